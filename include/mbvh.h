@@ -1,0 +1,183 @@
+/*
+ * mbvh.h — C ABI of the MI355X-native three-mesh-bvh query engine (libmbvh.so).
+ *
+ * The reference library (three-mesh-bvh 0.9.14) has no FFI: its boundary is the JavaScript
+ * method API of MeshBVH. Each entry point below is the native half of one of those methods and
+ * cites the reference code it replaces. The Node-API addon (csrc/napi/mbvh_napi.c) and the
+ * Python ctypes binding are the only callers; both marshal typed arrays into these signatures.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only. Every buffer is borrowed for the duration of the call.
+ *   - Functions return 0 (MBVH_OK) or a negative mbvh_status. mbvh_last_error() describes the
+ *     most recent failure on the calling thread.
+ *   - "_d" entry points take DEVICE pointers and enqueue asynchronously on `stream` (a
+ *     hipStream_t, or NULL for the context's own stream). They allocate nothing and never
+ *     synchronise, so a caller may capture them in a HIP graph. The other query entry points
+ *     take HOST pointers and are synchronous.
+ *   - Vectors are packed float triples (x, y, z) per element.
+ *   - A miss is reported as face index MBVH_NO_HIT and distance +Infinity.
+ */
+#ifndef MBVH_H
+#define MBVH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBVH_ABI_VERSION 1u
+#define MBVH_NO_HIT 0xFFFFFFFFu
+
+typedef enum mbvh_status {
+  MBVH_OK = 0,
+  MBVH_ERR_NO_DEVICE = -1,   /* no HIP device, or the device index is out of range */
+  MBVH_ERR_OOM = -2,         /* device or host allocation failed */
+  MBVH_ERR_BAD_HANDLE = -3,  /* NULL or foreign context / mesh / tree / batch handle */
+  MBVH_ERR_BAD_ARGS = -4,    /* invalid sizes, strides, ranges or option values */
+  MBVH_ERR_HIP = -5,         /* a HIP runtime call failed (message has the HIP error) */
+  MBVH_ERR_LIMIT = -6        /* input exceeds a documented limit of this build */
+} mbvh_status;
+
+/* three.js side constants (FrontSide = 0, BackSide = 1, DoubleSide = 2). */
+typedef enum mbvh_side {
+  MBVH_FRONT_SIDE = 0,
+  MBVH_BACK_SIDE = 1,
+  MBVH_DOUBLE_SIDE = 2,
+  MBVH_SIDE_PER_TRIANGLE = 3 /* use the per-triangle facing mask set by mbvh_mesh_set_side_mask */
+} mbvh_side;
+
+/* Split strategies, same values as src/core/Constants.js:9,19,29 (CENTER, AVERAGE, SAH). */
+typedef enum mbvh_strategy { MBVH_CENTER = 0, MBVH_AVERAGE = 1, MBVH_SAH = 2 } mbvh_strategy;
+
+typedef struct mbvh_ctx mbvh_ctx;     /* one HIP device + one stream + scratch buffers */
+typedef struct mbvh_tree mbvh_tree;   /* host-side build result (reference node buffers) */
+typedef struct mbvh_mesh mbvh_mesh;   /* device-resident BVH + triangles of one MeshBVH */
+typedef struct mbvh_batch mbvh_batch; /* several meshes concatenated for one batched launch */
+
+/* ------------------------------------------------------------------------------------------ */
+/* Library                                                                                    */
+/* ------------------------------------------------------------------------------------------ */
+
+uint32_t mbvh_abi_version(void);
+/* Message for the most recent failing call on this thread ("" if none). */
+const char* mbvh_last_error(void);
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int mbvh_device_count(void);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Host build (no device needed)                                                              */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct mbvh_build_opts {
+  int32_t strategy;         /* mbvh_strategy; reference default CENTER (Constants.js:75) */
+  int32_t max_depth;        /* default 40 (Constants.js:76) */
+  int32_t target_leaf_size; /* default 10 (Constants.js:77); `maxLeafTris` alias maps here */
+  int32_t indirect;         /* 0: permute `index` in place; 1: build an indirect buffer */
+} mbvh_build_opts;
+
+/*
+ * Builds the packed BVH in the reference's 32-byte node format.
+ * Replaces buildPackedTree (src/core/build/buildTree.js:123-152) together with
+ * MeshBVH.computePrimitiveBounds (src/core/MeshBVH.js:337-436), getBounds
+ * (src/core/build/computeBoundsUtils.js:4-68), getOptimalSplit (src/core/build/splitUtils.js:21-294),
+ * partition (src/core/build/sortUtils.js:4-62) and populateBuffer (src/core/build/buildUtils.js:20-91).
+ *
+ * positions       float32 vertex data; vertex v, axis k is positions[v*stride + offset + k]
+ * index           geometry index (elt bytes 2 or 4), or NULL for a non-indexed geometry (only
+ *                 allowed when opts->indirect). Non-indirect builds permute its triangle triplets
+ *                 IN PLACE, as the reference does through bvh.primitiveBuffer (GeometryBVH.js:86-89).
+ * root_ranges     n_roots (offset, count) pairs in triangles, as getRootPrimitiveRanges
+ *                 (src/core/build/geometryUtils.js:87-159) returns them. With opts->indirect the
+ *                 ranges only select the triangles listed in the generated indirect buffer and a
+ *                 single root covers it (generateIndirectBuffer GeometryBVH.js:9-35,
+ *                 getRootRanges GeometryBVH.js:169-181).
+ */
+int mbvh_build(const float* positions, uint64_t vertex_count, uint32_t position_stride,
+               uint32_t position_offset, void* index, int32_t index_elt_bytes, uint64_t index_count,
+               const uint32_t* root_ranges, uint32_t n_roots, const mbvh_build_opts* opts,
+               mbvh_tree** out_tree);
+/* Number of roots (= MeshBVH._roots.length), indirect buffer element size (0 when none) and count. */
+int mbvh_tree_info(const mbvh_tree* tree, uint32_t* n_roots, int32_t* indirect_elt_bytes,
+                   uint64_t* indirect_count);
+int mbvh_tree_root_bytes(const mbvh_tree* tree, uint32_t root, uint64_t* bytes);
+int mbvh_tree_copy_root(const mbvh_tree* tree, uint32_t root, void* dst, uint64_t dst_bytes);
+int mbvh_tree_copy_indirect(const mbvh_tree* tree, void* dst, uint64_t dst_bytes);
+void mbvh_tree_free(mbvh_tree* tree);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Device context and upload-once meshes                                                      */
+/* ------------------------------------------------------------------------------------------ */
+
+int mbvh_ctx_create(int32_t device, mbvh_ctx** out_ctx);
+void mbvh_ctx_destroy(mbvh_ctx* ctx);
+/* Waits for all work the context enqueued on its own stream. */
+int mbvh_ctx_synchronize(mbvh_ctx* ctx);
+
+/*
+ * Uploads one MeshBVH (its serialized form, MeshBVH.serialize src/core/MeshBVH.js:112-145:
+ * roots, index, indirectBuffer) plus its float32 positions to HBM, once. The library derives its
+ * traversal layout from those buffers on upload; later queries read only device memory.
+ * index may be NULL only when an indirect buffer is given (non-indexed indirect build).
+ */
+int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const uint64_t* root_bytes,
+                     uint32_t n_roots, const void* index, int32_t index_elt_bytes,
+                     uint64_t index_count, const void* indirect, int32_t indirect_elt_bytes,
+                     uint64_t indirect_count, const float* positions, uint64_t vertex_count,
+                     uint32_t position_stride, uint32_t position_offset, mbvh_mesh** out_mesh);
+void mbvh_mesh_free(mbvh_mesh* mesh);
+/* Triangles (BVH slots) and vertices held by a mesh, and the deepest root's node depth. */
+int mbvh_mesh_info(const mbvh_mesh* mesh, uint64_t* tri_count, uint64_t* vertex_count,
+                   uint32_t* max_depth);
+
+/*
+ * Per-triangle facing mask for material arrays with geometry groups (intersectTri's group loop,
+ * src/utils/ThreeRayIntersectUtilities.js:171-202). mask[f] for geometry triangle f (the
+ * reference faceIndex): bit 0 = front-facing hits allowed, bit 1 = back-facing hits allowed.
+ * Used by queries called with side = MBVH_SIDE_PER_TRIANGLE.
+ */
+int mbvh_mesh_set_side_mask(mbvh_ctx* ctx, mbvh_mesh* mesh, const uint8_t* mask, uint64_t n);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Queries                                                                                    */
+/* ------------------------------------------------------------------------------------------ */
+
+/*
+ * First hit per ray. Replaces MeshBVH.raycastFirst (src/core/MeshBVH.js:558-577) →
+ * raycastFirst (src/core/cast/raycastFirst.template.js:9-103) → intersectClosestTri
+ * (src/core/utils/iterationUtils.template.js:24-53) → intersectTri / checkIntersection
+ * (src/utils/ThreeRayIntersectUtilities.js:21-47,154-230), with three's Ray.intersectTriangle.
+ * Results per ray: face index (reference faceIndex), distance |point - origin|, point
+ * (origin + t*dir) and barycoord (weights of face a, b, c as Triangle.getBarycoord). Any of
+ * distance / point / barycoord may be NULL.
+ */
+int mbvh_raycast_first(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins,
+                       const float* directions, uint64_t n, int32_t side, float near_dist,
+                       float far_dist, uint32_t* face_index, float* distance, float* point,
+                       float* barycoord);
+int mbvh_raycast_first_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins,
+                         const float* directions, uint64_t n, int32_t side, float near_dist,
+                         float far_dist, uint32_t* face_index, float* distance, float* point,
+                         float* barycoord, void* stream);
+
+/*
+ * Closest point on the mesh per query point. Replaces MeshBVH.closestPointToPoint
+ * (src/core/MeshBVH.js:874-884) → closestPointToPoint (src/core/cast/closestPointToPoint.js:6-78)
+ * → shapecast (src/core/BVH.js:342-413, src/core/cast/shapecast.js:12-214) with three's
+ * Triangle.closestPointToPoint. Results: face index, distance, closest point and the barycoord of
+ * that point in its face (getTriangleHitPointInfo, src/utils/TriangleUtilities.js:65-147).
+ * Nothing found → MBVH_NO_HIT / +Infinity. distance / point / barycoord may be NULL.
+ */
+int mbvh_closest_point(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
+                       float min_threshold, float max_threshold, uint32_t* face_index,
+                       float* distance, float* closest, float* barycoord);
+int mbvh_closest_point_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
+                         float min_threshold, float max_threshold, uint32_t* face_index,
+                         float* distance, float* closest, float* barycoord, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MBVH_H */

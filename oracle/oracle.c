@@ -1,0 +1,729 @@
+/*
+ * oracle.c — FP64 CPU restatement of the three-mesh-bvh query path. TEST INFRASTRUCTURE ONLY:
+ * loaded by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as the checker,
+ * never by the product. See oracle.h for what each function restates.
+ *
+ * Built with -ffp-contract=off -fno-fast-math so every operation rounds exactly as the JS
+ * reference's IEEE double arithmetic does (JS never fuses multiply-adds).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NO_HIT 0xFFFFFFFFu
+
+/* ------------------------------------------------------------------------------------------ */
+/* three.js math restated (Vector3 operation order preserved)                                 */
+/* ------------------------------------------------------------------------------------------ */
+
+static void v_sub(const double* a, const double* b, double* o) {
+  o[0] = a[0] - b[0];
+  o[1] = a[1] - b[1];
+  o[2] = a[2] - b[2];
+}
+static double v_dot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static void v_cross(const double* a, const double* b, double* o) { /* Vector3.crossVectors */
+  const double ax = a[0], ay = a[1], az = a[2], bx = b[0], by = b[1], bz = b[2];
+  o[0] = ay * bz - az * by;
+  o[1] = az * bx - ax * bz;
+  o[2] = ax * by - ay * bx;
+}
+static double v_dist_sq(const double* a, const double* b) { /* Vector3.distanceToSquared */
+  const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return dx * dx + dy * dy + dz * dz;
+}
+
+/* Ray.intersectTriangle (GeometricTools IntrRay3Triangle3, as published in three.js). */
+int orc_ray_triangle(const double o[3], const double d[3], const double a[3], const double b[3],
+                     const double c[3], int backface_culling, double* t, double point[3]) {
+  double edge1[3], edge2[3], normal[3], diff[3], tmp[3];
+  v_sub(b, a, edge1);
+  v_sub(c, a, edge2);
+  v_cross(edge1, edge2, normal);
+  double DdN = v_dot(d, normal);
+  double sign;
+  if (DdN > 0) {
+    if (backface_culling) return 0;
+    sign = 1;
+  } else if (DdN < 0) {
+    sign = -1;
+    DdN = -DdN;
+  } else {
+    return 0;
+  }
+  v_sub(o, a, diff);
+  v_cross(diff, edge2, tmp);
+  const double DdQxE2 = sign * v_dot(d, tmp);
+  if (DdQxE2 < 0) return 0;
+  v_cross(edge1, diff, tmp);
+  const double DdE1xQ = sign * v_dot(d, tmp);
+  if (DdE1xQ < 0) return 0;
+  if (DdQxE2 + DdE1xQ > DdN) return 0;
+  const double QdN = -sign * v_dot(diff, normal);
+  if (QdN < 0) return 0;
+  const double tt = QdN / DdN;
+  *t = tt;
+  /* Ray.at: origin + direction * t */
+  point[0] = o[0] + d[0] * tt;
+  point[1] = o[1] + d[1] * tt;
+  point[2] = o[2] + d[2] * tt;
+  return 1;
+}
+
+/* Triangle.closestPointToPoint: Ericson, Real-Time Collision Detection §5.1.5. */
+void orc_closest_point_triangle(const double p[3], const double a[3], const double b[3],
+                                const double c[3], double out[3]) {
+  double vab[3], vac[3], vap[3], vbp[3], vcp[3];
+  v_sub(b, a, vab);
+  v_sub(c, a, vac);
+  v_sub(p, a, vap);
+  const double d1 = v_dot(vab, vap), d2 = v_dot(vac, vap);
+  if (d1 <= 0 && d2 <= 0) {
+    memcpy(out, a, sizeof(double) * 3);
+    return;
+  }
+  v_sub(p, b, vbp);
+  const double d3 = v_dot(vab, vbp), d4 = v_dot(vac, vbp);
+  if (d3 >= 0 && d4 <= d3) {
+    memcpy(out, b, sizeof(double) * 3);
+    return;
+  }
+  const double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+    const double v = d1 / (d1 - d3);
+    for (int k = 0; k < 3; k++) out[k] = a[k] + vab[k] * v;
+    return;
+  }
+  v_sub(p, c, vcp);
+  const double d5 = v_dot(vab, vcp), d6 = v_dot(vac, vcp);
+  if (d6 >= 0 && d5 <= d6) {
+    memcpy(out, c, sizeof(double) * 3);
+    return;
+  }
+  const double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+    const double w = d2 / (d2 - d6);
+    for (int k = 0; k < 3; k++) out[k] = a[k] + vac[k] * w;
+    return;
+  }
+  const double va = d3 * d6 - d5 * d4;
+  if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+    double vbc[3];
+    v_sub(c, b, vbc);
+    const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    for (int k = 0; k < 3; k++) out[k] = b[k] + vbc[k] * w;
+    return;
+  }
+  const double denom = 1 / (va + vb + vc);
+  const double v = vb * denom, w = vc * denom;
+  for (int k = 0; k < 3; k++) out[k] = (a[k] + vab[k] * v) + vac[k] * w;
+}
+
+/* Triangle.getBarycoord */
+int orc_barycoord(const double p[3], const double a[3], const double b[3], const double c[3],
+                  double out[3]) {
+  double v0[3], v1[3], v2[3];
+  v_sub(c, a, v0);
+  v_sub(b, a, v1);
+  v_sub(p, a, v2);
+  const double dot00 = v_dot(v0, v0), dot01 = v_dot(v0, v1), dot02 = v_dot(v0, v2);
+  const double dot11 = v_dot(v1, v1), dot12 = v_dot(v1, v2);
+  const double denom = dot00 * dot11 - dot01 * dot01;
+  if (denom == 0) {
+    out[0] = out[1] = out[2] = 0;
+    return 0;
+  }
+  const double inv = 1 / denom;
+  const double u = (dot11 * dot02 - dot01 * dot12) * inv;
+  const double v = (dot00 * dot12 - dot01 * dot02) * inv;
+  out[0] = 1 - u - v;
+  out[1] = v;
+  out[2] = u;
+  return 1;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Mesh access                                                                                */
+/* ------------------------------------------------------------------------------------------ */
+
+static uint64_t rd(const void* p, int eb, uint64_t i) {
+  return eb == 2 ? ((const uint16_t*)p)[i] : ((const uint32_t*)p)[i];
+}
+static uint32_t face_of_slot(const orc_mesh* m, uint64_t slot) {
+  return (uint32_t)(m->indirect ? rd(m->indirect, m->indirect_elt_bytes, slot) : slot);
+}
+static void face_vertices(const orc_mesh* m, uint32_t face, double a[3], double b[3], double c[3]) {
+  uint64_t v[3] = {3ull * face, 3ull * face + 1, 3ull * face + 2};
+  if (m->index)
+    for (int k = 0; k < 3; k++) v[k] = rd(m->index, m->index_elt_bytes, v[k]);
+  double* out[3] = {a, b, c};
+  for (int k = 0; k < 3; k++) {
+    const float* p = m->positions + v[k] * m->stride + m->offset;
+    out[k][0] = p[0];
+    out[k][1] = p[1];
+    out[k][2] = p[2];
+  }
+}
+
+/* Reference node accessors (src/core/utils/nodeBufferUtils.js:3-47) */
+typedef struct {
+  const uint8_t* bytes;
+  uint64_t n;
+} NodeBuf;
+static const float* nb_bounds(NodeBuf nb, uint64_t i) { return (const float*)(nb.bytes + 32 * i); }
+static uint32_t nb_u32(NodeBuf nb, uint64_t i, int w) {
+  uint32_t v;
+  memcpy(&v, nb.bytes + 32 * i + 4 * w, 4);
+  return v;
+}
+static uint16_t nb_u16(NodeBuf nb, uint64_t i, int h) {
+  uint16_t v;
+  memcpy(&v, nb.bytes + 32 * i + 2 * h, 2);
+  return v;
+}
+static int nb_leaf(NodeBuf nb, uint64_t i) { return nb_u16(nb, i, 15) == 0xFFFF; }
+static NodeBuf root_buf(const orc_mesh* m, uint32_t r) {
+  NodeBuf nb = {(const uint8_t*)m->roots[r], m->root_bytes[r] / 32};
+  return nb;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Raycast                                                                                    */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+  double o[3], d[3];
+  int side;
+  double near_d, far_d;
+  const orc_mesh* m;
+  uint32_t nodes, tris;
+} RayCtx;
+
+typedef struct {
+  int found;
+  uint32_t face;
+  double dist;
+  double point[3];
+} Hit;
+
+/* intersectsNodeBounds (src/core/utils/intersectUtils.js:2-78) */
+static int node_bounds_hit(const float* b, const RayCtx* r) {
+  double tmin, tmax, tymin, tymax, tzmin, tzmax;
+  const double invdirx = 1 / r->d[0], invdiry = 1 / r->d[1], invdirz = 1 / r->d[2];
+  const double ox = r->o[0], oy = r->o[1], oz = r->o[2];
+  const double minx = b[0], maxx = b[3], miny = b[1], maxy = b[4], minz = b[2], maxz = b[5];
+  if (invdirx >= 0) {
+    tmin = (minx - ox) * invdirx;
+    tmax = (maxx - ox) * invdirx;
+  } else {
+    tmin = (maxx - ox) * invdirx;
+    tmax = (minx - ox) * invdirx;
+  }
+  if (invdiry >= 0) {
+    tymin = (miny - oy) * invdiry;
+    tymax = (maxy - oy) * invdiry;
+  } else {
+    tymin = (maxy - oy) * invdiry;
+    tymax = (miny - oy) * invdiry;
+  }
+  if ((tmin > tymax) || (tymin > tmax)) return 0;
+  if (tymin > tmin || isnan(tmin)) tmin = tymin;
+  if (tymax < tmax || isnan(tmax)) tmax = tymax;
+  if (invdirz >= 0) {
+    tzmin = (minz - oz) * invdirz;
+    tzmax = (maxz - oz) * invdirz;
+  } else {
+    tzmin = (maxz - oz) * invdirz;
+    tzmax = (minz - oz) * invdirz;
+  }
+  if ((tmin > tzmax) || (tzmin > tmax)) return 0;
+  if (tzmin > tmin || tmin != tmin) tmin = tzmin;
+  if (tzmax < tmax || tmax != tmax) tmax = tzmax;
+  return tmin <= r->far_d && tmax >= r->near_d;
+}
+
+static int side_of_face(const RayCtx* r, uint32_t face) {
+  if (r->side != ORC_PER_FACE) return r->side;
+  const uint8_t mk = r->m->face_mask ? r->m->face_mask[face] & 3 : 3;
+  if (mk == 1) return ORC_FRONT;
+  if (mk == 2) return ORC_BACK;
+  if (mk == 3) return ORC_DOUBLE;
+  return -1;
+}
+
+/* intersectTri → checkBufferGeometryIntersection → checkIntersection
+ * (src/utils/ThreeRayIntersectUtilities.js:21-47,49-145,154-230) */
+static int intersect_face(RayCtx* r, uint32_t face, Hit* h) {
+  r->tris++;
+  const int side = side_of_face(r, face);
+  if (side < 0) return 0;
+  double a[3], b[3], c[3], t, p[3];
+  face_vertices(r->m, face, a, b, c);
+  int ok;
+  if (side == ORC_BACK)
+    ok = orc_ray_triangle(r->o, r->d, c, b, a, 1, &t, p);
+  else
+    ok = orc_ray_triangle(r->o, r->d, a, b, c, side != ORC_DOUBLE, &t, p);
+  if (!ok) return 0;
+  const double dist = sqrt(v_dist_sq(r->o, p)); /* ray.origin.distanceTo(point) */
+  if (dist < r->near_d || dist > r->far_d) return 0;
+  h->found = 1;
+  h->face = face;
+  h->dist = dist;
+  memcpy(h->point, p, sizeof(p));
+  return 1;
+}
+
+/* intersectClosestTri (src/core/utils/iterationUtils.template.js:24-53) */
+static Hit closest_in_leaf(RayCtx* r, uint64_t off, uint64_t cnt) {
+  Hit best;
+  best.found = 0;
+  double dist = INFINITY;
+  for (uint64_t s = off; s < off + cnt; s++) {
+    Hit h;
+    if (intersect_face(r, face_of_slot(r->m, s), &h) && h.dist < dist) {
+      best = h;
+      dist = h.dist;
+    }
+  }
+  return best;
+}
+
+/* _raycastFirst (src/core/cast/raycastFirst.template.js:19-103) */
+static Hit raycast_first_node(RayCtx* r, NodeBuf nb, uint64_t node) {
+  Hit none;
+  none.found = 0;
+  if (nb_leaf(nb, node)) return closest_in_leaf(r, nb_u32(nb, node, 6), nb_u16(nb, node, 14));
+  const uint32_t axis = nb_u32(nb, node, 7);
+  const int left_to_right = r->d[axis] >= 0;
+  const uint64_t left = node + 1, right = node + nb_u32(nb, node, 6);
+  const uint64_t c1 = left_to_right ? left : right, c2 = left_to_right ? right : left;
+  r->nodes++;
+  Hit r1 = none;
+  if (node_bounds_hit(nb_bounds(nb, c1), r)) r1 = raycast_first_node(r, nb, c1);
+  if (r1.found) {
+    const double p = r1.point[axis];
+    const float* b2 = nb_bounds(nb, c2);
+    const int outside = left_to_right ? p <= b2[axis] : p >= b2[axis + 3];
+    if (outside) return r1;
+  }
+  r->nodes++;
+  Hit r2 = none;
+  if (node_bounds_hit(nb_bounds(nb, c2), r)) r2 = raycast_first_node(r, nb, c2);
+  if (r1.found && r2.found) return r1.dist <= r2.dist ? r1 : r2;
+  if (r1.found) return r1;
+  return r2;
+}
+
+static Hit raycast_first_one(RayCtx* r, int brute) {
+  const orc_mesh* m = r->m;
+  Hit best;
+  best.found = 0;
+  for (uint32_t i = 0; i < m->n_roots; i++) {
+    NodeBuf nb = root_buf(m, i);
+    Hit h;
+    if (brute) {
+      /* every slot of this root's range, in slot order */
+      uint64_t lo = UINT64_MAX, hi = 0;
+      for (uint64_t k = 0; k < nb.n; k++) {
+        if (!nb_leaf(nb, k)) continue;
+        const uint64_t off = nb_u32(nb, k, 6), cnt = nb_u16(nb, k, 14);
+        if (off < lo) lo = off;
+        if (off + cnt > hi) hi = off + cnt;
+      }
+      h = lo < hi ? closest_in_leaf(r, lo, hi - lo) : best;
+      if (lo >= hi) h.found = 0;
+    } else {
+      h = raycast_first_node(r, nb, 0);
+    }
+    /* MeshBVH.raycastFirst: keep a root's result only if strictly closer (MeshBVH.js:567) */
+    if (h.found && (!best.found || h.dist < best.dist)) best = h;
+  }
+  return best;
+}
+
+/* _raycast (src/core/cast/raycast.template.js:16-53) */
+typedef struct {
+  uint64_t count, cap;
+  uint32_t* face;
+  double* dist;
+} HitList;
+static void raycast_all_node(RayCtx* r, NodeBuf nb, uint64_t node, HitList* out) {
+  if (nb_leaf(nb, node)) {
+    const uint64_t off = nb_u32(nb, node, 6), cnt = nb_u16(nb, node, 14);
+    for (uint64_t s = off; s < off + cnt; s++) {
+      Hit h;
+      if (intersect_face(r, face_of_slot(r->m, s), &h)) {
+        if (out->count < out->cap) {
+          out->face[out->count] = h.face;
+          out->dist[out->count] = h.dist;
+        }
+        out->count++;
+      }
+    }
+    return;
+  }
+  const uint64_t left = node + 1, right = node + nb_u32(nb, node, 6);
+  r->nodes++;
+  if (node_bounds_hit(nb_bounds(nb, left), r)) raycast_all_node(r, nb, left, out);
+  r->nodes++;
+  if (node_bounds_hit(nb_bounds(nb, right), r)) raycast_all_node(r, nb, right, out);
+}
+
+int64_t orc_raycast_all(const orc_mesh* m, const float origin[3], const float dir[3], int side,
+                        double near_d, double far_d, uint64_t cap, uint32_t* face, double* dist) {
+  RayCtx r;
+  memset(&r, 0, sizeof(r));
+  for (int k = 0; k < 3; k++) {
+    r.o[k] = origin[k];
+    r.d[k] = dir[k];
+  }
+  r.side = side;
+  r.near_d = near_d;
+  r.far_d = far_d;
+  r.m = m;
+  HitList out = {0, cap, face, dist};
+  for (uint32_t i = 0; i < m->n_roots; i++) raycast_all_node(&r, root_buf(m, i), 0, &out);
+  return (int64_t)out.count;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Closest point                                                                              */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+  double p[3];
+  double best_sq, min_sq, max_sq;
+  uint32_t best_face;
+  double best_point[3];
+  const orc_mesh* m;
+  uint32_t nodes, tris;
+} PointCtx;
+
+static double box_score(const float* b, const double p[3]) { /* clamp + distanceToSquared */
+  double t[3];
+  for (int k = 0; k < 3; k++) t[k] = fmax((double)b[k], fmin((double)b[k + 3], p[k]));
+  return v_dist_sq(t, p);
+}
+
+/* closestPointToPoint's intersectsTriangle callback (closestPointToPoint.js:39-61) */
+static int point_leaf(PointCtx* c, uint64_t off, uint64_t cnt) {
+  for (uint64_t s = off; s < off + cnt; s++) {
+    const uint32_t face = face_of_slot(c->m, s);
+    double a[3], b[3], cc[3], cp[3];
+    face_vertices(c->m, face, a, b, cc);
+    c->tris++;
+    orc_closest_point_triangle(c->p, a, b, cc, cp);
+    const double d2 = v_dist_sq(c->p, cp);
+    if (d2 < c->best_sq) {
+      memcpy(c->best_point, cp, sizeof(cp));
+      c->best_sq = d2;
+      c->best_face = face;
+    }
+    if (d2 < c->min_sq) return 1;
+  }
+  return 0;
+}
+
+/* shapecastTraverse (src/core/cast/shapecast.js:41-214) with the closest-point callbacks */
+static int point_node(PointCtx* c, NodeBuf nb, uint64_t node) {
+  if (nb_leaf(nb, node)) return point_leaf(c, nb_u32(nb, node, 6), nb_u16(nb, node, 14));
+  const uint64_t left = node + 1, right = node + nb_u32(nb, node, 6);
+  uint64_t c1 = left, c2 = right;
+  double s1 = box_score(nb_bounds(nb, left), c->p);
+  double s2 = box_score(nb_bounds(nb, right), c->p);
+  c->nodes += 2;
+  if (s2 < s1) {
+    c1 = right;
+    c2 = left;
+    const double t = s1;
+    s1 = s2;
+    s2 = t;
+  }
+  if (s1 < c->best_sq && s1 < c->max_sq)
+    if (point_node(c, nb, c1)) return 1;
+  if (s2 < c->best_sq && s2 < c->max_sq)
+    if (point_node(c, nb, c2)) return 1;
+  return 0;
+}
+
+static void closest_point_one(PointCtx* c, int brute) {
+  const orc_mesh* m = c->m;
+  for (uint32_t i = 0; i < m->n_roots; i++) {
+    NodeBuf nb = root_buf(m, i);
+    int stop;
+    if (brute) {
+      uint64_t lo = UINT64_MAX, hi = 0;
+      for (uint64_t k = 0; k < nb.n; k++) {
+        if (!nb_leaf(nb, k)) continue;
+        const uint64_t off = nb_u32(nb, k, 6), cnt = nb_u16(nb, k, 14);
+        if (off < lo) lo = off;
+        if (off + cnt > hi) hi = off + cnt;
+      }
+      stop = lo < hi ? point_leaf(c, lo, hi - lo) : 0;
+    } else {
+      stop = point_node(c, nb, 0);
+    }
+    if (stop) break;
+  }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Threaded drivers                                                                           */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+  const orc_mesh* m;
+  int kind; /* 0 raycast first, 1 closest point */
+  const float *a, *b;
+  uint64_t n;
+  int side, brute;
+  double p0, p1;
+  uint32_t* face;
+  double *dist, *point, *bary;
+  uint32_t *nodes, *tris;
+  uint64_t begin, end;
+} Job;
+
+static void run_ray(const Job* j, uint64_t i) {
+  RayCtx r;
+  memset(&r, 0, sizeof(r));
+  for (int k = 0; k < 3; k++) {
+    r.o[k] = j->a[3 * i + k];
+    r.d[k] = j->b[3 * i + k];
+  }
+  r.side = j->side;
+  r.near_d = j->p0;
+  r.far_d = j->p1;
+  r.m = j->m;
+  Hit h = raycast_first_one(&r, j->brute);
+  j->face[i] = h.found ? h.face : NO_HIT;
+  if (j->dist) j->dist[i] = h.found ? h.dist : INFINITY;
+  if (j->point)
+    for (int k = 0; k < 3; k++) j->point[3 * i + k] = h.found ? h.point[k] : NAN;
+  if (j->bary) {
+    double bc[3] = {NAN, NAN, NAN};
+    if (h.found) {
+      double a[3], b[3], c[3];
+      face_vertices(j->m, h.face, a, b, c);
+      orc_barycoord(h.point, a, b, c, bc);
+    }
+    for (int k = 0; k < 3; k++) j->bary[3 * i + k] = bc[k];
+  }
+  if (j->nodes) j->nodes[i] = r.nodes;
+  if (j->tris) j->tris[i] = r.tris;
+}
+
+static void run_point(const Job* j, uint64_t i) {
+  PointCtx c;
+  memset(&c, 0, sizeof(c));
+  for (int k = 0; k < 3; k++) c.p[k] = j->a[3 * i + k];
+  c.best_sq = INFINITY;
+  c.min_sq = j->p0 * j->p0;
+  c.max_sq = j->p1 * j->p1;
+  c.best_face = NO_HIT;
+  c.m = j->m;
+  closest_point_one(&c, j->brute);
+  const int found = c.best_sq != INFINITY;
+  j->face[i] = found ? c.best_face : NO_HIT;
+  if (j->dist) j->dist[i] = found ? sqrt(c.best_sq) : INFINITY;
+  if (j->point)
+    for (int k = 0; k < 3; k++) j->point[3 * i + k] = found ? c.best_point[k] : NAN;
+  if (j->nodes) j->nodes[i] = c.nodes;
+  if (j->tris) j->tris[i] = c.tris;
+}
+
+static void* worker(void* arg) {
+  const Job* j = (const Job*)arg;
+  for (uint64_t i = j->begin; i < j->end; i++) {
+    if (j->kind == 0)
+      run_ray(j, i);
+    else
+      run_point(j, i);
+  }
+  return NULL;
+}
+
+static int run_jobs(Job proto, int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+  if ((uint64_t)nthreads > proto.n) nthreads = proto.n ? (int)proto.n : 1;
+  Job* jobs = (Job*)calloc((size_t)nthreads, sizeof(Job));
+  pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+  if (!jobs || !th) {
+    free(jobs);
+    free(th);
+    return -1;
+  }
+  /* interleaved chunks of 256 queries keep the threads balanced */
+  const uint64_t chunk = 256;
+  const uint64_t nchunks = (proto.n + chunk - 1) / chunk;
+  for (int t = 0; t < nthreads; t++) {
+    jobs[t] = proto;
+    jobs[t].begin = nchunks * (uint64_t)t / (uint64_t)nthreads * chunk;
+    jobs[t].end = nchunks * (uint64_t)(t + 1) / (uint64_t)nthreads * chunk;
+    if (jobs[t].end > proto.n) jobs[t].end = proto.n;
+  }
+  for (int t = 1; t < nthreads; t++) pthread_create(&th[t], NULL, worker, &jobs[t]);
+  worker(&jobs[0]);
+  for (int t = 1; t < nthreads; t++) pthread_join(th[t], NULL);
+  free(jobs);
+  free(th);
+  return 0;
+}
+
+int orc_raycast_first(const orc_mesh* m, const float* origins, const float* dirs, uint64_t n,
+                      int side, double near_d, double far_d, int brute, int nthreads,
+                      uint32_t* face, double* dist, double* point, double* bary,
+                      uint32_t* nodes_read, uint32_t* tris_tested) {
+  Job j;
+  memset(&j, 0, sizeof(j));
+  j.m = m;
+  j.kind = 0;
+  j.a = origins;
+  j.b = dirs;
+  j.n = n;
+  j.side = side;
+  j.brute = brute;
+  j.p0 = near_d;
+  j.p1 = far_d;
+  j.face = face;
+  j.dist = dist;
+  j.point = point;
+  j.bary = bary;
+  j.nodes = nodes_read;
+  j.tris = tris_tested;
+  return run_jobs(j, nthreads);
+}
+
+int orc_closest_point(const orc_mesh* m, const float* points, uint64_t n, double min_t,
+                      double max_t, int brute, int nthreads, uint32_t* face, double* dist,
+                      double* point, uint32_t* nodes_read, uint32_t* tris_tested) {
+  Job j;
+  memset(&j, 0, sizeof(j));
+  j.m = m;
+  j.kind = 1;
+  j.a = points;
+  j.n = n;
+  j.brute = brute;
+  j.p0 = min_t;
+  j.p1 = max_t;
+  j.face = face;
+  j.dist = dist;
+  j.point = point;
+  j.nodes = nodes_read;
+  j.tris = tris_tested;
+  return run_jobs(j, nthreads);
+}
+
+double orc_ray_plane_distance(const orc_mesh* m, uint32_t face, const float origin[3],
+                              const float dir[3]) {
+  double a[3], b[3], c[3], e1[3], e2[3], n[3], q[3];
+  face_vertices(m, face, a, b, c);
+  v_sub(b, a, e1);
+  v_sub(c, a, e2);
+  v_cross(e1, e2, n);
+  const double o[3] = {origin[0], origin[1], origin[2]}, d[3] = {dir[0], dir[1], dir[2]};
+  const double ddn = v_dot(d, n);
+  if (ddn == 0) return INFINITY;
+  v_sub(a, o, q);
+  const double t = v_dot(q, n) / ddn;
+  return fabs(t) * sqrt(v_dot(d, d));
+}
+
+double orc_point_face_distance(const orc_mesh* m, uint32_t face, const float p[3]) {
+  double a[3], b[3], c[3], cp[3];
+  const double q[3] = {p[0], p[1], p[2]};
+  face_vertices(m, face, a, b, c);
+  orc_closest_point_triangle(q, a, b, c, cp);
+  return sqrt(v_dist_sq(q, cp));
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* validateBounds (src/debug/Debug.js:207-267) + structural checks                            */
+/* ------------------------------------------------------------------------------------------ */
+
+static int box_contains(const float* outer, const float* inner) { /* Box3.containsBox */
+  return outer[0] <= inner[0] && inner[3] <= outer[3] && outer[1] <= inner[1] && inner[4] <= outer[4] &&
+         outer[2] <= inner[2] && inner[5] <= outer[5];
+}
+
+int orc_validate_root(const orc_mesh* m, uint32_t root, orc_tree_stats* st) {
+  memset(st, 0, sizeof(*st));
+  if (root >= m->n_roots) return -1;
+  NodeBuf nb = root_buf(m, root);
+  if (nb.n == 0) {
+    st->structure_failures++;
+    return 1;
+  }
+  const uint64_t n_slots = m->indirect ? m->indirect_count : (m->index ? m->index_count / 3 : m->vertex_count / 3);
+  uint8_t* seen = (uint8_t*)calloc(n_slots ? n_slots : 1, 1);
+  uint64_t* stack = (uint64_t*)malloc(sizeof(uint64_t) * 2 * (nb.n + 1));
+  uint64_t sp = 0, lo = UINT64_MAX, hi = 0;
+  st->depth_min = UINT32_MAX;
+  st->leaf_tris_min = UINT32_MAX;
+  stack[sp++] = 0;
+  stack[sp++] = 0;
+  uint64_t visited = 0;
+  while (sp) {
+    const uint64_t depth = stack[--sp];
+    const uint64_t node = stack[--sp];
+    if (node >= nb.n || ++visited > nb.n) {
+      st->structure_failures++;
+      break;
+    }
+    st->node_count++;
+    const float* box = nb_bounds(nb, node);
+    if (nb_leaf(nb, node)) {
+      const uint64_t off = nb_u32(nb, node, 6), cnt = nb_u16(nb, node, 14);
+      st->leaf_count++;
+      st->tri_count += cnt;
+      if (depth < st->depth_min) st->depth_min = (uint32_t)depth;
+      if (depth > st->depth_max) st->depth_max = (uint32_t)depth;
+      if (cnt < st->leaf_tris_min) st->leaf_tris_min = (uint32_t)cnt;
+      if (cnt > st->leaf_tris_max) st->leaf_tris_max = (uint32_t)cnt;
+      if (off + cnt > n_slots) {
+        st->structure_failures++;
+        continue;
+      }
+      if (off < lo) lo = off;
+      if (off + cnt > hi) hi = off + cnt;
+      float tb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+      for (uint64_t s = off; s < off + cnt; s++) {
+        if (seen[s]) st->structure_failures++;
+        seen[s] = 1;
+        double a[3], b[3], c[3];
+        face_vertices(m, face_of_slot(m, s), a, b, c);
+        for (int k = 0; k < 3; k++) {
+          const double mn = fmin(a[k], fmin(b[k], c[k])), mx = fmax(a[k], fmax(b[k], c[k]));
+          if (mn < tb[k]) tb[k] = (float)mn;
+          if (mx > tb[k + 3]) tb[k + 3] = (float)mx;
+        }
+      }
+      if (cnt && !box_contains(box, tb)) st->bounds_failures++;
+    } else {
+      const uint64_t left = node + 1, right = node + nb_u32(nb, node, 6);
+      const uint32_t axis = nb_u32(nb, node, 7);
+      if (axis > 2 || right <= node || right >= nb.n || left >= nb.n) {
+        st->structure_failures++;
+        continue;
+      }
+      if (!box_contains(box, nb_bounds(nb, left))) st->bounds_failures++;
+      if (!box_contains(box, nb_bounds(nb, right))) st->bounds_failures++;
+      stack[sp++] = right;
+      stack[sp++] = depth + 1;
+      stack[sp++] = left;
+      stack[sp++] = depth + 1;
+    }
+  }
+  /* the root's leaves tile one contiguous slot range */
+  if (lo < hi)
+    for (uint64_t s = lo; s < hi; s++)
+      if (!seen[s]) st->structure_failures++;
+  if (visited != nb.n) st->structure_failures++;
+  free(seen);
+  free(stack);
+  return (st->bounds_failures || st->structure_failures) ? 1 : 0;
+}

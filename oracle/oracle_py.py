@@ -1,0 +1,180 @@
+"""ctypes wrapper of liboracle.so — the FP64 restatement of the reference query path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg, where it is the checker (or the timed CPU baseline). The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+NO_HIT = 0xFFFFFFFF
+FRONT, BACK, DOUBLE, PER_FACE = 0, 1, 2, 3
+
+
+class _Mesh(C.Structure):
+    _fields_ = [
+        ("positions", C.c_void_p), ("vertex_count", C.c_uint64), ("stride", C.c_uint32), ("offset", C.c_uint32),
+        ("index", C.c_void_p), ("index_elt_bytes", C.c_int32), ("index_count", C.c_uint64),
+        ("indirect", C.c_void_p), ("indirect_elt_bytes", C.c_int32), ("indirect_count", C.c_uint64),
+        ("roots", C.c_void_p), ("root_bytes", C.c_void_p), ("n_roots", C.c_uint32),
+        ("face_mask", C.c_void_p),
+    ]
+
+
+class TreeStats(C.Structure):
+    _fields_ = [("node_count", C.c_uint64), ("leaf_count", C.c_uint64), ("tri_count", C.c_uint64),
+                ("depth_min", C.c_uint32), ("depth_max", C.c_uint32),
+                ("leaf_tris_min", C.c_uint32), ("leaf_tris_max", C.c_uint32),
+                ("bounds_failures", C.c_uint64), ("structure_failures", C.c_uint64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+
+def library():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing; run `make -C oracle`")
+        lib = C.CDLL(LIB_PATH)
+        vp, u64, i32, f64, u32 = C.c_void_p, C.c_uint64, C.c_int32, C.c_double, C.c_uint32
+        lib.orc_ray_triangle.argtypes = [vp, vp, vp, vp, vp, C.c_int, vp, vp]
+        lib.orc_ray_triangle.restype = C.c_int
+        lib.orc_closest_point_triangle.argtypes = [vp, vp, vp, vp, vp]
+        lib.orc_closest_point_triangle.restype = None
+        lib.orc_barycoord.argtypes = [vp, vp, vp, vp, vp]
+        lib.orc_barycoord.restype = C.c_int
+        lib.orc_raycast_first.argtypes = [vp, vp, vp, u64, C.c_int, f64, f64, C.c_int, C.c_int,
+                                          vp, vp, vp, vp, vp, vp]
+        lib.orc_raycast_first.restype = C.c_int
+        lib.orc_raycast_all.argtypes = [vp, vp, vp, C.c_int, f64, f64, u64, vp, vp]
+        lib.orc_raycast_all.restype = C.c_int64
+        lib.orc_closest_point.argtypes = [vp, vp, u64, f64, f64, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+        lib.orc_closest_point.restype = C.c_int
+        lib.orc_ray_plane_distance.argtypes = [vp, u32, vp, vp]
+        lib.orc_ray_plane_distance.restype = f64
+        lib.orc_point_face_distance.argtypes = [vp, u32, vp]
+        lib.orc_point_face_distance.restype = f64
+        lib.orc_validate_root.argtypes = [vp, u32, C.POINTER(TreeStats)]
+        lib.orc_validate_root.restype = C.c_int
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _v3(x):
+    return np.ascontiguousarray(x, dtype=np.float64).reshape(3)
+
+
+def ray_triangle(o, d, a, b, c, backface_culling=True):
+    t = np.zeros(1)
+    p = np.zeros(3)
+    arrs = [_v3(o), _v3(d), _v3(a), _v3(b), _v3(c)]
+    ok = library().orc_ray_triangle(*[_p(x) for x in arrs], int(backface_culling), _p(t), _p(p))
+    return (float(t[0]), p) if ok else None
+
+
+def closest_point_triangle(p, a, b, c):
+    out = np.zeros(3)
+    arrs = [_v3(p), _v3(a), _v3(b), _v3(c)]
+    library().orc_closest_point_triangle(*[_p(x) for x in arrs], _p(out))
+    return out
+
+
+def barycoord(p, a, b, c):
+    out = np.zeros(3)
+    arrs = [_v3(p), _v3(a), _v3(b), _v3(c)]
+    ok = library().orc_barycoord(*[_p(x) for x in arrs], _p(out))
+    return out if ok else None
+
+
+class OracleMesh:
+    """Holds the arrays of one reference-format BVH for the oracle."""
+
+    def __init__(self, positions: np.ndarray, index: Optional[np.ndarray], roots: Sequence[np.ndarray],
+                 indirect: Optional[np.ndarray] = None, face_mask: Optional[np.ndarray] = None,
+                 stride: int = 3, offset: int = 0):
+        self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+        self.index = None if index is None else np.ascontiguousarray(index)
+        self.indirect = None if indirect is None else np.ascontiguousarray(indirect)
+        self.roots = [np.ascontiguousarray(r, dtype=np.uint8) for r in roots]
+        self.face_mask = None if face_mask is None else np.ascontiguousarray(face_mask, dtype=np.uint8)
+        self._root_ptrs = (C.c_void_p * len(self.roots))(*[r.ctypes.data for r in self.roots])
+        self._root_bytes = (C.c_uint64 * len(self.roots))(*[r.nbytes for r in self.roots])
+        m = _Mesh()
+        m.positions = self.positions.ctypes.data
+        m.vertex_count = self.positions.size // stride
+        m.stride, m.offset = stride, offset
+        if self.index is not None:
+            m.index, m.index_elt_bytes, m.index_count = self.index.ctypes.data, self.index.itemsize, self.index.size
+        if self.indirect is not None:
+            m.indirect, m.indirect_elt_bytes = self.indirect.ctypes.data, self.indirect.itemsize
+            m.indirect_count = self.indirect.size
+        m.roots = C.cast(self._root_ptrs, C.c_void_p)
+        m.root_bytes = C.cast(self._root_bytes, C.c_void_p)
+        m.n_roots = len(self.roots)
+        if self.face_mask is not None:
+            m.face_mask = self.face_mask.ctypes.data
+        self._m = m
+
+    @property
+    def ref(self):
+        return C.byref(self._m)
+
+    def raycast_first(self, origins, dirs, side=FRONT, near=0.0, far=float("inf"), brute=False,
+                      threads: Optional[int] = None):
+        o = np.ascontiguousarray(origins, dtype=np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        n = o.shape[0]
+        out = {"face": np.empty(n, np.uint32), "distance": np.empty(n), "point": np.empty((n, 3)),
+               "barycoord": np.empty((n, 3)), "nodes": np.empty(n, np.uint32), "tris": np.empty(n, np.uint32)}
+        library().orc_raycast_first(self.ref, _p(o), _p(d), n, side, near, far, int(brute),
+                                    threads or os.cpu_count() or 1, _p(out["face"]), _p(out["distance"]),
+                                    _p(out["point"]), _p(out["barycoord"]), _p(out["nodes"]), _p(out["tris"]))
+        return out
+
+    def raycast_all(self, origin, direction, side=FRONT, near=0.0, far=float("inf"), cap=4096):
+        o = np.ascontiguousarray(origin, dtype=np.float32).reshape(3)
+        d = np.ascontiguousarray(direction, dtype=np.float32).reshape(3)
+        face = np.empty(cap, np.uint32)
+        dist = np.empty(cap)
+        n = library().orc_raycast_all(self.ref, _p(o), _p(d), side, near, far, cap, _p(face), _p(dist))
+        n = min(int(n), cap)
+        return face[:n], dist[:n]
+
+    def closest_point(self, points, min_threshold=0.0, max_threshold=float("inf"), brute=False,
+                      threads: Optional[int] = None):
+        q = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        n = q.shape[0]
+        out = {"face": np.empty(n, np.uint32), "distance": np.empty(n), "point": np.empty((n, 3)),
+               "nodes": np.empty(n, np.uint32), "tris": np.empty(n, np.uint32)}
+        library().orc_closest_point(self.ref, _p(q), n, min_threshold, max_threshold, int(brute),
+                                    threads or os.cpu_count() or 1, _p(out["face"]), _p(out["distance"]),
+                                    _p(out["point"]), _p(out["nodes"]), _p(out["tris"]))
+        return out
+
+    def ray_plane_distance(self, face, origin, direction):
+        o = np.ascontiguousarray(origin, dtype=np.float32).reshape(3)
+        d = np.ascontiguousarray(direction, dtype=np.float32).reshape(3)
+        return library().orc_ray_plane_distance(self.ref, int(face), _p(o), _p(d))
+
+    def point_face_distance(self, face, p):
+        q = np.ascontiguousarray(p, dtype=np.float32).reshape(3)
+        return library().orc_point_face_distance(self.ref, int(face), _p(q))
+
+    def validate_root(self, root: int = 0):
+        st = TreeStats()
+        rc = library().orc_validate_root(self.ref, root, C.byref(st))
+        return rc, st.as_dict()

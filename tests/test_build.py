@@ -1,0 +1,141 @@
+"""Host BVH build (libmbvh.so mbvh_build) in the reference's packed format: invariants, a hand-derived
+SAH known answer, determinism, ranges, indirect builds — checked by the oracle's validateBounds."""
+import numpy as np
+import pytest
+
+import mbvh_amd as M
+from mbvh_amd import geometry as G
+import oracle_py as O
+
+GEOMS = {
+    "torus_knot": lambda: G.torus_knot(10, 3, 64, 16),
+    "torus": lambda: G.torus(1, 1, 40, 10),
+    "sphere": lambda: G.sphere(1, 32, 16),
+    "ico": lambda: G.icosahedron(1, 4).ensure_index(),
+    "box": lambda: G.box(),
+}
+
+
+def node_records(buf):
+    f = buf.view(np.float32).reshape(-1, 8)
+    u = buf.view(np.uint32).reshape(-1, 8)
+    h = buf.view(np.uint16).reshape(-1, 16)
+    return f, u, h
+
+
+@pytest.mark.parametrize("name", sorted(GEOMS))
+@pytest.mark.parametrize("strategy", [M.CENTER, M.AVERAGE, M.SAH])
+def test_invariants(name, strategy):
+    g = GEOMS[name]()
+    idx = g.index.copy()
+    tree = M.build(g.positions, idx, strategy=strategy)
+    assert len(tree.roots) == 1
+    om = O.OracleMesh(g.positions, idx, tree.roots)
+    rc, st = om.validate_root(0)
+    assert rc == 0, st
+    assert st["tri_count"] == g.tri_count
+    assert st["node_count"] == 2 * st["leaf_count"] - 1
+    assert st["depth_max"] <= 40
+    # the build permutes whole triangles of the index, nothing else
+    before = sorted(map(tuple, g.index.reshape(-1, 3).tolist()))
+    after = sorted(map(tuple, idx.reshape(-1, 3).tolist()))
+    assert before == after
+    # leaves only exceed the target size when the split heuristic declined to split
+    if strategy == M.SAH and name != "box":
+        assert st["leaf_tris_max"] <= 10
+
+
+def test_deterministic_and_layout():
+    g = G.torus_knot(10, 3, 64, 16)
+    i1, i2 = g.index.copy(), g.index.copy()
+    t1 = M.build(g.positions, i1)
+    t2 = M.build(g.positions, i2)
+    assert np.array_equal(t1.roots[0], t2.roots[0]) and np.array_equal(i1, i2)
+    f, u, h = node_records(t1.roots[0])
+    n = f.shape[0]
+    leaf = h[:, 15] == 0xFFFF
+    # internal nodes: left child follows the parent, right child is a forward relative offset
+    for k in np.nonzero(~leaf)[0]:
+        assert u[k, 7] in (0, 1, 2)
+        right = k + u[k, 6]
+        assert k + 1 < n and k < right < n
+
+
+def test_sah_known_answer_row():
+    """16 congruent triangles in a row along x: SAH cost 1 + 1.25 (k^2 + (16-k)^2)/16^2-ish is
+    minimal at k = 8, so the root splits 8 | 8 on axis 0 and both halves (8 <= 10) are leaves."""
+    pos, ind = [], []
+    for i in range(16):
+        base = len(pos)
+        pos += [[i, 0, 0], [i + 0.5, 0, 0], [i, 0.5, 0]]
+        ind += [base, base + 1, base + 2]
+    pos = np.array(pos, np.float32)
+    idx = np.array(ind, np.uint16)
+    tree = M.build(pos, idx, strategy=M.SAH)
+    f, u, h = node_records(tree.roots[0])
+    assert f.shape[0] == 3
+    assert h[0, 15] != 0xFFFF and u[0, 7] == 0 and u[0, 6] == 2
+    assert h[1, 15] == 0xFFFF and u[1, 6] == 0 and h[1, 14] == 8
+    assert h[2, 15] == 0xFFFF and u[2, 6] == 8 and h[2, 14] == 8
+    assert np.array_equal(idx, np.array(ind, np.uint16))  # already partitioned: no swaps
+    # root bounds: union of the triangles, widened by the float32-epsilon inflation
+    assert f[0, 0] <= 0 and f[0, 3] >= 15.5 and f[0, 3] < 15.5 + 1e-5
+
+
+def test_max_depth():
+    g = G.torus_knot(10, 3, 64, 16)
+    idx = g.index.copy()
+    tree = M.build(g.positions, idx, max_depth=3)
+    rc, st = O.OracleMesh(g.positions, idx, tree.roots).validate_root(0)
+    assert rc == 0 and st["depth_max"] <= 3 and st["leaf_count"] <= 8
+
+
+def test_ranges_make_roots():
+    g = G.torus(1, 1, 40, 10)
+    ntri = g.tri_count
+    ranges = [(k * ntri // 10, ntri // 10) for k in range(10)]
+    idx = g.index.copy()
+    tree = M.build(g.positions, idx, ranges=ranges)
+    assert len(tree.roots) == 10
+    om = O.OracleMesh(g.positions, idx, tree.roots)
+    orig = g.index.reshape(-1, 3)
+    perm = idx.reshape(-1, 3)
+    for r, (off, cnt) in enumerate(ranges):
+        rc, st = om.validate_root(r)
+        assert rc == 0 and st["tri_count"] == cnt
+        # triangles never leave their range (group membership is preserved)
+        assert sorted(map(tuple, orig[off:off + cnt].tolist())) == sorted(map(tuple, perm[off:off + cnt].tolist()))
+
+
+def test_indirect_build_keeps_index():
+    g = G.torus(1, 1, 40, 10)
+    ntri = g.tri_count
+    ranges = [(0, ntri // 3), (ntri // 2, ntri // 4)]
+    idx = g.index.copy()
+    tree = M.build(g.positions, idx, ranges=ranges, indirect=True)
+    assert np.array_equal(idx, g.index)
+    assert tree.indirect.dtype == np.uint16  # last range ends below 2^16
+    want = list(range(0, ntri // 3)) + list(range(ntri // 2, ntri // 2 + ntri // 4))
+    assert sorted(tree.indirect.tolist()) == want
+    rc, st = O.OracleMesh(g.positions, idx, tree.roots, indirect=tree.indirect).validate_root(0)
+    assert rc == 0 and st["tri_count"] == len(want)
+    # non-indexed geometry builds indirect without an index
+    g2 = G.icosahedron(1, 3)
+    t2 = M.build(g2.positions, None, indirect=True)
+    rc, st = O.OracleMesh(g2.positions, None, t2.roots, indirect=t2.indirect).validate_root(0)
+    assert rc == 0 and st["tri_count"] == g2.tri_count
+
+
+def test_build_errors():
+    g = G.torus(1, 1, 8, 4)
+    with pytest.raises(M.MbvhError) as e:
+        M.build(g.positions, None)  # non-indirect needs an index
+    assert e.value.code == -4
+    with pytest.raises(M.MbvhError):
+        M.build(g.positions, g.index.copy(), ranges=[(0, g.tri_count + 1)])
+    with pytest.raises(M.MbvhError):
+        M.build(g.positions, g.index.copy(), strategy=7)
+    bad = g.index.copy()
+    bad[0] = 60000
+    with pytest.raises(M.MbvhError):
+        M.build(g.positions, bad)

@@ -1,0 +1,115 @@
+"""GPU parity of closestPointToPoint against the FP64 oracle (reference traversal), via the C ABI."""
+import numpy as np
+import pytest
+
+import mbvh_amd as M
+from mbvh_amd import geometry as G, workloads as W
+import oracle_py as O
+from helpers import build_mesh, check_closest_parity
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = M.Context(0)
+    yield c
+    c.close()
+
+
+def _run(ctx, geometry, points, min_t=0.0, max_t=float("inf"), label="", ranges=None, indirect=False):
+    g, idx, tree = build_mesh(geometry, ranges=ranges, indirect=indirect)
+    mesh = ctx.upload(tree, g.positions, idx)
+    got = mesh.closest_point(points, min_t, max_t)
+    om = O.OracleMesh(g.positions, idx, tree.roots, indirect=tree.indirect)
+    ref = om.closest_point(points, min_t, max_t)
+    stats = check_closest_parity(om, points, got, ref, label)
+    print(label, stats, flush=True)
+    mesh.free()
+    return got, ref, stats, om
+
+
+def test_c2_grid(ctx):
+    """Config C2 geometry (TorusKnot(10,3,800,200)) on a 64^3 grid (full parity)."""
+    w = W.c2_closest(res=64)
+    got, ref, stats, om = _run(ctx, w.geometry, w.points, label="C2_64")
+    # barycoords of the closest point reconstruct it
+    assert np.all(np.abs(got["barycoord"].sum(axis=1) - 1) < 1e-5)
+
+
+def test_sphere_kats(ctx):
+    """ShapeCasts.test.js:715-752 restated on SphereGeometry(1, 200, 200)."""
+    g = G.sphere(1, 200, 200)
+    g2, idx, tree = build_mesh(g, strategy=M.CENTER)
+    mesh = ctx.upload(tree, g2.positions, idx)
+    r = mesh.closest_point(np.array([[0, 0, 0], [0, 1, 0]], np.float32))
+    assert 1 - 1e-3 <= r["distance"][0] <= 1
+    assert r["distance"][1] == 0
+    rng = np.random.default_rng(3)
+    v = rng.random((100, 3)) - 0.5
+    length = rng.random(100) * 3
+    v = (v / np.linalg.norm(v, axis=1, keepdims=True) * length[:, None]).astype(np.float32)
+    r = mesh.closest_point(v)
+    expected = np.abs(1 - np.linalg.norm(v.astype(np.float64), axis=1))
+    assert np.all(np.abs(r["distance"] - expected) <= 1e-3)
+    mesh.free()
+
+
+def test_thresholds(ctx):
+    g = G.torus_knot(10, 3, 120, 30)
+    lo, hi = g.bounds()
+    pts = W.grid_points(lo, hi, 24)
+    # maxThreshold prunes boxes; minThreshold stops at the first triangle under it
+    got, ref, _, om = _run(ctx, g, pts, max_t=2.0, label="maxT")
+    g2, idx, tree = build_mesh(g)
+    mesh = ctx.upload(tree, g2.positions, idx)
+    om2 = O.OracleMesh(g2.positions, idx, tree.roots)
+    got = mesh.closest_point(pts, 1.0, float("inf"))
+    ref = om2.closest_point(pts, 1.0, float("inf"))
+    # with a minThreshold the result is the first face (traversal order) under it, or the closest
+    close = ref["distance"] < 1.0
+    assert np.all(got["distance"][close] < 1.0 + 1e-6)
+    far = ~close
+    assert np.all(np.abs(got["distance"][far] - ref["distance"][far]) <= 1e-5 * np.maximum(ref["distance"][far], 1))
+    mesh.free()
+
+
+def test_groups_and_indirect(ctx):
+    g = G.torus(1, 1, 40, 10)
+    ntri = g.tri_count
+    rng = np.random.default_rng(4)
+    pts = (rng.random((4096, 3)) * 6 - 3).astype(np.float32)
+    _run(ctx, g, pts, ranges=[(k * ntri // 5, ntri // 5) for k in range(5)], label="groups")
+    _run(ctx, g, pts, ranges=[(0, ntri // 3), (ntri // 2, ntri // 4)], indirect=True, label="indirect")
+
+
+def test_c2_full_grid_properties(ctx):
+    """Full C2 grid (128^3): oracle parity on a strided sample, size-independent checks on all."""
+    w = W.c2_closest(res=128)
+    g, idx, tree = build_mesh(w.geometry)
+    mesh = ctx.upload(tree, g.positions, idx)
+    got = mesh.closest_point(w.points)
+    assert np.all(got["face"] != M.NO_HIT)
+    # distance equals |q - point| and the point lies in the reported face's plane span
+    d = np.linalg.norm(w.points.astype(np.float64) - got["point"], axis=1)
+    assert np.max(np.abs(d - got["distance"]) / np.maximum(got["distance"], 1)) < 2e-6
+    om = O.OracleMesh(g.positions, idx, tree.roots)
+    sample = np.arange(0, w.points.shape[0], 97)
+    ref = om.closest_point(w.points[sample])
+    sub = {k: v[sample] for k, v in got.items()}
+    stats = check_closest_parity(om, w.points[sample], sub, ref, "C2_128_sample")
+    print("C2_128_sample", stats, flush=True)
+    mesh.free()
+
+
+def test_degenerate_first_candidate(ctx):
+    """A zero-area triangle (two coincident vertices) whose FP32 closest-point formula yields NaN is
+    tested first; it must be ignored (reference: `distSq < best` is false for NaN) and the valid
+    triangle reported. Regression test for the exact-comparison branch with no best yet."""
+    a = [0.0, 0.0, 1.2246468525851679e-16]
+    c = [0.009960341267287731, 0.007236612029373646, -0.15643446147441864]
+    pos = np.array([a, a, c, [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32)
+    geo = G.Geometry(pos, np.array([0, 1, 2, 3, 4, 5], np.uint16))
+    q = np.array([[-2.514983892440796, 0.6441349983215332, -0.7410805225372314]], np.float32)
+    got, ref, stats, om = _run(ctx, geo, np.repeat(q, 64, axis=0), label="degenerate")
+    assert np.all(got["face"] == ref["face"])

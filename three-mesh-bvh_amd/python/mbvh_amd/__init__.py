@@ -1,0 +1,263 @@
+"""ctypes binding of libmbvh.so (the C ABI declared in include/mbvh.h).
+
+The JavaScript package is the drop-in host API; this module gives Python callers (the benchmark,
+the tests) the same entry points over plain numpy arrays and raw device pointers. It never falls
+back to a CPU path: if the library or a GPU is missing, calls raise MbvhError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+CENTER, AVERAGE, SAH = 0, 1, 2
+FRONT_SIDE, BACK_SIDE, DOUBLE_SIDE, SIDE_PER_TRIANGLE = 0, 1, 2, 3
+NO_HIT = 0xFFFFFFFF
+
+_PKG_ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
+LIB_PATH = os.path.join(_PKG_ROOT, "lib", "libmbvh.so")
+
+EXPORTED_SYMBOLS = (
+    "mbvh_abi_version", "mbvh_last_error", "mbvh_device_count",
+    "mbvh_build", "mbvh_tree_info", "mbvh_tree_root_bytes", "mbvh_tree_copy_root",
+    "mbvh_tree_copy_indirect", "mbvh_tree_free",
+    "mbvh_ctx_create", "mbvh_ctx_destroy", "mbvh_ctx_synchronize",
+    "mbvh_mesh_upload", "mbvh_mesh_free", "mbvh_mesh_info", "mbvh_mesh_set_side_mask",
+    "mbvh_raycast_first", "mbvh_raycast_first_d",
+    "mbvh_closest_point", "mbvh_closest_point_d",
+)
+
+
+class MbvhError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"mbvh error {code}: {message}")
+        self.code = code
+
+
+class BuildOpts(C.Structure):
+    _fields_ = [("strategy", C.c_int32), ("max_depth", C.c_int32),
+                ("target_leaf_size", C.c_int32), ("indirect", C.c_int32)]
+
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    """Loads libmbvh.so once. Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MbvhError(-100, f"{LIB_PATH} is missing; build it with __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    vp, u64, u32, i32, f32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32, C.c_float
+    sig = {
+        "mbvh_abi_version": (u32, []),
+        "mbvh_last_error": (C.c_char_p, []),
+        "mbvh_device_count": (C.c_int, []),
+        "mbvh_build": (C.c_int, [vp, u64, u32, u32, vp, i32, u64, vp, u32, C.POINTER(BuildOpts), C.POINTER(vp)]),
+        "mbvh_tree_info": (C.c_int, [vp, C.POINTER(u32), C.POINTER(i32), C.POINTER(u64)]),
+        "mbvh_tree_root_bytes": (C.c_int, [vp, u32, C.POINTER(u64)]),
+        "mbvh_tree_copy_root": (C.c_int, [vp, u32, vp, u64]),
+        "mbvh_tree_copy_indirect": (C.c_int, [vp, vp, u64]),
+        "mbvh_tree_free": (None, [vp]),
+        "mbvh_ctx_create": (C.c_int, [i32, C.POINTER(vp)]),
+        "mbvh_ctx_destroy": (None, [vp]),
+        "mbvh_ctx_synchronize": (C.c_int, [vp]),
+        "mbvh_mesh_upload": (C.c_int, [vp, C.POINTER(vp), C.POINTER(u64), u32, vp, i32, u64, vp, i32, u64,
+                                       vp, u64, u32, u32, C.POINTER(vp)]),
+        "mbvh_mesh_free": (None, [vp]),
+        "mbvh_mesh_info": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(u32)]),
+        "mbvh_mesh_set_side_mask": (C.c_int, [vp, vp, vp, u64]),
+        "mbvh_raycast_first": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp]),
+        "mbvh_raycast_first_d": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp]),
+        "mbvh_closest_point": (C.c_int, [vp, vp, vp, u64, f32, f32, vp, vp, vp, vp]),
+        "mbvh_closest_point_d": (C.c_int, [vp, vp, vp, u64, f32, f32, vp, vp, vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise MbvhError(rc, library().mbvh_last_error().decode("utf-8", "replace"))
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def device_count() -> int:
+    return int(library().mbvh_device_count())
+
+
+@dataclass
+class Tree:
+    """Reference-format build result: MeshBVH._roots and (indirect builds) the indirect buffer."""
+    roots: List[np.ndarray]          # uint8 node buffers, 32 bytes per node
+    indirect: Optional[np.ndarray]   # uint16 / uint32 or None
+
+
+def build(positions: np.ndarray, index: Optional[np.ndarray], ranges: Optional[Sequence] = None,
+          strategy: int = SAH, max_depth: int = 40, target_leaf_size: int = 10,
+          indirect: bool = False, stride: int = 3, offset: int = 0) -> Tree:
+    """Builds the BVH. A non-indirect build permutes `index` IN PLACE (like the reference).
+
+    positions: float32 array with `stride` floats per vertex. ranges: (offset, count) triangle
+    ranges of the roots (defaults to every triangle).
+    """
+    lib = library()
+    pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+    nverts = pos.size // stride
+    if index is not None:
+        if index.dtype not in (np.uint16, np.uint32) or not index.flags.c_contiguous:
+            raise ValueError("index must be a C-contiguous uint16 or uint32 array")
+        ntris = index.size // 3
+    else:
+        ntris = nverts // 3
+    if ranges is None:
+        ranges = [(0, ntris)]
+    rr = np.asarray(ranges, dtype=np.uint32).reshape(-1)
+    opts = BuildOpts(strategy, max_depth, target_leaf_size, 1 if indirect else 0)
+    tree = C.c_void_p()
+    _check(lib.mbvh_build(_ptr(pos), nverts, stride, offset, _ptr(index),
+                          0 if index is None else index.itemsize, 0 if index is None else index.size,
+                          _ptr(rr), len(rr) // 2, C.byref(opts), C.byref(tree)))
+    try:
+        nroots, ieb, icount = C.c_uint32(), C.c_int32(), C.c_uint64()
+        _check(lib.mbvh_tree_info(tree, C.byref(nroots), C.byref(ieb), C.byref(icount)))
+        roots = []
+        for r in range(nroots.value):
+            nbytes = C.c_uint64()
+            _check(lib.mbvh_tree_root_bytes(tree, r, C.byref(nbytes)))
+            buf = np.empty(nbytes.value, dtype=np.uint8)
+            _check(lib.mbvh_tree_copy_root(tree, r, _ptr(buf), nbytes.value))
+            roots.append(buf)
+        ind = None
+        if ieb.value:
+            ind = np.empty(icount.value, dtype=np.uint16 if ieb.value == 2 else np.uint32)
+            _check(lib.mbvh_tree_copy_indirect(tree, _ptr(ind), ind.nbytes))
+        return Tree(roots, ind)
+    finally:
+        lib.mbvh_tree_free(tree)
+
+
+class Context:
+    """One HIP device and stream (mbvh_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = library()
+        h = C.c_void_p()
+        _check(self._lib.mbvh_ctx_create(device, C.byref(h)))
+        self.handle = h
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.mbvh_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self) -> None:
+        _check(self._lib.mbvh_ctx_synchronize(self.handle))
+
+    def upload(self, tree: Tree, positions: np.ndarray, index: Optional[np.ndarray],
+               stride: int = 3, offset: int = 0) -> "Mesh":
+        return Mesh(self, tree, positions, index, stride, offset)
+
+
+class Mesh:
+    """Device-resident BVH (mbvh_mesh)."""
+
+    def __init__(self, ctx: Context, tree: Tree, positions: np.ndarray, index: Optional[np.ndarray],
+                 stride: int = 3, offset: int = 0):
+        lib = ctx._lib
+        self.ctx = ctx
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+        nroots = len(tree.roots)
+        root_ptrs = (C.c_void_p * nroots)(*[r.ctypes.data for r in tree.roots])
+        root_bytes = (C.c_uint64 * nroots)(*[r.nbytes for r in tree.roots])
+        h = C.c_void_p()
+        ind = tree.indirect
+        _check(lib.mbvh_mesh_upload(
+            ctx.handle, root_ptrs, root_bytes, nroots,
+            _ptr(index), 0 if index is None else index.itemsize, 0 if index is None else index.size,
+            _ptr(ind), 0 if ind is None else ind.itemsize, 0 if ind is None else ind.size,
+            _ptr(pos), pos.size // stride, stride, offset, C.byref(h)))
+        self.handle = h
+
+    def info(self):
+        t, v, d = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        _check(self.ctx._lib.mbvh_mesh_info(self.handle, C.byref(t), C.byref(v), C.byref(d)))
+        return {"tri_count": t.value, "vertex_count": v.value, "max_depth": d.value}
+
+    def free(self) -> None:
+        if self.handle:
+            self.ctx._lib.mbvh_mesh_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def set_side_mask(self, mask: np.ndarray) -> None:
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        _check(self.ctx._lib.mbvh_mesh_set_side_mask(self.ctx.handle, self.handle, _ptr(m), m.size))
+
+    # ---- host-pointer queries (synchronous) ----
+    def raycast_first(self, origins: np.ndarray, directions: np.ndarray, side: int = FRONT_SIDE,
+                      near: float = 0.0, far: float = float("inf")):
+        o = np.ascontiguousarray(origins, dtype=np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, dtype=np.float32).reshape(-1, 3)
+        if o.shape != d.shape:
+            raise ValueError("origins and directions must have the same shape")
+        n = o.shape[0]
+        face = np.empty(n, np.uint32)
+        dist = np.empty(n, np.float32)
+        point = np.empty((n, 3), np.float32)
+        bary = np.empty((n, 3), np.float32)
+        _check(self.ctx._lib.mbvh_raycast_first(self.ctx.handle, self.handle, _ptr(o), _ptr(d), n, side,
+                                                near, far, _ptr(face), _ptr(dist), _ptr(point), _ptr(bary)))
+        return {"face": face, "distance": dist, "point": point, "barycoord": bary}
+
+    def closest_point(self, points: np.ndarray, min_threshold: float = 0.0,
+                      max_threshold: float = float("inf")):
+        q = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        n = q.shape[0]
+        face = np.empty(n, np.uint32)
+        dist = np.empty(n, np.float32)
+        point = np.empty((n, 3), np.float32)
+        bary = np.empty((n, 3), np.float32)
+        _check(self.ctx._lib.mbvh_closest_point(self.ctx.handle, self.handle, _ptr(q), n, min_threshold,
+                                                max_threshold, _ptr(face), _ptr(dist), _ptr(point), _ptr(bary)))
+        return {"face": face, "distance": dist, "point": point, "barycoord": bary}
+
+    # ---- device-pointer queries (asynchronous on `stream`) ----
+    def raycast_first_d(self, origins_ptr: int, dirs_ptr: int, n: int, face_ptr: int, dist_ptr: int = 0,
+                        point_ptr: int = 0, bary_ptr: int = 0, side: int = FRONT_SIDE, near: float = 0.0,
+                        far: float = float("inf"), stream: int = 0) -> None:
+        _check(self.ctx._lib.mbvh_raycast_first_d(
+            self.ctx.handle, self.handle, C.c_void_p(origins_ptr), C.c_void_p(dirs_ptr), n, side, near, far,
+            C.c_void_p(face_ptr), C.c_void_p(dist_ptr or None), C.c_void_p(point_ptr or None),
+            C.c_void_p(bary_ptr or None), C.c_void_p(stream or None)))
+
+    def closest_point_d(self, points_ptr: int, n: int, face_ptr: int, dist_ptr: int = 0, point_ptr: int = 0,
+                        bary_ptr: int = 0, min_threshold: float = 0.0, max_threshold: float = float("inf"),
+                        stream: int = 0) -> None:
+        _check(self.ctx._lib.mbvh_closest_point_d(
+            self.ctx.handle, self.handle, C.c_void_p(points_ptr), n, min_threshold, max_threshold,
+            C.c_void_p(face_ptr), C.c_void_p(dist_ptr or None), C.c_void_p(point_ptr or None),
+            C.c_void_p(bary_ptr or None), C.c_void_p(stream or None)))
