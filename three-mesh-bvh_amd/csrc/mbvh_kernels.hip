@@ -1,11 +1,18 @@
 // Wave64 traversal kernels for gfx950: first-hit raycast and closest point.
 //
 // One query per lane. Each lane walks the uploaded tree with a private stack whose first
-// kStackLds entries live in LDS (column per lane, so the 64 lanes of a wave hit 64 distinct banks)
-// and whose deeper entries spill to a global overflow region sized from the tree depth at upload.
-// All arithmetic is FP32; triangles whose FP32 decision lies inside a rigorous error bound are
-// re-decided in double-float (see mbvh_device.h), and the winning triangle of each query is
-// re-evaluated in double-float for the reported distance / point / barycoord.
+// kStackLds entries live in LDS (one column per lane, so a wave's 64 lanes touch 64 distinct
+// banks) and whose deeper entries spill to a global overflow region sized from the tree depth at
+// upload. Traversal is "while-while": lanes at internal nodes step together, then lanes at
+// leaves test triangles together.
+//
+// Exactness. All traversal arithmetic is FP32. Every FP32 decision that could differ from the
+// exact one is bounded: triangle tests fall back to double-float (mbvh_device.h) inside a rigorous
+// error band, and candidate hits are compared through error intervals. Candidates whose interval
+// overlaps the best upper bound go to a short per-lane list (LDS); subtrees are pruned against
+// that bound; the list is resolved in double-float once, after traversal, when all lanes of the
+// wave do it together. The resolution keeps the first candidate in visit order among exactly
+// equal minima, which is the reference's strict-< rule.
 #include "mbvh_kernels.h"
 
 namespace mbvh {
@@ -14,43 +21,40 @@ namespace {
 constexpr float kU = 5.9604644775390625e-08f;  // unit roundoff 2^-24
 constexpr float kSlabEps = 8.0f * kU;          // widening of FP32 slab intervals (>= 3 roundings)
 constexpr uint32_t kNoHit = 0xFFFFFFFFu;
+constexpr uint32_t kOverflow = 0xFFFFFFFEu;    // reported face when a stack would overflow
 constexpr uint32_t kPruneFlag = 0x80000000u;
+constexpr uint32_t kDone = 0x7FFFFFFFu;        // "no more work" traversal reference
+constexpr int kCand = 4;                       // near-tie candidates per lane (LDS)
+constexpr float kInf = __builtin_huge_valf();
 
-__device__ __forceinline__ float lerp_bound_lo(float t) { return t * (t > 0.0f ? 1.0f - kSlabEps : 1.0f + kSlabEps); }
-__device__ __forceinline__ float lerp_bound_hi(float t) { return t * (t > 0.0f ? 1.0f + kSlabEps : 1.0f - kSlabEps); }
+// Per-lane LDS columns: stack entries (ref, aux) and candidates (slot, lower bound bits).
+__shared__ uint2 sh_stack[kStackLds * kBlock];
+__shared__ uint2 sh_cand[kCand * kBlock];
 
-// Stack with an LDS part and a global overflow part.
-struct LaneStack {
-  uint32_t* s_ref;
-  uint32_t* s_aux;
+__device__ __forceinline__ float widen_lo(float t) { return t * (t > 0.0f ? 1.0f - kSlabEps : 1.0f + kSlabEps); }
+__device__ __forceinline__ float widen_hi(float t) { return t * (t > 0.0f ? 1.0f + kSlabEps : 1.0f - kSlabEps); }
+
+struct Stack {
   uint2* ovf;
   uint64_t ovf_stride;
   uint64_t gtid;
   uint32_t lane;
-  uint32_t capacity;  // kStackLds + overflow entries
+  int capacity;  // kStackLds + overflow entries
   int sp;
 
   __device__ __forceinline__ bool push(uint32_t ref, uint32_t aux) {
-    if (sp >= static_cast<int>(capacity)) return false;
-    if (sp < kStackLds) {
-      s_ref[sp * kBlock + lane] = ref;
-      s_aux[sp * kBlock + lane] = aux;
-    } else {
+    if (sp >= capacity) return false;
+    if (sp < kStackLds)
+      sh_stack[sp * kBlock + lane] = make_uint2(ref, aux);
+    else
       ovf[static_cast<uint64_t>(sp - kStackLds) * ovf_stride + gtid] = make_uint2(ref, aux);
-    }
     sp++;
     return true;
   }
-  __device__ __forceinline__ void pop(uint32_t& ref, uint32_t& aux) {
+  __device__ __forceinline__ uint2 pop() {
     sp--;
-    if (sp < kStackLds) {
-      ref = s_ref[sp * kBlock + lane];
-      aux = s_aux[sp * kBlock + lane];
-    } else {
-      const uint2 e = ovf[static_cast<uint64_t>(sp - kStackLds) * ovf_stride + gtid];
-      ref = e.x;
-      aux = e.y;
-    }
+    if (sp < kStackLds) return sh_stack[sp * kBlock + lane];
+    return ovf[static_cast<uint64_t>(sp - kStackLds) * ovf_stride + gtid];
   }
 };
 
@@ -67,16 +71,20 @@ __device__ __forceinline__ void leaf_range(uint32_t ref, const uint2* bigleaf, u
   }
 }
 
-__device__ __forceinline__ void load_tri(const float4* tris, uint32_t slot, f3& a, f3& b, f3& c,
-                                         uint32_t& face, uint32_t& mask) {
-  const float4 p0 = tris[3u * slot + 0u];
-  const float4 p1 = tris[3u * slot + 1u];
-  const float4 p2 = tris[3u * slot + 2u];
-  a = mk(p0.x, p0.y, p0.z);
-  b = mk(p1.x, p1.y, p1.z);
-  c = mk(p2.x, p2.y, p2.z);
-  face = __float_as_uint(p0.w);
-  mask = __float_as_uint(p1.w);
+struct Tri {
+  float4 p0, p1, p2;  // [a faceId] [b facingMask] [c vertexA]
+  __device__ __forceinline__ f3 a() const { return mk(p0.x, p0.y, p0.z); }
+  __device__ __forceinline__ f3 b() const { return mk(p1.x, p1.y, p1.z); }
+  __device__ __forceinline__ f3 c() const { return mk(p2.x, p2.y, p2.z); }
+  __device__ __forceinline__ uint32_t face() const { return __float_as_uint(p0.w); }
+  __device__ __forceinline__ uint32_t mask() const { return __float_as_uint(p1.w); }
+};
+__device__ __forceinline__ Tri load_tri(const float4* tris, uint32_t slot) {
+  Tri t;
+  t.p0 = tris[3u * slot + 0u];
+  t.p1 = tris[3u * slot + 1u];
+  t.p2 = tris[3u * slot + 2u];
+  return t;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -99,8 +107,8 @@ __device__ __forceinline__ bool slab(const RayBox& r, float mnx, float mny, floa
   const float ty1 = ((r.sy ? mxy : mny) - r.oy) * r.iy;
   const float tz0 = ((r.sz ? mnz : mxz) - r.oz) * r.iz;
   const float tz1 = ((r.sz ? mxz : mnz) - r.oz) * r.iz;
-  const float t0 = lerp_bound_lo(fmaxf(fmaxf(tx0, ty0), tz0));
-  const float t1 = lerp_bound_hi(fminf(fminf(tx1, ty1), tz1));
+  const float t0 = widen_lo(fmaxf(fmaxf(tx0, ty0), tz0));
+  const float t1 = widen_hi(fminf(fminf(tx1, ty1), tz1));
   tmin = t0;
   return (t0 <= t1) && (t0 <= far_t) && (t1 >= near_t);
 }
@@ -139,9 +147,10 @@ __device__ TriTerms tri_terms_exact(f3 o, f3 d, f3 a, f3 b, f3 c) {
   return r;
 }
 
-__device__ __noinline__ bool hit_triangle_exact(f3 o, f3 d, float dlen, f3 a, f3 b, f3 c,
-                                                uint32_t mask, float near_d, float far_d,
-                                                float& t_out, float& et_out) {
+// Exact hit test; on a hit returns true and the ray parameter in double-float.
+// Arguments are passed by value so callers keep them in registers.
+__device__ __noinline__ bool hit_exact(f3 o, f3 d, float dlen, f3 a, f3 b, f3 c, uint32_t mask,
+                                       float near_d, float far_d, ff& t_out) {
   const TriTerms r = tri_terms_exact(o, d, a, b, c);
   if (!r.ok) return false;
   if (!(mask & (r.back ? 2u : 1u))) return false;
@@ -149,58 +158,100 @@ __device__ __noinline__ bool hit_triangle_exact(f3 o, f3 d, float dlen, f3 a, f3
   if (ff_sign(r.e1) < 0) return false;
   if (ff_sign(ff_sub(r.ddn_abs, ff_add(r.e0, r.e1))) < 0) return false;
   if (ff_sign(r.qdn) < 0) return false;
-  const float t = ff_to_float(ff_div(r.qdn, r.ddn_abs));
-  const float dist = t * dlen;
+  const ff t = ff_div(r.qdn, r.ddn_abs);
+  const float dist = ff_to_float(t) * dlen;
   if (dist < near_d || dist > far_d) return false;
   t_out = t;
-  et_out = 2.0f * kU * t;
   return true;
 }
 
-// t of a triangle the ray is known to hit, in double-float (for near-tie comparisons).
-__device__ __noinline__ ff exact_hit_t(f3 o, f3 d, const float4* tris, uint32_t slot) {
-  const float4 p0 = tris[3u * slot + 0u];
-  const float4 p1 = tris[3u * slot + 1u];
-  const float4 p2 = tris[3u * slot + 2u];
-  const TriTerms r = tri_terms_exact(o, d, mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z));
-  return ff_div(r.qdn, r.ddn_abs);
-}
+enum : int { kMiss = 0, kHit = 1, kUnsure = 2 };
 
-// On a hit, t_out is the FP32 ray parameter and et_out a bound on its error.
-__device__ __forceinline__ bool hit_triangle(f3 o, f3 d, float dl1, float dlen, f3 a, f3 b, f3 c,
-                                             uint32_t mask, float near_d, float far_d,
-                                             float& t_out, float& et_out) {
+// FP32 test. kHit comes with t and a bound et on its error; kUnsure means the FP32 decision lies
+// inside its error band and must be made exactly.
+__device__ __forceinline__ int hit_fast(f3 o, f3 d, float dl1, float dlen, const Tri& tr,
+                                        uint32_t mask, float near_d, float far_d, float& t_out,
+                                        float& et_out) {
+  const f3 a = tr.a(), b = tr.b(), c = tr.c();
   const f3 e1 = sub(b, a), e2 = sub(c, a);
   const f3 n = cross(e1, e2);
   const float ddn = dot(d, n);
   const float mE = fmaxf(maxabs(e1), maxabs(e2));
   const float bN = 32.0f * kU * dl1 * mE * mE;
-  if (!(fabsf(ddn) > bN)) return hit_triangle_exact(o, d, dlen, a, b, c, mask, near_d, far_d, t_out, et_out);
+  if (!(fabsf(ddn) > bN)) return kUnsure;
   const bool back = ddn > 0.0f;
-  if (!(mask & (back ? 2u : 1u))) return false;
+  if (!(mask & (back ? 2u : 1u))) return kMiss;
   const float s = back ? 1.0f : -1.0f;
   const float DdN = fabsf(ddn);
   const f3 q = sub(o, a);
   const float mQ = maxabs(q);
   const float bE = 32.0f * kU * dl1 * mQ * mE;
   const float e0 = s * dot(d, cross(q, e2));
-  if (e0 < -bE) return false;
+  if (e0 < -bE) return kMiss;
   const float f1 = s * dot(d, cross(e1, q));
-  if (f1 < -bE) return false;
+  if (f1 < -bE) return kMiss;
   const float g = DdN - e0 - f1;
   const float bS = bN + 2.0f * bE + 4.0f * kU * (DdN + fabsf(e0) + fabsf(f1));
-  if (g < -bS) return false;
+  if (g < -bS) return kMiss;
   const float qdn = -s * dot(q, n);
   const float bQ = 64.0f * kU * mQ * mE * mE;
-  if (qdn < -bQ) return false;
-  if (!(e0 > bE && f1 > bE && g > bS && qdn > bQ))
-    return hit_triangle_exact(o, d, dlen, a, b, c, mask, near_d, far_d, t_out, et_out);
+  if (qdn < -bQ) return kMiss;
+  if (!(e0 > bE && f1 > bE && g > bS && qdn > bQ)) return kUnsure;
   const float t = qdn / DdN;
   const float dist = t * dlen;
-  if (dist < near_d || dist > far_d) return false;
+  if (dist < near_d || dist > far_d) return kMiss;
   t_out = t;
   et_out = t * (bQ / qdn + bN / DdN + 4.0f * kU);
-  return true;
+  return kHit;
+}
+
+// ------------------------------------------------------------------------------------------
+// Candidate list (per lane, in visit order). `lb` is a lower bound of the candidate's exact key
+// (ray parameter or squared distance); -inf marks a candidate whose hit/miss is undecided. A
+// candidate whose lb exceeds the best upper bound U can never win and is dropped.
+// ------------------------------------------------------------------------------------------
+struct Cands {
+  uint32_t lane;
+  int n;
+  __device__ __forceinline__ void put(int i, uint32_t slot, float lb) {
+    sh_cand[i * kBlock + lane] = make_uint2(slot, __float_as_uint(lb));
+  }
+  __device__ __forceinline__ uint2 get(int i) const { return sh_cand[i * kBlock + lane]; }
+  // Drops candidates with lb > U, keeping order. Returns true if a slot is free afterwards.
+  __device__ __forceinline__ bool compact(float U) {
+    int m = 0;
+    for (int i = 0; i < n; i++) {
+      const uint2 e = get(i);
+      if (__uint_as_float(e.y) <= U || !(__uint_as_float(e.y) == __uint_as_float(e.y))) {
+        if (m != i) put(m, e.x, __uint_as_float(e.y));
+        m++;
+      }
+    }
+    n = m;
+    return n < kCand;
+  }
+};
+
+// Exact resolution for the raycast: the first candidate (visit order) with the smallest exact t.
+__device__ __noinline__ uint32_t resolve_ray(Cands cl, float U, f3 o, f3 d, float dlen,
+                                             const float4* tris, uint32_t side_mask, float near_d,
+                                             float far_d, ff& best_t) {
+  uint32_t win = kNoHit;
+  best_t = {kInf, 0.0f};
+  for (int i = 0; i < cl.n; i++) {
+    const uint2 e = cl.get(i);
+    const float lb = __uint_as_float(e.y);
+    if (lb > U) continue;
+    const Tri tr = load_tri(tris, e.x);
+    const uint32_t mask = side_mask ? side_mask : tr.mask();
+    ff t;
+    if (!hit_exact(o, d, dlen, tr.a(), tr.b(), tr.c(), mask, near_d, far_d, t)) continue;
+    if (win == kNoHit || ff_sign(ff_sub(t, best_t)) < 0) {
+      win = e.x;
+      best_t = t;
+    }
+  }
+  return win;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -209,107 +260,63 @@ __device__ __forceinline__ bool hit_triangle(f3 o, f3 d, float dl1, float dlen, 
 // comes from (dir[splitAxis] >= 0 ⇒ left) is finished first; inside a leaf the first closest
 // triangle wins (strict <, iterationUtils.template.js:24-53); across children and roots the
 // earlier-visited hit wins ties. The reference has no distance pruning; this kernel skips
-// subtrees whose entry distance already exceeds the best hit, which cannot change that result.
-// Its split-plane shortcut (raycastFirst.template.js:69-84) is implied by the distance pruning
-// whenever dir[splitAxis] != 0. Rays with a zero direction component are traced without distance
-// pruning and with that shortcut applied literally: on such an axis the near child's hit point
-// lies in the ray's own plane, so the shortcut fires iff the near subtree found any hit and
-// origin[axis] <= far.min[axis].
+// subtrees whose entry distance already exceeds the best hit's upper bound, which cannot change
+// that result. Its split-plane shortcut (raycastFirst.template.js:69-84) is implied by the
+// distance pruning whenever dir[splitAxis] != 0. Rays with a zero direction component are traced
+// without distance pruning and with that shortcut applied literally: on such an axis the near
+// child's hit point lies in the ray's own plane, so the shortcut fires iff the near subtree found
+// any hit and origin[axis] <= far.min[axis].
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_raycast_first(RaycastFirstArgs A) {
-  __shared__ uint32_t s_ref[kStackLds * kBlock];
-  __shared__ uint32_t s_aux[kStackLds * kBlock];
-
   const uint32_t lane = threadIdx.x;
   const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + lane;
   const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * kBlock;
+  const MeshView M = A.mesh;
 
-  LaneStack st;
-  st.s_ref = s_ref;
-  st.s_aux = s_aux;
+  Stack st;
   st.ovf = A.stack.ovf;
   st.ovf_stride = A.stack.ovf_stride;
   st.gtid = gtid;
   st.lane = lane;
-  st.capacity = kStackLds + A.stack.ovf_entries;
-
-  const MeshView M = A.mesh;
+  st.capacity = kStackLds + static_cast<int>(A.stack.ovf_entries);
 
   for (uint64_t ray = gtid; ray < A.n; ray += gstride) {
     const f3 o = mk(A.origins[3 * ray + 0], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
     const f3 d = mk(A.directions[3 * ray + 0], A.directions[3 * ray + 1], A.directions[3 * ray + 2]);
     RayBox rb;
-    rb.ox = o.x; rb.oy = o.y; rb.oz = o.z;
-    rb.ix = 1.0f / d.x; rb.iy = 1.0f / d.y; rb.iz = 1.0f / d.z;
-    rb.sx = rb.ix >= 0.0f; rb.sy = rb.iy >= 0.0f; rb.sz = rb.iz >= 0.0f;
+    rb.ox = o.x;
+    rb.oy = o.y;
+    rb.oz = o.z;
+    rb.ix = 1.0f / d.x;
+    rb.iy = 1.0f / d.y;
+    rb.iz = 1.0f / d.z;
+    rb.sx = rb.ix >= 0.0f;
+    rb.sy = rb.iy >= 0.0f;
+    rb.sz = rb.iz >= 0.0f;
     const float dlen = sqrtf(dot(d, d));
     const float dl1 = fabsf(d.x) + fabsf(d.y) + fabsf(d.z);
     const bool literal = (d.x == 0.0f) || (d.y == 0.0f) || (d.z == 0.0f);
 
-    float best_t = __int_as_float(0x7f800000);  // +inf
-    float best_et = 0.0f;                       // error bound of best_t
-    ff best_ff = {0.0f, 0.0f};                  // best_t in double-float when best_ff_ok
-    bool best_ff_ok = false;
-    uint32_t best_slot = kNoHit;
+    float U = kInf;  // upper bound of the best hit's t
     uint32_t hits = 0;
     bool overflow = false;
+    Cands cl;
+    cl.lane = lane;
+    cl.n = 0;
 
     for (uint32_t r = 0; r < M.n_roots && !overflow; r++) {
       uint32_t ref = M.roots[r];
       st.sp = 0;
       while (true) {
-        if (ref & kRefLeaf) {
-          uint32_t off, cnt;
-          leaf_range(ref, M.bigleaf, off, cnt);
-          for (uint32_t s = off; s < off + cnt; s++) {
-            f3 a, b, c;
-            uint32_t face, tmask;
-            load_tri(M.tris, s, a, b, c, face, tmask);
-            const uint32_t mask = A.side_mask ? A.side_mask : tmask;
-            float t, et;
-            if (hit_triangle(o, d, dl1, dlen, a, b, c, mask, A.near_dist, A.far_dist, t, et)) {
-              hits++;
-              // Strict < as in intersectClosestTri; when the FP32 values are within their error
-              // bands the order is decided in double-float.
-              bool better;
-              bool exact = false;
-              ff t_ff;
-              if (t + et < best_t - best_et) {
-                better = true;
-              } else if (t - et > best_t + best_et) {
-                better = false;
-              } else if (best_slot == kNoHit) {
-                better = true;  // any hit beats none (reached only when t or et is not finite)
-              } else {
-                t_ff = exact_hit_t(o, d, M.tris, s);
-                if (!best_ff_ok) best_ff = exact_hit_t(o, d, M.tris, best_slot);
-                better = ff_sign(ff_sub(t_ff, best_ff)) < 0;
-                exact = true;
-                best_ff_ok = true;
-              }
-              if (better) {
-                best_slot = s;
-                best_ff_ok = exact;
-                if (exact) {
-                  best_ff = t_ff;
-                  best_t = ff_to_float(t_ff);
-                  best_et = fmaxf(et, 2.0f * kU * best_t);
-                } else {
-                  best_t = t;
-                  best_et = et;
-                }
-              }
-            }
-          }
-        } else {
+        // ---- internal nodes ----
+        while (!(ref & kRefLeaf) && ref != kDone) {
           const WideNode nd = M.nodes[ref];
           float tl, tr;
           bool hl = slab(rb, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y, A.near_dist, A.far_dist, tl);
           bool hr = slab(rb, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w, A.near_dist, A.far_dist, tr);
-          const float t_reach = best_t + best_et;  // no hit in a box entered beyond this can win
           if (!literal) {
-            hl = hl && (tl <= t_reach);
-            hr = hr && (tr <= t_reach);
+            hl = hl && (tl <= U);
+            hr = hr && (tr <= U);
           }
           const uint32_t axis = nd.meta.z;
           const float daxis = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
@@ -323,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_raycast_first(RaycastFirstArgs A) {
             if (!literal) {
               aux = __float_as_uint(l2r ? tr : tl);
             } else {
-              // Far child is the right one whenever daxis is +-0 (l2r holds).
+              // daxis == +-0 means l2r, so the far child is the right one.
               const float oaxis = axis == 0 ? o.x : (axis == 1 ? o.y : o.z);
               const float fmin = axis == 0 ? nd.b1.z : (axis == 1 ? nd.b1.w : nd.b2.x);
               const bool prunable = (daxis == 0.0f) && (oaxis <= fmin);
@@ -331,65 +338,106 @@ __global__ __launch_bounds__(kBlock) void k_raycast_first(RaycastFirstArgs A) {
             }
             if (!st.push(c2, aux)) {
               overflow = true;
+              ref = kDone;
               break;
             }
             ref = c1;
-            continue;
-          }
-          if (h1) {
+          } else if (h1) {
             ref = c1;
-            continue;
-          }
-          if (h2) {
+          } else if (h2) {
             ref = c2;
-            continue;
+          } else {
+            ref = kDone;
+            while (st.sp > 0) {
+              const uint2 e = st.pop();
+              if (!literal ? (__uint_as_float(e.y) <= U)
+                           : !((e.y & kPruneFlag) && hits > (e.y & ~kPruneFlag))) {
+                ref = e.x;
+                break;
+              }
+            }
           }
         }
-        // pop the next pending subtree that can still matter
-        bool found = false;
-        while (st.sp > 0) {
-          uint32_t aux;
-          st.pop(ref, aux);
-          if (!literal) {
-            if (__uint_as_float(aux) <= best_t + best_et) {
-              found = true;
+        // ---- leaves ----
+        while (ref & kRefLeaf) {
+          uint32_t off, cnt;
+          leaf_range(ref, M.bigleaf, off, cnt);
+          Tri next;
+          if (cnt != 0) next = load_tri(M.tris, off);  // an empty leaf may sit at the slot count
+          for (uint32_t s = off; s < off + cnt; s++) {
+            const Tri tr = next;
+            if (s + 1 < off + cnt) next = load_tri(M.tris, s + 1);
+            const uint32_t mask = A.side_mask ? A.side_mask : tr.mask();
+            float t = 0.0f, et = 0.0f;
+            int res = hit_fast(o, d, dl1, dlen, tr, mask, A.near_dist, A.far_dist, t, et);
+            if (res == kUnsure && literal) {
+              // the literal shortcut counts hits, so an undecided hit is decided now
+              ff te;
+              res = hit_exact(o, d, dlen, tr.a(), tr.b(), tr.c(), mask, A.near_dist, A.far_dist, te) ? kHit : kMiss;
+              t = ff_to_float(te);
+              et = 2.0f * kU * t;
+            }
+            if (res == kMiss) continue;
+            if (res == kHit) {
+              hits++;
+              const float lb = t - et;
+              if (lb > U) continue;  // certainly behind a known hit
+              U = fminf(U, t + et);
+            }
+            if (cl.n == kCand && !cl.compact(U)) {
+              // full of live candidates: settle them now and keep the winner
+              ff bt;
+              const uint32_t w = resolve_ray(cl, U, o, d, dlen, M.tris, A.side_mask, A.near_dist, A.far_dist, bt);
+              cl.n = 0;
+              if (w != kNoHit) {
+                const float tw = ff_to_float(bt);
+                U = fminf(U, tw * (1.0f + 4.0f * kU));
+                cl.put(0, w, tw * (1.0f - 4.0f * kU));
+                cl.n = 1;
+              }
+            }
+            cl.put(cl.n++, s, res == kHit ? t - et : -kInf);
+          }
+          ref = kDone;
+          while (st.sp > 0) {
+            const uint2 e = st.pop();
+            if (!literal ? (__uint_as_float(e.y) <= U)
+                         : !((e.y & kPruneFlag) && hits > (e.y & ~kPruneFlag))) {
+              ref = e.x;
               break;
             }
-          } else {
-            if ((aux & kPruneFlag) && hits > (aux & ~kPruneFlag)) continue;
-            found = true;
-            break;
           }
         }
-        if (!found) break;
+        if (ref == kDone) break;
       }
     }
 
     uint32_t out_face = kNoHit;
-    float out_dist = __int_as_float(0x7f800000);
+    float out_dist = kInf;
     float px = __int_as_float(0x7fc00000), py = px, pz = px;
     float bw = px, bu = px, bv = px;
     if (overflow) {
-      // A tree deeper than the stack sized at upload; cannot happen for a validated mesh.
-      out_face = kNoHit - 1u;
-    } else if (best_slot != kNoHit) {
-      f3 a, b, c;
-      uint32_t face, tmask;
-      load_tri(M.tris, best_slot, a, b, c, face, tmask);
-      const TriTerms r = tri_terms_exact(o, d, a, b, c);
-      const ff t = ff_div(r.qdn, r.ddn_abs);
-      const ff u = ff_div(r.e0, r.ddn_abs);
-      const ff v = ff_div(r.e1, r.ddn_abs);
-      const ff w = ff_sub(ff_sub(ff_from(1.0f), u), v);
-      const ff dl = ff_sqrt(ff_add(ff_add(two_prod(d.x, d.x), two_prod(d.y, d.y)), two_prod(d.z, d.z)));
-      out_face = face;
-      out_dist = ff_to_float(ff_mul(t, dl));
-      px = ff_to_float(ff_add(ff_from(o.x), ff_mulf(t, d.x)));
-      py = ff_to_float(ff_add(ff_from(o.y), ff_mulf(t, d.y)));
-      pz = ff_to_float(ff_add(ff_from(o.z), ff_mulf(t, d.z)));
-      bw = ff_to_float(w);
-      bu = ff_to_float(u);
-      bv = ff_to_float(v);
+      out_face = kOverflow;
+    } else if (cl.n > 0) {
+      ff bt;
+      const uint32_t w = resolve_ray(cl, U, o, d, dlen, M.tris, A.side_mask, A.near_dist, A.far_dist, bt);
+      if (w != kNoHit) {
+        const Tri tr = load_tri(M.tris, w);
+        const TriTerms r = tri_terms_exact(o, d, tr.a(), tr.b(), tr.c());
+        const ff t = ff_div(r.qdn, r.ddn_abs);
+        const ff u = ff_div(r.e0, r.ddn_abs);
+        const ff v = ff_div(r.e1, r.ddn_abs);
+        const ff wgt = ff_sub(ff_sub(ff_from(1.0f), u), v);
+        const ff dl = ff_sqrt(ff_add(ff_add(two_prod(d.x, d.x), two_prod(d.y, d.y)), two_prod(d.z, d.z)));
+        out_face = tr.face();
+        out_dist = ff_to_float(ff_mul(t, dl));
+        px = ff_to_float(ff_add(ff_from(o.x), ff_mulf(t, d.x)));
+        py = ff_to_float(ff_add(ff_from(o.y), ff_mulf(t, d.y)));
+        pz = ff_to_float(ff_add(ff_from(o.z), ff_mulf(t, d.z)));
+        bw = ff_to_float(wgt);
+        bu = ff_to_float(u);
+        bv = ff_to_float(v);
+      }
     }
     A.out_face[ray] = out_face;
     if (A.out_dist) A.out_dist[ray] = out_dist;
@@ -449,22 +497,21 @@ struct ClosestFF {
   ff dist2;
 };
 
-__device__ __forceinline__ ff ff_dot3(const ff3& x, const ff3& y) { return ff3_dot(x, y); }
 __device__ __forceinline__ bool ff_le0(ff x) { return ff_sign(x) <= 0; }
 __device__ __forceinline__ bool ff_ge0(ff x) { return ff_sign(x) >= 0; }
 
 __device__ __noinline__ ClosestFF closest_on_triangle_ff(f3 p, f3 a, f3 b, f3 c) {
   const ff3 ab = ff3_diff(b, a), ac = ff3_diff(c, a), ap = ff3_diff(p, a);
   const ff one = ff_from(1.0f), zero = ff_from(0.0f);
-  const ff d1 = ff_dot3(ab, ap), d2 = ff_dot3(ac, ap);
+  const ff d1 = ff3_dot(ab, ap), d2 = ff3_dot(ac, ap);
   ClosestFF r;
   int region;  // 0 face, 1 a, 2 b, 3 c, 4 ab, 5 ac, 6 bc
   ff s = zero;  // edge parameter
   ff vb = zero, vc = zero, va = zero;
   const ff3 bp = ff3_diff(p, b);
-  const ff d3 = ff_dot3(ab, bp), d4 = ff_dot3(ac, bp);
+  const ff d3 = ff3_dot(ab, bp), d4 = ff3_dot(ac, bp);
   const ff3 cpv = ff3_diff(p, c);
-  const ff d5 = ff_dot3(ab, cpv), d6 = ff_dot3(ac, cpv);
+  const ff d5 = ff3_dot(ab, cpv), d6 = ff3_dot(ac, cpv);
   if (ff_le0(d1) && ff_le0(d2)) {
     region = 1;
   } else if (ff_ge0(d3) && ff_le0(ff_sub(d4, d3))) {
@@ -543,111 +590,70 @@ __device__ __forceinline__ float box_score(f3 q, float mnx, float mny, float mnz
   return dx * dx + dy * dy + dz * dz;
 }
 
+// Exact resolution for the closest point: the first candidate (visit order) at the smallest
+// exact squared distance.
+__device__ __noinline__ uint32_t resolve_point(Cands cl, float U, f3 q, const float4* tris,
+                                               ff& best_d2) {
+  uint32_t win = kNoHit;
+  best_d2 = {kInf, 0.0f};
+  for (int i = 0; i < cl.n; i++) {
+    const uint2 e = cl.get(i);
+    if (__uint_as_float(e.y) > U) continue;
+    const Tri tr = load_tri(tris, e.x);
+    const ff d2 = closest_on_triangle_ff(q, tr.a(), tr.b(), tr.c()).dist2;
+    if (!(d2.hi == d2.hi)) continue;  // degenerate: never wins (reference: `NaN < best` is false)
+    if (win == kNoHit || ff_sign(ff_sub(d2, best_d2)) < 0) {
+      win = e.x;
+      best_d2 = d2;
+    }
+  }
+  return win;
+}
+
 // ------------------------------------------------------------------------------------------
 // Closest point. Traversal of closestPointToPoint over shapecast
 // (src/core/cast/closestPointToPoint.js:6-78, src/core/cast/shapecast.js:41-214): children
 // ordered by squared box distance (right first only if strictly closer), a child is entered iff
-// score < best^2 and score < maxThreshold^2 (the far child's score is re-checked against the
-// best found meanwhile), root nodes are never bounds-tested, triangles update the best on strict
-// <, and the first triangle closer than minThreshold ends the whole query (all roots).
+// its score is below the best distance^2 (here its upper bound) and below maxThreshold^2 (the far
+// child's score is re-checked against the best found meanwhile), root nodes are never
+// bounds-tested, triangles update the best on strict <, and the first triangle closer than
+// minThreshold ends the whole query (all roots).
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
-  __shared__ uint32_t s_ref[kStackLds * kBlock];
-  __shared__ uint32_t s_aux[kStackLds * kBlock];
-
   const uint32_t lane = threadIdx.x;
   const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + lane;
   const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * kBlock;
+  const MeshView M = A.mesh;
 
-  LaneStack st;
-  st.s_ref = s_ref;
-  st.s_aux = s_aux;
+  Stack st;
   st.ovf = A.stack.ovf;
   st.ovf_stride = A.stack.ovf_stride;
   st.gtid = gtid;
   st.lane = lane;
-  st.capacity = kStackLds + A.stack.ovf_entries;
+  st.capacity = kStackLds + static_cast<int>(A.stack.ovf_entries);
 
-  const MeshView M = A.mesh;
   const float min2 = A.min_threshold * A.min_threshold;
   const float max2 = A.max_threshold * A.max_threshold;
-  // Scores are compared with a tiny relative slack so FP32 rounding never skips a box the exact
-  // score would enter.
   constexpr float kScoreSlack = 1.0f - 4.0f * kU;
 
   for (uint64_t qi = gtid; qi < A.n; qi += gstride) {
     const f3 q = mk(A.points[3 * qi + 0], A.points[3 * qi + 1], A.points[3 * qi + 2]);
     // Absolute error bound of an FP32 point-to-triangle distance for this query.
-    const float e_abs = 16.0f * kU * (maxabs(q) + M.coord_max);
-    float best = __int_as_float(0x7f800000);
-    float best_e = 0.0f;
-    ff best_ff = {0.0f, 0.0f};
-    bool best_ff_ok = false;
-    uint32_t best_slot = kNoHit;
-    bool done = false;
-    bool overflow = false;
+    const float e_abs = 8.0f * kU * (maxabs(q) + M.coord_max);
+    float U = kInf;       // upper bound of the best squared distance
+    float best = kInf;    // smallest FP32 squared distance seen
+    f3 best_cp = mk(kInf, kInf, kInf);
+    bool done = false, overflow = false;
+    Cands cl;
+    cl.lane = lane;
+    cl.n = 0;
 
     for (uint32_t r = 0; r < M.n_roots && !done; r++) {
       uint32_t ref = M.roots[r];
       st.sp = 0;
       while (true) {
-        if (ref & kRefLeaf) {
-          uint32_t off, cnt;
-          leaf_range(ref, M.bigleaf, off, cnt);
-          for (uint32_t s = off; s < off + cnt; s++) {
-            f3 a, b, c;
-            uint32_t face, tmask;
-            load_tri(M.tris, s, a, b, c, face, tmask);
-            const f3 cp = closest_on_triangle(q, a, b, c);
-            const f3 dv = sub(q, cp);
-            const float d2 = dot(dv, dv);
-            const float e2 = (2.0f * sqrtf(d2) + e_abs) * e_abs + 4.0f * kU * d2;
-            // Strict < as in closestPointToPoint; near-equal candidates are ordered in
-            // double-float (exact ties, e.g. a shared vertex, keep the first as the reference).
-            bool better;
-            bool exact = false;
-            ff d2_ff;
-            if (d2 + e2 < best - best_e) {
-              better = true;
-            } else if (d2 - e2 > best + best_e) {
-              better = false;
-            } else {
-              // Also reached when d2 is NaN (a degenerate triangle in FP32): the double-float
-              // value decides, and a NaN there never wins, as in the reference's `distSq <`.
-              d2_ff = closest_on_triangle_ff(q, a, b, c).dist2;
-              if (best_slot == kNoHit) {
-                better = d2_ff.hi < best;  // best is +inf here
-              } else {
-                if (!best_ff_ok) {
-                  f3 ba, bb, bc;
-                  uint32_t bface, bmask;
-                  load_tri(M.tris, best_slot, ba, bb, bc, bface, bmask);
-                  best_ff = closest_on_triangle_ff(q, ba, bb, bc).dist2;
-                  best_ff_ok = true;
-                }
-                better = ff_sign(ff_sub(d2_ff, best_ff)) < 0;
-              }
-              exact = true;
-            }
-            if (better) {
-              best_slot = s;
-              best_ff_ok = exact;
-              if (exact) {
-                best_ff = d2_ff;
-                best = ff_to_float(d2_ff);
-                best_e = (2.0f * sqrtf(best) + e_abs) * e_abs + 4.0f * kU * best;
-              } else {
-                best = d2;
-                best_e = e2;
-              }
-            }
-            if (d2 < min2) {
-              done = true;
-              break;
-            }
-          }
-          if (done) break;
-        } else {
+        // ---- internal nodes ----
+        while (!(ref & kRefLeaf) && ref != kDone) {
           const WideNode nd = M.nodes[ref];
           const float sl = box_score(q, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y);
           const float sr = box_score(q, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w);
@@ -656,55 +662,113 @@ __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
           const uint32_t c2 = swap ? nd.meta.x : nd.meta.y;
           const float s1 = (swap ? sr : sl) * kScoreSlack;
           const float s2 = (swap ? sl : sr) * kScoreSlack;
-          const float reach = best + best_e;  // a box scoring beyond this holds no better triangle
-          const bool in1 = s1 < reach && s1 < max2;
-          const bool in2 = s2 < reach && s2 < max2;
+          const bool in1 = s1 <= U && s1 < max2;
+          const bool in2 = s2 <= U && s2 < max2;
           if (in1) {
-            if (in2) {
-              if (!st.push(c2, __float_as_uint(s2))) {
-                overflow = true;
-                done = true;
+            if (in2 && !st.push(c2, __float_as_uint(s2))) {
+              overflow = done = true;
+              ref = kDone;
+              break;
+            }
+            ref = c1;
+          } else {
+            // s2 >= s1: if the near child is out, so is the far one
+            ref = kDone;
+            while (st.sp > 0) {
+              const uint2 e = st.pop();
+              const float sc = __uint_as_float(e.y);
+              if (sc <= U && sc < max2) {
+                ref = e.x;
                 break;
               }
             }
-            ref = c1;
-            continue;
-          }
-          // s2 >= s1, so if the near child is out the far one is too.
-        }
-        bool found = false;
-        while (st.sp > 0) {
-          uint32_t aux;
-          st.pop(ref, aux);
-          const float sc = __uint_as_float(aux);
-          if (sc < best + best_e && sc < max2) {
-            found = true;
-            break;
           }
         }
-        if (!found) break;
+        // ---- leaves ----
+        while (ref & kRefLeaf) {
+          uint32_t off, cnt;
+          leaf_range(ref, M.bigleaf, off, cnt);
+          Tri next;
+          if (cnt != 0) next = load_tri(M.tris, off);  // an empty leaf may sit at the slot count
+          for (uint32_t s = off; s < off + cnt; s++) {
+            const Tri tr = next;
+            if (s + 1 < off + cnt) next = load_tri(M.tris, s + 1);
+            const f3 cp = closest_on_triangle(q, tr.a(), tr.b(), tr.c());
+            const f3 dv = sub(q, cp);
+            const float d2 = dot(dv, dv);
+            float lb;
+            bool add;
+            if (!(d2 == d2)) {
+              add = true;  // FP32 failed (degenerate triangle): decide exactly at the end
+              lb = -kInf;
+            } else {
+              const float e2 = (2.0f * sqrtf(d2) + e_abs) * e_abs + 4.0f * kU * d2;
+              lb = d2 - e2;
+              // an exact copy of the best point (shared vertex) ties exactly: the earlier wins
+              add = lb <= U && !(d2 == best && cp.x == best_cp.x && cp.y == best_cp.y && cp.z == best_cp.z);
+              if (add) {
+                U = fminf(U, d2 + e2);
+                if (d2 < best) {
+                  best = d2;
+                  best_cp = cp;
+                }
+              }
+            }
+            if (add) {
+              if (cl.n == kCand && !cl.compact(U)) {
+                ff bd;
+                const uint32_t w = resolve_point(cl, U, q, M.tris, bd);
+                cl.n = 0;
+                if (w != kNoHit) {
+                  const float dw = ff_to_float(bd);
+                  U = fminf(U, dw * (1.0f + 4.0f * kU));
+                  cl.put(0, w, dw * (1.0f - 4.0f * kU));
+                  cl.n = 1;
+                }
+              }
+              cl.put(cl.n++, s, lb);
+            }
+            if (d2 < min2) {
+              done = true;
+              break;
+            }
+          }
+          ref = kDone;
+          if (done) break;
+          while (st.sp > 0) {
+            const uint2 e = st.pop();
+            const float sc = __uint_as_float(e.y);
+            if (sc <= U && sc < max2) {
+              ref = e.x;
+              break;
+            }
+          }
+        }
+        if (ref == kDone) break;
       }
     }
 
     uint32_t out_face = kNoHit;
-    float out_dist = __int_as_float(0x7f800000);
+    float out_dist = kInf;
     float px = __int_as_float(0x7fc00000), py = px, pz = px;
     float bw = px, bu = px, bv = px;
     if (overflow) {
-      out_face = kNoHit - 1u;
-    } else if (best_slot != kNoHit) {
-      f3 a, b, c;
-      uint32_t face, tmask;
-      load_tri(M.tris, best_slot, a, b, c, face, tmask);
-      const ClosestFF cf = closest_on_triangle_ff(q, a, b, c);
-      out_face = face;
-      out_dist = ff_to_float(ff_sqrt(cf.dist2));
-      px = ff_to_float(cf.point.x);
-      py = ff_to_float(cf.point.y);
-      pz = ff_to_float(cf.point.z);
-      bw = ff_to_float(cf.bw);
-      bu = ff_to_float(cf.bu);
-      bv = ff_to_float(cf.bv);
+      out_face = kOverflow;
+    } else if (cl.n > 0) {
+      ff bd;
+      const uint32_t w = resolve_point(cl, U, q, M.tris, bd);
+      if (w != kNoHit) {
+        const Tri tr = load_tri(M.tris, w);
+        const ClosestFF cf = closest_on_triangle_ff(q, tr.a(), tr.b(), tr.c());
+        out_face = tr.face();
+        out_dist = ff_to_float(ff_sqrt(cf.dist2));
+        px = ff_to_float(cf.point.x);
+        py = ff_to_float(cf.point.y);
+        pz = ff_to_float(cf.point.z);
+        bw = ff_to_float(cf.bw);
+        bu = ff_to_float(cf.bu);
+        bv = ff_to_float(cf.bv);
+      }
     }
     A.out_face[qi] = out_face;
     if (A.out_dist) A.out_dist[qi] = out_dist;
