@@ -10,7 +10,7 @@ Also reported in the same line: first-hit raycast throughput on TorusKnot(10, 3,
 
 The JSON line carries:
   roofline      algorithmic bytes of the closest-point kernel (per query: 32 B of I/O + 32 B per
-                node box read + 42 B per triangle tested (3 x u32 index + 3 x 12 B vertices), with
+                node box read + 48 B per triangle tested (3 x u32 index + 3 x 12 B vertices), with
                 the node / triangle counts of the reference traversal order measured by the FP64
                 oracle on a sample) divided by the kernel's average duration (HIP events on the
                 launch stream), against the 8 TB/s HBM peak.
@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=16, help="1/N stride of the CPU-baseline sample")
+    ap.add_argument("--traffic-calibration", action="store_true",
+                    help="also run one 256 MiB device copy, the reference kernel for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiling run")
     return ap.parse_args()
 
 
@@ -181,6 +184,12 @@ def main():
                           "kernel_ms": round(float(np.mean(kern1)), 4), **extra},
     }
     print(json.dumps(line), flush=True)
+    if args.traffic_calibration:
+        src = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
     mesh.free()
     mesh1.free()
     ctx.close()

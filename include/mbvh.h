@@ -162,6 +162,29 @@ int mbvh_raycast_first_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* orig
                          float* barycoord, void* stream);
 
 /*
+ * All hits per ray. Replaces MeshBVH.raycast (src/core/MeshBVH.js:532-545) → raycast
+ * (src/core/cast/raycast.template.js:7-53) → intersectTris (src/core/utils/iterationUtils.template.js:4-22).
+ * The hits of ray i are entries offsets[i] .. offsets[i+1]-1, in the reference's traversal order
+ * (unsorted, as the reference returns them; three's Raycaster sorts by distance afterwards).
+ * Host form: mbvh_raycast writes offsets (n + 1 entries, offsets[n] = total hits) and keeps the
+ * hits on the device until the next mbvh_raycast on this context; mbvh_raycast_fetch copies them.
+ */
+int mbvh_raycast(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins, const float* directions,
+                 uint64_t n, int32_t side, float near_dist, float far_dist, uint64_t* offsets);
+int mbvh_raycast_fetch(mbvh_ctx* ctx, uint32_t* face_index, float* distance, float* point,
+                       float* barycoord, uint64_t total);
+/* Device form in two passes: hits per ray, then the fill at caller-computed offsets (the exclusive
+ * prefix sum of the counts); `capacity` bounds the output arrays. */
+int mbvh_raycast_count_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins,
+                         const float* directions, uint64_t n, int32_t side, float near_dist,
+                         float far_dist, uint32_t* counts, void* stream);
+int mbvh_raycast_fill_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins,
+                        const float* directions, uint64_t n, int32_t side, float near_dist,
+                        float far_dist, const uint64_t* offsets, uint64_t capacity,
+                        uint32_t* face_index, float* distance, float* point, float* barycoord,
+                        void* stream);
+
+/*
  * Closest point on the mesh per query point. Replaces MeshBVH.closestPointToPoint
  * (src/core/MeshBVH.js:874-884) → closestPointToPoint (src/core/cast/closestPointToPoint.js:6-78)
  * → shapecast (src/core/BVH.js:342-413, src/core/cast/shapecast.js:12-214) with three's

@@ -1,0 +1,132 @@
+'use strict';
+// Drives the JavaScript API on the GPU for tests/test_gpu_js.py: reads a geometry and query sets
+// from <dir>, runs MeshBVH / acceleratedRaycast queries, writes the results (and the BVH the
+// build produced) back to <dir> for the oracle comparison, and asserts the API-level invariants
+// that need no oracle (per-call == batch, first hit == nearest of all hits, world/local
+// consistency under a rigid transform, BatchedMesh batchId bookkeeping).
+const assert = require('assert');
+const fs = require('fs');
+const path = require('path');
+const api = require(path.join(__dirname, '..', '..', 'three-mesh-bvh_amd', 'src', 'index.js'));
+const { geometryFrom, compose, Mesh, Raycaster } = require('./shim.js');
+
+const dir = process.argv[2];
+const man = JSON.parse(fs.readFileSync(path.join(dir, 'manifest.json'), 'utf8'));
+function readArr(name, Type) {
+  const b = fs.readFileSync(path.join(dir, name));
+  return new Type(b.buffer.slice(b.byteOffset, b.byteOffset + b.byteLength));
+}
+function writeArr(name, arr) {
+  fs.writeFileSync(path.join(dir, name), Buffer.from(arr.buffer, arr.byteOffset, arr.byteLength));
+}
+
+const positions = readArr('positions.bin', Float32Array);
+const index = readArr('index.bin', man.index_u32 ? Uint32Array : Uint16Array);
+const origins = readArr('origins.bin', Float32Array);
+const dirs = readArr('dirs.bin', Float32Array);
+const points = readArr('points.bin', Float32Array);
+const nRays = origins.length / 3;
+const nPts = points.length / 3;
+
+const geometry = geometryFrom(positions, index);
+const bvh = new api.MeshBVH(geometry, { strategy: api.SAH, maxLeafTris: 10 });
+writeArr('bvh_index.bin', geometry.index.array);
+writeArr('bvh_root0.bin', new Uint8Array(bvh._roots[0]));
+
+// --- batch queries -------------------------------------------------------------------------------
+const rf = bvh.raycastFirstBatch(origins, dirs);
+writeArr('rf_face.bin', rf.faceIndex);
+writeArr('rf_dist.bin', rf.distance);
+writeArr('rf_point.bin', rf.point);
+writeArr('rf_bary.bin', rf.barycoord);
+const cp = bvh.closestPointToPointBatch(points);
+writeArr('cp_face.bin', cp.faceIndex);
+writeArr('cp_dist.bin', cp.distance);
+writeArr('cp_point.bin', cp.point);
+
+const K = Math.min(256, nRays);
+const ra = bvh.raycastBatch(origins.subarray(0, 3 * K), dirs.subarray(0, 3 * K), { side: api.DoubleSide });
+writeArr('ra_offsets.bin', ra.offsets);
+writeArr('ra_face.bin', ra.faceIndex);
+writeArr('ra_dist.bin', ra.distance);
+
+// --- per-call API agrees with the batch API --------------------------------------------------
+for (let i = 0; i < K; i++) {
+  const ray = { origin: { x: origins[3 * i], y: origins[3 * i + 1], z: origins[3 * i + 2], isVector3: false },
+    direction: { x: dirs[3 * i], y: dirs[3 * i + 1], z: dirs[3 * i + 2] } };
+  const hit = bvh.raycastFirst(ray);
+  if (rf.faceIndex[i] === 0xFFFFFFFF) {
+    assert.strictEqual(hit, null);
+  } else {
+    assert.strictEqual(hit.faceIndex, rf.faceIndex[i]);
+    assert.strictEqual(hit.distance, rf.distance[i]);
+    assert.ok(Math.abs(hit.barycoord.x + hit.barycoord.y + hit.barycoord.z - 1) < 1e-5);
+    const f = hit.faceIndex;
+    assert.deepStrictEqual([hit.face.a, hit.face.b, hit.face.c], [geometry.index.array[3 * f], geometry.index.array[3 * f + 1], geometry.index.array[3 * f + 2]]);
+    assert.ok(Math.abs(Math.hypot(hit.face.normal.x, hit.face.normal.y, hit.face.normal.z) - 1) < 1e-6);
+  }
+  // the first hit is the nearest of all hits (both FrontSide)
+  const all = bvh.raycast(ray);
+  if (all.length) {
+    const nearest = all.reduce((m, h) => (h.distance < m.distance ? h : m));
+    assert.strictEqual(nearest.distance, rf.distance[i]);
+  } else {
+    assert.strictEqual(rf.faceIndex[i], 0xFFFFFFFF);
+  }
+}
+for (let i = 0; i < Math.min(256, nPts); i++) {
+  const t = bvh.closestPointToPoint({ x: points[3 * i], y: points[3 * i + 1], z: points[3 * i + 2] });
+  assert.strictEqual(t.faceIndex, cp.faceIndex[i]);
+  assert.strictEqual(t.distance, cp.distance[i]);
+}
+
+// --- raycastObject3D / acceleratedRaycast under a rigid transform ----------------------------
+const q = [0.2, -0.3, 0.4, 0];
+q[3] = Math.sqrt(1 - q[0] * q[0] - q[1] * q[1] - q[2] * q[2]);
+const M = compose([3, -2, 5], q, [1, 1, 1]);
+const e = M.elements;
+const apply = v => [e[0] * v[0] + e[4] * v[1] + e[8] * v[2] + e[12], e[1] * v[0] + e[5] * v[1] + e[9] * v[2] + e[13], e[2] * v[0] + e[6] * v[1] + e[10] * v[2] + e[14]];
+const rot = v => [e[0] * v[0] + e[4] * v[1] + e[8] * v[2], e[1] * v[0] + e[5] * v[1] + e[9] * v[2], e[2] * v[0] + e[6] * v[1] + e[10] * v[2]];
+geometry.boundsTree = bvh;
+const mesh = new Mesh(geometry, { isMaterial: true, side: api.FrontSide });
+mesh.matrixWorld = M;
+mesh.raycast = api.acceleratedRaycast;
+let worldChecked = 0;
+for (let i = 0; i < K; i++) {
+  const o = apply([origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]]);
+  const d = rot([dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]]);
+  const rc = new Raycaster(o, d);
+  rc.firstHitOnly = true;
+  const hits = rc.intersectObject(mesh);
+  if (rf.faceIndex[i] === 0xFFFFFFFF) continue;
+  // the world-space ray is rounded to float32 again, so near-grazing rays may differ; count agreement
+  if (hits.length === 1 && hits[0].faceIndex === rf.faceIndex[i]) {
+    assert.ok(Math.abs(hits[0].distance - rf.distance[i]) <= 1e-4 * Math.max(1, rf.distance[i]));
+    assert.strictEqual(hits[0].object, mesh);
+    worldChecked++;
+  }
+}
+assert.ok(worldChecked >= 0.98 * rf.faceIndex.subarray(0, K).filter(f => f !== 0xFFFFFFFF).length, `world/local agreement ${worldChecked}`);
+
+// --- BatchedMesh: two instances of one geometry, hits tagged with batchId --------------------
+const batched = {
+  isBatchedMesh: true,
+  geometry,
+  material: mesh.material,
+  matrixWorld: { elements: [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1] },
+  boundsTrees: [bvh],
+  _drawInfo: [{ geometryIndex: 0 }, { geometryIndex: 0 }],
+  getVisibleAt() { return true; },
+  getMatrixAt(i, target) {
+    target.elements = i === 0 ? [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1] : M.elements.slice();
+    return target;
+  },
+  raycast: api.acceleratedRaycast,
+};
+const rc = new Raycaster([origins[0], origins[1], origins[2]], [dirs[0], dirs[1], dirs[2]]);
+rc.firstHitOnly = true;
+const bh = rc.intersectObject(batched);
+assert.ok(bh.every(h => h.object === batched && (h.batchId === 0 || h.batchId === 1)));
+if (rf.faceIndex[0] !== 0xFFFFFFFF) assert.ok(bh.some(h => h.batchId === 0 && h.faceIndex === rf.faceIndex[0]));
+
+console.log(JSON.stringify({ ok: true, rays: nRays, points: nPts, worldChecked }));

@@ -1,0 +1,66 @@
+"""GPU parity of the JavaScript API (MeshBVH / acceleratedRaycast via the Node-API addon) against the
+FP64 oracle: the same geometry, rays and points go through Node, results come back here."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import mbvh_amd as M
+from mbvh_amd import workloads as W
+import oracle_py as O
+from helpers import check_raycast_parity, check_closest_parity
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")]
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_js_api_matches_oracle(tmp_path):
+    w = W.c1_raycast(8192)
+    g = w.geometry
+    lo, hi = g.bounds()
+    pts = W.grid_points(lo, hi, 20)
+    d = tmp_path
+    g.positions.tofile(d / "positions.bin")
+    g.index.tofile(d / "index.bin")
+    w.origins.tofile(d / "origins.bin")
+    w.directions.tofile(d / "dirs.bin")
+    pts.tofile(d / "points.bin")
+    (d / "manifest.json").write_text(json.dumps({"index_u32": g.index.dtype == np.uint32}))
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "gpu_run.js"), str(d)], capture_output=True,
+                       text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-4000:])
+    assert r.returncode == 0
+
+    idx = np.fromfile(d / "bvh_index.bin", dtype=g.index.dtype)
+    root = np.fromfile(d / "bvh_root0.bin", dtype=np.uint8)
+    # the JS build (through the addon) equals the Python-ABI build of the same geometry
+    idx_py = g.index.copy()
+    tree_py = M.build(g.positions, idx_py, strategy=M.SAH, target_leaf_size=10)
+    assert np.array_equal(idx, idx_py) and np.array_equal(root, tree_py.roots[0])
+
+    om = O.OracleMesh(g.positions, idx, [root])
+    got = {"face": np.fromfile(d / "rf_face.bin", np.uint32), "distance": np.fromfile(d / "rf_dist.bin", np.float32),
+           "point": np.fromfile(d / "rf_point.bin", np.float32).reshape(-1, 3),
+           "barycoord": np.fromfile(d / "rf_bary.bin", np.float32).reshape(-1, 3)}
+    ref = om.raycast_first(w.origins, w.directions)
+    print("js raycastFirst", check_raycast_parity(om, w.origins, w.directions, got, ref, "js_raycastFirst"))
+
+    gotc = {"face": np.fromfile(d / "cp_face.bin", np.uint32), "distance": np.fromfile(d / "cp_dist.bin", np.float32),
+            "point": np.fromfile(d / "cp_point.bin", np.float32).reshape(-1, 3)}
+    refc = om.closest_point(pts)
+    print("js closestPoint", check_closest_parity(om, pts, gotc, refc, "js_closest"))
+
+    # all hits, DoubleSide, in the reference's traversal order
+    offs = np.fromfile(d / "ra_offsets.bin", np.float64).astype(np.int64)
+    faces = np.fromfile(d / "ra_face.bin", np.uint32)
+    dists = np.fromfile(d / "ra_dist.bin", np.float32)
+    for i in range(len(offs) - 1):
+        rf_, rd_ = om.raycast_all(w.origins[i], w.directions[i], side=O.DOUBLE)
+        gf = faces[offs[i]:offs[i + 1]]
+        gd = dists[offs[i]:offs[i + 1]]
+        assert len(gf) == len(rf_), (i, gf, rf_)
+        assert np.array_equal(gf, rf_), (i, gf, rf_)
+        assert np.all(np.abs(gd - rd_) <= 1e-5 * np.maximum(rd_, 1))

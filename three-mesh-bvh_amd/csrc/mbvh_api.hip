@@ -32,6 +32,8 @@ struct mbvh_ctx {
   DeviceBuffer stack_ovf;       // traversal stack overflow region
   uint32_t stack_ovf_entries = 0;
   DeviceBuffer staging[8];      // host-API input / output staging
+  DeviceBuffer hits[5];         // host-API all-hits results: offsets, face, dist, point, bary
+  uint64_t hits_total = 0;
   std::mutex lock;              // host entry points on one context are serialised
 };
 
@@ -251,6 +253,8 @@ extern "C" void mbvh_ctx_destroy(mbvh_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->stack_ovf.ptr) (void)hipFree(ctx->stack_ovf.ptr);
   for (auto& b : ctx->staging)
+    if (b.ptr) (void)hipFree(b.ptr);
+  for (auto& b : ctx->hits)
     if (b.ptr) (void)hipFree(b.ptr);
   (void)hipStreamDestroy(ctx->stream);
   ctx->magic = 0;
@@ -592,4 +596,147 @@ extern "C" int mbvh_closest_point(mbvh_ctx* ctx, const mbvh_mesh* mesh, const fl
   if (barycoord) MBVH_HIP(hipMemcpyAsync(barycoord, ctx->staging[5].ptr, v3, hipMemcpyDeviceToHost, s));
   MBVH_HIP(hipStreamSynchronize(s));
   return check_no_overflow(face_index, n, "mbvh_closest_point");
+}
+
+// ---------------------------------------------------------------------------------------------
+// All hits
+// ---------------------------------------------------------------------------------------------
+
+namespace {
+
+int raycast_all_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o, const float* d, uint64_t n,
+                     int32_t side, float near_dist, float far_dist, uint32_t* counts, const uint64_t* offsets,
+                     uint64_t capacity, uint32_t* face, float* dist, float* point, float* bary, bool fill,
+                     hipStream_t s) {
+  if (side < MBVH_FRONT_SIDE || side > MBVH_SIDE_PER_TRIANGLE)
+    return set_error(MBVH_ERR_BAD_ARGS, "raycast: invalid side %d", side);
+  if (n && (!o || !d)) return set_error(MBVH_ERR_BAD_ARGS, "raycast: NULL ray buffer");
+  if (n && !fill && !counts) return set_error(MBVH_ERR_BAD_ARGS, "raycast: NULL counts buffer");
+  if (n && fill && (!offsets || (capacity && !face)))
+    return set_error(MBVH_ERR_BAD_ARGS, "raycast: NULL offsets or face buffer");
+  if (mesh->max_stack > kStackLds + ctx->stack_ovf_entries)
+    return set_error(MBVH_ERR_LIMIT, "raycast: stack scratch smaller than the tree depth");
+  const uint64_t threads = launch_threads(n, ctx->max_threads);
+  RaycastAllArgs a;
+  a.mesh = view_of(mesh);
+  a.stack = stack_of(ctx, threads);
+  a.origins = o;
+  a.directions = d;
+  a.n = n;
+  a.side_mask = side_mask_of(side);
+  a.near_dist = near_dist;
+  a.far_dist = far_dist;
+  a.counts = counts;
+  a.offsets = offsets;
+  a.out_face = face;
+  a.out_dist = dist;
+  a.out_point = point;
+  a.out_bary = bary;
+  a.capacity = capacity;
+  MBVH_HIP(launch_raycast_all(a, fill, threads, s));
+  return MBVH_OK;
+}
+
+}  // namespace
+
+extern "C" int mbvh_raycast_count_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins,
+                                    const float* directions, uint64_t n, int32_t side, float near_dist,
+                                    float far_dist, uint32_t* counts, void* stream) {
+  clear_error();
+  int rc = check_query("mbvh_raycast_count_d", ctx, mesh);
+  if (rc) return rc;
+  rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return raycast_all_impl(ctx, mesh, origins, directions, n, side, near_dist, far_dist, counts, nullptr, 0,
+                          nullptr, nullptr, nullptr, nullptr, false, s);
+}
+
+extern "C" int mbvh_raycast_fill_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins,
+                                   const float* directions, uint64_t n, int32_t side, float near_dist,
+                                   float far_dist, const uint64_t* offsets, uint64_t capacity,
+                                   uint32_t* face_index, float* distance, float* point, float* barycoord,
+                                   void* stream) {
+  clear_error();
+  int rc = check_query("mbvh_raycast_fill_d", ctx, mesh);
+  if (rc) return rc;
+  rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return raycast_all_impl(ctx, mesh, origins, directions, n, side, near_dist, far_dist, nullptr, offsets,
+                          capacity, face_index, distance, point, barycoord, true, s);
+}
+
+extern "C" int mbvh_raycast(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins, const float* directions,
+                            uint64_t n, int32_t side, float near_dist, float far_dist, uint64_t* offsets) {
+  clear_error();
+  int rc = check_query("mbvh_raycast", ctx, mesh);
+  if (rc) return rc;
+  if (!offsets) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_raycast: offsets is NULL");
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  ctx->hits_total = 0;
+  offsets[0] = 0;
+  if (n == 0) return MBVH_OK;
+  if (!origins || !directions) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_raycast: NULL ray buffer");
+  rc = set_device(ctx);
+  if (rc) return rc;
+  const size_t v3 = n * 3 * sizeof(float);
+  if ((rc = ensure_buffer(ctx->staging[0], v3)) || (rc = ensure_buffer(ctx->staging[1], v3)) ||
+      (rc = ensure_buffer(ctx->staging[2], n * sizeof(uint32_t))) ||
+      (rc = ensure_buffer(ctx->hits[0], (n + 1) * sizeof(uint64_t))))
+    return rc;
+  hipStream_t s = ctx->stream;
+  const float* dO = static_cast<float*>(ctx->staging[0].ptr);
+  const float* dD = static_cast<float*>(ctx->staging[1].ptr);
+  MBVH_HIP(hipMemcpyAsync(ctx->staging[0].ptr, origins, v3, hipMemcpyHostToDevice, s));
+  MBVH_HIP(hipMemcpyAsync(ctx->staging[1].ptr, directions, v3, hipMemcpyHostToDevice, s));
+  rc = raycast_all_impl(ctx, mesh, dO, dD, n, side, near_dist, far_dist, static_cast<uint32_t*>(ctx->staging[2].ptr),
+                        nullptr, 0, nullptr, nullptr, nullptr, nullptr, false, s);
+  if (rc) return rc;
+  std::vector<uint32_t> counts(n);
+  MBVH_HIP(hipMemcpyAsync(counts.data(), ctx->staging[2].ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipStreamSynchronize(s));
+  uint64_t total = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    if (counts[i] == MBVH_NO_HIT - 1u)
+      return set_error(MBVH_ERR_LIMIT, "mbvh_raycast: traversal stack overflow on ray %llu",
+                       static_cast<unsigned long long>(i));
+    offsets[i] = total;
+    total += counts[i];
+  }
+  offsets[n] = total;
+  const size_t t = static_cast<size_t>(total);
+  if ((rc = ensure_buffer(ctx->hits[1], t * sizeof(uint32_t))) || (rc = ensure_buffer(ctx->hits[2], t * sizeof(float))) ||
+      (rc = ensure_buffer(ctx->hits[3], t * 3 * sizeof(float))) || (rc = ensure_buffer(ctx->hits[4], t * 3 * sizeof(float))))
+    return rc;
+  MBVH_HIP(hipMemcpyAsync(ctx->hits[0].ptr, offsets, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  rc = raycast_all_impl(ctx, mesh, dO, dD, n, side, near_dist, far_dist, nullptr,
+                        static_cast<const uint64_t*>(ctx->hits[0].ptr), total, static_cast<uint32_t*>(ctx->hits[1].ptr),
+                        static_cast<float*>(ctx->hits[2].ptr), static_cast<float*>(ctx->hits[3].ptr),
+                        static_cast<float*>(ctx->hits[4].ptr), true, s);
+  if (rc) return rc;
+  MBVH_HIP(hipStreamSynchronize(s));
+  ctx->hits_total = total;
+  return MBVH_OK;
+}
+
+extern "C" int mbvh_raycast_fetch(mbvh_ctx* ctx, uint32_t* face_index, float* distance, float* point,
+                                  float* barycoord, uint64_t total) {
+  clear_error();
+  if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_raycast_fetch: invalid context handle");
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if (total != ctx->hits_total)
+    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_raycast_fetch: %llu hits are held, not %llu",
+                     static_cast<unsigned long long>(ctx->hits_total), static_cast<unsigned long long>(total));
+  if (total == 0) return MBVH_OK;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  const size_t t = static_cast<size_t>(total);
+  if (face_index) MBVH_HIP(hipMemcpyAsync(face_index, ctx->hits[1].ptr, t * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (distance) MBVH_HIP(hipMemcpyAsync(distance, ctx->hits[2].ptr, t * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (point) MBVH_HIP(hipMemcpyAsync(point, ctx->hits[3].ptr, t * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (barycoord) MBVH_HIP(hipMemcpyAsync(barycoord, ctx->hits[4].ptr, t * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipStreamSynchronize(s));
+  return MBVH_OK;
 }

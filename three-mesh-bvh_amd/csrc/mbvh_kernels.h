@@ -39,6 +39,24 @@ struct RaycastFirstArgs {
   float* out_bary;
 };
 
+struct RaycastAllArgs {
+  MeshView mesh;
+  StackScratch stack;
+  const float* origins;
+  const float* directions;
+  uint64_t n;
+  uint32_t side_mask;
+  float near_dist;
+  float far_dist;
+  uint32_t* counts;          // count pass: hits per ray
+  const uint64_t* offsets;   // fill pass: first output slot of each ray
+  uint32_t* out_face;
+  float* out_dist;
+  float* out_point;
+  float* out_bary;
+  uint64_t capacity;         // fill pass: output slots available
+};
+
 struct ClosestPointArgs {
   MeshView mesh;
   StackScratch stack;
@@ -58,6 +76,8 @@ uint64_t launch_threads(uint64_t n, uint64_t max_threads);
 
 hipError_t launch_raycast_first(const RaycastFirstArgs& a, uint64_t threads, hipStream_t s);
 hipError_t launch_closest_point(const ClosestPointArgs& a, uint64_t threads, hipStream_t s);
+// fill == false: count hits per ray into a.counts; fill == true: write them at a.offsets.
+hipError_t launch_raycast_all(const RaycastAllArgs& a, bool fill, uint64_t threads, hipStream_t s);
 hipError_t launch_apply_side_mask(float4* tris, uint64_t n_slots, const uint8_t* mask,
                                   hipStream_t s);
 

@@ -455,6 +455,117 @@ __global__ __launch_bounds__(kBlock) void k_raycast_first(RaycastFirstArgs A) {
 }
 
 // ------------------------------------------------------------------------------------------
+// All hits. _raycast (src/core/cast/raycast.template.js:16-53): left child before right child,
+// both bounds-tested against [near, far], no distance pruning; inside a leaf every triangle in
+// order (intersectTris, iterationUtils.template.js:4-22). The count pass and the fill pass walk
+// the identical order, so hit k of a ray lands at offsets[ray] + k in the reference's order.
+// Every accepted hit is re-evaluated in double-float for its reported values.
+// ------------------------------------------------------------------------------------------
+template <bool kFill>
+__global__ __launch_bounds__(kBlock) void k_raycast_all(RaycastAllArgs A) {
+  const uint32_t lane = threadIdx.x;
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + lane;
+  const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * kBlock;
+  const MeshView M = A.mesh;
+
+  Stack st;
+  st.ovf = A.stack.ovf;
+  st.ovf_stride = A.stack.ovf_stride;
+  st.gtid = gtid;
+  st.lane = lane;
+  st.capacity = kStackLds + static_cast<int>(A.stack.ovf_entries);
+
+  for (uint64_t ray = gtid; ray < A.n; ray += gstride) {
+    const f3 o = mk(A.origins[3 * ray + 0], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
+    const f3 d = mk(A.directions[3 * ray + 0], A.directions[3 * ray + 1], A.directions[3 * ray + 2]);
+    RayBox rb;
+    rb.ox = o.x;
+    rb.oy = o.y;
+    rb.oz = o.z;
+    rb.ix = 1.0f / d.x;
+    rb.iy = 1.0f / d.y;
+    rb.iz = 1.0f / d.z;
+    rb.sx = rb.ix >= 0.0f;
+    rb.sy = rb.iy >= 0.0f;
+    rb.sz = rb.iz >= 0.0f;
+    const float dlen = sqrtf(dot(d, d));
+    const float dl1 = fabsf(d.x) + fabsf(d.y) + fabsf(d.z);
+    uint32_t count = 0;
+    uint64_t base = 0;
+    if (kFill) base = A.offsets[ray];
+    bool overflow = false;
+
+    for (uint32_t r = 0; r < M.n_roots && !overflow; r++) {
+      uint32_t ref = M.roots[r];
+      st.sp = 0;
+      while (true) {
+        while (!(ref & kRefLeaf) && ref != kDone) {
+          const WideNode nd = M.nodes[ref];
+          float tl, tr;
+          const bool hl = slab(rb, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y, A.near_dist, A.far_dist, tl);
+          const bool hr = slab(rb, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w, A.near_dist, A.far_dist, tr);
+          if (hl && hr) {
+            if (!st.push(nd.meta.y, 0u)) {
+              overflow = true;
+              ref = kDone;
+              break;
+            }
+            ref = nd.meta.x;
+          } else if (hl) {
+            ref = nd.meta.x;
+          } else if (hr) {
+            ref = nd.meta.y;
+          } else {
+            ref = st.sp > 0 ? st.pop().x : kDone;
+          }
+        }
+        while (ref & kRefLeaf) {
+          uint32_t off, cnt;
+          leaf_range(ref, M.bigleaf, off, cnt);
+          for (uint32_t s = off; s < off + cnt; s++) {
+            const Tri tr = load_tri(M.tris, s);
+            const uint32_t mask = A.side_mask ? A.side_mask : tr.mask();
+            float t = 0.0f, et = 0.0f;
+            int res = hit_fast(o, d, dl1, dlen, tr, mask, A.near_dist, A.far_dist, t, et);
+            if (res == kUnsure) {
+              ff te;
+              res = hit_exact(o, d, dlen, tr.a(), tr.b(), tr.c(), mask, A.near_dist, A.far_dist, te) ? kHit : kMiss;
+            }
+            if (res != kHit) continue;
+            if (kFill) {
+              const uint64_t k = base + count;
+              if (k < A.capacity) {
+                const TriTerms q = tri_terms_exact(o, d, tr.a(), tr.b(), tr.c());
+                const ff tt = ff_div(q.qdn, q.ddn_abs);
+                const ff u = ff_div(q.e0, q.ddn_abs);
+                const ff v = ff_div(q.e1, q.ddn_abs);
+                const ff dl = ff_sqrt(ff_add(ff_add(two_prod(d.x, d.x), two_prod(d.y, d.y)), two_prod(d.z, d.z)));
+                A.out_face[k] = tr.face();
+                if (A.out_dist) A.out_dist[k] = ff_to_float(ff_mul(tt, dl));
+                if (A.out_point) {
+                  A.out_point[3 * k + 0] = ff_to_float(ff_add(ff_from(o.x), ff_mulf(tt, d.x)));
+                  A.out_point[3 * k + 1] = ff_to_float(ff_add(ff_from(o.y), ff_mulf(tt, d.y)));
+                  A.out_point[3 * k + 2] = ff_to_float(ff_add(ff_from(o.z), ff_mulf(tt, d.z)));
+                }
+                if (A.out_bary) {
+                  A.out_bary[3 * k + 0] = ff_to_float(ff_sub(ff_sub(ff_from(1.0f), u), v));
+                  A.out_bary[3 * k + 1] = ff_to_float(u);
+                  A.out_bary[3 * k + 2] = ff_to_float(v);
+                }
+              }
+            }
+            count++;
+          }
+          ref = st.sp > 0 ? st.pop().x : kDone;
+        }
+        if (ref == kDone) break;
+      }
+    }
+    if (!kFill) A.counts[ray] = overflow ? kOverflow : count;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // Closest point on a triangle: Ericson, Real-Time Collision Detection §5.1.5 (the algorithm of
 // three's Triangle.closestPointToPoint), Voronoi regions tested in the same order.
 // ------------------------------------------------------------------------------------------
@@ -810,6 +921,16 @@ hipError_t launch_raycast_first(const RaycastFirstArgs& a, uint64_t threads, hip
 hipError_t launch_closest_point(const ClosestPointArgs& a, uint64_t threads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_closest_point, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_raycast_all(const RaycastAllArgs& a, bool fill, uint64_t threads, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  const dim3 grid(static_cast<uint32_t>(threads / kBlock)), block(kBlock);
+  if (fill)
+    hipLaunchKernelGGL(k_raycast_all<true>, grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL(k_raycast_all<false>, grid, block, 0, s, a);
   return hipGetLastError();
 }
 

@@ -1,0 +1,511 @@
+/*
+ * mbvh_napi.c — Node-API addon over libmbvh.so. It only marshals typed arrays and numbers into
+ * the C ABI of include/mbvh.h and turns status codes into JavaScript exceptions; all logic lives
+ * in the library (kernels, upload) and in the JavaScript package (the MeshBVH API).
+ *
+ * Device contexts are process-wide, one per device, created on first use and destroyed by an
+ * environment cleanup hook, so mesh finalizers never outlive their context.
+ */
+#define NAPI_VERSION 3
+#include <node_api.h>
+
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mbvh.h"
+
+#define MAX_DEVICES 64
+#define MAX_ARGS 16
+
+static mbvh_ctx* g_ctx[MAX_DEVICES];
+
+typedef struct MeshHandle {
+  mbvh_mesh* mesh;
+  int device;
+} MeshHandle;
+
+/* ------------------------------------------------------------------------------------------ */
+
+static napi_value throw_status(napi_env env, int rc, const char* where) {
+  char msg[768];
+  snprintf(msg, sizeof(msg), "%s failed (mbvh status %d): %s", where, rc, mbvh_last_error());
+  char code[32];
+  snprintf(code, sizeof(code), "MBVH_%d", -rc);
+  napi_throw_error(env, code, msg);
+  return NULL;
+}
+
+static napi_value throw_msg(napi_env env, const char* msg) {
+  napi_throw_type_error(env, NULL, msg);
+  return NULL;
+}
+
+#define CHECK_NAPI(call)                                   \
+  do {                                                     \
+    if ((call) != napi_ok) return throw_msg(env, #call);   \
+  } while (0)
+
+static int get_args(napi_env env, napi_callback_info info, size_t want, napi_value* argv) {
+  size_t argc = MAX_ARGS;
+  napi_value tmp[MAX_ARGS];
+  if (napi_get_cb_info(env, info, &argc, tmp, NULL, NULL) != napi_ok) return 0;
+  for (size_t i = 0; i < want; i++) {
+    if (i < argc) {
+      argv[i] = tmp[i];
+    } else {
+      napi_get_undefined(env, &argv[i]);
+    }
+  }
+  return 1;
+}
+
+static int is_nullish(napi_env env, napi_value v) {
+  napi_valuetype t;
+  if (napi_typeof(env, v, &t) != napi_ok) return 1;
+  return t == napi_undefined || t == napi_null;
+}
+
+/* Typed array view: data pointer, element count and element size. */
+typedef struct {
+  void* data;
+  size_t length;
+  int elt;
+  napi_typedarray_type type;
+} View;
+
+static int get_view(napi_env env, napi_value v, View* out) {
+  memset(out, 0, sizeof(*out));
+  bool is_ta = false;
+  if (napi_is_typedarray(env, v, &is_ta) != napi_ok || !is_ta) return 0;
+  napi_value ab;
+  size_t off;
+  if (napi_get_typedarray_info(env, v, &out->type, &out->length, &out->data, &ab, &off) != napi_ok) return 0;
+  switch (out->type) {
+    case napi_int8_array: case napi_uint8_array: case napi_uint8_clamped_array: out->elt = 1; break;
+    case napi_int16_array: case napi_uint16_array: out->elt = 2; break;
+    case napi_int32_array: case napi_uint32_array: case napi_float32_array: out->elt = 4; break;
+    default: out->elt = 8; break;
+  }
+  return 1;
+}
+
+static int get_f64(napi_env env, napi_value v, double* out) {
+  return napi_get_value_double(env, v, out) == napi_ok;
+}
+
+static int get_i32(napi_env env, napi_value v, int32_t* out) {
+  return napi_get_value_int32(env, v, out) == napi_ok;
+}
+
+static int get_named_i32(napi_env env, napi_value obj, const char* name, int32_t dflt, int32_t* out) {
+  bool has = false;
+  *out = dflt;
+  if (napi_has_named_property(env, obj, name, &has) != napi_ok || !has) return 1;
+  napi_value v;
+  if (napi_get_named_property(env, obj, name, &v) != napi_ok) return 0;
+  if (is_nullish(env, v)) return 1;
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t == napi_boolean) {
+    bool b;
+    napi_get_value_bool(env, v, &b);
+    *out = b ? 1 : 0;
+    return 1;
+  }
+  return get_i32(env, v, out);
+}
+
+static float to_float(double v) { return (float)v; }
+
+static int context_for(napi_env env, int32_t device, mbvh_ctx** out) {
+  (void)env;
+  if (device < 0 || device >= MAX_DEVICES) return MBVH_ERR_NO_DEVICE;
+  if (!g_ctx[device]) {
+    int rc = mbvh_ctx_create(device, &g_ctx[device]);
+    if (rc) return rc;
+  }
+  *out = g_ctx[device];
+  return MBVH_OK;
+}
+
+static void cleanup_contexts(void* arg) {
+  (void)arg;
+  for (int i = 0; i < MAX_DEVICES; i++) {
+    if (g_ctx[i]) mbvh_ctx_destroy(g_ctx[i]);
+    g_ctx[i] = NULL;
+  }
+}
+
+static void finalize_mesh(napi_env env, void* data, void* hint) {
+  (void)env;
+  (void)hint;
+  MeshHandle* h = (MeshHandle*)data;
+  if (h) {
+    if (h->mesh) mbvh_mesh_free(h->mesh);
+    free(h);
+  }
+}
+
+static MeshHandle* get_mesh(napi_env env, napi_value v) {
+  void* p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok) return NULL;
+  MeshHandle* h = (MeshHandle*)p;
+  return (h && h->mesh) ? h : NULL;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* build(positions, stride, offset, index|null, ranges, opts) -> { roots, indirect }          */
+/* ------------------------------------------------------------------------------------------ */
+static napi_value js_build(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  if (!get_args(env, info, 6, argv)) return throw_msg(env, "build: bad arguments");
+  View pos, idx, rng;
+  if (!get_view(env, argv[0], &pos) || pos.type != napi_float32_array)
+    return throw_msg(env, "build: positions must be a Float32Array");
+  int32_t stride, offset;
+  if (!get_i32(env, argv[1], &stride) || !get_i32(env, argv[2], &offset) || stride <= 0 || offset < 0)
+    return throw_msg(env, "build: stride / offset must be integers");
+  int has_index = !is_nullish(env, argv[3]);
+  if (has_index && (!get_view(env, argv[3], &idx) || (idx.type != napi_uint16_array && idx.type != napi_uint32_array)))
+    return throw_msg(env, "build: index must be a Uint16Array or Uint32Array");
+  if (!get_view(env, argv[4], &rng) || rng.type != napi_uint32_array || rng.length < 2 || rng.length % 2)
+    return throw_msg(env, "build: ranges must be a Uint32Array of (offset, count) pairs");
+  mbvh_build_opts opts;
+  if (!get_named_i32(env, argv[5], "strategy", MBVH_CENTER, &opts.strategy) ||
+      !get_named_i32(env, argv[5], "maxDepth", 40, &opts.max_depth) ||
+      !get_named_i32(env, argv[5], "targetLeafSize", 10, &opts.target_leaf_size) ||
+      !get_named_i32(env, argv[5], "indirect", 0, &opts.indirect))
+    return throw_msg(env, "build: bad options");
+
+  mbvh_tree* tree = NULL;
+  int rc = mbvh_build((const float*)pos.data, pos.length / (size_t)stride, (uint32_t)stride, (uint32_t)offset,
+                      has_index ? idx.data : NULL, has_index ? idx.elt : 0, has_index ? idx.length : 0,
+                      (const uint32_t*)rng.data, (uint32_t)(rng.length / 2), &opts, &tree);
+  if (rc) return throw_status(env, rc, "mbvh_build");
+
+  uint32_t n_roots = 0;
+  int32_t ieb = 0;
+  uint64_t icount = 0;
+  mbvh_tree_info(tree, &n_roots, &ieb, &icount);
+  napi_value result, roots, indirect;
+  napi_create_object(env, &result);
+  napi_create_array_with_length(env, n_roots, &roots);
+  for (uint32_t r = 0; r < n_roots; r++) {
+    uint64_t bytes = 0;
+    mbvh_tree_root_bytes(tree, r, &bytes);
+    void* data = NULL;
+    napi_value ab;
+    if (napi_create_arraybuffer(env, (size_t)bytes, &data, &ab) != napi_ok) {
+      mbvh_tree_free(tree);
+      return throw_msg(env, "build: out of memory");
+    }
+    mbvh_tree_copy_root(tree, r, data, bytes);
+    napi_set_element(env, roots, r, ab);
+  }
+  if (ieb) {
+    void* data = NULL;
+    napi_value ab;
+    napi_create_arraybuffer(env, (size_t)(icount * (uint64_t)ieb), &data, &ab);
+    mbvh_tree_copy_indirect(tree, data, icount * (uint64_t)ieb);
+    napi_create_typedarray(env, ieb == 2 ? napi_uint16_array : napi_uint32_array, (size_t)icount, ab, 0, &indirect);
+  } else {
+    napi_get_null(env, &indirect);
+  }
+  mbvh_tree_free(tree);
+  napi_set_named_property(env, result, "roots", roots);
+  napi_set_named_property(env, result, "indirect", indirect);
+  return result;
+}
+
+/* deviceCount() */
+static napi_value js_device_count(napi_env env, napi_callback_info info) {
+  (void)info;
+  napi_value out;
+  napi_create_int32(env, mbvh_device_count(), &out);
+  return out;
+}
+
+/* uploadMesh(device, roots[], index|null, indirect|null, positions, stride, offset) -> external */
+static napi_value js_upload(napi_env env, napi_callback_info info) {
+  napi_value argv[7];
+  if (!get_args(env, info, 7, argv)) return throw_msg(env, "uploadMesh: bad arguments");
+  int32_t device;
+  if (!get_i32(env, argv[0], &device)) return throw_msg(env, "uploadMesh: device must be an integer");
+  bool is_arr = false;
+  napi_is_array(env, argv[1], &is_arr);
+  if (!is_arr) return throw_msg(env, "uploadMesh: roots must be an array of ArrayBuffers");
+  uint32_t n_roots = 0;
+  napi_get_array_length(env, argv[1], &n_roots);
+  if (n_roots == 0 || n_roots > 1u << 20) return throw_msg(env, "uploadMesh: roots must not be empty");
+  const void** root_ptrs = (const void**)calloc(n_roots, sizeof(void*));
+  uint64_t* root_bytes = (uint64_t*)calloc(n_roots, sizeof(uint64_t));
+  if (!root_ptrs || !root_bytes) {
+    free(root_ptrs);
+    free(root_bytes);
+    return throw_msg(env, "uploadMesh: out of memory");
+  }
+  for (uint32_t r = 0; r < n_roots; r++) {
+    napi_value e;
+    napi_get_element(env, argv[1], r, &e);
+    void* data = NULL;
+    size_t len = 0;
+    bool is_ab = false;
+    napi_is_arraybuffer(env, e, &is_ab);
+    if (is_ab) {
+      napi_get_arraybuffer_info(env, e, &data, &len);
+    } else {
+      View v;
+      if (!get_view(env, e, &v)) {
+        free(root_ptrs);
+        free(root_bytes);
+        return throw_msg(env, "uploadMesh: each root must be an ArrayBuffer");
+      }
+      data = v.data;
+      len = v.length * (size_t)v.elt;
+    }
+    root_ptrs[r] = data;
+    root_bytes[r] = len;
+  }
+  View idx, ind, pos;
+  int has_index = !is_nullish(env, argv[2]);
+  int has_ind = !is_nullish(env, argv[3]);
+  if ((has_index && !get_view(env, argv[2], &idx)) || (has_ind && !get_view(env, argv[3], &ind)) ||
+      !get_view(env, argv[4], &pos) || pos.type != napi_float32_array) {
+    free(root_ptrs);
+    free(root_bytes);
+    return throw_msg(env, "uploadMesh: index / indirect must be integer typed arrays, positions a Float32Array");
+  }
+  int32_t stride, offset;
+  if (!get_i32(env, argv[5], &stride) || !get_i32(env, argv[6], &offset) || stride <= 0 || offset < 0) {
+    free(root_ptrs);
+    free(root_bytes);
+    return throw_msg(env, "uploadMesh: stride / offset must be integers");
+  }
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, device, &ctx);
+  if (rc) {
+    free(root_ptrs);
+    free(root_bytes);
+    return throw_status(env, rc, "mbvh_ctx_create");
+  }
+  mbvh_mesh* mesh = NULL;
+  rc = mbvh_mesh_upload(ctx, root_ptrs, root_bytes, n_roots, has_index ? idx.data : NULL, has_index ? idx.elt : 0,
+                        has_index ? idx.length : 0, has_ind ? ind.data : NULL, has_ind ? ind.elt : 0,
+                        has_ind ? ind.length : 0, (const float*)pos.data, pos.length / (size_t)stride,
+                        (uint32_t)stride, (uint32_t)offset, &mesh);
+  free(root_ptrs);
+  free(root_bytes);
+  if (rc) return throw_status(env, rc, "mbvh_mesh_upload");
+  MeshHandle* h = (MeshHandle*)calloc(1, sizeof(MeshHandle));
+  if (!h) {
+    mbvh_mesh_free(mesh);
+    return throw_msg(env, "uploadMesh: out of memory");
+  }
+  h->mesh = mesh;
+  h->device = device;
+  napi_value ext;
+  if (napi_create_external(env, h, finalize_mesh, NULL, &ext) != napi_ok) {
+    mbvh_mesh_free(mesh);
+    free(h);
+    return throw_msg(env, "uploadMesh: cannot wrap the mesh");
+  }
+  return ext;
+}
+
+/* freeMesh(mesh): releases device memory now instead of at garbage collection */
+static napi_value js_free_mesh(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  void* p = NULL;
+  if (napi_get_value_external(env, argv[0], &p) == napi_ok && p) {
+    MeshHandle* h = (MeshHandle*)p;
+    if (h->mesh) mbvh_mesh_free(h->mesh);
+    h->mesh = NULL;
+  }
+  return NULL;
+}
+
+/* setSideMask(mesh, Uint8Array) */
+static napi_value js_set_side_mask(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  MeshHandle* h = get_mesh(env, argv[0]);
+  if (!h) return throw_msg(env, "setSideMask: invalid mesh");
+  View m;
+  if (!get_view(env, argv[1], &m) || m.elt != 1) return throw_msg(env, "setSideMask: mask must be a Uint8Array");
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, h->device, &ctx);
+  if (!rc) rc = mbvh_mesh_set_side_mask(ctx, h->mesh, (const uint8_t*)m.data, m.length);
+  if (rc) return throw_status(env, rc, "mbvh_mesh_set_side_mask");
+  return NULL;
+}
+
+static int out_view(napi_env env, napi_value v, napi_typedarray_type want, size_t n, void** ptr) {
+  *ptr = NULL;
+  if (is_nullish(env, v)) return 1;
+  View w;
+  if (!get_view(env, v, &w) || w.type != want || w.length < n) return 0;
+  *ptr = w.data;
+  return 1;
+}
+
+/* raycastFirst(mesh, origins, dirs, side, near, far, outFace, outDist, outPoint, outBary) */
+static napi_value js_raycast_first(napi_env env, napi_callback_info info) {
+  napi_value argv[10];
+  get_args(env, info, 10, argv);
+  MeshHandle* h = get_mesh(env, argv[0]);
+  if (!h) return throw_msg(env, "raycastFirst: invalid mesh");
+  View o, d;
+  if (!get_view(env, argv[1], &o) || !get_view(env, argv[2], &d) || o.type != napi_float32_array ||
+      d.type != napi_float32_array || o.length != d.length || o.length % 3)
+    return throw_msg(env, "raycastFirst: origins / directions must be Float32Arrays of equal length 3n");
+  const size_t n = o.length / 3;
+  int32_t side;
+  double nr, fr;
+  if (!get_i32(env, argv[3], &side) || !get_f64(env, argv[4], &nr) || !get_f64(env, argv[5], &fr))
+    return throw_msg(env, "raycastFirst: side / near / far must be numbers");
+  void *face, *dist, *pt, *bc;
+  if (!out_view(env, argv[6], napi_uint32_array, n, &face) || !face ||
+      !out_view(env, argv[7], napi_float32_array, n, &dist) || !out_view(env, argv[8], napi_float32_array, 3 * n, &pt) ||
+      !out_view(env, argv[9], napi_float32_array, 3 * n, &bc))
+    return throw_msg(env, "raycastFirst: output arrays have the wrong type or length");
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, h->device, &ctx);
+  if (!rc)
+    rc = mbvh_raycast_first(ctx, h->mesh, (const float*)o.data, (const float*)d.data, n, side, to_float(nr),
+                            to_float(fr), (uint32_t*)face, (float*)dist, (float*)pt, (float*)bc);
+  if (rc) return throw_status(env, rc, "mbvh_raycast_first");
+  return NULL;
+}
+
+/* raycast(mesh, origins, dirs, side, near, far) -> { offsets, faceIndex, distance, point, barycoord } */
+static napi_value js_raycast(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  get_args(env, info, 6, argv);
+  MeshHandle* h = get_mesh(env, argv[0]);
+  if (!h) return throw_msg(env, "raycast: invalid mesh");
+  View o, d;
+  if (!get_view(env, argv[1], &o) || !get_view(env, argv[2], &d) || o.type != napi_float32_array ||
+      d.type != napi_float32_array || o.length != d.length || o.length % 3)
+    return throw_msg(env, "raycast: origins / directions must be Float32Arrays of equal length 3n");
+  const size_t n = o.length / 3;
+  int32_t side;
+  double nr, fr;
+  if (!get_i32(env, argv[3], &side) || !get_f64(env, argv[4], &nr) || !get_f64(env, argv[5], &fr))
+    return throw_msg(env, "raycast: side / near / far must be numbers");
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, h->device, &ctx);
+  uint64_t* offs = (uint64_t*)calloc(n + 1, sizeof(uint64_t));
+  if (!offs) return throw_msg(env, "raycast: out of memory");
+  if (!rc)
+    rc = mbvh_raycast(ctx, h->mesh, (const float*)o.data, (const float*)d.data, n, side, to_float(nr), to_float(fr), offs);
+  if (rc) {
+    free(offs);
+    return throw_status(env, rc, "mbvh_raycast");
+  }
+  const size_t total = (size_t)offs[n];
+  napi_value result, offsets, face, dist, pt, bc, ab;
+  void *p_off, *p_face, *p_dist, *p_pt, *p_bc;
+  napi_create_arraybuffer(env, (n + 1) * sizeof(double), &p_off, &ab);
+  napi_create_typedarray(env, napi_float64_array, n + 1, ab, 0, &offsets);
+  for (size_t i = 0; i <= n; i++) ((double*)p_off)[i] = (double)offs[i];
+  free(offs);
+  napi_create_arraybuffer(env, total * 4, &p_face, &ab);
+  napi_create_typedarray(env, napi_uint32_array, total, ab, 0, &face);
+  napi_create_arraybuffer(env, total * 4, &p_dist, &ab);
+  napi_create_typedarray(env, napi_float32_array, total, ab, 0, &dist);
+  napi_create_arraybuffer(env, total * 12, &p_pt, &ab);
+  napi_create_typedarray(env, napi_float32_array, 3 * total, ab, 0, &pt);
+  napi_create_arraybuffer(env, total * 12, &p_bc, &ab);
+  napi_create_typedarray(env, napi_float32_array, 3 * total, ab, 0, &bc);
+  rc = mbvh_raycast_fetch(ctx, (uint32_t*)p_face, (float*)p_dist, (float*)p_pt, (float*)p_bc, total);
+  if (rc) return throw_status(env, rc, "mbvh_raycast_fetch");
+  napi_create_object(env, &result);
+  napi_set_named_property(env, result, "offsets", offsets);
+  napi_set_named_property(env, result, "faceIndex", face);
+  napi_set_named_property(env, result, "distance", dist);
+  napi_set_named_property(env, result, "point", pt);
+  napi_set_named_property(env, result, "barycoord", bc);
+  return result;
+}
+
+/* closestPoint(mesh, points, minT, maxT, outFace, outDist, outPoint, outBary) */
+static napi_value js_closest_point(napi_env env, napi_callback_info info) {
+  napi_value argv[8];
+  get_args(env, info, 8, argv);
+  MeshHandle* h = get_mesh(env, argv[0]);
+  if (!h) return throw_msg(env, "closestPoint: invalid mesh");
+  View q;
+  if (!get_view(env, argv[1], &q) || q.type != napi_float32_array || q.length % 3)
+    return throw_msg(env, "closestPoint: points must be a Float32Array of length 3n");
+  const size_t n = q.length / 3;
+  double mn, mx;
+  if (!get_f64(env, argv[2], &mn) || !get_f64(env, argv[3], &mx))
+    return throw_msg(env, "closestPoint: thresholds must be numbers");
+  void *face, *dist, *pt, *bc;
+  if (!out_view(env, argv[4], napi_uint32_array, n, &face) || !face ||
+      !out_view(env, argv[5], napi_float32_array, n, &dist) || !out_view(env, argv[6], napi_float32_array, 3 * n, &pt) ||
+      !out_view(env, argv[7], napi_float32_array, 3 * n, &bc))
+    return throw_msg(env, "closestPoint: output arrays have the wrong type or length");
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, h->device, &ctx);
+  if (!rc)
+    rc = mbvh_closest_point(ctx, h->mesh, (const float*)q.data, n, to_float(mn), to_float(mx), (uint32_t*)face,
+                            (float*)dist, (float*)pt, (float*)bc);
+  if (rc) return throw_status(env, rc, "mbvh_closest_point");
+  return NULL;
+}
+
+/* meshInfo(mesh) -> { triCount, vertexCount, maxDepth } */
+static napi_value js_mesh_info(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  MeshHandle* h = get_mesh(env, argv[0]);
+  if (!h) return throw_msg(env, "meshInfo: invalid mesh");
+  uint64_t t = 0, v = 0;
+  uint32_t d = 0;
+  int rc = mbvh_mesh_info(h->mesh, &t, &v, &d);
+  if (rc) return throw_status(env, rc, "mbvh_mesh_info");
+  napi_value out, a, b, c;
+  napi_create_object(env, &out);
+  napi_create_double(env, (double)t, &a);
+  napi_create_double(env, (double)v, &b);
+  napi_create_uint32(env, d, &c);
+  napi_set_named_property(env, out, "triCount", a);
+  napi_set_named_property(env, out, "vertexCount", b);
+  napi_set_named_property(env, out, "maxDepth", c);
+  return out;
+}
+
+static napi_value js_abi_version(napi_env env, napi_callback_info info) {
+  (void)info;
+  napi_value out;
+  napi_create_uint32(env, mbvh_abi_version(), &out);
+  return out;
+}
+
+#define EXPORT(name, fn)                                                        \
+  do {                                                                          \
+    napi_value f;                                                               \
+    napi_create_function(env, name, NAPI_AUTO_LENGTH, fn, NULL, &f);            \
+    napi_set_named_property(env, exports, name, f);                             \
+  } while (0)
+
+static napi_value init(napi_env env, napi_value exports) {
+  EXPORT("abiVersion", js_abi_version);
+  EXPORT("deviceCount", js_device_count);
+  EXPORT("build", js_build);
+  EXPORT("uploadMesh", js_upload);
+  EXPORT("freeMesh", js_free_mesh);
+  EXPORT("meshInfo", js_mesh_info);
+  EXPORT("setSideMask", js_set_side_mask);
+  EXPORT("raycastFirst", js_raycast_first);
+  EXPORT("closestPoint", js_closest_point);
+  EXPORT("raycast", js_raycast);
+  napi_add_env_cleanup_hook(env, cleanup_contexts, NULL);
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

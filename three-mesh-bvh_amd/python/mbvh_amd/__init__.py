@@ -28,6 +28,7 @@ EXPORTED_SYMBOLS = (
     "mbvh_mesh_upload", "mbvh_mesh_free", "mbvh_mesh_info", "mbvh_mesh_set_side_mask",
     "mbvh_raycast_first", "mbvh_raycast_first_d",
     "mbvh_closest_point", "mbvh_closest_point_d",
+    "mbvh_raycast", "mbvh_raycast_fetch", "mbvh_raycast_count_d", "mbvh_raycast_fill_d",
 )
 
 
@@ -76,6 +77,10 @@ def library() -> C.CDLL:
         "mbvh_raycast_first_d": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp]),
         "mbvh_closest_point": (C.c_int, [vp, vp, vp, u64, f32, f32, vp, vp, vp, vp]),
         "mbvh_closest_point_d": (C.c_int, [vp, vp, vp, u64, f32, f32, vp, vp, vp, vp, vp]),
+        "mbvh_raycast": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp]),
+        "mbvh_raycast_fetch": (C.c_int, [vp, vp, vp, vp, vp, u64]),
+        "mbvh_raycast_count_d": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp]),
+        "mbvh_raycast_fill_d": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, u64, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -233,6 +238,23 @@ class Mesh:
                                                 near, far, _ptr(face), _ptr(dist), _ptr(point), _ptr(bary)))
         return {"face": face, "distance": dist, "point": point, "barycoord": bary}
 
+    def raycast(self, origins: np.ndarray, directions: np.ndarray, side: int = FRONT_SIDE,
+                near: float = 0.0, far: float = float("inf")):
+        """All hits: returns offsets (n+1) and packed per-hit arrays in the reference's order."""
+        o = np.ascontiguousarray(origins, dtype=np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, dtype=np.float32).reshape(-1, 3)
+        n = o.shape[0]
+        offsets = np.zeros(n + 1, np.uint64)
+        lib = self.ctx._lib
+        _check(lib.mbvh_raycast(self.ctx.handle, self.handle, _ptr(o), _ptr(d), n, side, near, far, _ptr(offsets)))
+        total = int(offsets[-1])
+        face = np.empty(total, np.uint32)
+        dist = np.empty(total, np.float32)
+        point = np.empty((total, 3), np.float32)
+        bary = np.empty((total, 3), np.float32)
+        _check(lib.mbvh_raycast_fetch(self.ctx.handle, _ptr(face), _ptr(dist), _ptr(point), _ptr(bary), total))
+        return {"offsets": offsets, "face": face, "distance": dist, "point": point, "barycoord": bary}
+
     def closest_point(self, points: np.ndarray, min_threshold: float = 0.0,
                       max_threshold: float = float("inf")):
         q = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
@@ -253,6 +275,20 @@ class Mesh:
             self.ctx.handle, self.handle, C.c_void_p(origins_ptr), C.c_void_p(dirs_ptr), n, side, near, far,
             C.c_void_p(face_ptr), C.c_void_p(dist_ptr or None), C.c_void_p(point_ptr or None),
             C.c_void_p(bary_ptr or None), C.c_void_p(stream or None)))
+
+    def raycast_count_d(self, origins_ptr: int, dirs_ptr: int, n: int, counts_ptr: int, side: int = FRONT_SIDE,
+                        near: float = 0.0, far: float = float("inf"), stream: int = 0) -> None:
+        _check(self.ctx._lib.mbvh_raycast_count_d(
+            self.ctx.handle, self.handle, C.c_void_p(origins_ptr), C.c_void_p(dirs_ptr), n, side, near, far,
+            C.c_void_p(counts_ptr), C.c_void_p(stream or None)))
+
+    def raycast_fill_d(self, origins_ptr: int, dirs_ptr: int, n: int, offsets_ptr: int, capacity: int,
+                       face_ptr: int, dist_ptr: int = 0, point_ptr: int = 0, bary_ptr: int = 0,
+                       side: int = FRONT_SIDE, near: float = 0.0, far: float = float("inf"), stream: int = 0) -> None:
+        _check(self.ctx._lib.mbvh_raycast_fill_d(
+            self.ctx.handle, self.handle, C.c_void_p(origins_ptr), C.c_void_p(dirs_ptr), n, side, near, far,
+            C.c_void_p(offsets_ptr), capacity, C.c_void_p(face_ptr), C.c_void_p(dist_ptr or None),
+            C.c_void_p(point_ptr or None), C.c_void_p(bary_ptr or None), C.c_void_p(stream or None)))
 
     def closest_point_d(self, points_ptr: int, n: int, face_ptr: int, dist_ptr: int = 0, point_ptr: int = 0,
                         bary_ptr: int = 0, min_threshold: float = 0.0, max_threshold: float = float("inf"),

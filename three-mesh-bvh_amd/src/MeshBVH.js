@@ -1,0 +1,399 @@
+'use strict';
+// MeshBVH with the reference's query API (src/core/MeshBVH.js, src/index.d.ts:151-167), backed by
+// the MI355X kernels: the BVH is built on the host in the reference's packed format, uploaded to
+// HBM once (on first query), and every query runs on the GPU. Per-call methods take one ray or
+// point; the *Batch methods take packed Float32Arrays of many.
+const C = require('./Constants.js');
+const native = require('./native.js');
+const M = require('./math.js');
+const GU = require('./geometryUtils.js');
+
+const FLOATS_PER_VEC = 3;
+
+function getSide(materialOrSide) {
+  return materialOrSide && materialOrSide.isMaterial ? materialOrSide.side : materialOrSide;
+}
+
+// Facing mask for one material side: bit 0 front-facing hits, bit 1 back-facing hits
+// (checkIntersection, src/utils/ThreeRayIntersectUtilities.js:21-47; an undefined side behaves as
+// FrontSide there because only BackSide / DoubleSide are tested).
+function sideMask(side) {
+  if (side === C.BackSide) return 2;
+  if (side === C.DoubleSide) return 3;
+  return 1;
+}
+
+class MeshBVH {
+  constructor(geometry, options = {}) {
+    if (!geometry || !geometry.isBufferGeometry) {
+      throw new Error('BVH: Only BufferGeometries are supported.');
+    } else if (geometry.index && geometry.index.isInterleavedBufferAttribute) {
+      throw new Error('BVH: InterleavedBufferAttribute is not supported for the index attribute.');
+    }
+    if (options.useSharedArrayBuffer && typeof SharedArrayBuffer === 'undefined') {
+      throw new Error('BVH: SharedArrayBuffer is not available.');
+    }
+    if (options.maxLeafTris) {
+      options = Object.assign({}, options, { targetLeafSize: options.maxLeafTris });
+    }
+    this.geometry = geometry;
+    this._roots = null;
+    this._indirectBuffer = null;
+    this._gpu = null;        // native mesh handle (uploaded on first query)
+    this._maskKey = null;    // material-array signature of the uploaded facing masks
+    options = Object.assign({}, C.DEFAULT_OPTIONS, options);
+    if (!options[C.SKIP_GENERATION]) this.init(options);
+  }
+
+  get indirect() {
+    return !!this._indirectBuffer;
+  }
+
+  get resolveTriangleIndex() {
+    return this.indirect ? i => this._indirectBuffer[i] : i => i;
+  }
+
+  get primitiveStride() {
+    return 3;
+  }
+
+  init(options) {
+    const geometry = this.geometry;
+    let ranges;
+    if (options.indirect) {
+      ranges = GU.getRootPrimitiveRanges(geometry, options.range, 3);
+    } else {
+      GU.ensureIndex(geometry, options);
+      ranges = GU.getRootPrimitiveRanges(geometry, options.range, 3);
+    }
+    const flat = new Uint32Array(2 * ranges.length);
+    ranges.forEach((r, i) => {
+      flat[2 * i] = r.offset;
+      flat[2 * i + 1] = r.count;
+    });
+    const pos = GU.positionLayout(geometry);
+    const index = geometry.index ? geometry.index.array : null;
+    const out = native.load().build(pos.array, pos.stride, pos.offset, index, flat, {
+      strategy: options.strategy,
+      maxDepth: options.maxDepth,
+      targetLeafSize: options.targetLeafSize,
+      indirect: !!options.indirect,
+    });
+    let roots = out.roots;
+    if (options.useSharedArrayBuffer) {
+      roots = roots.map(r => {
+        const s = new SharedArrayBuffer(r.byteLength);
+        new Uint8Array(s).set(new Uint8Array(r));
+        return s;
+      });
+    }
+    this._roots = roots;
+    this._indirectBuffer = out.indirect;
+    if (geometry.index && !options.indirect) geometry.index.needsUpdate = true;
+    if (!geometry.boundingBox && options.setBoundingBox) {
+      const bb = this.getBoundingBox({ min: new M.Vec3(), max: new M.Vec3() });
+      geometry.boundingBox = bb;
+    }
+  }
+
+  // ---- serialization (src/core/MeshBVH.js:112-232) ------------------------------------------
+
+  static serialize(bvh, options = {}) {
+    options = Object.assign({ cloneBuffers: true }, options);
+    const geometry = bvh.geometry;
+    const indexAttr = geometry.index;
+    const result = { version: 1, roots: null, index: null, indirectBuffer: null };
+    if (options.cloneBuffers) {
+      result.roots = bvh._roots.map(r => r.slice(0));
+      result.index = indexAttr ? indexAttr.array.slice() : null;
+      result.indirectBuffer = bvh._indirectBuffer ? bvh._indirectBuffer.slice() : null;
+    } else {
+      result.roots = bvh._roots;
+      result.index = indexAttr ? indexAttr.array : null;
+      result.indirectBuffer = bvh._indirectBuffer;
+    }
+    return result;
+  }
+
+  static deserialize(data, geometry, options = {}) {
+    options = Object.assign({ setIndex: true, indirect: Boolean(data.indirectBuffer) }, options);
+    const { index, roots, indirectBuffer } = data;
+    if (!data.version) {
+      // version 0 stored absolute uint32 offsets of right children; convert to node distances
+      for (const root of roots) {
+        const u32 = new Uint32Array(root);
+        const u16 = new Uint16Array(root);
+        for (let node = 0, l = root.byteLength / C.BYTES_PER_NODE; node < l; node++) {
+          const n32 = C.UINT32_PER_NODE * node;
+          if (u16[2 * n32 + 15] !== C.IS_LEAFNODE_FLAG) u32[n32 + 6] = u32[n32 + 6] / C.UINT32_PER_NODE - node;
+        }
+      }
+    }
+    const bvh = new MeshBVH(geometry, Object.assign({}, options, { [C.SKIP_GENERATION]: true }));
+    bvh._roots = roots;
+    bvh._indirectBuffer = indirectBuffer || null;
+    if (options.setIndex) {
+      const indexAttr = geometry.index;
+      if (!indexAttr) {
+        const attr = GU.makeIndexAttribute(geometry, index);
+        if (typeof geometry.setIndex === 'function') geometry.setIndex(attr);
+        else geometry.index = attr;
+      } else if (indexAttr.array !== index) {
+        indexAttr.array.set(index);
+        indexAttr.needsUpdate = true;
+      }
+    }
+    return bvh;
+  }
+
+  getBoundingBox(target) {
+    let mnx = Infinity, mny = Infinity, mnz = Infinity, mxx = -Infinity, mxy = -Infinity, mxz = -Infinity;
+    for (const root of this._roots) {
+      const f = new Float32Array(root, 0, 6);
+      mnx = Math.min(mnx, f[0]);
+      mny = Math.min(mny, f[1]);
+      mnz = Math.min(mnz, f[2]);
+      mxx = Math.max(mxx, f[3]);
+      mxy = Math.max(mxy, f[4]);
+      mxz = Math.max(mxz, f[5]);
+    }
+    if (target.min && typeof target.min.set === 'function') {
+      target.min.set(mnx, mny, mnz);
+      target.max.set(mxx, mxy, mxz);
+    } else {
+      target.min = new M.Vec3(mnx, mny, mnz);
+      target.max = new M.Vec3(mxx, mxy, mxz);
+    }
+    return target;
+  }
+
+  // ---- device residency ------------------------------------------------------------------------
+
+  // Uploads the BVH, index and positions to the GPU (done automatically on the first query).
+  upload() {
+    const addon = native.load();
+    if (this._gpu) addon.freeMesh(this._gpu);
+    const geo = this.geometry;
+    const pos = GU.positionLayout(geo);
+    this._gpu = addon.uploadMesh(native.getDevice(), this._roots, geo.index ? geo.index.array : null,
+      this._indirectBuffer, pos.array, pos.stride, pos.offset);
+    this._maskKey = null;
+    return this;
+  }
+
+  // Releases the device copy now (otherwise released when the BVH is garbage collected).
+  dispose() {
+    if (this._gpu) native.load().freeMesh(this._gpu);
+    this._gpu = null;
+  }
+
+  _mesh() {
+    if (!this._gpu) this.upload();
+    return this._gpu;
+  }
+
+  // Side argument for the kernels; material arrays become per-face facing masks from the groups
+  // (intersectTri's group loop, src/utils/ThreeRayIntersectUtilities.js:171-202).
+  _sideFor(materialOrSide) {
+    if (!Array.isArray(materialOrSide)) {
+      const side = getSide(materialOrSide);
+      return side === C.BackSide ? C.BackSide : (side === C.DoubleSide ? C.DoubleSide : C.FrontSide);
+    }
+    const groups = this.geometry.groups || [];
+    const sides = groups.map(g => sideMask(getSide(materialOrSide[g.materialIndex])));
+    const key = groups.map((g, i) => `${g.start}:${g.count}:${sides[i]}`).join(',');
+    if (key !== this._maskKey) {
+      const faceCount = (this.geometry.index ? this.geometry.index.count : this.geometry.attributes.position.count) / 3;
+      const mask = new Uint8Array(faceCount);
+      groups.forEach((g, i) => {
+        const end = Math.min(faceCount, (g.start + g.count) / 3);
+        for (let f = Math.floor(g.start / 3); f < end; f++) mask[f] |= sides[i];
+      });
+      native.load().setSideMask(this._mesh(), mask);
+      this._maskKey = key;
+    }
+    return 3;  // MBVH_SIDE_PER_TRIANGLE
+  }
+
+  // ---- batch queries ---------------------------------------------------------------------------
+
+  // First hit for n rays: origins / directions are Float32Arrays of length 3n. Returns packed
+  // results; faceIndex is 0xFFFFFFFF and distance Infinity for a miss.
+  raycastFirstBatch(origins, directions, options = {}) {
+    const o = origins instanceof Float32Array ? origins : Float32Array.from(origins);
+    const d = directions instanceof Float32Array ? directions : Float32Array.from(directions);
+    const n = o.length / FLOATS_PER_VEC;
+    const side = this._sideFor(options.side !== undefined ? options.side : (options.material !== undefined ? options.material : C.FrontSide));
+    const out = {
+      faceIndex: new Uint32Array(n), distance: new Float32Array(n),
+      point: new Float32Array(3 * n), barycoord: new Float32Array(3 * n),
+    };
+    native.load().raycastFirst(this._mesh(), o, d, side, options.near !== undefined ? options.near : 0,
+      options.far !== undefined ? options.far : Infinity, out.faceIndex, out.distance, out.point, out.barycoord);
+    return out;
+  }
+
+  // All hits for n rays, in the reference's traversal order: hits of ray i are entries
+  // offsets[i] .. offsets[i + 1] - 1 of the packed arrays.
+  raycastBatch(origins, directions, options = {}) {
+    const o = origins instanceof Float32Array ? origins : Float32Array.from(origins);
+    const d = directions instanceof Float32Array ? directions : Float32Array.from(directions);
+    const side = this._sideFor(options.side !== undefined ? options.side : (options.material !== undefined ? options.material : C.FrontSide));
+    return native.load().raycast(this._mesh(), o, d, side, options.near !== undefined ? options.near : 0,
+      options.far !== undefined ? options.far : Infinity);
+  }
+
+  // Closest point for n query points (Float32Array of length 3n).
+  closestPointToPointBatch(points, options = {}) {
+    const q = points instanceof Float32Array ? points : Float32Array.from(points);
+    const n = q.length / FLOATS_PER_VEC;
+    const out = {
+      faceIndex: new Uint32Array(n), distance: new Float32Array(n),
+      point: new Float32Array(3 * n), barycoord: new Float32Array(3 * n),
+    };
+    native.load().closestPoint(this._mesh(), q, options.minThreshold || 0,
+      options.maxThreshold !== undefined ? options.maxThreshold : Infinity, out.faceIndex, out.distance, out.point,
+      out.barycoord);
+    return out;
+  }
+
+  // ---- the reference's per-call API ------------------------------------------------------------
+
+  raycastFirst(ray, materialOrSide = C.FrontSide, near = 0, far = Infinity) {
+    const o = new Float32Array([ray.origin.x, ray.origin.y, ray.origin.z]);
+    const d = new Float32Array([ray.direction.x, ray.direction.y, ray.direction.z]);
+    const r = this.raycastFirstBatch(o, d, { side: materialOrSide, near, far });
+    if (r.faceIndex[0] === C.NO_HIT) return null;
+    return this._makeHit(ray, materialOrSide, r.faceIndex[0], r.distance[0],
+      [r.point[0], r.point[1], r.point[2]], [r.barycoord[0], r.barycoord[1], r.barycoord[2]]);
+  }
+
+  // All hits of one ray (src/core/MeshBVH.js:532-545). With a material array a triangle yields
+  // one hit per group whose material admits it, in group order, as intersectTri does.
+  raycast(ray, materialOrSide = C.FrontSide, near = 0, far = Infinity) {
+    const o = new Float32Array([ray.origin.x, ray.origin.y, ray.origin.z]);
+    const d = new Float32Array([ray.direction.x, ray.direction.y, ray.direction.z]);
+    const r = this.raycastBatch(o, d, { side: materialOrSide, near, far });
+    const hits = [];
+    for (let k = 0; k < r.faceIndex.length; k++) {
+      const p = [r.point[3 * k], r.point[3 * k + 1], r.point[3 * k + 2]];
+      const b = [r.barycoord[3 * k], r.barycoord[3 * k + 1], r.barycoord[3 * k + 2]];
+      if (!Array.isArray(materialOrSide)) {
+        hits.push(this._makeHit(ray, materialOrSide, r.faceIndex[k], r.distance[k], p, b));
+        continue;
+      }
+      const first = this._makeHit(ray, materialOrSide, r.faceIndex[k], r.distance[k], p, b);
+      const back = first.face.normal.x * ray.direction.x + first.face.normal.y * ray.direction.y +
+        first.face.normal.z * ray.direction.z > 0;
+      const start = r.faceIndex[k] * 3;
+      for (const g of this.geometry.groups || []) {
+        if (start >= g.start && start < g.start + g.count &&
+          (sideMask(getSide(materialOrSide[g.materialIndex])) & (back ? 2 : 1))) {
+          const h = this._makeHit(ray, materialOrSide, r.faceIndex[k], r.distance[k], p, b);
+          h.face.materialIndex = g.materialIndex;
+          hits.push(h);
+        }
+      }
+    }
+    return hits;
+  }
+
+  closestPointToPoint(point, target = {}, minThreshold = 0, maxThreshold = Infinity) {
+    const q = new Float32Array([point.x, point.y, point.z]);
+    const r = this.closestPointToPointBatch(q, { minThreshold, maxThreshold });
+    if (r.faceIndex[0] === C.NO_HIT) return null;
+    if (!target.point) target.point = M.makeVec3(point, r.point[0], r.point[1], r.point[2]);
+    else target.point.set(r.point[0], r.point[1], r.point[2]);
+    target.distance = r.distance[0];
+    target.faceIndex = r.faceIndex[0];
+    return target;
+  }
+
+  // raycastObject3D (src/core/MeshBVH.js:447-494): world ray to local, query, hits back to world.
+  raycastObject3D(object, raycaster, intersects = []) {
+    const { material } = object;
+    if (material === undefined) return undefined;
+    const mw = object.matrixWorld.elements;
+    const inv = M.invert(mw);
+    const lo = M.applyToPoint(inv, raycaster.ray.origin);
+    const ld = M.transformDirection(inv, raycaster.ray.direction);
+    const s = M.matrixScale(mw);
+    const scaleFactor = Math.hypot(ld[0] * s[0], ld[1] * s[1], ld[2] * s[2]);
+    const near = raycaster.near / scaleFactor;
+    const far = raycaster.far / scaleFactor;
+    const localRay = {
+      origin: M.makeVec3(raycaster.ray.origin, lo[0], lo[1], lo[2]),
+      direction: M.makeVec3(raycaster.ray.direction, ld[0], ld[1], ld[2]),
+    };
+    const convert = hit => {
+      // convertRaycastIntersect (src/utils/GeometryRayIntersectUtilities.js:3-17)
+      const p = M.applyToPoint(mw, hit.point);
+      hit.point.set(p[0], p[1], p[2]);
+      hit.distance = hit.point.distanceTo(raycaster.ray.origin);
+      hit.object = object;
+      return hit;
+    };
+    if (raycaster.firstHitOnly === true) {
+      const hit = this.raycastFirst(localRay, material, near, far);
+      if (hit) intersects.push(convert(hit));
+    } else {
+      const hits = this.raycast(localRay, material, near, far);
+      for (const hit of hits) intersects.push(convert(hit));
+    }
+    return intersects;
+  }
+
+  // Builds the three.js intersection object (checkBufferGeometryIntersection,
+  // src/utils/ThreeRayIntersectUtilities.js:49-145) from the kernel's face, point and barycoord.
+  _makeHit(ray, materialOrSide, faceIndex, distance, p, bary) {
+    const geo = this.geometry;
+    const idx = geo.index ? geo.index.array : null;
+    const a = idx ? idx[3 * faceIndex] : 3 * faceIndex;
+    const b = idx ? idx[3 * faceIndex + 1] : 3 * faceIndex + 1;
+    const c = idx ? idx[3 * faceIndex + 2] : 3 * faceIndex + 2;
+    const posAttr = geo.attributes.position;
+    const va = [posAttr.getX(a), posAttr.getY(a), posAttr.getZ(a)];
+    const vb = [posAttr.getX(b), posAttr.getY(b), posAttr.getZ(b)];
+    const vc = [posAttr.getX(c), posAttr.getY(c), posAttr.getZ(c)];
+    const like = ray.origin;
+    const hit = { distance, point: M.makeVec3(like, p[0], p[1], p[2]) };
+    const interp = (attr, Make) => {
+      const n = attr.itemSize;
+      const get = (i, k) => (k === 0 ? attr.getX(i) : k === 1 ? attr.getY(i) : attr.getZ(i));
+      const v = [];
+      for (let k = 0; k < n && k < 3; k++) v.push(get(a, k) * bary[0] + get(b, k) * bary[1] + get(c, k) * bary[2]);
+      return Make(v);
+    };
+    const attrs = geo.attributes;
+    if (attrs.uv) hit.uv = interp(attrs.uv, v => M.makeVec2(null, v[0], v[1]));
+    if (attrs.uv1) hit.uv1 = interp(attrs.uv1, v => M.makeVec2(null, v[0], v[1]));
+    if (attrs.normal) {
+      const nrm = interp(attrs.normal, v => M.makeVec3(like, v[0], v[1], v[2]));
+      if (nrm.x * ray.direction.x + nrm.y * ray.direction.y + nrm.z * ray.direction.z > 0) nrm.multiplyScalar(-1);
+      hit.normal = nrm;
+    }
+    const n = M.triangleNormal(va, vb, vc);
+    let materialIndex = 0;
+    if (Array.isArray(materialOrSide)) {
+      // first group holding the face whose material admits the hit's facing
+      const e1 = [vb[0] - va[0], vb[1] - va[1], vb[2] - va[2]];
+      const e2 = [vc[0] - va[0], vc[1] - va[1], vc[2] - va[2]];
+      const nx = e1[1] * e2[2] - e1[2] * e2[1], ny = e1[2] * e2[0] - e1[0] * e2[2], nz = e1[0] * e2[1] - e1[1] * e2[0];
+      const back = nx * ray.direction.x + ny * ray.direction.y + nz * ray.direction.z > 0;
+      const first = faceIndex * 3;
+      for (const g of geo.groups || []) {
+        if (first >= g.start && first < g.start + g.count &&
+          (sideMask(getSide(materialOrSide[g.materialIndex])) & (back ? 2 : 1))) {
+          materialIndex = g.materialIndex;
+          break;
+        }
+      }
+    }
+    hit.face = { a, b, c, normal: M.makeVec3(like, n[0], n[1], n[2]), materialIndex };
+    hit.faceIndex = faceIndex;
+    hit.barycoord = M.makeVec3(like, bary[0], bary[1], bary[2]);
+    return hit;
+  }
+}
+
+module.exports = { MeshBVH };

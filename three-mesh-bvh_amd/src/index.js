@@ -1,0 +1,33 @@
+'use strict';
+// Public entry point: the query half of three-mesh-bvh's API (src/index.js of the reference).
+const C = require('./Constants.js');
+const { MeshBVH } = require('./MeshBVH.js');
+const ext = require('./ExtensionUtilities.js');
+const { getTriangleHitPointInfo } = require('./TriangleUtilities.js');
+const native = require('./native.js');
+const { Vec3 } = require('./math.js');
+
+module.exports = {
+  MeshBVH,
+  CENTER: C.CENTER,
+  AVERAGE: C.AVERAGE,
+  SAH: C.SAH,
+  NOT_INTERSECTED: C.NOT_INTERSECTED,
+  INTERSECTED: C.INTERSECTED,
+  CONTAINED: C.CONTAINED,
+  SKIP_GENERATION: C.SKIP_GENERATION,
+  FrontSide: C.FrontSide,
+  BackSide: C.BackSide,
+  DoubleSide: C.DoubleSide,
+  acceleratedRaycast: ext.acceleratedRaycast,
+  computeBoundsTree: ext.computeBoundsTree,
+  disposeBoundsTree: ext.disposeBoundsTree,
+  computeBatchedBoundsTree: ext.computeBatchedBoundsTree,
+  disposeBatchedBoundsTree: ext.disposeBatchedBoundsTree,
+  setFallbackRaycast: ext.setFallbackRaycast,
+  getTriangleHitPointInfo,
+  setDevice: native.setDevice,
+  getDevice: native.getDevice,
+  deviceCount: () => native.load().deviceCount(),
+  Vec3,
+};
