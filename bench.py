@@ -39,20 +39,34 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-order", action="store_true",
+                    help="trace queries in their given order (no spatial ordering pass in the step)")
     ap.add_argument("--cpu-sample", type=int, default=16, help="1/N stride of the CPU-baseline sample")
+    ap.add_argument("--only-primary", action="store_true",
+                    help="skip the differentiable (C3, C4) and batched (C5) measurements")
+    ap.add_argument("--c3-rays", type=int, default=1 << 20)
+    ap.add_argument("--c4-grid", type=int, default=256)
+    ap.add_argument("--c5-rays", type=int, default=1 << 22)
+    ap.add_argument("--c5-geometries", type=int, default=512)
     ap.add_argument("--traffic-calibration", action="store_true",
                     help="also run one 256 MiB device copy, the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiling run")
     return ap.parse_args()
 
 
-def timed_kernel(torch, launch, steps, warmup, stream):
+def timed_kernel(torch, launch, steps, warmup, stream, pre=None):
+    """Runs `pre` (if any) then `launch` per step; returns the wall time of all steps and the per-step
+    duration of `launch` alone from HIP events recorded on `stream` around it."""
     for _ in range(warmup):
+        if pre:
+            pre()
         launch()
     torch.cuda.synchronize()
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     t0 = time.perf_counter()
     for a, b in evs:
+        if pre:
+            pre()
         a.record(stream)
         launch()
         b.record(stream)
@@ -60,6 +74,194 @@ def timed_kernel(torch, launch, steps, warmup, stream):
     wall = time.perf_counter() - t0
     kern = [a.elapsed_time(b) for a, b in evs]
     return wall, kern
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def event_ms(torch, stream, fn, reps):
+    """Average duration of fn() over reps runs, timed with HIP events on `stream` (after one warm-up)."""
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def bench_differentiable(torch, M, W, ctx, stream, args):
+    """C3 / C4: forward alone vs forward + backward (deterministic and atomic vertex reductions)."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    sptr = stream.cuda_stream
+    out = {}
+    # C3: raycastFirst on the noise-displaced icosphere (detail 200), 1M rays
+    log("C3 setup")
+    w = W.c3_raycast(args.c3_rays, detail=200)
+    g = w.geometry.copy()
+    g.ensure_index()
+    idx = g.index.copy()
+    tree = M.build(g.positions, idx, strategy=M.SAH, target_leaf_size=10)
+    mesh = ctx.upload(tree, g.positions, idx)
+    n = w.origins.shape[0]
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    o, d = t(w.origins), t(w.directions)
+    face = torch.empty(n, dtype=torch.int32, device=dev)
+    dist, pt, bc = (torch.empty(s, device=dev) for s in (n, (n, 3), (n, 3)))
+    rng = np.random.default_rng(17)
+    g_dist, g_bary = t(rng.standard_normal(n).astype(np.float32)), t(rng.standard_normal((n, 3)).astype(np.float32))
+    g_o, g_d = torch.empty((n, 3), device=dev), torch.empty((n, 3), device=dev)
+    g_v = torch.empty((mesh.vertex_count, 3), device=dev)
+    wsb = M.Mesh.grad_workspace_bytes(n, M.GRAD_DETERMINISTIC)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+
+    def fwd():
+        mesh.raycast_first_d(o.data_ptr(), d.data_ptr(), n, face.data_ptr(), dist.data_ptr(), pt.data_ptr(),
+                             bc.data_ptr(), stream=sptr)
+
+    def bwd(mode):
+        mesh.raycast_first_bwd_d(d.data_ptr(), n, face.data_ptr(), dist.data_ptr(), bc.data_ptr(),
+                                 g_dist.data_ptr(), 0, g_bary.data_ptr(), g_o.data_ptr(), g_d.data_ptr(),
+                                 g_v.data_ptr(), mode, ws.data_ptr(), wsb, stream=sptr)
+
+    f_ms = event_ms(torch, stream, fwd, 5)
+    fb_det = event_ms(torch, stream, lambda: (fwd(), bwd(M.GRAD_DETERMINISTIC)), 5)
+    fb_atom = event_ms(torch, stream, lambda: (fwd(), bwd(M.GRAD_ATOMIC)), 5)
+    hits = int((face.cpu().numpy() >= 0).sum())
+    out["differentiable_raycast"] = {
+        "workload": f"raycastFirst displaced Icosahedron(1,200) {g.tri_count} tris, {n} rays, upstream "
+                    "dL/ddistance and dL/dbarycoord ~ N(0,1); grads wrt origins, directions, vertex positions",
+        "hits": hits, "fwd_ms": round(f_ms, 4), "fwd_bwd_ms": round(fb_det, 4),
+        "fwd_bwd_over_fwd": round(fb_det / f_ms, 3), "fwd_bwd_ms_atomic": round(fb_atom, 4),
+        "fwd_bwd_over_fwd_atomic": round(fb_atom / f_ms, 3), "fwd_mrays_per_s": round(n / f_ms / 1e3, 2)}
+    log(f"C3 {out['differentiable_raycast']}")
+    mesh.free()
+    del g_v, ws
+    # C4: closest point on the noise-displaced icosphere (detail 223), 256^3 grid
+    log("C4 setup")
+    w = W.c4_closest(args.c4_grid, detail=223)
+    g = w.geometry.copy()
+    g.ensure_index()
+    idx = g.index.copy()
+    tree = M.build(g.positions, idx, strategy=M.SAH, target_leaf_size=10)
+    mesh = ctx.upload(tree, g.positions, idx)
+    n = w.points.shape[0]
+    q = t(w.points)
+    face = torch.empty(n, dtype=torch.int32, device=dev)
+    dist, pt, bc = (torch.empty(s, device=dev) for s in (n, (n, 3), (n, 3)))
+    order = torch.empty(n, dtype=torch.int32, device=dev)
+    owsb = M.Context.order_workspace_bytes(n)
+    ows = torch.empty(owsb, dtype=torch.uint8, device=dev)
+    g_dist = t(rng.standard_normal(n).astype(np.float32))
+    g_q = torch.empty((n, 3), device=dev)
+    g_v = torch.empty((mesh.vertex_count, 3), device=dev)
+    wsb = M.Mesh.grad_workspace_bytes(n, M.GRAD_DETERMINISTIC)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+
+    def fwd_c():
+        ctx.order_points_d(q.data_ptr(), n, order.data_ptr(), ows.data_ptr(), owsb, stream=sptr)
+        mesh.closest_point_d(q.data_ptr(), n, face.data_ptr(), dist.data_ptr(), pt.data_ptr(), bc.data_ptr(),
+                             stream=sptr, order_ptr=order.data_ptr())
+
+    def bwd_c(mode):
+        mesh.closest_point_bwd_d(q.data_ptr(), n, face.data_ptr(), dist.data_ptr(), pt.data_ptr(), bc.data_ptr(),
+                                 g_dist.data_ptr(), g_q.data_ptr(), g_v.data_ptr(), mode, ws.data_ptr(), wsb,
+                                 stream=sptr)
+
+    f_ms = event_ms(torch, stream, fwd_c, 2)
+    b_det = event_ms(torch, stream, lambda: bwd_c(M.GRAD_DETERMINISTIC), 3)
+    b_atom = event_ms(torch, stream, lambda: bwd_c(M.GRAD_ATOMIC), 3)
+    out["differentiable_closest"] = {
+        "workload": f"closestPoint displaced Icosahedron(1,223) {g.tri_count} tris, {args.c4_grid}^3 grid, "
+                    "upstream dL/ddistance ~ N(0,1); grads wrt query points, vertex positions",
+        "fwd_ms": round(f_ms, 3), "bwd_ms_deterministic": round(b_det, 3), "bwd_ms_atomic": round(b_atom, 3),
+        "fwd_bwd_over_fwd": round((f_ms + b_det) / f_ms, 3),
+        "fwd_bwd_over_fwd_atomic": round((f_ms + b_atom) / f_ms, 3), "fwd_mqueries_per_s": round(n / f_ms / 1e3, 2)}
+    log(f"C4 {out['differentiable_closest']}")
+    mesh.free()
+    return out
+
+
+def bench_batched(torch, M, W, ctx, stream, args):
+    """C5: one persistent batched launch (captured in a HIP graph, autotuned) vs the per-mesh loop."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    dev = torch.device("cuda:0")
+    sptr = stream.cuda_stream
+    log("C5 setup")
+    w = W.c5_batched(args.c5_geometries, args.c5_rays)
+    geos = [g.copy() for g in w.geometries]
+    for g in geos:
+        g.ensure_index()
+    idxs = [g.index.copy() for g in geos]
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        trees = list(ex.map(lambda k: M.build(geos[k].positions, idxs[k], strategy=M.SAH, target_leaf_size=10),
+                            range(len(geos))))
+    meshes = [ctx.upload(trees[k], geos[k].positions, idxs[k]) for k in range(len(geos))]
+    batch = ctx.batch(meshes)
+    n = w.origins.shape[0]
+    n_inst = w.matrix_world.shape[0]
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    D = {"world": t(w.matrix_world), "inverse": t(w.matrix_world_inverse),
+         "instance_geometry": t(w.instance_geometry.view(np.int32)), "origins": t(w.origins),
+         "directions": t(w.directions), "ray_instance": t(w.ray_instance.view(np.int32))}
+    wsb = batch.workspace_bytes(n)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    out = {"face": torch.empty(n, dtype=torch.int32, device=dev), "distance": torch.empty(n, device=dev),
+           "point": torch.empty((n, 3), device=dev), "barycoord": torch.empty((n, 3), device=dev)}
+    p = {k: v.data_ptr() for k, v in D.items()}
+    p.update({k: v.data_ptr() for k, v in out.items()})
+    p.update(workspace=ws.data_ptr(), workspace_bytes=wsb, n_instances=n_inst, n=n)
+    log("C5 autotune")
+    tuning = batch.autotune(p)
+    graph = batch.graph(p)
+    b_ms = event_ms(torch, stream, lambda: graph.launch(sptr), 10)
+    e_ms = event_ms(torch, stream, lambda: batch.raycast_first_d(p, stream=sptr), 10)
+    res_b = {k: v.clone() for k, v in out.items()}
+    # the per-mesh plain loop: instance rays, one raycastFirst launch per mesh on its rays (grouped
+    # once, outside the timed region), world hits
+    lo, ld = torch.empty((n, 3), device=dev), torch.empty((n, 3), device=dev)
+    geo = w.instance_geometry[w.ray_instance]
+    order = np.argsort(geo, kind="stable")
+    starts = np.concatenate([[0], np.cumsum(np.bincount(geo, minlength=len(meshes)))])
+    gidx = t(order.astype(np.int64))
+    so = {k: torch.empty_like(v) for k, v in out.items()}
+    slo, sld = torch.empty_like(lo), torch.empty_like(ld)
+
+    def loop():
+        ctx.instance_rays_d(D["inverse"].data_ptr(), n_inst, D["origins"].data_ptr(), D["directions"].data_ptr(),
+                            D["ray_instance"].data_ptr(), n, lo.data_ptr(), ld.data_ptr(), stream=sptr)
+        torch.index_select(lo, 0, gidx, out=slo)
+        torch.index_select(ld, 0, gidx, out=sld)
+        for gi, mesh in enumerate(meshes):
+            a, b = int(starts[gi]), int(starts[gi + 1])
+            if a < b:
+                mesh.raycast_first_d(slo[a:].data_ptr(), sld[a:].data_ptr(), b - a, so["face"][a:].data_ptr(),
+                                     so["distance"][a:].data_ptr(), so["point"][a:].data_ptr(),
+                                     so["barycoord"][a:].data_ptr(), stream=sptr)
+        for k in out:
+            out[k].index_copy_(0, gidx, so[k])
+        ctx.instance_hits_d(D["world"].data_ptr(), n_inst, D["origins"].data_ptr(), D["ray_instance"].data_ptr(), n,
+                            out["face"].data_ptr(), out["point"].data_ptr(), out["distance"].data_ptr(), stream=sptr)
+
+    l_ms = event_ms(torch, stream, loop, 3)
+    same = all(torch.equal(out[k].view(torch.int32), res_b[k].view(torch.int32)) for k in out)
+    tris = int(sum(g.tri_count for g in geos))
+    r = {"workload": f"{len(meshes)} meshes (torus knots / spheres, {tris} tris, 100..200k each), one rigid "
+                     f"instance each, {n} rays split by Zipf(1.1)",
+         "batched_graph_ms": round(b_ms, 3), "batched_eager_ms": round(e_ms, 3), "plain_loop_ms": round(l_ms, 3),
+         "speedup_vs_loop": round(l_ms / b_ms, 2), "batched_mrays_per_s": round(n / b_ms / 1e3, 2),
+         "identical_to_loop": bool(same), "tuning": tuning.as_dict()}
+    log(f"C5 {r}")
+    graph.free()
+    batch.free()
+    for m in meshes:
+        m.free()
+    return {"batched_raycast_first": r}
 
 
 def main():
@@ -94,13 +296,21 @@ def main():
     d_dist = torch.empty(n, dtype=torch.float32, device=dev)
     d_pt = torch.empty((n, 3), dtype=torch.float32, device=dev)
     d_bc = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    # query order (Morton buckets), recomputed inside every step
+    d_order = torch.empty(n, dtype=torch.int32, device=dev)
+    ws_bytes = M.Context.order_workspace_bytes(n)
+    d_ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     torch.cuda.synchronize()
+
+    def order_cp():
+        if not args.no_order:
+            ctx.order_points_d(d_q.data_ptr(), n, d_order.data_ptr(), d_ws.data_ptr(), ws_bytes, stream=sptr)
 
     def launch_cp():
         mesh.closest_point_d(d_q.data_ptr(), n, d_face.data_ptr(), d_dist.data_ptr(), d_pt.data_ptr(),
-                             d_bc.data_ptr(), stream=sptr)
+                             d_bc.data_ptr(), stream=sptr, order_ptr=0 if args.no_order else d_order.data_ptr())
 
-    wall, kern = timed_kernel(torch, launch_cp, args.steps, args.warmup, stream)
+    wall, kern = timed_kernel(torch, launch_cp, args.steps, args.warmup, stream, pre=order_cp)
     ms_per_step = wall * 1000.0 / args.steps
     value = n / (wall / args.steps) / 1e6
     kern_ms = float(np.mean(kern))
@@ -118,12 +328,20 @@ def main():
     r_dist = torch.empty(nr, dtype=torch.float32, device=dev)
     r_pt = torch.empty((nr, 3), dtype=torch.float32, device=dev)
     r_bc = torch.empty((nr, 3), dtype=torch.float32, device=dev)
+    r_order = torch.empty(nr, dtype=torch.int32, device=dev)
+    r_ws_bytes = M.Context.order_workspace_bytes(nr)
+    r_ws = torch.empty(r_ws_bytes, dtype=torch.uint8, device=dev)
+
+    def order_rc():
+        if not args.no_order:
+            ctx.order_points_d(d_o.data_ptr(), nr, r_order.data_ptr(), r_ws.data_ptr(), r_ws_bytes, stream=sptr)
 
     def launch_rc():
         mesh1.raycast_first_d(d_o.data_ptr(), d_d.data_ptr(), nr, r_face.data_ptr(), r_dist.data_ptr(),
-                              r_pt.data_ptr(), r_bc.data_ptr(), stream=sptr)
+                              r_pt.data_ptr(), r_bc.data_ptr(), stream=sptr,
+                              order_ptr=0 if args.no_order else r_order.data_ptr())
 
-    wall1, kern1 = timed_kernel(torch, launch_rc, max(args.steps, 20) * 4, args.warmup, stream)
+    wall1, kern1 = timed_kernel(torch, launch_rc, max(args.steps, 20) * 4, args.warmup, stream, pre=order_rc)
     rc_steps = max(args.steps, 20) * 4
     mrays = nr / (wall1 / rc_steps) / 1e6
     mrays_kernel = nr / (float(np.mean(kern1)) / 1000.0) / 1e6
@@ -176,6 +394,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"closestPointToPoint TorusKnot(10,3,800,200) 320k tris SAH leaf10, "
                                f"{args.grid}^3 grid over AABB+10%", "queries": n,
+                   "query_order": "given" if args.no_order else "Morton buckets (in the step)",
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -183,6 +402,9 @@ def main():
                           "mrays_per_s": round(mrays, 2), "mrays_per_s_kernel_only": round(mrays_kernel, 2),
                           "kernel_ms": round(float(np.mean(kern1)), 4), **extra},
     }
+    if not args.only_primary:
+        line.update(bench_differentiable(torch, M, W, ctx, stream, args))
+        line.update(bench_batched(torch, M, W, ctx, stream, args))
     print(json.dumps(line), flush=True)
     if args.traffic_calibration:
         src = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define MBVH_ABI_VERSION 1u
+#define MBVH_ABI_VERSION 2u
 #define MBVH_NO_HIT 0xFFFFFFFFu
 
 typedef enum mbvh_status {
@@ -156,8 +156,11 @@ int mbvh_raycast_first(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origin
                        const float* directions, uint64_t n, int32_t side, float near_dist,
                        float far_dist, uint32_t* face_index, float* distance, float* point,
                        float* barycoord);
+/* `order` (device, may be NULL): a permutation of 0..n-1 giving the order in which the rays are
+ * traced, e.g. from mbvh_order_points_d on the origins. Results still go to each ray's own index.
+ * Entries >= n are skipped. */
 int mbvh_raycast_first_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins,
-                         const float* directions, uint64_t n, int32_t side, float near_dist,
+                         const float* directions, uint64_t n, const uint32_t* order, int32_t side, float near_dist,
                          float far_dist, uint32_t* face_index, float* distance, float* point,
                          float* barycoord, void* stream);
 
@@ -195,9 +198,139 @@ int mbvh_raycast_fill_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origi
 int mbvh_closest_point(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
                        float min_threshold, float max_threshold, uint32_t* face_index,
                        float* distance, float* closest, float* barycoord);
+/* `order` (device, may be NULL): a permutation of 0..n-1 giving the order in which the points are
+ * processed, e.g. from mbvh_order_points_d. Results still go to each point's own index. Entries
+ * >= n are skipped. */
 int mbvh_closest_point_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
-                         float min_threshold, float max_threshold, uint32_t* face_index,
+                         const uint32_t* order, float min_threshold, float max_threshold, uint32_t* face_index,
                          float* distance, float* closest, float* barycoord, void* stream);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Instances and the batched multi-mesh launch (BatchedMesh)                                  */
+/* ------------------------------------------------------------------------------------------ */
+/*
+ * acceleratedRaycast on a BatchedMesh (acceleratedBatchedMeshRaycast,
+ * src/utils/ExtensionUtilities.js:87-160) raycasts each visible instance through
+ * MeshBVH.raycastObject3D (src/core/MeshBVH.js:447-494): the world ray goes to the instance's
+ * frame with matrixWorld^-1 (three's Ray.applyMatrix4: origin by Vector3.applyMatrix4,
+ * direction by transformDirection, normalised), near / far are divided by
+ * |local direction * world scale|, raycastFirst runs on the instance's geometry BVH, and the hit
+ * comes back with matrixWorld (convertRaycastIntersect, src/utils/GeometryRayIntersectUtilities.js:3-17:
+ * world point, distance from the world origin). Here each ray names its instance
+ * (ray_instance[i]); instance k uses mesh instance_geometry[k] of the batch and the column-major
+ * 4x4 matrices matrix_world + 16k and matrix_world_inverse + 16k (three's Matrix4.elements, the
+ * inverse computed by the caller as raycastObject3D does). Results per ray: face index in that
+ * mesh, world distance, world point, barycoord. A ray whose instance or mesh index is out of
+ * range is a miss.
+ *
+ * mbvh_batch_raycast_first_d is ONE persistent launch over all meshes: rays are grouped by mesh
+ * (counting sort), then a fixed grid pulls tiles of rays from an atomic work counter until none
+ * remain. It allocates nothing and uses no memset; the device workspace holds the order.
+ * Concurrent launches of one batch race on its work counter: serialise them (one stream).
+ * tuning may be NULL (the batch's autotuned values, or defaults before mbvh_batch_autotune).
+ */
+typedef struct mbvh_batch_tuning {
+  uint32_t stack_lds;     /* LDS traversal-stack entries per lane: 8 or 16 */
+  uint32_t tile_rays;     /* rays per work item: a multiple of 64, at most 4096 */
+  uint32_t blocks_per_cu; /* persistent workgroups of 256 lanes per compute unit: 1..8 */
+  float ms;               /* mbvh_batch_autotune: measured time of the chosen configuration */
+} mbvh_batch_tuning;
+typedef struct mbvh_batch_graph mbvh_batch_graph;
+
+/* A batch references meshes uploaded on ctx; they must outlive it. */
+int mbvh_batch_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, uint32_t n_meshes, mbvh_batch** out_batch);
+void mbvh_batch_free(mbvh_batch* batch);
+int mbvh_batch_workspace_bytes(const mbvh_batch* batch, uint64_t n_rays, uint64_t* bytes);
+int mbvh_batch_raycast_first_d(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_world,
+                               const float* matrix_world_inverse, const uint32_t* instance_geometry,
+                               uint32_t n_instances, const float* origins, const float* directions,
+                               const uint32_t* ray_instance, uint64_t n, int32_t side, float near_dist,
+                               float far_dist, uint32_t* face_index, float* distance, float* point,
+                               float* barycoord, void* workspace, uint64_t workspace_bytes,
+                               const mbvh_batch_tuning* tuning, void* stream);
+/* Times every candidate (stack_lds x tile_rays x blocks_per_cu) on these inputs, keeps the fastest
+ * in the batch (used when tuning is NULL) and returns it. Synchronous; outputs hold the last run. */
+int mbvh_batch_autotune(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_world,
+                        const float* matrix_world_inverse, const uint32_t* instance_geometry,
+                        uint32_t n_instances, const float* origins, const float* directions,
+                        const uint32_t* ray_instance, uint64_t n, int32_t side, float near_dist, float far_dist,
+                        uint32_t* face_index, float* distance, float* point, float* barycoord, void* workspace,
+                        uint64_t workspace_bytes, mbvh_batch_tuning* out_tuning);
+/* Captures the whole batched launch (ordering + traversal) into a HIP graph for these buffers;
+ * mbvh_batch_graph_launch replays it on `stream` (NULL: the context's stream). */
+int mbvh_batch_graph_create(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_world,
+                            const float* matrix_world_inverse, const uint32_t* instance_geometry,
+                            uint32_t n_instances, const float* origins, const float* directions,
+                            const uint32_t* ray_instance, uint64_t n, int32_t side, float near_dist,
+                            float far_dist, uint32_t* face_index, float* distance, float* point,
+                            float* barycoord, void* workspace, uint64_t workspace_bytes,
+                            const mbvh_batch_tuning* tuning, mbvh_batch_graph** out_graph);
+int mbvh_batch_graph_launch(mbvh_batch_graph* graph, void* stream);
+void mbvh_batch_graph_free(mbvh_batch_graph* graph);
+/* The two halves of raycastObject3D's frame change, for callers that loop over meshes
+ * themselves: world rays -> instance-local rays, and local hits (face, point) -> world point and
+ * distance in place (misses untouched). Same arithmetic as the batched launch, bit for bit. */
+int mbvh_instance_rays_d(mbvh_ctx* ctx, const float* matrix_world_inverse, uint32_t n_instances,
+                         const float* origins, const float* directions, const uint32_t* ray_instance, uint64_t n,
+                         float* local_origins, float* local_directions, void* stream);
+int mbvh_instance_hits_d(mbvh_ctx* ctx, const float* matrix_world, uint32_t n_instances, const float* origins,
+                         const uint32_t* ray_instance, uint64_t n, const uint32_t* face_index, float* point,
+                         float* distance, void* stream);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Backward (adjoint) of the queries                                                          */
+/* ------------------------------------------------------------------------------------------ */
+/*
+ * No counterpart in the reference (which has no derivatives). For a fixed forward result (the
+ * saved face index, distance, barycoord and closest point of each query), the backward returns
+ * the gradients of a scalar loss L with respect to the query inputs and to the mesh's vertex
+ * positions, given the upstream gradients of L with respect to the forward outputs.
+ * Vertex gradients: vertex_count x 3 floats (the vertices the geometry index refers to, packed
+ * x, y, z whatever the position stride), overwritten. Misses contribute nothing; a closest point
+ * at distance 0 has no gradient (zero).
+ * mode MBVH_GRAD_ATOMIC sums vertex contributions with FP32 atomics (run-to-run rounding may
+ * differ); MBVH_GRAD_DETERMINISTIC sorts them by vertex (stable radix sort) and sums each
+ * vertex's contributions in (query, corner) order, bitwise reproducible. The deterministic mode
+ * needs a device workspace of mbvh_grad_workspace_bytes() bytes (256-byte aligned).
+ */
+#define MBVH_GRAD_ATOMIC 0
+#define MBVH_GRAD_DETERMINISTIC 1
+int mbvh_grad_workspace_bytes(uint64_t n, int32_t mode, uint64_t* bytes);
+/* Upstream: grad_distance (n), grad_point (3n), grad_barycoord (3n), each may be NULL (= 0).
+ * Outputs: grad_origins (3n), grad_directions (3n), grad_positions (3 x vertex_count), each
+ * may be NULL (not computed). */
+int mbvh_raycast_first_bwd_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* directions, uint64_t n,
+                             const uint32_t* face_index, const float* distance, const float* barycoord,
+                             const float* grad_distance, const float* grad_point, const float* grad_barycoord,
+                             float* grad_origins, float* grad_directions, float* grad_positions, int32_t mode,
+                             void* workspace, uint64_t workspace_bytes, void* stream);
+int mbvh_raycast_first_bwd(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* directions, uint64_t n,
+                           const uint32_t* face_index, const float* distance, const float* barycoord,
+                           const float* grad_distance, const float* grad_point, const float* grad_barycoord,
+                           float* grad_origins, float* grad_directions, float* grad_positions, int32_t mode);
+/* Upstream: grad_distance (n). Outputs: grad_points (3n), grad_positions (3 x vertex_count). */
+int mbvh_closest_point_bwd_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
+                             const uint32_t* face_index, const float* distance, const float* closest,
+                             const float* barycoord, const float* grad_distance, float* grad_points,
+                             float* grad_positions, int32_t mode, void* workspace, uint64_t workspace_bytes,
+                             void* stream);
+int mbvh_closest_point_bwd(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
+                           const uint32_t* face_index, const float* distance, const float* closest,
+                           const float* barycoord, const float* grad_distance, float* grad_points,
+                           float* grad_positions, int32_t mode);
+
+/*
+ * Spatial query order (no counterpart in the reference, which answers one query per call). A wave
+ * of 64 lanes costs the union of its lanes' traversal paths, so neighbouring queries should share
+ * a wave: mbvh_order_points_d writes a permutation `order` (n entries) that groups the n packed
+ * float triples by Morton cell of their bounding box. It needs a device workspace of
+ * mbvh_order_workspace_bytes(n) bytes (256-byte aligned), allocates nothing and uses no memset,
+ * so it may be captured in a HIP graph. The host query entry points order their queries this way
+ * by themselves. n must be below 2^32.
+ */
+int mbvh_order_workspace_bytes(uint64_t n, uint64_t* bytes);
+int mbvh_order_points_d(mbvh_ctx* ctx, const float* points, uint64_t n, uint32_t* order, void* workspace,
+                        uint64_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

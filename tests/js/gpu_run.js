@@ -50,6 +50,22 @@ writeArr('ra_offsets.bin', ra.offsets);
 writeArr('ra_face.bin', ra.faceIndex);
 writeArr('ra_dist.bin', ra.distance);
 
+// --- autograd-style forward / backward pairs -------------------------------------------------
+const gDist = readArr('g_dist.bin', Float32Array);
+const gBary = readArr('g_bary.bin', Float32Array);
+const gCp = readArr('g_cp.bin', Float32Array);
+const fr = bvh.raycastFirstForward(origins, dirs);
+assert.deepStrictEqual(Array.from(fr.faceIndex.subarray(0, 64)), Array.from(rf.faceIndex.subarray(0, 64)));
+const br = bvh.raycastFirstBackward(fr.saved, { distance: gDist, barycoord: gBary });
+writeArr('bw_origins.bin', br.origins);
+writeArr('bw_dirs.bin', br.directions);
+writeArr('bw_positions.bin', br.positions);
+const fc = bvh.closestPointToPointForward(points);
+const bc = bvh.closestPointToPointBackward(fc.saved, { distance: gCp }, { deterministic: true });
+writeArr('bc_points.bin', bc.points);
+writeArr('bc_positions.bin', bc.positions);
+assert.throws(() => bvh.raycastFirstBackward(fc.saved, {}), /raycastFirstForward/);
+
 // --- per-call API agrees with the batch API --------------------------------------------------
 for (let i = 0; i < K; i++) {
   const ray = { origin: { x: origins[3 * i], y: origins[3 * i + 1], z: origins[3 * i + 2], isVector3: false },

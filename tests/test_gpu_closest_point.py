@@ -113,3 +113,43 @@ def test_degenerate_first_candidate(ctx):
     q = np.array([[-2.514983892440796, 0.6441349983215332, -0.7410805225372314]], np.float32)
     got, ref, stats, om = _run(ctx, geo, np.repeat(q, 64, axis=0), label="degenerate")
     assert np.all(got["face"] == ref["face"])
+
+
+def test_query_order_changes_nothing(ctx):
+    """The spatial visit order is a permutation, and results with it equal results without it bit for
+    bit (each query's result depends on that query only)."""
+    torch = pytest.importorskip("torch")
+    g, idx, tree = build_mesh(G.torus_knot(10, 3, 200, 40))
+    mesh = ctx.upload(tree, g.positions, idx)
+    lo, hi = g.bounds()
+    pts = W.grid_points(lo, hi, 40)  # 64,000 points in row order
+    rng = np.random.default_rng(3)
+    pts = np.concatenate([pts, rng.standard_normal((1000, 3)).astype(np.float32) * 50,
+                          np.array([[np.nan, 0, 0], [np.inf, 1, 1]], np.float32)])
+    n = pts.shape[0]
+    dev = torch.device("cuda:0")
+    d_q = torch.from_numpy(pts).to(dev)
+    order = torch.empty(n, dtype=torch.int32, device=dev)
+    wsb = M.Context.order_workspace_bytes(n)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    ctx.order_points_d(d_q.data_ptr(), n, order.data_ptr(), ws.data_ptr(), wsb)
+    outs = []
+    for optr in (0, order.data_ptr()):
+        face = torch.empty(n, dtype=torch.int32, device=dev)
+        dist = torch.empty(n, dtype=torch.float32, device=dev)
+        pt = torch.empty((n, 3), dtype=torch.float32, device=dev)
+        bc = torch.empty((n, 3), dtype=torch.float32, device=dev)
+        mesh.closest_point_d(d_q.data_ptr(), n, face.data_ptr(), dist.data_ptr(), pt.data_ptr(), bc.data_ptr(),
+                             order_ptr=optr)
+        ctx.synchronize()
+        outs.append([x.cpu().numpy().view(np.uint32) for x in (face, dist, pt, bc)])
+    o = order.cpu().numpy().view(np.uint32)
+    assert np.array_equal(np.sort(o), np.arange(n, dtype=np.uint32))
+    assert not np.array_equal(o, np.arange(n, dtype=np.uint32))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    # the host entry point (which orders internally at this size) gives the same results
+    host = mesh.closest_point(pts)
+    assert np.array_equal(host["face"], outs[0][0])
+    assert np.array_equal(host["distance"].view(np.uint32), outs[0][1])
+    mesh.free()

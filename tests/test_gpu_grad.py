@@ -1,0 +1,214 @@
+"""GPU backward (adjoint) kernels through the C ABI, against the FP64 gradient oracle and FP64 central
+finite differences: raycastFirst (dL/d origins, directions, vertex positions) and closestPoint
+(dL/d query points, vertex positions), both vertex-gradient reductions (deterministic sorted
+segmented sum, FP32 atomics).
+
+Tolerances: per-query gradient vectors within 1e-3 relative of FP64 finite differences (the
+target for this path) and of the FP64 analytic gradient for the 99.9th percentile of queries,
+vertex gradients within 1e-5 of the summed size of the query gradients they share (FP32
+barycentric weights carry an absolute rounding error, so a vertex a query barely weights is judged
+against that query's whole gradient)."""
+import numpy as np
+import pytest
+
+import mbvh_amd as M
+from mbvh_amd import geometry as G, workloads as W
+import oracle_grad as OG
+from helpers import build_mesh
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-3
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = M.Context(0)
+    yield c
+    c.close()
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-6)
+
+
+def _vertex_check(got, ref, absum, label):
+    err = np.abs(got - ref) - 1e-5 * absum - 1e-6
+    print(label, "vertex max excess", float(err.max()))
+    assert err.max() <= 0, (label, np.unravel_index(np.argmax(err), err.shape))
+
+
+@pytest.fixture(scope="module")
+def ray_case(ctx):
+    w = W.c3_raycast(1 << 16, detail=40)
+    g, idx, tree = build_mesh(w.geometry)
+    mesh = ctx.upload(tree, g.positions, idx)
+    fwd = mesh.raycast_first(w.origins, w.directions)
+    hit = fwd["face"] != M.NO_HIT
+    rng = np.random.default_rng(11)
+    n = w.origins.shape[0]
+    up = {"grad_distance": rng.standard_normal(n).astype(np.float32),
+          "grad_point": rng.standard_normal((n, 3)).astype(np.float32),
+          "grad_barycoord": rng.standard_normal((n, 3)).astype(np.float32)}
+    yield w, g, idx, mesh, fwd, hit, up
+    mesh.free()
+
+
+def test_raycast_bwd_vs_oracle_and_determinism(ray_case):
+    w, g, idx, mesh, fwd, hit, up = ray_case
+    assert hit.mean() > 0.3
+    det = mesh.raycast_first_bwd(w.directions, fwd, mode=M.GRAD_DETERMINISTIC, **up)
+    det2 = mesh.raycast_first_bwd(w.directions, fwd, mode=M.GRAD_DETERMINISTIC, **up)
+    ato = mesh.raycast_first_bwd(w.directions, fwd, mode=M.GRAD_ATOMIC, **up)
+    for k in ("origins", "directions", "positions"):
+        assert np.array_equal(det[k].view(np.uint32), det2[k].view(np.uint32)), k  # bitwise reproducible
+    assert np.array_equal(det["origins"], ato["origins"]) and np.array_equal(det["directions"], ato["directions"])
+    assert np.all(det["origins"][~hit] == 0) and np.all(det["directions"][~hit] == 0)
+
+    f = fwd["face"][hit]
+    a, b, c = OG.face_vertices(g.positions, idx, f)
+    ref = OG.ray_grads(w.origins[hit], w.directions[hit], a, b, c, up["grad_distance"][hit],
+                       up["grad_point"][hit], up["grad_barycoord"][hit])
+    for k in ("origins", "directions"):
+        e = _rel(det[k][hit], ref[k])
+        print(k, "rel err p50/p99.9/max", np.quantile(e, [0.5, 0.999]), e.max())
+        assert np.quantile(e, 0.999) < REL, k
+    ref_v, absum = OG.scatter_vertices(idx, f, ref["a"], ref["b"], ref["c"], g.positions.shape[0])
+    _vertex_check(det["positions"], ref_v, absum, "ray det")
+    _vertex_check(ato["positions"], ref_v, absum, "ray atomic")
+    # vertices no hit touches get exactly zero
+    touched = np.zeros(g.positions.shape[0], bool)
+    touched[idx.reshape(-1, 3)[f].ravel()] = True
+    assert np.all(det["positions"][~touched] == 0)
+
+
+def test_raycast_bwd_vs_finite_differences(ray_case):
+    """1,024 sampled hits: FP64 central differences of the fixed-face forward map."""
+    w, g, idx, mesh, fwd, hit, up = ray_case
+    sel = np.flatnonzero(hit)[:: max(1, hit.sum() // 1024)][:1024]
+    got = mesh.raycast_first_bwd(w.directions, fwd, mode=M.GRAD_DETERMINISTIC, vertices=False, **up)
+    o, d = w.origins[sel].astype(np.float64), w.directions[sel].astype(np.float64)
+    a, b, c = OG.face_vertices(g.positions, idx, fwd["face"][sel])
+    gd, gp, gb = (up[k][sel].astype(np.float64) for k in ("grad_distance", "grad_point", "grad_barycoord"))
+
+    def loss(o, d, a, b, c):
+        dist, point, bary = OG.ray_forward(o, d, a, b, c)
+        return gd * dist + np.einsum("ij,ij->i", gp, point) + np.einsum("ij,ij->i", gb, bary)
+
+    args = [o, d, a, b, c]
+    for k, name in ((0, "origins"), (1, "directions")):
+        fd = np.zeros_like(args[k])
+        h = 1e-6
+        for j in range(3):
+            pl = [x.copy() for x in args]
+            mi = [x.copy() for x in args]
+            pl[k][:, j] += h
+            mi[k][:, j] -= h
+            fd[:, j] = (loss(*pl) - loss(*mi)) / (2 * h)
+        e = _rel(got[name][sel], fd)
+        print(name, "vs FD p50/p99/max", np.quantile(e, [0.5, 0.99]), e.max())
+        assert np.quantile(e, 0.99) < REL and np.median(e) < 1e-4, name
+
+
+@pytest.fixture(scope="module")
+def closest_case(ctx):
+    g0 = G.displaced_icosphere(30)
+    g, idx, tree = build_mesh(g0)
+    mesh = ctx.upload(tree, g.positions, idx)
+    lo, hi = g.bounds()
+    pts = W.grid_points(lo, hi, 40)
+    fwd = mesh.closest_point(pts)
+    rng = np.random.default_rng(12)
+    gd = rng.standard_normal(pts.shape[0]).astype(np.float32)
+    yield g, idx, mesh, pts, fwd, gd
+    mesh.free()
+
+
+def test_closest_bwd_vs_oracle_and_determinism(closest_case):
+    g, idx, mesh, pts, fwd, gd = closest_case
+    det = mesh.closest_point_bwd(pts, fwd, gd, mode=M.GRAD_DETERMINISTIC)
+    det2 = mesh.closest_point_bwd(pts, fwd, gd, mode=M.GRAD_DETERMINISTIC)
+    ato = mesh.closest_point_bwd(pts, fwd, gd, mode=M.GRAD_ATOMIC)
+    for k in ("points", "positions"):
+        assert np.array_equal(det[k].view(np.uint32), det2[k].view(np.uint32)), k
+    assert np.array_equal(det["points"], ato["points"])
+    a, b, c = OG.face_vertices(g.positions, idx, fwd["face"])
+    ref = OG.closest_grads(pts, a, b, c, gd)
+    e = _rel(det["points"], ref["points"])
+    print("points rel err p50/p99.9/max", np.quantile(e, [0.5, 0.999]), e.max())
+    assert np.quantile(e, 0.999) < REL
+    ref_v, absum = OG.scatter_vertices(idx, fwd["face"], ref["a"], ref["b"], ref["c"], g.positions.shape[0])
+    _vertex_check(det["positions"], ref_v, absum, "closest det")
+    _vertex_check(ato["positions"], ref_v, absum, "closest atomic")
+    # gradient of the distance field has unit length
+    nrm = np.linalg.norm(det["points"], axis=1) / np.abs(gd)
+    assert np.all(np.abs(nrm - 1) < 1e-4)
+
+
+def test_closest_bwd_vs_finite_differences(closest_case):
+    g, idx, mesh, pts, fwd, gd = closest_case
+    sel = np.arange(0, pts.shape[0], max(1, pts.shape[0] // 1024))[:1024]
+    got = mesh.closest_point_bwd(pts, fwd, gd, mode=M.GRAD_ATOMIC, vertices=False)
+    q = pts[sel].astype(np.float64)
+    a, b, c = OG.face_vertices(g.positions, idx, fwd["face"][sel])
+    h = 1e-6
+    fd = np.zeros_like(q)
+    for j in range(3):
+        qp, qm = q.copy(), q.copy()
+        qp[:, j] += h
+        qm[:, j] -= h
+        fd[:, j] = gd[sel] * (OG.closest_distance(qp, a, b, c) - OG.closest_distance(qm, a, b, c)) / (2 * h)
+    e = _rel(got["points"][sel], fd)
+    print("points vs FD p50/p99/max", np.quantile(e, [0.5, 0.99]), e.max())
+    assert np.quantile(e, 0.99) < REL and np.median(e) < 1e-4
+
+
+def test_bwd_edge_cases(ctx):
+    """Misses give zero gradients; n = 0 zeroes the vertex gradients; an indirect mesh maps faces
+    to its own slots; NULL outputs are skipped."""
+    g, idx, tree = build_mesh(G.torus_knot(10, 3, 64, 16), indirect=True)
+    mesh = ctx.upload(tree, g.positions, idx)
+    lo, hi = g.bounds()
+    o, d = W.rays_toward_box(4096, 40.0, lo, hi, seed_origin=5, seed_target=6)
+    d[::7] = -d[::7]  # some rays point away: misses
+    fwd = mesh.raycast_first(o, d)
+    hit = fwd["face"] != M.NO_HIT
+    assert 0.2 < hit.mean() < 0.95
+    rng = np.random.default_rng(13)
+    gdist = rng.standard_normal(4096).astype(np.float32)
+    got = mesh.raycast_first_bwd(d, fwd, grad_distance=gdist, mode=M.GRAD_DETERMINISTIC)
+    assert np.all(got["origins"][~hit] == 0)
+    a, b, c = OG.face_vertices(g.positions, idx, fwd["face"][hit])
+    ref = OG.ray_grads(o[hit], d[hit], a, b, c, gdist[hit])
+    assert np.quantile(_rel(got["origins"][hit], ref["origins"]), 0.999) < REL
+    ref_v, absum = OG.scatter_vertices(idx, fwd["face"][hit], ref["a"], ref["b"], ref["c"], g.positions.shape[0])
+    _vertex_check(got["positions"], ref_v, absum, "indirect")
+    empty = {"face": np.zeros(0, np.uint32), "distance": np.zeros(0, np.float32),
+             "barycoord": np.zeros((0, 3), np.float32)}
+    z = mesh.raycast_first_bwd(np.zeros((0, 3), np.float32), empty, mode=M.GRAD_DETERMINISTIC)
+    assert z["positions"].shape == (g.positions.shape[0], 3) and np.all(z["positions"] == 0)
+    mesh.free()
+
+
+def test_device_path_matches_host(ctx, closest_case):
+    torch = pytest.importorskip("torch")
+    g, idx, mesh, pts, fwd, gd = closest_case
+    dev = torch.device("cuda:0")
+    n = pts.shape[0]
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    d_q, d_f = t(pts), t(fwd["face"].view(np.int32))
+    d_dist, d_pt, d_bc, d_g = t(fwd["distance"]), t(fwd["point"]), t(fwd["barycoord"]), t(gd)
+    g_q = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    g_v = torch.empty((mesh.vertex_count, 3), dtype=torch.float32, device=dev)
+    wsb = M.Mesh.grad_workspace_bytes(n, M.GRAD_DETERMINISTIC)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    mesh.closest_point_bwd_d(d_q.data_ptr(), n, d_f.data_ptr(), d_dist.data_ptr(), d_pt.data_ptr(), d_bc.data_ptr(),
+                             d_g.data_ptr(), g_q.data_ptr(), g_v.data_ptr(), M.GRAD_DETERMINISTIC, ws.data_ptr(), wsb)
+    torch.cuda.synchronize()
+    host = mesh.closest_point_bwd(pts, fwd, gd, mode=M.GRAD_DETERMINISTIC)
+    assert np.array_equal(g_q.cpu().numpy(), host["points"])
+    assert np.array_equal(g_v.cpu().numpy(), host["positions"])
+    with pytest.raises(M.MbvhError):
+        mesh.closest_point_bwd_d(d_q.data_ptr(), n, d_f.data_ptr(), d_dist.data_ptr(), d_pt.data_ptr(),
+                                 d_bc.data_ptr(), d_g.data_ptr(), g_q.data_ptr(), g_v.data_ptr(),
+                                 M.GRAD_DETERMINISTIC, ws.data_ptr(), wsb - 1)

@@ -28,6 +28,13 @@ def test_js_api_matches_oracle(tmp_path):
     w.origins.tofile(d / "origins.bin")
     w.directions.tofile(d / "dirs.bin")
     pts.tofile(d / "points.bin")
+    rng = np.random.default_rng(31)
+    g_dist = rng.standard_normal(w.origins.shape[0]).astype(np.float32)
+    g_bary = rng.standard_normal((w.origins.shape[0], 3)).astype(np.float32)
+    g_cp = rng.standard_normal(pts.shape[0]).astype(np.float32)
+    g_dist.tofile(d / "g_dist.bin")
+    g_bary.tofile(d / "g_bary.bin")
+    g_cp.tofile(d / "g_cp.bin")
     (d / "manifest.json").write_text(json.dumps({"index_u32": g.index.dtype == np.uint32}))
     r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "gpu_run.js"), str(d)], capture_output=True,
                        text=True, timeout=300)
@@ -64,3 +71,20 @@ def test_js_api_matches_oracle(tmp_path):
         assert len(gf) == len(rf_), (i, gf, rf_)
         assert np.array_equal(gf, rf_), (i, gf, rf_)
         assert np.all(np.abs(gd - rd_) <= 1e-5 * np.maximum(rd_, 1))
+
+    # forward / backward pair through the addon == the same backward through the C ABI here
+    # (deterministic reduction: bitwise), whose values test_gpu_grad.py checks against FP64
+    ctx = M.Context(0)
+    mesh = ctx.upload(tree_py, g.positions, idx)
+    fwd = mesh.raycast_first(w.origins, w.directions)
+    ref = mesh.raycast_first_bwd(w.directions, fwd, grad_distance=g_dist, grad_barycoord=g_bary,
+                                 mode=M.GRAD_DETERMINISTIC)
+    assert np.array_equal(np.fromfile(d / "bw_origins.bin", np.float32).reshape(-1, 3), ref["origins"])
+    assert np.array_equal(np.fromfile(d / "bw_dirs.bin", np.float32).reshape(-1, 3), ref["directions"])
+    assert np.array_equal(np.fromfile(d / "bw_positions.bin", np.float32).reshape(-1, 3), ref["positions"])
+    fwdc = mesh.closest_point(pts)
+    refc = mesh.closest_point_bwd(pts, fwdc, g_cp, mode=M.GRAD_DETERMINISTIC)
+    assert np.array_equal(np.fromfile(d / "bc_points.bin", np.float32).reshape(-1, 3), refc["points"])
+    assert np.array_equal(np.fromfile(d / "bc_positions.bin", np.float32).reshape(-1, 3), refc["positions"])
+    mesh.free()
+    ctx.close()

@@ -148,3 +148,30 @@ def test_c3_sample(ctx):
     """Config C3 geometry (noise-displaced Icosahedron detail 200, ~808k tris), 131,072 rays."""
     w = W.c3_raycast(n_rays=1 << 17)
     _run(ctx, w.geometry, w.origins, w.directions, label="C3sample")
+
+
+def test_ray_order_changes_nothing(ctx):
+    """Rays traced in the spatial order of their origins give bit-identical results."""
+    torch = pytest.importorskip("torch")
+    w = W.c1_raycast(20000)
+    g, idx, tree = build_mesh(w.geometry)
+    mesh = ctx.upload(tree, g.positions, idx)
+    n = w.origins.shape[0]
+    dev = torch.device("cuda:0")
+    d_o = torch.from_numpy(w.origins).to(dev)
+    d_d = torch.from_numpy(w.directions).to(dev)
+    order = torch.empty(n, dtype=torch.int32, device=dev)
+    wsb = M.Context.order_workspace_bytes(n)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    ctx.order_points_d(d_o.data_ptr(), n, order.data_ptr(), ws.data_ptr(), wsb)
+    outs = []
+    for optr in (0, order.data_ptr()):
+        face = torch.empty(n, dtype=torch.int32, device=dev)
+        dist = torch.empty(n, dtype=torch.float32, device=dev)
+        mesh.raycast_first_d(d_o.data_ptr(), d_d.data_ptr(), n, face.data_ptr(), dist.data_ptr(), order_ptr=optr)
+        ctx.synchronize()
+        outs.append((face.cpu().numpy().view(np.uint32), dist.cpu().numpy().view(np.uint32)))
+    o = order.cpu().numpy().view(np.uint32)
+    assert np.array_equal(np.sort(o), np.arange(n, dtype=np.uint32))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    mesh.free()

@@ -32,9 +32,30 @@ struct mbvh_ctx {
   DeviceBuffer stack_ovf;       // traversal stack overflow region
   uint32_t stack_ovf_entries = 0;
   DeviceBuffer staging[8];      // host-API input / output staging
+  DeviceBuffer order;           // host-API query order
+  DeviceBuffer order_ws;        // host-API ordering workspace
   DeviceBuffer hits[5];         // host-API all-hits results: offsets, face, dist, point, bary
+  DeviceBuffer grad[11];        // host-API backward staging
+  DeviceBuffer grad_ws;         // host-API backward workspace
   uint64_t hits_total = 0;
   std::mutex lock;              // host entry points on one context are serialised
+};
+
+struct mbvh_batch {
+  uint32_t magic = 0x48435442u;  // "BTCH"
+  mbvh_ctx* ctx = nullptr;
+  uint32_t n_meshes = 0;
+  MeshView* d_meshes = nullptr;
+  uint32_t* d_counter = nullptr;  // work-queue counter
+  uint32_t max_stack = 0;
+  mbvh_batch_tuning tuning{16, 256, 4, 0.0f};
+};
+
+struct mbvh_batch_graph {
+  uint32_t magic = 0x48505247u;  // "GRPH"
+  mbvh_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
 };
 
 struct mbvh_mesh {
@@ -45,6 +66,7 @@ struct mbvh_mesh {
   uint2* d_bigleaf = nullptr;
   float4* d_tris = nullptr;
   uint32_t* d_vidx = nullptr;
+  uint32_t* d_face_slot = nullptr;  // indirect meshes: face -> slot (0xFFFFFFFF if absent)
   uint64_t n_nodes = 0;
   uint32_t n_roots = 0;
   uint64_t n_slots = 0;
@@ -81,6 +103,27 @@ int ensure_buffer(DeviceBuffer& b, size_t bytes) {
                      hipGetErrorString(e));
   }
   b.bytes = want;
+  return MBVH_OK;
+}
+
+// Per-thread stack overflow entries of a context: enough for a stack of max_stack entries with
+// the smallest LDS part any kernel uses (8). Sized for a 64-deep stack on first need, so the region
+// (whose address launches and captured graphs hold) moves only for deeper trees.
+constexpr uint32_t kMinLdsStack = 8;
+constexpr uint32_t kOvfEntriesDefault = 64 - kMinLdsStack;
+
+int reserve_stack(mbvh_ctx* ctx, uint32_t max_stack) {
+  if (max_stack <= kMinLdsStack) return MBVH_OK;
+  const uint32_t need = max_stack - kMinLdsStack;
+  if (need <= ctx->stack_ovf_entries) return MBVH_OK;
+  const uint32_t entries = std::max(need, kOvfEntriesDefault);
+  if (ctx->stack_ovf.ptr) {
+    const hipError_t e = hipDeviceSynchronize();  // earlier launches may still use the old region
+    if (e != hipSuccess) return set_error(MBVH_ERR_HIP, "stack scratch: %s", hipGetErrorString(e));
+  }
+  const int rc = ensure_buffer(ctx->stack_ovf, static_cast<size_t>(entries) * ctx->max_threads * sizeof(uint2));
+  if (rc) return rc;
+  ctx->stack_ovf_entries = entries;
   return MBVH_OK;
 }
 
@@ -164,6 +207,8 @@ void free_mesh_buffers(mbvh_mesh* m) {
   if (m->d_bigleaf) (void)hipFree(m->d_bigleaf);
   if (m->d_tris) (void)hipFree(m->d_tris);
   if (m->d_vidx) (void)hipFree(m->d_vidx);
+  if (m->d_face_slot) (void)hipFree(m->d_face_slot);
+  m->d_face_slot = nullptr;
   m->d_nodes = nullptr;
   m->d_roots = nullptr;
   m->d_bigleaf = nullptr;
@@ -181,6 +226,21 @@ MeshView view_of(const mbvh_mesh* m) {
   v.max_stack = m->max_stack;
   v.coord_max = m->coord_max;
   return v;
+}
+
+GradMesh grad_view_of(const mbvh_mesh* m) {
+  GradMesh g;
+  g.tris = m->d_tris;
+  g.vidx = m->d_vidx;
+  g.face_slot = m->d_face_slot;
+  g.face_limit = m->face_limit;
+  return g;
+}
+
+uint32_t tris_face(const std::vector<float4>& tris, uint64_t slot) {
+  uint32_t f;
+  std::memcpy(&f, &tris[3 * slot].w, 4);
+  return f;
 }
 
 StackScratch stack_of(const mbvh_ctx* c, uint64_t threads) {
@@ -256,6 +316,11 @@ extern "C" void mbvh_ctx_destroy(mbvh_ctx* ctx) {
     if (b.ptr) (void)hipFree(b.ptr);
   for (auto& b : ctx->hits)
     if (b.ptr) (void)hipFree(b.ptr);
+  if (ctx->order.ptr) (void)hipFree(ctx->order.ptr);
+  for (auto& b : ctx->grad)
+    if (b.ptr) (void)hipFree(b.ptr);
+  if (ctx->grad_ws.ptr) (void)hipFree(ctx->grad_ws.ptr);
+  if (ctx->order_ws.ptr) (void)hipFree(ctx->order_ws.ptr);
   (void)hipStreamDestroy(ctx->stream);
   ctx->magic = 0;
   delete ctx;
@@ -356,14 +421,7 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
 
   // Stack: one pending entry per internal ancestor at most.
   const uint32_t max_stack = max_depth + 1;
-  if (max_stack > static_cast<uint32_t>(kStackLds)) {
-    const uint32_t need = max_stack - kStackLds;
-    if (need > ctx->stack_ovf_entries) {
-      rc = ensure_buffer(ctx->stack_ovf, static_cast<size_t>(need) * ctx->max_threads * sizeof(uint2));
-      if (rc) return rc;
-      ctx->stack_ovf_entries = need;
-    }
-  }
+  if ((rc = reserve_stack(ctx, max_stack))) return rc;
 
   mbvh_mesh* m = new (std::nothrow) mbvh_mesh();
   if (!m) return set_error(MBVH_ERR_OOM, "mbvh_mesh_upload: out of host memory");
@@ -394,6 +452,11 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_bigleaf), bigleaf.data(), bigleaf.size() * sizeof(uint2));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_tris), tris.data(), tris.size() * sizeof(float4));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_vidx), vidx.data(), vidx.size() * sizeof(uint32_t));
+  if (!rc && indirect) {
+    std::vector<uint32_t> face_slot(face_limit, 0xFFFFFFFFu);
+    for (uint64_t sl = 0; sl < n_slots; sl++) face_slot[tris_face(tris, sl)] = static_cast<uint32_t>(sl);
+    rc = upload(reinterpret_cast<void**>(&m->d_face_slot), face_slot.data(), face_slot.size() * sizeof(uint32_t));
+  }
   if (rc) {
     free_mesh_buffers(m);
     m->magic = 0;
@@ -450,7 +513,7 @@ extern "C" int mbvh_mesh_set_side_mask(mbvh_ctx* ctx, mbvh_mesh* mesh, const uin
 namespace {
 
 int raycast_first_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o, const float* d,
-                       uint64_t n, int32_t side, float near_dist, float far_dist, uint32_t* face,
+                       uint64_t n, const uint32_t* order, int32_t side, float near_dist, float far_dist, uint32_t* face,
                        float* dist, float* point, float* bary, hipStream_t s) {
   if (side < MBVH_FRONT_SIDE || side > MBVH_SIDE_PER_TRIANGLE)
     return set_error(MBVH_ERR_BAD_ARGS, "raycast_first: invalid side %d", side);
@@ -463,6 +526,7 @@ int raycast_first_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o, con
   a.stack = stack_of(ctx, threads);
   a.origins = o;
   a.directions = d;
+  a.order = order;
   a.n = n;
   a.side_mask = side_mask_of(side);
   a.near_dist = near_dist;
@@ -475,7 +539,8 @@ int raycast_first_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o, con
   return MBVH_OK;
 }
 
-int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uint64_t n, float min_t,
+int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uint64_t n,
+                       const uint32_t* order, float min_t,
                        float max_t, uint32_t* face, float* dist, float* point, float* bary, hipStream_t s) {
   if (n && (!q || !face)) return set_error(MBVH_ERR_BAD_ARGS, "closest_point: NULL point or face buffer");
   if (mesh->max_stack > kStackLds + ctx->stack_ovf_entries)
@@ -485,6 +550,7 @@ int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uin
   a.mesh = view_of(mesh);
   a.stack = stack_of(ctx, threads);
   a.points = q;
+  a.order = order;
   a.n = n;
   a.min_threshold = min_t;
   a.max_threshold = max_t;
@@ -493,6 +559,23 @@ int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uin
   a.out_point = point;
   a.out_bary = bary;
   MBVH_HIP(launch_closest_point(a, threads, s));
+  return MBVH_OK;
+}
+
+// Host entry points order queries spatially from this many on (below it a launch has few waves and
+// the ordering passes cost more than they save).
+constexpr uint64_t kOrderMin = 4096;
+
+// Orders the n packed triples at `p` (device) into ctx->order; returns the order or NULL (small n).
+int host_order(mbvh_ctx* ctx, const float* p, uint64_t n, const uint32_t** out) {
+  *out = nullptr;
+  if (n < kOrderMin || n >= (uint64_t{1} << 32)) return MBVH_OK;
+  int rc;
+  if ((rc = ensure_buffer(ctx->order, n * sizeof(uint32_t))) ||
+      (rc = ensure_buffer(ctx->order_ws, order_workspace_bytes(n))))
+    return rc;
+  MBVH_HIP(launch_order_points(p, n, static_cast<uint32_t*>(ctx->order.ptr), ctx->order_ws.ptr, ctx->stream));
+  *out = static_cast<const uint32_t*>(ctx->order.ptr);
   return MBVH_OK;
 }
 
@@ -507,7 +590,7 @@ int check_no_overflow(const uint32_t* face, uint64_t n, const char* fn) {
 }  // namespace
 
 extern "C" int mbvh_raycast_first_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins,
-                                    const float* directions, uint64_t n, int32_t side, float near_dist,
+                                    const float* directions, uint64_t n, const uint32_t* order, int32_t side, float near_dist,
                                     float far_dist, uint32_t* face_index, float* distance, float* point,
                                     float* barycoord, void* stream) {
   clear_error();
@@ -516,7 +599,7 @@ extern "C" int mbvh_raycast_first_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const 
   rc = set_device(ctx);
   if (rc) return rc;
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-  return raycast_first_impl(ctx, mesh, origins, directions, n, side, near_dist, far_dist, face_index,
+  return raycast_first_impl(ctx, mesh, origins, directions, n, order, side, near_dist, far_dist, face_index,
                             distance, point, barycoord, s);
 }
 
@@ -542,8 +625,10 @@ extern "C" int mbvh_raycast_first(mbvh_ctx* ctx, const mbvh_mesh* mesh, const fl
   hipStream_t s = ctx->stream;
   MBVH_HIP(hipMemcpyAsync(ctx->staging[0].ptr, origins, v3, hipMemcpyHostToDevice, s));
   MBVH_HIP(hipMemcpyAsync(ctx->staging[1].ptr, directions, v3, hipMemcpyHostToDevice, s));
+  const uint32_t* order = nullptr;
+  if ((rc = host_order(ctx, static_cast<float*>(ctx->staging[0].ptr), n, &order))) return rc;
   rc = raycast_first_impl(ctx, mesh, static_cast<float*>(ctx->staging[0].ptr),
-                          static_cast<float*>(ctx->staging[1].ptr), n, side, near_dist, far_dist,
+                          static_cast<float*>(ctx->staging[1].ptr), n, order, side, near_dist, far_dist,
                           static_cast<uint32_t*>(ctx->staging[2].ptr), static_cast<float*>(ctx->staging[3].ptr),
                           static_cast<float*>(ctx->staging[4].ptr), static_cast<float*>(ctx->staging[5].ptr), s);
   if (rc) return rc;
@@ -556,7 +641,7 @@ extern "C" int mbvh_raycast_first(mbvh_ctx* ctx, const mbvh_mesh* mesh, const fl
 }
 
 extern "C" int mbvh_closest_point_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
-                                    float min_threshold, float max_threshold, uint32_t* face_index,
+                                    const uint32_t* order, float min_threshold, float max_threshold, uint32_t* face_index,
                                     float* distance, float* closest, float* barycoord, void* stream) {
   clear_error();
   int rc = check_query("mbvh_closest_point_d", ctx, mesh);
@@ -564,7 +649,7 @@ extern "C" int mbvh_closest_point_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const 
   rc = set_device(ctx);
   if (rc) return rc;
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-  return closest_point_impl(ctx, mesh, points, n, min_threshold, max_threshold, face_index, distance,
+  return closest_point_impl(ctx, mesh, points, n, order, min_threshold, max_threshold, face_index, distance,
                             closest, barycoord, s);
 }
 
@@ -586,7 +671,9 @@ extern "C" int mbvh_closest_point(mbvh_ctx* ctx, const mbvh_mesh* mesh, const fl
     return rc;
   hipStream_t s = ctx->stream;
   MBVH_HIP(hipMemcpyAsync(ctx->staging[0].ptr, points, v3, hipMemcpyHostToDevice, s));
-  rc = closest_point_impl(ctx, mesh, static_cast<float*>(ctx->staging[0].ptr), n, min_threshold, max_threshold,
+  const uint32_t* order = nullptr;
+  if ((rc = host_order(ctx, static_cast<float*>(ctx->staging[0].ptr), n, &order))) return rc;
+  rc = closest_point_impl(ctx, mesh, static_cast<float*>(ctx->staging[0].ptr), n, order, min_threshold, max_threshold,
                           static_cast<uint32_t*>(ctx->staging[2].ptr), static_cast<float*>(ctx->staging[3].ptr),
                           static_cast<float*>(ctx->staging[4].ptr), static_cast<float*>(ctx->staging[5].ptr), s);
   if (rc) return rc;
@@ -738,5 +825,576 @@ extern "C" int mbvh_raycast_fetch(mbvh_ctx* ctx, uint32_t* face_index, float* di
   if (point) MBVH_HIP(hipMemcpyAsync(point, ctx->hits[3].ptr, t * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
   if (barycoord) MBVH_HIP(hipMemcpyAsync(barycoord, ctx->hits[4].ptr, t * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
   MBVH_HIP(hipStreamSynchronize(s));
+  return MBVH_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Query ordering
+// ---------------------------------------------------------------------------------------------
+
+extern "C" int mbvh_order_workspace_bytes(uint64_t n, uint64_t* bytes) {
+  clear_error();
+  if (!bytes) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_order_workspace_bytes: NULL output");
+  *bytes = order_workspace_bytes(n);
+  return MBVH_OK;
+}
+
+extern "C" int mbvh_order_points_d(mbvh_ctx* ctx, const float* points, uint64_t n, uint32_t* order,
+                                   void* workspace, uint64_t workspace_bytes, void* stream) {
+  clear_error();
+  if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_order_points_d: invalid context");
+  if (n == 0) return MBVH_OK;
+  if (n >= (uint64_t{1} << 32)) return set_error(MBVH_ERR_LIMIT, "mbvh_order_points_d: n must be below 2^32");
+  if (!points || !order || !workspace)
+    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_order_points_d: NULL points, order or workspace");
+  if (workspace_bytes < order_workspace_bytes(n))
+    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_order_points_d: workspace of %llu bytes, %llu needed",
+                     static_cast<unsigned long long>(workspace_bytes),
+                     static_cast<unsigned long long>(order_workspace_bytes(n)));
+  if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
+    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_order_points_d: workspace must be 256-byte aligned");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  MBVH_HIP(launch_order_points(points, n, order, workspace, s));
+  return MBVH_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Backward
+// ---------------------------------------------------------------------------------------------
+
+namespace {
+
+int check_mode(const char* fn, int32_t mode) {
+  if (mode != MBVH_GRAD_ATOMIC && mode != MBVH_GRAD_DETERMINISTIC)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: mode must be MBVH_GRAD_ATOMIC or MBVH_GRAD_DETERMINISTIC", fn);
+  return MBVH_OK;
+}
+
+int check_workspace(const char* fn, uint64_t n, int32_t mode, const float* grad_positions, const void* ws,
+                    uint64_t ws_bytes) {
+  if (!grad_positions || mode != MBVH_GRAD_DETERMINISTIC || n == 0) return MBVH_OK;
+  const uint64_t need = grad_workspace_bytes(n, mode);
+  if (!ws || ws_bytes < need)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: deterministic mode needs a workspace of %llu bytes", fn,
+                     static_cast<unsigned long long>(need));
+  if (reinterpret_cast<uintptr_t>(ws) % 256 != 0)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: workspace must be 256-byte aligned", fn);
+  return MBVH_OK;
+}
+
+int raycast_bwd_impl(const char* fn, mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* directions, uint64_t n,
+                     const uint32_t* face, const float* distance, const float* bary, const float* g_dist,
+                     const float* g_point, const float* g_bary, float* g_origins, float* g_dirs, float* g_pos,
+                     int32_t mode, void* ws, uint64_t ws_bytes, hipStream_t s) {
+  int rc = check_mode(fn, mode);
+  if (rc) return rc;
+  if (n && (!directions || !face || !distance || !bary))
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL direction or saved forward buffer", fn);
+  if (mesh->vertex_count >= 0xFFFFFFFFull) return set_error(MBVH_ERR_LIMIT, "%s: too many vertices", fn);
+  if (n >= (uint64_t{1} << 32) / 3) return set_error(MBVH_ERR_LIMIT, "%s: too many queries", fn);
+  if ((rc = check_workspace(fn, n, mode, g_pos, ws, ws_bytes))) return rc;
+  RaycastBwdArgs a{};
+  a.mesh = grad_view_of(mesh);
+  a.directions = directions;
+  a.face = face;
+  a.distance = distance;
+  a.barycoord = bary;
+  a.grad_distance = g_dist;
+  a.grad_point = g_point;
+  a.grad_barycoord = g_bary;
+  a.grad_origins = g_origins;
+  a.grad_directions = g_dirs;
+  a.n = n;
+  MBVH_HIP(launch_raycast_first_bwd(a, mode, g_pos, mesh->vertex_count, ws, s));
+  return MBVH_OK;
+}
+
+int closest_bwd_impl(const char* fn, mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
+                     const uint32_t* face, const float* distance, const float* closest, const float* bary,
+                     const float* g_dist, float* g_points, float* g_pos, int32_t mode, void* ws,
+                     uint64_t ws_bytes, hipStream_t s) {
+  (void)ctx;
+  int rc = check_mode(fn, mode);
+  if (rc) return rc;
+  if (n && (!points || !face || !distance || !closest || !bary || !g_dist))
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL point, saved forward or upstream buffer", fn);
+  if (mesh->vertex_count >= 0xFFFFFFFFull) return set_error(MBVH_ERR_LIMIT, "%s: too many vertices", fn);
+  if (n >= (uint64_t{1} << 32) / 3) return set_error(MBVH_ERR_LIMIT, "%s: too many queries", fn);
+  if ((rc = check_workspace(fn, n, mode, g_pos, ws, ws_bytes))) return rc;
+  ClosestBwdArgs a{};
+  a.mesh = grad_view_of(mesh);
+  a.points = points;
+  a.face = face;
+  a.distance = distance;
+  a.closest = closest;
+  a.barycoord = bary;
+  a.grad_distance = g_dist;
+  a.grad_points = g_points;
+  a.n = n;
+  MBVH_HIP(launch_closest_point_bwd(a, mode, g_pos, mesh->vertex_count, ws, s));
+  return MBVH_OK;
+}
+
+// Copies `bytes` of host data into staging buffer b (or returns NULL for a NULL source).
+int stage_in(mbvh_ctx* ctx, DeviceBuffer& b, const void* src, size_t bytes, void** out) {
+  *out = nullptr;
+  if (!src) return MBVH_OK;
+  int rc = ensure_buffer(b, bytes);
+  if (rc) return rc;
+  MBVH_HIP(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  *out = b.ptr;
+  return MBVH_OK;
+}
+
+int stage_out(DeviceBuffer& b, const void* dst, size_t bytes, void** out) {
+  *out = nullptr;
+  if (!dst) return MBVH_OK;
+  int rc = ensure_buffer(b, bytes);
+  if (rc) return rc;
+  *out = b.ptr;
+  return MBVH_OK;
+}
+
+}  // namespace
+
+extern "C" int mbvh_grad_workspace_bytes(uint64_t n, int32_t mode, uint64_t* bytes) {
+  clear_error();
+  if (!bytes) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_grad_workspace_bytes: NULL output");
+  int rc = check_mode("mbvh_grad_workspace_bytes", mode);
+  if (rc) return rc;
+  *bytes = grad_workspace_bytes(n, mode);
+  return MBVH_OK;
+}
+
+extern "C" int mbvh_raycast_first_bwd_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* directions, uint64_t n,
+                                        const uint32_t* face_index, const float* distance, const float* barycoord,
+                                        const float* grad_distance, const float* grad_point,
+                                        const float* grad_barycoord, float* grad_origins, float* grad_directions,
+                                        float* grad_positions, int32_t mode, void* workspace,
+                                        uint64_t workspace_bytes, void* stream) {
+  clear_error();
+  int rc = check_query("mbvh_raycast_first_bwd_d", ctx, mesh);
+  if (rc) return rc;
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return raycast_bwd_impl("mbvh_raycast_first_bwd_d", ctx, mesh, directions, n, face_index, distance, barycoord,
+                          grad_distance, grad_point, grad_barycoord, grad_origins, grad_directions, grad_positions,
+                          mode, workspace, workspace_bytes, s);
+}
+
+extern "C" int mbvh_raycast_first_bwd(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* directions, uint64_t n,
+                                      const uint32_t* face_index, const float* distance, const float* barycoord,
+                                      const float* grad_distance, const float* grad_point,
+                                      const float* grad_barycoord, float* grad_origins, float* grad_directions,
+                                      float* grad_positions, int32_t mode) {
+  clear_error();
+  const char* fn = "mbvh_raycast_first_bwd";
+  int rc = check_query(fn, ctx, mesh);
+  if (rc) return rc;
+  if ((rc = check_mode(fn, mode))) return rc;
+  if (n && (!directions || !face_index || !distance || !barycoord))
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL direction or saved forward buffer", fn);
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  const size_t v3 = n * 3 * sizeof(float), s1 = n * sizeof(float), vp = mesh->vertex_count * 3 * sizeof(float);
+  void *dD, *dF, *dT, *dB, *gT, *gP, *gB, *oO, *oD, *oV;
+  if ((rc = stage_in(ctx, ctx->grad[0], directions, v3, &dD)) || (rc = stage_in(ctx, ctx->grad[1], face_index, s1, &dF)) ||
+      (rc = stage_in(ctx, ctx->grad[2], distance, s1, &dT)) || (rc = stage_in(ctx, ctx->grad[3], barycoord, v3, &dB)) ||
+      (rc = stage_in(ctx, ctx->grad[4], grad_distance, s1, &gT)) ||
+      (rc = stage_in(ctx, ctx->grad[5], grad_point, v3, &gP)) ||
+      (rc = stage_in(ctx, ctx->grad[6], grad_barycoord, v3, &gB)) ||
+      (rc = stage_out(ctx->grad[7], grad_origins, v3, &oO)) || (rc = stage_out(ctx->grad[8], grad_directions, v3, &oD)) ||
+      (rc = stage_out(ctx->grad[9], grad_positions, vp, &oV)))
+    return rc;
+  const uint64_t wsb = grad_positions ? grad_workspace_bytes(n, mode) : 0;
+  if (wsb && (rc = ensure_buffer(ctx->grad_ws, wsb))) return rc;
+  rc = raycast_bwd_impl(fn, ctx, mesh, static_cast<float*>(dD), n, static_cast<uint32_t*>(dF), static_cast<float*>(dT),
+                        static_cast<float*>(dB), static_cast<float*>(gT), static_cast<float*>(gP),
+                        static_cast<float*>(gB), static_cast<float*>(oO), static_cast<float*>(oD),
+                        static_cast<float*>(oV), mode, ctx->grad_ws.ptr, ctx->grad_ws.bytes, ctx->stream);
+  if (rc) return rc;
+  if (oO) MBVH_HIP(hipMemcpyAsync(grad_origins, oO, v3, hipMemcpyDeviceToHost, ctx->stream));
+  if (oD) MBVH_HIP(hipMemcpyAsync(grad_directions, oD, v3, hipMemcpyDeviceToHost, ctx->stream));
+  if (oV) MBVH_HIP(hipMemcpyAsync(grad_positions, oV, vp, hipMemcpyDeviceToHost, ctx->stream));
+  MBVH_HIP(hipStreamSynchronize(ctx->stream));
+  return MBVH_OK;
+}
+
+extern "C" int mbvh_closest_point_bwd_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
+                                        const uint32_t* face_index, const float* distance, const float* closest,
+                                        const float* barycoord, const float* grad_distance, float* grad_points,
+                                        float* grad_positions, int32_t mode, void* workspace,
+                                        uint64_t workspace_bytes, void* stream) {
+  clear_error();
+  int rc = check_query("mbvh_closest_point_bwd_d", ctx, mesh);
+  if (rc) return rc;
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return closest_bwd_impl("mbvh_closest_point_bwd_d", ctx, mesh, points, n, face_index, distance, closest, barycoord,
+                          grad_distance, grad_points, grad_positions, mode, workspace, workspace_bytes, s);
+}
+
+extern "C" int mbvh_closest_point_bwd(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
+                                      const uint32_t* face_index, const float* distance, const float* closest,
+                                      const float* barycoord, const float* grad_distance, float* grad_points,
+                                      float* grad_positions, int32_t mode) {
+  clear_error();
+  const char* fn = "mbvh_closest_point_bwd";
+  int rc = check_query(fn, ctx, mesh);
+  if (rc) return rc;
+  if ((rc = check_mode(fn, mode))) return rc;
+  if (n && (!points || !face_index || !distance || !closest || !barycoord || !grad_distance))
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL point, saved forward or upstream buffer", fn);
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  const size_t v3 = n * 3 * sizeof(float), s1 = n * sizeof(float), vp = mesh->vertex_count * 3 * sizeof(float);
+  void *dQ, *dF, *dT, *dC, *dB, *gT, *oQ, *oV;
+  if ((rc = stage_in(ctx, ctx->grad[0], points, v3, &dQ)) || (rc = stage_in(ctx, ctx->grad[1], face_index, s1, &dF)) ||
+      (rc = stage_in(ctx, ctx->grad[2], distance, s1, &dT)) || (rc = stage_in(ctx, ctx->grad[3], closest, v3, &dC)) ||
+      (rc = stage_in(ctx, ctx->grad[4], barycoord, v3, &dB)) ||
+      (rc = stage_in(ctx, ctx->grad[5], grad_distance, s1, &gT)) ||
+      (rc = stage_out(ctx->grad[7], grad_points, v3, &oQ)) || (rc = stage_out(ctx->grad[9], grad_positions, vp, &oV)))
+    return rc;
+  const uint64_t wsb = grad_positions ? grad_workspace_bytes(n, mode) : 0;
+  if (wsb && (rc = ensure_buffer(ctx->grad_ws, wsb))) return rc;
+  rc = closest_bwd_impl(fn, ctx, mesh, static_cast<float*>(dQ), n, static_cast<uint32_t*>(dF), static_cast<float*>(dT),
+                        static_cast<float*>(dC), static_cast<float*>(dB), static_cast<float*>(gT),
+                        static_cast<float*>(oQ), static_cast<float*>(oV), mode, ctx->grad_ws.ptr, ctx->grad_ws.bytes,
+                        ctx->stream);
+  if (rc) return rc;
+  if (oQ) MBVH_HIP(hipMemcpyAsync(grad_points, oQ, v3, hipMemcpyDeviceToHost, ctx->stream));
+  if (oV) MBVH_HIP(hipMemcpyAsync(grad_positions, oV, vp, hipMemcpyDeviceToHost, ctx->stream));
+  MBVH_HIP(hipStreamSynchronize(ctx->stream));
+  return MBVH_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Instances and batches
+// ---------------------------------------------------------------------------------------------
+
+namespace {
+
+constexpr uint32_t kBatchMagic = 0x48435442u;
+constexpr uint32_t kGraphMagic = 0x48505247u;
+
+bool valid_batch(const mbvh_batch* b) { return b && b->magic == kBatchMagic; }
+
+uint64_t batch_ws_bytes(const mbvh_batch* b, uint64_t n) {
+  return ((4 * n + 255) & ~uint64_t{255}) + order_by_mesh_workspace_bytes(n, b->n_meshes);
+}
+
+int check_tuning(const char* fn, const mbvh_batch_tuning& t) {
+  if ((t.stack_lds != 8 && t.stack_lds != 16) || t.tile_rays == 0 || t.tile_rays % 64 != 0 || t.tile_rays > 4096 ||
+      t.blocks_per_cu < 1 || t.blocks_per_cu > 8)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: invalid tuning (stack_lds 8|16, tile_rays 64..4096 step 64, "
+                     "blocks_per_cu 1..8)", fn);
+  return MBVH_OK;
+}
+
+struct BatchCall {
+  const float* world;
+  const float* inverse;
+  const uint32_t* inst_geom;
+  uint32_t n_inst;
+  const float* origins;
+  const float* dirs;
+  const uint32_t* ray_inst;
+  uint64_t n;
+  int32_t side;
+  float near_d, far_d;
+  uint32_t* face;
+  float* dist;
+  float* point;
+  float* bary;
+  void* ws;
+  uint64_t ws_bytes;
+};
+
+int check_batch_call(const char* fn, const mbvh_ctx* ctx, const mbvh_batch* b, const BatchCall& c) {
+  if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid context handle", fn);
+  if (!valid_batch(b) || b->ctx != ctx) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid batch handle", fn);
+  if (c.side < MBVH_FRONT_SIDE || c.side > MBVH_SIDE_PER_TRIANGLE)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: invalid side %d", fn, c.side);
+  if (c.n >= (uint64_t{1} << 32)) return set_error(MBVH_ERR_LIMIT, "%s: n must be below 2^32", fn);
+  if (c.n == 0) return MBVH_OK;
+  if (!c.world || !c.inverse || !c.inst_geom || !c.origins || !c.dirs || !c.ray_inst || !c.face)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL matrix, instance, ray or face buffer", fn);
+  const uint64_t need = batch_ws_bytes(b, c.n);
+  if (!c.ws || c.ws_bytes < need)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: workspace of %llu bytes needed", fn, static_cast<unsigned long long>(need));
+  if (reinterpret_cast<uintptr_t>(c.ws) % 256 != 0)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: workspace must be 256-byte aligned", fn);
+  return MBVH_OK;
+}
+
+int enqueue_batch(mbvh_ctx* ctx, mbvh_batch* b, const BatchCall& c, const mbvh_batch_tuning& t, hipStream_t s) {
+  uint32_t* order = static_cast<uint32_t*>(c.ws);
+  char* ows = static_cast<char*>(c.ws) + ((4 * c.n + 255) & ~uint64_t{255});
+  if (c.n) MBVH_HIP(launch_order_by_mesh(c.ray_inst, c.inst_geom, c.n_inst, c.n, b->n_meshes, order, ows, s));
+  int device_cus = static_cast<int>(ctx->max_threads / kMaxThreadsPerCU);
+  const uint64_t tiles = (c.n + t.tile_rays - 1) / t.tile_rays;
+  uint64_t blocks = static_cast<uint64_t>(device_cus) * t.blocks_per_cu;
+  const uint64_t useful = (tiles + kBlock / 64 - 1) / (kBlock / 64);  // one wave per tile at most
+  if (blocks > useful) blocks = std::max<uint64_t>(useful, 1);
+  BatchArgs a{};
+  a.meshes = b->d_meshes;
+  a.n_meshes = b->n_meshes;
+  a.world = c.world;
+  a.inverse = c.inverse;
+  a.instance_geometry = c.inst_geom;
+  a.n_instances = c.n_inst;
+  a.origins = c.origins;
+  a.directions = c.dirs;
+  a.ray_instance = c.ray_inst;
+  a.order = c.n ? order : nullptr;
+  a.n = c.n;
+  a.side_mask = side_mask_of(c.side);
+  a.near_dist = c.near_d;
+  a.far_dist = c.far_d;
+  a.out_face = c.face;
+  a.out_dist = c.dist;
+  a.out_point = c.point;
+  a.out_bary = c.bary;
+  a.counter = b->d_counter;
+  a.tile = t.tile_rays;
+  a.n_tiles = tiles;
+  a.stack = stack_of(ctx, blocks * kBlock);
+  MBVH_HIP(launch_batch_raycast_first(a, t.stack_lds, static_cast<uint32_t>(blocks), s));
+  return MBVH_OK;
+}
+
+}  // namespace
+
+extern "C" int mbvh_batch_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, uint32_t n_meshes,
+                                 mbvh_batch** out_batch) {
+  clear_error();
+  if (!out_batch) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_batch_create: out_batch is NULL");
+  *out_batch = nullptr;
+  if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_batch_create: invalid context handle");
+  if (n_meshes == 0 || !meshes) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_batch_create: no meshes");
+  std::vector<MeshView> views(n_meshes);
+  uint32_t max_stack = 0;
+  for (uint32_t i = 0; i < n_meshes; i++) {
+    if (!valid_mesh(meshes[i]) || meshes[i]->ctx != ctx)
+      return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_batch_create: mesh %u is not a mesh of this context", i);
+    views[i] = view_of(meshes[i]);
+    max_stack = std::max(max_stack, meshes[i]->max_stack);
+  }
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  if ((rc = reserve_stack(ctx, max_stack))) return rc;
+  mbvh_batch* b = new (std::nothrow) mbvh_batch();
+  if (!b) return set_error(MBVH_ERR_OOM, "mbvh_batch_create: out of host memory");
+  b->ctx = ctx;
+  b->n_meshes = n_meshes;
+  b->max_stack = max_stack;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&b->d_meshes), views.size() * sizeof(MeshView));
+  if (e == hipSuccess) e = hipMemcpy(b->d_meshes, views.data(), views.size() * sizeof(MeshView), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_counter), 256);
+  if (e != hipSuccess) {
+    if (b->d_meshes) (void)hipFree(b->d_meshes);
+    if (b->d_counter) (void)hipFree(b->d_counter);
+    delete b;
+    return set_error(MBVH_ERR_OOM, "mbvh_batch_create: device allocation failed (%s)", hipGetErrorString(e));
+  }
+  *out_batch = b;
+  return MBVH_OK;
+}
+
+extern "C" void mbvh_batch_free(mbvh_batch* batch) {
+  if (!valid_batch(batch)) return;
+  (void)hipSetDevice(batch->ctx->device);
+  (void)hipDeviceSynchronize();
+  (void)hipFree(batch->d_meshes);
+  (void)hipFree(batch->d_counter);
+  batch->magic = 0;
+  delete batch;
+}
+
+extern "C" int mbvh_batch_workspace_bytes(const mbvh_batch* batch, uint64_t n_rays, uint64_t* bytes) {
+  clear_error();
+  if (!valid_batch(batch)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_batch_workspace_bytes: invalid batch handle");
+  if (!bytes) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_batch_workspace_bytes: NULL output");
+  *bytes = batch_ws_bytes(batch, n_rays);
+  return MBVH_OK;
+}
+
+#define MBVH_BATCH_CALL                                                                          \
+  BatchCall c{matrix_world, matrix_world_inverse, instance_geometry, n_instances, origins, directions, \
+              ray_instance, n, side, near_dist, far_dist, face_index, distance, point, barycoord,  \
+              workspace, workspace_bytes}
+
+extern "C" int mbvh_batch_raycast_first_d(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_world,
+                                          const float* matrix_world_inverse, const uint32_t* instance_geometry,
+                                          uint32_t n_instances, const float* origins, const float* directions,
+                                          const uint32_t* ray_instance, uint64_t n, int32_t side, float near_dist,
+                                          float far_dist, uint32_t* face_index, float* distance, float* point,
+                                          float* barycoord, void* workspace, uint64_t workspace_bytes,
+                                          const mbvh_batch_tuning* tuning, void* stream) {
+  clear_error();
+  const char* fn = "mbvh_batch_raycast_first_d";
+  MBVH_BATCH_CALL;
+  int rc = check_batch_call(fn, ctx, batch, c);
+  if (rc) return rc;
+  const mbvh_batch_tuning t = tuning ? *tuning : batch->tuning;
+  if ((rc = check_tuning(fn, t))) return rc;
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return enqueue_batch(ctx, batch, c, t, s);
+}
+
+extern "C" int mbvh_batch_autotune(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_world,
+                                   const float* matrix_world_inverse, const uint32_t* instance_geometry,
+                                   uint32_t n_instances, const float* origins, const float* directions,
+                                   const uint32_t* ray_instance, uint64_t n, int32_t side, float near_dist,
+                                   float far_dist, uint32_t* face_index, float* distance, float* point,
+                                   float* barycoord, void* workspace, uint64_t workspace_bytes,
+                                   mbvh_batch_tuning* out_tuning) {
+  clear_error();
+  const char* fn = "mbvh_batch_autotune";
+  MBVH_BATCH_CALL;
+  int rc = check_batch_call(fn, ctx, batch, c);
+  if (rc) return rc;
+  if (!out_tuning) return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL output", fn);
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t s = ctx->stream;
+  hipEvent_t e0, e1;
+  MBVH_HIP(hipEventCreate(&e0));
+  MBVH_HIP(hipEventCreate(&e1));
+  mbvh_batch_tuning best{16, 256, 4, 3.4e38f};
+  static const uint32_t kStacks[] = {8, 16};
+  static const uint32_t kTiles[] = {64, 128, 256, 512};
+  static const uint32_t kBlocks[] = {1, 2, 4, 8};
+  for (uint32_t sl : kStacks)
+    for (uint32_t tl : kTiles)
+      for (uint32_t bl : kBlocks) {
+        const mbvh_batch_tuning t{sl, tl, bl, 0.0f};
+        if ((rc = enqueue_batch(ctx, batch, c, t, s))) break;  // warm-up
+        float total = 0.0f;
+        for (int rep = 0; rep < 3 && !rc; rep++) {
+          float ms = 0.0f;
+          if (hipEventRecord(e0, s) != hipSuccess || (rc = enqueue_batch(ctx, batch, c, t, s)) ||
+              hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            if (!rc) rc = set_error(MBVH_ERR_HIP, "%s: timing failed", fn);
+            break;
+          }
+          total += ms;
+        }
+        if (rc) break;
+        if (total / 3.0f < best.ms) best = mbvh_batch_tuning{sl, tl, bl, total / 3.0f};
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  batch->tuning = best;
+  *out_tuning = best;
+  return MBVH_OK;
+}
+
+extern "C" int mbvh_batch_graph_create(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_world,
+                                       const float* matrix_world_inverse, const uint32_t* instance_geometry,
+                                       uint32_t n_instances, const float* origins, const float* directions,
+                                       const uint32_t* ray_instance, uint64_t n, int32_t side, float near_dist,
+                                       float far_dist, uint32_t* face_index, float* distance, float* point,
+                                       float* barycoord, void* workspace, uint64_t workspace_bytes,
+                                       const mbvh_batch_tuning* tuning, mbvh_batch_graph** out_graph) {
+  clear_error();
+  const char* fn = "mbvh_batch_graph_create";
+  if (!out_graph) return set_error(MBVH_ERR_BAD_ARGS, "%s: out_graph is NULL", fn);
+  *out_graph = nullptr;
+  MBVH_BATCH_CALL;
+  int rc = check_batch_call(fn, ctx, batch, c);
+  if (rc) return rc;
+  const mbvh_batch_tuning t = tuning ? *tuning : batch->tuning;
+  if ((rc = check_tuning(fn, t))) return rc;
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t cs;
+  MBVH_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    rc = enqueue_batch(ctx, batch, c, t, cs);
+    e = hipStreamEndCapture(cs, &graph);
+  }
+  (void)hipStreamDestroy(cs);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return set_error(MBVH_ERR_HIP, "%s: capture failed (%s)", fn, hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return set_error(MBVH_ERR_HIP, "%s: instantiate failed (%s)", fn, hipGetErrorString(e));
+  }
+  mbvh_batch_graph* g = new (std::nothrow) mbvh_batch_graph();
+  if (!g) {
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    return set_error(MBVH_ERR_OOM, "%s: out of host memory", fn);
+  }
+  g->ctx = ctx;
+  g->graph = graph;
+  g->exec = exec;
+  *out_graph = g;
+  return MBVH_OK;
+}
+
+extern "C" int mbvh_batch_graph_launch(mbvh_batch_graph* graph, void* stream) {
+  clear_error();
+  if (!graph || graph->magic != kGraphMagic || !valid_ctx(graph->ctx))
+    return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_batch_graph_launch: invalid graph handle");
+  int rc = set_device(graph->ctx);
+  if (rc) return rc;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : graph->ctx->stream;
+  MBVH_HIP(hipGraphLaunch(graph->exec, s));
+  return MBVH_OK;
+}
+
+extern "C" void mbvh_batch_graph_free(mbvh_batch_graph* graph) {
+  if (!graph || graph->magic != kGraphMagic) return;
+  (void)hipSetDevice(graph->ctx->device);
+  (void)hipDeviceSynchronize();
+  (void)hipGraphExecDestroy(graph->exec);
+  (void)hipGraphDestroy(graph->graph);
+  graph->magic = 0;
+  delete graph;
+}
+
+extern "C" int mbvh_instance_rays_d(mbvh_ctx* ctx, const float* matrix_world_inverse, uint32_t n_instances,
+                                    const float* origins, const float* directions, const uint32_t* ray_instance,
+                                    uint64_t n, float* local_origins, float* local_directions, void* stream) {
+  clear_error();
+  if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_instance_rays_d: invalid context handle");
+  if (n == 0) return MBVH_OK;
+  if (!matrix_world_inverse || !origins || !directions || !ray_instance || !local_origins || !local_directions)
+    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_instance_rays_d: NULL buffer");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  InstanceRayArgs a{matrix_world_inverse, n_instances, origins, directions, ray_instance, n, local_origins,
+                    local_directions};
+  MBVH_HIP(launch_instance_rays(a, stream ? static_cast<hipStream_t>(stream) : ctx->stream));
+  return MBVH_OK;
+}
+
+extern "C" int mbvh_instance_hits_d(mbvh_ctx* ctx, const float* matrix_world, uint32_t n_instances,
+                                    const float* origins, const uint32_t* ray_instance, uint64_t n,
+                                    const uint32_t* face_index, float* point, float* distance, void* stream) {
+  clear_error();
+  if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_instance_hits_d: invalid context handle");
+  if (n == 0) return MBVH_OK;
+  if (!matrix_world || !origins || !ray_instance || !face_index || !point || !distance)
+    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_instance_hits_d: NULL buffer");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  InstanceHitArgs a{matrix_world, n_instances, origins, ray_instance, n, face_index, point, distance};
+  MBVH_HIP(launch_instance_hits(a, stream ? static_cast<hipStream_t>(stream) : ctx->stream));
   return MBVH_OK;
 }

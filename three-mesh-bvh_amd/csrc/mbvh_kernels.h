@@ -29,6 +29,7 @@ struct RaycastFirstArgs {
   StackScratch stack;
   const float* origins;
   const float* directions;
+  const uint32_t* order;  // visit order of the rays (NULL: 0..n-1); results go to each ray's own index
   uint64_t n;
   uint32_t side_mask;  // 1 front, 2 back, 3 both; 0 = per-triangle masks stored in the mesh
   float near_dist;
@@ -61,6 +62,7 @@ struct ClosestPointArgs {
   MeshView mesh;
   StackScratch stack;
   const float* points;
+  const uint32_t* order;  // visit order of the points (NULL: 0..n-1); results go to each point's own index
   uint64_t n;
   float min_threshold;
   float max_threshold;
@@ -70,6 +72,121 @@ struct ClosestPointArgs {
   float* out_bary;
 };
 
+// ---- instances and the batched multi-mesh launch -------------------------------------------------
+struct InstanceRayArgs {
+  const float* inverse;          // 16 floats per instance (world -> local, column-major)
+  uint32_t n_instances;
+  const float* origins;          // world rays
+  const float* directions;
+  const uint32_t* ray_instance;
+  uint64_t n;
+  float* out_origins;            // local rays (NaN for an invalid instance)
+  float* out_directions;
+};
+
+struct InstanceHitArgs {
+  const float* world;            // 16 floats per instance (local -> world)
+  uint32_t n_instances;
+  const float* origins;          // world ray origins
+  const uint32_t* ray_instance;
+  uint64_t n;
+  const uint32_t* face;          // hits (misses are left alone)
+  float* point;                  // local in, world out
+  float* distance;               // world distance out
+};
+
+struct BatchArgs {
+  const MeshView* meshes;
+  uint32_t n_meshes;
+  const float* world;
+  const float* inverse;
+  const uint32_t* instance_geometry;
+  uint32_t n_instances;
+  const float* origins;
+  const float* directions;
+  const uint32_t* ray_instance;
+  const uint32_t* order;         // visit order (rays grouped by mesh), may be NULL
+  uint64_t n;
+  uint32_t side_mask;
+  float near_dist;
+  float far_dist;
+  uint32_t* out_face;
+  float* out_dist;
+  float* out_point;
+  float* out_bary;
+  uint32_t* counter;             // work-queue counter (reset by the launch)
+  uint32_t tile;                 // rays per work item
+  uint64_t n_tiles;
+  StackScratch stack;
+};
+
+hipError_t launch_instance_rays(const InstanceRayArgs& a, hipStream_t s);
+hipError_t launch_instance_hits(const InstanceHitArgs& a, hipStream_t s);
+// stack_lds: 8 or 16 LDS stack entries per lane; blocks: persistent grid size
+hipError_t launch_batch_raycast_first(const BatchArgs& a, uint32_t stack_lds, uint32_t blocks, hipStream_t s);
+// Visit order grouping rays by the mesh of their instance (counting sort over n_meshes keys);
+// workspace of order_by_mesh_workspace_bytes(n, n_meshes) bytes.
+uint64_t order_by_mesh_workspace_bytes(uint64_t n, uint32_t n_meshes);
+hipError_t launch_order_by_mesh(const uint32_t* ray_instance, const uint32_t* instance_geometry, uint32_t n_instances,
+                                uint64_t n, uint32_t n_meshes, uint32_t* order, void* workspace, hipStream_t s);
+
+// ---- backward ----------------------------------------------------------------------------------
+constexpr int32_t kGradAtomic = 0;
+constexpr int32_t kGradDeterministic = 1;
+
+struct GradMesh {
+  const float4* tris;         // 3 per slot (vertex positions in .xyz)
+  const uint32_t* vidx;       // 3 vertex ids per slot
+  const uint32_t* face_slot;  // face -> slot for indirect meshes (NULL: slot = face)
+  uint64_t face_limit;        // faces are < face_limit
+};
+
+struct VertexSink {
+  float* grad;        // atomic mode: vertex gradients, 3 floats per vertex
+  uint32_t* keys;     // deterministic mode: vertex id of each contribution (3 per query)
+  uint32_t* vals;     // contribution index
+  float* contrib;     // 3 floats per contribution
+  uint32_t sentinel;  // vertex id meaning "no contribution" (= vertex count)
+};
+
+struct RaycastBwdArgs {
+  GradMesh mesh;
+  const float* directions;
+  const uint32_t* face;         // saved forward outputs
+  const float* distance;
+  const float* barycoord;
+  const float* grad_distance;   // upstream gradients (each may be NULL = 0)
+  const float* grad_point;
+  const float* grad_barycoord;
+  float* grad_origins;          // outputs (each may be NULL)
+  float* grad_directions;
+  uint64_t n;
+  bool want_vertices;
+  VertexSink sink;
+};
+
+struct ClosestBwdArgs {
+  GradMesh mesh;
+  const float* points;
+  const uint32_t* face;         // saved forward outputs
+  const float* distance;
+  const float* closest;
+  const float* barycoord;
+  const float* grad_distance;   // upstream gradient
+  float* grad_points;           // output (may be NULL)
+  uint64_t n;
+  bool want_vertices;
+  VertexSink sink;
+};
+
+// Device workspace the backward needs for n queries (0 for the atomic mode), 256-byte aligned.
+uint64_t grad_workspace_bytes(uint64_t n_queries, int32_t mode);
+// grad_positions (NULL: no vertex gradients) receives n_vertices x 3 floats, overwritten.
+hipError_t launch_raycast_first_bwd(const RaycastBwdArgs& a, int32_t mode, float* grad_positions,
+                                    uint64_t n_vertices, void* workspace, hipStream_t s);
+hipError_t launch_closest_point_bwd(const ClosestBwdArgs& a, int32_t mode, float* grad_positions,
+                                    uint64_t n_vertices, void* workspace, hipStream_t s);
+
 // Threads a launch uses for n queries: one query per thread, capped at `max_threads` (the
 // launch then strides over the remaining queries).
 uint64_t launch_threads(uint64_t n, uint64_t max_threads);
@@ -78,6 +195,12 @@ hipError_t launch_raycast_first(const RaycastFirstArgs& a, uint64_t threads, hip
 hipError_t launch_closest_point(const ClosestPointArgs& a, uint64_t threads, hipStream_t s);
 // fill == false: count hits per ray into a.counts; fill == true: write them at a.offsets.
 hipError_t launch_raycast_all(const RaycastAllArgs& a, bool fill, uint64_t threads, hipStream_t s);
+// Spatial order of n packed float triples (Morton bucket sort); workspace of
+// order_workspace_bytes(n) bytes, 256-byte aligned.
+constexpr uint64_t kOrderHeaderBytes = 256;
+uint32_t order_axis_bits(uint64_t n);
+uint64_t order_workspace_bytes(uint64_t n);
+hipError_t launch_order_points(const float* points, uint64_t n, uint32_t* order, void* workspace, hipStream_t s);
 hipError_t launch_apply_side_mask(float4* tris, uint64_t n_slots, const uint8_t* mask,
                                   hipStream_t s);
 

@@ -31,6 +31,33 @@ constexpr float kInf = __builtin_huge_valf();
 __shared__ uint2 sh_stack[kStackLds * kBlock];
 __shared__ uint2 sh_cand[kCand * kBlock];
 
+// Same as Stack, with the LDS part in a kernel-local array of kL entries per lane.
+template <int kL>
+struct StackL {
+  uint2* lds;
+  uint2* ovf;
+  uint64_t ovf_stride;
+  uint64_t gtid;
+  uint32_t lane;
+  int capacity;  // kL + overflow entries
+  int sp;
+
+  __device__ __forceinline__ bool push(uint32_t ref, uint32_t aux) {
+    if (sp >= capacity) return false;
+    if (sp < kL)
+      lds[sp * kBlock + lane] = make_uint2(ref, aux);
+    else
+      ovf[static_cast<uint64_t>(sp - kL) * ovf_stride + gtid] = make_uint2(ref, aux);
+    sp++;
+    return true;
+  }
+  __device__ __forceinline__ uint2 pop() {
+    sp--;
+    if (sp < kL) return lds[sp * kBlock + lane];
+    return ovf[static_cast<uint64_t>(sp - kL) * ovf_stride + gtid];
+  }
+};
+
 __device__ __forceinline__ float widen_lo(float t) { return t * (t > 0.0f ? 1.0f - kSlabEps : 1.0f + kSlabEps); }
 __device__ __forceinline__ float widen_hi(float t) { return t * (t > 0.0f ? 1.0f + kSlabEps : 1.0f - kSlabEps); }
 
@@ -267,137 +294,81 @@ __device__ __noinline__ uint32_t resolve_ray(Cands cl, float U, f3 o, f3 d, floa
 // child's hit point lies in the ray's own plane, so the shortcut fires iff the near subtree found
 // any hit and origin[axis] <= far.min[axis].
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_raycast_first(RaycastFirstArgs A) {
-  const uint32_t lane = threadIdx.x;
-  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + lane;
-  const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * kBlock;
-  const MeshView M = A.mesh;
+struct FirstHit {
+  uint32_t face;  // MBVH_NO_HIT, or kOverflow when the stack would overflow
+  float dist;
+  f3 p;
+  f3 bary;
+};
 
-  Stack st;
-  st.ovf = A.stack.ovf;
-  st.ovf_stride = A.stack.ovf_stride;
-  st.gtid = gtid;
-  st.lane = lane;
-  st.capacity = kStackLds + static_cast<int>(A.stack.ovf_entries);
+// One ray against one mesh; `st` is the lane's traversal stack (any LDS depth).
+template <typename StackT>
+__device__ __forceinline__ FirstHit trace_first(const MeshView& M, StackT& st, uint32_t lane, f3 o, f3 d,
+                                                uint32_t side_mask, float near_dist, float far_dist) {
+  RayBox rb;
+  rb.ox = o.x;
+  rb.oy = o.y;
+  rb.oz = o.z;
+  rb.ix = 1.0f / d.x;
+  rb.iy = 1.0f / d.y;
+  rb.iz = 1.0f / d.z;
+  rb.sx = rb.ix >= 0.0f;
+  rb.sy = rb.iy >= 0.0f;
+  rb.sz = rb.iz >= 0.0f;
+  const float dlen = sqrtf(dot(d, d));
+  const float dl1 = fabsf(d.x) + fabsf(d.y) + fabsf(d.z);
+  const bool literal = (d.x == 0.0f) || (d.y == 0.0f) || (d.z == 0.0f);
 
-  for (uint64_t ray = gtid; ray < A.n; ray += gstride) {
-    const f3 o = mk(A.origins[3 * ray + 0], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
-    const f3 d = mk(A.directions[3 * ray + 0], A.directions[3 * ray + 1], A.directions[3 * ray + 2]);
-    RayBox rb;
-    rb.ox = o.x;
-    rb.oy = o.y;
-    rb.oz = o.z;
-    rb.ix = 1.0f / d.x;
-    rb.iy = 1.0f / d.y;
-    rb.iz = 1.0f / d.z;
-    rb.sx = rb.ix >= 0.0f;
-    rb.sy = rb.iy >= 0.0f;
-    rb.sz = rb.iz >= 0.0f;
-    const float dlen = sqrtf(dot(d, d));
-    const float dl1 = fabsf(d.x) + fabsf(d.y) + fabsf(d.z);
-    const bool literal = (d.x == 0.0f) || (d.y == 0.0f) || (d.z == 0.0f);
+  float U = kInf;  // upper bound of the best hit's t
+  uint32_t hits = 0;
+  bool overflow = false;
+  Cands cl;
+  cl.lane = lane;
+  cl.n = 0;
 
-    float U = kInf;  // upper bound of the best hit's t
-    uint32_t hits = 0;
-    bool overflow = false;
-    Cands cl;
-    cl.lane = lane;
-    cl.n = 0;
-
-    for (uint32_t r = 0; r < M.n_roots && !overflow; r++) {
-      uint32_t ref = M.roots[r];
-      st.sp = 0;
-      while (true) {
-        // ---- internal nodes ----
-        while (!(ref & kRefLeaf) && ref != kDone) {
-          const WideNode nd = M.nodes[ref];
-          float tl, tr;
-          bool hl = slab(rb, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y, A.near_dist, A.far_dist, tl);
-          bool hr = slab(rb, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w, A.near_dist, A.far_dist, tr);
-          if (!literal) {
-            hl = hl && (tl <= U);
-            hr = hr && (tr <= U);
-          }
-          const uint32_t axis = nd.meta.z;
-          const float daxis = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
-          const bool l2r = daxis >= 0.0f;  // raycastFirst.template.js:46-48
-          const uint32_t c1 = l2r ? nd.meta.x : nd.meta.y;
-          const uint32_t c2 = l2r ? nd.meta.y : nd.meta.x;
-          const bool h1 = l2r ? hl : hr;
-          const bool h2 = l2r ? hr : hl;
-          if (h1 && h2) {
-            uint32_t aux;
-            if (!literal) {
-              aux = __float_as_uint(l2r ? tr : tl);
-            } else {
-              // daxis == +-0 means l2r, so the far child is the right one.
-              const float oaxis = axis == 0 ? o.x : (axis == 1 ? o.y : o.z);
-              const float fmin = axis == 0 ? nd.b1.z : (axis == 1 ? nd.b1.w : nd.b2.x);
-              const bool prunable = (daxis == 0.0f) && (oaxis <= fmin);
-              aux = (hits & ~kPruneFlag) | (prunable ? kPruneFlag : 0u);
-            }
-            if (!st.push(c2, aux)) {
-              overflow = true;
-              ref = kDone;
-              break;
-            }
-            ref = c1;
-          } else if (h1) {
-            ref = c1;
-          } else if (h2) {
-            ref = c2;
-          } else {
-            ref = kDone;
-            while (st.sp > 0) {
-              const uint2 e = st.pop();
-              if (!literal ? (__uint_as_float(e.y) <= U)
-                           : !((e.y & kPruneFlag) && hits > (e.y & ~kPruneFlag))) {
-                ref = e.x;
-                break;
-              }
-            }
-          }
+  for (uint32_t r = 0; r < M.n_roots && !overflow; r++) {
+    uint32_t ref = M.roots[r];
+    st.sp = 0;
+    while (true) {
+      // ---- internal nodes ----
+      while (!(ref & kRefLeaf) && ref != kDone) {
+        const WideNode nd = M.nodes[ref];
+        float tl, tr;
+        bool hl = slab(rb, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y, near_dist, far_dist, tl);
+        bool hr = slab(rb, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w, near_dist, far_dist, tr);
+        if (!literal) {
+          hl = hl && (tl <= U);
+          hr = hr && (tr <= U);
         }
-        // ---- leaves ----
-        while (ref & kRefLeaf) {
-          uint32_t off, cnt;
-          leaf_range(ref, M.bigleaf, off, cnt);
-          Tri next;
-          if (cnt != 0) next = load_tri(M.tris, off);  // an empty leaf may sit at the slot count
-          for (uint32_t s = off; s < off + cnt; s++) {
-            const Tri tr = next;
-            if (s + 1 < off + cnt) next = load_tri(M.tris, s + 1);
-            const uint32_t mask = A.side_mask ? A.side_mask : tr.mask();
-            float t = 0.0f, et = 0.0f;
-            int res = hit_fast(o, d, dl1, dlen, tr, mask, A.near_dist, A.far_dist, t, et);
-            if (res == kUnsure && literal) {
-              // the literal shortcut counts hits, so an undecided hit is decided now
-              ff te;
-              res = hit_exact(o, d, dlen, tr.a(), tr.b(), tr.c(), mask, A.near_dist, A.far_dist, te) ? kHit : kMiss;
-              t = ff_to_float(te);
-              et = 2.0f * kU * t;
-            }
-            if (res == kMiss) continue;
-            if (res == kHit) {
-              hits++;
-              const float lb = t - et;
-              if (lb > U) continue;  // certainly behind a known hit
-              U = fminf(U, t + et);
-            }
-            if (cl.n == kCand && !cl.compact(U)) {
-              // full of live candidates: settle them now and keep the winner
-              ff bt;
-              const uint32_t w = resolve_ray(cl, U, o, d, dlen, M.tris, A.side_mask, A.near_dist, A.far_dist, bt);
-              cl.n = 0;
-              if (w != kNoHit) {
-                const float tw = ff_to_float(bt);
-                U = fminf(U, tw * (1.0f + 4.0f * kU));
-                cl.put(0, w, tw * (1.0f - 4.0f * kU));
-                cl.n = 1;
-              }
-            }
-            cl.put(cl.n++, s, res == kHit ? t - et : -kInf);
+        const uint32_t axis = nd.meta.z;
+        const float daxis = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
+        const bool l2r = daxis >= 0.0f;  // raycastFirst.template.js:46-48
+        const uint32_t c1 = l2r ? nd.meta.x : nd.meta.y;
+        const uint32_t c2 = l2r ? nd.meta.y : nd.meta.x;
+        const bool h1 = l2r ? hl : hr;
+        const bool h2 = l2r ? hr : hl;
+        if (h1 && h2) {
+          uint32_t aux;
+          if (!literal) {
+            aux = __float_as_uint(l2r ? tr : tl);
+          } else {
+            // daxis == +-0 means l2r, so the far child is the right one.
+            const float oaxis = axis == 0 ? o.x : (axis == 1 ? o.y : o.z);
+            const float fmin = axis == 0 ? nd.b1.z : (axis == 1 ? nd.b1.w : nd.b2.x);
+            const bool prunable = (daxis == 0.0f) && (oaxis <= fmin);
+            aux = (hits & ~kPruneFlag) | (prunable ? kPruneFlag : 0u);
           }
+          if (!st.push(c2, aux)) {
+            overflow = true;
+            ref = kDone;
+            break;
+          }
+          ref = c1;
+        } else if (h1) {
+          ref = c1;
+        } else if (h2) {
+          ref = c2;
+        } else {
           ref = kDone;
           while (st.sp > 0) {
             const uint2 e = st.pop();
@@ -408,48 +379,257 @@ __global__ __launch_bounds__(kBlock) void k_raycast_first(RaycastFirstArgs A) {
             }
           }
         }
-        if (ref == kDone) break;
       }
+      // ---- leaves ----
+      while (ref & kRefLeaf) {
+        uint32_t off, cnt;
+        leaf_range(ref, M.bigleaf, off, cnt);
+        Tri next;
+        if (cnt != 0) next = load_tri(M.tris, off);  // an empty leaf may sit at the slot count
+        for (uint32_t s = off; s < off + cnt; s++) {
+          const Tri tr = next;
+          if (s + 1 < off + cnt) next = load_tri(M.tris, s + 1);
+          const uint32_t mask = side_mask ? side_mask : tr.mask();
+          float t = 0.0f, et = 0.0f;
+          int res = hit_fast(o, d, dl1, dlen, tr, mask, near_dist, far_dist, t, et);
+          if (res == kUnsure && literal) {
+            // the literal shortcut counts hits, so an undecided hit is decided now
+            ff te;
+            res = hit_exact(o, d, dlen, tr.a(), tr.b(), tr.c(), mask, near_dist, far_dist, te) ? kHit : kMiss;
+            t = ff_to_float(te);
+            et = 2.0f * kU * t;
+          }
+          if (res == kMiss) continue;
+          if (res == kHit) {
+            hits++;
+            const float lb = t - et;
+            if (lb > U) continue;  // certainly behind a known hit
+            U = fminf(U, t + et);
+          }
+          if (cl.n == kCand && !cl.compact(U)) {
+            // full of live candidates: settle them now and keep the winner
+            ff bt;
+            const uint32_t w = resolve_ray(cl, U, o, d, dlen, M.tris, side_mask, near_dist, far_dist, bt);
+            cl.n = 0;
+            if (w != kNoHit) {
+              const float tw = ff_to_float(bt);
+              U = fminf(U, tw * (1.0f + 4.0f * kU));
+              cl.put(0, w, tw * (1.0f - 4.0f * kU));
+              cl.n = 1;
+            }
+          }
+          cl.put(cl.n++, s, res == kHit ? t - et : -kInf);
+        }
+        ref = kDone;
+        while (st.sp > 0) {
+          const uint2 e = st.pop();
+          if (!literal ? (__uint_as_float(e.y) <= U)
+                       : !((e.y & kPruneFlag) && hits > (e.y & ~kPruneFlag))) {
+            ref = e.x;
+            break;
+          }
+        }
+      }
+      if (ref == kDone) break;
     }
+  }
 
-    uint32_t out_face = kNoHit;
-    float out_dist = kInf;
-    float px = __int_as_float(0x7fc00000), py = px, pz = px;
-    float bw = px, bu = px, bv = px;
-    if (overflow) {
-      out_face = kOverflow;
-    } else if (cl.n > 0) {
-      ff bt;
-      const uint32_t w = resolve_ray(cl, U, o, d, dlen, M.tris, A.side_mask, A.near_dist, A.far_dist, bt);
-      if (w != kNoHit) {
-        const Tri tr = load_tri(M.tris, w);
-        const TriTerms r = tri_terms_exact(o, d, tr.a(), tr.b(), tr.c());
-        const ff t = ff_div(r.qdn, r.ddn_abs);
-        const ff u = ff_div(r.e0, r.ddn_abs);
-        const ff v = ff_div(r.e1, r.ddn_abs);
-        const ff wgt = ff_sub(ff_sub(ff_from(1.0f), u), v);
-        const ff dl = ff_sqrt(ff_add(ff_add(two_prod(d.x, d.x), two_prod(d.y, d.y)), two_prod(d.z, d.z)));
-        out_face = tr.face();
-        out_dist = ff_to_float(ff_mul(t, dl));
-        px = ff_to_float(ff_add(ff_from(o.x), ff_mulf(t, d.x)));
-        py = ff_to_float(ff_add(ff_from(o.y), ff_mulf(t, d.y)));
-        pz = ff_to_float(ff_add(ff_from(o.z), ff_mulf(t, d.z)));
-        bw = ff_to_float(wgt);
-        bu = ff_to_float(u);
-        bv = ff_to_float(v);
-      }
+  FirstHit h;
+  h.face = kNoHit;
+  h.dist = kInf;
+  const float qnan = __int_as_float(0x7fc00000);
+  h.p = mk(qnan, qnan, qnan);
+  h.bary = h.p;
+  if (overflow) {
+    h.face = kOverflow;
+  } else if (cl.n > 0) {
+    ff bt;
+    const uint32_t w = resolve_ray(cl, U, o, d, dlen, M.tris, side_mask, near_dist, far_dist, bt);
+    if (w != kNoHit) {
+      const Tri tr = load_tri(M.tris, w);
+      const TriTerms r = tri_terms_exact(o, d, tr.a(), tr.b(), tr.c());
+      const ff t = ff_div(r.qdn, r.ddn_abs);
+      const ff u = ff_div(r.e0, r.ddn_abs);
+      const ff v = ff_div(r.e1, r.ddn_abs);
+      const ff wgt = ff_sub(ff_sub(ff_from(1.0f), u), v);
+      const ff dl = ff_sqrt(ff_add(ff_add(two_prod(d.x, d.x), two_prod(d.y, d.y)), two_prod(d.z, d.z)));
+      h.face = tr.face();
+      h.dist = ff_to_float(ff_mul(t, dl));
+      h.p = mk(ff_to_float(ff_add(ff_from(o.x), ff_mulf(t, d.x))), ff_to_float(ff_add(ff_from(o.y), ff_mulf(t, d.y))),
+               ff_to_float(ff_add(ff_from(o.z), ff_mulf(t, d.z))));
+      h.bary = mk(ff_to_float(wgt), ff_to_float(u), ff_to_float(v));
     }
-    A.out_face[ray] = out_face;
-    if (A.out_dist) A.out_dist[ray] = out_dist;
+  }
+  return h;
+}
+
+__global__ __launch_bounds__(kBlock) void k_raycast_first(RaycastFirstArgs A) {
+  const uint32_t lane = threadIdx.x;
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + lane;
+  const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * kBlock;
+
+  Stack st;
+  st.ovf = A.stack.ovf;
+  st.ovf_stride = A.stack.ovf_stride;
+  st.gtid = gtid;
+  st.lane = lane;
+  st.capacity = kStackLds + static_cast<int>(A.stack.ovf_entries);
+
+  for (uint64_t visit = gtid; visit < A.n; visit += gstride) {
+    const uint64_t ray = A.order ? A.order[visit] : visit;
+    if (ray >= A.n) continue;  // not a permutation of 0..n-1: skip rather than read out of bounds
+    const f3 o = mk(A.origins[3 * ray + 0], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
+    const f3 d = mk(A.directions[3 * ray + 0], A.directions[3 * ray + 1], A.directions[3 * ray + 2]);
+    const FirstHit h = trace_first(A.mesh, st, lane, o, d, A.side_mask, A.near_dist, A.far_dist);
+    A.out_face[ray] = h.face;
+    if (A.out_dist) A.out_dist[ray] = h.dist;
     if (A.out_point) {
-      A.out_point[3 * ray + 0] = px;
-      A.out_point[3 * ray + 1] = py;
-      A.out_point[3 * ray + 2] = pz;
+      A.out_point[3 * ray + 0] = h.p.x;
+      A.out_point[3 * ray + 1] = h.p.y;
+      A.out_point[3 * ray + 2] = h.p.z;
     }
     if (A.out_bary) {
-      A.out_bary[3 * ray + 0] = bw;
-      A.out_bary[3 * ray + 1] = bu;
-      A.out_bary[3 * ray + 2] = bv;
+      A.out_bary[3 * ray + 0] = h.bary.x;
+      A.out_bary[3 * ray + 1] = h.bary.y;
+      A.out_bary[3 * ray + 2] = h.bary.z;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Instances (BatchedMesh). MeshBVH.raycastObject3D (src/core/MeshBVH.js:447-494): the world ray
+// is brought into the instance's frame with three's Ray.applyMatrix4(matrixWorld^-1) (origin by
+// Vector3.applyMatrix4, direction by Vector3.transformDirection, i.e. normalised), near/far are
+// divided by |local direction * world scale|, and the hit point goes back with
+// Vector3.applyMatrix4(matrixWorld), its distance measured from the world origin
+// (convertRaycastIntersect, src/utils/GeometryRayIntersectUtilities.js:3-17). Matrices are
+// column-major 4x4 (three's Matrix4.elements). The arithmetic is fixed (no contraction) so that
+// every kernel computing an instance ray gets the same bits.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ f3 apply_matrix4(const float* e, f3 v) {
+  MBVH_NO_CONTRACT
+  const float w = 1.0f / (e[3] * v.x + e[7] * v.y + e[11] * v.z + e[15]);
+  return mk((e[0] * v.x + e[4] * v.y + e[8] * v.z + e[12]) * w, (e[1] * v.x + e[5] * v.y + e[9] * v.z + e[13]) * w,
+            (e[2] * v.x + e[6] * v.y + e[10] * v.z + e[14]) * w);
+}
+__device__ __forceinline__ f3 transform_direction(const float* e, f3 v) {
+  MBVH_NO_CONTRACT
+  const f3 r = mk(e[0] * v.x + e[4] * v.y + e[8] * v.z, e[1] * v.x + e[5] * v.y + e[9] * v.z,
+                  e[2] * v.x + e[6] * v.y + e[10] * v.z);
+  const float len = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
+  const float inv = 1.0f / (len != 0.0f ? len : 1.0f);
+  return mk(r.x * inv, r.y * inv, r.z * inv);
+}
+// |local direction * column lengths of matrixWorld| (raycastObject3D's scaleFactor)
+__device__ __forceinline__ float instance_scale(const float* e, f3 dl) {
+  MBVH_NO_CONTRACT
+  const float sx = sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+  const float sy = sqrtf(e[4] * e[4] + e[5] * e[5] + e[6] * e[6]);
+  const float sz = sqrtf(e[8] * e[8] + e[9] * e[9] + e[10] * e[10]);
+  const f3 v = mk(dl.x * sx, dl.y * sy, dl.z * sz);
+  return sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+}
+__device__ __forceinline__ float world_distance(f3 pw, f3 o) {
+  MBVH_NO_CONTRACT
+  const f3 dv = sub(pw, o);
+  return sqrtf(dv.x * dv.x + dv.y * dv.y + dv.z * dv.z);
+}
+
+__global__ __launch_bounds__(kBlock) void k_instance_rays(InstanceRayArgs A) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= A.n) return;
+  const uint32_t inst = A.ray_instance[i];
+  const float qnan = __int_as_float(0x7fc00000);
+  f3 o = mk(qnan, qnan, qnan), d = o;
+  if (inst < A.n_instances) {
+    const float* inv = A.inverse + 16ull * inst;
+    o = apply_matrix4(inv, mk(A.origins[3 * i], A.origins[3 * i + 1], A.origins[3 * i + 2]));
+    d = transform_direction(inv, mk(A.directions[3 * i], A.directions[3 * i + 1], A.directions[3 * i + 2]));
+  }
+  A.out_origins[3 * i] = o.x;
+  A.out_origins[3 * i + 1] = o.y;
+  A.out_origins[3 * i + 2] = o.z;
+  A.out_directions[3 * i] = d.x;
+  A.out_directions[3 * i + 1] = d.y;
+  A.out_directions[3 * i + 2] = d.z;
+}
+
+__global__ __launch_bounds__(kBlock) void k_instance_hits(InstanceHitArgs A) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= A.n) return;
+  const uint32_t inst = A.ray_instance[i];
+  if (A.face[i] >= kOverflow || inst >= A.n_instances) return;
+  const f3 pw = apply_matrix4(A.world + 16ull * inst, mk(A.point[3 * i], A.point[3 * i + 1], A.point[3 * i + 2]));
+  A.point[3 * i] = pw.x;
+  A.point[3 * i + 1] = pw.y;
+  A.point[3 * i + 2] = pw.z;
+  A.distance[i] = world_distance(pw, mk(A.origins[3 * i], A.origins[3 * i + 1], A.origins[3 * i + 2]));
+}
+
+__global__ void k_batch_reset(uint32_t* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
+}
+
+// Batched first hit over many meshes: a persistent grid pulls tiles of `tile` rays (in the given
+// visit order, grouped by mesh) from an atomic counter until none are left; each lane brings its
+// ray into its instance's frame, traces it against that instance's mesh and converts the hit to
+// world space.
+template <int kL>
+__global__ __launch_bounds__(kBlock) void k_batch_raycast_first(BatchArgs A) {
+  __shared__ uint2 lds_stack[kL * kBlock];
+  const uint32_t tid = threadIdx.x, lane64 = tid & 63u;
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + tid;
+  StackL<kL> st;
+  st.lds = lds_stack;
+  st.ovf = A.stack.ovf;
+  st.ovf_stride = A.stack.ovf_stride;
+  st.gtid = gtid;
+  st.lane = tid;
+  st.capacity = kL + static_cast<int>(A.stack.ovf_entries);
+  const float qnan = __int_as_float(0x7fc00000);
+  while (true) {
+    uint32_t t = 0;
+    if (lane64 == 0) t = atomicAdd(A.counter, 1u);
+    t = __shfl(t, 0, 64);
+    if (t >= A.n_tiles) break;  // every wave leaves once the counter passes the last tile
+    const uint64_t base = static_cast<uint64_t>(t) * A.tile;
+    for (uint32_t k = lane64; k < A.tile; k += 64) {
+      const uint64_t visit = base + k;
+      if (visit >= A.n) break;
+      const uint64_t ray = A.order ? A.order[visit] : visit;
+      if (ray >= A.n) continue;
+      const uint32_t inst = A.ray_instance[ray];
+      const uint32_t geom = inst < A.n_instances ? A.instance_geometry[inst] : 0xFFFFFFFFu;
+      FirstHit h;
+      h.face = kNoHit;
+      h.dist = kInf;
+      h.p = mk(qnan, qnan, qnan);
+      h.bary = h.p;
+      const f3 ow = mk(A.origins[3 * ray], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
+      if (geom < A.n_meshes) {
+        const float* inv = A.inverse + 16ull * inst;
+        const float* wld = A.world + 16ull * inst;
+        const f3 o = apply_matrix4(inv, ow);
+        const f3 d = transform_direction(inv, mk(A.directions[3 * ray], A.directions[3 * ray + 1], A.directions[3 * ray + 2]));
+        const float sc = instance_scale(wld, d);
+        h = trace_first(A.meshes[geom], st, tid, o, d, A.side_mask, A.near_dist / sc, A.far_dist / sc);
+        if (h.face < kOverflow) {
+          h.p = apply_matrix4(wld, h.p);
+          h.dist = world_distance(h.p, ow);
+        }
+      }
+      A.out_face[ray] = h.face;
+      if (A.out_dist) A.out_dist[ray] = h.dist;
+      if (A.out_point) {
+        A.out_point[3 * ray + 0] = h.p.x;
+        A.out_point[3 * ray + 1] = h.p.y;
+        A.out_point[3 * ray + 2] = h.p.z;
+      }
+      if (A.out_bary) {
+        A.out_bary[3 * ray + 0] = h.bary.x;
+        A.out_bary[3 * ray + 1] = h.bary.y;
+        A.out_bary[3 * ray + 2] = h.bary.z;
+      }
     }
   }
 }
@@ -570,34 +750,46 @@ __global__ __launch_bounds__(kBlock) void k_raycast_all(RaycastAllArgs A) {
 // three's Triangle.closestPointToPoint), Voronoi regions tested in the same order.
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ f3 closest_on_triangle(f3 p, f3 a, f3 b, f3 c) {
-  const f3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+  // Branch-free: all six dot products, then the region by the same ordered tests (first match
+  // wins), then one division for the region's parameter. Lanes of a wave sit in different
+  // regions, so a branchy version would run every region's code one after another.
+  const f3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a), bp = sub(p, b), cp = sub(p, c);
   const float d1 = dot(ab, ap), d2 = dot(ac, ap);
-  if (d1 <= 0.0f && d2 <= 0.0f) return a;
-  const f3 bp = sub(p, b);
   const float d3 = dot(ab, bp), d4 = dot(ac, bp);
-  if (d3 >= 0.0f && d4 <= d3) return b;
-  const float vc = d1 * d4 - d3 * d2;
-  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
-    const float v = d1 / (d1 - d3);
-    return mk(a.x + ab.x * v, a.y + ab.y * v, a.z + ab.z * v);
-  }
-  const f3 cp = sub(p, c);
   const float d5 = dot(ab, cp), d6 = dot(ac, cp);
-  if (d6 >= 0.0f && d5 <= d6) return c;
+  const float vc = d1 * d4 - d3 * d2;
   const float vb = d5 * d2 - d1 * d6;
-  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
-    const float w = d2 / (d2 - d6);
-    return mk(a.x + ac.x * w, a.y + ac.y * w, a.z + ac.z * w);
-  }
   const float va = d3 * d6 - d5 * d4;
-  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
-    const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
-    const f3 bc = sub(c, b);
-    return mk(b.x + bc.x * w, b.y + bc.y * w, b.z + bc.z * w);
-  }
-  const float denom = 1.0f / (va + vb + vc);
-  const float v = vb * denom, w = vc * denom;
-  return mk(a.x + ab.x * v + ac.x * w, a.y + ab.y * v + ac.y * w, a.z + ab.z * v + ac.z * w);
+  const float d43 = d4 - d3, d56 = d5 - d6;
+  const bool rA = d1 <= 0.0f && d2 <= 0.0f;
+  const bool rB = !rA && d3 >= 0.0f && d4 <= d3;
+  const bool rAB = !rA && !rB && vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f;
+  const bool rC = !rA && !rB && !rAB && d6 >= 0.0f && d5 <= d6;
+  const bool rAC = !rA && !rB && !rAB && !rC && vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f;
+  const bool rBC = !rA && !rB && !rAB && !rC && !rAC && va <= 0.0f && d43 >= 0.0f && d56 >= 0.0f;
+  const bool rF = !rA && !rB && !rAB && !rC && !rAC && !rBC;
+  // parameter: AB d1/(d1-d3), AC d2/(d2-d6), BC d43/(d43+d56), face 1/(va+vb+vc); every
+  // candidate is computed, then selected (selects, not branches)
+  const float den_ab = d1 - d3, den_ac = d2 - d6, den_bc = d43 + d56, den_f = va + vb + vc;
+  float num = 1.0f, den = 1.0f;
+  num = rAB ? d1 : num;
+  num = rAC ? d2 : num;
+  num = rBC ? d43 : num;
+  den = rF ? den_f : den;
+  den = rAB ? den_ab : den;
+  den = rAC ? den_ac : den;
+  den = rBC ? den_bc : den;
+  const float q = num / den;
+  // point = base + dir1 * s1 + dir2 * s2 (unused terms are exact zeros)
+  const f3 base = (rB || rBC) ? b : (rC ? c : a);
+  const f3 bc = sub(c, b);
+  const f3 z = mk(0.0f, 0.0f, 0.0f);
+  const f3 dir1 = (rAB || rF) ? ab : (rAC ? ac : (rBC ? bc : z));
+  const float s1 = (rAB || rAC || rBC) ? q : (rF ? vb * q : 0.0f);
+  const f3 dir2 = rF ? ac : z;
+  const float s2 = rF ? vc * q : 0.0f;
+  return mk(base.x + dir1.x * s1 + dir2.x * s2, base.y + dir1.y * s1 + dir2.y * s2,
+            base.z + dir1.z * s1 + dir2.z * s2);
 }
 
 // Same regions in double-float; returns the point, its barycoord (weights of a, b, c) and the
@@ -747,7 +939,9 @@ __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
   const float max2 = A.max_threshold * A.max_threshold;
   constexpr float kScoreSlack = 1.0f - 4.0f * kU;
 
-  for (uint64_t qi = gtid; qi < A.n; qi += gstride) {
+  for (uint64_t visit = gtid; visit < A.n; visit += gstride) {
+    const uint64_t qi = A.order ? A.order[visit] : visit;
+    if (qi >= A.n) continue;  // not a permutation of 0..n-1: skip rather than read out of bounds
     const f3 q = mk(A.points[3 * qi + 0], A.points[3 * qi + 1], A.points[3 * qi + 2]);
     // Absolute error bound of an FP32 point-to-triangle distance for this query.
     const float e_abs = 8.0f * kU * (maxabs(q) + M.coord_max);
@@ -915,6 +1109,28 @@ uint64_t launch_threads(uint64_t n, uint64_t max_threads) {
 hipError_t launch_raycast_first(const RaycastFirstArgs& a, uint64_t threads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_raycast_first, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_instance_rays(const InstanceRayArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_instance_rays, dim3(static_cast<uint32_t>((a.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_instance_hits(const InstanceHitArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_instance_hits, dim3(static_cast<uint32_t>((a.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_raycast_first(const BatchArgs& a, uint32_t stack_lds, uint32_t blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.counter);
+  if (a.n == 0) return hipGetLastError();
+  if (stack_lds == 8)
+    hipLaunchKernelGGL(k_batch_raycast_first<8>, dim3(blocks), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_batch_raycast_first<16>, dim3(blocks), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 

@@ -479,6 +479,81 @@ static napi_value js_mesh_info(napi_env env, napi_callback_info info) {
   return out;
 }
 
+/* raycastFirstBackward(mesh, dirs, face, dist, bary, gDist|null, gPoint|null, gBary|null,
+ *                      outGradOrigins|null, outGradDirs|null, outGradPositions|null, mode) */
+static napi_value js_raycast_first_bwd(napi_env env, napi_callback_info info) {
+  napi_value argv[12];
+  get_args(env, info, 12, argv);
+  MeshHandle* h = get_mesh(env, argv[0]);
+  if (!h) return throw_msg(env, "raycastFirstBackward: invalid mesh");
+  View d;
+  if (!get_view(env, argv[1], &d) || d.type != napi_float32_array || d.length % 3)
+    return throw_msg(env, "raycastFirstBackward: directions must be a Float32Array of length 3n");
+  const size_t n = d.length / 3;
+  uint64_t t = 0, v = 0;
+  uint32_t depth = 0;
+  int rc = mbvh_mesh_info(h->mesh, &t, &v, &depth);
+  if (rc) return throw_status(env, rc, "mbvh_mesh_info");
+  void *face, *dist, *bary, *gd, *gp, *gb, *oo, *od, *ov;
+  if (!out_view(env, argv[2], napi_uint32_array, n, &face) || !face ||
+      !out_view(env, argv[3], napi_float32_array, n, &dist) || !dist ||
+      !out_view(env, argv[4], napi_float32_array, 3 * n, &bary) || !bary)
+    return throw_msg(env, "raycastFirstBackward: saved faceIndex / distance / barycoord have the wrong type or length");
+  if (!out_view(env, argv[5], napi_float32_array, n, &gd) || !out_view(env, argv[6], napi_float32_array, 3 * n, &gp) ||
+      !out_view(env, argv[7], napi_float32_array, 3 * n, &gb))
+    return throw_msg(env, "raycastFirstBackward: upstream gradients have the wrong type or length");
+  if (!out_view(env, argv[8], napi_float32_array, 3 * n, &oo) || !out_view(env, argv[9], napi_float32_array, 3 * n, &od) ||
+      !out_view(env, argv[10], napi_float32_array, 3 * (size_t)v, &ov))
+    return throw_msg(env, "raycastFirstBackward: output gradients have the wrong type or length");
+  int32_t mode;
+  if (!get_i32(env, argv[11], &mode)) return throw_msg(env, "raycastFirstBackward: mode must be a number");
+  mbvh_ctx* ctx = NULL;
+  rc = context_for(env, h->device, &ctx);
+  if (!rc)
+    rc = mbvh_raycast_first_bwd(ctx, h->mesh, (const float*)d.data, n, (const uint32_t*)face, (const float*)dist,
+                                (const float*)bary, (const float*)gd, (const float*)gp, (const float*)gb, (float*)oo,
+                                (float*)od, (float*)ov, mode);
+  if (rc) return throw_status(env, rc, "mbvh_raycast_first_bwd");
+  return NULL;
+}
+
+/* closestPointBackward(mesh, points, face, dist, closest, bary, gDist, outGradPoints|null,
+ *                      outGradPositions|null, mode) */
+static napi_value js_closest_point_bwd(napi_env env, napi_callback_info info) {
+  napi_value argv[10];
+  get_args(env, info, 10, argv);
+  MeshHandle* h = get_mesh(env, argv[0]);
+  if (!h) return throw_msg(env, "closestPointBackward: invalid mesh");
+  View q;
+  if (!get_view(env, argv[1], &q) || q.type != napi_float32_array || q.length % 3)
+    return throw_msg(env, "closestPointBackward: points must be a Float32Array of length 3n");
+  const size_t n = q.length / 3;
+  uint64_t t = 0, v = 0;
+  uint32_t depth = 0;
+  int rc = mbvh_mesh_info(h->mesh, &t, &v, &depth);
+  if (rc) return throw_status(env, rc, "mbvh_mesh_info");
+  void *face, *dist, *cp, *bary, *gd, *oq, *ov;
+  if (!out_view(env, argv[2], napi_uint32_array, n, &face) || !face ||
+      !out_view(env, argv[3], napi_float32_array, n, &dist) || !dist ||
+      !out_view(env, argv[4], napi_float32_array, 3 * n, &cp) || !cp ||
+      !out_view(env, argv[5], napi_float32_array, 3 * n, &bary) || !bary)
+    return throw_msg(env, "closestPointBackward: saved faceIndex / distance / point / barycoord have the wrong type or length");
+  if (!out_view(env, argv[6], napi_float32_array, n, &gd) || !gd)
+    return throw_msg(env, "closestPointBackward: the upstream distance gradient must be a Float32Array of length n");
+  if (!out_view(env, argv[7], napi_float32_array, 3 * n, &oq) ||
+      !out_view(env, argv[8], napi_float32_array, 3 * (size_t)v, &ov))
+    return throw_msg(env, "closestPointBackward: output gradients have the wrong type or length");
+  int32_t mode;
+  if (!get_i32(env, argv[9], &mode)) return throw_msg(env, "closestPointBackward: mode must be a number");
+  mbvh_ctx* ctx = NULL;
+  rc = context_for(env, h->device, &ctx);
+  if (!rc)
+    rc = mbvh_closest_point_bwd(ctx, h->mesh, (const float*)q.data, n, (const uint32_t*)face, (const float*)dist,
+                                (const float*)cp, (const float*)bary, (const float*)gd, (float*)oq, (float*)ov, mode);
+  if (rc) return throw_status(env, rc, "mbvh_closest_point_bwd");
+  return NULL;
+}
+
 static napi_value js_abi_version(napi_env env, napi_callback_info info) {
   (void)info;
   napi_value out;
@@ -504,6 +579,8 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("raycastFirst", js_raycast_first);
   EXPORT("closestPoint", js_closest_point);
   EXPORT("raycast", js_raycast);
+  EXPORT("raycastFirstBackward", js_raycast_first_bwd);
+  EXPORT("closestPointBackward", js_closest_point_bwd);
   napi_add_env_cleanup_hook(env, cleanup_contexts, NULL);
   return exports;
 }

@@ -3,6 +3,9 @@
 The JavaScript package is the drop-in host API; this module gives Python callers (the benchmark,
 the tests) the same entry points over plain numpy arrays and raw device pointers. It never falls
 back to a CPU path: if the library or a GPU is missing, calls raise MbvhError.
+
+A process that also uses torch must import torch first: torch bundles its own HIP runtime under the
+same soname as /opt/rocm's, so whichever loads first serves both, and torch cannot run on the other.
 """
 from __future__ import annotations
 
@@ -15,6 +18,7 @@ import numpy as np
 
 CENTER, AVERAGE, SAH = 0, 1, 2
 FRONT_SIDE, BACK_SIDE, DOUBLE_SIDE, SIDE_PER_TRIANGLE = 0, 1, 2, 3
+GRAD_ATOMIC, GRAD_DETERMINISTIC = 0, 1
 NO_HIT = 0xFFFFFFFF
 
 _PKG_ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
@@ -29,6 +33,12 @@ EXPORTED_SYMBOLS = (
     "mbvh_raycast_first", "mbvh_raycast_first_d",
     "mbvh_closest_point", "mbvh_closest_point_d",
     "mbvh_raycast", "mbvh_raycast_fetch", "mbvh_raycast_count_d", "mbvh_raycast_fill_d",
+    "mbvh_order_workspace_bytes", "mbvh_order_points_d",
+    "mbvh_batch_create", "mbvh_batch_free", "mbvh_batch_workspace_bytes", "mbvh_batch_raycast_first_d",
+    "mbvh_batch_autotune", "mbvh_batch_graph_create", "mbvh_batch_graph_launch", "mbvh_batch_graph_free",
+    "mbvh_instance_rays_d", "mbvh_instance_hits_d",
+    "mbvh_grad_workspace_bytes", "mbvh_raycast_first_bwd", "mbvh_raycast_first_bwd_d",
+    "mbvh_closest_point_bwd", "mbvh_closest_point_bwd_d",
 )
 
 
@@ -44,6 +54,15 @@ class BuildOpts(C.Structure):
 
 
 _lib = None
+
+
+class BatchTuning(C.Structure):
+    """mbvh_batch_tuning: LDS stack entries (8|16), rays per work item, workgroups per CU, measured ms."""
+    _fields_ = [("stack_lds", C.c_uint32), ("tile_rays", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+                ("ms", C.c_float)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 def library() -> C.CDLL:
@@ -74,9 +93,30 @@ def library() -> C.CDLL:
         "mbvh_mesh_info": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(u32)]),
         "mbvh_mesh_set_side_mask": (C.c_int, [vp, vp, vp, u64]),
         "mbvh_raycast_first": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp]),
-        "mbvh_raycast_first_d": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp]),
+        "mbvh_raycast_first_d": (C.c_int, [vp, vp, vp, vp, u64, vp, i32, f32, f32, vp, vp, vp, vp, vp]),
         "mbvh_closest_point": (C.c_int, [vp, vp, vp, u64, f32, f32, vp, vp, vp, vp]),
-        "mbvh_closest_point_d": (C.c_int, [vp, vp, vp, u64, f32, f32, vp, vp, vp, vp, vp]),
+        "mbvh_closest_point_d": (C.c_int, [vp, vp, vp, u64, vp, f32, f32, vp, vp, vp, vp, vp]),
+        "mbvh_order_workspace_bytes": (C.c_int, [u64, C.POINTER(u64)]),
+        "mbvh_order_points_d": (C.c_int, [vp, vp, u64, vp, vp, u64, vp]),
+        "mbvh_batch_create": (C.c_int, [vp, C.POINTER(vp), u32, C.POINTER(vp)]),
+        "mbvh_batch_free": (None, [vp]),
+        "mbvh_batch_workspace_bytes": (C.c_int, [vp, u64, C.POINTER(u64)]),
+        "mbvh_batch_raycast_first_d": (C.c_int, [vp, vp, vp, vp, vp, u32, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp,
+                                                 vp, vp, u64, C.POINTER(BatchTuning), vp]),
+        "mbvh_batch_autotune": (C.c_int, [vp, vp, vp, vp, vp, u32, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp,
+                                          vp, u64, C.POINTER(BatchTuning)]),
+        "mbvh_batch_graph_create": (C.c_int, [vp, vp, vp, vp, vp, u32, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp,
+                                              vp, vp, u64, C.POINTER(BatchTuning), C.POINTER(vp)]),
+        "mbvh_batch_graph_launch": (C.c_int, [vp, vp]),
+        "mbvh_batch_graph_free": (None, [vp]),
+        "mbvh_instance_rays_d": (C.c_int, [vp, vp, u32, vp, vp, vp, u64, vp, vp, vp]),
+        "mbvh_instance_hits_d": (C.c_int, [vp, vp, u32, vp, vp, u64, vp, vp, vp, vp]),
+        "mbvh_grad_workspace_bytes": (C.c_int, [u64, i32, C.POINTER(u64)]),
+        "mbvh_raycast_first_bwd": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32]),
+        "mbvh_raycast_first_bwd_d": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, u64,
+                                               vp]),
+        "mbvh_closest_point_bwd": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp, i32]),
+        "mbvh_closest_point_bwd_d": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp, i32, vp, u64, vp]),
         "mbvh_raycast": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp]),
         "mbvh_raycast_fetch": (C.c_int, [vp, vp, vp, vp, vp, u64]),
         "mbvh_raycast_count_d": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp]),
@@ -181,6 +221,33 @@ class Context:
                stride: int = 3, offset: int = 0) -> "Mesh":
         return Mesh(self, tree, positions, index, stride, offset)
 
+    def batch(self, meshes: Sequence["Mesh"]) -> "Batch":
+        return Batch(self, meshes)
+
+    def instance_rays_d(self, inverse_ptr: int, n_instances: int, origins_ptr: int, dirs_ptr: int, ray_inst_ptr: int,
+                        n: int, out_origins_ptr: int, out_dirs_ptr: int, stream: int = 0) -> None:
+        v = lambda x: C.c_void_p(x or None)
+        _check(self._lib.mbvh_instance_rays_d(self.handle, v(inverse_ptr), n_instances, v(origins_ptr), v(dirs_ptr),
+                                              v(ray_inst_ptr), n, v(out_origins_ptr), v(out_dirs_ptr), v(stream)))
+
+    def instance_hits_d(self, world_ptr: int, n_instances: int, origins_ptr: int, ray_inst_ptr: int, n: int,
+                        face_ptr: int, point_ptr: int, dist_ptr: int, stream: int = 0) -> None:
+        v = lambda x: C.c_void_p(x or None)
+        _check(self._lib.mbvh_instance_hits_d(self.handle, v(world_ptr), n_instances, v(origins_ptr), v(ray_inst_ptr),
+                                              n, v(face_ptr), v(point_ptr), v(dist_ptr), v(stream)))
+
+    @staticmethod
+    def order_workspace_bytes(n: int) -> int:
+        out = C.c_uint64()
+        _check(library().mbvh_order_workspace_bytes(n, C.byref(out)))
+        return out.value
+
+    def order_points_d(self, points_ptr: int, n: int, order_ptr: int, workspace_ptr: int, workspace_bytes: int,
+                       stream: int = 0) -> None:
+        """Spatial (Morton bucket) visit order of n device float triples into order_ptr (n uint32)."""
+        _check(self._lib.mbvh_order_points_d(self.handle, C.c_void_p(points_ptr), n, C.c_void_p(order_ptr),
+                                             C.c_void_p(workspace_ptr), workspace_bytes, C.c_void_p(stream or None)))
+
 
 class Mesh:
     """Device-resident BVH (mbvh_mesh)."""
@@ -201,6 +268,10 @@ class Mesh:
             _ptr(ind), 0 if ind is None else ind.itemsize, 0 if ind is None else ind.size,
             _ptr(pos), pos.size // stride, stride, offset, C.byref(h)))
         self.handle = h
+
+    @property
+    def vertex_count(self) -> int:
+        return self.info()["vertex_count"]
 
     def info(self):
         t, v, d = C.c_uint64(), C.c_uint64(), C.c_uint32()
@@ -270,9 +341,10 @@ class Mesh:
     # ---- device-pointer queries (asynchronous on `stream`) ----
     def raycast_first_d(self, origins_ptr: int, dirs_ptr: int, n: int, face_ptr: int, dist_ptr: int = 0,
                         point_ptr: int = 0, bary_ptr: int = 0, side: int = FRONT_SIDE, near: float = 0.0,
-                        far: float = float("inf"), stream: int = 0) -> None:
+                        far: float = float("inf"), stream: int = 0, order_ptr: int = 0) -> None:
         _check(self.ctx._lib.mbvh_raycast_first_d(
-            self.ctx.handle, self.handle, C.c_void_p(origins_ptr), C.c_void_p(dirs_ptr), n, side, near, far,
+            self.ctx.handle, self.handle, C.c_void_p(origins_ptr), C.c_void_p(dirs_ptr), n,
+            C.c_void_p(order_ptr or None), side, near, far,
             C.c_void_p(face_ptr), C.c_void_p(dist_ptr or None), C.c_void_p(point_ptr or None),
             C.c_void_p(bary_ptr or None), C.c_void_p(stream or None)))
 
@@ -292,8 +364,142 @@ class Mesh:
 
     def closest_point_d(self, points_ptr: int, n: int, face_ptr: int, dist_ptr: int = 0, point_ptr: int = 0,
                         bary_ptr: int = 0, min_threshold: float = 0.0, max_threshold: float = float("inf"),
-                        stream: int = 0) -> None:
+                        stream: int = 0, order_ptr: int = 0) -> None:
         _check(self.ctx._lib.mbvh_closest_point_d(
-            self.ctx.handle, self.handle, C.c_void_p(points_ptr), n, min_threshold, max_threshold,
+            self.ctx.handle, self.handle, C.c_void_p(points_ptr), n, C.c_void_p(order_ptr or None),
+            min_threshold, max_threshold,
             C.c_void_p(face_ptr), C.c_void_p(dist_ptr or None), C.c_void_p(point_ptr or None),
             C.c_void_p(bary_ptr or None), C.c_void_p(stream or None)))
+
+    # ---- backward (host arrays) ----
+    def raycast_first_bwd(self, directions: np.ndarray, fwd: dict, grad_distance: Optional[np.ndarray] = None,
+                          grad_point: Optional[np.ndarray] = None, grad_barycoord: Optional[np.ndarray] = None,
+                          mode: int = GRAD_DETERMINISTIC, vertices: bool = True) -> dict:
+        """Gradients of a loss w.r.t. ray origins, directions and vertex positions, for the fixed hits in
+        `fwd` (the dict raycast_first returned) and the upstream gradients of its outputs."""
+        d = np.ascontiguousarray(directions, np.float32)
+        n = d.shape[0]
+        g_o = np.empty((n, 3), np.float32)
+        g_d = np.empty((n, 3), np.float32)
+        g_v = np.empty((self.vertex_count, 3), np.float32) if vertices else None
+        f32 = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+        _check(self.ctx._lib.mbvh_raycast_first_bwd(
+            self.ctx.handle, self.handle, _ptr(d), n, _ptr(np.ascontiguousarray(fwd["face"], np.uint32)),
+            _ptr(f32(fwd["distance"])), _ptr(f32(fwd["barycoord"])), _ptr(f32(grad_distance)),
+            _ptr(f32(grad_point)), _ptr(f32(grad_barycoord)), _ptr(g_o), _ptr(g_d), _ptr(g_v), mode))
+        return {"origins": g_o, "directions": g_d, "positions": g_v}
+
+    def closest_point_bwd(self, points: np.ndarray, fwd: dict, grad_distance: np.ndarray,
+                          mode: int = GRAD_DETERMINISTIC, vertices: bool = True) -> dict:
+        """Gradients of a loss w.r.t. query points and vertex positions for the fixed closest points in
+        `fwd` (the dict closest_point returned) and the upstream gradient of the distances."""
+        q = np.ascontiguousarray(points, np.float32)
+        n = q.shape[0]
+        g_q = np.empty((n, 3), np.float32)
+        g_v = np.empty((self.vertex_count, 3), np.float32) if vertices else None
+        f32 = lambda a: np.ascontiguousarray(a, np.float32)
+        _check(self.ctx._lib.mbvh_closest_point_bwd(
+            self.ctx.handle, self.handle, _ptr(q), n, _ptr(np.ascontiguousarray(fwd["face"], np.uint32)),
+            _ptr(f32(fwd["distance"])), _ptr(f32(fwd["point"])), _ptr(f32(fwd["barycoord"])),
+            _ptr(f32(grad_distance)), _ptr(g_q), _ptr(g_v), mode))
+        return {"points": g_q, "positions": g_v}
+
+    # ---- backward (device pointers, asynchronous on `stream`) ----
+    @staticmethod
+    def grad_workspace_bytes(n: int, mode: int = GRAD_DETERMINISTIC) -> int:
+        out = C.c_uint64()
+        _check(library().mbvh_grad_workspace_bytes(n, mode, C.byref(out)))
+        return out.value
+
+    def raycast_first_bwd_d(self, dirs_ptr: int, n: int, face_ptr: int, dist_ptr: int, bary_ptr: int,
+                            g_dist_ptr: int = 0, g_point_ptr: int = 0, g_bary_ptr: int = 0, g_origins_ptr: int = 0,
+                            g_dirs_ptr: int = 0, g_pos_ptr: int = 0, mode: int = GRAD_DETERMINISTIC,
+                            ws_ptr: int = 0, ws_bytes: int = 0, stream: int = 0) -> None:
+        v = lambda x: C.c_void_p(x or None)
+        _check(self.ctx._lib.mbvh_raycast_first_bwd_d(
+            self.ctx.handle, self.handle, v(dirs_ptr), n, v(face_ptr), v(dist_ptr), v(bary_ptr), v(g_dist_ptr),
+            v(g_point_ptr), v(g_bary_ptr), v(g_origins_ptr), v(g_dirs_ptr), v(g_pos_ptr), mode, v(ws_ptr), ws_bytes,
+            v(stream)))
+
+    def closest_point_bwd_d(self, points_ptr: int, n: int, face_ptr: int, dist_ptr: int, closest_ptr: int,
+                            bary_ptr: int, g_dist_ptr: int, g_points_ptr: int = 0, g_pos_ptr: int = 0,
+                            mode: int = GRAD_DETERMINISTIC, ws_ptr: int = 0, ws_bytes: int = 0,
+                            stream: int = 0) -> None:
+        v = lambda x: C.c_void_p(x or None)
+        _check(self.ctx._lib.mbvh_closest_point_bwd_d(
+            self.ctx.handle, self.handle, v(points_ptr), n, v(face_ptr), v(dist_ptr), v(closest_ptr), v(bary_ptr),
+            v(g_dist_ptr), v(g_points_ptr), v(g_pos_ptr), mode, v(ws_ptr), ws_bytes, v(stream)))
+
+
+class Batch:
+    """Several uploaded meshes for one persistent batched launch (mbvh_batch)."""
+
+    def __init__(self, ctx: Context, meshes: Sequence[Mesh]):
+        self.ctx = ctx
+        self.meshes = list(meshes)  # keeps the meshes alive
+        arr = (C.c_void_p * len(self.meshes))(*[m.handle for m in self.meshes])
+        h = C.c_void_p()
+        _check(ctx._lib.mbvh_batch_create(ctx.handle, arr, len(self.meshes), C.byref(h)))
+        self.handle = h
+
+    def free(self) -> None:
+        if self.handle:
+            self.ctx._lib.mbvh_batch_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def workspace_bytes(self, n: int) -> int:
+        out = C.c_uint64()
+        _check(self.ctx._lib.mbvh_batch_workspace_bytes(self.handle, n, C.byref(out)))
+        return out.value
+
+    def _args(self, p):
+        v = lambda x: C.c_void_p(x or None)
+        return [self.ctx.handle, self.handle, v(p["world"]), v(p["inverse"]), v(p["instance_geometry"]),
+                p["n_instances"], v(p["origins"]), v(p["directions"]), v(p["ray_instance"]), p["n"],
+                p.get("side", FRONT_SIDE), p.get("near", 0.0), p.get("far", float("inf")), v(p["face"]),
+                v(p.get("distance")), v(p.get("point")), v(p.get("barycoord")), v(p["workspace"]),
+                p["workspace_bytes"]]
+
+    def raycast_first_d(self, p: dict, tuning: Optional[BatchTuning] = None, stream: int = 0) -> None:
+        """p: device pointers world, inverse, instance_geometry, origins, directions, ray_instance, face,
+        distance, point, barycoord, workspace (+ n_instances, n, workspace_bytes, side, near, far)."""
+        _check(self.ctx._lib.mbvh_batch_raycast_first_d(*self._args(p), C.byref(tuning) if tuning else None,
+                                                        C.c_void_p(stream or None)))
+
+    def autotune(self, p: dict) -> BatchTuning:
+        t = BatchTuning()
+        _check(self.ctx._lib.mbvh_batch_autotune(*self._args(p), C.byref(t)))
+        return t
+
+    def graph(self, p: dict, tuning: Optional[BatchTuning] = None) -> "BatchGraph":
+        h = C.c_void_p()
+        _check(self.ctx._lib.mbvh_batch_graph_create(*self._args(p), C.byref(tuning) if tuning else None, C.byref(h)))
+        return BatchGraph(self, h)
+
+
+class BatchGraph:
+    """A captured batched launch (mbvh_batch_graph)."""
+
+    def __init__(self, batch: Batch, handle):
+        self.batch = batch
+        self.handle = handle
+
+    def launch(self, stream: int = 0) -> None:
+        _check(self.batch.ctx._lib.mbvh_batch_graph_launch(self.handle, C.c_void_p(stream or None)))
+
+    def free(self) -> None:
+        if self.handle:
+            self.batch.ctx._lib.mbvh_batch_graph_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

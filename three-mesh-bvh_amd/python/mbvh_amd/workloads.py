@@ -90,3 +90,87 @@ def c4_closest(res: int = 256, detail: int = 223) -> Workload:
     lo, hi = g.bounds()
     return Workload(f"C4 closest point displaced Icosahedron(1,{detail}), {res}^3 grid", g,
                     points=grid_points(lo, hi, res))
+
+
+@dataclass
+class BatchedWorkload:
+    """C5: many geometries, one rigid instance each, rays split across instances."""
+    geometries: list           # G.Geometry per mesh
+    instance_geometry: np.ndarray  # uint32 (n_instances,)
+    matrix_world: np.ndarray   # float32 (n_instances, 16), column-major (three's Matrix4.elements)
+    matrix_world_inverse: np.ndarray
+    origins: np.ndarray
+    directions: np.ndarray
+    ray_instance: np.ndarray   # uint32 (n_rays,)
+
+
+def _rotation(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def c5_batched(n_geometries: int = 512, n_rays: int = 1 << 22, tri_min: int = 100, tri_max: int = 200_000,
+               zipf: float = 1.1, seed: int = 5) -> BatchedWorkload:
+    """C5 batched multi-BVH raycastFirst: torus knots and spheres with seeded parameters and triangle
+    counts log-uniform in [tri_min, tri_max], one random rigid instance each (spread over a cube),
+    rays split across instances by Zipf(zipf) over a random ranking; each ray starts on a sphere of
+    3x the instance's bounding radius and aims at a uniform point of its world AABB."""
+    rng = np.random.default_rng(seed)
+    geos = []
+    for i in range(n_geometries):
+        t = float(np.exp(rng.uniform(np.log(tri_min), np.log(tri_max))))
+        if i % 2 == 0:
+            radial = max(3, int(round(np.sqrt(t / 16))))
+            tubular = max(3, int(round(t / (2 * radial))))
+            p, q = int(rng.integers(1, 4)), int(rng.integers(2, 6))
+            if np.gcd(p, q) != 1:
+                p, q = 2, 3
+            geos.append(G.torus_knot(rng.uniform(0.5, 2.0), rng.uniform(0.1, 0.5), tubular, radial, p, q))
+        else:
+            h = max(2, int(round(np.sqrt(t / 4))))
+            w = max(3, int(round(t / (2 * h))))
+            geos.append(G.sphere(rng.uniform(0.5, 2.0), w, h))
+    n_inst = n_geometries
+    inst_geom = rng.permutation(n_geometries).astype(np.uint32)
+    world = np.zeros((n_inst, 16), np.float64)
+    span = 8.0 * np.cbrt(n_inst)
+    for k in range(n_inst):
+        q = rng.standard_normal(4)
+        q /= np.linalg.norm(q)
+        m = np.eye(4)
+        m[:3, :3] = _rotation(q)
+        m[:3, 3] = rng.uniform(-span, span, 3)
+        world[k] = m.T.reshape(-1)  # column-major
+    world32 = world.astype(np.float32)
+    inv32 = np.stack([np.linalg.inv(world32[k].astype(np.float64).reshape(4, 4).T).T.reshape(-1)
+                      for k in range(n_inst)]).astype(np.float32)
+    weights = 1.0 / np.arange(1, n_inst + 1) ** zipf
+    weights /= weights.sum()
+    counts = rng.multinomial(n_rays, weights)[rng.permutation(n_inst)]
+    ray_inst = np.repeat(np.arange(n_inst, dtype=np.uint32), counts)
+    rng.shuffle(ray_inst)
+    o = np.empty((n_rays, 3))
+    tgt = np.empty((n_rays, 3))
+    by_inst = np.argsort(ray_inst, kind="stable")
+    starts = np.concatenate([[0], np.cumsum(np.bincount(ray_inst, minlength=n_inst))])
+    for k in range(n_inst):
+        sel = by_inst[starts[k]:starts[k + 1]]
+        m = sel.shape[0]
+        if m == 0:
+            continue
+        g = geos[inst_geom[k]]
+        lo, hi = g.bounds()
+        corners = np.array([[x, y, z] for x in (lo[0], hi[0]) for y in (lo[1], hi[1]) for z in (lo[2], hi[2])])
+        mw = world32[k].astype(np.float64).reshape(4, 4).T
+        wc = corners @ mw[:3, :3].T + mw[:3, 3]
+        wlo, whi = wc.min(axis=0), wc.max(axis=0)
+        c = (wlo + whi) / 2
+        r = np.linalg.norm(whi - wlo) / 2
+        o[sel] = c + 3 * r * unit_sphere(rng, m)
+        tgt[sel] = wlo + (whi - wlo) * rng.random((m, 3))
+    o32 = o.astype(np.float32)
+    d = tgt - o32.astype(np.float64)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return BatchedWorkload(geos, inst_geom, world32, inv32, o32, d.astype(np.float32), ray_inst)

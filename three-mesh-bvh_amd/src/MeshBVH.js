@@ -23,6 +23,11 @@ function sideMask(side) {
   return 1;
 }
 
+function f32OrNull(a) {
+  if (a === undefined || a === null) return null;
+  return a instanceof Float32Array ? a : Float32Array.from(a);
+}
+
 class MeshBVH {
   constructor(geometry, options = {}) {
     if (!geometry || !geometry.isBufferGeometry) {
@@ -254,6 +259,66 @@ class MeshBVH {
     native.load().closestPoint(this._mesh(), q, options.minThreshold || 0,
       options.maxThreshold !== undefined ? options.maxThreshold : Infinity, out.faceIndex, out.distance, out.point,
       out.barycoord);
+    return out;
+  }
+
+  // ---- differentiable queries: autograd-style forward / backward pairs ---------------------------
+  // (no counterpart in the reference). forward() runs the batch query and returns its outputs
+  // plus `saved`, the hit data the backward reuses (it never re-traverses). backward(saved, grads)
+  // takes the upstream gradients of a scalar loss with respect to the forward outputs and returns
+  // the gradients with respect to the inputs and to the vertex positions (one x, y, z triple per
+  // vertex of the position attribute). options.deterministic (default true) selects the sorted
+  // segmented vertex reduction (bitwise reproducible) instead of FP32 atomics;
+  // options.positions === false skips vertex gradients.
+
+  raycastFirstForward(origins, directions, options = {}) {
+    const d = directions instanceof Float32Array ? directions.slice() : Float32Array.from(directions);
+    const out = this.raycastFirstBatch(origins, d, options);
+    out.saved = { kind: 'raycastFirst', directions: d, faceIndex: out.faceIndex, distance: out.distance,
+      barycoord: out.barycoord };
+    return out;
+  }
+
+  // grads: { distance?, point?, barycoord? } (Float32Arrays of n, 3n, 3n; missing = 0).
+  raycastFirstBackward(saved, grads = {}, options = {}) {
+    if (!saved || saved.kind !== 'raycastFirst') throw new Error('raycastFirstBackward: `saved` must come from raycastFirstForward');
+    const n = saved.faceIndex.length;
+    const addon = native.load();
+    const mesh = this._mesh();
+    const out = {
+      origins: new Float32Array(3 * n), directions: new Float32Array(3 * n),
+      positions: options.positions === false ? null : new Float32Array(3 * addon.meshInfo(mesh).vertexCount),
+    };
+    addon.raycastFirstBackward(mesh, saved.directions, saved.faceIndex, saved.distance, saved.barycoord,
+      f32OrNull(grads.distance), f32OrNull(grads.point), f32OrNull(grads.barycoord),
+      out.origins, out.directions, out.positions, options.deterministic === false ? 0 : 1);
+    return out;
+  }
+
+  closestPointToPointForward(points, options = {}) {
+    const q = points instanceof Float32Array ? points.slice() : Float32Array.from(points);
+    const out = this.closestPointToPointBatch(q, options);
+    out.saved = { kind: 'closestPointToPoint', points: q, faceIndex: out.faceIndex, distance: out.distance,
+      point: out.point, barycoord: out.barycoord };
+    return out;
+  }
+
+  // grads: { distance } (Float32Array of n).
+  closestPointToPointBackward(saved, grads = {}, options = {}) {
+    if (!saved || saved.kind !== 'closestPointToPoint') {
+      throw new Error('closestPointToPointBackward: `saved` must come from closestPointToPointForward');
+    }
+    const n = saved.faceIndex.length;
+    const addon = native.load();
+    const mesh = this._mesh();
+    const gd = f32OrNull(grads.distance);
+    if (!gd || gd.length !== n) throw new Error('closestPointToPointBackward: grads.distance must have one entry per point');
+    const out = {
+      points: new Float32Array(3 * n),
+      positions: options.positions === false ? null : new Float32Array(3 * addon.meshInfo(mesh).vertexCount),
+    };
+    addon.closestPointBackward(mesh, saved.points, saved.faceIndex, saved.distance, saved.point, saved.barycoord,
+      gd, out.points, out.positions, options.deterministic === false ? 0 : 1);
     return out;
   }
 
