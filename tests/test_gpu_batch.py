@@ -38,6 +38,7 @@ def case(ctx):
          "directions": _t(w.directions), "ray_instance": _t(w.ray_instance.view(np.int32))}
     wsb = batch.workspace_bytes(n)
     D["workspace"] = torch.empty(wsb, dtype=torch.uint8, device=DEV)
+    torch.cuda.synchronize()
     yield w, built, meshes, batch, D, wsb
     batch.free()
     for m in meshes:
@@ -76,6 +77,7 @@ def _plain_loop(ctx, w, meshes, D):
     ld = torch.empty((n, 3), device=DEV)
     ctx.instance_rays_d(D["inverse"].data_ptr(), w.matrix_world.shape[0], D["origins"].data_ptr(),
                         D["directions"].data_ptr(), D["ray_instance"].data_ptr(), n, lo.data_ptr(), ld.data_ptr())
+    torch.cuda.synchronize()  # the library's calls run on the context's stream, torch's on its own
     geo = w.instance_geometry[w.ray_instance]
     order = np.argsort(geo, kind="stable")
     starts = np.concatenate([[0], np.cumsum(np.bincount(geo, minlength=len(meshes)))])
@@ -89,10 +91,12 @@ def _plain_loop(ctx, w, meshes, D):
         mesh.raycast_first_d(slo[a:].data_ptr(), sld[a:].data_ptr(), b - a, so["face"][a:].data_ptr(),
                              so["distance"][a:].data_ptr(), so["point"][a:].data_ptr(),
                              so["barycoord"][a:].data_ptr())
+    torch.cuda.synchronize()
     out = _outputs(n)
     for k in out:
         out[k][idx] = so[k]
     local = _host(out)
+    torch.cuda.synchronize()
     ctx.instance_hits_d(D["world"].data_ptr(), w.matrix_world.shape[0], D["origins"].data_ptr(),
                         D["ray_instance"].data_ptr(), n, out["face"].data_ptr(), out["point"].data_ptr(),
                         out["distance"].data_ptr())
@@ -154,6 +158,7 @@ def test_graph_replays_between_other_work(ctx, case):
         for v in out.values():
             v.fill_(-7)  # eager work on the outputs between replays
         junk = torch.randn(1 << 20, device=DEV).sum().item()  # and unrelated work
+        torch.cuda.synchronize()  # torch's stream and the context's stream are not ordered
         graph.launch()
         torch.cuda.synchronize()
         _same(_host(out), base)
@@ -162,6 +167,7 @@ def test_graph_replays_between_other_work(ctx, case):
     saved = D["directions"].clone()
     perm = _t(rng.permutation(n).astype(np.int64))
     D["directions"].copy_(saved[perm])
+    torch.cuda.synchronize()
     graph.launch()
     torch.cuda.synchronize()
     _same(_host(out), _batched(w, batch, D, wsb))
@@ -176,6 +182,7 @@ def test_invalid_instances_miss(ctx, case):
     bad[::3] = 10_000  # no such instance
     D2 = dict(D)
     D2["ray_instance"] = _t(bad.view(np.int32))
+    torch.cuda.synchronize()
     out = _outputs(n)
     batch.raycast_first_d(_params(w, D2, out, batch.workspace_bytes(n), n=n))
     torch.cuda.synchronize()

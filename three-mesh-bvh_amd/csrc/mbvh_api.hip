@@ -30,6 +30,7 @@ struct mbvh_ctx {
   hipStream_t stream = nullptr;
   uint64_t max_threads = 0;     // resident-thread cap for one launch
   DeviceBuffer stack_ovf;       // traversal stack overflow region
+  DeviceBuffer queue;           // work-queue counter of the single-mesh query kernels
   uint32_t stack_ovf_entries = 0;
   DeviceBuffer staging[8];      // host-API input / output staging
   DeviceBuffer order;           // host-API query order
@@ -303,6 +304,11 @@ extern "C" int mbvh_ctx_create(int32_t device, mbvh_ctx** out_ctx) {
     delete c;
     return set_error(MBVH_ERR_HIP, "mbvh_ctx_create: hipStreamCreate failed (%s)", hipGetErrorString(e));
   }
+  if (ensure_buffer(c->queue, 256)) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MBVH_ERR_OOM;
+  }
   *out_ctx = c;
   return MBVH_OK;
 }
@@ -312,6 +318,7 @@ extern "C" void mbvh_ctx_destroy(mbvh_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->stack_ovf.ptr) (void)hipFree(ctx->stack_ovf.ptr);
+  if (ctx->queue.ptr) (void)hipFree(ctx->queue.ptr);
   for (auto& b : ctx->staging)
     if (b.ptr) (void)hipFree(b.ptr);
   for (auto& b : ctx->hits)
@@ -551,6 +558,7 @@ int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uin
   a.stack = stack_of(ctx, threads);
   a.points = q;
   a.order = order;
+  a.queue = static_cast<uint32_t*>(ctx->queue.ptr);
   a.n = n;
   a.min_threshold = min_t;
   a.max_threshold = max_t;

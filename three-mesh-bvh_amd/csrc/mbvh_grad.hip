@@ -27,6 +27,7 @@
 //                  by (vertex, query, corner); a segmented sum over fixed-size chunks of the
 //                  sorted list, then across chunk boundaries in chunk order, gives bitwise
 //                  identical results on every run.
+#include "mbvh_closest_ff.h"
 #include "mbvh_kernels.h"
 
 namespace mbvh {
@@ -88,14 +89,24 @@ __global__ __launch_bounds__(kGradBlock) void k_raycast_first_bwd(RaycastBwdArgs
     const float g_dist = A.grad_distance ? A.grad_distance[i] : 0.0f;
     const f3 g_pt = A.grad_point ? load3(A.grad_point, i) : mk(0.0f, 0.0f, 0.0f);
     const f3 g_bc = A.grad_barycoord ? load3(A.grad_barycoord, i) : mk(0.0f, 0.0f, 0.0f);
-    const float g_t = g_dist * dlen + dot(g_pt, d);
-    const float g_u = g_bc.y - g_bc.x;
-    const float g_v = g_bc.z - g_bc.x;
-    const f3 e1 = sub(b, a), e2 = sub(c, a), nd = mk(-d.x, -d.y, -d.z);
-    const f3 n = cross(e1, e2);
-    const float det = dot(nd, n);
-    const f3 num = add3(add3(scale3(n, g_t), scale3(cross(e2, nd), g_u)), scale3(cross(nd, e1), g_v));
-    const f3 lam = scale3(num, 1.0f / det);
+    // upstream of (t, u, v) in double-float too: the three terms of lambda's numerator can cancel
+    const ff dlen_ff = ff_sqrt(ff_add(ff_add(two_prod(d.x, d.x), two_prod(d.y, d.y)), two_prod(d.z, d.z)));
+    const ff g_t = ff_add(ff_mulf(dlen_ff, g_dist),
+                          ff_add(ff_add(two_prod(g_pt.x, d.x), two_prod(g_pt.y, d.y)), two_prod(g_pt.z, d.z)));
+    const ff g_u = two_sum(g_bc.y, -g_bc.x);
+    const ff g_v = two_sum(g_bc.z, -g_bc.x);
+    // lambda in double-float: edges of a small triangle far from the origin lose most of their
+    // FP32 bits to cancellation, and det loses more on grazing rays; the exact edge differences
+    // keep lambda at FP32 accuracy.
+    const ff3 e1 = ff3_diff(b, a), e2 = ff3_diff(c, a);
+    const ff3 nd = {ff_from(-d.x), ff_from(-d.y), ff_from(-d.z)};
+    const ff3 n = ff3_cross(e1, e2);
+    const ff det = ff3_dot(nd, n);
+    const ff3 c2 = ff3_cross(e2, nd), c3 = ff3_cross(nd, e1);
+    const auto comp = [&](ff nk, ff c2k, ff c3k) {
+      return ff_to_float(ff_div(ff_add(ff_add(ff_mul(nk, g_t), ff_mul(c2k, g_u)), ff_mul(c3k, g_v)), det));
+    };
+    const f3 lam = mk(comp(n.x, c2.x, c3.x), comp(n.y, c2.y, c3.y), comp(n.z, c2.z, c3.z));
     d_o = add3(lam, g_pt);
     d_d = add3(scale3(add3(lam, g_pt), t), scale3(d, g_dist * t / dlen));
     ga = scale3(lam, -bc.x);
@@ -123,15 +134,23 @@ __global__ __launch_bounds__(kGradBlock) void k_closest_point_bwd(ClosestBwdArgs
     v0 = A.mesh.vidx[3ull * slot];
     v1 = A.mesh.vidx[3ull * slot + 1];
     v2 = A.mesh.vidx[3ull * slot + 2];
-    const float D = A.distance[i];
-    if (D > 0.0f) {
-      const f3 q = load3(A.points, i), p = load3(A.closest, i), bc = load3(A.barycoord, i);
-      const float g = A.grad_distance[i];
-      const f3 nrm = scale3(sub(q, p), 1.0f / D);
-      d_q = scale3(nrm, g);
-      ga = scale3(nrm, -g * bc.x);
-      gb = scale3(nrm, -g * bc.y);
-      gc = scale3(nrm, -g * bc.z);
+    if (A.distance[i] > 0.0f) {
+      // q - p in double-float from the saved face: an FP32 closest point loses the direction's
+      // bits to cancellation when the query is near the surface
+      const float4 pa = A.mesh.tris[3ull * slot], pb = A.mesh.tris[3ull * slot + 1], pc = A.mesh.tris[3ull * slot + 2];
+      const f3 q = load3(A.points, i);
+      const ClosestFF cf = closest_on_triangle_ff(q, mk(pa.x, pa.y, pa.z), mk(pb.x, pb.y, pb.z), mk(pc.x, pc.y, pc.z));
+      const ff D = ff_sqrt(cf.dist2);
+      if (D.hi > 0.0f) {
+        const float g = A.grad_distance[i];
+        const f3 nrm = mk(ff_to_float(ff_div(ff_sub(ff_from(q.x), cf.point.x), D)),
+                          ff_to_float(ff_div(ff_sub(ff_from(q.y), cf.point.y), D)),
+                          ff_to_float(ff_div(ff_sub(ff_from(q.z), cf.point.z), D)));
+        d_q = scale3(nrm, g);
+        ga = scale3(nrm, -g * ff_to_float(cf.bw));
+        gb = scale3(nrm, -g * ff_to_float(cf.bu));
+        gc = scale3(nrm, -g * ff_to_float(cf.bv));
+      }
     }
   }
   if (A.grad_points) store3(A.grad_points, i, d_q);

@@ -63,6 +63,7 @@ struct ClosestPointArgs {
   StackScratch stack;
   const float* points;
   const uint32_t* order;  // visit order of the points (NULL: 0..n-1); results go to each point's own index
+  uint32_t* queue;        // work-queue counter (reset by the launch)
   uint64_t n;
   float min_threshold;
   float max_threshold;

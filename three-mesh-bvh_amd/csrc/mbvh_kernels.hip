@@ -14,6 +14,7 @@
 // wave do it together. The resolution keeps the first candidate in visit order among exactly
 // equal minima, which is the reference's strict-< rule.
 #include "mbvh_kernels.h"
+#include "mbvh_closest_ff.h"
 
 namespace mbvh {
 namespace {
@@ -750,137 +751,34 @@ __global__ __launch_bounds__(kBlock) void k_raycast_all(RaycastAllArgs A) {
 // three's Triangle.closestPointToPoint), Voronoi regions tested in the same order.
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ f3 closest_on_triangle(f3 p, f3 a, f3 b, f3 c) {
-  // Branch-free: all six dot products, then the region by the same ordered tests (first match
-  // wins), then one division for the region's parameter. Lanes of a wave sit in different
-  // regions, so a branchy version would run every region's code one after another.
-  const f3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a), bp = sub(p, b), cp = sub(p, c);
+  const f3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
   const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) return a;
+  const f3 bp = sub(p, b);
   const float d3 = dot(ab, bp), d4 = dot(ac, bp);
-  const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d3 >= 0.0f && d4 <= d3) return b;
   const float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    const float v = d1 / (d1 - d3);
+    return mk(a.x + ab.x * v, a.y + ab.y * v, a.z + ab.z * v);
+  }
+  const f3 cp = sub(p, c);
+  const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) return c;
   const float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    const float w = d2 / (d2 - d6);
+    return mk(a.x + ac.x * w, a.y + ac.y * w, a.z + ac.z * w);
+  }
   const float va = d3 * d6 - d5 * d4;
-  const float d43 = d4 - d3, d56 = d5 - d6;
-  const bool rA = d1 <= 0.0f && d2 <= 0.0f;
-  const bool rB = !rA && d3 >= 0.0f && d4 <= d3;
-  const bool rAB = !rA && !rB && vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f;
-  const bool rC = !rA && !rB && !rAB && d6 >= 0.0f && d5 <= d6;
-  const bool rAC = !rA && !rB && !rAB && !rC && vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f;
-  const bool rBC = !rA && !rB && !rAB && !rC && !rAC && va <= 0.0f && d43 >= 0.0f && d56 >= 0.0f;
-  const bool rF = !rA && !rB && !rAB && !rC && !rAC && !rBC;
-  // parameter: AB d1/(d1-d3), AC d2/(d2-d6), BC d43/(d43+d56), face 1/(va+vb+vc); every
-  // candidate is computed, then selected (selects, not branches)
-  const float den_ab = d1 - d3, den_ac = d2 - d6, den_bc = d43 + d56, den_f = va + vb + vc;
-  float num = 1.0f, den = 1.0f;
-  num = rAB ? d1 : num;
-  num = rAC ? d2 : num;
-  num = rBC ? d43 : num;
-  den = rF ? den_f : den;
-  den = rAB ? den_ab : den;
-  den = rAC ? den_ac : den;
-  den = rBC ? den_bc : den;
-  const float q = num / den;
-  // point = base + dir1 * s1 + dir2 * s2 (unused terms are exact zeros)
-  const f3 base = (rB || rBC) ? b : (rC ? c : a);
-  const f3 bc = sub(c, b);
-  const f3 z = mk(0.0f, 0.0f, 0.0f);
-  const f3 dir1 = (rAB || rF) ? ab : (rAC ? ac : (rBC ? bc : z));
-  const float s1 = (rAB || rAC || rBC) ? q : (rF ? vb * q : 0.0f);
-  const f3 dir2 = rF ? ac : z;
-  const float s2 = rF ? vc * q : 0.0f;
-  return mk(base.x + dir1.x * s1 + dir2.x * s2, base.y + dir1.y * s1 + dir2.y * s2,
-            base.z + dir1.z * s1 + dir2.z * s2);
-}
-
-// Same regions in double-float; returns the point, its barycoord (weights of a, b, c) and the
-// squared distance.
-struct ClosestFF {
-  ff3 point;
-  ff bw, bu, bv;
-  ff dist2;
-};
-
-__device__ __forceinline__ bool ff_le0(ff x) { return ff_sign(x) <= 0; }
-__device__ __forceinline__ bool ff_ge0(ff x) { return ff_sign(x) >= 0; }
-
-__device__ __noinline__ ClosestFF closest_on_triangle_ff(f3 p, f3 a, f3 b, f3 c) {
-  const ff3 ab = ff3_diff(b, a), ac = ff3_diff(c, a), ap = ff3_diff(p, a);
-  const ff one = ff_from(1.0f), zero = ff_from(0.0f);
-  const ff d1 = ff3_dot(ab, ap), d2 = ff3_dot(ac, ap);
-  ClosestFF r;
-  int region;  // 0 face, 1 a, 2 b, 3 c, 4 ab, 5 ac, 6 bc
-  ff s = zero;  // edge parameter
-  ff vb = zero, vc = zero, va = zero;
-  const ff3 bp = ff3_diff(p, b);
-  const ff d3 = ff3_dot(ab, bp), d4 = ff3_dot(ac, bp);
-  const ff3 cpv = ff3_diff(p, c);
-  const ff d5 = ff3_dot(ab, cpv), d6 = ff3_dot(ac, cpv);
-  if (ff_le0(d1) && ff_le0(d2)) {
-    region = 1;
-  } else if (ff_ge0(d3) && ff_le0(ff_sub(d4, d3))) {
-    region = 2;
-  } else {
-    vc = ff_sub(ff_mul(d1, d4), ff_mul(d3, d2));
-    if (ff_le0(vc) && ff_ge0(d1) && ff_le0(d3)) {
-      region = 4;
-      s = ff_div(d1, ff_sub(d1, d3));
-    } else if (ff_ge0(d6) && ff_le0(ff_sub(d5, d6))) {
-      region = 3;
-    } else {
-      vb = ff_sub(ff_mul(d5, d2), ff_mul(d1, d6));
-      if (ff_le0(vb) && ff_ge0(d2) && ff_le0(d6)) {
-        region = 5;
-        s = ff_div(d2, ff_sub(d2, d6));
-      } else {
-        va = ff_sub(ff_mul(d3, d6), ff_mul(d5, d4));
-        const ff d43 = ff_sub(d4, d3), d56 = ff_sub(d5, d6);
-        if (ff_le0(va) && ff_ge0(d43) && ff_ge0(d56)) {
-          region = 6;
-          s = ff_div(d43, ff_add(d43, d56));
-        } else {
-          region = 0;
-        }
-      }
-    }
+  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+    const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    const f3 bc = sub(c, b);
+    return mk(b.x + bc.x * w, b.y + bc.y * w, b.z + bc.z * w);
   }
-  ff bw = zero, bu = zero, bv = zero;
-  switch (region) {
-    case 1: bw = one; break;
-    case 2: bu = one; break;
-    case 3: bv = one; break;
-    case 4: bu = s; bw = ff_sub(one, s); break;
-    case 5: bv = s; bw = ff_sub(one, s); break;
-    case 6: bv = s; bu = ff_sub(one, s); break;
-    default: {
-      const ff sum = ff_add(ff_add(va, vb), vc);
-      bu = ff_div(vb, sum);
-      bv = ff_div(vc, sum);
-      bw = ff_sub(ff_sub(one, bu), bv);
-    }
-  }
-  // point = a + u*(b - a) + v*(c - a) (vertex regions return the vertex itself, exactly)
-  if (region == 1) {
-    r.point = {ff_from(a.x), ff_from(a.y), ff_from(a.z)};
-  } else if (region == 2) {
-    r.point = {ff_from(b.x), ff_from(b.y), ff_from(b.z)};
-  } else if (region == 3) {
-    r.point = {ff_from(c.x), ff_from(c.y), ff_from(c.z)};
-  } else if (region == 6) {
-    const ff3 bc = ff3_diff(c, b);
-    r.point = {ff_add(ff_from(b.x), ff_mul(bc.x, s)), ff_add(ff_from(b.y), ff_mul(bc.y, s)),
-               ff_add(ff_from(b.z), ff_mul(bc.z, s))};
-  } else {
-    r.point = {ff_add(ff_add(ff_from(a.x), ff_mul(ab.x, bu)), ff_mul(ac.x, bv)),
-               ff_add(ff_add(ff_from(a.y), ff_mul(ab.y, bu)), ff_mul(ac.y, bv)),
-               ff_add(ff_add(ff_from(a.z), ff_mul(ab.z, bu)), ff_mul(ac.z, bv))};
-  }
-  const ff3 diff = {ff_sub(ff_from(p.x), r.point.x), ff_sub(ff_from(p.y), r.point.y),
-                    ff_sub(ff_from(p.z), r.point.z)};
-  r.dist2 = ff3_dot(diff, diff);
-  r.bw = bw;
-  r.bu = bu;
-  r.bv = bv;
-  return r;
+  const float denom = 1.0f / (va + vb + vc);
+  const float v = vb * denom, w = vc * denom;
+  return mk(a.x + ab.x * v + ac.x * w, a.y + ab.y * v + ac.y * w, a.z + ab.z * v + ac.z * w);
 }
 
 __device__ __forceinline__ float box_score(f3 q, float mnx, float mny, float mnz, float mxx,
@@ -925,7 +823,6 @@ __device__ __noinline__ uint32_t resolve_point(Cands cl, float U, f3 q, const fl
 __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
   const uint32_t lane = threadIdx.x;
   const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + lane;
-  const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * kBlock;
   const MeshView M = A.mesh;
 
   Stack st;
@@ -939,7 +836,15 @@ __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
   const float max2 = A.max_threshold * A.max_threshold;
   constexpr float kScoreSlack = 1.0f - 4.0f * kU;
 
-  for (uint64_t visit = gtid; visit < A.n; visit += gstride) {
+  // Work queue: each wave takes the next 64 queries until none are left (query costs vary a lot,
+  // so a dynamic assignment keeps the tail short). Every wave reaches the exit test.
+  while (true) {
+    uint32_t tile = 0;
+    if ((lane & 63u) == 0) tile = atomicAdd(A.queue, 1u);
+    tile = __shfl(tile, 0, 64);
+    if (static_cast<uint64_t>(tile) * 64 >= A.n) break;
+    const uint64_t visit = static_cast<uint64_t>(tile) * 64 + (lane & 63u);
+    if (visit >= A.n) continue;
     const uint64_t qi = A.order ? A.order[visit] : visit;
     if (qi >= A.n) continue;  // not a permutation of 0..n-1: skip rather than read out of bounds
     const f3 q = mk(A.points[3 * qi + 0], A.points[3 * qi + 1], A.points[3 * qi + 2]);
@@ -1136,6 +1041,7 @@ hipError_t launch_batch_raycast_first(const BatchArgs& a, uint32_t stack_lds, ui
 
 hipError_t launch_closest_point(const ClosestPointArgs& a, uint64_t threads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.queue);
   hipLaunchKernelGGL(k_closest_point, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
