@@ -248,7 +248,9 @@ def bench_batched(torch, M, W, ctx, stream, args):
         ctx.instance_hits_d(D["world"].data_ptr(), n_inst, D["origins"].data_ptr(), D["ray_instance"].data_ptr(), n,
                             out["face"].data_ptr(), out["point"].data_ptr(), out["distance"].data_ptr(), stream=sptr)
 
+    ctx.set_ray_lanes(tuning.ray_lanes)  # the loop traces with the same traversal as the tuned launch
     l_ms = event_ms(torch, stream, loop, 3)
+    ctx.set_ray_lanes(0)
     same = all(torch.equal(out[k].view(torch.int32), res_b[k].view(torch.int32)) for k in out)
     tris = int(sum(g.tri_count for g in geos))
     r = {"workload": f"{len(meshes)} meshes (torus knots / spheres, {tris} tris, 100..200k each), one rigid "
@@ -328,20 +330,14 @@ def main():
     r_dist = torch.empty(nr, dtype=torch.float32, device=dev)
     r_pt = torch.empty((nr, 3), dtype=torch.float32, device=dev)
     r_bc = torch.empty((nr, 3), dtype=torch.float32, device=dev)
-    r_order = torch.empty(nr, dtype=torch.int32, device=dev)
-    r_ws_bytes = M.Context.order_workspace_bytes(nr)
-    r_ws = torch.empty(r_ws_bytes, dtype=torch.uint8, device=dev)
 
-    def order_rc():
-        if not args.no_order:
-            ctx.order_points_d(d_o.data_ptr(), nr, r_order.data_ptr(), r_ws.data_ptr(), r_ws_bytes, stream=sptr)
-
+    # rays in their given order: origin-ordering does not make these incoherent rays (random
+    # origins, random targets) share paths, and the extra pass costs more than it saves
     def launch_rc():
         mesh1.raycast_first_d(d_o.data_ptr(), d_d.data_ptr(), nr, r_face.data_ptr(), r_dist.data_ptr(),
-                              r_pt.data_ptr(), r_bc.data_ptr(), stream=sptr,
-                              order_ptr=0 if args.no_order else r_order.data_ptr())
+                              r_pt.data_ptr(), r_bc.data_ptr(), stream=sptr)
 
-    wall1, kern1 = timed_kernel(torch, launch_rc, max(args.steps, 20) * 4, args.warmup, stream, pre=order_rc)
+    wall1, kern1 = timed_kernel(torch, launch_rc, max(args.steps, 20) * 4, args.warmup, stream)
     rc_steps = max(args.steps, 20) * 4
     mrays = nr / (wall1 / rc_steps) / 1e6
     mrays_kernel = nr / (float(np.mean(kern1)) / 1000.0) / 1e6

@@ -112,6 +112,10 @@ void mbvh_tree_free(mbvh_tree* tree);
 
 int mbvh_ctx_create(int32_t device, mbvh_ctx** out_ctx);
 void mbvh_ctx_destroy(mbvh_ctx* ctx);
+/* raycastFirst traversal width: 1 lane per ray on the binary tree, 4 lanes per ray on a 4-wide
+ * tree built at upload, or 0 (default): 4 lanes when all rays of a call fit in one resident round
+ * of the device, else 1. Results are the same up to exactly equal hits (ties). */
+int mbvh_ctx_set_ray_lanes(mbvh_ctx* ctx, int32_t lanes);
 /* Waits for all work the context enqueued on its own stream. */
 int mbvh_ctx_synchronize(mbvh_ctx* ctx);
 
@@ -230,7 +234,8 @@ int mbvh_closest_point_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* poin
  * tuning may be NULL (the batch's autotuned values, or defaults before mbvh_batch_autotune).
  */
 typedef struct mbvh_batch_tuning {
-  uint32_t stack_lds;     /* LDS traversal-stack entries per lane: 8 or 16 */
+  uint32_t ray_lanes;     /* lanes per ray: 1 (binary trees) or 4 (4-wide trees) */
+  uint32_t stack_lds;     /* LDS traversal-stack entries per lane (1 lane per ray): 8 or 16 */
   uint32_t tile_rays;     /* rays per work item: a multiple of 64, at most 4096 */
   uint32_t blocks_per_cu; /* persistent workgroups of 256 lanes per compute unit: 1..8 */
   float ms;               /* mbvh_batch_autotune: measured time of the chosen configuration */
@@ -248,7 +253,7 @@ int mbvh_batch_raycast_first_d(mbvh_ctx* ctx, mbvh_batch* batch, const float* ma
                                float far_dist, uint32_t* face_index, float* distance, float* point,
                                float* barycoord, void* workspace, uint64_t workspace_bytes,
                                const mbvh_batch_tuning* tuning, void* stream);
-/* Times every candidate (stack_lds x tile_rays x blocks_per_cu) on these inputs, keeps the fastest
+/* Times every candidate (ray_lanes x stack_lds x tile_rays x blocks_per_cu) on these inputs, keeps the fastest
  * in the batch (used when tuning is NULL) and returns it. Synchronous; outputs hold the last run. */
 int mbvh_batch_autotune(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_world,
                         const float* matrix_world_inverse, const uint32_t* instance_geometry,

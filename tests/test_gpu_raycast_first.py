@@ -175,3 +175,23 @@ def test_ray_order_changes_nothing(ctx):
     assert np.array_equal(np.sort(o), np.arange(n, dtype=np.uint32))
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
     mesh.free()
+
+
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_lanes_per_ray(lanes):
+    """One lane per ray on the binary tree and four lanes per ray on the 4-wide tree both meet the
+    parity rule on C1 and on rays with zero direction components (which four-lane calls trace on
+    the binary tree)."""
+    c = M.Context(0)
+    c.set_ray_lanes(lanes)
+    try:
+        w = W.c1_raycast()
+        _run(c, w.geometry, w.origins, w.directions, label=f"C1 lanes={lanes}")
+        g = G.torus_knot(10, 3, 120, 30)
+        lo, hi = g.bounds()
+        o, d = W.rays_toward_box(4096, 40.0, lo, hi, seed_origin=41, seed_target=42)
+        d[::2, 1] = 0.0  # axis-parallel in y
+        d[1::4, 2] = 0.0
+        _run(c, g, o, d, label=f"zero components lanes={lanes}")
+    finally:
+        c.close()

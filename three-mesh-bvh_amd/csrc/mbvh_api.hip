@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -31,6 +32,7 @@ struct mbvh_ctx {
   uint64_t max_threads = 0;     // resident-thread cap for one launch
   DeviceBuffer stack_ovf;       // traversal stack overflow region
   DeviceBuffer queue;           // work-queue counter of the single-mesh query kernels
+  int32_t ray_lanes = 0;        // raycastFirst lanes per ray: 0 automatic, 1, or 4
   uint32_t stack_ovf_entries = 0;
   DeviceBuffer staging[8];      // host-API input / output staging
   DeviceBuffer order;           // host-API query order
@@ -49,7 +51,8 @@ struct mbvh_batch {
   MeshView* d_meshes = nullptr;
   uint32_t* d_counter = nullptr;  // work-queue counter
   uint32_t max_stack = 0;
-  mbvh_batch_tuning tuning{16, 256, 4, 0.0f};
+  uint32_t max_stack4 = 0;
+  mbvh_batch_tuning tuning{4, 16, 256, 4, 0.0f};
 };
 
 struct mbvh_batch_graph {
@@ -68,6 +71,9 @@ struct mbvh_mesh {
   float4* d_tris = nullptr;
   uint32_t* d_vidx = nullptr;
   uint32_t* d_face_slot = nullptr;  // indirect meshes: face -> slot (0xFFFFFFFF if absent)
+  float4* d_nodes4 = nullptr;       // 4-wide tree
+  uint32_t* d_roots4 = nullptr;
+  uint32_t max_stack4 = 0;
   uint64_t n_nodes = 0;
   uint32_t n_roots = 0;
   uint64_t n_slots = 0;
@@ -197,6 +203,84 @@ int convert_root(const NodeView& v, uint64_t n_slots, std::vector<WideNode>& wid
   return MBVH_OK;
 }
 
+// 4-wide tree from the wide binary nodes: a 4-wide node holds the children of a binary node's
+// children (a child that is a leaf stays a child), so it replaces two binary levels. Child k of
+// node n is nodes4[8n + 2k] = {min.x, min.y, min.z, max.x}, nodes4[8n + 2k + 1] = {max.y, max.z,
+// ref, 0} with the reference's float32 bounds bit for bit; unused children have ref kDone4 and an
+// empty box. Leaf references are the binary tree's.
+constexpr uint32_t kDone4 = 0x7FFFFFFFu;
+
+int build_bvh4(const std::vector<WideNode>& wide, uint32_t root_ref, std::vector<float4>& nodes4,
+               uint32_t* root4, uint32_t* depth4) {
+  struct Item {
+    uint32_t bref;    // binary (wide) node
+    int64_t parent4;  // -1: the root
+    int slot;
+    uint32_t depth;
+  };
+  if (root_ref & kRefLeaf) {
+    *root4 = root_ref;
+    return MBVH_OK;
+  }
+  std::vector<Item> todo{{root_ref, -1, 0, 1}};
+  auto as_float = [](uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+  };
+  while (!todo.empty()) {
+    const Item it = todo.back();
+    todo.pop_back();
+    if (nodes4.size() / 8 >= 0x7FFFFFFEull) return set_error(MBVH_ERR_LIMIT, "mbvh_mesh_upload: too many nodes");
+    const uint32_t n4 = static_cast<uint32_t>(nodes4.size() / 8);
+    *depth4 = std::max(*depth4, it.depth);
+    if (it.parent4 < 0)
+      *root4 = n4;
+    else
+      nodes4[8 * it.parent4 + 2 * it.slot + 1].z = as_float(n4);
+    struct Child {
+      float box[6];
+      uint32_t ref;
+    };
+    Child ch[4];
+    int nc = 0;
+    const WideNode& w = wide[it.bref];
+    const float lb[6] = {w.b0.x, w.b0.y, w.b0.z, w.b0.w, w.b1.x, w.b1.y};
+    const float rb[6] = {w.b1.z, w.b1.w, w.b2.x, w.b2.y, w.b2.z, w.b2.w};
+    const uint32_t refs[2] = {w.meta.x, w.meta.y};
+    const float* boxes[2] = {lb, rb};
+    for (int side = 0; side < 2; side++) {
+      const uint32_t c = refs[side];
+      if (c & kRefLeaf) {
+        std::memcpy(ch[nc].box, boxes[side], sizeof(ch[nc].box));
+        ch[nc++].ref = c;
+      } else {
+        const WideNode& g = wide[c];
+        const float gl[6] = {g.b0.x, g.b0.y, g.b0.z, g.b0.w, g.b1.x, g.b1.y};
+        const float gr[6] = {g.b1.z, g.b1.w, g.b2.x, g.b2.y, g.b2.z, g.b2.w};
+        std::memcpy(ch[nc].box, gl, sizeof(gl));
+        ch[nc++].ref = g.meta.x;
+        std::memcpy(ch[nc].box, gr, sizeof(gr));
+        ch[nc++].ref = g.meta.y;
+      }
+    }
+    const float inf = std::numeric_limits<float>::infinity();
+    for (int k = 0; k < 4; k++) {
+      if (k < nc) {
+        const Child& c = ch[k];
+        nodes4.push_back(make_float4(c.box[0], c.box[1], c.box[2], c.box[3]));
+        nodes4.push_back(make_float4(c.box[4], c.box[5], as_float(c.ref), 0.0f));
+      } else {
+        nodes4.push_back(make_float4(inf, inf, inf, -inf));
+        nodes4.push_back(make_float4(-inf, -inf, as_float(kDone4), 0.0f));
+      }
+    }
+    for (int k = 0; k < nc; k++)
+      if (!(ch[k].ref & kRefLeaf)) todo.push_back({ch[k].ref, static_cast<int64_t>(n4), k, it.depth + 1});
+  }
+  return MBVH_OK;
+}
+
 uint64_t read_u(const void* p, int elt_bytes, uint64_t i) {
   if (elt_bytes == 2) return static_cast<const uint16_t*>(p)[i];
   return static_cast<const uint32_t*>(p)[i];
@@ -209,7 +293,11 @@ void free_mesh_buffers(mbvh_mesh* m) {
   if (m->d_tris) (void)hipFree(m->d_tris);
   if (m->d_vidx) (void)hipFree(m->d_vidx);
   if (m->d_face_slot) (void)hipFree(m->d_face_slot);
+  if (m->d_nodes4) (void)hipFree(m->d_nodes4);
+  if (m->d_roots4) (void)hipFree(m->d_roots4);
   m->d_face_slot = nullptr;
+  m->d_nodes4 = nullptr;
+  m->d_roots4 = nullptr;
   m->d_nodes = nullptr;
   m->d_roots = nullptr;
   m->d_bigleaf = nullptr;
@@ -221,6 +309,9 @@ MeshView view_of(const mbvh_mesh* m) {
   MeshView v;
   v.nodes = m->d_nodes;
   v.roots = m->d_roots;
+  v.nodes4 = m->d_nodes4;
+  v.roots4 = m->d_roots4;
+  v.max_stack4 = m->max_stack4;
   v.bigleaf = m->d_bigleaf;
   v.tris = m->d_tris;
   v.n_roots = m->n_roots;
@@ -333,6 +424,15 @@ extern "C" void mbvh_ctx_destroy(mbvh_ctx* ctx) {
   delete ctx;
 }
 
+extern "C" int mbvh_ctx_set_ray_lanes(mbvh_ctx* ctx, int32_t lanes) {
+  clear_error();
+  if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_ctx_set_ray_lanes: invalid context handle");
+  if (lanes != 0 && lanes != 1 && lanes != 4)
+    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_ctx_set_ray_lanes: lanes must be 0 (automatic), 1 or 4");
+  ctx->ray_lanes = lanes;
+  return MBVH_OK;
+}
+
 extern "C" int mbvh_ctx_synchronize(mbvh_ctx* ctx) {
   if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_ctx_synchronize: invalid context handle");
   int rc = set_device(ctx);
@@ -381,6 +481,13 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
     if (rc) return rc;
   }
   if (bigleaf.size() > kRefOffsetMask) return set_error(MBVH_ERR_LIMIT, "mbvh_mesh_upload: too many large leaves");
+  std::vector<float4> nodes4;
+  std::vector<uint32_t> roots4(n_roots);
+  uint32_t depth4 = 0;
+  for (uint32_t r = 0; r < n_roots; r++) {
+    const int rc4 = build_bvh4(wide, root_refs[r], nodes4, &roots4[r], &depth4);
+    if (rc4) return rc4;
+  }
 
   // Per-slot triangle records in BVH order.
   std::vector<float4> tris;
@@ -428,7 +535,10 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
 
   // Stack: one pending entry per internal ancestor at most.
   const uint32_t max_stack = max_depth + 1;
-  if ((rc = reserve_stack(ctx, max_stack))) return rc;
+  // a quad pushes up to three children per 4-wide level, and runs binary traversals for rays with a
+  // zero direction component
+  const uint32_t max_stack4 = std::max(max_stack, 3 * depth4 + 1);
+  if ((rc = reserve_stack(ctx, max_stack4))) return rc;
 
   mbvh_mesh* m = new (std::nothrow) mbvh_mesh();
   if (!m) return set_error(MBVH_ERR_OOM, "mbvh_mesh_upload: out of host memory");
@@ -440,6 +550,7 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
   m->face_limit = face_limit;
   m->max_depth = max_depth;
   m->max_stack = max_stack;
+  m->max_stack4 = max_stack4;
   m->coord_max = coord_max;
 
   auto upload = [&](void** dst, const void* src, size_t bytes) -> int {
@@ -459,6 +570,8 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_bigleaf), bigleaf.data(), bigleaf.size() * sizeof(uint2));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_tris), tris.data(), tris.size() * sizeof(float4));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_vidx), vidx.data(), vidx.size() * sizeof(uint32_t));
+  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_nodes4), nodes4.data(), nodes4.size() * sizeof(float4));
+  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_roots4), roots4.data(), roots4.size() * sizeof(uint32_t));
   if (!rc && indirect) {
     std::vector<uint32_t> face_slot(face_limit, 0xFFFFFFFFu);
     for (uint64_t sl = 0; sl < n_slots; sl++) face_slot[tris_face(tris, sl)] = static_cast<uint32_t>(sl);
@@ -525,12 +638,19 @@ int raycast_first_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o, con
   if (side < MBVH_FRONT_SIDE || side > MBVH_SIDE_PER_TRIANGLE)
     return set_error(MBVH_ERR_BAD_ARGS, "raycast_first: invalid side %d", side);
   if (n && (!o || !d || !face)) return set_error(MBVH_ERR_BAD_ARGS, "raycast_first: NULL ray or face buffer");
-  if (mesh->max_stack > kStackLds + ctx->stack_ovf_entries)
+  // four lanes per ray (4-wide tree) when all rays fit in one resident round: a small launch is
+  // latency-bound and gains from shorter paths and more waves; big launches keep one lane per ray
+  const bool quads = ctx->ray_lanes == 4 || (ctx->ray_lanes == 0 && 4 * n <= ctx->max_threads);
+  if (quads) {
+    if (mesh->max_stack4 > 16 + ctx->stack_ovf_entries)
+      return set_error(MBVH_ERR_LIMIT, "raycast_first: stack scratch smaller than the tree depth");
+  } else if (mesh->max_stack > kStackLds + ctx->stack_ovf_entries) {
     return set_error(MBVH_ERR_LIMIT, "raycast_first: stack scratch smaller than the tree depth");
-  const uint64_t threads = launch_threads(n, ctx->max_threads);
+  }
+  const uint64_t threads = quads ? launch_threads(4 * n, ctx->max_threads) : launch_threads(n, ctx->max_threads);
   RaycastFirstArgs a;
   a.mesh = view_of(mesh);
-  a.stack = stack_of(ctx, threads);
+  a.stack = stack_of(ctx, quads ? threads / 4 : threads);
   a.origins = o;
   a.directions = d;
   a.order = order;
@@ -542,7 +662,10 @@ int raycast_first_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o, con
   a.out_dist = dist;
   a.out_point = point;
   a.out_bary = bary;
-  MBVH_HIP(launch_raycast_first(a, threads, s));
+  if (quads)
+    MBVH_HIP(launch_raycast_first4(a, threads / 4, s));
+  else
+    MBVH_HIP(launch_raycast_first(a, threads, s));
   return MBVH_OK;
 }
 
@@ -1094,10 +1217,10 @@ uint64_t batch_ws_bytes(const mbvh_batch* b, uint64_t n) {
 }
 
 int check_tuning(const char* fn, const mbvh_batch_tuning& t) {
-  if ((t.stack_lds != 8 && t.stack_lds != 16) || t.tile_rays == 0 || t.tile_rays % 64 != 0 || t.tile_rays > 4096 ||
-      t.blocks_per_cu < 1 || t.blocks_per_cu > 8)
-    return set_error(MBVH_ERR_BAD_ARGS, "%s: invalid tuning (stack_lds 8|16, tile_rays 64..4096 step 64, "
-                     "blocks_per_cu 1..8)", fn);
+  if ((t.ray_lanes != 1 && t.ray_lanes != 4) || (t.stack_lds != 8 && t.stack_lds != 16) || t.tile_rays == 0 ||
+      t.tile_rays % 64 != 0 || t.tile_rays > 4096 || t.blocks_per_cu < 1 || t.blocks_per_cu > 8)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: invalid tuning (ray_lanes 1|4, stack_lds 8|16, tile_rays 64..4096 "
+                     "step 64, blocks_per_cu 1..8)", fn);
   return MBVH_OK;
 }
 
@@ -1145,6 +1268,8 @@ int enqueue_batch(mbvh_ctx* ctx, mbvh_batch* b, const BatchCall& c, const mbvh_b
   const uint64_t tiles = (c.n + t.tile_rays - 1) / t.tile_rays;
   uint64_t blocks = static_cast<uint64_t>(device_cus) * t.blocks_per_cu;
   const uint64_t useful = (tiles + kBlock / 64 - 1) / (kBlock / 64);  // one wave per tile at most
+  if (t.ray_lanes == 4 && b->max_stack4 > 16 + ctx->stack_ovf_entries)
+    return set_error(MBVH_ERR_LIMIT, "batch: stack scratch smaller than the tree depth");
   if (blocks > useful) blocks = std::max<uint64_t>(useful, 1);
   BatchArgs a{};
   a.meshes = b->d_meshes;
@@ -1168,8 +1293,8 @@ int enqueue_batch(mbvh_ctx* ctx, mbvh_batch* b, const BatchCall& c, const mbvh_b
   a.counter = b->d_counter;
   a.tile = t.tile_rays;
   a.n_tiles = tiles;
-  a.stack = stack_of(ctx, blocks * kBlock);
-  MBVH_HIP(launch_batch_raycast_first(a, t.stack_lds, static_cast<uint32_t>(blocks), s));
+  a.stack = stack_of(ctx, t.ray_lanes == 4 ? blocks * kBlock / 4 : blocks * kBlock);
+  MBVH_HIP(launch_batch_raycast_first(a, t.stack_lds, t.ray_lanes, static_cast<uint32_t>(blocks), s));
   return MBVH_OK;
 }
 
@@ -1183,22 +1308,24 @@ extern "C" int mbvh_batch_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, 
   if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_batch_create: invalid context handle");
   if (n_meshes == 0 || !meshes) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_batch_create: no meshes");
   std::vector<MeshView> views(n_meshes);
-  uint32_t max_stack = 0;
+  uint32_t max_stack = 0, max_stack4 = 0;
   for (uint32_t i = 0; i < n_meshes; i++) {
     if (!valid_mesh(meshes[i]) || meshes[i]->ctx != ctx)
       return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_batch_create: mesh %u is not a mesh of this context", i);
     views[i] = view_of(meshes[i]);
     max_stack = std::max(max_stack, meshes[i]->max_stack);
+    max_stack4 = std::max(max_stack4, meshes[i]->max_stack4);
   }
   std::lock_guard<std::mutex> guard(ctx->lock);
   int rc = set_device(ctx);
   if (rc) return rc;
-  if ((rc = reserve_stack(ctx, max_stack))) return rc;
+  if ((rc = reserve_stack(ctx, std::max(max_stack, max_stack4)))) return rc;
   mbvh_batch* b = new (std::nothrow) mbvh_batch();
   if (!b) return set_error(MBVH_ERR_OOM, "mbvh_batch_create: out of host memory");
   b->ctx = ctx;
   b->n_meshes = n_meshes;
   b->max_stack = max_stack;
+  b->max_stack4 = max_stack4;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&b->d_meshes), views.size() * sizeof(MeshView));
   if (e == hipSuccess) e = hipMemcpy(b->d_meshes, views.data(), views.size() * sizeof(MeshView), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_counter), 256);
@@ -1273,14 +1400,17 @@ extern "C" int mbvh_batch_autotune(mbvh_ctx* ctx, mbvh_batch* batch, const float
   hipEvent_t e0, e1;
   MBVH_HIP(hipEventCreate(&e0));
   MBVH_HIP(hipEventCreate(&e1));
-  mbvh_batch_tuning best{16, 256, 4, 3.4e38f};
-  static const uint32_t kStacks[] = {8, 16};
+  mbvh_batch_tuning best{4, 16, 256, 4, 3.4e38f};
+  struct Shape {
+    uint32_t lanes, stack;
+  };
+  static const Shape kShapes[] = {{1, 8}, {1, 16}, {4, 16}};
   static const uint32_t kTiles[] = {64, 128, 256, 512};
   static const uint32_t kBlocks[] = {1, 2, 4, 8};
-  for (uint32_t sl : kStacks)
+  for (const Shape& sh : kShapes)
     for (uint32_t tl : kTiles)
       for (uint32_t bl : kBlocks) {
-        const mbvh_batch_tuning t{sl, tl, bl, 0.0f};
+        const mbvh_batch_tuning t{sh.lanes, sh.stack, tl, bl, 0.0f};
         if ((rc = enqueue_batch(ctx, batch, c, t, s))) break;  // warm-up
         float total = 0.0f;
         for (int rep = 0; rep < 3 && !rc; rep++) {
@@ -1294,7 +1424,7 @@ extern "C" int mbvh_batch_autotune(mbvh_ctx* ctx, mbvh_batch* batch, const float
           total += ms;
         }
         if (rc) break;
-        if (total / 3.0f < best.ms) best = mbvh_batch_tuning{sl, tl, bl, total / 3.0f};
+        if (total / 3.0f < best.ms) best = mbvh_batch_tuning{sh.lanes, sh.stack, tl, bl, total / 3.0f};
       }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -11,6 +11,9 @@ namespace mbvh {
 struct MeshView {
   const WideNode* nodes;
   const uint32_t* roots;
+  const float4* nodes4;   // 4-wide tree: 8 float4 per node (4 children x {min.xyz max.x}{max.yz ref 0})
+  const uint32_t* roots4; // 4-wide root reference per root
+  uint32_t max_stack4;    // stack entries a quad traversal can need (4-wide or binary)
   const uint2* bigleaf;
   const float4* tris;  // 3 per slot
   uint32_t n_roots;
@@ -123,8 +126,10 @@ struct BatchArgs {
 
 hipError_t launch_instance_rays(const InstanceRayArgs& a, hipStream_t s);
 hipError_t launch_instance_hits(const InstanceHitArgs& a, hipStream_t s);
-// stack_lds: 8 or 16 LDS stack entries per lane; blocks: persistent grid size
-hipError_t launch_batch_raycast_first(const BatchArgs& a, uint32_t stack_lds, uint32_t blocks, hipStream_t s);
+// lanes 1: one lane per ray (stack_lds 8 or 16 LDS stack entries per lane); lanes 4: four lanes
+// per ray on the 4-wide trees; blocks: persistent grid size
+hipError_t launch_batch_raycast_first(const BatchArgs& a, uint32_t stack_lds, uint32_t lanes, uint32_t blocks,
+                                      hipStream_t s);
 // Visit order grouping rays by the mesh of their instance (counting sort over n_meshes keys);
 // workspace of order_by_mesh_workspace_bytes(n, n_meshes) bytes.
 uint64_t order_by_mesh_workspace_bytes(uint64_t n, uint32_t n_meshes);
@@ -193,6 +198,8 @@ hipError_t launch_closest_point_bwd(const ClosestBwdArgs& a, int32_t mode, float
 uint64_t launch_threads(uint64_t n, uint64_t max_threads);
 
 hipError_t launch_raycast_first(const RaycastFirstArgs& a, uint64_t threads, hipStream_t s);
+// four lanes per ray on the 4-wide tree; `quads` = rays traced at once (a multiple of 64)
+hipError_t launch_raycast_first4(const RaycastFirstArgs& a, uint64_t quads, hipStream_t s);
 hipError_t launch_closest_point(const ClosestPointArgs& a, uint64_t threads, hipStream_t s);
 // fill == false: count hits per ray into a.counts; fill == true: write them at a.offsets.
 hipError_t launch_raycast_all(const RaycastAllArgs& a, bool fill, uint64_t threads, hipStream_t s);

@@ -464,7 +464,249 @@ __device__ __forceinline__ FirstHit trace_first(const MeshView& M, StackT& st, u
   return h;
 }
 
-__global__ __launch_bounds__(kBlock) void k_raycast_first(RaycastFirstArgs A) {
+// ------------------------------------------------------------------------------------------
+// First hit with four lanes per ray on the 4-wide tree (a "quad" of lanes traces one ray).
+// Node step: lane q tests child q of the current node; the quad exchanges (entry t, ref), sorts
+// the hit children by t, continues with the nearest and pushes the others (the pushed t is
+// re-checked against the best bound on pop). Leaf step: lane q tests triangles q, q+4, ... with
+// its own candidate list; the bound is shared after each leaf. After traversal each lane
+// resolves its candidates exactly and the quad keeps the smallest exact t. Any visit order gives
+// the nearest hit; exactly equal hits may resolve to a different face than the reference's order,
+// which ties allow. Rays with a zero direction component need the reference's split-plane
+// shortcut literally, which is defined on the binary tree: they take the binary traversal, run
+// identically by the four lanes.
+// Sixteen rays per wave instead of 64, and four times the waves for a given ray count: a launch
+// of few rays (latency-bound) gets more waves to overlap and shorter paths per wave.
+// ------------------------------------------------------------------------------------------
+constexpr int kQuadLds = 16;  // LDS stack entries per quad
+
+// Stack shared by the four lanes of a quad: every lane pushes and pops the same entries.
+struct QuadStack {
+  uint2* lds;
+  uint2* ovf;
+  uint64_t ovf_stride;  // quads in the launch
+  uint64_t gquad;
+  uint32_t qid;         // quad within the block
+  int capacity;
+  int sp;
+
+  __device__ __forceinline__ bool push(uint32_t ref, uint32_t aux) {
+    if (sp >= capacity) return false;
+    if (sp < kQuadLds)
+      lds[sp * (kBlock / 4) + qid] = make_uint2(ref, aux);
+    else
+      ovf[static_cast<uint64_t>(sp - kQuadLds) * ovf_stride + gquad] = make_uint2(ref, aux);
+    sp++;
+    return true;
+  }
+  __device__ __forceinline__ uint2 pop() {
+    sp--;
+    if (sp < kQuadLds) return lds[sp * (kBlock / 4) + qid];
+    return ovf[static_cast<uint64_t>(sp - kQuadLds) * ovf_stride + gquad];
+  }
+};
+
+__device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
+  if (tb < ta) {
+    const float t = ta;
+    ta = tb;
+    tb = t;
+    const uint32_t r = ra;
+    ra = rb;
+    rb = r;
+  }
+}
+
+__device__ __forceinline__ FirstHit trace_first4(const MeshView& M, QuadStack& st, uint32_t lane, f3 o, f3 d,
+                                                 uint32_t side_mask, float near_dist, float far_dist) {
+  const uint32_t q = lane & 3u;
+  RayBox rb;
+  rb.ox = o.x;
+  rb.oy = o.y;
+  rb.oz = o.z;
+  rb.ix = 1.0f / d.x;
+  rb.iy = 1.0f / d.y;
+  rb.iz = 1.0f / d.z;
+  rb.sx = rb.ix >= 0.0f;
+  rb.sy = rb.iy >= 0.0f;
+  rb.sz = rb.iz >= 0.0f;
+  const float dlen = sqrtf(dot(d, d));
+  const float dl1 = fabsf(d.x) + fabsf(d.y) + fabsf(d.z);
+  float U = kInf;  // quad-wide upper bound of the best hit's t (after each leaf)
+  bool overflow = false;
+  Cands cl;
+  cl.lane = lane;
+  cl.n = 0;
+
+  for (uint32_t r = 0; r < M.n_roots && !overflow; r++) {
+    uint32_t ref = M.roots4[r];
+    st.sp = 0;
+    while (true) {
+      // ---- internal nodes (quad-uniform) ----
+      while (!(ref & kRefLeaf) && ref != kDone) {
+        const float4 c0 = M.nodes4[8ull * ref + 2 * q];
+        const float4 c1 = M.nodes4[8ull * ref + 2 * q + 1];
+        const uint32_t cref = __float_as_uint(c1.z);
+        float t;
+        const bool hit = cref != kDone && slab(rb, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, near_dist, far_dist, t) &&
+                         t <= U;
+        const float tq = hit ? t : kInf;
+        const uint32_t rq = hit ? cref : kDone;
+        float t0 = __shfl(tq, 0, 4), t1 = __shfl(tq, 1, 4), t2 = __shfl(tq, 2, 4), t3 = __shfl(tq, 3, 4);
+        uint32_t r0 = __shfl(rq, 0, 4), r1 = __shfl(rq, 1, 4), r2 = __shfl(rq, 2, 4), r3 = __shfl(rq, 3, 4);
+        // misses carry t = inf and sort last; among hits, nearest first
+        cswap(t0, r0, t1, r1);
+        cswap(t2, r2, t3, r3);
+        cswap(t0, r0, t2, r2);
+        cswap(t1, r1, t3, r3);
+        cswap(t1, r1, t2, r2);
+        if (r0 == kDone) {
+          ref = kDone;
+          while (st.sp > 0) {
+            const uint2 e = st.pop();
+            if (__uint_as_float(e.y) <= U) {
+              ref = e.x;
+              break;
+            }
+          }
+        } else {
+          bool ok = true;
+          if (r3 != kDone) ok = ok && st.push(r3, __float_as_uint(t3));
+          if (r2 != kDone) ok = ok && st.push(r2, __float_as_uint(t2));
+          if (r1 != kDone) ok = ok && st.push(r1, __float_as_uint(t1));
+          if (!ok) {
+            overflow = true;
+            ref = kDone;
+            break;
+          }
+          ref = r0;
+        }
+      }
+      // ---- leaves: the quad splits the triangles ----
+      while (ref & kRefLeaf) {
+        uint32_t off, cnt;
+        leaf_range(ref, M.bigleaf, off, cnt);
+        float Ul = U;
+        for (uint32_t s = off + q; s < off + cnt; s += 4) {
+          const Tri tr = load_tri(M.tris, s);
+          const uint32_t mask = side_mask ? side_mask : tr.mask();
+          float t = 0.0f, et = 0.0f;
+          const int res = hit_fast(o, d, dl1, dlen, tr, mask, near_dist, far_dist, t, et);
+          if (res == kMiss) continue;
+          if (res == kHit) {
+            if (t - et > Ul) continue;  // certainly behind a known hit
+            Ul = fminf(Ul, t + et);
+          }
+          if (cl.n == kCand && !cl.compact(Ul)) {
+            ff bt;
+            const uint32_t w = resolve_ray(cl, Ul, o, d, dlen, M.tris, side_mask, near_dist, far_dist, bt);
+            cl.n = 0;
+            if (w != kNoHit) {
+              const float tw = ff_to_float(bt);
+              Ul = fminf(Ul, tw * (1.0f + 4.0f * kU));
+              cl.put(0, w, tw * (1.0f - 4.0f * kU));
+              cl.n = 1;
+            }
+          }
+          cl.put(cl.n++, s, res == kHit ? t - et : -kInf);
+        }
+        Ul = fminf(Ul, __shfl_xor(Ul, 1, 4));
+        U = fminf(Ul, __shfl_xor(Ul, 2, 4));
+        ref = kDone;
+        while (st.sp > 0) {
+          const uint2 e = st.pop();
+          if (__uint_as_float(e.y) <= U) {
+            ref = e.x;
+            break;
+          }
+        }
+      }
+      if (ref == kDone) break;
+    }
+  }
+
+  FirstHit h;
+  h.face = kNoHit;
+  h.dist = kInf;
+  const float qnan = __int_as_float(0x7fc00000);
+  h.p = mk(qnan, qnan, qnan);
+  h.bary = h.p;
+  if (overflow) {
+    h.face = kOverflow;
+    return h;
+  }
+  ff bt = {kInf, 0.0f};
+  uint32_t w = kNoHit;
+  if (cl.n > 0) {
+    w = resolve_ray(cl, U, o, d, dlen, M.tris, side_mask, near_dist, far_dist, bt);
+    if (w == kNoHit) bt = {kInf, 0.0f};
+  }
+  // smallest exact t over the quad (then the smaller slot)
+  for (int m = 1; m <= 2; m <<= 1) {
+    const float oh = __shfl_xor(bt.hi, m, 4), ol = __shfl_xor(bt.lo, m, 4);
+    const uint32_t ow = __shfl_xor(w, m, 4);
+    if (oh < bt.hi || (oh == bt.hi && (ol < bt.lo || (ol == bt.lo && ow < w)))) {
+      bt = {oh, ol};
+      w = ow;
+    }
+  }
+  if (w != kNoHit) {
+    const Tri tr = load_tri(M.tris, w);
+    const TriTerms r = tri_terms_exact(o, d, tr.a(), tr.b(), tr.c());
+    const ff t = ff_div(r.qdn, r.ddn_abs);
+    const ff u = ff_div(r.e0, r.ddn_abs);
+    const ff v = ff_div(r.e1, r.ddn_abs);
+    const ff wgt = ff_sub(ff_sub(ff_from(1.0f), u), v);
+    const ff dl = ff_sqrt(ff_add(ff_add(two_prod(d.x, d.x), two_prod(d.y, d.y)), two_prod(d.z, d.z)));
+    h.face = tr.face();
+    h.dist = ff_to_float(ff_mul(t, dl));
+    h.p = mk(ff_to_float(ff_add(ff_from(o.x), ff_mulf(t, d.x))), ff_to_float(ff_add(ff_from(o.y), ff_mulf(t, d.y))),
+             ff_to_float(ff_add(ff_from(o.z), ff_mulf(t, d.z))));
+    h.bary = mk(ff_to_float(wgt), ff_to_float(u), ff_to_float(v));
+  }
+  return h;
+}
+
+__global__ __launch_bounds__(kBlock, 4) void k_raycast_first4(RaycastFirstArgs A) {
+  __shared__ uint2 qstack[kQuadLds * (kBlock / 4)];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t qid = lane >> 2;
+  const uint64_t gquad = static_cast<uint64_t>(blockIdx.x) * (kBlock / 4) + qid;
+  const uint64_t nquads = static_cast<uint64_t>(gridDim.x) * (kBlock / 4);
+  QuadStack st;
+  st.lds = qstack;
+  st.ovf = A.stack.ovf;
+  st.ovf_stride = nquads;
+  st.gquad = gquad;
+  st.qid = qid;
+  st.capacity = kQuadLds + static_cast<int>(A.stack.ovf_entries);
+  for (uint64_t visit = gquad; visit < A.n; visit += nquads) {
+    const uint64_t ray = A.order ? A.order[visit] : visit;
+    if (ray >= A.n) continue;
+    const f3 o = mk(A.origins[3 * ray + 0], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
+    const f3 d = mk(A.directions[3 * ray + 0], A.directions[3 * ray + 1], A.directions[3 * ray + 2]);
+    const bool literal = (d.x == 0.0f) || (d.y == 0.0f) || (d.z == 0.0f);
+    const FirstHit h = literal ? trace_first(A.mesh, st, lane, o, d, A.side_mask, A.near_dist, A.far_dist)
+                               : trace_first4(A.mesh, st, lane, o, d, A.side_mask, A.near_dist, A.far_dist);
+    if ((lane & 3u) == 0) {
+      A.out_face[ray] = h.face;
+      if (A.out_dist) A.out_dist[ray] = h.dist;
+      if (A.out_point) {
+        A.out_point[3 * ray + 0] = h.p.x;
+        A.out_point[3 * ray + 1] = h.p.y;
+        A.out_point[3 * ray + 2] = h.p.z;
+      }
+      if (A.out_bary) {
+        A.out_bary[3 * ray + 0] = h.bary.x;
+        A.out_bary[3 * ray + 1] = h.bary.y;
+        A.out_bary[3 * ray + 2] = h.bary.z;
+      }
+    }
+  }
+}
+
+// (kBlock, 4): at most 128 VGPRs, so four waves fit on a SIMD (the LDS use allows four blocks per CU)
+__global__ __launch_bounds__(kBlock, 4) void k_raycast_first(RaycastFirstArgs A) {
   const uint32_t lane = threadIdx.x;
   const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + lane;
   const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * kBlock;
@@ -576,7 +818,7 @@ __global__ void k_batch_reset(uint32_t* counter) {
 // ray into its instance's frame, traces it against that instance's mesh and converts the hit to
 // world space.
 template <int kL>
-__global__ __launch_bounds__(kBlock) void k_batch_raycast_first(BatchArgs A) {
+__global__ __launch_bounds__(kBlock, 4) void k_batch_raycast_first(BatchArgs A) {
   __shared__ uint2 lds_stack[kL * kBlock];
   const uint32_t tid = threadIdx.x, lane64 = tid & 63u;
   const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + tid;
@@ -606,17 +848,18 @@ __global__ __launch_bounds__(kBlock) void k_batch_raycast_first(BatchArgs A) {
       h.dist = kInf;
       h.p = mk(qnan, qnan, qnan);
       h.bary = h.p;
-      const f3 ow = mk(A.origins[3 * ray], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
       if (geom < A.n_meshes) {
+        const f3 ow = mk(A.origins[3 * ray], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
         const float* inv = A.inverse + 16ull * inst;
-        const float* wld = A.world + 16ull * inst;
         const f3 o = apply_matrix4(inv, ow);
         const f3 d = transform_direction(inv, mk(A.directions[3 * ray], A.directions[3 * ray + 1], A.directions[3 * ray + 2]));
-        const float sc = instance_scale(wld, d);
+        const float sc = instance_scale(A.world + 16ull * inst, d);
         h = trace_first(A.meshes[geom], st, tid, o, d, A.side_mask, A.near_dist / sc, A.far_dist / sc);
         if (h.face < kOverflow) {
-          h.p = apply_matrix4(wld, h.p);
-          h.dist = world_distance(h.p, ow);
+          // matrices and the world origin are re-read here rather than kept live across the trace
+          const f3 ow2 = mk(A.origins[3 * ray], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
+          h.p = apply_matrix4(A.world + 16ull * inst, h.p);
+          h.dist = world_distance(h.p, ow2);
         }
       }
       A.out_face[ray] = h.face;
@@ -630,6 +873,72 @@ __global__ __launch_bounds__(kBlock) void k_batch_raycast_first(BatchArgs A) {
         A.out_bary[3 * ray + 0] = h.bary.x;
         A.out_bary[3 * ray + 1] = h.bary.y;
         A.out_bary[3 * ray + 2] = h.bary.z;
+      }
+    }
+  }
+}
+
+// Same work queue with four lanes per ray (quads, the 4-wide trees): a wave traces 16 rays at a
+// time; rays with a zero direction component take the binary traversal (see trace_first4).
+__global__ __launch_bounds__(kBlock, 4) void k_batch_raycast_first4(BatchArgs A) {
+  __shared__ uint2 qstack[kQuadLds * (kBlock / 4)];
+  const uint32_t tid = threadIdx.x, lane64 = tid & 63u, q = tid & 3u, wq = lane64 >> 2;
+  const uint32_t qid = tid >> 2;
+  QuadStack st;
+  st.lds = qstack;
+  st.ovf = A.stack.ovf;
+  st.ovf_stride = static_cast<uint64_t>(gridDim.x) * (kBlock / 4);
+  st.gquad = static_cast<uint64_t>(blockIdx.x) * (kBlock / 4) + qid;
+  st.qid = qid;
+  st.capacity = kQuadLds + static_cast<int>(A.stack.ovf_entries);
+  const float qnan = __int_as_float(0x7fc00000);
+  while (true) {
+    uint32_t t = 0;
+    if (lane64 == 0) t = atomicAdd(A.counter, 1u);
+    t = __shfl(t, 0, 64);
+    if (t >= A.n_tiles) break;  // every wave leaves once the counter passes the last tile
+    const uint64_t base = static_cast<uint64_t>(t) * A.tile;
+    for (uint32_t k = wq; k < A.tile; k += 16) {
+      const uint64_t visit = base + k;
+      if (visit >= A.n) break;
+      const uint64_t ray = A.order ? A.order[visit] : visit;
+      if (ray >= A.n) continue;
+      const uint32_t inst = A.ray_instance[ray];
+      const uint32_t geom = inst < A.n_instances ? A.instance_geometry[inst] : 0xFFFFFFFFu;
+      FirstHit h;
+      h.face = kNoHit;
+      h.dist = kInf;
+      h.p = mk(qnan, qnan, qnan);
+      h.bary = h.p;
+      if (geom < A.n_meshes) {
+        const f3 ow = mk(A.origins[3 * ray], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
+        const float* inv = A.inverse + 16ull * inst;
+        const f3 o = apply_matrix4(inv, ow);
+        const f3 d = transform_direction(inv, mk(A.directions[3 * ray], A.directions[3 * ray + 1], A.directions[3 * ray + 2]));
+        const float sc = instance_scale(A.world + 16ull * inst, d);
+        const bool literal = (d.x == 0.0f) || (d.y == 0.0f) || (d.z == 0.0f);
+        const MeshView& M = A.meshes[geom];
+        h = literal ? trace_first(M, st, tid, o, d, A.side_mask, A.near_dist / sc, A.far_dist / sc)
+                    : trace_first4(M, st, tid, o, d, A.side_mask, A.near_dist / sc, A.far_dist / sc);
+        if (h.face < kOverflow) {
+          const f3 ow2 = mk(A.origins[3 * ray], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
+          h.p = apply_matrix4(A.world + 16ull * inst, h.p);
+          h.dist = world_distance(h.p, ow2);
+        }
+      }
+      if (q == 0) {
+        A.out_face[ray] = h.face;
+        if (A.out_dist) A.out_dist[ray] = h.dist;
+        if (A.out_point) {
+          A.out_point[3 * ray + 0] = h.p.x;
+          A.out_point[3 * ray + 1] = h.p.y;
+          A.out_point[3 * ray + 2] = h.p.z;
+        }
+        if (A.out_bary) {
+          A.out_bary[3 * ray + 0] = h.bary.x;
+          A.out_bary[3 * ray + 1] = h.bary.y;
+          A.out_bary[3 * ray + 2] = h.bary.z;
+        }
       }
     }
   }
@@ -1011,6 +1320,12 @@ uint64_t launch_threads(uint64_t n, uint64_t max_threads) {
   return t;
 }
 
+hipError_t launch_raycast_first4(const RaycastFirstArgs& a, uint64_t quads, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_raycast_first4, dim3(static_cast<uint32_t>(quads / (kBlock / 4))), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_raycast_first(const RaycastFirstArgs& a, uint64_t threads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_raycast_first, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
@@ -1029,10 +1344,13 @@ hipError_t launch_instance_hits(const InstanceHitArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_batch_raycast_first(const BatchArgs& a, uint32_t stack_lds, uint32_t blocks, hipStream_t s) {
+hipError_t launch_batch_raycast_first(const BatchArgs& a, uint32_t stack_lds, uint32_t lanes, uint32_t blocks,
+                                      hipStream_t s) {
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.counter);
   if (a.n == 0) return hipGetLastError();
-  if (stack_lds == 8)
+  if (lanes == 4)
+    hipLaunchKernelGGL(k_batch_raycast_first4, dim3(blocks), dim3(kBlock), 0, s, a);
+  else if (stack_lds == 8)
     hipLaunchKernelGGL(k_batch_raycast_first<8>, dim3(blocks), dim3(kBlock), 0, s, a);
   else
     hipLaunchKernelGGL(k_batch_raycast_first<16>, dim3(blocks), dim3(kBlock), 0, s, a);

@@ -28,7 +28,7 @@ EXPORTED_SYMBOLS = (
     "mbvh_abi_version", "mbvh_last_error", "mbvh_device_count",
     "mbvh_build", "mbvh_tree_info", "mbvh_tree_root_bytes", "mbvh_tree_copy_root",
     "mbvh_tree_copy_indirect", "mbvh_tree_free",
-    "mbvh_ctx_create", "mbvh_ctx_destroy", "mbvh_ctx_synchronize",
+    "mbvh_ctx_create", "mbvh_ctx_destroy", "mbvh_ctx_synchronize", "mbvh_ctx_set_ray_lanes",
     "mbvh_mesh_upload", "mbvh_mesh_free", "mbvh_mesh_info", "mbvh_mesh_set_side_mask",
     "mbvh_raycast_first", "mbvh_raycast_first_d",
     "mbvh_closest_point", "mbvh_closest_point_d",
@@ -57,9 +57,10 @@ _lib = None
 
 
 class BatchTuning(C.Structure):
-    """mbvh_batch_tuning: LDS stack entries (8|16), rays per work item, workgroups per CU, measured ms."""
-    _fields_ = [("stack_lds", C.c_uint32), ("tile_rays", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("ms", C.c_float)]
+    """mbvh_batch_tuning: lanes per ray (1|4), LDS stack entries (8|16), rays per work item,
+    workgroups per CU, measured ms."""
+    _fields_ = [("ray_lanes", C.c_uint32), ("stack_lds", C.c_uint32), ("tile_rays", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32), ("ms", C.c_float)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -87,6 +88,7 @@ def library() -> C.CDLL:
         "mbvh_ctx_create": (C.c_int, [i32, C.POINTER(vp)]),
         "mbvh_ctx_destroy": (None, [vp]),
         "mbvh_ctx_synchronize": (C.c_int, [vp]),
+        "mbvh_ctx_set_ray_lanes": (C.c_int, [vp, i32]),
         "mbvh_mesh_upload": (C.c_int, [vp, C.POINTER(vp), C.POINTER(u64), u32, vp, i32, u64, vp, i32, u64,
                                        vp, u64, u32, u32, C.POINTER(vp)]),
         "mbvh_mesh_free": (None, [vp]),
@@ -216,6 +218,10 @@ class Context:
 
     def synchronize(self) -> None:
         _check(self._lib.mbvh_ctx_synchronize(self.handle))
+
+    def set_ray_lanes(self, lanes: int) -> None:
+        """raycastFirst lanes per ray: 0 automatic, 1 (binary tree) or 4 (4-wide tree)."""
+        _check(self._lib.mbvh_ctx_set_ray_lanes(self.handle, lanes))
 
     def upload(self, tree: Tree, positions: np.ndarray, index: Optional[np.ndarray],
                stride: int = 3, offset: int = 0) -> "Mesh":
