@@ -116,6 +116,10 @@ void mbvh_ctx_destroy(mbvh_ctx* ctx);
  * tree built at upload, or 0 (default): 4 lanes when all rays of a call fit in one resident round
  * of the device, else 1. Results are the same up to exactly equal hits (ties). */
 int mbvh_ctx_set_ray_lanes(mbvh_ctx* ctx, int32_t lanes);
+/* closestPoint traversal width: 1 lane per query on the binary tree, or 4 lanes per query on the
+ * 4-wide tree; 0 (default) = 4. Calls with a nonzero minThreshold always use 1 lane: the
+ * reference returns the first triangle under minThreshold in its own (binary) visit order. */
+int mbvh_ctx_set_point_lanes(mbvh_ctx* ctx, int32_t lanes);
 /* Waits for all work the context enqueued on its own stream. */
 int mbvh_ctx_synchronize(mbvh_ctx* ctx);
 

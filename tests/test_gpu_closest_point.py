@@ -153,3 +153,20 @@ def test_query_order_changes_nothing(ctx):
     assert np.array_equal(host["face"], outs[0][0])
     assert np.array_equal(host["distance"].view(np.uint32), outs[0][1])
     mesh.free()
+
+
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_lanes_per_query(lanes):
+    """One lane per query (binary tree) and four lanes per query (4-wide tree) both meet the parity
+    rule on the C2 mesh; with a minThreshold the call keeps one lane (the reference's early exit
+    follows its own visit order)."""
+    c = M.Context(0)
+    c.set_point_lanes(lanes)
+    try:
+        w = W.c2_closest(res=48)
+        _run(c, w.geometry, w.points, label=f"C2_48 lanes={lanes}")
+        lo, hi = w.geometry.bounds()
+        pts = W.grid_points(lo, hi, 16)
+        _run(c, w.geometry, pts, min_t=0.5, label=f"minThreshold lanes={lanes}")
+    finally:
+        c.close()

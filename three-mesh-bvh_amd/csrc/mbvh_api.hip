@@ -33,6 +33,7 @@ struct mbvh_ctx {
   DeviceBuffer stack_ovf;       // traversal stack overflow region
   DeviceBuffer queue;           // work-queue counter of the single-mesh query kernels
   int32_t ray_lanes = 0;        // raycastFirst lanes per ray: 0 automatic, 1, or 4
+  int32_t point_lanes = 0;      // closestPoint lanes per query: 0 automatic, 1, or 4
   uint32_t stack_ovf_entries = 0;
   DeviceBuffer staging[8];      // host-API input / output staging
   DeviceBuffer order;           // host-API query order
@@ -433,6 +434,15 @@ extern "C" int mbvh_ctx_set_ray_lanes(mbvh_ctx* ctx, int32_t lanes) {
   return MBVH_OK;
 }
 
+extern "C" int mbvh_ctx_set_point_lanes(mbvh_ctx* ctx, int32_t lanes) {
+  clear_error();
+  if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_ctx_set_point_lanes: invalid context handle");
+  if (lanes != 0 && lanes != 1 && lanes != 4)
+    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_ctx_set_point_lanes: lanes must be 0 (automatic), 1 or 4");
+  ctx->point_lanes = lanes;
+  return MBVH_OK;
+}
+
 extern "C" int mbvh_ctx_synchronize(mbvh_ctx* ctx) {
   if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_ctx_synchronize: invalid context handle");
   int rc = set_device(ctx);
@@ -673,12 +683,14 @@ int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uin
                        const uint32_t* order, float min_t,
                        float max_t, uint32_t* face, float* dist, float* point, float* bary, hipStream_t s) {
   if (n && (!q || !face)) return set_error(MBVH_ERR_BAD_ARGS, "closest_point: NULL point or face buffer");
-  if (mesh->max_stack > kStackLds + ctx->stack_ovf_entries)
+  // four lanes per query only without a minThreshold (its early exit follows the binary order)
+  const bool quads = min_t == 0.0f && (ctx->point_lanes == 4 || ctx->point_lanes == 0);
+  if (quads ? mesh->max_stack4 > 16 + ctx->stack_ovf_entries : mesh->max_stack > kStackLds + ctx->stack_ovf_entries)
     return set_error(MBVH_ERR_LIMIT, "closest_point: stack scratch smaller than the tree depth");
-  const uint64_t threads = launch_threads(n, ctx->max_threads);
+  const uint64_t threads = quads ? launch_threads(4 * n, ctx->max_threads) : launch_threads(n, ctx->max_threads);
   ClosestPointArgs a;
   a.mesh = view_of(mesh);
-  a.stack = stack_of(ctx, threads);
+  a.stack = stack_of(ctx, quads ? threads / 4 : threads);
   a.points = q;
   a.order = order;
   a.queue = static_cast<uint32_t*>(ctx->queue.ptr);
@@ -689,7 +701,10 @@ int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uin
   a.out_dist = dist;
   a.out_point = point;
   a.out_bary = bary;
-  MBVH_HIP(launch_closest_point(a, threads, s));
+  if (quads)
+    MBVH_HIP(launch_closest_point4(a, threads, s));
+  else
+    MBVH_HIP(launch_closest_point(a, threads, s));
   return MBVH_OK;
 }
 

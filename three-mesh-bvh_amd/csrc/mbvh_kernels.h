@@ -201,6 +201,8 @@ hipError_t launch_raycast_first(const RaycastFirstArgs& a, uint64_t threads, hip
 // four lanes per ray on the 4-wide tree; `quads` = rays traced at once (a multiple of 64)
 hipError_t launch_raycast_first4(const RaycastFirstArgs& a, uint64_t quads, hipStream_t s);
 hipError_t launch_closest_point(const ClosestPointArgs& a, uint64_t threads, hipStream_t s);
+// four lanes per query on the 4-wide tree (minThreshold must be 0); `threads` lanes resident
+hipError_t launch_closest_point4(const ClosestPointArgs& a, uint64_t threads, hipStream_t s);
 // fill == false: count hits per ray into a.counts; fill == true: write them at a.offsets.
 hipError_t launch_raycast_all(const RaycastAllArgs& a, bool fill, uint64_t threads, hipStream_t s);
 // Spatial order of n packed float triples (Morton bucket sort); workspace of

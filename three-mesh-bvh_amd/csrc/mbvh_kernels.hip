@@ -1129,6 +1129,191 @@ __device__ __noinline__ uint32_t resolve_point(Cands cl, float U, f3 q, const fl
 // bounds-tested, triangles update the best on strict <, and the first triangle closer than
 // minThreshold ends the whole query (all roots).
 // ------------------------------------------------------------------------------------------
+// Closest point with four lanes per query on the 4-wide tree (used when minThreshold is 0: the
+// reference's minThreshold exit returns the FIRST triangle closer than it in its own visit order,
+// which only the binary traversal reproduces). Node step: lane q scores child q; the quad sorts
+// the entered children by score, continues with the nearest and pushes the others (scores
+// re-checked on pop). Leaf step: lane q tests triangles q, q+4, ...; the bound is shared after
+// each leaf. Each lane resolves its candidates exactly; the quad keeps the smallest exact squared
+// distance (exactly equal distances are ties).
+__global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A) {
+  __shared__ uint2 qstack[kQuadLds * (kBlock / 4)];
+  const uint32_t lane = threadIdx.x, q4 = lane & 3u, wq = (lane & 63u) >> 2;
+  const uint32_t qid = lane >> 2;
+  const MeshView M = A.mesh;
+  QuadStack st;
+  st.lds = qstack;
+  st.ovf = A.stack.ovf;
+  st.ovf_stride = static_cast<uint64_t>(gridDim.x) * (kBlock / 4);
+  st.gquad = static_cast<uint64_t>(blockIdx.x) * (kBlock / 4) + qid;
+  st.qid = qid;
+  st.capacity = kQuadLds + static_cast<int>(A.stack.ovf_entries);
+  const float max2 = A.max_threshold * A.max_threshold;
+  constexpr float kScoreSlack = 1.0f - 4.0f * kU;
+  const float qnan = __int_as_float(0x7fc00000);
+
+  while (true) {
+    uint32_t tile = 0;
+    if ((lane & 63u) == 0) tile = atomicAdd(A.queue, 1u);
+    tile = __shfl(tile, 0, 64);
+    if (static_cast<uint64_t>(tile) * 16 >= A.n) break;  // every wave reaches this exit
+    const uint64_t visit = static_cast<uint64_t>(tile) * 16 + wq;
+    if (visit >= A.n) continue;
+    const uint64_t qi = A.order ? A.order[visit] : visit;
+    if (qi >= A.n) continue;
+    const f3 q = mk(A.points[3 * qi + 0], A.points[3 * qi + 1], A.points[3 * qi + 2]);
+    const float e_abs = 8.0f * kU * (maxabs(q) + M.coord_max);
+    float U = kInf;  // quad-wide upper bound of the best squared distance (after each leaf)
+    bool overflow = false;
+    Cands cl;
+    cl.lane = lane;
+    cl.n = 0;
+    for (uint32_t r = 0; r < M.n_roots && !overflow; r++) {
+      uint32_t ref = M.roots4[r];
+      st.sp = 0;
+      while (true) {
+        while (!(ref & kRefLeaf) && ref != kDone) {
+          const float4 c0 = M.nodes4[8ull * ref + 2 * q4];
+          const float4 c1 = M.nodes4[8ull * ref + 2 * q4 + 1];
+          const uint32_t cref = __float_as_uint(c1.z);
+          const float sc = box_score(q, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y) * kScoreSlack;
+          const bool in = cref != kDone && sc <= U && sc < max2;
+          const float sq = in ? sc : kInf;
+          const uint32_t rq = in ? cref : kDone;
+          float t0 = __shfl(sq, 0, 4), t1 = __shfl(sq, 1, 4), t2 = __shfl(sq, 2, 4), t3 = __shfl(sq, 3, 4);
+          uint32_t r0 = __shfl(rq, 0, 4), r1 = __shfl(rq, 1, 4), r2 = __shfl(rq, 2, 4), r3 = __shfl(rq, 3, 4);
+          cswap(t0, r0, t1, r1);
+          cswap(t2, r2, t3, r3);
+          cswap(t0, r0, t2, r2);
+          cswap(t1, r1, t3, r3);
+          cswap(t1, r1, t2, r2);
+          if (r0 == kDone) {
+            ref = kDone;
+            while (st.sp > 0) {
+              const uint2 e = st.pop();
+              const float se = __uint_as_float(e.y);
+              if (se <= U && se < max2) {
+                ref = e.x;
+                break;
+              }
+            }
+          } else {
+            bool ok = true;
+            if (r3 != kDone) ok = ok && st.push(r3, __float_as_uint(t3));
+            if (r2 != kDone) ok = ok && st.push(r2, __float_as_uint(t2));
+            if (r1 != kDone) ok = ok && st.push(r1, __float_as_uint(t1));
+            if (!ok) {
+              overflow = true;
+              ref = kDone;
+              break;
+            }
+            ref = r0;
+          }
+        }
+        while (ref & kRefLeaf) {
+          uint32_t off, cnt;
+          leaf_range(ref, M.bigleaf, off, cnt);
+          float Ul = U;
+          float best = kInf;
+          f3 best_cp = mk(kInf, kInf, kInf);
+          for (uint32_t s = off + q4; s < off + cnt; s += 4) {
+            const Tri tr = load_tri(M.tris, s);
+            const f3 cp = closest_on_triangle(q, tr.a(), tr.b(), tr.c());
+            const f3 dv = sub(q, cp);
+            const float d2 = dot(dv, dv);
+            float lb;
+            bool add;
+            if (!(d2 == d2)) {
+              add = true;  // FP32 failed (degenerate triangle): decide exactly at the end
+              lb = -kInf;
+            } else {
+              const float e2 = (2.0f * sqrtf(d2) + e_abs) * e_abs + 4.0f * kU * d2;
+              lb = d2 - e2;
+              add = lb <= Ul && !(d2 == best && cp.x == best_cp.x && cp.y == best_cp.y && cp.z == best_cp.z);
+              if (add) {
+                Ul = fminf(Ul, d2 + e2);
+                if (d2 < best) {
+                  best = d2;
+                  best_cp = cp;
+                }
+              }
+            }
+            if (add) {
+              if (cl.n == kCand && !cl.compact(Ul)) {
+                ff bd;
+                const uint32_t w = resolve_point(cl, Ul, q, M.tris, bd);
+                cl.n = 0;
+                if (w != kNoHit) {
+                  const float dw = ff_to_float(bd);
+                  Ul = fminf(Ul, dw * (1.0f + 4.0f * kU));
+                  cl.put(0, w, dw * (1.0f - 4.0f * kU));
+                  cl.n = 1;
+                }
+              }
+              cl.put(cl.n++, s, lb);
+            }
+          }
+          Ul = fminf(Ul, __shfl_xor(Ul, 1, 4));
+          U = fminf(Ul, __shfl_xor(Ul, 2, 4));
+          ref = kDone;
+          while (st.sp > 0) {
+            const uint2 e = st.pop();
+            const float se = __uint_as_float(e.y);
+            if (se <= U && se < max2) {
+              ref = e.x;
+              break;
+            }
+          }
+        }
+        if (ref == kDone) break;
+      }
+    }
+    ff bd = {kInf, 0.0f};
+    uint32_t w = kNoHit;
+    if (!overflow && cl.n > 0) {
+      w = resolve_point(cl, U, q, M.tris, bd);
+      if (w == kNoHit) bd = {kInf, 0.0f};
+    }
+    for (int m = 1; m <= 2; m <<= 1) {
+      const float oh = __shfl_xor(bd.hi, m, 4), ol = __shfl_xor(bd.lo, m, 4);
+      const uint32_t ow = __shfl_xor(w, m, 4);
+      if (oh < bd.hi || (oh == bd.hi && (ol < bd.lo || (ol == bd.lo && ow < w)))) {
+        bd = {oh, ol};
+        w = ow;
+      }
+    }
+    uint32_t out_face = overflow ? kOverflow : kNoHit;
+    float out_dist = kInf;
+    float px = qnan, py = qnan, pz = qnan, bw = qnan, bu = qnan, bv = qnan;
+    if (!overflow && w != kNoHit && q4 == 0) {
+      const Tri tr = load_tri(M.tris, w);
+      const ClosestFF cf = closest_on_triangle_ff(q, tr.a(), tr.b(), tr.c());
+      out_face = tr.face();
+      out_dist = ff_to_float(ff_sqrt(cf.dist2));
+      px = ff_to_float(cf.point.x);
+      py = ff_to_float(cf.point.y);
+      pz = ff_to_float(cf.point.z);
+      bw = ff_to_float(cf.bw);
+      bu = ff_to_float(cf.bu);
+      bv = ff_to_float(cf.bv);
+    }
+    if (q4 == 0) {
+      A.out_face[qi] = out_face;
+      if (A.out_dist) A.out_dist[qi] = out_dist;
+      if (A.out_point) {
+        A.out_point[3 * qi + 0] = px;
+        A.out_point[3 * qi + 1] = py;
+        A.out_point[3 * qi + 2] = pz;
+      }
+      if (A.out_bary) {
+        A.out_bary[3 * qi + 0] = bw;
+        A.out_bary[3 * qi + 1] = bu;
+        A.out_bary[3 * qi + 2] = bv;
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
   const uint32_t lane = threadIdx.x;
   const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + lane;
@@ -1354,6 +1539,13 @@ hipError_t launch_batch_raycast_first(const BatchArgs& a, uint32_t stack_lds, ui
     hipLaunchKernelGGL(k_batch_raycast_first<8>, dim3(blocks), dim3(kBlock), 0, s, a);
   else
     hipLaunchKernelGGL(k_batch_raycast_first<16>, dim3(blocks), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_closest_point4(const ClosestPointArgs& a, uint64_t threads, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.queue);
+  hipLaunchKernelGGL(k_closest_point4, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 

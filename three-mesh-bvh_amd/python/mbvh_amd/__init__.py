@@ -29,6 +29,7 @@ EXPORTED_SYMBOLS = (
     "mbvh_build", "mbvh_tree_info", "mbvh_tree_root_bytes", "mbvh_tree_copy_root",
     "mbvh_tree_copy_indirect", "mbvh_tree_free",
     "mbvh_ctx_create", "mbvh_ctx_destroy", "mbvh_ctx_synchronize", "mbvh_ctx_set_ray_lanes",
+    "mbvh_ctx_set_point_lanes",
     "mbvh_mesh_upload", "mbvh_mesh_free", "mbvh_mesh_info", "mbvh_mesh_set_side_mask",
     "mbvh_raycast_first", "mbvh_raycast_first_d",
     "mbvh_closest_point", "mbvh_closest_point_d",
@@ -89,6 +90,7 @@ def library() -> C.CDLL:
         "mbvh_ctx_destroy": (None, [vp]),
         "mbvh_ctx_synchronize": (C.c_int, [vp]),
         "mbvh_ctx_set_ray_lanes": (C.c_int, [vp, i32]),
+        "mbvh_ctx_set_point_lanes": (C.c_int, [vp, i32]),
         "mbvh_mesh_upload": (C.c_int, [vp, C.POINTER(vp), C.POINTER(u64), u32, vp, i32, u64, vp, i32, u64,
                                        vp, u64, u32, u32, C.POINTER(vp)]),
         "mbvh_mesh_free": (None, [vp]),
@@ -218,6 +220,10 @@ class Context:
 
     def synchronize(self) -> None:
         _check(self._lib.mbvh_ctx_synchronize(self.handle))
+
+    def set_point_lanes(self, lanes: int) -> None:
+        """closestPoint lanes per query: 0 automatic, 1 (binary tree) or 4 (4-wide tree)."""
+        _check(self._lib.mbvh_ctx_set_point_lanes(self.handle, lanes))
 
     def set_ray_lanes(self, lanes: int) -> None:
         """raycastFirst lanes per ray: 0 automatic, 1 (binary tree) or 4 (4-wide tree)."""
