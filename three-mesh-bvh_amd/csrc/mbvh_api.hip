@@ -1228,7 +1228,8 @@ constexpr uint32_t kGraphMagic = 0x48505247u;
 bool valid_batch(const mbvh_batch* b) { return b && b->magic == kBatchMagic; }
 
 uint64_t batch_ws_bytes(const mbvh_batch* b, uint64_t n) {
-  return ((4 * n + 255) & ~uint64_t{255}) + order_by_mesh_workspace_bytes(n, b->n_meshes);
+  (void)b;
+  return order_by_mesh_workspace_bytes(n);
 }
 
 int check_tuning(const char* fn, const mbvh_batch_tuning& t) {
@@ -1276,9 +1277,8 @@ int check_batch_call(const char* fn, const mbvh_ctx* ctx, const mbvh_batch* b, c
 }
 
 int enqueue_batch(mbvh_ctx* ctx, mbvh_batch* b, const BatchCall& c, const mbvh_batch_tuning& t, hipStream_t s) {
-  uint32_t* order = static_cast<uint32_t*>(c.ws);
-  char* ows = static_cast<char*>(c.ws) + ((4 * c.n + 255) & ~uint64_t{255});
-  if (c.n) MBVH_HIP(launch_order_by_mesh(c.ray_inst, c.inst_geom, c.n_inst, c.n, b->n_meshes, order, ows, s));
+  const uint32_t* order = nullptr;
+  if (c.n) MBVH_HIP(launch_order_by_mesh(c.ray_inst, c.inst_geom, c.n_inst, c.n, b->n_meshes, c.ws, &order, s));
   int device_cus = static_cast<int>(ctx->max_threads / kMaxThreadsPerCU);
   const uint64_t tiles = (c.n + t.tile_rays - 1) / t.tile_rays;
   uint64_t blocks = static_cast<uint64_t>(device_cus) * t.blocks_per_cu;

@@ -406,6 +406,64 @@ hipError_t reduce_deterministic(const GradLayout& L, char* ws, uint32_t sentinel
 
 }  // namespace
 
+// ---- generic stable radix sort (also orders the rays of a batched launch by mesh) ----------------
+namespace {
+struct SortLayout {
+  uint64_t n_tiles, keys0, vals0, keys1, vals1, hist, totals, total;
+};
+SortLayout sort_layout(uint64_t n) {
+  SortLayout L{};
+  L.n_tiles = (n + kSortTile - 1) / kSortTile;
+  uint64_t o = 0;
+  auto take = [&](uint64_t bytes) {
+    const uint64_t at = o;
+    o += align256(bytes);
+    return at;
+  };
+  L.keys0 = take(4 * n);
+  L.vals0 = take(4 * n);
+  L.keys1 = take(4 * n);
+  L.vals1 = take(4 * n);
+  L.hist = take(4 * 256 * std::max<uint64_t>(L.n_tiles, 1));
+  L.totals = take(4 * 256);
+  L.total = o;
+  return L;
+}
+}  // namespace
+
+uint64_t radix_sort_workspace_bytes(uint64_t n) { return sort_layout(n).total; }
+
+void radix_sort_inputs(void* ws, uint64_t n, uint32_t** keys, uint32_t** vals) {
+  const SortLayout L = sort_layout(n);
+  *keys = reinterpret_cast<uint32_t*>(static_cast<char*>(ws) + L.keys0);
+  *vals = reinterpret_cast<uint32_t*>(static_cast<char*>(ws) + L.vals0);
+}
+
+hipError_t launch_radix_sort(void* ws, uint64_t n, uint32_t bits, uint32_t** sorted_keys, uint32_t** sorted_vals,
+                             hipStream_t s) {
+  const SortLayout L = sort_layout(n);
+  char* w = static_cast<char*>(ws);
+  uint32_t* keys[2] = {reinterpret_cast<uint32_t*>(w + L.keys0), reinterpret_cast<uint32_t*>(w + L.keys1)};
+  uint32_t* vals[2] = {reinterpret_cast<uint32_t*>(w + L.vals0), reinterpret_cast<uint32_t*>(w + L.vals1)};
+  uint32_t* hist = reinterpret_cast<uint32_t*>(w + L.hist);
+  uint32_t* totals = reinterpret_cast<uint32_t*>(w + L.totals);
+  const uint32_t n_tiles = static_cast<uint32_t>(L.n_tiles);
+  int cur = 0;
+  if (n > 0) {
+    for (uint32_t shift = 0; shift < bits; shift += 8) {
+      hipLaunchKernelGGL(k_sort_hist, dim3(n_tiles), dim3(kSortBlock), 0, s, keys[cur], n, shift, n_tiles, hist);
+      hipLaunchKernelGGL(k_sort_totals, dim3(256), dim3(kSortBlock), 0, s, hist, n_tiles, totals);
+      hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(kSortBlock), 0, s, hist, n_tiles, totals);
+      hipLaunchKernelGGL(k_sort_scatter, dim3(n_tiles), dim3(kSortBlock), 0, s, keys[cur], vals[cur], n, shift,
+                         n_tiles, hist, keys[cur ^ 1], vals[cur ^ 1]);
+      cur ^= 1;
+    }
+  }
+  *sorted_keys = keys[cur];
+  *sorted_vals = vals[cur];
+  return hipGetLastError();
+}
+
 uint64_t grad_workspace_bytes(uint64_t n_queries, int32_t mode) {
   return mode == kGradDeterministic ? grad_layout(n_queries).total : 0;
 }

@@ -130,11 +130,20 @@ hipError_t launch_instance_hits(const InstanceHitArgs& a, hipStream_t s);
 // per ray on the 4-wide trees; blocks: persistent grid size
 hipError_t launch_batch_raycast_first(const BatchArgs& a, uint32_t stack_lds, uint32_t lanes, uint32_t blocks,
                                       hipStream_t s);
-// Visit order grouping rays by the mesh of their instance (counting sort over n_meshes keys);
-// workspace of order_by_mesh_workspace_bytes(n, n_meshes) bytes.
-uint64_t order_by_mesh_workspace_bytes(uint64_t n, uint32_t n_meshes);
+// Visit order grouping rays by the mesh of their instance (stable radix sort on the mesh id; a
+// hot mesh does not serialise on one counter); workspace of order_by_mesh_workspace_bytes(n)
+// bytes. *order points into the workspace.
+uint64_t order_by_mesh_workspace_bytes(uint64_t n);
 hipError_t launch_order_by_mesh(const uint32_t* ray_instance, const uint32_t* instance_geometry, uint32_t n_instances,
-                                uint64_t n, uint32_t n_meshes, uint32_t* order, void* workspace, hipStream_t s);
+                                uint64_t n, uint32_t n_meshes, void* workspace, const uint32_t** order, hipStream_t s);
+
+// Stable LSD radix sort (8-bit digits) of n (key, value) pairs by the low `bits` key bits. The
+// caller fills the input arrays radix_sort_inputs() names inside the workspace; the sorted arrays
+// are returned (also inside the workspace).
+uint64_t radix_sort_workspace_bytes(uint64_t n);
+void radix_sort_inputs(void* ws, uint64_t n, uint32_t** keys, uint32_t** vals);
+hipError_t launch_radix_sort(void* ws, uint64_t n, uint32_t bits, uint32_t** sorted_keys, uint32_t** sorted_vals,
+                             hipStream_t s);
 
 // ---- backward ----------------------------------------------------------------------------------
 constexpr int32_t kGradAtomic = 0;

@@ -147,42 +147,37 @@ __global__ __launch_bounds__(kOrderBlock) void k_order_scatter(const uint32_t* k
   }
 }
 
-// Bucket of a ray in the by-mesh order: the mesh of its instance, or n_meshes when invalid.
-__global__ __launch_bounds__(kOrderBlock) void k_mesh_hist(const uint32_t* ray_instance, const uint32_t* instance_geometry,
+// Sort key of a ray in the by-mesh order: the mesh of its instance, or n_meshes when invalid.
+__global__ __launch_bounds__(kOrderBlock) void k_mesh_keys(const uint32_t* ray_instance, const uint32_t* instance_geometry,
                                                            uint32_t n_instances, uint64_t n, uint32_t n_meshes,
-                                                           uint32_t* counts, uint32_t* keys) {
+                                                           uint32_t* keys, uint32_t* vals) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kOrderBlock;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kOrderBlock + threadIdx.x; i < n; i += stride) {
     const uint32_t inst = ray_instance[i];
     uint32_t g = inst < n_instances ? instance_geometry[inst] : n_meshes;
     if (g > n_meshes) g = n_meshes;
     keys[i] = g;
-    atomicAdd(&counts[g], 1u);
+    vals[i] = static_cast<uint32_t>(i);
   }
 }
 
 }  // namespace
 
-uint64_t order_by_mesh_workspace_bytes(uint64_t n, uint32_t n_meshes) {
-  return kOrderHeaderBytes + ((4 * (uint64_t{n_meshes} + 1) + 255) & ~uint64_t{255}) + 4 * n;
-}
+uint64_t order_by_mesh_workspace_bytes(uint64_t n) { return radix_sort_workspace_bytes(n); }
 
 hipError_t launch_order_by_mesh(const uint32_t* ray_instance, const uint32_t* instance_geometry, uint32_t n_instances,
-                                uint64_t n, uint32_t n_meshes, uint32_t* order, void* workspace, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  const uint32_t n_buckets = n_meshes + 1;
-  uint32_t* bounds = static_cast<uint32_t*>(workspace);
-  uint32_t* counts = reinterpret_cast<uint32_t*>(static_cast<char*>(workspace) + kOrderHeaderBytes);
-  uint32_t* keys = reinterpret_cast<uint32_t*>(static_cast<char*>(workspace) + kOrderHeaderBytes +
-                                               ((4 * (uint64_t{n_meshes} + 1) + 255) & ~uint64_t{255}));
-  const uint32_t init_blocks = std::max<uint32_t>(1, std::min<uint32_t>(1024, (n_buckets + kOrderBlock - 1) / kOrderBlock));
-  const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(4096, (n + kOrderBlock - 1) / kOrderBlock));
-  hipLaunchKernelGGL(k_order_init, dim3(init_blocks), dim3(kOrderBlock), 0, s, bounds, counts, n_buckets);
-  hipLaunchKernelGGL(k_mesh_hist, dim3(blocks), dim3(kOrderBlock), 0, s, ray_instance, instance_geometry, n_instances, n,
-                     n_meshes, counts, keys);
-  hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(kScanBlock), 0, s, counts, n_buckets);
-  hipLaunchKernelGGL(k_order_scatter, dim3(blocks), dim3(kOrderBlock), 0, s, keys, n, counts, order);
-  return hipGetLastError();
+                                uint64_t n, uint32_t n_meshes, void* workspace, const uint32_t** order, hipStream_t s) {
+  uint32_t *keys, *vals, *sk, *sv;
+  radix_sort_inputs(workspace, n, &keys, &vals);
+  if (n) {
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(4096, (n + kOrderBlock - 1) / kOrderBlock));
+    hipLaunchKernelGGL(k_mesh_keys, dim3(blocks), dim3(kOrderBlock), 0, s, ray_instance, instance_geometry, n_instances,
+                       n, n_meshes, keys, vals);
+  }
+  const uint32_t bits = 32u - static_cast<uint32_t>(__builtin_clz(n_meshes | 1u));
+  const hipError_t e = launch_radix_sort(workspace, n, bits, &sk, &sv, s);
+  *order = sv;
+  return e;
 }
 
 uint32_t order_axis_bits(uint64_t n) {
