@@ -652,7 +652,7 @@ int raycast_first_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o, con
   // latency-bound and gains from shorter paths and more waves; big launches keep one lane per ray
   const bool quads = ctx->ray_lanes == 4 || (ctx->ray_lanes == 0 && 4 * n <= ctx->max_threads);
   if (quads) {
-    if (mesh->max_stack4 > 16 + ctx->stack_ovf_entries)
+    if (mesh->max_stack4 > kQuadLdsStack + ctx->stack_ovf_entries)
       return set_error(MBVH_ERR_LIMIT, "raycast_first: stack scratch smaller than the tree depth");
   } else if (mesh->max_stack > kStackLds + ctx->stack_ovf_entries) {
     return set_error(MBVH_ERR_LIMIT, "raycast_first: stack scratch smaller than the tree depth");
@@ -685,7 +685,7 @@ int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uin
   if (n && (!q || !face)) return set_error(MBVH_ERR_BAD_ARGS, "closest_point: NULL point or face buffer");
   // four lanes per query only without a minThreshold (its early exit follows the binary order)
   const bool quads = min_t == 0.0f && (ctx->point_lanes == 4 || ctx->point_lanes == 0);
-  if (quads ? mesh->max_stack4 > 16 + ctx->stack_ovf_entries : mesh->max_stack > kStackLds + ctx->stack_ovf_entries)
+  if (quads ? mesh->max_stack4 > kQuadLdsStack + ctx->stack_ovf_entries : mesh->max_stack > kStackLds + ctx->stack_ovf_entries)
     return set_error(MBVH_ERR_LIMIT, "closest_point: stack scratch smaller than the tree depth");
   const uint64_t threads = quads ? launch_threads(4 * n, ctx->max_threads) : launch_threads(n, ctx->max_threads);
   ClosestPointArgs a;
@@ -1283,7 +1283,7 @@ int enqueue_batch(mbvh_ctx* ctx, mbvh_batch* b, const BatchCall& c, const mbvh_b
   const uint64_t tiles = (c.n + t.tile_rays - 1) / t.tile_rays;
   uint64_t blocks = static_cast<uint64_t>(device_cus) * t.blocks_per_cu;
   const uint64_t useful = (tiles + kBlock / 64 - 1) / (kBlock / 64);  // one wave per tile at most
-  if (t.ray_lanes == 4 && b->max_stack4 > 16 + ctx->stack_ovf_entries)
+  if (t.ray_lanes == 4 && b->max_stack4 > kQuadLdsStack + ctx->stack_ovf_entries)
     return set_error(MBVH_ERR_LIMIT, "batch: stack scratch smaller than the tree depth");
   if (blocks > useful) blocks = std::max<uint64_t>(useful, 1);
   BatchArgs a{};

@@ -40,6 +40,11 @@ static_assert(sizeof(WideNode) == 64, "wide node is one 64-byte record");
 // (indexed [depth - kStackLds][thread]) sized at upload from the deepest tree.
 constexpr int kStackLds = 16;
 constexpr int kBlock = 256;
+// LDS stack entries per quad (four lanes tracing one query on the 4-wide tree). A closest-point
+// descent enters every child until a first bound exists (three pushes per 4-wide level), so the
+// stack holds a few levels' worth in LDS: 48 x 8 B x 64 quads = 24 KB per block, which with the
+// 8 KB candidate lists still lets four blocks (the VGPR limit) share a CU's LDS.
+constexpr int kQuadLdsStack = 48;
 
 // ---- small float3 helpers -------------------------------------------------------------------
 struct f3 {

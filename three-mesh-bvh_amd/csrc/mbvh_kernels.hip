@@ -478,7 +478,7 @@ __device__ __forceinline__ FirstHit trace_first(const MeshView& M, StackT& st, u
 // Sixteen rays per wave instead of 64, and four times the waves for a given ray count: a launch
 // of few rays (latency-bound) gets more waves to overlap and shorter paths per wave.
 // ------------------------------------------------------------------------------------------
-constexpr int kQuadLds = 16;  // LDS stack entries per quad
+constexpr int kQuadLds = kQuadLdsStack;
 
 // Stack shared by the four lanes of a quad: every lane pushes and pops the same entries.
 struct QuadStack {
