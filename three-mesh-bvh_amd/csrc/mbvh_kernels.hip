@@ -506,6 +506,29 @@ struct QuadStack {
   }
 };
 
+// Quad exchanges as DPP quad permutations (VALU operand modifiers) rather than LDS permutes.
+// quad_perm selects, for each lane of a group of four, the source lane within the group.
+template <int kCtrl>
+__device__ __forceinline__ uint32_t quad_dpp(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), kCtrl, 0xF, 0xF, false));
+}
+template <int kCtrl>
+__device__ __forceinline__ float quad_dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xF, 0xF, false));
+}
+template <int kLane, typename T>
+__device__ __forceinline__ T quad_bcast(T v) {  // value of lane kLane of the quad
+  return quad_dpp<kLane | (kLane << 2) | (kLane << 4) | (kLane << 6)>(v);
+}
+template <typename T>
+__device__ __forceinline__ T quad_xor1(T v) {  // lanes 1 0 3 2
+  return quad_dpp<0xB1>(v);
+}
+template <typename T>
+__device__ __forceinline__ T quad_xor2(T v) {  // lanes 2 3 0 1
+  return quad_dpp<0x4E>(v);
+}
+
 __device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
   if (tb < ta) {
     const float t = ta;
@@ -552,8 +575,8 @@ __device__ __forceinline__ FirstHit trace_first4(const MeshView& M, QuadStack& s
                          t <= U;
         const float tq = hit ? t : kInf;
         const uint32_t rq = hit ? cref : kDone;
-        float t0 = __shfl(tq, 0, 4), t1 = __shfl(tq, 1, 4), t2 = __shfl(tq, 2, 4), t3 = __shfl(tq, 3, 4);
-        uint32_t r0 = __shfl(rq, 0, 4), r1 = __shfl(rq, 1, 4), r2 = __shfl(rq, 2, 4), r3 = __shfl(rq, 3, 4);
+        float t0 = quad_bcast<0>(tq), t1 = quad_bcast<1>(tq), t2 = quad_bcast<2>(tq), t3 = quad_bcast<3>(tq);
+        uint32_t r0 = quad_bcast<0>(rq), r1 = quad_bcast<1>(rq), r2 = quad_bcast<2>(rq), r3 = quad_bcast<3>(rq);
         // misses carry t = inf and sort last; among hits, nearest first
         cswap(t0, r0, t1, r1);
         cswap(t2, r2, t3, r3);
@@ -610,8 +633,8 @@ __device__ __forceinline__ FirstHit trace_first4(const MeshView& M, QuadStack& s
           }
           cl.put(cl.n++, s, res == kHit ? t - et : -kInf);
         }
-        Ul = fminf(Ul, __shfl_xor(Ul, 1, 4));
-        U = fminf(Ul, __shfl_xor(Ul, 2, 4));
+        Ul = fminf(Ul, quad_xor1(Ul));
+        U = fminf(Ul, quad_xor2(Ul));
         ref = kDone;
         while (st.sp > 0) {
           const uint2 e = st.pop();
@@ -643,8 +666,9 @@ __device__ __forceinline__ FirstHit trace_first4(const MeshView& M, QuadStack& s
   }
   // smallest exact t over the quad (then the smaller slot)
   for (int m = 1; m <= 2; m <<= 1) {
-    const float oh = __shfl_xor(bt.hi, m, 4), ol = __shfl_xor(bt.lo, m, 4);
-    const uint32_t ow = __shfl_xor(w, m, 4);
+    const float oh = m == 1 ? quad_xor1(bt.hi) : quad_xor2(bt.hi);
+    const float ol = m == 1 ? quad_xor1(bt.lo) : quad_xor2(bt.lo);
+    const uint32_t ow = m == 1 ? quad_xor1(w) : quad_xor2(w);
     if (oh < bt.hi || (oh == bt.hi && (ol < bt.lo || (ol == bt.lo && ow < w)))) {
       bt = {oh, ol};
       w = ow;
@@ -1180,8 +1204,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
           const bool in = cref != kDone && sc <= U && sc < max2;
           const float sq = in ? sc : kInf;
           const uint32_t rq = in ? cref : kDone;
-          float t0 = __shfl(sq, 0, 4), t1 = __shfl(sq, 1, 4), t2 = __shfl(sq, 2, 4), t3 = __shfl(sq, 3, 4);
-          uint32_t r0 = __shfl(rq, 0, 4), r1 = __shfl(rq, 1, 4), r2 = __shfl(rq, 2, 4), r3 = __shfl(rq, 3, 4);
+          float t0 = quad_bcast<0>(sq), t1 = quad_bcast<1>(sq), t2 = quad_bcast<2>(sq), t3 = quad_bcast<3>(sq);
+          uint32_t r0 = quad_bcast<0>(rq), r1 = quad_bcast<1>(rq), r2 = quad_bcast<2>(rq), r3 = quad_bcast<3>(rq);
           cswap(t0, r0, t1, r1);
           cswap(t2, r2, t3, r3);
           cswap(t0, r0, t2, r2);
@@ -1253,8 +1277,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
               cl.put(cl.n++, s, lb);
             }
           }
-          Ul = fminf(Ul, __shfl_xor(Ul, 1, 4));
-          U = fminf(Ul, __shfl_xor(Ul, 2, 4));
+          Ul = fminf(Ul, quad_xor1(Ul));
+          U = fminf(Ul, quad_xor2(Ul));
           ref = kDone;
           while (st.sp > 0) {
             const uint2 e = st.pop();
@@ -1275,8 +1299,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
       if (w == kNoHit) bd = {kInf, 0.0f};
     }
     for (int m = 1; m <= 2; m <<= 1) {
-      const float oh = __shfl_xor(bd.hi, m, 4), ol = __shfl_xor(bd.lo, m, 4);
-      const uint32_t ow = __shfl_xor(w, m, 4);
+      const float oh = m == 1 ? quad_xor1(bd.hi) : quad_xor2(bd.hi);
+      const float ol = m == 1 ? quad_xor1(bd.lo) : quad_xor2(bd.lo);
+      const uint32_t ow = m == 1 ? quad_xor1(w) : quad_xor2(w);
       if (oh < bd.hi || (oh == bd.hi && (ol < bd.lo || (ol == bd.lo && ow < w)))) {
         bd = {oh, ol};
         w = ow;
