@@ -257,6 +257,13 @@ int mbvh_batch_raycast_first_d(mbvh_ctx* ctx, mbvh_batch* batch, const float* ma
                                float far_dist, uint32_t* face_index, float* distance, float* point,
                                float* barycoord, void* workspace, uint64_t workspace_bytes,
                                const mbvh_batch_tuning* tuning, void* stream);
+/* Host-pointer form of mbvh_batch_raycast_first_d (the batch's tuning, the context's stream and
+ * scratch): stages the inputs, launches, copies the results back and synchronises. */
+int mbvh_batch_raycast_first(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_world,
+                             const float* matrix_world_inverse, const uint32_t* instance_geometry,
+                             uint32_t n_instances, const float* origins, const float* directions,
+                             const uint32_t* ray_instance, uint64_t n, int32_t side, float near_dist, float far_dist,
+                             uint32_t* face_index, float* distance, float* point, float* barycoord);
 /* Times every candidate (ray_lanes x stack_lds x tile_rays x blocks_per_cu) on these inputs, keeps the fastest
  * in the batch (used when tuning is NULL) and returns it. Synchronous; outputs hold the last run. */
 int mbvh_batch_autotune(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_world,

@@ -144,5 +144,30 @@ rc.firstHitOnly = true;
 const bh = rc.intersectObject(batched);
 assert.ok(bh.every(h => h.object === batched && (h.batchId === 0 || h.batchId === 1)));
 if (rf.faceIndex[0] !== 0xFFFFFFFF) assert.ok(bh.some(h => h.batchId === 0 && h.faceIndex === rf.faceIndex[0]));
+// firstHitOnly goes through one batched device launch; compare it with the per-instance path (all
+// hits of each instance, nearest kept) over many rays
+let batchedChecked = 0, presenceDiffers = 0;
+for (let i = 0; i < Math.min(K, 128); i++) {
+  const r1 = new Raycaster([origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]], [dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]]);
+  r1.firstHitOnly = true;
+  const fast = r1.intersectObject(batched);
+  const r2 = new Raycaster([origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]], [dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]]);
+  const all = r2.intersectObject(batched);
+  for (const id of [0, 1]) {
+    const f = fast.filter(h => h.batchId === id);
+    const a = all.filter(h => h.batchId === id);
+    assert.ok(f.length <= 1, `ray ${i} instance ${id}`);
+    // a grazing ray may hit on one side of the FP32 / FP64 frame-change rounding and miss on the other
+    if (f.length !== (a.length ? 1 : 0)) { presenceDiffers++; continue; }
+    if (!a.length) continue;
+    const nearest = a.reduce((m, h) => (h.distance < m.distance ? h : m));
+    assert.ok(Math.abs(f[0].distance - nearest.distance) <= 1e-4 * Math.max(1, nearest.distance), `ray ${i} instance ${id} distance`);
+    assert.strictEqual(f[0].object, batched);
+    batchedChecked++;
+  }
+}
+assert.ok(batchedChecked > 64 && presenceDiffers <= 2, `batched BatchedMesh hits ${batchedChecked}, ${presenceDiffers} differ`);
+api.disposeBatchedBoundsTree.call(Object.assign(batched, { boundsTrees: [bvh] }), -1);
+assert.strictEqual(batched._mbvhBatch, null);
 
-console.log(JSON.stringify({ ok: true, rays: nRays, points: nPts, worldChecked }));
+console.log(JSON.stringify({ ok: true, rays: nRays, points: nPts, worldChecked, batchedChecked, presenceDiffers }));

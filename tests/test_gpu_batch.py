@@ -212,3 +212,13 @@ def test_invalid_instances_miss(ctx, case):
     assert np.array_equal(got["face"][keep], ref["face"][:n][keep])
     with pytest.raises(M.MbvhError):
         batch.raycast_first_d(_params(w, D2, out, 16, n=n))  # workspace too small
+
+
+def test_host_form_equals_device_form(ctx, case):
+    w, built, meshes, batch, D, wsb = case
+    got = batch.raycast_first(w.matrix_world, w.matrix_world_inverse, w.instance_geometry, w.origins,
+                              w.directions, w.ray_instance)
+    _same(got, _batched(w, batch, D, wsb))
+    empty = batch.raycast_first(w.matrix_world, w.matrix_world_inverse, w.instance_geometry,
+                                np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32), np.zeros(0, np.uint32))
+    assert empty["face"].shape == (0,)

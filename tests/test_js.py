@@ -25,4 +25,5 @@ def test_addon_exports():
     assert r.returncode == 0, r.stderr
     names = set(r.stdout.strip().split(","))
     assert {"build", "uploadMesh", "freeMesh", "raycastFirst", "raycast", "closestPoint", "setSideMask",
-            "deviceCount"} <= names
+            "deviceCount", "raycastFirstBackward", "closestPointBackward", "batchCreate", "freeBatch",
+            "batchRaycastFirst"} <= names

@@ -1396,6 +1396,50 @@ extern "C" int mbvh_batch_raycast_first_d(mbvh_ctx* ctx, mbvh_batch* batch, cons
   return enqueue_batch(ctx, batch, c, t, s);
 }
 
+extern "C" int mbvh_batch_raycast_first(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_world,
+                                        const float* matrix_world_inverse, const uint32_t* instance_geometry,
+                                        uint32_t n_instances, const float* origins, const float* directions,
+                                        const uint32_t* ray_instance, uint64_t n, int32_t side, float near_dist,
+                                        float far_dist, uint32_t* face_index, float* distance, float* point,
+                                        float* barycoord) {
+  clear_error();
+  const char* fn = "mbvh_batch_raycast_first";
+  if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid context handle", fn);
+  if (!valid_batch(batch) || batch->ctx != ctx) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid batch handle", fn);
+  if (n == 0) return MBVH_OK;
+  if (!matrix_world || !matrix_world_inverse || !instance_geometry || !origins || !directions || !ray_instance ||
+      !face_index)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL matrix, instance, ray or face buffer", fn);
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  const size_t v3 = n * 3 * sizeof(float), m16 = static_cast<size_t>(n_instances) * 16 * sizeof(float);
+  void *dW, *dI, *dG, *dO, *dD, *dR, *oF, *oT, *oP, *oB;
+  if ((rc = stage_in(ctx, ctx->grad[0], matrix_world, m16, &dW)) ||
+      (rc = stage_in(ctx, ctx->grad[1], matrix_world_inverse, m16, &dI)) ||
+      (rc = stage_in(ctx, ctx->grad[2], instance_geometry, n_instances * sizeof(uint32_t), &dG)) ||
+      (rc = stage_in(ctx, ctx->grad[3], origins, v3, &dO)) || (rc = stage_in(ctx, ctx->grad[4], directions, v3, &dD)) ||
+      (rc = stage_in(ctx, ctx->grad[5], ray_instance, n * sizeof(uint32_t), &dR)) ||
+      (rc = stage_out(ctx->grad[6], face_index, n * sizeof(uint32_t), &oF)) ||
+      (rc = stage_out(ctx->grad[7], distance, n * sizeof(float), &oT)) ||
+      (rc = stage_out(ctx->grad[8], point, v3, &oP)) || (rc = stage_out(ctx->grad[9], barycoord, v3, &oB)) ||
+      (rc = ensure_buffer(ctx->grad_ws, batch_ws_bytes(batch, n))))
+    return rc;
+  const BatchCall c{static_cast<float*>(dW), static_cast<float*>(dI), static_cast<uint32_t*>(dG), n_instances,
+                    static_cast<float*>(dO), static_cast<float*>(dD), static_cast<uint32_t*>(dR), n, side, near_dist,
+                    far_dist, static_cast<uint32_t*>(oF), static_cast<float*>(oT), static_cast<float*>(oP),
+                    static_cast<float*>(oB), ctx->grad_ws.ptr, ctx->grad_ws.bytes};
+  if ((rc = check_batch_call(fn, ctx, batch, c))) return rc;
+  if ((rc = enqueue_batch(ctx, batch, c, batch->tuning, s))) return rc;
+  MBVH_HIP(hipMemcpyAsync(face_index, oF, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (distance) MBVH_HIP(hipMemcpyAsync(distance, oT, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (point) MBVH_HIP(hipMemcpyAsync(point, oP, v3, hipMemcpyDeviceToHost, s));
+  if (barycoord) MBVH_HIP(hipMemcpyAsync(barycoord, oB, v3, hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipStreamSynchronize(s));
+  return MBVH_OK;
+}
+
 extern "C" int mbvh_batch_autotune(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_world,
                                    const float* matrix_world_inverse, const uint32_t* instance_geometry,
                                    uint32_t n_instances, const float* origins, const float* directions,

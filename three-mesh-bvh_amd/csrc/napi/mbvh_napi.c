@@ -24,7 +24,16 @@ static mbvh_ctx* g_ctx[MAX_DEVICES];
 typedef struct MeshHandle {
   mbvh_mesh* mesh;
   int device;
+  int batch_refs;  /* batches using this mesh: it cannot be freed explicitly meanwhile */
 } MeshHandle;
+
+typedef struct BatchHandle {
+  mbvh_batch* batch;
+  int device;
+  uint32_t n_meshes;
+  MeshHandle** meshes;  /* kept alive (and unfreeable) while the batch exists */
+  napi_ref* refs;
+} BatchHandle;
 
 /* ------------------------------------------------------------------------------------------ */
 
@@ -90,6 +99,8 @@ static int get_view(napi_env env, napi_value v, View* out) {
   }
   return 1;
 }
+
+static int out_view(napi_env env, napi_value v, napi_typedarray_type want, size_t n, void** ptr);
 
 static int get_f64(napi_env env, napi_value v, double* out) {
   return napi_get_value_double(env, v, out) == napi_ok;
@@ -321,9 +332,138 @@ static napi_value js_free_mesh(napi_env env, napi_callback_info info) {
   void* p = NULL;
   if (napi_get_value_external(env, argv[0], &p) == napi_ok && p) {
     MeshHandle* h = (MeshHandle*)p;
+    if (h->batch_refs > 0) return throw_msg(env, "freeMesh: the mesh is used by a batch; free the batch first");
     if (h->mesh) mbvh_mesh_free(h->mesh);
     h->mesh = NULL;
   }
+  return NULL;
+}
+
+static void release_batch(napi_env env, BatchHandle* b) {
+  if (!b) return;
+  if (b->batch) mbvh_batch_free(b->batch);
+  b->batch = NULL;
+  for (uint32_t i = 0; i < b->n_meshes; i++) {
+    if (b->meshes && b->meshes[i]) b->meshes[i]->batch_refs--;
+    if (b->refs && b->refs[i]) napi_delete_reference(env, b->refs[i]);
+  }
+  free(b->meshes);
+  free(b->refs);
+  b->meshes = NULL;
+  b->refs = NULL;
+  b->n_meshes = 0;
+}
+
+static void finalize_batch(napi_env env, void* data, void* hint) {
+  (void)hint;
+  BatchHandle* b = (BatchHandle*)data;
+  release_batch(env, b);
+  free(b);
+}
+
+/* batchCreate(device, [mesh, ...]) -> batch (external) */
+static napi_value js_batch_create(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  int32_t device;
+  if (!get_i32(env, argv[0], &device)) return throw_msg(env, "batchCreate: device must be a number");
+  bool is_array = false;
+  if (napi_is_array(env, argv[1], &is_array) != napi_ok || !is_array)
+    return throw_msg(env, "batchCreate: meshes must be an array");
+  uint32_t n = 0;
+  napi_get_array_length(env, argv[1], &n);
+  if (n == 0) return throw_msg(env, "batchCreate: no meshes");
+  BatchHandle* b = (BatchHandle*)calloc(1, sizeof(BatchHandle));
+  mbvh_mesh** raw = (mbvh_mesh**)calloc(n, sizeof(mbvh_mesh*));
+  if (!b || !raw) {
+    free(b);
+    free(raw);
+    return throw_msg(env, "batchCreate: out of memory");
+  }
+  b->device = device;
+  b->meshes = (MeshHandle**)calloc(n, sizeof(MeshHandle*));
+  b->refs = (napi_ref*)calloc(n, sizeof(napi_ref));
+  if (!b->meshes || !b->refs) {
+    free(raw);
+    release_batch(env, b);
+    free(b);
+    return throw_msg(env, "batchCreate: out of memory");
+  }
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value v;
+    napi_get_element(env, argv[1], i, &v);
+    MeshHandle* h = get_mesh(env, v);
+    if (!h || h->device != device) {
+      free(raw);
+      release_batch(env, b);
+      free(b);
+      return throw_msg(env, "batchCreate: every mesh must be an uploaded mesh of this device");
+    }
+    b->n_meshes = i + 1;
+    b->meshes[i] = h;
+    h->batch_refs++;
+    napi_create_reference(env, v, 1, &b->refs[i]);
+    raw[i] = h->mesh;
+  }
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, device, &ctx);
+  if (!rc) rc = mbvh_batch_create(ctx, (const mbvh_mesh* const*)raw, n, &b->batch);
+  free(raw);
+  if (rc) {
+    release_batch(env, b);
+    free(b);
+    return throw_status(env, rc, "mbvh_batch_create");
+  }
+  napi_value out;
+  napi_create_external(env, b, finalize_batch, NULL, &out);
+  return out;
+}
+
+/* freeBatch(batch) */
+static napi_value js_free_batch(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  void* p = NULL;
+  if (napi_get_value_external(env, argv[0], &p) == napi_ok && p) release_batch(env, (BatchHandle*)p);
+  return NULL;
+}
+
+/* batchRaycastFirst(batch, matrixWorld, matrixWorldInverse, instanceGeometry, origins, dirs, rayInstance,
+ *                   side, near, far, outFace, outDist, outPoint, outBary) */
+static napi_value js_batch_raycast_first(napi_env env, napi_callback_info info) {
+  napi_value argv[14];
+  get_args(env, info, 14, argv);
+  void* p = NULL;
+  if (napi_get_value_external(env, argv[0], &p) != napi_ok || !p || !((BatchHandle*)p)->batch)
+    return throw_msg(env, "batchRaycastFirst: invalid batch");
+  BatchHandle* b = (BatchHandle*)p;
+  View mw, mi, ig, o, d, ri;
+  if (!get_view(env, argv[1], &mw) || !get_view(env, argv[2], &mi) || !get_view(env, argv[3], &ig) ||
+      mw.type != napi_float32_array || mi.type != napi_float32_array || ig.type != napi_uint32_array ||
+      mw.length != 16 * ig.length || mi.length != mw.length)
+    return throw_msg(env, "batchRaycastFirst: matrices must be Float32Arrays of 16 per instance, instanceGeometry a Uint32Array");
+  if (!get_view(env, argv[4], &o) || !get_view(env, argv[5], &d) || !get_view(env, argv[6], &ri) ||
+      o.type != napi_float32_array || d.type != napi_float32_array || ri.type != napi_uint32_array ||
+      o.length != d.length || o.length % 3 || ri.length * 3 != o.length)
+    return throw_msg(env, "batchRaycastFirst: origins / directions (3n floats) and rayInstance (n) do not match");
+  const size_t n = ri.length;
+  int32_t side;
+  double nr, fr;
+  if (!get_i32(env, argv[7], &side) || !get_f64(env, argv[8], &nr) || !get_f64(env, argv[9], &fr))
+    return throw_msg(env, "batchRaycastFirst: side / near / far must be numbers");
+  void *face, *dist, *pt, *bc;
+  if (!out_view(env, argv[10], napi_uint32_array, n, &face) || !face ||
+      !out_view(env, argv[11], napi_float32_array, n, &dist) || !out_view(env, argv[12], napi_float32_array, 3 * n, &pt) ||
+      !out_view(env, argv[13], napi_float32_array, 3 * n, &bc))
+    return throw_msg(env, "batchRaycastFirst: output arrays have the wrong type or length");
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, b->device, &ctx);
+  if (!rc)
+    rc = mbvh_batch_raycast_first(ctx, b->batch, (const float*)mw.data, (const float*)mi.data, (const uint32_t*)ig.data,
+                                  (uint32_t)ig.length, (const float*)o.data, (const float*)d.data,
+                                  (const uint32_t*)ri.data, n, side, to_float(nr), to_float(fr), (uint32_t*)face,
+                                  (float*)dist, (float*)pt, (float*)bc);
+  if (rc) return throw_status(env, rc, "mbvh_batch_raycast_first");
   return NULL;
 }
 
@@ -580,6 +720,9 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("closestPoint", js_closest_point);
   EXPORT("raycast", js_raycast);
   EXPORT("raycastFirstBackward", js_raycast_first_bwd);
+  EXPORT("batchCreate", js_batch_create);
+  EXPORT("freeBatch", js_free_batch);
+  EXPORT("batchRaycastFirst", js_batch_raycast_first);
   EXPORT("closestPointBackward", js_closest_point_bwd);
   napi_add_env_cleanup_hook(env, cleanup_contexts, NULL);
   return exports;

@@ -36,6 +36,7 @@ EXPORTED_SYMBOLS = (
     "mbvh_raycast", "mbvh_raycast_fetch", "mbvh_raycast_count_d", "mbvh_raycast_fill_d",
     "mbvh_order_workspace_bytes", "mbvh_order_points_d",
     "mbvh_batch_create", "mbvh_batch_free", "mbvh_batch_workspace_bytes", "mbvh_batch_raycast_first_d",
+    "mbvh_batch_raycast_first",
     "mbvh_batch_autotune", "mbvh_batch_graph_create", "mbvh_batch_graph_launch", "mbvh_batch_graph_free",
     "mbvh_instance_rays_d", "mbvh_instance_hits_d",
     "mbvh_grad_workspace_bytes", "mbvh_raycast_first_bwd", "mbvh_raycast_first_bwd_d",
@@ -107,6 +108,8 @@ def library() -> C.CDLL:
         "mbvh_batch_workspace_bytes": (C.c_int, [vp, u64, C.POINTER(u64)]),
         "mbvh_batch_raycast_first_d": (C.c_int, [vp, vp, vp, vp, vp, u32, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp,
                                                  vp, vp, u64, C.POINTER(BatchTuning), vp]),
+        "mbvh_batch_raycast_first": (C.c_int, [vp, vp, vp, vp, vp, u32, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp,
+                                               vp]),
         "mbvh_batch_autotune": (C.c_int, [vp, vp, vp, vp, vp, u32, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp,
                                           vp, u64, C.POINTER(BatchTuning)]),
         "mbvh_batch_graph_create": (C.c_int, [vp, vp, vp, vp, vp, u32, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp,
@@ -483,6 +486,28 @@ class Batch:
         distance, point, barycoord, workspace (+ n_instances, n, workspace_bytes, side, near, far)."""
         _check(self.ctx._lib.mbvh_batch_raycast_first_d(*self._args(p), C.byref(tuning) if tuning else None,
                                                         C.c_void_p(stream or None)))
+
+    def raycast_first(self, matrix_world: np.ndarray, matrix_world_inverse: np.ndarray,
+                      instance_geometry: np.ndarray, origins: np.ndarray, directions: np.ndarray,
+                      ray_instance: np.ndarray, side: int = FRONT_SIDE, near: float = 0.0,
+                      far: float = float("inf")):
+        """Host arrays: column-major 4x4 matrices per instance, the batch slot of each instance's
+        mesh, world rays and the instance of each ray. Returns face / distance / point / barycoord."""
+        mw = np.ascontiguousarray(matrix_world, np.float32).reshape(-1, 16)
+        mi = np.ascontiguousarray(matrix_world_inverse, np.float32).reshape(-1, 16)
+        ig = np.ascontiguousarray(instance_geometry, np.uint32).reshape(-1)
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
+        ri = np.ascontiguousarray(ray_instance, np.uint32).reshape(-1)
+        if mw.shape != mi.shape or mw.shape[0] != ig.shape[0] or o.shape != d.shape or o.shape[0] != ri.shape[0]:
+            raise ValueError("inconsistent batched ray inputs")
+        n = o.shape[0]
+        out = {"face": np.empty(n, np.uint32), "distance": np.empty(n, np.float32),
+               "point": np.empty((n, 3), np.float32), "barycoord": np.empty((n, 3), np.float32)}
+        _check(self.ctx._lib.mbvh_batch_raycast_first(
+            self.ctx.handle, self.handle, _ptr(mw), _ptr(mi), _ptr(ig), mw.shape[0], _ptr(o), _ptr(d), _ptr(ri), n,
+            side, near, far, _ptr(out["face"]), _ptr(out["distance"]), _ptr(out["point"]), _ptr(out["barycoord"])))
+        return out
 
     def autotune(self, p: dict) -> BatchTuning:
         t = BatchTuning()
