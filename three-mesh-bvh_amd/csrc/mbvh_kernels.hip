@@ -32,6 +32,23 @@ constexpr float kInf = __builtin_huge_valf();
 __shared__ uint2 sh_stack[kStackLds * kBlock];
 __shared__ uint2 sh_cand[kCand * kBlock];
 
+// Stack entries through address-space-qualified pointers: LDS accesses compile to ds_* and
+// overflow accesses to global_* instructions. (Through generic pointers the compiler merges the
+// two branches of a push or pop into one flat access, which waits on both memory counters.) The
+// overflow load carries the nontemporal hint so it stays a distinct global load.
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+typedef __attribute__((address_space(1))) uint64_t glb_u64;
+__device__ __forceinline__ uint64_t pack_entry(uint2 v) { return static_cast<uint64_t>(v.x) | (static_cast<uint64_t>(v.y) << 32); }
+__device__ __forceinline__ uint2 unpack_entry(uint64_t v) {
+  return make_uint2(static_cast<uint32_t>(v), static_cast<uint32_t>(v >> 32));
+}
+__device__ __forceinline__ void lds_store(uint2* base, uint32_t i, uint2 v) { ((lds_u64*)base)[i] = pack_entry(v); }
+__device__ __forceinline__ uint2 lds_load(uint2* base, uint32_t i) { return unpack_entry(((lds_u64*)base)[i]); }
+__device__ __forceinline__ void ovf_store(uint2* base, uint64_t i, uint2 v) { ((glb_u64*)base)[i] = pack_entry(v); }
+__device__ __forceinline__ uint2 ovf_load(uint2* base, uint64_t i) {
+  return unpack_entry(__builtin_nontemporal_load((glb_u64*)base + i));
+}
+
 // Same as Stack, with the LDS part in a kernel-local array of kL entries per lane.
 template <int kL>
 struct StackL {
@@ -46,16 +63,16 @@ struct StackL {
   __device__ __forceinline__ bool push(uint32_t ref, uint32_t aux) {
     if (sp >= capacity) return false;
     if (sp < kL)
-      lds[sp * kBlock + lane] = make_uint2(ref, aux);
+      lds_store(lds, sp * kBlock + lane, make_uint2(ref, aux));
     else
-      ovf[static_cast<uint64_t>(sp - kL) * ovf_stride + gtid] = make_uint2(ref, aux);
+      ovf_store(ovf, static_cast<uint64_t>(sp - kL) * ovf_stride + gtid, make_uint2(ref, aux));
     sp++;
     return true;
   }
   __device__ __forceinline__ uint2 pop() {
     sp--;
-    if (sp < kL) return lds[sp * kBlock + lane];
-    return ovf[static_cast<uint64_t>(sp - kL) * ovf_stride + gtid];
+    if (sp < kL) return lds_load(lds, sp * kBlock + lane);
+    return ovf_load(ovf, static_cast<uint64_t>(sp - kL) * ovf_stride + gtid);
   }
 };
 
@@ -73,16 +90,16 @@ struct Stack {
   __device__ __forceinline__ bool push(uint32_t ref, uint32_t aux) {
     if (sp >= capacity) return false;
     if (sp < kStackLds)
-      sh_stack[sp * kBlock + lane] = make_uint2(ref, aux);
+      lds_store(sh_stack, sp * kBlock + lane, make_uint2(ref, aux));
     else
-      ovf[static_cast<uint64_t>(sp - kStackLds) * ovf_stride + gtid] = make_uint2(ref, aux);
+      ovf_store(ovf, static_cast<uint64_t>(sp - kStackLds) * ovf_stride + gtid, make_uint2(ref, aux));
     sp++;
     return true;
   }
   __device__ __forceinline__ uint2 pop() {
     sp--;
-    if (sp < kStackLds) return sh_stack[sp * kBlock + lane];
-    return ovf[static_cast<uint64_t>(sp - kStackLds) * ovf_stride + gtid];
+    if (sp < kStackLds) return lds_load(sh_stack, sp * kBlock + lane);
+    return ovf_load(ovf, static_cast<uint64_t>(sp - kStackLds) * ovf_stride + gtid);
   }
 };
 
@@ -139,6 +156,62 @@ __device__ __forceinline__ bool slab(const RayBox& r, float mnx, float mny, floa
   const float t1 = widen_hi(fminf(fminf(tx1, ty1), tz1));
   tmin = t0;
   return (t0 <= t1) && (t0 <= far_t) && (t1 >= near_t);
+}
+
+// The quad traversal's box test, for rays with no zero and no subnormal direction component (all
+// of 1/dir finite): one FMA per slab plane, b * (1/d) + (-o/d), with the interval widened per axis
+// by an absolute bound instead of the relative ulps above. With S = (B + |o|) |1/d| per axis
+// (B = largest |coordinate| of the mesh, so |b| <= B), each computed plane distance is within
+// 5 u S of (b - o) / d: u S from -o * (1/d), 2 u S from 1/d and the FMA, u S from folding the
+// widening into the offset, u S from the FMA's own rounding. The widening is 8 u S.
+// ------------------------------------------------------------------------------------------
+struct RayBox4 {
+  float ix, iy, iz;  // 1 / direction
+  float lx, ly, lz;  // -o / d - E (entry offsets)
+  float hx, hy, hz;  // -o / d + E (exit offsets)
+  bool sx, sy, sz;   // 1/dir >= 0
+};
+
+__device__ __forceinline__ RayBox4 ray_box4(f3 o, f3 d, float coord_max) {
+  RayBox4 r;
+  r.ix = 1.0f / d.x;
+  r.iy = 1.0f / d.y;
+  r.iz = 1.0f / d.z;
+  const float ox = -o.x * r.ix, oy = -o.y * r.iy, oz = -o.z * r.iz;
+  const float ex = 8.0f * kU * ((coord_max + fabsf(o.x)) * fabsf(r.ix));
+  const float ey = 8.0f * kU * ((coord_max + fabsf(o.y)) * fabsf(r.iy));
+  const float ez = 8.0f * kU * ((coord_max + fabsf(o.z)) * fabsf(r.iz));
+  r.lx = ox - ex;
+  r.ly = oy - ey;
+  r.lz = oz - ez;
+  r.hx = ox + ex;
+  r.hy = oy + ey;
+  r.hz = oz + ez;
+  r.sx = r.ix >= 0.0f;
+  r.sy = r.iy >= 0.0f;
+  r.sz = r.iz >= 0.0f;
+  return r;
+}
+
+// Rays the quad traversal does not take: a zero direction component (the reference's split-plane
+// shortcut is then needed literally; see trace_first), or slab constants that are not finite.
+__device__ __forceinline__ bool literal_ray(f3 o, f3 d, float coord_max) {
+  if (d.x == 0.0f || d.y == 0.0f || d.z == 0.0f) return true;
+  const RayBox4 r = ray_box4(o, d, coord_max);
+  const float m = fabsf(r.ix) + fabsf(r.iy) + fabsf(r.iz) + fabsf(r.lx) + fabsf(r.ly) + fabsf(r.lz) + fabsf(r.hx) +
+                  fabsf(r.hy) + fabsf(r.hz);
+  return !(m < kInf);
+}
+
+__device__ __forceinline__ bool slab4(const RayBox4& r, float4 c0, float4 c1, float near_t, float far_t,
+                                      float& tmin) {
+  // child box: c0 = (min.x, min.y, min.z, max.x), c1 = (max.y, max.z, ref, 0)
+  const float t0 = fmaxf(fmaxf(fmaf(r.sx ? c0.x : c0.w, r.ix, r.lx), fmaf(r.sy ? c0.y : c1.x, r.iy, r.ly)),
+                         fmaf(r.sz ? c0.z : c1.y, r.iz, r.lz));
+  const float t1 = fminf(fminf(fmaf(r.sx ? c0.w : c0.x, r.ix, r.hx), fmaf(r.sy ? c1.x : c0.y, r.iy, r.hy)),
+                         fmaf(r.sz ? c1.y : c0.z, r.iz, r.hz));
+  tmin = t0;
+  return (t0 <= t1) & (t0 <= far_t) & (t1 >= near_t);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -490,20 +563,24 @@ struct QuadStack {
   int capacity;
   int sp;
 
+  // entry `pos`, written or read by one lane
+  __device__ __forceinline__ void put(int pos, uint32_t ref, uint32_t aux) {
+    if (pos < kQuadLds)
+      lds_store(lds, pos * (kBlock / 4) + qid, make_uint2(ref, aux));
+    else
+      ovf_store(ovf, static_cast<uint64_t>(pos - kQuadLds) * ovf_stride + gquad, make_uint2(ref, aux));
+  }
+  __device__ __forceinline__ uint2 get(int pos) const {
+    if (pos < kQuadLds) return lds_load(lds, pos * (kBlock / 4) + qid);
+    return ovf_load(ovf, static_cast<uint64_t>(pos - kQuadLds) * ovf_stride + gquad);
+  }
+  // the same entry pushed / popped by all four lanes (binary traversal run by a quad)
   __device__ __forceinline__ bool push(uint32_t ref, uint32_t aux) {
     if (sp >= capacity) return false;
-    if (sp < kQuadLds)
-      lds[sp * (kBlock / 4) + qid] = make_uint2(ref, aux);
-    else
-      ovf[static_cast<uint64_t>(sp - kQuadLds) * ovf_stride + gquad] = make_uint2(ref, aux);
-    sp++;
+    put(sp++, ref, aux);
     return true;
   }
-  __device__ __forceinline__ uint2 pop() {
-    sp--;
-    if (sp < kQuadLds) return lds[sp * (kBlock / 4) + qid];
-    return ovf[static_cast<uint64_t>(sp - kQuadLds) * ovf_stride + gquad];
-  }
+  __device__ __forceinline__ uint2 pop() { return get(--sp); }
 };
 
 // Quad exchanges as DPP quad permutations (VALU operand modifiers) rather than LDS permutes.
@@ -540,19 +617,40 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32
   }
 }
 
+// One node step of a quad: lane q holds child q of the node, `in` saying whether it is entered,
+// with its key (entry t or box score, a lower bound) and ref. The entered children are ranked by
+// (key, lane) through three quad exchanges; the rank-0 child is returned to every lane (kDone if
+// none is entered) and each other entered child is written by its own lane onto the quad stack,
+// farthest deepest, so that pops come nearest first. Sets `overflow` (and returns kDone) when the
+// stack cannot take them.
+__device__ __forceinline__ uint32_t quad_children(QuadStack& st, uint32_t lane, bool in, float key, uint32_t cref,
+                                                  bool& overflow) {
+  const uint32_t q = lane & 3u;
+  const float kq = in ? key : __int_as_float(0x7fc00000);  // NaN: never ranks ahead
+  const float k1 = quad_xor1(kq), k2 = quad_xor2(kq), k3 = quad_dpp<0x1B>(kq);  // lanes q^1, q^2, q^3
+  // lane q^1 precedes q iff bit 0 of q is set; lanes q^2 and q^3 precede q iff bit 1 is set
+  const bool p1 = (q & 1u) != 0u, p2 = (q & 2u) != 0u;
+  const int rank = static_cast<int>(p1 ? (k1 <= kq) : (k1 < kq)) + static_cast<int>(p2 ? (k2 <= kq) : (k2 < kq)) +
+                   static_cast<int>(p2 ? (k3 <= kq) : (k3 < kq));
+  const uint64_t entered = __ballot(in);
+  const int nh = __popcll((entered >> ((lane & 63u) & ~3u)) & 0xFull);
+  if (nh == 0) return kDone;
+  uint32_t first = (in && rank == 0) ? cref : 0u;  // child refs are never 0 (node 0 is a root)
+  first |= quad_xor1(first);
+  first |= quad_xor2(first);
+  if (st.sp + nh - 1 > st.capacity) {
+    overflow = true;
+    return kDone;
+  }
+  if (in && rank > 0) st.put(st.sp + nh - 1 - rank, cref, __float_as_uint(key));
+  st.sp += nh - 1;
+  return first;
+}
+
 __device__ __forceinline__ FirstHit trace_first4(const MeshView& M, QuadStack& st, uint32_t lane, f3 o, f3 d,
                                                  uint32_t side_mask, float near_dist, float far_dist) {
   const uint32_t q = lane & 3u;
-  RayBox rb;
-  rb.ox = o.x;
-  rb.oy = o.y;
-  rb.oz = o.z;
-  rb.ix = 1.0f / d.x;
-  rb.iy = 1.0f / d.y;
-  rb.iz = 1.0f / d.z;
-  rb.sx = rb.ix >= 0.0f;
-  rb.sy = rb.iy >= 0.0f;
-  rb.sz = rb.iz >= 0.0f;
+  const RayBox4 rb = ray_box4(o, d, M.coord_max);
   const float dlen = sqrtf(dot(d, d));
   const float dl1 = fabsf(d.x) + fabsf(d.y) + fabsf(d.z);
   float U = kInf;  // quad-wide upper bound of the best hit's t (after each leaf)
@@ -571,20 +669,10 @@ __device__ __forceinline__ FirstHit trace_first4(const MeshView& M, QuadStack& s
         const float4 c1 = M.nodes4[8ull * ref + 2 * q + 1];
         const uint32_t cref = __float_as_uint(c1.z);
         float t;
-        const bool hit = cref != kDone && slab(rb, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, near_dist, far_dist, t) &&
-                         t <= U;
-        const float tq = hit ? t : kInf;
-        const uint32_t rq = hit ? cref : kDone;
-        float t0 = quad_bcast<0>(tq), t1 = quad_bcast<1>(tq), t2 = quad_bcast<2>(tq), t3 = quad_bcast<3>(tq);
-        uint32_t r0 = quad_bcast<0>(rq), r1 = quad_bcast<1>(rq), r2 = quad_bcast<2>(rq), r3 = quad_bcast<3>(rq);
-        // misses carry t = inf and sort last; among hits, nearest first
-        cswap(t0, r0, t1, r1);
-        cswap(t2, r2, t3, r3);
-        cswap(t0, r0, t2, r2);
-        cswap(t1, r1, t3, r3);
-        cswap(t1, r1, t2, r2);
-        if (r0 == kDone) {
-          ref = kDone;
+        const bool box = slab4(rb, c0, c1, near_dist, far_dist, t);
+        ref = quad_children(st, lane, (cref != kDone) & box & (t <= U), t, cref, overflow);
+        if (overflow) break;
+        if (ref == kDone) {
           while (st.sp > 0) {
             const uint2 e = st.pop();
             if (__uint_as_float(e.y) <= U) {
@@ -592,17 +680,6 @@ __device__ __forceinline__ FirstHit trace_first4(const MeshView& M, QuadStack& s
               break;
             }
           }
-        } else {
-          bool ok = true;
-          if (r3 != kDone) ok = ok && st.push(r3, __float_as_uint(t3));
-          if (r2 != kDone) ok = ok && st.push(r2, __float_as_uint(t2));
-          if (r1 != kDone) ok = ok && st.push(r1, __float_as_uint(t1));
-          if (!ok) {
-            overflow = true;
-            ref = kDone;
-            break;
-          }
-          ref = r0;
         }
       }
       // ---- leaves: the quad splits the triangles ----
@@ -709,7 +786,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_raycast_first4(RaycastFirstArgs A
     if (ray >= A.n) continue;
     const f3 o = mk(A.origins[3 * ray + 0], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
     const f3 d = mk(A.directions[3 * ray + 0], A.directions[3 * ray + 1], A.directions[3 * ray + 2]);
-    const bool literal = (d.x == 0.0f) || (d.y == 0.0f) || (d.z == 0.0f);
+    const bool literal = literal_ray(o, d, A.mesh.coord_max);
     const FirstHit h = literal ? trace_first(A.mesh, st, lane, o, d, A.side_mask, A.near_dist, A.far_dist)
                                : trace_first4(A.mesh, st, lane, o, d, A.side_mask, A.near_dist, A.far_dist);
     if ((lane & 3u) == 0) {
@@ -940,8 +1017,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_batch_raycast_first4(BatchArgs A)
         const f3 o = apply_matrix4(inv, ow);
         const f3 d = transform_direction(inv, mk(A.directions[3 * ray], A.directions[3 * ray + 1], A.directions[3 * ray + 2]));
         const float sc = instance_scale(A.world + 16ull * inst, d);
-        const bool literal = (d.x == 0.0f) || (d.y == 0.0f) || (d.z == 0.0f);
         const MeshView& M = A.meshes[geom];
+        const bool literal = literal_ray(o, d, M.coord_max);
         h = literal ? trace_first(M, st, tid, o, d, A.side_mask, A.near_dist / sc, A.far_dist / sc)
                     : trace_first4(M, st, tid, o, d, A.side_mask, A.near_dist / sc, A.far_dist / sc);
         if (h.face < kOverflow) {
@@ -1201,18 +1278,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
           const float4 c1 = M.nodes4[8ull * ref + 2 * q4 + 1];
           const uint32_t cref = __float_as_uint(c1.z);
           const float sc = box_score(q, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y) * kScoreSlack;
-          const bool in = cref != kDone && sc <= U && sc < max2;
-          const float sq = in ? sc : kInf;
-          const uint32_t rq = in ? cref : kDone;
-          float t0 = quad_bcast<0>(sq), t1 = quad_bcast<1>(sq), t2 = quad_bcast<2>(sq), t3 = quad_bcast<3>(sq);
-          uint32_t r0 = quad_bcast<0>(rq), r1 = quad_bcast<1>(rq), r2 = quad_bcast<2>(rq), r3 = quad_bcast<3>(rq);
-          cswap(t0, r0, t1, r1);
-          cswap(t2, r2, t3, r3);
-          cswap(t0, r0, t2, r2);
-          cswap(t1, r1, t3, r3);
-          cswap(t1, r1, t2, r2);
-          if (r0 == kDone) {
-            ref = kDone;
+          ref = quad_children(st, lane, (cref != kDone) & (sc <= U) & (sc < max2), sc, cref, overflow);
+          if (overflow) break;
+          if (ref == kDone) {
             while (st.sp > 0) {
               const uint2 e = st.pop();
               const float se = __uint_as_float(e.y);
@@ -1221,17 +1289,6 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
                 break;
               }
             }
-          } else {
-            bool ok = true;
-            if (r3 != kDone) ok = ok && st.push(r3, __float_as_uint(t3));
-            if (r2 != kDone) ok = ok && st.push(r2, __float_as_uint(t2));
-            if (r1 != kDone) ok = ok && st.push(r1, __float_as_uint(t1));
-            if (!ok) {
-              overflow = true;
-              ref = kDone;
-              break;
-            }
-            ref = r0;
           }
         }
         while (ref & kRefLeaf) {
