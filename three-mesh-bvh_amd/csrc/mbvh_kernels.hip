@@ -1158,8 +1158,13 @@ __global__ __launch_bounds__(kBlock) void k_raycast_all(RaycastAllArgs A) {
 
 // ------------------------------------------------------------------------------------------
 // Closest point on a triangle: Ericson, Real-Time Collision Detection §5.1.5 (the algorithm of
-// three's Triangle.closestPointToPoint), Voronoi regions tested in the same order.
+// three's Triangle.closestPointToPoint), Voronoi regions tested in the same order. This FP32
+// version only feeds error-bounded comparisons (the decisions are made in double-float), so its
+// divisions use the hardware reciprocal (1 ulp) and the face region clamps the three sub-areas at
+// 0: every branch returns a convex combination of a, b, c with weights valid to a few ulps.
 // ------------------------------------------------------------------------------------------
+__device__ __forceinline__ float rcp_approx(float x) { return __builtin_amdgcn_rcpf(x); }
+
 __device__ __forceinline__ f3 closest_on_triangle(f3 p, f3 a, f3 b, f3 c) {
   const f3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
   const float d1 = dot(ab, ap), d2 = dot(ac, ap);
@@ -1169,7 +1174,7 @@ __device__ __forceinline__ f3 closest_on_triangle(f3 p, f3 a, f3 b, f3 c) {
   if (d3 >= 0.0f && d4 <= d3) return b;
   const float vc = d1 * d4 - d3 * d2;
   if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
-    const float v = d1 / (d1 - d3);
+    const float v = d1 * rcp_approx(d1 - d3);
     return mk(a.x + ab.x * v, a.y + ab.y * v, a.z + ab.z * v);
   }
   const f3 cp = sub(p, c);
@@ -1177,26 +1182,31 @@ __device__ __forceinline__ f3 closest_on_triangle(f3 p, f3 a, f3 b, f3 c) {
   if (d6 >= 0.0f && d5 <= d6) return c;
   const float vb = d5 * d2 - d1 * d6;
   if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
-    const float w = d2 / (d2 - d6);
+    const float w = d2 * rcp_approx(d2 - d6);
     return mk(a.x + ac.x * w, a.y + ac.y * w, a.z + ac.z * w);
   }
   const float va = d3 * d6 - d5 * d4;
   if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
-    const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    const float w = (d4 - d3) * rcp_approx((d4 - d3) + (d5 - d6));
     const f3 bc = sub(c, b);
     return mk(b.x + bc.x * w, b.y + bc.y * w, b.z + bc.z * w);
   }
-  const float denom = 1.0f / (va + vb + vc);
-  const float v = vb * denom, w = vc * denom;
+  const float ua = fmaxf(va, 0.0f), ub = fmaxf(vb, 0.0f), uc = fmaxf(vc, 0.0f);
+  const float denom = rcp_approx(ua + ub + uc);  // all three 0: inf, then NaN, decided exactly
+  const float v = ub * denom, w = uc * denom;
   return mk(a.x + ab.x * v + ac.x * w, a.y + ab.y * v + ac.y * w, a.z + ab.z * v + ac.z * w);
 }
 
+// An upper bound of sqrt(x) for x >= 0 (hardware square root, 1 ulp, raised by 4 ulps).
+__device__ __forceinline__ float sqrt_up(float x) { return __builtin_amdgcn_sqrtf(x) * (1.0f + 4.0f * kU); }
+
 __device__ __forceinline__ float box_score(f3 q, float mnx, float mny, float mnz, float mxx,
                                            float mxy, float mxz) {
-  // distance^2 from q to the box (three Vector3.clamp then distanceToSquared)
-  const float cx = fmaxf(mnx, fminf(mxx, q.x));
-  const float cy = fmaxf(mny, fminf(mxy, q.y));
-  const float cz = fmaxf(mnz, fminf(mxz, q.z));
+  // distance^2 from q to the box (three Vector3.clamp then distanceToSquared); the median of
+  // (q, min, max) is the clamp for min <= max
+  const float cx = __builtin_amdgcn_fmed3f(q.x, mnx, mxx);
+  const float cy = __builtin_amdgcn_fmed3f(q.y, mny, mxy);
+  const float cz = __builtin_amdgcn_fmed3f(q.z, mnz, mxz);
   const float dx = cx - q.x, dy = cy - q.y, dz = cz - q.z;
   return dx * dx + dy * dy + dz * dz;
 }
@@ -1308,7 +1318,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
               add = true;  // FP32 failed (degenerate triangle): decide exactly at the end
               lb = -kInf;
             } else {
-              const float e2 = (2.0f * sqrtf(d2) + e_abs) * e_abs + 4.0f * kU * d2;
+              const float e2 = (2.0f * sqrt_up(d2) + e_abs) * e_abs + 4.0f * kU * d2;
               lb = d2 - e2;
               add = lb <= Ul && !(d2 == best && cp.x == best_cp.x && cp.y == best_cp.y && cp.z == best_cp.z);
               if (add) {
@@ -1488,7 +1498,7 @@ __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
               add = true;  // FP32 failed (degenerate triangle): decide exactly at the end
               lb = -kInf;
             } else {
-              const float e2 = (2.0f * sqrtf(d2) + e_abs) * e_abs + 4.0f * kU * d2;
+              const float e2 = (2.0f * sqrt_up(d2) + e_abs) * e_abs + 4.0f * kU * d2;
               lb = d2 - e2;
               // an exact copy of the best point (shared vertex) ties exactly: the earlier wins
               add = lb <= U && !(d2 == best && cp.x == best_cp.x && cp.y == best_cp.y && cp.z == best_cp.z);
