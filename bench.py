@@ -76,6 +76,23 @@ def timed_kernel(torch, launch, steps, warmup, stream, pre=None):
     return wall, kern
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "closest_point4_torusknot320k_grid128.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the closest-point kernel, from the committed counter summary: the
+    FETCH_SIZE and WRITE_SIZE of `rocprofv3 --pmc` runs of this workload, each scaled by what the
+    same counter reads for a 256 MiB device copy (None if the summary is absent)."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -362,8 +379,11 @@ def main():
         tris = float(ref["tris"].mean())
         bq = 32.0 + 32.0 * nodes + (3 * IDX_BYTES + 36) * tris
         achieved = n * bq / (kern_ms / 1000.0) / 1e9
+        traffic = profiled_traffic()
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": round(traffic) if traffic else None,
+                "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
+                "algorithmic_bytes_per_launch": round(n * bq),
                 "bytes_per_query": round(bq, 1), "nodes_per_query": round(nodes, 2),
                 "tris_per_query": round(tris, 2), "kernel_ms": round(kern_ms, 4)}
         cpu = {"value": round(sample.shape[0] / cpu_s / 1e6, 4), "unit": "Mqueries/s", "cores": threads,

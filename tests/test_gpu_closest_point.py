@@ -115,6 +115,30 @@ def test_degenerate_first_candidate(ctx):
     assert np.all(got["face"] == ref["face"])
 
 
+def _morton_buckets(pts):
+    """Bucket of each point as the ordering pass computes it: cells of the finite bounds, 1..7 bits
+    per axis (about 8 points per bucket), a NaN coordinate in cell 0 and an infinite one clamped to the
+    end cell, bits interleaved x, y, z."""
+    n = pts.shape[0]
+    bits = 1
+    while bits < 7 and (8 << (3 * bits)) < n:
+        bits += 1
+    cells = 1 << bits
+    key = np.zeros(n, np.uint64)
+    for k in range(3):
+        v = pts[:, k]
+        fin = v[np.isfinite(v)]
+        lo, hi = np.float32(fin.min()), np.float32(fin.max())
+        ext = np.float32(hi - lo)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            t = (v - lo) / ext if ext > 0 else np.zeros(n, np.float32)
+        t = np.where(np.isnan(t), np.float32(0), np.clip(t, 0, 1)).astype(np.float32)
+        c = np.minimum((t * np.float32(cells)).astype(np.uint64), cells - 1)
+        for b in range(bits):
+            key |= ((c >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + k)
+    return key
+
+
 def test_query_order_changes_nothing(ctx):
     """The spatial visit order is a permutation, and results with it equal results without it bit for
     bit (each query's result depends on that query only)."""
@@ -146,6 +170,9 @@ def test_query_order_changes_nothing(ctx):
     o = order.cpu().numpy().view(np.uint32)
     assert np.array_equal(np.sort(o), np.arange(n, dtype=np.uint32))
     assert not np.array_equal(o, np.arange(n, dtype=np.uint32))
+    # the visit order is sorted by Morton bucket (same FP32 cell arithmetic on the host)
+    keys = _morton_buckets(pts)
+    assert np.all(np.diff(keys[o].astype(np.int64)) >= 0)
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
     # the host entry point (which orders internally at this size) gives the same results

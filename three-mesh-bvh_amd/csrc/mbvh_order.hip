@@ -10,8 +10,9 @@
 //   init     bounds <- (+inf, -inf), bucket counts <- 0
 //   bounds   min / max of the finite coordinates (wave reduction, one atomic per wave and axis)
 //   hist     bucket of each query (kept) and count per Morton bucket
-//   scan     exclusive prefix sum of the counts (one workgroup)
-//   scatter  order[cursor[bucket]++] = query index, from the kept buckets
+//   scan     exclusive prefix sum of the counts within chunks of kScanChunk buckets (one
+//            workgroup per chunk), then of the chunk totals (one workgroup)
+//   scatter  order[chunk base + cursor[bucket]++] = query index, from the kept buckets
 // Within a bucket the order depends on atomic timing; it carries no meaning.
 #include <algorithm>
 
@@ -21,7 +22,10 @@ namespace mbvh {
 namespace {
 
 constexpr int kOrderBlock = 256;
-constexpr int kScanBlock = 1024;
+constexpr int kScanBlock = 256;
+constexpr int kScanPer = 16;                         // buckets per thread of a chunk
+constexpr uint32_t kScanChunk = kScanBlock * kScanPer;  // buckets per chunk
+constexpr uint32_t kMaxChunks = 1024;                 // chunk totals are scanned by one workgroup
 
 // Total order on float bits as unsigned integers (negative values reversed).
 __device__ __forceinline__ uint32_t ordered_bits(float f) {
@@ -50,7 +54,7 @@ __device__ __forceinline__ uint32_t bucket_of(const float* p, const uint32_t* bo
     const float hi = from_ordered_bits(bounds[3 + k]);
     const float ext = hi - lo;
     float t = (ext > 0.0f) ? (p[k] - lo) / ext : 0.0f;
-    t = (t == t) ? fminf(fmaxf(t, 0.0f), 1.0f) : 0.0f;  // NaN / inf coordinates go to cell 0
+    t = (t == t) ? fminf(fmaxf(t, 0.0f), 1.0f) : 0.0f;  // NaN goes to cell 0, +-inf to an end cell
     c[k] = min(static_cast<uint32_t>(t * static_cast<float>(cells)), cells - 1u);
   }
   return spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
@@ -112,37 +116,70 @@ __global__ __launch_bounds__(kOrderBlock) void k_order_hist(const float* p, uint
   }
 }
 
-// Exclusive scan of counts[0, n_buckets) in place; one workgroup, each thread a contiguous run.
-__global__ __launch_bounds__(kScanBlock) void k_order_scan(uint32_t* counts, uint32_t n_buckets) {
-  __shared__ uint32_t part[kScanBlock];
-  const uint32_t per = (n_buckets + kScanBlock - 1) / kScanBlock;
-  const uint32_t b0 = threadIdx.x * per;
-  const uint32_t b1 = min(b0 + per, n_buckets);
-  uint32_t s = 0;
-  for (uint32_t b = b0; b < b1; b++) s += counts[b];
-  part[threadIdx.x] = s;
+// Exclusive prefix sum over the kScanBlock threads of a workgroup (block-wide, LDS for the waves).
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wave_sums[kScanBlock / 64];
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(x), off, 64));
+    if (lane >= static_cast<uint32_t>(off)) x += y;
+  }
+  if (lane == 63u) wave_sums[w] = x;
   __syncthreads();
-  // Hillis-Steele inclusive scan over the per-thread sums
-  for (uint32_t off = 1; off < kScanBlock; off <<= 1) {
-    const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (uint32_t k = 0; k < kScanBlock / 64; k++) {
+    if (k < w) before += wave_sums[k];
+    all += wave_sums[k];
   }
-  uint32_t run = part[threadIdx.x] - s;
-  for (uint32_t b = b0; b < b1; b++) {
-    const uint32_t c = counts[b];
-    counts[b] = run;
-    run += c;
-  }
+  *total = all;
+  return before + x - v;
 }
 
+// Exclusive scan of each chunk of kScanChunk counts in place (thread t: a contiguous run of
+// kScanPer buckets); the chunk's total goes to chunk_sums[chunk].
+__global__ __launch_bounds__(kScanBlock) void k_order_scan_chunks(uint32_t* counts, uint32_t n_buckets,
+                                                                  uint32_t* chunk_sums) {
+  const uint32_t b0 = blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+  uint32_t c[kScanPer];
+  uint32_t s = 0;
+  for (int k = 0; k < kScanPer; k++) {
+    c[k] = b0 + k < n_buckets ? counts[b0 + k] : 0u;
+    s += c[k];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(s, &total);
+  for (int k = 0; k < kScanPer; k++) {
+    if (b0 + k < n_buckets) counts[b0 + k] = run;
+    run += c[k];
+  }
+  if (threadIdx.x == 0) chunk_sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of the chunk totals in place (one workgroup, n_chunks <= kMaxChunks).
+__global__ __launch_bounds__(kScanBlock) void k_order_scan_sums(uint32_t* chunk_sums, uint32_t n_chunks) {
+  constexpr uint32_t kPer = kMaxChunks / kScanBlock;
+  const uint32_t b0 = threadIdx.x * kPer;
+  uint32_t c[kPer];
+  uint32_t s = 0;
+  for (uint32_t k = 0; k < kPer; k++) {
+    c[k] = b0 + k < n_chunks ? chunk_sums[b0 + k] : 0u;
+    s += c[k];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(s, &total);
+  for (uint32_t k = 0; k < kPer; k++) {
+    if (b0 + k < n_chunks) chunk_sums[b0 + k] = run;
+    run += c[k];
+  }
+}
 // Every bucket's cursor range holds exactly the queries counted into it, so pos < n.
 __global__ __launch_bounds__(kOrderBlock) void k_order_scatter(const uint32_t* keys, uint64_t n, uint32_t* cursor,
-                                                               uint32_t* order) {
+                                                               const uint32_t* chunk_base, uint32_t* order) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kOrderBlock;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kOrderBlock + threadIdx.x; i < n; i += stride) {
-    const uint32_t pos = atomicAdd(&cursor[keys[i]], 1u);
+    const uint32_t key = keys[i];
+    const uint32_t pos = chunk_base[key / kScanChunk] + atomicAdd(&cursor[key], 1u);
     order[pos] = static_cast<uint32_t>(i);
   }
 }
@@ -187,9 +224,9 @@ uint32_t order_axis_bits(uint64_t n) {
   return bits;
 }
 
-// header (bounds) | bucket counts | per-query bucket
+// header (bounds) | bucket counts | chunk totals | per-query bucket
 uint64_t order_workspace_bytes(uint64_t n) {
-  return kOrderHeaderBytes + (uint64_t{4} << (3 * order_axis_bits(n))) + 4 * n;
+  return kOrderHeaderBytes + (uint64_t{4} << (3 * order_axis_bits(n))) + 4 * kMaxChunks + 4 * n;
 }
 
 hipError_t launch_order_points(const float* points, uint64_t n, uint32_t* order, void* workspace, hipStream_t s) {
@@ -198,14 +235,18 @@ hipError_t launch_order_points(const float* points, uint64_t n, uint32_t* order,
   const uint32_t n_buckets = 1u << (3 * axis_bits);
   uint32_t* bounds = static_cast<uint32_t*>(workspace);
   uint32_t* counts = reinterpret_cast<uint32_t*>(static_cast<char*>(workspace) + kOrderHeaderBytes);
-  uint32_t* keys = counts + n_buckets;
+  uint32_t* chunk_sums = counts + n_buckets;
+  uint32_t* keys = chunk_sums + kMaxChunks;
+  const uint32_t n_chunks = (n_buckets + kScanChunk - 1) / kScanChunk;
+  static_assert((uint64_t{1} << 21) / kScanChunk <= kMaxChunks, "chunk totals fit one workgroup");
   const uint32_t init_blocks = std::max<uint32_t>(1, std::min<uint32_t>(1024, (n_buckets + kOrderBlock - 1) / kOrderBlock));
   const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(4096, (n + kOrderBlock - 1) / kOrderBlock));
   hipLaunchKernelGGL(k_order_init, dim3(init_blocks), dim3(kOrderBlock), 0, s, bounds, counts, n_buckets);
   hipLaunchKernelGGL(k_order_bounds, dim3(std::min<uint32_t>(blocks, 512)), dim3(kOrderBlock), 0, s, points, n, bounds);
   hipLaunchKernelGGL(k_order_hist, dim3(blocks), dim3(kOrderBlock), 0, s, points, n, bounds, axis_bits, counts, keys);
-  hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(kScanBlock), 0, s, counts, n_buckets);
-  hipLaunchKernelGGL(k_order_scatter, dim3(blocks), dim3(kOrderBlock), 0, s, keys, n, counts, order);
+  hipLaunchKernelGGL(k_order_scan_chunks, dim3(n_chunks), dim3(kScanBlock), 0, s, counts, n_buckets, chunk_sums);
+  hipLaunchKernelGGL(k_order_scan_sums, dim3(1), dim3(kScanBlock), 0, s, chunk_sums, n_chunks);
+  hipLaunchKernelGGL(k_order_scatter, dim3(blocks), dim3(kOrderBlock), 0, s, keys, n, counts, chunk_sums, order);
   return hipGetLastError();
 }
 
