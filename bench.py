@@ -14,8 +14,9 @@ The JSON line carries:
                 the node / triangle counts of the reference traversal order measured by the FP64
                 oracle on a sample) divided by the kernel's average duration (HIP events on the
                 launch stream), against the 8 TB/s HBM peak.
-  cpu_baseline  the FP64 oracle's restatement of the reference traversal, timed on this host's
-                cores over a bounded sample of the same queries.
+  cpu_baseline  the reference side timed on this host: an FP64 JavaScript restatement of the
+                reference's closestPointToPoint (oracle/js) in Node, a worker pool over every query
+                (`value`) and one thread over a strided sample; the C oracle's rate beside it.
 """
 from __future__ import annotations
 
@@ -91,6 +92,44 @@ def profiled_traffic():
     except OSError:
         pass
     return None
+
+
+def js_cpu_baseline(kind, positions, index, roots, queries, threads, single, **meta):
+    """Times the FP64 JavaScript restatement of the reference query path (oracle/js) in Node on this
+    host: one thread over `single` queries (the first rows of `queries`), and a worker pool of
+    `threads` over all of them. Returns Node's JSON report plus the single-thread faces, or None when
+    Node is absent."""
+    import shutil
+    import subprocess
+    import tempfile
+    import numpy as np
+    node = shutil.which("node")
+    if node is None:
+        return None
+    d = tempfile.mkdtemp(prefix="mbvh_cpu_")
+    try:
+        idx = np.ascontiguousarray(index)
+        ib = 2 if idx.dtype == np.uint16 else 4
+        np.ascontiguousarray(positions, np.float32).tofile(os.path.join(d, "positions.f32"))
+        idx.astype(np.uint16 if ib == 2 else np.uint32).tofile(os.path.join(d, "index.u16" if ib == 2 else "index.u32"))
+        for i, r in enumerate(roots):
+            with open(os.path.join(d, f"root{i}.bin"), "wb") as f:
+                f.write(bytes(r))
+        np.ascontiguousarray(queries, np.float32).tofile(os.path.join(d, "queries.f32"))
+        n = queries.shape[0] if kind == "closest" else queries.shape[0] // 2
+        enc = {k: ("inf" if v == float("inf") else v) for k, v in meta.items()}  # JSON has no infinity
+        with open(os.path.join(d, "meta.json"), "w") as f:
+            json.dump(dict(kind=kind, n=int(n), roots=len(roots), index_bytes=ib, **enc), f)
+        r = subprocess.run([node, os.path.join(ROOT, "oracle", "js", "cpu_bench.js"), d, str(threads), str(single)],
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            log(f"node CPU baseline failed: {r.stderr[-500:]}")
+            return None
+        rep = json.loads(r.stdout.strip().splitlines()[-1])
+        rep["faces"] = np.fromfile(os.path.join(d, "faces_single.u32"), np.uint32)
+        return rep
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 def log(msg):
@@ -371,7 +410,7 @@ def main():
 
         om = O.OracleMesh(g.positions, idx, tree.roots)
         sample = w.points[:: args.cpu_sample]
-        threads = os.cpu_count() or 1
+        threads = min(16, os.cpu_count() or 1)  # this host's CPU share for one GPU
         t0 = time.perf_counter()
         ref = om.closest_point(sample, threads=threads)
         cpu_s = time.perf_counter() - t0
@@ -386,10 +425,37 @@ def main():
                 "algorithmic_bytes_per_launch": round(n * bq),
                 "bytes_per_query": round(bq, 1), "nodes_per_query": round(nodes, 2),
                 "tris_per_query": round(tris, 2), "kernel_ms": round(kern_ms, 4)}
-        cpu = {"value": round(sample.shape[0] / cpu_s / 1e6, 4), "unit": "Mqueries/s", "cores": threads,
-               "kind": "port",
-               "sample": f"every {args.cpu_sample}th grid point ({sample.shape[0]} queries), FP64 oracle "
-                         f"restatement of the reference traversal, {threads} threads"}
+        oracle_rate = sample.shape[0] / cpu_s / 1e6
+        # the reference side in its own language: the JS restatement in Node, single thread over a
+        # strided sample, then a worker pool over every query
+        single = w.points[:: max(1, n // 32768)][:32768]
+        log("CPU baseline (Node)")
+        js = js_cpu_baseline("closest", g.positions, idx, tree.roots, np.concatenate([single, w.points]), threads,
+                             single.shape[0], min_threshold=0.0, max_threshold=float("inf"), side=0, near=0.0,
+                             far=float("inf"))
+        if js is not None:
+            gf = d_face.cpu().numpy().view(np.uint32)[:: max(1, n // 32768)][:32768]
+            # where the faces differ, both must be at the same FP64 distance (an exact tie, e.g. a
+            # shared vertex: the reference keeps the first in its visit order)
+            tie_ok = np.ones(single.shape[0], bool)
+            for i in np.flatnonzero(js["faces"] != gf):
+                da = om.point_face_distance(int(js["faces"][i]), single[i])
+                db = om.point_face_distance(int(gf[i]), single[i])
+                tie_ok[i] = abs(da - db) <= 1e-6 * max(da, 1e-30)
+            cpu = {"value": round(js["pool"]["queries_per_s"] / 1e6, 4),
+                   "unit": "Mqueries/s", "cores": threads, "kind": "port",
+                   "sample": f"FP64 JavaScript restatement of the reference closestPointToPoint (oracle/js) in "
+                             f"Node {js['node']} on {js['cpu']}: worker pool of {threads} over all {n} grid points "
+                             f"and the {single.shape[0]} single-thread ones, median of 3 rounds after a prewarm",
+                   "single_thread_mqueries_per_s": round(js["single"]["queries_per_s"] / 1e6, 5),
+                   "single_thread_sample": f"{single.shape[0]} points (every {max(1, n // 32768)}th)",
+                   "face_agreement_with_gpu": round(float((js["faces"] == gf).mean()), 6),
+                   "face_or_exact_tie_agreement_with_gpu": round(float(tie_ok.mean()), 6),
+                   "oracle_c_mqueries_per_s": round(oracle_rate, 4)}
+        else:
+            cpu = {"value": round(oracle_rate, 4), "unit": "Mqueries/s", "cores": threads, "kind": "port",
+                   "sample": f"every {args.cpu_sample}th grid point ({sample.shape[0]} queries), FP64 C oracle "
+                             f"restatement of the reference traversal, {threads} threads (Node absent)"}
         # raycast parity spot check + counts for the secondary line
         om1 = O.OracleMesh(g1.positions, idx1, tree1.roots)
         r1 = om1.raycast_first(w1.origins, w1.directions, threads=threads)
@@ -397,6 +463,16 @@ def main():
         extra["raycast_face_agreement"] = float((gf == r1["face"]).mean())
         extra["raycast_nodes_per_ray"] = round(float(r1["nodes"].mean()), 2)
         extra["raycast_tris_per_ray"] = round(float(r1["tris"].mean()), 2)
+        js1 = js_cpu_baseline("raycast", g1.positions, idx1, tree1.roots, np.concatenate([w1.origins, w1.directions]),
+                              threads, nr, min_threshold=0.0, max_threshold=float("inf"), side=0, near=0.0,
+                              far=float("inf"))
+        if js1 is not None:
+            extra["cpu_baseline"] = {
+                "single_thread_mrays_per_s": round(js1["single"]["queries_per_s"] / 1e6, 5),
+                "pool_mrays_per_s": round(js1["pool"]["queries_per_s"] / 1e6, 4), "cores": threads,
+                "kind": "port", "sample": f"all {nr} rays, FP64 JavaScript restatement of the reference "
+                                          f"raycastFirst (oracle/js) in Node {js1['node']}",
+                "face_agreement_with_gpu": round(float((js1["faces"] == gf).mean()), 6)}
     line = {
         "metric": "Mrays/s raycastFirst, Mqueries/s closestPoint, fwd+bwd/fwd ratio, 1 MI355X",
         "value": round(value, 3),

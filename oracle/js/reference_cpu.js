@@ -1,0 +1,188 @@
+'use strict';
+// FP64 JavaScript restatement of the reference's CPU query path, for timing the reference side
+// beside the GPU. TEST INFRASTRUCTURE ONLY: run by bench.py's CPU-baseline leg and by tests/, never
+// by the product.
+//
+// It follows the same files as oracle/oracle.c, in JavaScript doubles (JS never fuses multiply-adds,
+// so every operation rounds as the reference's does):
+//   raycastFirst   src/core/cast/raycastFirst.template.js:19-103 (ordered recursion, split-plane
+//                  shortcut), intersectsNodeBounds src/core/utils/intersectUtils.js:2-78,
+//                  intersectClosestTri src/core/utils/iterationUtils.template.js:24-53, three's
+//                  Ray.intersectTriangle (GeometricTools IntrRay3Triangle3) as called by
+//                  src/utils/ThreeRayIntersectUtilities.js:21-47, roots combined with strict <
+//                  (src/core/MeshBVH.js:558-577)
+//   closestPoint   src/core/cast/closestPointToPoint.js:6-78 over shapecastTraverse
+//                  (src/core/cast/shapecast.js:41-214): children by box distance^2, right first only
+//                  if strictly closer, score < best^2 and < maxThreshold^2, strict < triangle update,
+//                  minThreshold exit; three's Triangle.closestPointToPoint (Ericson RTCD 5.1.5)
+// Node buffers are the reference's 32-byte format (src/core/Constants.js:61-66): float32 bounds,
+// u32[6] right offset or first triangle, u32[7] split axis, u16[14] count, u16[15] 0xFFFF for leaves.
+// The reference allocates result objects and Vector3 temporaries per call; this restatement does
+// not, so it is at least as fast as the reference's own code on the same machine.
+
+const FRONT = 0, BACK = 1, DOUBLE = 2;
+
+function makeMesh(positions, index, roots) {
+  return {
+    pos: positions,
+    index,
+    roots: roots.map(buf => ({ f32: new Float32Array(buf), u32: new Uint32Array(buf), u16: new Uint16Array(buf) })),
+  };
+}
+
+// ---- ray -------------------------------------------------------------------------------------
+function boundsHit(f, n, r) {
+  const b = 8 * n;
+  let tmin, tmax, tymin, tymax, tzmin, tzmax;
+  if (r.ix >= 0) { tmin = (f[b] - r.ox) * r.ix; tmax = (f[b + 3] - r.ox) * r.ix; } else { tmin = (f[b + 3] - r.ox) * r.ix; tmax = (f[b] - r.ox) * r.ix; }
+  if (r.iy >= 0) { tymin = (f[b + 1] - r.oy) * r.iy; tymax = (f[b + 4] - r.oy) * r.iy; } else { tymin = (f[b + 4] - r.oy) * r.iy; tymax = (f[b + 1] - r.oy) * r.iy; }
+  if (tmin > tymax || tymin > tmax) return false;
+  if (tymin > tmin || tmin !== tmin) tmin = tymin;
+  if (tymax < tmax || tmax !== tmax) tmax = tymax;
+  if (r.iz >= 0) { tzmin = (f[b + 2] - r.oz) * r.iz; tzmax = (f[b + 5] - r.oz) * r.iz; } else { tzmin = (f[b + 5] - r.oz) * r.iz; tzmax = (f[b + 2] - r.oz) * r.iz; }
+  if (tmin > tzmax || tzmin > tmax) return false;
+  if (tzmin > tmin || tmin !== tmin) tmin = tzmin;
+  if (tzmax < tmax || tmax !== tmax) tmax = tzmax;
+  return tmin <= r.far && tmax >= r.near;
+}
+
+// Ray / triangle; on a hit fills r.hx, r.hy, r.hz (the point) and returns the distance, else -1.
+function rayTri(r, ax, ay, az, bx, by, bz, cx, cy, cz, cull) {
+  const e1x = bx - ax, e1y = by - ay, e1z = bz - az, e2x = cx - ax, e2y = cy - ay, e2z = cz - az;
+  const nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+  let ddn = r.dx * nx + r.dy * ny + r.dz * nz;
+  let sign;
+  if (ddn > 0) { if (cull) return -1; sign = 1; } else if (ddn < 0) { sign = -1; ddn = -ddn; } else return -1;
+  const qx = r.ox - ax, qy = r.oy - ay, qz = r.oz - az;
+  const t1x = qy * e2z - qz * e2y, t1y = qz * e2x - qx * e2z, t1z = qx * e2y - qy * e2x;
+  const e0 = sign * (r.dx * t1x + r.dy * t1y + r.dz * t1z);
+  if (e0 < 0) return -1;
+  const t2x = e1y * qz - e1z * qy, t2y = e1z * qx - e1x * qz, t2z = e1x * qy - e1y * qx;
+  const e1 = sign * (r.dx * t2x + r.dy * t2y + r.dz * t2z);
+  if (e1 < 0) return -1;
+  if (e0 + e1 > ddn) return -1;
+  const qdn = -sign * (qx * nx + qy * ny + qz * nz);
+  if (qdn < 0) return -1;
+  const t = qdn / ddn;
+  r.hx = r.ox + r.dx * t; r.hy = r.oy + r.dy * t; r.hz = r.oz + r.dz * t;
+  const dx = r.ox - r.hx, dy = r.oy - r.hy, dz = r.oz - r.hz;
+  const dist = Math.sqrt(dx * dx + dy * dy + dz * dz);
+  return dist < r.near || dist > r.far ? -1 : dist;
+}
+
+function leafFirst(m, r, off, cnt) {
+  const p = m.pos, ix = m.index;
+  let best = -1, bestDist = Infinity, px = 0, py = 0, pz = 0;
+  for (let s = off; s < off + cnt; s++) {
+    const a = 3 * ix[3 * s], b = 3 * ix[3 * s + 1], c = 3 * ix[3 * s + 2];
+    let d;
+    if (r.side === BACK) d = rayTri(r, p[c], p[c + 1], p[c + 2], p[b], p[b + 1], p[b + 2], p[a], p[a + 1], p[a + 2], true);
+    else d = rayTri(r, p[a], p[a + 1], p[a + 2], p[b], p[b + 1], p[b + 2], p[c], p[c + 1], p[c + 2], r.side === FRONT);
+    if (d >= 0 && d < bestDist) { best = s; bestDist = d; px = r.hx; py = r.hy; pz = r.hz; }
+  }
+  if (best < 0) return null;
+  return { face: best, dist: bestDist, p: [px, py, pz] };
+}
+
+function firstNode(m, R, r, n) {
+  if (R.u16[16 * n + 15] === 0xFFFF) return leafFirst(m, r, R.u32[8 * n + 6], R.u16[16 * n + 14]);
+  const axis = R.u32[8 * n + 7];
+  const d = axis === 0 ? r.dx : axis === 1 ? r.dy : r.dz;
+  const ltr = d >= 0;
+  const left = n + 1, right = n + R.u32[8 * n + 6];
+  const c1 = ltr ? left : right, c2 = ltr ? right : left;
+  const h1 = boundsHit(R.f32, c1, r) ? firstNode(m, R, r, c1) : null;
+  if (h1) {
+    const pv = h1.p[axis];
+    const outside = ltr ? pv <= R.f32[8 * c2 + axis] : pv >= R.f32[8 * c2 + axis + 3];
+    if (outside) return h1;
+  }
+  const h2 = boundsHit(R.f32, c2, r) ? firstNode(m, R, r, c2) : null;
+  if (h1 && h2) return h1.dist <= h2.dist ? h1 : h2;
+  return h1 || h2;
+}
+
+function raycastFirst(m, ox, oy, oz, dx, dy, dz, side, near, far) {
+  const r = { ox, oy, oz, dx, dy, dz, ix: 1 / dx, iy: 1 / dy, iz: 1 / dz, side, near, far, hx: 0, hy: 0, hz: 0 };
+  let best = null;
+  for (const R of m.roots) {
+    const h = firstNode(m, R, r, 0);
+    if (h && (!best || h.dist < best.dist)) best = h;
+  }
+  return best;
+}
+
+// ---- closest point -----------------------------------------------------------------------------
+function boxScore(f, n, px, py, pz) {
+  const b = 8 * n;
+  const x = Math.max(f[b], Math.min(f[b + 3], px)) - px;
+  const y = Math.max(f[b + 1], Math.min(f[b + 4], py)) - py;
+  const z = Math.max(f[b + 2], Math.min(f[b + 5], pz)) - pz;
+  return x * x + y * y + z * z;
+}
+
+// Squared distance from (px, py, pz) to the triangle's closest point (Ericson's regions in order).
+function triDistSq(px, py, pz, ax, ay, az, bx, by, bz, cx, cy, cz) {
+  const abx = bx - ax, aby = by - ay, abz = bz - az, acx = cx - ax, acy = cy - ay, acz = cz - az;
+  const apx = px - ax, apy = py - ay, apz = pz - az;
+  const d1 = abx * apx + aby * apy + abz * apz, d2 = acx * apx + acy * apy + acz * apz;
+  let qx, qy, qz;
+  if (d1 <= 0 && d2 <= 0) { qx = ax; qy = ay; qz = az; } else {
+    const bpx = px - bx, bpy = py - by, bpz = pz - bz;
+    const d3 = abx * bpx + aby * bpy + abz * bpz, d4 = acx * bpx + acy * bpy + acz * bpz;
+    const vc = d1 * d4 - d3 * d2;
+    if (d3 >= 0 && d4 <= d3) { qx = bx; qy = by; qz = bz; } else if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+      const v = d1 / (d1 - d3);
+      qx = ax + abx * v; qy = ay + aby * v; qz = az + abz * v;
+    } else {
+      const cpx = px - cx, cpy = py - cy, cpz = pz - cz;
+      const d5 = abx * cpx + aby * cpy + abz * cpz, d6 = acx * cpx + acy * cpy + acz * cpz;
+      const vb = d5 * d2 - d1 * d6;
+      if (d6 >= 0 && d5 <= d6) { qx = cx; qy = cy; qz = cz; } else if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+        const w = d2 / (d2 - d6);
+        qx = ax + acx * w; qy = ay + acy * w; qz = az + acz * w;
+      } else {
+        const va = d3 * d6 - d5 * d4;
+        if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+          const w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+          qx = bx + (cx - bx) * w; qy = by + (cy - by) * w; qz = bz + (cz - bz) * w;
+        } else {
+          const denom = 1 / (va + vb + vc);
+          const v = vb * denom, w = vc * denom;
+          qx = (ax + abx * v) + acx * w; qy = (ay + aby * v) + acy * w; qz = (az + abz * v) + acz * w;
+        }
+      }
+    }
+  }
+  const x = px - qx, y = py - qy, z = pz - qz;
+  return x * x + y * y + z * z;
+}
+
+function pointLeaf(m, c, off, cnt) {
+  const p = m.pos, ix = m.index;
+  for (let s = off; s < off + cnt; s++) {
+    const a = 3 * ix[3 * s], b = 3 * ix[3 * s + 1], cc = 3 * ix[3 * s + 2];
+    const d2 = triDistSq(c.px, c.py, c.pz, p[a], p[a + 1], p[a + 2], p[b], p[b + 1], p[b + 2], p[cc], p[cc + 1], p[cc + 2]);
+    if (d2 < c.best) { c.best = d2; c.face = s; }
+    if (d2 < c.min2) return true;
+  }
+  return false;
+}
+
+function pointNode(m, R, c, n) {
+  if (R.u16[16 * n + 15] === 0xFFFF) return pointLeaf(m, c, R.u32[8 * n + 6], R.u16[16 * n + 14]);
+  let c1 = n + 1, c2 = n + R.u32[8 * n + 6];
+  let s1 = boxScore(R.f32, c1, c.px, c.py, c.pz), s2 = boxScore(R.f32, c2, c.px, c.py, c.pz);
+  if (s2 < s1) { const t = c1; c1 = c2; c2 = t; const u = s1; s1 = s2; s2 = u; }
+  if (s1 < c.best && s1 < c.max2 && pointNode(m, R, c, c1)) return true;
+  if (s2 < c.best && s2 < c.max2 && pointNode(m, R, c, c2)) return true;
+  return false;
+}
+
+function closestPoint(m, px, py, pz, minT, maxT) {
+  const c = { px, py, pz, best: Infinity, face: -1, min2: minT * minT, max2: maxT * maxT };
+  for (const R of m.roots) if (pointNode(m, R, c, 0)) break;
+  return c.face < 0 ? null : { face: c.face, distSq: c.best };
+}
+
+module.exports = { FRONT, BACK, DOUBLE, makeMesh, raycastFirst, closestPoint };

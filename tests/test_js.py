@@ -27,3 +27,44 @@ def test_addon_exports():
     assert {"build", "uploadMesh", "freeMesh", "raycastFirst", "raycast", "closestPoint", "setSideMask",
             "deviceCount", "raycastFirstBackward", "closestPointBackward", "batchCreate", "freeBatch",
             "batchRaycastFirst"} <= names
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_js_reference_restatement_matches_c_oracle():
+    """The JavaScript restatement of the reference query path that the benchmark times as the CPU
+    baseline (oracle/js) returns the same faces as the C oracle: first hits for FrontSide, BackSide and
+    DoubleSide rays (a sphere seen from inside and outside), closest points on a torus knot grid."""
+    import sys
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import bench
+    import mbvh_amd as M
+    import oracle_py as O
+    from mbvh_amd import geometry as G
+    from mbvh_amd import workloads as W
+
+    g = G.torus_knot(10, 3, 64, 16)
+    g.ensure_index()
+    idx = g.index.copy()
+    tree = M.build(g.positions, idx, strategy=M.SAH, target_leaf_size=10)
+    om = O.OracleMesh(g.positions, idx, tree.roots)
+    lo, hi = g.bounds()
+    pts = W.grid_points(lo, hi, 12)
+    js = bench.js_cpu_baseline("closest", g.positions, idx, tree.roots, pts, 2, pts.shape[0], min_threshold=0.0,
+                               max_threshold=float("inf"), side=0, near=0.0, far=float("inf"))
+    assert np.array_equal(js["faces"], om.closest_point(pts)["face"])
+    rng = np.random.default_rng(5)
+    n = 2000
+    o = (rng.standard_normal((n, 3)) * 20).astype(np.float32)
+    o[: n // 2] *= 0.05  # half of the rays start inside the knot's bounds
+    tgt = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    d = (tgt - o).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True).astype(np.float32)
+    for side, oside in ((0, O.FRONT), (1, O.BACK), (2, O.DOUBLE)):
+        js = bench.js_cpu_baseline("raycast", g.positions, idx, tree.roots, np.concatenate([o, d]), 2, n,
+                                   min_threshold=0.0, max_threshold=float("inf"), side=side, near=0.0,
+                                   far=float("inf"))
+        ref = om.raycast_first(o, d, side=oside)
+        assert np.array_equal(js["faces"], ref["face"]), side
+        assert 0.2 < (ref["face"] != O.NO_HIT).mean() < 1.0
