@@ -400,6 +400,23 @@ def main():
     mrays = nr / (wall1 / rc_steps) / 1e6
     mrays_kernel = nr / (float(np.mean(kern1)) / 1000.0) / 1e6
 
+    # ---------------- host-pointer entry points (PCIe included) ----------------
+    # what the JS API sees: pageable host arrays in, results back in host arrays, ordering included
+    torch.cuda.synchronize()
+    mesh.closest_point(w.points)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        mesh.closest_point(w.points)
+    host_cp = n / ((time.perf_counter() - t0) / 3) / 1e6
+    mesh1.raycast_first(w1.origins, w1.directions)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        mesh1.raycast_first(w1.origins, w1.directions)
+    host_rc = nr / ((time.perf_counter() - t0) / 10) / 1e6
+    host_api = {"closest_point_mqueries_per_s": round(host_cp, 2), "raycast_first_mrays_per_s": round(host_rc, 2),
+                "note": "host-pointer entry points (the JS path): pageable host arrays in and out, copies over "
+                        "PCIe and the query ordering included"}
+
     # ---------------- oracle counts + CPU baseline ----------------
     cpu = None
     roof = None
@@ -463,6 +480,10 @@ def main():
         extra["raycast_face_agreement"] = float((gf == r1["face"]).mean())
         extra["raycast_nodes_per_ray"] = round(float(r1["nodes"].mean()), 2)
         extra["raycast_tris_per_ray"] = round(float(r1["tris"].mean()), 2)
+        # algorithmic bytes per ray: 56 B of I/O, 32 B per node box, 42 B per triangle (3 x u16 + 36 B)
+        b_ray = 56.0 + 32.0 * float(r1["nodes"].mean()) + 42.0 * float(r1["tris"].mean())
+        extra["raycast_bytes_per_ray"] = round(b_ray, 1)
+        extra["raycast_achieved_gbs"] = round(nr * b_ray / (float(np.mean(kern1)) / 1000.0) / 1e9, 1)
         js1 = js_cpu_baseline("raycast", g1.positions, idx1, tree1.roots, np.concatenate([w1.origins, w1.directions]),
                               threads, nr, min_threshold=0.0, max_threshold=float("inf"), side=0, near=0.0,
                               far=float("inf"))
@@ -492,6 +513,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "host_api": host_api,
         "raycast_first": {"workload": "TorusKnot(10,3,400,100) 80k tris SAH leaf10, 65,536 rays from r=40",
                           "mrays_per_s": round(mrays, 2), "mrays_per_s_kernel_only": round(mrays_kernel, 2),
                           "kernel_ms": round(float(np.mean(kern1)), 4), **extra},

@@ -606,32 +606,22 @@ __device__ __forceinline__ T quad_xor2(T v) {  // lanes 2 3 0 1
   return quad_dpp<0x4E>(v);
 }
 
-__device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
-  if (tb < ta) {
-    const float t = ta;
-    ta = tb;
-    tb = t;
-    const uint32_t r = ra;
-    ra = rb;
-    rb = r;
-  }
-}
-
 // One node step of a quad: lane q holds child q of the node, `in` saying whether it is entered,
 // with its key (entry t or box score, a lower bound) and ref. The entered children are ranked by
-// (key, lane) through three quad exchanges; the rank-0 child is returned to every lane (kDone if
-// none is entered) and each other entered child is written by its own lane onto the quad stack,
-// farthest deepest, so that pops come nearest first. Sets `overflow` (and returns kDone) when the
-// stack cannot take them.
+// their key (the order only steers the search, any order gives the same result): the key's bits,
+// made monotonic as an unsigned integer, with the lane in the two low bits so that all four are
+// distinct. The rank-0 child is returned to every lane (kDone if none is entered) and each other
+// entered child is written by its own lane onto the quad stack, farthest deepest, so that pops come
+// nearest first; the stack keeps the exact key. Sets `overflow` (and returns kDone) when the stack
+// cannot take them.
 __device__ __forceinline__ uint32_t quad_children(QuadStack& st, uint32_t lane, bool in, float key, uint32_t cref,
                                                   bool& overflow) {
   const uint32_t q = lane & 3u;
-  const float kq = in ? key : __int_as_float(0x7fc00000);  // NaN: never ranks ahead
-  const float k1 = quad_xor1(kq), k2 = quad_xor2(kq), k3 = quad_dpp<0x1B>(kq);  // lanes q^1, q^2, q^3
-  // lane q^1 precedes q iff bit 0 of q is set; lanes q^2 and q^3 precede q iff bit 1 is set
-  const bool p1 = (q & 1u) != 0u, p2 = (q & 2u) != 0u;
-  const int rank = static_cast<int>(p1 ? (k1 <= kq) : (k1 < kq)) + static_cast<int>(p2 ? (k2 <= kq) : (k2 < kq)) +
-                   static_cast<int>(p2 ? (k3 <= kq) : (k3 < kq));
+  const uint32_t kb = __float_as_uint(key);
+  const uint32_t mono = kb ^ ((kb >> 31) ? 0xFFFFFFFFu : 0x80000000u);  // float order as unsigned order
+  const uint32_t kq = in ? ((mono & ~3u) | q) : 0xFFFFFFFFu;           // not entered: ranks last
+  const uint32_t k1 = quad_xor1(kq), k2 = quad_xor2(kq), k3 = quad_dpp<0x1B>(kq);  // lanes q^1, q^2, q^3
+  const int rank = static_cast<int>(k1 < kq) + static_cast<int>(k2 < kq) + static_cast<int>(k3 < kq);
   const uint64_t entered = __ballot(in);
   const int nh = __popcll((entered >> ((lane & 63u) & ~3u)) & 0xFull);
   if (nh == 0) return kDone;
