@@ -127,6 +127,29 @@ check('positions through accessors for non-float attributes', () => {
   assert.ok(bvh._roots[0].byteLength > 32);
 });
 
+check('getTriangleHitPointInfo: face, material group, normal, barycoord, uv (closed form)', () => {
+  // two right triangles; the second one in material group 2
+  const pos = new Float32Array([0, 0, 0, 2, 0, 0, 0, 2, 0, 0, 0, 1, 0, 2, 1, 2, 0, 1]);
+  const g = geometryFrom(pos, new Uint16Array([0, 1, 2, 3, 4, 5]));
+  g.setAttribute('uv', new BufferAttribute(new Float32Array([0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1]), 2));
+  g.addGroup(0, 3, 0);
+  g.addGroup(3, 3, 2);
+  const near = (x, y) => Math.abs(x - y) < 1e-12;
+  // p = 0.5 a + 0.25 b + 0.25 c on triangle 0: normal (b - a) x (c - a) = +z
+  const r0 = api.getTriangleHitPointInfo({ x: 0.5, y: 0.5, z: 0 }, g, 0, null);
+  assert.deepStrictEqual([r0.face.a, r0.face.b, r0.face.c, r0.face.materialIndex], [0, 1, 2, 0]);
+  assert.ok(near(r0.barycoord.x, 0.5) && near(r0.barycoord.y, 0.25) && near(r0.barycoord.z, 0.25));
+  assert.ok(near(r0.face.normal.x, 0) && near(r0.face.normal.y, 0) && near(r0.face.normal.z, 1));
+  assert.ok(near(r0.uv.x, 0.25) && near(r0.uv.y, 0.25));
+  // vertex c of triangle 1 (reversed winding: normal -z), material 2, written into `target`
+  const target = { face: { normal: { x: 0, y: 0, z: 0, set(x, y, z) { this.x = x; this.y = y; this.z = z; } } } };
+  const r1 = api.getTriangleHitPointInfo({ x: 2, y: 0, z: 1 }, g, 1, target);
+  assert.strictEqual(r1, target);
+  assert.deepStrictEqual([r1.face.a, r1.face.b, r1.face.c, r1.face.materialIndex], [3, 4, 5, 2]);
+  assert.ok(near(r1.barycoord.x, 0) && near(r1.barycoord.y, 0) && near(r1.barycoord.z, 1));
+  assert.ok(near(r1.face.normal.z, -1) && near(r1.uv.x, 0) && near(r1.uv.y, 1));
+});
+
 let failed = 0;
 for (const [name, fn] of checks) {
   try {
