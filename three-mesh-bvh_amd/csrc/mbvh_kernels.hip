@@ -125,10 +125,11 @@ struct Tri {
   __device__ __forceinline__ uint32_t mask() const { return __float_as_uint(p1.w); }
 };
 __device__ __forceinline__ Tri load_tri(const float4* tris, uint32_t slot) {
+  const float4* p = tris + 3ull * slot;  // one address, the three rows at immediate offsets
   Tri t;
-  t.p0 = tris[3u * slot + 0u];
-  t.p1 = tris[3u * slot + 1u];
-  t.p2 = tris[3u * slot + 2u];
+  t.p0 = p[0];
+  t.p1 = p[1];
+  t.p2 = p[2];
   return t;
 }
 
