@@ -238,8 +238,54 @@ def bench_differentiable(torch, M, W, ctx, stream, args):
         "fwd_bwd_over_fwd": round((f_ms + b_det) / f_ms, 3),
         "fwd_bwd_over_fwd_atomic": round((f_ms + b_atom) / f_ms, 3), "fwd_mqueries_per_s": round(n / f_ms / 1e3, 2)}
     log(f"C4 {out['differentiable_closest']}")
+    out["refit"] = bench_refit(mesh, g, idx, tree)
+    log(f"refit {out['refit']}")
     mesh.free()
     return out
+
+
+def bench_refit(mesh, g, idx, tree):
+    """MeshBVH.refit on the device (host entry point: positions in, refit nodes out) on the C4 mesh
+    after a vertex displacement, against the reference's refit restated in JavaScript (one thread)."""
+    import numpy as np
+    p0 = g.positions.reshape(-1, 3)
+    pos = [(p0 * (1.0 + 0.01 * np.sin(7.0 * p0[:, :1] + k))).astype(np.float32) for k in range(2)]
+    first = mesh.refit(pos[0])  # from the uploaded tree: compared with the JavaScript refit below
+    times = []
+    for k in range(6):
+        t0 = time.perf_counter()
+        mesh.refit(pos[(k + 1) % 2])  # every call moves the vertices again
+        times.append(time.perf_counter() - t0)
+    res = {"workload": f"refit of the C4 mesh ({g.tri_count} tris, {p0.shape[0]} vertices, "
+                       f"{tree.roots[0].nbytes // 32} nodes) after a vertex displacement",
+           "gpu_host_api_ms": round(1000 * float(np.median(times)), 3),
+           "note": "host entry point: positions copied in, kernels, refit nodes copied out, synchronised"}
+    import shutil
+    import subprocess
+    import tempfile
+    node = shutil.which("node")
+    if node is not None:
+        d = tempfile.mkdtemp(prefix="mbvh_refit_")
+        try:
+            pos[0].tofile(os.path.join(d, "positions.f32"))
+            ib = 2 if idx.dtype == np.uint16 else 4
+            idx.tofile(os.path.join(d, "index.u16" if ib == 2 else "index.u32"))
+            for i, rt in enumerate(tree.roots):
+                rt.tofile(os.path.join(d, f"root{i}.bin"))
+            with open(os.path.join(d, "meta.json"), "w") as f:
+                json.dump({"roots": len(tree.roots), "index_bytes": ib}, f)
+            out = subprocess.run([node, os.path.join(ROOT, "oracle", "js", "refit_bench.js"), d], capture_output=True,
+                                 text=True, timeout=600)
+            if out.returncode == 0:
+                rep = json.loads(out.stdout.strip().splitlines()[-1])
+                js_roots = [np.fromfile(os.path.join(d, f"refit{i}.bin"), np.uint8) for i in range(len(tree.roots))]
+                res["cpu_baseline"] = {"single_thread_ms": round(1000 * rep["seconds"], 3), "cores": 1, "kind": "port",
+                                       "sample": f"FP64 JavaScript restatement of the reference refit (oracle/js) in "
+                                                 f"Node {rep['node']}, the whole tree, median of 3"}
+                res["nodes_equal_to_cpu"] = bool(all(np.array_equal(a, b) for a, b in zip(first, js_roots)))
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    return res
 
 
 def bench_batched(torch, M, W, ctx, stream, args):

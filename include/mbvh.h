@@ -147,6 +147,22 @@ int mbvh_mesh_info(const mbvh_mesh* mesh, uint64_t* tri_count, uint64_t* vertex_
  */
 int mbvh_mesh_set_side_mask(mbvh_ctx* ctx, mbvh_mesh* mesh, const uint8_t* mask, uint64_t n);
 
+/*
+ * Refit (MeshBVH.refit with nodeIndices = null, src/core/MeshBVH.js:504-508 over
+ * src/core/cast/refit.template.js:4-196): recomputes the node boxes of the uploaded tree from new
+ * positions of the same vertices (same count and layout as at upload), on the device and with the
+ * reference's rules - a leaf box is the min / max of its triangles' vertices and is rewritten only
+ * if it differs; an internal node is the union of its children, recomputed only if a child changed.
+ * The device copies of the triangles and of every node layout are updated, so later queries see the
+ * new geometry. If roots_out is not NULL, it has n_roots entries (the mesh's root count) and
+ * roots_out[r] (root_bytes[r] bytes, the size of root r at upload; NULL entries skipped) receives
+ * the refit reference-format node buffer of root r.
+ * Synchronous. Queries on this mesh must not run concurrently with a refit.
+ */
+int mbvh_mesh_refit(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint64_t vertex_count,
+                    uint32_t position_stride, uint32_t position_offset, void* const* roots_out,
+                    const uint64_t* root_bytes, uint32_t n_roots);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Queries                                                                                    */
 /* ------------------------------------------------------------------------------------------ */

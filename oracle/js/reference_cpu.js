@@ -11,6 +11,7 @@
 //                  Ray.intersectTriangle (GeometricTools IntrRay3Triangle3) as called by
 //                  src/utils/ThreeRayIntersectUtilities.js:21-47, roots combined with strict <
 //                  (src/core/MeshBVH.js:558-577)
+//   refit          src/core/cast/refit.template.js:4-196 (nodeIndices = null)
 //   closestPoint   src/core/cast/closestPointToPoint.js:6-78 over shapecastTraverse
 //                  (src/core/cast/shapecast.js:41-214): children by box distance^2, right first only
 //                  if strictly closer, score < best^2 and < maxThreshold^2, strict < triangle update,
@@ -185,4 +186,47 @@ function closestPoint(m, px, py, pz, minT, maxT) {
   return c.face < 0 ? null : { face: c.face, distSq: c.best };
 }
 
-module.exports = { FRONT, BACK, DOUBLE, makeMesh, raycastFirst, closestPoint };
+// ---- refit (src/core/cast/refit.template.js:4-196, nodeIndices = null) ---------------------------
+// Rewrites the boxes of every root of `m` in place from its positions: a leaf's box is the min / max
+// of its triangles' vertices (strict < / >, slot order), written only if a component differs; an
+// internal node is the union of its children, recomputed only when a child changed.
+function refitNode(m, R, n) {
+  const f = R.f32, b = 8 * n;
+  if (R.u16[16 * n + 15] === 0xFFFF) {
+    const off = R.u32[b + 6], cnt = R.u16[16 * n + 14];
+    const p = m.pos, ix = m.index;
+    let minx = Infinity, miny = Infinity, minz = Infinity, maxx = -Infinity, maxy = -Infinity, maxz = -Infinity;
+    for (let i = 3 * off, l = 3 * (off + cnt); i < l; i++) {
+      const v = 3 * ix[i];
+      const x = p[v], y = p[v + 1], z = p[v + 2];
+      if (x < minx) minx = x;
+      if (x > maxx) maxx = x;
+      if (y < miny) miny = y;
+      if (y > maxy) maxy = y;
+      if (z < minz) minz = z;
+      if (z > maxz) maxz = z;
+    }
+    if (f[b] !== minx || f[b + 1] !== miny || f[b + 2] !== minz || f[b + 3] !== maxx || f[b + 4] !== maxy || f[b + 5] !== maxz) {
+      f[b] = minx; f[b + 1] = miny; f[b + 2] = minz; f[b + 3] = maxx; f[b + 4] = maxy; f[b + 5] = maxz;
+      return true;
+    }
+    return false;
+  }
+  const left = n + 1, right = n + R.u32[b + 6];
+  const lc = refitNode(m, R, left);
+  const rc = refitNode(m, R, right);
+  if (lc || rc) {
+    const l = 8 * left, r = 8 * right;
+    for (let i = 0; i < 3; i++) {
+      f[b + i] = f[l + i] < f[r + i] ? f[l + i] : f[r + i];
+      f[b + i + 3] = f[l + i + 3] > f[r + i + 3] ? f[l + i + 3] : f[r + i + 3];
+    }
+  }
+  return lc || rc;
+}
+
+function refit(m) {
+  for (const R of m.roots) refitNode(m, R, 0);
+}
+
+module.exports = { FRONT, BACK, DOUBLE, makeMesh, raycastFirst, closestPoint, refit };

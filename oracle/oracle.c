@@ -727,3 +727,60 @@ int orc_validate_root(const orc_mesh* m, uint32_t root, orc_tree_stats* st) {
   free(stack);
   return (st->bounds_failures || st->structure_failures) ? 1 : 0;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* refit (src/core/cast/refit.template.js:4-196 with nodeIndices = null)                       */
+/* ------------------------------------------------------------------------------------------ */
+
+/* One node of _traverse: a leaf's box is the min / max over its triangles' vertices in slot order
+ * (strict < and >), written only if a component differs (!==); an internal node is the union of its
+ * children, recomputed only when a child changed. Returns whether the node changed. */
+static int refit_node(const orc_mesh* m, uint8_t* nodes, uint64_t n_nodes, uint64_t node, int depth) {
+  if (node >= n_nodes || depth > 4096) return 0;
+  float* box = (float*)(nodes + 32 * node);
+  uint16_t flag, count;
+  uint32_t word6;
+  memcpy(&flag, nodes + 32 * node + 30, 2);
+  memcpy(&count, nodes + 32 * node + 28, 2);
+  memcpy(&word6, nodes + 32 * node + 24, 4);
+  if (flag == 0xFFFF) {
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint64_t s = word6; s < (uint64_t)word6 + count; s++) {
+      const uint32_t face = face_of_slot(m, s);
+      for (int j = 0; j < 3; j++) {
+        uint64_t v = 3ull * face + j;
+        if (m->index) v = rd(m->index, m->index_elt_bytes, v);
+        const float* p = m->positions + v * m->stride + m->offset;
+        for (int k = 0; k < 3; k++) {
+          const double x = p[k];
+          if (x < mn[k]) mn[k] = x;
+          if (x > mx[k]) mx[k] = x;
+        }
+      }
+    }
+    int changed = 0;
+    for (int k = 0; k < 3; k++) changed = changed || !((double)box[k] == mn[k]) || !((double)box[k + 3] == mx[k]);
+    if (changed)
+      for (int k = 0; k < 3; k++) {
+        box[k] = (float)mn[k];
+        box[k + 3] = (float)mx[k];
+      }
+    return changed;
+  }
+  const uint64_t left = node + 1, right = node + word6;
+  const int lc = refit_node(m, nodes, n_nodes, left, depth + 1);
+  const int rc = refit_node(m, nodes, n_nodes, right, depth + 1);
+  if (lc || rc) {
+    const float* lb = (const float*)(nodes + 32 * left);
+    const float* rb = (const float*)(nodes + 32 * right);
+    for (int k = 0; k < 3; k++) {
+      box[k] = lb[k] < rb[k] ? lb[k] : rb[k];
+      box[k + 3] = lb[k + 3] > rb[k + 3] ? lb[k + 3] : rb[k + 3];
+    }
+  }
+  return lc || rc;
+}
+
+void orc_refit_root(const orc_mesh* m, uint8_t* nodes, uint64_t n_bytes) {
+  if (n_bytes >= 32) refit_node(m, nodes, n_bytes / 32, 0, 0);
+}

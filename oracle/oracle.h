@@ -94,6 +94,11 @@ typedef struct orc_tree_stats {
  * [first, first+count) of that root in exactly one leaf. Returns 0 when all checks pass. */
 int orc_validate_root(const orc_mesh* m, uint32_t root, orc_tree_stats* stats);
 
+/* MeshBVH.refit with nodeIndices = null (src/core/cast/refit.template.js:4-196): rewrites the boxes
+ * of one root's node buffer `nodes` (n_bytes, modified in place) from the mesh's positions, index
+ * and indirect buffer. */
+void orc_refit_root(const orc_mesh* m, uint8_t* nodes, uint64_t n_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,8 @@ def library():
         lib.orc_ray_plane_distance.restype = f64
         lib.orc_point_face_distance.argtypes = [vp, u32, vp]
         lib.orc_point_face_distance.restype = f64
+        lib.orc_refit_root.argtypes = [vp, vp, C.c_uint64]
+        lib.orc_refit_root.restype = None
         lib.orc_validate_root.argtypes = [vp, u32, C.POINTER(TreeStats)]
         lib.orc_validate_root.restype = C.c_int
         _lib = lib
@@ -173,6 +175,15 @@ class OracleMesh:
     def point_face_distance(self, face, p):
         q = np.ascontiguousarray(p, dtype=np.float32).reshape(3)
         return library().orc_point_face_distance(self.ref, int(face), _p(q))
+
+    def refit(self):
+        """The roots after MeshBVH.refit over this mesh's positions (new arrays; self.roots unchanged)."""
+        out = []
+        for r in self.roots:
+            nodes = r.copy()
+            library().orc_refit_root(self.ref, nodes.ctypes.data, nodes.nbytes)
+            out.append(nodes)
+        return out
 
     def validate_root(self, root: int = 0):
         st = TreeStats()

@@ -170,4 +170,21 @@ assert.ok(batchedChecked > 64 && presenceDiffers <= 2, `batched BatchedMesh hits
 api.disposeBatchedBoundsTree.call(Object.assign(batched, { boundsTrees: [bvh] }), -1);
 assert.strictEqual(batched._mbvhBatch, null);
 
+// --- refit after a deformation: nodes and queries go back to the oracle ------------------------
+for (let v = 0; v < positions.length / 3; v++) {
+  const x = positions[3 * v], y = positions[3 * v + 1], z = positions[3 * v + 2];
+  positions[3 * v] = x * 1.05 + 0.3 * Math.sin(0.7 * y);
+  positions[3 * v + 1] = y * 0.97 + 0.2 * Math.cos(0.5 * z);
+  positions[3 * v + 2] = z + 0.25 * Math.sin(0.3 * x);
+}
+bvh.refit();
+assert.throws(() => bvh.refit([0, 1]), /nodeIndices/);
+writeArr('refit_positions.bin', positions);
+writeArr('refit_root0.bin', new Uint8Array(bvh._roots[0]));
+const rr = bvh.raycastFirstBatch(origins, dirs);
+writeArr('refit_rf_face.bin', rr.faceIndex);
+writeArr('refit_rf_dist.bin', rr.distance);
+writeArr('refit_rf_point.bin', rr.point);
+writeArr('refit_rf_bary.bin', rr.barycoord);
+
 console.log(JSON.stringify({ ok: true, rays: nRays, points: nPts, worldChecked, batchedChecked, presenceDiffers }));

@@ -88,3 +88,17 @@ def test_js_api_matches_oracle(tmp_path):
     assert np.array_equal(np.fromfile(d / "bc_positions.bin", np.float32).reshape(-1, 3), refc["positions"])
     mesh.free()
     ctx.close()
+
+    # MeshBVH.refit after a deformation: the refit nodes equal the oracle's refit, and raycasts on
+    # the refit tree match the oracle on the deformed geometry
+    pos2 = np.fromfile(d / "refit_positions.bin", np.float32)
+    root2 = np.fromfile(d / "refit_root0.bin", np.uint8)
+    assert not np.array_equal(pos2, g.positions.reshape(-1))
+    assert np.array_equal(root2, O.OracleMesh(pos2, idx, [root]).refit()[0])
+    om2 = O.OracleMesh(pos2, idx, [root2])
+    got2 = {"face": np.fromfile(d / "refit_rf_face.bin", np.uint32),
+            "distance": np.fromfile(d / "refit_rf_dist.bin", np.float32),
+            "point": np.fromfile(d / "refit_rf_point.bin", np.float32).reshape(-1, 3),
+            "barycoord": np.fromfile(d / "refit_rf_bary.bin", np.float32).reshape(-1, 3)}
+    print("js refit raycastFirst", check_raycast_parity(om2, w.origins, w.directions, got2,
+                                                        om2.raycast_first(w.origins, w.directions), "js_refit"))

@@ -1,0 +1,121 @@
+"""GPU refit (MeshBVH.refit on the device) through the C ABI: the refit reference-format nodes equal
+the oracle's refit byte for byte (deformed positions, an inconsistent internal box, groups,
+indirect); queries after the refit equal queries on a fresh upload of the refit tree bit for bit and
+match the FP64 oracle on the deformed geometry; a batch containing the mesh sees the refit."""
+import numpy as np
+import pytest
+
+import mbvh_amd as M
+from mbvh_amd import geometry as G, workloads as W
+import oracle_py as O
+from helpers import build_mesh, check_closest_parity, check_raycast_parity
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = M.Context(0)
+    yield c
+    c.close()
+
+
+def _deform(positions, seed, scale=0.2):
+    rng = np.random.default_rng(seed)
+    p = positions.reshape(-1, 3).copy()
+    p += rng.normal(0, scale, p.shape).astype(np.float32)
+    p[rng.integers(0, len(p), 10)] *= np.float32(1.5)
+    return p.astype(np.float32)
+
+
+def _rays(lo, hi, n, seed):
+    rng = np.random.default_rng(seed)
+    o = rng.standard_normal((n, 3)).astype(np.float32)
+    o = (o / np.linalg.norm(o, axis=1, keepdims=True) * np.float32(2.5 * np.abs(hi - lo).max())).astype(np.float32)
+    t = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    d = (t - o).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True).astype(np.float32)
+    return o, d
+
+
+def _same(a, b):
+    for k in ("face", "distance", "point", "barycoord"):
+        assert np.array_equal(np.asarray(a[k]).view(np.uint32), np.asarray(b[k]).view(np.uint32)), k
+
+
+@pytest.mark.parametrize("indirect", [False, True])
+def test_refit_nodes_and_queries(ctx, indirect):
+    g, idx, tree = build_mesh(G.torus_knot(10, 3, 300, 40), indirect=indirect)
+    mesh = ctx.upload(tree, g.positions, idx)
+    pos = _deform(g.positions, 1)
+    roots = mesh.refit(pos)
+    om = O.OracleMesh(pos, idx, tree.roots, indirect=tree.indirect)
+    ref_roots = om.refit()
+    for a, b in zip(roots, ref_roots):
+        assert np.array_equal(a, b)
+    # queries on the refit mesh == queries on a fresh upload of the refit tree, and parity
+    fresh = ctx.upload(M.Tree(ref_roots, tree.indirect), pos, idx)
+    lo, hi = pos.min(axis=0), pos.max(axis=0)
+    o, d = _rays(lo, hi, 20000, 2)
+    got = mesh.raycast_first(o, d)
+    omn = O.OracleMesh(pos, idx, ref_roots, indirect=tree.indirect)
+    check_raycast_parity(omn, o, d, got, omn.raycast_first(o, d), "refit rays")
+    pts = W.grid_points(lo, hi, 24)
+    gotc = mesh.closest_point(pts)
+    check_closest_parity(omn, pts, gotc, omn.closest_point(pts), "refit points")
+    _same(got, fresh.raycast_first(o, d))
+    _same(gotc, fresh.closest_point(pts))
+    fresh.free()
+    mesh.free()
+
+
+def test_refit_keeps_unchanged_paths(ctx):
+    """An inconsistent internal box above unchanged leaves is kept; a changed leaf recomputes its path."""
+    g, idx, tree = build_mesh(G.torus_knot(10, 3, 120, 16))
+    tight = O.OracleMesh(g.positions, idx, tree.roots).refit()
+    buf = tight[0].copy()
+    f = buf.view(np.float32).reshape(-1, 8)
+    h = buf.view(np.uint16).reshape(-1, 16)
+    k = [i for i in range(f.shape[0]) if h[i, 15] != 0xFFFF][5]
+    f[k, 0] -= np.float32(3.0)
+    mesh = ctx.upload(M.Tree([buf], None), g.positions, idx)
+    same = mesh.refit(g.positions)
+    assert np.array_equal(same[0], buf)
+    pos = _deform(g.positions, 3, scale=0.05)
+    got = mesh.refit(pos)
+    assert np.array_equal(got[0], O.OracleMesh(pos, idx, [buf]).refit()[0])
+    mesh.free()
+
+
+def test_refit_groups_and_batch(ctx):
+    """Several roots (geometry groups); a batch holding the mesh sees the refit geometry."""
+    geo = G.torus_knot(10, 3, 200, 24)
+    geo.ensure_index()
+    ntri = geo.index.size // 3
+    ranges = [(0, ntri // 3), (ntri // 3, ntri - ntri // 3)]
+    g, idx, tree = build_mesh(geo, ranges=ranges)
+    assert len(tree.roots) == 2
+    mesh = ctx.upload(tree, g.positions, idx)
+    batch = ctx.batch([mesh])
+    pos = _deform(g.positions, 5)
+    roots = mesh.refit(pos)
+    for a, b in zip(roots, O.OracleMesh(pos, idx, tree.roots).refit()):
+        assert np.array_equal(a, b)
+    lo, hi = pos.min(axis=0), pos.max(axis=0)
+    o, d = _rays(lo, hi, 5000, 6)
+    ident = np.eye(4, dtype=np.float32).reshape(1, 16)
+    res = batch.raycast_first(ident, ident, np.zeros(1, np.uint32), o, d, np.zeros(len(o), np.uint32))
+    single = mesh.raycast_first(o, d)
+    hit = single["face"] != M.NO_HIT
+    assert np.array_equal(res["face"] != M.NO_HIT, hit)
+    assert np.all(np.abs(res["distance"][hit] - single["distance"][hit]) <= 1e-5 * np.maximum(single["distance"][hit], 1))
+    batch.free()
+    mesh.free()
+
+
+def test_refit_rejects_other_vertex_counts(ctx):
+    g, idx, tree = build_mesh(G.torus_knot(10, 3, 60, 8))
+    mesh = ctx.upload(tree, g.positions, idx)
+    with pytest.raises(M.MbvhError):
+        mesh.refit(g.positions.reshape(-1, 3)[:-1])
+    mesh.free()

@@ -26,7 +26,7 @@ def test_addon_exports():
     names = set(r.stdout.strip().split(","))
     assert {"build", "uploadMesh", "freeMesh", "raycastFirst", "raycast", "closestPoint", "setSideMask",
             "deviceCount", "raycastFirstBackward", "closestPointBackward", "batchCreate", "freeBatch",
-            "batchRaycastFirst"} <= names
+            "batchRaycastFirst", "refitMesh"} <= names
 
 
 @pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
@@ -68,3 +68,33 @@ def test_js_reference_restatement_matches_c_oracle():
         ref = om.raycast_first(o, d, side=oside)
         assert np.array_equal(js["faces"], ref["face"]), side
         assert 0.2 < (ref["face"] != O.NO_HIT).mean() < 1.0
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_js_refit_restatement_matches_c_oracle(tmp_path):
+    """The JavaScript refit timed as the refit's CPU baseline equals the C oracle's refit byte for byte
+    (a displaced torus knot, 16-bit index)."""
+    import json
+    import sys
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mbvh_amd as M
+    import oracle_py as O
+    from mbvh_amd import geometry as G
+
+    g = G.torus_knot(10, 3, 200, 30)
+    g.ensure_index()
+    idx = g.index.copy()
+    tree = M.build(g.positions, idx, strategy=M.SAH, target_leaf_size=10)
+    p0 = g.positions.reshape(-1, 3)
+    pos = (p0 * (1.0 + 0.01 * np.sin(7.0 * p0[:, :1]))).astype(np.float32)
+    pos.tofile(tmp_path / "positions.f32")
+    idx.tofile(tmp_path / ("index.u16" if idx.dtype == np.uint16 else "index.u32"))
+    tree.roots[0].tofile(tmp_path / "root0.bin")
+    (tmp_path / "meta.json").write_text(json.dumps({"roots": 1, "index_bytes": idx.itemsize}))
+    r = subprocess.run(["node", os.path.join(ROOT, "oracle", "js", "refit_bench.js"), str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    js = np.fromfile(tmp_path / "refit0.bin", np.uint8)
+    assert np.array_equal(js, O.OracleMesh(pos, idx, tree.roots).refit()[0])
+    assert not np.array_equal(js, tree.roots[0])

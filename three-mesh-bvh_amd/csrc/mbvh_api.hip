@@ -54,11 +54,14 @@ struct mbvh_batch {
   uint32_t max_stack = 0;
   uint32_t max_stack4 = 0;
   mbvh_batch_tuning tuning{4, 16, 256, 4, 0.0f};
+  std::vector<const mbvh_mesh*> meshes;  // for refreshing d_meshes after a refit
+  std::vector<uint64_t> versions;
 };
 
 struct mbvh_batch_graph {
   uint32_t magic = 0x48505247u;  // "GRPH"
   mbvh_ctx* ctx = nullptr;
+  mbvh_batch* batch = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -82,7 +85,18 @@ struct mbvh_mesh {
   uint64_t face_limit = 0;  // face ids are < face_limit
   uint32_t max_depth = 0;   // deepest reference node depth over all roots
   uint32_t max_stack = 0;   // traversal stack entries needed
-  float coord_max = 0.0f;   // largest |vertex coordinate| over the mesh's triangles
+  float coord_max = 0.0f;   // largest |vertex coordinate| over the mesh's triangles (an upper bound after a refit)
+  uint64_t version = 0;     // incremented by each refit (batches refresh their copies of the mesh view)
+  // refit: the reference-format nodes of all roots and how their boxes map onto the device layouts
+  uint32_t* d_ref_nodes = nullptr;   // 8 words per node, roots concatenated
+  int32_t* d_ref_parent = nullptr;   // parent node (-1 for a root)
+  int32_t* d_ref_wide = nullptr;     // float offset of the node's box in the wide nodes (-1: none)
+  uint32_t* d_ref_leaves = nullptr;  // leaf nodes
+  uint32_t* d_ref_scratch = nullptr; // 2 words per node: arrival counter, changed flag
+  uint2* d_four_map = nullptr;       // 4-wide child slot <- wide box (float offsets)
+  uint64_t n_ref_nodes = 0, n_ref_leaves = 0, n_four_map = 0;
+  std::vector<uint64_t> root_base;   // first node of each root
+  std::vector<uint64_t> root_nodes;  // nodes of each root
 };
 
 namespace {
@@ -142,10 +156,20 @@ uint32_t inline_leaf_ref(uint64_t off, uint64_t cnt, std::vector<uint2>& bigleaf
   return kRefLeaf | static_cast<uint32_t>(bigleaf.size() - 1);
 }
 
+// Where each reference node's box lives in the device layouts (for refit); node indices count over
+// all roots, `base` being this root's first.
+struct RefitMaps {
+  uint64_t base = 0;
+  std::vector<int32_t> parent;    // parent node, -1 for a root (and for nodes no traversal reaches)
+  std::vector<int32_t> wide_off;  // float offset of the node's box in the wide nodes, -1: none
+  std::vector<uint32_t> leaves;   // reachable leaves
+  std::vector<uint2> four_map;    // (float offset of a 4-wide child slot, float offset of its wide box)
+};
+
 // Converts one reference root into wide nodes. Validates every index it follows so a corrupt or
 // foreign buffer is rejected on the host instead of steering a kernel out of bounds.
 int convert_root(const NodeView& v, uint64_t n_slots, std::vector<WideNode>& wide,
-                 std::vector<uint2>& bigleaf, uint32_t* root_ref, uint32_t* max_depth) {
+                 std::vector<uint2>& bigleaf, uint32_t* root_ref, uint32_t* max_depth, RefitMaps& maps) {
   if (v.n_nodes == 0) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_mesh_upload: empty root buffer");
   auto leaf_ok = [&](uint64_t node) {
     const uint64_t off = v.leaf_offset(node), cnt = v.leaf_count(node);
@@ -156,9 +180,13 @@ int convert_root(const NodeView& v, uint64_t n_slots, std::vector<WideNode>& wid
     uint32_t depth;
     int64_t parent_wide;  // -1 for the root
     int side;             // 0 left, 1 right
+    int64_t parent_node;  // -1 for the root
   };
   std::vector<Item> todo;
-  todo.push_back({0, 0, -1, 0});
+  todo.push_back({0, 0, -1, 0, -1});
+  if (maps.base + v.n_nodes > 0x7FFFFFFFull) return set_error(MBVH_ERR_LIMIT, "mbvh_mesh_upload: too many nodes");
+  maps.parent.resize(maps.base + v.n_nodes, -1);
+  maps.wide_off.resize(maps.base + v.n_nodes, -1);
   uint64_t visited = 0;
   while (!todo.empty()) {
     const Item it = todo.back();
@@ -166,11 +194,15 @@ int convert_root(const NodeView& v, uint64_t n_slots, std::vector<WideNode>& wid
     if (++visited > v.n_nodes)
       return set_error(MBVH_ERR_BAD_ARGS, "mbvh_mesh_upload: node buffer contains a cycle");
     *max_depth = std::max(*max_depth, it.depth);
+    const uint64_t g = maps.base + it.node;
+    maps.parent[g] = it.parent_node < 0 ? -1 : static_cast<int32_t>(maps.base + it.parent_node);
+    if (it.parent_wide >= 0) maps.wide_off[g] = static_cast<int32_t>(16 * it.parent_wide + 6 * it.side);
     uint32_t ref;
     if (v.is_leaf(it.node)) {
       if (!leaf_ok(it.node))
         return set_error(MBVH_ERR_BAD_ARGS, "mbvh_mesh_upload: leaf %llu references triangles past the end",
                          static_cast<unsigned long long>(it.node));
+      maps.leaves.push_back(static_cast<uint32_t>(g));
       ref = inline_leaf_ref(v.leaf_offset(it.node), v.leaf_count(it.node), bigleaf);
     } else {
       const uint64_t left = it.node + 1, right = v.right_child(it.node);
@@ -178,7 +210,7 @@ int convert_root(const NodeView& v, uint64_t n_slots, std::vector<WideNode>& wid
       if (axis > 2 || left >= v.n_nodes || right >= v.n_nodes || right <= it.node)
         return set_error(MBVH_ERR_BAD_ARGS, "mbvh_mesh_upload: malformed internal node %llu",
                          static_cast<unsigned long long>(it.node));
-      if (wide.size() >= 0x7FFFFFFFull)
+      if (wide.size() >= 0x7FFFFFFull)  // float offsets of wide boxes fit 31 bits
         return set_error(MBVH_ERR_LIMIT, "mbvh_mesh_upload: too many nodes");
       const uint32_t w = static_cast<uint32_t>(wide.size());
       WideNode wn;
@@ -190,8 +222,8 @@ int convert_root(const NodeView& v, uint64_t n_slots, std::vector<WideNode>& wid
       wn.meta = make_uint4(0, 0, axis, 0);
       wide.push_back(wn);
       ref = w;
-      todo.push_back({right, it.depth + 1, static_cast<int64_t>(w), 1});
-      todo.push_back({left, it.depth + 1, static_cast<int64_t>(w), 0});
+      todo.push_back({right, it.depth + 1, static_cast<int64_t>(w), 1, static_cast<int64_t>(it.node)});
+      todo.push_back({left, it.depth + 1, static_cast<int64_t>(w), 0, static_cast<int64_t>(it.node)});
     }
     if (it.parent_wide < 0) {
       *root_ref = ref;
@@ -212,7 +244,7 @@ int convert_root(const NodeView& v, uint64_t n_slots, std::vector<WideNode>& wid
 constexpr uint32_t kDone4 = 0x7FFFFFFFu;
 
 int build_bvh4(const std::vector<WideNode>& wide, uint32_t root_ref, std::vector<float4>& nodes4,
-               uint32_t* root4, uint32_t* depth4) {
+               uint32_t* root4, uint32_t* depth4, std::vector<uint2>& four_map) {
   struct Item {
     uint32_t bref;    // binary (wide) node
     int64_t parent4;  // -1: the root
@@ -242,6 +274,7 @@ int build_bvh4(const std::vector<WideNode>& wide, uint32_t root_ref, std::vector
     struct Child {
       float box[6];
       uint32_t ref;
+      uint32_t wide_off;  // float offset of the box in the wide nodes
     };
     Child ch[4];
     int nc = 0;
@@ -254,14 +287,17 @@ int build_bvh4(const std::vector<WideNode>& wide, uint32_t root_ref, std::vector
       const uint32_t c = refs[side];
       if (c & kRefLeaf) {
         std::memcpy(ch[nc].box, boxes[side], sizeof(ch[nc].box));
+        ch[nc].wide_off = 16 * it.bref + 6 * side;
         ch[nc++].ref = c;
       } else {
         const WideNode& g = wide[c];
         const float gl[6] = {g.b0.x, g.b0.y, g.b0.z, g.b0.w, g.b1.x, g.b1.y};
         const float gr[6] = {g.b1.z, g.b1.w, g.b2.x, g.b2.y, g.b2.z, g.b2.w};
         std::memcpy(ch[nc].box, gl, sizeof(gl));
+        ch[nc].wide_off = 16 * c;
         ch[nc++].ref = g.meta.x;
         std::memcpy(ch[nc].box, gr, sizeof(gr));
+        ch[nc].wide_off = 16 * c + 6;
         ch[nc++].ref = g.meta.y;
       }
     }
@@ -269,6 +305,8 @@ int build_bvh4(const std::vector<WideNode>& wide, uint32_t root_ref, std::vector
     for (int k = 0; k < 4; k++) {
       if (k < nc) {
         const Child& c = ch[k];
+        if (4 * nodes4.size() > 0xFFFFFFF0ull) return set_error(MBVH_ERR_LIMIT, "mbvh_mesh_upload: too many nodes");
+        four_map.push_back(make_uint2(static_cast<uint32_t>(4 * nodes4.size()), c.wide_off));
         nodes4.push_back(make_float4(c.box[0], c.box[1], c.box[2], c.box[3]));
         nodes4.push_back(make_float4(c.box[4], c.box[5], as_float(c.ref), 0.0f));
       } else {
@@ -296,6 +334,16 @@ void free_mesh_buffers(mbvh_mesh* m) {
   if (m->d_face_slot) (void)hipFree(m->d_face_slot);
   if (m->d_nodes4) (void)hipFree(m->d_nodes4);
   if (m->d_roots4) (void)hipFree(m->d_roots4);
+  for (void* p : {static_cast<void*>(m->d_ref_nodes), static_cast<void*>(m->d_ref_parent),
+                  static_cast<void*>(m->d_ref_wide), static_cast<void*>(m->d_ref_leaves),
+                  static_cast<void*>(m->d_ref_scratch), static_cast<void*>(m->d_four_map)})
+    if (p) (void)hipFree(p);
+  m->d_ref_nodes = nullptr;
+  m->d_ref_parent = nullptr;
+  m->d_ref_wide = nullptr;
+  m->d_ref_leaves = nullptr;
+  m->d_ref_scratch = nullptr;
+  m->d_four_map = nullptr;
   m->d_face_slot = nullptr;
   m->d_nodes4 = nullptr;
   m->d_roots4 = nullptr;
@@ -483,19 +531,24 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
   std::vector<uint2> bigleaf;
   std::vector<uint32_t> root_refs(n_roots);
   uint32_t max_depth = 0;
+  RefitMaps maps;
+  std::vector<uint64_t> root_base(n_roots), root_nodes(n_roots);
   for (uint32_t r = 0; r < n_roots; r++) {
     if (!roots[r] || root_bytes[r] % kBytesPerNode != 0)
       return set_error(MBVH_ERR_BAD_ARGS, "mbvh_mesh_upload: root %u size is not a multiple of 32 bytes", r);
     NodeView v{static_cast<const uint8_t*>(roots[r]), root_bytes[r] / kBytesPerNode};
-    int rc = convert_root(v, n_slots, wide, bigleaf, &root_refs[r], &max_depth);
+    root_base[r] = maps.base;
+    root_nodes[r] = v.n_nodes;
+    int rc = convert_root(v, n_slots, wide, bigleaf, &root_refs[r], &max_depth, maps);
     if (rc) return rc;
+    maps.base += v.n_nodes;
   }
   if (bigleaf.size() > kRefOffsetMask) return set_error(MBVH_ERR_LIMIT, "mbvh_mesh_upload: too many large leaves");
   std::vector<float4> nodes4;
   std::vector<uint32_t> roots4(n_roots);
   uint32_t depth4 = 0;
   for (uint32_t r = 0; r < n_roots; r++) {
-    const int rc4 = build_bvh4(wide, root_refs[r], nodes4, &roots4[r], &depth4);
+    const int rc4 = build_bvh4(wide, root_refs[r], nodes4, &roots4[r], &depth4, maps.four_map);
     if (rc4) return rc4;
   }
 
@@ -562,6 +615,11 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
   m->max_stack = max_stack;
   m->max_stack4 = max_stack4;
   m->coord_max = coord_max;
+  m->n_ref_nodes = maps.base;
+  m->n_ref_leaves = maps.leaves.size();
+  m->n_four_map = maps.four_map.size();
+  m->root_base = root_base;
+  m->root_nodes = root_nodes;
 
   auto upload = [&](void** dst, const void* src, size_t bytes) -> int {
     hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 16));
@@ -569,7 +627,7 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
       *dst = nullptr;
       return set_error(MBVH_ERR_OOM, "mbvh_mesh_upload: device allocation of %zu bytes failed", bytes);
     }
-    if (bytes) {
+    if (bytes && src) {
       e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
       if (e != hipSuccess) return set_error(MBVH_ERR_HIP, "mbvh_mesh_upload: copy failed (%s)", hipGetErrorString(e));
     }
@@ -582,6 +640,18 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_vidx), vidx.data(), vidx.size() * sizeof(uint32_t));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_nodes4), nodes4.data(), nodes4.size() * sizeof(float4));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_roots4), roots4.data(), roots4.size() * sizeof(uint32_t));
+  // refit tables: the reference nodes as uploaded, parents, box places, leaves, arrival scratch
+  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_ref_nodes), nullptr, maps.base * kBytesPerNode);
+  for (uint32_t r = 0; !rc && r < n_roots; r++) {
+    const hipError_t e = hipMemcpy(reinterpret_cast<uint8_t*>(m->d_ref_nodes) + root_base[r] * kBytesPerNode, roots[r],
+                                   root_bytes[r], hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = set_error(MBVH_ERR_HIP, "mbvh_mesh_upload: copy failed (%s)", hipGetErrorString(e));
+  }
+  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_ref_parent), maps.parent.data(), maps.parent.size() * sizeof(int32_t));
+  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_ref_wide), maps.wide_off.data(), maps.wide_off.size() * sizeof(int32_t));
+  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_ref_leaves), maps.leaves.data(), maps.leaves.size() * sizeof(uint32_t));
+  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_ref_scratch), nullptr, 2 * maps.base * sizeof(uint32_t));
+  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_four_map), maps.four_map.data(), maps.four_map.size() * sizeof(uint2));
   if (!rc && indirect) {
     std::vector<uint32_t> face_slot(face_limit, 0xFFFFFFFFu);
     for (uint64_t sl = 0; sl < n_slots; sl++) face_slot[tris_face(tris, sl)] = static_cast<uint32_t>(sl);
@@ -633,6 +703,71 @@ extern "C" int mbvh_mesh_set_side_mask(mbvh_ctx* ctx, mbvh_mesh* mesh, const uin
   MBVH_HIP(hipMemcpyAsync(ctx->staging[7].ptr, mask, n, hipMemcpyHostToDevice, ctx->stream));
   MBVH_HIP(launch_apply_side_mask(mesh->d_tris, mesh->n_slots, static_cast<const uint8_t*>(ctx->staging[7].ptr), ctx->stream));
   MBVH_HIP(hipStreamSynchronize(ctx->stream));
+  return MBVH_OK;
+}
+
+extern "C" int mbvh_mesh_refit(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint64_t vertex_count,
+                               uint32_t position_stride, uint32_t position_offset, void* const* roots_out,
+                               const uint64_t* root_bytes, uint32_t n_roots) {
+  clear_error();
+  const char* fn = "mbvh_mesh_refit";
+  int rc = check_query(fn, ctx, mesh);
+  if (rc) return rc;
+  if (vertex_count != mesh->vertex_count)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: %llu vertices given, the mesh was uploaded with %llu", fn,
+                     static_cast<unsigned long long>(vertex_count), static_cast<unsigned long long>(mesh->vertex_count));
+  if (position_stride < position_offset + 3) return set_error(MBVH_ERR_BAD_ARGS, "%s: position stride too small", fn);
+  if (!positions && vertex_count) return set_error(MBVH_ERR_BAD_ARGS, "%s: positions is NULL", fn);
+  if (roots_out) {
+    if (!root_bytes) return set_error(MBVH_ERR_BAD_ARGS, "%s: root sizes are NULL", fn);
+    if (n_roots != mesh->n_roots)
+      return set_error(MBVH_ERR_BAD_ARGS, "%s: %u root buffers given, the mesh has %u roots", fn, n_roots, mesh->n_roots);
+    for (uint32_t r = 0; r < mesh->n_roots; r++)
+      if (roots_out[r] && root_bytes[r] != mesh->root_nodes[r] * kBytesPerNode)
+        return set_error(MBVH_ERR_BAD_ARGS, "%s: root %u buffer holds %llu bytes, the tree %llu", fn, r,
+                         static_cast<unsigned long long>(root_bytes[r]),
+                         static_cast<unsigned long long>(mesh->root_nodes[r] * kBytesPerNode));
+  }
+  // an upper bound of |coordinate| over the vertices (the error bounds of the queries use it)
+  float coord_max = 0.0f;
+  for (uint64_t v = 0; v < vertex_count; v++) {
+    const float* p = positions + v * position_stride + position_offset;
+    coord_max = std::max(coord_max, std::max(std::fabs(p[0]), std::max(std::fabs(p[1]), std::fabs(p[2]))));
+  }
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t s = ctx->stream;
+  const size_t bytes = vertex_count ? ((vertex_count - 1) * position_stride + position_offset + 3) * sizeof(float) : 0;
+  if ((rc = ensure_buffer(ctx->staging[0], bytes))) return rc;
+  if (bytes) MBVH_HIP(hipMemcpyAsync(ctx->staging[0].ptr, positions, bytes, hipMemcpyHostToDevice, s));
+  RefitArgs a{};
+  a.ref_nodes = mesh->d_ref_nodes;
+  a.parent = mesh->d_ref_parent;
+  a.wide_off = mesh->d_ref_wide;
+  a.leaves = mesh->d_ref_leaves;
+  a.n_nodes = mesh->n_ref_nodes;
+  a.n_leaves = mesh->n_ref_leaves;
+  a.counters = mesh->d_ref_scratch;
+  a.changed = mesh->d_ref_scratch + mesh->n_ref_nodes;
+  a.wide = reinterpret_cast<float*>(mesh->d_nodes);
+  a.four = reinterpret_cast<float*>(mesh->d_nodes4);
+  a.four_map = mesh->d_four_map;
+  a.n_four = mesh->n_four_map;
+  a.tris = mesh->d_tris;
+  a.vidx = mesh->d_vidx;
+  a.n_slots = mesh->n_slots;
+  a.positions = static_cast<const float*>(ctx->staging[0].ptr);
+  a.stride = position_stride;
+  a.offset = position_offset;
+  MBVH_HIP(launch_refit(a, s));
+  if (roots_out)
+    for (uint32_t r = 0; r < mesh->n_roots; r++)
+      if (roots_out[r])
+        MBVH_HIP(hipMemcpyAsync(roots_out[r], reinterpret_cast<uint8_t*>(mesh->d_ref_nodes) + mesh->root_base[r] * kBytesPerNode,
+                                root_bytes[r], hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipStreamSynchronize(s));
+  mesh->coord_max = coord_max;
+  mesh->version++;
   return MBVH_OK;
 }
 
@@ -1259,6 +1394,22 @@ struct BatchCall {
   uint64_t ws_bytes;
 };
 
+// A refit changes a mesh's view (its coordinate bound); the batch's device copy of the views is
+// rewritten before its next launch (after the device is idle, so no launch still reads it).
+int refresh_views(mbvh_batch* b) {
+  bool stale = false;
+  for (uint32_t i = 0; i < b->n_meshes; i++) stale = stale || b->meshes[i]->version != b->versions[i];
+  if (!stale) return MBVH_OK;
+  std::vector<MeshView> views(b->n_meshes);
+  for (uint32_t i = 0; i < b->n_meshes; i++) {
+    views[i] = view_of(b->meshes[i]);
+    b->versions[i] = b->meshes[i]->version;
+  }
+  MBVH_HIP(hipDeviceSynchronize());
+  MBVH_HIP(hipMemcpy(b->d_meshes, views.data(), views.size() * sizeof(MeshView), hipMemcpyHostToDevice));
+  return MBVH_OK;
+}
+
 int check_batch_call(const char* fn, const mbvh_ctx* ctx, const mbvh_batch* b, const BatchCall& c) {
   if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid context handle", fn);
   if (!valid_batch(b) || b->ctx != ctx) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid batch handle", fn);
@@ -1341,6 +1492,8 @@ extern "C" int mbvh_batch_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, 
   b->n_meshes = n_meshes;
   b->max_stack = max_stack;
   b->max_stack4 = max_stack4;
+  b->meshes.assign(meshes, meshes + n_meshes);
+  for (uint32_t i = 0; i < n_meshes; i++) b->versions.push_back(meshes[i]->version);
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&b->d_meshes), views.size() * sizeof(MeshView));
   if (e == hipSuccess) e = hipMemcpy(b->d_meshes, views.data(), views.size() * sizeof(MeshView), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_counter), 256);
@@ -1392,6 +1545,7 @@ extern "C" int mbvh_batch_raycast_first_d(mbvh_ctx* ctx, mbvh_batch* batch, cons
   const mbvh_batch_tuning t = tuning ? *tuning : batch->tuning;
   if ((rc = check_tuning(fn, t))) return rc;
   if ((rc = set_device(ctx))) return rc;
+  if ((rc = refresh_views(batch))) return rc;
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
   return enqueue_batch(ctx, batch, c, t, s);
 }
@@ -1413,6 +1567,7 @@ extern "C" int mbvh_batch_raycast_first(mbvh_ctx* ctx, mbvh_batch* batch, const 
   std::lock_guard<std::mutex> guard(ctx->lock);
   int rc = set_device(ctx);
   if (rc) return rc;
+  if ((rc = refresh_views(batch))) return rc;
   hipStream_t s = ctx->stream;
   const size_t v3 = n * 3 * sizeof(float), m16 = static_cast<size_t>(n_instances) * 16 * sizeof(float);
   void *dW, *dI, *dG, *dO, *dD, *dR, *oF, *oT, *oP, *oB;
@@ -1455,6 +1610,7 @@ extern "C" int mbvh_batch_autotune(mbvh_ctx* ctx, mbvh_batch* batch, const float
   if (!out_tuning) return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL output", fn);
   std::lock_guard<std::mutex> guard(ctx->lock);
   if ((rc = set_device(ctx))) return rc;
+  if ((rc = refresh_views(batch))) return rc;
   hipStream_t s = ctx->stream;
   hipEvent_t e0, e1;
   MBVH_HIP(hipEventCreate(&e0));
@@ -1511,6 +1667,7 @@ extern "C" int mbvh_batch_graph_create(mbvh_ctx* ctx, mbvh_batch* batch, const f
   if ((rc = check_tuning(fn, t))) return rc;
   std::lock_guard<std::mutex> guard(ctx->lock);
   if ((rc = set_device(ctx))) return rc;
+  if ((rc = refresh_views(batch))) return rc;
   hipStream_t cs;
   MBVH_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   hipGraph_t graph = nullptr;
@@ -1538,6 +1695,7 @@ extern "C" int mbvh_batch_graph_create(mbvh_ctx* ctx, mbvh_batch* batch, const f
     return set_error(MBVH_ERR_OOM, "%s: out of host memory", fn);
   }
   g->ctx = ctx;
+  g->batch = batch;
   g->graph = graph;
   g->exec = exec;
   *out_graph = g;
@@ -1550,6 +1708,7 @@ extern "C" int mbvh_batch_graph_launch(mbvh_batch_graph* graph, void* stream) {
     return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_batch_graph_launch: invalid graph handle");
   int rc = set_device(graph->ctx);
   if (rc) return rc;
+  if (valid_batch(graph->batch) && (rc = refresh_views(graph->batch))) return rc;
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : graph->ctx->stream;
   MBVH_HIP(hipGraphLaunch(graph->exec, s));
   return MBVH_OK;

@@ -76,6 +76,29 @@ struct ClosestPointArgs {
   float* out_bary;
 };
 
+// ---- refit ------------------------------------------------------------------------------------
+struct RefitArgs {
+  uint32_t* ref_nodes;        // the reference-format nodes of all roots, 8 words each
+  const int32_t* parent;      // parent node index (-1 for a root)
+  const int32_t* wide_off;    // float offset of the node's box in the wide nodes (-1: none)
+  const uint32_t* leaves;     // leaf node indices
+  uint64_t n_nodes;
+  uint64_t n_leaves;
+  uint32_t* counters;         // per node: children arrived
+  uint32_t* changed;          // per node: box recomputed
+  float* wide;                // wide nodes as floats (16 per node)
+  float* four;                // 4-wide nodes as floats
+  const uint2* four_map;      // (float offset in the 4-wide tree, float offset in the wide nodes)
+  uint64_t n_four;
+  float4* tris;
+  const uint32_t* vidx;
+  uint64_t n_slots;
+  const float* positions;     // new vertex positions (device)
+  uint32_t stride;
+  uint32_t offset;
+};
+hipError_t launch_refit(const RefitArgs& a, hipStream_t s);
+
 // ---- instances and the batched multi-mesh launch -------------------------------------------------
 struct InstanceRayArgs {
   const float* inverse;          // 16 floats per instance (world -> local, column-major)

@@ -708,6 +708,56 @@ static napi_value js_abi_version(napi_env env, napi_callback_info info) {
     napi_set_named_property(env, exports, name, f);                             \
   } while (0)
 
+/* refitMesh(mesh, positions, stride, offset, roots[]): device refit from new positions; the refit
+ * reference-format nodes are written into the roots' ArrayBuffers */
+static napi_value js_refit(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  if (!get_args(env, info, 5, argv)) return throw_msg(env, "refitMesh: bad arguments");
+  MeshHandle* h = get_mesh(env, argv[0]);
+  if (!h) return throw_msg(env, "refitMesh: invalid mesh");
+  View pos;
+  if (!get_view(env, argv[1], &pos) || pos.type != napi_float32_array)
+    return throw_msg(env, "refitMesh: positions must be a Float32Array");
+  int32_t stride, offset;
+  if (!get_i32(env, argv[2], &stride) || !get_i32(env, argv[3], &offset) || stride <= 0 || offset < 0)
+    return throw_msg(env, "refitMesh: stride / offset must be integers");
+  bool is_arr = false;
+  napi_is_array(env, argv[4], &is_arr);
+  if (!is_arr) return throw_msg(env, "refitMesh: roots must be an array of ArrayBuffers");
+  uint32_t n_roots = 0;
+  napi_get_array_length(env, argv[4], &n_roots);
+  if (n_roots == 0 || n_roots > 1u << 20) return throw_msg(env, "refitMesh: roots must not be empty");
+  void** root_ptrs = (void**)calloc(n_roots, sizeof(void*));
+  uint64_t* root_bytes = (uint64_t*)calloc(n_roots, sizeof(uint64_t));
+  if (!root_ptrs || !root_bytes) {
+    free(root_ptrs);
+    free(root_bytes);
+    return throw_msg(env, "refitMesh: out of memory");
+  }
+  for (uint32_t r = 0; r < n_roots; r++) {
+    napi_value e;
+    napi_get_element(env, argv[4], r, &e);
+    bool is_ab = false;
+    napi_is_arraybuffer(env, e, &is_ab);
+    size_t len = 0;
+    if (!is_ab || napi_get_arraybuffer_info(env, e, &root_ptrs[r], &len) != napi_ok) {
+      free(root_ptrs);
+      free(root_bytes);
+      return throw_msg(env, "refitMesh: each root must be an ArrayBuffer");
+    }
+    root_bytes[r] = len;
+  }
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, h->device, &ctx);
+  if (!rc)
+    rc = mbvh_mesh_refit(ctx, h->mesh, (const float*)pos.data, pos.length / (size_t)stride, (uint32_t)stride,
+                         (uint32_t)offset, root_ptrs, root_bytes, n_roots);
+  free(root_ptrs);
+  free(root_bytes);
+  if (rc) return throw_status(env, rc, "mbvh_mesh_refit");
+  return NULL;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
   EXPORT("abiVersion", js_abi_version);
   EXPORT("deviceCount", js_device_count);
@@ -724,6 +774,7 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("freeBatch", js_free_batch);
   EXPORT("batchRaycastFirst", js_batch_raycast_first);
   EXPORT("closestPointBackward", js_closest_point_bwd);
+  EXPORT("refitMesh", js_refit);
   napi_add_env_cleanup_hook(env, cleanup_contexts, NULL);
   return exports;
 }

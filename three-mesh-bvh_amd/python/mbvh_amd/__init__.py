@@ -30,7 +30,7 @@ EXPORTED_SYMBOLS = (
     "mbvh_tree_copy_indirect", "mbvh_tree_free",
     "mbvh_ctx_create", "mbvh_ctx_destroy", "mbvh_ctx_synchronize", "mbvh_ctx_set_ray_lanes",
     "mbvh_ctx_set_point_lanes",
-    "mbvh_mesh_upload", "mbvh_mesh_free", "mbvh_mesh_info", "mbvh_mesh_set_side_mask",
+    "mbvh_mesh_upload", "mbvh_mesh_free", "mbvh_mesh_info", "mbvh_mesh_set_side_mask", "mbvh_mesh_refit",
     "mbvh_raycast_first", "mbvh_raycast_first_d",
     "mbvh_closest_point", "mbvh_closest_point_d",
     "mbvh_raycast", "mbvh_raycast_fetch", "mbvh_raycast_count_d", "mbvh_raycast_fill_d",
@@ -97,6 +97,7 @@ def library() -> C.CDLL:
         "mbvh_mesh_free": (None, [vp]),
         "mbvh_mesh_info": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(u32)]),
         "mbvh_mesh_set_side_mask": (C.c_int, [vp, vp, vp, u64]),
+        "mbvh_mesh_refit": (C.c_int, [vp, vp, vp, u64, u32, u32, vp, vp, u32]),
         "mbvh_raycast_first": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp]),
         "mbvh_raycast_first_d": (C.c_int, [vp, vp, vp, vp, u64, vp, i32, f32, f32, vp, vp, vp, vp, vp]),
         "mbvh_closest_point": (C.c_int, [vp, vp, vp, u64, f32, f32, vp, vp, vp, vp]),
@@ -283,6 +284,19 @@ class Mesh:
             _ptr(ind), 0 if ind is None else ind.itemsize, 0 if ind is None else ind.size,
             _ptr(pos), pos.size // stride, stride, offset, C.byref(h)))
         self.handle = h
+        self.root_bytes = [r.nbytes for r in tree.roots]
+
+    def refit(self, positions: np.ndarray, stride: int = 3, offset: int = 0, roots: bool = True):
+        """MeshBVH.refit on the device from new positions of the same vertices; returns the refit
+        reference-format root buffers (uint8 arrays) when `roots` is true."""
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+        n = len(self.root_bytes)
+        outs = [np.empty(b, np.uint8) for b in self.root_bytes] if roots else None
+        ptrs = (C.c_void_p * n)(*[o.ctypes.data for o in outs]) if roots else None
+        sizes = (C.c_uint64 * n)(*self.root_bytes)
+        _check(self.ctx._lib.mbvh_mesh_refit(self.ctx.handle, self.handle, _ptr(pos), pos.size // stride, stride,
+                                             offset, ptrs, sizes, n))
+        return outs
 
     @property
     def vertex_count(self) -> int:

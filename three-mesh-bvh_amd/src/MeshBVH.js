@@ -186,6 +186,20 @@ class MeshBVH {
     return this;
   }
 
+  // refit (src/core/MeshBVH.js:504-508 over src/core/cast/refit.template.js): the node boxes are
+  // recomputed on the GPU from the geometry's current positions with the reference's rules (a leaf
+  // is the min / max of its triangles' vertices and is rewritten only if it differs, an internal
+  // node is the union of its children, recomputed only if a child changed). The refit nodes are
+  // written back into this._roots, so serialize() and later uploads see them, and the device copy
+  // is updated in place. Partial refits (nodeIndices from shapecast) are not supported.
+  refit(nodeIndices = null) {
+    if (nodeIndices !== null && nodeIndices !== undefined) {
+      throw new Error('MeshBVH.refit: partial refit (nodeIndices) is not supported; call refit()');
+    }
+    const pos = GU.positionLayout(this.geometry);
+    native.load().refitMesh(this._mesh(), pos.array, pos.stride, pos.offset, this._roots);
+  }
+
   // Releases the device copy now (otherwise released when the BVH is garbage collected).
   dispose() {
     if (this._gpu) native.load().freeMesh(this._gpu);
