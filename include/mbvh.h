@@ -162,6 +162,15 @@ int mbvh_mesh_set_side_mask(mbvh_ctx* ctx, mbvh_mesh* mesh, const uint8_t* mask,
 int mbvh_mesh_refit(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint64_t vertex_count,
                     uint32_t position_stride, uint32_t position_offset, void* const* roots_out,
                     const uint64_t* root_bytes, uint32_t n_roots);
+/* Device form of mbvh_mesh_refit: positions in device memory (for example updated by an optimiser
+ * from the vertex gradients), enqueued on `stream`, no copy back, no synchronisation, no allocation.
+ * Queries launched after it on the same stream see the refit geometry. */
+int mbvh_mesh_refit_d(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint64_t vertex_count,
+                      uint32_t position_stride, uint32_t position_offset, void* stream);
+/* Copies the mesh's current reference-format node buffers (as refit) into roots_out[r]
+ * (root_bytes[r] bytes each, n_roots entries; NULL entries skipped). Synchronous. */
+int mbvh_mesh_copy_roots(mbvh_ctx* ctx, const mbvh_mesh* mesh, void* const* roots_out, const uint64_t* root_bytes,
+                         uint32_t n_roots);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Queries                                                                                    */

@@ -119,3 +119,37 @@ def test_refit_rejects_other_vertex_counts(ctx):
     with pytest.raises(M.MbvhError):
         mesh.refit(g.positions.reshape(-1, 3)[:-1])
     mesh.free()
+
+
+def test_refit_device_form_in_an_optimisation_step(ctx):
+    """Forward, backward (vertex gradients), a gradient step on the device positions, refit from
+    those device positions: the refit tree equals the oracle's refit of the same positions, and the
+    next forward pass matches the oracle on the new geometry and the host refit bit for bit."""
+    torch = pytest.importorskip("torch")
+    dev = torch.device("cuda:0")
+    g, idx, tree = build_mesh(G.torus_knot(10, 3, 240, 32))
+    mesh = ctx.upload(tree, g.positions, idx)
+    lo, hi = g.bounds()
+    o, d = _rays(lo, hi, 8000, 9)
+    fwd = mesh.raycast_first(o, d)
+    gd = np.ones(len(o), np.float32)
+    gv = mesh.raycast_first_bwd(d, fwd, grad_distance=gd, mode=M.GRAD_DETERMINISTIC)["positions"]
+    pos_t = torch.from_numpy(g.positions.reshape(-1, 3).copy()).to(dev)
+    pos_t -= 0.5 * torch.from_numpy(gv).to(dev)  # descend the summed hit distances
+    pos_t *= 3.0  # and grow the mesh: the coordinate bound must follow
+    torch.cuda.synchronize()
+    mesh.refit_d(pos_t.data_ptr(), pos_t.shape[0])
+    ctx.synchronize()
+    pos = pos_t.cpu().numpy()
+    roots = mesh.copy_roots()
+    ref_roots = O.OracleMesh(pos, idx, tree.roots).refit()
+    assert all(np.array_equal(a, b) for a, b in zip(roots, ref_roots))
+    o2, d2 = _rays(pos.min(axis=0), pos.max(axis=0), 8000, 10)
+    got = mesh.raycast_first(o2, d2)
+    om = O.OracleMesh(pos, idx, ref_roots)
+    check_raycast_parity(om, o2, d2, got, om.raycast_first(o2, d2), "refit_d rays")
+    host = ctx.upload(tree, g.positions, idx)
+    host.refit(pos)
+    _same(got, host.raycast_first(o2, d2))
+    host.free()
+    mesh.free()

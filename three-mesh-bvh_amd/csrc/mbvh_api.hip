@@ -54,14 +54,11 @@ struct mbvh_batch {
   uint32_t max_stack = 0;
   uint32_t max_stack4 = 0;
   mbvh_batch_tuning tuning{4, 16, 256, 4, 0.0f};
-  std::vector<const mbvh_mesh*> meshes;  // for refreshing d_meshes after a refit
-  std::vector<uint64_t> versions;
 };
 
 struct mbvh_batch_graph {
   uint32_t magic = 0x48505247u;  // "GRPH"
   mbvh_ctx* ctx = nullptr;
-  mbvh_batch* batch = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -85,8 +82,7 @@ struct mbvh_mesh {
   uint64_t face_limit = 0;  // face ids are < face_limit
   uint32_t max_depth = 0;   // deepest reference node depth over all roots
   uint32_t max_stack = 0;   // traversal stack entries needed
-  float coord_max = 0.0f;   // largest |vertex coordinate| over the mesh's triangles (an upper bound after a refit)
-  uint64_t version = 0;     // incremented by each refit (batches refresh their copies of the mesh view)
+  float* d_coord_max = nullptr;  // device: largest |vertex coordinate| (after a refit: over all vertices)
   // refit: the reference-format nodes of all roots and how their boxes map onto the device layouts
   uint32_t* d_ref_nodes = nullptr;   // 8 words per node, roots concatenated
   int32_t* d_ref_parent = nullptr;   // parent node (-1 for a root)
@@ -336,7 +332,8 @@ void free_mesh_buffers(mbvh_mesh* m) {
   if (m->d_roots4) (void)hipFree(m->d_roots4);
   for (void* p : {static_cast<void*>(m->d_ref_nodes), static_cast<void*>(m->d_ref_parent),
                   static_cast<void*>(m->d_ref_wide), static_cast<void*>(m->d_ref_leaves),
-                  static_cast<void*>(m->d_ref_scratch), static_cast<void*>(m->d_four_map)})
+                  static_cast<void*>(m->d_ref_scratch), static_cast<void*>(m->d_four_map),
+                  static_cast<void*>(m->d_coord_max)})
     if (p) (void)hipFree(p);
   m->d_ref_nodes = nullptr;
   m->d_ref_parent = nullptr;
@@ -344,6 +341,7 @@ void free_mesh_buffers(mbvh_mesh* m) {
   m->d_ref_leaves = nullptr;
   m->d_ref_scratch = nullptr;
   m->d_four_map = nullptr;
+  m->d_coord_max = nullptr;
   m->d_face_slot = nullptr;
   m->d_nodes4 = nullptr;
   m->d_roots4 = nullptr;
@@ -365,7 +363,7 @@ MeshView view_of(const mbvh_mesh* m) {
   v.tris = m->d_tris;
   v.n_roots = m->n_roots;
   v.max_stack = m->max_stack;
-  v.coord_max = m->coord_max;
+  v.coord_max = m->d_coord_max;
   return v;
 }
 
@@ -614,7 +612,6 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
   m->max_depth = max_depth;
   m->max_stack = max_stack;
   m->max_stack4 = max_stack4;
-  m->coord_max = coord_max;
   m->n_ref_nodes = maps.base;
   m->n_ref_leaves = maps.leaves.size();
   m->n_four_map = maps.four_map.size();
@@ -652,6 +649,7 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_ref_leaves), maps.leaves.data(), maps.leaves.size() * sizeof(uint32_t));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_ref_scratch), nullptr, 2 * maps.base * sizeof(uint32_t));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_four_map), maps.four_map.data(), maps.four_map.size() * sizeof(uint2));
+  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_coord_max), &coord_max, sizeof(float));
   if (!rc && indirect) {
     std::vector<uint32_t> face_slot(face_limit, 0xFFFFFFFFu);
     for (uint64_t sl = 0; sl < n_slots; sl++) face_slot[tris_face(tris, sl)] = static_cast<uint32_t>(sl);
@@ -706,11 +704,10 @@ extern "C" int mbvh_mesh_set_side_mask(mbvh_ctx* ctx, mbvh_mesh* mesh, const uin
   return MBVH_OK;
 }
 
-extern "C" int mbvh_mesh_refit(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint64_t vertex_count,
-                               uint32_t position_stride, uint32_t position_offset, void* const* roots_out,
-                               const uint64_t* root_bytes, uint32_t n_roots) {
-  clear_error();
-  const char* fn = "mbvh_mesh_refit";
+namespace {
+
+int check_refit(const char* fn, mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* positions, uint64_t vertex_count,
+                uint32_t position_stride, uint32_t position_offset) {
   int rc = check_query(fn, ctx, mesh);
   if (rc) return rc;
   if (vertex_count != mesh->vertex_count)
@@ -718,28 +715,25 @@ extern "C" int mbvh_mesh_refit(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* posi
                      static_cast<unsigned long long>(vertex_count), static_cast<unsigned long long>(mesh->vertex_count));
   if (position_stride < position_offset + 3) return set_error(MBVH_ERR_BAD_ARGS, "%s: position stride too small", fn);
   if (!positions && vertex_count) return set_error(MBVH_ERR_BAD_ARGS, "%s: positions is NULL", fn);
-  if (roots_out) {
-    if (!root_bytes) return set_error(MBVH_ERR_BAD_ARGS, "%s: root sizes are NULL", fn);
-    if (n_roots != mesh->n_roots)
-      return set_error(MBVH_ERR_BAD_ARGS, "%s: %u root buffers given, the mesh has %u roots", fn, n_roots, mesh->n_roots);
-    for (uint32_t r = 0; r < mesh->n_roots; r++)
-      if (roots_out[r] && root_bytes[r] != mesh->root_nodes[r] * kBytesPerNode)
-        return set_error(MBVH_ERR_BAD_ARGS, "%s: root %u buffer holds %llu bytes, the tree %llu", fn, r,
-                         static_cast<unsigned long long>(root_bytes[r]),
-                         static_cast<unsigned long long>(mesh->root_nodes[r] * kBytesPerNode));
-  }
-  // an upper bound of |coordinate| over the vertices (the error bounds of the queries use it)
-  float coord_max = 0.0f;
-  for (uint64_t v = 0; v < vertex_count; v++) {
-    const float* p = positions + v * position_stride + position_offset;
-    coord_max = std::max(coord_max, std::max(std::fabs(p[0]), std::max(std::fabs(p[1]), std::fabs(p[2]))));
-  }
-  std::lock_guard<std::mutex> guard(ctx->lock);
-  if ((rc = set_device(ctx))) return rc;
-  hipStream_t s = ctx->stream;
-  const size_t bytes = vertex_count ? ((vertex_count - 1) * position_stride + position_offset + 3) * sizeof(float) : 0;
-  if ((rc = ensure_buffer(ctx->staging[0], bytes))) return rc;
-  if (bytes) MBVH_HIP(hipMemcpyAsync(ctx->staging[0].ptr, positions, bytes, hipMemcpyHostToDevice, s));
+  return MBVH_OK;
+}
+
+int check_roots_out(const char* fn, const mbvh_mesh* mesh, void* const* roots_out, const uint64_t* root_bytes,
+                    uint32_t n_roots) {
+  if (!roots_out) return MBVH_OK;
+  if (!root_bytes) return set_error(MBVH_ERR_BAD_ARGS, "%s: root sizes are NULL", fn);
+  if (n_roots != mesh->n_roots)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: %u root buffers given, the mesh has %u roots", fn, n_roots, mesh->n_roots);
+  for (uint32_t r = 0; r < mesh->n_roots; r++)
+    if (roots_out[r] && root_bytes[r] != mesh->root_nodes[r] * kBytesPerNode)
+      return set_error(MBVH_ERR_BAD_ARGS, "%s: root %u buffer holds %llu bytes, the tree %llu", fn, r,
+                       static_cast<unsigned long long>(root_bytes[r]),
+                       static_cast<unsigned long long>(mesh->root_nodes[r] * kBytesPerNode));
+  return MBVH_OK;
+}
+
+int enqueue_refit(mbvh_mesh* mesh, const float* d_positions, uint64_t vertex_count, uint32_t stride, uint32_t offset,
+                  hipStream_t s) {
   RefitArgs a{};
   a.ref_nodes = mesh->d_ref_nodes;
   a.parent = mesh->d_ref_parent;
@@ -756,18 +750,72 @@ extern "C" int mbvh_mesh_refit(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* posi
   a.tris = mesh->d_tris;
   a.vidx = mesh->d_vidx;
   a.n_slots = mesh->n_slots;
-  a.positions = static_cast<const float*>(ctx->staging[0].ptr);
-  a.stride = position_stride;
-  a.offset = position_offset;
+  a.positions = d_positions;
+  a.n_vertices = vertex_count;
+  a.stride = stride;
+  a.offset = offset;
+  a.coord_bound = mesh->d_coord_max;
   MBVH_HIP(launch_refit(a, s));
-  if (roots_out)
-    for (uint32_t r = 0; r < mesh->n_roots; r++)
-      if (roots_out[r])
-        MBVH_HIP(hipMemcpyAsync(roots_out[r], reinterpret_cast<uint8_t*>(mesh->d_ref_nodes) + mesh->root_base[r] * kBytesPerNode,
-                                root_bytes[r], hipMemcpyDeviceToHost, s));
+  return MBVH_OK;
+}
+
+int enqueue_copy_roots(const mbvh_mesh* mesh, void* const* roots_out, const uint64_t* root_bytes, hipStream_t s) {
+  if (!roots_out) return MBVH_OK;
+  for (uint32_t r = 0; r < mesh->n_roots; r++)
+    if (roots_out[r])
+      MBVH_HIP(hipMemcpyAsync(roots_out[r], reinterpret_cast<uint8_t*>(mesh->d_ref_nodes) + mesh->root_base[r] * kBytesPerNode,
+                              root_bytes[r], hipMemcpyDeviceToHost, s));
+  return MBVH_OK;
+}
+
+}  // namespace
+
+extern "C" int mbvh_mesh_refit(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint64_t vertex_count,
+                               uint32_t position_stride, uint32_t position_offset, void* const* roots_out,
+                               const uint64_t* root_bytes, uint32_t n_roots) {
+  clear_error();
+  const char* fn = "mbvh_mesh_refit";
+  int rc = check_refit(fn, ctx, mesh, positions, vertex_count, position_stride, position_offset);
+  if (!rc) rc = check_roots_out(fn, mesh, roots_out, root_bytes, n_roots);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t s = ctx->stream;
+  const size_t bytes = vertex_count ? ((vertex_count - 1) * position_stride + position_offset + 3) * sizeof(float) : 0;
+  if ((rc = ensure_buffer(ctx->staging[0], bytes))) return rc;
+  if (bytes) MBVH_HIP(hipMemcpyAsync(ctx->staging[0].ptr, positions, bytes, hipMemcpyHostToDevice, s));
+  if ((rc = enqueue_refit(mesh, static_cast<const float*>(ctx->staging[0].ptr), vertex_count, position_stride,
+                          position_offset, s)) ||
+      (rc = enqueue_copy_roots(mesh, roots_out, root_bytes, s)))
+    return rc;
   MBVH_HIP(hipStreamSynchronize(s));
-  mesh->coord_max = coord_max;
-  mesh->version++;
+  return MBVH_OK;
+}
+
+extern "C" int mbvh_mesh_refit_d(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint64_t vertex_count,
+                                 uint32_t position_stride, uint32_t position_offset, void* stream) {
+  clear_error();
+  const char* fn = "mbvh_mesh_refit_d";
+  int rc = check_refit(fn, ctx, mesh, positions, vertex_count, position_stride, position_offset);
+  if (rc) return rc;
+  if ((rc = set_device(ctx))) return rc;
+  return enqueue_refit(mesh, positions, vertex_count, position_stride, position_offset,
+                       stream ? static_cast<hipStream_t>(stream) : ctx->stream);
+}
+
+extern "C" int mbvh_mesh_copy_roots(mbvh_ctx* ctx, const mbvh_mesh* mesh, void* const* roots_out,
+                                    const uint64_t* root_bytes, uint32_t n_roots) {
+  clear_error();
+  const char* fn = "mbvh_mesh_copy_roots";
+  int rc = check_query(fn, ctx, mesh);
+  if (!rc && !roots_out) rc = set_error(MBVH_ERR_BAD_ARGS, "%s: roots_out is NULL", fn);
+  if (!rc) rc = check_roots_out(fn, mesh, roots_out, root_bytes, n_roots);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  MBVH_HIP(hipDeviceSynchronize());  // a refit may be in flight on another stream
+  if ((rc = enqueue_copy_roots(mesh, roots_out, root_bytes, ctx->stream))) return rc;
+  MBVH_HIP(hipStreamSynchronize(ctx->stream));
   return MBVH_OK;
 }
 
@@ -1394,22 +1442,6 @@ struct BatchCall {
   uint64_t ws_bytes;
 };
 
-// A refit changes a mesh's view (its coordinate bound); the batch's device copy of the views is
-// rewritten before its next launch (after the device is idle, so no launch still reads it).
-int refresh_views(mbvh_batch* b) {
-  bool stale = false;
-  for (uint32_t i = 0; i < b->n_meshes; i++) stale = stale || b->meshes[i]->version != b->versions[i];
-  if (!stale) return MBVH_OK;
-  std::vector<MeshView> views(b->n_meshes);
-  for (uint32_t i = 0; i < b->n_meshes; i++) {
-    views[i] = view_of(b->meshes[i]);
-    b->versions[i] = b->meshes[i]->version;
-  }
-  MBVH_HIP(hipDeviceSynchronize());
-  MBVH_HIP(hipMemcpy(b->d_meshes, views.data(), views.size() * sizeof(MeshView), hipMemcpyHostToDevice));
-  return MBVH_OK;
-}
-
 int check_batch_call(const char* fn, const mbvh_ctx* ctx, const mbvh_batch* b, const BatchCall& c) {
   if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid context handle", fn);
   if (!valid_batch(b) || b->ctx != ctx) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid batch handle", fn);
@@ -1492,8 +1524,6 @@ extern "C" int mbvh_batch_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, 
   b->n_meshes = n_meshes;
   b->max_stack = max_stack;
   b->max_stack4 = max_stack4;
-  b->meshes.assign(meshes, meshes + n_meshes);
-  for (uint32_t i = 0; i < n_meshes; i++) b->versions.push_back(meshes[i]->version);
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&b->d_meshes), views.size() * sizeof(MeshView));
   if (e == hipSuccess) e = hipMemcpy(b->d_meshes, views.data(), views.size() * sizeof(MeshView), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_counter), 256);
@@ -1545,7 +1575,6 @@ extern "C" int mbvh_batch_raycast_first_d(mbvh_ctx* ctx, mbvh_batch* batch, cons
   const mbvh_batch_tuning t = tuning ? *tuning : batch->tuning;
   if ((rc = check_tuning(fn, t))) return rc;
   if ((rc = set_device(ctx))) return rc;
-  if ((rc = refresh_views(batch))) return rc;
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
   return enqueue_batch(ctx, batch, c, t, s);
 }
@@ -1567,7 +1596,6 @@ extern "C" int mbvh_batch_raycast_first(mbvh_ctx* ctx, mbvh_batch* batch, const 
   std::lock_guard<std::mutex> guard(ctx->lock);
   int rc = set_device(ctx);
   if (rc) return rc;
-  if ((rc = refresh_views(batch))) return rc;
   hipStream_t s = ctx->stream;
   const size_t v3 = n * 3 * sizeof(float), m16 = static_cast<size_t>(n_instances) * 16 * sizeof(float);
   void *dW, *dI, *dG, *dO, *dD, *dR, *oF, *oT, *oP, *oB;
@@ -1610,7 +1638,6 @@ extern "C" int mbvh_batch_autotune(mbvh_ctx* ctx, mbvh_batch* batch, const float
   if (!out_tuning) return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL output", fn);
   std::lock_guard<std::mutex> guard(ctx->lock);
   if ((rc = set_device(ctx))) return rc;
-  if ((rc = refresh_views(batch))) return rc;
   hipStream_t s = ctx->stream;
   hipEvent_t e0, e1;
   MBVH_HIP(hipEventCreate(&e0));
@@ -1667,7 +1694,6 @@ extern "C" int mbvh_batch_graph_create(mbvh_ctx* ctx, mbvh_batch* batch, const f
   if ((rc = check_tuning(fn, t))) return rc;
   std::lock_guard<std::mutex> guard(ctx->lock);
   if ((rc = set_device(ctx))) return rc;
-  if ((rc = refresh_views(batch))) return rc;
   hipStream_t cs;
   MBVH_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   hipGraph_t graph = nullptr;
@@ -1695,7 +1721,6 @@ extern "C" int mbvh_batch_graph_create(mbvh_ctx* ctx, mbvh_batch* batch, const f
     return set_error(MBVH_ERR_OOM, "%s: out of host memory", fn);
   }
   g->ctx = ctx;
-  g->batch = batch;
   g->graph = graph;
   g->exec = exec;
   *out_graph = g;
@@ -1708,7 +1733,6 @@ extern "C" int mbvh_batch_graph_launch(mbvh_batch_graph* graph, void* stream) {
     return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_batch_graph_launch: invalid graph handle");
   int rc = set_device(graph->ctx);
   if (rc) return rc;
-  if (valid_batch(graph->batch) && (rc = refresh_views(graph->batch))) return rc;
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : graph->ctx->stream;
   MBVH_HIP(hipGraphLaunch(graph->exec, s));
   return MBVH_OK;

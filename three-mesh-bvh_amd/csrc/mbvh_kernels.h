@@ -18,7 +18,8 @@ struct MeshView {
   const float4* tris;  // 3 per slot
   uint32_t n_roots;
   uint32_t max_stack;  // stack entries a traversal of this mesh can need
-  float coord_max;     // largest |vertex coordinate| (scales FP32 error bounds)
+  const float* coord_max;  // device: an upper bound of |vertex coordinate| (scales FP32 error bounds;
+                           // rewritten by a refit)
 };
 
 struct StackScratch {
@@ -94,8 +95,10 @@ struct RefitArgs {
   const uint32_t* vidx;
   uint64_t n_slots;
   const float* positions;     // new vertex positions (device)
+  uint64_t n_vertices;
   uint32_t stride;
   uint32_t offset;
+  float* coord_bound;         // the mesh's coordinate bound, recomputed from the positions
 };
 hipError_t launch_refit(const RefitArgs& a, hipStream_t s);
 

@@ -641,7 +641,7 @@ __device__ __forceinline__ uint32_t quad_children(QuadStack& st, uint32_t lane, 
 __device__ __forceinline__ FirstHit trace_first4(const MeshView& M, QuadStack& st, uint32_t lane, f3 o, f3 d,
                                                  uint32_t side_mask, float near_dist, float far_dist) {
   const uint32_t q = lane & 3u;
-  const RayBox4 rb = ray_box4(o, d, M.coord_max);
+  const RayBox4 rb = ray_box4(o, d, *M.coord_max);
   const float dlen = sqrtf(dot(d, d));
   const float dl1 = fabsf(d.x) + fabsf(d.y) + fabsf(d.z);
   float U = kInf;  // quad-wide upper bound of the best hit's t (after each leaf)
@@ -777,7 +777,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_raycast_first4(RaycastFirstArgs A
     if (ray >= A.n) continue;
     const f3 o = mk(A.origins[3 * ray + 0], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
     const f3 d = mk(A.directions[3 * ray + 0], A.directions[3 * ray + 1], A.directions[3 * ray + 2]);
-    const bool literal = literal_ray(o, d, A.mesh.coord_max);
+    const bool literal = literal_ray(o, d, *A.mesh.coord_max);
     const FirstHit h = literal ? trace_first(A.mesh, st, lane, o, d, A.side_mask, A.near_dist, A.far_dist)
                                : trace_first4(A.mesh, st, lane, o, d, A.side_mask, A.near_dist, A.far_dist);
     if ((lane & 3u) == 0) {
@@ -1009,7 +1009,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_batch_raycast_first4(BatchArgs A)
         const f3 d = transform_direction(inv, mk(A.directions[3 * ray], A.directions[3 * ray + 1], A.directions[3 * ray + 2]));
         const float sc = instance_scale(A.world + 16ull * inst, d);
         const MeshView& M = A.meshes[geom];
-        const bool literal = literal_ray(o, d, M.coord_max);
+        const bool literal = literal_ray(o, d, *M.coord_max);
         h = literal ? trace_first(M, st, tid, o, d, A.side_mask, A.near_dist / sc, A.far_dist / sc)
                     : trace_first4(M, st, tid, o, d, A.side_mask, A.near_dist / sc, A.far_dist / sc);
         if (h.face < kOverflow) {
@@ -1264,7 +1264,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
     const uint64_t qi = A.order ? A.order[visit] : visit;
     if (qi >= A.n) continue;
     const f3 q = mk(A.points[3 * qi + 0], A.points[3 * qi + 1], A.points[3 * qi + 2]);
-    const float e_abs = 8.0f * kU * (maxabs(q) + M.coord_max);
+    const float e_abs = 8.0f * kU * (maxabs(q) + *M.coord_max);
     float U = kInf;  // quad-wide upper bound of the best squared distance (after each leaf)
     bool overflow = false;
     Cands cl;
@@ -1426,7 +1426,7 @@ __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
     if (qi >= A.n) continue;  // not a permutation of 0..n-1: skip rather than read out of bounds
     const f3 q = mk(A.points[3 * qi + 0], A.points[3 * qi + 1], A.points[3 * qi + 2]);
     // Absolute error bound of an FP32 point-to-triangle distance for this query.
-    const float e_abs = 8.0f * kU * (maxabs(q) + M.coord_max);
+    const float e_abs = 8.0f * kU * (maxabs(q) + *M.coord_max);
     float U = kInf;       // upper bound of the best squared distance
     float best = kInf;    // smallest FP32 squared distance seen
     f3 best_cp = mk(kInf, kInf, kInf);

@@ -9,7 +9,8 @@
 // changed itself.
 //
 // Device form: one thread per leaf walks up through parent links. Each node has an arrival counter
-// (cleared by k_refit_clear); the second child to arrive computes the parent (release fence before
+// (cleared by k_refit_clear, which also resets the coordinate bound that k_refit_bound recomputes);
+// the second child to arrive computes the parent (release fence before
 // the counter increment, acquire fence after it), so every internal node is decided once, after
 // both children. A node's box is written to the device copy of the reference nodes and, where the
 // node has one, to its place in the parent's wide node; k_refit_four then copies the changed wide
@@ -22,12 +23,34 @@ namespace {
 
 constexpr int kRefitBlock = 256;
 
-__global__ __launch_bounds__(kRefitBlock) void k_refit_clear(uint32_t* counters, uint32_t* changed, uint64_t n) {
+__global__ __launch_bounds__(kRefitBlock) void k_refit_clear(uint32_t* counters, uint32_t* changed, uint64_t n,
+                                                              float* coord_bound) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kRefitBlock;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *coord_bound = 0.0f;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kRefitBlock + threadIdx.x; i < n; i += stride) {
     counters[i] = 0u;
     changed[i] = 0u;
   }
+}
+
+// The mesh's coordinate bound: max |coordinate| over the vertices (NaN ignored; an infinite one
+// makes the bound infinite), through one atomic max per wave on the float bits (monotonic for
+// non-negative floats).
+__global__ __launch_bounds__(kRefitBlock) void k_refit_bound(RefitArgs a) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kRefitBlock;
+  float m = 0.0f;
+  for (uint64_t v = static_cast<uint64_t>(blockIdx.x) * kRefitBlock + threadIdx.x; v < a.n_vertices; v += stride) {
+    const float* p = a.positions + v * a.stride + a.offset;
+    for (int k = 0; k < 3; k++) {
+      const float x = fabsf(p[k]);
+      if (x > m) m = x;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const float o = __shfl_xor(m, off, 64);
+    if (o > m) m = o;
+  }
+  if ((threadIdx.x & 63u) == 0u) atomicMax(reinterpret_cast<unsigned int*>(a.coord_bound), __float_as_uint(m));
 }
 
 __global__ __launch_bounds__(kRefitBlock) void k_refit_tris(RefitArgs a) {
@@ -118,7 +141,9 @@ uint32_t blocks_for(uint64_t n) {
 
 hipError_t launch_refit(const RefitArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_refit_clear, dim3(blocks_for(a.n_nodes)), dim3(kRefitBlock), 0, s, a.counters, a.changed,
-                     a.n_nodes);
+                     a.n_nodes, a.coord_bound);
+  hipLaunchKernelGGL(k_refit_bound, dim3(blocks_for(a.n_vertices) < 1024 ? blocks_for(a.n_vertices) : 1024),
+                     dim3(kRefitBlock), 0, s, a);
   if (a.n_slots) hipLaunchKernelGGL(k_refit_tris, dim3(blocks_for(3 * a.n_slots)), dim3(kRefitBlock), 0, s, a);
   if (a.n_leaves) hipLaunchKernelGGL(k_refit_up, dim3(blocks_for(a.n_leaves)), dim3(kRefitBlock), 0, s, a);
   if (a.n_four) hipLaunchKernelGGL(k_refit_four, dim3(blocks_for(a.n_four)), dim3(kRefitBlock), 0, s, a);

@@ -31,6 +31,7 @@ EXPORTED_SYMBOLS = (
     "mbvh_ctx_create", "mbvh_ctx_destroy", "mbvh_ctx_synchronize", "mbvh_ctx_set_ray_lanes",
     "mbvh_ctx_set_point_lanes",
     "mbvh_mesh_upload", "mbvh_mesh_free", "mbvh_mesh_info", "mbvh_mesh_set_side_mask", "mbvh_mesh_refit",
+    "mbvh_mesh_refit_d", "mbvh_mesh_copy_roots",
     "mbvh_raycast_first", "mbvh_raycast_first_d",
     "mbvh_closest_point", "mbvh_closest_point_d",
     "mbvh_raycast", "mbvh_raycast_fetch", "mbvh_raycast_count_d", "mbvh_raycast_fill_d",
@@ -98,6 +99,8 @@ def library() -> C.CDLL:
         "mbvh_mesh_info": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(u32)]),
         "mbvh_mesh_set_side_mask": (C.c_int, [vp, vp, vp, u64]),
         "mbvh_mesh_refit": (C.c_int, [vp, vp, vp, u64, u32, u32, vp, vp, u32]),
+        "mbvh_mesh_refit_d": (C.c_int, [vp, vp, vp, u64, u32, u32, vp]),
+        "mbvh_mesh_copy_roots": (C.c_int, [vp, vp, vp, vp, u32]),
         "mbvh_raycast_first": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp]),
         "mbvh_raycast_first_d": (C.c_int, [vp, vp, vp, vp, u64, vp, i32, f32, f32, vp, vp, vp, vp, vp]),
         "mbvh_closest_point": (C.c_int, [vp, vp, vp, u64, f32, f32, vp, vp, vp, vp]),
@@ -296,6 +299,21 @@ class Mesh:
         sizes = (C.c_uint64 * n)(*self.root_bytes)
         _check(self.ctx._lib.mbvh_mesh_refit(self.ctx.handle, self.handle, _ptr(pos), pos.size // stride, stride,
                                              offset, ptrs, sizes, n))
+        return outs
+
+    def refit_d(self, positions_ptr: int, vertex_count: int, stride: int = 3, offset: int = 0,
+                stream: int = 0) -> None:
+        """Refit from device positions, enqueued on `stream` (no synchronisation)."""
+        _check(self.ctx._lib.mbvh_mesh_refit_d(self.ctx.handle, self.handle, C.c_void_p(positions_ptr), vertex_count,
+                                               stride, offset, C.c_void_p(stream or None)))
+
+    def copy_roots(self):
+        """The mesh's current reference-format root buffers (uint8 arrays)."""
+        n = len(self.root_bytes)
+        outs = [np.empty(b, np.uint8) for b in self.root_bytes]
+        ptrs = (C.c_void_p * n)(*[o.ctypes.data for o in outs])
+        sizes = (C.c_uint64 * n)(*self.root_bytes)
+        _check(self.ctx._lib.mbvh_mesh_copy_roots(self.ctx.handle, self.handle, ptrs, sizes, n))
         return outs
 
     @property
