@@ -85,14 +85,15 @@ struct mbvh_mesh {
   float* d_coord_max = nullptr;  // device: largest |vertex coordinate| (after a refit: over all vertices)
   // refit: the reference-format nodes of all roots and how their boxes map onto the device layouts
   uint32_t* d_ref_nodes = nullptr;   // 8 words per node, roots concatenated
-  int32_t* d_ref_parent = nullptr;   // parent node (-1 for a root)
+  uint32_t* d_ref_levels = nullptr;  // reachable internal nodes, grouped by depth
   int32_t* d_ref_wide = nullptr;     // float offset of the node's box in the wide nodes (-1: none)
   uint32_t* d_ref_leaves = nullptr;  // leaf nodes
-  uint32_t* d_ref_scratch = nullptr; // 2 words per node: arrival counter, changed flag
+  uint32_t* d_ref_scratch = nullptr; // per node: changed flag
   uint2* d_four_map = nullptr;       // 4-wide child slot <- wide box (float offsets)
   uint64_t n_ref_nodes = 0, n_ref_leaves = 0, n_four_map = 0;
   std::vector<uint64_t> root_base;   // first node of each root
   std::vector<uint64_t> root_nodes;  // nodes of each root
+  std::vector<uint64_t> level_off;   // d_ref_levels[level_off[d] .. level_off[d + 1]) are at depth d
 };
 
 namespace {
@@ -156,8 +157,8 @@ uint32_t inline_leaf_ref(uint64_t off, uint64_t cnt, std::vector<uint2>& bigleaf
 // all roots, `base` being this root's first.
 struct RefitMaps {
   uint64_t base = 0;
-  std::vector<int32_t> parent;    // parent node, -1 for a root (and for nodes no traversal reaches)
   std::vector<int32_t> wide_off;  // float offset of the node's box in the wide nodes, -1: none
+  std::vector<std::vector<uint32_t>> internal_by_depth;  // reachable internal nodes per depth
   std::vector<uint32_t> leaves;   // reachable leaves
   std::vector<uint2> four_map;    // (float offset of a 4-wide child slot, float offset of its wide box)
 };
@@ -176,12 +177,10 @@ int convert_root(const NodeView& v, uint64_t n_slots, std::vector<WideNode>& wid
     uint32_t depth;
     int64_t parent_wide;  // -1 for the root
     int side;             // 0 left, 1 right
-    int64_t parent_node;  // -1 for the root
   };
   std::vector<Item> todo;
-  todo.push_back({0, 0, -1, 0, -1});
+  todo.push_back({0, 0, -1, 0});
   if (maps.base + v.n_nodes > 0x7FFFFFFFull) return set_error(MBVH_ERR_LIMIT, "mbvh_mesh_upload: too many nodes");
-  maps.parent.resize(maps.base + v.n_nodes, -1);
   maps.wide_off.resize(maps.base + v.n_nodes, -1);
   uint64_t visited = 0;
   while (!todo.empty()) {
@@ -191,7 +190,6 @@ int convert_root(const NodeView& v, uint64_t n_slots, std::vector<WideNode>& wid
       return set_error(MBVH_ERR_BAD_ARGS, "mbvh_mesh_upload: node buffer contains a cycle");
     *max_depth = std::max(*max_depth, it.depth);
     const uint64_t g = maps.base + it.node;
-    maps.parent[g] = it.parent_node < 0 ? -1 : static_cast<int32_t>(maps.base + it.parent_node);
     if (it.parent_wide >= 0) maps.wide_off[g] = static_cast<int32_t>(16 * it.parent_wide + 6 * it.side);
     uint32_t ref;
     if (v.is_leaf(it.node)) {
@@ -201,6 +199,8 @@ int convert_root(const NodeView& v, uint64_t n_slots, std::vector<WideNode>& wid
       maps.leaves.push_back(static_cast<uint32_t>(g));
       ref = inline_leaf_ref(v.leaf_offset(it.node), v.leaf_count(it.node), bigleaf);
     } else {
+      if (maps.internal_by_depth.size() <= it.depth) maps.internal_by_depth.resize(it.depth + 1);
+      maps.internal_by_depth[it.depth].push_back(static_cast<uint32_t>(g));
       const uint64_t left = it.node + 1, right = v.right_child(it.node);
       const uint32_t axis = v.split_axis(it.node);
       if (axis > 2 || left >= v.n_nodes || right >= v.n_nodes || right <= it.node)
@@ -218,8 +218,8 @@ int convert_root(const NodeView& v, uint64_t n_slots, std::vector<WideNode>& wid
       wn.meta = make_uint4(0, 0, axis, 0);
       wide.push_back(wn);
       ref = w;
-      todo.push_back({right, it.depth + 1, static_cast<int64_t>(w), 1, static_cast<int64_t>(it.node)});
-      todo.push_back({left, it.depth + 1, static_cast<int64_t>(w), 0, static_cast<int64_t>(it.node)});
+      todo.push_back({right, it.depth + 1, static_cast<int64_t>(w), 1});
+      todo.push_back({left, it.depth + 1, static_cast<int64_t>(w), 0});
     }
     if (it.parent_wide < 0) {
       *root_ref = ref;
@@ -330,13 +330,13 @@ void free_mesh_buffers(mbvh_mesh* m) {
   if (m->d_face_slot) (void)hipFree(m->d_face_slot);
   if (m->d_nodes4) (void)hipFree(m->d_nodes4);
   if (m->d_roots4) (void)hipFree(m->d_roots4);
-  for (void* p : {static_cast<void*>(m->d_ref_nodes), static_cast<void*>(m->d_ref_parent),
+  for (void* p : {static_cast<void*>(m->d_ref_nodes), static_cast<void*>(m->d_ref_levels),
                   static_cast<void*>(m->d_ref_wide), static_cast<void*>(m->d_ref_leaves),
                   static_cast<void*>(m->d_ref_scratch), static_cast<void*>(m->d_four_map),
                   static_cast<void*>(m->d_coord_max)})
     if (p) (void)hipFree(p);
   m->d_ref_nodes = nullptr;
-  m->d_ref_parent = nullptr;
+  m->d_ref_levels = nullptr;
   m->d_ref_wide = nullptr;
   m->d_ref_leaves = nullptr;
   m->d_ref_scratch = nullptr;
@@ -644,10 +644,16 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
                                    root_bytes[r], hipMemcpyHostToDevice);
     if (e != hipSuccess) rc = set_error(MBVH_ERR_HIP, "mbvh_mesh_upload: copy failed (%s)", hipGetErrorString(e));
   }
-  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_ref_parent), maps.parent.data(), maps.parent.size() * sizeof(int32_t));
+  std::vector<uint32_t> levels;
+  m->level_off.push_back(0);
+  for (const auto& lv : maps.internal_by_depth) {
+    levels.insert(levels.end(), lv.begin(), lv.end());
+    m->level_off.push_back(levels.size());
+  }
+  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_ref_levels), levels.data(), levels.size() * sizeof(uint32_t));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_ref_wide), maps.wide_off.data(), maps.wide_off.size() * sizeof(int32_t));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_ref_leaves), maps.leaves.data(), maps.leaves.size() * sizeof(uint32_t));
-  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_ref_scratch), nullptr, 2 * maps.base * sizeof(uint32_t));
+  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_ref_scratch), nullptr, maps.base * sizeof(uint32_t));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_four_map), maps.four_map.data(), maps.four_map.size() * sizeof(uint2));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_coord_max), &coord_max, sizeof(float));
   if (!rc && indirect) {
@@ -736,13 +742,14 @@ int enqueue_refit(mbvh_mesh* mesh, const float* d_positions, uint64_t vertex_cou
                   hipStream_t s) {
   RefitArgs a{};
   a.ref_nodes = mesh->d_ref_nodes;
-  a.parent = mesh->d_ref_parent;
   a.wide_off = mesh->d_ref_wide;
   a.leaves = mesh->d_ref_leaves;
   a.n_nodes = mesh->n_ref_nodes;
   a.n_leaves = mesh->n_ref_leaves;
-  a.counters = mesh->d_ref_scratch;
-  a.changed = mesh->d_ref_scratch + mesh->n_ref_nodes;
+  a.changed = mesh->d_ref_scratch;
+  a.levels = mesh->d_ref_levels;
+  a.level_off = mesh->level_off.data();
+  a.n_levels = static_cast<uint32_t>(mesh->level_off.size() - 1);
   a.wide = reinterpret_cast<float*>(mesh->d_nodes);
   a.four = reinterpret_cast<float*>(mesh->d_nodes4);
   a.four_map = mesh->d_four_map;

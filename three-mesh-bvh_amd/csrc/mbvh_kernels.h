@@ -80,13 +80,14 @@ struct ClosestPointArgs {
 // ---- refit ------------------------------------------------------------------------------------
 struct RefitArgs {
   uint32_t* ref_nodes;        // the reference-format nodes of all roots, 8 words each
-  const int32_t* parent;      // parent node index (-1 for a root)
   const int32_t* wide_off;    // float offset of the node's box in the wide nodes (-1: none)
   const uint32_t* leaves;     // leaf node indices
   uint64_t n_nodes;
   uint64_t n_leaves;
-  uint32_t* counters;         // per node: children arrived
   uint32_t* changed;          // per node: box recomputed
+  const uint32_t* levels;     // internal nodes grouped by depth
+  const uint64_t* level_off;  // host: levels[level_off[d] .. level_off[d + 1]) are at depth d
+  uint32_t n_levels;
   float* wide;                // wide nodes as floats (16 per node)
   float* four;                // 4-wide nodes as floats
   const uint2* four_map;      // (float offset in the 4-wide tree, float offset in the wide nodes)

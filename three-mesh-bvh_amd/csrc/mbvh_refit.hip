@@ -8,14 +8,13 @@
 // left : right, max: left > right ? left : right) exactly when a child changed, and then counts as
 // changed itself.
 //
-// Device form: one thread per leaf walks up through parent links. Each node has an arrival counter
-// (cleared by k_refit_clear, which also resets the coordinate bound that k_refit_bound recomputes);
-// the second child to arrive computes the parent (release fence before
-// the counter increment, acquire fence after it), so every internal node is decided once, after
-// both children. A node's box is written to the device copy of the reference nodes and, where the
-// node has one, to its place in the parent's wide node; k_refit_four then copies the changed wide
-// boxes into the 4-wide tree, and k_refit_tris re-gathers the triangle corners (their .w words -
-// face id, facing mask, vertex id - are kept).
+// Device form, level by level: one launch decides every leaf, then one launch per depth, deepest
+// first, decides the internal nodes of that depth from their children (decided by an earlier
+// launch on the same stream, so no fences or counters are needed). A node's box is written to the
+// device copy of the reference nodes and, where the node has one, to its place in the parent's wide
+// node; k_refit_four then copies the wide boxes into the 4-wide tree, and k_refit_tris re-gathers
+// the triangle corners (their .w words - face id, facing mask, vertex id - are kept). The
+// coordinate bound the query kernels scale their error bounds by is recomputed from the positions.
 #include "mbvh_kernels.h"
 
 namespace mbvh {
@@ -23,14 +22,8 @@ namespace {
 
 constexpr int kRefitBlock = 256;
 
-__global__ __launch_bounds__(kRefitBlock) void k_refit_clear(uint32_t* counters, uint32_t* changed, uint64_t n,
-                                                              float* coord_bound) {
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kRefitBlock;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *coord_bound = 0.0f;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kRefitBlock + threadIdx.x; i < n; i += stride) {
-    counters[i] = 0u;
-    changed[i] = 0u;
-  }
+__global__ __launch_bounds__(64) void k_refit_reset_bound(float* coord_bound) {
+  if (threadIdx.x == 0) *coord_bound = 0.0f;
 }
 
 // The mesh's coordinate bound: max |coordinate| over the vertices (NaN ignored; an infinite one
@@ -76,14 +69,14 @@ __device__ __forceinline__ void put_box(const RefitArgs& a, uint64_t i, const fl
   }
 }
 
-__global__ __launch_bounds__(kRefitBlock) void k_refit_up(RefitArgs a) {
+__global__ __launch_bounds__(kRefitBlock) void k_refit_leaves(RefitArgs a) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kRefitBlock;
+  const float inf = __builtin_huge_valf();
   for (uint64_t li = static_cast<uint64_t>(blockIdx.x) * kRefitBlock + threadIdx.x; li < a.n_leaves; li += stride) {
-    uint64_t node = a.leaves[li];
+    const uint64_t node = a.leaves[li];
     const uint32_t* w = a.ref_nodes + 8 * node;
     const uint32_t off = w[6];
     const uint32_t cnt = w[7] & 0xFFFFu;  // u16[14]: count (u16[15] is the leaf flag)
-    const float inf = __builtin_huge_valf();
     float b[6] = {inf, inf, inf, -inf, -inf, -inf};
     for (uint64_t c = 3ull * off; c < 3ull * (static_cast<uint64_t>(off) + cnt); c++) {
       const float* p = a.positions + static_cast<uint64_t>(a.vidx[c]) * a.stride + a.offset;
@@ -97,29 +90,28 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_up(RefitArgs a) {
     for (int k = 0; k < 6; k++) changed = changed || !(old[k] == b[k]);
     if (changed) put_box(a, node, b);
     a.changed[node] = changed ? 1u : 0u;
-    // walk up: the second child to arrive at a parent decides it
-    while (true) {
-      const int32_t parent = a.parent[node];
-      if (parent < 0) break;
-      __threadfence();  // this subtree's boxes and flags before the arrival
-      if (atomicAdd(&a.counters[parent], 1u) == 0u) break;  // the sibling is not done yet
-      __threadfence();  // the sibling's boxes and flags after its arrival
-      const uint64_t left = static_cast<uint64_t>(parent) + 1;
-      const uint64_t right = static_cast<uint64_t>(parent) + a.ref_nodes[8 * parent + 6];
-      const bool any = a.changed[left] != 0u || a.changed[right] != 0u;
-      if (any) {
-        const float* lb = reinterpret_cast<const float*>(a.ref_nodes + 8 * left);
-        const float* rb = reinterpret_cast<const float*>(a.ref_nodes + 8 * right);
-        float u[6];
-        for (int k = 0; k < 3; k++) {
-          u[k] = lb[k] < rb[k] ? lb[k] : rb[k];
-          u[3 + k] = lb[3 + k] > rb[3 + k] ? lb[3 + k] : rb[3 + k];
-        }
-        put_box(a, static_cast<uint64_t>(parent), u);
+  }
+}
+
+// The internal nodes of one depth, whose children were decided by earlier launches.
+__global__ __launch_bounds__(kRefitBlock) void k_refit_level(RefitArgs a, const uint32_t* nodes, uint64_t count) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kRefitBlock;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kRefitBlock + threadIdx.x; i < count; i += stride) {
+    const uint64_t node = nodes[i];
+    const uint64_t left = node + 1;
+    const uint64_t right = node + a.ref_nodes[8 * node + 6];
+    const bool any = a.changed[left] != 0u || a.changed[right] != 0u;
+    if (any) {
+      const float* lb = reinterpret_cast<const float*>(a.ref_nodes + 8 * left);
+      const float* rb = reinterpret_cast<const float*>(a.ref_nodes + 8 * right);
+      float u[6];
+      for (int k = 0; k < 3; k++) {
+        u[k] = lb[k] < rb[k] ? lb[k] : rb[k];
+        u[3 + k] = lb[3 + k] > rb[3 + k] ? lb[3 + k] : rb[3 + k];
       }
-      a.changed[parent] = any ? 1u : 0u;
-      node = static_cast<uint64_t>(parent);
+      put_box(a, node, u);
     }
+    a.changed[node] = any ? 1u : 0u;
   }
 }
 
@@ -140,12 +132,17 @@ uint32_t blocks_for(uint64_t n) {
 }  // namespace
 
 hipError_t launch_refit(const RefitArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_refit_clear, dim3(blocks_for(a.n_nodes)), dim3(kRefitBlock), 0, s, a.counters, a.changed,
-                     a.n_nodes, a.coord_bound);
+  hipLaunchKernelGGL(k_refit_reset_bound, dim3(1), dim3(64), 0, s, a.coord_bound);
   hipLaunchKernelGGL(k_refit_bound, dim3(blocks_for(a.n_vertices) < 1024 ? blocks_for(a.n_vertices) : 1024),
                      dim3(kRefitBlock), 0, s, a);
   if (a.n_slots) hipLaunchKernelGGL(k_refit_tris, dim3(blocks_for(3 * a.n_slots)), dim3(kRefitBlock), 0, s, a);
-  if (a.n_leaves) hipLaunchKernelGGL(k_refit_up, dim3(blocks_for(a.n_leaves)), dim3(kRefitBlock), 0, s, a);
+  if (a.n_leaves) hipLaunchKernelGGL(k_refit_leaves, dim3(blocks_for(a.n_leaves)), dim3(kRefitBlock), 0, s, a);
+  for (uint32_t d = a.n_levels; d-- > 0;) {
+    const uint64_t count = a.level_off[d + 1] - a.level_off[d];
+    if (count)
+      hipLaunchKernelGGL(k_refit_level, dim3(blocks_for(count)), dim3(kRefitBlock), 0, s, a, a.levels + a.level_off[d],
+                         count);
+  }
   if (a.n_four) hipLaunchKernelGGL(k_refit_four, dim3(blocks_for(a.n_four)), dim3(kRefitBlock), 0, s, a);
   return hipGetLastError();
 }
