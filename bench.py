@@ -472,7 +472,13 @@ def main():
     for _ in range(10):
         mesh1.raycast_first(w1.origins, w1.directions)
     host_rc = nr / ((time.perf_counter() - t0) / 10) / 1e6
+    radii = np.full(n, 0.5, np.float32)  # about half of the grid points reach the surface
+    mesh.intersects_sphere(w.points, radii)
+    t0 = time.perf_counter()
+    hit_frac = float(mesh.intersects_sphere(w.points, radii).mean())
+    host_sp = n / (time.perf_counter() - t0) / 1e6
     host_api = {"closest_point_mqueries_per_s": round(host_cp, 2), "raycast_first_mrays_per_s": round(host_rc, 2),
+                "intersects_sphere_mspheres_per_s": round(host_sp, 2), "intersects_sphere_hit_fraction": round(hit_frac, 3),
                 "note": "host-pointer entry points (the JS path): pageable host arrays in and out, copies over "
                         "PCIe and the query ordering included"}
 

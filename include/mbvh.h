@@ -198,6 +198,15 @@ int mbvh_raycast_first_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* orig
                          float* barycoord, void* stream);
 
 /*
+ * intersectsSphere (MeshBVH.intersectsSphere src/core/MeshBVH.js:800-809 with
+ * sphereIntersectTriangle src/math/MathUtilities.js:160-204) for n spheres: out[i] = 1 when some
+ * triangle lies within radii[i] of centers[i] (inclusive), else 0. Decided from the closest
+ * triangle's squared distance in double-float against radius^2. Synchronous.
+ */
+int mbvh_intersects_sphere(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* centers, const float* radii,
+                           uint64_t n, uint8_t* out);
+
+/*
  * All hits per ray. Replaces MeshBVH.raycast (src/core/MeshBVH.js:532-545) → raycast
  * (src/core/cast/raycast.template.js:7-53) → intersectTris (src/core/utils/iterationUtils.template.js:4-22).
  * The hits of ray i are entries offsets[i] .. offsets[i+1]-1, in the reference's traversal order

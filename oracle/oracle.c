@@ -784,3 +784,57 @@ static int refit_node(const orc_mesh* m, uint8_t* nodes, uint64_t n_nodes, uint6
 void orc_refit_root(const orc_mesh* m, uint8_t* nodes, uint64_t n_bytes) {
   if (n_bytes >= 32) refit_node(m, nodes, n_bytes / 32, 0, 0);
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* intersectsSphere: brute force over the triangles with sphereIntersectTriangle              */
+/* (src/math/MathUtilities.js:160-204)                                                        */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Line3.closestPointToPoint(point, clampToLine = true) */
+static void segment_closest(const double s[3], const double e[3], const double p[3], double out[3]) {
+  double se[3], sp[3];
+  v_sub(e, s, se);
+  v_sub(p, s, sp);
+  double t = v_dot(se, sp) / v_dot(se, se);
+  t = fmax(0.0, fmin(1.0, t)); /* MathUtils.clamp: Math.max(min, Math.min(max, t)) */
+  if (t != t) {
+    out[0] = out[1] = out[2] = NAN; /* degenerate segment: 0 / 0 */
+    return;
+  }
+  for (int k = 0; k < 3; k++) out[k] = s[k] + se[k] * t;
+}
+
+static int sphere_triangle(const double c[3], double r, const double a[3], const double b[3], const double cc[3]) {
+  double q[3];
+  const double* ends[3][2] = {{a, b}, {a, cc}, {b, cc}};
+  for (int k = 0; k < 3; k++) {
+    segment_closest(ends[k][0], ends[k][1], c, q);
+    if (sqrt(v_dist_sq(q, c)) <= r) return 1;
+  }
+  /* Triangle.getPlane: setFromCoplanarPoints(a, b, c), normal = normalize((c - b) x (a - b)) */
+  double cb[3], ab[3], n[3];
+  v_sub(cc, b, cb);
+  v_sub(a, b, ab);
+  v_cross(cb, ab, n);
+  const double len = sqrt(v_dot(n, n));
+  for (int k = 0; k < 3; k++) n[k] /= len;
+  const double constant = -v_dot(a, n);
+  const double dist = v_dot(n, c) + constant;
+  if (fabs(dist) <= r) {
+    double pp[3], bc[3];
+    for (int k = 0; k < 3; k++) pp[k] = c[k] - n[k] * dist; /* projectPoint */
+    if (orc_barycoord(pp, a, b, cc, bc) && bc[0] >= 0 && bc[1] >= 0 && bc[0] + bc[1] <= 1) return 1;
+  }
+  return 0;
+}
+
+int orc_intersects_sphere(const orc_mesh* m, const float center[3], double radius) {
+  const uint64_t n_tri = m->index ? m->index_count / 3 : m->vertex_count / 3;
+  const double c[3] = {center[0], center[1], center[2]};
+  for (uint64_t f = 0; f < n_tri; f++) {
+    double a[3], b[3], cc[3];
+    face_vertices(m, (uint32_t)f, a, b, cc);
+    if (sphere_triangle(c, radius, a, b, cc)) return 1;
+  }
+  return 0;
+}

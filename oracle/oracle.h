@@ -99,6 +99,11 @@ int orc_validate_root(const orc_mesh* m, uint32_t root, orc_tree_stats* stats);
  * and indirect buffer. */
 void orc_refit_root(const orc_mesh* m, uint8_t* nodes, uint64_t n_bytes);
 
+/* MeshBVH.intersectsSphere by brute force over the geometry's triangles with the reference's
+ * sphereIntersectTriangle (src/math/MathUtilities.js:160-204): segment closest points, then the
+ * plane distance and projected-point containment. */
+int orc_intersects_sphere(const orc_mesh* m, const float center[3], double radius);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,8 @@ def library():
         lib.orc_ray_plane_distance.restype = f64
         lib.orc_point_face_distance.argtypes = [vp, u32, vp]
         lib.orc_point_face_distance.restype = f64
+        lib.orc_intersects_sphere.argtypes = [vp, vp, f64]
+        lib.orc_intersects_sphere.restype = C.c_int
         lib.orc_refit_root.argtypes = [vp, vp, C.c_uint64]
         lib.orc_refit_root.restype = None
         lib.orc_validate_root.argtypes = [vp, u32, C.POINTER(TreeStats)]
@@ -175,6 +177,12 @@ class OracleMesh:
     def point_face_distance(self, face, p):
         q = np.ascontiguousarray(p, dtype=np.float32).reshape(3)
         return library().orc_point_face_distance(self.ref, int(face), _p(q))
+
+    def intersects_sphere(self, centers, radii):
+        """intersectsSphere per sphere (brute force, the reference's sphere-triangle test)."""
+        c = np.ascontiguousarray(centers, dtype=np.float32).reshape(-1, 3)
+        r = np.asarray(radii, dtype=np.float64).reshape(-1)
+        return np.array([bool(library().orc_intersects_sphere(self.ref, _p(c[i]), float(r[i]))) for i in range(len(c))])
 
     def refit(self):
         """The roots after MeshBVH.refit over this mesh's positions (new arrays; self.roots unchanged)."""

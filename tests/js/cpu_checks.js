@@ -33,7 +33,7 @@ check('exports the query API', () => {
     'computeBatchedBoundsTree', 'disposeBatchedBoundsTree', 'getTriangleHitPointInfo', 'SAH', 'CENTER', 'AVERAGE']) {
     assert.ok(api[k] !== undefined, k);
   }
-  for (const m of ['raycast', 'raycastFirst', 'closestPointToPoint', 'raycastObject3D', 'raycastFirstBatch', 'refit',
+  for (const m of ['raycast', 'raycastFirst', 'closestPointToPoint', 'raycastObject3D', 'raycastFirstBatch', 'refit', 'intersectsSphere', 'intersectsSphereBatch',
     'raycastBatch', 'closestPointToPointBatch', 'getBoundingBox']) {
     assert.strictEqual(typeof api.MeshBVH.prototype[m], 'function', m);
   }

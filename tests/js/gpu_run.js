@@ -170,6 +170,20 @@ assert.ok(batchedChecked > 64 && presenceDiffers <= 2, `batched BatchedMesh hits
 api.disposeBatchedBoundsTree.call(Object.assign(batched, { boundsTrees: [bvh] }), -1);
 assert.strictEqual(batched._mbvhBatch, null);
 
+// --- intersectsSphere: batch results go back to the oracle; the per-call form agrees -------------
+const sc = new Float32Array(3 * nPts), sr = new Float32Array(nPts);
+for (let i = 0; i < nPts; i++) {
+  sc.set(points.subarray(3 * i, 3 * i + 3), 3 * i);
+  sr[i] = cp.distance[i] * (i % 2 ? 1.0001 : 0.9999);  // just outside / just inside the closest distance
+}
+const si = bvh.intersectsSphereBatch(sc, sr);
+writeArr('sphere_radii.bin', sr);
+writeArr('sphere_hit.bin', si);
+for (let i = 0; i < nPts; i += 97) {
+  const one = bvh.intersectsSphere({ center: { x: sc[3 * i], y: sc[3 * i + 1], z: sc[3 * i + 2] }, radius: sr[i] });
+  assert.strictEqual(one, si[i] === 1, `sphere ${i}`);
+}
+
 // --- refit after a deformation: nodes and queries go back to the oracle ------------------------
 for (let v = 0; v < positions.length / 3; v++) {
   const x = positions[3 * v], y = positions[3 * v + 1], z = positions[3 * v + 2];

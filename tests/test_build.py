@@ -139,3 +139,28 @@ def test_build_errors():
     bad[0] = 60000
     with pytest.raises(M.MbvhError):
         M.build(g.positions, bad)
+
+
+def test_validate_bounds_catches_planted_violations():
+    """validateBounds (the oracle's orc_validate_root) reports a leaf box shrunk inside its triangles,
+    an internal box shrunk inside a child, and a slot left out of every leaf - each planted in a
+    copy of a valid tree."""
+    g = G.torus_knot(10, 3, 80, 12)
+    g.ensure_index()
+    idx = g.index.copy()
+    tree = M.build(g.positions, idx)
+    assert O.OracleMesh(g.positions, idx, tree.roots).validate_root(0)[0] == 0
+    f0 = tree.roots[0].view(np.float32).reshape(-1, 8)
+    h0 = tree.roots[0].view(np.uint16).reshape(-1, 16)
+    leaf = int(np.flatnonzero(h0[:, 15] == 0xFFFF)[3])
+    inner = int(np.flatnonzero(h0[:, 15] != 0xFFFF)[2])
+    for node, k, delta in ((leaf, 0, 0.5), (leaf, 4, -0.5), (inner, 2, 0.5)):
+        buf = tree.roots[0].copy()
+        buf.view(np.float32).reshape(-1, 8)[node, k] += np.float32(delta)
+        rc, st = O.OracleMesh(g.positions, idx, [buf]).validate_root(0)
+        assert rc != 0 and st["bounds_failures"] > 0, (node, k)
+    buf = tree.roots[0].copy()
+    h = buf.view(np.uint16).reshape(-1, 16)
+    h[leaf, 14] -= 1  # the leaf's last slot is no longer covered
+    rc, st = O.OracleMesh(g.positions, idx, [buf]).validate_root(0)
+    assert rc != 0 and st["structure_failures"] > 0

@@ -89,6 +89,13 @@ def test_js_api_matches_oracle(tmp_path):
     mesh.free()
     ctx.close()
 
+    # intersectsSphere batch == the oracle's brute force with the reference's sphere-triangle test
+    sr = np.fromfile(d / "sphere_radii.bin", np.float32)
+    si = np.fromfile(d / "sphere_hit.bin", np.uint8).astype(bool)
+    sub = np.arange(0, len(sr), 20)  # brute force over 80k triangles: a sample of the spheres
+    assert np.array_equal(si[sub], om.intersects_sphere(pts[sub], sr[sub]))
+    assert 0.3 < si.mean() < 0.7
+
     # MeshBVH.refit after a deformation: the refit nodes equal the oracle's refit, and raycasts on
     # the refit tree match the oracle on the deformed geometry
     pos2 = np.fromfile(d / "refit_positions.bin", np.float32)

@@ -1021,6 +1021,54 @@ extern "C" int mbvh_closest_point(mbvh_ctx* ctx, const mbvh_mesh* mesh, const fl
   return check_no_overflow(face_index, n, "mbvh_closest_point");
 }
 
+extern "C" int mbvh_intersects_sphere(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* centers, const float* radii,
+                                     uint64_t n, uint8_t* out) {
+  clear_error();
+  const char* fn = "mbvh_intersects_sphere";
+  int rc = check_query(fn, ctx, mesh);
+  if (rc) return rc;
+  if (n == 0) return MBVH_OK;
+  if (!centers || !radii || !out) return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL centre, radius or output buffer", fn);
+  // the closest-point search runs with the largest radius (slightly enlarged: its pruning compares
+  // rounded squares) as maxThreshold; the exact decision is per sphere
+  float r_max = -1.0f;
+  for (uint64_t i = 0; i < n; i++)
+    if (radii[i] > r_max) r_max = radii[i];
+  if (!(r_max >= 0.0f)) {
+    std::memset(out, 0, n);
+    return MBVH_OK;
+  }
+  const float max_t = r_max * (1.0f + 1.0e-6f) + std::numeric_limits<float>::min();
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  const size_t v3 = n * 3 * sizeof(float);
+  if ((rc = ensure_buffer(ctx->staging[0], v3)) || (rc = ensure_buffer(ctx->staging[1], n * sizeof(float))) ||
+      (rc = ensure_buffer(ctx->staging[2], n * sizeof(uint32_t))) || (rc = ensure_buffer(ctx->staging[7], n)))
+    return rc;
+  hipStream_t s = ctx->stream;
+  MBVH_HIP(hipMemcpyAsync(ctx->staging[0].ptr, centers, v3, hipMemcpyHostToDevice, s));
+  MBVH_HIP(hipMemcpyAsync(ctx->staging[1].ptr, radii, n * sizeof(float), hipMemcpyHostToDevice, s));
+  const uint32_t* order = nullptr;
+  if ((rc = host_order(ctx, static_cast<float*>(ctx->staging[0].ptr), n, &order))) return rc;
+  if ((rc = closest_point_impl(ctx, mesh, static_cast<float*>(ctx->staging[0].ptr), n, order, 0.0f, max_t,
+                               static_cast<uint32_t*>(ctx->staging[2].ptr), nullptr, nullptr, nullptr, s)))
+    return rc;
+  SphereArgs a;
+  a.mesh = grad_view_of(mesh);
+  a.centers = static_cast<const float*>(ctx->staging[0].ptr);
+  a.radii = static_cast<const float*>(ctx->staging[1].ptr);
+  a.face = static_cast<const uint32_t*>(ctx->staging[2].ptr);
+  a.n = n;
+  a.out = static_cast<uint8_t*>(ctx->staging[7].ptr);
+  MBVH_HIP(launch_sphere_decide(a, s));
+  MBVH_HIP(hipMemcpyAsync(out, ctx->staging[7].ptr, n, hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipStreamSynchronize(s));
+  for (uint64_t i = 0; i < n; i++)
+    if (out[i] == 2)
+      return set_error(MBVH_ERR_LIMIT, "%s: traversal stack overflow on sphere %llu", fn, static_cast<unsigned long long>(i));
+  return MBVH_OK;
+}
+
 // ---------------------------------------------------------------------------------------------
 // All hits
 // ---------------------------------------------------------------------------------------------

@@ -183,6 +183,17 @@ struct GradMesh {
   uint64_t face_limit;        // faces are < face_limit
 };
 
+// ---- sphere intersection --------------------------------------------------------------------------
+struct SphereArgs {
+  GradMesh mesh;            // triangles and the face -> slot map
+  const float* centers;     // 3 per sphere
+  const float* radii;
+  const uint32_t* face;     // closest face within the largest radius (MBVH_NO_HIT: none)
+  uint64_t n;
+  uint8_t* out;             // 1 intersects, 0 not, 2 the closest-point search overflowed
+};
+hipError_t launch_sphere_decide(const SphereArgs& a, hipStream_t s);
+
 struct VertexSink {
   float* grad;        // atomic mode: vertex gradients, 3 floats per vertex
   uint32_t* keys;     // deterministic mode: vertex id of each contribution (3 per query)

@@ -1632,6 +1632,39 @@ hipError_t launch_closest_point4(const ClosestPointArgs& a, uint64_t threads, hi
   return hipGetLastError();
 }
 
+// intersectsSphere (src/core/MeshBVH.js:800-809, sphereIntersectTriangle src/math/MathUtilities.js:160-204):
+// any triangle within `radius` of the centre, the comparison inclusive. The closest triangle (found
+// within the largest radius by the closest-point kernels) decides it: its squared distance in
+// double-float against radius^2 (exact as two_prod of the float radius).
+__global__ __launch_bounds__(kBlock) void k_sphere_decide(SphereArgs a) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < a.n; i += stride) {
+    const uint32_t f = a.face[i];
+    const float r = a.radii[i];
+    uint8_t res = 0;
+    if (f == kOverflow) {
+      res = 2;
+    } else if (f != kNoHit && r >= 0.0f && f < a.mesh.face_limit) {
+      const uint32_t slot = a.mesh.face_slot ? a.mesh.face_slot[f] : f;
+      if (slot != kNoHit) {
+        const Tri tr = load_tri(a.mesh.tris, slot);
+        const f3 c = mk(a.centers[3 * i + 0], a.centers[3 * i + 1], a.centers[3 * i + 2]);
+        const ClosestFF cf = closest_on_triangle_ff(c, tr.a(), tr.b(), tr.c());
+        res = (cf.dist2.hi == cf.dist2.hi) && ff_sign(ff_sub(cf.dist2, two_prod(r, r))) <= 0 ? 1 : 0;
+      }
+    }
+    a.out[i] = res;
+  }
+}
+
+hipError_t launch_sphere_decide(const SphereArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  const uint64_t blocks = (a.n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_sphere_decide, dim3(static_cast<uint32_t>(blocks < 16384 ? blocks : 16384)), dim3(kBlock), 0, s,
+                     a);
+  return hipGetLastError();
+}
+
 hipError_t launch_closest_point(const ClosestPointArgs& a, uint64_t threads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.queue);

@@ -708,6 +708,27 @@ static napi_value js_abi_version(napi_env env, napi_callback_info info) {
     napi_set_named_property(env, exports, name, f);                             \
   } while (0)
 
+/* intersectsSphere(mesh, centers Float32Array(3n), radii Float32Array(n), out Uint8Array(n)) */
+static napi_value js_intersects_sphere(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  get_args(env, info, 4, argv);
+  MeshHandle* h = get_mesh(env, argv[0]);
+  if (!h) return throw_msg(env, "intersectsSphere: invalid mesh");
+  View c, r;
+  if (!get_view(env, argv[1], &c) || c.type != napi_float32_array || c.length % 3 || !get_view(env, argv[2], &r) ||
+      r.type != napi_float32_array || r.length != c.length / 3)
+    return throw_msg(env, "intersectsSphere: centers must be a Float32Array of length 3n, radii one of length n");
+  const size_t n = r.length;
+  void* out;
+  if (!out_view(env, argv[3], napi_uint8_array, n, &out) || !out)
+    return throw_msg(env, "intersectsSphere: out must be a Uint8Array of length n");
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, h->device, &ctx);
+  if (!rc) rc = mbvh_intersects_sphere(ctx, h->mesh, (const float*)c.data, (const float*)r.data, n, (uint8_t*)out);
+  if (rc) return throw_status(env, rc, "mbvh_intersects_sphere");
+  return NULL;
+}
+
 /* refitMesh(mesh, positions, stride, offset, roots[]): device refit from new positions; the refit
  * reference-format nodes are written into the roots' ArrayBuffers */
 static napi_value js_refit(napi_env env, napi_callback_info info) {
@@ -775,6 +796,7 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("batchRaycastFirst", js_batch_raycast_first);
   EXPORT("closestPointBackward", js_closest_point_bwd);
   EXPORT("refitMesh", js_refit);
+  EXPORT("intersectsSphere", js_intersects_sphere);
   napi_add_env_cleanup_hook(env, cleanup_contexts, NULL);
   return exports;
 }

@@ -31,7 +31,7 @@ EXPORTED_SYMBOLS = (
     "mbvh_ctx_create", "mbvh_ctx_destroy", "mbvh_ctx_synchronize", "mbvh_ctx_set_ray_lanes",
     "mbvh_ctx_set_point_lanes",
     "mbvh_mesh_upload", "mbvh_mesh_free", "mbvh_mesh_info", "mbvh_mesh_set_side_mask", "mbvh_mesh_refit",
-    "mbvh_mesh_refit_d", "mbvh_mesh_copy_roots",
+    "mbvh_mesh_refit_d", "mbvh_mesh_copy_roots", "mbvh_intersects_sphere",
     "mbvh_raycast_first", "mbvh_raycast_first_d",
     "mbvh_closest_point", "mbvh_closest_point_d",
     "mbvh_raycast", "mbvh_raycast_fetch", "mbvh_raycast_count_d", "mbvh_raycast_fill_d",
@@ -100,6 +100,7 @@ def library() -> C.CDLL:
         "mbvh_mesh_set_side_mask": (C.c_int, [vp, vp, vp, u64]),
         "mbvh_mesh_refit": (C.c_int, [vp, vp, vp, u64, u32, u32, vp, vp, u32]),
         "mbvh_mesh_refit_d": (C.c_int, [vp, vp, vp, u64, u32, u32, vp]),
+        "mbvh_intersects_sphere": (C.c_int, [vp, vp, vp, vp, u64, vp]),
         "mbvh_mesh_copy_roots": (C.c_int, [vp, vp, vp, vp, u32]),
         "mbvh_raycast_first": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp]),
         "mbvh_raycast_first_d": (C.c_int, [vp, vp, vp, vp, u64, vp, i32, f32, f32, vp, vp, vp, vp, vp]),
@@ -384,6 +385,17 @@ class Mesh:
         _check(self.ctx._lib.mbvh_closest_point(self.ctx.handle, self.handle, _ptr(q), n, min_threshold,
                                                 max_threshold, _ptr(face), _ptr(dist), _ptr(point), _ptr(bary)))
         return {"face": face, "distance": dist, "point": point, "barycoord": bary}
+
+    def intersects_sphere(self, centers: np.ndarray, radii: np.ndarray) -> np.ndarray:
+        """intersectsSphere for n spheres: a bool per sphere."""
+        c = np.ascontiguousarray(centers, dtype=np.float32).reshape(-1, 3)
+        r = np.ascontiguousarray(radii, dtype=np.float32).reshape(-1)
+        if r.shape[0] != c.shape[0]:
+            raise ValueError("one radius per centre")
+        out = np.empty(c.shape[0], np.uint8)
+        _check(self.ctx._lib.mbvh_intersects_sphere(self.ctx.handle, self.handle, _ptr(c), _ptr(r), c.shape[0],
+                                                    _ptr(out)))
+        return out.astype(bool)
 
     # ---- device-pointer queries (asynchronous on `stream`) ----
     def raycast_first_d(self, origins_ptr: int, dirs_ptr: int, n: int, face_ptr: int, dist_ptr: int = 0,

@@ -276,6 +276,16 @@ class MeshBVH {
     return out;
   }
 
+  // intersectsSphere for n spheres: centers a Float32Array of length 3n, radii one of length n.
+  // Returns a Uint8Array (1: some triangle within the radius, inclusive).
+  intersectsSphereBatch(centers, radii) {
+    const c = centers instanceof Float32Array ? centers : Float32Array.from(centers);
+    const r = radii instanceof Float32Array ? radii : Float32Array.from(radii);
+    const out = new Uint8Array(r.length);
+    native.load().intersectsSphere(this._mesh(), c, r, out);
+    return out;
+  }
+
   // ---- differentiable queries: autograd-style forward / backward pairs ---------------------------
   // (no counterpart in the reference). forward() runs the batch query and returns its outputs
   // plus `saved`, the hit data the backward reuses (it never re-traverses). backward(saved, grads)
@@ -375,6 +385,12 @@ class MeshBVH {
       }
     }
     return hits;
+  }
+
+  // intersectsSphere (src/core/MeshBVH.js:800-809): whether any triangle lies within the sphere.
+  intersectsSphere(sphere) {
+    const c = sphere.center;
+    return this.intersectsSphereBatch(new Float32Array([c.x, c.y, c.z]), new Float32Array([sphere.radius]))[0] === 1;
   }
 
   closestPointToPoint(point, target = {}, minThreshold = 0, maxThreshold = Infinity) {
