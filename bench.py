@@ -383,6 +383,32 @@ def bench_batched(torch, M, W, ctx, stream, args):
     return {"batched_raycast_first": r}
 
 
+def bench_host_api(mesh, mesh1, w, w1, n, nr):
+    """Host-pointer entry points (the JS path): pageable host arrays in, results back, PCIe and the
+    query ordering included."""
+    import numpy as np
+    mesh.closest_point(w.points)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        mesh.closest_point(w.points)
+    host_cp = n / ((time.perf_counter() - t0) / 3) / 1e6
+    mesh1.raycast_first(w1.origins, w1.directions)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        mesh1.raycast_first(w1.origins, w1.directions)
+    host_rc = nr / ((time.perf_counter() - t0) / 10) / 1e6
+    radii = np.full(n, 0.5, np.float32)  # about half of the grid points reach the surface
+    mesh.intersects_sphere(w.points, radii)
+    t0 = time.perf_counter()
+    hit_frac = float(mesh.intersects_sphere(w.points, radii).mean())
+    host_sp = n / (time.perf_counter() - t0) / 1e6
+    host_api = {"closest_point_mqueries_per_s": round(host_cp, 2), "raycast_first_mrays_per_s": round(host_rc, 2),
+                "intersects_sphere_mspheres_per_s": round(host_sp, 2), "intersects_sphere_hit_fraction": round(hit_frac, 3),
+                "note": "host-pointer entry points (the JS path): pageable host arrays in and out, copies over "
+                        "PCIe and the query ordering included"}
+    return host_api
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -461,27 +487,12 @@ def main():
 
     # ---------------- host-pointer entry points (PCIe included) ----------------
     # what the JS API sees: pageable host arrays in, results back in host arrays, ordering included
-    torch.cuda.synchronize()
-    mesh.closest_point(w.points)
-    t0 = time.perf_counter()
-    for _ in range(3):
-        mesh.closest_point(w.points)
-    host_cp = n / ((time.perf_counter() - t0) / 3) / 1e6
-    mesh1.raycast_first(w1.origins, w1.directions)
-    t0 = time.perf_counter()
-    for _ in range(10):
-        mesh1.raycast_first(w1.origins, w1.directions)
-    host_rc = nr / ((time.perf_counter() - t0) / 10) / 1e6
-    radii = np.full(n, 0.5, np.float32)  # about half of the grid points reach the surface
-    mesh.intersects_sphere(w.points, radii)
-    t0 = time.perf_counter()
-    hit_frac = float(mesh.intersects_sphere(w.points, radii).mean())
-    host_sp = n / (time.perf_counter() - t0) / 1e6
-    host_api = {"closest_point_mqueries_per_s": round(host_cp, 2), "raycast_first_mrays_per_s": round(host_rc, 2),
-                "intersects_sphere_mspheres_per_s": round(host_sp, 2), "intersects_sphere_hit_fraction": round(hit_frac, 3),
-                "note": "host-pointer entry points (the JS path): pageable host arrays in and out, copies over "
-                        "PCIe and the query ordering included"}
+    # (skipped with --only-primary, whose profiles then hold the primary launches alone)
+    host_api = None
+    if not args.only_primary:
+        host_api = bench_host_api(mesh, mesh1, w, w1, n, nr)
 
+    # ---------------- oracle counts + CPU baseline ----------------
     # ---------------- oracle counts + CPU baseline ----------------
     cpu = None
     roof = None

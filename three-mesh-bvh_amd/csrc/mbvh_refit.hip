@@ -27,9 +27,13 @@ __global__ __launch_bounds__(64) void k_refit_reset_bound(float* coord_bound) {
 }
 
 // The mesh's coordinate bound: max |coordinate| over the vertices (NaN ignored; an infinite one
-// makes the bound infinite), through one atomic max per wave on the float bits (monotonic for
-// non-negative floats).
+// makes the bound infinite): a wave and then a workgroup reduction, and one atomic max per
+// workgroup on the float bits (monotonic for non-negative floats; atomics on one address serialise,
+// so the grid is kept small).
+constexpr uint32_t kBoundBlocks = 256;
+
 __global__ __launch_bounds__(kRefitBlock) void k_refit_bound(RefitArgs a) {
+  __shared__ float wave_max[kRefitBlock / 64];
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kRefitBlock;
   float m = 0.0f;
   for (uint64_t v = static_cast<uint64_t>(blockIdx.x) * kRefitBlock + threadIdx.x; v < a.n_vertices; v += stride) {
@@ -43,7 +47,13 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_bound(RefitArgs a) {
     const float o = __shfl_xor(m, off, 64);
     if (o > m) m = o;
   }
-  if ((threadIdx.x & 63u) == 0u) atomicMax(reinterpret_cast<unsigned int*>(a.coord_bound), __float_as_uint(m));
+  if ((threadIdx.x & 63u) == 0u) wave_max[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kRefitBlock / 64; w++)
+      if (wave_max[w] > m) m = wave_max[w];
+    atomicMax(reinterpret_cast<unsigned int*>(a.coord_bound), __float_as_uint(m));
+  }
 }
 
 __global__ __launch_bounds__(kRefitBlock) void k_refit_tris(RefitArgs a) {
@@ -133,7 +143,7 @@ uint32_t blocks_for(uint64_t n) {
 
 hipError_t launch_refit(const RefitArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_refit_reset_bound, dim3(1), dim3(64), 0, s, a.coord_bound);
-  hipLaunchKernelGGL(k_refit_bound, dim3(blocks_for(a.n_vertices) < 1024 ? blocks_for(a.n_vertices) : 1024),
+  hipLaunchKernelGGL(k_refit_bound, dim3(blocks_for(a.n_vertices) < kBoundBlocks ? blocks_for(a.n_vertices) : kBoundBlocks),
                      dim3(kRefitBlock), 0, s, a);
   if (a.n_slots) hipLaunchKernelGGL(k_refit_tris, dim3(blocks_for(3 * a.n_slots)), dim3(kRefitBlock), 0, s, a);
   if (a.n_leaves) hipLaunchKernelGGL(k_refit_leaves, dim3(blocks_for(a.n_leaves)), dim3(kRefitBlock), 0, s, a);
