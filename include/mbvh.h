@@ -123,6 +123,36 @@ int mbvh_ctx_set_point_lanes(mbvh_ctx* ctx, int32_t lanes);
 /* Waits for all work the context enqueued on its own stream. */
 int mbvh_ctx_synchronize(mbvh_ctx* ctx);
 
+/* ------------------------------------------------------------------------------------------ */
+/* Device build                                                                               */
+/* ------------------------------------------------------------------------------------------ */
+
+/*
+ * mbvh_build on the device: the same arguments, the same tree handle and byte-for-byte the same
+ * node buffers, permuted index (in place, host memory) or indirect buffer. Replaces the same
+ * reference functions as mbvh_build (buildPackedTree src/core/build/buildTree.js:123-152 with
+ * computePrimitiveBounds src/core/MeshBVH.js:337-436, getBounds, getOptimalSplit, partition and
+ * populateBuffer). Inputs are copied to the device, the result back; the call returns when done.
+ * Limit: 2 x triangles + roots below 2^32. Errors as mbvh_build; a leaf over 65535 triangles is
+ * reported with the largest such count.
+ */
+int mbvh_build_gpu(mbvh_ctx* ctx, const float* positions, uint64_t vertex_count, uint32_t position_stride,
+                   uint32_t position_offset, void* index, int32_t index_elt_bytes, uint64_t index_count,
+                   const uint32_t* root_ranges, uint32_t n_roots, const mbvh_build_opts* opts,
+                   mbvh_tree** out_tree);
+/*
+ * Device-pointer form: positions and index in device memory (the index is permuted in place on the
+ * device); root_ranges and opts stay on the host. The trees are written one after another to
+ * nodes_out (capacity at least (2 x triangles + roots) x 32 bytes), their node counts to the host
+ * array tree_nodes; an indirect build writes its indirect buffer (2-byte entries when the last
+ * range ends at or below 65536, else 4) to indirect_out. Waits for the build before returning.
+ */
+int mbvh_build_gpu_d(mbvh_ctx* ctx, const float* positions, uint64_t vertex_count, uint32_t position_stride,
+                     uint32_t position_offset, void* index, int32_t index_elt_bytes, uint64_t index_count,
+                     const uint32_t* root_ranges, uint32_t n_roots, const mbvh_build_opts* opts,
+                     void* nodes_out, uint64_t nodes_capacity, uint64_t* tree_nodes, uint32_t n_tree_slots,
+                     void* indirect_out, uint64_t indirect_capacity, void* stream);
+
 /*
  * Uploads one MeshBVH (its serialized form, MeshBVH.serialize src/core/MeshBVH.js:112-145:
  * roots, index, indirectBuffer) plus its float32 positions to HBM, once. The library derives its

@@ -41,6 +41,7 @@ struct mbvh_ctx {
   DeviceBuffer hits[5];         // host-API all-hits results: offsets, face, dist, point, bary
   DeviceBuffer grad[11];        // host-API backward staging
   DeviceBuffer grad_ws;         // host-API backward workspace
+  DeviceBuffer build_ws;        // device build workspace
   uint64_t hits_total = 0;
   std::mutex lock;              // host entry points on one context are serialised
 };
@@ -466,6 +467,7 @@ extern "C" void mbvh_ctx_destroy(mbvh_ctx* ctx) {
     if (b.ptr) (void)hipFree(b.ptr);
   if (ctx->grad_ws.ptr) (void)hipFree(ctx->grad_ws.ptr);
   if (ctx->order_ws.ptr) (void)hipFree(ctx->order_ws.ptr);
+  if (ctx->build_ws.ptr) (void)hipFree(ctx->build_ws.ptr);
   (void)hipStreamDestroy(ctx->stream);
   ctx->magic = 0;
   delete ctx;
@@ -1831,5 +1833,269 @@ extern "C" int mbvh_instance_hits_d(mbvh_ctx* ctx, const float* matrix_world, ui
   if (rc) return rc;
   InstanceHitArgs a{matrix_world, n_instances, origins, ray_instance, n, face_index, point, distance};
   MBVH_HIP(launch_instance_hits(a, stream ? static_cast<hipStream_t>(stream) : ctx->stream));
+  return MBVH_OK;
+}
+
+// ---- device build ----------------------------------------------------------------------------------
+
+namespace {
+
+// Layout of one build: the primitive buffer (positions [0, span), triangle base + q of the index, or
+// the generated indirect buffer) and the trees over it (buildTree.js:128-140, GeometryBVH.js:9-35,
+// 169-181).
+struct BuildPlan {
+  bool indirect = false;
+  uint64_t base = 0, span = 0;
+  std::vector<uint32_t> src;    // (first triangle, first position) of each source range
+  std::vector<uint32_t> roots;  // (first position, count) of each tree
+  int32_t out_elt = 4;
+};
+
+int plan_build(const char* fn, const uint32_t* root_ranges, uint32_t n_roots, const mbvh_build_opts* opts,
+               int32_t index_elt, BuildPlan* p) {
+  const uint64_t last_end = static_cast<uint64_t>(root_ranges[2 * (n_roots - 1)]) + root_ranges[2 * (n_roots - 1) + 1];
+  p->indirect = opts->indirect != 0;
+  if (p->indirect) {
+    uint64_t total = 0;
+    for (uint32_t r = 0; r < n_roots; r++) {
+      p->src.push_back(root_ranges[2 * r]);
+      p->src.push_back(static_cast<uint32_t>(total));
+      total += root_ranges[2 * r + 1];
+    }
+    p->base = 0;
+    p->span = total;
+    p->roots = {0u, static_cast<uint32_t>(total)};
+    p->out_elt = last_end > 65536 ? 4 : 2;
+  } else {
+    p->base = root_ranges[0];
+    p->span = last_end - p->base;
+    p->src = {static_cast<uint32_t>(p->base), 0u};
+    for (uint32_t r = 0; r < n_roots; r++) {
+      p->roots.push_back(static_cast<uint32_t>(root_ranges[2 * r] - p->base));
+      p->roots.push_back(root_ranges[2 * r + 1]);
+    }
+    p->out_elt = index_elt;
+  }
+  const uint64_t n_trees = p->roots.size() / 2;
+  if (2 * p->span + n_trees + 2 > 0xFFFFFFFFull || last_end > 0xFFFFFFFFull)
+    return set_error(MBVH_ERR_LIMIT, "%s: %llu triangles exceed the device build's 32-bit node and position ids", fn,
+                     static_cast<unsigned long long>(p->span));
+  return MBVH_OK;
+}
+
+struct BuildOut {
+  const uint32_t* packed = nullptr;   // device, trees one after another
+  std::vector<uint64_t> tree_nodes;   // nodes per tree
+  const void* prims = nullptr;        // device: permuted triplets or the indirect buffer
+  uint64_t prims_bytes = 0;
+};
+
+size_t carve(size_t& at, size_t bytes) {
+  const size_t here = at;
+  at = (at + bytes + 255) & ~static_cast<size_t>(255);
+  return here;
+}
+
+// Runs the device build on the context's stream (inputs already on the device) and waits for it.
+int gpu_build(const char* fn, mbvh_ctx* ctx, const float* d_positions, uint64_t vertex_count, uint32_t stride,
+              uint32_t offset, const void* d_index, int32_t index_elt, uint64_t index_count, const BuildPlan& p,
+              const mbvh_build_opts* opts, hipStream_t s, BuildOut* out) {
+  const uint32_t n_trees = static_cast<uint32_t>(p.roots.size() / 2);
+  const uint64_t max_nodes = 2 * p.span + n_trees + 2;
+  const uint32_t n_lvl = static_cast<uint32_t>(opts->max_depth) + 2;
+  const uint64_t prims_bytes = p.indirect ? p.span * p.out_elt : 3 * p.span * index_elt;
+  size_t at = 0;
+  const size_t o_prim = carve(at, 6 * p.span * sizeof(float));
+  const size_t o_perm = carve(at, p.span * 4), o_pl = carve(at, p.span * 4), o_pr = carve(at, p.span * 4);
+  const size_t o_box = carve(at, 6 * max_nodes * 4), o_cen = carve(at, 6 * max_nodes * 4);
+  const size_t o_info = carve(at, max_nodes * 16), o_root = carve(at, max_nodes * 4);
+  const size_t o_size = carve(at, max_nodes * 4), o_slot = carve(at, max_nodes * 4);
+  const size_t o_lvl = carve(at, 4ull * n_lvl * 4), o_big = carve(at, max_nodes * 4);
+  const size_t o_rkeys = carve(at, 12ull * n_trees * 8), o_err = carve(at, 4 * 8);
+  const size_t o_packed = carve(at, max_nodes * kBytesPerNode), o_rn = carve(at, n_trees * 4ull);
+  const size_t o_out = carve(at, prims_bytes), o_src = carve(at, p.src.size() * 4), o_roots = carve(at, p.roots.size() * 4);
+  int rc = ensure_buffer(ctx->build_ws, at);
+  if (rc) return rc;
+  uint8_t* ws = static_cast<uint8_t*>(ctx->build_ws.ptr);
+  GpuBuildArgs a{};
+  a.positions = d_positions;
+  a.n_vertices = vertex_count;
+  a.stride = stride;
+  a.offset = offset;
+  a.index = d_index;
+  a.index_elt = index_elt;
+  a.index_count = d_index ? index_count : 0;
+  a.src = reinterpret_cast<const uint32_t*>(ws + o_src);
+  a.n_src = static_cast<uint32_t>(p.src.size() / 2);
+  a.roots = reinterpret_cast<const uint32_t*>(ws + o_roots);
+  a.n_roots = n_trees;
+  a.base = p.base;
+  a.span = p.span;
+  a.strategy = opts->strategy;
+  a.max_depth = opts->max_depth;
+  a.leaf_size = opts->target_leaf_size;
+  a.prim = reinterpret_cast<float*>(ws + o_prim);
+  a.perm = reinterpret_cast<uint32_t*>(ws + o_perm);
+  a.pair_l = reinterpret_cast<uint32_t*>(ws + o_pl);
+  a.pair_r = reinterpret_cast<uint32_t*>(ws + o_pr);
+  a.nbox = reinterpret_cast<float*>(ws + o_box);
+  a.ncen = reinterpret_cast<float*>(ws + o_cen);
+  a.ninfo = reinterpret_cast<uint4*>(ws + o_info);
+  a.nroot = reinterpret_cast<uint32_t*>(ws + o_root);
+  a.nsize = reinterpret_cast<uint32_t*>(ws + o_size);
+  a.nslot = reinterpret_cast<uint32_t*>(ws + o_slot);
+  a.max_nodes = max_nodes;
+  a.lvl = reinterpret_cast<uint32_t*>(ws + o_lvl);
+  a.big = reinterpret_cast<uint32_t*>(ws + o_big);
+  a.rkeys = reinterpret_cast<unsigned long long*>(ws + o_rkeys);
+  a.err = reinterpret_cast<unsigned long long*>(ws + o_err);
+  a.packed = reinterpret_cast<uint32_t*>(ws + o_packed);
+  a.root_nodes = reinterpret_cast<uint32_t*>(ws + o_rn);
+  a.prims_out = ws + o_out;
+  a.out_elt = p.out_elt;
+  a.indirect = p.indirect ? 1 : 0;
+  MBVH_HIP(hipMemcpyAsync(ws + o_src, p.src.data(), p.src.size() * 4, hipMemcpyHostToDevice, s));
+  MBVH_HIP(hipMemcpyAsync(ws + o_roots, p.roots.data(), p.roots.size() * 4, hipMemcpyHostToDevice, s));
+  MBVH_HIP(launch_build_levels(a, s));
+  std::vector<uint32_t> lvl(4ull * n_lvl);
+  unsigned long long err[3] = {0, 0, 0};
+  MBVH_HIP(hipMemcpyAsync(lvl.data(), a.lvl, lvl.size() * 4, hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipMemcpyAsync(err, a.err, sizeof(err), hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipStreamSynchronize(s));
+  if (err[0] & 2u) {
+    if (err[2] != ~0ull)
+      return set_error(MBVH_ERR_BAD_ARGS, "%s: index[%llu] is out of range", fn, err[2]);
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: a triangle references a vertex past the end", fn);
+  }
+  if (err[0] & 1u)
+    return set_error(MBVH_ERR_LIMIT,
+                     "%s: a leaf would hold %llu triangles; the node format stores at most 65535 (raise max_depth)", fn,
+                     err[1]);
+  uint32_t levels = 0;
+  for (uint32_t d = 0; d + 1 < n_lvl; d++)
+    if (lvl[n_lvl + d]) levels = d + 1;
+  const uint64_t total = levels ? static_cast<uint64_t>(lvl[levels - 1]) + lvl[n_lvl + levels - 1] : 0;
+  MBVH_HIP(launch_build_pack(a, levels, total, s));
+  std::vector<uint32_t> rn(n_trees);
+  MBVH_HIP(hipMemcpyAsync(rn.data(), a.root_nodes, n_trees * 4ull, hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipStreamSynchronize(s));
+  out->packed = a.packed;
+  out->tree_nodes.assign(rn.begin(), rn.end());
+  out->prims = a.prims_out;
+  out->prims_bytes = prims_bytes;
+  return MBVH_OK;
+}
+
+}  // namespace
+
+extern "C" int mbvh_build_gpu(mbvh_ctx* ctx, const float* positions, uint64_t vertex_count, uint32_t position_stride,
+                              uint32_t position_offset, void* index, int32_t index_elt_bytes, uint64_t index_count,
+                              const uint32_t* root_ranges, uint32_t n_roots, const mbvh_build_opts* opts,
+                              mbvh_tree** out_tree) {
+  clear_error();
+  const char* fn = "mbvh_build_gpu";
+  if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid context handle", fn);
+  if (!out_tree) return set_error(MBVH_ERR_BAD_ARGS, "%s: out_tree is NULL", fn);
+  *out_tree = nullptr;
+  int rc = check_build_args(fn, positions, vertex_count, position_stride, position_offset, index, index_elt_bytes,
+                            index_count, root_ranges, n_roots, opts, false);
+  BuildPlan plan;
+  if (!rc) rc = plan_build(fn, root_ranges, n_roots, opts, index_elt_bytes, &plan);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t s = ctx->stream;
+  const size_t pos_bytes = vertex_count ? ((vertex_count - 1) * position_stride + position_offset + 3) * sizeof(float) : 0;
+  const size_t idx_bytes = index ? index_count * static_cast<size_t>(index_elt_bytes) : 0;
+  void* d_pos = nullptr;
+  void* d_idx = nullptr;
+  if ((rc = stage_in(ctx, ctx->staging[0], pos_bytes ? positions : nullptr, pos_bytes, &d_pos)) ||
+      (rc = stage_in(ctx, ctx->staging[1], idx_bytes ? index : nullptr, idx_bytes, &d_idx)))
+    return rc;
+  BuildOut res;
+  if ((rc = gpu_build(fn, ctx, static_cast<const float*>(d_pos), vertex_count, position_stride, position_offset, d_idx,
+                      index_elt_bytes, index_count, plan, opts, s, &res)))
+    return rc;
+  mbvh_tree* tree = new (std::nothrow) mbvh_tree();
+  if (!tree) return set_error(MBVH_ERR_OOM, "%s: out of host memory", fn);
+  try {
+    tree->roots.resize(res.tree_nodes.size());
+    for (size_t r = 0; r < res.tree_nodes.size(); r++) tree->roots[r].resize(res.tree_nodes[r] * kBytesPerNode);
+    if (plan.indirect) {
+      tree->indirect_elt_bytes = plan.out_elt;
+      tree->indirect_count = plan.span;
+      tree->indirect.resize(res.prims_bytes);
+    }
+  } catch (...) {
+    delete tree;
+    return set_error(MBVH_ERR_OOM, "%s: out of host memory", fn);
+  }
+  uint64_t node_at = 0;
+  for (size_t r = 0; r < tree->roots.size(); r++) {
+    const size_t bytes = tree->roots[r].size();
+    hipError_t e = hipMemcpyAsync(tree->roots[r].data(), res.packed + node_at * kU32PerNode, bytes, hipMemcpyDeviceToHost, s);
+    node_at += res.tree_nodes[r];
+    if (e != hipSuccess) {
+      delete tree;
+      return set_error(MBVH_ERR_HIP, "%s: node copy failed (%s)", fn, hipGetErrorString(e));
+    }
+  }
+  hipError_t e = hipSuccess;
+  if (plan.indirect) {
+    if (res.prims_bytes) e = hipMemcpyAsync(tree->indirect.data(), res.prims, res.prims_bytes, hipMemcpyDeviceToHost, s);
+  } else if (res.prims_bytes) {
+    e = hipMemcpyAsync(static_cast<uint8_t*>(index) + 3 * plan.base * index_elt_bytes, res.prims, res.prims_bytes,
+                       hipMemcpyDeviceToHost, s);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    delete tree;
+    return set_error(MBVH_ERR_HIP, "%s: result copy failed (%s)", fn, hipGetErrorString(e));
+  }
+  *out_tree = tree;
+  return MBVH_OK;
+}
+
+extern "C" int mbvh_build_gpu_d(mbvh_ctx* ctx, const float* positions, uint64_t vertex_count, uint32_t position_stride,
+                                uint32_t position_offset, void* index, int32_t index_elt_bytes, uint64_t index_count,
+                                const uint32_t* root_ranges, uint32_t n_roots, const mbvh_build_opts* opts,
+                                void* nodes_out, uint64_t nodes_capacity, uint64_t* tree_nodes, uint32_t n_tree_slots,
+                                void* indirect_out, uint64_t indirect_capacity, void* stream) {
+  clear_error();
+  const char* fn = "mbvh_build_gpu_d";
+  if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid context handle", fn);
+  int rc = check_build_args(fn, positions, vertex_count, position_stride, position_offset, index, index_elt_bytes,
+                            index_count, root_ranges, n_roots, opts, false);
+  BuildPlan plan;
+  if (!rc) rc = plan_build(fn, root_ranges, n_roots, opts, index_elt_bytes, &plan);
+  if (rc) return rc;
+  const uint32_t n_trees = static_cast<uint32_t>(plan.roots.size() / 2);
+  if (!tree_nodes || n_tree_slots < n_trees)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: tree_nodes must hold %u counts", fn, n_trees);
+  const uint64_t need_nodes = (2 * plan.span + n_trees) * kBytesPerNode;
+  if (!nodes_out || nodes_capacity < need_nodes)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: nodes_out must hold %llu bytes (2 x triangles + trees nodes)", fn,
+                     static_cast<unsigned long long>(need_nodes));
+  if (plan.indirect && (!indirect_out || indirect_capacity < plan.span * plan.out_elt))
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: indirect_out must hold %llu bytes", fn,
+                     static_cast<unsigned long long>(plan.span * plan.out_elt));
+  std::lock_guard<std::mutex> guard(ctx->lock);  // the build workspace belongs to the context
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  BuildOut res;
+  if ((rc = gpu_build(fn, ctx, positions, vertex_count, position_stride, position_offset, index, index_elt_bytes,
+                      index_count, plan, opts, s, &res)))
+    return rc;
+  uint64_t total = 0;
+  for (uint32_t r = 0; r < n_trees; r++) {
+    tree_nodes[r] = res.tree_nodes[r];
+    total += res.tree_nodes[r];
+  }
+  if (total) MBVH_HIP(hipMemcpyAsync(nodes_out, res.packed, total * kBytesPerNode, hipMemcpyDeviceToDevice, s));
+  if (res.prims_bytes) {
+    void* dst = plan.indirect ? indirect_out : static_cast<uint8_t*>(index) + 3 * plan.base * index_elt_bytes;
+    MBVH_HIP(hipMemcpyAsync(dst, res.prims, res.prims_bytes, hipMemcpyDeviceToDevice, s));
+  }
+  MBVH_HIP(hipStreamSynchronize(s));
   return MBVH_OK;
 }

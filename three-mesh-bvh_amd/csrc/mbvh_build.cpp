@@ -453,6 +453,55 @@ uint64_t read_index(const void* index, int elt_bytes, uint64_t i) {
 
 using namespace mbvh;
 
+namespace mbvh {
+
+int check_build_args(const char* fn, const float* positions, uint64_t vertex_count, uint32_t position_stride,
+                     uint32_t position_offset, const void* index, int32_t index_elt_bytes, uint64_t index_count,
+                     const uint32_t* root_ranges, uint32_t n_roots, const mbvh_build_opts* opts,
+                     bool check_index_values) {
+  if (!opts) return set_error(MBVH_ERR_BAD_ARGS, "%s: opts is NULL", fn);
+  if (!positions && vertex_count > 0)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: positions is NULL", fn);
+  if (position_stride < position_offset + 3)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: position stride %u too small for offset %u", fn,
+                     position_stride, position_offset);
+  if (opts->strategy < MBVH_CENTER || opts->strategy > MBVH_SAH)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: invalid build strategy value %d", fn, opts->strategy);
+  if (opts->max_depth < 0 || opts->target_leaf_size < 0)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: negative max_depth / target_leaf_size", fn);
+  if (index) {
+    if (index_elt_bytes != 2 && index_elt_bytes != 4)
+      return set_error(MBVH_ERR_BAD_ARGS, "%s: index element size must be 2 or 4", fn);
+    if (index_count % 3 != 0)
+      return set_error(MBVH_ERR_BAD_ARGS, "%s: index count %llu is not a multiple of 3", fn,
+                       static_cast<unsigned long long>(index_count));
+  } else if (!opts->indirect) {
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: a non-indirect build needs an index (ensureIndex)", fn);
+  }
+  if (n_roots == 0 || !root_ranges)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: at least one root range is required", fn);
+
+  const uint64_t tri_total = index ? index_count / 3 : vertex_count / 3;
+  for (uint32_t r = 0; r < n_roots; r++) {
+    const uint64_t off = root_ranges[2 * r], cnt = root_ranges[2 * r + 1];
+    if (off + cnt > tri_total)
+      return set_error(MBVH_ERR_BAD_ARGS, "%s: root range %u [%llu, +%llu) exceeds %llu triangles", fn,
+                       r, static_cast<unsigned long long>(off), static_cast<unsigned long long>(cnt),
+                       static_cast<unsigned long long>(tri_total));
+    if (r > 0 && off < static_cast<uint64_t>(root_ranges[2 * r - 2]) + root_ranges[2 * r - 1])
+      return set_error(MBVH_ERR_BAD_ARGS, "%s: root ranges must be sorted and disjoint", fn);
+  }
+  if (index && check_index_values) {
+    for (uint64_t i = 0; i < index_count; i++)
+      if (read_index(index, index_elt_bytes, i) >= vertex_count)
+        return set_error(MBVH_ERR_BAD_ARGS, "%s: index[%llu] is out of range", fn,
+                         static_cast<unsigned long long>(i));
+  }
+  return MBVH_OK;
+}
+
+}  // namespace mbvh
+
 extern "C" int mbvh_build(const float* positions, uint64_t vertex_count, uint32_t position_stride,
                           uint32_t position_offset, void* index, int32_t index_elt_bytes,
                           uint64_t index_count, const uint32_t* root_ranges, uint32_t n_roots,
@@ -460,46 +509,9 @@ extern "C" int mbvh_build(const float* positions, uint64_t vertex_count, uint32_
   clear_error();
   if (!out_tree) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_build: out_tree is NULL");
   *out_tree = nullptr;
-  if (!opts) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_build: opts is NULL");
-  if (!positions && vertex_count > 0)
-    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_build: positions is NULL");
-  if (position_stride < position_offset + 3)
-    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_build: position stride %u too small for offset %u",
-                     position_stride, position_offset);
-  if (opts->strategy < MBVH_CENTER || opts->strategy > MBVH_SAH)
-    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_build: invalid build strategy value %d",
-                     opts->strategy);
-  if (opts->max_depth < 0 || opts->target_leaf_size < 0)
-    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_build: negative max_depth / target_leaf_size");
-  if (index) {
-    if (index_elt_bytes != 2 && index_elt_bytes != 4)
-      return set_error(MBVH_ERR_BAD_ARGS, "mbvh_build: index element size must be 2 or 4");
-    if (index_count % 3 != 0)
-      return set_error(MBVH_ERR_BAD_ARGS, "mbvh_build: index count %llu is not a multiple of 3",
-                       static_cast<unsigned long long>(index_count));
-  } else if (!opts->indirect) {
-    return set_error(MBVH_ERR_BAD_ARGS,
-                     "mbvh_build: a non-indirect build needs an index (ensureIndex)");
-  }
-  if (n_roots == 0 || !root_ranges)
-    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_build: at least one root range is required");
-
-  const uint64_t tri_total = index ? index_count / 3 : vertex_count / 3;
-  for (uint32_t r = 0; r < n_roots; r++) {
-    const uint64_t off = root_ranges[2 * r], cnt = root_ranges[2 * r + 1];
-    if (off + cnt > tri_total)
-      return set_error(MBVH_ERR_BAD_ARGS, "mbvh_build: root range %u [%llu, +%llu) exceeds %llu triangles",
-                       r, static_cast<unsigned long long>(off), static_cast<unsigned long long>(cnt),
-                       static_cast<unsigned long long>(tri_total));
-    if (r > 0 && off < static_cast<uint64_t>(root_ranges[2 * r - 2]) + root_ranges[2 * r - 1])
-      return set_error(MBVH_ERR_BAD_ARGS, "mbvh_build: root ranges must be sorted and disjoint");
-  }
-  if (index) {
-    for (uint64_t i = 0; i < index_count; i++)
-      if (read_index(index, index_elt_bytes, i) >= vertex_count)
-        return set_error(MBVH_ERR_BAD_ARGS, "mbvh_build: index[%llu] is out of range",
-                         static_cast<unsigned long long>(i));
-  }
+  int rc = check_build_args("mbvh_build", positions, vertex_count, position_stride, position_offset, index,
+                            index_elt_bytes, index_count, root_ranges, n_roots, opts, true);
+  if (rc) return rc;
 
   mbvh_tree* tree = new (std::nothrow) mbvh_tree();
   if (!tree) return set_error(MBVH_ERR_OOM, "mbvh_build: out of host memory");

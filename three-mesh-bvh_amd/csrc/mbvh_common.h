@@ -24,6 +24,13 @@ constexpr uint32_t kMeshMagic = 0x4853454Du;   // "MESH"
 int set_error(int code, const char* fmt, ...);
 void clear_error();
 
+// Argument checks shared by the host and the device builds (messages prefixed with `fn`); the
+// index values are checked only when `check_index_values` (the device build checks them itself).
+int check_build_args(const char* fn, const float* positions, uint64_t vertex_count, uint32_t position_stride,
+                     uint32_t position_offset, const void* index, int32_t index_elt_bytes, uint64_t index_count,
+                     const uint32_t* root_ranges, uint32_t n_roots, const mbvh_build_opts* opts,
+                     bool check_index_values);
+
 // Read-only view of one reference-format node buffer.
 struct NodeView {
   const uint8_t* bytes;

@@ -103,6 +103,56 @@ struct RefitArgs {
 };
 hipError_t launch_refit(const RefitArgs& a, hipStream_t s);
 
+// ---- device build -------------------------------------------------------------------------------
+// Positions q in [0, span) of the primitive buffer being built; a non-indirect build's buffer is the
+// triangle range [base, base + span) of the index, an indirect build's is the generated indirect
+// buffer (base 0).
+struct GpuBuildArgs {
+  const float* positions;      // device vertex data; vertex v, axis k at positions[v * stride + offset + k]
+  uint64_t n_vertices;
+  uint32_t stride;
+  uint32_t offset;
+  const void* index;           // device geometry index (NULL: non-indexed, triangle t = vertices 3t..3t+2)
+  int32_t index_elt;           // 2 or 4
+  uint64_t index_count;
+  const uint32_t* src;         // n_src (first triangle, first position) pairs: position q of source range s
+  uint32_t n_src;              // holds triangle src[2s] + (q - src[2s + 1])
+  const uint32_t* roots;       // n_roots (first position, count) pairs of the trees to build
+  uint32_t n_roots;
+  uint64_t base;               // added to positions in leaf offsets and index writes
+  uint64_t span;
+  int32_t strategy;
+  int32_t max_depth;
+  int32_t leaf_size;
+  // workspace
+  float* prim;                 // 6 planes of span floats: center x, half x, center y, half y, center z, half z
+  uint32_t* perm;              // triangle at each position
+  uint32_t* pair_l;            // partition scratch: misplaced left / right elements by rank
+  uint32_t* pair_r;
+  float* nbox;                 // 6 per node
+  float* ncen;                 // 6 per node: box of the triangle centers
+  uint4* ninfo;                // (first position, count, left child or leaf mark, split axis)
+  uint32_t* nroot;             // tree of each node
+  uint32_t* nsize;             // nodes in the subtree
+  uint32_t* nslot;             // packed slot
+  uint64_t max_nodes;
+  uint32_t* lvl;               // 4 x (max_depth + 2): level start, level count, big-list start, big-list count
+  uint32_t* big;               // nodes split by a whole workgroup
+  unsigned long long* rkeys;   // 12 per root: bound keys of the root ranges
+  unsigned long long* err;     // [0] error flags, [1] largest leaf count over the limit, [2] first bad index entry
+  // outputs
+  uint32_t* packed;            // 8 words per node, trees one after another
+  uint32_t* root_nodes;        // nodes per tree
+  void* prims_out;             // permuted triangle triplets (3 * span entries, index_elt bytes each) or the
+  int32_t out_elt;             // indirect buffer (span entries of out_elt bytes)
+  int32_t indirect;
+};
+constexpr uint32_t kBuildBigCount = 2048;  // nodes with more triangles are split by a whole workgroup
+// Builds every tree level by level; the level counts and error words are then read by the host.
+hipError_t launch_build_levels(const GpuBuildArgs& a, hipStream_t s);
+// Packs levels [0, n_levels) into `packed` (depth-first, left child first) and writes the primitives out.
+hipError_t launch_build_pack(const GpuBuildArgs& a, uint32_t n_levels, uint64_t total_nodes, hipStream_t s);
+
 // ---- instances and the batched multi-mesh launch -------------------------------------------------
 struct InstanceRayArgs {
   const float* inverse;          // 16 floats per instance (world -> local, column-major)

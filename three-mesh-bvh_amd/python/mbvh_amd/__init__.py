@@ -27,7 +27,7 @@ LIB_PATH = os.path.join(_PKG_ROOT, "lib", "libmbvh.so")
 EXPORTED_SYMBOLS = (
     "mbvh_abi_version", "mbvh_last_error", "mbvh_device_count",
     "mbvh_build", "mbvh_tree_info", "mbvh_tree_root_bytes", "mbvh_tree_copy_root",
-    "mbvh_tree_copy_indirect", "mbvh_tree_free",
+    "mbvh_tree_copy_indirect", "mbvh_tree_free", "mbvh_build_gpu", "mbvh_build_gpu_d",
     "mbvh_ctx_create", "mbvh_ctx_destroy", "mbvh_ctx_synchronize", "mbvh_ctx_set_ray_lanes",
     "mbvh_ctx_set_point_lanes",
     "mbvh_mesh_upload", "mbvh_mesh_free", "mbvh_mesh_info", "mbvh_mesh_set_side_mask", "mbvh_mesh_refit",
@@ -88,6 +88,9 @@ def library() -> C.CDLL:
         "mbvh_tree_copy_root": (C.c_int, [vp, u32, vp, u64]),
         "mbvh_tree_copy_indirect": (C.c_int, [vp, vp, u64]),
         "mbvh_tree_free": (None, [vp]),
+        "mbvh_build_gpu": (C.c_int, [vp, vp, u64, u32, u32, vp, i32, u64, vp, u32, C.POINTER(BuildOpts), C.POINTER(vp)]),
+        "mbvh_build_gpu_d": (C.c_int, [vp, vp, u64, u32, u32, vp, i32, u64, vp, u32, C.POINTER(BuildOpts),
+                                       vp, u64, C.POINTER(u64), u32, vp, u64, vp]),
         "mbvh_ctx_create": (C.c_int, [i32, C.POINTER(vp)]),
         "mbvh_ctx_destroy": (None, [vp]),
         "mbvh_ctx_synchronize": (C.c_int, [vp]),
@@ -164,11 +167,12 @@ class Tree:
 
 def build(positions: np.ndarray, index: Optional[np.ndarray], ranges: Optional[Sequence] = None,
           strategy: int = SAH, max_depth: int = 40, target_leaf_size: int = 10,
-          indirect: bool = False, stride: int = 3, offset: int = 0) -> Tree:
+          indirect: bool = False, stride: int = 3, offset: int = 0, ctx: Optional["Context"] = None) -> Tree:
     """Builds the BVH. A non-indirect build permutes `index` IN PLACE (like the reference).
 
     positions: float32 array with `stride` floats per vertex. ranges: (offset, count) triangle
-    ranges of the roots (defaults to every triangle).
+    ranges of the roots (defaults to every triangle). With a Context the build runs on its device
+    (mbvh_build_gpu) and returns the same bytes.
     """
     lib = library()
     pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
@@ -184,9 +188,12 @@ def build(positions: np.ndarray, index: Optional[np.ndarray], ranges: Optional[S
     rr = np.asarray(ranges, dtype=np.uint32).reshape(-1)
     opts = BuildOpts(strategy, max_depth, target_leaf_size, 1 if indirect else 0)
     tree = C.c_void_p()
-    _check(lib.mbvh_build(_ptr(pos), nverts, stride, offset, _ptr(index),
-                          0 if index is None else index.itemsize, 0 if index is None else index.size,
-                          _ptr(rr), len(rr) // 2, C.byref(opts), C.byref(tree)))
+    args = (_ptr(pos), nverts, stride, offset, _ptr(index), 0 if index is None else index.itemsize,
+            0 if index is None else index.size, _ptr(rr), len(rr) // 2, C.byref(opts), C.byref(tree))
+    if ctx is None:
+        _check(lib.mbvh_build(*args))
+    else:
+        _check(lib.mbvh_build_gpu(ctx.handle, *args))
     try:
         nroots, ieb, icount = C.c_uint32(), C.c_int32(), C.c_uint64()
         _check(lib.mbvh_tree_info(tree, C.byref(nroots), C.byref(ieb), C.byref(icount)))
@@ -236,6 +243,22 @@ class Context:
     def set_ray_lanes(self, lanes: int) -> None:
         """raycastFirst lanes per ray: 0 automatic, 1 (binary tree) or 4 (4-wide tree)."""
         _check(self._lib.mbvh_ctx_set_ray_lanes(self.handle, lanes))
+
+    def build_d(self, positions_ptr: int, vertex_count: int, index_ptr: int, index_elt: int, index_count: int,
+                ranges: Sequence, nodes_ptr: int, nodes_capacity: int, strategy: int = SAH, max_depth: int = 40,
+                target_leaf_size: int = 10, indirect: bool = False, indirect_ptr: int = 0,
+                indirect_capacity: int = 0, stride: int = 3, offset: int = 0, stream: int = 0) -> List[int]:
+        """mbvh_build_gpu_d: device positions / index (permuted in place); returns the node count of
+        each tree written one after another at nodes_ptr."""
+        rr = np.asarray(ranges, dtype=np.uint32).reshape(-1)
+        n_trees = 1 if indirect else len(rr) // 2
+        counts = (C.c_uint64 * n_trees)()
+        opts = BuildOpts(strategy, max_depth, target_leaf_size, 1 if indirect else 0)
+        _check(self._lib.mbvh_build_gpu_d(self.handle, positions_ptr, vertex_count, stride, offset, index_ptr or None,
+                                          index_elt, index_count, _ptr(rr), len(rr) // 2, C.byref(opts), nodes_ptr,
+                                          nodes_capacity, counts, n_trees, indirect_ptr or None, indirect_capacity,
+                                          stream or None))
+        return [int(c) for c in counts]
 
     def upload(self, tree: Tree, positions: np.ndarray, index: Optional[np.ndarray],
                stride: int = 3, offset: int = 0) -> "Mesh":
