@@ -1912,6 +1912,15 @@ int gpu_build(const char* fn, mbvh_ctx* ctx, const float* d_positions, uint64_t 
   const size_t o_size = carve(at, max_nodes * 4), o_slot = carve(at, max_nodes * 4);
   const size_t o_lvl = carve(at, 4ull * n_lvl * 4), o_big = carve(at, max_nodes * 4);
   const size_t o_rkeys = carve(at, 12ull * n_trees * 8), o_err = carve(at, 4 * 8);
+  const uint64_t max_big = p.span / kBuildBigCount + n_trees + 1;
+  const uint64_t max_chunks = p.span / kBuildChunk + max_big + 1;
+  const size_t o_wbins = carve(at, max_big * 3 * 32 * 7 * 4), o_waxis = carve(at, max_big * 4);
+  const size_t o_wpos = carve(at, max_big * 8), o_wbase = carve(at, (max_big + 1) * 4);
+  const size_t o_wcnt = carve(at, max_chunks * 4), o_wpre = carve(at, max_chunks * 4);
+  const size_t o_wnl = carve(at, max_big * 4), o_wm = carve(at, max_big * 4), o_wkeys = carve(at, max_big * 24 * 8);
+  const uint64_t max_width = p.span + n_trees + 1;  // nodes of one level (disjoint ranges, empty roots aside)
+  const size_t o_tsplit = carve(at, max_width * 8), o_tbound = carve(at, max_width * 24 * 4);
+  const size_t o_ttile = carve(at, (max_width / 1024 + 2) * 4);
   const size_t o_packed = carve(at, max_nodes * kBytesPerNode), o_rn = carve(at, n_trees * 4ull);
   const size_t o_out = carve(at, prims_bytes), o_src = carve(at, p.src.size() * 4), o_roots = carve(at, p.roots.size() * 4);
   int rc = ensure_buffer(ctx->build_ws, at);
@@ -1949,6 +1958,18 @@ int gpu_build(const char* fn, mbvh_ctx* ctx, const float* d_positions, uint64_t 
   a.big = reinterpret_cast<uint32_t*>(ws + o_big);
   a.rkeys = reinterpret_cast<unsigned long long*>(ws + o_rkeys);
   a.err = reinterpret_cast<unsigned long long*>(ws + o_err);
+  a.wbins = reinterpret_cast<uint32_t*>(ws + o_wbins);
+  a.waxis = reinterpret_cast<int32_t*>(ws + o_waxis);
+  a.wpos = reinterpret_cast<double*>(ws + o_wpos);
+  a.wbase = reinterpret_cast<uint32_t*>(ws + o_wbase);
+  a.wcnt = reinterpret_cast<uint32_t*>(ws + o_wcnt);
+  a.wpre = reinterpret_cast<uint32_t*>(ws + o_wpre);
+  a.wnl = reinterpret_cast<uint32_t*>(ws + o_wnl);
+  a.wm = reinterpret_cast<uint32_t*>(ws + o_wm);
+  a.wkeys = reinterpret_cast<unsigned long long*>(ws + o_wkeys);
+  a.tsplit = reinterpret_cast<uint2*>(ws + o_tsplit);
+  a.tbound = reinterpret_cast<float*>(ws + o_tbound);
+  a.ttile = reinterpret_cast<uint32_t*>(ws + o_ttile);
   a.packed = reinterpret_cast<uint32_t*>(ws + o_packed);
   a.root_nodes = reinterpret_cast<uint32_t*>(ws + o_rn);
   a.prims_out = ws + o_out;
@@ -1956,7 +1977,22 @@ int gpu_build(const char* fn, mbvh_ctx* ctx, const float* d_positions, uint64_t 
   a.indirect = p.indirect ? 1 : 0;
   MBVH_HIP(hipMemcpyAsync(ws + o_src, p.src.data(), p.src.size() * 4, hipMemcpyHostToDevice, s));
   MBVH_HIP(hipMemcpyAsync(ws + o_roots, p.roots.data(), p.roots.size() * 4, hipMemcpyHostToDevice, s));
-  MBVH_HIP(launch_build_levels(a, s));
+  MBVH_HIP(launch_build_init(a, s));
+  // Level by level. While a level has big nodes the host reads the next level's counts after it
+  // (the big-node launches are needed only then); afterwards every fourth level, to stop at the
+  // first empty one.
+  bool wide = true;
+  uint32_t counts[2] = {0, 0};
+  for (uint32_t d = 0; d <= static_cast<uint32_t>(opts->max_depth); d++) {
+    MBVH_HIP(launch_build_level(a, d, wide, s));
+    if (wide || d % 4 == 3) {
+      MBVH_HIP(hipMemcpyAsync(&counts[0], a.lvl + n_lvl + d + 1, 4, hipMemcpyDeviceToHost, s));
+      MBVH_HIP(hipMemcpyAsync(&counts[1], a.lvl + 3 * n_lvl + d + 1, 4, hipMemcpyDeviceToHost, s));
+      MBVH_HIP(hipStreamSynchronize(s));
+      if (counts[0] == 0) break;
+      wide = counts[1] != 0;
+    }
+  }
   std::vector<uint32_t> lvl(4ull * n_lvl);
   unsigned long long err[3] = {0, 0, 0};
   MBVH_HIP(hipMemcpyAsync(lvl.data(), a.lvl, lvl.size() * 4, hipMemcpyDeviceToHost, s));

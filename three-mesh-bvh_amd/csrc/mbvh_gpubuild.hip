@@ -7,10 +7,12 @@
 //   recursion         buildTree                        src/core/build/buildTree.js:10-121
 //   packing           populateBuffer                   src/core/build/buildUtils.js:35-91
 //
-// Level-synchronous: the nodes of one depth are split by one launch (a whole workgroup for a node of
-// more than kBuildBigCount triangles, one wave for the others); children are allocated in pairs at
-// the next level, so node ids are grouped by depth. Packing then counts subtree sizes deepest level
-// first, hands out depth-first slots from the roots down and writes every node record.
+// Level-synchronous: the nodes of one depth are split together, one wave per node of at most
+// kBuildBigCount triangles and, for the bigger nodes, a sequence of launches that each spread every
+// big node's triangles over all workgroups in chunks (bins, split choice, counts, ranks, swaps,
+// children). Children are allocated in pairs at the next level, so node ids are grouped by depth.
+// Packing then counts subtree sizes deepest level first, hands out depth-first slots from the roots
+// down and writes every node record.
 //
 // Why the results are identical to the sequential build:
 //  - Arithmetic is IEEE double wherever the reference computes with JS numbers, rounded to float
@@ -45,9 +47,8 @@ constexpr uint32_t kLeafMark = 0xFFFFFFFFu;
 constexpr uint32_t kBinWords = 3 * kBins * 7;       // per axis and bin: count, 3 min keys, 3 max keys
 constexpr uint32_t kKeyPosInf = 0xFF800000u;        // key of +inf
 constexpr uint32_t kKeyNegInf = 0x007FFFFFu;        // key of -inf
-constexpr int kBigWaves = 8;
-constexpr int kSmallWaves = 4;                      // waves (one node each) per small-node workgroup
 constexpr uint32_t kBuildBlock = 256;
+constexpr uint32_t kWideChunk = kBuildChunk;
 
 enum : uint32_t { kErrLeafLimit = 1u, kErrIndex = 2u };
 
@@ -120,6 +121,8 @@ __device__ float key_value(const GpuBuildArgs& A, int j, unsigned long long key)
   const float h = A.prim[(2 * d + 1) * A.span + q];
   return is_min ? lo_of(c, h) : hi_of(c, h);
 }
+
+__device__ __forceinline__ float pick3(const float* v, int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
 
 __device__ __forceinline__ void load_prim(const GpuBuildArgs& A, uint32_t q, float* c, float* h) {
   for (int d = 0; d < 3; d++) {
@@ -261,29 +264,12 @@ __device__ void serial_split(const GpuBuildArgs& A, SerialWs& ws, const float* b
   *pos_out = pos;
 }
 
-// ---- group primitives -------------------------------------------------------------------------
-// A group is W waves working on one node: the whole workgroup (W > 1) or a single wave (W = 1, several
-// independent groups per workgroup).
+// ---- wave and workgroup primitives ----------------------------------------------------------------
 
-template <int W>
-struct GroupLds {
-  uint32_t bins[W][kBinWords];
-  uint32_t wcount[W];
-  unsigned long long keys[W][24];
-  double pos;
-  int32_t axis;
-  uint32_t first_child;
-};
-
-template <int W>
-__device__ __forceinline__ void gsync() {
-  if constexpr (W == 1) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  } else {
-    __syncthreads();
-  }
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {
@@ -302,61 +288,171 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
-template <int W>
-__device__ uint32_t group_sum(GroupLds<W>& L, uint32_t v, int w) {
-  v = wave_sum(v);
-  if constexpr (W == 1) {
-    return v;
-  } else {
-    if ((threadIdx.x & 63u) == 0u) L.wcount[w] = v;
-    __syncthreads();
-    uint32_t t = 0;
-    for (int i = 0; i < W; i++) t += L.wcount[i];
-    __syncthreads();
-    return t;
-  }
-}
-
-// Reduces both children's keys over the group into L.keys[0][0..23] (left child 0..11, right 12..23).
-template <int W>
-__device__ void group_keys(GroupLds<W>& L, BoundKeys& kl, BoundKeys& kr, int w, int lane) {
+__device__ __forceinline__ void keys_wave_reduce(BoundKeys& k) {
   for (int o = 32; o > 0; o >>= 1) {
     for (int i = 0; i < 6; i++) {
-      unsigned long long t = shfl_xor_u64(kl.mn[i], o);
-      kl.mn[i] = t < kl.mn[i] ? t : kl.mn[i];
-      t = shfl_xor_u64(kl.mx[i], o);
-      kl.mx[i] = t > kl.mx[i] ? t : kl.mx[i];
-      t = shfl_xor_u64(kr.mn[i], o);
-      kr.mn[i] = t < kr.mn[i] ? t : kr.mn[i];
-      t = shfl_xor_u64(kr.mx[i], o);
-      kr.mx[i] = t > kr.mx[i] ? t : kr.mx[i];
+      unsigned long long t = shfl_xor_u64(k.mn[i], o);
+      k.mn[i] = t < k.mn[i] ? t : k.mn[i];
+      t = shfl_xor_u64(k.mx[i], o);
+      k.mx[i] = t > k.mx[i] ? t : k.mx[i];
     }
-  }
-  if (lane == 0) {
-    for (int i = 0; i < 6; i++) {
-      L.keys[w][i] = kl.mn[i];
-      L.keys[w][6 + i] = kl.mx[i];
-      L.keys[w][12 + i] = kr.mn[i];
-      L.keys[w][18 + i] = kr.mx[i];
-    }
-  }
-  gsync<W>();
-  if constexpr (W > 1) {
-    const int t = w * 64 + lane;
-    if (t < 24) {
-      unsigned long long k = L.keys[0][t];
-      const bool is_min = (t % 12) < 6;
-      for (int i = 1; i < W; i++) {
-        const unsigned long long o = L.keys[i][t];
-        k = is_min ? (o < k ? o : k) : (o > k ? o : k);
-      }
-      L.keys[0][t] = k;
-    }
-    __syncthreads();
   }
 }
 
-// ---- node split ---------------------------------------------------------------------------------
+constexpr int kBlockWaves = kBuildBlock / 64;
+
+// Sum over a workgroup of kBuildBlock threads (every thread gets it).
+__device__ uint32_t block_sum(uint32_t v, uint32_t* sh) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63u) == 0u) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  for (int i = 0; i < kBlockWaves; i++) t += sh[i];
+  __syncthreads();
+  return t;
+}
+
+// Both children's keys reduced over the workgroup and merged into dst[0..23] with 64-bit atomics.
+__device__ void block_keys_to(BoundKeys& kl, BoundKeys& kr, unsigned long long (*sh)[24], unsigned long long* dst) {
+  keys_wave_reduce(kl);
+  keys_wave_reduce(kr);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63u) == 0u) {
+    for (int i = 0; i < 6; i++) {
+      sh[w][i] = kl.mn[i];
+      sh[w][6 + i] = kl.mx[i];
+      sh[w][12 + i] = kr.mn[i];
+      sh[w][18 + i] = kr.mx[i];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 24) {
+    const int t = threadIdx.x;
+    const bool is_min = (t % 12) < 6;
+    unsigned long long v = sh[0][t];
+    for (int i = 1; i < kBlockWaves; i++) {
+      const unsigned long long o = sh[i][t];
+      v = is_min ? (o < v ? o : v) : (o > v ? o : v);
+    }
+    if (is_min) {
+      if (v != ~0ull) atomicMin(dst + t, v);
+    } else if (v != 0ull) {
+      atomicMax(dst + t, v);
+    }
+  }
+  __syncthreads();
+}
+
+// ---- split choice ---------------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t bin_init(uint32_t word) {
+  return word == 0 ? 0u : (word <= 3 ? kKeyPosInf : kKeyNegInf);
+}
+
+struct BinFrame {  // splitUtils.js:180-186: bins start at the centroid box's min, 32 equal widths
+  double left[3];
+  double width[3];
+};
+
+__device__ __forceinline__ BinFrame bin_frame(const float* cen) {
+  BinFrame f;
+  for (int a = 0; a < 3; a++) {
+    f.left[a] = cen[a];
+    f.width[a] = (static_cast<double>(cen[a + 3]) - f.left[a]) / kBins;
+  }
+  return f;
+}
+
+// Adds a primitive to its bin on every axis (splitUtils.js:197-213): count and box keys.
+__device__ __forceinline__ void bin_prim(uint32_t* bins, const BinFrame& f, const float* c, const float* h) {
+  float lo[3], hi[3];
+  for (int d = 0; d < 3; d++) {
+    lo[d] = lo_of(c[d], h[d]);
+    hi[d] = hi_of(c[d], h[d]);
+  }
+  for (int a = 0; a < 3; a++) {
+    int32_t idx = js_to_int32((static_cast<double>(c[a]) - f.left[a]) / f.width[a]);
+    if (idx >= kBins) idx = kBins - 1;
+    if (idx < 0) idx = 0;
+    uint32_t* e = bins + (a * kBins + idx) * 7;
+    atomicAdd(e, 1u);
+    for (int d = 0; d < 3; d++) {
+      if (lo[d] == lo[d]) atomicMin(e + 1 + d, fkey(lo[d]));
+      if (hi[d] == hi[d]) atomicMax(e + 4 + d, fkey(hi[d]));
+    }
+  }
+}
+
+// The first cheapest of the 93 binned SAH candidates (splitUtils.js:215-284), by one whole wave;
+// every lane returns the choice (axis -1: no candidate beats the leaf cost).
+__device__ void sah_select(const uint32_t* bins, const float* box, const float* cen, uint32_t cnt, int lane,
+                           int* axis_out, double* pos_out) {
+  const BinFrame f = bin_frame(cen);
+  const double root_area = surface_area(box);
+  const double dcount = static_cast<double>(cnt);
+  const double leaf_cost = kPrimitiveIntersectCost * dcount;
+  double best = __builtin_huge_val();
+  uint32_t best_id = 0xFFFFFFFFu;
+  for (int rep = 0; rep < 2; rep++) {
+    const uint32_t cid = lane + 64 * rep;
+    if (cid >= 3 * kBins || (cid % kBins) == kBins - 1) continue;
+    const int a = cid / kBins, i = cid % kBins;
+    uint32_t lc = 0;
+    uint32_t lk[6] = {kKeyPosInf, kKeyPosInf, kKeyPosInf, kKeyNegInf, kKeyNegInf, kKeyNegInf};
+    uint32_t rk[6] = {kKeyPosInf, kKeyPosInf, kKeyPosInf, kKeyNegInf, kKeyNegInf, kKeyNegInf};
+    for (int b = 0; b < kBins; b++) {
+      const uint32_t* e = bins + (a * kBins + b) * 7;
+      const bool to_left = b <= i;
+      if (to_left) lc += e[0];
+      for (int d = 0; d < 3; d++) {
+        const uint32_t mn = e[1 + d], mx = e[4 + d];
+        if (to_left) {
+          lk[d] = mn < lk[d] ? mn : lk[d];
+          lk[3 + d] = mx > lk[3 + d] ? mx : lk[3 + d];
+        } else {
+          rk[d] = mn < rk[d] ? mn : rk[d];
+          rk[3 + d] = mx > rk[3 + d] ? mx : rk[3 + d];
+        }
+      }
+    }
+    float lb[6], rb[6];
+    for (int d = 0; d < 6; d++) {
+      lb[d] = key_float(lk[d]);
+      rb[d] = key_float(rk[d]);
+    }
+    const double lcd = static_cast<double>(lc);
+    const double rcd = dcount - lcd;
+    const double lp = lc != 0 ? surface_area(lb) / root_area : 0.0;
+    const double rp = rcd != 0 ? surface_area(rb) / root_area : 0.0;
+    const double cost = kTraversalCost + kPrimitiveIntersectCost * (lp * lcd + rp * rcd);
+    if (cost < leaf_cost && (cost < best || (cost == best && cid < best_id))) {
+      best = cost;
+      best_id = cid;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o, 64);
+    const uint32_t oi = __shfl_xor(best_id, o, 64);
+    if (ob < best || (ob == best && oi < best_id)) {
+      best = ob;
+      best_id = oi;
+    }
+  }
+  if (best_id == 0xFFFFFFFFu) {
+    *axis_out = -1;
+    *pos_out = 0.0;
+  } else {
+    const int a = best_id / kBins, i = best_id % kBins;
+    const double left = a == 0 ? f.left[0] : (a == 1 ? f.left[1] : f.left[2]);
+    const double width = a == 0 ? f.width[0] : (a == 1 ? f.width[1] : f.width[2]);
+    *axis_out = a;
+    *pos_out = left + width + i * width;
+  }
+}
+
+// ---- node bookkeeping -----------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t n_lvl(const GpuBuildArgs& A) { return static_cast<uint32_t>(A.max_depth) + 2; }
 
 __device__ __forceinline__ bool is_big(const GpuBuildArgs& A, uint32_t cnt, uint32_t depth) {
   return cnt > kBuildBigCount && cnt > static_cast<uint32_t>(A.leaf_size) && depth < static_cast<uint32_t>(A.max_depth);
@@ -370,127 +466,36 @@ __device__ void make_leaf(const GpuBuildArgs& A, uint32_t node, uint32_t off, ui
   }
 }
 
-// Binned SAH over the node's primitives (splitUtils.js:168-284), then the first cheapest candidate.
-template <int W>
-__device__ void sah_binned(const GpuBuildArgs& A, GroupLds<W>& L, const float* box, const float* cen, uint32_t off,
-                           uint32_t cnt, int w, int lane) {
-  uint32_t* mine = L.bins[w];
-  for (uint32_t i = lane; i < kBinWords; i += 64) {
-    const uint32_t j = i % 7;
-    mine[i] = j == 0 ? 0u : (j <= 3 ? kKeyPosInf : kKeyNegInf);
-  }
-  gsync<W>();
-  double left[3], width[3];
-  for (int a = 0; a < 3; a++) {
-    left[a] = cen[a];
-    width[a] = (static_cast<double>(cen[a + 3]) - left[a]) / kBins;
-  }
-  for (uint32_t q = off + w * 64 + lane; q < off + cnt; q += 64 * W) {
-    float c[3], h[3];
-    load_prim(A, q, c, h);
-    float lo[3], hi[3];
-    for (int d = 0; d < 3; d++) {
-      lo[d] = lo_of(c[d], h[d]);
-      hi[d] = hi_of(c[d], h[d]);
-    }
-    for (int a = 0; a < 3; a++) {
-      int32_t idx = js_to_int32((static_cast<double>(c[a]) - left[a]) / width[a]);
-      if (idx >= kBins) idx = kBins - 1;
-      if (idx < 0) idx = 0;
-      uint32_t* e = mine + (a * kBins + idx) * 7;
-      atomicAdd(e, 1u);
-      for (int d = 0; d < 3; d++) {
-        if (lo[d] == lo[d]) atomicMin(e + 1 + d, fkey(lo[d]));
-        if (hi[d] == hi[d]) atomicMax(e + 4 + d, fkey(hi[d]));
-      }
-    }
-  }
-  gsync<W>();
-  if constexpr (W > 1) {
-    for (uint32_t i = w * 64 + lane; i < kBinWords; i += 64 * W) {
-      const uint32_t j = i % 7;
-      uint32_t v = L.bins[0][i];
-      for (int k = 1; k < W; k++) {
-        const uint32_t o = L.bins[k][i];
-        v = j == 0 ? v + o : (j <= 3 ? (o < v ? o : v) : (o > v ? o : v));
-      }
-      L.bins[0][i] = v;
-    }
-    __syncthreads();
-  }
-  if (w == 0) {
-    const uint32_t* bins = L.bins[0];
-    const double root_area = surface_area(box);
-    const double dcount = static_cast<double>(cnt);
-    const double leaf_cost = kPrimitiveIntersectCost * dcount;
-    double best = __builtin_huge_val();
-    uint32_t best_id = 0xFFFFFFFFu;
-    for (int rep = 0; rep < 2; rep++) {
-      const uint32_t cid = lane + 64 * rep;
-      if (cid >= 3 * kBins || (cid % kBins) == kBins - 1) continue;
-      const int a = cid / kBins, i = cid % kBins;
-      uint32_t lc = 0;
-      uint32_t lk[6] = {kKeyPosInf, kKeyPosInf, kKeyPosInf, kKeyNegInf, kKeyNegInf, kKeyNegInf};
-      uint32_t rk[6] = {kKeyPosInf, kKeyPosInf, kKeyPosInf, kKeyNegInf, kKeyNegInf, kKeyNegInf};
-      for (int b = 0; b < kBins; b++) {
-        const uint32_t* e = bins + (a * kBins + b) * 7;
-        const bool to_left = b <= i;
-        if (to_left) lc += e[0];
-        for (int d = 0; d < 3; d++) {
-          const uint32_t mn = e[1 + d], mx = e[4 + d];
-          if (to_left) {
-            lk[d] = mn < lk[d] ? mn : lk[d];
-            lk[3 + d] = mx > lk[3 + d] ? mx : lk[3 + d];
-          } else {
-            rk[d] = mn < rk[d] ? mn : rk[d];
-            rk[3 + d] = mx > rk[3 + d] ? mx : rk[3 + d];
-          }
-        }
-      }
-      float lb[6], rb[6];
-      for (int d = 0; d < 6; d++) {
-        lb[d] = key_float(lk[d]);
-        rb[d] = key_float(rk[d]);
-      }
-      const double lcd = static_cast<double>(lc);
-      const double rcd = dcount - lcd;
-      const double lp = lc != 0 ? surface_area(lb) / root_area : 0.0;
-      const double rp = rcd != 0 ? surface_area(rb) / root_area : 0.0;
-      const double cost = kTraversalCost + kPrimitiveIntersectCost * (lp * lcd + rp * rcd);
-      if (cost < leaf_cost && (cost < best || (cost == best && cid < best_id))) {
-        best = cost;
-        best_id = cid;
-      }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ob = __shfl_xor(best, o, 64);
-      const uint32_t oi = __shfl_xor(best_id, o, 64);
-      if (ob < best || (ob == best && oi < best_id)) {
-        best = ob;
-        best_id = oi;
-      }
-    }
-    if (lane == 0) {
-      if (best_id == 0xFFFFFFFFu) {
-        L.axis = -1;
-      } else {
-        const int a = best_id / kBins, i = best_id % kBins;
-        L.axis = a;
-        L.pos = left[a] + width[a] + i * width[a];
-      }
-    }
-  }
+// A node's outcome at its level position p (node id - level start): split[p] = (elements left of the
+// plane, axis), (0, 0) for a leaf; bound[24 p + j] = the children's bounds (left child j = 0..11,
+// right 12..23: box min xyz, box max xyz, center-box min xyz, center-box max xyz). Children ids are
+// handed out afterwards in level order (k_level_finish), so no counter is shared between nodes.
+__device__ __forceinline__ void put_bound(const GpuBuildArgs& A, uint32_t p, int j, unsigned long long key) {
+  const int jj = j % 12;                    // keys: box min, center min (0..5), box max, center max (6..11)
+  const bool is_min = jj < 6;
+  const int k = is_min ? jj : jj - 6;       // 0..2 box, 3..5 centers
+  const int slot = (k < 3 ? 0 : 6) + (is_min ? 0 : 3) + k % 3;
+  A.tbound[24ull * p + 12 * (j / 12) + slot] = key_value(A, jj, key);
 }
 
-// Splits (or makes a leaf of) one node at depth `depth`; children go to the next level.
-template <int W>
-__device__ void split_node(const GpuBuildArgs& A, GroupLds<W>& L, uint32_t node, uint32_t depth, uint32_t next_start,
-                           uint32_t big_next, int w, int lane) {
-  const int tid = w * 64 + lane;
+// ---- small nodes: one wave per node ---------------------------------------------------------------
+
+struct WaveLds {
+  uint32_t bins[kBinWords];
+  unsigned long long keys[24];
+};
+
+// Leaf at level position p.
+__device__ __forceinline__ void leaf_at(const GpuBuildArgs& A, uint32_t node, uint32_t p, uint32_t off, uint32_t cnt) {
+  make_leaf(A, node, off, cnt);
+  A.tsplit[p] = make_uint2(0u, 0u);
+}
+
+__device__ void split_small(const GpuBuildArgs& A, WaveLds& L, uint32_t node, uint32_t p, uint32_t depth, int lane) {
   const uint4 info = A.ninfo[node];
   const uint32_t off = info.x, cnt = info.y;
   if (cnt <= static_cast<uint32_t>(A.leaf_size) || depth >= static_cast<uint32_t>(A.max_depth)) {
-    if (tid == 0) make_leaf(A, node, off, cnt);
+    if (lane == 0) leaf_at(A, node, p, off, cnt);
     return;
   }
   float box[6], cen[6];
@@ -498,47 +503,37 @@ __device__ void split_node(const GpuBuildArgs& A, GroupLds<W>& L, uint32_t node,
     box[i] = A.nbox[6ull * node + i];
     cen[i] = A.ncen[6ull * node + i];
   }
+  int axis = -1;
+  double pos = 0.0;
   if (A.strategy == MBVH_SAH && cnt >= kBins / 4) {
-    sah_binned<W>(A, L, box, cen, off, cnt, w, lane);
-  } else if (tid == 0) {
-    int axis;
-    double pos;
-    serial_split(A, *reinterpret_cast<SerialWs*>(L.bins[w]), box, cen, off, cnt, &axis, &pos);
-    L.axis = axis;
-    L.pos = pos;
+    for (uint32_t i = lane; i < kBinWords; i += 64) L.bins[i] = bin_init(i % 7);
+    wave_sync();
+    const BinFrame f = bin_frame(cen);
+    for (uint32_t q = off + lane; q < off + cnt; q += 64) {
+      float c[3], h[3];
+      load_prim(A, q, c, h);
+      bin_prim(L.bins, f, c, h);
+    }
+    wave_sync();
+    sah_select(L.bins, box, cen, cnt, lane, &axis, &pos);
+  } else {
+    if (lane == 0) serial_split(A, *reinterpret_cast<SerialWs*>(L.bins), box, cen, off, cnt, &axis, &pos);
+    axis = __shfl(axis, 0, 64);
+    pos = __shfl(pos, 0, 64);
   }
-  gsync<W>();
-  const int axis = L.axis;
-  const double pos = L.pos;
   if (axis < 0) {
-    if (tid == 0) make_leaf(A, node, off, cnt);
+    if (lane == 0) leaf_at(A, node, p, off, cnt);
     return;
   }
 
-  // Count the primitives below the plane, per wave over a contiguous part of the range.
   const float* cax = A.prim + (2ull * axis) * A.span;
-  const uint32_t part = ((cnt + 64 * W - 1) / (64 * W)) * 64;
-  const uint32_t ws = off + (w * part < cnt ? w * part : cnt);
-  const uint32_t we = off + ((w + 1) * part < cnt ? (w + 1) * part : cnt);
-  uint32_t below = 0;
-  for (uint32_t q0 = ws; q0 < we; q0 += 64) {
+  uint32_t n_left = 0;
+  for (uint32_t q0 = off; q0 < off + cnt; q0 += 64) {
     const uint32_t q = q0 + lane;
-    const bool f = q < we && static_cast<double>(cax[q]) < pos;
-    below += __popcll(__ballot(f));
-  }
-  uint32_t run = 0, n_left = below;
-  if constexpr (W > 1) {
-    if (lane == 0) L.wcount[w] = below;
-    __syncthreads();
-    n_left = 0;
-    for (int i = 0; i < W; i++) {
-      if (i < w) run += L.wcount[i];
-      n_left += L.wcount[i];
-    }
-    __syncthreads();
+    n_left += __popcll(__ballot(q < off + cnt && static_cast<double>(cax[q]) < pos));
   }
   if (n_left == 0 || n_left == cnt) {
-    if (tid == 0) make_leaf(A, node, off, cnt);
+    if (lane == 0) leaf_at(A, node, p, off, cnt);
     return;
   }
   const uint32_t split = off + n_left;
@@ -547,13 +542,13 @@ __device__ void split_node(const GpuBuildArgs& A, GroupLds<W>& L, uint32_t node,
   BoundKeys kl, kr;
   keys_init(kl);
   keys_init(kr);
-  uint32_t misplaced = 0;
-  for (uint32_t q0 = ws; q0 < we; q0 += 64) {
+  uint32_t misplaced = 0, run = 0;
+  for (uint32_t q0 = off; q0 < off + cnt; q0 += 64) {
     const uint32_t q = q0 + lane;
-    const bool valid = q < we;
+    const bool valid = q < off + cnt;
     float c[3] = {0.f, 0.f, 0.f}, h[3] = {0.f, 0.f, 0.f};
     if (valid) load_prim(A, q, c, h);
-    const bool f = valid && static_cast<double>(c[axis]) < pos;
+    const bool f = valid && static_cast<double>(pick3(c, axis)) < pos;
     const unsigned long long ball = __ballot(f);
     const uint32_t below_q = run + lanes_below(ball);  // elements below the plane in [off, q)
     run += __popcll(ball);
@@ -573,11 +568,11 @@ __device__ void split_node(const GpuBuildArgs& A, GroupLds<W>& L, uint32_t node,
       }
     }
   }
-  const uint32_t m = group_sum<W>(L, misplaced, w);
-  gsync<W>();
+  const uint32_t m = wave_sum(misplaced);
+  wave_sync();
 
   // Swap pair i: the i-th misplaced element from the left with the i-th from the right.
-  for (uint32_t i = tid; i < m; i += 64 * W) {
+  for (uint32_t i = lane; i < m; i += 64) {
     const uint32_t pl = A.pair_l[off + i];
     const uint32_t pr = A.pair_r[off + n_left - 1 - i];
     float cl[3], hl[3], cr[3], hr[3];
@@ -595,44 +590,25 @@ __device__ void split_node(const GpuBuildArgs& A, GroupLds<W>& L, uint32_t node,
     keys_add(kr, cl, hl, pr);
     keys_add(kl, cr, hr, pl);
   }
-  group_keys<W>(L, kl, kr, w, lane);
-
-  if (w == 0) {
-    uint32_t first = 0;
-    if (lane == 0) {
-      first = next_start + atomicAdd(&A.lvl[(A.max_depth + 2) + depth + 1], 2u);
-      A.ninfo[node] = make_uint4(off, cnt, first, static_cast<uint32_t>(axis));
-      A.ninfo[first] = make_uint4(off, n_left, 0u, 0u);
-      A.ninfo[first + 1] = make_uint4(split, cnt - n_left, 0u, 0u);
-      A.nroot[first] = A.nroot[node];
-      A.nroot[first + 1] = A.nroot[node];
-      const uint32_t counts[2] = {n_left, cnt - n_left};
-      for (int s = 0; s < 2; s++) {
-        if (is_big(A, counts[s], depth + 1)) {
-          const uint32_t at = atomicAdd(&A.lvl[3 * (A.max_depth + 2) + depth + 1], 1u);
-          A.big[big_next + at] = first + s;
-        }
-      }
-    }
-    first = __shfl(first, 0, 64);
-    if (lane < 24) {
-      const int side = lane / 12, j = lane % 12;
-      const float v = key_value(A, j, L.keys[0][lane]);
-      const uint32_t child = first + side;
-      const int k = j < 6 ? j : j - 6;      // 0..2 box, 3..5 centers
-      const int d = k % 3 + (j < 6 ? 0 : 3);  // min xyz then max xyz
-      if (k < 3)
-        A.nbox[6ull * child + d] = v;
-      else
-        A.ncen[6ull * child + d] = v;
+  keys_wave_reduce(kl);
+  keys_wave_reduce(kr);
+  if (lane == 0) {
+    for (int i = 0; i < 6; i++) {
+      L.keys[i] = kl.mn[i];
+      L.keys[6 + i] = kl.mx[i];
+      L.keys[12 + i] = kr.mn[i];
+      L.keys[18 + i] = kr.mx[i];
     }
   }
+  wave_sync();  // also orders the swaps before the bounds re-read them
+  if (lane == 0) A.tsplit[p] = make_uint2(n_left, static_cast<uint32_t>(axis));
+  if (lane < 24) put_bound(A, p, lane, L.keys[lane]);
 }
 
-// ---- kernels ------------------------------------------------------------------------------------
+// ---- kernels: set-up ------------------------------------------------------------------------------
 
 __global__ __launch_bounds__(kBuildBlock) void k_build_init(GpuBuildArgs A) {
-  const uint32_t L = static_cast<uint32_t>(A.max_depth) + 2;
+  const uint32_t L = n_lvl(A);
   for (uint32_t i = threadIdx.x; i < 4 * L; i += kBuildBlock) A.lvl[i] = 0u;
   for (uint32_t i = threadIdx.x; i < 3; i += kBuildBlock) A.err[i] = i == 2 ? ~0ull : 0ull;
   __syncthreads();
@@ -702,9 +678,9 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_prims(GpuBuildArgs A) {
   }
 }
 
-// Root bounds (getBounds over each root range): block reductions, then 64-bit atomics per block.
+// Root bounds (getBounds over each root range): workgroup reductions, then 64-bit atomics.
 __global__ __launch_bounds__(kBuildBlock) void k_build_root_bounds(GpuBuildArgs A) {
-  __shared__ unsigned long long part[kBuildBlock / 64][12];
+  __shared__ unsigned long long part[kBlockWaves][12];
   for (uint32_t r = blockIdx.y; r < A.n_roots; r += gridDim.y) {
     const uint32_t off = A.roots[2 * r], cnt = A.roots[2 * r + 1];
     BoundKeys k;
@@ -715,14 +691,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_root_bounds(GpuBuildArgs 
       load_prim(A, q, c, h);
       keys_add(k, c, h, q);
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      for (int i = 0; i < 6; i++) {
-        unsigned long long t = shfl_xor_u64(k.mn[i], o);
-        k.mn[i] = t < k.mn[i] ? t : k.mn[i];
-        t = shfl_xor_u64(k.mx[i], o);
-        k.mx[i] = t > k.mx[i] ? t : k.mx[i];
-      }
-    }
+    keys_wave_reduce(k);
     const uint32_t w = threadIdx.x >> 6;
     if ((threadIdx.x & 63u) == 0u) {
       for (int i = 0; i < 6; i++) {
@@ -733,7 +702,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_root_bounds(GpuBuildArgs 
     __syncthreads();
     if (threadIdx.x < 12) {
       unsigned long long v = part[0][threadIdx.x];
-      for (uint32_t i = 1; i < kBuildBlock / 64; i++) {
+      for (int i = 1; i < kBlockWaves; i++) {
         const unsigned long long o = part[i][threadIdx.x];
         v = threadIdx.x < 6 ? (o < v ? o : v) : (o > v ? o : v);
       }
@@ -763,43 +732,430 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_roots(GpuBuildArgs A) {
     A.ninfo[r] = make_uint4(off, cnt, 0u, 0u);
     A.nroot[r] = r;
     if (is_big(A, cnt, 0)) {
-      const uint32_t at = atomicAdd(&A.lvl[3 * (A.max_depth + 2)], 1u);
+      const uint32_t at = atomicAdd(&A.lvl[3 * n_lvl(A)], 1u);
       A.big[at] = r;
     }
   }
 }
 
-// Level `depth`, nodes of at most kBuildBigCount primitives: one wave per node.
+// ---- kernels: one level ---------------------------------------------------------------------------
+
+// Nodes of at most kBuildBigCount primitives: one wave per node.
 __global__ __launch_bounds__(kBuildBlock) void k_build_level_small(GpuBuildArgs A, uint32_t depth) {
-  __shared__ GroupLds<1> lds[kSmallWaves];
-  const uint32_t L = static_cast<uint32_t>(A.max_depth) + 2;
+  __shared__ WaveLds lds[kBlockWaves];
+  const uint32_t L = n_lvl(A);
   const uint32_t start = A.lvl[depth], count = A.lvl[L + depth];
-  const uint32_t next_start = start + count;
-  const uint32_t big_next = A.lvl[2 * L + depth] + A.lvl[3 * L + depth];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    A.lvl[depth + 1] = next_start;
-    A.lvl[2 * L + depth + 1] = big_next;
+    A.lvl[depth + 1] = start + count;                            // next level's first id
+    A.lvl[2 * L + depth + 1] = A.lvl[2 * L + depth] + A.lvl[3 * L + depth];  // next big-list start
   }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (uint32_t i = blockIdx.x * kSmallWaves + w; i < count; i += gridDim.x * kSmallWaves) {
+  for (uint32_t i = blockIdx.x * kBlockWaves + w; i < count; i += gridDim.x * kBlockWaves) {
     const uint32_t node = start + i;
     if (is_big(A, A.ninfo[node].y, depth)) continue;
-    gsync<1>();
-    split_node<1>(A, lds[w], node, depth, next_start, big_next, 0, lane);
+    wave_sync();
+    split_small(A, lds[w], node, i, depth, lane);
   }
 }
 
-// Level `depth`, nodes of more than kBuildBigCount primitives: one workgroup per node.
-__global__ __launch_bounds__(kBigWaves * 64) void k_build_level_big(GpuBuildArgs A, uint32_t depth) {
-  __shared__ GroupLds<kBigWaves> lds;
-  const uint32_t L = static_cast<uint32_t>(A.max_depth) + 2;
-  const uint32_t next_start = A.lvl[depth] + A.lvl[L + depth];
-  const uint32_t big_start = A.lvl[2 * L + depth], big_count = A.lvl[3 * L + depth];
-  const uint32_t big_next = big_start + big_count;
+// Nodes of more than kBuildBigCount primitives ("wide" nodes, the big-node list of the level) are
+// split by all workgroups together: the node's primitives are cut into chunks of kWideChunk, and
+// each step below is one launch over every chunk (or every node) of the level.
+struct WideLevel {
+  uint32_t start;  // first big-list entry of the level
+  uint32_t n;      // big nodes in the level
+};
+
+__device__ __forceinline__ WideLevel wide_level(const GpuBuildArgs& A, uint32_t depth) {
+  const uint32_t L = n_lvl(A);
+  return {A.lvl[2 * L + depth], A.lvl[3 * L + depth]};
+}
+
+// Big node holding chunk c (the last node whose first chunk is at or before c).
+__device__ __forceinline__ uint32_t chunk_node(const GpuBuildArgs& A, uint32_t n, uint32_t c) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (A.wbase[mid] <= c) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Chunk counts and first chunks of the level's big nodes; clears their bins, keys and pair counts.
+__global__ __launch_bounds__(1024) void k_wide_plan(GpuBuildArgs A, uint32_t depth) {
+  __shared__ uint32_t wsum[16];
+  const WideLevel lv = wide_level(A, depth);
+  if (lv.n == 0) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (uint32_t i = blockIdx.x; i < big_count; i += gridDim.x) {
+  uint32_t running = 0;
+  for (uint32_t b = 0; b < lv.n; b += 1024) {
+    const uint32_t i = b + threadIdx.x;
+    const uint32_t c = i < lv.n ? (A.ninfo[A.big[lv.start + i]].y + kWideChunk - 1) / kWideChunk : 0u;
+    uint32_t v = c;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = __shfl_up(v, o, 64);
+      if (lane >= o) v += t;
+    }
+    if (lane == 63) wsum[w] = v;
     __syncthreads();
-    split_node<kBigWaves>(A, lds, A.big[big_start + i], depth, next_start, big_next, w, lane);
+    uint32_t before = 0, total = 0;
+    for (int k = 0; k < 16; k++) {
+      if (k < w) before += wsum[k];
+      total += wsum[k];
+    }
+    if (i < lv.n) A.wbase[i] = running + before + v - c;
+    running += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) A.wbase[lv.n] = running;
+  for (uint64_t i = threadIdx.x; i < static_cast<uint64_t>(lv.n) * kBinWords; i += 1024)
+    A.wbins[i] = bin_init(static_cast<uint32_t>(i % 7));
+  for (uint64_t i = threadIdx.x; i < 24ull * lv.n; i += 1024) A.wkeys[i] = (i % 12) < 6 ? ~0ull : 0ull;
+  for (uint32_t i = threadIdx.x; i < lv.n; i += 1024) A.wm[i] = 0u;
+}
+
+// SAH bins of every chunk, merged into the node's bins.
+__global__ __launch_bounds__(kBuildBlock) void k_wide_bin(GpuBuildArgs A, uint32_t depth) {
+  __shared__ uint32_t bins[kBlockWaves][kBinWords];
+  const WideLevel lv = wide_level(A, depth);
+  if (lv.n == 0) return;
+  const uint32_t total = A.wbase[lv.n];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    const uint32_t i = chunk_node(A, lv.n, c);
+    const uint32_t node = A.big[lv.start + i];
+    const uint4 info = A.ninfo[node];
+    const uint32_t first = info.x + (c - A.wbase[i]) * kWideChunk, end = info.x + info.y;
+    for (uint32_t t = lane; t < kBinWords; t += 64) bins[w][t] = bin_init(t % 7);
+    float cen[6];
+    for (int k = 0; k < 6; k++) cen[k] = A.ncen[6ull * node + k];
+    const BinFrame f = bin_frame(cen);
+    wave_sync();
+    for (uint32_t r = 0; r < kWideChunk; r += kBuildBlock) {
+      const uint32_t q = first + r + threadIdx.x;
+      if (q < end) {
+        float cc[3], hh[3];
+        load_prim(A, q, cc, hh);
+        bin_prim(bins[w], f, cc, hh);
+      }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < kBinWords; t += kBuildBlock) {
+      const uint32_t word = t % 7;
+      uint32_t v = bins[0][t];
+      for (int k = 1; k < kBlockWaves; k++) {
+        const uint32_t o = bins[k][t];
+        v = word == 0 ? v + o : (word <= 3 ? (o < v ? o : v) : (o > v ? o : v));
+      }
+      uint32_t* dst = A.wbins + static_cast<uint64_t>(i) * kBinWords + t;
+      if (word == 0) {
+        if (v) atomicAdd(dst, v);
+      } else if (word <= 3) {
+        if (v != kKeyPosInf) atomicMin(dst, v);
+      } else if (v != kKeyNegInf) {
+        atomicMax(dst, v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Split choice of every big node, one wave per node; a node no candidate improves becomes a leaf.
+__global__ __launch_bounds__(kBuildBlock) void k_wide_split(GpuBuildArgs A, uint32_t depth) {
+  __shared__ SerialWs ws[kBlockWaves];
+  const WideLevel lv = wide_level(A, depth);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (uint32_t i = blockIdx.x * kBlockWaves + w; i < lv.n; i += gridDim.x * kBlockWaves) {
+    const uint32_t node = A.big[lv.start + i];
+    const uint4 info = A.ninfo[node];
+    float box[6], cen[6];
+    for (int k = 0; k < 6; k++) {
+      box[k] = A.nbox[6ull * node + k];
+      cen[k] = A.ncen[6ull * node + k];
+    }
+    int axis = -1;
+    double pos = 0.0;
+    if (A.strategy == MBVH_SAH) {
+      sah_select(A.wbins + static_cast<uint64_t>(i) * kBinWords, box, cen, info.y, lane, &axis, &pos);
+    } else {
+      if (lane == 0) serial_split(A, ws[w], box, cen, info.x, info.y, &axis, &pos);
+      axis = __shfl(axis, 0, 64);
+      pos = __shfl(pos, 0, 64);
+    }
+    if (lane == 0) {
+      A.waxis[i] = axis;
+      A.wpos[i] = pos;
+      if (axis < 0) make_leaf(A, node, info.x, info.y);
+    }
+  }
+}
+
+// Elements below the plane in every chunk.
+__global__ __launch_bounds__(kBuildBlock) void k_wide_count(GpuBuildArgs A, uint32_t depth) {
+  __shared__ uint32_t sh[kBlockWaves];
+  const WideLevel lv = wide_level(A, depth);
+  if (lv.n == 0) return;
+  const uint32_t total = A.wbase[lv.n];
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    const uint32_t i = chunk_node(A, lv.n, c);
+    const int axis = A.waxis[i];
+    if (axis < 0) continue;
+    const double pos = A.wpos[i];
+    const uint4 info = A.ninfo[A.big[lv.start + i]];
+    const uint32_t first = info.x + (c - A.wbase[i]) * kWideChunk, end = info.x + info.y;
+    const float* cax = A.prim + (2ull * axis) * A.span;
+    uint32_t below = 0;
+    for (uint32_t r = 0; r < kWideChunk; r += kBuildBlock) {
+      const uint32_t q = first + r + threadIdx.x;
+      below += (q < end && static_cast<double>(cax[q]) < pos) ? 1u : 0u;
+    }
+    below = block_sum(below, sh);
+    if (threadIdx.x == 0) A.wcnt[c] = below;
+  }
+}
+
+// Per node: below-plane elements before each chunk, and the split position; a node whose plane
+// leaves one side empty becomes a leaf.
+__global__ __launch_bounds__(kBuildBlock) void k_wide_scan(GpuBuildArgs A, uint32_t depth) {
+  const WideLevel lv = wide_level(A, depth);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (uint32_t i = blockIdx.x * kBlockWaves + w; i < lv.n; i += gridDim.x * kBlockWaves) {
+    if (A.waxis[i] < 0) continue;
+    const uint32_t c0 = A.wbase[i], c1 = A.wbase[i + 1];
+    uint32_t running = 0;
+    for (uint32_t cb = c0; cb < c1; cb += 64) {
+      const uint32_t c = cb + lane;
+      const uint32_t v = c < c1 ? A.wcnt[c] : 0u;
+      uint32_t incl = v;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+      }
+      if (c < c1) A.wpre[c] = running + incl - v;
+      running += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) {
+      const uint32_t node = A.big[lv.start + i];
+      const uint4 info = A.ninfo[node];
+      A.wnl[i] = running;
+      if (running == 0 || running == info.y) {
+        make_leaf(A, node, info.x, info.y);
+        A.waxis[i] = -1;
+      }
+    }
+  }
+}
+
+// Ranks of the misplaced elements of every chunk (pair lists) and the bounds of the elements that stay.
+__global__ __launch_bounds__(kBuildBlock) void k_wide_rank(GpuBuildArgs A, uint32_t depth) {
+  __shared__ uint32_t sh[kBlockWaves];
+  __shared__ unsigned long long shk[kBlockWaves][24];
+  const WideLevel lv = wide_level(A, depth);
+  if (lv.n == 0) return;
+  const uint32_t total = A.wbase[lv.n];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    const uint32_t i = chunk_node(A, lv.n, c);
+    const int axis = A.waxis[i];
+    if (axis < 0) continue;
+    const double pos = A.wpos[i];
+    const uint4 info = A.ninfo[A.big[lv.start + i]];
+    const uint32_t off = info.x, end = info.x + info.y;
+    const uint32_t n_left = A.wnl[i], split = off + n_left;
+    const uint32_t first = off + (c - A.wbase[i]) * kWideChunk;
+    uint32_t run = A.wpre[c], misplaced = 0;
+    BoundKeys kl, kr;
+    keys_init(kl);
+    keys_init(kr);
+    for (uint32_t r = 0; r < kWideChunk; r += kBuildBlock) {
+      const uint32_t q = first + r + threadIdx.x;
+      const bool valid = q < end;
+      float cc[3] = {0.f, 0.f, 0.f}, hh[3] = {0.f, 0.f, 0.f};
+      if (valid) load_prim(A, q, cc, hh);
+      const bool f = valid && static_cast<double>(pick3(cc, axis)) < pos;
+      const unsigned long long ball = __ballot(f);
+      if (lane == 0) sh[w] = __popcll(ball);
+      __syncthreads();
+      uint32_t before = 0, all = 0;
+      for (int k = 0; k < kBlockWaves; k++) {
+        if (k < w) before += sh[k];
+        all += sh[k];
+      }
+      const uint32_t below_q = run + before + lanes_below(ball);
+      run += all;
+      if (valid) {
+        if (q < split) {
+          if (!f) {
+            A.pair_l[off + (q - off) - below_q] = q;
+            misplaced++;
+          } else {
+            keys_add(kl, cc, hh, q);
+          }
+        } else {
+          if (f)
+            A.pair_r[off + below_q] = q;
+          else
+            keys_add(kr, cc, hh, q);
+        }
+      }
+      __syncthreads();
+    }
+    misplaced = block_sum(misplaced, sh);
+    if (threadIdx.x == 0 && misplaced) atomicAdd(&A.wm[i], misplaced);
+    block_keys_to(kl, kr, shk, A.wkeys + 24ull * i);
+  }
+}
+
+// Swaps of the pairs (pair chunk j of a node = pairs [j * kWideChunk, (j + 1) * kWideChunk)).
+__global__ __launch_bounds__(kBuildBlock) void k_wide_swap(GpuBuildArgs A, uint32_t depth) {
+  __shared__ unsigned long long shk[kBlockWaves][24];
+  const WideLevel lv = wide_level(A, depth);
+  if (lv.n == 0) return;
+  const uint32_t total = A.wbase[lv.n];
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    const uint32_t i = chunk_node(A, lv.n, c);
+    if (A.waxis[i] < 0) continue;
+    const uint32_t m = A.wm[i];
+    const uint32_t p0 = (c - A.wbase[i]) * kWideChunk;
+    if (p0 >= m) continue;
+    const uint4 info = A.ninfo[A.big[lv.start + i]];
+    const uint32_t off = info.x, n_left = A.wnl[i];
+    BoundKeys kl, kr;
+    keys_init(kl);
+    keys_init(kr);
+    for (uint32_t r = 0; r < kWideChunk; r += kBuildBlock) {
+      const uint32_t p = p0 + r + threadIdx.x;
+      if (p < m) {
+        const uint32_t pl = A.pair_l[off + p];
+        const uint32_t pr = A.pair_r[off + n_left - 1 - p];
+        float cl[3], hl[3], cr[3], hr[3];
+        load_prim(A, pl, cl, hl);
+        load_prim(A, pr, cr, hr);
+        const uint32_t tl = A.perm[pl], tr = A.perm[pr];
+        for (int d = 0; d < 3; d++) {
+          A.prim[(2 * d) * A.span + pl] = cr[d];
+          A.prim[(2 * d + 1) * A.span + pl] = hr[d];
+          A.prim[(2 * d) * A.span + pr] = cl[d];
+          A.prim[(2 * d + 1) * A.span + pr] = hl[d];
+        }
+        A.perm[pl] = tr;
+        A.perm[pr] = tl;
+        keys_add(kr, cl, hl, pr);
+        keys_add(kl, cr, hr, pl);
+      }
+    }
+    block_keys_to(kl, kr, shk, A.wkeys + 24ull * i);
+  }
+}
+
+// Outcome of every big node at its level position (leaves were written by the split or scan step).
+__global__ __launch_bounds__(kBuildBlock) void k_wide_finish(GpuBuildArgs A, uint32_t depth) {
+  const WideLevel lv = wide_level(A, depth);
+  const uint32_t start = A.lvl[depth];
+  for (uint32_t i = blockIdx.x * kBuildBlock + threadIdx.x; i < lv.n; i += gridDim.x * kBuildBlock) {
+    const int axis = A.waxis[i];
+    const uint32_t p = A.big[lv.start + i] - start;
+    if (axis < 0) {
+      A.tsplit[p] = make_uint2(0u, 0u);
+      continue;
+    }
+    A.tsplit[p] = make_uint2(A.wnl[i], static_cast<uint32_t>(axis));
+    for (int j = 0; j < 24; j++) put_bound(A, p, j, A.wkeys[24ull * i + j]);
+  }
+}
+
+// Split nodes per tile of kLevelTile level positions.
+constexpr uint32_t kLevelTile = 1024;
+
+__global__ __launch_bounds__(kLevelTile) void k_level_tiles(GpuBuildArgs A, uint32_t depth) {
+  __shared__ uint32_t sh[kLevelTile / 64];
+  const uint32_t count = A.lvl[n_lvl(A) + depth];
+  const uint32_t tiles = (count + kLevelTile - 1) / kLevelTile;
+  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const uint32_t p = t * kLevelTile + threadIdx.x;
+    uint32_t v = (p < count && A.tsplit[p].x != 0u) ? 1u : 0u;
+    v = wave_sum(v);
+    if ((threadIdx.x & 63u) == 0u) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t s = 0;
+      for (int k = 0; k < kLevelTile / 64; k++) s += sh[k];
+      A.ttile[t] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// Children of the level's split nodes, in level order (buildTree.js:97-117): ids at the next level,
+// ranges, tree, bounds, and the next level's big-node list.
+__global__ __launch_bounds__(kLevelTile) void k_level_finish(GpuBuildArgs A, uint32_t depth) {
+  __shared__ uint32_t sh[kLevelTile / 64];
+  __shared__ uint32_t base_sh;
+  const uint32_t L = n_lvl(A);
+  const uint32_t start = A.lvl[depth], count = A.lvl[L + depth];
+  const uint32_t next_start = start + count;
+  const uint32_t big_next = A.lvl[2 * L + depth] + A.lvl[3 * L + depth];
+  const uint32_t tiles = (count + kLevelTile - 1) / kLevelTile;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    // splits in the earlier tiles (and, for block 0, the level total: the next level's size)
+    uint32_t before = 0, all = 0;
+    for (uint32_t k = threadIdx.x; k < tiles; k += kLevelTile) {
+      const uint32_t v = A.ttile[k];
+      all += v;
+      if (k < t) before += v;
+    }
+    before = wave_sum(before);
+    all = wave_sum(all);
+    if (lane == 0) sh[w] = before;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t s = 0;
+      for (int k = 0; k < kLevelTile / 64; k++) s += sh[k];
+      base_sh = s;
+    }
+    __syncthreads();
+    if (t == 0 && lane == 0) sh[w] = all;
+    __syncthreads();
+    if (t == 0 && threadIdx.x == 0) {
+      uint32_t s = 0;
+      for (int k = 0; k < kLevelTile / 64; k++) s += sh[k];
+      A.lvl[L + depth + 1] = 2 * s;
+    }
+    const uint32_t p = t * kLevelTile + threadIdx.x;
+    const uint2 sp = p < count ? A.tsplit[p] : make_uint2(0u, 0u);
+    const bool split = sp.x != 0u;
+    const unsigned long long ball = __ballot(split);
+    __syncthreads();
+    if (lane == 0) sh[w] = __popcll(ball);
+    __syncthreads();
+    uint32_t rank = base_sh + lanes_below(ball);
+    for (int k = 0; k < w; k++) rank += sh[k];
+    if (split) {
+      const uint32_t node = start + p;
+      const uint4 info = A.ninfo[node];
+      const uint32_t first = next_start + 2 * rank;
+      const uint32_t n_left = sp.x;
+      A.ninfo[node] = make_uint4(info.x, info.y, first, sp.y);
+      A.ninfo[first] = make_uint4(info.x, n_left, 0u, 0u);
+      A.ninfo[first + 1] = make_uint4(info.x + n_left, info.y - n_left, 0u, 0u);
+      const uint32_t tree = A.nroot[node];
+      A.nroot[first] = tree;
+      A.nroot[first + 1] = tree;
+      for (int side = 0; side < 2; side++) {
+        const float* b = A.tbound + 24ull * p + 12 * side;
+        for (int k = 0; k < 6; k++) {
+          A.nbox[6ull * (first + side) + k] = b[k];
+          A.ncen[6ull * (first + side) + k] = b[6 + k];
+        }
+        const uint32_t c = side == 0 ? n_left : info.y - n_left;
+        if (is_big(A, c, depth + 1)) {
+          const uint32_t at = atomicAdd(&A.lvl[3 * L + depth + 1], 1u);  // big nodes are few
+          A.big[big_next + at] = first + side;
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -885,7 +1241,7 @@ uint32_t grid_for(uint64_t n, uint32_t cap) {
 
 }  // namespace
 
-hipError_t launch_build_levels(const GpuBuildArgs& a, hipStream_t s) {
+hipError_t launch_build_init(const GpuBuildArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_build_init, dim3(1), dim3(kBuildBlock), 0, s, a);
   hipLaunchKernelGGL(k_build_root_keys, dim3(grid_for(12ull * a.n_roots, 1024)), dim3(kBuildBlock), 0, s, a);
   if (a.index && a.index_count)
@@ -895,10 +1251,23 @@ hipError_t launch_build_levels(const GpuBuildArgs& a, hipStream_t s) {
   const uint32_t gy = a.n_roots < 1024 ? a.n_roots : 1024;
   hipLaunchKernelGGL(k_build_root_bounds, dim3(gx, gy), dim3(kBuildBlock), 0, s, a);
   hipLaunchKernelGGL(k_build_roots, dim3(grid_for(a.n_roots, 1024)), dim3(kBuildBlock), 0, s, a);
-  for (uint32_t d = 0; d <= static_cast<uint32_t>(a.max_depth); d++) {
-    hipLaunchKernelGGL(k_build_level_small, dim3(2048), dim3(kBuildBlock), 0, s, a, d);
-    hipLaunchKernelGGL(k_build_level_big, dim3(512), dim3(kBigWaves * 64), 0, s, a, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_level(const GpuBuildArgs& a, uint32_t depth, bool wide, hipStream_t s) {
+  hipLaunchKernelGGL(k_build_level_small, dim3(2048), dim3(kBuildBlock), 0, s, a, depth);
+  if (wide) {
+    hipLaunchKernelGGL(k_wide_plan, dim3(1), dim3(1024), 0, s, a, depth);
+    if (a.strategy == MBVH_SAH) hipLaunchKernelGGL(k_wide_bin, dim3(1024), dim3(kBuildBlock), 0, s, a, depth);
+    hipLaunchKernelGGL(k_wide_split, dim3(256), dim3(kBuildBlock), 0, s, a, depth);
+    hipLaunchKernelGGL(k_wide_count, dim3(1024), dim3(kBuildBlock), 0, s, a, depth);
+    hipLaunchKernelGGL(k_wide_scan, dim3(256), dim3(kBuildBlock), 0, s, a, depth);
+    hipLaunchKernelGGL(k_wide_rank, dim3(1024), dim3(kBuildBlock), 0, s, a, depth);
+    hipLaunchKernelGGL(k_wide_swap, dim3(1024), dim3(kBuildBlock), 0, s, a, depth);
+    hipLaunchKernelGGL(k_wide_finish, dim3(16), dim3(kBuildBlock), 0, s, a, depth);
   }
+  hipLaunchKernelGGL(k_level_tiles, dim3(256), dim3(kLevelTile), 0, s, a, depth);
+  hipLaunchKernelGGL(k_level_finish, dim3(256), dim3(kLevelTile), 0, s, a, depth);
   return hipGetLastError();
 }
 

@@ -139,6 +139,20 @@ struct GpuBuildArgs {
   uint32_t* lvl;               // 4 x (max_depth + 2): level start, level count, big-list start, big-list count
   uint32_t* big;               // nodes split by a whole workgroup
   unsigned long long* rkeys;   // 12 per root: bound keys of the root ranges
+  // big nodes of the level being split (index i = position in the level's big-node list)
+  uint32_t* wbins;             // SAH bins, 3 x 32 x 7 words per node
+  int32_t* waxis;              // split axis (-1: made a leaf)
+  double* wpos;                // split plane
+  uint32_t* wbase;             // first chunk of each node (+ the level's chunk total)
+  uint32_t* wcnt;              // per chunk: elements below the plane
+  uint32_t* wpre;              // per chunk: elements below the plane in the node's earlier chunks
+  uint32_t* wnl;               // elements below the plane
+  uint32_t* wm;                // misplaced pairs
+  unsigned long long* wkeys;   // 24 per node: bound keys of both children
+  // outcome of every node of the level being split (index = node id - level start)
+  uint2* tsplit;               // (elements left of the plane, axis); (0, 0) for a leaf
+  float* tbound;               // 24 per node: both children's box and center box
+  uint32_t* ttile;             // split nodes per tile of 1024 positions
   unsigned long long* err;     // [0] error flags, [1] largest leaf count over the limit, [2] first bad index entry
   // outputs
   uint32_t* packed;            // 8 words per node, trees one after another
@@ -147,9 +161,12 @@ struct GpuBuildArgs {
   int32_t out_elt;             // indirect buffer (span entries of out_elt bytes)
   int32_t indirect;
 };
-constexpr uint32_t kBuildBigCount = 2048;  // nodes with more triangles are split by a whole workgroup
-// Builds every tree level by level; the level counts and error words are then read by the host.
-hipError_t launch_build_levels(const GpuBuildArgs& a, hipStream_t s);
+constexpr uint32_t kBuildBigCount = 2048;  // nodes with more triangles are split by all workgroups together
+constexpr uint32_t kBuildChunk = 1024;     // big-node primitives per workgroup pass
+// Primitive bounds, root nodes and the level-0 big-node list.
+hipError_t launch_build_init(const GpuBuildArgs& a, hipStream_t s);
+// Splits the nodes of one depth; `wide` launches the big-node steps (needed while the level has big nodes).
+hipError_t launch_build_level(const GpuBuildArgs& a, uint32_t depth, bool wide, hipStream_t s);
 // Packs levels [0, n_levels) into `packed` (depth-first, left child first) and writes the primitives out.
 hipError_t launch_build_pack(const GpuBuildArgs& a, uint32_t n_levels, uint64_t total_nodes, hipStream_t s);
 
