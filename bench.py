@@ -240,6 +240,8 @@ def bench_differentiable(torch, M, W, ctx, stream, args):
     log(f"C4 {out['differentiable_closest']}")
     out["refit"] = bench_refit(torch, stream, mesh, g, idx, tree)
     log(f"refit {out['refit']}")
+    out["build"] = bench_build(torch, M, ctx, w.geometry, "the C4 mesh (displaced Icosahedron(1,223))")
+    log(f"build {out['build']}")
     mesh.free()
     return out
 
@@ -299,6 +301,57 @@ def bench_refit(torch, stream, mesh, g, idx, tree):
         finally:
             shutil.rmtree(d, ignore_errors=True)
     return res
+
+
+def bench_build(torch, M, ctx, g, label):
+    """MeshBVH construction (SAH, leaf 10) on the device: the device-pointer entry point (positions and
+    index in HBM, wall clock of the call, which waits for the build) and the host entry point (arrays
+    copied in, tree and permuted index copied out), against the host builder (the reference's build
+    restated in C++, one thread). Both device results are compared with the host build byte for byte."""
+    import numpy as np
+    g = g.copy()
+    g.ensure_index()
+    idx_h = g.index.copy()
+    t0 = time.perf_counter()
+    th = M.build(g.positions, idx_h)
+    host_s = time.perf_counter() - t0
+    api = []
+    for _ in range(4):
+        idx_d = g.index.copy()
+        t0 = time.perf_counter()
+        td = M.build(g.positions, idx_d, ctx=ctx)
+        api.append(time.perf_counter() - t0)
+    same = all(np.array_equal(a, b) for a, b in zip(th.roots, td.roots)) and np.array_equal(idx_h, idx_d)
+    dev = torch.device("cuda:0")
+    pos = torch.from_numpy(g.positions.reshape(-1)).to(dev)
+    idx_view = g.index.view(np.int32) if g.index.dtype == np.uint32 else g.index.view(np.int16)
+    idx0 = torch.from_numpy(idx_view).to(dev)
+    idx = idx0.clone()
+    cap = (2 * g.tri_count + 1) * 32
+    nodes = torch.empty(cap, dtype=torch.uint8, device=dev)
+    dev_s = []
+    counts = None
+    for _ in range(6):
+        idx.copy_(idx0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        counts = ctx.build_d(pos.data_ptr(), g.positions.shape[0], idx.data_ptr(), g.index.itemsize, g.index.size,
+                             [(0, g.tri_count)], nodes.data_ptr(), cap)
+        dev_s.append(time.perf_counter() - t0)
+    same_d = bool(np.array_equal(nodes[:counts[0] * 32].cpu().numpy(), th.roots[0])
+                  and np.array_equal(idx.cpu().numpy().view(g.index.dtype), idx_h))
+    dev_ms = 1000 * float(np.median(dev_s[1:]))
+    return {"workload": f"SAH build (leaf 10, max depth 40) of {label}: {g.tri_count} tris, "
+                        f"{th.roots[0].nbytes // 32} nodes",
+            "gpu_device_ms": round(dev_ms, 3), "gpu_mtris_per_s": round(g.tri_count / dev_ms / 1e3, 1),
+            "gpu_host_api_ms": round(1000 * float(np.median(api[1:])), 3),
+            "identical_to_cpu": bool(same and same_d),
+            "cpu_baseline": {"single_thread_ms": round(1000 * host_s, 1), "cores": 1, "kind": "port",
+                             "sample": "the host builder (mbvh_build: the reference's buildTree / getOptimalSplit / "
+                                       "partition restated in C++), the whole mesh, one run"},
+            "speedup_vs_cpu": round(1000 * host_s / dev_ms, 1),
+            "note": "device: positions and index in HBM, wall clock of mbvh_build_gpu_d (which waits for the build); "
+                    "host: pageable arrays in, node buffers and permuted index out"}
 
 
 def bench_batched(torch, M, W, ctx, stream, args):
@@ -595,6 +648,8 @@ def main():
                           "kernel_ms": round(float(np.mean(kern1)), 4), **extra},
     }
     if not args.only_primary:
+        line["build_closest_mesh"] = bench_build(torch, M, ctx, g, "the closest-point mesh TorusKnot(10,3,800,200)")
+        log(f"build {line['build_closest_mesh']}")
         line.update(bench_differentiable(torch, M, W, ctx, stream, args))
         line.update(bench_batched(torch, M, W, ctx, stream, args))
     print(json.dumps(line), flush=True)

@@ -28,10 +28,28 @@ const points = readArr('points.bin', Float32Array);
 const nRays = origins.length / 3;
 const nPts = points.length / 3;
 
+const index0 = index.slice();
 const geometry = geometryFrom(positions, index);
 const bvh = new api.MeshBVH(geometry, { strategy: api.SAH, maxLeafTris: 10 });
 writeArr('bvh_index.bin', geometry.index.array);
 writeArr('bvh_root0.bin', new Uint8Array(bvh._roots[0]));
+
+// --- the gpuBuild option: the same node buffers, permuted index and indirect buffer ---------------
+{
+  const same = (a, b, what) => assert.ok(Buffer.from(a).equals(Buffer.from(b)), what);
+  const gd = geometryFrom(positions, index0.slice());
+  const bd = new api.MeshBVH(gd, { strategy: api.SAH, maxLeafTris: 10, gpuBuild: true });
+  same(new Uint8Array(bd._roots[0]), new Uint8Array(bvh._roots[0]), 'gpuBuild nodes');
+  same(new Uint8Array(gd.index.array.buffer), new Uint8Array(geometry.index.array.buffer), 'gpuBuild index');
+  for (const strategy of [api.CENTER, api.AVERAGE]) {
+    const opts = { strategy, indirect: true, range: { start: 30, count: 3 * Math.floor((index0.length - 60) / 3) } };
+    const h = new api.MeshBVH(geometryFrom(positions, index0.slice()), opts);
+    const d = new api.MeshBVH(geometryFrom(positions, index0.slice()), Object.assign({ gpuBuild: true }, opts));
+    assert.strictEqual(h._roots.length, d._roots.length);
+    h._roots.forEach((r, i) => same(new Uint8Array(d._roots[i]), new Uint8Array(r), `gpuBuild indirect nodes ${strategy}`));
+    same(d._indirectBuffer, h._indirectBuffer, 'gpuBuild indirect buffer');
+  }
+}
 
 // --- batch queries -------------------------------------------------------------------------------
 const rf = bvh.raycastFirstBatch(origins, dirs);

@@ -189,12 +189,25 @@ static napi_value js_build(napi_env env, napi_callback_info info) {
       !get_named_i32(env, argv[5], "targetLeafSize", 10, &opts.target_leaf_size) ||
       !get_named_i32(env, argv[5], "indirect", 0, &opts.indirect))
     return throw_msg(env, "build: bad options");
+  int32_t device = -1; /* >= 0: build on that device (mbvh_build_gpu), same result */
+  if (!get_named_i32(env, argv[5], "device", -1, &device)) return throw_msg(env, "build: bad options");
 
   mbvh_tree* tree = NULL;
-  int rc = mbvh_build((const float*)pos.data, pos.length / (size_t)stride, (uint32_t)stride, (uint32_t)offset,
-                      has_index ? idx.data : NULL, has_index ? idx.elt : 0, has_index ? idx.length : 0,
-                      (const uint32_t*)rng.data, (uint32_t)(rng.length / 2), &opts, &tree);
-  if (rc) return throw_status(env, rc, "mbvh_build");
+  int rc;
+  if (device >= 0) {
+    mbvh_ctx* ctx = NULL;
+    rc = context_for(env, device, &ctx);
+    if (rc) return throw_status(env, rc, "mbvh_ctx_create");
+    rc = mbvh_build_gpu(ctx, (const float*)pos.data, pos.length / (size_t)stride, (uint32_t)stride, (uint32_t)offset,
+                        has_index ? idx.data : NULL, has_index ? idx.elt : 0, has_index ? idx.length : 0,
+                        (const uint32_t*)rng.data, (uint32_t)(rng.length / 2), &opts, &tree);
+    if (rc) return throw_status(env, rc, "mbvh_build_gpu");
+  } else {
+    rc = mbvh_build((const float*)pos.data, pos.length / (size_t)stride, (uint32_t)stride, (uint32_t)offset,
+                    has_index ? idx.data : NULL, has_index ? idx.elt : 0, has_index ? idx.length : 0,
+                    (const uint32_t*)rng.data, (uint32_t)(rng.length / 2), &opts, &tree);
+    if (rc) return throw_status(env, rc, "mbvh_build");
+  }
 
   uint32_t n_roots = 0;
   int32_t ieb = 0;

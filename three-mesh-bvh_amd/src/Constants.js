@@ -29,6 +29,7 @@ const DEFAULT_OPTIONS = {
   indirect: false,
   verbose: true,
   range: null,
+  gpuBuild: false,  // build on the device (this package's option; the tree is the same)
   [SKIP_GENERATION]: false,
 };
 

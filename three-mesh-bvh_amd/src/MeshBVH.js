@@ -78,11 +78,13 @@ class MeshBVH {
     });
     const pos = GU.positionLayout(geometry);
     const index = geometry.index ? geometry.index.array : null;
+    // gpuBuild: the same tree built on the current device (byte-for-byte the host build's)
     const out = native.load().build(pos.array, pos.stride, pos.offset, index, flat, {
       strategy: options.strategy,
       maxDepth: options.maxDepth,
       targetLeafSize: options.targetLeafSize,
       indirect: !!options.indirect,
+      device: options.gpuBuild ? native.getDevice() : -1,
     });
     let roots = out.roots;
     if (options.useSharedArrayBuffer) {
