@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=16, help="1/N stride of the CPU-baseline sample")
     ap.add_argument("--only-primary", action="store_true",
                     help="skip the differentiable (C3, C4) and batched (C5) measurements")
+    ap.add_argument("--only-build", action="store_true",
+                    help="time the device BVH build of the 1M-triangle icosphere alone (for profiling) and exit")
     ap.add_argument("--c3-rays", type=int, default=1 << 20)
     ap.add_argument("--c4-grid", type=int, default=256)
     ap.add_argument("--c5-rays", type=int, default=1 << 22)
@@ -480,6 +482,13 @@ def main():
     stream = torch.cuda.Stream(device=dev)
     torch.cuda.set_stream(stream)
     sptr = stream.cuda_stream
+
+    if args.only_build:
+        from mbvh_amd import geometry as G
+        ctx = M.Context(0)
+        print(json.dumps({"build": bench_build(torch, M, ctx, G.displaced_icosphere(223),
+                                               "the C4 mesh (displaced Icosahedron(1,223))")}), flush=True)
+        return
 
     # ---------------- closest point (primary) ----------------
     w = W.c2_closest(args.grid)
