@@ -55,6 +55,7 @@ typedef struct mbvh_ctx mbvh_ctx;     /* one HIP device + one stream + scratch b
 typedef struct mbvh_tree mbvh_tree;   /* host-side build result (reference node buffers) */
 typedef struct mbvh_mesh mbvh_mesh;   /* device-resident BVH + triangles of one MeshBVH */
 typedef struct mbvh_batch mbvh_batch; /* several meshes concatenated for one batched launch */
+typedef struct mbvh_scene mbvh_scene; /* instances of meshes under a top-level tree (TLAS) */
 
 /* ------------------------------------------------------------------------------------------ */
 /* Library                                                                                    */
@@ -328,6 +329,40 @@ int mbvh_batch_raycast_first(mbvh_ctx* ctx, mbvh_batch* batch, const float* matr
                              uint32_t n_instances, const float* origins, const float* directions,
                              const uint32_t* ray_instance, uint64_t n, int32_t side, float near_dist, float far_dist,
                              uint32_t* face_index, float* distance, float* point, float* barycoord);
+/* ------------------------------------------------------------------------------------------ */
+/* Scenes: closest hit over all instances in one launch                                      */
+/* ------------------------------------------------------------------------------------------ */
+
+/*
+ * A scene of instances (mesh meshes[instance_geometry[i]] placed by matrix_world[i], 16 floats,
+ * column-major as three's Matrix4.elements, with its inverse) under a top-level tree over the
+ * instances' world boxes, built at creation (the ClusteredBVH top level, src/webgpu/ClusteredBVH.js:18-325,
+ * with one primitive per instance; its traversal src/webgpu/shapecastFns/getShapecastFn.js:97-217).
+ * visible (NULL: all) excludes instances as getVisibleAt does (src/utils/ExtensionUtilities.js:112-116).
+ * The world boxes come from the meshes' current root boxes: re-create the scene after a refit or
+ * after instances move. The meshes must outlive the scene.
+ */
+int mbvh_scene_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, uint32_t n_meshes, const float* matrix_world,
+                      const float* matrix_world_inverse, const uint32_t* instance_geometry, const uint8_t* visible,
+                      uint32_t n_instances, mbvh_scene** out_scene);
+void mbvh_scene_free(mbvh_scene* scene);
+/*
+ * Closest first hit of each world ray over every visible instance: the result the reference's
+ * acceleratedBatchedMeshRaycast (src/utils/ExtensionUtilities.js:87-161) with firstHitOnly puts first
+ * after three's Raycaster sorts the per-instance hits by distance (equal distances: the lower instance
+ * index). face_index is the face in the hit instance's mesh (MBVH_NO_HIT on a miss), instance the
+ * instance index; distance and point in world space; barycoord as raycastFirst. near / far in world
+ * units (scaled into each instance's frame as raycastObject3D does, src/core/MeshBVH.js:447-494).
+ * Device pointers; instance, distance, point, barycoord may be NULL. Enqueues on `stream`.
+ */
+int mbvh_scene_raycast_first_d(mbvh_ctx* ctx, const mbvh_scene* scene, const float* origins, const float* directions,
+                               uint64_t n, int32_t side, float near_dist, float far_dist, uint32_t* face_index,
+                               uint32_t* instance, float* distance, float* point, float* barycoord, void* stream);
+/* Host-pointer form: stages the inputs, launches, copies the results back and synchronises. */
+int mbvh_scene_raycast_first(mbvh_ctx* ctx, const mbvh_scene* scene, const float* origins, const float* directions,
+                             uint64_t n, int32_t side, float near_dist, float far_dist, uint32_t* face_index,
+                             uint32_t* instance, float* distance, float* point, float* barycoord);
+
 /* Times every candidate (ray_lanes x stack_lds x tile_rays x blocks_per_cu) on these inputs, keeps the fastest
  * in the batch (used when tuning is NULL) and returns it. Synchronous; outputs hold the last run. */
 int mbvh_batch_autotune(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_world,

@@ -1,0 +1,171 @@
+"""GPU scene raycast (mbvh_scene_raycast_first: a top-level tree over the instances' world boxes and
+each instance's mesh, one launch) through the C ABI. The scene's answer for a ray is the closest of
+the per-instance first hits: it must equal, bit for bit, the batched launch run on every (ray,
+instance) pair (whose per-instance hits test_gpu_batch.py pins to the oracle) reduced by world
+distance with ties to the lower instance index — the hit three's stable distance sort puts first
+among the reference's per-instance BatchedMesh hits. A sample is also checked against the FP64
+oracle directly (instance-local rays, world distances)."""
+import numpy as np
+import pytest
+
+import mbvh_amd as M
+from mbvh_amd import geometry as G, workloads as W
+import oracle_py as O
+from helpers import build_mesh
+
+pytestmark = pytest.mark.gpu
+
+GEOMS = [lambda: G.torus_knot(1.0, 0.3, 120, 16), lambda: G.sphere(1.2, 40, 20), lambda: G.box(1.5, 0.7, 1.1),
+         lambda: G.torus(1.0, 0.35, 16, 60), lambda: G.displaced_icosphere(12).ensure_index()]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = M.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def meshes(ctx):
+    built = [build_mesh(f()) for f in GEOMS]
+    ms = [ctx.upload(t, g.positions, idx) for g, idx, t in built]
+    yield built, ms
+    for m in ms:
+        m.free()
+
+
+def _pairwise(ctx, ms, w, visible, side=M.FRONT_SIDE, near=0.0, far=float("inf")):
+    """Every (instance, ray) pair through the batched launch, reduced to the closest instance."""
+    batch = ctx.batch(ms)
+    n = w.origins.shape[0]
+    inst = [i for i in range(w.instance_geometry.shape[0]) if visible is None or visible[i]]
+    face = np.full(n, M.NO_HIT, np.uint32)
+    which = np.full(n, M.NO_HIT, np.uint32)
+    dist = np.full(n, np.inf, np.float32)
+    point = np.full((n, 3), np.nan, np.float32)
+    bary = np.full((n, 3), np.nan, np.float32)
+    if inst:
+        k = len(inst)
+        ri = np.repeat(np.asarray(inst, np.uint32), n)
+        res = batch.raycast_first(w.matrix_world, w.matrix_world_inverse, w.instance_geometry,
+                                  np.tile(w.origins, (k, 1)), np.tile(w.directions, (k, 1)), ri, side, near, far)
+        F = res["face"].reshape(k, n)
+        D = np.where(F == M.NO_HIT, np.float32(np.inf), res["distance"].reshape(k, n))
+        best = np.argmin(D, axis=0)  # the first minimum: the lowest instance index on ties
+        cols = np.arange(n)
+        hit = F[best, cols] != M.NO_HIT
+        face[hit] = F[best, cols][hit]
+        which[hit] = np.asarray(inst, np.uint32)[best[hit]]
+        dist[hit] = D[best, cols][hit]
+        point[hit] = res["point"].reshape(k, n, 3)[best, cols][hit]
+        bary[hit] = res["barycoord"].reshape(k, n, 3)[best, cols][hit]
+    batch.free()
+    return {"face": face, "instance": which, "distance": dist, "point": point, "barycoord": bary}
+
+
+def _same(got, ref):
+    assert np.array_equal(got["face"], ref["face"])
+    assert np.array_equal(got["instance"], ref["instance"])
+    hit = ref["face"] != M.NO_HIT
+    for k in ("distance", "point", "barycoord"):
+        assert np.array_equal(got[k][hit].view(np.uint32), ref[k][hit].view(np.uint32)), k
+    assert np.all(np.isinf(got["distance"][~hit]))
+
+
+@pytest.mark.parametrize("side", [M.FRONT_SIDE, M.DOUBLE_SIDE])
+def test_scene_equals_closest_of_instances(ctx, meshes, side):
+    built, ms = meshes
+    geos = [b[0] for b in built]
+    w = W.scene_rays(geos, 60, 4000, seed=3, non_uniform=0.2)
+    visible = np.ones(60, np.uint8)
+    visible[[4, 17, 33]] = 0
+    print(f"case scene 60 instances side {side}", flush=True)
+    sc = ctx.scene(ms, w.matrix_world, w.matrix_world_inverse, w.instance_geometry, visible)
+    got = sc.raycast_first(w.origins, w.directions, side)
+    ref = _pairwise(ctx, ms, w, visible, side)
+    _same(got, ref)
+    hits = ref["face"] != M.NO_HIT
+    assert hits.mean() > 0.3 and len(np.unique(ref["instance"][hits])) > 20
+    assert not np.any(np.isin(got["instance"][hits], [4, 17, 33]))
+    sc.free()
+
+
+def test_near_far_and_ties(ctx, meshes):
+    built, ms = meshes
+    geos = [b[0] for b in built]
+    w = W.scene_rays(geos, 24, 3000, seed=5)
+    # instance 7 duplicated as 8 and 20: equal distances go to the lowest index
+    for k in (8, 20):
+        w.matrix_world[k] = w.matrix_world[7]
+        w.matrix_world_inverse[k] = w.matrix_world_inverse[7]
+        w.instance_geometry[k] = w.instance_geometry[7]
+    print("case near/far and ties", flush=True)
+    sc = ctx.scene(ms, w.matrix_world, w.matrix_world_inverse, w.instance_geometry)
+    for near, far in ((0.0, float("inf")), (5.0, 40.0), (30.0, 31.0)):
+        got = sc.raycast_first(w.origins, w.directions, M.FRONT_SIDE, near, far)
+        _same(got, _pairwise(ctx, ms, w, None, M.FRONT_SIDE, near, far))
+    got = sc.raycast_first(w.origins, w.directions)
+    assert not np.any(np.isin(got["instance"], [8, 20]))
+    sc.free()
+
+
+def test_oracle_sample(ctx, meshes):
+    built, ms = meshes
+    geos = [b[0] for b in built]
+    w = W.scene_rays(geos, 30, 400, seed=9)
+    sc = ctx.scene(ms, w.matrix_world, w.matrix_world_inverse, w.instance_geometry)
+    got = sc.raycast_first(w.origins, w.directions)
+    oms = [O.OracleMesh(g.positions, idx, t.roots) for g, idx, t in built]
+    n = w.origins.shape[0]
+    best = np.full(n, np.inf)
+    per = np.full((30, n), np.inf)
+    for i in range(30):
+        mw = w.matrix_world[i].astype(np.float64).reshape(4, 4).T
+        mi = w.matrix_world_inverse[i].astype(np.float64).reshape(4, 4).T
+        lo = w.origins @ mi[:3, :3].T + mi[:3, 3]
+        ld = w.directions @ mi[:3, :3].T
+        r = oms[w.instance_geometry[i]].raycast_first(lo.astype(np.float32), ld.astype(np.float32))
+        hit = r["face"] != M.NO_HIT
+        pw = r["point"] @ mw[:3, :3].T + mw[:3, 3]
+        dw = np.linalg.norm(pw - w.origins.astype(np.float64), axis=1)
+        per[i, hit] = dw[hit]
+    best = per.min(axis=0)
+    gh = got["face"] != M.NO_HIT
+    assert np.array_equal(gh, np.isfinite(best))
+    gi = got["instance"][gh]
+    cols = np.nonzero(gh)[0]
+    # the chosen instance's oracle distance is the best one (ties within 1e-6), and the distance agrees
+    assert np.all(per[gi, cols] <= best[gh] * (1 + 1e-6) + 1e-12)
+    assert np.all(np.abs(got["distance"][gh] - best[gh]) <= 1e-5 * np.maximum(best[gh], 1.0))
+    sc.free()
+
+
+def test_empty_and_device_form(ctx, meshes):
+    torch = pytest.importorskip("torch")
+    built, ms = meshes
+    geos = [b[0] for b in built]
+    w = W.scene_rays(geos, 12, 2000, seed=13)
+    print("case empty / device form", flush=True)
+    sc = ctx.scene(ms, w.matrix_world, w.matrix_world_inverse, w.instance_geometry, np.zeros(12, np.uint8))
+    got = sc.raycast_first(w.origins, w.directions)
+    assert np.all(got["face"] == M.NO_HIT) and np.all(got["instance"] == M.NO_HIT)
+    sc.free()
+    sc = ctx.scene(ms, w.matrix_world, w.matrix_world_inverse, w.instance_geometry)
+    host = sc.raycast_first(w.origins, w.directions)
+    dev = torch.device("cuda:0")
+    o = torch.from_numpy(w.origins).to(dev)
+    d = torch.from_numpy(w.directions).to(dev)
+    n = o.shape[0]
+    face = torch.empty(n, dtype=torch.int32, device=dev)
+    inst = torch.empty(n, dtype=torch.int32, device=dev)
+    dist = torch.empty(n, device=dev)
+    torch.cuda.synchronize()
+    sc.raycast_first_d(o.data_ptr(), d.data_ptr(), n, face.data_ptr(), inst.data_ptr(), dist.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(face.cpu().numpy().view(np.uint32), host["face"])
+    assert np.array_equal(inst.cpu().numpy().view(np.uint32), host["instance"])
+    assert np.array_equal(dist.cpu().numpy().view(np.uint32), host["distance"].view(np.uint32))
+    with pytest.raises(M.MbvhError):
+        ctx.scene(ms, w.matrix_world, w.matrix_world_inverse, np.full(12, 99, np.uint32))
+    sc.free()

@@ -2135,3 +2135,271 @@ extern "C" int mbvh_build_gpu_d(mbvh_ctx* ctx, const float* positions, uint64_t 
   MBVH_HIP(hipStreamSynchronize(s));
   return MBVH_OK;
 }
+
+// ---- scenes ------------------------------------------------------------------------------------------
+
+struct mbvh_scene {
+  uint32_t magic = 0x4E454353u;  // "SCEN"
+  mbvh_ctx* ctx = nullptr;
+  uint32_t n_meshes = 0;
+  uint32_t n_instances = 0;
+  MeshView* d_meshes = nullptr;
+  float* d_world = nullptr;
+  float* d_inverse = nullptr;
+  uint32_t* d_geom = nullptr;
+  uint32_t* d_tlas = nullptr;
+  uint32_t* d_tlas_inst = nullptr;
+  uint32_t max_stack = 0;
+};
+
+namespace {
+
+bool valid_scene(const mbvh_scene* s) { return s && s->magic == 0x4E454353u; }
+
+void free_scene_buffers(mbvh_scene* sc) {
+  void* bufs[] = {sc->d_meshes, sc->d_world, sc->d_inverse, sc->d_geom, sc->d_tlas, sc->d_tlas_inst};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+}
+
+// Union of a mesh's root boxes (the device copy of its reference nodes).
+int mesh_local_box(const mbvh_mesh* m, double* box) {
+  for (int k = 0; k < 3; k++) {
+    box[k] = std::numeric_limits<double>::infinity();
+    box[k + 3] = -std::numeric_limits<double>::infinity();
+  }
+  for (uint32_t r = 0; r < m->root_base.size(); r++) {
+    float b[6];
+    MBVH_HIP(hipMemcpy(b, m->d_ref_nodes + 8 * m->root_base[r], sizeof(b), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; k++) {
+      if (b[k] <= b[k + 3]) {
+        box[k] = std::min(box[k], static_cast<double>(b[k]));
+        box[k + 3] = std::max(box[k + 3], static_cast<double>(b[k + 3]));
+      }
+    }
+  }
+  return MBVH_OK;
+}
+
+}  // namespace
+
+extern "C" int mbvh_scene_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, uint32_t n_meshes,
+                                 const float* matrix_world, const float* matrix_world_inverse,
+                                 const uint32_t* instance_geometry, const uint8_t* visible, uint32_t n_instances,
+                                 mbvh_scene** out_scene) {
+  clear_error();
+  const char* fn = "mbvh_scene_create";
+  if (!out_scene) return set_error(MBVH_ERR_BAD_ARGS, "%s: out_scene is NULL", fn);
+  *out_scene = nullptr;
+  if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid context handle", fn);
+  if (n_meshes == 0 || !meshes) return set_error(MBVH_ERR_BAD_ARGS, "%s: no meshes", fn);
+  if (n_instances && (!matrix_world || !matrix_world_inverse || !instance_geometry))
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL matrices or instance geometry", fn);
+  if (n_instances >= (1u << 24)) return set_error(MBVH_ERR_LIMIT, "%s: at most 2^24 - 1 instances", fn);
+  std::vector<MeshView> views(n_meshes);
+  uint32_t max_stack = 0;
+  for (uint32_t i = 0; i < n_meshes; i++) {
+    if (!valid_mesh(meshes[i]) || meshes[i]->ctx != ctx)
+      return set_error(MBVH_ERR_BAD_HANDLE, "%s: mesh %u is not a mesh of this context", fn, i);
+    views[i] = view_of(meshes[i]);
+    max_stack = std::max(max_stack, meshes[i]->max_stack);
+  }
+  for (uint32_t i = 0; i < n_instances; i++)
+    if (instance_geometry[i] >= n_meshes)
+      return set_error(MBVH_ERR_BAD_ARGS, "%s: instance %u names mesh %u of %u", fn, i, instance_geometry[i], n_meshes);
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  std::vector<double> local(6ull * n_meshes);
+  for (uint32_t i = 0; i < n_meshes; i++)
+    if ((rc = mesh_local_box(meshes[i], &local[6ull * i]))) return rc;
+
+  // World boxes of the listed instances (8 transformed corners, rounded outward and widened by
+  // 2^-20 of their magnitude so that no hit rounds outside its instance's box); one build
+  // "triangle" per instance spanning its box.
+  std::vector<uint32_t> listed;
+  std::vector<float> prim_pos;
+  for (uint32_t i = 0; i < n_instances; i++) {
+    if (visible && !visible[i]) continue;
+    const double* lb = &local[6ull * instance_geometry[i]];
+    if (!(lb[0] <= lb[3] && lb[1] <= lb[4] && lb[2] <= lb[5])) continue;  // no triangles
+    const float* e = matrix_world + 16ull * i;
+    double mn[3] = {std::numeric_limits<double>::infinity(), std::numeric_limits<double>::infinity(),
+                    std::numeric_limits<double>::infinity()};
+    double mx[3] = {-mn[0], -mn[1], -mn[2]};
+    for (int c = 0; c < 8; c++) {
+      const double x = lb[(c & 1) ? 3 : 0], y = lb[(c & 2) ? 4 : 1], z = lb[(c & 4) ? 5 : 2];
+      const double w = e[3] * x + e[7] * y + e[11] * z + e[15];
+      for (int k = 0; k < 3; k++) {
+        const double v = (e[k] * x + e[4 + k] * y + e[8 + k] * z + e[12 + k]) / w;
+        mn[k] = std::min(mn[k], v);
+        mx[k] = std::max(mx[k], v);
+      }
+    }
+    bool finite = true;
+    float fb[6];
+    for (int k = 0; k < 3; k++) {
+      const double pad = std::max(std::max(std::fabs(mn[k]), std::fabs(mx[k])) * 9.5367431640625e-07, 1e-30);
+      fb[k] = std::nextafter(static_cast<float>(mn[k] - pad), -std::numeric_limits<float>::infinity());
+      fb[k + 3] = std::nextafter(static_cast<float>(mx[k] + pad), std::numeric_limits<float>::infinity());
+      finite = finite && std::isfinite(mn[k]) && std::isfinite(mx[k]);
+    }
+    if (!finite) return set_error(MBVH_ERR_BAD_ARGS, "%s: instance %u has a non-finite world box", fn, i);
+    listed.push_back(i);
+    const float v[9] = {fb[0], fb[1], fb[2], fb[3], fb[4], fb[5], fb[0], fb[1], fb[2]};
+    prim_pos.insert(prim_pos.end(), v, v + 9);
+  }
+  const uint32_t n_listed = static_cast<uint32_t>(listed.size());
+  std::vector<uint32_t> index(3ull * n_listed);
+  for (uint32_t k = 0; k < 3 * n_listed; k++) index[k] = k;
+  std::vector<uint8_t> nodes(kBytesPerNode, 0);
+  if (n_listed == 0) {  // one empty leaf (its empty box is never entered)
+    const float empty[6] = {std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity(),
+                            std::numeric_limits<float>::infinity(), -std::numeric_limits<float>::infinity(),
+                            -std::numeric_limits<float>::infinity(), -std::numeric_limits<float>::infinity()};
+    const uint32_t leaf_word = 0xFFFF0000u;
+    std::memcpy(nodes.data(), empty, sizeof(empty));
+    std::memcpy(nodes.data() + 28, &leaf_word, 4);
+  } else {
+    const uint32_t range[2] = {0u, n_listed};
+    const mbvh_build_opts opts{MBVH_SAH, 24, 2, 0};  // depth bound = the kernel's TLAS stack
+    mbvh_tree* tlas = nullptr;
+    rc = mbvh_build(prim_pos.data(), 3ull * n_listed, 3, 0, index.data(), 4, index.size(), range, 1, &opts, &tlas);
+    if (rc) return rc;
+    nodes = tlas->roots[0];
+    mbvh_tree_free(tlas);
+  }
+  std::vector<uint32_t> slot_inst(std::max<uint32_t>(n_listed, 1), 0u);
+  for (uint32_t s = 0; s < n_listed; s++) slot_inst[s] = listed[index[3ull * s] / 3];
+
+  if ((rc = reserve_stack(ctx, max_stack))) return rc;
+  mbvh_scene* sc = new (std::nothrow) mbvh_scene();
+  if (!sc) return set_error(MBVH_ERR_OOM, "%s: out of host memory", fn);
+  sc->ctx = ctx;
+  sc->n_meshes = n_meshes;
+  sc->n_instances = n_instances;
+  sc->max_stack = max_stack;
+  const size_t mat_bytes = std::max<size_t>(16ull * n_instances * sizeof(float), 64);
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&sc->d_meshes), views.size() * sizeof(MeshView));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_world), mat_bytes);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_inverse), mat_bytes);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_geom), std::max<size_t>(4ull * n_instances, 4));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_tlas), nodes.size());
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_tlas_inst), slot_inst.size() * 4);
+  if (e == hipSuccess) e = hipMemcpy(sc->d_meshes, views.data(), views.size() * sizeof(MeshView), hipMemcpyHostToDevice);
+  if (e == hipSuccess && n_instances) {
+    e = hipMemcpy(sc->d_world, matrix_world, 16ull * n_instances * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(sc->d_inverse, matrix_world_inverse, 16ull * n_instances * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(sc->d_geom, instance_geometry, 4ull * n_instances, hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess) e = hipMemcpy(sc->d_tlas, nodes.data(), nodes.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(sc->d_tlas_inst, slot_inst.data(), slot_inst.size() * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    free_scene_buffers(sc);
+    delete sc;
+    return set_error(MBVH_ERR_OOM, "%s: device allocation or copy failed (%s)", fn, hipGetErrorString(e));
+  }
+  *out_scene = sc;
+  return MBVH_OK;
+}
+
+extern "C" void mbvh_scene_free(mbvh_scene* scene) {
+  if (!valid_scene(scene)) return;
+  if (valid_ctx(scene->ctx)) {
+    (void)hipSetDevice(scene->ctx->device);
+    (void)hipStreamSynchronize(scene->ctx->stream);
+  }
+  free_scene_buffers(scene);
+  scene->magic = 0;
+  delete scene;
+}
+
+namespace {
+
+int check_scene_call(const char* fn, const mbvh_ctx* ctx, const mbvh_scene* sc, int32_t side, uint64_t n,
+                     const void* o, const void* d, const void* face) {
+  if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid context handle", fn);
+  if (!valid_scene(sc) || sc->ctx != ctx) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid scene handle", fn);
+  if (side < MBVH_FRONT_SIDE || side > MBVH_SIDE_PER_TRIANGLE)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: invalid side %d", fn, side);
+  if (n && (!o || !d || !face)) return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL ray or face buffer", fn);
+  return MBVH_OK;
+}
+
+int enqueue_scene(mbvh_ctx* ctx, const mbvh_scene* sc, const float* o, const float* d, uint64_t n, int32_t side,
+                  float near_d, float far_d, uint32_t* face, uint32_t* inst, float* dist, float* point, float* bary,
+                  hipStream_t s) {
+  if (sc->max_stack > 8 + ctx->stack_ovf_entries)
+    return set_error(MBVH_ERR_LIMIT, "scene: stack scratch smaller than the tree depth");
+  SceneArgs a{};
+  a.meshes = sc->d_meshes;
+  a.n_meshes = sc->n_meshes;
+  a.tlas = sc->d_tlas;
+  a.tlas_inst = sc->d_tlas_inst;
+  a.world = sc->d_world;
+  a.inverse = sc->d_inverse;
+  a.instance_geometry = sc->d_geom;
+  a.origins = o;
+  a.directions = d;
+  a.n = n;
+  a.side_mask = side_mask_of(side);
+  a.near_dist = near_d;
+  a.far_dist = far_d;
+  a.out_face = face;
+  a.out_instance = inst;
+  a.out_dist = dist;
+  a.out_point = point;
+  a.out_bary = bary;
+  const uint64_t threads = launch_threads(n, ctx->max_threads);
+  a.stack = stack_of(ctx, threads);
+  MBVH_HIP(launch_scene_raycast_first(a, threads, s));
+  return MBVH_OK;
+}
+
+}  // namespace
+
+extern "C" int mbvh_scene_raycast_first_d(mbvh_ctx* ctx, const mbvh_scene* scene, const float* origins,
+                                          const float* directions, uint64_t n, int32_t side, float near_dist,
+                                          float far_dist, uint32_t* face_index, uint32_t* instance, float* distance,
+                                          float* point, float* barycoord, void* stream) {
+  clear_error();
+  const char* fn = "mbvh_scene_raycast_first_d";
+  int rc = check_scene_call(fn, ctx, scene, side, n, origins, directions, face_index);
+  if (rc) return rc;
+  if ((rc = set_device(ctx))) return rc;
+  return enqueue_scene(ctx, scene, origins, directions, n, side, near_dist, far_dist, face_index, instance, distance,
+                       point, barycoord, stream ? static_cast<hipStream_t>(stream) : ctx->stream);
+}
+
+extern "C" int mbvh_scene_raycast_first(mbvh_ctx* ctx, const mbvh_scene* scene, const float* origins,
+                                        const float* directions, uint64_t n, int32_t side, float near_dist,
+                                        float far_dist, uint32_t* face_index, uint32_t* instance, float* distance,
+                                        float* point, float* barycoord) {
+  clear_error();
+  const char* fn = "mbvh_scene_raycast_first";
+  int rc = check_scene_call(fn, ctx, scene, side, n, origins, directions, face_index);
+  if (rc) return rc;
+  if (n == 0) return MBVH_OK;
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t s = ctx->stream;
+  const size_t v3 = 3 * n * sizeof(float), s1 = n * sizeof(float);
+  void *dO, *dD, *dF, *dI = nullptr, *dT = nullptr, *dP = nullptr, *dB = nullptr;
+  if ((rc = stage_in(ctx, ctx->staging[0], origins, v3, &dO)) || (rc = stage_in(ctx, ctx->staging[1], directions, v3, &dD)) ||
+      (rc = stage_out(ctx->staging[2], face_index, s1, &dF)) || (rc = stage_out(ctx->staging[3], instance, s1, &dI)) ||
+      (rc = stage_out(ctx->staging[4], distance, s1, &dT)) || (rc = stage_out(ctx->staging[5], point, v3, &dP)) ||
+      (rc = stage_out(ctx->staging[6], barycoord, v3, &dB)))
+    return rc;
+  if ((rc = enqueue_scene(ctx, scene, static_cast<const float*>(dO), static_cast<const float*>(dD), n, side, near_dist,
+                          far_dist, static_cast<uint32_t*>(dF), static_cast<uint32_t*>(dI), static_cast<float*>(dT),
+                          static_cast<float*>(dP), static_cast<float*>(dB), s)))
+    return rc;
+  MBVH_HIP(hipMemcpyAsync(face_index, dF, s1, hipMemcpyDeviceToHost, s));
+  if (instance) MBVH_HIP(hipMemcpyAsync(instance, dI, s1, hipMemcpyDeviceToHost, s));
+  if (distance) MBVH_HIP(hipMemcpyAsync(distance, dT, s1, hipMemcpyDeviceToHost, s));
+  if (point) MBVH_HIP(hipMemcpyAsync(point, dP, v3, hipMemcpyDeviceToHost, s));
+  if (barycoord) MBVH_HIP(hipMemcpyAsync(barycoord, dB, v3, hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipStreamSynchronize(s));
+  return check_no_overflow(face_index, n, fn);
+}

@@ -218,6 +218,30 @@ struct BatchArgs {
   StackScratch stack;
 };
 
+// Scene: closest first hit over instances, one launch (TLAS over the instances' world boxes).
+struct SceneArgs {
+  const MeshView* meshes;
+  uint32_t n_meshes;
+  const uint32_t* tlas;            // reference-format nodes (8 words) over the instances' world boxes
+  const uint32_t* tlas_inst;       // instance at each TLAS primitive slot
+  const float* world;              // 16 floats per instance (local -> world, column-major)
+  const float* inverse;            // 16 floats per instance (world -> local)
+  const uint32_t* instance_geometry;
+  const float* origins;            // world rays
+  const float* directions;
+  uint64_t n;
+  uint32_t side_mask;
+  float near_dist;
+  float far_dist;
+  uint32_t* out_face;
+  uint32_t* out_instance;          // instance of the hit (MBVH_NO_HIT on a miss)
+  float* out_dist;                 // world distance
+  float* out_point;                // world point
+  float* out_bary;
+  StackScratch stack;
+};
+hipError_t launch_scene_raycast_first(const SceneArgs& a, uint64_t threads, hipStream_t s);
+
 hipError_t launch_instance_rays(const InstanceRayArgs& a, hipStream_t s);
 hipError_t launch_instance_hits(const InstanceHitArgs& a, hipStream_t s);
 // lanes 1: one lane per ray (stack_lds 8 or 16 LDS stack entries per lane); lanes 4: four lanes

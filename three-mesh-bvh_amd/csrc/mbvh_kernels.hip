@@ -1572,6 +1572,119 @@ __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
   }
 }
 
+// ---- scenes: closest first hit over instances (TLAS over instance world boxes) ------------------
+
+constexpr int kSceneStack = 25;  // TLAS stack entries per lane: the TLAS is at most 24 levels deep
+
+// Entry distance of a world ray into a box of the TLAS (near plane respected).
+__device__ __forceinline__ bool box_entry(f3 o, f3 inv, const float* b, float near_dist, float& t_enter) {
+  const float tx0 = (b[0] - o.x) * inv.x, tx1 = (b[3] - o.x) * inv.x;
+  const float ty0 = (b[1] - o.y) * inv.y, ty1 = (b[4] - o.y) * inv.y;
+  const float tz0 = (b[2] - o.z) * inv.z, tz1 = (b[5] - o.z) * inv.z;
+  const float tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+  const float tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+  t_enter = tmin;
+  return tmin <= tmax && tmax >= near_dist;
+}
+
+// One lane per ray. The TLAS (reference node format over the instances' world boxes, expanded
+// outward when built) is walked near child first and pruned by the best world distance so far
+// (with a 2^-16 relative margin: box distances and hit distances round differently); each
+// instance leaf brings the ray into the instance's frame and traces its mesh exactly as the batched
+// per-instance launch does (near / far scaled by the instance, hit point and distance back in world
+// space), so every instance's hit is that launch's hit. The closest world distance wins; equal
+// distances keep the lowest instance index, the order a stable sort of the reference's
+// per-instance hits (acceleratedBatchedMeshRaycast) gives.
+__global__ __launch_bounds__(kBlock) void k_scene_raycast_first(SceneArgs A) {
+  __shared__ uint2 lds_stack[8 * kBlock];
+  __shared__ uint32_t tstack[kSceneStack * kBlock];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + tid;
+  StackL<8> st;
+  st.lds = lds_stack;
+  st.ovf = A.stack.ovf;
+  st.ovf_stride = A.stack.ovf_stride;
+  st.gtid = gtid;
+  st.lane = tid;
+  st.capacity = 8 + static_cast<int>(A.stack.ovf_entries);
+  const float qnan = __int_as_float(0x7fc00000);
+  const float margin = 1.0f + 1.0f / 65536.0f;
+  for (uint64_t ray = gtid; ray < A.n; ray += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    const f3 ow = mk(A.origins[3 * ray], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
+    const f3 dw = mk(A.directions[3 * ray], A.directions[3 * ray + 1], A.directions[3 * ray + 2]);
+    const f3 iw = mk(1.0f / dw.x, 1.0f / dw.y, 1.0f / dw.z);
+    FirstHit best;
+    best.face = kNoHit;
+    best.dist = kInf;
+    best.p = mk(qnan, qnan, qnan);
+    best.bary = best.p;
+    uint32_t best_inst = kNoHit;
+    bool overflow = false;
+    int sp = 1;
+    tstack[tid] = 0u;
+    while (sp > 0 && !overflow) {
+      sp--;
+      const uint32_t node = tstack[sp * kBlock + tid];
+      const uint32_t* nw = A.tlas + 8ull * node;
+      float te;
+      if (!box_entry(ow, iw, reinterpret_cast<const float*>(nw), A.near_dist, te)) continue;
+      if (te > fminf(best.dist, A.far_dist) * margin + 1e-30f) continue;
+      const uint32_t w7 = nw[7];
+      if ((w7 >> 16) == 0xFFFFu) {
+        const uint32_t off = nw[6], cnt = w7 & 0xFFFFu;
+        for (uint32_t s = off; s < off + cnt; s++) {
+          const uint32_t inst = A.tlas_inst[s];
+          const uint32_t geom = A.instance_geometry[inst];
+          const float* inv = A.inverse + 16ull * inst;
+          const f3 o = apply_matrix4(inv, ow);
+          const f3 d = transform_direction(inv, dw);
+          const float sc = instance_scale(A.world + 16ull * inst, d);
+          FirstHit h = trace_first(A.meshes[geom], st, tid, o, d, A.side_mask, A.near_dist / sc, A.far_dist / sc);
+          if (h.face == kOverflow) {
+            overflow = true;
+            break;
+          }
+          if (h.face != kNoHit) {
+            h.p = apply_matrix4(A.world + 16ull * inst, h.p);
+            h.dist = world_distance(h.p, ow);
+            if (h.dist < best.dist || (h.dist == best.dist && inst < best_inst)) {
+              best = h;
+              best_inst = inst;
+            }
+          }
+        }
+      } else {
+        if (sp + 2 > kSceneStack) {
+          overflow = true;
+          break;
+        }
+        const uint32_t left = node + 1, right = node + nw[6];
+        const bool l2r = comp(dw, static_cast<int>(w7)) >= 0.0f;
+        tstack[sp * kBlock + tid] = l2r ? right : left;  // far child below the near one
+        tstack[(sp + 1) * kBlock + tid] = l2r ? left : right;
+        sp += 2;
+      }
+    }
+    if (overflow) {
+      best.face = kOverflow;
+      best_inst = kNoHit;
+    }
+    A.out_face[ray] = best.face;
+    if (A.out_instance) A.out_instance[ray] = best_inst;
+    if (A.out_dist) A.out_dist[ray] = best.dist;
+    if (A.out_point) {
+      A.out_point[3 * ray + 0] = best.p.x;
+      A.out_point[3 * ray + 1] = best.p.y;
+      A.out_point[3 * ray + 2] = best.p.z;
+    }
+    if (A.out_bary) {
+      A.out_bary[3 * ray + 0] = best.bary.x;
+      A.out_bary[3 * ray + 1] = best.bary.y;
+      A.out_bary[3 * ray + 2] = best.bary.z;
+    }
+  }
+}
+
 __global__ void k_apply_side_mask(float4* tris, uint64_t n_slots, const uint8_t* mask) {
   const uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n_slots) return;
@@ -1622,6 +1735,12 @@ hipError_t launch_batch_raycast_first(const BatchArgs& a, uint32_t stack_lds, ui
     hipLaunchKernelGGL(k_batch_raycast_first<8>, dim3(blocks), dim3(kBlock), 0, s, a);
   else
     hipLaunchKernelGGL(k_batch_raycast_first<16>, dim3(blocks), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_scene_raycast_first(const SceneArgs& a, uint64_t threads, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scene_raycast_first, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 

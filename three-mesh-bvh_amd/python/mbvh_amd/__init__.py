@@ -37,7 +37,8 @@ EXPORTED_SYMBOLS = (
     "mbvh_raycast", "mbvh_raycast_fetch", "mbvh_raycast_count_d", "mbvh_raycast_fill_d",
     "mbvh_order_workspace_bytes", "mbvh_order_points_d",
     "mbvh_batch_create", "mbvh_batch_free", "mbvh_batch_workspace_bytes", "mbvh_batch_raycast_first_d",
-    "mbvh_batch_raycast_first",
+    "mbvh_batch_raycast_first", "mbvh_scene_create", "mbvh_scene_free", "mbvh_scene_raycast_first",
+    "mbvh_scene_raycast_first_d",
     "mbvh_batch_autotune", "mbvh_batch_graph_create", "mbvh_batch_graph_launch", "mbvh_batch_graph_free",
     "mbvh_instance_rays_d", "mbvh_instance_hits_d",
     "mbvh_grad_workspace_bytes", "mbvh_raycast_first_bwd", "mbvh_raycast_first_bwd_d",
@@ -122,6 +123,10 @@ def library() -> C.CDLL:
                                           vp, u64, C.POINTER(BatchTuning)]),
         "mbvh_batch_graph_create": (C.c_int, [vp, vp, vp, vp, vp, u32, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp,
                                               vp, vp, u64, C.POINTER(BatchTuning), C.POINTER(vp)]),
+        "mbvh_scene_create": (C.c_int, [vp, C.POINTER(vp), u32, vp, vp, vp, vp, u32, C.POINTER(vp)]),
+        "mbvh_scene_free": (None, [vp]),
+        "mbvh_scene_raycast_first_d": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp, vp]),
+        "mbvh_scene_raycast_first": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp]),
         "mbvh_batch_graph_launch": (C.c_int, [vp, vp]),
         "mbvh_batch_graph_free": (None, [vp]),
         "mbvh_instance_rays_d": (C.c_int, [vp, vp, u32, vp, vp, vp, u64, vp, vp, vp]),
@@ -259,6 +264,10 @@ class Context:
                                           nodes_capacity, counts, n_trees, indirect_ptr or None, indirect_capacity,
                                           stream or None))
         return [int(c) for c in counts]
+
+    def scene(self, meshes: Sequence["Mesh"], matrix_world: np.ndarray, matrix_world_inverse: np.ndarray,
+              instance_geometry: np.ndarray, visible: Optional[np.ndarray] = None) -> "Scene":
+        return Scene(self, meshes, matrix_world, matrix_world_inverse, instance_geometry, visible)
 
     def upload(self, tree: Tree, positions: np.ndarray, index: Optional[np.ndarray],
                stride: int = 3, offset: int = 0) -> "Mesh":
@@ -585,6 +594,61 @@ class Batch:
         h = C.c_void_p()
         _check(self.ctx._lib.mbvh_batch_graph_create(*self._args(p), C.byref(tuning) if tuning else None, C.byref(h)))
         return BatchGraph(self, h)
+
+
+class Scene:
+    """Instances of uploaded meshes under a top-level tree (mbvh_scene): closest hit over all
+    instances in one launch."""
+
+    def __init__(self, ctx: Context, meshes: Sequence["Mesh"], matrix_world, matrix_world_inverse,
+                 instance_geometry, visible=None):
+        self.ctx = ctx
+        self.meshes = list(meshes)  # the meshes must outlive the scene
+        arr = (C.c_void_p * len(self.meshes))(*[m.handle for m in self.meshes])
+        mw = np.ascontiguousarray(matrix_world, np.float32).reshape(-1, 16)
+        mi = np.ascontiguousarray(matrix_world_inverse, np.float32).reshape(-1, 16)
+        ig = np.ascontiguousarray(instance_geometry, np.uint32).reshape(-1)
+        vis = None if visible is None else np.ascontiguousarray(visible, np.uint8).reshape(-1)
+        if mw.shape != mi.shape or mw.shape[0] != ig.shape[0] or (vis is not None and vis.shape[0] != ig.shape[0]):
+            raise ValueError("inconsistent instance inputs")
+        h = C.c_void_p()
+        _check(ctx._lib.mbvh_scene_create(ctx.handle, arr, len(self.meshes), _ptr(mw), _ptr(mi), _ptr(ig),
+                                          _ptr(vis), ig.shape[0], C.byref(h)))
+        self.handle = h
+
+    def free(self) -> None:
+        if self.handle:
+            self.ctx._lib.mbvh_scene_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def raycast_first(self, origins: np.ndarray, directions: np.ndarray, side: int = FRONT_SIDE,
+                      near: float = 0.0, far: float = float("inf")):
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
+        if o.shape != d.shape:
+            raise ValueError("origins and directions differ in shape")
+        n = o.shape[0]
+        out = {"face": np.empty(n, np.uint32), "instance": np.empty(n, np.uint32),
+               "distance": np.empty(n, np.float32), "point": np.empty((n, 3), np.float32),
+               "barycoord": np.empty((n, 3), np.float32)}
+        _check(self.ctx._lib.mbvh_scene_raycast_first(
+            self.ctx.handle, self.handle, _ptr(o), _ptr(d), n, side, near, far, _ptr(out["face"]),
+            _ptr(out["instance"]), _ptr(out["distance"]), _ptr(out["point"]), _ptr(out["barycoord"])))
+        return out
+
+    def raycast_first_d(self, origins_ptr: int, dirs_ptr: int, n: int, face_ptr: int, instance_ptr: int = 0,
+                        dist_ptr: int = 0, point_ptr: int = 0, bary_ptr: int = 0, side: int = FRONT_SIDE,
+                        near: float = 0.0, far: float = float("inf"), stream: int = 0) -> None:
+        v = lambda x: C.c_void_p(x or None)
+        _check(self.ctx._lib.mbvh_scene_raycast_first_d(
+            self.ctx.handle, self.handle, v(origins_ptr), v(dirs_ptr), n, side, near, far, v(face_ptr),
+            v(instance_ptr), v(dist_ptr), v(point_ptr), v(bary_ptr), v(stream)))
 
 
 class BatchGraph:

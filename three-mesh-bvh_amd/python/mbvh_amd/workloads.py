@@ -174,3 +174,55 @@ def c5_batched(n_geometries: int = 512, n_rays: int = 1 << 22, tri_min: int = 10
     d = tgt - o32.astype(np.float64)
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     return BatchedWorkload(geos, inst_geom, world32, inv32, o32, d.astype(np.float32), ray_inst)
+
+
+@dataclass
+class SceneWorkload:
+    """Instances of meshes in one scene and world rays through it (closest hit over all instances)."""
+    matrix_world: np.ndarray       # float32 (n_instances, 16), column-major
+    matrix_world_inverse: np.ndarray
+    instance_geometry: np.ndarray  # uint32 (n_instances,)
+    origins: np.ndarray            # float32 (n, 3)
+    directions: np.ndarray         # float32 (n, 3), unit
+
+
+def scene_rays(geometries: list, n_instances: int, n_rays: int, seed: int = 11, spread: float = 6.0,
+               non_uniform: float = 0.0) -> SceneWorkload:
+    """Instances of `geometries` (random mesh, rotation, uniform scale in [0.5, 2] — or, for a fraction
+    `non_uniform` of them, a different scale per axis — translation in a cube of half-width
+    spread * cbrt(n_instances)); each ray starts on a sphere of 1.25x the scene's bounding radius and
+    aims at a uniform point of a random instance's world box, so it crosses other instances' boxes."""
+    rng = np.random.default_rng(seed)
+    n_geo = len(geometries)
+    geom = rng.integers(0, n_geo, n_instances).astype(np.uint32)
+    span = spread * np.cbrt(n_instances)
+    world = np.zeros((n_instances, 16), np.float64)
+    for k in range(n_instances):
+        q = rng.standard_normal(4)
+        q /= np.linalg.norm(q)
+        s = np.full(3, rng.uniform(0.5, 2.0))
+        if rng.random() < non_uniform:
+            s = rng.uniform(0.5, 2.0, 3)
+        m = np.eye(4)
+        m[:3, :3] = _rotation(q) @ np.diag(s)
+        m[:3, 3] = rng.uniform(-span, span, 3)
+        world[k] = m.T.reshape(-1)
+    world32 = world.astype(np.float32)
+    inv32 = np.stack([np.linalg.inv(world32[k].astype(np.float64).reshape(4, 4).T).T.reshape(-1)
+                      for k in range(n_instances)]).astype(np.float32)
+    boxes = np.empty((n_instances, 2, 3))
+    for k in range(n_instances):
+        lo, hi = geometries[geom[k]].bounds()
+        corners = np.array([[x, y, z] for x in (lo[0], hi[0]) for y in (lo[1], hi[1]) for z in (lo[2], hi[2])])
+        mw = world32[k].astype(np.float64).reshape(4, 4).T
+        wc = corners @ mw[:3, :3].T + mw[:3, 3]
+        boxes[k, 0], boxes[k, 1] = wc.min(axis=0), wc.max(axis=0)
+    slo, shi = boxes[:, 0].min(axis=0), boxes[:, 1].max(axis=0)
+    c = (slo + shi) / 2
+    r = np.linalg.norm(shi - slo) / 2
+    o = (c + 1.25 * r * unit_sphere(rng, n_rays)).astype(np.float32)
+    pick = rng.integers(0, n_instances, n_rays)
+    tgt = boxes[pick, 0] + (boxes[pick, 1] - boxes[pick, 0]) * rng.random((n_rays, 3))
+    d = tgt - o.astype(np.float64)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return SceneWorkload(world32, inv32, geom, o, d.astype(np.float32))
