@@ -51,6 +51,7 @@ def parse():
     ap.add_argument("--c4-grid", type=int, default=256)
     ap.add_argument("--c5-rays", type=int, default=1 << 22)
     ap.add_argument("--c5-geometries", type=int, default=512)
+    ap.add_argument("--scene-rays", type=int, default=1 << 20)
     ap.add_argument("--traffic-calibration", action="store_true",
                     help="also run one 256 MiB device copy, the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiling run")
@@ -432,10 +433,107 @@ def bench_batched(torch, M, W, ctx, stream, args):
          "identical_to_loop": bool(same), "tuning": tuning.as_dict()}
     log(f"C5 {r}")
     graph.free()
+    scene = bench_scene(torch, M, W, ctx, stream, args, meshes, batch, geos, idxs, trees, w)
+    log(f"scene {scene}")
     batch.free()
     for m in meshes:
         m.free()
-    return {"batched_raycast_first": r}
+    return {"batched_raycast_first": r, "scene_raycast_first": scene}
+
+
+def bench_scene(torch, M, W, ctx, stream, args, meshes, batch, geos, idxs, trees, w):
+    """Scene raycast (closest hit over all instances, one launch over a top-level tree) on the C5 meshes
+    and placements, against the batched launch on every (ray, instance) pair reduced to the closest
+    (what a loop over the instances computes), and against the FP64 oracle's per-instance loop on the
+    host (the reference's per-instance traversal, one call per instance)."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    sptr = stream.cuda_stream
+    rng = np.random.default_rng(17)
+    n = args.scene_rays
+    o, d = W.rays_through_instances(geos, w.matrix_world, w.instance_geometry, n, rng)
+    sc = ctx.scene(meshes, w.matrix_world, w.matrix_world_inverse, w.instance_geometry)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    do, dd = t(o), t(d)
+    face = torch.empty(n, dtype=torch.int32, device=dev)
+    inst = torch.empty(n, dtype=torch.int32, device=dev)
+    dist = torch.empty(n, device=dev)
+    point = torch.empty((n, 3), device=dev)
+    bary = torch.empty((n, 3), device=dev)
+    torch.cuda.synchronize()
+
+    def scene_d():
+        sc.raycast_first_d(do.data_ptr(), dd.data_ptr(), n, face.data_ptr(), inst.data_ptr(), dist.data_ptr(),
+                           point.data_ptr(), bary.data_ptr(), stream=sptr)
+
+    s_ms = event_ms(torch, stream, scene_d, 10)
+    # every (ray, instance) pair of a subsample through the batched launch, then the closest per ray
+    n_inst = w.matrix_world.shape[0]
+    m = max(1, min(n, (1 << 21) // n_inst))
+    ri = np.repeat(np.arange(n_inst, dtype=np.uint32), m)
+    P = {"world": t(w.matrix_world), "inverse": t(w.matrix_world_inverse),
+         "instance_geometry": t(w.instance_geometry.view(np.int32)), "origins": t(np.tile(o[:m], (n_inst, 1))),
+         "directions": t(np.tile(d[:m], (n_inst, 1))), "ray_instance": t(ri.view(np.int32))}
+    npair = n_inst * m
+    wsb = batch.workspace_bytes(npair)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    pout = {"face": torch.empty(npair, dtype=torch.int32, device=dev), "distance": torch.empty(npair, device=dev),
+            "point": torch.empty((npair, 3), device=dev), "barycoord": torch.empty((npair, 3), device=dev)}
+    p = {k: v.data_ptr() for k, v in P.items()}
+    p.update({k: v.data_ptr() for k, v in pout.items()})
+    p.update(workspace=ws.data_ptr(), workspace_bytes=wsb, n_instances=n_inst, n=npair)
+    sel = {}
+
+    def pairwise():
+        batch.raycast_first_d(p, stream=sptr)
+        dm = pout["distance"].view(n_inst, m)
+        miss = pout["face"].view(n_inst, m) == -1
+        dm = torch.where(miss, torch.full_like(dm, float("inf")), dm)
+        sel["best"] = torch.argmin(dm, dim=0)  # first minimum: the lowest instance on ties
+        sel["dm"] = dm
+
+    pw_ms = event_ms(torch, stream, pairwise, 3)
+    best = sel["best"].cpu().numpy()
+    fm = pout["face"].view(n_inst, m).cpu().numpy().view(np.uint32)
+    dmn = sel["dm"].cpu().numpy()
+    cols = np.arange(m)
+    pf = fm[best, cols]
+    hit = pf != M.NO_HIT
+    sf = face[:m].cpu().numpy().view(np.uint32)
+    si = inst[:m].cpu().numpy().view(np.uint32)
+    sdist = dist[:m].cpu().numpy()
+    same = bool(np.array_equal(sf, np.where(hit, pf, M.NO_HIT)) and np.array_equal(si[hit], best[hit].astype(np.uint32))
+                and np.array_equal(sdist[hit].view(np.uint32), dmn[best, cols][hit].astype(np.float32).view(np.uint32)))
+    pw_per_ray_ms = pw_ms / m
+    res = {"workload": f"closest first hit over {n_inst} instances (the C5 meshes and placements, {len(meshes)} "
+                       f"meshes, {sum(g.tri_count for g in geos)} tris), {n} world rays from outside the scene towards "
+                       "random instances",
+           "scene_ms": round(s_ms, 3), "scene_mrays_per_s": round(n / s_ms / 1e3, 2),
+           "hit_fraction": round(float((face.cpu().numpy().view(np.uint32) != M.NO_HIT).mean()), 3),
+           "pairwise_ms_for_all_rays": round(pw_per_ray_ms * n, 1),
+           "speedup_vs_pairwise": round(pw_per_ray_ms * n / s_ms, 1),
+           "pairwise_sample_rays": m, "identical_to_pairwise": same,
+           "note": "pairwise = the batched launch on every (ray, instance) pair of the first rays, then the "
+                   "closest per ray (torch argmin), scaled to all rays"}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        oms = [O.OracleMesh(geos[k].positions, idxs[k], trees[k].roots) for k in range(len(geos))]
+        ms = min(n, 256)
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        for i in range(n_inst):
+            mi = w.matrix_world_inverse[i].astype(np.float64).reshape(4, 4).T
+            lo = (o[:ms] @ mi[:3, :3].T + mi[:3, 3]).astype(np.float32)
+            ld = (d[:ms] @ mi[:3, :3].T).astype(np.float32)
+            oms[w.instance_geometry[i]].raycast_first(lo, ld, threads=threads)
+        cpu_s = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(ms / cpu_s / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+                               "sample": f"{ms} rays x {n_inst} instances: the FP64 C oracle's restatement of the "
+                                         "reference raycastFirst traversal, one call per instance over the sample "
+                                         "(the per-instance loop of acceleratedBatchedMeshRaycast)"}
+    sc.free()
+    return res
 
 
 def bench_host_api(mesh, mesh1, w, w1, n, nr):

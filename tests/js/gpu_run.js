@@ -185,6 +185,33 @@ for (let i = 0; i < Math.min(K, 128); i++) {
   }
 }
 assert.ok(batchedChecked > 64 && presenceDiffers <= 2, `batched BatchedMesh hits ${batchedChecked}, ${presenceDiffers} differ`);
+// SceneBVH over the BatchedMesh's instances: one launch gives the hit firstHitOnly's per-instance hits
+// put first after the Raycaster's distance sort (same device frame change, so the same bits)
+{
+  const scene = api.SceneBVH.fromBatchedMesh(batched);
+  const nS = Math.min(K, 128);
+  const many = scene.raycastFirstBatch(origins.subarray(0, 3 * nS), dirs.subarray(0, 3 * nS));
+  let sceneHits = 0;
+  for (let i = 0; i < nS; i++) {
+    const ray = new Raycaster([origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]], [dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]]);
+    ray.firstHitOnly = true;
+    const first = ray.intersectObject(batched)[0];
+    const hit = scene.raycastFirst(ray.ray, batched.material);
+    if (!first) {
+      assert.strictEqual(hit, null, `scene ray ${i}`);
+      assert.strictEqual(many.faceIndex[i], 0xFFFFFFFF);
+      continue;
+    }
+    assert.ok(hit && hit.batchId === first.batchId && hit.faceIndex === first.faceIndex, `scene ray ${i}`);
+    assert.strictEqual(Math.fround(hit.distance), Math.fround(first.distance));
+    assert.strictEqual(hit.object, batched);
+    assert.strictEqual(many.instance[i], first.batchId);
+    assert.strictEqual(many.faceIndex[i], first.faceIndex);
+    sceneHits++;
+  }
+  assert.ok(sceneHits > 32, `scene hits ${sceneHits}`);
+  scene.dispose();
+}
 api.disposeBatchedBoundsTree.call(Object.assign(batched, { boundsTrees: [bvh] }), -1);
 assert.strictEqual(batched._mbvhBatch, null);
 

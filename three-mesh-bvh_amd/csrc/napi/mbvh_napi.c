@@ -29,6 +29,7 @@ typedef struct MeshHandle {
 
 typedef struct BatchHandle {
   mbvh_batch* batch;
+  mbvh_scene* scene;    /* scenes use the same handle (one of batch / scene is set) */
   int device;
   uint32_t n_meshes;
   MeshHandle** meshes;  /* kept alive (and unfreeable) while the batch exists */
@@ -355,7 +356,9 @@ static napi_value js_free_mesh(napi_env env, napi_callback_info info) {
 static void release_batch(napi_env env, BatchHandle* b) {
   if (!b) return;
   if (b->batch) mbvh_batch_free(b->batch);
+  if (b->scene) mbvh_scene_free(b->scene);
   b->batch = NULL;
+  b->scene = NULL;
   for (uint32_t i = 0; i < b->n_meshes; i++) {
     if (b->meshes && b->meshes[i]) b->meshes[i]->batch_refs--;
     if (b->refs && b->refs[i]) napi_delete_reference(env, b->refs[i]);
@@ -374,6 +377,47 @@ static void finalize_batch(napi_env env, void* data, void* hint) {
   free(b);
 }
 
+/* A batch / scene handle holding n uploaded meshes of `device` (kept alive and unfreeable meanwhile);
+ * raw receives their C handles. Returns an error message, or NULL. */
+static const char* hold_meshes(napi_env env, napi_value arr, uint32_t n, int32_t device, BatchHandle** out,
+                               mbvh_mesh*** raw_out) {
+  BatchHandle* b = (BatchHandle*)calloc(1, sizeof(BatchHandle));
+  mbvh_mesh** raw = (mbvh_mesh**)calloc(n, sizeof(mbvh_mesh*));
+  if (!b || !raw) {
+    free(b);
+    free(raw);
+    return "out of memory";
+  }
+  b->device = device;
+  b->meshes = (MeshHandle**)calloc(n, sizeof(MeshHandle*));
+  b->refs = (napi_ref*)calloc(n, sizeof(napi_ref));
+  if (!b->meshes || !b->refs) {
+    free(raw);
+    release_batch(env, b);
+    free(b);
+    return "out of memory";
+  }
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value v;
+    napi_get_element(env, arr, i, &v);
+    MeshHandle* h = get_mesh(env, v);
+    if (!h || h->device != device) {
+      free(raw);
+      release_batch(env, b);
+      free(b);
+      return "every mesh must be an uploaded mesh of this device";
+    }
+    b->n_meshes = i + 1;
+    b->meshes[i] = h;
+    h->batch_refs++;
+    napi_create_reference(env, v, 1, &b->refs[i]);
+    raw[i] = h->mesh;
+  }
+  *out = b;
+  *raw_out = raw;
+  return NULL;
+}
+
 /* batchCreate(device, [mesh, ...]) -> batch (external) */
 static napi_value js_batch_create(napi_env env, napi_callback_info info) {
   napi_value argv[2];
@@ -386,38 +430,10 @@ static napi_value js_batch_create(napi_env env, napi_callback_info info) {
   uint32_t n = 0;
   napi_get_array_length(env, argv[1], &n);
   if (n == 0) return throw_msg(env, "batchCreate: no meshes");
-  BatchHandle* b = (BatchHandle*)calloc(1, sizeof(BatchHandle));
-  mbvh_mesh** raw = (mbvh_mesh**)calloc(n, sizeof(mbvh_mesh*));
-  if (!b || !raw) {
-    free(b);
-    free(raw);
-    return throw_msg(env, "batchCreate: out of memory");
-  }
-  b->device = device;
-  b->meshes = (MeshHandle**)calloc(n, sizeof(MeshHandle*));
-  b->refs = (napi_ref*)calloc(n, sizeof(napi_ref));
-  if (!b->meshes || !b->refs) {
-    free(raw);
-    release_batch(env, b);
-    free(b);
-    return throw_msg(env, "batchCreate: out of memory");
-  }
-  for (uint32_t i = 0; i < n; i++) {
-    napi_value v;
-    napi_get_element(env, argv[1], i, &v);
-    MeshHandle* h = get_mesh(env, v);
-    if (!h || h->device != device) {
-      free(raw);
-      release_batch(env, b);
-      free(b);
-      return throw_msg(env, "batchCreate: every mesh must be an uploaded mesh of this device");
-    }
-    b->n_meshes = i + 1;
-    b->meshes[i] = h;
-    h->batch_refs++;
-    napi_create_reference(env, v, 1, &b->refs[i]);
-    raw[i] = h->mesh;
-  }
+  BatchHandle* b = NULL;
+  mbvh_mesh** raw = NULL;
+  const char* err = hold_meshes(env, argv[1], n, device, &b, &raw);
+  if (err) return throw_msg(env, err);
   mbvh_ctx* ctx = NULL;
   int rc = context_for(env, device, &ctx);
   if (!rc) rc = mbvh_batch_create(ctx, (const mbvh_mesh* const*)raw, n, &b->batch);
@@ -477,6 +493,78 @@ static napi_value js_batch_raycast_first(napi_env env, napi_callback_info info) 
                                   (const uint32_t*)ri.data, n, side, to_float(nr), to_float(fr), (uint32_t*)face,
                                   (float*)dist, (float*)pt, (float*)bc);
   if (rc) return throw_status(env, rc, "mbvh_batch_raycast_first");
+  return NULL;
+}
+
+/* sceneCreate(device, [mesh, ...], matrixWorld, matrixWorldInverse, instanceGeometry, visible|null) -> scene */
+static napi_value js_scene_create(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  get_args(env, info, 6, argv);
+  int32_t device;
+  if (!get_i32(env, argv[0], &device)) return throw_msg(env, "sceneCreate: device must be a number");
+  bool is_array = false;
+  if (napi_is_array(env, argv[1], &is_array) != napi_ok || !is_array)
+    return throw_msg(env, "sceneCreate: meshes must be an array");
+  uint32_t n = 0;
+  napi_get_array_length(env, argv[1], &n);
+  if (n == 0) return throw_msg(env, "sceneCreate: no meshes");
+  View mw, mi, ig, vis;
+  if (!get_view(env, argv[2], &mw) || !get_view(env, argv[3], &mi) || !get_view(env, argv[4], &ig) ||
+      mw.type != napi_float32_array || mi.type != napi_float32_array || ig.type != napi_uint32_array ||
+      mw.length != 16 * ig.length || mi.length != mw.length)
+    return throw_msg(env, "sceneCreate: matrices must be Float32Arrays of 16 per instance, instanceGeometry a Uint32Array");
+  const int has_vis = !is_nullish(env, argv[5]);
+  if (has_vis && (!get_view(env, argv[5], &vis) || vis.elt != 1 || vis.length != ig.length))
+    return throw_msg(env, "sceneCreate: visible must be a Uint8Array of one entry per instance");
+  BatchHandle* b = NULL;
+  mbvh_mesh** raw = NULL;
+  const char* err = hold_meshes(env, argv[1], n, device, &b, &raw);
+  if (err) return throw_msg(env, err);
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, device, &ctx);
+  if (!rc)
+    rc = mbvh_scene_create(ctx, (const mbvh_mesh* const*)raw, n, (const float*)mw.data, (const float*)mi.data,
+                           (const uint32_t*)ig.data, has_vis ? (const uint8_t*)vis.data : NULL, (uint32_t)ig.length,
+                           &b->scene);
+  free(raw);
+  if (rc) {
+    release_batch(env, b);
+    free(b);
+    return throw_status(env, rc, "mbvh_scene_create");
+  }
+  napi_value out;
+  napi_create_external(env, b, finalize_batch, NULL, &out);
+  return out;
+}
+
+/* sceneRaycastFirst(scene, origins, dirs, side, near, far, outFace, outInstance, outDist, outPoint, outBary) */
+static napi_value js_scene_raycast_first(napi_env env, napi_callback_info info) {
+  napi_value argv[11];
+  get_args(env, info, 11, argv);
+  void* p = NULL;
+  if (napi_get_value_external(env, argv[0], &p) != napi_ok || !p || !((BatchHandle*)p)->scene)
+    return throw_msg(env, "sceneRaycastFirst: invalid scene");
+  BatchHandle* b = (BatchHandle*)p;
+  View o, d;
+  if (!get_view(env, argv[1], &o) || !get_view(env, argv[2], &d) || o.type != napi_float32_array ||
+      d.type != napi_float32_array || o.length != d.length || o.length % 3)
+    return throw_msg(env, "sceneRaycastFirst: origins / directions must be Float32Arrays of 3n floats");
+  const size_t n = o.length / 3;
+  int32_t side;
+  double nr, fr;
+  if (!get_i32(env, argv[3], &side) || !get_f64(env, argv[4], &nr) || !get_f64(env, argv[5], &fr))
+    return throw_msg(env, "sceneRaycastFirst: side / near / far must be numbers");
+  void *face, *inst, *dist, *pt, *bc;
+  if (!out_view(env, argv[6], napi_uint32_array, n, &face) || !face ||
+      !out_view(env, argv[7], napi_uint32_array, n, &inst) || !out_view(env, argv[8], napi_float32_array, n, &dist) ||
+      !out_view(env, argv[9], napi_float32_array, 3 * n, &pt) || !out_view(env, argv[10], napi_float32_array, 3 * n, &bc))
+    return throw_msg(env, "sceneRaycastFirst: output arrays have the wrong type or length");
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, b->device, &ctx);
+  if (!rc)
+    rc = mbvh_scene_raycast_first(ctx, b->scene, (const float*)o.data, (const float*)d.data, n, side, to_float(nr),
+                                  to_float(fr), (uint32_t*)face, (uint32_t*)inst, (float*)dist, (float*)pt, (float*)bc);
+  if (rc) return throw_status(env, rc, "mbvh_scene_raycast_first");
   return NULL;
 }
 
@@ -807,6 +895,9 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("batchCreate", js_batch_create);
   EXPORT("freeBatch", js_free_batch);
   EXPORT("batchRaycastFirst", js_batch_raycast_first);
+  EXPORT("sceneCreate", js_scene_create);
+  EXPORT("freeScene", js_free_batch);
+  EXPORT("sceneRaycastFirst", js_scene_raycast_first);
   EXPORT("closestPointBackward", js_closest_point_bwd);
   EXPORT("refitMesh", js_refit);
   EXPORT("intersectsSphere", js_intersects_sphere);

@@ -210,11 +210,20 @@ def scene_rays(geometries: list, n_instances: int, n_rays: int, seed: int = 11, 
     world32 = world.astype(np.float32)
     inv32 = np.stack([np.linalg.inv(world32[k].astype(np.float64).reshape(4, 4).T).T.reshape(-1)
                       for k in range(n_instances)]).astype(np.float32)
+    o, d = rays_through_instances(geometries, world32, geom, n_rays, rng)
+    return SceneWorkload(world32, inv32, geom, o, d)
+
+
+def rays_through_instances(geometries: list, matrix_world: np.ndarray, instance_geometry: np.ndarray, n_rays: int,
+                           rng: np.random.Generator):
+    """World rays from a sphere of 1.25x the instances' bounding radius towards uniform points of
+    random instances' world boxes."""
+    n_instances = matrix_world.shape[0]
     boxes = np.empty((n_instances, 2, 3))
     for k in range(n_instances):
-        lo, hi = geometries[geom[k]].bounds()
+        lo, hi = geometries[instance_geometry[k]].bounds()
         corners = np.array([[x, y, z] for x in (lo[0], hi[0]) for y in (lo[1], hi[1]) for z in (lo[2], hi[2])])
-        mw = world32[k].astype(np.float64).reshape(4, 4).T
+        mw = matrix_world[k].astype(np.float64).reshape(4, 4).T
         wc = corners @ mw[:3, :3].T + mw[:3, 3]
         boxes[k, 0], boxes[k, 1] = wc.min(axis=0), wc.max(axis=0)
     slo, shi = boxes[:, 0].min(axis=0), boxes[:, 1].max(axis=0)
@@ -225,4 +234,4 @@ def scene_rays(geometries: list, n_instances: int, n_rays: int, seed: int = 11, 
     tgt = boxes[pick, 0] + (boxes[pick, 1] - boxes[pick, 0]) * rng.random((n_rays, 3))
     d = tgt - o.astype(np.float64)
     d /= np.linalg.norm(d, axis=1, keepdims=True)
-    return SceneWorkload(world32, inv32, geom, o, d.astype(np.float32))
+    return o, d.astype(np.float32)

@@ -2,6 +2,7 @@
 // Public entry point: the query half of three-mesh-bvh's API (src/index.js of the reference).
 const C = require('./Constants.js');
 const { MeshBVH } = require('./MeshBVH.js');
+const { SceneBVH } = require('./SceneBVH.js');
 const ext = require('./ExtensionUtilities.js');
 const { getTriangleHitPointInfo } = require('./TriangleUtilities.js');
 const native = require('./native.js');
@@ -9,6 +10,7 @@ const { Vec3 } = require('./math.js');
 
 module.exports = {
   MeshBVH,
+  SceneBVH,
   CENTER: C.CENTER,
   AVERAGE: C.AVERAGE,
   SAH: C.SAH,
