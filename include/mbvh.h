@@ -330,6 +330,46 @@ int mbvh_batch_raycast_first(mbvh_ctx* ctx, mbvh_batch* batch, const float* matr
                              const uint32_t* ray_instance, uint64_t n, int32_t side, float near_dist, float far_dist,
                              uint32_t* face_index, float* distance, float* point, float* barycoord);
 /* ------------------------------------------------------------------------------------------ */
+/* Export to the reference's shader data layouts                                              */
+/* ------------------------------------------------------------------------------------------ */
+
+/*
+ * Texture sides (width = height) of MeshBVHUniformStruct's textures for this mesh, in texels:
+ * sides[0] bvhBounds (2 * ceil(sqrt(nodes / 2))), sides[1] bvhContents (ceil(sqrt(nodes))),
+ * sides[2] the index texture (ceil(sqrt(triangles)), at least 1), sides[3] the position texture
+ * (ceil(sqrt(vertices)), at least 1); src/webgl/MeshBVHUniformStruct.js:146-150,
+ * src/webgl/VertexAttributeTexture.js:251.
+ */
+int mbvh_mesh_texture_sizes(const mbvh_mesh* mesh, uint32_t* sides);
+/*
+ * MeshBVHUniformStruct.updateFrom's texture data (bvhToTextures src/webgl/MeshBVHUniformStruct.js:129-210
+ * and VertexAttributeTexture.updateFrom src/webgl/VertexAttributeTexture.js:106-300) written from the
+ * device copy of a single-root mesh (multi-root meshes are rejected, as the reference does):
+ * bounds = RGBA32F texels, two per node (min.xyz, 0 / max.xyz, 0); contents = RG32UI texels
+ * ((0xFFFF0000 | count, offset) for a leaf, (split axis, right child distance) otherwise); index
+ * (optional) = RGBA texels (a, b, c, 1) of the permuted or, for an indirect mesh, dereferenced index, as
+ * 32-bit values (the reference keeps the index array's element type); position (written when
+ * positions is given) = RGBA32F texels (x, y, z, 1). Texels past the data are zero. Sizes from
+ * mbvh_mesh_texture_sizes. Device pointers; enqueues on `stream`.
+ */
+int mbvh_mesh_export_textures_d(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint32_t position_stride,
+                                uint32_t position_offset, float* bounds, uint32_t* contents, uint32_t* index,
+                                float* position, void* stream);
+/* Host-pointer form (synchronous). */
+int mbvh_mesh_export_textures(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint32_t position_stride,
+                              uint32_t position_offset, float* bounds, uint32_t* contents, uint32_t* index,
+                              float* position);
+/*
+ * The WebGPU BVHNode storage layout of one mesh (bvhNodeStruct src/webgpu/tsl/structs.js:3-14, as
+ * appendBVHSubtree src/webgpu/utils/packBVHBufferUtils.js writes a mesh's roots): 8 words per node,
+ * roots one after another; a root whose bounds are empty (min > max) gets [1, -1] on that axis.
+ * The node buffers are the device copy of the mesh (refits included). Device pointer form enqueues
+ * on `stream`; the host form is synchronous.
+ */
+int mbvh_mesh_export_nodes_d(mbvh_ctx* ctx, mbvh_mesh* mesh, uint32_t* nodes, uint64_t capacity_bytes, void* stream);
+int mbvh_mesh_export_nodes(mbvh_ctx* ctx, mbvh_mesh* mesh, uint32_t* nodes, uint64_t capacity_bytes);
+
+/* ------------------------------------------------------------------------------------------ */
 /* Scenes: closest hit over all instances in one launch                                      */
 /* ------------------------------------------------------------------------------------------ */
 

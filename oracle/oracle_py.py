@@ -197,3 +197,85 @@ class OracleMesh:
         st = TreeStats()
         rc = library().orc_validate_root(self.ref, root, C.byref(st))
         return rc, st.as_dict()
+
+
+# ---- shader data layouts (numpy restatements of the reference's packing) ------------------------------
+
+def _ceil_sqrt(x: float) -> int:
+    return int(np.ceil(np.sqrt(x)))
+
+
+def uniform_textures(root: np.ndarray):
+    """bvhToTextures (src/webgl/MeshBVHUniformStruct.js:129-210) for one root buffer: (bounds float32
+    (side^2, 4), contents uint32 (side^2, 2)). Bounds side = 2 * ceil(sqrt(nodes / 2)): texel 2i holds
+    the node's min and texel 2i + 1 its max (w = 0); contents side = ceil(sqrt(nodes)): a leaf is
+    (LEAFNODE_MASK_32 | count, offset), an internal node (split axis, right child offset)."""
+    buf = np.ascontiguousarray(root, np.uint8)
+    n = buf.size // 32
+    f = buf.view(np.float32).reshape(n, 8)
+    u = buf.view(np.uint32).reshape(n, 8)
+    h = buf.view(np.uint16).reshape(n, 16)
+    bside = 2 * _ceil_sqrt(n / 2)
+    cside = _ceil_sqrt(n)
+    bounds = np.zeros((bside * bside, 4), np.float32)
+    contents = np.zeros((cside * cside, 2), np.uint32)
+    for i in range(n):
+        bounds[2 * i, :3] = f[i, 0:3]
+        bounds[2 * i + 1, :3] = f[i, 3:6]
+        if h[i, 15] == 0xFFFF:  # IS_LEAF
+            contents[i] = (0xFFFF0000 | int(h[i, 14]), u[i, 6])
+        else:
+            contents[i] = (u[i, 7], u[i, 6])
+    return bounds, contents
+
+
+def textures_to_root(bounds: np.ndarray, contents: np.ndarray, n_nodes: int) -> np.ndarray:
+    """The inverse of uniform_textures: the 32-byte node buffer the two textures were made from."""
+    out = np.zeros((n_nodes, 8), np.uint32)
+    b = np.asarray(bounds, np.float32).reshape(-1, 4)
+    c = np.asarray(contents, np.uint32).reshape(-1, 2)
+    for i in range(n_nodes):
+        out[i, 0:3] = b[2 * i, :3].view(np.uint32)
+        out[i, 3:6] = b[2 * i + 1, :3].view(np.uint32)
+        if (c[i, 0] & 0xFFFF0000) == 0xFFFF0000:
+            out[i, 6] = c[i, 1]
+            out[i, 7] = 0xFFFF0000 | (c[i, 0] & 0xFFFF)
+        else:
+            out[i, 6] = c[i, 1]
+            out[i, 7] = c[i, 0]
+    return out.view(np.uint8).reshape(-1)
+
+
+def attribute_texture(values: np.ndarray, item_size: int) -> np.ndarray:
+    """VertexAttributeTexture.updateFrom (src/webgl/VertexAttributeTexture.js:106-300) for a
+    non-normalized attribute: side = ceil(sqrt(count)) or 1, item size 3 stored as RGBA with w = 1,
+    the rest zero. Returns (side^2, 4 or item_size) in the attribute's element type."""
+    v = np.asarray(values).reshape(-1, item_size)
+    count = v.shape[0]
+    side = _ceil_sqrt(count) or 1
+    stride = 4 if item_size == 3 else item_size
+    out = np.zeros((side * side, stride), v.dtype)
+    out[:count, :item_size] = v
+    if item_size == 3:
+        out[:count, 3] = 1
+    return out
+
+
+def webgpu_nodes(roots: Sequence[np.ndarray], geometry_offset: int = 0) -> np.ndarray:
+    """appendBVHSubtree (src/webgpu/utils/packBVHBufferUtils.js) of every root of a mesh, one after
+    another: bounds copied (an empty root's min > max axis written as [1, -1]), leaf offsets rebased by
+    geometry_offset, internal words copied."""
+    parts = []
+    for root in roots:
+        buf = np.ascontiguousarray(root, np.uint8)
+        n = buf.size // 32
+        u = buf.view(np.uint32).reshape(n, 8).copy()
+        f = u.view(np.float32)
+        for c in range(3):
+            if f[0, c] > f[0, c + 3]:
+                f[0, c], f[0, c + 3] = 1.0, -1.0
+        h = buf.view(np.uint16).reshape(n, 16)
+        leaf = h[:, 15] == 0xFFFF
+        u[leaf, 6] += np.uint32(geometry_offset)
+        parts.append(u.view(np.uint8).reshape(-1))
+    return np.concatenate(parts) if parts else np.zeros(0, np.uint8)

@@ -215,6 +215,35 @@ assert.ok(batchedChecked > 64 && presenceDiffers <= 2, `batched BatchedMesh hits
 api.disposeBatchedBoundsTree.call(Object.assign(batched, { boundsTrees: [bvh] }), -1);
 assert.strictEqual(batched._mbvhBatch, null);
 
+// --- MeshBVHUniformStruct / WebGPU node buffer of the main BVH ----------------------------------
+{
+  const u = new api.MeshBVHUniformStruct();
+  u.updateFrom(bvh);
+  const f32 = new Float32Array(bvh._roots[0]), u32 = new Uint32Array(bvh._roots[0]), u16 = new Uint16Array(bvh._roots[0]);
+  const nodes = bvh._roots[0].byteLength / 32;
+  assert.strictEqual(u.bvhBounds.image.width, 2 * Math.ceil(Math.sqrt(nodes / 2)));
+  assert.strictEqual(u.bvhContents.image.width, Math.ceil(Math.sqrt(nodes)));
+  for (let i = 0; i < nodes; i += 97) {
+    for (let c = 0; c < 3; c++) {
+      assert.strictEqual(u.bvhBounds.image.data[8 * i + c], f32[8 * i + c]);
+      assert.strictEqual(u.bvhBounds.image.data[8 * i + 4 + c], f32[8 * i + 3 + c]);
+    }
+    const leaf = u16[16 * i + 15] === 0xFFFF;
+    assert.strictEqual(u.bvhContents.image.data[2 * i], leaf ? (0xFFFF0000 | u16[16 * i + 14]) >>> 0 : u32[8 * i + 7]);
+    assert.strictEqual(u.bvhContents.image.data[2 * i + 1], u32[8 * i + 6]);
+  }
+  const ia = geometry.index.array, tex = u.index.image.data;
+  assert.strictEqual(tex.constructor, ia.constructor);
+  for (let t = 0; t < ia.length / 3; t += 13) {
+    assert.ok(tex[4 * t] === ia[3 * t] && tex[4 * t + 1] === ia[3 * t + 1] && tex[4 * t + 2] === ia[3 * t + 2] && tex[4 * t + 3] === 1);
+  }
+  const pd = u.position.image.data;
+  for (let v = 0; v < positions.length / 3; v += 101) {
+    assert.ok(pd[4 * v] === positions[3 * v] && pd[4 * v + 1] === positions[3 * v + 1] && pd[4 * v + 2] === positions[3 * v + 2] && pd[4 * v + 3] === 1);
+  }
+  assert.ok(Buffer.from(api.bvhNodeBuffer(bvh).buffer).equals(Buffer.from(bvh._roots[0])));
+}
+
 // --- intersectsSphere: batch results go back to the oracle; the per-call form agrees -------------
 const sc = new Float32Array(3 * nPts), sr = new Float32Array(nPts);
 for (let i = 0; i < nPts; i++) {

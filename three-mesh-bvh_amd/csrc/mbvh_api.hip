@@ -91,6 +91,7 @@ struct mbvh_mesh {
   uint32_t* d_ref_leaves = nullptr;  // leaf nodes
   uint32_t* d_ref_scratch = nullptr; // per node: changed flag
   uint2* d_four_map = nullptr;       // 4-wide child slot <- wide box (float offsets)
+  uint64_t* d_root_first = nullptr;  // first reference node of each root (the exports)
   uint64_t n_ref_nodes = 0, n_ref_leaves = 0, n_four_map = 0;
   std::vector<uint64_t> root_base;   // first node of each root
   std::vector<uint64_t> root_nodes;  // nodes of each root
@@ -334,7 +335,7 @@ void free_mesh_buffers(mbvh_mesh* m) {
   for (void* p : {static_cast<void*>(m->d_ref_nodes), static_cast<void*>(m->d_ref_levels),
                   static_cast<void*>(m->d_ref_wide), static_cast<void*>(m->d_ref_leaves),
                   static_cast<void*>(m->d_ref_scratch), static_cast<void*>(m->d_four_map),
-                  static_cast<void*>(m->d_coord_max)})
+                  static_cast<void*>(m->d_coord_max), static_cast<void*>(m->d_root_first)})
     if (p) (void)hipFree(p);
   m->d_ref_nodes = nullptr;
   m->d_ref_levels = nullptr;
@@ -343,6 +344,7 @@ void free_mesh_buffers(mbvh_mesh* m) {
   m->d_ref_scratch = nullptr;
   m->d_four_map = nullptr;
   m->d_coord_max = nullptr;
+  m->d_root_first = nullptr;
   m->d_face_slot = nullptr;
   m->d_nodes4 = nullptr;
   m->d_roots4 = nullptr;
@@ -658,6 +660,7 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_ref_scratch), nullptr, maps.base * sizeof(uint32_t));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_four_map), maps.four_map.data(), maps.four_map.size() * sizeof(uint2));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_coord_max), &coord_max, sizeof(float));
+  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_root_first), root_base.data(), root_base.size() * sizeof(uint64_t));
   if (!rc && indirect) {
     std::vector<uint32_t> face_slot(face_limit, 0xFFFFFFFFu);
     for (uint64_t sl = 0; sl < n_slots; sl++) face_slot[tris_face(tris, sl)] = static_cast<uint32_t>(sl);
@@ -2402,4 +2405,154 @@ extern "C" int mbvh_scene_raycast_first(mbvh_ctx* ctx, const mbvh_scene* scene, 
   if (barycoord) MBVH_HIP(hipMemcpyAsync(barycoord, dB, v3, hipMemcpyDeviceToHost, s));
   MBVH_HIP(hipStreamSynchronize(s));
   return check_no_overflow(face_index, n, fn);
+}
+
+// ---- export to the reference's shader data layouts ---------------------------------------------------
+
+namespace {
+
+uint32_t ceil_sqrt(double x) { return static_cast<uint32_t>(std::ceil(std::sqrt(x))); }
+
+// Texture sides: MeshBVHUniformStruct.js:146-150 (bounds 2 * ceil(sqrt(nodes / 2)), contents
+// ceil(sqrt(nodes))), VertexAttributeTexture.js:251 (ceil(sqrt(count)) || 1).
+void texture_sides(const mbvh_mesh* m, uint32_t* sides) {
+  const double nodes = m->root_nodes.empty() ? 0.0 : static_cast<double>(m->root_nodes[0]);
+  sides[0] = 2 * ceil_sqrt(nodes / 2);
+  sides[1] = ceil_sqrt(nodes);
+  sides[2] = std::max<uint32_t>(ceil_sqrt(static_cast<double>(m->n_slots)), 1u);
+  sides[3] = std::max<uint32_t>(ceil_sqrt(static_cast<double>(m->vertex_count)), 1u);
+}
+
+int enqueue_export_textures(mbvh_mesh* m, const float* d_positions, uint32_t stride, uint32_t offset, float* bounds,
+                            uint32_t* contents, uint32_t* index, float* position, hipStream_t s) {
+  uint32_t sides[4];
+  texture_sides(m, sides);
+  ExportArgs a{};
+  a.nodes = m->d_ref_nodes;
+  a.n_nodes = m->root_nodes[0];
+  a.bounds = bounds;
+  a.bounds_texels = static_cast<uint64_t>(sides[0]) * sides[0];
+  a.contents = contents;
+  a.contents_texels = static_cast<uint64_t>(sides[1]) * sides[1];
+  a.vidx = m->d_vidx;
+  a.n_tris = m->n_slots;
+  a.index = index;
+  a.index_texels = static_cast<uint64_t>(sides[2]) * sides[2];
+  a.positions = d_positions;
+  a.stride = stride;
+  a.offset = offset;
+  a.n_vertices = m->vertex_count;
+  a.position = d_positions ? position : nullptr;
+  a.position_texels = static_cast<uint64_t>(sides[3]) * sides[3];
+  MBVH_HIP(launch_export_textures(a, s));
+  return MBVH_OK;
+}
+
+int check_export(const char* fn, mbvh_ctx* ctx, const mbvh_mesh* mesh, const void* bounds, const void* contents,
+                 const void* positions, const void* position, uint32_t stride, uint32_t offset) {
+  int rc = check_query(fn, ctx, mesh);
+  if (rc) return rc;
+  if (mesh->root_nodes.size() != 1)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: Multi-root BVHs not supported (%zu roots)", fn, mesh->root_nodes.size());
+  if (!bounds || !contents) return set_error(MBVH_ERR_BAD_ARGS, "%s: bounds and contents are required", fn);
+  if (positions && (!position || stride < offset + 3))
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: positions need a position texture and a stride >= offset + 3", fn);
+  return MBVH_OK;
+}
+
+}  // namespace
+
+extern "C" int mbvh_mesh_texture_sizes(const mbvh_mesh* mesh, uint32_t* sides) {
+  clear_error();
+  if (!valid_mesh(mesh)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_mesh_texture_sizes: invalid mesh handle");
+  if (!sides) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_mesh_texture_sizes: sides is NULL");
+  texture_sides(mesh, sides);
+  return MBVH_OK;
+}
+
+extern "C" int mbvh_mesh_export_textures_d(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions,
+                                           uint32_t position_stride, uint32_t position_offset, float* bounds,
+                                           uint32_t* contents, uint32_t* index, float* position, void* stream) {
+  clear_error();
+  const char* fn = "mbvh_mesh_export_textures_d";
+  int rc = check_export(fn, ctx, mesh, bounds, contents, positions, position, position_stride, position_offset);
+  if (rc) return rc;
+  if ((rc = set_device(ctx))) return rc;
+  return enqueue_export_textures(mesh, positions, position_stride, position_offset, bounds, contents, index, position,
+                                 stream ? static_cast<hipStream_t>(stream) : ctx->stream);
+}
+
+extern "C" int mbvh_mesh_export_textures(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions,
+                                         uint32_t position_stride, uint32_t position_offset, float* bounds,
+                                         uint32_t* contents, uint32_t* index, float* position) {
+  clear_error();
+  const char* fn = "mbvh_mesh_export_textures";
+  int rc = check_export(fn, ctx, mesh, bounds, contents, positions, position, position_stride, position_offset);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t s = ctx->stream;
+  uint32_t sides[4];
+  texture_sides(mesh, sides);
+  const size_t nb = 16ull * sides[0] * sides[0], nc = 8ull * sides[1] * sides[1];
+  const size_t ni = 16ull * sides[2] * sides[2], np = 16ull * sides[3] * sides[3];
+  const size_t pos_bytes = mesh->vertex_count ? ((mesh->vertex_count - 1) * position_stride + position_offset + 3) * 4 : 0;
+  void *dB, *dC, *dI = nullptr, *dPin = nullptr, *dP = nullptr;
+  if ((rc = stage_out(ctx->staging[2], bounds, nb, &dB)) || (rc = stage_out(ctx->staging[3], contents, nc, &dC)) ||
+      (rc = stage_out(ctx->staging[4], index, ni, &dI)) ||
+      (rc = stage_in(ctx, ctx->staging[0], pos_bytes ? positions : nullptr, pos_bytes, &dPin)) ||
+      (rc = stage_out(ctx->staging[5], dPin ? position : nullptr, np, &dP)))
+    return rc;
+  if ((rc = enqueue_export_textures(mesh, static_cast<const float*>(dPin), position_stride, position_offset,
+                                    static_cast<float*>(dB), static_cast<uint32_t*>(dC), static_cast<uint32_t*>(dI),
+                                    static_cast<float*>(dP), s)))
+    return rc;
+  MBVH_HIP(hipMemcpyAsync(bounds, dB, nb, hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipMemcpyAsync(contents, dC, nc, hipMemcpyDeviceToHost, s));
+  if (index) MBVH_HIP(hipMemcpyAsync(index, dI, ni, hipMemcpyDeviceToHost, s));
+  if (dP) MBVH_HIP(hipMemcpyAsync(position, dP, np, hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipStreamSynchronize(s));
+  return MBVH_OK;
+}
+
+namespace {
+
+int enqueue_export_nodes(mbvh_mesh* m, uint32_t* out, hipStream_t s) {
+  MBVH_HIP(launch_export_nodes(m->d_ref_nodes, m->n_ref_nodes, m->d_root_first,
+                               static_cast<uint32_t>(m->root_base.size()), out, s));
+  return MBVH_OK;
+}
+
+}  // namespace
+
+extern "C" int mbvh_mesh_export_nodes_d(mbvh_ctx* ctx, mbvh_mesh* mesh, uint32_t* nodes, uint64_t capacity_bytes,
+                                        void* stream) {
+  clear_error();
+  const char* fn = "mbvh_mesh_export_nodes_d";
+  int rc = check_query(fn, ctx, mesh);
+  if (rc) return rc;
+  if (!nodes || capacity_bytes < mesh->n_ref_nodes * kBytesPerNode)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: nodes must hold %llu bytes", fn,
+                     static_cast<unsigned long long>(mesh->n_ref_nodes * kBytesPerNode));
+  if ((rc = set_device(ctx))) return rc;
+  return enqueue_export_nodes(mesh, nodes, stream ? static_cast<hipStream_t>(stream) : ctx->stream);
+}
+
+extern "C" int mbvh_mesh_export_nodes(mbvh_ctx* ctx, mbvh_mesh* mesh, uint32_t* nodes, uint64_t capacity_bytes) {
+  clear_error();
+  const char* fn = "mbvh_mesh_export_nodes";
+  int rc = check_query(fn, ctx, mesh);
+  if (rc) return rc;
+  const size_t bytes = mesh->n_ref_nodes * kBytesPerNode;
+  if (!nodes || capacity_bytes < bytes)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: nodes must hold %llu bytes", fn, static_cast<unsigned long long>(bytes));
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t s = ctx->stream;
+  void* d = nullptr;
+  if ((rc = stage_out(ctx->staging[6], nodes, bytes, &d))) return rc;
+  if ((rc = enqueue_export_nodes(mesh, static_cast<uint32_t*>(d), s))) return rc;
+  MBVH_HIP(hipMemcpyAsync(nodes, d, bytes, hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipStreamSynchronize(s));
+  return MBVH_OK;
 }

@@ -242,6 +242,31 @@ struct SceneArgs {
 };
 hipError_t launch_scene_raycast_first(const SceneArgs& a, uint64_t threads, hipStream_t s);
 
+// ---- export to the reference's shader data layouts ------------------------------------------------
+struct ExportArgs {
+  const uint32_t* nodes;     // reference-format nodes of one root (8 words each)
+  uint64_t n_nodes;
+  float* bounds;             // MeshBVHUniformStruct bvhBounds: 2 RGBA32F texels per node, side^2 texels
+  uint64_t bounds_texels;
+  uint32_t* contents;        // bvhContents: 1 RG32UI texel per node
+  uint64_t contents_texels;
+  const uint32_t* vidx;      // 3 vertex ids per triangle slot (the permuted / dereferenced index)
+  uint64_t n_tris;
+  uint32_t* index;           // index texture: RGBA texels (a, b, c, 1) per triangle
+  uint64_t index_texels;
+  const float* positions;    // optional: vertex positions, vertex v at positions[v * stride + offset]
+  uint32_t stride;
+  uint32_t offset;
+  uint64_t n_vertices;
+  float* position;           // position texture: RGBA32F texels (x, y, z, 1) per vertex
+  uint64_t position_texels;
+};
+hipError_t launch_export_textures(const ExportArgs& a, hipStream_t s);
+// WebGPU BVHNode buffer of one mesh (appendBVHSubtree, src/webgpu/utils/packBVHBufferUtils.js): the
+// nodes of every root in order, an empty root's inverted bounds written as [1, -1].
+hipError_t launch_export_nodes(const uint32_t* nodes, uint64_t n_nodes, const uint64_t* root_first, uint32_t n_roots,
+                               uint32_t* out, hipStream_t s);
+
 hipError_t launch_instance_rays(const InstanceRayArgs& a, hipStream_t s);
 hipError_t launch_instance_hits(const InstanceHitArgs& a, hipStream_t s);
 // lanes 1: one lane per ray (stack_lds 8 or 16 LDS stack entries per lane); lanes 4: four lanes

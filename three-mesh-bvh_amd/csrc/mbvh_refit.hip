@@ -1,4 +1,5 @@
-// Refit: every node box of an uploaded tree recomputed from new vertex positions, on the device.
+// Refit: every node box of an uploaded tree recomputed from new vertex positions, on the device; and
+// the exports of the tree to the reference's shader data layouts.
 //
 // Semantics of the reference's refit with nodeIndices = null (src/core/cast/refit.template.js:4-196,
 // src/core/MeshBVH.js:504-508): a leaf's box becomes the min / max over its triangles' vertices,
@@ -139,7 +140,87 @@ uint32_t blocks_for(uint64_t n) {
   return static_cast<uint32_t>(b < 1 ? 1 : (b > 16384 ? 16384 : b));
 }
 
+// MeshBVHUniformStruct's textures (src/webgl/MeshBVHUniformStruct.js:129-210, VertexAttributeTexture
+// updateFrom src/webgl/VertexAttributeTexture.js:106-300): one thread per texel of the largest
+// texture; texels past the data are zero (the reference's arrays start zeroed).
+__global__ __launch_bounds__(kRefitBlock) void k_export_textures(ExportArgs a) {
+  uint64_t most = a.bounds_texels > a.contents_texels ? a.bounds_texels : a.contents_texels;
+  if (a.index && a.index_texels > most) most = a.index_texels;
+  if (a.position && a.position_texels > most) most = a.position_texels;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kRefitBlock;
+  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * kRefitBlock + threadIdx.x; t < most; t += stride) {
+    if (t < a.bounds_texels) {  // texel 2i: min.xyz, 0; texel 2i + 1: max.xyz, 0
+      const uint64_t node = t / 2, half = t % 2;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (node < a.n_nodes) {
+        const float* b = reinterpret_cast<const float*>(a.nodes + 8 * node) + 3 * half;
+        v = make_float4(b[0], b[1], b[2], 0.f);
+      }
+      reinterpret_cast<float4*>(a.bounds)[t] = v;
+    }
+    if (t < a.contents_texels) {  // leaf: (0xFFFF0000 | count, offset); internal: (split axis, right offset)
+      uint2 v = make_uint2(0u, 0u);
+      if (t < a.n_nodes) {
+        const uint32_t* w = a.nodes + 8 * t;
+        const bool leaf = (w[7] >> 16) == 0xFFFFu;
+        v = leaf ? make_uint2(0xFFFF0000u | (w[7] & 0xFFFFu), w[6]) : make_uint2(w[7], w[6]);
+      }
+      reinterpret_cast<uint2*>(a.contents)[t] = v;
+    }
+    if (a.index && t < a.index_texels) {
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (t < a.n_tris) v = make_uint4(a.vidx[3 * t], a.vidx[3 * t + 1], a.vidx[3 * t + 2], 1u);
+      reinterpret_cast<uint4*>(a.index)[t] = v;
+    }
+    if (a.position && t < a.position_texels) {
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (t < a.n_vertices) {
+        const float* p = a.positions + t * a.stride + a.offset;
+        v = make_float4(p[0], p[1], p[2], 1.0f);
+      }
+      reinterpret_cast<float4*>(a.position)[t] = v;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kRefitBlock) void k_export_nodes(const uint32_t* nodes, uint64_t n_nodes,
+                                                               const uint64_t* root_first, uint32_t n_roots,
+                                                               uint32_t* out) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kRefitBlock;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kRefitBlock + threadIdx.x; i < n_nodes; i += stride) {
+    uint32_t w[8];
+    for (int k = 0; k < 8; k++) w[k] = nodes[8 * i + k];
+    bool is_root = false;
+    for (uint32_t r = 0; r < n_roots; r++) is_root = is_root || root_first[r] == i;
+    if (is_root) {
+      for (int c = 0; c < 3; c++) {
+        if (__uint_as_float(w[c]) > __uint_as_float(w[c + 3])) {
+          w[c] = __float_as_uint(1.0f);
+          w[c + 3] = __float_as_uint(-1.0f);
+        }
+      }
+    }
+    for (int k = 0; k < 8; k++) out[8 * i + k] = w[k];
+  }
+}
+
 }  // namespace
+
+hipError_t launch_export_textures(const ExportArgs& a, hipStream_t s) {
+  uint64_t most = a.bounds_texels > a.contents_texels ? a.bounds_texels : a.contents_texels;
+  if (a.index && a.index_texels > most) most = a.index_texels;
+  if (a.position && a.position_texels > most) most = a.position_texels;
+  if (most) hipLaunchKernelGGL(k_export_textures, dim3(blocks_for(most)), dim3(kRefitBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_export_nodes(const uint32_t* nodes, uint64_t n_nodes, const uint64_t* root_first, uint32_t n_roots,
+                               uint32_t* out, hipStream_t s) {
+  if (n_nodes)
+    hipLaunchKernelGGL(k_export_nodes, dim3(blocks_for(n_nodes)), dim3(kRefitBlock), 0, s, nodes, n_nodes, root_first,
+                       n_roots, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_refit(const RefitArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_refit_reset_bound, dim3(1), dim3(64), 0, s, a.coord_bound);

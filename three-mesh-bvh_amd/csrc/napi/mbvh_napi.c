@@ -568,6 +568,73 @@ static napi_value js_scene_raycast_first(napi_env env, napi_callback_info info) 
   return NULL;
 }
 
+/* textureSizes(mesh) -> [bounds, contents, index, position] texture sides */
+static napi_value js_texture_sizes(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  MeshHandle* h = get_mesh(env, argv[0]);
+  if (!h) return throw_msg(env, "textureSizes: invalid mesh");
+  uint32_t sides[4];
+  int rc = mbvh_mesh_texture_sizes(h->mesh, sides);
+  if (rc) return throw_status(env, rc, "mbvh_mesh_texture_sizes");
+  napi_value out;
+  napi_create_array_with_length(env, 4, &out);
+  for (uint32_t i = 0; i < 4; i++) {
+    napi_value v;
+    napi_create_uint32(env, sides[i], &v);
+    napi_set_element(env, out, i, v);
+  }
+  return out;
+}
+
+/* exportTextures(mesh, positions|null, stride, offset, bounds F32, contents U32, index U32|null, position F32|null) */
+static napi_value js_export_textures(napi_env env, napi_callback_info info) {
+  napi_value argv[8];
+  get_args(env, info, 8, argv);
+  MeshHandle* h = get_mesh(env, argv[0]);
+  if (!h) return throw_msg(env, "exportTextures: invalid mesh");
+  uint32_t sides[4];
+  int rc = mbvh_mesh_texture_sizes(h->mesh, sides);
+  if (rc) return throw_status(env, rc, "mbvh_mesh_texture_sizes");
+  View pos;
+  const int has_pos = !is_nullish(env, argv[1]);
+  if (has_pos && (!get_view(env, argv[1], &pos) || pos.type != napi_float32_array))
+    return throw_msg(env, "exportTextures: positions must be a Float32Array");
+  int32_t stride = 3, offset = 0;
+  if (!get_i32(env, argv[2], &stride) || !get_i32(env, argv[3], &offset) || stride <= 0 || offset < 0)
+    return throw_msg(env, "exportTextures: stride / offset must be integers");
+  void *bounds, *contents, *index, *position;
+  if (!out_view(env, argv[4], napi_float32_array, 4ull * sides[0] * sides[0], &bounds) || !bounds ||
+      !out_view(env, argv[5], napi_uint32_array, 2ull * sides[1] * sides[1], &contents) || !contents ||
+      !out_view(env, argv[6], napi_uint32_array, 4ull * sides[2] * sides[2], &index) ||
+      !out_view(env, argv[7], napi_float32_array, 4ull * sides[3] * sides[3], &position))
+    return throw_msg(env, "exportTextures: output arrays have the wrong type or length");
+  mbvh_ctx* ctx = NULL;
+  rc = context_for(env, h->device, &ctx);
+  if (!rc)
+    rc = mbvh_mesh_export_textures(ctx, h->mesh, has_pos ? (const float*)pos.data : NULL, (uint32_t)stride,
+                                   (uint32_t)offset, (float*)bounds, (uint32_t*)contents, (uint32_t*)index,
+                                   (float*)position);
+  if (rc) return throw_status(env, rc, "mbvh_mesh_export_textures");
+  return NULL;
+}
+
+/* exportNodes(mesh, out Uint32Array of 8 words per node) */
+static napi_value js_export_nodes(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  MeshHandle* h = get_mesh(env, argv[0]);
+  if (!h) return throw_msg(env, "exportNodes: invalid mesh");
+  View out;
+  if (!get_view(env, argv[1], &out) || out.type != napi_uint32_array)
+    return throw_msg(env, "exportNodes: out must be a Uint32Array");
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, h->device, &ctx);
+  if (!rc) rc = mbvh_mesh_export_nodes(ctx, h->mesh, (uint32_t*)out.data, 4ull * out.length);
+  if (rc) return throw_status(env, rc, "mbvh_mesh_export_nodes");
+  return NULL;
+}
+
 /* setSideMask(mesh, Uint8Array) */
 static napi_value js_set_side_mask(napi_env env, napi_callback_info info) {
   napi_value argv[2];
@@ -896,6 +963,9 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("freeBatch", js_free_batch);
   EXPORT("batchRaycastFirst", js_batch_raycast_first);
   EXPORT("sceneCreate", js_scene_create);
+  EXPORT("textureSizes", js_texture_sizes);
+  EXPORT("exportTextures", js_export_textures);
+  EXPORT("exportNodes", js_export_nodes);
   EXPORT("freeScene", js_free_batch);
   EXPORT("sceneRaycastFirst", js_scene_raycast_first);
   EXPORT("closestPointBackward", js_closest_point_bwd);

@@ -38,7 +38,8 @@ EXPORTED_SYMBOLS = (
     "mbvh_order_workspace_bytes", "mbvh_order_points_d",
     "mbvh_batch_create", "mbvh_batch_free", "mbvh_batch_workspace_bytes", "mbvh_batch_raycast_first_d",
     "mbvh_batch_raycast_first", "mbvh_scene_create", "mbvh_scene_free", "mbvh_scene_raycast_first",
-    "mbvh_scene_raycast_first_d",
+    "mbvh_scene_raycast_first_d", "mbvh_mesh_texture_sizes", "mbvh_mesh_export_textures",
+    "mbvh_mesh_export_textures_d", "mbvh_mesh_export_nodes", "mbvh_mesh_export_nodes_d",
     "mbvh_batch_autotune", "mbvh_batch_graph_create", "mbvh_batch_graph_launch", "mbvh_batch_graph_free",
     "mbvh_instance_rays_d", "mbvh_instance_hits_d",
     "mbvh_grad_workspace_bytes", "mbvh_raycast_first_bwd", "mbvh_raycast_first_bwd_d",
@@ -127,6 +128,11 @@ def library() -> C.CDLL:
         "mbvh_scene_free": (None, [vp]),
         "mbvh_scene_raycast_first_d": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp, vp]),
         "mbvh_scene_raycast_first": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp]),
+        "mbvh_mesh_texture_sizes": (C.c_int, [vp, vp]),
+        "mbvh_mesh_export_textures": (C.c_int, [vp, vp, vp, u32, u32, vp, vp, vp, vp]),
+        "mbvh_mesh_export_textures_d": (C.c_int, [vp, vp, vp, u32, u32, vp, vp, vp, vp, vp]),
+        "mbvh_mesh_export_nodes": (C.c_int, [vp, vp, vp, u64]),
+        "mbvh_mesh_export_nodes_d": (C.c_int, [vp, vp, vp, u64, vp]),
         "mbvh_batch_graph_launch": (C.c_int, [vp, vp]),
         "mbvh_batch_graph_free": (None, [vp]),
         "mbvh_instance_rays_d": (C.c_int, [vp, vp, u32, vp, vp, vp, u64, vp, vp, vp]),
@@ -339,6 +345,32 @@ class Mesh:
         """Refit from device positions, enqueued on `stream` (no synchronisation)."""
         _check(self.ctx._lib.mbvh_mesh_refit_d(self.ctx.handle, self.handle, C.c_void_p(positions_ptr), vertex_count,
                                                stride, offset, C.c_void_p(stream or None)))
+
+    def texture_sizes(self):
+        """Texture sides (bounds, contents, index, position) of MeshBVHUniformStruct for this mesh."""
+        sides = np.zeros(4, np.uint32)
+        _check(self.ctx._lib.mbvh_mesh_texture_sizes(self.handle, _ptr(sides)))
+        return [int(x) for x in sides]
+
+    def export_textures(self, positions: Optional[np.ndarray] = None, stride: int = 3, offset: int = 0):
+        """MeshBVHUniformStruct texture data: bounds (side^2, 4) float32, contents (side^2, 2) uint32,
+        index (side^2, 4) uint32, position (side^2, 4) float32 (when positions are given)."""
+        sb, sc, si, sp = self.texture_sizes()
+        out = {"bounds": np.empty((sb * sb, 4), np.float32), "contents": np.empty((sc * sc, 2), np.uint32),
+               "index": np.empty((si * si, 4), np.uint32),
+               "position": None if positions is None else np.empty((sp * sp, 4), np.float32)}
+        pos = None if positions is None else np.ascontiguousarray(positions, np.float32).reshape(-1)
+        _check(self.ctx._lib.mbvh_mesh_export_textures(self.ctx.handle, self.handle, _ptr(pos), stride, offset,
+                                                       _ptr(out["bounds"]), _ptr(out["contents"]), _ptr(out["index"]),
+                                                       _ptr(out["position"])))
+        return out
+
+    def export_nodes(self) -> np.ndarray:
+        """The WebGPU BVHNode buffer of the mesh (roots one after another)."""
+        n = sum(self.root_bytes)
+        buf = np.empty(int(n), np.uint8)
+        _check(self.ctx._lib.mbvh_mesh_export_nodes(self.ctx.handle, self.handle, _ptr(buf), buf.nbytes))
+        return buf
 
     def copy_roots(self):
         """The mesh's current reference-format root buffers (uint8 arrays)."""

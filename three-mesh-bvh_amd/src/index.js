@@ -3,6 +3,7 @@
 const C = require('./Constants.js');
 const { MeshBVH } = require('./MeshBVH.js');
 const { SceneBVH } = require('./SceneBVH.js');
+const { MeshBVHUniformStruct, bvhNodeBuffer } = require('./MeshBVHUniformStruct.js');
 const ext = require('./ExtensionUtilities.js');
 const { getTriangleHitPointInfo } = require('./TriangleUtilities.js');
 const native = require('./native.js');
@@ -11,6 +12,8 @@ const { Vec3 } = require('./math.js');
 module.exports = {
   MeshBVH,
   SceneBVH,
+  MeshBVHUniformStruct,
+  bvhNodeBuffer,
   CENTER: C.CENTER,
   AVERAGE: C.AVERAGE,
   SAH: C.SAH,
