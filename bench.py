@@ -241,6 +241,8 @@ def bench_differentiable(torch, M, W, ctx, stream, args):
         "fwd_bwd_over_fwd": round((f_ms + b_det) / f_ms, 3),
         "fwd_bwd_over_fwd_atomic": round((f_ms + b_atom) / f_ms, 3), "fwd_mqueries_per_s": round(n / f_ms / 1e3, 2)}
     log(f"C4 {out['differentiable_closest']}")
+    out["export_textures"] = bench_export(torch, ctx, stream, mesh, g, idx, tree)  # before the refit moves the mesh
+    log(f"export {out['export_textures']}")
     out["refit"] = bench_refit(torch, stream, mesh, g, idx, tree)
     log(f"refit {out['refit']}")
     out["build"] = bench_build(torch, M, ctx, w.geometry, "the C4 mesh (displaced Icosahedron(1,223))")
@@ -301,6 +303,66 @@ def bench_refit(torch, stream, mesh, g, idx, tree):
                                        "sample": f"FP64 JavaScript restatement of the reference refit (oracle/js) in "
                                                  f"Node {rep['node']}, the whole tree, median of 3"}
                 res["nodes_equal_to_cpu"] = bool(all(np.array_equal(a, b) for a, b in zip(first, js_roots)))
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    return res
+
+
+def bench_export(torch, ctx, stream, mesh, g, idx, tree):
+    """MeshBVHUniformStruct texture data (bounds, contents, index, position) of the C4 mesh written on
+    the device from the resident mesh (events on the launch stream), against a JavaScript restatement
+    of the reference's packing loops on one thread; the arrays are compared."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    sptr = stream.cuda_stream
+    sb, sc, si, sp = mesh.texture_sizes()
+    pos = torch.from_numpy(g.positions.reshape(-1)).to(dev)
+    outs = [torch.empty((sb * sb, 4), device=dev), torch.empty((sc * sc, 2), dtype=torch.int32, device=dev),
+            torch.empty((si * si, 4), dtype=torch.int32, device=dev), torch.empty((sp * sp, 4), device=dev)]
+    torch.cuda.synchronize()
+    lib = ctx._lib
+
+    def export_d():
+        rc = lib.mbvh_mesh_export_textures_d(ctx.handle, mesh.handle, pos.data_ptr(), 3, 0, *[o.data_ptr() for o in outs],
+                                             sptr)
+        if rc:
+            raise RuntimeError(f"export failed ({rc})")
+
+    d_ms = event_ms(torch, stream, export_d, 10)
+    moved = sum(o.numel() * 4 for o in outs) + tree.roots[0].nbytes + 12 * g.tri_count + g.positions.nbytes
+    res = {"workload": f"MeshBVHUniformStruct textures of the C4 mesh ({tree.roots[0].nbytes // 32} nodes, "
+                       f"{g.tri_count} tris, {g.positions.shape[0]} vertices)",
+           "gpu_device_ms": round(d_ms, 4), "achieved_gb_per_s": round(moved / (d_ms / 1000) / 1e9, 1)}
+    import shutil
+    import subprocess
+    import tempfile
+    node = shutil.which("node")
+    if node is not None:
+        d = tempfile.mkdtemp(prefix="mbvh_export_")
+        try:
+            ib = 2 if idx.dtype == np.uint16 else 4
+            tree.roots[0].tofile(os.path.join(d, "root0.bin"))
+            idx.tofile(os.path.join(d, "index.u16" if ib == 2 else "index.u32"))
+            g.positions.tofile(os.path.join(d, "positions.f32"))
+            with open(os.path.join(d, "meta.json"), "w") as f:
+                json.dump({"index_bytes": ib}, f)
+            out = subprocess.run([node, os.path.join(ROOT, "oracle", "js", "export_bench.js"), d], capture_output=True,
+                                 text=True, timeout=600)
+            if out.returncode == 0:
+                rep = json.loads(out.stdout.strip().splitlines()[-1])
+                got = [o.cpu().numpy() for o in outs]
+                ref_b = np.fromfile(os.path.join(d, "bounds.f32"), np.uint32)
+                ref_c = np.fromfile(os.path.join(d, "contents.u32"), np.uint32)
+                ref_i = np.fromfile(os.path.join(d, "index_tex.bin"), np.uint16 if ib == 2 else np.uint32)
+                ref_p = np.fromfile(os.path.join(d, "position_tex.f32"), np.uint32)
+                same = (np.array_equal(got[0].view(np.uint32).reshape(-1), ref_b)
+                        and np.array_equal(got[1].view(np.uint32).reshape(-1), ref_c)
+                        and np.array_equal(got[2].view(np.uint32).reshape(-1), ref_i.astype(np.uint32))
+                        and np.array_equal(got[3].view(np.uint32).reshape(-1), ref_p))
+                res["cpu_baseline"] = {"single_thread_ms": round(1000 * rep["seconds"], 3), "cores": 1, "kind": "port",
+                                       "sample": f"JavaScript restatement of bvhToTextures and VertexAttributeTexture."
+                                                 f"updateFrom (oracle/js) in Node {rep['node']}, median of 3"}
+                res["identical_to_cpu"] = bool(same)
         finally:
             shutil.rmtree(d, ignore_errors=True)
     return res
