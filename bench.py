@@ -308,6 +308,78 @@ def bench_refit(torch, stream, mesh, g, idx, tree):
     return res
 
 
+def bench_boxes(torch, ctx, stream, mesh, g, idx, tree, n=1 << 20):
+    """intersectsBox on the closest-point mesh: n random oriented boxes (the device entry point, events on
+    the launch stream) against the JavaScript restatement of the reference's shapecast with OrientedBox
+    tests on one thread over a sample; the sample's answers are compared."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    dev = torch.device("cuda:0")
+    sptr = stream.cuda_stream
+    lo, hi = g.bounds()
+    rng = np.random.default_rng(21)
+    ext = np.linalg.norm(hi - lo)
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y + w * z), 2 * (x * z - w * y),
+                  2 * (x * y - w * z), 1 - 2 * (x * x + z * z), 2 * (y * z + w * x),
+                  2 * (x * z + w * y), 2 * (y * z - w * x), 1 - 2 * (x * x + y * y)], axis=1).reshape(n, 3, 3)
+    mats = np.zeros((n, 16))
+    mats[:, [0, 1, 2, 4, 5, 6, 8, 9, 10]] = R.reshape(n, 9)
+    mats[:, 12:15] = lo + (hi - lo) * rng.random((n, 3))
+    mats[:, 15] = 1.0
+    half = ext * rng.uniform(0.002, 0.03, (n, 3))
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+    dmn, dmx, dm = t(-half), t(half), t(mats)
+    out = torch.empty(n, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+
+    def boxes_d():
+        rc = ctx._lib.mbvh_intersects_box_d(ctx.handle, mesh.handle, dmn.data_ptr(), dmx.data_ptr(), dm.data_ptr(), n,
+                                            out.data_ptr(), sptr)
+        if rc:
+            raise RuntimeError(f"intersectsBox failed ({rc})")
+
+    d_ms = event_ms(torch, stream, boxes_d, 10)
+    got = out.cpu().numpy()
+    res = {"workload": f"intersectsBox: {n} random oriented boxes (half extents 0.2-3% of the mesh diagonal) on "
+                       f"the closest-point mesh ({g.tri_count} tris)", "gpu_device_ms": round(d_ms, 3),
+           "mboxes_per_s": round(n / d_ms / 1e3, 2), "hit_fraction": round(float((got == 1).mean()), 3)}
+    import shutil
+    import subprocess
+    import tempfile
+    node = shutil.which("node")
+    if node is not None:
+        d = tempfile.mkdtemp(prefix="mbvh_box_")
+        try:
+            m = 4096
+            ib = 2 if idx.dtype == np.uint16 else 4
+            g.positions.tofile(os.path.join(d, "positions.f32"))
+            idx.tofile(os.path.join(d, "index.u16" if ib == 2 else "index.u32"))
+            for i, rt in enumerate(tree.roots):
+                rt.tofile(os.path.join(d, f"root{i}.bin"))
+            (-half[:m]).tofile(os.path.join(d, "box_min.f64"))
+            half[:m].tofile(os.path.join(d, "box_max.f64"))
+            mats[:m].tofile(os.path.join(d, "box_matrix.f64"))
+            with open(os.path.join(d, "meta.json"), "w") as f:
+                json.dump({"roots": len(tree.roots), "index_bytes": ib, "boxes": m}, f)
+            r = subprocess.run([node, os.path.join(ROOT, "oracle", "js", "box_bench.js"), d], capture_output=True,
+                               text=True, timeout=600)
+            if r.returncode == 0:
+                rep = json.loads(r.stdout.strip().splitlines()[-1])
+                js = np.fromfile(os.path.join(d, "box_hit.u8"), np.uint8)
+                res["cpu_baseline"] = {"value": round(m / rep["seconds"] / 1e6, 4), "unit": "Mboxes/s", "cores": 1,
+                                       "kind": "port",
+                                       "sample": f"the first {m} boxes: JavaScript restatement of the reference's "
+                                                 f"intersectsBox (shapecast + OrientedBox, oracle/js) in Node "
+                                                 f"{rep['node']}, one thread, median of 3"}
+                res["identical_to_cpu_sample"] = bool(np.array_equal(js, got[:m]))
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    return res
+
+
 def bench_export(torch, ctx, stream, mesh, g, idx, tree):
     """MeshBVHUniformStruct texture data (bounds, contents, index, position) of the C4 mesh written on
     the device from the resident mesh (events on the launch stream), against a JavaScript restatement
@@ -817,6 +889,8 @@ def main():
                           "kernel_ms": round(float(np.mean(kern1)), 4), **extra},
     }
     if not args.only_primary:
+        line["intersects_box"] = bench_boxes(torch, ctx, stream, mesh, g, idx, tree)
+        log(f"boxes {line['intersects_box']}")
         line["build_closest_mesh"] = bench_build(torch, M, ctx, g, "the closest-point mesh TorusKnot(10,3,800,200)")
         log(f"build {line['build_closest_mesh']}")
         line.update(bench_differentiable(torch, M, W, ctx, stream, args))

@@ -236,6 +236,19 @@ int mbvh_raycast_first_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* orig
  */
 int mbvh_intersects_sphere(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* centers, const float* radii,
                            uint64_t n, uint8_t* out);
+/*
+ * intersectsBox (src/core/MeshBVH.js:780-792): whether any triangle intersects the oriented box
+ * (box_min, box_max: 3 doubles per box in the box's frame; box_to_mesh: 16 doubles per box,
+ * column-major, the box frame -> the mesh frame). The separating-axis tests of OrientedBox
+ * (src/math/OrientedBox.js:79-260) and ExtendedTriangle (src/math/ExtendedTriangle.js:47-75) are
+ * evaluated in double in the reference's order, so the answers are the reference's. out: 1 / 0 per
+ * box. Host-pointer form (synchronous) and device-pointer form (enqueues on `stream`; out 2 marks a
+ * traversal stack overflow).
+ */
+int mbvh_intersects_box(mbvh_ctx* ctx, const mbvh_mesh* mesh, const double* box_min, const double* box_max,
+                        const double* box_to_mesh, uint64_t n, uint8_t* out);
+int mbvh_intersects_box_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const double* box_min, const double* box_max,
+                          const double* box_to_mesh, uint64_t n, uint8_t* out, void* stream);
 
 /*
  * All hits per ray. Replaces MeshBVH.raycast (src/core/MeshBVH.js:532-545) → raycast

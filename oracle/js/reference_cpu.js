@@ -12,6 +12,7 @@
 //                  src/utils/ThreeRayIntersectUtilities.js:21-47, roots combined with strict <
 //                  (src/core/MeshBVH.js:558-577)
 //   refit          src/core/cast/refit.template.js:4-196 (nodeIndices = null)
+//   intersectsBox  src/core/MeshBVH.js:780-792, OrientedBox src/math/OrientedBox.js:79-260
 //   closestPoint   src/core/cast/closestPointToPoint.js:6-78 over shapecastTraverse
 //                  (src/core/cast/shapecast.js:41-214): children by box distance^2, right first only
 //                  if strictly closer, score < best^2 and < maxThreshold^2, strict < triangle update,
@@ -229,4 +230,105 @@ function refit(m) {
   for (const R of m.roots) refitNode(m, R, 0);
 }
 
-module.exports = { FRONT, BACK, DOUBLE, makeMesh, raycastFirst, closestPoint, refit };
+// ---- intersectsBox (src/core/MeshBVH.js:780-792 over shapecast) ---------------------------------
+// OrientedBox.update / intersectsBox / intersectsTriangle (src/math/OrientedBox.js:79-260) with
+// SeparatingAxisBounds and ExtendedTriangle.update (src/math/ExtendedTriangle.js:47-75); node boxes by
+// the six-axis test, triangles by the full test, depth first, the first intersecting triangle ends it.
+function sepPoints(ax, pts, out) {
+  let mn = Infinity, mx = -Infinity;
+  for (let i = 0; i < pts.length; i += 3) {
+    const v = ax[0] * pts[i] + ax[1] * pts[i + 1] + ax[2] * pts[i + 2];
+    mn = v < mn ? v : mn;
+    mx = v > mx ? v : mx;
+  }
+  out[0] = mn; out[1] = mx;
+  return out;
+}
+const sep = (a, b) => a[0] > b[1] || b[0] > a[1];
+
+function makeObb(bmin, bmax, e) {
+  const p = new Float64Array(24);
+  for (let x = 0; x <= 1; x++) for (let y = 0; y <= 1; y++) for (let z = 0; z <= 1; z++) {
+    const i = x | (y << 1) | (z << 2);
+    const vx = x ? bmax[0] : bmin[0], vy = y ? bmax[1] : bmin[1], vz = z ? bmax[2] : bmin[2];
+    const w = 1 / (e[3] * vx + e[7] * vy + e[11] * vz + e[15]);
+    p[3 * i] = (e[0] * vx + e[4] * vy + e[8] * vz + e[12]) * w;
+    p[3 * i + 1] = (e[1] * vx + e[5] * vy + e[9] * vz + e[13]) * w;
+    p[3 * i + 2] = (e[2] * vx + e[6] * vy + e[10] * vz + e[14]) * w;
+  }
+  const ax = [], sb = [], ab = [];
+  for (let i = 0; i < 3; i++) {
+    const j = 3 * (1 << i);
+    const a = [p[0] - p[j], p[1] - p[j + 1], p[2] - p[j + 2]];
+    ax.push(a);
+    sb.push(sepPoints(a, p, [0, 0]));
+    let mn = Infinity, mx = -Infinity;
+    for (let k = 0; k < 8; k++) { const v = p[3 * k + i]; mn = v < mn ? v : mn; mx = v > mx ? v : mx; }
+    ab.push([mn, mx]);
+  }
+  return { p, ax, sb, ab, t: [0, 0], u: [0, 0] };
+}
+
+function obbBox(o, f, n) {
+  const b = 8 * n;
+  for (let k = 0; k < 3; k++) if (o.ab[k][0] > f[b + 3 + k] || f[b + k] > o.ab[k][1]) return false;
+  for (let i = 0; i < 3; i++) {
+    const ax = o.ax[i];
+    let mn = Infinity, mx = -Infinity;
+    for (let x = 0; x <= 1; x++) for (let y = 0; y <= 1; y++) for (let z = 0; z <= 1; z++) {
+      const px = f[b] * x + f[b + 3] * (1 - x), py = f[b + 1] * y + f[b + 4] * (1 - y), pz = f[b + 2] * z + f[b + 5] * (1 - z);
+      const v = ax[0] * px + ax[1] * py + ax[2] * pz;
+      mn = Math.min(v, mn);
+      mx = Math.max(v, mx);
+    }
+    if (o.sb[i][0] > mx || mn > o.sb[i][1]) return false;
+  }
+  return true;
+}
+
+function obbTri(o, tri) {
+  const a = [tri[0], tri[1], tri[2]], b = [tri[3], tri[4], tri[5]], c = [tri[6], tri[7], tri[8]];
+  const cb = [c[0] - b[0], c[1] - b[1], c[2] - b[2]], ab = [a[0] - b[0], a[1] - b[1], a[2] - b[2]];
+  let n = [cb[1] * ab[2] - cb[2] * ab[1], cb[2] * ab[0] - cb[0] * ab[2], cb[0] * ab[1] - cb[1] * ab[0]];
+  const l2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  if (l2 > 0) { const s = 1 / Math.sqrt(l2); n = [n[0] * s, n[1] * s, n[2] * s]; } else n = [0, 0, 0];
+  const tax = [n, ab, [b[0] - c[0], b[1] - c[1], b[2] - c[2]], [c[0] - a[0], c[1] - a[1], c[2] - a[2]]];
+  for (let i = 0; i < 3; i++) if (sep(o.sb[i], sepPoints(o.ax[i], tri, o.t))) return false;
+  for (let i = 0; i < 3; i++) if (sep(sepPoints(tax[i], tri, o.t), sepPoints(tax[i], o.p, o.u))) return false;
+  for (let i = 0; i < 3; i++) {
+    const s1 = o.ax[i];
+    for (let i2 = 0; i2 < 4; i2++) {
+      const s2 = tax[i2];
+      const x = [s1[1] * s2[2] - s1[2] * s2[1], s1[2] * s2[0] - s1[0] * s2[2], s1[0] * s2[1] - s1[1] * s2[0]];
+      if (sep(sepPoints(x, tri, o.t), sepPoints(x, o.p, o.u))) return false;
+    }
+  }
+  return true;
+}
+
+function boxNode(m, R, o, n, tri) {
+  if (R.u16[16 * n + 15] === 0xFFFF) {
+    const off = R.u32[8 * n + 6], cnt = R.u16[16 * n + 14], p = m.pos, ix = m.index;
+    for (let s = off; s < off + cnt; s++) {
+      for (let k = 0; k < 3; k++) {
+        const v = 3 * ix[3 * s + k];
+        tri[3 * k] = p[v]; tri[3 * k + 1] = p[v + 1]; tri[3 * k + 2] = p[v + 2];
+      }
+      if (obbTri(o, tri)) return true;
+    }
+    return false;
+  }
+  const c1 = n + 1, c2 = n + R.u32[8 * n + 6];
+  if (obbBox(o, R.f32, c1) && boxNode(m, R, o, c1, tri)) return true;
+  if (obbBox(o, R.f32, c2) && boxNode(m, R, o, c2, tri)) return true;
+  return false;
+}
+
+function intersectsBox(m, bmin, bmax, e) {
+  const o = makeObb(bmin, bmax, e);
+  const tri = new Float64Array(9);
+  for (const R of m.roots) if (obbBox(o, R.f32, 0) && boxNode(m, R, o, 0, tri)) return true;
+  return false;
+}
+
+module.exports = { FRONT, BACK, DOUBLE, makeMesh, raycastFirst, closestPoint, refit, intersectsBox };

@@ -838,3 +838,145 @@ int orc_intersects_sphere(const orc_mesh* m, const float center[3], double radiu
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* intersectsBox (src/core/MeshBVH.js:780-792): whether any triangle intersects an oriented    */
+/* box, decided as OrientedBox.intersectsTriangle does (src/math/OrientedBox.js:190-260 after   */
+/* update, :79-130) with SeparatingAxisBounds (src/math/SeparatingAxisBounds.js) and the        */
+/* ExtendedTriangle.update axes (src/math/ExtendedTriangle.js:47-75), in double, in the         */
+/* reference's operation order (three's Vector3.applyMatrix4 / dot / crossVectors and           */
+/* Triangle.getNormal). Brute force over the triangles.                                          */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+  double mn, mx;
+} sab;
+
+static sab sab_points(const double ax[3], const double (*p)[3], int n) {
+  sab b = {INFINITY, -INFINITY};
+  for (int i = 0; i < n; i++) {
+    const double v = ax[0] * p[i][0] + ax[1] * p[i][1] + ax[2] * p[i][2];
+    b.mn = v < b.mn ? v : b.mn;
+    b.mx = v > b.mx ? v : b.mx;
+  }
+  return b;
+}
+
+static int sab_separated(sab a, sab b) { return a.mn > b.mx || b.mn > a.mx; }
+
+static void obb_corners(const double bmin[3], const double bmax[3], const double e[16], double p[8][3], double ax[3][3],
+                        sab sb[3]) {
+  for (int x = 0; x <= 1; x++)
+    for (int y = 0; y <= 1; y++)
+      for (int z = 0; z <= 1; z++) {
+        const int i = x | (y << 1) | (z << 2);
+        const double vx = x ? bmax[0] : bmin[0], vy = y ? bmax[1] : bmin[1], vz = z ? bmax[2] : bmin[2];
+        const double w = 1 / (e[3] * vx + e[7] * vy + e[11] * vz + e[15]);
+        p[i][0] = (e[0] * vx + e[4] * vy + e[8] * vz + e[12]) * w;
+        p[i][1] = (e[1] * vx + e[5] * vy + e[9] * vz + e[13]) * w;
+        p[i][2] = (e[2] * vx + e[6] * vy + e[10] * vz + e[14]) * w;
+      }
+  for (int i = 0; i < 3; i++) {
+    for (int k = 0; k < 3; k++) ax[i][k] = p[0][k] - p[1 << i][k];
+    sb[i] = sab_points(ax[i], (const double(*)[3])p, 8);
+  }
+}
+
+static void cross3(const double a[3], const double b[3], double o[3]) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+int orc_box_triangle_sat(const double bmin[3], const double bmax[3], const double e[16], const double a[3],
+                         const double b[3], const double c[3]) {
+  double p[8][3], ax[3][3];
+  sab sb[3];
+  obb_corners(bmin, bmax, e, p, ax, sb);
+  const double pts[3][3] = {{a[0], a[1], a[2]}, {b[0], b[1], b[2]}, {c[0], c[1], c[2]}};
+  double tax[4][3], cb[3], ab[3];
+  for (int k = 0; k < 3; k++) {
+    cb[k] = c[k] - b[k];
+    ab[k] = a[k] - b[k];
+  }
+  cross3(cb, ab, tax[0]);
+  const double l2 = tax[0][0] * tax[0][0] + tax[0][1] * tax[0][1] + tax[0][2] * tax[0][2];
+  if (l2 > 0) {
+    const double s = 1 / sqrt(l2);
+    for (int k = 0; k < 3; k++) tax[0][k] = tax[0][k] * s;
+  } else {
+    tax[0][0] = tax[0][1] = tax[0][2] = 0;
+  }
+  for (int k = 0; k < 3; k++) {
+    tax[1][k] = a[k] - b[k];
+    tax[2][k] = b[k] - c[k];
+    tax[3][k] = c[k] - a[k];
+  }
+  for (int i = 0; i < 3; i++)
+    if (sab_separated(sb[i], sab_points(ax[i], pts, 3))) return 0;
+  for (int i = 0; i < 3; i++)
+    if (sab_separated(sab_points(tax[i], pts, 3), sab_points(tax[i], (const double(*)[3])p, 8))) return 0;
+  for (int i = 0; i < 3; i++)
+    for (int i2 = 0; i2 < 4; i2++) {
+      double axis[3];
+      cross3(ax[i], tax[i2], axis);
+      if (sab_separated(sab_points(axis, pts, 3), sab_points(axis, (const double(*)[3])p, 8))) return 0;
+    }
+  return 1;
+}
+
+int orc_intersects_box(const orc_mesh* m, const double bmin[3], const double bmax[3], const double e[16]) {
+  const uint64_t n_tri = m->index ? m->index_count / 3 : m->vertex_count / 3;
+  for (uint64_t f = 0; f < n_tri; f++) {
+    double a[3], b[3], c[3];
+    face_vertices(m, (uint32_t)f, a, b, c);
+    if (orc_box_triangle_sat(bmin, bmax, e, a, b, c)) return 1;
+  }
+  return 0;
+}
+
+/* An independent formulation for checking the one above: the triangle brought into the box frame by
+   the inverse of an affine e (cofactors, double), then clipped against the six slabs
+   (Sutherland-Hodgman, points on a plane kept); intersecting iff something is left. */
+int orc_box_triangle_clip(const double bmin[3], const double bmax[3], const double e[16], const double a[3],
+                          const double b[3], const double c[3]) {
+  const double m00 = e[0], m01 = e[4], m02 = e[8], m10 = e[1], m11 = e[5], m12 = e[9], m20 = e[2], m21 = e[6],
+               m22 = e[10];
+  const double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
+  if (det == 0) return -1;
+  const double inv[3][3] = {{(m11 * m22 - m12 * m21) / det, (m02 * m21 - m01 * m22) / det, (m01 * m12 - m02 * m11) / det},
+                            {(m12 * m20 - m10 * m22) / det, (m00 * m22 - m02 * m20) / det, (m02 * m10 - m00 * m12) / det},
+                            {(m10 * m21 - m11 * m20) / det, (m01 * m20 - m00 * m21) / det, (m00 * m11 - m01 * m10) / det}};
+  double poly[16][3], tmp[16][3];
+  int n = 3;
+  const double* src[3] = {a, b, c};
+  for (int v = 0; v < 3; v++) {
+    const double d[3] = {src[v][0] - e[12], src[v][1] - e[13], src[v][2] - e[14]};
+    for (int k = 0; k < 3; k++) poly[v][k] = inv[k][0] * d[0] + inv[k][1] * d[1] + inv[k][2] * d[2];
+  }
+  for (int plane = 0; plane < 6 && n > 0; plane++) {
+    const int k = plane % 3;
+    const int keep_above = plane < 3;  /* x >= min, then x <= max */
+    const double lim = keep_above ? bmin[k] : bmax[k];
+    int m = 0;
+    for (int i = 0; i < n; i++) {
+      const double* P = poly[i];
+      const double* Q = poly[(i + 1) % n];
+      const double dp = keep_above ? P[k] - lim : lim - P[k];
+      const double dq = keep_above ? Q[k] - lim : lim - Q[k];
+      if (dp >= 0) {
+        for (int j = 0; j < 3; j++) tmp[m][j] = P[j];
+        m++;
+      }
+      if ((dp >= 0) != (dq >= 0)) {
+        const double t = dp / (dp - dq);
+        for (int j = 0; j < 3; j++) tmp[m][j] = P[j] + (Q[j] - P[j]) * t;
+        m++;
+      }
+    }
+    n = m;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < 3; j++) poly[i][j] = tmp[i][j];
+  }
+  return n > 0;
+}

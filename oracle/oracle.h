@@ -103,6 +103,14 @@ void orc_refit_root(const orc_mesh* m, uint8_t* nodes, uint64_t n_bytes);
  * sphereIntersectTriangle (src/math/MathUtilities.js:160-204): segment closest points, then the
  * plane distance and projected-point containment. */
 int orc_intersects_sphere(const orc_mesh* m, const float center[3], double radius);
+/* intersectsBox: any triangle intersecting the box (min / max in its frame, e = box -> mesh, 16 doubles
+   column-major), OrientedBox's separating-axis test in double (brute force). */
+int orc_intersects_box(const orc_mesh* m, const double bmin[3], const double bmax[3], const double e[16]);
+int orc_box_triangle_sat(const double bmin[3], const double bmax[3], const double e[16], const double a[3],
+                         const double b[3], const double c[3]);
+/* The same question by clipping the triangle to the box in the box frame (affine e; -1 if singular). */
+int orc_box_triangle_clip(const double bmin[3], const double bmax[3], const double e[16], const double a[3],
+                          const double b[3], const double c[3]);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,12 @@ def library():
         lib.orc_point_face_distance.restype = f64
         lib.orc_intersects_sphere.argtypes = [vp, vp, f64]
         lib.orc_intersects_sphere.restype = C.c_int
+        lib.orc_intersects_box.argtypes = [vp, vp, vp, vp]
+        lib.orc_intersects_box.restype = C.c_int
+        lib.orc_box_triangle_sat.argtypes = [vp, vp, vp, vp, vp, vp]
+        lib.orc_box_triangle_sat.restype = C.c_int
+        lib.orc_box_triangle_clip.argtypes = [vp, vp, vp, vp, vp, vp]
+        lib.orc_box_triangle_clip.restype = C.c_int
         lib.orc_refit_root.argtypes = [vp, vp, C.c_uint64]
         lib.orc_refit_root.restype = None
         lib.orc_validate_root.argtypes = [vp, u32, C.POINTER(TreeStats)]
@@ -184,6 +190,14 @@ class OracleMesh:
         r = np.asarray(radii, dtype=np.float64).reshape(-1)
         return np.array([bool(library().orc_intersects_sphere(self.ref, _p(c[i]), float(r[i]))) for i in range(len(c))])
 
+    def intersects_box(self, box_min, box_max, box_to_mesh):
+        """intersectsBox per box (brute force, the reference's OrientedBox separating-axis test)."""
+        mn = np.ascontiguousarray(box_min, np.float64).reshape(-1, 3)
+        mx = np.ascontiguousarray(box_max, np.float64).reshape(-1, 3)
+        m = np.ascontiguousarray(box_to_mesh, np.float64).reshape(-1, 16)
+        return np.array([bool(library().orc_intersects_box(self.ref, _p(mn[i]), _p(mx[i]), _p(m[i])))
+                         for i in range(mn.shape[0])])
+
     def refit(self):
         """The roots after MeshBVH.refit over this mesh's positions (new arrays; self.roots unchanged)."""
         out = []
@@ -279,3 +293,12 @@ def webgpu_nodes(roots: Sequence[np.ndarray], geometry_offset: int = 0) -> np.nd
         u[leaf, 6] += np.uint32(geometry_offset)
         parts.append(u.view(np.uint8).reshape(-1))
     return np.concatenate(parts) if parts else np.zeros(0, np.uint8)
+
+
+def box_triangle(box_min, box_max, box_to_mesh, a, b, c, clip=False) -> int:
+    """One box against one triangle: OrientedBox's separating-axis test (clip=False) or the independent
+    clipping formulation (clip=True; -1 for a singular matrix)."""
+    f = library().orc_box_triangle_clip if clip else library().orc_box_triangle_sat
+    arr = lambda v, n: np.ascontiguousarray(v, np.float64).reshape(n)
+    return int(f(_p(arr(box_min, 3)), _p(arr(box_max, 3)), _p(arr(box_to_mesh, 16)), _p(arr(a, 3)), _p(arr(b, 3)),
+                 _p(arr(c, 3))))

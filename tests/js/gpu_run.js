@@ -215,6 +215,37 @@ assert.ok(batchedChecked > 64 && presenceDiffers <= 2, `batched BatchedMesh hits
 api.disposeBatchedBoundsTree.call(Object.assign(batched, { boundsTrees: [bvh] }), -1);
 assert.strictEqual(batched._mbvhBatch, null);
 
+// --- intersectsBox: a box around the whole mesh, one far away, and the batch form against the JS
+// restatement of the reference (oracle/js) -------------------------------------------------------
+{
+  const R = require(path.join(__dirname, '..', '..', 'oracle', 'js', 'reference_cpu.js'));
+  const I = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1];
+  let lo = [Infinity, Infinity, Infinity], hi = [-Infinity, -Infinity, -Infinity];
+  for (let v = 0; v < positions.length; v += 3) for (let k = 0; k < 3; k++) {
+    lo[k] = Math.min(lo[k], positions[v + k]); hi[k] = Math.max(hi[k], positions[v + k]);
+  }
+  const V = (x, y, z) => ({ x, y, z });
+  assert.strictEqual(bvh.intersectsBox({ min: V(lo[0] - 1, lo[1] - 1, lo[2] - 1), max: V(hi[0] + 1, hi[1] + 1, hi[2] + 1) }, { elements: I }), true);
+  assert.strictEqual(bvh.intersectsBox({ min: V(1e6, 1e6, 1e6), max: V(1e6 + 1, 1e6 + 1, 1e6 + 1) }, I), false);
+  const nb = 400, mins = new Float64Array(3 * nb), maxs = new Float64Array(3 * nb), mats = new Float64Array(16 * nb);
+  let seed = 7;
+  const rnd = () => ((seed = (seed * 1103515245 + 12345) >>> 0) / 4294967296);
+  for (let i = 0; i < nb; i++) {
+    const ang = 6.283 * rnd(), c = Math.cos(ang), s = Math.sin(ang);
+    mats.set([c, s, 0, 0, -s, c, 0, 0, 0, 0, 1, 0, lo[0] + (hi[0] - lo[0]) * rnd(), lo[1] + (hi[1] - lo[1]) * rnd(), lo[2] + (hi[2] - lo[2]) * rnd(), 1], 16 * i);
+    for (let k = 0; k < 3; k++) { const h = 0.02 * (hi[k] - lo[k]) * (0.2 + rnd()); mins[3 * i + k] = -h; maxs[3 * i + k] = h; }
+  }
+  const got = bvh.intersectsBoxBatch(mins, maxs, mats);
+  const rm = R.makeMesh(positions, geometry.index.array, bvh._roots);
+  let nHit = 0;
+  for (let i = 0; i < nb; i++) {
+    const ref = R.intersectsBox(rm, mins.subarray(3 * i, 3 * i + 3), maxs.subarray(3 * i, 3 * i + 3), mats.subarray(16 * i, 16 * i + 16));
+    assert.strictEqual(got[i] === 1, ref, `box ${i}`);
+    nHit += ref ? 1 : 0;
+  }
+  assert.ok(nHit > 10 && nHit < nb - 10, `box hits ${nHit}`);
+}
+
 // --- MeshBVHUniformStruct / WebGPU node buffer of the main BVH ----------------------------------
 {
   const u = new api.MeshBVHUniformStruct();

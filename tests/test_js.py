@@ -98,3 +98,36 @@ def test_js_refit_restatement_matches_c_oracle(tmp_path):
     js = np.fromfile(tmp_path / "refit0.bin", np.uint8)
     assert np.array_equal(js, O.OracleMesh(pos, idx, tree.roots).refit()[0])
     assert not np.array_equal(js, tree.roots[0])
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_js_box_restatement_matches_c_oracle(tmp_path):
+    """The JavaScript restatement of intersectsBox (timed as the CPU baseline) gives the C oracle's answers."""
+    import json
+    import sys
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import mbvh_amd as M
+    import oracle_py as O
+    from mbvh_amd import geometry as G
+    from test_gpu_box import _boxes
+
+    g = G.torus_knot(1, 0.3, 80, 12)
+    idx = g.index.copy()
+    tree = M.build(g.positions, idx)
+    lo, hi = g.bounds()
+    mn, mx, mats = _boxes(1500, lo, hi, 3, size=(0.003, 0.12))
+    g.positions.tofile(tmp_path / "positions.f32")
+    idx.tofile(tmp_path / ("index.u16" if idx.dtype == np.uint16 else "index.u32"))
+    tree.roots[0].tofile(tmp_path / "root0.bin")
+    mn.tofile(tmp_path / "box_min.f64")
+    mx.tofile(tmp_path / "box_max.f64")
+    mats.tofile(tmp_path / "box_matrix.f64")
+    (tmp_path / "meta.json").write_text(json.dumps({"roots": 1, "index_bytes": idx.itemsize, "boxes": len(mn)}))
+    r = subprocess.run(["node", os.path.join(ROOT, "oracle", "js", "box_bench.js"), str(tmp_path)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    js = np.fromfile(tmp_path / "box_hit.u8", np.uint8).astype(bool)
+    ref = O.OracleMesh(g.positions, idx, tree.roots).intersects_box(mn, mx, mats)
+    assert np.array_equal(js, ref) and 0.05 < ref.mean() < 0.95

@@ -2556,3 +2556,71 @@ extern "C" int mbvh_mesh_export_nodes(mbvh_ctx* ctx, mbvh_mesh* mesh, uint32_t* 
   MBVH_HIP(hipStreamSynchronize(s));
   return MBVH_OK;
 }
+
+// ---- intersectsBox ---------------------------------------------------------------------------------
+
+namespace {
+
+int enqueue_boxes(mbvh_ctx* ctx, const mbvh_mesh* mesh, const double* mn, const double* mx, const double* mats,
+                  uint64_t n, uint8_t* out, hipStream_t s) {
+  if (mesh->max_stack > 8 + ctx->stack_ovf_entries)
+    return set_error(MBVH_ERR_LIMIT, "intersectsBox: stack scratch smaller than the tree depth");
+  BoxArgs a{};
+  a.mesh = view_of(mesh);
+  a.box_min = mn;
+  a.box_max = mx;
+  a.matrices = mats;
+  a.n = n;
+  a.out = out;
+  const uint64_t threads = launch_threads(n, ctx->max_threads);
+  a.stack = stack_of(ctx, threads);
+  MBVH_HIP(launch_intersects_box(a, threads, s));
+  return MBVH_OK;
+}
+
+}  // namespace
+
+extern "C" int mbvh_intersects_box_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const double* box_min, const double* box_max,
+                                     const double* box_to_mesh, uint64_t n, uint8_t* out, void* stream) {
+  clear_error();
+  const char* fn = "mbvh_intersects_box_d";
+  int rc = check_query(fn, ctx, mesh);
+  if (rc) return rc;
+  if (n && (!box_min || !box_max || !box_to_mesh || !out))
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL box, matrix or output buffer", fn);
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = reserve_stack(ctx, mesh->max_stack))) return rc;
+  return enqueue_boxes(ctx, mesh, box_min, box_max, box_to_mesh, n, out,
+                       stream ? static_cast<hipStream_t>(stream) : ctx->stream);
+}
+
+extern "C" int mbvh_intersects_box(mbvh_ctx* ctx, const mbvh_mesh* mesh, const double* box_min, const double* box_max,
+                                   const double* box_to_mesh, uint64_t n, uint8_t* out) {
+  clear_error();
+  const char* fn = "mbvh_intersects_box";
+  int rc = check_query(fn, ctx, mesh);
+  if (rc) return rc;
+  if (n == 0) return MBVH_OK;
+  if (!box_min || !box_max || !box_to_mesh || !out)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL box, matrix or output buffer", fn);
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = reserve_stack(ctx, mesh->max_stack))) return rc;
+  hipStream_t s = ctx->stream;
+  void *dmn, *dmx, *dm, *dout;
+  if ((rc = stage_in(ctx, ctx->staging[0], box_min, 3 * n * sizeof(double), &dmn)) ||
+      (rc = stage_in(ctx, ctx->staging[1], box_max, 3 * n * sizeof(double), &dmx)) ||
+      (rc = stage_in(ctx, ctx->staging[2], box_to_mesh, 16 * n * sizeof(double), &dm)) ||
+      (rc = stage_out(ctx->staging[7], out, n, &dout)))
+    return rc;
+  if ((rc = enqueue_boxes(ctx, mesh, static_cast<const double*>(dmn), static_cast<const double*>(dmx),
+                          static_cast<const double*>(dm), n, static_cast<uint8_t*>(dout), s)))
+    return rc;
+  MBVH_HIP(hipMemcpyAsync(out, dout, n, hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipStreamSynchronize(s));
+  for (uint64_t i = 0; i < n; i++)
+    if (out[i] == 2)
+      return set_error(MBVH_ERR_LIMIT, "%s: traversal stack overflow on box %llu", fn, static_cast<unsigned long long>(i));
+  return MBVH_OK;
+}
+

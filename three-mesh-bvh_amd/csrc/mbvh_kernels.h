@@ -242,6 +242,18 @@ struct SceneArgs {
 };
 hipError_t launch_scene_raycast_first(const SceneArgs& a, uint64_t threads, hipStream_t s);
 
+// ---- intersectsBox ----------------------------------------------------------------------------------
+struct BoxArgs {
+  MeshView mesh;
+  StackScratch stack;
+  const double* box_min;    // 3 per box
+  const double* box_max;    // 3 per box
+  const double* matrices;   // 16 per box: box -> mesh frame (column-major)
+  uint64_t n;
+  uint8_t* out;             // 1 intersects, 0 not, 2 stack overflow
+};
+hipError_t launch_intersects_box(const BoxArgs& a, uint64_t threads, hipStream_t s);
+
 // ---- export to the reference's shader data layouts ------------------------------------------------
 struct ExportArgs {
   const uint32_t* nodes;     // reference-format nodes of one root (8 words each)

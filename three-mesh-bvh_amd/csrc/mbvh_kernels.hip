@@ -1685,6 +1685,215 @@ __global__ __launch_bounds__(kBlock) void k_scene_raycast_first(SceneArgs A) {
   }
 }
 
+// ---- intersectsBox: an oriented box against the mesh ----------------------------------------------
+// The reference decides it in JS numbers (IEEE double) with separating-axis tests
+// (OrientedBox.update / intersectsBox / intersectsTriangle src/math/OrientedBox.js:79-260,
+// SeparatingAxisBounds src/math/SeparatingAxisBounds.js, ExtendedTriangle.update
+// src/math/ExtendedTriangle.js:47-75 and three's Vector3 / Triangle arithmetic). The same expressions
+// are evaluated here in double, in the same order and without contraction, so every projection and
+// every comparison has the reference's value. The node test only uses six axes, so it never rejects a
+// node that holds an intersecting triangle; the answer is whether any triangle intersects.
+
+struct D3 {
+  double x, y, z;
+};
+__device__ __forceinline__ D3 d3(double x, double y, double z) { return {x, y, z}; }
+__device__ __forceinline__ D3 dsub(D3 a, D3 b) {
+  MBVH_NO_CONTRACT
+  return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+__device__ __forceinline__ double ddot(D3 a, D3 b) {  // Vector3.dot
+  MBVH_NO_CONTRACT
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ __forceinline__ D3 dcross(D3 a, D3 b) {  // Vector3.crossVectors
+  MBVH_NO_CONTRACT
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double dcomp(D3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+// JavaScript Math.min / Math.max: NaN if either operand is NaN; -0 below +0.
+__device__ __forceinline__ double js_min(double a, double b) {
+  if (a != a || b != b) return __builtin_nan("");
+  if (a == b) return __builtin_signbit(a) ? a : b;
+  return a < b ? a : b;
+}
+__device__ __forceinline__ double js_max(double a, double b) {
+  if (a != a || b != b) return __builtin_nan("");
+  if (a == b) return __builtin_signbit(a) ? b : a;
+  return a > b ? a : b;
+}
+
+struct SepBounds {  // SeparatingAxisBounds
+  double mn, mx;
+};
+// setFromPoints: min / max of the projections with the ternaries (a NaN projection is skipped)
+__device__ __forceinline__ SepBounds sep_points(D3 axis, const D3* pts, int n) {
+  SepBounds b{__builtin_huge_val(), -__builtin_huge_val()};
+  for (int i = 0; i < n; i++) {
+    const double v = ddot(axis, pts[i]);
+    b.mn = v < b.mn ? v : b.mn;
+    b.mx = v > b.mx ? v : b.mx;
+  }
+  return b;
+}
+__device__ __forceinline__ bool separated(SepBounds a, SepBounds b) { return a.mn > b.mx || b.mn > a.mx; }
+
+struct Obb {  // OrientedBox after update()
+  D3 p[8];
+  D3 ax[3];
+  SepBounds sb[3];
+  SepBounds ab[3];  // alignedSatBounds
+};
+
+__device__ void obb_update(const double* bmin, const double* bmax, const double* e, Obb& o) {
+  MBVH_NO_CONTRACT
+  for (int x = 0; x <= 1; x++) {
+    for (int y = 0; y <= 1; y++) {
+      for (int z = 0; z <= 1; z++) {
+        const int i = x | (y << 1) | (z << 2);
+        const double vx = x ? bmax[0] : bmin[0], vy = y ? bmax[1] : bmin[1], vz = z ? bmax[2] : bmin[2];
+        const double w = 1 / (e[3] * vx + e[7] * vy + e[11] * vz + e[15]);  // Vector3.applyMatrix4
+        o.p[i] = d3((e[0] * vx + e[4] * vy + e[8] * vz + e[12]) * w, (e[1] * vx + e[5] * vy + e[9] * vz + e[13]) * w,
+                    (e[2] * vx + e[6] * vy + e[10] * vz + e[14]) * w);
+      }
+    }
+  }
+  for (int i = 0; i < 3; i++) {
+    o.ax[i] = dsub(o.p[0], o.p[1 << i]);
+    o.sb[i] = sep_points(o.ax[i], o.p, 8);
+  }
+  for (int k = 0; k < 3; k++) {  // setFromPointsField
+    SepBounds b{__builtin_huge_val(), -__builtin_huge_val()};
+    for (int i = 0; i < 8; i++) {
+      const double v = dcomp(o.p[i], k);
+      b.mn = v < b.mn ? v : b.mn;
+      b.mx = v > b.mx ? v : b.mx;
+    }
+    o.ab[k] = b;
+  }
+}
+
+// OrientedBox.intersectsBox against a node box (float bounds as JS numbers).
+__device__ bool obb_box(const Obb& o, const double* mn, const double* mx) {
+  MBVH_NO_CONTRACT
+  for (int k = 0; k < 3; k++)
+    if (separated(o.ab[k], SepBounds{mn[k], mx[k]})) return false;
+  for (int i = 0; i < 3; i++) {
+    SepBounds b{__builtin_huge_val(), -__builtin_huge_val()};  // setFromBox: corners, Math.min / Math.max
+    for (int x = 0; x <= 1; x++) {
+      for (int y = 0; y <= 1; y++) {
+        for (int z = 0; z <= 1; z++) {
+          const D3 q = d3(mn[0] * x + mx[0] * (1 - x), mn[1] * y + mx[1] * (1 - y), mn[2] * z + mx[2] * (1 - z));
+          const double v = ddot(o.ax[i], q);
+          b.mn = js_min(v, b.mn);
+          b.mx = js_max(v, b.mx);
+        }
+      }
+    }
+    if (separated(o.sb[i], b)) return false;
+  }
+  return true;
+}
+
+// OrientedBox.intersectsTriangle with the triangle's ExtendedTriangle.update axes: the normal
+// (Triangle.getNormal) and the edge vectors a - b, b - c, c - a.
+__device__ bool obb_triangle(const Obb& o, D3 a, D3 b, D3 c) {
+  MBVH_NO_CONTRACT
+  const D3 pts[3] = {a, b, c};
+  D3 tax[4];
+  D3 nrm = dcross(dsub(c, b), dsub(a, b));
+  const double len2 = nrm.x * nrm.x + nrm.y * nrm.y + nrm.z * nrm.z;
+  if (len2 > 0) {
+    const double s = 1 / sqrt(len2);
+    nrm = d3(nrm.x * s, nrm.y * s, nrm.z * s);
+  } else {
+    nrm = d3(0, 0, 0);
+  }
+  tax[0] = nrm;
+  tax[1] = dsub(a, b);
+  tax[2] = dsub(b, c);
+  tax[3] = dsub(c, a);
+  for (int i = 0; i < 3; i++)
+    if (separated(o.sb[i], sep_points(o.ax[i], pts, 3))) return false;
+  for (int i = 0; i < 3; i++)
+    if (separated(sep_points(tax[i], pts, 3), sep_points(tax[i], o.p, 8))) return false;
+  for (int i = 0; i < 3; i++) {
+    for (int i2 = 0; i2 < 4; i2++) {
+      const D3 axis = dcross(o.ax[i], tax[i2]);
+      if (separated(sep_points(axis, pts, 3), sep_points(axis, o.p, 8))) return false;
+    }
+  }
+  return true;
+}
+
+__device__ __forceinline__ void node_box_d(float a, float b, float c, float d, float e, float f, double* mn, double* mx) {
+  mn[0] = a;
+  mn[1] = b;
+  mn[2] = c;
+  mx[0] = d;
+  mx[1] = e;
+  mx[2] = f;
+}
+
+// One lane per box: depth-first any-hit traversal of the binary tree (node boxes by the six-axis test,
+// triangles by the full test), stopping at the first intersecting triangle.
+__global__ __launch_bounds__(kBlock) void k_intersects_box(BoxArgs A) {
+  __shared__ uint2 lds_stack[8 * kBlock];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + tid;
+  const MeshView M = A.mesh;
+  StackL<8> st;
+  st.lds = lds_stack;
+  st.ovf = A.stack.ovf;
+  st.ovf_stride = A.stack.ovf_stride;
+  st.gtid = gtid;
+  st.lane = tid;
+  st.capacity = 8 + static_cast<int>(A.stack.ovf_entries);
+  for (uint64_t q = gtid; q < A.n; q += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    Obb o;
+    obb_update(A.box_min + 3 * q, A.box_max + 3 * q, A.matrices + 16 * q, o);
+    uint8_t hit = 0;
+    bool overflow = false;
+    for (uint32_t r = 0; r < M.n_roots && !hit && !overflow; r++) {
+      st.sp = 0;
+      uint32_t ref = M.roots[r];
+      while (true) {
+        if (ref & kRefLeaf) {
+          uint32_t off, cnt;
+          leaf_range(ref, M.bigleaf, off, cnt);
+          for (uint32_t s = off; s < off + cnt && !hit; s++) {
+            const Tri t = load_tri(M.tris, s);
+            if (obb_triangle(o, d3(t.p0.x, t.p0.y, t.p0.z), d3(t.p1.x, t.p1.y, t.p1.z), d3(t.p2.x, t.p2.y, t.p2.z)))
+              hit = 1;
+          }
+          if (hit) break;
+          ref = kDone;
+        } else if (ref != kDone) {
+          const WideNode nd = M.nodes[ref];
+          double mn[3], mx[3];
+          node_box_d(nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y, mn, mx);
+          const bool hl = obb_box(o, mn, mx);
+          node_box_d(nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w, mn, mx);
+          const bool hr = obb_box(o, mn, mx);
+          if (hl && hr) {
+            if (!st.push(nd.meta.y, 0u)) {
+              overflow = true;
+              break;
+            }
+            ref = nd.meta.x;
+          } else {
+            ref = hl ? nd.meta.x : (hr ? nd.meta.y : kDone);
+          }
+          continue;
+        }
+        if (st.sp == 0) break;
+        ref = st.pop().x;
+      }
+    }
+    A.out[q] = overflow ? 2 : hit;
+  }
+}
+
 __global__ void k_apply_side_mask(float4* tris, uint64_t n_slots, const uint8_t* mask) {
   const uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n_slots) return;
@@ -1741,6 +1950,12 @@ hipError_t launch_batch_raycast_first(const BatchArgs& a, uint32_t stack_lds, ui
 hipError_t launch_scene_raycast_first(const SceneArgs& a, uint64_t threads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_scene_raycast_first, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_intersects_box(const BoxArgs& a, uint64_t threads, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_intersects_box, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 

@@ -568,6 +568,27 @@ static napi_value js_scene_raycast_first(napi_env env, napi_callback_info info) 
   return NULL;
 }
 
+/* intersectsBox(mesh, boxMin F64 (3n), boxMax F64 (3n), boxToMesh F64 (16n), out U8 (n)) */
+static napi_value js_intersects_box(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  get_args(env, info, 5, argv);
+  MeshHandle* h = get_mesh(env, argv[0]);
+  if (!h) return throw_msg(env, "intersectsBox: invalid mesh");
+  View mn, mx, m, out;
+  if (!get_view(env, argv[1], &mn) || !get_view(env, argv[2], &mx) || !get_view(env, argv[3], &m) ||
+      !get_view(env, argv[4], &out) || mn.type != napi_float64_array || mx.type != napi_float64_array ||
+      m.type != napi_float64_array || out.elt != 1 || mn.length != 3 * out.length || mx.length != mn.length ||
+      m.length != 16 * out.length)
+    return throw_msg(env, "intersectsBox: boxes must be Float64Arrays (3n, 3n, 16n) and out a Uint8Array (n)");
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, h->device, &ctx);
+  if (!rc)
+    rc = mbvh_intersects_box(ctx, h->mesh, (const double*)mn.data, (const double*)mx.data, (const double*)m.data,
+                             out.length, (uint8_t*)out.data);
+  if (rc) return throw_status(env, rc, "mbvh_intersects_box");
+  return NULL;
+}
+
 /* textureSizes(mesh) -> [bounds, contents, index, position] texture sides */
 static napi_value js_texture_sizes(napi_env env, napi_callback_info info) {
   napi_value argv[1];
@@ -964,6 +985,7 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("batchRaycastFirst", js_batch_raycast_first);
   EXPORT("sceneCreate", js_scene_create);
   EXPORT("textureSizes", js_texture_sizes);
+  EXPORT("intersectsBox", js_intersects_box);
   EXPORT("exportTextures", js_export_textures);
   EXPORT("exportNodes", js_export_nodes);
   EXPORT("freeScene", js_free_batch);

@@ -38,7 +38,7 @@ EXPORTED_SYMBOLS = (
     "mbvh_order_workspace_bytes", "mbvh_order_points_d",
     "mbvh_batch_create", "mbvh_batch_free", "mbvh_batch_workspace_bytes", "mbvh_batch_raycast_first_d",
     "mbvh_batch_raycast_first", "mbvh_scene_create", "mbvh_scene_free", "mbvh_scene_raycast_first",
-    "mbvh_scene_raycast_first_d", "mbvh_mesh_texture_sizes", "mbvh_mesh_export_textures",
+    "mbvh_scene_raycast_first_d", "mbvh_intersects_box", "mbvh_intersects_box_d", "mbvh_mesh_texture_sizes", "mbvh_mesh_export_textures",
     "mbvh_mesh_export_textures_d", "mbvh_mesh_export_nodes", "mbvh_mesh_export_nodes_d",
     "mbvh_batch_autotune", "mbvh_batch_graph_create", "mbvh_batch_graph_launch", "mbvh_batch_graph_free",
     "mbvh_instance_rays_d", "mbvh_instance_hits_d",
@@ -128,6 +128,8 @@ def library() -> C.CDLL:
         "mbvh_scene_free": (None, [vp]),
         "mbvh_scene_raycast_first_d": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp, vp]),
         "mbvh_scene_raycast_first": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp]),
+        "mbvh_intersects_box": (C.c_int, [vp, vp, vp, vp, vp, u64, vp]),
+        "mbvh_intersects_box_d": (C.c_int, [vp, vp, vp, vp, vp, u64, vp, vp]),
         "mbvh_mesh_texture_sizes": (C.c_int, [vp, vp]),
         "mbvh_mesh_export_textures": (C.c_int, [vp, vp, vp, u32, u32, vp, vp, vp, vp]),
         "mbvh_mesh_export_textures_d": (C.c_int, [vp, vp, vp, u32, u32, vp, vp, vp, vp, vp]),
@@ -345,6 +347,19 @@ class Mesh:
         """Refit from device positions, enqueued on `stream` (no synchronisation)."""
         _check(self.ctx._lib.mbvh_mesh_refit_d(self.ctx.handle, self.handle, C.c_void_p(positions_ptr), vertex_count,
                                                stride, offset, C.c_void_p(stream or None)))
+
+    def intersects_box(self, box_min: np.ndarray, box_max: np.ndarray, box_to_mesh: np.ndarray) -> np.ndarray:
+        """Per box: whether any triangle intersects it (min / max in the box frame, 4x4 column-major
+        box -> mesh matrices), as the reference's OrientedBox tests decide it."""
+        mn = np.ascontiguousarray(box_min, np.float64).reshape(-1, 3)
+        mx = np.ascontiguousarray(box_max, np.float64).reshape(-1, 3)
+        m = np.ascontiguousarray(box_to_mesh, np.float64).reshape(-1, 16)
+        if mn.shape != mx.shape or m.shape[0] != mn.shape[0]:
+            raise ValueError("inconsistent box inputs")
+        out = np.empty(mn.shape[0], np.uint8)
+        _check(self.ctx._lib.mbvh_intersects_box(self.ctx.handle, self.handle, _ptr(mn), _ptr(mx), _ptr(m),
+                                                 mn.shape[0], _ptr(out)))
+        return out.astype(bool)
 
     def texture_sizes(self):
         """Texture sides (bounds, contents, index, position) of MeshBVHUniformStruct for this mesh."""

@@ -280,6 +280,23 @@ class MeshBVH {
 
   // intersectsSphere for n spheres: centers a Float32Array of length 3n, radii one of length n.
   // Returns a Uint8Array (1: some triangle within the radius, inclusive).
+  // intersectsBox (src/core/MeshBVH.js:780-792): box = { min, max } in its own frame, boxToMesh its
+  // Matrix4 (or 16 numbers, column-major) into the mesh frame. Decided on the GPU in double with the
+  // reference's separating-axis tests.
+  intersectsBox(box, boxToMesh) {
+    const e = boxToMesh && boxToMesh.elements ? boxToMesh.elements : boxToMesh;
+    return this.intersectsBoxBatch(new Float64Array([box.min.x, box.min.y, box.min.z]),
+      new Float64Array([box.max.x, box.max.y, box.max.z]), Float64Array.from(e))[0] === 1;
+  }
+
+  // Many boxes: Float64Arrays of 3n mins, 3n maxs and 16n matrices -> Uint8Array (1 intersects).
+  intersectsBoxBatch(mins, maxs, matrices) {
+    const out = new Uint8Array(mins.length / 3);
+    native.load().intersectsBox(this._mesh(), Float64Array.from(mins), Float64Array.from(maxs),
+      Float64Array.from(matrices), out);
+    return out;
+  }
+
   intersectsSphereBatch(centers, radii) {
     const c = centers instanceof Float32Array ? centers : Float32Array.from(centers);
     const r = radii instanceof Float32Array ? radii : Float32Array.from(radii);
