@@ -47,6 +47,8 @@ def parse():
                     help="skip the differentiable (C3, C4) and batched (C5) measurements")
     ap.add_argument("--only-build", action="store_true",
                     help="time the device BVH build of the 1M-triangle icosphere alone (for profiling) and exit")
+    ap.add_argument("--only-geometry", action="store_true",
+                    help="time closestPointToGeometry on the closest-point mesh alone (for profiling) and exit")
     ap.add_argument("--c3-rays", type=int, default=1 << 20)
     ap.add_argument("--c4-grid", type=int, default=256)
     ap.add_argument("--c5-rays", type=int, default=1 << 22)
@@ -306,6 +308,55 @@ def bench_refit(torch, stream, mesh, g, idx, tree):
         finally:
             shutil.rmtree(d, ignore_errors=True)
     return res
+
+
+def bench_geometry(torch, M, ctx, stream, mesh, g, idx, tree):
+    """closestPointToGeometry on the closest-point mesh: one whole query per call (the device entry point,
+    events on the launch stream) for a separated placement (a 16k-triangle sphere in the knot's hole) and
+    an interpenetrating one (an 80k-triangle knot shifted through it), against the oracle's tree walk on
+    one host thread over the same query; the answers are compared."""
+    import numpy as np
+    from mbvh_amd import geometry as G
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    dev = torch.device("cuda:0")
+    sptr = stream.cuda_stream
+    om = O.OracleMesh(g.positions, idx, tree.roots)
+    out = {}
+    for name, other, off in (("separated", G.sphere(1, 128, 64), (0.0, 0.0, 0.0)),
+                             ("interpenetrating", G.torus_knot(10, 3, 400, 100), (29.5, 0.0, 0.0))):
+        other.ensure_index()
+        oi = other.index.astype(np.uint32)
+        m = np.eye(4)
+        m[:3, 3] = off
+        e = np.ascontiguousarray(m.T.reshape(-1))
+        pos = torch.from_numpy(other.positions.reshape(-1).copy()).to(dev)
+        di = torch.from_numpy(oi.view(np.int32)).to(dev)
+        rec = torch.zeros(M.GEOMETRY_HIT.itemsize, dtype=torch.uint8, device=dev)
+        torch.cuda.synchronize()
+
+        def query():
+            rc = ctx._lib.mbvh_closest_point_to_geometry_d(ctx.handle, mesh.handle, pos.data_ptr(),
+                                                           other.positions.shape[0], 3, 0, di.data_ptr(), 4,
+                                                           other.tri_count, e.ctypes.data, 0.0, float("inf"),
+                                                           rec.data_ptr(), sptr)
+            if rc:
+                raise RuntimeError(f"closestPointToGeometry failed ({rc})")
+
+        d_ms = event_ms(torch, stream, query, 20)
+        r = np.frombuffer(rec.cpu().numpy().tobytes(), M.GEOMETRY_HIT)[0]
+        t0 = time.perf_counter()
+        ref = om.closest_to_geometry(other.positions, oi, e)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        same = ref is not None and abs(float(r["distance"]) - ref["distance"]) <= 1e-9 * max(ref["distance"], 1.0)
+        out[name] = {"other_tris": other.tri_count, "gpu_device_ms": round(d_ms, 4), "distance": float(r["distance"]),
+                     "cpu_oracle_ms_1thread": round(cpu_ms, 2), "speedup_vs_cpu": round(cpu_ms / d_ms, 1),
+                     "same_distance_as_oracle": bool(same),
+                     "same_pair_as_oracle": bool(ref is not None and int(r["face_index"]) == ref["face_index"] and
+                                                 int(r["other_face_index"]) == ref["other_face_index"])}
+    out["workload"] = (f"closestPointToGeometry on the closest-point mesh ({g.tri_count} tris): a sphere (radius 1, "
+                       f"16,128 tris) in the knot's hole, and TorusKnot(10,3,400,100) shifted by 29.5 in x")
+    return out
 
 
 def bench_boxes(torch, ctx, stream, mesh, g, idx, tree, n=1 << 20):
@@ -715,6 +766,17 @@ def main():
     torch.cuda.set_stream(stream)
     sptr = stream.cuda_stream
 
+    if args.only_geometry:
+        from mbvh_amd import geometry as G
+        ctx = M.Context(0)
+        g = G.torus_knot(10, 3, 800, 200).ensure_index()
+        idx = g.index.copy()
+        tree = M.build(g.positions, idx, strategy=M.SAH, target_leaf_size=10)
+        mesh = ctx.upload(tree, g.positions, idx)
+        print(json.dumps({"closest_point_to_geometry": bench_geometry(torch, M, ctx, stream, mesh, g, idx, tree)}),
+              flush=True)
+        return
+
     if args.only_build:
         from mbvh_amd import geometry as G
         ctx = M.Context(0)
@@ -891,6 +953,8 @@ def main():
     if not args.only_primary:
         line["intersects_box"] = bench_boxes(torch, ctx, stream, mesh, g, idx, tree)
         log(f"boxes {line['intersects_box']}")
+        line["closest_point_to_geometry"] = bench_geometry(torch, M, ctx, stream, mesh, g, idx, tree)
+        log(f"geometry {line['closest_point_to_geometry']}")
         line["build_closest_mesh"] = bench_build(torch, M, ctx, g, "the closest-point mesh TorusKnot(10,3,800,200)")
         log(f"build {line['build_closest_mesh']}")
         line.update(bench_differentiable(torch, M, W, ctx, stream, args))

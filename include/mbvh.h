@@ -251,6 +251,42 @@ int mbvh_intersects_box_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const double* bo
                           const double* box_to_mesh, uint64_t n, uint8_t* out, void* stream);
 
 /*
+ * closestPointToGeometry (MeshBVH.closestPointToGeometry src/core/MeshBVH.js:840-853,
+ * src/core/cast/closestPointToGeometry.template.js:15-271 with ExtendedTriangle.distanceToTriangle
+ * src/math/ExtendedTriangle.js:774-860): the closest pair of triangles between the mesh and another
+ * geometry (float positions with `stride` floats between vertices and x at `offset`; an optional
+ * 2- or 4-byte index, NULL for a triangle soup; n_tris triangles) placed in the mesh frame by
+ * geometry_to_bvh (16 doubles, column-major, three's applyMatrix4). Distances in double. The answer
+ * is the closest pair overall (equal distances: the lowest other triangle, then the lowest mesh
+ * slot); a pair counts only when its distance is below max_threshold. min_threshold is accepted for
+ * the reference's signature: the reference stops at the first pair under it, and the overall
+ * closest pair is under it whenever that one is. `face_index` is the mesh triangle slot (the
+ * reference's closestDistanceTriIndex), `other_point` / `other_distance` are target2's point and
+ * distance in the other geometry's frame. No pair: distance +inf, faces 0xFFFFFFFF.
+ * Host-pointer form (synchronous) and device-pointer form (positions, index and `out` on the device;
+ * enqueues on `stream`; out->status 1 marks a traversal stack overflow, 2 an index entry beyond
+ * n_vertices). The context's workspace is shared: device-pointer calls on one context must not overlap.
+ */
+typedef struct mbvh_geometry_hit {
+  double distance;
+  double point[3];        /* on the mesh triangle, mesh frame (target1.point) */
+  double other_point[3];  /* on the other triangle, the other geometry's frame (target2.point) */
+  double other_distance;  /* target2.distance */
+  uint32_t face_index;    /* target1.faceIndex */
+  uint32_t other_face_index; /* target2.faceIndex */
+  int32_t status;
+  uint32_t pad;
+} mbvh_geometry_hit;
+int mbvh_closest_point_to_geometry(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* positions, uint64_t n_vertices,
+                                   uint32_t stride, uint32_t offset, const void* index, uint32_t index_elt,
+                                   uint64_t n_tris, const double* geometry_to_bvh, double min_threshold,
+                                   double max_threshold, mbvh_geometry_hit* out);
+int mbvh_closest_point_to_geometry_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* positions,
+                                     uint64_t n_vertices, uint32_t stride, uint32_t offset, const void* index,
+                                     uint32_t index_elt, uint64_t n_tris, const double* geometry_to_bvh,
+                                     double min_threshold, double max_threshold, mbvh_geometry_hit* out, void* stream);
+
+/*
  * All hits per ray. Replaces MeshBVH.raycast (src/core/MeshBVH.js:532-545) → raycast
  * (src/core/cast/raycast.template.js:7-53) → intersectTris (src/core/utils/iterationUtils.template.js:4-22).
  * The hits of ray i are entries offsets[i] .. offsets[i+1]-1, in the reference's traversal order

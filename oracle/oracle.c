@@ -980,3 +980,269 @@ int orc_box_triangle_clip(const double bmin[3], const double bmax[3], const doub
   }
   return n > 0;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* closestPointToGeometry                                                                     */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Line3.at: (end - start) * t + start */
+static void line_at(const double s[3], const double e[3], double t, double o[3]) {
+  for (int k = 0; k < 3; k++) o[k] = (e[k] - s[k]) * t + s[k];
+}
+/* Line3.closestPointToPoint(p, clamp = true) */
+static void line_closest(const double s[3], const double e[3], const double p[3], double o[3]) {
+  double sp[3], se[3];
+  v_sub(p, s, sp);
+  v_sub(e, s, se);
+  double t = v_dot(se, sp) / v_dot(se, se);
+  if (!isnan(t)) t = fmax(0.0, fmin(1.0, t)); /* MathUtils.clamp (Math.max / Math.min keep a NaN) */
+  line_at(s, e, t, o);
+}
+
+/* closestPointsSegmentToSegment (src/math/MathUtilities.js:58-150, after MathGeoLib's
+   LineSegment::ClosestPoint) with closestPointLineToLine (MathUtilities.js:3-56). */
+void orc_segment_segment(const double s1[3], const double e1[3], const double s2[3], const double e2[3], double o1[3],
+                         double o2[3]) {
+  double v02[3], v10[3], v32[3];
+  v_sub(s1, s2, v02);
+  v_sub(e1, s1, v10);
+  v_sub(e2, s2, v32);
+  const double d0232 = v_dot(v02, v32), d3210 = v_dot(v32, v10), d3232 = v_dot(v32, v32), d0210 = v_dot(v02, v10),
+               d1010 = v_dot(v10, v10);
+  const double denom = d1010 * d3232 - d3210 * d3210;
+  const double d = denom != 0 ? (d0232 * d3210 - d0210 * d3232) / denom : 0;
+  const double d2 = (d0232 + d * d3210) / d3232;
+  const int in1 = d >= 0 && d <= 1, in2 = d2 >= 0 && d2 <= 1;
+  if (in1 && in2) {
+    line_at(s1, e1, d, o1);
+    line_at(s2, e2, d2, o2);
+  } else if (in1) {
+    memcpy(o2, d2 < 0 ? s2 : e2, sizeof(double) * 3);
+    line_closest(s1, e1, o2, o1);
+  } else if (in2) {
+    memcpy(o1, d < 0 ? s1 : e1, sizeof(double) * 3);
+    line_closest(s2, e2, o1, o2);
+  } else {
+    const double* p = d < 0 ? s1 : e1;
+    const double* p2 = d2 < 0 ? s2 : e2;
+    double c1[3], c2[3];
+    line_closest(s1, e1, p2, c1);
+    line_closest(s2, e2, p, c2);
+    if (v_dist_sq(c1, p2) <= v_dist_sq(c2, p)) {
+      memcpy(o1, c1, sizeof(c1));
+      memcpy(o2, p2, sizeof(double) * 3);
+    } else {
+      memcpy(o1, p, sizeof(double) * 3);
+      memcpy(o2, c2, sizeof(c2));
+    }
+  }
+}
+
+/* Where segment pq meets triangle abc, by the parametric solve p + t (q - p) = a + u (b - a) + v (c - a)
+   (Cramer's rule as in Moller-Trumbore); 0 for a segment parallel to the triangle's plane. */
+static int segment_meets_triangle(const double p[3], const double q[3], const double a[3], const double b[3],
+                                  const double c[3], double x[3]) {
+  double e1[3], e2[3], d[3], h[3], s[3], qv[3];
+  v_sub(b, a, e1);
+  v_sub(c, a, e2);
+  v_sub(q, p, d);
+  v_cross(d, e2, h);
+  const double det = v_dot(e1, h);
+  if (det == 0) return 0;
+  const double f = 1 / det;
+  v_sub(p, a, s);
+  const double u = f * v_dot(s, h);
+  if (u < 0 || u > 1) return 0;
+  v_cross(s, e1, qv);
+  const double v = f * v_dot(d, qv);
+  if (v < 0 || u + v > 1) return 0;
+  const double t = f * v_dot(e2, qv);
+  if (t < 0 || t > 1) return 0;
+  for (int k = 0; k < 3; k++) x[k] = p[k] + d[k] * t;
+  return 1;
+}
+
+/* ExtendedTriangle.distanceToTriangle (src/math/ExtendedTriangle.js:774-860): 0 when the triangles
+   intersect (the points: the middle of the intersection segment, whose ends are where edges of one
+   triangle meet the other), else the least of the vertex-to-triangle and edge-to-edge distances in
+   the reference's order, strict improvements only. */
+double orc_triangle_distance(const double* t1, const double* t2, double o1[3], double o2[3]) {
+  const double* A[3] = {t1, t1 + 3, t1 + 6};
+  const double* B[3] = {t2, t2 + 3, t2 + 6};
+  double n1[3], n2[3], ea[3], eb[3], line[3];
+  v_sub(A[1], A[0], ea);
+  v_sub(A[2], A[0], eb);
+  v_cross(ea, eb, n1);
+  v_sub(B[1], B[0], ea);
+  v_sub(B[2], B[0], eb);
+  v_cross(ea, eb, n2);
+  v_cross(n1, n2, line);
+  int found = 0;
+  double lo = INFINITY, hi = -INFINITY, plo[3] = {0, 0, 0}, phi[3] = {0, 0, 0};
+  for (int side = 0; side < 2; side++) {
+    const double* const* E = side ? B : A;
+    const double* const* T = side ? A : B;
+    for (int k = 0; k < 3; k++) {
+      double x[3];
+      if (!segment_meets_triangle(E[k], E[(k + 1) % 3], T[0], T[1], T[2], x)) continue;
+      const double s = v_dot(x, line);
+      if (!found || s < lo) {
+        lo = s;
+        memcpy(plo, x, sizeof(x));
+      }
+      if (!found || s > hi) {
+        hi = s;
+        memcpy(phi, x, sizeof(x));
+      }
+      found = 1;
+    }
+  }
+  if (found) {
+    for (int k = 0; k < 3; k++) o1[k] = o2[k] = (plo[k] + phi[k]) * 0.5;
+    return 0.0;
+  }
+  double best = INFINITY, q[3];
+  for (int i = 0; i < 3; i++) {
+    orc_closest_point_triangle(B[i], A[0], A[1], A[2], q);
+    double d = v_dist_sq(B[i], q);
+    if (d < best) {
+      best = d;
+      memcpy(o1, q, sizeof(q));
+      memcpy(o2, B[i], sizeof(q));
+    }
+    orc_closest_point_triangle(A[i], B[0], B[1], B[2], q);
+    d = v_dist_sq(A[i], q);
+    if (d < best) {
+      best = d;
+      memcpy(o1, A[i], sizeof(q));
+      memcpy(o2, q, sizeof(q));
+    }
+  }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double c1[3], c2[3];
+      orc_segment_segment(A[i], A[(i + 1) % 3], B[j], B[(j + 1) % 3], c1, c2);
+      const double d = v_dist_sq(c1, c2);
+      if (d < best) {
+        best = d;
+        memcpy(o1, c1, sizeof(c1));
+        memcpy(o2, c2, sizeof(c2));
+      }
+    }
+  return sqrt(best);
+}
+
+/* Vector3.applyMatrix4 */
+static void apply_m4(const double e[16], const double v[3], double o[3]) {
+  const double x = v[0], y = v[1], z = v[2];
+  const double w = 1 / (e[3] * x + e[7] * y + e[11] * z + e[15]);
+  o[0] = (e[0] * x + e[4] * y + e[8] * z + e[12]) * w;
+  o[1] = (e[1] * x + e[5] * y + e[9] * z + e[13]) * w;
+  o[2] = (e[2] * x + e[6] * y + e[10] * z + e[14]) * w;
+}
+
+typedef struct {
+  const orc_mesh* m;
+  const double* t2;  /* 9 doubles */
+  double mn[3], mx[3];
+  double best;      /* this other triangle's best */
+  double bound;     /* the best over earlier other triangles */
+  double max_t;
+  uint32_t best_slot;
+  double o1[3], o2[3];
+} GeoCtx;
+
+/* A lower bound of the distance from the triangle's box to a node box (the gap less 1e-12 of the
+   largest coordinate, as roundings of the triangle distance can go below the exact gap). */
+static double geo_box_lb(const GeoCtx* g, const float* b) {
+  double s = 0, m = 0;
+  for (int k = 0; k < 3; k++) {
+    const double gap = fmax(fmax((double)b[k] - g->mx[k], g->mn[k] - (double)b[k + 3]), 0.0);
+    s += gap * gap;
+    m = fmax(m, fmax(fmax(fabs((double)b[k]), fabs((double)b[k + 3])), fmax(fabs(g->mn[k]), fabs(g->mx[k]))));
+  }
+  return fmax(sqrt(s) * (1 - 1e-12) - 1e-12 * m, 0.0);
+}
+
+static void geo_leaf(GeoCtx* g, uint64_t off, uint64_t cnt) {
+  for (uint64_t s = off; s < off + cnt; s++) {
+    double t1[9], o1[3], o2[3];
+    face_vertices(g->m, face_of_slot(g->m, s), t1, t1 + 3, t1 + 6);
+    const double d = orc_triangle_distance(t1, g->t2, o1, o2);
+    if (d < g->max_t && (d < g->best || (d == g->best && s < g->best_slot))) {
+      g->best = d;
+      g->best_slot = (uint32_t)s;
+      memcpy(g->o1, o1, sizeof(o1));
+      memcpy(g->o2, o2, sizeof(o2));
+    }
+  }
+}
+
+static void geo_node(GeoCtx* g, NodeBuf nb, uint64_t node) {
+  if (nb_leaf(nb, node)) {
+    geo_leaf(g, nb_u32(nb, node, 6), nb_u16(nb, node, 14));
+    return;
+  }
+  const uint64_t kids[2] = {node + 1, node + nb_u32(nb, node, 6)};
+  double s[2] = {geo_box_lb(g, nb_bounds(nb, kids[0])), geo_box_lb(g, nb_bounds(nb, kids[1]))};
+  const int first = s[1] < s[0];
+  for (int i = 0; i < 2; i++) {
+    const int c = i ? 1 - first : first;
+    if (s[c] <= fmin(g->best, g->bound) && s[c] < g->max_t) geo_node(g, nb, kids[c]);
+  }
+}
+
+/* closestPointToGeometry (src/core/cast/closestPointToGeometry.template.js:15-271): the closest pair
+   over every triangle of the other geometry (brought into the mesh frame by e, applyMatrix4) and every
+   triangle of the mesh; pairs at or beyond max_t do not count. Equal distances: the lowest other
+   triangle, then the lowest mesh slot. brute != 0 tries every pair; otherwise each other triangle
+   walks the tree, skipping nodes whose box lower bound exceeds the best so far. Returns the other
+   triangle (NO_HIT when no pair counts) and writes the slot, distance and points (mesh frame). */
+uint32_t orc_closest_to_geometry(const orc_mesh* m, const float* positions, uint64_t n_vertices, uint32_t stride,
+                                 uint32_t offset, const void* index, int elt, uint64_t n_tris, const double e[16],
+                                 double max_t, int brute, uint32_t* slot, double* dist, double p1[3], double p2[3]) {
+  (void)n_vertices;
+  uint32_t win = NO_HIT;
+  double best = INFINITY;
+  *slot = NO_HIT;
+  *dist = INFINITY;
+  for (uint64_t j = 0; j < n_tris; j++) {
+    double t2[9];
+    for (int k = 0; k < 3; k++) {
+      const uint64_t v = index ? rd(index, elt, 3 * j + k) : 3 * j + k;
+      const float* p = positions + v * stride + offset;
+      const double src[3] = {p[0], p[1], p[2]};
+      apply_m4(e, src, t2 + 3 * k);
+    }
+    GeoCtx g;
+    memset(&g, 0, sizeof(g));
+    g.m = m;
+    g.t2 = t2;
+    g.best = INFINITY;
+    g.bound = best;
+    g.max_t = max_t;
+    g.best_slot = NO_HIT;
+    for (int k = 0; k < 3; k++) {
+      g.mn[k] = fmin(fmin(t2[k], t2[3 + k]), t2[6 + k]);
+      g.mx[k] = fmax(fmax(t2[k], t2[3 + k]), t2[6 + k]);
+    }
+    for (uint32_t r = 0; r < m->n_roots; r++) {
+      NodeBuf nb = root_buf(m, r);
+      if (brute) {
+        for (uint64_t k = 0; k < nb.n; k++)
+          if (nb_leaf(nb, k)) geo_leaf(&g, nb_u32(nb, k, 6), nb_u16(nb, k, 14));
+      } else {
+        geo_node(&g, nb, 0);
+      }
+    }
+    if (g.best_slot != NO_HIT && g.best < best) {
+      best = g.best;
+      win = (uint32_t)j;
+      *slot = g.best_slot;
+      *dist = g.best;
+      memcpy(p1, g.o1, sizeof(g.o1));
+      memcpy(p2, g.o2, sizeof(g.o2));
+    }
+  }
+  return win;
+}

@@ -112,6 +112,16 @@ int orc_box_triangle_sat(const double bmin[3], const double bmax[3], const doubl
 int orc_box_triangle_clip(const double bmin[3], const double bmax[3], const double e[16], const double a[3],
                           const double b[3], const double c[3]);
 
+/* closestPointsSegmentToSegment (MathGeoLib's form, as the reference restates it) */
+void orc_segment_segment(const double s1[3], const double e1[3], const double s2[3], const double e2[3], double o1[3],
+                         double o2[3]);
+/* ExtendedTriangle.distanceToTriangle of two triangles (9 doubles each); writes the closest points. */
+double orc_triangle_distance(const double* t1, const double* t2, double o1[3], double o2[3]);
+/* closestPointToGeometry over the mesh and another geometry placed by e (16 doubles, column-major). */
+uint32_t orc_closest_to_geometry(const orc_mesh* m, const float* positions, uint64_t n_vertices, uint32_t stride,
+                                 uint32_t offset, const void* index, int elt, uint64_t n_tris, const double e[16],
+                                 double max_t, int brute, uint32_t* slot, double* dist, double p1[3], double p2[3]);
+
 #ifdef __cplusplus
 }
 #endif

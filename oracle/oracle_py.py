@@ -72,6 +72,13 @@ def library():
         lib.orc_box_triangle_sat.restype = C.c_int
         lib.orc_box_triangle_clip.argtypes = [vp, vp, vp, vp, vp, vp]
         lib.orc_box_triangle_clip.restype = C.c_int
+        lib.orc_segment_segment.argtypes = [vp, vp, vp, vp, vp, vp]
+        lib.orc_segment_segment.restype = None
+        lib.orc_triangle_distance.argtypes = [vp, vp, vp, vp]
+        lib.orc_triangle_distance.restype = f64
+        lib.orc_closest_to_geometry.argtypes = [vp, vp, u64, u32, u32, vp, C.c_int, u64, vp, f64, C.c_int, vp, vp,
+                                                vp, vp]
+        lib.orc_closest_to_geometry.restype = u32
         lib.orc_refit_root.argtypes = [vp, vp, C.c_uint64]
         lib.orc_refit_root.restype = None
         lib.orc_validate_root.argtypes = [vp, u32, C.POINTER(TreeStats)]
@@ -190,6 +197,27 @@ class OracleMesh:
         r = np.asarray(radii, dtype=np.float64).reshape(-1)
         return np.array([bool(library().orc_intersects_sphere(self.ref, _p(c[i]), float(r[i]))) for i in range(len(c))])
 
+    def closest_to_geometry(self, positions, index=None, matrix=None, max_threshold=float("inf"), brute=False):
+        """closestPointToGeometry: the closest pair over every mesh triangle and every triangle of the
+        other geometry placed by `matrix` (16 numbers, column-major). Equal distances go to the lowest
+        other triangle, then the lowest mesh slot. None when no pair is below max_threshold."""
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        idx = None if index is None else np.ascontiguousarray(index)
+        n = (idx.size if idx is not None else pos.shape[0]) // 3
+        m = np.ascontiguousarray(np.eye(4).reshape(-1) if matrix is None else np.asarray(matrix, np.float64).reshape(-1),
+                                 np.float64)
+        slot = C.c_uint32()
+        dist = C.c_double()
+        p1 = np.zeros(3)
+        p2 = np.zeros(3)
+        j = library().orc_closest_to_geometry(self.ref, _p(pos), pos.shape[0], 3, 0, _p(idx),
+                                              0 if idx is None else idx.itemsize, n, _p(m), float(max_threshold),
+                                              int(brute), C.byref(slot), C.byref(dist), _p(p1), _p(p2))
+        if j == NO_HIT:
+            return None
+        return {"distance": dist.value, "face_index": slot.value, "other_face_index": int(j), "point": p1,
+                "point_in_mesh_frame": p2}
+
     def intersects_box(self, box_min, box_max, box_to_mesh):
         """intersectsBox per box (brute force, the reference's OrientedBox separating-axis test)."""
         mn = np.ascontiguousarray(box_min, np.float64).reshape(-1, 3)
@@ -302,3 +330,22 @@ def box_triangle(box_min, box_max, box_to_mesh, a, b, c, clip=False) -> int:
     arr = lambda v, n: np.ascontiguousarray(v, np.float64).reshape(n)
     return int(f(_p(arr(box_min, 3)), _p(arr(box_max, 3)), _p(arr(box_to_mesh, 16)), _p(arr(a, 3)), _p(arr(b, 3)),
                  _p(arr(c, 3))))
+
+
+def triangle_distance(t1, t2):
+    """ExtendedTriangle.distanceToTriangle of two triangles ((3, 3) each): (distance, point on t1, point on t2)."""
+    a = np.ascontiguousarray(t1, np.float64).reshape(9)
+    b = np.ascontiguousarray(t2, np.float64).reshape(9)
+    o1 = np.zeros(3)
+    o2 = np.zeros(3)
+    d = library().orc_triangle_distance(_p(a), _p(b), _p(o1), _p(o2))
+    return d, o1, o2
+
+
+def segment_segment(s1, e1, s2, e2):
+    """closestPointsSegmentToSegment: the closest points of two segments."""
+    v = [np.ascontiguousarray(x, np.float64).reshape(3) for x in (s1, e1, s2, e2)]
+    o1 = np.zeros(3)
+    o2 = np.zeros(3)
+    library().orc_segment_segment(*[_p(x) for x in v], _p(o1), _p(o2))
+    return o1, o2

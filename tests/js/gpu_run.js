@@ -246,6 +246,29 @@ assert.strictEqual(batched._mbvhBatch, null);
   assert.ok(nHit > 10 && nHit < nb - 10, `box hits ${nHit}`);
 }
 
+// --- closestPointToGeometry: a large triangle in the plane one unit past the mesh's largest x, given in
+// a frame shifted by (-5, 0, 0): the closest pair is at distance 1 from a vertex of largest x ----------
+{
+  let lo = [Infinity, Infinity, Infinity], hi = [-Infinity, -Infinity, -Infinity];
+  for (let v = 0; v < positions.length; v += 3) for (let k = 0; k < 3; k++) {
+    lo[k] = Math.min(lo[k], positions[v + k]); hi[k] = Math.max(hi[k], positions[v + k]);
+  }
+  // a right triangle whose legs (from a corner below and left of the mesh's y-z extent) are long enough
+  // that the hypotenuse passes beyond that extent: every vertex projects inside it
+  const x = hi[0] + 1 - 5, s = 1, L = 2 * (hi[1] - lo[1] + hi[2] - lo[2] + 2 * s);
+  const tri = new Float32Array([x, lo[1] - s, lo[2] - s, x, lo[1] - s + L, lo[2] - s, x, lo[1] - s, lo[2] - s + L]);
+  const other = { attributes: { position: { array: tri, itemSize: 3, count: 3 } }, index: null };
+  const shift = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 5, 0, 0, 1];
+  const t1 = {}, t2 = {};
+  const r = bvh.closestPointToGeometry(other, { elements: shift }, t1, t2);
+  assert.strictEqual(r, t1);
+  assert.ok(Math.abs(t1.distance - (Math.fround(hi[0] + 1 - 5) + 5 - hi[0])) < 1e-9, `distance ${t1.distance}`);
+  assert.strictEqual(t1.point.x, hi[0]);
+  assert.ok(Math.abs(t2.point.x - Math.fround(x)) < 1e-9 && t2.faceIndex === 0 && Math.abs(t2.distance - t1.distance) < 1e-9);
+  assert.ok(t1.faceIndex >= 0 && t1.faceIndex < geometry.index.count / 3);
+  assert.strictEqual(bvh.closestPointToGeometry(other, shift, {}, null, 0, 0.5), null);
+}
+
 // --- MeshBVHUniformStruct / WebGPU node buffer of the main BVH ----------------------------------
 {
   const u = new api.MeshBVHUniformStruct();

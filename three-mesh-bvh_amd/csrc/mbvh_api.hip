@@ -42,6 +42,7 @@ struct mbvh_ctx {
   DeviceBuffer grad[11];        // host-API backward staging
   DeviceBuffer grad_ws;         // host-API backward workspace
   DeviceBuffer build_ws;        // device build workspace
+  DeviceBuffer geo_ws;          // closestPointToGeometry workspace (per other triangle + answer)
   uint64_t hits_total = 0;
   std::mutex lock;              // host entry points on one context are serialised
 };
@@ -470,6 +471,7 @@ extern "C" void mbvh_ctx_destroy(mbvh_ctx* ctx) {
   if (ctx->grad_ws.ptr) (void)hipFree(ctx->grad_ws.ptr);
   if (ctx->order_ws.ptr) (void)hipFree(ctx->order_ws.ptr);
   if (ctx->build_ws.ptr) (void)hipFree(ctx->build_ws.ptr);
+  if (ctx->geo_ws.ptr) (void)hipFree(ctx->geo_ws.ptr);
   (void)hipStreamDestroy(ctx->stream);
   ctx->magic = 0;
   delete ctx;
@@ -2624,3 +2626,146 @@ extern "C" int mbvh_intersects_box(mbvh_ctx* ctx, const mbvh_mesh* mesh, const d
   return MBVH_OK;
 }
 
+
+// ---- closestPointToGeometry ------------------------------------------------------------------------
+namespace {
+
+static_assert(sizeof(GeometryHit) == sizeof(mbvh_geometry_hit), "mbvh_geometry_hit layout");
+
+// three's Matrix4.invert (the cofactor expansion; a zero determinant gives the zero matrix), in double.
+void invert4(const double* te, double* out) {
+  const double n11 = te[0], n21 = te[1], n31 = te[2], n41 = te[3];
+  const double n12 = te[4], n22 = te[5], n32 = te[6], n42 = te[7];
+  const double n13 = te[8], n23 = te[9], n33 = te[10], n43 = te[11];
+  const double n14 = te[12], n24 = te[13], n34 = te[14], n44 = te[15];
+  const double t11 = n23 * n34 * n42 - n24 * n33 * n42 + n24 * n32 * n43 - n22 * n34 * n43 - n23 * n32 * n44 + n22 * n33 * n44;
+  const double t12 = n14 * n33 * n42 - n13 * n34 * n42 - n14 * n32 * n43 + n12 * n34 * n43 + n13 * n32 * n44 - n12 * n33 * n44;
+  const double t13 = n13 * n24 * n42 - n14 * n23 * n42 + n14 * n22 * n43 - n12 * n24 * n43 - n13 * n22 * n44 + n12 * n23 * n44;
+  const double t14 = n14 * n23 * n32 - n13 * n24 * n32 - n14 * n22 * n33 + n12 * n24 * n33 + n13 * n22 * n34 - n12 * n23 * n34;
+  const double det = n11 * t11 + n21 * t12 + n31 * t13 + n41 * t14;
+  if (det == 0) {
+    for (int i = 0; i < 16; i++) out[i] = 0;
+    return;
+  }
+  const double d = 1 / det;
+  out[0] = t11 * d;
+  out[1] = (n24 * n33 * n41 - n23 * n34 * n41 - n24 * n31 * n43 + n21 * n34 * n43 + n23 * n31 * n44 - n21 * n33 * n44) * d;
+  out[2] = (n22 * n34 * n41 - n24 * n32 * n41 + n24 * n31 * n42 - n21 * n34 * n42 - n22 * n31 * n44 + n21 * n32 * n44) * d;
+  out[3] = (n23 * n32 * n41 - n22 * n33 * n41 - n23 * n31 * n42 + n21 * n33 * n42 + n22 * n31 * n43 - n21 * n32 * n43) * d;
+  out[4] = t12 * d;
+  out[5] = (n13 * n34 * n41 - n14 * n33 * n41 + n14 * n31 * n43 - n11 * n34 * n43 - n13 * n31 * n44 + n11 * n33 * n44) * d;
+  out[6] = (n14 * n32 * n41 - n12 * n34 * n41 - n14 * n31 * n42 + n11 * n34 * n42 + n12 * n31 * n44 - n11 * n32 * n44) * d;
+  out[7] = (n12 * n33 * n41 - n13 * n32 * n41 + n13 * n31 * n42 - n11 * n33 * n42 - n12 * n31 * n43 + n11 * n32 * n43) * d;
+  out[8] = t13 * d;
+  out[9] = (n14 * n23 * n41 - n13 * n24 * n41 - n14 * n21 * n43 + n11 * n24 * n43 + n13 * n21 * n44 - n11 * n23 * n44) * d;
+  out[10] = (n12 * n24 * n41 - n14 * n22 * n41 + n14 * n21 * n42 - n11 * n24 * n42 - n12 * n21 * n44 + n11 * n22 * n44) * d;
+  out[11] = (n13 * n22 * n41 - n12 * n23 * n41 - n13 * n21 * n42 + n11 * n23 * n42 + n12 * n21 * n43 - n11 * n22 * n43) * d;
+  out[12] = t14 * d;
+  out[13] = (n13 * n24 * n31 - n14 * n23 * n31 + n14 * n21 * n33 - n11 * n24 * n33 - n13 * n21 * n34 + n11 * n23 * n34) * d;
+  out[14] = (n14 * n22 * n31 - n12 * n24 * n31 - n14 * n21 * n32 + n11 * n24 * n32 + n12 * n21 * n34 - n11 * n22 * n34) * d;
+  out[15] = (n12 * n23 * n31 - n13 * n22 * n31 + n13 * n21 * n32 - n11 * n23 * n32 - n12 * n21 * n33 + n11 * n22 * n33) * d;
+}
+
+int check_geometry(const char* fn, const float* positions, uint64_t n_vertices, uint32_t stride, uint32_t offset,
+                   const void* index, uint32_t index_elt, uint64_t n_tris, const double* matrix, void* out) {
+  if (!matrix || !out) return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL matrix or output", fn);
+  if (n_tris && !positions) return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL positions", fn);
+  if (stride < 3 || offset > stride - 3) return set_error(MBVH_ERR_BAD_ARGS, "%s: bad stride %u / offset %u", fn, stride, offset);
+  if (index && index_elt != 2 && index_elt != 4)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: index element size must be 2 or 4", fn);
+  if (n_tris >= 0xFFFFFFFFull) return set_error(MBVH_ERR_LIMIT, "%s: too many triangles", fn);
+  if (!index && 3 * n_tris > n_vertices)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: %llu triangles need more than %llu vertices", fn,
+                     static_cast<unsigned long long>(n_tris), static_cast<unsigned long long>(n_vertices));
+  return MBVH_OK;
+}
+
+int enqueue_geometry(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* positions, uint64_t n_vertices, uint32_t stride,
+                     uint32_t offset, const void* index, uint32_t index_elt, uint64_t n_tris, const double* matrix,
+                     double max_threshold, mbvh_geometry_hit* out, hipStream_t s) {
+  if (mesh->max_stack > 8 + ctx->stack_ovf_entries)
+    return set_error(MBVH_ERR_LIMIT, "closestPointToGeometry: stack scratch smaller than the tree depth");
+  const size_t per = sizeof(double) + sizeof(uint32_t);
+  int rc = ensure_buffer(ctx->geo_ws, 256 + n_tris * per);
+  if (rc) return rc;
+  uint8_t* ws = static_cast<uint8_t*>(ctx->geo_ws.ptr);
+  GeometryArgs a{};
+  a.mesh = view_of(mesh);
+  a.other_positions = positions;
+  a.other_vertices = n_vertices;
+  a.other_stride = stride;
+  a.other_offset = offset;
+  a.other_index = index;
+  a.other_elt = static_cast<int32_t>(index_elt);
+  a.n_other = n_tris;
+  for (int i = 0; i < 16; i++) a.matrix[i] = matrix[i];
+  invert4(matrix, a.inverse);
+  a.max_threshold = max_threshold;
+  a.best_bits = reinterpret_cast<unsigned long long*>(ws);
+  a.winner = reinterpret_cast<uint32_t*>(ws + 8);
+  a.status = reinterpret_cast<uint32_t*>(ws + 12);
+  a.lane_dist = reinterpret_cast<double*>(ws + 256);
+  a.lane_face = reinterpret_cast<uint32_t*>(ws + 256 + n_tris * sizeof(double));
+  a.out = reinterpret_cast<GeometryHit*>(out);
+  const uint64_t threads = launch_threads(n_tris ? n_tris : 1, ctx->max_threads);
+  a.stack = stack_of(ctx, threads);
+  MBVH_HIP(launch_closest_to_geometry(a, threads, s));
+  return MBVH_OK;
+}
+
+}  // namespace
+
+extern "C" int mbvh_closest_point_to_geometry_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* positions,
+                                                uint64_t n_vertices, uint32_t stride, uint32_t offset, const void* index,
+                                                uint32_t index_elt, uint64_t n_tris, const double* geometry_to_bvh,
+                                                double min_threshold, double max_threshold, mbvh_geometry_hit* out,
+                                                void* stream) {
+  clear_error();
+  const char* fn = "mbvh_closest_point_to_geometry_d";
+  (void)min_threshold;
+  int rc = check_query(fn, ctx, mesh);
+  if (rc || (rc = check_geometry(fn, positions, n_vertices, stride, offset, index, index_elt, n_tris, geometry_to_bvh, out)))
+    return rc;
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = reserve_stack(ctx, mesh->max_stack))) return rc;
+  return enqueue_geometry(ctx, mesh, positions, n_vertices, stride, offset, index, index_elt, n_tris, geometry_to_bvh,
+                          max_threshold, out, stream ? static_cast<hipStream_t>(stream) : ctx->stream);
+}
+
+extern "C" int mbvh_closest_point_to_geometry(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* positions,
+                                              uint64_t n_vertices, uint32_t stride, uint32_t offset, const void* index,
+                                              uint32_t index_elt, uint64_t n_tris, const double* geometry_to_bvh,
+                                              double min_threshold, double max_threshold, mbvh_geometry_hit* out) {
+  clear_error();
+  const char* fn = "mbvh_closest_point_to_geometry";
+  (void)min_threshold;
+  int rc = check_query(fn, ctx, mesh);
+  if (rc || (rc = check_geometry(fn, positions, n_vertices, stride, offset, index, index_elt, n_tris, geometry_to_bvh, out)))
+    return rc;
+  if (index)
+    for (uint64_t i = 0; i < 3 * n_tris; i++) {
+      const uint64_t v = index_elt == 2 ? static_cast<const uint16_t*>(index)[i] : static_cast<const uint32_t*>(index)[i];
+      if (v >= n_vertices)
+        return set_error(MBVH_ERR_BAD_ARGS, "%s: index[%llu] = %llu is not below the %llu vertices", fn,
+                         static_cast<unsigned long long>(i), static_cast<unsigned long long>(v),
+                         static_cast<unsigned long long>(n_vertices));
+    }
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = reserve_stack(ctx, mesh->max_stack))) return rc;
+  hipStream_t s = ctx->stream;
+  const uint64_t n_floats = n_vertices ? (n_vertices - 1) * stride + offset + 3 : 0;
+  void *dp = nullptr, *di = nullptr, *dout = nullptr;
+  if ((rc = stage_in(ctx, ctx->staging[0], positions, n_floats * sizeof(float), &dp)) ||
+      (index && (rc = stage_in(ctx, ctx->staging[1], index, 3 * n_tris * index_elt, &di))) ||
+      (rc = stage_out(ctx->staging[7], out, sizeof(mbvh_geometry_hit), &dout)))
+    return rc;
+  if ((rc = enqueue_geometry(ctx, mesh, static_cast<const float*>(dp), n_vertices, stride, offset, di, index_elt, n_tris,
+                             geometry_to_bvh, max_threshold, static_cast<mbvh_geometry_hit*>(dout), s)))
+    return rc;
+  MBVH_HIP(hipMemcpyAsync(out, dout, sizeof(mbvh_geometry_hit), hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipStreamSynchronize(s));
+  if (out->status == 1) return set_error(MBVH_ERR_LIMIT, "%s: traversal stack overflow", fn);
+  if (out->status == 2) return set_error(MBVH_ERR_BAD_ARGS, "%s: an index entry is out of range", fn);
+  return MBVH_OK;
+}

@@ -254,6 +254,42 @@ struct BoxArgs {
 };
 hipError_t launch_intersects_box(const BoxArgs& a, uint64_t threads, hipStream_t s);
 
+// ---- closestPointToGeometry --------------------------------------------------------------------------
+// The answer record (the layout of mbvh_geometry_hit in the C ABI header).
+struct GeometryHit {
+  double distance;        // +inf: no pair closer than maxThreshold
+  double point[3];        // on the mesh triangle, mesh frame
+  double other_point[3];  // on the other triangle, the other geometry's frame
+  double other_distance;  // |inverse(point) - other_point|
+  uint32_t face_index;    // mesh triangle slot
+  uint32_t other_face_index;
+  int32_t status;         // 0, 1 traversal stack overflow, 2 an index entry out of range
+  uint32_t pad;
+};
+
+struct GeometryArgs {
+  MeshView mesh;
+  StackScratch stack;
+  const float* other_positions;   // the other geometry (device)
+  uint64_t other_vertices;
+  uint32_t other_stride;          // floats between vertices
+  uint32_t other_offset;          // float offset of x
+  const void* other_index;        // NULL: non-indexed
+  int32_t other_elt;              // 2 or 4 bytes
+  uint64_t n_other;               // triangles of the other geometry
+  double matrix[16];              // other geometry frame -> mesh frame (column-major)
+  double inverse[16];             // mesh frame -> other geometry frame
+  double max_threshold;
+  unsigned long long* best_bits;  // workspace: the best distance of all lanes (bits of a non-negative double)
+  uint32_t* winner;               // workspace: the winning other triangle
+  uint32_t* status;               // workspace: 1 stack overflow, 2 index out of range
+  double* lane_dist;              // workspace, per other triangle: its closest distance found
+  uint32_t* lane_face;            // workspace, per other triangle: the mesh slot of that pair
+  GeometryHit* out;               // device
+};
+// Enqueues the kernels of one query (clear, seed bound, per-triangle traversal, pick, gather).
+hipError_t launch_closest_to_geometry(const GeometryArgs& a, uint64_t threads, hipStream_t s);
+
 // ---- export to the reference's shader data layouts ------------------------------------------------
 struct ExportArgs {
   const uint32_t* nodes;     // reference-format nodes of one root (8 words each)

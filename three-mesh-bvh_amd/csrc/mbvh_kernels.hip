@@ -1894,6 +1894,395 @@ __global__ __launch_bounds__(kBlock) void k_intersects_box(BoxArgs A) {
   }
 }
 
+// ---- closestPointToGeometry: the closest pair of triangles between the mesh and another geometry ----
+// (src/core/cast/closestPointToGeometry.template.js:15-271 with ExtendedTriangle.distanceToTriangle
+// src/math/ExtendedTriangle.js:774-860). In double: 0 when an edge of one triangle pierces the other
+// (intersecting triangles; coplanar overlaps give 0 below), else the least of the six vertex-to-triangle
+// distances and the nine edge-to-edge distances, as distanceToTriangle takes it.
+
+__device__ __forceinline__ D3 dadd(D3 a, D3 b) {
+  MBVH_NO_CONTRACT
+  return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+__device__ __forceinline__ D3 dscale(D3 a, double s) {
+  MBVH_NO_CONTRACT
+  return {a.x * s, a.y * s, a.z * s};
+}
+__device__ __forceinline__ double ddist2(D3 a, D3 b) {
+  const D3 d = dsub(a, b);
+  return ddot(d, d);
+}
+
+// Closest point of triangle abc to p (Ericson, Real-Time Collision Detection 5.1.5; three's
+// Triangle.closestPointToPoint).
+__device__ D3 closest_on_tri_d(D3 p, D3 a, D3 b, D3 c) {
+  MBVH_NO_CONTRACT
+  const D3 ab = dsub(b, a), ac = dsub(c, a), ap = dsub(p, a);
+  const double d1 = ddot(ab, ap), d2 = ddot(ac, ap);
+  if (d1 <= 0 && d2 <= 0) return a;
+  const D3 bp = dsub(p, b);
+  const double d3 = ddot(ab, bp), d4 = ddot(ac, bp);
+  if (d3 >= 0 && d4 <= d3) return b;
+  const double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0 && d1 >= 0 && d3 <= 0) return dadd(a, dscale(ab, d1 / (d1 - d3)));
+  const D3 cp = dsub(p, c);
+  const double d5 = ddot(ab, cp), d6 = ddot(ac, cp);
+  if (d6 >= 0 && d5 <= d6) return c;
+  const double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0 && d2 >= 0 && d6 <= 0) return dadd(a, dscale(ac, d2 / (d2 - d6)));
+  const double va = d3 * d6 - d5 * d4;
+  if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0)
+    return dadd(b, dscale(dsub(c, b), (d4 - d3) / ((d4 - d3) + (d5 - d6))));
+  const double denom = 1 / (va + vb + vc);
+  return dadd(dadd(a, dscale(ab, vb * denom)), dscale(ac, vc * denom));
+}
+
+// Closest points of segments p1q1 and p2q2 (Ericson 5.1.9, clamped).
+__device__ void seg_seg_d(D3 p1, D3 q1, D3 p2, D3 q2, D3& c1, D3& c2) {
+  MBVH_NO_CONTRACT
+  const D3 d1 = dsub(q1, p1), d2 = dsub(q2, p2), r = dsub(p1, p2);
+  const double a = ddot(d1, d1), e = ddot(d2, d2), f = ddot(d2, r);
+  double s, t;
+  if (a <= 0 && e <= 0) {
+    s = t = 0;
+  } else if (a <= 0) {
+    s = 0;
+    t = fmin(fmax(f / e, 0.0), 1.0);
+  } else {
+    const double c = ddot(d1, r);
+    if (e <= 0) {
+      t = 0;
+      s = fmin(fmax(-c / a, 0.0), 1.0);
+    } else {
+      const double b = ddot(d1, d2), den = a * e - b * b;
+      s = den != 0 ? fmin(fmax((b * f - c * e) / den, 0.0), 1.0) : 0.0;
+      t = (b * s + f) / e;
+      if (t < 0) {
+        t = 0;
+        s = fmin(fmax(-c / a, 0.0), 1.0);
+      } else if (t > 1) {
+        t = 1;
+        s = fmin(fmax((b - c) / a, 0.0), 1.0);
+      }
+    }
+  }
+  c1 = dadd(p1, dscale(d1, s));
+  c2 = dadd(p2, dscale(d2, t));
+}
+
+__device__ __forceinline__ double orient3(D3 a, D3 b, D3 c, D3 d) {  // (b - a) x (c - a) . (d - a)
+  return ddot(dcross(dsub(b, a), dsub(c, a)), dsub(d, a));
+}
+
+// Whether segment pq crosses triangle abc at a single point (strictly opposite sides, or one end on the
+// plane; both ends on the plane is left to the coplanar distances), writing that point.
+__device__ bool seg_pierces_d(D3 p, D3 q, D3 a, D3 b, D3 c, D3& x) {
+  MBVH_NO_CONTRACT
+  const double sp = orient3(a, b, c, p), sq = orient3(a, b, c, q);
+  if ((sp > 0 && sq > 0) || (sp < 0 && sq < 0) || (sp == 0 && sq == 0)) return false;
+  const double u = orient3(p, q, a, b), v = orient3(p, q, b, c), w = orient3(p, q, c, a);
+  if (!((u >= 0 && v >= 0 && w >= 0) || (u <= 0 && v <= 0 && w <= 0))) return false;
+  x = dadd(p, dscale(dsub(q, p), sp / (sp - sq)));
+  return true;
+}
+
+// distanceToTriangle: the distance and the closest points on t1 (a1 b1 c1) and t2. Intersecting
+// triangles give 0 and, as the reference's intersection segment does, the middle of the segment
+// where they cross (its ends are where edges of one meet the other).
+__device__ double tri_tri_d(const D3* t1, const D3* t2, D3& o1, D3& o2) {
+  MBVH_NO_CONTRACT
+  const D3 line = dcross(dcross(dsub(t1[1], t1[0]), dsub(t1[2], t1[0])), dcross(dsub(t2[1], t2[0]), dsub(t2[2], t2[0])));
+  bool found = false;
+  double lo = 0, hi = 0;
+  D3 plo = d3(0, 0, 0), phi = d3(0, 0, 0);
+  for (int k = 0; k < 6; k++) {
+    const D3* e = k < 3 ? t1 : t2;
+    const D3* t = k < 3 ? t2 : t1;
+    const int i = k % 3;
+    D3 x;
+    if (!seg_pierces_d(e[i], e[(i + 1) % 3], t[0], t[1], t[2], x)) continue;
+    const double sx = ddot(x, line);
+    if (!found || sx < lo) {
+      lo = sx;
+      plo = x;
+    }
+    if (!found || sx > hi) {
+      hi = sx;
+      phi = x;
+    }
+    found = true;
+  }
+  if (found) {
+    o1 = o2 = dscale(dadd(plo, phi), 0.5);
+    return 0.0;
+  }
+  double best = __builtin_huge_val();
+  for (int i = 0; i < 3; i++) {
+    D3 q = closest_on_tri_d(t2[i], t1[0], t1[1], t1[2]);
+    double d = ddist2(t2[i], q);
+    if (d < best) {
+      best = d;
+      o1 = q;
+      o2 = t2[i];
+    }
+    q = closest_on_tri_d(t1[i], t2[0], t2[1], t2[2]);
+    d = ddist2(t1[i], q);
+    if (d < best) {
+      best = d;
+      o1 = t1[i];
+      o2 = q;
+    }
+  }
+  for (int i = 0; i < 3; i++) {
+    for (int j = 0; j < 3; j++) {
+      D3 c1, c2;
+      seg_seg_d(t1[i], t1[(i + 1) % 3], t2[j], t2[(j + 1) % 3], c1, c2);
+      const double d = ddist2(c1, c2);
+      if (d < best) {
+        best = d;
+        o1 = c1;
+        o2 = c2;
+      }
+    }
+  }
+  return sqrt(best);
+}
+
+// three's Vector3.applyMatrix4 in double (column-major elements, divided by w).
+__device__ __forceinline__ D3 apply_matrix4_d(const double* e, double x, double y, double z) {
+  MBVH_NO_CONTRACT
+  const double w = 1 / (e[3] * x + e[7] * y + e[11] * z + e[15]);
+  return d3((e[0] * x + e[4] * y + e[8] * z + e[12]) * w, (e[1] * x + e[5] * y + e[9] * z + e[13]) * w,
+            (e[2] * x + e[6] * y + e[10] * z + e[14]) * w);
+}
+
+// Triangle j of the other geometry in the mesh frame; false when an index entry is out of range.
+__device__ __forceinline__ bool other_triangle(const GeometryArgs& A, uint64_t j, D3* t2) {
+  for (int k = 0; k < 3; k++) {
+    uint64_t v = 3 * j + k;
+    if (A.other_index)
+      v = A.other_elt == 2 ? static_cast<const uint16_t*>(A.other_index)[v] : static_cast<const uint32_t*>(A.other_index)[v];
+    if (v >= A.other_vertices) return false;
+    const float* p = A.other_positions + v * A.other_stride + A.other_offset;
+    t2[k] = apply_matrix4_d(A.matrix, p[0], p[1], p[2]);
+  }
+  return true;
+}
+
+// A lower bound of the computed distance between a triangle inside the double box (mn, mx) and one
+// inside the float node box: the boxes' gap, less a margin for the roundings of both computations
+// (1e-12 of the largest coordinate involved), never negative.
+__device__ __forceinline__ double box_box_lb(const double* mn, const double* mx, double mag, float a, float b, float c,
+                                             float d, float e, float f) {
+  MBVH_NO_CONTRACT
+  const double lo[3] = {a, b, c}, hi[3] = {d, e, f};
+  double s = 0, m = mag;
+  for (int k = 0; k < 3; k++) {
+    const double g = fmax(fmax(lo[k] - mx[k], mn[k] - hi[k]), 0.0);
+    s += g * g;
+    m = fmax(m, fmax(fabs(lo[k]), fabs(hi[k])));
+  }
+  return fmax(sqrt(s) * (1 - 1e-12) - 1e-12 * m, 0.0);
+}
+
+__device__ __forceinline__ double shared_best(const unsigned long long* bits) {
+  return __longlong_as_double(static_cast<long long>(__hip_atomic_load(bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+}
+
+// One lane per triangle of the other geometry (brought into the mesh frame by `matrix`, three's
+// applyMatrix4): a depth-first traversal of the mesh, near child first. A node is skipped when its lower
+// bound exceeds the lane's best distance or the best of all lanes (an atomic minimum on the distance's
+// bits), or reaches maxThreshold. Equal bounds are still visited and equal distances go to the lower
+// triangle slot, so a lane whose closest distance is the overall minimum finds it, and the lowest slot
+// at it, whatever the other lanes have published by then: the reduction over lanes is deterministic.
+__global__ __launch_bounds__(kBlock) void k_closest_to_geometry(GeometryArgs A) {
+  __shared__ uint2 lds_stack[8 * kBlock];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + tid;
+  const MeshView M = A.mesh;
+  StackL<8> st;
+  st.lds = lds_stack;
+  st.ovf = A.stack.ovf;
+  st.ovf_stride = A.stack.ovf_stride;
+  st.gtid = gtid;
+  st.lane = tid;
+  st.capacity = 8 + static_cast<int>(A.stack.ovf_entries);
+  const double cap = A.max_threshold;
+  for (uint64_t j = gtid; j < A.n_other; j += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    D3 t2[3];
+    double mn[3] = {__builtin_huge_val(), __builtin_huge_val(), __builtin_huge_val()};
+    double mx[3] = {-__builtin_huge_val(), -__builtin_huge_val(), -__builtin_huge_val()};
+    if (!other_triangle(A, j, t2)) {
+      atomicOr(A.status, 2u);
+      A.lane_dist[j] = __builtin_huge_val();
+      A.lane_face[j] = kNoHit;
+      continue;
+    }
+    for (int k = 0; k < 3; k++) {
+      mn[0] = fmin(mn[0], t2[k].x);
+      mn[1] = fmin(mn[1], t2[k].y);
+      mn[2] = fmin(mn[2], t2[k].z);
+      mx[0] = fmax(mx[0], t2[k].x);
+      mx[1] = fmax(mx[1], t2[k].y);
+      mx[2] = fmax(mx[2], t2[k].z);
+    }
+    const double mag = fmax(fmax(fmax(fabs(mn[0]), fabs(mn[1])), fmax(fabs(mn[2]), fabs(mx[0]))), fmax(fabs(mx[1]), fabs(mx[2])));
+    double best = __builtin_huge_val();
+    uint32_t best_face = kNoHit;
+    bool overflow = false;
+    for (uint32_t r = 0; r < M.n_roots && !overflow; r++) {
+      st.sp = 0;
+      uint32_t ref = M.roots[r];
+      while (true) {
+        if (ref & kRefLeaf) {
+          uint32_t off, cnt;
+          leaf_range(ref, M.bigleaf, off, cnt);
+          const double leaf_bound = fmin(best, shared_best(A.best_bits));
+          for (uint32_t s = off; s < off + cnt; s++) {
+            const Tri t = load_tri(M.tris, s);
+            // the triangle's own box first: the full test only where the pair could still win or tie
+            const double tlb = box_box_lb(mn, mx, mag, fminf(fminf(t.p0.x, t.p1.x), t.p2.x),
+                                          fminf(fminf(t.p0.y, t.p1.y), t.p2.y), fminf(fminf(t.p0.z, t.p1.z), t.p2.z),
+                                          fmaxf(fmaxf(t.p0.x, t.p1.x), t.p2.x), fmaxf(fmaxf(t.p0.y, t.p1.y), t.p2.y),
+                                          fmaxf(fmaxf(t.p0.z, t.p1.z), t.p2.z));
+            if (tlb > fmin(best, leaf_bound) || tlb >= cap) continue;
+            const D3 t1[3] = {d3(t.p0.x, t.p0.y, t.p0.z), d3(t.p1.x, t.p1.y, t.p1.z), d3(t.p2.x, t.p2.y, t.p2.z)};
+            D3 o1, o2;
+            const double dist = tri_tri_d(t1, t2, o1, o2);
+            if (dist < cap && (dist < best || (dist == best && s < best_face))) {
+              if (dist < best) atomicMin(A.best_bits, static_cast<unsigned long long>(__double_as_longlong(dist)));
+              best = dist;
+              best_face = s;
+            }
+          }
+        } else {
+          const double bound = fmin(best, shared_best(A.best_bits));
+          const WideNode nd = M.nodes[ref];
+          const double dl = box_box_lb(mn, mx, mag, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y);
+          const double dr = box_box_lb(mn, mx, mag, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w);
+          const bool hl = dl <= bound && dl < cap, hr = dr <= bound && dr < cap;
+          if (hl && hr) {
+            const bool left_first = dl <= dr;
+            // the upper word of a non-negative double rounds it down: still a lower bound when popped
+            if (!st.push(left_first ? nd.meta.y : nd.meta.x,
+                         static_cast<uint32_t>(static_cast<uint64_t>(__double_as_longlong(left_first ? dr : dl)) >> 32))) {
+              overflow = true;
+              break;
+            }
+            ref = left_first ? nd.meta.x : nd.meta.y;
+            continue;
+          }
+          if (hl || hr) {
+            ref = hl ? nd.meta.x : nd.meta.y;
+            continue;
+          }
+        }
+        ref = kDone;
+        while (st.sp > 0) {
+          const uint2 en = st.pop();
+          const double lb = __longlong_as_double(static_cast<long long>(static_cast<uint64_t>(en.y) << 32));
+          if (lb <= fmin(best, shared_best(A.best_bits))) {
+            ref = en.x;
+            break;
+          }
+        }
+        if (ref == kDone) break;
+      }
+    }
+    if (overflow) atomicOr(A.status, 1u);
+    A.lane_dist[j] = best;
+    A.lane_face[j] = best_face;
+  }
+}
+
+// A first bound for every lane: each other triangle follows the nearer child down to one leaf per root
+// and publishes its closest pair there. Always a real pair's distance, so never below the overall
+// minimum: the traversal that follows finds the same answer, with less to visit.
+__global__ __launch_bounds__(kBlock) void k_geometry_seed(GeometryArgs A) {
+  const MeshView M = A.mesh;
+  for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; j < A.n_other;
+       j += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    D3 t2[3];
+    if (!other_triangle(A, j, t2)) continue;
+    double mn[3], mx[3];
+    for (int k = 0; k < 3; k++) {
+      const double a = k == 0 ? t2[0].x : (k == 1 ? t2[0].y : t2[0].z);
+      const double b = k == 0 ? t2[1].x : (k == 1 ? t2[1].y : t2[1].z);
+      const double c = k == 0 ? t2[2].x : (k == 1 ? t2[2].y : t2[2].z);
+      mn[k] = fmin(fmin(a, b), c);
+      mx[k] = fmax(fmax(a, b), c);
+    }
+    double best = __builtin_huge_val();
+    for (uint32_t r = 0; r < M.n_roots; r++) {
+      uint32_t ref = M.roots[r];
+      while (!(ref & kRefLeaf)) {
+        const WideNode nd = M.nodes[ref];
+        const double dl = box_box_lb(mn, mx, 0.0, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y);
+        const double dr = box_box_lb(mn, mx, 0.0, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w);
+        ref = dl <= dr ? nd.meta.x : nd.meta.y;
+      }
+      uint32_t off, cnt;
+      leaf_range(ref, M.bigleaf, off, cnt);
+      for (uint32_t s = off; s < off + cnt; s++) {
+        const Tri t = load_tri(M.tris, s);
+        const D3 t1[3] = {d3(t.p0.x, t.p0.y, t.p0.z), d3(t.p1.x, t.p1.y, t.p1.z), d3(t.p2.x, t.p2.y, t.p2.z)};
+        D3 o1, o2;
+        best = fmin(best, tri_tri_d(t1, t2, o1, o2));
+      }
+    }
+    if (best < A.max_threshold) atomicMin(A.best_bits, static_cast<unsigned long long>(__double_as_longlong(best)));
+  }
+}
+
+__global__ void k_geometry_clear(GeometryArgs A) {
+  *A.best_bits = static_cast<unsigned long long>(__double_as_longlong(__builtin_huge_val()));
+  *A.winner = kNoHit;
+  *A.status = 0;
+}
+
+// The lowest other triangle whose closest distance is the overall minimum.
+__global__ __launch_bounds__(kBlock) void k_geometry_pick(GeometryArgs A) {
+  const double best = __longlong_as_double(static_cast<long long>(*A.best_bits));
+  for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; j < A.n_other;
+       j += static_cast<uint64_t>(gridDim.x) * kBlock)
+    if (A.lane_face[j] != kNoHit && A.lane_dist[j] == best) atomicMin(A.winner, static_cast<uint32_t>(j));
+}
+
+// The answer record: the winning pair's closest points (recomputed by the same code, so the same
+// values), the other point and the reference's target2.distance in the other geometry's frame.
+__global__ void k_geometry_gather(GeometryArgs A) {
+  MBVH_NO_CONTRACT
+  GeometryHit h{};
+  h.status = static_cast<int32_t>(*A.status);
+  const uint32_t j = *A.winner;
+  const double nan = __builtin_nan("");
+  if (j == kNoHit || h.status) {
+    h.distance = __builtin_huge_val();
+    h.face_index = h.other_face_index = kNoHit;
+    for (int k = 0; k < 3; k++) h.point[k] = h.other_point[k] = nan;
+    h.other_distance = __builtin_huge_val();
+  } else {
+    const uint32_t s = A.lane_face[j];
+    D3 t2[3];
+    other_triangle(A, j, t2);
+    const Tri t = load_tri(A.mesh.tris, s);
+    const D3 t1[3] = {d3(t.p0.x, t.p0.y, t.p0.z), d3(t.p1.x, t.p1.y, t.p1.z), d3(t.p2.x, t.p2.y, t.p2.z)};
+    D3 o1, o2;
+    h.distance = tri_tri_d(t1, t2, o1, o2);
+    h.face_index = s;
+    h.other_face_index = j;
+    h.point[0] = o1.x;
+    h.point[1] = o1.y;
+    h.point[2] = o1.z;
+    const D3 q1 = apply_matrix4_d(A.inverse, o1.x, o1.y, o1.z), q2 = apply_matrix4_d(A.inverse, o2.x, o2.y, o2.z);
+    h.other_point[0] = q2.x;
+    h.other_point[1] = q2.y;
+    h.other_point[2] = q2.z;
+    h.other_distance = sqrt(ddist2(q1, q2));
+  }
+  *A.out = h;
+}
+
 __global__ void k_apply_side_mask(float4* tris, uint64_t n_slots, const uint8_t* mask) {
   const uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n_slots) return;
@@ -1956,6 +2345,18 @@ hipError_t launch_scene_raycast_first(const SceneArgs& a, uint64_t threads, hipS
 hipError_t launch_intersects_box(const BoxArgs& a, uint64_t threads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_intersects_box, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_closest_to_geometry(const GeometryArgs& a, uint64_t threads, hipStream_t s) {
+  hipLaunchKernelGGL(k_geometry_clear, dim3(1), dim3(1), 0, s, a);
+  if (a.n_other) {
+    const dim3 grid(static_cast<uint32_t>(threads / kBlock));
+    hipLaunchKernelGGL(k_geometry_seed, grid, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_closest_to_geometry, grid, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_geometry_pick, grid, dim3(kBlock), 0, s, a);
+  }
+  hipLaunchKernelGGL(k_geometry_gather, dim3(1), dim3(1), 0, s, a);
   return hipGetLastError();
 }
 

@@ -589,6 +589,49 @@ static napi_value js_intersects_box(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* closestPointToGeometry(mesh, positions F32, stride, offset, index U16/U32|null, triCount, geometryToBvh F64 (16),
+   minThreshold, maxThreshold, out F64 (10): distance, point xyz, otherPoint xyz, otherDistance, faceIndex,
+   otherFaceIndex) -> true when a pair was found */
+static napi_value js_closest_to_geometry(napi_env env, napi_callback_info info) {
+  napi_value argv[10];
+  get_args(env, info, 10, argv);
+  MeshHandle* h = get_mesh(env, argv[0]);
+  if (!h) return throw_msg(env, "closestPointToGeometry: invalid mesh");
+  View pos, idx, mat, out;
+  int32_t stride = 3, offset = 0;
+  double tris = 0, lo = 0, hi = 0;
+  const int has_index = !is_nullish(env, argv[4]);
+  if (!get_view(env, argv[1], &pos) || pos.type != napi_float32_array || !get_i32(env, argv[2], &stride) ||
+      !get_i32(env, argv[3], &offset) || (has_index && (!get_view(env, argv[4], &idx) ||
+      (idx.type != napi_uint16_array && idx.type != napi_uint32_array))) || !get_f64(env, argv[5], &tris) ||
+      !get_view(env, argv[6], &mat) || mat.type != napi_float64_array || mat.length != 16 ||
+      !get_f64(env, argv[7], &lo) || !get_f64(env, argv[8], &hi) || !get_view(env, argv[9], &out) ||
+      out.type != napi_float64_array || out.length != 10 || stride < 3 || offset < 0 || tris < 0 ||
+      (has_index && 3 * (size_t)tris > idx.length))
+    return throw_msg(env, "closestPointToGeometry: bad arguments");
+  const uint64_t nv = pos.length >= (size_t)offset + 3 ? (pos.length - (size_t)offset - 3) / (size_t)stride + 1 : 0;
+  mbvh_ctx* ctx = NULL;
+  mbvh_geometry_hit r;
+  int rc = context_for(env, h->device, &ctx);
+  if (!rc)
+    rc = mbvh_closest_point_to_geometry(ctx, h->mesh, (const float*)pos.data, nv, (uint32_t)stride, (uint32_t)offset,
+                                        has_index ? idx.data : NULL, has_index ? (uint32_t)idx.elt : 0, (uint64_t)tris,
+                                        (const double*)mat.data, lo, hi, &r);
+  if (rc) return throw_status(env, rc, "mbvh_closest_point_to_geometry");
+  double* o = (double*)out.data;
+  o[0] = r.distance;
+  for (int k = 0; k < 3; k++) {
+    o[1 + k] = r.point[k];
+    o[4 + k] = r.other_point[k];
+  }
+  o[7] = r.other_distance;
+  o[8] = r.face_index;
+  o[9] = r.other_face_index;
+  napi_value res;
+  napi_get_boolean(env, r.face_index != 0xFFFFFFFFu, &res);
+  return res;
+}
+
 /* textureSizes(mesh) -> [bounds, contents, index, position] texture sides */
 static napi_value js_texture_sizes(napi_env env, napi_callback_info info) {
   napi_value argv[1];
@@ -986,6 +1029,7 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("sceneCreate", js_scene_create);
   EXPORT("textureSizes", js_texture_sizes);
   EXPORT("intersectsBox", js_intersects_box);
+  EXPORT("closestPointToGeometry", js_closest_to_geometry);
   EXPORT("exportTextures", js_export_textures);
   EXPORT("exportNodes", js_export_nodes);
   EXPORT("freeScene", js_free_batch);

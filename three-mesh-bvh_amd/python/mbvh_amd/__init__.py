@@ -38,7 +38,8 @@ EXPORTED_SYMBOLS = (
     "mbvh_order_workspace_bytes", "mbvh_order_points_d",
     "mbvh_batch_create", "mbvh_batch_free", "mbvh_batch_workspace_bytes", "mbvh_batch_raycast_first_d",
     "mbvh_batch_raycast_first", "mbvh_scene_create", "mbvh_scene_free", "mbvh_scene_raycast_first",
-    "mbvh_scene_raycast_first_d", "mbvh_intersects_box", "mbvh_intersects_box_d", "mbvh_mesh_texture_sizes", "mbvh_mesh_export_textures",
+    "mbvh_scene_raycast_first_d", "mbvh_intersects_box", "mbvh_intersects_box_d",
+    "mbvh_closest_point_to_geometry", "mbvh_closest_point_to_geometry_d", "mbvh_mesh_texture_sizes", "mbvh_mesh_export_textures",
     "mbvh_mesh_export_textures_d", "mbvh_mesh_export_nodes", "mbvh_mesh_export_nodes_d",
     "mbvh_batch_autotune", "mbvh_batch_graph_create", "mbvh_batch_graph_launch", "mbvh_batch_graph_free",
     "mbvh_instance_rays_d", "mbvh_instance_hits_d",
@@ -79,7 +80,7 @@ def library() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise MbvhError(-100, f"{LIB_PATH} is missing; build it with __graft_entry__.build()")
     lib = C.CDLL(LIB_PATH)
-    vp, u64, u32, i32, f32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32, C.c_float
+    vp, u64, u32, i32, f32, f64 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32, C.c_float, C.c_double
     sig = {
         "mbvh_abi_version": (u32, []),
         "mbvh_last_error": (C.c_char_p, []),
@@ -130,6 +131,8 @@ def library() -> C.CDLL:
         "mbvh_scene_raycast_first": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp]),
         "mbvh_intersects_box": (C.c_int, [vp, vp, vp, vp, vp, u64, vp]),
         "mbvh_intersects_box_d": (C.c_int, [vp, vp, vp, vp, vp, u64, vp, vp]),
+        "mbvh_closest_point_to_geometry": (C.c_int, [vp, vp, vp, u64, u32, u32, vp, u32, u64, vp, f64, f64, vp]),
+        "mbvh_closest_point_to_geometry_d": (C.c_int, [vp, vp, vp, u64, u32, u32, vp, u32, u64, vp, f64, f64, vp, vp]),
         "mbvh_mesh_texture_sizes": (C.c_int, [vp, vp]),
         "mbvh_mesh_export_textures": (C.c_int, [vp, vp, vp, u32, u32, vp, vp, vp, vp]),
         "mbvh_mesh_export_textures_d": (C.c_int, [vp, vp, vp, u32, u32, vp, vp, vp, vp, vp]),
@@ -309,6 +312,19 @@ class Context:
                                              C.c_void_p(workspace_ptr), workspace_bytes, C.c_void_p(stream or None)))
 
 
+# mbvh_geometry_hit
+GEOMETRY_HIT = np.dtype([("distance", "f8"), ("point", "f8", 3), ("other_point", "f8", 3), ("other_distance", "f8"),
+                         ("face_index", "u4"), ("other_face_index", "u4"), ("status", "i4"), ("pad", "u4")])
+
+
+def _geometry_hit(r):
+    if not np.isfinite(r["distance"]):
+        return None
+    return {"distance": float(r["distance"]), "point": r["point"].copy(), "face_index": int(r["face_index"]),
+            "other_point": r["other_point"].copy(), "other_distance": float(r["other_distance"]),
+            "other_face_index": int(r["other_face_index"])}
+
+
 class Mesh:
     """Device-resident BVH (mbvh_mesh)."""
 
@@ -360,6 +376,29 @@ class Mesh:
         _check(self.ctx._lib.mbvh_intersects_box(self.ctx.handle, self.handle, _ptr(mn), _ptr(mx), _ptr(m),
                                                  mn.shape[0], _ptr(out)))
         return out.astype(bool)
+
+    def closest_point_to_geometry(self, positions: np.ndarray, index: Optional[np.ndarray] = None,
+                                  geometry_to_bvh: Optional[np.ndarray] = None, min_threshold: float = 0.0,
+                                  max_threshold: float = float("inf")):
+        """closestPointToGeometry: the closest pair between this mesh's triangles and another geometry's
+        (positions (V, 3) float32, optional uint16 / uint32 index), the other placed in this mesh's frame
+        by geometry_to_bvh (4x4 column-major, 16 numbers). Returns None when no pair is closer than
+        max_threshold, else a dict: distance, point (on this mesh), face_index (triangle slot),
+        other_point / other_distance (in the other geometry's frame), other_face_index."""
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        idx = None if index is None else np.ascontiguousarray(index)
+        if idx is not None and idx.dtype not in (np.uint16, np.uint32):
+            raise ValueError("index must be uint16 or uint32")
+        n_tris = (idx.size if idx is not None else pos.shape[0]) // 3
+        m = np.eye(4).reshape(-1) if geometry_to_bvh is None else np.asarray(geometry_to_bvh, np.float64).reshape(-1)
+        if m.size != 16:
+            raise ValueError("geometry_to_bvh must hold 16 numbers")
+        m = np.ascontiguousarray(m, np.float64)
+        out = np.zeros(1, GEOMETRY_HIT)
+        _check(self.ctx._lib.mbvh_closest_point_to_geometry(
+            self.ctx.handle, self.handle, _ptr(pos), pos.shape[0], 3, 0, _ptr(idx), 0 if idx is None else idx.itemsize,
+            n_tris, _ptr(m), float(min_threshold), float(max_threshold), _ptr(out)))
+        return _geometry_hit(out[0])
 
     def texture_sizes(self):
         """Texture sides (bounds, contents, index, position) of MeshBVHUniformStruct for this mesh."""

@@ -297,6 +297,39 @@ class MeshBVH {
     return out;
   }
 
+  // closestPointToGeometry (src/core/MeshBVH.js:840-853): the closest pair of triangles between this
+  // BVH's geometry and otherGeometry placed by geometryToBvh (Matrix4 or 16 numbers). target1 gets the
+  // point on this geometry, distance and faceIndex (the triangle slot the reference reports); target2
+  // (unless null) the point on otherGeometry in its own frame, its distance and faceIndex. Returns
+  // target1, or null when no pair is closer than maxThreshold. The pair is the closest overall (equal
+  // distances: the lowest other triangle, then the lowest slot); the reference may stop earlier at any
+  // pair under minThreshold, and the closest pair is under it whenever such a pair exists.
+  closestPointToGeometry(otherGeometry, geometryToBvh, target1 = {}, target2 = {}, minThreshold = 0,
+    maxThreshold = Infinity) {
+    const e = geometryToBvh && geometryToBvh.elements ? geometryToBvh.elements : geometryToBvh;
+    const pos = GU.positionLayout(otherGeometry);
+    const index = otherGeometry.index ? otherGeometry.index.array : null;
+    const triCount = Math.floor((otherGeometry.index ? otherGeometry.index.count
+      : otherGeometry.attributes.position.count) / 3);
+    const out = new Float64Array(10);
+    const found = native.load().closestPointToGeometry(this._mesh(), pos.array, pos.stride, pos.offset,
+      index, triCount, Float64Array.from(e), minThreshold, maxThreshold, out);
+    if (!found) return null;
+    const vec = (t, x, y, z) => {
+      if (t.point && t.point.set) t.point.set(x, y, z);
+      else t.point = M.makeVec3(t.point, x, y, z);
+    };
+    vec(target1, out[1], out[2], out[3]);
+    target1.distance = out[0];
+    target1.faceIndex = out[8];
+    if (target2) {
+      vec(target2, out[4], out[5], out[6]);
+      target2.distance = out[7];
+      target2.faceIndex = out[9];
+    }
+    return target1;
+  }
+
   intersectsSphereBatch(centers, radii) {
     const c = centers instanceof Float32Array ? centers : Float32Array.from(centers);
     const r = radii instanceof Float32Array ? radii : Float32Array.from(radii);
