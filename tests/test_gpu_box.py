@@ -101,3 +101,37 @@ def test_device_form(ctx):
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy().astype(bool), host)
     mesh.free()
+
+
+def _euler_xyz(a, b, c):
+    ca, sa, cb, sb, cc, sc = np.cos(a), np.sin(a), np.cos(b), np.sin(b), np.cos(c), np.sin(c)
+    rx = np.array([[1, 0, 0], [0, ca, -sa], [0, sa, ca]])
+    ry = np.array([[cb, 0, sb], [0, 1, 0], [-sb, 0, cb]])
+    rz = np.array([[cc, -sc, 0], [sc, cc, 0], [0, 0, 1]])
+    return rx @ ry @ rz  # three's Euler order 'XYZ'
+
+
+def test_reference_intersects_cases(ctx):
+    """The reference's own IntersectsSphere / IntersectsBox cases (test/ShapeCasts.test.js:567-698) on
+    SphereGeometry(1, 50, 50): spheres on, inside and outside the surface; boxes rotated by
+    Euler(pi/4, pi/4, 0) outside, with a corner inside, enclosing, inside, and cut by the surface."""
+    g, idx, tree = build_mesh(G.sphere(1, 50, 50))
+    m = ctx.upload(tree, g.positions, idx)
+    print("case reference sphere / box cases", flush=True)
+    got = m.intersects_sphere(np.array([[0, 1, 0], [0, 0, 0], [0, 2.01, 0]], np.float32),
+                              np.array([0.01, 0.9, 0.9], np.float32))
+    assert [bool(x) for x in got] == [True, False, False]
+    R = _euler_xyz(np.pi / 4, np.pi / 4, 0)
+    cases = [((-1, -1, -1), (1, 1, 1), (0, 3, 0), False), ((-1, -1, -1), (1, 1, 1), (0, 2, 0), True),
+             ((-10, -10, -10), (10, 10, 10), (0, 0, 0), True), ((-.5, -.5, -.5), (.5, .5, .5), (0, 0, 0), False),
+             ((-10, 0, -10), (10, 10, 10), (0, 0, 0), True)]
+    mats = []
+    for _, _, pos, _ in cases:
+        mm = np.eye(4)
+        mm[:3, :3] = R
+        mm[:3, 3] = pos
+        mats.append(mm.T.reshape(-1))
+    res = m.intersects_box(np.array([c[0] for c in cases], np.float64), np.array([c[1] for c in cases], np.float64),
+                           np.array(mats))
+    assert [bool(x) for x in res] == [c[3] for c in cases]
+    m.free()

@@ -158,3 +158,46 @@ def test_groups_and_device_form(ctx):
     assert rec["distance"] == got["distance"] and rec["face_index"] == got["face_index"]
     assert rec["other_face_index"] == got["other_face_index"]
     m.free()
+
+
+def _compose(pos, scale):
+    """Matrix4.compose(pos, identity quaternion, (scale, scale, scale)) as column-major elements."""
+    m = np.eye(4)
+    m[:3, :3] *= scale
+    m[:3, 3] = pos
+    return m.T.reshape(-1)
+
+
+def test_reference_distance_to_geometry_cases(ctx):
+    """The reference's own 'Distance To Geometry' cases (test/ShapeCasts.test.js:755-870): a BVH of
+    SphereGeometry(1, 20, 20) against SphereGeometry(1, 5, 5) shrunk to the center (distance ~ the
+    radius), placed on the surface (0), and at random distances from it (|1 - r| - 0.1 within the
+    faceting error); spheres (1, 2n, n) three units apart (the returned points are ~1 apart)."""
+    g, idx, tree = build_mesh(G.sphere(1, 20, 20))
+    m = ctx.upload(tree, g.positions, idx)
+    other = G.sphere(1, 5, 5).ensure_index()
+    print("case reference sphere cases", flush=True)
+    r = m.closest_point_to_geometry(other.positions, other.index, _compose([0, 0, 0], 0.001))
+    assert 1 - 0.05 <= r["distance"] <= 1
+    r = m.closest_point_to_geometry(other.positions, other.index, _compose([0, 1, 0], 0.1))
+    assert r["distance"] == 0
+    rng = np.random.default_rng(11)
+    for _ in range(10):
+        p = rng.random(3) - 0.5
+        length = rng.random() * 3
+        p = p / np.linalg.norm(p) * length
+        expected = abs(1 - length)
+        expected = 0 if expected < 0.1 else expected - 0.1
+        r = m.closest_point_to_geometry(other.positions, other.index, _compose(p, 0.1))
+        assert expected - 0.1 <= r["distance"] <= expected + 0.1
+    m.free()
+    for n in (4, 8, 16):
+        g, idx, tree = build_mesh(G.sphere(1, 2 * n, n))
+        m = ctx.upload(tree, g.positions, idx)
+        o = G.sphere(1, 2 * n, n).ensure_index()
+        e = _compose([3, 0, 0], 1.0)
+        r = m.closest_point_to_geometry(o.positions, o.index, e)
+        p2 = r["other_point"] + np.array([3.0, 0, 0])  # target2.point.applyMatrix4(matrix)
+        assert abs(np.linalg.norm(r["point"] - p2) - 1) < 0.05
+        assert abs(r["distance"] - np.linalg.norm(r["point"] - p2)) < 1e-9
+        m.free()
