@@ -1300,6 +1300,13 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
           f3 best_cp = mk(kInf, kInf, kInf);
           for (uint32_t s = off + q4; s < off + cnt; s += 4) {
             const Tri tr = load_tri(M.tris, s);
+            // the triangle's own box first: its squared distance (clamp exact, three roundings per
+            // axis and two in the sum, covered by 8 ulps) bounds the triangle's from below, so a
+            // box beyond the upper bound Ul rules the triangle out without the full test
+            const float sb = box_score(q, fminf(fminf(tr.p0.x, tr.p1.x), tr.p2.x), fminf(fminf(tr.p0.y, tr.p1.y), tr.p2.y),
+                                       fminf(fminf(tr.p0.z, tr.p1.z), tr.p2.z), fmaxf(fmaxf(tr.p0.x, tr.p1.x), tr.p2.x),
+                                       fmaxf(fmaxf(tr.p0.y, tr.p1.y), tr.p2.y), fmaxf(fmaxf(tr.p0.z, tr.p1.z), tr.p2.z));
+            if (sb * (1.0f - 8.0f * kU) > Ul) continue;
             const f3 cp = closest_on_triangle(q, tr.a(), tr.b(), tr.c());
             const f3 dv = sub(q, cp);
             const float d2 = dot(dv, dv);
