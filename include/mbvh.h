@@ -447,6 +447,22 @@ void mbvh_scene_free(mbvh_scene* scene);
 int mbvh_scene_raycast_first_d(mbvh_ctx* ctx, const mbvh_scene* scene, const float* origins, const float* directions,
                                uint64_t n, int32_t side, float near_dist, float far_dist, uint32_t* face_index,
                                uint32_t* instance, float* distance, float* point, float* barycoord, void* stream);
+/*
+ * Closest point over the instances of a scene: the reference's bvh_ClosestPointToPoint over a
+ * ClusteredBVH (src/webgpu/shapecastFns/getClosestPointToPointFn.js:20-151, traversed as in
+ * src/webgpu/shapecastFns/getShapecastFn.js:97-217) — node boxes and triangles taken to world space
+ * by each instance's matrix, the least world distance wins — for n world points (3n floats). Triangles
+ * and distances in double; equal distances go to the lowest instance, then the lowest face. Outputs
+ * per point: face_index (in the instance's geometry), instance, world distance, world point and
+ * barycoord (NULL outputs other than face_index are skipped). Host-pointer form (synchronous) and
+ * device-pointer form (enqueues on `stream`; face 0xFFFFFFFE marks a traversal stack overflow).
+ */
+int mbvh_scene_closest_point(mbvh_ctx* ctx, const mbvh_scene* scene, const float* points, uint64_t n,
+                             uint32_t* face_index, uint32_t* instance, float* distance, float* point,
+                             float* barycoord);
+int mbvh_scene_closest_point_d(mbvh_ctx* ctx, const mbvh_scene* scene, const float* points, uint64_t n,
+                               uint32_t* face_index, uint32_t* instance, float* distance, float* point,
+                               float* barycoord, void* stream);
 /* Host-pointer form: stages the inputs, launches, copies the results back and synchronises. */
 int mbvh_scene_raycast_first(mbvh_ctx* ctx, const mbvh_scene* scene, const float* origins, const float* directions,
                              uint64_t n, int32_t side, float near_dist, float far_dist, uint32_t* face_index,

@@ -1246,3 +1246,42 @@ uint32_t orc_closest_to_geometry(const orc_mesh* m, const float* positions, uint
   }
   return win;
 }
+
+/* The scene closest point (bvh_ClosestPointToPoint over a ClusteredBVH,
+   src/webgpu/shapecastFns/getClosestPointToPointFn.js:20-151) by brute force: every triangle of every
+   visible instance taken to world space (applyMatrix4 of the float matrix in double), three's
+   Triangle.closestPointToPoint, the least squared world distance; equal distances to the lowest
+   instance, then the lowest face. meshes[g] is instance geometry g; world: 16 floats per instance.
+   Returns the instance (NO_HIT when none) and writes the face, squared distance and world point. */
+uint32_t orc_scene_closest_point(const orc_mesh* const* meshes, const float* world, const uint32_t* geometry,
+                                 const uint8_t* visible, uint32_t n_instances, const float p[3], uint32_t* face,
+                                 double* d2_out, double point[3]) {
+  const double q[3] = {p[0], p[1], p[2]};
+  double best = INFINITY;
+  uint32_t win = NO_HIT;
+  *face = NO_HIT;
+  for (uint32_t i = 0; i < n_instances; i++) {
+    if (visible && !visible[i]) continue;
+    const orc_mesh* m = meshes[geometry[i]];
+    double e[16];
+    for (int k = 0; k < 16; k++) e[k] = world[16 * (uint64_t)i + k];
+    const uint64_t n_tri = m->index ? m->index_count / 3 : m->vertex_count / 3;
+    for (uint64_t f = 0; f < n_tri; f++) {
+      double a[3], b[3], c[3], wa[3], wb[3], wc[3], cp[3];
+      face_vertices(m, (uint32_t)f, a, b, c);
+      apply_m4(e, a, wa);
+      apply_m4(e, b, wb);
+      apply_m4(e, c, wc);
+      orc_closest_point_triangle(q, wa, wb, wc, cp);
+      const double d2 = v_dist_sq(q, cp);
+      if (d2 < best) {
+        best = d2;
+        win = i;
+        *face = (uint32_t)f;
+        memcpy(point, cp, sizeof(cp));
+      }
+    }
+  }
+  *d2_out = best;
+  return win;
+}

@@ -112,6 +112,10 @@ int orc_box_triangle_sat(const double bmin[3], const double bmax[3], const doubl
 int orc_box_triangle_clip(const double bmin[3], const double bmax[3], const double e[16], const double a[3],
                           const double b[3], const double c[3]);
 
+/* Scene closest point by brute force over every visible instance's world triangles. */
+uint32_t orc_scene_closest_point(const orc_mesh* const* meshes, const float* world, const uint32_t* geometry,
+                                 const uint8_t* visible, uint32_t n_instances, const float p[3], uint32_t* face,
+                                 double* d2_out, double point[3]);
 /* closestPointsSegmentToSegment (MathGeoLib's form, as the reference restates it) */
 void orc_segment_segment(const double s1[3], const double e1[3], const double s2[3], const double e2[3], double o1[3],
                          double o2[3]);

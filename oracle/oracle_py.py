@@ -72,6 +72,8 @@ def library():
         lib.orc_box_triangle_sat.restype = C.c_int
         lib.orc_box_triangle_clip.argtypes = [vp, vp, vp, vp, vp, vp]
         lib.orc_box_triangle_clip.restype = C.c_int
+        lib.orc_scene_closest_point.argtypes = [vp, vp, vp, vp, u32, vp, vp, vp, vp]
+        lib.orc_scene_closest_point.restype = u32
         lib.orc_segment_segment.argtypes = [vp, vp, vp, vp, vp, vp]
         lib.orc_segment_segment.restype = None
         lib.orc_triangle_distance.argtypes = [vp, vp, vp, vp]
@@ -349,3 +351,28 @@ def segment_segment(s1, e1, s2, e2):
     o2 = np.zeros(3)
     library().orc_segment_segment(*[_p(x) for x in v], _p(o1), _p(o2))
     return o1, o2
+
+
+def scene_closest_point(meshes, world, geometry, visible, points):
+    """Scene closest point by brute force (world space, double): per point (instance, face, distance,
+    world point); equal distances to the lowest instance, then the lowest face."""
+    refs = (C.c_void_p * len(meshes))(*[C.cast(C.pointer(m._m), C.c_void_p) for m in meshes])
+    w = np.ascontiguousarray(world, np.float32).reshape(-1)
+    g = np.ascontiguousarray(geometry, np.uint32)
+    vis = None if visible is None else np.ascontiguousarray(visible, np.uint8)
+    q = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    n = q.shape[0]
+    inst = np.empty(n, np.uint32)
+    face = np.empty(n, np.uint32)
+    dist = np.empty(n)
+    pts = np.empty((n, 3))
+    f = C.c_uint32()
+    d2 = C.c_double()
+    for i in range(n):
+        p = np.zeros(3)
+        inst[i] = library().orc_scene_closest_point(refs, _p(w), _p(g), _p(vis), g.size, _p(q[i]), C.byref(f),
+                                                    C.byref(d2), _p(p))
+        face[i] = f.value
+        dist[i] = np.sqrt(d2.value)
+        pts[i] = p
+    return {"instance": inst, "face": face, "distance": dist, "point": pts}

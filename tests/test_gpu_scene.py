@@ -169,3 +169,80 @@ def test_empty_and_device_form(ctx, meshes):
     with pytest.raises(M.MbvhError):
         ctx.scene(ms, w.matrix_world, w.matrix_world_inverse, np.full(12, 99, np.uint32))
     sc.free()
+
+
+def _scene_points(w, n, seed):
+    """World points around the instances: each near a random instance's origin, up to ~2 units away."""
+    rng = np.random.default_rng(seed)
+    t = w.matrix_world[:, 12:15].astype(np.float64)
+    pick = rng.integers(0, t.shape[0], n)
+    return (t[pick] + rng.normal(0, 1.2, (n, 3))).astype(np.float32)
+
+
+def _world_tri(built, w, inst, face):
+    g, idx, _ = built[w.instance_geometry[inst]]
+    v = g.positions[idx[3 * face:3 * face + 3]].astype(np.float64)
+    m = w.matrix_world[inst].astype(np.float64).reshape(4, 4).T
+    return v @ m[:3, :3].T + m[:3, 3]
+
+
+@pytest.mark.parametrize("non_uniform", [0.0, 0.3])
+def test_scene_closest_point_equals_oracle(ctx, meshes, non_uniform):
+    """Scene closest point (mbvh_scene_closest_point) against the oracle's brute force over every
+    visible instance's world triangles: the instance and face are the oracle's, or an equal-distance
+    tie (the oracle's distance to the GPU's world triangle within 1e-6 of the best); distance and point
+    to 1e-5 of the scene scale. Rigid and non-uniformly scaled instances, some hidden."""
+    built, ms = meshes
+    geos = [b[0] for b in built]
+    w = W.scene_rays(geos, 24, 8, seed=17, non_uniform=non_uniform)
+    visible = np.ones(24, np.uint8)
+    visible[[3, 11]] = 0
+    pts = _scene_points(w, 300, 5)
+    print(f"case scene closest point non_uniform {non_uniform}", flush=True)
+    sc = ctx.scene(ms, w.matrix_world, w.matrix_world_inverse, w.instance_geometry, visible)
+    got = sc.closest_point(pts)
+    oms = [O.OracleMesh(g.positions, idx, t.roots) for g, idx, t in built]
+    ref = O.scene_closest_point(oms, w.matrix_world, w.instance_geometry, visible, pts)
+    scale = float(np.abs(w.matrix_world[:, 12:15]).max()) + 1.0
+    assert np.all(got["face"] != M.NO_HIT) and not np.any(np.isin(got["instance"], [3, 11]))
+    np.testing.assert_allclose(got["distance"], ref["distance"], rtol=1e-5, atol=1e-5 * scale)
+    same = (got["instance"] == ref["instance"]) & (got["face"] == ref["face"])
+    for i in np.nonzero(~same)[0]:
+        d, _, _ = None, None, None
+        cp = O.closest_point_triangle(pts[i].astype(np.float64), *_world_tri(built, w, got["instance"][i], got["face"][i]))
+        d = np.linalg.norm(cp - pts[i])
+        assert d <= ref["distance"][i] * (1 + 1e-6) + 1e-9, (i, d, ref["distance"][i])
+    np.testing.assert_allclose(got["point"][same], ref["point"][same], atol=1e-5 * scale)
+    assert same.mean() > 0.95
+    # barycoords reproduce the point on the world triangle
+    for i in np.nonzero(same)[0][:50]:
+        a, b, c = _world_tri(built, w, got["instance"][i], got["face"][i])
+        bc = got["barycoord"][i].astype(np.float64)
+        np.testing.assert_allclose(bc[0] * a + bc[1] * b + bc[2] * c, got["point"][i], atol=1e-4 * scale)
+    sc.free()
+
+
+def test_scene_closest_point_device_form(ctx, meshes):
+    torch = pytest.importorskip("torch")
+    built, ms = meshes
+    geos = [b[0] for b in built]
+    w = W.scene_rays(geos, 16, 8, seed=23)
+    pts = _scene_points(w, 2000, 9)
+    print("case scene closest point device form", flush=True)
+    sc = ctx.scene(ms, w.matrix_world, w.matrix_world_inverse, w.instance_geometry)
+    host = sc.closest_point(pts)
+    dev = torch.device("cuda:0")
+    q = torch.from_numpy(pts).to(dev)
+    face = torch.empty(pts.shape[0], dtype=torch.int32, device=dev)
+    inst = torch.empty(pts.shape[0], dtype=torch.int32, device=dev)
+    dist = torch.empty(pts.shape[0], device=dev)
+    torch.cuda.synchronize()
+    sc.closest_point_d(q.data_ptr(), pts.shape[0], face.data_ptr(), inst.data_ptr(), dist.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(face.cpu().numpy().view(np.uint32), host["face"])
+    assert np.array_equal(inst.cpu().numpy().view(np.uint32), host["instance"])
+    assert np.array_equal(dist.cpu().numpy().view(np.uint32), host["distance"].view(np.uint32))
+    # repeatable
+    again = sc.closest_point(pts)
+    assert np.array_equal(again["face"], host["face"]) and np.array_equal(again["instance"], host["instance"])
+    sc.free()

@@ -2409,6 +2409,78 @@ extern "C" int mbvh_scene_raycast_first(mbvh_ctx* ctx, const mbvh_scene* scene, 
   return check_no_overflow(face_index, n, fn);
 }
 
+// ---- scenes: closest point over instances -------------------------------------------------------------
+namespace {
+
+int enqueue_scene_points(mbvh_ctx* ctx, const mbvh_scene* sc, const float* points, uint64_t n, uint32_t* face,
+                         uint32_t* inst, float* dist, float* point, float* bary, hipStream_t s) {
+  if (sc->max_stack > 8 + ctx->stack_ovf_entries)
+    return set_error(MBVH_ERR_LIMIT, "scene: stack scratch smaller than the tree depth");
+  SceneArgs a{};
+  a.meshes = sc->d_meshes;
+  a.n_meshes = sc->n_meshes;
+  a.tlas = sc->d_tlas;
+  a.tlas_inst = sc->d_tlas_inst;
+  a.world = sc->d_world;
+  a.inverse = sc->d_inverse;
+  a.instance_geometry = sc->d_geom;
+  a.origins = points;
+  a.n = n;
+  a.out_face = face;
+  a.out_instance = inst;
+  a.out_dist = dist;
+  a.out_point = point;
+  a.out_bary = bary;
+  const uint64_t threads = launch_threads(n, ctx->max_threads);
+  a.stack = stack_of(ctx, threads);
+  MBVH_HIP(launch_scene_closest_point(a, threads, s));
+  return MBVH_OK;
+}
+
+}  // namespace
+
+extern "C" int mbvh_scene_closest_point_d(mbvh_ctx* ctx, const mbvh_scene* scene, const float* points, uint64_t n,
+                                          uint32_t* face_index, uint32_t* instance, float* distance, float* point,
+                                          float* barycoord, void* stream) {
+  clear_error();
+  const char* fn = "mbvh_scene_closest_point_d";
+  int rc = check_scene_call(fn, ctx, scene, MBVH_FRONT_SIDE, n, points, points, face_index);
+  if (rc) return rc;
+  if ((rc = set_device(ctx))) return rc;
+  return enqueue_scene_points(ctx, scene, points, n, face_index, instance, distance, point, barycoord,
+                              stream ? static_cast<hipStream_t>(stream) : ctx->stream);
+}
+
+extern "C" int mbvh_scene_closest_point(mbvh_ctx* ctx, const mbvh_scene* scene, const float* points, uint64_t n,
+                                        uint32_t* face_index, uint32_t* instance, float* distance, float* point,
+                                        float* barycoord) {
+  clear_error();
+  const char* fn = "mbvh_scene_closest_point";
+  int rc = check_scene_call(fn, ctx, scene, MBVH_FRONT_SIDE, n, points, points, face_index);
+  if (rc) return rc;
+  if (n == 0) return MBVH_OK;
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t s = ctx->stream;
+  const size_t v3 = 3 * n * sizeof(float), s1 = n * sizeof(float);
+  void *dQ, *dF, *dI = nullptr, *dT = nullptr, *dP = nullptr, *dB = nullptr;
+  if ((rc = stage_in(ctx, ctx->staging[0], points, v3, &dQ)) || (rc = stage_out(ctx->staging[2], face_index, s1, &dF)) ||
+      (rc = stage_out(ctx->staging[3], instance, s1, &dI)) || (rc = stage_out(ctx->staging[4], distance, s1, &dT)) ||
+      (rc = stage_out(ctx->staging[5], point, v3, &dP)) || (rc = stage_out(ctx->staging[6], barycoord, v3, &dB)))
+    return rc;
+  if ((rc = enqueue_scene_points(ctx, scene, static_cast<const float*>(dQ), n, static_cast<uint32_t*>(dF),
+                                 static_cast<uint32_t*>(dI), static_cast<float*>(dT), static_cast<float*>(dP),
+                                 static_cast<float*>(dB), s)))
+    return rc;
+  MBVH_HIP(hipMemcpyAsync(face_index, dF, s1, hipMemcpyDeviceToHost, s));
+  if (instance) MBVH_HIP(hipMemcpyAsync(instance, dI, s1, hipMemcpyDeviceToHost, s));
+  if (distance) MBVH_HIP(hipMemcpyAsync(distance, dT, s1, hipMemcpyDeviceToHost, s));
+  if (point) MBVH_HIP(hipMemcpyAsync(point, dP, v3, hipMemcpyDeviceToHost, s));
+  if (barycoord) MBVH_HIP(hipMemcpyAsync(barycoord, dB, v3, hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipStreamSynchronize(s));
+  return check_no_overflow(face_index, n, fn);
+}
+
 // ---- export to the reference's shader data layouts ---------------------------------------------------
 
 namespace {

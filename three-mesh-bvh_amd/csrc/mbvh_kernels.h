@@ -241,6 +241,8 @@ struct SceneArgs {
   StackScratch stack;
 };
 hipError_t launch_scene_raycast_first(const SceneArgs& a, uint64_t threads, hipStream_t s);
+// Closest point over the instances: `origins` holds the world points (directions, side, near / far unused).
+hipError_t launch_scene_closest_point(const SceneArgs& a, uint64_t threads, hipStream_t s);
 
 // ---- intersectsBox ----------------------------------------------------------------------------------
 struct BoxArgs {

@@ -2290,6 +2290,206 @@ __global__ void k_geometry_gather(GeometryArgs A) {
   *A.out = h;
 }
 
+// ---- scenes: closest point over instances ------------------------------------------------------------
+// The reference's scene closest point (bvh_ClosestPointToPoint, src/webgpu/shapecastFns/
+// getClosestPointToPointFn.js:20-151 over ClusteredBVH's top level) measures in world space: node
+// boxes of an instance's tree are taken to world boxes through its matrix, triangles are taken to
+// world space, and the least world distance wins. Here the triangles and distances are in double;
+// the boxes are FP32 lower bounds with margins for their roundings.
+
+// Squared distance from p to the world box of a local box under W (local -> world, column-major): the
+// box's centre through W, its half extents through |W|, both widened by 2^-16 of the magnitudes
+// involved; a lower bound of the squared distance to anything W puts inside it.
+__device__ __forceinline__ double world_box_d2_lb(const float* W, float px, float py, float pz, float mnx, float mny,
+                                                  float mnz, float mxx, float mxy, float mxz) {
+  const float cx = 0.5f * (mnx + mxx), cy = 0.5f * (mny + mxy), cz = 0.5f * (mnz + mxz);
+  const float hx = 0.5f * (mxx - mnx), hy = 0.5f * (mxy - mny), hz = 0.5f * (mxz - mnz);
+  const float ax = fabsf(cx) + hx, ay = fabsf(cy) + hy, az = fabsf(cz) + hz;
+  float s = 0.0f;
+  for (int i = 0; i < 3; i++) {
+    const float w0 = W[i], w1 = W[4 + i], w2 = W[8 + i], t = W[12 + i];
+    const float wc = w0 * cx + w1 * cy + w2 * cz + t;
+    const float wh = fabsf(w0) * hx + fabsf(w1) * hy + fabsf(w2) * hz;
+    const float m = (fabsf(w0) * ax + fabsf(w1) * ay + fabsf(w2) * az + fabsf(t)) * (1.0f / 65536.0f);
+    const float q = i == 0 ? px : (i == 1 ? py : pz);
+    const float g = fmaxf(fabsf(q - wc) - (wh + m), 0.0f);
+    s += g * g;
+  }
+  return static_cast<double>(s) * (1.0 - 1.0 / 65536.0);
+}
+
+// The same for a world box (the top level), widened by 2^-16 of its coordinates.
+__device__ __forceinline__ double tlas_box_d2_lb(const float* b, float px, float py, float pz) {
+  float s = 0.0f;
+  for (int i = 0; i < 3; i++) {
+    const float q = i == 0 ? px : (i == 1 ? py : pz);
+    const float e = fmaxf(fabsf(b[i]), fabsf(b[3 + i])) * (1.0f / 65536.0f);
+    const float g = fmaxf(fmaxf(b[i] - e - q, q - b[3 + i] - e), 0.0f);
+    s += g * g;
+  }
+  return static_cast<double>(s) * (1.0 - 1.0 / 65536.0);
+}
+
+__device__ __forceinline__ D3 world_vertex(const double* Wd, float4 v) { return apply_matrix4_d(Wd, v.x, v.y, v.z); }
+
+struct ScenePointBest {
+  double d2;
+  uint32_t inst, face, slot;
+  D3 p;
+};
+
+// One instance's tree walked near child first in world space; updates `b` (least squared distance,
+// then lowest instance, then lowest face). Returns false on a stack overflow.
+__device__ bool scene_point_instance(const MeshView& M, const float* W, uint32_t inst, D3 p, float px, float py,
+                                     float pz, StackL<8>& st, ScenePointBest& b) {
+  double Wd[16];
+  for (int k = 0; k < 16; k++) Wd[k] = W[k];
+  for (uint32_t r = 0; r < M.n_roots; r++) {
+    st.sp = 0;
+    uint32_t ref = M.roots[r];
+    while (true) {
+      if (ref & kRefLeaf) {
+        uint32_t off, cnt;
+        leaf_range(ref, M.bigleaf, off, cnt);
+        for (uint32_t s = off; s < off + cnt; s++) {
+          const Tri t = load_tri(M.tris, s);
+          const D3 q = closest_on_tri_d(p, world_vertex(Wd, t.p0), world_vertex(Wd, t.p1), world_vertex(Wd, t.p2));
+          const double d2 = ddist2(p, q);
+          const uint32_t face = t.face();
+          if (d2 < b.d2 || (d2 == b.d2 && (inst < b.inst || (inst == b.inst && face < b.face)))) {
+            b.d2 = d2;
+            b.inst = inst;
+            b.face = face;
+            b.slot = s;
+            b.p = q;
+          }
+        }
+      } else {
+        const WideNode nd = M.nodes[ref];
+        const double dl = world_box_d2_lb(W, px, py, pz, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y);
+        const double dr = world_box_d2_lb(W, px, py, pz, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w);
+        const bool hl = dl <= b.d2, hr = dr <= b.d2;
+        if (hl && hr) {
+          const bool left_first = dl <= dr;
+          // the upper word of a non-negative double rounds it down: still a lower bound when popped
+          if (!st.push(left_first ? nd.meta.y : nd.meta.x,
+                       static_cast<uint32_t>(static_cast<uint64_t>(__double_as_longlong(left_first ? dr : dl)) >> 32)))
+            return false;
+          ref = left_first ? nd.meta.x : nd.meta.y;
+          continue;
+        }
+        if (hl || hr) {
+          ref = hl ? nd.meta.x : nd.meta.y;
+          continue;
+        }
+      }
+      ref = kDone;
+      while (st.sp > 0) {
+        const uint2 e = st.pop();
+        if (__longlong_as_double(static_cast<long long>(static_cast<uint64_t>(e.y) << 32)) <= b.d2) {
+          ref = e.x;
+          break;
+        }
+      }
+      if (ref == kDone) break;
+    }
+  }
+  return true;
+}
+
+// One lane per world point: the top level near child first (pruned by the best squared distance,
+// equal bounds still visited so ties resolve the same way every time), each instance leaf walks
+// that instance's tree. Outputs: face (in the instance's geometry), instance, world distance, world
+// point and the barycoords of the point in the world triangle (those of the local triangle too).
+__global__ __launch_bounds__(kBlock) void k_scene_closest_point(SceneArgs A) {
+  __shared__ uint2 lds_stack[8 * kBlock];
+  __shared__ uint32_t tstack[kSceneStack * kBlock];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + tid;
+  StackL<8> st;
+  st.lds = lds_stack;
+  st.ovf = A.stack.ovf;
+  st.ovf_stride = A.stack.ovf_stride;
+  st.gtid = gtid;
+  st.lane = tid;
+  st.capacity = 8 + static_cast<int>(A.stack.ovf_entries);
+  const float qnan = __int_as_float(0x7fc00000);
+  for (uint64_t i = gtid; i < A.n; i += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    const float px = A.origins[3 * i], py = A.origins[3 * i + 1], pz = A.origins[3 * i + 2];
+    const D3 p = d3(px, py, pz);
+    ScenePointBest b;
+    b.d2 = __builtin_huge_val();
+    b.inst = b.face = b.slot = kNoHit;
+    b.p = d3(0, 0, 0);
+    bool overflow = false;
+    int sp = 1;
+    tstack[tid] = 0u;
+    while (sp > 0 && !overflow) {
+      sp--;
+      const uint32_t node = tstack[sp * kBlock + tid];
+      const uint32_t* nw = A.tlas + 8ull * node;
+      if (tlas_box_d2_lb(reinterpret_cast<const float*>(nw), px, py, pz) > b.d2) continue;
+      const uint32_t w7 = nw[7];
+      if ((w7 >> 16) == 0xFFFFu) {
+        const uint32_t off = nw[6], cnt = w7 & 0xFFFFu;
+        for (uint32_t s = off; s < off + cnt && !overflow; s++) {
+          const uint32_t inst = A.tlas_inst[s];
+          overflow = !scene_point_instance(A.meshes[A.instance_geometry[inst]], A.world + 16ull * inst, inst, p, px, py,
+                                           pz, st, b);
+        }
+      } else {
+        if (sp + 2 > kSceneStack) {
+          overflow = true;
+          break;
+        }
+        const uint32_t left = node + 1, right = node + nw[6];
+        const double dl = tlas_box_d2_lb(reinterpret_cast<const float*>(A.tlas + 8ull * left), px, py, pz);
+        const double dr = tlas_box_d2_lb(reinterpret_cast<const float*>(A.tlas + 8ull * right), px, py, pz);
+        const bool left_first = dl <= dr;
+        tstack[sp * kBlock + tid] = left_first ? right : left;  // far child below the near one
+        tstack[(sp + 1) * kBlock + tid] = left_first ? left : right;
+        sp += 2;
+      }
+    }
+    uint32_t face = overflow ? kOverflow : b.face;
+    float dist = kInf, wx = qnan, wy = qnan, wz = qnan, bw = qnan, bu = qnan, bv = qnan;
+    if (!overflow && b.inst != kNoHit) {
+      // three's Triangle.getBarycoord of the point in the winning world triangle
+      const MeshView& M = A.meshes[A.instance_geometry[b.inst]];
+      double Wd[16];
+      for (int k = 0; k < 16; k++) Wd[k] = A.world[16ull * b.inst + k];
+      const Tri t = load_tri(M.tris, b.slot);
+      const D3 a = world_vertex(Wd, t.p0), bb = world_vertex(Wd, t.p1), c = world_vertex(Wd, t.p2);
+      const D3 v0 = dsub(c, a), v1 = dsub(bb, a), v2 = dsub(b.p, a);
+      const double d00 = ddot(v0, v0), d01 = ddot(v0, v1), d02 = ddot(v0, v2), d11 = ddot(v1, v1), d12 = ddot(v1, v2);
+      const double den = d00 * d11 - d01 * d01;
+      if (den != 0) {
+        const double inv = 1 / den, u = (d11 * d02 - d01 * d12) * inv, v = (d00 * d12 - d01 * d02) * inv;
+        bw = static_cast<float>(1 - u - v);
+        bu = static_cast<float>(v);
+        bv = static_cast<float>(u);
+      }
+      dist = static_cast<float>(sqrt(b.d2));
+      wx = static_cast<float>(b.p.x);
+      wy = static_cast<float>(b.p.y);
+      wz = static_cast<float>(b.p.z);
+    }
+    A.out_face[i] = face;
+    if (A.out_instance) A.out_instance[i] = overflow ? kNoHit : b.inst;
+    if (A.out_dist) A.out_dist[i] = dist;
+    if (A.out_point) {
+      A.out_point[3 * i] = wx;
+      A.out_point[3 * i + 1] = wy;
+      A.out_point[3 * i + 2] = wz;
+    }
+    if (A.out_bary) {
+      A.out_bary[3 * i] = bw;
+      A.out_bary[3 * i + 1] = bu;
+      A.out_bary[3 * i + 2] = bv;
+    }
+  }
+}
+
 __global__ void k_apply_side_mask(float4* tris, uint64_t n_slots, const uint8_t* mask) {
   const uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n_slots) return;
@@ -2352,6 +2552,12 @@ hipError_t launch_scene_raycast_first(const SceneArgs& a, uint64_t threads, hipS
 hipError_t launch_intersects_box(const BoxArgs& a, uint64_t threads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_intersects_box, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_scene_closest_point(const SceneArgs& a, uint64_t threads, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scene_closest_point, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 

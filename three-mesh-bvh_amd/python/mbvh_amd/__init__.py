@@ -39,7 +39,8 @@ EXPORTED_SYMBOLS = (
     "mbvh_batch_create", "mbvh_batch_free", "mbvh_batch_workspace_bytes", "mbvh_batch_raycast_first_d",
     "mbvh_batch_raycast_first", "mbvh_scene_create", "mbvh_scene_free", "mbvh_scene_raycast_first",
     "mbvh_scene_raycast_first_d", "mbvh_intersects_box", "mbvh_intersects_box_d",
-    "mbvh_closest_point_to_geometry", "mbvh_closest_point_to_geometry_d", "mbvh_mesh_texture_sizes", "mbvh_mesh_export_textures",
+    "mbvh_closest_point_to_geometry", "mbvh_closest_point_to_geometry_d",
+    "mbvh_scene_closest_point", "mbvh_scene_closest_point_d", "mbvh_mesh_texture_sizes", "mbvh_mesh_export_textures",
     "mbvh_mesh_export_textures_d", "mbvh_mesh_export_nodes", "mbvh_mesh_export_nodes_d",
     "mbvh_batch_autotune", "mbvh_batch_graph_create", "mbvh_batch_graph_launch", "mbvh_batch_graph_free",
     "mbvh_instance_rays_d", "mbvh_instance_hits_d",
@@ -129,6 +130,8 @@ def library() -> C.CDLL:
         "mbvh_scene_free": (None, [vp]),
         "mbvh_scene_raycast_first_d": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp, vp]),
         "mbvh_scene_raycast_first": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp]),
+        "mbvh_scene_closest_point": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp]),
+        "mbvh_scene_closest_point_d": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp, vp]),
         "mbvh_intersects_box": (C.c_int, [vp, vp, vp, vp, vp, u64, vp]),
         "mbvh_intersects_box_d": (C.c_int, [vp, vp, vp, vp, vp, u64, vp, vp]),
         "mbvh_closest_point_to_geometry": (C.c_int, [vp, vp, vp, u64, u32, u32, vp, u32, u64, vp, f64, f64, vp]),
@@ -727,6 +730,26 @@ class Scene:
             self.ctx.handle, self.handle, _ptr(o), _ptr(d), n, side, near, far, _ptr(out["face"]),
             _ptr(out["instance"]), _ptr(out["distance"]), _ptr(out["point"]), _ptr(out["barycoord"])))
         return out
+
+    def closest_point(self, points: np.ndarray):
+        """Closest point over the instances (world points in, world results out): face (in the
+        instance's geometry), instance, distance, point, barycoord."""
+        q = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        n = q.shape[0]
+        out = {"face": np.empty(n, np.uint32), "instance": np.empty(n, np.uint32),
+               "distance": np.empty(n, np.float32), "point": np.empty((n, 3), np.float32),
+               "barycoord": np.empty((n, 3), np.float32)}
+        _check(self.ctx._lib.mbvh_scene_closest_point(
+            self.ctx.handle, self.handle, _ptr(q), n, _ptr(out["face"]), _ptr(out["instance"]),
+            _ptr(out["distance"]), _ptr(out["point"]), _ptr(out["barycoord"])))
+        return out
+
+    def closest_point_d(self, points_ptr: int, n: int, face_ptr: int, instance_ptr: int = 0, dist_ptr: int = 0,
+                        point_ptr: int = 0, bary_ptr: int = 0, stream: int = 0) -> None:
+        v = lambda x: C.c_void_p(x or None)
+        _check(self.ctx._lib.mbvh_scene_closest_point_d(
+            self.ctx.handle, self.handle, v(points_ptr), n, v(face_ptr), v(instance_ptr), v(dist_ptr), v(point_ptr),
+            v(bary_ptr), v(stream)))
 
     def raycast_first_d(self, origins_ptr: int, dirs_ptr: int, n: int, face_ptr: int, instance_ptr: int = 0,
                         dist_ptr: int = 0, point_ptr: int = 0, bary_ptr: int = 0, side: int = FRONT_SIDE,
