@@ -626,6 +626,52 @@ def bench_batched(torch, M, W, ctx, stream, args):
     return {"batched_raycast_first": r, "scene_raycast_first": scene}
 
 
+def bench_scene_points(torch, M, ctx, stream, args, sc, geos, idxs, trees, w, n=1 << 20):
+    """Scene closest point (world space, one launch) for n world points near random instances of the C5
+    scene, against the FP64 oracle's closest point run once per instance on a sample (the instances are
+    rigid, so each instance's local answer is its world answer); the sample's distances are compared."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(29)
+    tr = w.matrix_world[:, 12:15].astype(np.float64)
+    pts = (tr[rng.integers(0, tr.shape[0], n)] + rng.normal(0, 1.5, (n, 3))).astype(np.float32)
+    q = torch.from_numpy(pts).to(dev)
+    face = torch.empty(n, dtype=torch.int32, device=dev)
+    inst = torch.empty(n, dtype=torch.int32, device=dev)
+    dist = torch.empty(n, device=dev)
+    torch.cuda.synchronize()
+
+    def run():
+        sc.closest_point_d(q.data_ptr(), n, face.data_ptr(), inst.data_ptr(), dist.data_ptr(), stream=stream.cuda_stream)
+
+    ms_gpu = event_ms(torch, stream, run, 10)
+    res = {"workload": f"closest point over the {w.matrix_world.shape[0]} instances of the same scene, {n} world "
+                       "points within a few units of random instances", "scene_ms": round(ms_gpu, 3),
+           "mqueries_per_s": round(n / ms_gpu / 1e3, 2)}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        oms = [O.OracleMesh(geos[k].positions, idxs[k], trees[k].roots) for k in range(len(geos))]
+        m = 64
+        threads = min(16, os.cpu_count() or 1)
+        best = np.full(m, np.inf)
+        t0 = time.perf_counter()
+        for i in range(w.matrix_world.shape[0]):
+            mi = w.matrix_world_inverse[i].astype(np.float64).reshape(4, 4).T
+            mw = w.matrix_world[i].astype(np.float64).reshape(4, 4).T
+            lp = (pts[:m] @ mi[:3, :3].T + mi[:3, 3]).astype(np.float32)
+            r = oms[w.instance_geometry[i]].closest_point(lp, threads=threads)
+            wp = r["point"] @ mw[:3, :3].T + mw[:3, 3]
+            best = np.minimum(best, np.linalg.norm(wp - pts[:m].astype(np.float64), axis=1))
+        cpu_s = time.perf_counter() - t0
+        gd = dist[:m].cpu().numpy().astype(np.float64)
+        res["cpu_baseline"] = {"value": round(m / cpu_s / 1e6, 5), "unit": "Mqueries/s", "cores": threads,
+                               "kind": "port", "sample": f"{m} points x {w.matrix_world.shape[0]} instances: the FP64 "
+                               "C oracle's closestPointToPoint once per instance, the least world distance"}
+        res["sample_max_rel_diff_vs_oracle"] = float(np.max(np.abs(gd - best) / np.maximum(best, 1.0)))
+    return res
+
+
 def bench_scene(torch, M, W, ctx, stream, args, meshes, batch, geos, idxs, trees, w):
     """Scene raycast (closest hit over all instances, one launch over a top-level tree) on the C5 meshes
     and placements, against the batched launch on every (ray, instance) pair reduced to the closest
@@ -717,6 +763,7 @@ def bench_scene(torch, M, W, ctx, stream, args, meshes, batch, geos, idxs, trees
                                "sample": f"{ms} rays x {n_inst} instances: the FP64 C oracle's restatement of the "
                                          "reference raycastFirst traversal, one call per instance over the sample "
                                          "(the per-instance loop of acceleratedBatchedMeshRaycast)"}
+    res["closest_point"] = bench_scene_points(torch, M, ctx, stream, args, sc, geos, idxs, trees, w)
     sc.free()
     return res
 

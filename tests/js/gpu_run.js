@@ -210,6 +210,34 @@ assert.ok(batchedChecked > 64 && presenceDiffers <= 2, `batched BatchedMesh hits
     sceneHits++;
   }
   assert.ok(sceneHits > 32, `scene hits ${sceneHits}`);
+  // closest point over the instances: never farther than any visible instance's own closest point
+  // (its local answer taken to world space), and the returned point is at the returned distance
+  const apply = (e, o, x, y, z) => [e[o] * x + e[o + 4] * y + e[o + 8] * z + e[o + 12],
+    e[o + 1] * x + e[o + 5] * y + e[o + 9] * z + e[o + 13], e[o + 2] * x + e[o + 6] * y + e[o + 10] * z + e[o + 14]];
+  const nP = 24, pts = new Float32Array(3 * nP);
+  for (let i = 0; i < nP; i++) {
+    const k = (7 * i) % scene.instances.length;
+    for (let c = 0; c < 3; c++) pts[3 * i + c] = scene._world[16 * k + 12 + c] + 0.7 * Math.sin(3 * i + c);
+  }
+  const cpb = scene.closestPointToPointBatch(pts);
+  for (let i = 0; i < nP; i++) {
+    const px = pts[3 * i], py = pts[3 * i + 1], pz = pts[3 * i + 2];
+    let best = Infinity;
+    scene.instances.forEach((inst, k) => {
+      if (inst.visible === false) return;
+      const lp = apply(scene._inverse, 16 * k, px, py, pz);
+      const r = inst.bvh.closestPointToPoint({ x: lp[0], y: lp[1], z: lp[2] });
+      if (!r) return;
+      const wp = apply(scene._world, 16 * k, r.point.x, r.point.y, r.point.z);
+      best = Math.min(best, Math.hypot(wp[0] - px, wp[1] - py, wp[2] - pz));
+    });
+    const d = cpb.distance[i];
+    assert.ok(d <= best * (1 + 1e-4) + 1e-5, `scene closest point ${i}: ${d} vs ${best}`);
+    const q = [cpb.point[3 * i], cpb.point[3 * i + 1], cpb.point[3 * i + 2]];
+    assert.ok(Math.abs(Math.hypot(q[0] - px, q[1] - py, q[2] - pz) - d) <= 1e-4 * Math.max(d, 1));
+    const one = scene.closestPointToPoint({ x: px, y: py, z: pz });
+    assert.ok(one && one.distance === d && one.batchId === cpb.instance[i] && one.faceIndex === cpb.faceIndex[i]);
+  }
   scene.dispose();
 }
 api.disposeBatchedBoundsTree.call(Object.assign(batched, { boundsTrees: [bvh] }), -1);

@@ -568,6 +568,33 @@ static napi_value js_scene_raycast_first(napi_env env, napi_callback_info info) 
   return NULL;
 }
 
+/* sceneClosestPoint(scene, points F32 (3n), face U32 (n), instance U32 (n), dist F32 (n), point F32 (3n),
+   bary F32 (3n)) */
+static napi_value js_scene_closest_point(napi_env env, napi_callback_info info) {
+  napi_value argv[7];
+  get_args(env, info, 7, argv);
+  void* p = NULL;
+  if (napi_get_value_external(env, argv[0], &p) != napi_ok || !p || !((BatchHandle*)p)->scene)
+    return throw_msg(env, "sceneClosestPoint: invalid scene");
+  BatchHandle* b = (BatchHandle*)p;
+  View q;
+  if (!get_view(env, argv[1], &q) || q.type != napi_float32_array || q.length % 3)
+    return throw_msg(env, "sceneClosestPoint: points must be a Float32Array of 3n");
+  const size_t n = q.length / 3;
+  void *face, *inst, *dist, *pt, *bc;
+  if (!out_view(env, argv[2], napi_uint32_array, n, &face) || !out_view(env, argv[3], napi_uint32_array, n, &inst) ||
+      !out_view(env, argv[4], napi_float32_array, n, &dist) || !out_view(env, argv[5], napi_float32_array, 3 * n, &pt) ||
+      !out_view(env, argv[6], napi_float32_array, 3 * n, &bc))
+    return throw_msg(env, "sceneClosestPoint: output arrays have the wrong type or length");
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, b->device, &ctx);
+  if (!rc)
+    rc = mbvh_scene_closest_point(ctx, b->scene, (const float*)q.data, n, (uint32_t*)face, (uint32_t*)inst,
+                                  (float*)dist, (float*)pt, (float*)bc);
+  if (rc) return throw_status(env, rc, "mbvh_scene_closest_point");
+  return NULL;
+}
+
 /* intersectsBox(mesh, boxMin F64 (3n), boxMax F64 (3n), boxToMesh F64 (16n), out U8 (n)) */
 static napi_value js_intersects_box(napi_env env, napi_callback_info info) {
   napi_value argv[5];
@@ -1029,6 +1056,7 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("sceneCreate", js_scene_create);
   EXPORT("textureSizes", js_texture_sizes);
   EXPORT("intersectsBox", js_intersects_box);
+  EXPORT("sceneClosestPoint", js_scene_closest_point);
   EXPORT("closestPointToGeometry", js_closest_to_geometry);
   EXPORT("exportTextures", js_export_textures);
   EXPORT("exportNodes", js_export_nodes);

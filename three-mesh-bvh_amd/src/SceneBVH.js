@@ -107,6 +107,35 @@ class SceneBVH {
     return hit;
   }
 
+  // Closest point over the instances in world space (the reference's bvh_ClosestPointToPoint over a
+  // ClusteredBVH, src/webgpu/shapecastFns/getClosestPointToPointFn.js:20-151): world points in (a
+  // Float32Array of 3n); faceIndex (in the instance's geometry), instance, world distance and point,
+  // barycoord out. Equal distances go to the lowest instance, then the lowest face.
+  closestPointToPointBatch(points) {
+    const q = points instanceof Float32Array ? points : Float32Array.from(points);
+    const n = q.length / 3;
+    const out = {
+      faceIndex: new Uint32Array(n), instance: new Uint32Array(n), distance: new Float32Array(n),
+      point: new Float32Array(3 * n), barycoord: new Float32Array(3 * n),
+    };
+    native.load().sceneClosestPoint(this._scene, q, out.faceIndex, out.instance, out.distance, out.point,
+      out.barycoord);
+    return out;
+  }
+
+  // One point (three's Vector3): { point, distance, faceIndex, instance } (and batchId for a
+  // BatchedMesh scene), or null for a scene without visible instances.
+  closestPointToPoint(point, target = {}) {
+    const r = this.closestPointToPointBatch(new Float32Array([point.x, point.y, point.z]));
+    if (r.faceIndex[0] === C.NO_HIT) return null;
+    target.point = M.makeVec3(target.point || point, r.point[0], r.point[1], r.point[2]);
+    target.distance = r.distance[0];
+    target.faceIndex = r.faceIndex[0];
+    target.instance = r.instance[0];
+    if (this.object) target.batchId = r.instance[0];
+    return target;
+  }
+
   dispose() {
     if (this._scene) native.load().freeScene(this._scene);
     this._scene = null;
