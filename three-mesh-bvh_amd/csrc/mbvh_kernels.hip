@@ -2353,6 +2353,12 @@ __device__ bool scene_point_instance(const MeshView& M, const float* W, uint32_t
         leaf_range(ref, M.bigleaf, off, cnt);
         for (uint32_t s = off; s < off + cnt; s++) {
           const Tri t = load_tri(M.tris, s);
+          // the triangle's own box through the matrix first: a lower bound, so only a triangle that
+          // could still win or tie gets the double-precision step
+          if (world_box_d2_lb(W, px, py, pz, fminf(fminf(t.p0.x, t.p1.x), t.p2.x), fminf(fminf(t.p0.y, t.p1.y), t.p2.y),
+                              fminf(fminf(t.p0.z, t.p1.z), t.p2.z), fmaxf(fmaxf(t.p0.x, t.p1.x), t.p2.x),
+                              fmaxf(fmaxf(t.p0.y, t.p1.y), t.p2.y), fmaxf(fmaxf(t.p0.z, t.p1.z), t.p2.z)) > b.d2)
+            continue;
           const D3 q = closest_on_tri_d(p, world_vertex(Wd, t.p0), world_vertex(Wd, t.p1), world_vertex(Wd, t.p2));
           const double d2 = ddist2(p, q);
           const uint32_t face = t.face();
