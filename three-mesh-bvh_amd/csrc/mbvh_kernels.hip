@@ -2332,6 +2332,47 @@ __device__ __forceinline__ double tlas_box_d2_lb(const float* b, float px, float
 
 __device__ __forceinline__ D3 world_vertex(const double* Wd, float4 v) { return apply_matrix4_d(Wd, v.x, v.y, v.z); }
 
+// The point in an instance's frame, for box bounds measured there: for an affine W, the world
+// distance to anything W puts in a box is at least sigma * (the local distance to the box), sigma the
+// least singular value of W's 3x3 part; s2 = a lower bound of sigma^2 (Gershgorin on W^T W, shrunk for
+// its FP32 roundings; just under 1 for a rotation), or 0 when that bound is not positive (then world
+// boxes are used). d[k] bounds the FP32 error of each local coordinate (the inverse is taken to be the
+// correctly rounded inverse of W, as the scene's creator supplies it).
+struct LocalFrame {
+  float l[3], d[3], s2;
+};
+__device__ __forceinline__ LocalFrame local_frame(const float* W, const float* I, float px, float py, float pz) {
+  LocalFrame f;
+  for (int k = 0; k < 3; k++) {
+    const float a = I[k], b = I[4 + k], c = I[8 + k], t = I[12 + k];
+    f.l[k] = a * px + b * py + c * pz + t;
+    f.d[k] = (fabsf(a * px) + fabsf(b * py) + fabsf(c * pz) + fabsf(t)) * (1.0f / 65536.0f);
+  }
+  float g[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      g[i][j] = W[4 * i] * W[4 * j] + W[4 * i + 1] * W[4 * j + 1] + W[4 * i + 2] * W[4 * j + 2];
+  // each entry's FP32 error is a few ulps of |col i| |col j|: 2^-20 of that per off-diagonal entry
+  float s2 = kInf;
+  for (int i = 0; i < 3; i++) {
+    const int j = (i + 1) % 3, k = (i + 2) % 3;
+    const float off = fabsf(g[i][j]) + fabsf(g[i][k]) +
+                      (sqrtf(g[i][i] * g[j][j]) + sqrtf(g[i][i] * g[k][k])) * (1.0f / 1048576.0f);
+    s2 = fminf(s2, g[i][i] * (1.0f - 1.0f / 65536.0f) - off * (1.0f + 1.0f / 65536.0f));
+  }
+  f.s2 = s2 > 0.0f ? s2 : 0.0f;
+  return f;
+}
+
+// A lower bound of the squared world distance to anything W puts in the local box (frame f with s2 > 0).
+__device__ __forceinline__ double local_box_d2_lb(const LocalFrame& f, float mnx, float mny, float mnz, float mxx,
+                                                  float mxy, float mxz) {
+  const float gx = fmaxf(fmaxf(mnx - f.l[0], f.l[0] - mxx) - f.d[0], 0.0f);
+  const float gy = fmaxf(fmaxf(mny - f.l[1], f.l[1] - mxy) - f.d[1], 0.0f);
+  const float gz = fmaxf(fmaxf(mnz - f.l[2], f.l[2] - mxz) - f.d[2], 0.0f);
+  return static_cast<double>((gx * gx + gy * gy + gz * gz) * f.s2) * (1.0 - 1.0 / 16384.0);
+}
+
 struct ScenePointBest {
   double d2;
   uint32_t inst, face, slot;
@@ -2340,10 +2381,15 @@ struct ScenePointBest {
 
 // One instance's tree walked near child first in world space; updates `b` (least squared distance,
 // then lowest instance, then lowest face). Returns false on a stack overflow.
-__device__ bool scene_point_instance(const MeshView& M, const float* W, uint32_t inst, D3 p, float px, float py,
-                                     float pz, StackL<8>& st, ScenePointBest& b) {
+__device__ bool scene_point_instance(const MeshView& M, const float* W, const float* I, uint32_t inst, D3 p, float px,
+                                     float py, float pz, StackL<8>& st, ScenePointBest& b) {
   double Wd[16];
   for (int k = 0; k < 16; k++) Wd[k] = W[k];
+  const LocalFrame lf = local_frame(W, I, px, py, pz);
+  const bool local = lf.s2 > 0.0f;
+  auto box_lb = [&](float a, float bb, float c, float d, float e, float g) {
+    return local ? local_box_d2_lb(lf, a, bb, c, d, e, g) : world_box_d2_lb(W, px, py, pz, a, bb, c, d, e, g);
+  };
   for (uint32_t r = 0; r < M.n_roots; r++) {
     st.sp = 0;
     uint32_t ref = M.roots[r];
@@ -2355,9 +2401,9 @@ __device__ bool scene_point_instance(const MeshView& M, const float* W, uint32_t
           const Tri t = load_tri(M.tris, s);
           // the triangle's own box through the matrix first: a lower bound, so only a triangle that
           // could still win or tie gets the double-precision step
-          if (world_box_d2_lb(W, px, py, pz, fminf(fminf(t.p0.x, t.p1.x), t.p2.x), fminf(fminf(t.p0.y, t.p1.y), t.p2.y),
-                              fminf(fminf(t.p0.z, t.p1.z), t.p2.z), fmaxf(fmaxf(t.p0.x, t.p1.x), t.p2.x),
-                              fmaxf(fmaxf(t.p0.y, t.p1.y), t.p2.y), fmaxf(fmaxf(t.p0.z, t.p1.z), t.p2.z)) > b.d2)
+          if (box_lb(fminf(fminf(t.p0.x, t.p1.x), t.p2.x), fminf(fminf(t.p0.y, t.p1.y), t.p2.y),
+                     fminf(fminf(t.p0.z, t.p1.z), t.p2.z), fmaxf(fmaxf(t.p0.x, t.p1.x), t.p2.x),
+                     fmaxf(fmaxf(t.p0.y, t.p1.y), t.p2.y), fmaxf(fmaxf(t.p0.z, t.p1.z), t.p2.z)) > b.d2)
             continue;
           const D3 q = closest_on_tri_d(p, world_vertex(Wd, t.p0), world_vertex(Wd, t.p1), world_vertex(Wd, t.p2));
           const double d2 = ddist2(p, q);
@@ -2372,8 +2418,8 @@ __device__ bool scene_point_instance(const MeshView& M, const float* W, uint32_t
         }
       } else {
         const WideNode nd = M.nodes[ref];
-        const double dl = world_box_d2_lb(W, px, py, pz, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y);
-        const double dr = world_box_d2_lb(W, px, py, pz, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w);
+        const double dl = box_lb(nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y);
+        const double dr = box_lb(nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w);
         const bool hl = dl <= b.d2, hr = dr <= b.d2;
         if (hl && hr) {
           const bool left_first = dl <= dr;
@@ -2440,8 +2486,8 @@ __global__ __launch_bounds__(kBlock) void k_scene_closest_point(SceneArgs A) {
         const uint32_t off = nw[6], cnt = w7 & 0xFFFFu;
         for (uint32_t s = off; s < off + cnt && !overflow; s++) {
           const uint32_t inst = A.tlas_inst[s];
-          overflow = !scene_point_instance(A.meshes[A.instance_geometry[inst]], A.world + 16ull * inst, inst, p, px, py,
-                                           pz, st, b);
+          overflow = !scene_point_instance(A.meshes[A.instance_geometry[inst]], A.world + 16ull * inst,
+                                           A.inverse + 16ull * inst, inst, p, px, py, pz, st, b);
         }
       } else {
         if (sp + 2 > kSceneStack) {
