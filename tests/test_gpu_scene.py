@@ -246,3 +246,39 @@ def test_scene_closest_point_device_form(ctx, meshes):
     again = sc.closest_point(pts)
     assert np.array_equal(again["face"], host["face"]) and np.array_equal(again["instance"], host["instance"])
     sc.free()
+
+
+def test_scene_closest_point_sheared_and_anisotropic(ctx, meshes):
+    """Instance matrices at the edges of the frame bound: strongly anisotropic scales (the local-frame
+    bound with its rounding margins) and near-parallel columns (no positive singular-value bound, so
+    the world-box path), against the oracle's brute force."""
+    built, ms = meshes
+    geos = [b[0] for b in built]
+    w = W.scene_rays(geos, 8, 8, seed=31)
+    rng = np.random.default_rng(3)
+    mats = [np.diag([30.0, 1.0, 0.03, 1.0]), np.diag([0.05, 2.0, 40.0, 1.0])]
+    sh = np.eye(4)
+    sh[:3, 2] = [0.98, 0.05, 0.2]  # third column nearly the first: Gershgorin gives no positive bound
+    mats.append(sh)
+    sh2 = np.eye(4)
+    sh2[:3, :3] = [[1.0, 0.9, 0.0], [0.0, 0.5, 0.0], [0.0, 0.0, 3.0]]
+    mats.append(sh2)
+    for k, m in enumerate(mats):
+        m = m.copy()
+        m[:3, 3] = rng.uniform(-8, 8, 3)
+        w.matrix_world[k] = m.T.reshape(-1).astype(np.float32)
+        w.matrix_world_inverse[k] = np.linalg.inv(w.matrix_world[k].astype(np.float64).reshape(4, 4).T).T.reshape(-1)
+    pts = _scene_points(w, 300, 13)
+    print("case scene closest point sheared / anisotropic", flush=True)
+    sc = ctx.scene(ms, w.matrix_world, w.matrix_world_inverse, w.instance_geometry)
+    got = sc.closest_point(pts)
+    oms = [O.OracleMesh(g.positions, idx, t.roots) for g, idx, t in built]
+    ref = O.scene_closest_point(oms, w.matrix_world, w.instance_geometry, None, pts)
+    scale = float(np.abs(w.matrix_world[:, 12:15]).max()) + 40.0
+    np.testing.assert_allclose(got["distance"], ref["distance"], rtol=1e-5, atol=1e-5 * scale)
+    same = (got["instance"] == ref["instance"]) & (got["face"] == ref["face"])
+    for i in np.nonzero(~same)[0]:
+        cp = O.closest_point_triangle(pts[i].astype(np.float64), *_world_tri(built, w, got["instance"][i], got["face"][i]))
+        assert np.linalg.norm(cp - pts[i]) <= ref["distance"][i] * (1 + 1e-6) + 1e-9
+    assert np.isin(got["instance"], np.arange(4)).sum() > 20  # the edge-case instances do win points
+    sc.free()
