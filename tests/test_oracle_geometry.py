@@ -154,3 +154,38 @@ def test_tree_walk_equals_brute_force():
     r = om.closest_to_geometry(other.positions, other.index, far.T.reshape(-1))
     assert r is not None and r["distance"] > 2
     assert om.closest_to_geometry(other.positions, other.index, far.T.reshape(-1), max_threshold=r["distance"]) is None
+
+
+def test_scene_brute_force_against_single_mesh_walk():
+    """The oracle's scene closest point (brute force over world triangles) against its closest-point
+    tree walk: one identity instance gives the same faces and distances; a rotated, uniformly scaled
+    and translated copy scales the distances by the scale and maps the points; a hidden instance never
+    answers."""
+    g, idx, t = build_mesh(G.torus_knot(1.0, 0.3, 40, 8))
+    om = O.OracleMesh(g.positions, idx, t.roots)
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(-2, 2, (60, 3)).astype(np.float32)
+    walk = om.closest_point(pts, threads=1)
+    ident = np.eye(4, dtype=np.float32).reshape(1, 16)
+    r = O.scene_closest_point([om], ident, np.zeros(1, np.uint32), None, pts)
+    np.testing.assert_allclose(r["distance"], walk["distance"], rtol=1e-12)
+    # faces: the walk keeps the reference's first visited among exact ties, the brute force the lowest
+    diff = np.nonzero(r["face"] != walk["face"])[0]
+    assert diff.size < 20
+    for i in diff:
+        assert r["face"][i] < walk["face"][i]
+        assert om.point_face_distance(int(walk["face"][i]), pts[i]) == pytest.approx(r["distance"][i], rel=1e-12)
+    c, s_ = np.cos(0.7), np.sin(0.7)
+    m = np.eye(4)
+    m[:3, :3] = np.array([[c, -s_, 0], [s_, c, 0], [0, 0, 1]]) * 2.0
+    m[:3, 3] = [5.0, -3.0, 1.0]
+    world = np.stack([m.T.reshape(-1), np.eye(4).reshape(-1)]).astype(np.float32)
+    wp = (pts.astype(np.float64) @ m[:3, :3].T + m[:3, 3]).astype(np.float32)
+    vis = np.array([1, 0], np.uint8)
+    r2 = O.scene_closest_point([om], world, np.zeros(2, np.uint32), vis, wp)
+    assert np.all(r2["instance"] == 0)
+    lp = (np.linalg.inv(m) @ np.c_[wp.astype(np.float64), np.ones(60)].T).T[:, :3].astype(np.float32)
+    walk2 = om.closest_point(lp, threads=1)
+    # the world and local query points are each rounded to float: 1e-6 of the coordinates (~10) absolute
+    np.testing.assert_allclose(r2["distance"], 2.0 * walk2["distance"], rtol=1e-6, atol=1e-5)
+    np.testing.assert_allclose(r2["point"], walk2["point"] @ m[:3, :3].T + m[:3, 3], atol=1e-5)
