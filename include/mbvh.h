@@ -14,6 +14,11 @@
  *     hipStream_t, or NULL for the context's own stream). They allocate nothing and never
  *     synchronise, so a caller may capture them in a HIP graph. The other query entry points
  *     take HOST pointers and are synchronous.
+ *   - All query kernels of one context share that context's work-queue counter and traversal
+ *     stack overflow region. The "_d" calls of one context must therefore be serialised: issue
+ *     them on one stream (or order the streams with events). Two "_d" calls of one context that
+ *     run at the same time on different streams overwrite each other's spilled stack entries.
+ *     Use one context per stream for concurrent queries.
  *   - Vectors are packed float triples (x, y, z) per element.
  *   - A miss is reported as face index MBVH_NO_HIT and distance +Infinity.
  */
@@ -430,6 +435,11 @@ int mbvh_mesh_export_nodes(mbvh_ctx* ctx, mbvh_mesh* mesh, uint32_t* nodes, uint
  * visible (NULL: all) excludes instances as getVisibleAt does (src/utils/ExtensionUtilities.js:112-116).
  * The world boxes come from the meshes' current root boxes: re-create the scene after a refit or
  * after instances move. The meshes must outlive the scene.
+ * Every matrix_world must be affine (last row 0 0 0 1, as Matrix4.compose makes it): others are
+ * rejected with MBVH_ERR_BAD_ARGS. matrix_world_inverse is used as given to bring rays into each
+ * instance's frame (as raycastObject3D inverts matrixWorld itself); the closest point's pruning
+ * bounds never rely on it: the scene computes its own inverse in double and uses world boxes for an
+ * instance whose inverse fails a residual check.
  */
 int mbvh_scene_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, uint32_t n_meshes, const float* matrix_world,
                       const float* matrix_world_inverse, const uint32_t* instance_geometry, const uint8_t* visible,
@@ -477,7 +487,11 @@ int mbvh_batch_autotune(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_wo
                         uint32_t* face_index, float* distance, float* point, float* barycoord, void* workspace,
                         uint64_t workspace_bytes, mbvh_batch_tuning* out_tuning);
 /* Captures the whole batched launch (ordering + traversal) into a HIP graph for these buffers;
- * mbvh_batch_graph_launch replays it on `stream` (NULL: the context's stream). */
+ * mbvh_batch_graph_launch replays it on `stream` (NULL: the context's stream). The graph holds the
+ * addresses of these buffers, of the batch and of the context's traversal stack region: free the
+ * graph before the batch, its meshes or the buffers. When a later upload, batch or scene makes the
+ * context grow its stack region (a deeper tree), the graph is stale and mbvh_batch_graph_launch
+ * returns MBVH_ERR_BAD_HANDLE without launching; capture it again. */
 int mbvh_batch_graph_create(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_world,
                             const float* matrix_world_inverse, const uint32_t* instance_geometry,
                             uint32_t n_instances, const float* origins, const float* directions,

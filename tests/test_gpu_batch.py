@@ -193,6 +193,57 @@ def test_graph_replays_between_other_work(ctx, case):
     graph.free()
 
 
+def test_graph_refused_after_stack_scratch_moves():
+    """A captured graph holds the address of the context's traversal stack overflow region. Uploading
+    a mesh deeper than that region covers makes the context reallocate it; a later replay of the old
+    graph is refused with an error (and never launched)."""
+    c = M.Context(0)
+    try:
+        w = W.c5_batched(n_geometries=4, n_rays=2000, tri_max=3000, seed=4)
+        built = [build_mesh(g) for g in w.geometries]
+        meshes = [c.upload(t, g.positions, idx) for g, idx, t in built]
+        batch = c.batch(meshes)
+        n = w.origins.shape[0]
+        D = {"world": _t(w.matrix_world), "inverse": _t(w.matrix_world_inverse),
+             "instance_geometry": _t(w.instance_geometry.view(np.int32)), "origins": _t(w.origins),
+             "directions": _t(w.directions), "ray_instance": _t(w.ray_instance.view(np.int32))}
+        wsb = batch.workspace_bytes(n)
+        D["workspace"] = torch.empty(wsb, dtype=torch.uint8, device=DEV)
+        out = _outputs(n)
+        torch.cuda.synchronize()
+        graph = batch.graph(_params(w, D, out, wsb))
+        graph.launch()  # the scratch has not moved: replays
+        c.synchronize()
+        # 100 triangles at x = 2^i: CENTER splits peel one triangle per level, so the tree is as deep
+        # as maxDepth allows; a depth of 64 needs 57 overflow entries per thread (56 are reserved)
+        tris = []
+        for i in range(100):
+            x = float(2.0 ** i)
+            tris += [[x, 0, 0], [x, 1, 0], [x, 0, 1]]
+        pos = np.array(tris, np.float32)
+        idx = np.arange(300, dtype=np.uint32)
+        tree = M.build(pos, idx, strategy=M.CENTER, max_depth=70, target_leaf_size=1)
+        deep = c.upload(tree, pos, idx)
+        assert deep.info()["max_depth"] >= 64
+        with pytest.raises(M.MbvhError) as e:
+            graph.launch()
+        assert e.value.code == -3 and "reallocated" in str(e.value)
+        # a graph captured now replays, and matches the eager launch
+        graph.free()
+        graph = batch.graph(_params(w, D, out, wsb))
+        graph.launch()
+        c.synchronize()
+        got = _host(out)
+        _same(got, _batched(w, batch, D, wsb))
+        graph.free()
+        deep.free()
+        batch.free()
+        for m in meshes:
+            m.free()
+    finally:
+        c.close()
+
+
 def test_invalid_instances_miss(ctx, case):
     w, built, meshes, batch, D, wsb = case
     n = 2000

@@ -222,6 +222,38 @@ def test_scene_closest_point_equals_oracle(ctx, meshes, non_uniform):
     sc.free()
 
 
+def test_scene_closest_point_ignores_a_stale_inverse_and_rejects_projective(ctx, meshes):
+    """The closest point's pruning bounds use the scene's own double-precision inverse of each
+    matrixWorld: a caller inverse that is stale by 5% (or garbage) changes nothing, and the results
+    equal the oracle's brute force. A matrixWorld whose last row is not 0 0 0 1 is rejected."""
+    built, ms = meshes
+    geos = [b[0] for b in built]
+    w = W.scene_rays(geos, 12, 8, seed=41, non_uniform=0.3)
+    pts = _scene_points(w, 300, 7)
+    print("case scene closest point stale inverse", flush=True)
+    good = ctx.scene(ms, w.matrix_world, w.matrix_world_inverse, w.instance_geometry)
+    ref_gpu = good.closest_point(pts)
+    good.free()
+    stale = w.matrix_world_inverse.copy()
+    stale[:, [0, 1, 2, 4, 5, 6, 8, 9, 10]] *= 1.05
+    stale[1] = np.float32(7.0)
+    sc = ctx.scene(ms, w.matrix_world, stale, w.instance_geometry)
+    got = sc.closest_point(pts)
+    sc.free()
+    for k in ("face", "instance"):
+        assert np.array_equal(got[k], ref_gpu[k]), k
+    assert np.array_equal(got["distance"].view(np.uint32), ref_gpu["distance"].view(np.uint32))
+    oms = [O.OracleMesh(g.positions, idx, t.roots) for g, idx, t in built]
+    ref = O.scene_closest_point(oms, w.matrix_world, w.instance_geometry, None, pts)
+    scale = float(np.abs(w.matrix_world[:, 12:15]).max()) + 1.0
+    np.testing.assert_allclose(got["distance"], ref["distance"], rtol=1e-5, atol=1e-5 * scale)
+    proj = w.matrix_world.copy()
+    proj[5, 15] = 2.0
+    with pytest.raises(M.MbvhError) as e:
+        ctx.scene(ms, proj, w.matrix_world_inverse, w.instance_geometry)
+    assert e.value.code == -4 and "affine" in str(e.value)
+
+
 def test_scene_closest_point_device_form(ctx, meshes):
     torch = pytest.importorskip("torch")
     built, ms = meshes

@@ -35,6 +35,7 @@ struct mbvh_ctx {
   int32_t ray_lanes = 0;        // raycastFirst lanes per ray: 0 automatic, 1, or 4
   int32_t point_lanes = 0;      // closestPoint lanes per query: 0 automatic, 1, or 4
   uint32_t stack_ovf_entries = 0;
+  uint64_t stack_gen = 0;       // bumped whenever stack_ovf moves (captured graphs hold its address)
   DeviceBuffer staging[8];      // host-API input / output staging
   DeviceBuffer order;           // host-API query order
   DeviceBuffer order_ws;        // host-API ordering workspace
@@ -63,6 +64,7 @@ struct mbvh_batch_graph {
   mbvh_ctx* ctx = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  uint64_t stack_gen = 0;  // the context's stack_gen at capture
 };
 
 struct mbvh_mesh {
@@ -143,8 +145,12 @@ int reserve_stack(mbvh_ctx* ctx, uint32_t max_stack) {
     const hipError_t e = hipDeviceSynchronize();  // earlier launches may still use the old region
     if (e != hipSuccess) return set_error(MBVH_ERR_HIP, "stack scratch: %s", hipGetErrorString(e));
   }
+  ctx->stack_gen++;  // a graph captured before this point holds the old address
   const int rc = ensure_buffer(ctx->stack_ovf, static_cast<size_t>(entries) * ctx->max_threads * sizeof(uint2));
-  if (rc) return rc;
+  if (rc) {
+    ctx->stack_ovf_entries = 0;
+    return rc;
+  }
   ctx->stack_ovf_entries = entries;
   return MBVH_OK;
 }
@@ -1785,6 +1791,7 @@ extern "C" int mbvh_batch_graph_create(mbvh_ctx* ctx, mbvh_batch* batch, const f
   g->ctx = ctx;
   g->graph = graph;
   g->exec = exec;
+  g->stack_gen = ctx->stack_gen;
   *out_graph = g;
   return MBVH_OK;
 }
@@ -1793,6 +1800,10 @@ extern "C" int mbvh_batch_graph_launch(mbvh_batch_graph* graph, void* stream) {
   clear_error();
   if (!graph || graph->magic != kGraphMagic || !valid_ctx(graph->ctx))
     return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_batch_graph_launch: invalid graph handle");
+  if (graph->stack_gen != graph->ctx->stack_gen)
+    return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_batch_graph_launch: the context's traversal stack scratch was "
+                     "reallocated after capture (a deeper mesh, batch or scene was created); free the graph and "
+                     "capture it again");
   int rc = set_device(graph->ctx);
   if (rc) return rc;
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : graph->ctx->stream;
@@ -2151,6 +2162,8 @@ struct mbvh_scene {
   MeshView* d_meshes = nullptr;
   float* d_world = nullptr;
   float* d_inverse = nullptr;
+  float* d_frame = nullptr;        // the library's own inverse of each matrixWorld (closest point bounds)
+  uint8_t* d_frame_ok = nullptr;   // that inverse passed its residual check
   uint32_t* d_geom = nullptr;
   uint32_t* d_tlas = nullptr;
   uint32_t* d_tlas_inst = nullptr;
@@ -2161,8 +2174,47 @@ namespace {
 
 bool valid_scene(const mbvh_scene* s) { return s && s->magic == 0x4E454353u; }
 
+// The inverse of an affine column-major 4x4 (local -> world) in double, rounded to float, and whether
+// W times that rounded inverse is the identity to 2^-20 in every entry of the 3x3 part and of the
+// translation relative to the magnitudes involved (false for singular or badly conditioned W).
+bool frame_inverse(const float* e, float* out) {
+  const double a = e[0], b = e[4], c = e[8], d = e[1], f = e[5], g = e[9], h = e[2], i = e[6], j = e[10];
+  const double c0 = f * j - g * i, c1 = g * h - d * j, c2 = d * i - f * h;
+  const double det = a * c0 + b * c1 + c * c2;
+  const double m[9] = {c0 / det, (c * i - b * j) / det, (b * g - c * f) / det,
+                       c1 / det, (a * j - c * h) / det, (c * d - a * g) / det,
+                       c2 / det, (b * h - a * i) / det, (a * f - b * d) / det};  // row-major inverse of the 3x3
+  const double t[3] = {e[12], e[13], e[14]};
+  float r[16] = {0};
+  for (int row = 0; row < 3; row++) {
+    for (int col = 0; col < 3; col++) r[4 * col + row] = static_cast<float>(m[3 * row + col]);
+    r[12 + row] = static_cast<float>(-(m[3 * row] * t[0] + m[3 * row + 1] * t[1] + m[3 * row + 2] * t[2]));
+  }
+  r[15] = 1.0f;
+  std::memcpy(out, r, sizeof(r));
+  if (!(std::isfinite(det) && det != 0.0)) return false;
+  // residual W * R - I over the 3x3 part, and the translation W * (R t) + t_W - 0
+  for (int row = 0; row < 3; row++) {
+    double scale = 0.0, tr = e[12 + row], trs = std::fabs(e[12 + row]);
+    for (int col = 0; col < 3; col++) {
+      double s = 0.0, sa = 0.0;
+      for (int k = 0; k < 3; k++) {
+        s += static_cast<double>(e[4 * k + row]) * r[4 * col + k];
+        sa += std::fabs(static_cast<double>(e[4 * k + row]) * r[4 * col + k]);
+      }
+      if (!(std::fabs(s - (row == col ? 1.0 : 0.0)) <= 9.5367431640625e-07 * std::max(sa, 1e-300))) return false;
+      scale = std::max(scale, sa);
+      tr += static_cast<double>(e[4 * col + row]) * r[12 + col];
+      trs += std::fabs(static_cast<double>(e[4 * col + row]) * r[12 + col]);
+    }
+    if (!(std::fabs(tr) <= 9.5367431640625e-07 * std::max(trs, 1e-300)) || !(scale > 0.0)) return false;
+  }
+  return true;
+}
+
 void free_scene_buffers(mbvh_scene* sc) {
-  void* bufs[] = {sc->d_meshes, sc->d_world, sc->d_inverse, sc->d_geom, sc->d_tlas, sc->d_tlas_inst};
+  void* bufs[] = {sc->d_meshes, sc->d_world, sc->d_inverse, sc->d_frame, sc->d_frame_ok, sc->d_geom, sc->d_tlas,
+                  sc->d_tlas_inst};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
 }
@@ -2212,6 +2264,16 @@ extern "C" int mbvh_scene_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, 
   for (uint32_t i = 0; i < n_instances; i++)
     if (instance_geometry[i] >= n_meshes)
       return set_error(MBVH_ERR_BAD_ARGS, "%s: instance %u names mesh %u of %u", fn, i, instance_geometry[i], n_meshes);
+  // matrixWorld of an Object3D is affine (Matrix4.compose); the world boxes and bounds rely on it
+  std::vector<float> frame(std::max<size_t>(16ull * n_instances, 16), 0.0f);
+  std::vector<uint8_t> frame_ok(std::max<uint32_t>(n_instances, 1), 0);
+  for (uint32_t i = 0; i < n_instances; i++) {
+    const float* e = matrix_world + 16ull * i;
+    if (!(e[3] == 0.0f && e[7] == 0.0f && e[11] == 0.0f && e[15] == 1.0f))
+      return set_error(MBVH_ERR_BAD_ARGS, "%s: matrixWorld of instance %u is not affine (its last row must be 0 0 0 1)",
+                       fn, i);
+    frame_ok[i] = frame_inverse(e, &frame[16ull * i]) ? 1 : 0;
+  }
   std::lock_guard<std::mutex> guard(ctx->lock);
   int rc = set_device(ctx);
   if (rc) return rc;
@@ -2288,6 +2350,10 @@ extern "C" int mbvh_scene_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, 
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&sc->d_meshes), views.size() * sizeof(MeshView));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_world), mat_bytes);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_inverse), mat_bytes);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_frame), frame.size() * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_frame_ok), frame_ok.size());
+  if (e == hipSuccess) e = hipMemcpy(sc->d_frame, frame.data(), frame.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(sc->d_frame_ok, frame_ok.data(), frame_ok.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_geom), std::max<size_t>(4ull * n_instances, 4));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_tlas), nodes.size());
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_tlas_inst), slot_inst.size() * 4);
@@ -2344,6 +2410,8 @@ int enqueue_scene(mbvh_ctx* ctx, const mbvh_scene* sc, const float* o, const flo
   a.tlas_inst = sc->d_tlas_inst;
   a.world = sc->d_world;
   a.inverse = sc->d_inverse;
+  a.frame = sc->d_frame;
+  a.frame_ok = sc->d_frame_ok;
   a.instance_geometry = sc->d_geom;
   a.origins = o;
   a.directions = d;
@@ -2423,6 +2491,8 @@ int enqueue_scene_points(mbvh_ctx* ctx, const mbvh_scene* sc, const float* point
   a.tlas_inst = sc->d_tlas_inst;
   a.world = sc->d_world;
   a.inverse = sc->d_inverse;
+  a.frame = sc->d_frame;
+  a.frame_ok = sc->d_frame_ok;
   a.instance_geometry = sc->d_geom;
   a.origins = points;
   a.n = n;

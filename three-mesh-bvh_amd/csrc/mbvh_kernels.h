@@ -226,6 +226,11 @@ struct SceneArgs {
   const uint32_t* tlas_inst;       // instance at each TLAS primitive slot
   const float* world;              // 16 floats per instance (local -> world, column-major)
   const float* inverse;            // 16 floats per instance (world -> local)
+  // closest point's box bounds in the instance frame: 16 floats per instance, the float rounding of
+  // the double-precision inverse of `world` computed at scene creation, and whether that inverse
+  // passed its residual check (0: world boxes only)
+  const float* frame;
+  const uint8_t* frame_ok;
   const uint32_t* instance_geometry;
   const float* origins;            // world rays
   const float* directions;

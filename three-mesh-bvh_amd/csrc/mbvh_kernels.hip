@@ -2336,8 +2336,9 @@ __device__ __forceinline__ D3 world_vertex(const double* Wd, float4 v) { return 
 // distance to anything W puts in a box is at least sigma * (the local distance to the box), sigma the
 // least singular value of W's 3x3 part; s2 = a lower bound of sigma^2 (Gershgorin on W^T W, shrunk for
 // its FP32 roundings; just under 1 for a rotation), or 0 when that bound is not positive (then world
-// boxes are used). d[k] bounds the FP32 error of each local coordinate (the inverse is taken to be the
-// correctly rounded inverse of W, as the scene's creator supplies it).
+// boxes are used). d[k] bounds the FP32 error of each local coordinate; I is the float rounding of
+// the double-precision inverse of the affine W, computed and residual-checked at scene creation (a
+// caller-supplied inverse is never used for bounds).
 struct LocalFrame {
   float l[3], d[3], s2;
 };
@@ -2381,12 +2382,12 @@ struct ScenePointBest {
 
 // One instance's tree walked near child first in world space; updates `b` (least squared distance,
 // then lowest instance, then lowest face). Returns false on a stack overflow.
-__device__ bool scene_point_instance(const MeshView& M, const float* W, const float* I, uint32_t inst, D3 p, float px,
-                                     float py, float pz, StackL<8>& st, ScenePointBest& b) {
+__device__ bool scene_point_instance(const MeshView& M, const float* W, const float* I, bool frame_ok, uint32_t inst,
+                                     D3 p, float px, float py, float pz, StackL<8>& st, ScenePointBest& b) {
   double Wd[16];
   for (int k = 0; k < 16; k++) Wd[k] = W[k];
   const LocalFrame lf = local_frame(W, I, px, py, pz);
-  const bool local = lf.s2 > 0.0f;
+  const bool local = frame_ok && lf.s2 > 0.0f;
   auto box_lb = [&](float a, float bb, float c, float d, float e, float g) {
     return local ? local_box_d2_lb(lf, a, bb, c, d, e, g) : world_box_d2_lb(W, px, py, pz, a, bb, c, d, e, g);
   };
@@ -2487,7 +2488,7 @@ __global__ __launch_bounds__(kBlock) void k_scene_closest_point(SceneArgs A) {
         for (uint32_t s = off; s < off + cnt && !overflow; s++) {
           const uint32_t inst = A.tlas_inst[s];
           overflow = !scene_point_instance(A.meshes[A.instance_geometry[inst]], A.world + 16ull * inst,
-                                           A.inverse + 16ull * inst, inst, p, px, py, pz, st, b);
+                                           A.frame + 16ull * inst, A.frame_ok[inst] != 0, inst, p, px, py, pz, st, b);
         }
       } else {
         if (sp + 2 > kSceneStack) {
