@@ -332,6 +332,28 @@ int mbvh_closest_point_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* poin
                          const uint32_t* order, float min_threshold, float max_threshold, uint32_t* face_index,
                          float* distance, float* closest, float* barycoord, void* stream);
 
+/* Size of the per-device scratch the allocation-free entry point below uses, and its query limit. */
+#define MBVH_NOALLOC_SCRATCH_BYTES (32ull << 20)
+#define MBVH_NOALLOC_MAX_QUERIES (1ull << 21)
+/*
+ * Allocation-free closest point. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and results as mbvh_closest_point_d (device pointers, enqueued on
+ * `stream`, NULL = the context's stream), except that the query order is computed inside (the host
+ * entry point's spatial order, from 4,096 queries on) and that the work-queue counter, the order,
+ * its workspace and the traversal stack overflow live in a fixed MBVH_NOALLOC_SCRATCH_BYTES region
+ * of device memory that the library reserves once per GPU for the life of the process. The call
+ * makes no device allocation and no library call that allocates.
+ *   - n > MBVH_NOALLOC_MAX_QUERIES is rejected with MBVH_ERR_LIMIT (nothing is enqueued).
+ *   - A tree too deep for the stack region even at one workgroup is rejected with MBVH_ERR_LIMIT;
+ *     a deep tree otherwise runs on fewer workers (same results, lower throughput).
+ *   - Every call on one GPU shares that GPU's region: calls that overlap in time on one GPU (two
+ *     streams, two contexts, two threads) race and give undefined results. Serialise them.
+ */
+int mbvh_closest_point_noalloc_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
+                                 float min_threshold, float max_threshold, uint32_t* face_index,
+                                 float* distance, float* closest, float* barycoord, void* stream);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Instances and the batched multi-mesh launch (BatchedMesh)                                  */
 /* ------------------------------------------------------------------------------------------ */

@@ -48,6 +48,39 @@ struct mbvh_ctx {
   std::mutex lock;              // host entry points on one context are serialised
 };
 
+// Device memory of the allocation-free closest-point entry point (mbvh_closest_point_noalloc_d):
+// work-queue counter, query order, ordering workspace and traversal stack overflow. Every GPU the
+// library runs on holds its own copy until the process ends; host code must never pass this name
+// to a kernel or a copy (it is a host-side stand-in), only the per-device address looked up below.
+static __device__ __align__(256) unsigned char scratch[MBVH_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_lock;
+void* g_scratch_ptr[kMaxScratchDevices] = {};  // device address of `scratch`, per device
+
+// Looks up (once per device, with that device current) the device address of `scratch`.
+int scratch_init(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return MBVH_OK;  // the entry point reports it
+  std::lock_guard<std::mutex> guard(g_scratch_lock);
+  if (g_scratch_ptr[device]) return MBVH_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return set_error(MBVH_ERR_HIP, "scratch lookup on device %d failed (%s)", device, hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return MBVH_OK;
+}
+
+void* scratch_of(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> guard(g_scratch_lock);
+  return g_scratch_ptr[device];
+}
+
+}  // namespace
+
 struct mbvh_batch {
   uint32_t magic = 0x48435442u;  // "BTCH"
   mbvh_ctx* ctx = nullptr;
@@ -443,6 +476,7 @@ extern "C" int mbvh_ctx_create(int32_t device, mbvh_ctx** out_ctx) {
   MBVH_HIP(hipSetDevice(device));
   hipDeviceProp_t prop;
   MBVH_HIP(hipGetDeviceProperties(&prop, device));
+  if (const int rc = scratch_init(device)) return rc;
   mbvh_ctx* c = new (std::nothrow) mbvh_ctx();
   if (!c) return set_error(MBVH_ERR_OOM, "mbvh_ctx_create: out of host memory");
   c->device = device;
@@ -882,21 +916,50 @@ int raycast_first_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o, con
   return MBVH_OK;
 }
 
+// A caller-provided region for a query launch's work-queue counter (256 bytes) and traversal stack
+// overflow, used instead of the context's buffers. The launch is narrowed (the kernels are
+// persistent: fewer workers pull the same tiles) until every worker's overflow rows fit.
+struct FixedScratch {
+  uint32_t* queue;
+  uint2* ovf;
+  uint64_t ovf_bytes;
+};
+
 int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uint64_t n,
                        const uint32_t* order, float min_t,
-                       float max_t, uint32_t* face, float* dist, float* point, float* bary, hipStream_t s) {
+                       float max_t, uint32_t* face, float* dist, float* point, float* bary, hipStream_t s,
+                       const FixedScratch* fixed = nullptr) {
   if (n && (!q || !face)) return set_error(MBVH_ERR_BAD_ARGS, "closest_point: NULL point or face buffer");
   // four lanes per query only without a minThreshold (its early exit follows the binary order)
   const bool quads = min_t == 0.0f && (ctx->point_lanes == 4 || ctx->point_lanes == 0);
-  if (quads ? mesh->max_stack4 > kQuadLdsStack + ctx->stack_ovf_entries : mesh->max_stack > kStackLds + ctx->stack_ovf_entries)
-    return set_error(MBVH_ERR_LIMIT, "closest_point: stack scratch smaller than the tree depth");
-  const uint64_t threads = quads ? launch_threads(4 * n, ctx->max_threads) : launch_threads(n, ctx->max_threads);
+  uint64_t threads = quads ? launch_threads(4 * n, ctx->max_threads) : launch_threads(n, ctx->max_threads);
   ClosestPointArgs a;
+  if (fixed) {
+    const uint32_t need = quads ? (mesh->max_stack4 > kQuadLdsStack ? mesh->max_stack4 - kQuadLdsStack : 0)
+                                : (mesh->max_stack > kStackLds ? mesh->max_stack - kStackLds : 0);
+    const uint64_t per_thread = quads ? 4 : 1;  // threads per stack (one stack per quad)
+    if (need) {
+      const uint64_t stacks = fixed->ovf_bytes / (uint64_t{need} * sizeof(uint2));
+      const uint64_t fit = stacks * per_thread / kBlock * kBlock;
+      if (fit == 0)
+        return set_error(MBVH_ERR_LIMIT, "closest_point: the tree (stack of %u entries) is too deep for %llu bytes of "
+                         "stack scratch", quads ? mesh->max_stack4 : mesh->max_stack,
+                         static_cast<unsigned long long>(fixed->ovf_bytes));
+      threads = std::min(threads, fit);
+    }
+    a.stack.ovf = fixed->ovf;
+    a.stack.ovf_stride = threads / per_thread;
+    a.stack.ovf_entries = need;
+    a.queue = fixed->queue;
+  } else {
+    if (quads ? mesh->max_stack4 > kQuadLdsStack + ctx->stack_ovf_entries : mesh->max_stack > kStackLds + ctx->stack_ovf_entries)
+      return set_error(MBVH_ERR_LIMIT, "closest_point: stack scratch smaller than the tree depth");
+    a.stack = stack_of(ctx, quads ? threads / 4 : threads);
+    a.queue = static_cast<uint32_t*>(ctx->queue.ptr);
+  }
   a.mesh = view_of(mesh);
-  a.stack = stack_of(ctx, quads ? threads / 4 : threads);
   a.points = q;
   a.order = order;
-  a.queue = static_cast<uint32_t*>(ctx->queue.ptr);
   a.n = n;
   a.min_threshold = min_t;
   a.max_threshold = max_t;
@@ -1000,6 +1063,61 @@ extern "C" int mbvh_closest_point_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const 
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
   return closest_point_impl(ctx, mesh, points, n, order, min_threshold, max_threshold, face_index, distance,
                             closest, barycoord, s);
+}
+
+namespace {
+
+// Regions of `scratch`, at fixed offsets: queue counter | order (4 B per query) | ordering
+// workspace for the largest n | traversal stack overflow (the rest).
+struct NoAllocLayout {
+  uint64_t order_off, ws_off, ovf_off;
+};
+
+NoAllocLayout noalloc_layout() {
+  auto up = [](uint64_t x) { return (x + 255) / 256 * 256; };
+  NoAllocLayout l;
+  l.order_off = 256;
+  l.ws_off = up(l.order_off + 4 * uint64_t{MBVH_NOALLOC_MAX_QUERIES});
+  l.ovf_off = up(l.ws_off + order_workspace_bytes(MBVH_NOALLOC_MAX_QUERIES));
+  return l;
+}
+
+}  // namespace
+
+extern "C" int mbvh_closest_point_noalloc_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
+                                            float min_threshold, float max_threshold, uint32_t* face_index,
+                                            float* distance, float* closest, float* barycoord, void* stream) {
+  clear_error();
+  int rc = check_query("mbvh_closest_point_noalloc_d", ctx, mesh);
+  if (rc) return rc;
+  if (n > MBVH_NOALLOC_MAX_QUERIES)
+    return set_error(MBVH_ERR_LIMIT, "mbvh_closest_point_noalloc_d: %llu queries exceed the allocation-free maximum "
+                     "of %llu", static_cast<unsigned long long>(n),
+                     static_cast<unsigned long long>(MBVH_NOALLOC_MAX_QUERIES));
+  if (n && (!points || !face_index))
+    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_closest_point_noalloc_d: NULL point or face buffer");
+  char* base = static_cast<char*>(scratch_of(ctx->device));
+  if (!base)
+    return set_error(MBVH_ERR_NO_DEVICE, "mbvh_closest_point_noalloc_d: no scratch on device %d", ctx->device);
+  const NoAllocLayout l = noalloc_layout();
+  if (l.ovf_off >= MBVH_NOALLOC_SCRATCH_BYTES)
+    return set_error(MBVH_ERR_LIMIT, "mbvh_closest_point_noalloc_d: scratch layout exceeds its size");
+  if ((rc = set_device(ctx))) return rc;
+  if (n == 0) return MBVH_OK;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  // the host entry points' spatial order (results are written at each query's own index)
+  const uint32_t* order = nullptr;
+  if (n >= kOrderMin) {
+    uint32_t* o = reinterpret_cast<uint32_t*>(base + l.order_off);
+    MBVH_HIP(launch_order_points(points, n, o, base + l.ws_off, s));
+    order = o;
+  }
+  FixedScratch fx;
+  fx.queue = reinterpret_cast<uint32_t*>(base);
+  fx.ovf = reinterpret_cast<uint2*>(base + l.ovf_off);
+  fx.ovf_bytes = MBVH_NOALLOC_SCRATCH_BYTES - l.ovf_off;
+  return closest_point_impl(ctx, mesh, points, n, order, min_threshold, max_threshold, face_index, distance,
+                            closest, barycoord, s, &fx);
 }
 
 extern "C" int mbvh_closest_point(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,

@@ -20,6 +20,7 @@ CENTER, AVERAGE, SAH = 0, 1, 2
 FRONT_SIDE, BACK_SIDE, DOUBLE_SIDE, SIDE_PER_TRIANGLE = 0, 1, 2, 3
 GRAD_ATOMIC, GRAD_DETERMINISTIC = 0, 1
 NO_HIT = 0xFFFFFFFF
+NOALLOC_MAX_QUERIES = 1 << 21  # MBVH_NOALLOC_MAX_QUERIES
 
 _PKG_ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
 LIB_PATH = os.path.join(_PKG_ROOT, "lib", "libmbvh.so")
@@ -33,7 +34,7 @@ EXPORTED_SYMBOLS = (
     "mbvh_mesh_upload", "mbvh_mesh_free", "mbvh_mesh_info", "mbvh_mesh_set_side_mask", "mbvh_mesh_refit",
     "mbvh_mesh_refit_d", "mbvh_mesh_copy_roots", "mbvh_intersects_sphere",
     "mbvh_raycast_first", "mbvh_raycast_first_d",
-    "mbvh_closest_point", "mbvh_closest_point_d",
+    "mbvh_closest_point", "mbvh_closest_point_d", "mbvh_closest_point_noalloc_d",
     "mbvh_raycast", "mbvh_raycast_fetch", "mbvh_raycast_count_d", "mbvh_raycast_fill_d",
     "mbvh_order_workspace_bytes", "mbvh_order_points_d",
     "mbvh_batch_create", "mbvh_batch_free", "mbvh_batch_workspace_bytes", "mbvh_batch_raycast_first_d",
@@ -113,6 +114,7 @@ def library() -> C.CDLL:
         "mbvh_raycast_first_d": (C.c_int, [vp, vp, vp, vp, u64, vp, i32, f32, f32, vp, vp, vp, vp, vp]),
         "mbvh_closest_point": (C.c_int, [vp, vp, vp, u64, f32, f32, vp, vp, vp, vp]),
         "mbvh_closest_point_d": (C.c_int, [vp, vp, vp, u64, vp, f32, f32, vp, vp, vp, vp, vp]),
+        "mbvh_closest_point_noalloc_d": (C.c_int, [vp, vp, vp, u64, f32, f32, vp, vp, vp, vp, vp]),
         "mbvh_order_workspace_bytes": (C.c_int, [u64, C.POINTER(u64)]),
         "mbvh_order_points_d": (C.c_int, [vp, vp, u64, vp, vp, u64, vp]),
         "mbvh_batch_create": (C.c_int, [vp, C.POINTER(vp), u32, C.POINTER(vp)]),
@@ -548,6 +550,16 @@ class Mesh:
         _check(self.ctx._lib.mbvh_closest_point_d(
             self.ctx.handle, self.handle, C.c_void_p(points_ptr), n, C.c_void_p(order_ptr or None),
             min_threshold, max_threshold,
+            C.c_void_p(face_ptr), C.c_void_p(dist_ptr or None), C.c_void_p(point_ptr or None),
+            C.c_void_p(bary_ptr or None), C.c_void_p(stream or None)))
+
+    def closest_point_noalloc_d(self, points_ptr: int, n: int, face_ptr: int, dist_ptr: int = 0, point_ptr: int = 0,
+                                bary_ptr: int = 0, min_threshold: float = 0.0, max_threshold: float = float("inf"),
+                                stream: int = 0) -> None:
+        """mbvh_closest_point_noalloc_d: as closest_point_d, with the library's fixed per-device scratch
+        (no device allocation; at most NOALLOC_MAX_QUERIES points; overlapping calls on one GPU race)."""
+        _check(self.ctx._lib.mbvh_closest_point_noalloc_d(
+            self.ctx.handle, self.handle, C.c_void_p(points_ptr), n, min_threshold, max_threshold,
             C.c_void_p(face_ptr), C.c_void_p(dist_ptr or None), C.c_void_p(point_ptr or None),
             C.c_void_p(bary_ptr or None), C.c_void_p(stream or None)))
 
