@@ -131,8 +131,11 @@ def scatter_vertices(index: np.ndarray, faces: np.ndarray, ga, gb, gc, vertex_co
     out = np.zeros((vertex_count, 3))
     scale = np.zeros((vertex_count, 3))
     keep = np.ones(tri.shape[0], bool) if hit is None else np.asarray(hit, bool)
-    whole = np.abs(np.asarray(ga) + np.asarray(gb) + np.asarray(gc))
+    whole = np.abs(np.asarray(ga, np.float64) + np.asarray(gb, np.float64) + np.asarray(gc, np.float64))[keep]
     for k, g in enumerate((ga, gb, gc)):
-        np.add.at(out, tri[keep, k], g[keep])
-        np.add.at(scale, tri[keep, k], whole[keep])
+        v = tri[keep, k]
+        g = np.asarray(g, np.float64)[keep]
+        for j in range(3):
+            out[:, j] += np.bincount(v, weights=g[:, j], minlength=vertex_count)
+            scale[:, j] += np.bincount(v, weights=whole[:, j], minlength=vertex_count)
     return out, scale
