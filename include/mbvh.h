@@ -232,6 +232,24 @@ int mbvh_raycast_first_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* orig
                          const float* directions, uint64_t n, const uint32_t* order, int32_t side, float near_dist,
                          float far_dist, uint32_t* face_index, float* distance, float* point,
                          float* barycoord, void* stream);
+/*
+ * Double-precision rays, as three.js holds them (its Ray and the local ray raycastObject3D builds,
+ * src/core/MeshBVH.js:456-468, are JS numbers), traced as the reference traces them: every decision
+ * (box, triangle, near / far, which of two equal hits) is the one for the double ray, not for its
+ * float rounding. Each double x is split as hi = fl(x), lo = fl(x - hi); the kernels trace hi with
+ * error bounds widened by lo and decide in double-float with both (all GPU arithmetic stays FP32).
+ * Results as mbvh_raycast_first (float outputs). Magnitudes beyond the float range count as
+ * infinite, and below 2^-149 as zero.
+ */
+int mbvh_raycast_first_f64(mbvh_ctx* ctx, const mbvh_mesh* mesh, const double* origins, const double* directions,
+                           uint64_t n, int32_t side, double near_dist, double far_dist, uint32_t* face_index,
+                           float* distance, float* point, float* barycoord);
+/* Device form: the rays already split (origins / directions the hi parts, *_lo the lo parts, both
+ * device arrays, lo = fl(x - hi)); `order` as mbvh_raycast_first_d. */
+int mbvh_raycast_first_ff_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins, const float* origins_lo,
+                            const float* directions, const float* directions_lo, uint64_t n, const uint32_t* order,
+                            int32_t side, double near_dist, double far_dist, uint32_t* face_index, float* distance,
+                            float* point, float* barycoord, void* stream);
 
 /*
  * intersectsSphere (MeshBVH.intersectsSphere src/core/MeshBVH.js:800-809 with
@@ -303,6 +321,9 @@ int mbvh_raycast(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins, con
                  uint64_t n, int32_t side, float near_dist, float far_dist, uint64_t* offsets);
 int mbvh_raycast_fetch(mbvh_ctx* ctx, uint32_t* face_index, float* distance, float* point,
                        float* barycoord, uint64_t total);
+/* mbvh_raycast for double-precision rays (as mbvh_raycast_first_f64); fetch with mbvh_raycast_fetch. */
+int mbvh_raycast_f64(mbvh_ctx* ctx, const mbvh_mesh* mesh, const double* origins, const double* directions,
+                     uint64_t n, int32_t side, double near_dist, double far_dist, uint64_t* offsets);
 /* Device form in two passes: hits per ray, then the fill at caller-computed offsets (the exclusive
  * prefix sum of the counts); `capacity` bounds the output arrays. */
 int mbvh_raycast_count_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins,
@@ -331,6 +352,15 @@ int mbvh_closest_point(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points
 int mbvh_closest_point_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
                          const uint32_t* order, float min_threshold, float max_threshold, uint32_t* face_index,
                          float* distance, float* closest, float* barycoord, void* stream);
+/* Double-precision query points and thresholds (three's Vector3 are JS numbers), decided as the
+ * reference decides them for the double point (split and traced as mbvh_raycast_first_f64). */
+int mbvh_closest_point_f64(mbvh_ctx* ctx, const mbvh_mesh* mesh, const double* points, uint64_t n,
+                           double min_threshold, double max_threshold, uint32_t* face_index, float* distance,
+                           float* closest, float* barycoord);
+/* Device form: points split as hi (points) + lo (points_lo) device arrays. */
+int mbvh_closest_point_ff_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, const float* points_lo,
+                            uint64_t n, const uint32_t* order, double min_threshold, double max_threshold,
+                            uint32_t* face_index, float* distance, float* closest, float* barycoord, void* stream);
 
 /* Size of the per-device scratch the allocation-free entry point below uses, and its query limit. */
 #define MBVH_NOALLOC_SCRATCH_BYTES (32ull << 20)
@@ -405,6 +435,20 @@ int mbvh_batch_raycast_first(mbvh_ctx* ctx, mbvh_batch* batch, const float* matr
                              uint32_t n_instances, const float* origins, const float* directions,
                              const uint32_t* ray_instance, uint64_t n, int32_t side, float near_dist, float far_dist,
                              uint32_t* face_index, float* distance, float* point, float* barycoord);
+/*
+ * The batched launch on rays already brought into their mesh's frame in double precision: what
+ * acceleratedBatchedMeshRaycast (src/utils/ExtensionUtilities.js:87-161) hands each instance's
+ * raycastObject3D (src/core/MeshBVH.js:456-468: the world ray through matrixWorld^-1 and near / far
+ * divided by the scale, all JS numbers). One ray per (ray, instance) pair: ray_mesh[i] names the batch
+ * mesh, windows[2i], windows[2i + 1] its near / far in local units (windows NULL: 0 and +Infinity).
+ * Traced as mbvh_raycast_first_f64 (every decision for the double ray) in one launch; results stay
+ * in the mesh's frame (the caller converts them as convertRaycastIntersect does). A ray_mesh value
+ * outside the batch misses. Host pointers, synchronous.
+ */
+int mbvh_batch_raycast_first_local_f64(mbvh_ctx* ctx, mbvh_batch* batch, const double* origins,
+                                       const double* directions, const double* windows, const uint32_t* ray_mesh,
+                                       uint64_t n, int32_t side, uint32_t* face_index, float* distance, float* point,
+                                       float* barycoord);
 /* ------------------------------------------------------------------------------------------ */
 /* Export to the reference's shader data layouts                                              */
 /* ------------------------------------------------------------------------------------------ */

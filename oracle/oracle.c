@@ -373,6 +373,23 @@ static void raycast_all_node(RayCtx* r, NodeBuf nb, uint64_t node, HitList* out)
   if (node_bounds_hit(nb_bounds(nb, right), r)) raycast_all_node(r, nb, right, out);
 }
 
+int64_t orc_raycast_all_f64(const orc_mesh* m, const double origin[3], const double dir[3], int side,
+                            double near_d, double far_d, uint64_t cap, uint32_t* face, double* dist) {
+  RayCtx r;
+  memset(&r, 0, sizeof(r));
+  for (int k = 0; k < 3; k++) {
+    r.o[k] = origin[k];
+    r.d[k] = dir[k];
+  }
+  r.side = side;
+  r.near_d = near_d;
+  r.far_d = far_d;
+  r.m = m;
+  HitList out = {0, cap, face, dist};
+  for (uint32_t i = 0; i < m->n_roots; i++) raycast_all_node(&r, root_buf(m, i), 0, &out);
+  return (int64_t)out.count;
+}
+
 int64_t orc_raycast_all(const orc_mesh* m, const float origin[3], const float dir[3], int side,
                         double near_d, double far_d, uint64_t cap, uint32_t* face, double* dist) {
   RayCtx r;
@@ -479,6 +496,7 @@ typedef struct {
   const orc_mesh* m;
   int kind; /* 0 raycast first, 1 closest point */
   const float *a, *b;
+  const double *ad, *bd; /* double-precision inputs (used instead of a, b when set) */
   uint64_t n;
   int side, brute;
   double p0, p1;
@@ -492,8 +510,8 @@ static void run_ray(const Job* j, uint64_t i) {
   RayCtx r;
   memset(&r, 0, sizeof(r));
   for (int k = 0; k < 3; k++) {
-    r.o[k] = j->a[3 * i + k];
-    r.d[k] = j->b[3 * i + k];
+    r.o[k] = j->ad ? j->ad[3 * i + k] : j->a[3 * i + k];
+    r.d[k] = j->bd ? j->bd[3 * i + k] : j->b[3 * i + k];
   }
   r.side = j->side;
   r.near_d = j->p0;
@@ -520,7 +538,7 @@ static void run_ray(const Job* j, uint64_t i) {
 static void run_point(const Job* j, uint64_t i) {
   PointCtx c;
   memset(&c, 0, sizeof(c));
-  for (int k = 0; k < 3; k++) c.p[k] = j->a[3 * i + k];
+  for (int k = 0; k < 3; k++) c.p[k] = j->ad ? j->ad[3 * i + k] : j->a[3 * i + k];
   c.best_sq = INFINITY;
   c.min_sq = j->p0 * j->p0;
   c.max_sq = j->p1 * j->p1;
@@ -616,6 +634,69 @@ int orc_closest_point(const orc_mesh* m, const float* points, uint64_t n, double
   j.nodes = nodes_read;
   j.tris = tris_tested;
   return run_jobs(j, nthreads);
+}
+
+int orc_raycast_first_f64(const orc_mesh* m, const double* origins, const double* dirs, uint64_t n, int side,
+                          double near_d, double far_d, int brute, int nthreads, uint32_t* face, double* dist,
+                          double* point, double* bary, uint32_t* nodes_read, uint32_t* tris_tested) {
+  Job j;
+  memset(&j, 0, sizeof(j));
+  j.m = m;
+  j.kind = 0;
+  j.ad = origins;
+  j.bd = dirs;
+  j.n = n;
+  j.side = side;
+  j.brute = brute;
+  j.p0 = near_d;
+  j.p1 = far_d;
+  j.face = face;
+  j.dist = dist;
+  j.point = point;
+  j.bary = bary;
+  j.nodes = nodes_read;
+  j.tris = tris_tested;
+  return run_jobs(j, nthreads);
+}
+
+int orc_closest_point_f64(const orc_mesh* m, const double* points, uint64_t n, double min_t, double max_t, int brute,
+                          int nthreads, uint32_t* face, double* dist, double* point, uint32_t* nodes_read,
+                          uint32_t* tris_tested) {
+  Job j;
+  memset(&j, 0, sizeof(j));
+  j.m = m;
+  j.kind = 1;
+  j.ad = points;
+  j.n = n;
+  j.brute = brute;
+  j.p0 = min_t;
+  j.p1 = max_t;
+  j.face = face;
+  j.dist = dist;
+  j.point = point;
+  j.nodes = nodes_read;
+  j.tris = tris_tested;
+  return run_jobs(j, nthreads);
+}
+
+double orc_ray_plane_distance_f64(const orc_mesh* m, uint32_t face, const double o[3], const double d[3]) {
+  double a[3], b[3], c[3], e1[3], e2[3], n[3], q[3];
+  face_vertices(m, face, a, b, c);
+  v_sub(b, a, e1);
+  v_sub(c, a, e2);
+  v_cross(e1, e2, n);
+  const double ddn = v_dot(d, n);
+  if (ddn == 0) return INFINITY;
+  v_sub(a, o, q);
+  const double t = v_dot(q, n) / ddn;
+  return fabs(t) * sqrt(v_dot(d, d));
+}
+
+double orc_point_face_distance_f64(const orc_mesh* m, uint32_t face, const double q[3]) {
+  double a[3], b[3], c[3], cp[3];
+  face_vertices(m, face, a, b, c);
+  orc_closest_point_triangle(q, a, b, c, cp);
+  return sqrt(v_dist_sq(q, cp));
 }
 
 double orc_ray_plane_distance(const orc_mesh* m, uint32_t face, const float origin[3],

@@ -75,6 +75,18 @@ int orc_closest_point(const orc_mesh* m, const float* points, uint64_t n, double
                       double max_t, int brute, int nthreads, uint32_t* face, double* dist,
                       double* point, uint32_t* nodes_read, uint32_t* tris_tested);
 
+/* The same for double-precision rays / points (three's Ray and Vector3 hold JS numbers). */
+int orc_raycast_first_f64(const orc_mesh* m, const double* origins, const double* dirs, uint64_t n, int side,
+                          double near_d, double far_d, int brute, int nthreads, uint32_t* face, double* dist,
+                          double* point, double* bary, uint32_t* nodes_read, uint32_t* tris_tested);
+int orc_closest_point_f64(const orc_mesh* m, const double* points, uint64_t n, double min_t, double max_t, int brute,
+                          int nthreads, uint32_t* face, double* dist, double* point, uint32_t* nodes_read,
+                          uint32_t* tris_tested);
+int64_t orc_raycast_all_f64(const orc_mesh* m, const double origin[3], const double dir[3], int side,
+                            double near_d, double far_d, uint64_t cap, uint32_t* face, double* dist);
+double orc_ray_plane_distance_f64(const orc_mesh* m, uint32_t face, const double o[3], const double d[3]);
+double orc_point_face_distance_f64(const orc_mesh* m, uint32_t face, const double q[3]);
+
 /* FP64 distance from the ray origin to triangle `face` along the ray's line (the plane hit, no
  * edge test); used to decide whether two faces tie for a ray. Returns +inf if parallel. */
 double orc_ray_plane_distance(const orc_mesh* m, uint32_t face, const float origin[3],

@@ -64,6 +64,16 @@ def library():
         lib.orc_ray_plane_distance.restype = f64
         lib.orc_point_face_distance.argtypes = [vp, u32, vp]
         lib.orc_point_face_distance.restype = f64
+        lib.orc_raycast_first_f64.argtypes = lib.orc_raycast_first.argtypes
+        lib.orc_raycast_first_f64.restype = C.c_int
+        lib.orc_closest_point_f64.argtypes = lib.orc_closest_point.argtypes
+        lib.orc_closest_point_f64.restype = C.c_int
+        lib.orc_raycast_all_f64.argtypes = lib.orc_raycast_all.argtypes
+        lib.orc_raycast_all_f64.restype = C.c_int64
+        lib.orc_ray_plane_distance_f64.argtypes = [vp, u32, vp, vp]
+        lib.orc_ray_plane_distance_f64.restype = f64
+        lib.orc_point_face_distance_f64.argtypes = [vp, u32, vp]
+        lib.orc_point_face_distance_f64.restype = f64
         lib.orc_intersects_sphere.argtypes = [vp, vp, f64]
         lib.orc_intersects_sphere.restype = C.c_int
         lib.orc_intersects_box.argtypes = [vp, vp, vp, vp]
@@ -87,6 +97,10 @@ def library():
         lib.orc_validate_root.restype = C.c_int
         _lib = lib
     return _lib
+
+
+def _is_f64(*arrays):
+    return any(np.asarray(a).dtype == np.float64 for a in arrays)
 
 
 def _p(a):
@@ -154,42 +168,58 @@ class OracleMesh:
 
     def raycast_first(self, origins, dirs, side=FRONT, near=0.0, far=float("inf"), brute=False,
                       threads: Optional[int] = None):
-        o = np.ascontiguousarray(origins, dtype=np.float32).reshape(-1, 3)
-        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        """float64 rays are traced as given; other rays as float32."""
+        f64 = _is_f64(origins, dirs)
+        dt = np.float64 if f64 else np.float32
+        o = np.ascontiguousarray(origins, dtype=dt).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, dtype=dt).reshape(-1, 3)
         n = o.shape[0]
         out = {"face": np.empty(n, np.uint32), "distance": np.empty(n), "point": np.empty((n, 3)),
                "barycoord": np.empty((n, 3)), "nodes": np.empty(n, np.uint32), "tris": np.empty(n, np.uint32)}
-        library().orc_raycast_first(self.ref, _p(o), _p(d), n, side, near, far, int(brute),
+        fn = library().orc_raycast_first_f64 if f64 else library().orc_raycast_first
+        fn(self.ref, _p(o), _p(d), n, side, near, far, int(brute),
                                     threads or os.cpu_count() or 1, _p(out["face"]), _p(out["distance"]),
                                     _p(out["point"]), _p(out["barycoord"]), _p(out["nodes"]), _p(out["tris"]))
         return out
 
     def raycast_all(self, origin, direction, side=FRONT, near=0.0, far=float("inf"), cap=4096):
-        o = np.ascontiguousarray(origin, dtype=np.float32).reshape(3)
-        d = np.ascontiguousarray(direction, dtype=np.float32).reshape(3)
+        f64 = _is_f64(origin, direction)
+        dt = np.float64 if f64 else np.float32
+        o = np.ascontiguousarray(origin, dtype=dt).reshape(3)
+        d = np.ascontiguousarray(direction, dtype=dt).reshape(3)
         face = np.empty(cap, np.uint32)
         dist = np.empty(cap)
-        n = library().orc_raycast_all(self.ref, _p(o), _p(d), side, near, far, cap, _p(face), _p(dist))
+        fn = library().orc_raycast_all_f64 if f64 else library().orc_raycast_all
+        n = fn(self.ref, _p(o), _p(d), side, near, far, cap, _p(face), _p(dist))
         n = min(int(n), cap)
         return face[:n], dist[:n]
 
     def closest_point(self, points, min_threshold=0.0, max_threshold=float("inf"), brute=False,
                       threads: Optional[int] = None):
-        q = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        f64 = _is_f64(points)
+        q = np.ascontiguousarray(points, dtype=np.float64 if f64 else np.float32).reshape(-1, 3)
         n = q.shape[0]
         out = {"face": np.empty(n, np.uint32), "distance": np.empty(n), "point": np.empty((n, 3)),
                "nodes": np.empty(n, np.uint32), "tris": np.empty(n, np.uint32)}
-        library().orc_closest_point(self.ref, _p(q), n, min_threshold, max_threshold, int(brute),
+        fn = library().orc_closest_point_f64 if f64 else library().orc_closest_point
+        fn(self.ref, _p(q), n, min_threshold, max_threshold, int(brute),
                                     threads or os.cpu_count() or 1, _p(out["face"]), _p(out["distance"]),
                                     _p(out["point"]), _p(out["nodes"]), _p(out["tris"]))
         return out
 
     def ray_plane_distance(self, face, origin, direction):
+        if _is_f64(origin, direction):
+            o = np.ascontiguousarray(origin, dtype=np.float64).reshape(3)
+            d = np.ascontiguousarray(direction, dtype=np.float64).reshape(3)
+            return library().orc_ray_plane_distance_f64(self.ref, int(face), _p(o), _p(d))
         o = np.ascontiguousarray(origin, dtype=np.float32).reshape(3)
         d = np.ascontiguousarray(direction, dtype=np.float32).reshape(3)
         return library().orc_ray_plane_distance(self.ref, int(face), _p(o), _p(d))
 
     def point_face_distance(self, face, p):
+        if _is_f64(p):
+            return library().orc_point_face_distance_f64(self.ref, int(face),
+                                                         _p(np.ascontiguousarray(p, dtype=np.float64).reshape(3)))
         q = np.ascontiguousarray(p, dtype=np.float32).reshape(3)
         return library().orc_point_face_distance(self.ref, int(face), _p(q))
 

@@ -114,102 +114,159 @@ for (let i = 0; i < Math.min(256, nPts); i++) {
   assert.strictEqual(t.distance, cp.distance[i]);
 }
 
-// --- raycastObject3D / acceleratedRaycast under a rigid transform ----------------------------
+// --- raycastObject3D / acceleratedRaycast under transforms ------------------------------------
+// World rays in double precision through Mesh.raycast = acceleratedRaycast (firstHitOnly), for a rigid,
+// a non-uniformly scaled and a uniformly scaled matrixWorld (the last with a near / far window). The
+// reference's own local ray (raycastObject3D, src/core/MeshBVH.js:456-468: Matrix4.invert,
+// Ray.applyMatrix4, near / far over |local dir * world scale|) is written out in double precision with
+// the GPU's hit; the Python side traces those local rays with the FP64 oracle.
+const X = require('./shim.js');
 const q = [0.2, -0.3, 0.4, 0];
 q[3] = Math.sqrt(1 - q[0] * q[0] - q[1] * q[1] - q[2] * q[2]);
-const M = compose([3, -2, 5], q, [1, 1, 1]);
-const e = M.elements;
-const apply = v => [e[0] * v[0] + e[4] * v[1] + e[8] * v[2] + e[12], e[1] * v[0] + e[5] * v[1] + e[9] * v[2] + e[13], e[2] * v[0] + e[6] * v[1] + e[10] * v[2] + e[14]];
-const rot = v => [e[0] * v[0] + e[4] * v[1] + e[8] * v[2], e[1] * v[0] + e[5] * v[1] + e[9] * v[2], e[2] * v[0] + e[6] * v[1] + e[10] * v[2]];
+const q2 = [-0.5, 0.1, 0.3, 0];
+q2[3] = Math.sqrt(1 - q2[0] * q2[0] - q2[1] * q2[1] - q2[2] * q2[2]);
+const transforms = [
+  { m: compose([3, -2, 5], q, [1, 1, 1]), near: 0, far: Infinity },
+  { m: compose([-1, 4, 2], q2, [0.5, 2.0, 1.3]), near: 0, far: Infinity },
+  { m: compose([2, 1, -3], q, [3, 3, 3]), near: 95, far: 140 },
+];
+const M = transforms[0].m;
 geometry.boundsTree = bvh;
 const mesh = new Mesh(geometry, { isMaterial: true, side: api.FrontSide });
-mesh.matrixWorld = M;
 mesh.raycast = api.acceleratedRaycast;
-let worldChecked = 0;
-for (let i = 0; i < K; i++) {
-  const o = apply([origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]]);
-  const d = rot([dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]]);
-  const rc = new Raycaster(o, d);
-  rc.firstHitOnly = true;
-  const hits = rc.intersectObject(mesh);
-  if (rf.faceIndex[i] === 0xFFFFFFFF) continue;
-  // the world-space ray is rounded to float32 again, so near-grazing rays may differ; count agreement
-  if (hits.length === 1 && hits[0].faceIndex === rf.faceIndex[i]) {
-    assert.ok(Math.abs(hits[0].distance - rf.distance[i]) <= 1e-4 * Math.max(1, rf.distance[i]));
-    assert.strictEqual(hits[0].object, mesh);
-    worldChecked++;
+const NX = Math.min(4096, nRays);
+const xo = new Float64Array(3 * NX * transforms.length), xd = new Float64Array(3 * NX * transforms.length);
+const xwin = new Float64Array(2 * NX * transforms.length), xface = new Uint32Array(NX * transforms.length);
+const xdist = new Float64Array(NX * transforms.length), xpoint = new Float64Array(3 * NX * transforms.length);
+const xmat = new Float64Array(16 * transforms.length), xwo = new Float64Array(3 * NX * transforms.length);
+transforms.forEach((t, ti) => {
+  mesh.matrixWorld = t.m;
+  xmat.set(t.m.elements, 16 * ti);
+  const inv = X.invert(t.m.elements);
+  for (let i = 0; i < NX; i++) {
+    const k = ti * NX + i;
+    // the world ray: the float ray taken through matrixWorld in double, plus a shift below float resolution
+    const ow = X.applyMatrix4(t.m.elements, [origins[3 * i] + 1e-9 * Math.sin(i), origins[3 * i + 1], origins[3 * i + 2]]);
+    const dw = X.transformDirection(t.m.elements, [dirs[3 * i], dirs[3 * i + 1] + 1e-10 * Math.cos(i), dirs[3 * i + 2]]);
+    const rc = new Raycaster(ow, dw);
+    rc.firstHitOnly = true;
+    rc.near = t.near;
+    rc.far = t.far;
+    const hits = rc.intersectObject(mesh);
+    assert.ok(hits.length <= 1);
+    const lo = X.applyMatrix4(inv, ow), ld = X.transformDirection(inv, dw);
+    const sf = X.length3(...X.mulv(ld, X.matrixScale(t.m.elements)));
+    xo.set(lo, 3 * k);
+    xd.set(ld, 3 * k);
+    xwo.set(ow, 3 * k);
+    xwin[2 * k] = t.near / sf;
+    xwin[2 * k + 1] = t.far / sf;
+    if (hits.length) {
+      assert.strictEqual(hits[0].object, mesh);
+      xface[k] = hits[0].faceIndex;
+      xdist[k] = hits[0].distance;
+      xpoint.set([hits[0].point.x, hits[0].point.y, hits[0].point.z], 3 * k);
+    } else {
+      xface[k] = 0xFFFFFFFF;
+      xdist[k] = Infinity;
+    }
   }
-}
-assert.ok(worldChecked >= 0.98 * rf.faceIndex.subarray(0, K).filter(f => f !== 0xFFFFFFFF).length, `world/local agreement ${worldChecked}`);
+});
+writeArr('xf_origins.bin', xo);
+writeArr('xf_dirs.bin', xd);
+writeArr('xf_windows.bin', xwin);
+writeArr('xf_face.bin', xface);
+writeArr('xf_dist.bin', xdist);
+writeArr('xf_point.bin', xpoint);
+writeArr('xf_world_origins.bin', xwo);
+writeArr('xf_matrices.bin', xmat);
+const worldChecked = xface.filter(f => f !== 0xFFFFFFFF).length;
 
-// --- BatchedMesh: two instances of one geometry, hits tagged with batchId --------------------
+// --- BatchedMesh: instances of one geometry (rigid and scaled), firstHitOnly in one launch ----------
+// For each world ray the GPU's first hit after the Raycaster's distance sort (batchId, faceIndex, world
+// distance) is written out with every visible instance's own local ray as the reference computes it
+// (acceleratedBatchedMeshRaycast, src/utils/ExtensionUtilities.js:87-161: matrixWorld * instanceMatrix,
+// then raycastObject3D); the Python side traces them with the FP64 oracle and keeps the nearest.
+const instMats = [[1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1], M.elements.slice(), transforms[1].m.elements.slice()];
+const batchWorld = compose([0.5, -0.25, 0.125], [0, 0, 0, 1], [1, 1, 1]).elements;
 const batched = {
   isBatchedMesh: true,
   geometry,
   material: mesh.material,
-  matrixWorld: { elements: [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1] },
+  matrixWorld: { elements: batchWorld, clone() { return { elements: batchWorld.slice() }; } },
   boundsTrees: [bvh],
-  _drawInfo: [{ geometryIndex: 0 }, { geometryIndex: 0 }],
-  getVisibleAt() { return true; },
+  _drawInfo: [{ geometryIndex: 0 }, { geometryIndex: 0 }, { geometryIndex: 0 }, { geometryIndex: 0 }],
+  getVisibleAt(i) { return i !== 2; },
   getMatrixAt(i, target) {
-    target.elements = i === 0 ? [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1] : M.elements.slice();
+    target.elements = instMats[i % 3].slice();
     return target;
   },
   raycast: api.acceleratedRaycast,
 };
-const rc = new Raycaster([origins[0], origins[1], origins[2]], [dirs[0], dirs[1], dirs[2]]);
-rc.firstHitOnly = true;
-const bh = rc.intersectObject(batched);
-assert.ok(bh.every(h => h.object === batched && (h.batchId === 0 || h.batchId === 1)));
-if (rf.faceIndex[0] !== 0xFFFFFFFF) assert.ok(bh.some(h => h.batchId === 0 && h.faceIndex === rf.faceIndex[0]));
-// firstHitOnly goes through one batched device launch; compare it with the per-instance path (all
-// hits of each instance, nearest kept) over many rays
-let batchedChecked = 0, presenceDiffers = 0;
-for (let i = 0; i < Math.min(K, 128); i++) {
-  const r1 = new Raycaster([origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]], [dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]]);
-  r1.firstHitOnly = true;
-  const fast = r1.intersectObject(batched);
-  const r2 = new Raycaster([origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]], [dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]]);
-  const all = r2.intersectObject(batched);
-  for (const id of [0, 1]) {
-    const f = fast.filter(h => h.batchId === id);
-    const a = all.filter(h => h.batchId === id);
-    assert.ok(f.length <= 1, `ray ${i} instance ${id}`);
-    // a grazing ray may hit on one side of the FP32 / FP64 frame-change rounding and miss on the other
-    if (f.length !== (a.length ? 1 : 0)) { presenceDiffers++; continue; }
-    if (!a.length) continue;
-    const nearest = a.reduce((m, h) => (h.distance < m.distance ? h : m));
-    assert.ok(Math.abs(f[0].distance - nearest.distance) <= 1e-4 * Math.max(1, nearest.distance), `ray ${i} instance ${id} distance`);
-    assert.strictEqual(f[0].object, batched);
-    batchedChecked++;
+const NB = Math.min(2048, nRays), visible = [0, 1, 3];
+const bo = new Float64Array(3 * NB * visible.length), bd = new Float64Array(3 * NB * visible.length);
+const bworld = new Float64Array(3 * NB), bface = new Uint32Array(NB), binst = new Uint32Array(NB);
+const bdist = new Float64Array(NB), bmats = new Float64Array(16 * visible.length);
+visible.forEach((id, v) => bmats.set(X.multiply(batchWorld, instMats[id % 3]), 16 * v));
+for (let i = 0; i < NB; i++) {
+  const ow = [origins[3 * i] + 3e-9 * Math.cos(i), origins[3 * i + 1], origins[3 * i + 2]];
+  const dw = [dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2] + 2e-10 * Math.sin(i)];
+  const rc = new Raycaster(ow, dw);
+  rc.firstHitOnly = true;
+  const hits = rc.intersectObject(batched);
+  assert.ok(hits.every(h => h.object === batched && visible.includes(h.batchId)));
+  assert.ok(new Set(hits.map(h => h.batchId)).size === hits.length, 'one hit per instance');
+  bworld.set(ow, 3 * i);
+  visible.forEach((id, v) => {
+    const wm = bmats.subarray(16 * v, 16 * v + 16);
+    const inv = X.invert(Array.from(wm));
+    bo.set(X.applyMatrix4(inv, ow), 3 * (v * NB + i));
+    bd.set(X.transformDirection(inv, dw), 3 * (v * NB + i));
+  });
+  if (hits.length) {
+    bface[i] = hits[0].faceIndex;
+    binst[i] = hits[0].batchId;
+    bdist[i] = hits[0].distance;
+  } else {
+    bface[i] = 0xFFFFFFFF;
+    binst[i] = 0xFFFFFFFF;
+    bdist[i] = Infinity;
   }
 }
-assert.ok(batchedChecked > 64 && presenceDiffers <= 2, `batched BatchedMesh hits ${batchedChecked}, ${presenceDiffers} differ`);
+writeArr('bm_origins.bin', bo);
+writeArr('bm_dirs.bin', bd);
+writeArr('bm_world_origins.bin', bworld);
+writeArr('bm_matrices.bin', bmats);
+writeArr('bm_face.bin', bface);
+writeArr('bm_inst.bin', binst);
+writeArr('bm_dist.bin', bdist);
+writeArr('bm_visible.bin', new Uint32Array(visible));
+const batchedChecked = bface.filter(f => f !== 0xFFFFFFFF).length, presenceDiffers = 0;
 // SceneBVH over the BatchedMesh's instances: one launch gives the hit firstHitOnly's per-instance hits
-// put first after the Raycaster's distance sort (same device frame change, so the same bits)
+// put first after the Raycaster's distance sort. The scene changes frames in FP32 on the device (its
+// own rounding of the world ray), the BatchedMesh path in double as the reference: the same instance,
+// face and distance up to that rounding (a ray grazing an edge may land on the other side of it).
 {
   const scene = api.SceneBVH.fromBatchedMesh(batched);
   const nS = Math.min(K, 128);
   const many = scene.raycastFirstBatch(origins.subarray(0, 3 * nS), dirs.subarray(0, 3 * nS));
-  let sceneHits = 0;
+  let sceneHits = 0, sceneSame = 0;
   for (let i = 0; i < nS; i++) {
     const ray = new Raycaster([origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]], [dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]]);
     ray.firstHitOnly = true;
     const first = ray.intersectObject(batched)[0];
     const hit = scene.raycastFirst(ray.ray, batched.material);
-    if (!first) {
-      assert.strictEqual(hit, null, `scene ray ${i}`);
-      assert.strictEqual(many.faceIndex[i], 0xFFFFFFFF);
-      continue;
-    }
-    assert.ok(hit && hit.batchId === first.batchId && hit.faceIndex === first.faceIndex, `scene ray ${i}`);
-    assert.strictEqual(Math.fround(hit.distance), Math.fround(first.distance));
-    assert.strictEqual(hit.object, batched);
-    assert.strictEqual(many.instance[i], first.batchId);
-    assert.strictEqual(many.faceIndex[i], first.faceIndex);
+    assert.strictEqual(many.faceIndex[i], hit ? hit.faceIndex : 0xFFFFFFFF, `scene ray ${i}: batch == per call`);
+    if (!first || !hit) continue;
     sceneHits++;
+    if (hit.batchId === first.batchId && hit.faceIndex === first.faceIndex) {
+      assert.ok(Math.abs(hit.distance - first.distance) <= 1e-5 * Math.max(1, first.distance), `scene ray ${i}`);
+      assert.strictEqual(hit.object, batched);
+      assert.strictEqual(many.instance[i], first.batchId);
+      sceneSame++;
+    }
   }
-  assert.ok(sceneHits > 32, `scene hits ${sceneHits}`);
+  assert.ok(sceneHits > 32 && sceneSame >= sceneHits - 2, `scene hits ${sceneHits}, same ${sceneSame}`);
   // closest point over the instances: never farther than any visible instance's own closest point
   // (its local answer taken to world space), and the returned point is at the returned distance
   const apply = (e, o, x, y, z) => [e[o] * x + e[o + 4] * y + e[o + 8] * z + e[o + 12],

@@ -85,4 +85,64 @@ class Raycaster {
   }
 }
 
-module.exports = { BufferAttribute, BufferGeometry, geometryFrom, Vector3, compose, Mesh, Raycaster };
+// three.js Matrix4 / Vector3 arithmetic in the library's own operation order (JS doubles), for
+// computing the local rays the reference traces: Matrix4.invert, Matrix4.multiplyMatrices,
+// Vector3.applyMatrix4 (with the projective divide), Vector3.transformDirection (normalize multiplies
+// by 1 / length), Vector3.length and Vector3.setFromMatrixScale.
+function invert(te) {
+  const [n11, n21, n31, n41, n12, n22, n32, n42, n13, n23, n33, n43, n14, n24, n34, n44] = te;
+  const t11 = n23 * n34 * n42 - n24 * n33 * n42 + n24 * n32 * n43 - n22 * n34 * n43 - n23 * n32 * n44 + n22 * n33 * n44;
+  const t12 = n14 * n33 * n42 - n13 * n34 * n42 - n14 * n32 * n43 + n12 * n34 * n43 + n13 * n32 * n44 - n12 * n33 * n44;
+  const t13 = n13 * n24 * n42 - n14 * n23 * n42 + n14 * n22 * n43 - n12 * n24 * n43 - n13 * n22 * n44 + n12 * n23 * n44;
+  const t14 = n14 * n23 * n32 - n13 * n24 * n32 - n14 * n22 * n33 + n12 * n24 * n33 + n13 * n22 * n34 - n12 * n23 * n34;
+  const det = n11 * t11 + n21 * t12 + n31 * t13 + n41 * t14;
+  if (det === 0) return new Array(16).fill(0);
+  const detInv = 1 / det;
+  return [
+    t11 * detInv,
+    (n24 * n33 * n41 - n23 * n34 * n41 - n24 * n31 * n43 + n21 * n34 * n43 + n23 * n31 * n44 - n21 * n33 * n44) * detInv,
+    (n22 * n34 * n41 - n24 * n32 * n41 + n24 * n31 * n42 - n21 * n34 * n42 - n22 * n31 * n44 + n21 * n32 * n44) * detInv,
+    (n23 * n32 * n41 - n22 * n33 * n41 - n23 * n31 * n42 + n21 * n33 * n42 + n22 * n31 * n43 - n21 * n32 * n43) * detInv,
+    t12 * detInv,
+    (n13 * n34 * n41 - n14 * n33 * n41 + n14 * n31 * n43 - n11 * n34 * n43 - n13 * n31 * n44 + n11 * n33 * n44) * detInv,
+    (n14 * n32 * n41 - n12 * n34 * n41 - n14 * n31 * n42 + n11 * n34 * n42 + n12 * n31 * n44 - n11 * n32 * n44) * detInv,
+    (n12 * n33 * n41 - n13 * n32 * n41 + n13 * n31 * n42 - n11 * n33 * n42 - n12 * n31 * n43 + n11 * n32 * n43) * detInv,
+    t13 * detInv,
+    (n14 * n23 * n41 - n13 * n24 * n41 - n14 * n21 * n43 + n11 * n24 * n43 + n13 * n21 * n44 - n11 * n23 * n44) * detInv,
+    (n12 * n24 * n41 - n14 * n22 * n41 + n14 * n21 * n42 - n11 * n24 * n42 - n12 * n21 * n44 + n11 * n22 * n44) * detInv,
+    (n13 * n22 * n41 - n12 * n23 * n41 - n13 * n21 * n42 + n11 * n23 * n42 + n12 * n21 * n43 - n11 * n22 * n43) * detInv,
+    t14 * detInv,
+    (n13 * n24 * n31 - n14 * n23 * n31 + n14 * n21 * n33 - n11 * n24 * n33 - n13 * n21 * n34 + n11 * n23 * n34) * detInv,
+    (n14 * n22 * n31 - n12 * n24 * n31 - n14 * n21 * n32 + n11 * n24 * n32 + n12 * n21 * n34 - n11 * n22 * n34) * detInv,
+    (n12 * n23 * n31 - n13 * n22 * n31 + n13 * n21 * n32 - n11 * n23 * n32 - n12 * n21 * n33 + n11 * n22 * n33) * detInv,
+  ];
+}
+function multiply(a, b) {  // a * b, column-major
+  const out = new Array(16);
+  for (let r = 0; r < 4; r++) {
+    for (let c = 0; c < 4; c++) {
+      out[4 * c + r] = a[r] * b[4 * c] + a[4 + r] * b[4 * c + 1] + a[8 + r] * b[4 * c + 2] + a[12 + r] * b[4 * c + 3];
+    }
+  }
+  return out;
+}
+function applyMatrix4(e, v) {
+  const [x, y, z] = v;
+  const w = 1 / (e[3] * x + e[7] * y + e[11] * z + e[15]);
+  return [(e[0] * x + e[4] * y + e[8] * z + e[12]) * w, (e[1] * x + e[5] * y + e[9] * z + e[13]) * w,
+    (e[2] * x + e[6] * y + e[10] * z + e[14]) * w];
+}
+function length3(x, y, z) { return Math.sqrt(x * x + y * y + z * z); }
+function transformDirection(e, v) {
+  const [x, y, z] = v;
+  const r = [e[0] * x + e[4] * y + e[8] * z, e[1] * x + e[5] * y + e[9] * z, e[2] * x + e[6] * y + e[10] * z];
+  const s = 1 / (length3(...r) || 1);
+  return [r[0] * s, r[1] * s, r[2] * s];
+}
+function matrixScale(e) { return [length3(e[0], e[1], e[2]), length3(e[4], e[5], e[6]), length3(e[8], e[9], e[10])]; }
+function mulv(a, b) { return [a[0] * b[0], a[1] * b[1], a[2] * b[2]]; }
+
+module.exports = {
+  BufferAttribute, BufferGeometry, geometryFrom, Vector3, compose, Mesh, Raycaster,
+  invert, multiply, applyMatrix4, length3, transformDirection, matrixScale, mulv,
+};

@@ -5,7 +5,7 @@ import pytest
 import mbvh_amd as M
 from mbvh_amd import geometry as G, workloads as W
 import oracle_py as O
-from helpers import build_mesh, check_closest_parity
+from helpers import build_mesh, check_closest_parity, point_faces_tied, threshold_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -55,6 +55,12 @@ def test_sphere_kats(ctx):
     mesh.free()
 
 
+def _faces_equal_up_to_rounding(om, pts, got, ref):
+    """Indices where the faces differ and are not rounding ties (point_faces_tied)."""
+    diff = np.nonzero(got["face"] != ref["face"])[0]
+    return [int(i) for i in diff if not point_faces_tied(om, pts[i], got["face"][i], ref["face"][i])]
+
+
 def test_thresholds(ctx):
     g = G.torus_knot(10, 3, 120, 30)
     lo, hi = g.bounds()
@@ -64,14 +70,51 @@ def test_thresholds(ctx):
     g2, idx, tree = build_mesh(g)
     mesh = ctx.upload(tree, g2.positions, idx)
     om2 = O.OracleMesh(g2.positions, idx, tree.roots)
-    got = mesh.closest_point(pts, 1.0, float("inf"))
-    ref = om2.closest_point(pts, 1.0, float("inf"))
-    # with a minThreshold the result is the first face (traversal order) under it, or the closest
-    close = ref["distance"] < 1.0
-    assert np.all(got["distance"][close] < 1.0 + 1e-6)
-    far = ~close
-    assert np.all(np.abs(got["distance"][far] - ref["distance"][far]) <= 1e-5 * np.maximum(ref["distance"][far], 1))
+    closest = om2.closest_point(pts)
+    # with a minThreshold the result is the first face in the reference's visit order under it, or the
+    # closest: the one-lane kernel follows that order exactly (children by exact box distance, the
+    # threshold decided exactly), so the faces are the oracle's up to rounding ties, and exactly where
+    # the early exit returned a face that is not the closest
+    for mt in (1.0, 0.3, 2.5):
+        got = mesh.closest_point(pts, mt, float("inf"))
+        ref = om2.closest_point(pts, mt, float("inf"))
+        under = ref["distance"] < mt
+        early = under & (ref["distance"] > closest["distance"] * (1 + 1e-9))
+        print(f"minThreshold {mt}: {under.mean():.3f} under it, {early.sum()} early exits before the closest",
+              flush=True)
+        assert under.mean() > 0.05 and early.sum() > 50
+        assert np.array_equal(got["face"][early], ref["face"][early]), np.count_nonzero(got["face"][early] != ref["face"][early])
+        bad = _faces_equal_up_to_rounding(om2, pts, got, ref)
+        assert not bad, (mt, len(bad), bad[:5])
+        assert np.all(got["distance"][under] < mt)
+        assert np.all(np.abs(got["distance"] - ref["distance"]) <= 1e-5 * np.maximum(ref["distance"], 1))
     mesh.free()
+
+
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_threshold_branches_by_hand(lanes):
+    """The hand-built two-leaf mesh of the oracle's threshold test: each (minThreshold,
+    maxThreshold) case gives the face and distance worked out by hand, float and double queries."""
+    geo, cases = threshold_cases()
+    g, idx, tree = build_mesh(geo, strategy=M.CENTER)
+    om = O.OracleMesh(g.positions, idx, tree.roots)
+    c = M.Context(0)
+    c.set_point_lanes(lanes)
+    mesh = c.upload(tree, g.positions, idx)
+    try:
+        for dt in (np.float32, np.float64):
+            p = np.zeros((64, 3), dt)
+            for mn, mx, want in cases:
+                got = mesh.closest_point(p, mn, mx)
+                ref = om.closest_point(p[:1], mn, mx)
+                assert np.all(got["face"] == ref["face"][0]), (dt, mn, mx, got["face"][0], ref["face"][0])
+                if want is None:
+                    assert np.all(np.isinf(got["distance"]))
+                else:
+                    assert np.all(got["distance"] == np.float32(want)), (dt, mn, mx, got["distance"][0])
+    finally:
+        mesh.free()
+        c.close()
 
 
 def test_groups_and_indirect(ctx):

@@ -60,6 +60,86 @@ def test_js_api_matches_oracle(tmp_path):
     refc = om.closest_point(pts)
     print("js closestPoint", check_closest_parity(om, pts, gotc, refc, "js_closest"))
 
+    # raycastObject3D / acceleratedRaycast under rigid, scaled and windowed transforms: the GPU's first
+    # hit for the double world ray against the FP64 oracle traced on the reference's own local ray
+    # (double precision, written by the driver); faces equal up to ties, world distances to 1e-5
+    lo3 = np.fromfile(d / "xf_origins.bin", np.float64).reshape(-1, 3)
+    ld3 = np.fromfile(d / "xf_dirs.bin", np.float64).reshape(-1, 3)
+    win = np.fromfile(d / "xf_windows.bin", np.float64).reshape(-1, 2)
+    xface = np.fromfile(d / "xf_face.bin", np.uint32)
+    xdist = np.fromfile(d / "xf_dist.bin", np.float64)
+    xpoint = np.fromfile(d / "xf_point.bin", np.float64).reshape(-1, 3)
+    xwo = np.fromfile(d / "xf_world_origins.bin", np.float64).reshape(-1, 3)
+    mats = np.fromfile(d / "xf_matrices.bin", np.float64).reshape(-1, 4, 4).transpose(0, 2, 1)
+    nx = xface.shape[0] // mats.shape[0]
+    assert nx * mats.shape[0] >= 10000
+    bad = ties = hits = 0
+    for ti in range(mats.shape[0]):
+        sl = slice(ti * nx, (ti + 1) * nx)
+        if np.all(np.isinf(win[sl, 1])) and np.all(win[sl, 0] == 0):
+            ref = om.raycast_first(lo3[sl], ld3[sl])
+        else:  # the window differs per ray (near / far over each ray's scale factor)
+            parts = [om.raycast_first(lo3[sl][i:i + 1], ld3[sl][i:i + 1], near=win[sl][i, 0], far=win[sl][i, 1])
+                     for i in range(nx)]
+            ref = {k: np.concatenate([p[k] for p in parts]) for k in ("face", "distance", "point")}
+        gf = xface[sl]
+        assert np.array_equal(gf == M.NO_HIT, ref["face"] == O.NO_HIT), f"transform {ti}: hit / miss differs"
+        for i in np.nonzero((gf != ref["face"]) & (gf != M.NO_HIT))[0]:
+            dplane = om.ray_plane_distance(int(gf[i]), lo3[sl][i], ld3[sl][i])
+            if abs(dplane - ref["distance"][i]) <= 1e-6 * max(ref["distance"][i], 1.0):
+                ties += 1
+            else:
+                bad += 1
+        h = gf != M.NO_HIT
+        hits += int(h.sum())
+        # the oracle's local hit point to world space, and its distance from the world origin
+        pw = ref["point"][h] @ mats[ti, :3, :3].T + mats[ti, :3, 3]
+        dw = np.linalg.norm(pw - xwo[sl][h], axis=1)
+        assert np.all(np.abs(xdist[sl][h] - dw) <= 1e-5 * np.maximum(dw, 1)), f"transform {ti}: world distance"
+        assert np.all(np.abs(xpoint[sl][h] - pw) <= 1e-5 * np.maximum(np.abs(pw), 1)), f"transform {ti}: world point"
+    print("js acceleratedRaycast under transforms:", {"rays": int(xface.shape[0]), "hits": hits, "ties": ties})
+    assert bad == 0 and hits > 3000
+
+    # BatchedMesh firstHitOnly (one launch over the visible instances): the GPU's first hit after the
+    # distance sort against the nearest of the FP64 oracle's per-instance hits on the reference's local
+    # rays (equal world distances: the lower instance, as the stable sort keeps it)
+    visible = np.fromfile(d / "bm_visible.bin", np.uint32)
+    bo = np.fromfile(d / "bm_origins.bin", np.float64).reshape(len(visible), -1, 3)
+    bd = np.fromfile(d / "bm_dirs.bin", np.float64).reshape(len(visible), -1, 3)
+    bw = np.fromfile(d / "bm_world_origins.bin", np.float64).reshape(-1, 3)
+    bm = np.fromfile(d / "bm_matrices.bin", np.float64).reshape(-1, 4, 4).transpose(0, 2, 1)
+    bface = np.fromfile(d / "bm_face.bin", np.uint32)
+    binst = np.fromfile(d / "bm_inst.bin", np.uint32)
+    bdist = np.fromfile(d / "bm_dist.bin", np.float64)
+    nb = bface.shape[0]
+    best_d = np.full(nb, np.inf)
+    best_f = np.full(nb, O.NO_HIT, np.uint32)
+    best_i = np.full(nb, O.NO_HIT, np.uint32)
+    per = []
+    for v in range(len(visible)):
+        ref = om.raycast_first(bo[v], bd[v])
+        per.append(ref)
+        h = ref["face"] != O.NO_HIT
+        pw = ref["point"] @ bm[v, :3, :3].T + bm[v, :3, 3]
+        dw = np.where(h, np.linalg.norm(pw - bw, axis=1), np.inf)
+        better = dw < best_d
+        best_d[better], best_f[better], best_i[better] = dw[better], ref["face"][better], visible[v]
+    assert np.array_equal(bface == M.NO_HIT, best_f == O.NO_HIT), "BatchedMesh hit / miss differs"
+    bbad = bties = 0
+    for i in np.nonzero((bface != best_f) | (binst != best_i))[0]:
+        v = int(np.nonzero(visible == binst[i])[0][0])
+        dplane = om.ray_plane_distance(int(bface[i]), bo[v][i], bd[v][i])
+        pw = bo[v][i] + dplane * bd[v][i] / np.linalg.norm(bd[v][i])
+        dw = np.linalg.norm(pw @ bm[v, :3, :3].T + bm[v, :3, 3] - bw[i])
+        if abs(dw - best_d[i]) <= 1e-6 * max(best_d[i], 1.0):
+            bties += 1
+        else:
+            bbad += 1
+    h = bface != M.NO_HIT
+    assert np.all(np.abs(bdist[h] - best_d[h]) <= 1e-5 * np.maximum(best_d[h], 1))
+    print("js BatchedMesh firstHitOnly:", {"rays": nb, "hits": int(h.sum()), "ties": bties})
+    assert bbad == 0 and h.sum() > 500
+
     # all hits, DoubleSide, in the reference's traversal order
     offs = np.fromfile(d / "ra_offsets.bin", np.float64).astype(np.int64)
     faces = np.fromfile(d / "ra_face.bin", np.uint32)

@@ -44,6 +44,7 @@ struct mbvh_ctx {
   DeviceBuffer grad_ws;         // host-API backward workspace
   DeviceBuffer build_ws;        // device build workspace
   DeviceBuffer geo_ws;          // closestPointToGeometry workspace (per other triangle + answer)
+  DeviceBuffer lo_stage[3];     // host-API lo parts of double-precision inputs (origins / points, directions, windows)
   uint64_t hits_total = 0;
   std::mutex lock;              // host entry points on one context are serialised
 };
@@ -105,6 +106,7 @@ struct mbvh_mesh {
   mbvh_ctx* ctx = nullptr;
   WideNode* d_nodes = nullptr;
   uint32_t* d_roots = nullptr;
+  uint2* d_root_slots = nullptr;    // slot range (first, end) of each root
   uint2* d_bigleaf = nullptr;
   float4* d_tris = nullptr;
   uint32_t* d_vidx = nullptr;
@@ -274,6 +276,46 @@ int convert_root(const NodeView& v, uint64_t n_slots, std::vector<WideNode>& wid
   return MBVH_OK;
 }
 
+// Slot ranges: the reference partitions a node's triangles into [left | right] and lays leaves out
+// depth-first, so each subtree covers a contiguous run of slots. Stores in meta.w of every wide node
+// the first slot of its right subtree, and returns each root's slot range (first, end). Wide nodes
+// are in preorder (children after parents), so one backward pass sees children first.
+void assign_split_slots(std::vector<WideNode>& wide, const std::vector<uint2>& bigleaf,
+                        const std::vector<uint32_t>& root_refs, std::vector<uint2>& root_slots) {
+  auto leaf = [&](uint32_t ref, uint32_t& off, uint32_t& cnt) {
+    const uint32_t c = (ref >> kRefCountShift) & kRefCountMask;
+    if (c) {
+      off = ref & kRefOffsetMask;
+      cnt = c;
+    } else {
+      off = bigleaf[ref & kRefOffsetMask].x;
+      cnt = bigleaf[ref & kRefOffsetMask].y;
+    }
+  };
+  std::vector<uint32_t> first(wide.size()), end(wide.size());
+  auto range = [&](uint32_t ref, uint32_t& lo, uint32_t& hi) {
+    if (ref & kRefLeaf) {
+      uint32_t off, cnt;
+      leaf(ref, off, cnt);
+      lo = off;
+      hi = off + cnt;
+    } else {
+      lo = first[ref];
+      hi = end[ref];
+    }
+  };
+  for (size_t w = wide.size(); w-- > 0;) {
+    uint32_t llo, lhi, rlo, rhi;
+    range(wide[w].meta.x, llo, lhi);
+    range(wide[w].meta.y, rlo, rhi);
+    wide[w].meta.w = rlo;
+    first[w] = std::min(llo, rlo);
+    end[w] = std::max(lhi, rhi);
+  }
+  root_slots.resize(root_refs.size());
+  for (size_t r = 0; r < root_refs.size(); r++) range(root_refs[r], root_slots[r].x, root_slots[r].y);
+}
+
 // 4-wide tree from the wide binary nodes: a 4-wide node holds the children of a binary node's
 // children (a child that is a leaf stays a child), so it replaces two binary levels. Child k of
 // node n is nodes4[8n + 2k] = {min.x, min.y, min.z, max.x}, nodes4[8n + 2k + 1] = {max.y, max.z,
@@ -366,6 +408,7 @@ uint64_t read_u(const void* p, int elt_bytes, uint64_t i) {
 void free_mesh_buffers(mbvh_mesh* m) {
   if (m->d_nodes) (void)hipFree(m->d_nodes);
   if (m->d_roots) (void)hipFree(m->d_roots);
+  if (m->d_root_slots) (void)hipFree(m->d_root_slots);
   if (m->d_bigleaf) (void)hipFree(m->d_bigleaf);
   if (m->d_tris) (void)hipFree(m->d_tris);
   if (m->d_vidx) (void)hipFree(m->d_vidx);
@@ -399,6 +442,7 @@ MeshView view_of(const mbvh_mesh* m) {
   MeshView v;
   v.nodes = m->d_nodes;
   v.roots = m->d_roots;
+  v.root_slots = m->d_root_slots;
   v.nodes4 = m->d_nodes4;
   v.roots4 = m->d_roots4;
   v.max_stack4 = m->max_stack4;
@@ -508,6 +552,8 @@ extern "C" void mbvh_ctx_destroy(mbvh_ctx* ctx) {
   if (ctx->order.ptr) (void)hipFree(ctx->order.ptr);
   for (auto& b : ctx->grad)
     if (b.ptr) (void)hipFree(b.ptr);
+  for (auto& b : ctx->lo_stage)
+    if (b.ptr) (void)hipFree(b.ptr);
   if (ctx->grad_ws.ptr) (void)hipFree(ctx->grad_ws.ptr);
   if (ctx->order_ws.ptr) (void)hipFree(ctx->order_ws.ptr);
   if (ctx->build_ws.ptr) (void)hipFree(ctx->build_ws.ptr);
@@ -588,6 +634,8 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
     maps.base += v.n_nodes;
   }
   if (bigleaf.size() > kRefOffsetMask) return set_error(MBVH_ERR_LIMIT, "mbvh_mesh_upload: too many large leaves");
+  std::vector<uint2> root_slots;
+  assign_split_slots(wide, bigleaf, root_refs, root_slots);
   std::vector<float4> nodes4;
   std::vector<uint32_t> roots4(n_roots);
   uint32_t depth4 = 0;
@@ -678,6 +726,7 @@ extern "C" int mbvh_mesh_upload(mbvh_ctx* ctx, const void* const* roots, const u
   };
   rc = upload(reinterpret_cast<void**>(&m->d_nodes), wide.data(), wide.size() * sizeof(WideNode));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_roots), root_refs.data(), root_refs.size() * sizeof(uint32_t));
+  if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_root_slots), root_slots.data(), root_slots.size() * sizeof(uint2));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_bigleaf), bigleaf.data(), bigleaf.size() * sizeof(uint2));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_tris), tris.data(), tris.size() * sizeof(float4));
   if (!rc) rc = upload(reinterpret_cast<void**>(&m->d_vidx), vidx.data(), vidx.size() * sizeof(uint32_t));
@@ -879,9 +928,29 @@ extern "C" int mbvh_mesh_copy_roots(mbvh_ctx* ctx, const mbvh_mesh* mesh, void* 
 
 namespace {
 
+// Double-precision inputs travel as float pairs hi = fl(x), lo = fl(x - hi): the kernels trace the hi
+// parts with error bounds widened by the lo parts and decide in double-float with both. These are
+// the lo parts (device arrays of origins / points and directions, and the lo parts of near / far or
+// of the thresholds); all NULL / 0 for FP32 inputs.
+struct LoInputs {
+  const float* a = nullptr;
+  const float* b = nullptr;
+  float near_lo = 0.0f, far_lo = 0.0f;
+};
+
+void split_f64(double x, float& hi, float& lo) {
+  hi = static_cast<float>(x);
+  lo = std::isfinite(x) && std::isfinite(hi) ? static_cast<float>(x - static_cast<double>(hi)) : 0.0f;
+}
+void split_f64(const double* x, size_t n, std::vector<float>& hi, std::vector<float>& lo) {
+  hi.resize(n);
+  lo.resize(n);
+  for (size_t i = 0; i < n; i++) split_f64(x[i], hi[i], lo[i]);
+}
+
 int raycast_first_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o, const float* d,
                        uint64_t n, const uint32_t* order, int32_t side, float near_dist, float far_dist, uint32_t* face,
-                       float* dist, float* point, float* bary, hipStream_t s) {
+                       float* dist, float* point, float* bary, hipStream_t s, const LoInputs& lo = LoInputs{}) {
   if (side < MBVH_FRONT_SIDE || side > MBVH_SIDE_PER_TRIANGLE)
     return set_error(MBVH_ERR_BAD_ARGS, "raycast_first: invalid side %d", side);
   if (n && (!o || !d || !face)) return set_error(MBVH_ERR_BAD_ARGS, "raycast_first: NULL ray or face buffer");
@@ -895,16 +964,20 @@ int raycast_first_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o, con
     return set_error(MBVH_ERR_LIMIT, "raycast_first: stack scratch smaller than the tree depth");
   }
   const uint64_t threads = quads ? launch_threads(4 * n, ctx->max_threads) : launch_threads(n, ctx->max_threads);
-  RaycastFirstArgs a;
+  RaycastFirstArgs a{};
   a.mesh = view_of(mesh);
   a.stack = stack_of(ctx, quads ? threads / 4 : threads);
   a.origins = o;
   a.directions = d;
+  a.origins_lo = lo.a;
+  a.directions_lo = lo.b;
   a.order = order;
   a.n = n;
   a.side_mask = side_mask_of(side);
   a.near_dist = near_dist;
   a.far_dist = far_dist;
+  a.near_lo = lo.near_lo;
+  a.far_lo = lo.far_lo;
   a.out_face = face;
   a.out_dist = dist;
   a.out_point = point;
@@ -928,12 +1001,12 @@ struct FixedScratch {
 int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uint64_t n,
                        const uint32_t* order, float min_t,
                        float max_t, uint32_t* face, float* dist, float* point, float* bary, hipStream_t s,
-                       const FixedScratch* fixed = nullptr) {
+                       const FixedScratch* fixed = nullptr, const LoInputs& lo = LoInputs{}) {
   if (n && (!q || !face)) return set_error(MBVH_ERR_BAD_ARGS, "closest_point: NULL point or face buffer");
   // four lanes per query only without a minThreshold (its early exit follows the binary order)
   const bool quads = min_t == 0.0f && (ctx->point_lanes == 4 || ctx->point_lanes == 0);
   uint64_t threads = quads ? launch_threads(4 * n, ctx->max_threads) : launch_threads(n, ctx->max_threads);
-  ClosestPointArgs a;
+  ClosestPointArgs a{};
   if (fixed) {
     const uint32_t need = quads ? (mesh->max_stack4 > kQuadLdsStack ? mesh->max_stack4 - kQuadLdsStack : 0)
                                 : (mesh->max_stack > kStackLds ? mesh->max_stack - kStackLds : 0);
@@ -959,6 +1032,9 @@ int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uin
   }
   a.mesh = view_of(mesh);
   a.points = q;
+  a.points_lo = lo.a;
+  a.min_lo = lo.near_lo;
+  a.max_lo = lo.far_lo;
   a.order = order;
   a.n = n;
   a.min_threshold = min_t;
@@ -1015,18 +1091,16 @@ extern "C" int mbvh_raycast_first_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const 
                             distance, point, barycoord, s);
 }
 
-extern "C" int mbvh_raycast_first(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins,
-                                  const float* directions, uint64_t n, int32_t side, float near_dist,
-                                  float far_dist, uint32_t* face_index, float* distance, float* point,
-                                  float* barycoord) {
-  clear_error();
-  int rc = check_query("mbvh_raycast_first", ctx, mesh);
-  if (rc) return rc;
-  if (n == 0) return MBVH_OK;
-  if (!origins || !directions || !face_index)
-    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_raycast_first: NULL ray or face buffer");
+namespace {
+
+// Host-pointer raycastFirst: stages the rays (and the lo parts of double rays), orders them, launches,
+// copies the results back and synchronises.
+int host_raycast_first(const char* fn, mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins,
+                       const float* directions, const float* origins_lo, const float* directions_lo, uint64_t n,
+                       int32_t side, float near_dist, float far_dist, float near_lo, float far_lo,
+                       uint32_t* face_index, float* distance, float* point, float* barycoord) {
   std::lock_guard<std::mutex> guard(ctx->lock);
-  rc = set_device(ctx);
+  int rc = set_device(ctx);
   if (rc) return rc;
   const size_t v3 = n * 3 * sizeof(float);
   if ((rc = ensure_buffer(ctx->staging[0], v3)) || (rc = ensure_buffer(ctx->staging[1], v3)) ||
@@ -1037,19 +1111,92 @@ extern "C" int mbvh_raycast_first(mbvh_ctx* ctx, const mbvh_mesh* mesh, const fl
   hipStream_t s = ctx->stream;
   MBVH_HIP(hipMemcpyAsync(ctx->staging[0].ptr, origins, v3, hipMemcpyHostToDevice, s));
   MBVH_HIP(hipMemcpyAsync(ctx->staging[1].ptr, directions, v3, hipMemcpyHostToDevice, s));
+  LoInputs lo;
+  lo.near_lo = near_lo;
+  lo.far_lo = far_lo;
+  if (origins_lo) {
+    if ((rc = ensure_buffer(ctx->lo_stage[0], v3)) || (rc = ensure_buffer(ctx->lo_stage[1], v3))) return rc;
+    MBVH_HIP(hipMemcpyAsync(ctx->lo_stage[0].ptr, origins_lo, v3, hipMemcpyHostToDevice, s));
+    MBVH_HIP(hipMemcpyAsync(ctx->lo_stage[1].ptr, directions_lo, v3, hipMemcpyHostToDevice, s));
+    lo.a = static_cast<const float*>(ctx->lo_stage[0].ptr);
+    lo.b = static_cast<const float*>(ctx->lo_stage[1].ptr);
+  }
   const uint32_t* order = nullptr;
   if ((rc = host_order(ctx, static_cast<float*>(ctx->staging[0].ptr), n, &order))) return rc;
   rc = raycast_first_impl(ctx, mesh, static_cast<float*>(ctx->staging[0].ptr),
                           static_cast<float*>(ctx->staging[1].ptr), n, order, side, near_dist, far_dist,
                           static_cast<uint32_t*>(ctx->staging[2].ptr), static_cast<float*>(ctx->staging[3].ptr),
-                          static_cast<float*>(ctx->staging[4].ptr), static_cast<float*>(ctx->staging[5].ptr), s);
+                          static_cast<float*>(ctx->staging[4].ptr), static_cast<float*>(ctx->staging[5].ptr), s, lo);
   if (rc) return rc;
   MBVH_HIP(hipMemcpyAsync(face_index, ctx->staging[2].ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   if (distance) MBVH_HIP(hipMemcpyAsync(distance, ctx->staging[3].ptr, n * sizeof(float), hipMemcpyDeviceToHost, s));
   if (point) MBVH_HIP(hipMemcpyAsync(point, ctx->staging[4].ptr, v3, hipMemcpyDeviceToHost, s));
   if (barycoord) MBVH_HIP(hipMemcpyAsync(barycoord, ctx->staging[5].ptr, v3, hipMemcpyDeviceToHost, s));
   MBVH_HIP(hipStreamSynchronize(s));
-  return check_no_overflow(face_index, n, "mbvh_raycast_first");
+  return check_no_overflow(face_index, n, fn);
+}
+
+}  // namespace
+
+extern "C" int mbvh_raycast_first(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins,
+                                  const float* directions, uint64_t n, int32_t side, float near_dist,
+                                  float far_dist, uint32_t* face_index, float* distance, float* point,
+                                  float* barycoord) {
+  clear_error();
+  int rc = check_query("mbvh_raycast_first", ctx, mesh);
+  if (rc) return rc;
+  if (n == 0) return MBVH_OK;
+  if (!origins || !directions || !face_index)
+    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_raycast_first: NULL ray or face buffer");
+  return host_raycast_first("mbvh_raycast_first", ctx, mesh, origins, directions, nullptr, nullptr, n, side, near_dist,
+                            far_dist, 0.0f, 0.0f, face_index, distance, point, barycoord);
+}
+
+extern "C" int mbvh_raycast_first_f64(mbvh_ctx* ctx, const mbvh_mesh* mesh, const double* origins,
+                                      const double* directions, uint64_t n, int32_t side, double near_dist,
+                                      double far_dist, uint32_t* face_index, float* distance, float* point,
+                                      float* barycoord) {
+  clear_error();
+  const char* fn = "mbvh_raycast_first_f64";
+  int rc = check_query(fn, ctx, mesh);
+  if (rc) return rc;
+  if (n == 0) return MBVH_OK;
+  if (!origins || !directions || !face_index) return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL ray or face buffer", fn);
+  std::vector<float> oh, ol, dh, dl;
+  try {
+    split_f64(origins, 3 * n, oh, ol);
+    split_f64(directions, 3 * n, dh, dl);
+  } catch (...) {
+    return set_error(MBVH_ERR_OOM, "%s: out of host memory", fn);
+  }
+  float nh, nl, fh, fl;
+  split_f64(near_dist, nh, nl);
+  split_f64(far_dist, fh, fl);
+  return host_raycast_first(fn, ctx, mesh, oh.data(), dh.data(), ol.data(), dl.data(), n, side, nh, fh, nl, fl,
+                            face_index, distance, point, barycoord);
+}
+
+extern "C" int mbvh_raycast_first_ff_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins,
+                                       const float* origins_lo, const float* directions, const float* directions_lo,
+                                       uint64_t n, const uint32_t* order, int32_t side, double near_dist,
+                                       double far_dist, uint32_t* face_index, float* distance, float* point,
+                                       float* barycoord, void* stream) {
+  clear_error();
+  int rc = check_query("mbvh_raycast_first_ff_d", ctx, mesh);
+  if (rc) return rc;
+  if (n && (!origins_lo || !directions_lo))
+    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_raycast_first_ff_d: NULL lo parts");
+  rc = set_device(ctx);
+  if (rc) return rc;
+  LoInputs lo;
+  lo.a = origins_lo;
+  lo.b = directions_lo;
+  float nh, fh;
+  split_f64(near_dist, nh, lo.near_lo);
+  split_f64(far_dist, fh, lo.far_lo);
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return raycast_first_impl(ctx, mesh, origins, directions, n, order, side, nh, fh, face_index, distance, point,
+                            barycoord, s, lo);
 }
 
 extern "C" int mbvh_closest_point_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
@@ -1120,16 +1267,15 @@ extern "C" int mbvh_closest_point_noalloc_d(mbvh_ctx* ctx, const mbvh_mesh* mesh
                             closest, barycoord, s, &fx);
 }
 
-extern "C" int mbvh_closest_point(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
-                                  float min_threshold, float max_threshold, uint32_t* face_index,
-                                  float* distance, float* closest, float* barycoord) {
-  clear_error();
-  int rc = check_query("mbvh_closest_point", ctx, mesh);
-  if (rc) return rc;
-  if (n == 0) return MBVH_OK;
-  if (!points || !face_index) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_closest_point: NULL point or face buffer");
+namespace {
+
+// Host-pointer closest point: stages the points (and the lo parts of double points), orders them,
+// launches, copies the results back and synchronises.
+int host_closest_point(const char* fn, mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, const float* points_lo,
+                       uint64_t n, float min_threshold, float max_threshold, float min_lo, float max_lo,
+                       uint32_t* face_index, float* distance, float* closest, float* barycoord) {
   std::lock_guard<std::mutex> guard(ctx->lock);
-  rc = set_device(ctx);
+  int rc = set_device(ctx);
   if (rc) return rc;
   const size_t v3 = n * 3 * sizeof(float);
   if ((rc = ensure_buffer(ctx->staging[0], v3)) || (rc = ensure_buffer(ctx->staging[2], n * sizeof(uint32_t))) ||
@@ -1138,18 +1284,83 @@ extern "C" int mbvh_closest_point(mbvh_ctx* ctx, const mbvh_mesh* mesh, const fl
     return rc;
   hipStream_t s = ctx->stream;
   MBVH_HIP(hipMemcpyAsync(ctx->staging[0].ptr, points, v3, hipMemcpyHostToDevice, s));
+  LoInputs lo;
+  lo.near_lo = min_lo;
+  lo.far_lo = max_lo;
+  if (points_lo) {
+    if ((rc = ensure_buffer(ctx->lo_stage[0], v3))) return rc;
+    MBVH_HIP(hipMemcpyAsync(ctx->lo_stage[0].ptr, points_lo, v3, hipMemcpyHostToDevice, s));
+    lo.a = static_cast<const float*>(ctx->lo_stage[0].ptr);
+  }
   const uint32_t* order = nullptr;
   if ((rc = host_order(ctx, static_cast<float*>(ctx->staging[0].ptr), n, &order))) return rc;
   rc = closest_point_impl(ctx, mesh, static_cast<float*>(ctx->staging[0].ptr), n, order, min_threshold, max_threshold,
                           static_cast<uint32_t*>(ctx->staging[2].ptr), static_cast<float*>(ctx->staging[3].ptr),
-                          static_cast<float*>(ctx->staging[4].ptr), static_cast<float*>(ctx->staging[5].ptr), s);
+                          static_cast<float*>(ctx->staging[4].ptr), static_cast<float*>(ctx->staging[5].ptr), s,
+                          nullptr, lo);
   if (rc) return rc;
   MBVH_HIP(hipMemcpyAsync(face_index, ctx->staging[2].ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   if (distance) MBVH_HIP(hipMemcpyAsync(distance, ctx->staging[3].ptr, n * sizeof(float), hipMemcpyDeviceToHost, s));
   if (closest) MBVH_HIP(hipMemcpyAsync(closest, ctx->staging[4].ptr, v3, hipMemcpyDeviceToHost, s));
   if (barycoord) MBVH_HIP(hipMemcpyAsync(barycoord, ctx->staging[5].ptr, v3, hipMemcpyDeviceToHost, s));
   MBVH_HIP(hipStreamSynchronize(s));
-  return check_no_overflow(face_index, n, "mbvh_closest_point");
+  return check_no_overflow(face_index, n, fn);
+}
+
+}  // namespace
+
+extern "C" int mbvh_closest_point(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, uint64_t n,
+                                  float min_threshold, float max_threshold, uint32_t* face_index,
+                                  float* distance, float* closest, float* barycoord) {
+  clear_error();
+  int rc = check_query("mbvh_closest_point", ctx, mesh);
+  if (rc) return rc;
+  if (n == 0) return MBVH_OK;
+  if (!points || !face_index) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_closest_point: NULL point or face buffer");
+  return host_closest_point("mbvh_closest_point", ctx, mesh, points, nullptr, n, min_threshold, max_threshold, 0.0f,
+                            0.0f, face_index, distance, closest, barycoord);
+}
+
+extern "C" int mbvh_closest_point_f64(mbvh_ctx* ctx, const mbvh_mesh* mesh, const double* points, uint64_t n,
+                                      double min_threshold, double max_threshold, uint32_t* face_index,
+                                      float* distance, float* closest, float* barycoord) {
+  clear_error();
+  const char* fn = "mbvh_closest_point_f64";
+  int rc = check_query(fn, ctx, mesh);
+  if (rc) return rc;
+  if (n == 0) return MBVH_OK;
+  if (!points || !face_index) return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL point or face buffer", fn);
+  std::vector<float> ph, pl;
+  try {
+    split_f64(points, 3 * n, ph, pl);
+  } catch (...) {
+    return set_error(MBVH_ERR_OOM, "%s: out of host memory", fn);
+  }
+  float mh, ml, xh, xl;
+  split_f64(min_threshold, mh, ml);
+  split_f64(max_threshold, xh, xl);
+  return host_closest_point(fn, ctx, mesh, ph.data(), pl.data(), n, mh, xh, ml, xl, face_index, distance, closest,
+                            barycoord);
+}
+
+extern "C" int mbvh_closest_point_ff_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* points, const float* points_lo,
+                                       uint64_t n, const uint32_t* order, double min_threshold, double max_threshold,
+                                       uint32_t* face_index, float* distance, float* closest, float* barycoord,
+                                       void* stream) {
+  clear_error();
+  int rc = check_query("mbvh_closest_point_ff_d", ctx, mesh);
+  if (rc) return rc;
+  if (n && !points_lo) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_closest_point_ff_d: NULL lo parts");
+  rc = set_device(ctx);
+  if (rc) return rc;
+  LoInputs lo;
+  lo.a = points_lo;
+  float mh, xh;
+  split_f64(min_threshold, mh, lo.near_lo);
+  split_f64(max_threshold, xh, lo.far_lo);
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  return closest_point_impl(ctx, mesh, points, n, order, mh, xh, face_index, distance, closest, barycoord, s, nullptr,
+                            lo);
 }
 
 extern "C" int mbvh_intersects_sphere(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* centers, const float* radii,
@@ -1184,7 +1395,7 @@ extern "C" int mbvh_intersects_sphere(mbvh_ctx* ctx, const mbvh_mesh* mesh, cons
   if ((rc = closest_point_impl(ctx, mesh, static_cast<float*>(ctx->staging[0].ptr), n, order, 0.0f, max_t,
                                static_cast<uint32_t*>(ctx->staging[2].ptr), nullptr, nullptr, nullptr, s)))
     return rc;
-  SphereArgs a;
+  SphereArgs a{};
   a.mesh = grad_view_of(mesh);
   a.centers = static_cast<const float*>(ctx->staging[0].ptr);
   a.radii = static_cast<const float*>(ctx->staging[1].ptr);
@@ -1209,7 +1420,7 @@ namespace {
 int raycast_all_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o, const float* d, uint64_t n,
                      int32_t side, float near_dist, float far_dist, uint32_t* counts, const uint64_t* offsets,
                      uint64_t capacity, uint32_t* face, float* dist, float* point, float* bary, bool fill,
-                     hipStream_t s) {
+                     hipStream_t s, const LoInputs& lo = LoInputs{}) {
   if (side < MBVH_FRONT_SIDE || side > MBVH_SIDE_PER_TRIANGLE)
     return set_error(MBVH_ERR_BAD_ARGS, "raycast: invalid side %d", side);
   if (n && (!o || !d)) return set_error(MBVH_ERR_BAD_ARGS, "raycast: NULL ray buffer");
@@ -1219,15 +1430,19 @@ int raycast_all_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o, const
   if (mesh->max_stack > kStackLds + ctx->stack_ovf_entries)
     return set_error(MBVH_ERR_LIMIT, "raycast: stack scratch smaller than the tree depth");
   const uint64_t threads = launch_threads(n, ctx->max_threads);
-  RaycastAllArgs a;
+  RaycastAllArgs a{};
   a.mesh = view_of(mesh);
   a.stack = stack_of(ctx, threads);
   a.origins = o;
   a.directions = d;
+  a.origins_lo = lo.a;
+  a.directions_lo = lo.b;
   a.n = n;
   a.side_mask = side_mask_of(side);
   a.near_dist = near_dist;
   a.far_dist = far_dist;
+  a.near_lo = lo.near_lo;
+  a.far_lo = lo.far_lo;
   a.counts = counts;
   a.offsets = offsets;
   a.out_face = face;
@@ -1269,18 +1484,18 @@ extern "C" int mbvh_raycast_fill_d(mbvh_ctx* ctx, const mbvh_mesh* mesh, const f
                           capacity, face_index, distance, point, barycoord, true, s);
 }
 
-extern "C" int mbvh_raycast(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins, const float* directions,
-                            uint64_t n, int32_t side, float near_dist, float far_dist, uint64_t* offsets) {
-  clear_error();
-  int rc = check_query("mbvh_raycast", ctx, mesh);
-  if (rc) return rc;
-  if (!offsets) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_raycast: offsets is NULL");
+namespace {
+
+// Host-pointer all-hits raycast: a count pass, the offsets on the host, a fill pass into the
+// context's hit buffers (mbvh_raycast_fetch copies them out).
+int host_raycast_all(const char* fn, mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins, const float* directions,
+                     const float* origins_lo, const float* directions_lo, uint64_t n, int32_t side, float near_dist,
+                     float far_dist, float near_lo, float far_lo, uint64_t* offsets) {
   std::lock_guard<std::mutex> guard(ctx->lock);
   ctx->hits_total = 0;
   offsets[0] = 0;
   if (n == 0) return MBVH_OK;
-  if (!origins || !directions) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_raycast: NULL ray buffer");
-  rc = set_device(ctx);
+  int rc = set_device(ctx);
   if (rc) return rc;
   const size_t v3 = n * 3 * sizeof(float);
   if ((rc = ensure_buffer(ctx->staging[0], v3)) || (rc = ensure_buffer(ctx->staging[1], v3)) ||
@@ -1292,8 +1507,18 @@ extern "C" int mbvh_raycast(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o
   const float* dD = static_cast<float*>(ctx->staging[1].ptr);
   MBVH_HIP(hipMemcpyAsync(ctx->staging[0].ptr, origins, v3, hipMemcpyHostToDevice, s));
   MBVH_HIP(hipMemcpyAsync(ctx->staging[1].ptr, directions, v3, hipMemcpyHostToDevice, s));
+  LoInputs lo;
+  lo.near_lo = near_lo;
+  lo.far_lo = far_lo;
+  if (origins_lo) {
+    if ((rc = ensure_buffer(ctx->lo_stage[0], v3)) || (rc = ensure_buffer(ctx->lo_stage[1], v3))) return rc;
+    MBVH_HIP(hipMemcpyAsync(ctx->lo_stage[0].ptr, origins_lo, v3, hipMemcpyHostToDevice, s));
+    MBVH_HIP(hipMemcpyAsync(ctx->lo_stage[1].ptr, directions_lo, v3, hipMemcpyHostToDevice, s));
+    lo.a = static_cast<const float*>(ctx->lo_stage[0].ptr);
+    lo.b = static_cast<const float*>(ctx->lo_stage[1].ptr);
+  }
   rc = raycast_all_impl(ctx, mesh, dO, dD, n, side, near_dist, far_dist, static_cast<uint32_t*>(ctx->staging[2].ptr),
-                        nullptr, 0, nullptr, nullptr, nullptr, nullptr, false, s);
+                        nullptr, 0, nullptr, nullptr, nullptr, nullptr, false, s, lo);
   if (rc) return rc;
   std::vector<uint32_t> counts(n);
   MBVH_HIP(hipMemcpyAsync(counts.data(), ctx->staging[2].ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1301,8 +1526,7 @@ extern "C" int mbvh_raycast(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o
   uint64_t total = 0;
   for (uint64_t i = 0; i < n; i++) {
     if (counts[i] == MBVH_NO_HIT - 1u)
-      return set_error(MBVH_ERR_LIMIT, "mbvh_raycast: traversal stack overflow on ray %llu",
-                       static_cast<unsigned long long>(i));
+      return set_error(MBVH_ERR_LIMIT, "%s: traversal stack overflow on ray %llu", fn, static_cast<unsigned long long>(i));
     offsets[i] = total;
     total += counts[i];
   }
@@ -1315,11 +1539,45 @@ extern "C" int mbvh_raycast(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* o
   rc = raycast_all_impl(ctx, mesh, dO, dD, n, side, near_dist, far_dist, nullptr,
                         static_cast<const uint64_t*>(ctx->hits[0].ptr), total, static_cast<uint32_t*>(ctx->hits[1].ptr),
                         static_cast<float*>(ctx->hits[2].ptr), static_cast<float*>(ctx->hits[3].ptr),
-                        static_cast<float*>(ctx->hits[4].ptr), true, s);
+                        static_cast<float*>(ctx->hits[4].ptr), true, s, lo);
   if (rc) return rc;
   MBVH_HIP(hipStreamSynchronize(s));
   ctx->hits_total = total;
   return MBVH_OK;
+}
+
+}  // namespace
+
+extern "C" int mbvh_raycast(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* origins, const float* directions,
+                            uint64_t n, int32_t side, float near_dist, float far_dist, uint64_t* offsets) {
+  clear_error();
+  int rc = check_query("mbvh_raycast", ctx, mesh);
+  if (rc) return rc;
+  if (!offsets) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_raycast: offsets is NULL");
+  if (n && (!origins || !directions)) return set_error(MBVH_ERR_BAD_ARGS, "mbvh_raycast: NULL ray buffer");
+  return host_raycast_all("mbvh_raycast", ctx, mesh, origins, directions, nullptr, nullptr, n, side, near_dist,
+                          far_dist, 0.0f, 0.0f, offsets);
+}
+
+extern "C" int mbvh_raycast_f64(mbvh_ctx* ctx, const mbvh_mesh* mesh, const double* origins, const double* directions,
+                                uint64_t n, int32_t side, double near_dist, double far_dist, uint64_t* offsets) {
+  clear_error();
+  const char* fn = "mbvh_raycast_f64";
+  int rc = check_query(fn, ctx, mesh);
+  if (rc) return rc;
+  if (!offsets) return set_error(MBVH_ERR_BAD_ARGS, "%s: offsets is NULL", fn);
+  if (n && (!origins || !directions)) return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL ray buffer", fn);
+  std::vector<float> oh, ol, dh, dl;
+  try {
+    split_f64(origins, 3 * n, oh, ol);
+    split_f64(directions, 3 * n, dh, dl);
+  } catch (...) {
+    return set_error(MBVH_ERR_OOM, "%s: out of host memory", fn);
+  }
+  float nh, nl, fh, fl;
+  split_f64(near_dist, nh, nl);
+  split_f64(far_dist, fh, fl);
+  return host_raycast_all(fn, ctx, mesh, oh.data(), dh.data(), ol.data(), dl.data(), n, side, nh, fh, nl, fl, offsets);
 }
 
 extern "C" int mbvh_raycast_fetch(mbvh_ctx* ctx, uint32_t* face_index, float* distance, float* point,
@@ -1626,6 +1884,11 @@ struct BatchCall {
   float* bary;
   void* ws;
   uint64_t ws_bytes;
+  // local mode (windows != NULL): rays in their mesh's frame as hi + lo pairs, ray_inst naming the mesh,
+  // a window (near, near lo, far, far lo) per ray; no matrices or instance table
+  const float* origins_lo;
+  const float* dirs_lo;
+  const float* windows;
 };
 
 int check_batch_call(const char* fn, const mbvh_ctx* ctx, const mbvh_batch* b, const BatchCall& c) {
@@ -1635,7 +1898,9 @@ int check_batch_call(const char* fn, const mbvh_ctx* ctx, const mbvh_batch* b, c
     return set_error(MBVH_ERR_BAD_ARGS, "%s: invalid side %d", fn, c.side);
   if (c.n >= (uint64_t{1} << 32)) return set_error(MBVH_ERR_LIMIT, "%s: n must be below 2^32", fn);
   if (c.n == 0) return MBVH_OK;
-  if (!c.world || !c.inverse || !c.inst_geom || !c.origins || !c.dirs || !c.ray_inst || !c.face)
+  const bool local = c.windows != nullptr;
+  if ((!local && (!c.world || !c.inverse || !c.inst_geom)) || !c.origins || !c.dirs || !c.ray_inst || !c.face ||
+      (local && (!c.origins_lo || !c.dirs_lo)))
     return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL matrix, instance, ray or face buffer", fn);
   const uint64_t need = batch_ws_bytes(b, c.n);
   if (!c.ws || c.ws_bytes < need)
@@ -1646,8 +1911,10 @@ int check_batch_call(const char* fn, const mbvh_ctx* ctx, const mbvh_batch* b, c
 }
 
 int enqueue_batch(mbvh_ctx* ctx, mbvh_batch* b, const BatchCall& c, const mbvh_batch_tuning& t, hipStream_t s) {
+  const bool local = c.windows != nullptr;
   const uint32_t* order = nullptr;
-  if (c.n) MBVH_HIP(launch_order_by_mesh(c.ray_inst, c.inst_geom, c.n_inst, c.n, b->n_meshes, c.ws, &order, s));
+  if (c.n)
+    MBVH_HIP(launch_order_by_mesh(c.ray_inst, local ? nullptr : c.inst_geom, c.n_inst, c.n, b->n_meshes, c.ws, &order, s));
   int device_cus = static_cast<int>(ctx->max_threads / kMaxThreadsPerCU);
   const uint64_t tiles = (c.n + t.tile_rays - 1) / t.tile_rays;
   uint64_t blocks = static_cast<uint64_t>(device_cus) * t.blocks_per_cu;
@@ -1658,9 +1925,12 @@ int enqueue_batch(mbvh_ctx* ctx, mbvh_batch* b, const BatchCall& c, const mbvh_b
   BatchArgs a{};
   a.meshes = b->d_meshes;
   a.n_meshes = b->n_meshes;
-  a.world = c.world;
-  a.inverse = c.inverse;
-  a.instance_geometry = c.inst_geom;
+  a.world = local ? nullptr : c.world;
+  a.inverse = local ? nullptr : c.inverse;
+  a.instance_geometry = local ? nullptr : c.inst_geom;
+  a.origins_lo = c.origins_lo;
+  a.directions_lo = c.dirs_lo;
+  a.windows = c.windows;
   a.n_instances = c.n_inst;
   a.origins = c.origins;
   a.directions = c.dirs;
@@ -1807,6 +2077,60 @@ extern "C" int mbvh_batch_raycast_first(mbvh_ctx* ctx, mbvh_batch* batch, const 
   if (barycoord) MBVH_HIP(hipMemcpyAsync(barycoord, oB, v3, hipMemcpyDeviceToHost, s));
   MBVH_HIP(hipStreamSynchronize(s));
   return MBVH_OK;
+}
+
+extern "C" int mbvh_batch_raycast_first_local_f64(mbvh_ctx* ctx, mbvh_batch* batch, const double* origins,
+                                                  const double* directions, const double* windows,
+                                                  const uint32_t* ray_mesh, uint64_t n, int32_t side,
+                                                  uint32_t* face_index, float* distance, float* point,
+                                                  float* barycoord) {
+  clear_error();
+  const char* fn = "mbvh_batch_raycast_first_local_f64";
+  if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid context handle", fn);
+  if (!valid_batch(batch) || batch->ctx != ctx) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid batch handle", fn);
+  if (n == 0) return MBVH_OK;
+  if (!origins || !directions || !ray_mesh || !face_index)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL ray, mesh or face buffer", fn);
+  std::vector<float> oh, ol, dh, dl, win(4 * n);
+  try {
+    split_f64(origins, 3 * n, oh, ol);
+    split_f64(directions, 3 * n, dh, dl);
+  } catch (...) {
+    return set_error(MBVH_ERR_OOM, "%s: out of host memory", fn);
+  }
+  for (uint64_t i = 0; i < n; i++) {
+    split_f64(windows ? windows[2 * i] : 0.0, win[4 * i], win[4 * i + 1]);
+    split_f64(windows ? windows[2 * i + 1] : std::numeric_limits<double>::infinity(), win[4 * i + 2], win[4 * i + 3]);
+  }
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  const size_t v3 = n * 3 * sizeof(float);
+  void *dO, *dD, *dOl, *dDl, *dWn, *dR, *oF, *oT, *oP, *oB;
+  if ((rc = stage_in(ctx, ctx->grad[3], oh.data(), v3, &dO)) || (rc = stage_in(ctx, ctx->grad[4], dh.data(), v3, &dD)) ||
+      (rc = stage_in(ctx, ctx->lo_stage[0], ol.data(), v3, &dOl)) ||
+      (rc = stage_in(ctx, ctx->lo_stage[1], dl.data(), v3, &dDl)) ||
+      (rc = stage_in(ctx, ctx->lo_stage[2], win.data(), win.size() * sizeof(float), &dWn)) ||
+      (rc = stage_in(ctx, ctx->grad[5], ray_mesh, n * sizeof(uint32_t), &dR)) ||
+      (rc = stage_out(ctx->grad[6], face_index, n * sizeof(uint32_t), &oF)) ||
+      (rc = stage_out(ctx->grad[7], distance, n * sizeof(float), &oT)) ||
+      (rc = stage_out(ctx->grad[8], point, v3, &oP)) || (rc = stage_out(ctx->grad[9], barycoord, v3, &oB)) ||
+      (rc = ensure_buffer(ctx->grad_ws, batch_ws_bytes(batch, n))))
+    return rc;
+  BatchCall c{nullptr, nullptr, nullptr, 0, static_cast<float*>(dO), static_cast<float*>(dD),
+              static_cast<uint32_t*>(dR), n, side, 0.0f, std::numeric_limits<float>::infinity(), static_cast<uint32_t*>(oF),
+              static_cast<float*>(oT),
+              static_cast<float*>(oP), static_cast<float*>(oB), ctx->grad_ws.ptr, ctx->grad_ws.bytes,
+              static_cast<float*>(dOl), static_cast<float*>(dDl), static_cast<float*>(dWn)};
+  if ((rc = check_batch_call(fn, ctx, batch, c))) return rc;
+  if ((rc = enqueue_batch(ctx, batch, c, batch->tuning, s))) return rc;
+  MBVH_HIP(hipMemcpyAsync(face_index, oF, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (distance) MBVH_HIP(hipMemcpyAsync(distance, oT, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (point) MBVH_HIP(hipMemcpyAsync(point, oP, v3, hipMemcpyDeviceToHost, s));
+  if (barycoord) MBVH_HIP(hipMemcpyAsync(barycoord, oB, v3, hipMemcpyDeviceToHost, s));
+  MBVH_HIP(hipStreamSynchronize(s));
+  return check_no_overflow(face_index, n, fn);
 }
 
 extern "C" int mbvh_batch_autotune(mbvh_ctx* ctx, mbvh_batch* batch, const float* matrix_world,

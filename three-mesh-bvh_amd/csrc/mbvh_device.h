@@ -32,7 +32,7 @@ struct alignas(16) WideNode {
   float4 b0;  // L.min.x L.min.y L.min.z L.max.x
   float4 b1;  // L.max.y L.max.z R.min.x R.min.y
   float4 b2;  // R.min.z R.max.x R.max.y R.max.z
-  uint4 meta; // refL, refR, split axis, unused
+  uint4 meta; // refL, refR, split axis, first slot of the right subtree (the left one's slots are below it)
 };
 static_assert(sizeof(WideNode) == 64, "wide node is one 64-byte record");
 

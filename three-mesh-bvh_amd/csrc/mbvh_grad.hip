@@ -139,13 +139,13 @@ __global__ __launch_bounds__(kGradBlock) void k_closest_point_bwd(ClosestBwdArgs
       // bits to cancellation when the query is near the surface
       const float4 pa = A.mesh.tris[3ull * slot], pb = A.mesh.tris[3ull * slot + 1], pc = A.mesh.tris[3ull * slot + 2];
       const f3 q = load3(A.points, i);
-      const ClosestFF cf = closest_on_triangle_ff(q, mk(pa.x, pa.y, pa.z), mk(pb.x, pb.y, pb.z), mk(pc.x, pc.y, pc.z));
+      const ClosestFF cf = closest_on_triangle_ff<false>(q, mk(0.0f, 0.0f, 0.0f), mk(pa.x, pa.y, pa.z), mk(pb.x, pb.y, pb.z),
+                                                          mk(pc.x, pc.y, pc.z));
       const ff D = ff_sqrt(cf.dist2);
       if (D.hi > 0.0f) {
         const float g = A.grad_distance[i];
-        const f3 nrm = mk(ff_to_float(ff_div(ff_sub(ff_from(q.x), cf.point.x), D)),
-                          ff_to_float(ff_div(ff_sub(ff_from(q.y), cf.point.y), D)),
-                          ff_to_float(ff_div(ff_sub(ff_from(q.z), cf.point.z), D)));
+        const f3 nrm = mk(ff_to_float(ff_div(cf.qp.x, D)), ff_to_float(ff_div(cf.qp.y, D)),
+                          ff_to_float(ff_div(cf.qp.z, D)));
         d_q = scale3(nrm, g);
         ga = scale3(nrm, -g * ff_to_float(cf.bw));
         gb = scale3(nrm, -g * ff_to_float(cf.bu));

@@ -11,6 +11,7 @@ namespace mbvh {
 struct MeshView {
   const WideNode* nodes;
   const uint32_t* roots;
+  const uint2* root_slots;  // slot range (first, end) of each root (ties across roots: the earlier root)
   const float4* nodes4;   // 4-wide tree: 8 float4 per node (4 children x {min.xyz max.x}{max.yz ref 0})
   const uint32_t* roots4; // 4-wide root reference per root
   uint32_t max_stack4;    // stack entries a quad traversal can need (4-wide or binary)
@@ -33,11 +34,16 @@ struct RaycastFirstArgs {
   StackScratch stack;
   const float* origins;
   const float* directions;
+  // double-precision rays as hi + lo pairs: the lo parts (both NULL for FP32 rays), lo = fl(x - hi)
+  const float* origins_lo;
+  const float* directions_lo;
   const uint32_t* order;  // visit order of the rays (NULL: 0..n-1); results go to each ray's own index
   uint64_t n;
   uint32_t side_mask;  // 1 front, 2 back, 3 both; 0 = per-triangle masks stored in the mesh
   float near_dist;
   float far_dist;
+  float near_lo;  // lo parts of near / far (0 for FP32 bounds)
+  float far_lo;
   uint32_t* out_face;
   float* out_dist;
   float* out_point;
@@ -49,10 +55,14 @@ struct RaycastAllArgs {
   StackScratch stack;
   const float* origins;
   const float* directions;
+  const float* origins_lo;     // lo parts (NULL for FP32 rays), as RaycastFirstArgs
+  const float* directions_lo;
   uint64_t n;
   uint32_t side_mask;
   float near_dist;
   float far_dist;
+  float near_lo;
+  float far_lo;
   uint32_t* counts;          // count pass: hits per ray
   const uint64_t* offsets;   // fill pass: first output slot of each ray
   uint32_t* out_face;
@@ -66,11 +76,14 @@ struct ClosestPointArgs {
   MeshView mesh;
   StackScratch stack;
   const float* points;
+  const float* points_lo;  // lo parts of double-precision points (NULL for FP32 points)
   const uint32_t* order;  // visit order of the points (NULL: 0..n-1); results go to each point's own index
   uint32_t* queue;        // work-queue counter (reset by the launch)
   uint64_t n;
   float min_threshold;
   float max_threshold;
+  float min_lo;  // lo parts of the thresholds (0 for FP32 thresholds)
+  float max_lo;
   uint32_t* out_face;
   float* out_dist;
   float* out_point;
@@ -208,6 +221,12 @@ struct BatchArgs {
   uint32_t side_mask;
   float near_dist;
   float far_dist;
+  // Local mode (inverse == NULL): the rays are already in their mesh's frame as hi + lo pairs
+  // (origins / directions the hi parts, these the lo parts), ray_instance[i] names the MESH, each
+  // ray has its own window (4 floats: near, near lo, far, far lo) and results stay local.
+  const float* origins_lo;
+  const float* directions_lo;
+  const float* windows;
   uint32_t* out_face;
   float* out_dist;
   float* out_point;

@@ -143,7 +143,20 @@ struct RayBox {
   float ox, oy, oz;
   float ix, iy, iz;  // 1 / direction
   bool sx, sy, sz;   // 1/dir >= 0 (reference's `invdir >= 0` selection)
+  float wa;          // absolute widening for the ray's low parts (0 for an FP32 ray)
 };
+
+// A ray given as hi + lo float pairs (a double split as hi = fl(x), lo = fl(x - hi)): the FP32
+// kernels trace the hi parts with every error bound widened by the lo parts, and the exact steps use
+// both. |lo| <= ulp(hi) / 2, so dividing by the direction changes each plane distance by at most
+// |ol / d| (1 + 2u) plus a relative u (inside the relative widening); zero direction components
+// have zero lo parts.
+__device__ __forceinline__ float slab_lo_widening(f3 ol, float ix, float iy, float iz) {
+  const float ax = ol.x == 0.0f ? 0.0f : fabsf(ol.x * ix);
+  const float ay = ol.y == 0.0f ? 0.0f : fabsf(ol.y * iy);
+  const float az = ol.z == 0.0f ? 0.0f : fabsf(ol.z * iz);
+  return fmaxf(fmaxf(ax, ay), az) * (1.0f + 4.0f * kU);
+}
 
 __device__ __forceinline__ bool slab(const RayBox& r, float mnx, float mny, float mnz, float mxx,
                                      float mxy, float mxz, float near_t, float far_t, float& tmin) {
@@ -153,8 +166,8 @@ __device__ __forceinline__ bool slab(const RayBox& r, float mnx, float mny, floa
   const float ty1 = ((r.sy ? mxy : mny) - r.oy) * r.iy;
   const float tz0 = ((r.sz ? mnz : mxz) - r.oz) * r.iz;
   const float tz1 = ((r.sz ? mxz : mnz) - r.oz) * r.iz;
-  const float t0 = widen_lo(fmaxf(fmaxf(tx0, ty0), tz0));
-  const float t1 = widen_hi(fminf(fminf(tx1, ty1), tz1));
+  const float t0 = widen_lo(fmaxf(fmaxf(tx0, ty0), tz0)) - r.wa;
+  const float t1 = widen_hi(fminf(fminf(tx1, ty1), tz1)) + r.wa;
   tmin = t0;
   return (t0 <= t1) && (t0 <= far_t) && (t1 >= near_t);
 }
@@ -173,15 +186,23 @@ struct RayBox4 {
   bool sx, sy, sz;   // 1/dir >= 0
 };
 
-__device__ __forceinline__ RayBox4 ray_box4(f3 o, f3 d, float coord_max) {
+// With low parts (ol, dl) the exact plane distance moves by at most (|ol| + S |dl|) |1/d| (1 + 2u)
+// more per axis, which is added to that axis's widening (zero for an FP32 ray).
+template <bool kLo>
+__device__ __forceinline__ RayBox4 ray_box4(f3 o, f3 d, f3 ol, f3 dl, float coord_max) {
   RayBox4 r;
   r.ix = 1.0f / d.x;
   r.iy = 1.0f / d.y;
   r.iz = 1.0f / d.z;
   const float ox = -o.x * r.ix, oy = -o.y * r.iy, oz = -o.z * r.iz;
-  const float ex = 8.0f * kU * ((coord_max + fabsf(o.x)) * fabsf(r.ix));
-  const float ey = 8.0f * kU * ((coord_max + fabsf(o.y)) * fabsf(r.iy));
-  const float ez = 8.0f * kU * ((coord_max + fabsf(o.z)) * fabsf(r.iz));
+  float ex = 8.0f * kU * ((coord_max + fabsf(o.x)) * fabsf(r.ix));
+  float ey = 8.0f * kU * ((coord_max + fabsf(o.y)) * fabsf(r.iy));
+  float ez = 8.0f * kU * ((coord_max + fabsf(o.z)) * fabsf(r.iz));
+  if (kLo) {
+    ex += (fabsf(ol.x) + (coord_max + fabsf(o.x)) * fabsf(r.ix) * fabsf(dl.x)) * fabsf(r.ix) * (1.0f + 4.0f * kU);
+    ey += (fabsf(ol.y) + (coord_max + fabsf(o.y)) * fabsf(r.iy) * fabsf(dl.y)) * fabsf(r.iy) * (1.0f + 4.0f * kU);
+    ez += (fabsf(ol.z) + (coord_max + fabsf(o.z)) * fabsf(r.iz) * fabsf(dl.z)) * fabsf(r.iz) * (1.0f + 4.0f * kU);
+  }
   r.lx = ox - ex;
   r.ly = oy - ey;
   r.lz = oz - ez;
@@ -196,9 +217,10 @@ __device__ __forceinline__ RayBox4 ray_box4(f3 o, f3 d, float coord_max) {
 
 // Rays the quad traversal does not take: a zero direction component (the reference's split-plane
 // shortcut is then needed literally; see trace_first), or slab constants that are not finite.
-__device__ __forceinline__ bool literal_ray(f3 o, f3 d, float coord_max) {
+template <bool kLo>
+__device__ __forceinline__ bool literal_ray(f3 o, f3 d, f3 ol, f3 dl, float coord_max) {
   if (d.x == 0.0f || d.y == 0.0f || d.z == 0.0f) return true;
-  const RayBox4 r = ray_box4(o, d, coord_max);
+  const RayBox4 r = ray_box4<kLo>(o, d, ol, dl, coord_max);
   const float m = fabsf(r.ix) + fabsf(r.iy) + fabsf(r.iz) + fabsf(r.lx) + fabsf(r.ly) + fabsf(r.lz) + fabsf(r.hx) +
                   fabsf(r.hy) + fabsf(r.hz);
   return !(m < kInf);
@@ -230,18 +252,37 @@ struct TriTerms {
   bool ok;
 };
 
-__device__ TriTerms tri_terms_exact(f3 o, f3 d, f3 a, f3 b, f3 c) {
+// The ray's direction in double-float (hi + lo; the lo part is 0 for an FP32 ray).
+template <bool kLo>
+__device__ __forceinline__ ff3 ray_dir_ff(f3 d, f3 dl) {
+  if (kLo) return {two_sum(d.x, dl.x), two_sum(d.y, dl.y), two_sum(d.z, dl.z)};
+  return {ff_from(d.x), ff_from(d.y), ff_from(d.z)};
+}
+
+// The exact sign terms for the ray o + ol + t (d + dl).
+template <bool kLo>
+__device__ TriTerms tri_terms_exact(f3 o, f3 d, f3 ol, f3 dl, f3 a, f3 b, f3 c) {
   TriTerms r;
-  const ff3 E1 = ff3_diff(b, a), E2 = ff3_diff(c, a), Q = ff3_diff(o, a);
+  const ff3 E1 = ff3_diff(b, a), E2 = ff3_diff(c, a);
+  ff3 Q = ff3_diff(o, a);
+  if (kLo) Q = {ff_add(Q.x, ff_from(ol.x)), ff_add(Q.y, ff_from(ol.y)), ff_add(Q.z, ff_from(ol.z))};
   const ff3 N = ff3_cross(E1, E2);
-  const ff ddn = ff3_dotf(d, N);
+  ff ddn, e0, e1;
+  if (kLo) {
+    const ff3 D = ray_dir_ff<true>(d, dl);
+    ddn = ff3_dot(D, N);
+    e0 = ff3_dot(D, ff3_cross(Q, E2));
+    e1 = ff3_dot(D, ff3_cross(E1, Q));
+  } else {
+    ddn = ff3_dotf(d, N);
+    e0 = ff3_dotf(d, ff3_cross(Q, E2));
+    e1 = ff3_dotf(d, ff3_cross(E1, Q));
+  }
   const int sg = ff_sign(ddn);
   r.ok = sg != 0;
   r.back = sg > 0;
   const bool neg = !r.back;  // sign = -1 for front-facing
   r.ddn_abs = neg ? ff_neg(ddn) : ddn;
-  ff e0 = ff3_dotf(d, ff3_cross(Q, E2));
-  ff e1 = ff3_dotf(d, ff3_cross(E1, Q));
   ff qn = ff3_dot(Q, N);
   r.e0 = neg ? ff_neg(e0) : e0;
   r.e1 = neg ? ff_neg(e1) : e1;
@@ -249,11 +290,31 @@ __device__ TriTerms tri_terms_exact(f3 o, f3 d, f3 a, f3 b, f3 c) {
   return r;
 }
 
-// Exact hit test; on a hit returns true and the ray parameter in double-float.
+// The near / far window of a query in double-float (lo parts 0 for FP32 bounds).
+struct Window {
+  float near_d, near_lo, far_d, far_lo;
+};
+__device__ __forceinline__ Window window_of(float near_d, float near_lo, float far_d, float far_lo) {
+  return {near_d, near_lo, far_d, far_lo};
+}
+
+// |d + dl| in double-float.
+template <bool kLo>
+__device__ __forceinline__ ff ray_len_ff(f3 d, f3 dl) {
+  if (kLo) {
+    const ff3 D = ray_dir_ff<true>(d, dl);
+    return ff_sqrt(ff3_dot(D, D));
+  }
+  return ff_sqrt(ff_add(ff_add(two_prod(d.x, d.x), two_prod(d.y, d.y)), two_prod(d.z, d.z)));
+}
+
+// Exact hit test; on a hit returns true and the ray parameter in double-float. The distance
+// t |d| is compared with near / far in double-float.
 // Arguments are passed by value so callers keep them in registers.
-__device__ __noinline__ bool hit_exact(f3 o, f3 d, float dlen, f3 a, f3 b, f3 c, uint32_t mask,
-                                       float near_d, float far_d, ff& t_out) {
-  const TriTerms r = tri_terms_exact(o, d, a, b, c);
+template <bool kLo>
+__device__ __noinline__ bool hit_exact(f3 o, f3 d, f3 ol, f3 dl, f3 a, f3 b, f3 c, uint32_t mask, Window win,
+                                       ff& t_out) {
+  const TriTerms r = tri_terms_exact<kLo>(o, d, ol, dl, a, b, c);
   if (!r.ok) return false;
   if (!(mask & (r.back ? 2u : 1u))) return false;
   if (ff_sign(r.e0) < 0) return false;
@@ -261,17 +322,45 @@ __device__ __noinline__ bool hit_exact(f3 o, f3 d, float dlen, f3 a, f3 b, f3 c,
   if (ff_sign(ff_sub(r.ddn_abs, ff_add(r.e0, r.e1))) < 0) return false;
   if (ff_sign(r.qdn) < 0) return false;
   const ff t = ff_div(r.qdn, r.ddn_abs);
-  const float dist = ff_to_float(t) * dlen;
-  if (dist < near_d || dist > far_d) return false;
+  if (win.near_d > 0.0f || win.far_d < kInf) {
+    const ff dist = ff_mul(t, ray_len_ff<kLo>(d, dl));
+    if (win.near_d > 0.0f && ff_sign(ff_sub(dist, {win.near_d, win.near_lo})) < 0) return false;
+    if (win.far_d < kInf && ff_sign(ff_sub(dist, {win.far_d, win.far_lo})) > 0) return false;
+  }
   t_out = t;
   return true;
 }
 
 enum : int { kMiss = 0, kHit = 1, kUnsure = 2 };
 
+// Per-ray constants of the FP32 test's error bounds. For an FP32 ray: kd = 32 u |d|_1. With low
+// parts (|ol|_inf = w, |dl|_1 = e) each sign term moves by at most: d.N by e |N|_inf <= 2 e mE^2;
+// the edge terms by 2 mE (e mQ + w (|d|_1 + e)); Q.N by 3 w |N|_inf <= 6 w mE^2 (the factors
+// 2.0001 / 6.0001 cover the FP32 roundings of the bounds themselves).
+struct RayBand {
+  float kd, a, c6, dm;
+};
+template <bool kLo>
+__device__ __forceinline__ RayBand ray_band(f3 ol, f3 dl, float dl1) {
+  RayBand b;
+  b.kd = 32.0f * kU * dl1;
+  b.a = 0.0f;
+  b.c6 = 0.0f;
+  b.dm = 0.0f;
+  if (kLo) {
+    const float w = maxabs(ol), e = fabsf(dl.x) + fabsf(dl.y) + fabsf(dl.z);
+    b.kd += 2.0001f * e;
+    b.a = 2.0001f * w * (dl1 + e);
+    b.c6 = 6.0001f * w;
+    b.dm = e;
+  }
+  return b;
+}
+
 // FP32 test. kHit comes with t and a bound et on its error; kUnsure means the FP32 decision lies
 // inside its error band and must be made exactly.
-__device__ __forceinline__ int hit_fast(f3 o, f3 d, float dl1, float dlen, const Tri& tr,
+template <bool kLo>
+__device__ __forceinline__ int hit_fast(f3 o, f3 d, float dlen, const RayBand& bd, const Tri& tr,
                                         uint32_t mask, float near_d, float far_d, float& t_out,
                                         float& et_out) {
   const f3 a = tr.a(), b = tr.b(), c = tr.c();
@@ -279,7 +368,7 @@ __device__ __forceinline__ int hit_fast(f3 o, f3 d, float dl1, float dlen, const
   const f3 n = cross(e1, e2);
   const float ddn = dot(d, n);
   const float mE = fmaxf(maxabs(e1), maxabs(e2));
-  const float bN = 32.0f * kU * dl1 * mE * mE;
+  const float bN = bd.kd * mE * mE;
   if (!(fabsf(ddn) > bN)) return kUnsure;
   const bool back = ddn > 0.0f;
   if (!(mask & (back ? 2u : 1u))) return kMiss;
@@ -287,7 +376,7 @@ __device__ __forceinline__ int hit_fast(f3 o, f3 d, float dl1, float dlen, const
   const float DdN = fabsf(ddn);
   const f3 q = sub(o, a);
   const float mQ = maxabs(q);
-  const float bE = 32.0f * kU * dl1 * mQ * mE;
+  const float bE = kLo ? (bd.kd * mQ + bd.a) * mE : bd.kd * mQ * mE;
   const float e0 = s * dot(d, cross(q, e2));
   if (e0 < -bE) return kMiss;
   const float f1 = s * dot(d, cross(e1, q));
@@ -296,14 +385,20 @@ __device__ __forceinline__ int hit_fast(f3 o, f3 d, float dl1, float dlen, const
   const float bS = bN + 2.0f * bE + 4.0f * kU * (DdN + fabsf(e0) + fabsf(f1));
   if (g < -bS) return kMiss;
   const float qdn = -s * dot(q, n);
-  const float bQ = 64.0f * kU * mQ * mE * mE;
+  const float bQ = kLo ? (64.0f * kU * mQ + bd.c6) * mE * mE : 64.0f * kU * mQ * mE * mE;
   if (qdn < -bQ) return kMiss;
   if (!(e0 > bE && f1 > bE && g > bS && qdn > bQ)) return kUnsure;
   const float t = qdn / DdN;
+  const float et = t * (bQ / qdn + bN / DdN + 4.0f * kU);
   const float dist = t * dlen;
-  if (dist < near_d || dist > far_d) return kMiss;
+  if (near_d > 0.0f || far_d < kInf) {
+    // |o + t d - o| against the window: undecided inside the distance's error band
+    const float ed = (8.0f * kU * dist + et * dlen + (kLo ? t * bd.dm : 0.0f)) * (1.0f + 4.0f * kU);
+    if (dist + ed < near_d || dist - ed > far_d) return kMiss;
+    if (dist - ed < near_d || dist + ed > far_d) return kUnsure;
+  }
   t_out = t;
-  et_out = t * (bQ / qdn + bN / DdN + 4.0f * kU);
+  et_out = et;
   return kHit;
 }
 
@@ -334,12 +429,118 @@ struct Cands {
   }
 };
 
-// Exact resolution for the raycast: the first candidate (visit order) with the smallest exact t.
-__device__ __noinline__ uint32_t resolve_ray(Cands cl, float U, f3 o, f3 d, float dlen,
-                                             const float4* tris, uint32_t side_mask, float near_d,
-                                             float far_d, ff& best_t) {
+// ------------------------------------------------------------------------------------------
+// Ties. Exactly equal results (a point whose closest feature is a vertex shared by several
+// triangles) compute to identical values; results equal in exact arithmetic but computed along
+// different paths (a shared edge seen from its two triangles) agree to about 2^-44. Values within
+// 2^-40 of each other are ties, and a tie goes to the triangle the reference meets first. Both
+// reference traversals are depth-first over the binary tree, roots in order, a leaf's triangles in
+// slot order, and keep the first of equal results (strict < within leaves and across roots,
+// raycastFirst.template.js:91-97 `<=` preferring the near child). At the deepest node whose subtree
+// holds both slots, raycastFirst enters the child on the side the ray comes from (dir[splitAxis] >= 0:
+// left, raycastFirst.template.js:46-48) and closestPointToPoint the child at the smaller box
+// distance^2, the right one only if strictly smaller (src/core/cast/shapecast.js:76-92).
+// ------------------------------------------------------------------------------------------
+// A resolution's winner: its slot (kNoHit if none), exact value and that value's error bound.
+struct Winner {
+  uint32_t w;
+  ff v;
+  float err;
+};
+
+// Two hit distances are tied when within 2^-40 of each other, relatively.
+__device__ __forceinline__ bool near_tie(ff a, ff b) {
+  const ff d = ff_sub(a, b);
+  return fabsf(d.hi) <= 9.094947017729282e-13f * fmaxf(fabsf(a.hi), fabsf(b.hi));  // 2^-40
+}
+// Two squared point distances are tied when their difference is within the sum of their
+// evaluations' error bounds (ClosestFF::err): neither can then be shown the smaller.
+__device__ __forceinline__ bool point_tie(ff a, float ea, ff b, float eb) {
+  return fabsf(ff_sub(a, b).hi) <= (ea + eb) * (1.0f + 2.0f * kU);
+}
+
+// What the tie order reads of a mesh.
+struct TieTree {
+  const WideNode* nodes;
+  const uint32_t* roots;
+  const uint2* root_slots;
+  uint32_t n_roots;
+};
+__device__ __forceinline__ TieTree tie_tree(const MeshView& M) { return {M.nodes, M.roots, M.root_slots, M.n_roots}; }
+
+__device__ __forceinline__ uint32_t root_of(const TieTree& T, uint32_t s) {
+  uint32_t r = 0;
+  while (r < T.n_roots && !(s >= T.root_slots[r].x && s < T.root_slots[r].y)) r++;
+  return r;
+}
+
+// The deepest wide node whose subtree holds slots a and b of root r, or kDone when they share a
+// leaf. Each step descends one level (children follow their parents in the node array).
+__device__ __forceinline__ uint32_t split_node(const TieTree& T, uint32_t r, uint32_t a, uint32_t b) {
+  uint32_t ref = T.roots[r];
+  while (!(ref & kRefLeaf)) {
+    const uint4 meta = T.nodes[ref].meta;
+    const bool ra = a >= meta.w, rb = b >= meta.w;
+    if (ra != rb) return ref;
+    ref = ra ? meta.y : meta.x;
+  }
+  return kDone;
+}
+
+// Whether the reference's raycastFirst meets slot a before slot b (a != b) on direction d.
+__device__ __noinline__ bool ray_first_of(TieTree T, float dx, float dy, float dz, uint32_t a, uint32_t b) {
+  const uint32_t ra = root_of(T, a), rb = root_of(T, b);
+  if (ra != rb) return ra < rb;
+  if (ra >= T.n_roots) return a < b;  // a slot outside every root's range: no tree order to follow
+  const uint32_t n = split_node(T, ra, a, b);
+  if (n == kDone) return a < b;
+  const uint4 meta = T.nodes[n].meta;
+  const float dax = meta.z == 0 ? dx : (meta.z == 1 ? dy : dz);
+  return (a < meta.w) == (dax >= 0.0f);
+}
+
+// Squared distance from q + ql to a box in double-float (the clamp is exact; the differences are
+// exact, the squares and sum carry double-float roundings).
+template <bool kLo>
+__device__ __forceinline__ ff box_score_ff(f3 q, f3 ql, float mnx, float mny, float mnz, float mxx, float mxy,
+                                           float mxz) {
+  auto axis = [](float p, float pl, float mn, float mx) {
+    ff g = {0.0f, 0.0f};
+    if (p < mn || (kLo && p == mn && pl < 0.0f)) {
+      g = two_sum(mn, -p);
+      if (kLo) g = ff_sub(g, ff_from(pl));
+    } else if (p > mx || (kLo && p == mx && pl > 0.0f)) {
+      g = two_sum(p, -mx);
+      if (kLo) g = ff_add(g, ff_from(pl));
+    }
+    return ff_mul(g, g);
+  };
+  return ff_add(ff_add(axis(q.x, ql.x, mnx, mxx), axis(q.y, ql.y, mny, mxy)), axis(q.z, ql.z, mnz, mxz));
+}
+
+// Whether the reference's closestPointToPoint meets slot a before slot b (a != b) from q + ql.
+template <bool kLo>
+__device__ __noinline__ bool point_first_of(TieTree T, f3 q, f3 ql, uint32_t a, uint32_t b) {
+  const uint32_t ra = root_of(T, a), rb = root_of(T, b);
+  if (ra != rb) return ra < rb;
+  if (ra >= T.n_roots) return a < b;  // a slot outside every root's range: no tree order to follow
+  const uint32_t n = split_node(T, ra, a, b);
+  if (n == kDone) return a < b;
+  const WideNode nd = T.nodes[n];
+  const ff sl = box_score_ff<kLo>(q, ql, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y);
+  const ff sr = box_score_ff<kLo>(q, ql, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w);
+  const bool right_first = ff_sign(ff_sub(sr, sl)) < 0;
+  return (a < nd.meta.w) != right_first;
+}
+
+// Exact resolution for the raycast: the smallest exact t; ties (near_tie) go to the reference's
+// first. kRefOrder: the candidates are not in the reference's order (four lanes per ray), so ties
+// are ordered by ray_first_of; otherwise the list order is the reference's and the earlier wins.
+template <bool kLo, bool kRefOrder>
+__device__ __noinline__ Winner resolve_ray(Cands cl, float U, f3 o, f3 d, f3 ol, f3 dl, const float4* tris,
+                                           TieTree T, uint32_t side_mask, Window win_d) {
   uint32_t win = kNoHit;
-  best_t = {kInf, 0.0f};
+  ff best_t = {kInf, 0.0f};
   for (int i = 0; i < cl.n; i++) {
     const uint2 e = cl.get(i);
     const float lb = __uint_as_float(e.y);
@@ -347,13 +548,20 @@ __device__ __noinline__ uint32_t resolve_ray(Cands cl, float U, f3 o, f3 d, floa
     const Tri tr = load_tri(tris, e.x);
     const uint32_t mask = side_mask ? side_mask : tr.mask();
     ff t;
-    if (!hit_exact(o, d, dlen, tr.a(), tr.b(), tr.c(), mask, near_d, far_d, t)) continue;
-    if (win == kNoHit || ff_sign(ff_sub(t, best_t)) < 0) {
+    if (!hit_exact<kLo>(o, d, ol, dl, tr.a(), tr.b(), tr.c(), mask, win_d, t)) continue;
+    bool take = win == kNoHit;
+    if (!take) {
+      if (near_tie(t, best_t))
+        take = kRefOrder && e.x != win && ray_first_of(T, d.x, d.y, d.z, e.x, win);
+      else
+        take = ff_sign(ff_sub(t, best_t)) < 0;
+    }
+    if (take) {
       win = e.x;
       best_t = t;
     }
   }
-  return win;
+  return {win, best_t, 0.0f};
 }
 
 // ------------------------------------------------------------------------------------------
@@ -376,10 +584,47 @@ struct FirstHit {
   f3 bary;
 };
 
-// One ray against one mesh; `st` is the lane's traversal stack (any LDS depth).
-template <typename StackT>
-__device__ __forceinline__ FirstHit trace_first(const MeshView& M, StackT& st, uint32_t lane, f3 o, f3 d,
-                                                uint32_t side_mask, float near_dist, float far_dist) {
+// The reported values of the winning triangle, re-evaluated in double-float: distance t |d|, point
+// o + t d and barycoord (the exact terms of three's Ray.intersectTriangle).
+template <bool kLo>
+__device__ __forceinline__ FirstHit finish_hit(const float4* tris, uint32_t w, f3 o, f3 d, f3 ol, f3 dl) {
+  const Tri tr = load_tri(tris, w);
+  const TriTerms r = tri_terms_exact<kLo>(o, d, ol, dl, tr.a(), tr.b(), tr.c());
+  const ff t = ff_div(r.qdn, r.ddn_abs);
+  const ff u = ff_div(r.e0, r.ddn_abs);
+  const ff v = ff_div(r.e1, r.ddn_abs);
+  const ff wgt = ff_sub(ff_sub(ff_from(1.0f), u), v);
+  FirstHit h;
+  h.face = tr.face();
+  h.dist = ff_to_float(ff_mul(t, ray_len_ff<kLo>(d, dl)));
+  if (kLo) {
+    const ff3 D = ray_dir_ff<true>(d, dl);
+    h.p = mk(ff_to_float(ff_add(two_sum(o.x, ol.x), ff_mul(t, D.x))),
+             ff_to_float(ff_add(two_sum(o.y, ol.y), ff_mul(t, D.y))),
+             ff_to_float(ff_add(two_sum(o.z, ol.z), ff_mul(t, D.z))));
+  } else {
+    h.p = mk(ff_to_float(ff_add(ff_from(o.x), ff_mulf(t, d.x))), ff_to_float(ff_add(ff_from(o.y), ff_mulf(t, d.y))),
+             ff_to_float(ff_add(ff_from(o.z), ff_mulf(t, d.z))));
+  }
+  h.bary = mk(ff_to_float(wgt), ff_to_float(u), ff_to_float(v));
+  return h;
+}
+
+__device__ __forceinline__ FirstHit no_hit() {
+  FirstHit h;
+  h.face = kNoHit;
+  h.dist = kInf;
+  const float qnan = __int_as_float(0x7fc00000);
+  h.p = mk(qnan, qnan, qnan);
+  h.bary = h.p;
+  return h;
+}
+
+// One ray (o + ol) + t (d + dl) against one mesh; `st` is the lane's traversal stack (any LDS depth).
+// kLo = false: an FP32 ray (ol, dl unused).
+template <bool kLo, typename StackT>
+__device__ __forceinline__ FirstHit trace_first(const MeshView& M, StackT& st, uint32_t lane, f3 o, f3 d, f3 ol,
+                                                f3 dl, uint32_t side_mask, Window win) {
   RayBox rb;
   rb.ox = o.x;
   rb.oy = o.y;
@@ -390,8 +635,10 @@ __device__ __forceinline__ FirstHit trace_first(const MeshView& M, StackT& st, u
   rb.sx = rb.ix >= 0.0f;
   rb.sy = rb.iy >= 0.0f;
   rb.sz = rb.iz >= 0.0f;
+  rb.wa = kLo ? slab_lo_widening(ol, rb.ix, rb.iy, rb.iz) : 0.0f;
+  const float near_dist = win.near_d, far_dist = win.far_d;
   const float dlen = sqrtf(dot(d, d));
-  const float dl1 = fabsf(d.x) + fabsf(d.y) + fabsf(d.z);
+  const RayBand band = ray_band<kLo>(ol, dl, fabsf(d.x) + fabsf(d.y) + fabsf(d.z));
   const bool literal = (d.x == 0.0f) || (d.y == 0.0f) || (d.z == 0.0f);
 
   float U = kInf;  // upper bound of the best hit's t
@@ -430,7 +677,9 @@ __device__ __forceinline__ FirstHit trace_first(const MeshView& M, StackT& st, u
             // daxis == +-0 means l2r, so the far child is the right one.
             const float oaxis = axis == 0 ? o.x : (axis == 1 ? o.y : o.z);
             const float fmin = axis == 0 ? nd.b1.z : (axis == 1 ? nd.b1.w : nd.b2.x);
-            const bool prunable = (daxis == 0.0f) && (oaxis <= fmin);
+            bool below = oaxis <= fmin;
+            if (kLo && oaxis == fmin) below = (axis == 0 ? ol.x : (axis == 1 ? ol.y : ol.z)) <= 0.0f;
+            const bool prunable = (daxis == 0.0f) && below;
             aux = (hits & ~kPruneFlag) | (prunable ? kPruneFlag : 0u);
           }
           if (!st.push(c2, aux)) {
@@ -466,11 +715,11 @@ __device__ __forceinline__ FirstHit trace_first(const MeshView& M, StackT& st, u
           if (s + 1 < off + cnt) next = load_tri(M.tris, s + 1);
           const uint32_t mask = side_mask ? side_mask : tr.mask();
           float t = 0.0f, et = 0.0f;
-          int res = hit_fast(o, d, dl1, dlen, tr, mask, near_dist, far_dist, t, et);
+          int res = hit_fast<kLo>(o, d, dlen, band, tr, mask, near_dist, far_dist, t, et);
           if (res == kUnsure && literal) {
             // the literal shortcut counts hits, so an undecided hit is decided now
             ff te;
-            res = hit_exact(o, d, dlen, tr.a(), tr.b(), tr.c(), mask, near_dist, far_dist, te) ? kHit : kMiss;
+            res = hit_exact<kLo>(o, d, ol, dl, tr.a(), tr.b(), tr.c(), mask, win, te) ? kHit : kMiss;
             t = ff_to_float(te);
             et = 2.0f * kU * t;
           }
@@ -483,11 +732,11 @@ __device__ __forceinline__ FirstHit trace_first(const MeshView& M, StackT& st, u
           }
           if (cl.n == kCand && !cl.compact(U)) {
             // full of live candidates: settle them now and keep the winner
-            ff bt;
-            const uint32_t w = resolve_ray(cl, U, o, d, dlen, M.tris, side_mask, near_dist, far_dist, bt);
+            const Winner r = resolve_ray<kLo, false>(cl, U, o, d, ol, dl, M.tris, tie_tree(M), side_mask, win);
+            const uint32_t w = r.w;
             cl.n = 0;
             if (w != kNoHit) {
-              const float tw = ff_to_float(bt);
+              const float tw = ff_to_float(r.v);
               U = fminf(U, tw * (1.0f + 4.0f * kU));
               cl.put(0, w, tw * (1.0f - 4.0f * kU));
               cl.n = 1;
@@ -509,31 +758,12 @@ __device__ __forceinline__ FirstHit trace_first(const MeshView& M, StackT& st, u
     }
   }
 
-  FirstHit h;
-  h.face = kNoHit;
-  h.dist = kInf;
-  const float qnan = __int_as_float(0x7fc00000);
-  h.p = mk(qnan, qnan, qnan);
-  h.bary = h.p;
+  FirstHit h = no_hit();
   if (overflow) {
     h.face = kOverflow;
   } else if (cl.n > 0) {
-    ff bt;
-    const uint32_t w = resolve_ray(cl, U, o, d, dlen, M.tris, side_mask, near_dist, far_dist, bt);
-    if (w != kNoHit) {
-      const Tri tr = load_tri(M.tris, w);
-      const TriTerms r = tri_terms_exact(o, d, tr.a(), tr.b(), tr.c());
-      const ff t = ff_div(r.qdn, r.ddn_abs);
-      const ff u = ff_div(r.e0, r.ddn_abs);
-      const ff v = ff_div(r.e1, r.ddn_abs);
-      const ff wgt = ff_sub(ff_sub(ff_from(1.0f), u), v);
-      const ff dl = ff_sqrt(ff_add(ff_add(two_prod(d.x, d.x), two_prod(d.y, d.y)), two_prod(d.z, d.z)));
-      h.face = tr.face();
-      h.dist = ff_to_float(ff_mul(t, dl));
-      h.p = mk(ff_to_float(ff_add(ff_from(o.x), ff_mulf(t, d.x))), ff_to_float(ff_add(ff_from(o.y), ff_mulf(t, d.y))),
-               ff_to_float(ff_add(ff_from(o.z), ff_mulf(t, d.z))));
-      h.bary = mk(ff_to_float(wgt), ff_to_float(u), ff_to_float(v));
-    }
+    const uint32_t w = resolve_ray<kLo, false>(cl, U, o, d, ol, dl, M.tris, tie_tree(M), side_mask, win).w;
+    if (w != kNoHit) h = finish_hit<kLo>(M.tris, w, o, d, ol, dl);
   }
   return h;
 }
@@ -638,12 +868,26 @@ __device__ __forceinline__ uint32_t quad_children(QuadStack& st, uint32_t lane, 
   return first;
 }
 
-__device__ __forceinline__ FirstHit trace_first4(const MeshView& M, QuadStack& st, uint32_t lane, f3 o, f3 d,
-                                                 uint32_t side_mask, float near_dist, float far_dist) {
+// Whether another lane's exact result (o, ow) beats this lane's (b, w): a smaller value, or a tie
+// (`tie`, a symmetric test of the two) that the reference meets first. Antisymmetric, so the lanes
+// of a quad agree.
+template <typename FirstOf>
+__device__ __forceinline__ bool beats(ff o, uint32_t ow, ff b, uint32_t w, bool tie, FirstOf first_of) {
+  if (ow == kNoHit || ow == w) return false;
+  if (w == kNoHit) return true;
+  if (tie) return first_of(ow, w);
+  return ff_sign(ff_sub(o, b)) < 0;
+}
+
+template <bool kLo>
+__device__ __forceinline__ FirstHit trace_first4(const MeshView& M, QuadStack& st, uint32_t lane, f3 o, f3 d, f3 ol,
+                                                 f3 dl, uint32_t side_mask, Window win) {
   const uint32_t q = lane & 3u;
-  const RayBox4 rb = ray_box4(o, d, *M.coord_max);
+  const RayBox4 rb = ray_box4<kLo>(o, d, ol, dl, *M.coord_max);
+  const float near_dist = win.near_d, far_dist = win.far_d;
   const float dlen = sqrtf(dot(d, d));
-  const float dl1 = fabsf(d.x) + fabsf(d.y) + fabsf(d.z);
+  const RayBand band = ray_band<kLo>(ol, dl, fabsf(d.x) + fabsf(d.y) + fabsf(d.z));
+  const TieTree T = tie_tree(M);
   float U = kInf;  // quad-wide upper bound of the best hit's t (after each leaf)
   bool overflow = false;
   Cands cl;
@@ -682,18 +926,18 @@ __device__ __forceinline__ FirstHit trace_first4(const MeshView& M, QuadStack& s
           const Tri tr = load_tri(M.tris, s);
           const uint32_t mask = side_mask ? side_mask : tr.mask();
           float t = 0.0f, et = 0.0f;
-          const int res = hit_fast(o, d, dl1, dlen, tr, mask, near_dist, far_dist, t, et);
+          const int res = hit_fast<kLo>(o, d, dlen, band, tr, mask, near_dist, far_dist, t, et);
           if (res == kMiss) continue;
           if (res == kHit) {
             if (t - et > Ul) continue;  // certainly behind a known hit
             Ul = fminf(Ul, t + et);
           }
           if (cl.n == kCand && !cl.compact(Ul)) {
-            ff bt;
-            const uint32_t w = resolve_ray(cl, Ul, o, d, dlen, M.tris, side_mask, near_dist, far_dist, bt);
+            const Winner r = resolve_ray<kLo, true>(cl, Ul, o, d, ol, dl, M.tris, T, side_mask, win);
+            const uint32_t w = r.w;
             cl.n = 0;
             if (w != kNoHit) {
-              const float tw = ff_to_float(bt);
+              const float tw = ff_to_float(r.v);
               Ul = fminf(Ul, tw * (1.0f + 4.0f * kU));
               cl.put(0, w, tw * (1.0f - 4.0f * kU));
               cl.n = 1;
@@ -716,49 +960,37 @@ __device__ __forceinline__ FirstHit trace_first4(const MeshView& M, QuadStack& s
     }
   }
 
-  FirstHit h;
-  h.face = kNoHit;
-  h.dist = kInf;
-  const float qnan = __int_as_float(0x7fc00000);
-  h.p = mk(qnan, qnan, qnan);
-  h.bary = h.p;
+  FirstHit h = no_hit();
   if (overflow) {
     h.face = kOverflow;
     return h;
   }
-  ff bt = {kInf, 0.0f};
-  uint32_t w = kNoHit;
-  if (cl.n > 0) {
-    w = resolve_ray(cl, U, o, d, dlen, M.tris, side_mask, near_dist, far_dist, bt);
-    if (w == kNoHit) bt = {kInf, 0.0f};
-  }
-  // smallest exact t over the quad (then the smaller slot)
+  Winner r = {kNoHit, {kInf, 0.0f}, 0.0f};
+  if (cl.n > 0) r = resolve_ray<kLo, true>(cl, U, o, d, ol, dl, M.tris, T, side_mask, win);
+  ff bt = r.w == kNoHit ? ff{kInf, 0.0f} : r.v;
+  uint32_t w = r.w;
+  // the quad's winner: smallest exact t, ties to the reference's first
   for (int m = 1; m <= 2; m <<= 1) {
     const float oh = m == 1 ? quad_xor1(bt.hi) : quad_xor2(bt.hi);
-    const float ol = m == 1 ? quad_xor1(bt.lo) : quad_xor2(bt.lo);
+    const float olo = m == 1 ? quad_xor1(bt.lo) : quad_xor2(bt.lo);
     const uint32_t ow = m == 1 ? quad_xor1(w) : quad_xor2(w);
-    if (oh < bt.hi || (oh == bt.hi && (ol < bt.lo || (ol == bt.lo && ow < w)))) {
-      bt = {oh, ol};
+    const ff ot = {oh, olo};
+    if (beats(ot, ow, bt, w, near_tie(ot, bt),
+              [&](uint32_t a, uint32_t b) { return ray_first_of(T, d.x, d.y, d.z, a, b); })) {
+      bt = ot;
       w = ow;
     }
   }
-  if (w != kNoHit) {
-    const Tri tr = load_tri(M.tris, w);
-    const TriTerms r = tri_terms_exact(o, d, tr.a(), tr.b(), tr.c());
-    const ff t = ff_div(r.qdn, r.ddn_abs);
-    const ff u = ff_div(r.e0, r.ddn_abs);
-    const ff v = ff_div(r.e1, r.ddn_abs);
-    const ff wgt = ff_sub(ff_sub(ff_from(1.0f), u), v);
-    const ff dl = ff_sqrt(ff_add(ff_add(two_prod(d.x, d.x), two_prod(d.y, d.y)), two_prod(d.z, d.z)));
-    h.face = tr.face();
-    h.dist = ff_to_float(ff_mul(t, dl));
-    h.p = mk(ff_to_float(ff_add(ff_from(o.x), ff_mulf(t, d.x))), ff_to_float(ff_add(ff_from(o.y), ff_mulf(t, d.y))),
-             ff_to_float(ff_add(ff_from(o.z), ff_mulf(t, d.z))));
-    h.bary = mk(ff_to_float(wgt), ff_to_float(u), ff_to_float(v));
-  }
+  if (w != kNoHit) h = finish_hit<kLo>(M.tris, w, o, d, ol, dl);
   return h;
 }
 
+// The triple at index i of a packed float array (zero when the array is NULL).
+__device__ __forceinline__ f3 load3(const float* p, uint64_t i) {
+  return p ? mk(p[3 * i + 0], p[3 * i + 1], p[3 * i + 2]) : mk(0.0f, 0.0f, 0.0f);
+}
+
+template <bool kLo>
 __global__ __launch_bounds__(kBlock, 4) void k_raycast_first4(RaycastFirstArgs A) {
   __shared__ uint2 qstack[kQuadLds * (kBlock / 4)];
   const uint32_t lane = threadIdx.x;
@@ -777,9 +1009,12 @@ __global__ __launch_bounds__(kBlock, 4) void k_raycast_first4(RaycastFirstArgs A
     if (ray >= A.n) continue;
     const f3 o = mk(A.origins[3 * ray + 0], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
     const f3 d = mk(A.directions[3 * ray + 0], A.directions[3 * ray + 1], A.directions[3 * ray + 2]);
-    const bool literal = literal_ray(o, d, *A.mesh.coord_max);
-    const FirstHit h = literal ? trace_first(A.mesh, st, lane, o, d, A.side_mask, A.near_dist, A.far_dist)
-                               : trace_first4(A.mesh, st, lane, o, d, A.side_mask, A.near_dist, A.far_dist);
+    const f3 ol = kLo ? load3(A.origins_lo, ray) : mk(0.0f, 0.0f, 0.0f);
+    const f3 dl = kLo ? load3(A.directions_lo, ray) : mk(0.0f, 0.0f, 0.0f);
+    const Window win = window_of(A.near_dist, A.near_lo, A.far_dist, A.far_lo);
+    const bool literal = literal_ray<kLo>(o, d, ol, dl, *A.mesh.coord_max);
+    const FirstHit h = literal ? trace_first<kLo>(A.mesh, st, lane, o, d, ol, dl, A.side_mask, win)
+                               : trace_first4<kLo>(A.mesh, st, lane, o, d, ol, dl, A.side_mask, win);
     if ((lane & 3u) == 0) {
       A.out_face[ray] = h.face;
       if (A.out_dist) A.out_dist[ray] = h.dist;
@@ -798,6 +1033,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_raycast_first4(RaycastFirstArgs A
 }
 
 // (kBlock, 4): at most 128 VGPRs, so four waves fit on a SIMD (the LDS use allows four blocks per CU)
+template <bool kLo>
 __global__ __launch_bounds__(kBlock, 4) void k_raycast_first(RaycastFirstArgs A) {
   const uint32_t lane = threadIdx.x;
   const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + lane;
@@ -815,7 +1051,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_raycast_first(RaycastFirstArgs A)
     if (ray >= A.n) continue;  // not a permutation of 0..n-1: skip rather than read out of bounds
     const f3 o = mk(A.origins[3 * ray + 0], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
     const f3 d = mk(A.directions[3 * ray + 0], A.directions[3 * ray + 1], A.directions[3 * ray + 2]);
-    const FirstHit h = trace_first(A.mesh, st, lane, o, d, A.side_mask, A.near_dist, A.far_dist);
+    const f3 ol = kLo ? load3(A.origins_lo, ray) : mk(0.0f, 0.0f, 0.0f);
+    const f3 dl = kLo ? load3(A.directions_lo, ray) : mk(0.0f, 0.0f, 0.0f);
+    const FirstHit h = trace_first<kLo>(A.mesh, st, lane, o, d, ol, dl, A.side_mask,
+                                        window_of(A.near_dist, A.near_lo, A.far_dist, A.far_lo));
     A.out_face[ray] = h.face;
     if (A.out_dist) A.out_dist[ray] = h.dist;
     if (A.out_point) {
@@ -909,7 +1148,9 @@ __global__ void k_batch_reset(uint32_t* counter) {
 // visit order, grouped by mesh) from an atomic counter until none are left; each lane brings its
 // ray into its instance's frame, traces it against that instance's mesh and converts the hit to
 // world space.
-template <int kL>
+// kLo (local mode, A.inverse == NULL): rays already in their mesh's frame as hi + lo pairs, ray_instance
+// naming the mesh, a window per ray, results left in the mesh's frame.
+template <int kL, bool kLo>
 __global__ __launch_bounds__(kBlock, 4) void k_batch_raycast_first(BatchArgs A) {
   __shared__ uint2 lds_stack[kL * kBlock];
   const uint32_t tid = threadIdx.x, lane64 = tid & 63u;
@@ -934,19 +1175,26 @@ __global__ __launch_bounds__(kBlock, 4) void k_batch_raycast_first(BatchArgs A) 
       const uint64_t ray = A.order ? A.order[visit] : visit;
       if (ray >= A.n) continue;
       const uint32_t inst = A.ray_instance[ray];
-      const uint32_t geom = inst < A.n_instances ? A.instance_geometry[inst] : 0xFFFFFFFFu;
+      const uint32_t geom = kLo ? inst : (inst < A.n_instances ? A.instance_geometry[inst] : 0xFFFFFFFFu);
       FirstHit h;
       h.face = kNoHit;
       h.dist = kInf;
       h.p = mk(qnan, qnan, qnan);
       h.bary = h.p;
-      if (geom < A.n_meshes) {
+      if (kLo && geom < A.n_meshes) {
+        const float* wn = A.windows + 4 * ray;
+        h = trace_first<true>(A.meshes[geom], st, tid, load3(A.origins, ray), load3(A.directions, ray),
+                              load3(A.origins_lo, ray), load3(A.directions_lo, ray), A.side_mask,
+                              window_of(wn[0], wn[1], wn[2], wn[3]));
+      } else if (!kLo && geom < A.n_meshes) {
         const f3 ow = mk(A.origins[3 * ray], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
         const float* inv = A.inverse + 16ull * inst;
         const f3 o = apply_matrix4(inv, ow);
         const f3 d = transform_direction(inv, mk(A.directions[3 * ray], A.directions[3 * ray + 1], A.directions[3 * ray + 2]));
         const float sc = instance_scale(A.world + 16ull * inst, d);
-        h = trace_first(A.meshes[geom], st, tid, o, d, A.side_mask, A.near_dist / sc, A.far_dist / sc);
+        const f3 z = mk(0.0f, 0.0f, 0.0f);
+        h = trace_first<false>(A.meshes[geom], st, tid, o, d, z, z, A.side_mask,
+                               window_of(A.near_dist / sc, 0.0f, A.far_dist / sc, 0.0f));
         if (h.face < kOverflow) {
           // matrices and the world origin are re-read here rather than kept live across the trace
           const f3 ow2 = mk(A.origins[3 * ray], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
@@ -972,6 +1220,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_batch_raycast_first(BatchArgs A) 
 
 // Same work queue with four lanes per ray (quads, the 4-wide trees): a wave traces 16 rays at a
 // time; rays with a zero direction component take the binary traversal (see trace_first4).
+template <bool kLo>
 __global__ __launch_bounds__(kBlock, 4) void k_batch_raycast_first4(BatchArgs A) {
   __shared__ uint2 qstack[kQuadLds * (kBlock / 4)];
   const uint32_t tid = threadIdx.x, lane64 = tid & 63u, q = tid & 3u, wq = lane64 >> 2;
@@ -996,22 +1245,32 @@ __global__ __launch_bounds__(kBlock, 4) void k_batch_raycast_first4(BatchArgs A)
       const uint64_t ray = A.order ? A.order[visit] : visit;
       if (ray >= A.n) continue;
       const uint32_t inst = A.ray_instance[ray];
-      const uint32_t geom = inst < A.n_instances ? A.instance_geometry[inst] : 0xFFFFFFFFu;
+      const uint32_t geom = kLo ? inst : (inst < A.n_instances ? A.instance_geometry[inst] : 0xFFFFFFFFu);
       FirstHit h;
       h.face = kNoHit;
       h.dist = kInf;
       h.p = mk(qnan, qnan, qnan);
       h.bary = h.p;
-      if (geom < A.n_meshes) {
+      if (kLo && geom < A.n_meshes) {
+        const MeshView& M = A.meshes[geom];
+        const f3 o = load3(A.origins, ray), d = load3(A.directions, ray);
+        const f3 ol = load3(A.origins_lo, ray), dl = load3(A.directions_lo, ray);
+        const float* wn = A.windows + 4 * ray;
+        const Window win = window_of(wn[0], wn[1], wn[2], wn[3]);
+        h = literal_ray<true>(o, d, ol, dl, *M.coord_max) ? trace_first<true>(M, st, tid, o, d, ol, dl, A.side_mask, win)
+                                                         : trace_first4<true>(M, st, tid, o, d, ol, dl, A.side_mask, win);
+      } else if (!kLo && geom < A.n_meshes) {
         const f3 ow = mk(A.origins[3 * ray], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
         const float* inv = A.inverse + 16ull * inst;
         const f3 o = apply_matrix4(inv, ow);
         const f3 d = transform_direction(inv, mk(A.directions[3 * ray], A.directions[3 * ray + 1], A.directions[3 * ray + 2]));
         const float sc = instance_scale(A.world + 16ull * inst, d);
         const MeshView& M = A.meshes[geom];
-        const bool literal = literal_ray(o, d, *M.coord_max);
-        h = literal ? trace_first(M, st, tid, o, d, A.side_mask, A.near_dist / sc, A.far_dist / sc)
-                    : trace_first4(M, st, tid, o, d, A.side_mask, A.near_dist / sc, A.far_dist / sc);
+        const f3 z = mk(0.0f, 0.0f, 0.0f);
+        const Window win = window_of(A.near_dist / sc, 0.0f, A.far_dist / sc, 0.0f);
+        const bool literal = literal_ray<false>(o, d, z, z, *M.coord_max);
+        h = literal ? trace_first<false>(M, st, tid, o, d, z, z, A.side_mask, win)
+                    : trace_first4<false>(M, st, tid, o, d, z, z, A.side_mask, win);
         if (h.face < kOverflow) {
           const f3 ow2 = mk(A.origins[3 * ray], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
           h.p = apply_matrix4(A.world + 16ull * inst, h.p);
@@ -1043,7 +1302,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_batch_raycast_first4(BatchArgs A)
 // the identical order, so hit k of a ray lands at offsets[ray] + k in the reference's order.
 // Every accepted hit is re-evaluated in double-float for its reported values.
 // ------------------------------------------------------------------------------------------
-template <bool kFill>
+template <bool kFill, bool kLo>
 __global__ __launch_bounds__(kBlock) void k_raycast_all(RaycastAllArgs A) {
   const uint32_t lane = threadIdx.x;
   const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + lane;
@@ -1060,6 +1319,9 @@ __global__ __launch_bounds__(kBlock) void k_raycast_all(RaycastAllArgs A) {
   for (uint64_t ray = gtid; ray < A.n; ray += gstride) {
     const f3 o = mk(A.origins[3 * ray + 0], A.origins[3 * ray + 1], A.origins[3 * ray + 2]);
     const f3 d = mk(A.directions[3 * ray + 0], A.directions[3 * ray + 1], A.directions[3 * ray + 2]);
+    const f3 ol = kLo ? load3(A.origins_lo, ray) : mk(0.0f, 0.0f, 0.0f);
+    const f3 dl = kLo ? load3(A.directions_lo, ray) : mk(0.0f, 0.0f, 0.0f);
+    const Window win = window_of(A.near_dist, A.near_lo, A.far_dist, A.far_lo);
     RayBox rb;
     rb.ox = o.x;
     rb.oy = o.y;
@@ -1070,8 +1332,9 @@ __global__ __launch_bounds__(kBlock) void k_raycast_all(RaycastAllArgs A) {
     rb.sx = rb.ix >= 0.0f;
     rb.sy = rb.iy >= 0.0f;
     rb.sz = rb.iz >= 0.0f;
+    rb.wa = kLo ? slab_lo_widening(ol, rb.ix, rb.iy, rb.iz) : 0.0f;
     const float dlen = sqrtf(dot(d, d));
-    const float dl1 = fabsf(d.x) + fabsf(d.y) + fabsf(d.z);
+    const RayBand band = ray_band<kLo>(ol, dl, fabsf(d.x) + fabsf(d.y) + fabsf(d.z));
     uint32_t count = 0;
     uint64_t base = 0;
     if (kFill) base = A.offsets[ray];
@@ -1084,8 +1347,8 @@ __global__ __launch_bounds__(kBlock) void k_raycast_all(RaycastAllArgs A) {
         while (!(ref & kRefLeaf) && ref != kDone) {
           const WideNode nd = M.nodes[ref];
           float tl, tr;
-          const bool hl = slab(rb, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y, A.near_dist, A.far_dist, tl);
-          const bool hr = slab(rb, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w, A.near_dist, A.far_dist, tr);
+          const bool hl = slab(rb, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y, win.near_d, win.far_d, tl);
+          const bool hr = slab(rb, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w, win.near_d, win.far_d, tr);
           if (hl && hr) {
             if (!st.push(nd.meta.y, 0u)) {
               overflow = true;
@@ -1108,31 +1371,27 @@ __global__ __launch_bounds__(kBlock) void k_raycast_all(RaycastAllArgs A) {
             const Tri tr = load_tri(M.tris, s);
             const uint32_t mask = A.side_mask ? A.side_mask : tr.mask();
             float t = 0.0f, et = 0.0f;
-            int res = hit_fast(o, d, dl1, dlen, tr, mask, A.near_dist, A.far_dist, t, et);
+            int res = hit_fast<kLo>(o, d, dlen, band, tr, mask, A.near_dist, A.far_dist, t, et);
             if (res == kUnsure) {
               ff te;
-              res = hit_exact(o, d, dlen, tr.a(), tr.b(), tr.c(), mask, A.near_dist, A.far_dist, te) ? kHit : kMiss;
+              res = hit_exact<kLo>(o, d, ol, dl, tr.a(), tr.b(), tr.c(), mask, win, te) ? kHit : kMiss;
             }
             if (res != kHit) continue;
             if (kFill) {
               const uint64_t k = base + count;
               if (k < A.capacity) {
-                const TriTerms q = tri_terms_exact(o, d, tr.a(), tr.b(), tr.c());
-                const ff tt = ff_div(q.qdn, q.ddn_abs);
-                const ff u = ff_div(q.e0, q.ddn_abs);
-                const ff v = ff_div(q.e1, q.ddn_abs);
-                const ff dl = ff_sqrt(ff_add(ff_add(two_prod(d.x, d.x), two_prod(d.y, d.y)), two_prod(d.z, d.z)));
-                A.out_face[k] = tr.face();
-                if (A.out_dist) A.out_dist[k] = ff_to_float(ff_mul(tt, dl));
+                const FirstHit h = finish_hit<kLo>(M.tris, s, o, d, ol, dl);
+                A.out_face[k] = h.face;
+                if (A.out_dist) A.out_dist[k] = h.dist;
                 if (A.out_point) {
-                  A.out_point[3 * k + 0] = ff_to_float(ff_add(ff_from(o.x), ff_mulf(tt, d.x)));
-                  A.out_point[3 * k + 1] = ff_to_float(ff_add(ff_from(o.y), ff_mulf(tt, d.y)));
-                  A.out_point[3 * k + 2] = ff_to_float(ff_add(ff_from(o.z), ff_mulf(tt, d.z)));
+                  A.out_point[3 * k + 0] = h.p.x;
+                  A.out_point[3 * k + 1] = h.p.y;
+                  A.out_point[3 * k + 2] = h.p.z;
                 }
                 if (A.out_bary) {
-                  A.out_bary[3 * k + 0] = ff_to_float(ff_sub(ff_sub(ff_from(1.0f), u), v));
-                  A.out_bary[3 * k + 1] = ff_to_float(u);
-                  A.out_bary[3 * k + 2] = ff_to_float(v);
+                  A.out_bary[3 * k + 0] = h.bary.x;
+                  A.out_bary[3 * k + 1] = h.bary.y;
+                  A.out_bary[3 * k + 2] = h.bary.z;
                 }
               }
             }
@@ -1202,24 +1461,54 @@ __device__ __forceinline__ float box_score(f3 q, float mnx, float mny, float mnz
   return dx * dx + dy * dy + dz * dz;
 }
 
-// Exact resolution for the closest point: the first candidate (visit order) at the smallest
-// exact squared distance.
-__device__ __noinline__ uint32_t resolve_point(Cands cl, float U, f3 q, const float4* tris,
-                                               ff& best_d2) {
+// Exact resolution for the closest point: the smallest exact squared distance from q + ql; ties
+// (point_tie) go to the reference's first (kRefOrder: by point_first_of, the candidates coming from
+// four lanes; otherwise the list is in the reference's order and the earlier wins).
+template <bool kLo, bool kRefOrder>
+__device__ __noinline__ Winner resolve_point(Cands cl, float U, f3 q, f3 ql, const float4* tris, TieTree T) {
   uint32_t win = kNoHit;
-  best_d2 = {kInf, 0.0f};
+  ff best_d2 = {kInf, 0.0f};
+  float best_err = 0.0f;
   for (int i = 0; i < cl.n; i++) {
     const uint2 e = cl.get(i);
     if (__uint_as_float(e.y) > U) continue;
     const Tri tr = load_tri(tris, e.x);
-    const ff d2 = closest_on_triangle_ff(q, tr.a(), tr.b(), tr.c()).dist2;
+    const ClosestFF cf = closest_on_triangle_ff<kLo>(q, ql, tr.a(), tr.b(), tr.c());
+    const ff d2 = cf.dist2;
     if (!(d2.hi == d2.hi)) continue;  // degenerate: never wins (reference: `NaN < best` is false)
-    if (win == kNoHit || ff_sign(ff_sub(d2, best_d2)) < 0) {
+    bool take = win == kNoHit;
+    if (!take) {
+      if (point_tie(d2, cf.err, best_d2, best_err))
+        take = kRefOrder && e.x != win && point_first_of<kLo>(T, q, ql, e.x, win);
+      else
+        take = ff_sign(ff_sub(d2, best_d2)) < 0;
+    }
+    if (take) {
       win = e.x;
       best_d2 = d2;
+      best_err = cf.err;
     }
   }
-  return win;
+  return {win, best_d2, best_err};
+}
+
+// Exact decisions against a threshold given as hi + lo floats: whether the squared distance of
+// triangle `tr` from q + ql is below t^2, and whether a box's squared distance is below t^2. The
+// FP32 value x (within a relative error rel, plus e2 absolute for a triangle) decides when it is
+// clear of the band; otherwise double-float does.
+__device__ __forceinline__ ff thr_sq(float t, float tl) {
+  const ff T = two_sum(t, tl);
+  return ff_mul(T, T);
+}
+template <bool kLo>
+__device__ __noinline__ bool tri_below(f3 q, f3 ql, const Tri& tr, ff t2) {
+  const ff d2 = closest_on_triangle_ff<kLo>(q, ql, tr.a(), tr.b(), tr.c()).dist2;
+  return d2.hi == d2.hi && ff_sign(ff_sub(d2, t2)) < 0;  // NaN (degenerate) is never below
+}
+template <bool kLo>
+__device__ __noinline__ bool box_below(f3 q, f3 ql, float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
+                                       ff t2) {
+  return ff_sign(ff_sub(box_score_ff<kLo>(q, ql, mnx, mny, mnz, mxx, mxy, mxz), t2)) < 0;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1238,11 +1527,53 @@ __device__ __noinline__ uint32_t resolve_point(Cands cl, float U, f3 q, const fl
 // re-checked on pop). Leaf step: lane q tests triangles q, q+4, ...; the bound is shared after
 // each leaf. Each lane resolves its candidates exactly; the quad keeps the smallest exact squared
 // distance (exactly equal distances are ties).
+// The closest point's reported values: distance, point and barycoord in double-float.
+template <bool kLo>
+__device__ __forceinline__ void finish_point(const float4* tris, uint32_t w, f3 q, f3 ql, uint32_t& face, float& dist,
+                                             float& px, float& py, float& pz, float& bw, float& bu, float& bv) {
+  const Tri tr = load_tri(tris, w);
+  const ClosestFF cf = closest_on_triangle_ff<kLo>(q, ql, tr.a(), tr.b(), tr.c());
+  face = tr.face();
+  dist = ff_to_float(ff_sqrt(cf.dist2));
+  px = ff_to_float(cf.point.x);
+  py = ff_to_float(cf.point.y);
+  pz = ff_to_float(cf.point.z);
+  bw = ff_to_float(cf.bw);
+  bu = ff_to_float(cf.bu);
+  bv = ff_to_float(cf.bv);
+}
+
+// The absolute error bound of an FP32 point-to-triangle distance for the query q (+ ql): FP32
+// roundings (8u of the magnitudes), and for a double query twice its distance to q (so that bounds
+// measured from q also bound the distances from q + ql, and the boxes pruned from q stay pruned).
+template <bool kLo>
+__device__ __forceinline__ float query_error(f3 q, f3 ql, float coord_max) {
+  float e = 8.0f * kU * (maxabs(q) + coord_max);
+  if (kLo) e += 2.0f * (fabsf(ql.x) + fabsf(ql.y) + fabsf(ql.z)) * (1.0f + 4.0f * kU);
+  return e;
+}
+
+// Whether a box (FP32 squared distance s from q) is entered under maxThreshold^2 (t2): the FP32
+// value decides when it is clear of its error band, double-float otherwise. Double queries always
+// decide in double-float (their FP32 score is measured from q, not from q + ql).
+template <bool kLo>
+__device__ __forceinline__ bool below_max(bool finite, float s, ff t2, f3 q, f3 ql, float mnx, float mny, float mnz,
+                                          float mxx, float mxy, float mxz) {
+  if (!finite) return true;
+  if (!kLo) {
+    if (s * (1.0f + 8.0f * kU) < t2.hi * (1.0f - 4.0f * kU)) return true;
+    if (s * (1.0f - 8.0f * kU) > t2.hi * (1.0f + 4.0f * kU)) return false;
+  }
+  return box_below<kLo>(q, ql, mnx, mny, mnz, mxx, mxy, mxz, t2);
+}
+
+template <bool kLo>
 __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A) {
   __shared__ uint2 qstack[kQuadLds * (kBlock / 4)];
   const uint32_t lane = threadIdx.x, q4 = lane & 3u, wq = (lane & 63u) >> 2;
   const uint32_t qid = lane >> 2;
   const MeshView M = A.mesh;
+  const TieTree T = tie_tree(M);
   QuadStack st;
   st.lds = qstack;
   st.ovf = A.stack.ovf;
@@ -1250,7 +1581,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
   st.gquad = static_cast<uint64_t>(blockIdx.x) * (kBlock / 4) + qid;
   st.qid = qid;
   st.capacity = kQuadLds + static_cast<int>(A.stack.ovf_entries);
-  const float max2 = A.max_threshold * A.max_threshold;
+  const bool max_finite = A.max_threshold < kInf;
+  const ff max_t2 = max_finite ? thr_sq(A.max_threshold, A.max_lo) : ff{kInf, 0.0f};
   constexpr float kScoreSlack = 1.0f - 4.0f * kU;
   const float qnan = __int_as_float(0x7fc00000);
 
@@ -1264,7 +1596,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
     const uint64_t qi = A.order ? A.order[visit] : visit;
     if (qi >= A.n) continue;
     const f3 q = mk(A.points[3 * qi + 0], A.points[3 * qi + 1], A.points[3 * qi + 2]);
-    const float e_abs = 8.0f * kU * (maxabs(q) + *M.coord_max);
+    const f3 ql = kLo ? load3(A.points_lo, qi) : mk(0.0f, 0.0f, 0.0f);
+    const float e_abs = query_error<kLo>(q, ql, *M.coord_max);
     float U = kInf;  // quad-wide upper bound of the best squared distance (after each leaf)
     bool overflow = false;
     Cands cl;
@@ -1278,14 +1611,16 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
           const float4 c0 = M.nodes4[8ull * ref + 2 * q4];
           const float4 c1 = M.nodes4[8ull * ref + 2 * q4 + 1];
           const uint32_t cref = __float_as_uint(c1.z);
-          const float sc = box_score(q, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y) * kScoreSlack;
-          ref = quad_children(st, lane, (cref != kDone) & (sc <= U) & (sc < max2), sc, cref, overflow);
+          const float raw = box_score(q, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y);
+          const float sc = raw * kScoreSlack;
+          bool in = (cref != kDone) & (sc <= U);
+          if (in && max_finite) in = below_max<kLo>(true, raw, max_t2, q, ql, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y);
+          ref = quad_children(st, lane, in, sc, cref, overflow);
           if (overflow) break;
           if (ref == kDone) {
             while (st.sp > 0) {
               const uint2 e = st.pop();
-              const float se = __uint_as_float(e.y);
-              if (se <= U && se < max2) {
+              if (__uint_as_float(e.y) <= U) {  // maxThreshold was decided when it was pushed
                 ref = e.x;
                 break;
               }
@@ -1296,8 +1631,6 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
           uint32_t off, cnt;
           leaf_range(ref, M.bigleaf, off, cnt);
           float Ul = U;
-          float best = kInf;
-          f3 best_cp = mk(kInf, kInf, kInf);
           for (uint32_t s = off + q4; s < off + cnt; s += 4) {
             const Tri tr = load_tri(M.tris, s);
             // the triangle's own box first: its squared distance (clamp exact, three roundings per
@@ -1310,45 +1643,33 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
             const f3 cp = closest_on_triangle(q, tr.a(), tr.b(), tr.c());
             const f3 dv = sub(q, cp);
             const float d2 = dot(dv, dv);
-            float lb;
-            bool add;
-            if (!(d2 == d2)) {
-              add = true;  // FP32 failed (degenerate triangle): decide exactly at the end
-              lb = -kInf;
-            } else {
+            float lb = -kInf;  // FP32 failed (degenerate triangle, NaN): decide exactly at the end
+            if (d2 == d2) {
               const float e2 = (2.0f * sqrt_up(d2) + e_abs) * e_abs + 4.0f * kU * d2;
               lb = d2 - e2;
-              add = lb <= Ul && !(d2 == best && cp.x == best_cp.x && cp.y == best_cp.y && cp.z == best_cp.z);
-              if (add) {
-                Ul = fminf(Ul, d2 + e2);
-                if (d2 < best) {
-                  best = d2;
-                  best_cp = cp;
-                }
+              if (lb > Ul) continue;
+              // every candidate that may tie stays listed: ties go to the reference's first
+              Ul = fminf(Ul, d2 + e2);
+            }
+            if (cl.n == kCand && !cl.compact(Ul)) {
+              const Winner r = resolve_point<kLo, true>(cl, Ul, q, ql, M.tris, T);
+              const uint32_t w = r.w;
+              cl.n = 0;
+              if (w != kNoHit) {
+                const float dw = ff_to_float(r.v);
+                Ul = fminf(Ul, dw * (1.0f + 4.0f * kU));
+                cl.put(0, w, dw * (1.0f - 4.0f * kU));
+                cl.n = 1;
               }
             }
-            if (add) {
-              if (cl.n == kCand && !cl.compact(Ul)) {
-                ff bd;
-                const uint32_t w = resolve_point(cl, Ul, q, M.tris, bd);
-                cl.n = 0;
-                if (w != kNoHit) {
-                  const float dw = ff_to_float(bd);
-                  Ul = fminf(Ul, dw * (1.0f + 4.0f * kU));
-                  cl.put(0, w, dw * (1.0f - 4.0f * kU));
-                  cl.n = 1;
-                }
-              }
-              cl.put(cl.n++, s, lb);
-            }
+            cl.put(cl.n++, s, lb);
           }
           Ul = fminf(Ul, quad_xor1(Ul));
           U = fminf(Ul, quad_xor2(Ul));
           ref = kDone;
           while (st.sp > 0) {
             const uint2 e = st.pop();
-            const float se = __uint_as_float(e.y);
-            if (se <= U && se < max2) {
+            if (__uint_as_float(e.y) <= U) {
               ref = e.x;
               break;
             }
@@ -1357,36 +1678,29 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
         if (ref == kDone) break;
       }
     }
-    ff bd = {kInf, 0.0f};
-    uint32_t w = kNoHit;
-    if (!overflow && cl.n > 0) {
-      w = resolve_point(cl, U, q, M.tris, bd);
-      if (w == kNoHit) bd = {kInf, 0.0f};
-    }
+    Winner r = {kNoHit, {kInf, 0.0f}, 0.0f};
+    if (!overflow && cl.n > 0) r = resolve_point<kLo, true>(cl, U, q, ql, M.tris, T);
+    ff bd = r.w == kNoHit ? ff{kInf, 0.0f} : r.v;
+    float be = r.w == kNoHit ? 0.0f : r.err;
+    uint32_t w = r.w;
+    // the quad's winner: smallest exact squared distance, ties to the reference's first
     for (int m = 1; m <= 2; m <<= 1) {
       const float oh = m == 1 ? quad_xor1(bd.hi) : quad_xor2(bd.hi);
-      const float ol = m == 1 ? quad_xor1(bd.lo) : quad_xor2(bd.lo);
+      const float olo = m == 1 ? quad_xor1(bd.lo) : quad_xor2(bd.lo);
+      const float oe = m == 1 ? quad_xor1(be) : quad_xor2(be);
       const uint32_t ow = m == 1 ? quad_xor1(w) : quad_xor2(w);
-      if (oh < bd.hi || (oh == bd.hi && (ol < bd.lo || (ol == bd.lo && ow < w)))) {
-        bd = {oh, ol};
+      const ff od = {oh, olo};
+      if (beats(od, ow, bd, w, point_tie(od, oe, bd, be),
+                [&](uint32_t a, uint32_t b) { return point_first_of<kLo>(T, q, ql, a, b); })) {
+        bd = od;
+        be = oe;
         w = ow;
       }
     }
     uint32_t out_face = overflow ? kOverflow : kNoHit;
     float out_dist = kInf;
     float px = qnan, py = qnan, pz = qnan, bw = qnan, bu = qnan, bv = qnan;
-    if (!overflow && w != kNoHit && q4 == 0) {
-      const Tri tr = load_tri(M.tris, w);
-      const ClosestFF cf = closest_on_triangle_ff(q, tr.a(), tr.b(), tr.c());
-      out_face = tr.face();
-      out_dist = ff_to_float(ff_sqrt(cf.dist2));
-      px = ff_to_float(cf.point.x);
-      py = ff_to_float(cf.point.y);
-      pz = ff_to_float(cf.point.z);
-      bw = ff_to_float(cf.bw);
-      bu = ff_to_float(cf.bu);
-      bv = ff_to_float(cf.bv);
-    }
+    if (!overflow && w != kNoHit && q4 == 0) finish_point<kLo>(M.tris, w, q, ql, out_face, out_dist, px, py, pz, bw, bu, bv);
     if (q4 == 0) {
       A.out_face[qi] = out_face;
       if (A.out_dist) A.out_dist[qi] = out_dist;
@@ -1404,10 +1718,17 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
   }
 }
 
+// One lane per query on the binary tree in the reference's own visit order (minThreshold's early
+// exit returns the first triangle under it in that order): children ordered by their squared box
+// distances (decided in double-float when the FP32 values are within their error band of each
+// other), maxThreshold decided exactly, and the first triangle whose exact squared distance is below
+// minThreshold^2 ends the query.
+template <bool kLo>
 __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
   const uint32_t lane = threadIdx.x;
   const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + lane;
   const MeshView M = A.mesh;
+  const TieTree T = tie_tree(M);
 
   Stack st;
   st.ovf = A.stack.ovf;
@@ -1416,8 +1737,10 @@ __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
   st.lane = lane;
   st.capacity = kStackLds + static_cast<int>(A.stack.ovf_entries);
 
-  const float min2 = A.min_threshold * A.min_threshold;
-  const float max2 = A.max_threshold * A.max_threshold;
+  const bool min_pos = A.min_threshold > 0.0f;
+  const ff min_t2 = min_pos ? thr_sq(A.min_threshold, A.min_lo) : ff{0.0f, 0.0f};
+  const bool max_finite = A.max_threshold < kInf;
+  const ff max_t2 = max_finite ? thr_sq(A.max_threshold, A.max_lo) : ff{kInf, 0.0f};
   constexpr float kScoreSlack = 1.0f - 4.0f * kU;
 
   // Work queue: each wave takes the next 64 queries until none are left (query costs vary a lot,
@@ -1432,11 +1755,10 @@ __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
     const uint64_t qi = A.order ? A.order[visit] : visit;
     if (qi >= A.n) continue;  // not a permutation of 0..n-1: skip rather than read out of bounds
     const f3 q = mk(A.points[3 * qi + 0], A.points[3 * qi + 1], A.points[3 * qi + 2]);
-    // Absolute error bound of an FP32 point-to-triangle distance for this query.
-    const float e_abs = 8.0f * kU * (maxabs(q) + *M.coord_max);
+    const f3 ql = kLo ? load3(A.points_lo, qi) : mk(0.0f, 0.0f, 0.0f);
+    const float e_abs = query_error<kLo>(q, ql, *M.coord_max);
+    const float w1 = kLo ? fabsf(ql.x) + fabsf(ql.y) + fabsf(ql.z) : 0.0f;
     float U = kInf;       // upper bound of the best squared distance
-    float best = kInf;    // smallest FP32 squared distance seen
-    f3 best_cp = mk(kInf, kInf, kInf);
     bool done = false, overflow = false;
     Cands cl;
     cl.lane = lane;
@@ -1451,27 +1773,39 @@ __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
           const WideNode nd = M.nodes[ref];
           const float sl = box_score(q, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y);
           const float sr = box_score(q, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w);
-          const bool swap = sr < sl;
+          // right first only if strictly closer, decided exactly when the FP32 scores are too close
+          // (each within 6u of its exact value; a double query's shift moves each by (2 sqrt(s) + w) w)
+          bool swap = sr < sl;
+          const float smax = fmaxf(sl, sr);
+          const float band = 12.0f * kU * smax + (kLo ? (2.0f * sqrt_up(smax) + w1) * w1 * 2.0f : 0.0f);
+          if (!(fabsf(sr - sl) > band))
+            swap = ff_sign(ff_sub(box_score_ff<kLo>(q, ql, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w),
+                                  box_score_ff<kLo>(q, ql, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y))) < 0;
           const uint32_t c1 = swap ? nd.meta.y : nd.meta.x;
           const uint32_t c2 = swap ? nd.meta.x : nd.meta.y;
           const float s1 = (swap ? sr : sl) * kScoreSlack;
           const float s2 = (swap ? sl : sr) * kScoreSlack;
-          const bool in1 = s1 <= U && s1 < max2;
-          const bool in2 = s2 <= U && s2 < max2;
-          if (in1) {
-            if (in2 && !st.push(c2, __float_as_uint(s2))) {
+          bool in1 = s1 <= U, in2 = s2 <= U;
+          if (in1 && max_finite)
+            in1 = swap ? below_max<kLo>(true, sr, max_t2, q, ql, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w)
+                       : below_max<kLo>(true, sl, max_t2, q, ql, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y);
+          if (in2 && max_finite)
+            in2 = swap ? below_max<kLo>(true, sl, max_t2, q, ql, nd.b0.x, nd.b0.y, nd.b0.z, nd.b0.w, nd.b1.x, nd.b1.y)
+                       : below_max<kLo>(true, sr, max_t2, q, ql, nd.b1.z, nd.b1.w, nd.b2.x, nd.b2.y, nd.b2.z, nd.b2.w);
+          if (in1 && in2) {
+            if (!st.push(c2, __float_as_uint(s2))) {
               overflow = done = true;
               ref = kDone;
               break;
             }
             ref = c1;
+          } else if (in1 || in2) {
+            ref = in1 ? c1 : c2;
           } else {
-            // s2 >= s1: if the near child is out, so is the far one
             ref = kDone;
             while (st.sp > 0) {
               const uint2 e = st.pop();
-              const float sc = __uint_as_float(e.y);
-              if (sc <= U && sc < max2) {
+              if (__uint_as_float(e.y) <= U) {  // maxThreshold was decided when it was pushed
                 ref = e.x;
                 break;
               }
@@ -1490,31 +1824,25 @@ __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
             const f3 cp = closest_on_triangle(q, tr.a(), tr.b(), tr.c());
             const f3 dv = sub(q, cp);
             const float d2 = dot(dv, dv);
-            float lb;
+            float lb, e2 = 0.0f;
             bool add;
             if (!(d2 == d2)) {
               add = true;  // FP32 failed (degenerate triangle): decide exactly at the end
               lb = -kInf;
             } else {
-              const float e2 = (2.0f * sqrt_up(d2) + e_abs) * e_abs + 4.0f * kU * d2;
+              e2 = (2.0f * sqrt_up(d2) + e_abs) * e_abs + 4.0f * kU * d2;
               lb = d2 - e2;
-              // an exact copy of the best point (shared vertex) ties exactly: the earlier wins
-              add = lb <= U && !(d2 == best && cp.x == best_cp.x && cp.y == best_cp.y && cp.z == best_cp.z);
-              if (add) {
-                U = fminf(U, d2 + e2);
-                if (d2 < best) {
-                  best = d2;
-                  best_cp = cp;
-                }
-              }
+              // every candidate that may tie stays listed (equal FP32 values do not make exact ties)
+              add = lb <= U;
+              if (add) U = fminf(U, d2 + e2);
             }
             if (add) {
               if (cl.n == kCand && !cl.compact(U)) {
-                ff bd;
-                const uint32_t w = resolve_point(cl, U, q, M.tris, bd);
+                const Winner r = resolve_point<kLo, false>(cl, U, q, ql, M.tris, T);
+                const uint32_t w = r.w;
                 cl.n = 0;
                 if (w != kNoHit) {
-                  const float dw = ff_to_float(bd);
+                  const float dw = ff_to_float(r.v);
                   U = fminf(U, dw * (1.0f + 4.0f * kU));
                   cl.put(0, w, dw * (1.0f - 4.0f * kU));
                   cl.n = 1;
@@ -1522,17 +1850,27 @@ __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
               }
               cl.put(cl.n++, s, lb);
             }
-            if (d2 < min2) {
-              done = true;
-              break;
+            // minThreshold: the first triangle whose exact squared distance is below it ends the
+            // query (a degenerate triangle's NaN never is)
+            if (min_pos && d2 == d2) {
+              bool under;
+              if (d2 + e2 < min_t2.hi * (1.0f - 4.0f * kU))
+                under = true;
+              else if (d2 - e2 > min_t2.hi * (1.0f + 4.0f * kU))
+                under = false;
+              else
+                under = tri_below<kLo>(q, ql, tr, min_t2);
+              if (under) {
+                done = true;
+                break;
+              }
             }
           }
           ref = kDone;
           if (done) break;
           while (st.sp > 0) {
             const uint2 e = st.pop();
-            const float sc = __uint_as_float(e.y);
-            if (sc <= U && sc < max2) {
+            if (__uint_as_float(e.y) <= U) {
               ref = e.x;
               break;
             }
@@ -1549,20 +1887,8 @@ __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
     if (overflow) {
       out_face = kOverflow;
     } else if (cl.n > 0) {
-      ff bd;
-      const uint32_t w = resolve_point(cl, U, q, M.tris, bd);
-      if (w != kNoHit) {
-        const Tri tr = load_tri(M.tris, w);
-        const ClosestFF cf = closest_on_triangle_ff(q, tr.a(), tr.b(), tr.c());
-        out_face = tr.face();
-        out_dist = ff_to_float(ff_sqrt(cf.dist2));
-        px = ff_to_float(cf.point.x);
-        py = ff_to_float(cf.point.y);
-        pz = ff_to_float(cf.point.z);
-        bw = ff_to_float(cf.bw);
-        bu = ff_to_float(cf.bu);
-        bv = ff_to_float(cf.bv);
-      }
+      const uint32_t w = resolve_point<kLo, false>(cl, U, q, ql, M.tris, T).w;
+      if (w != kNoHit) finish_point<kLo>(M.tris, w, q, ql, out_face, out_dist, px, py, pz, bw, bu, bv);
     }
     A.out_face[qi] = out_face;
     if (A.out_dist) A.out_dist[qi] = out_dist;
@@ -1646,7 +1972,9 @@ __global__ __launch_bounds__(kBlock) void k_scene_raycast_first(SceneArgs A) {
           const f3 o = apply_matrix4(inv, ow);
           const f3 d = transform_direction(inv, dw);
           const float sc = instance_scale(A.world + 16ull * inst, d);
-          FirstHit h = trace_first(A.meshes[geom], st, tid, o, d, A.side_mask, A.near_dist / sc, A.far_dist / sc);
+          const f3 z = mk(0.0f, 0.0f, 0.0f);
+          FirstHit h = trace_first<false>(A.meshes[geom], st, tid, o, d, z, z, A.side_mask,
+                                          window_of(A.near_dist / sc, 0.0f, A.far_dist / sc, 0.0f));
           if (h.face == kOverflow) {
             overflow = true;
             break;
@@ -2561,13 +2889,21 @@ uint64_t launch_threads(uint64_t n, uint64_t max_threads) {
 
 hipError_t launch_raycast_first4(const RaycastFirstArgs& a, uint64_t quads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_raycast_first4, dim3(static_cast<uint32_t>(quads / (kBlock / 4))), dim3(kBlock), 0, s, a);
+  const dim3 grid(static_cast<uint32_t>(quads / (kBlock / 4)));
+  if (a.origins_lo || a.directions_lo)
+    hipLaunchKernelGGL(k_raycast_first4<true>, grid, dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_raycast_first4<false>, grid, dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_raycast_first(const RaycastFirstArgs& a, uint64_t threads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_raycast_first, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
+  const dim3 grid(static_cast<uint32_t>(threads / kBlock));
+  if (a.origins_lo || a.directions_lo)
+    hipLaunchKernelGGL(k_raycast_first<true>, grid, dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_raycast_first<false>, grid, dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
@@ -2587,12 +2923,19 @@ hipError_t launch_batch_raycast_first(const BatchArgs& a, uint32_t stack_lds, ui
                                       hipStream_t s) {
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.counter);
   if (a.n == 0) return hipGetLastError();
-  if (lanes == 4)
-    hipLaunchKernelGGL(k_batch_raycast_first4, dim3(blocks), dim3(kBlock), 0, s, a);
+  const bool local = a.inverse == nullptr;  // rays already in their mesh's frame (hi + lo pairs)
+  if (lanes == 4 && local)
+    hipLaunchKernelGGL(k_batch_raycast_first4<true>, dim3(blocks), dim3(kBlock), 0, s, a);
+  else if (lanes == 4)
+    hipLaunchKernelGGL(k_batch_raycast_first4<false>, dim3(blocks), dim3(kBlock), 0, s, a);
+  else if (stack_lds == 8 && local)
+    hipLaunchKernelGGL((k_batch_raycast_first<8, true>), dim3(blocks), dim3(kBlock), 0, s, a);
   else if (stack_lds == 8)
-    hipLaunchKernelGGL(k_batch_raycast_first<8>, dim3(blocks), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_batch_raycast_first<8, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+  else if (local)
+    hipLaunchKernelGGL((k_batch_raycast_first<16, true>), dim3(blocks), dim3(kBlock), 0, s, a);
   else
-    hipLaunchKernelGGL(k_batch_raycast_first<16>, dim3(blocks), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_batch_raycast_first<16, false>), dim3(blocks), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
@@ -2629,7 +2972,11 @@ hipError_t launch_closest_to_geometry(const GeometryArgs& a, uint64_t threads, h
 hipError_t launch_closest_point4(const ClosestPointArgs& a, uint64_t threads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.queue);
-  hipLaunchKernelGGL(k_closest_point4, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
+  const dim3 grid(static_cast<uint32_t>(threads / kBlock));
+  if (a.points_lo)
+    hipLaunchKernelGGL(k_closest_point4<true>, grid, dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_closest_point4<false>, grid, dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
@@ -2650,7 +2997,7 @@ __global__ __launch_bounds__(kBlock) void k_sphere_decide(SphereArgs a) {
       if (slot != kNoHit) {
         const Tri tr = load_tri(a.mesh.tris, slot);
         const f3 c = mk(a.centers[3 * i + 0], a.centers[3 * i + 1], a.centers[3 * i + 2]);
-        const ClosestFF cf = closest_on_triangle_ff(c, tr.a(), tr.b(), tr.c());
+        const ClosestFF cf = closest_on_triangle_ff<false>(c, mk(0.0f, 0.0f, 0.0f), tr.a(), tr.b(), tr.c());
         res = (cf.dist2.hi == cf.dist2.hi) && ff_sign(ff_sub(cf.dist2, two_prod(r, r))) <= 0 ? 1 : 0;
       }
     }
@@ -2669,17 +3016,26 @@ hipError_t launch_sphere_decide(const SphereArgs& a, hipStream_t s) {
 hipError_t launch_closest_point(const ClosestPointArgs& a, uint64_t threads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.queue);
-  hipLaunchKernelGGL(k_closest_point, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
+  const dim3 grid(static_cast<uint32_t>(threads / kBlock));
+  if (a.points_lo)
+    hipLaunchKernelGGL(k_closest_point<true>, grid, dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_closest_point<false>, grid, dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_raycast_all(const RaycastAllArgs& a, bool fill, uint64_t threads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   const dim3 grid(static_cast<uint32_t>(threads / kBlock)), block(kBlock);
-  if (fill)
-    hipLaunchKernelGGL(k_raycast_all<true>, grid, block, 0, s, a);
+  const bool lo = a.origins_lo || a.directions_lo;
+  if (fill && lo)
+    hipLaunchKernelGGL((k_raycast_all<true, true>), grid, block, 0, s, a);
+  else if (fill)
+    hipLaunchKernelGGL((k_raycast_all<true, false>), grid, block, 0, s, a);
+  else if (lo)
+    hipLaunchKernelGGL((k_raycast_all<false, true>), grid, block, 0, s, a);
   else
-    hipLaunchKernelGGL(k_raycast_all<false>, grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_raycast_all<false, false>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 

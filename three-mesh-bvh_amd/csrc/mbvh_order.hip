@@ -191,7 +191,8 @@ __global__ __launch_bounds__(kOrderBlock) void k_mesh_keys(const uint32_t* ray_i
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kOrderBlock;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kOrderBlock + threadIdx.x; i < n; i += stride) {
     const uint32_t inst = ray_instance[i];
-    uint32_t g = inst < n_instances ? instance_geometry[inst] : n_meshes;
+    // without an instance table the ray names its mesh directly
+    uint32_t g = !instance_geometry ? inst : (inst < n_instances ? instance_geometry[inst] : n_meshes);
     if (g > n_meshes) g = n_meshes;
     keys[i] = g;
     vals[i] = static_cast<uint32_t>(i);

@@ -496,6 +496,44 @@ static napi_value js_batch_raycast_first(napi_env env, napi_callback_info info) 
   return NULL;
 }
 
+/* batchRaycastFirstLocal(batch, origins, dirs, windows|null, rayMesh, side, outFace, outDist, outPoint, outBary):
+ * double-precision rays already in their mesh's frame (Float64Arrays), windows 2 per ray (near, far) */
+static napi_value js_batch_raycast_first_local(napi_env env, napi_callback_info info) {
+  napi_value argv[10];
+  get_args(env, info, 10, argv);
+  void* p = NULL;
+  if (napi_get_value_external(env, argv[0], &p) != napi_ok || !p || !((BatchHandle*)p)->batch)
+    return throw_msg(env, "batchRaycastFirstLocal: invalid batch");
+  BatchHandle* b = (BatchHandle*)p;
+  View o, d, w, rm;
+  if (!get_view(env, argv[1], &o) || !get_view(env, argv[2], &d) || !get_view(env, argv[4], &rm) ||
+      o.type != napi_float64_array || d.type != napi_float64_array || rm.type != napi_uint32_array ||
+      o.length != d.length || o.length != 3 * rm.length)
+    return throw_msg(env, "batchRaycastFirstLocal: origins / directions (Float64Array, 3n) and rayMesh (Uint32Array, n) do not match");
+  const size_t n = rm.length;
+  const double* win = NULL;
+  if (!is_nullish(env, argv[3])) {
+    if (!get_view(env, argv[3], &w) || w.type != napi_float64_array || w.length != 2 * n)
+      return throw_msg(env, "batchRaycastFirstLocal: windows must be a Float64Array of 2n (near, far)");
+    win = (const double*)w.data;
+  }
+  int32_t side;
+  if (!get_i32(env, argv[5], &side)) return throw_msg(env, "batchRaycastFirstLocal: side must be a number");
+  void *face, *dist, *pt, *bc;
+  if (!out_view(env, argv[6], napi_uint32_array, n, &face) || !face ||
+      !out_view(env, argv[7], napi_float32_array, n, &dist) || !out_view(env, argv[8], napi_float32_array, 3 * n, &pt) ||
+      !out_view(env, argv[9], napi_float32_array, 3 * n, &bc))
+    return throw_msg(env, "batchRaycastFirstLocal: output arrays have the wrong type or length");
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, b->device, &ctx);
+  if (!rc)
+    rc = mbvh_batch_raycast_first_local_f64(ctx, b->batch, (const double*)o.data, (const double*)d.data, win,
+                                            (const uint32_t*)rm.data, n, side, (uint32_t*)face, (float*)dist,
+                                            (float*)pt, (float*)bc);
+  if (rc) return throw_status(env, rc, "mbvh_batch_raycast_first_local_f64");
+  return NULL;
+}
+
 /* sceneCreate(device, [mesh, ...], matrixWorld, matrixWorldInverse, instanceGeometry, visible|null) -> scene */
 static napi_value js_scene_create(napi_env env, napi_callback_info info) {
   napi_value argv[6];
@@ -750,16 +788,26 @@ static int out_view(napi_env env, napi_value v, napi_typedarray_type want, size_
   return 1;
 }
 
-/* raycastFirst(mesh, origins, dirs, side, near, far, outFace, outDist, outPoint, outBary) */
+/* Float32Array or Float64Array rays of equal type and length 3n; *f64 tells which. */
+static int get_rays(napi_env env, napi_value vo, napi_value vd, View* o, View* d, int* f64) {
+  if (!get_view(env, vo, o) || !get_view(env, vd, d) || o->type != d->type || o->length != d->length ||
+      o->length % 3)
+    return 0;
+  *f64 = o->type == napi_float64_array;
+  return o->type == napi_float32_array || *f64;
+}
+
+/* raycastFirst(mesh, origins, dirs, side, near, far, outFace, outDist, outPoint, outBary); Float64Array
+ * rays are traced as the double rays (mbvh_raycast_first_f64). */
 static napi_value js_raycast_first(napi_env env, napi_callback_info info) {
   napi_value argv[10];
   get_args(env, info, 10, argv);
   MeshHandle* h = get_mesh(env, argv[0]);
   if (!h) return throw_msg(env, "raycastFirst: invalid mesh");
   View o, d;
-  if (!get_view(env, argv[1], &o) || !get_view(env, argv[2], &d) || o.type != napi_float32_array ||
-      d.type != napi_float32_array || o.length != d.length || o.length % 3)
-    return throw_msg(env, "raycastFirst: origins / directions must be Float32Arrays of equal length 3n");
+  int f64 = 0;
+  if (!get_rays(env, argv[1], argv[2], &o, &d, &f64))
+    return throw_msg(env, "raycastFirst: origins / directions must be Float32Arrays or Float64Arrays of equal length 3n");
   const size_t n = o.length / 3;
   int32_t side;
   double nr, fr;
@@ -772,10 +820,13 @@ static napi_value js_raycast_first(napi_env env, napi_callback_info info) {
     return throw_msg(env, "raycastFirst: output arrays have the wrong type or length");
   mbvh_ctx* ctx = NULL;
   int rc = context_for(env, h->device, &ctx);
-  if (!rc)
+  if (!rc && f64)
+    rc = mbvh_raycast_first_f64(ctx, h->mesh, (const double*)o.data, (const double*)d.data, n, side, nr, fr,
+                                (uint32_t*)face, (float*)dist, (float*)pt, (float*)bc);
+  else if (!rc)
     rc = mbvh_raycast_first(ctx, h->mesh, (const float*)o.data, (const float*)d.data, n, side, to_float(nr),
                             to_float(fr), (uint32_t*)face, (float*)dist, (float*)pt, (float*)bc);
-  if (rc) return throw_status(env, rc, "mbvh_raycast_first");
+  if (rc) return throw_status(env, rc, f64 ? "mbvh_raycast_first_f64" : "mbvh_raycast_first");
   return NULL;
 }
 
@@ -786,9 +837,9 @@ static napi_value js_raycast(napi_env env, napi_callback_info info) {
   MeshHandle* h = get_mesh(env, argv[0]);
   if (!h) return throw_msg(env, "raycast: invalid mesh");
   View o, d;
-  if (!get_view(env, argv[1], &o) || !get_view(env, argv[2], &d) || o.type != napi_float32_array ||
-      d.type != napi_float32_array || o.length != d.length || o.length % 3)
-    return throw_msg(env, "raycast: origins / directions must be Float32Arrays of equal length 3n");
+  int f64 = 0;
+  if (!get_rays(env, argv[1], argv[2], &o, &d, &f64))
+    return throw_msg(env, "raycast: origins / directions must be Float32Arrays or Float64Arrays of equal length 3n");
   const size_t n = o.length / 3;
   int32_t side;
   double nr, fr;
@@ -798,7 +849,9 @@ static napi_value js_raycast(napi_env env, napi_callback_info info) {
   int rc = context_for(env, h->device, &ctx);
   uint64_t* offs = (uint64_t*)calloc(n + 1, sizeof(uint64_t));
   if (!offs) return throw_msg(env, "raycast: out of memory");
-  if (!rc)
+  if (!rc && f64)
+    rc = mbvh_raycast_f64(ctx, h->mesh, (const double*)o.data, (const double*)d.data, n, side, nr, fr, offs);
+  else if (!rc)
     rc = mbvh_raycast(ctx, h->mesh, (const float*)o.data, (const float*)d.data, n, side, to_float(nr), to_float(fr), offs);
   if (rc) {
     free(offs);
@@ -830,15 +883,17 @@ static napi_value js_raycast(napi_env env, napi_callback_info info) {
   return result;
 }
 
-/* closestPoint(mesh, points, minT, maxT, outFace, outDist, outPoint, outBary) */
+/* closestPoint(mesh, points, minT, maxT, outFace, outDist, outPoint, outBary); Float64Array points are
+ * decided for the double points (mbvh_closest_point_f64). */
 static napi_value js_closest_point(napi_env env, napi_callback_info info) {
   napi_value argv[8];
   get_args(env, info, 8, argv);
   MeshHandle* h = get_mesh(env, argv[0]);
   if (!h) return throw_msg(env, "closestPoint: invalid mesh");
   View q;
-  if (!get_view(env, argv[1], &q) || q.type != napi_float32_array || q.length % 3)
-    return throw_msg(env, "closestPoint: points must be a Float32Array of length 3n");
+  if (!get_view(env, argv[1], &q) || (q.type != napi_float32_array && q.type != napi_float64_array) || q.length % 3)
+    return throw_msg(env, "closestPoint: points must be a Float32Array or Float64Array of length 3n");
+  const int f64 = q.type == napi_float64_array;
   const size_t n = q.length / 3;
   double mn, mx;
   if (!get_f64(env, argv[2], &mn) || !get_f64(env, argv[3], &mx))
@@ -850,10 +905,13 @@ static napi_value js_closest_point(napi_env env, napi_callback_info info) {
     return throw_msg(env, "closestPoint: output arrays have the wrong type or length");
   mbvh_ctx* ctx = NULL;
   int rc = context_for(env, h->device, &ctx);
-  if (!rc)
+  if (!rc && f64)
+    rc = mbvh_closest_point_f64(ctx, h->mesh, (const double*)q.data, n, mn, mx, (uint32_t*)face, (float*)dist,
+                                (float*)pt, (float*)bc);
+  else if (!rc)
     rc = mbvh_closest_point(ctx, h->mesh, (const float*)q.data, n, to_float(mn), to_float(mx), (uint32_t*)face,
                             (float*)dist, (float*)pt, (float*)bc);
-  if (rc) return throw_status(env, rc, "mbvh_closest_point");
+  if (rc) return throw_status(env, rc, f64 ? "mbvh_closest_point_f64" : "mbvh_closest_point");
   return NULL;
 }
 
@@ -1053,6 +1111,7 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("batchCreate", js_batch_create);
   EXPORT("freeBatch", js_free_batch);
   EXPORT("batchRaycastFirst", js_batch_raycast_first);
+  EXPORT("batchRaycastFirstLocal", js_batch_raycast_first_local);
   EXPORT("sceneCreate", js_scene_create);
   EXPORT("textureSizes", js_texture_sizes);
   EXPORT("intersectsBox", js_intersects_box);

@@ -35,6 +35,8 @@ EXPORTED_SYMBOLS = (
     "mbvh_mesh_refit_d", "mbvh_mesh_copy_roots", "mbvh_intersects_sphere",
     "mbvh_raycast_first", "mbvh_raycast_first_d",
     "mbvh_closest_point", "mbvh_closest_point_d", "mbvh_closest_point_noalloc_d",
+    "mbvh_raycast_first_f64", "mbvh_raycast_first_ff_d", "mbvh_closest_point_f64", "mbvh_closest_point_ff_d",
+    "mbvh_raycast_f64", "mbvh_batch_raycast_first_local_f64",
     "mbvh_raycast", "mbvh_raycast_fetch", "mbvh_raycast_count_d", "mbvh_raycast_fill_d",
     "mbvh_order_workspace_bytes", "mbvh_order_points_d",
     "mbvh_batch_create", "mbvh_batch_free", "mbvh_batch_workspace_bytes", "mbvh_batch_raycast_first_d",
@@ -115,6 +117,12 @@ def library() -> C.CDLL:
         "mbvh_closest_point": (C.c_int, [vp, vp, vp, u64, f32, f32, vp, vp, vp, vp]),
         "mbvh_closest_point_d": (C.c_int, [vp, vp, vp, u64, vp, f32, f32, vp, vp, vp, vp, vp]),
         "mbvh_closest_point_noalloc_d": (C.c_int, [vp, vp, vp, u64, f32, f32, vp, vp, vp, vp, vp]),
+        "mbvh_raycast_first_f64": (C.c_int, [vp, vp, vp, vp, u64, i32, f64, f64, vp, vp, vp, vp]),
+        "mbvh_raycast_first_ff_d": (C.c_int, [vp, vp, vp, vp, vp, vp, u64, vp, i32, f64, f64, vp, vp, vp, vp, vp]),
+        "mbvh_closest_point_f64": (C.c_int, [vp, vp, vp, u64, f64, f64, vp, vp, vp, vp]),
+        "mbvh_closest_point_ff_d": (C.c_int, [vp, vp, vp, vp, u64, vp, f64, f64, vp, vp, vp, vp, vp]),
+        "mbvh_raycast_f64": (C.c_int, [vp, vp, vp, vp, u64, i32, f64, f64, vp]),
+        "mbvh_batch_raycast_first_local_f64": (C.c_int, [vp, vp, vp, vp, vp, vp, u64, i32, vp, vp, vp, vp]),
         "mbvh_order_workspace_bytes": (C.c_int, [u64, C.POINTER(u64)]),
         "mbvh_order_points_d": (C.c_int, [vp, vp, u64, vp, vp, u64, vp]),
         "mbvh_batch_create": (C.c_int, [vp, C.POINTER(vp), u32, C.POINTER(vp)]),
@@ -467,8 +475,11 @@ class Mesh:
     # ---- host-pointer queries (synchronous) ----
     def raycast_first(self, origins: np.ndarray, directions: np.ndarray, side: int = FRONT_SIDE,
                       near: float = 0.0, far: float = float("inf")):
-        o = np.ascontiguousarray(origins, dtype=np.float32).reshape(-1, 3)
-        d = np.ascontiguousarray(directions, dtype=np.float32).reshape(-1, 3)
+        """float64 rays go to mbvh_raycast_first_f64 (traced as the double rays), others as float32."""
+        f64 = np.asarray(origins).dtype == np.float64 or np.asarray(directions).dtype == np.float64
+        dt = np.float64 if f64 else np.float32
+        o = np.ascontiguousarray(origins, dtype=dt).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, dtype=dt).reshape(-1, 3)
         if o.shape != d.shape:
             raise ValueError("origins and directions must have the same shape")
         n = o.shape[0]
@@ -476,19 +487,23 @@ class Mesh:
         dist = np.empty(n, np.float32)
         point = np.empty((n, 3), np.float32)
         bary = np.empty((n, 3), np.float32)
-        _check(self.ctx._lib.mbvh_raycast_first(self.ctx.handle, self.handle, _ptr(o), _ptr(d), n, side,
-                                                near, far, _ptr(face), _ptr(dist), _ptr(point), _ptr(bary)))
+        fn = self.ctx._lib.mbvh_raycast_first_f64 if f64 else self.ctx._lib.mbvh_raycast_first
+        _check(fn(self.ctx.handle, self.handle, _ptr(o), _ptr(d), n, side, near, far, _ptr(face), _ptr(dist),
+                  _ptr(point), _ptr(bary)))
         return {"face": face, "distance": dist, "point": point, "barycoord": bary}
 
     def raycast(self, origins: np.ndarray, directions: np.ndarray, side: int = FRONT_SIDE,
                 near: float = 0.0, far: float = float("inf")):
         """All hits: returns offsets (n+1) and packed per-hit arrays in the reference's order."""
-        o = np.ascontiguousarray(origins, dtype=np.float32).reshape(-1, 3)
-        d = np.ascontiguousarray(directions, dtype=np.float32).reshape(-1, 3)
+        f64 = np.asarray(origins).dtype == np.float64 or np.asarray(directions).dtype == np.float64
+        dt = np.float64 if f64 else np.float32
+        o = np.ascontiguousarray(origins, dtype=dt).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, dtype=dt).reshape(-1, 3)
         n = o.shape[0]
         offsets = np.zeros(n + 1, np.uint64)
         lib = self.ctx._lib
-        _check(lib.mbvh_raycast(self.ctx.handle, self.handle, _ptr(o), _ptr(d), n, side, near, far, _ptr(offsets)))
+        fn = lib.mbvh_raycast_f64 if f64 else lib.mbvh_raycast
+        _check(fn(self.ctx.handle, self.handle, _ptr(o), _ptr(d), n, side, near, far, _ptr(offsets)))
         total = int(offsets[-1])
         face = np.empty(total, np.uint32)
         dist = np.empty(total, np.float32)
@@ -499,14 +514,16 @@ class Mesh:
 
     def closest_point(self, points: np.ndarray, min_threshold: float = 0.0,
                       max_threshold: float = float("inf")):
-        q = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        f64 = np.asarray(points).dtype == np.float64
+        q = np.ascontiguousarray(points, dtype=np.float64 if f64 else np.float32).reshape(-1, 3)
         n = q.shape[0]
         face = np.empty(n, np.uint32)
         dist = np.empty(n, np.float32)
         point = np.empty((n, 3), np.float32)
         bary = np.empty((n, 3), np.float32)
-        _check(self.ctx._lib.mbvh_closest_point(self.ctx.handle, self.handle, _ptr(q), n, min_threshold,
-                                                max_threshold, _ptr(face), _ptr(dist), _ptr(point), _ptr(bary)))
+        fn = self.ctx._lib.mbvh_closest_point_f64 if f64 else self.ctx._lib.mbvh_closest_point
+        _check(fn(self.ctx.handle, self.handle, _ptr(q), n, min_threshold, max_threshold, _ptr(face), _ptr(dist),
+                  _ptr(point), _ptr(bary)))
         return {"face": face, "distance": dist, "point": point, "barycoord": bary}
 
     def intersects_sphere(self, centers: np.ndarray, radii: np.ndarray) -> np.ndarray:
@@ -552,6 +569,25 @@ class Mesh:
             min_threshold, max_threshold,
             C.c_void_p(face_ptr), C.c_void_p(dist_ptr or None), C.c_void_p(point_ptr or None),
             C.c_void_p(bary_ptr or None), C.c_void_p(stream or None)))
+
+    def raycast_first_ff_d(self, origins_ptr: int, origins_lo_ptr: int, dirs_ptr: int, dirs_lo_ptr: int, n: int,
+                           face_ptr: int, dist_ptr: int = 0, point_ptr: int = 0, bary_ptr: int = 0,
+                           side: int = FRONT_SIDE, near: float = 0.0, far: float = float("inf"), stream: int = 0,
+                           order_ptr: int = 0) -> None:
+        """mbvh_raycast_first_ff_d: double rays split by the caller (hi, lo float arrays on the device)."""
+        v = lambda p: C.c_void_p(p or None)
+        _check(self.ctx._lib.mbvh_raycast_first_ff_d(
+            self.ctx.handle, self.handle, v(origins_ptr), v(origins_lo_ptr), v(dirs_ptr), v(dirs_lo_ptr), n,
+            v(order_ptr), side, near, far, v(face_ptr), v(dist_ptr), v(point_ptr), v(bary_ptr), v(stream)))
+
+    def closest_point_ff_d(self, points_ptr: int, points_lo_ptr: int, n: int, face_ptr: int, dist_ptr: int = 0,
+                           point_ptr: int = 0, bary_ptr: int = 0, min_threshold: float = 0.0,
+                           max_threshold: float = float("inf"), stream: int = 0, order_ptr: int = 0) -> None:
+        """mbvh_closest_point_ff_d: double points split by the caller (hi, lo float arrays on the device)."""
+        v = lambda p: C.c_void_p(p or None)
+        _check(self.ctx._lib.mbvh_closest_point_ff_d(
+            self.ctx.handle, self.handle, v(points_ptr), v(points_lo_ptr), n, v(order_ptr), min_threshold,
+            max_threshold, v(face_ptr), v(dist_ptr), v(point_ptr), v(bary_ptr), v(stream)))
 
     def closest_point_noalloc_d(self, points_ptr: int, n: int, face_ptr: int, dist_ptr: int = 0, point_ptr: int = 0,
                                 bary_ptr: int = 0, min_threshold: float = 0.0, max_threshold: float = float("inf"),
@@ -684,6 +720,24 @@ class Batch:
         _check(self.ctx._lib.mbvh_batch_raycast_first(
             self.ctx.handle, self.handle, _ptr(mw), _ptr(mi), _ptr(ig), mw.shape[0], _ptr(o), _ptr(d), _ptr(ri), n,
             side, near, far, _ptr(out["face"]), _ptr(out["distance"]), _ptr(out["point"]), _ptr(out["barycoord"])))
+        return out
+
+    def raycast_first_local(self, origins: np.ndarray, directions: np.ndarray, ray_mesh: np.ndarray,
+                            windows: Optional[np.ndarray] = None, side: int = FRONT_SIDE):
+        """mbvh_batch_raycast_first_local_f64: double rays already in their mesh's frame, ray_mesh naming
+        the batch mesh of each ray, windows (n, 2) near / far per ray (None: 0, inf); local results."""
+        o = np.ascontiguousarray(origins, np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, np.float64).reshape(-1, 3)
+        rm = np.ascontiguousarray(ray_mesh, np.uint32).reshape(-1)
+        if o.shape != d.shape or o.shape[0] != rm.shape[0]:
+            raise ValueError("inconsistent local ray inputs")
+        w = None if windows is None else np.ascontiguousarray(windows, np.float64).reshape(-1, 2)
+        n = o.shape[0]
+        out = {"face": np.empty(n, np.uint32), "distance": np.empty(n, np.float32),
+               "point": np.empty((n, 3), np.float32), "barycoord": np.empty((n, 3), np.float32)}
+        _check(self.ctx._lib.mbvh_batch_raycast_first_local_f64(
+            self.ctx.handle, self.handle, _ptr(o), _ptr(d), _ptr(w), _ptr(rm), n, side, _ptr(out["face"]),
+            _ptr(out["distance"]), _ptr(out["point"]), _ptr(out["barycoord"])))
         return out
 
     def autotune(self, p: dict) -> BatchTuning:

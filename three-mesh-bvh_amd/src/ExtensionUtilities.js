@@ -68,53 +68,60 @@ function releaseBatch(self) {
 }
 
 // firstHitOnly on a BatchedMesh with one material: every visible instance in ONE device launch
-// (mbvh_batch_raycast_first), instead of one raycastFirst call per instance. The frame change is
-// raycastObject3D's, done on the device in FP32 from the float32 matrices (the per-instance path
-// does it in FP64 and rounds the local ray), so hits agree with that path up to that rounding.
+// instead of one raycastFirst call per instance. Each instance's local ray is raycastObject3D's
+// (src/core/MeshBVH.js:456-468) computed here in double precision exactly as the reference computes
+// it (matrixWorld * instanceMatrix, Matrix4.invert, Ray.applyMatrix4, near / far over the scale
+// factor); the launch traces those double rays (mbvh_batch_raycast_first_local_f64) and each hit goes
+// back to world space as convertRaycastIntersect does (src/utils/GeometryRayIntersectUtilities.js:3-17).
 function batchedFirstHits(self, raycaster, intersects) {
   const b = batchFor(self);
   if (!b) return;
   const drawInfo = self._drawInfo || self._instanceInfo;
   const matrixWorld = self.matrixWorld.elements;
   const tmp = self.matrixWorld.clone ? self.matrixWorld.clone() : { elements: new Array(16) };
-  const ids = [];
-  const mats = [];
+  const ro = raycaster.ray.origin, rd = raycaster.ray.direction;
+  const inst = [];
   for (let i = 0, l = drawInfo.length; i < l; i++) {
     if (!self.getVisibleAt(i)) continue;
     const g = drawInfo[i].geometryIndex;
     if (!self.boundsTrees[g]) continue;
     self.getMatrixAt(i, tmp);
     const world = M.multiply(matrixWorld, tmp.elements);
-    ids.push(i);
-    mats.push({ g, world, inv: M.invert(world) });
+    const inv = M.invert(world);
+    const lo = M.applyToPoint(inv, ro);
+    const ld = M.transformDirection(inv, rd);
+    const s = M.matrixScale(world);
+    const scaleFactor = M.length3(ld[0] * s[0], ld[1] * s[1], ld[2] * s[2]);
+    inst.push({ i, g, world, lo, ld, near: raycaster.near / scaleFactor, far: raycaster.far / scaleFactor });
   }
-  const n = ids.length;
+  const n = inst.length;
   if (!n) return;
-  const mw = new Float32Array(16 * n), mi = new Float32Array(16 * n), ig = new Uint32Array(n);
-  const o = new Float32Array(3 * n), d = new Float32Array(3 * n), ri = new Uint32Array(n);
-  const ro = raycaster.ray.origin, rd = raycaster.ray.direction;
+  const o = new Float64Array(3 * n), d = new Float64Array(3 * n), win = new Float64Array(2 * n);
+  const mesh = new Uint32Array(n);
   for (let k = 0; k < n; k++) {
-    mw.set(mats[k].world, 16 * k);
-    mi.set(mats[k].inv, 16 * k);
-    ig[k] = b.slot[mats[k].g];
-    o[3 * k] = ro.x; o[3 * k + 1] = ro.y; o[3 * k + 2] = ro.z;
-    d[3 * k] = rd.x; d[3 * k + 1] = rd.y; d[3 * k + 2] = rd.z;
-    ri[k] = k;
+    const it = inst[k];
+    o.set(it.lo, 3 * k);
+    d.set(it.ld, 3 * k);
+    win[2 * k] = it.near;
+    win[2 * k + 1] = it.far;
+    mesh[k] = b.slot[it.g];
   }
   const face = new Uint32Array(n), dist = new Float32Array(n), pt = new Float32Array(3 * n), bc = new Float32Array(3 * n);
-  native.load().batchRaycastFirst(b.batch, mw, mi, ig, o, d, ri, sideOf(self.material), raycaster.near, raycaster.far,
-    face, dist, pt, bc);
+  native.load().batchRaycastFirstLocal(b.batch, o, d, win, mesh, sideOf(self.material), face, dist, pt, bc);
   for (let k = 0; k < n; k++) {
     if (face[k] === C.NO_HIT) continue;
-    const tree = self.boundsTrees[mats[k].g];
-    // face data in the instance's frame (three's intersection keeps face / normals local)
-    const lo = M.applyToPoint(mats[k].inv, ro);
-    const ld = M.transformDirection(mats[k].inv, rd);
-    const localRay = { origin: M.makeVec3(ro, lo[0], lo[1], lo[2]), direction: M.makeVec3(rd, ld[0], ld[1], ld[2]) };
+    const it = inst[k];
+    const tree = self.boundsTrees[it.g];
+    const localRay = { origin: M.makeVec3(ro, it.lo[0], it.lo[1], it.lo[2]),
+      direction: M.makeVec3(rd, it.ld[0], it.ld[1], it.ld[2]) };
     const hit = tree._makeHit(localRay, self.material, face[k], dist[k], [pt[3 * k], pt[3 * k + 1], pt[3 * k + 2]],
       [bc[3 * k], bc[3 * k + 1], bc[3 * k + 2]]);
+    // convertRaycastIntersect: the point to world space, the distance from the world ray's origin
+    const pw = M.applyToPoint(it.world, hit.point);
+    hit.point.set(pw[0], pw[1], pw[2]);
+    hit.distance = hit.point.distanceTo(ro);
     hit.object = self;
-    hit.batchId = ids[k];
+    hit.batchId = it.i;
     intersects.push(hit);
   }
 }

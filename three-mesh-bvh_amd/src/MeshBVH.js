@@ -23,6 +23,17 @@ function sideMask(side) {
   return 1;
 }
 
+// Query arrays: a Float32Array is traced as FP32 values; a Float64Array (or a plain array of JS
+// numbers) as the doubles it holds, every decision the reference would make for those doubles.
+function queryArray(a) {
+  return a instanceof Float32Array || a instanceof Float64Array ? a : Float64Array.from(a);
+}
+function queryArrays(a, b) {
+  const x = queryArray(a), y = queryArray(b);
+  if (x.constructor === y.constructor) return [x, y];
+  return [Float64Array.from(x), Float64Array.from(y)];
+}
+
 function f32OrNull(a) {
   if (a === undefined || a === null) return null;
   return a instanceof Float32Array ? a : Float32Array.from(a);
@@ -238,11 +249,11 @@ class MeshBVH {
 
   // ---- batch queries ---------------------------------------------------------------------------
 
-  // First hit for n rays: origins / directions are Float32Arrays of length 3n. Returns packed
+  // First hit for n rays: origins / directions are arrays of length 3n (Float32Array: FP32 rays;
+  // Float64Array or plain arrays: double rays, traced as the reference traces them). Returns packed
   // results; faceIndex is 0xFFFFFFFF and distance Infinity for a miss.
   raycastFirstBatch(origins, directions, options = {}) {
-    const o = origins instanceof Float32Array ? origins : Float32Array.from(origins);
-    const d = directions instanceof Float32Array ? directions : Float32Array.from(directions);
+    const [o, d] = queryArrays(origins, directions);
     const n = o.length / FLOATS_PER_VEC;
     const side = this._sideFor(options.side !== undefined ? options.side : (options.material !== undefined ? options.material : C.FrontSide));
     const out = {
@@ -257,16 +268,16 @@ class MeshBVH {
   // All hits for n rays, in the reference's traversal order: hits of ray i are entries
   // offsets[i] .. offsets[i + 1] - 1 of the packed arrays.
   raycastBatch(origins, directions, options = {}) {
-    const o = origins instanceof Float32Array ? origins : Float32Array.from(origins);
-    const d = directions instanceof Float32Array ? directions : Float32Array.from(directions);
+    const [o, d] = queryArrays(origins, directions);
     const side = this._sideFor(options.side !== undefined ? options.side : (options.material !== undefined ? options.material : C.FrontSide));
     return native.load().raycast(this._mesh(), o, d, side, options.near !== undefined ? options.near : 0,
       options.far !== undefined ? options.far : Infinity);
   }
 
-  // Closest point for n query points (Float32Array of length 3n).
+  // Closest point for n query points (length 3n; Float32Array: FP32 points, Float64Array or plain
+  // arrays: double points).
   closestPointToPointBatch(points, options = {}) {
-    const q = points instanceof Float32Array ? points : Float32Array.from(points);
+    const q = queryArray(points);
     const n = q.length / FLOATS_PER_VEC;
     const out = {
       faceIndex: new Uint32Array(n), distance: new Float32Array(n),
@@ -401,8 +412,9 @@ class MeshBVH {
   // ---- the reference's per-call API ------------------------------------------------------------
 
   raycastFirst(ray, materialOrSide = C.FrontSide, near = 0, far = Infinity) {
-    const o = new Float32Array([ray.origin.x, ray.origin.y, ray.origin.z]);
-    const d = new Float32Array([ray.direction.x, ray.direction.y, ray.direction.z]);
+    // the ray's JS numbers as they are (the reference traces them in double precision)
+    const o = new Float64Array([ray.origin.x, ray.origin.y, ray.origin.z]);
+    const d = new Float64Array([ray.direction.x, ray.direction.y, ray.direction.z]);
     const r = this.raycastFirstBatch(o, d, { side: materialOrSide, near, far });
     if (r.faceIndex[0] === C.NO_HIT) return null;
     return this._makeHit(ray, materialOrSide, r.faceIndex[0], r.distance[0],
@@ -412,8 +424,8 @@ class MeshBVH {
   // All hits of one ray (src/core/MeshBVH.js:532-545). With a material array a triangle yields
   // one hit per group whose material admits it, in group order, as intersectTri does.
   raycast(ray, materialOrSide = C.FrontSide, near = 0, far = Infinity) {
-    const o = new Float32Array([ray.origin.x, ray.origin.y, ray.origin.z]);
-    const d = new Float32Array([ray.direction.x, ray.direction.y, ray.direction.z]);
+    const o = new Float64Array([ray.origin.x, ray.origin.y, ray.origin.z]);
+    const d = new Float64Array([ray.direction.x, ray.direction.y, ray.direction.z]);
     const r = this.raycastBatch(o, d, { side: materialOrSide, near, far });
     const hits = [];
     for (let k = 0; k < r.faceIndex.length; k++) {
@@ -446,7 +458,7 @@ class MeshBVH {
   }
 
   closestPointToPoint(point, target = {}, minThreshold = 0, maxThreshold = Infinity) {
-    const q = new Float32Array([point.x, point.y, point.z]);
+    const q = new Float64Array([point.x, point.y, point.z]);
     const r = this.closestPointToPointBatch(q, { minThreshold, maxThreshold });
     if (r.faceIndex[0] === C.NO_HIT) return null;
     if (!target.point) target.point = M.makeVec3(point, r.point[0], r.point[1], r.point[2]);
@@ -465,7 +477,7 @@ class MeshBVH {
     const lo = M.applyToPoint(inv, raycaster.ray.origin);
     const ld = M.transformDirection(inv, raycaster.ray.direction);
     const s = M.matrixScale(mw);
-    const scaleFactor = Math.hypot(ld[0] * s[0], ld[1] * s[1], ld[2] * s[2]);
+    const scaleFactor = M.length3(ld[0] * s[0], ld[1] * s[1], ld[2] * s[2]);
     const near = raycaster.near / scaleFactor;
     const far = raycaster.far / scaleFactor;
     const localRay = {

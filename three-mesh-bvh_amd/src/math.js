@@ -113,19 +113,25 @@ function applyToPoint(e, v) {
     (e[2] * x + e[6] * y + e[10] * z + e[14]) * w];
 }
 
-// Vector3.transformDirection: upper 3x3 then normalize.
+// Vector3.length: the square root of the sum of squares (not Math.hypot, which rounds differently).
+function length3(x, y, z) {
+  return Math.sqrt(x * x + y * y + z * z);
+}
+
+// Vector3.transformDirection: upper 3x3, then normalize (Vector3.divideScalar multiplies by the
+// reciprocal of the length, or of 1 for a zero vector).
 function transformDirection(e, v) {
   const x = v.x, y = v.y, z = v.z;
   const rx = e[0] * x + e[4] * y + e[8] * z;
   const ry = e[1] * x + e[5] * y + e[9] * z;
   const rz = e[2] * x + e[6] * y + e[10] * z;
-  const len = Math.sqrt(rx * rx + ry * ry + rz * rz) || 1;
-  return [rx / len, ry / len, rz / len];
+  const inv = 1 / (length3(rx, ry, rz) || 1);
+  return [rx * inv, ry * inv, rz * inv];
 }
 
 // Vector3.setFromMatrixScale: lengths of the basis columns.
 function matrixScale(e) {
-  return [Math.hypot(e[0], e[1], e[2]), Math.hypot(e[4], e[5], e[6]), Math.hypot(e[8], e[9], e[10])];
+  return [length3(e[0], e[1], e[2]), length3(e[4], e[5], e[6]), length3(e[8], e[9], e[10])];
 }
 
 // Triangle.getNormal: normalize((c - b) x (a - b)); zero for a degenerate triangle.
@@ -146,5 +152,5 @@ function triangleNormal(a, b, c) {
 }
 
 module.exports = {
-  Vec3, makeVec3, makeVec2, invert, multiply, applyToPoint, transformDirection, matrixScale, triangleNormal,
+  Vec3, makeVec3, makeVec2, invert, multiply, applyToPoint, transformDirection, matrixScale, triangleNormal, length3,
 };
