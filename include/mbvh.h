@@ -122,9 +122,13 @@ void mbvh_ctx_destroy(mbvh_ctx* ctx);
  * tree built at upload, or 0 (default): 4 lanes when all rays of a call fit in one resident round
  * of the device, else 1. Results are the same up to exactly equal hits (ties). */
 int mbvh_ctx_set_ray_lanes(mbvh_ctx* ctx, int32_t lanes);
-/* closestPoint traversal width: 1 lane per query on the binary tree, or 4 lanes per query on the
- * 4-wide tree; 0 (default) = 4. Calls with a nonzero minThreshold always use 1 lane: the
- * reference returns the first triangle under minThreshold in its own (binary) visit order. */
+/* closestPoint traversal width: 1 lane per query on the binary tree, 4 lanes per query on the
+ * 4-wide tree, or 64: packets of 64 queries (consecutive in the spatial visit order) walking the
+ * 4-wide tree together, one lane each, node records and triangles read once per packet (faster for
+ * coherent queries such as grid samples, slower for scattered ones; trees too deep for its stack
+ * use 4); 0 (default) = 4. Widths 4 and 64 give the same results bit for bit. Calls with a
+ * nonzero minThreshold always use 1 lane: the reference returns the first triangle under
+ * minThreshold in its own (binary) visit order. */
 int mbvh_ctx_set_point_lanes(mbvh_ctx* ctx, int32_t lanes);
 /* Waits for all work the context enqueued on its own stream. */
 int mbvh_ctx_synchronize(mbvh_ctx* ctx);

@@ -225,11 +225,11 @@ def test_query_order_changes_nothing(ctx):
     mesh.free()
 
 
-@pytest.mark.parametrize("lanes", [1, 4])
+@pytest.mark.parametrize("lanes", [1, 4, 64])
 def test_lanes_per_query(lanes):
-    """One lane per query (binary tree) and four lanes per query (4-wide tree) both meet the parity
-    rule on the C2 mesh; with a minThreshold the call keeps one lane (the reference's early exit
-    follows its own visit order)."""
+    """One lane per query (binary tree), four lanes per query (4-wide tree) and packets of 64 queries
+    walking the 4-wide tree together all meet the parity rule on the C2 mesh; with a minThreshold the
+    call keeps one lane (the reference's early exit follows its own visit order)."""
     c = M.Context(0)
     c.set_point_lanes(lanes)
     try:
@@ -240,3 +240,36 @@ def test_lanes_per_query(lanes):
         _run(c, w.geometry, pts, min_t=0.5, label=f"minThreshold lanes={lanes}")
     finally:
         c.close()
+
+
+def test_packets_equal_quads_bit_for_bit():
+    """Packets of 64 queries return exactly the quad kernel's results (faces, distances, points,
+    barycoords, bit for bit): grid and random points, double-precision points, a finite maxThreshold,
+    several roots (groups), an indirect mesh, a ragged last packet, NaN and far points."""
+    g = G.torus_knot(10, 3, 200, 40)
+    lo, hi = g.bounds()
+    rng = np.random.default_rng(11)
+    grid = W.grid_points(lo, hi, 30)  # 27,000 points: the last packet is ragged
+    rand = (lo + (hi - lo) * rng.random((20001, 3))).astype(np.float32)
+    rand[::997] = np.nan
+    rand[5] = [1e6, -1e6, 3e5]
+    dbl = rand.astype(np.float64) + rng.standard_normal(rand.shape) * 1e-9
+    ntri = g.tri_count
+    meshes = [("plain", dict()), ("groups", dict(ranges=[(k * ntri // 3, ntri // 3) for k in range(3)])),
+              ("indirect", dict(ranges=[(0, ntri // 2), (ntri // 2, ntri - ntri // 2)], indirect=True))]
+    for name, kw in meshes:
+        gg, idx, tree = build_mesh(g, **kw)
+        out = {}
+        for lanes in (4, 64):
+            c = M.Context(0)
+            c.set_point_lanes(lanes)
+            mesh = c.upload(tree, gg.positions, idx)
+            out[lanes] = [mesh.closest_point(grid), mesh.closest_point(rand), mesh.closest_point(dbl),
+                          mesh.closest_point(rand, 0.0, 2.0)]
+            mesh.free()
+            c.close()
+        for k, (a4, a64) in enumerate(zip(out[4], out[64])):
+            for key in ("face", "distance", "point", "barycoord"):
+                assert np.array_equal(a4[key].view(np.uint32) if a4[key].dtype != np.uint32 else a4[key],
+                                      a64[key].view(np.uint32) if a64[key].dtype != np.uint32 else a64[key]), (name, k, key)
+        print(name, "packets == quads on", [len(x["face"]) for x in out[64]], flush=True)

@@ -575,8 +575,8 @@ extern "C" int mbvh_ctx_set_ray_lanes(mbvh_ctx* ctx, int32_t lanes) {
 extern "C" int mbvh_ctx_set_point_lanes(mbvh_ctx* ctx, int32_t lanes) {
   clear_error();
   if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_ctx_set_point_lanes: invalid context handle");
-  if (lanes != 0 && lanes != 1 && lanes != 4)
-    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_ctx_set_point_lanes: lanes must be 0 (automatic), 1 or 4");
+  if (lanes != 0 && lanes != 1 && lanes != 4 && lanes != 64)
+    return set_error(MBVH_ERR_BAD_ARGS, "mbvh_ctx_set_point_lanes: lanes must be 0 (automatic), 1, 4 or 64");
   ctx->point_lanes = lanes;
   return MBVH_OK;
 }
@@ -1003,13 +1003,16 @@ int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uin
                        float max_t, uint32_t* face, float* dist, float* point, float* bary, hipStream_t s,
                        const FixedScratch* fixed = nullptr, const LoInputs& lo = LoInputs{}) {
   if (n && (!q || !face)) return set_error(MBVH_ERR_BAD_ARGS, "closest_point: NULL point or face buffer");
-  // four lanes per query only without a minThreshold (its early exit follows the binary order)
-  const bool quads = min_t == 0.0f && (ctx->point_lanes == 4 || ctx->point_lanes == 0);
+  // packets and quads only without a minThreshold (its early exit follows the binary order); packets
+  // only where the tree's 4-wide stack fits the packet kernel's
+  const bool packet = !fixed && min_t == 0.0f && ctx->point_lanes == 64 && mesh->max_stack4 <= kPacketStackEntries;
+  const bool quads = min_t == 0.0f && !packet && (ctx->point_lanes == 4 || ctx->point_lanes == 0 || ctx->point_lanes == 64);
   uint64_t threads = quads ? launch_threads(4 * n, ctx->max_threads) : launch_threads(n, ctx->max_threads);
   ClosestPointArgs a{};
   if (fixed) {
-    const uint32_t need = quads ? (mesh->max_stack4 > kQuadLdsStack ? mesh->max_stack4 - kQuadLdsStack : 0)
-                                : (mesh->max_stack > kStackLds ? mesh->max_stack - kStackLds : 0);
+    const uint32_t need = packet ? 0
+                          : quads ? (mesh->max_stack4 > kQuadLdsStack ? mesh->max_stack4 - kQuadLdsStack : 0)
+                                  : (mesh->max_stack > kStackLds ? mesh->max_stack - kStackLds : 0);
     const uint64_t per_thread = quads ? 4 : 1;  // threads per stack (one stack per quad)
     if (need) {
       const uint64_t stacks = fixed->ovf_bytes / (uint64_t{need} * sizeof(uint2));
@@ -1025,7 +1028,8 @@ int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uin
     a.stack.ovf_entries = need;
     a.queue = fixed->queue;
   } else {
-    if (quads ? mesh->max_stack4 > kQuadLdsStack + ctx->stack_ovf_entries : mesh->max_stack > kStackLds + ctx->stack_ovf_entries)
+    if (!packet && (quads ? mesh->max_stack4 > kQuadLdsStack + ctx->stack_ovf_entries
+                          : mesh->max_stack > kStackLds + ctx->stack_ovf_entries))
       return set_error(MBVH_ERR_LIMIT, "closest_point: stack scratch smaller than the tree depth");
     a.stack = stack_of(ctx, quads ? threads / 4 : threads);
     a.queue = static_cast<uint32_t*>(ctx->queue.ptr);
@@ -1043,7 +1047,11 @@ int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uin
   a.out_dist = dist;
   a.out_point = point;
   a.out_bary = bary;
-  if (quads)
+  if (packet) {
+    a.tile0 = 0;
+    a.tiles = (n + 63) / 64;
+    MBVH_HIP(launch_closest_point_packet(a, threads, s));
+  } else if (quads)
     MBVH_HIP(launch_closest_point4(a, threads, s));
   else
     MBVH_HIP(launch_closest_point(a, threads, s));

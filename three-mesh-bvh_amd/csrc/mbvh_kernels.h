@@ -88,6 +88,9 @@ struct ClosestPointArgs {
   float* out_dist;
   float* out_point;
   float* out_bary;
+  // packet kernel only: the launch covers packets [tile0, tile0 + tiles) of 64 queries in visit order
+  uint64_t tile0;
+  uint64_t tiles;
 };
 
 // ---- refit ------------------------------------------------------------------------------------
@@ -440,6 +443,10 @@ hipError_t launch_raycast_first4(const RaycastFirstArgs& a, uint64_t quads, hipS
 hipError_t launch_closest_point(const ClosestPointArgs& a, uint64_t threads, hipStream_t s);
 // four lanes per query on the 4-wide tree (minThreshold must be 0); `threads` lanes resident
 hipError_t launch_closest_point4(const ClosestPointArgs& a, uint64_t threads, hipStream_t s);
+// One wave of 64 queries per packet walking the 4-wide tree together (coherent queries); the tree's
+// 4-wide stack must fit kPacketStackEntries.
+constexpr uint32_t kPacketStackEntries = 64;
+hipError_t launch_closest_point_packet(const ClosestPointArgs& a, uint64_t threads, hipStream_t s);
 // fill == false: count hits per ray into a.counts; fill == true: write them at a.offsets.
 hipError_t launch_raycast_all(const RaycastAllArgs& a, bool fill, uint64_t threads, hipStream_t s);
 // Spatial order of n packed float triples (Morton bucket sort); workspace of

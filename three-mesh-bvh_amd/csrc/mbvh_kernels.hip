@@ -1718,6 +1718,215 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
   }
 }
 
+// ------------------------------------------------------------------------------------------
+// Closest point for coherent queries (grid samples, Morton-ordered point sets): a packet of 64
+// queries, one per lane of a wave, walks the 4-wide tree together. A child is entered when any
+// lane's query may need it (its squared box distance under that lane's upper bound and under
+// maxThreshold^2); node records and triangles are read at wave-uniform addresses (scalar loads),
+// so the node and leaf steps run with every lane busy instead of one quad in three. Each lane keeps
+// its own bound and candidate list and resolves them exactly as the quad kernel does, so the
+// results are bit for bit those of k_closest_point4: the packet's order only steers the search.
+// Children are ordered by the scores of one lane near the middle of the packet. The stack (the
+// children's own 32-byte records, re-scored per lane on pop) lives in LDS; the host uses this
+// kernel only for trees whose 4-wide stack fits it.
+// ------------------------------------------------------------------------------------------
+constexpr int kPacketStack = static_cast<int>(kPacketStackEntries);
+
+__device__ __forceinline__ float4 pick4(uint32_t k, float4 a, float4 b, float4 c, float4 d) {
+  return k == 0 ? a : (k == 1 ? b : (k == 2 ? c : d));
+}
+
+typedef __attribute__((address_space(4))) const float ConstF;  // read-only for the launch: scalar loads
+__device__ __forceinline__ float4 cload4(ConstF* p) { return make_float4(p[0], p[1], p[2], p[3]); }
+
+template <bool kLo>
+__global__ __launch_bounds__(kBlock, 8) void k_closest_point_packet(ClosestPointArgs A) {
+  __shared__ float4 pstack[(kBlock / 64) * kPacketStack * 2];
+  const uint32_t lane = threadIdx.x, wl = lane & 63u;
+  float4* const stk = pstack + (lane >> 6) * kPacketStack * 2;
+  const MeshView M = A.mesh;
+  const TieTree T = tie_tree(M);
+  ConstF* const nodes4 = (ConstF*)(M.nodes4);
+  ConstF* const tris = (ConstF*)(M.tris);
+  const bool max_finite = A.max_threshold < kInf;
+  const ff max_t2 = max_finite ? thr_sq(A.max_threshold, A.max_lo) : ff{kInf, 0.0f};
+  constexpr float kScoreSlack = 1.0f - 4.0f * kU;
+  const float qnan = __int_as_float(0x7fc00000);
+
+  while (true) {
+    uint32_t tile = 0;
+    if (wl == 0) tile = atomicAdd(A.queue, 1u);
+    tile = __builtin_amdgcn_readfirstlane(__shfl(tile, 0, 64));
+    if (tile >= A.tiles || (A.tile0 + tile) * 64 >= A.n) break;  // every wave reaches this exit
+    const uint64_t visit = (A.tile0 + tile) * 64 + wl;
+    uint64_t qi = visit;
+    if (visit < A.n && A.order) qi = A.order[visit];
+    const bool active = visit < A.n && qi < A.n;
+    f3 q = mk(qnan, qnan, qnan), ql = mk(0.0f, 0.0f, 0.0f);
+    if (active) {
+      q = mk(A.points[3 * qi + 0], A.points[3 * qi + 1], A.points[3 * qi + 2]);
+      if (kLo) ql = load3(A.points_lo, qi);
+    }
+    const float e_abs = query_error<kLo>(q, ql, *M.coord_max);
+    float U = active ? kInf : -1.0f;  // this lane's upper bound of its best squared distance
+    const uint64_t act = __ballot(active);
+    if (act == 0) continue;
+    // the lane whose scores order the children: lane 32 if its query is live, else the first live one
+    const uint32_t leader = (act >> 32) & 1u ? 32u : static_cast<uint32_t>(__builtin_ctzll(act));
+    Cands cl;
+    cl.lane = lane;
+    cl.n = 0;
+    bool overflow = false;
+    for (uint32_t r = 0; r < M.n_roots && !overflow; r++) {
+      uint32_t ref = __builtin_amdgcn_readfirstlane(M.roots4[r]);
+      int sp = 0;
+      // the current node's own box record (a root has none: roots are never bounds-tested)
+      float4 cb0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), cb1 = cb0;
+      bool has_box = false;
+      while (ref != kDone) {
+        if (!(ref & kRefLeaf)) {
+          // ---- node step: the four children, scored by every lane ----
+          ConstF* const nd = nodes4 + 32ull * ref;
+          uint32_t keys[4];
+          uint64_t need_any = 0;
+          float4 rec0[4], rec1[4];
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            rec0[k] = cload4(nd + 8 * k);
+            rec1[k] = cload4(nd + 8 * k + 4);
+            const uint32_t cref = __float_as_uint(rec1[k].z);
+            const float raw = box_score(q, rec0[k].x, rec0[k].y, rec0[k].z, rec0[k].w, rec1[k].x, rec1[k].y);
+            bool in = cref != kDone && raw * kScoreSlack <= U;
+            if (in && max_finite)
+              in = below_max<kLo>(true, raw, max_t2, q, ql, rec0[k].x, rec0[k].y, rec0[k].z, rec0[k].w, rec1[k].x,
+                                  rec1[k].y);
+            const uint64_t m = __ballot(in);
+            need_any |= m ? (1ull << k) : 0ull;
+            // the leader's score (bits of a non-negative float order as unsigned) with the child's index
+            const uint32_t kb = __builtin_amdgcn_readlane(__float_as_uint(raw), leader);
+            keys[k] = m ? ((kb & ~3u) | static_cast<uint32_t>(k)) : 0xFFFFFFFFu;
+          }
+          // sort the (at most four) entered children nearest first
+#define MBVH_CSWAP(i, j)                  \
+  {                                       \
+    const uint32_t lo_ = min(keys[i], keys[j]); \
+    keys[j] = max(keys[i], keys[j]);      \
+    keys[i] = lo_;                        \
+  }
+          MBVH_CSWAP(0, 1) MBVH_CSWAP(2, 3) MBVH_CSWAP(0, 2) MBVH_CSWAP(1, 3) MBVH_CSWAP(1, 2)
+#undef MBVH_CSWAP
+          const int nh = __builtin_popcount(static_cast<uint32_t>(need_any));
+          if (nh == 0) {
+            ref = kDone;
+          } else {
+            if (sp + nh - 1 > kPacketStack) {
+              overflow = true;
+              break;
+            }
+            // push the farther entered children (farthest deepest), continue with the nearest
+            for (int i = nh - 1; i >= 1; i--) {
+              const uint32_t k = keys[i] & 3u;
+              if (wl == 0) {
+                stk[2 * sp] = pick4(k, rec0[0], rec0[1], rec0[2], rec0[3]);
+                stk[2 * sp + 1] = pick4(k, rec1[0], rec1[1], rec1[2], rec1[3]);
+              }
+              sp++;
+            }
+            cb0 = pick4(keys[0] & 3u, rec0[0], rec0[1], rec0[2], rec0[3]);
+            cb1 = pick4(keys[0] & 3u, rec1[0], rec1[1], rec1[2], rec1[3]);
+            has_box = true;
+            ref = __float_as_uint(cb1.z);
+          }
+        } else {
+          // ---- leaf step: every lane tests its query against each triangle ----
+          uint32_t off, cnt;
+          leaf_range(ref, M.bigleaf, off, cnt);
+          // the packet entered this leaf for some lane; a lane whose maxThreshold excludes the leaf's
+          // box (and so, the boxes nesting, the leaf) must not record its triangles
+          bool reach = active;
+          if (reach && max_finite && has_box) {
+            const float raw = box_score(q, cb0.x, cb0.y, cb0.z, cb0.w, cb1.x, cb1.y);
+            reach = below_max<kLo>(true, raw, max_t2, q, ql, cb0.x, cb0.y, cb0.z, cb0.w, cb1.x, cb1.y);
+          }
+          for (uint32_t s = off; s < off + cnt; s++) {
+            ConstF* const tp = tris + 12ull * s;
+            Tri tr;
+            tr.p0 = cload4(tp);
+            tr.p1 = cload4(tp + 4);
+            tr.p2 = cload4(tp + 8);
+            const float sb = box_score(q, fminf(fminf(tr.p0.x, tr.p1.x), tr.p2.x), fminf(fminf(tr.p0.y, tr.p1.y), tr.p2.y),
+                                       fminf(fminf(tr.p0.z, tr.p1.z), tr.p2.z), fmaxf(fmaxf(tr.p0.x, tr.p1.x), tr.p2.x),
+                                       fmaxf(fmaxf(tr.p0.y, tr.p1.y), tr.p2.y), fmaxf(fmaxf(tr.p0.z, tr.p1.z), tr.p2.z));
+            const bool near = reach && sb * (1.0f - 8.0f * kU) <= U;
+            if (__ballot(near) == 0) continue;
+            if (near) {
+              const f3 cp = closest_on_triangle(q, tr.a(), tr.b(), tr.c());
+              const f3 dv = sub(q, cp);
+              const float d2 = dot(dv, dv);
+              float lb = -kInf;  // FP32 failed (degenerate triangle, NaN): decide exactly at the end
+              bool add = true;
+              if (d2 == d2) {
+                const float e2 = (2.0f * sqrt_up(d2) + e_abs) * e_abs + 4.0f * kU * d2;
+                lb = d2 - e2;
+                add = lb <= U;
+                if (add) U = fminf(U, d2 + e2);
+              }
+              if (add) {
+                if (cl.n == kCand && !cl.compact(U)) {
+                  const Winner w = resolve_point<kLo, true>(cl, U, q, ql, M.tris, T);
+                  cl.n = 0;
+                  if (w.w != kNoHit) {
+                    const float dw = ff_to_float(w.v);
+                    U = fminf(U, dw * (1.0f + 4.0f * kU));
+                    cl.put(0, w.w, dw * (1.0f - 4.0f * kU));
+                    cl.n = 1;
+                  }
+                }
+                cl.put(cl.n++, s, lb);
+              }
+            }
+          }
+          ref = kDone;
+        }
+        // ---- pop: the nearest pushed child some lane still needs ----
+        while (ref == kDone && sp > 0) {
+          sp--;
+          const float4 e0 = stk[2 * sp], e1 = stk[2 * sp + 1];
+          const float raw = box_score(q, e0.x, e0.y, e0.z, e0.w, e1.x, e1.y);
+          bool in = raw * kScoreSlack <= U;
+          if (in && max_finite) in = below_max<kLo>(true, raw, max_t2, q, ql, e0.x, e0.y, e0.z, e0.w, e1.x, e1.y);
+          if (__ballot(in)) {
+            cb0 = e0;
+            cb1 = e1;
+            has_box = true;
+            ref = __builtin_amdgcn_readfirstlane(__float_as_uint(e1.z));
+          }
+        }
+      }
+    }
+    Winner w = {kNoHit, {kInf, 0.0f}, 0.0f};
+    if (active && !overflow && cl.n > 0) w = resolve_point<kLo, true>(cl, U, q, ql, M.tris, T);
+    if (active) {
+      uint32_t out_face = overflow ? kOverflow : kNoHit;
+      float out_dist = kInf;
+      float px = qnan, py = qnan, pz = qnan, bw = qnan, bu = qnan, bv = qnan;
+      if (!overflow && w.w != kNoHit) finish_point<kLo>(M.tris, w.w, q, ql, out_face, out_dist, px, py, pz, bw, bu, bv);
+      A.out_face[qi] = out_face;
+      if (A.out_dist) A.out_dist[qi] = out_dist;
+      if (A.out_point) {
+        A.out_point[3 * qi + 0] = px;
+        A.out_point[3 * qi + 1] = py;
+        A.out_point[3 * qi + 2] = pz;
+      }
+      if (A.out_bary) {
+        A.out_bary[3 * qi + 0] = bw;
+        A.out_bary[3 * qi + 1] = bu;
+        A.out_bary[3 * qi + 2] = bv;
+      }
+    }
+  }
+}
+
 // One lane per query on the binary tree in the reference's own visit order (minThreshold's early
 // exit returns the first triangle under it in that order): children ordered by their squared box
 // distances (decided in double-float when the FP32 values are within their error band of each
@@ -3010,6 +3219,17 @@ hipError_t launch_sphere_decide(const SphereArgs& a, hipStream_t s) {
   const uint64_t blocks = (a.n + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_sphere_decide, dim3(static_cast<uint32_t>(blocks < 16384 ? blocks : 16384)), dim3(kBlock), 0, s,
                      a);
+  return hipGetLastError();
+}
+
+hipError_t launch_closest_point_packet(const ClosestPointArgs& a, uint64_t threads, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.queue);
+  const dim3 grid(static_cast<uint32_t>(threads / kBlock));
+  if (a.points_lo)
+    hipLaunchKernelGGL(k_closest_point_packet<true>, grid, dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_closest_point_packet<false>, grid, dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
