@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-order", action="store_true",
                     help="trace queries in their given order (no spatial ordering pass in the step)")
     ap.add_argument("--cpu-sample", type=int, default=16, help="1/N stride of the CPU-baseline sample")
+    ap.add_argument("--only-differentiable", action="store_true",
+                    help="run only the C3 / C4 forward and backward comparison (for profiling)")
     ap.add_argument("--only-primary", action="store_true",
                     help="skip the differentiable (C3, C4) and batched (C5) measurements")
     ap.add_argument("--only-build", action="store_true",
@@ -82,21 +84,73 @@ def timed_kernel(torch, launch, steps, warmup, stream, pre=None):
     return wall, kern
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "closest_point4_torusknot320k_grid128.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "closest_point_packet_torusknot320k_grid128.md")
+SIMDS, CLOCK_GHZ = 1024, 2.4  # MI355X: 256 CUs x 4 SIMDs; one wave64 VALU op per SIMD per 4 cycles
 
 
-def profiled_traffic():
-    """HBM bytes per launch of the closest-point kernel, from the committed counter summary: the
-    FETCH_SIZE and WRITE_SIZE of `rocprofv3 --pmc` runs of this workload, each scaled by what the
-    same counter reads for a 256 MiB device copy (None if the summary is absent)."""
+def profiled_value(key):
+    """A figure of the committed counter summary of the closest-point launch (`key: value` lines):
+    traffic_bytes_per_launch (FETCH_SIZE and WRITE_SIZE of `rocprofv3 --pmc` runs, each scaled by
+    what the same counter reads for a 256 MiB device copy) or valu_insts_per_launch (SQ_INSTS_VALU).
+    None if the summary is absent."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return float(line.split(":", 1)[1])
     except OSError:
         pass
     return None
+
+
+def percall_latency(positions, index, rays, points, calls=2000):
+    """Per-call latency of MeshBVH.raycastFirst / closestPointToPoint through the JS package, the
+    Node-API addon and the C ABI (three-mesh-bvh_amd/bench/percall.js), one query per call. None
+    when Node or the addon is absent."""
+    import shutil
+    import subprocess
+    import tempfile
+    import numpy as np
+    node = shutil.which("node")
+    if node is None or not os.path.exists(os.path.join(ROOT, "three-mesh-bvh_amd", "lib", "mbvh.node")):
+        return None
+    d = tempfile.mkdtemp(prefix="mbvh_percall_")
+    try:
+        idx = np.ascontiguousarray(index)
+        ib = 2 if idx.dtype == np.uint16 else 4
+        np.ascontiguousarray(positions, np.float32).tofile(os.path.join(d, "positions.f32"))
+        idx.tofile(os.path.join(d, "index.u16" if ib == 2 else "index.u32"))
+        np.ascontiguousarray(rays, np.float32).tofile(os.path.join(d, "rays.f32"))
+        np.ascontiguousarray(points, np.float32).tofile(os.path.join(d, "points.f32"))
+        with open(os.path.join(d, "meta.json"), "w") as f:
+            json.dump({"index_bytes": ib}, f)
+        r = subprocess.run([node, os.path.join(ROOT, "three-mesh-bvh_amd", "bench", "percall.js"), d, str(calls)],
+                           capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            log(f"per-call latency run failed: {r.stderr[-500:]}")
+            return None
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def max_rel(a, b):
+    import numpy as np
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1.0))) if a.size else 0.0
+
+
+def barycoords64(p, a, b, c):
+    """Barycentric weights (a, b, c) of points p on triangles (a, b, c), FP64 (three's getBarycoord)."""
+    import numpy as np
+    v0, v1, v2 = c - a, b - a, p - a
+    d00, d01, d02 = (v0 * v0).sum(1), (v0 * v1).sum(1), (v0 * v2).sum(1)
+    d11, d12 = (v1 * v1).sum(1), (v1 * v2).sum(1)
+    den = d00 * d11 - d01 * d01
+    u = (d11 * d02 - d01 * d12) / den
+    v = (d00 * d12 - d01 * d02) / den
+    return np.stack([1 - u - v, v, u], axis=1)
 
 
 def js_cpu_baseline(kind, positions, index, roots, queries, threads, single, **meta):
@@ -233,13 +287,17 @@ def bench_differentiable(torch, M, W, ctx, stream, args):
                                  g_dist.data_ptr(), g_q.data_ptr(), g_v.data_ptr(), mode, ws.data_ptr(), wsb,
                                  stream=sptr)
 
+    ctx.set_point_lanes(4)
+    f4_ms = event_ms(torch, stream, fwd_c, 2)
+    ctx.set_point_lanes(64)  # grid queries: packets of 64 (identical results)
     f_ms = event_ms(torch, stream, fwd_c, 2)
     b_det = event_ms(torch, stream, lambda: bwd_c(M.GRAD_DETERMINISTIC), 3)
     b_atom = event_ms(torch, stream, lambda: bwd_c(M.GRAD_ATOMIC), 3)
     out["differentiable_closest"] = {
         "workload": f"closestPoint displaced Icosahedron(1,223) {g.tri_count} tris, {args.c4_grid}^3 grid, "
                     "upstream dL/ddistance ~ N(0,1); grads wrt query points, vertex positions",
-        "fwd_ms": round(f_ms, 3), "bwd_ms_deterministic": round(b_det, 3), "bwd_ms_atomic": round(b_atom, 3),
+        "fwd_ms": round(f_ms, 3), "fwd_ms_quad_mode": round(f4_ms, 3), "fwd_mode": "packets of 64 queries",
+        "bwd_ms_deterministic": round(b_det, 3), "bwd_ms_atomic": round(b_atom, 3),
         "fwd_bwd_over_fwd": round((f_ms + b_det) / f_ms, 3),
         "fwd_bwd_over_fwd_atomic": round((f_ms + b_atom) / f_ms, 3), "fwd_mqueries_per_s": round(n / f_ms / 1e3, 2)}
     log(f"C4 {out['differentiable_closest']}")
@@ -824,6 +882,11 @@ def main():
               flush=True)
         return
 
+    if args.only_differentiable:
+        ctx = M.Context(0)
+        print(json.dumps(bench_differentiable(torch, M, W, ctx, stream, args)), flush=True)
+        return
+
     if args.only_build:
         from mbvh_amd import geometry as G
         ctx = M.Context(0)
@@ -837,7 +900,13 @@ def main():
     idx = g.index.copy()
     tree = M.build(g.positions, idx, strategy=M.SAH, target_leaf_size=10)
     ctx = M.Context(0)
+    ctx.upload(tree, g.positions, idx).free()  # first upload: allocator and module warm-up
+    t_up = time.perf_counter()
     mesh = ctx.upload(tree, g.positions, idx)
+    ctx.synchronize()
+    upload_ms = (time.perf_counter() - t_up) * 1e3
+    # the grid's queries are coherent: packets of 64 neighbouring queries walk the tree together
+    ctx.set_point_lanes(64)
     n = w.points.shape[0]
     d_q = torch.from_numpy(w.points).to(dev)
     d_face = torch.empty(n, dtype=torch.int32, device=dev)
@@ -862,13 +931,25 @@ def main():
     ms_per_step = wall * 1000.0 / args.steps
     value = n / (wall / args.steps) / 1e6
     kern_ms = float(np.mean(kern))
+    # the same launch with four lanes per query (the default for scattered queries): identical results
+    packet_out = [x.cpu().numpy().view(np.uint32).copy() for x in (d_face, d_dist, d_pt, d_bc)]
+    ctx.set_point_lanes(4)
+    _, kern4 = timed_kernel(torch, launch_cp, max(3, args.steps // 5), 2, stream, pre=order_cp)
+    quad_same = all(np.array_equal(a, x.cpu().numpy().view(np.uint32))
+                    for a, x in zip(packet_out, (d_face, d_dist, d_pt, d_bc)))
+    ctx.set_point_lanes(64)
+    launch_cp()
+    torch.cuda.synchronize()
 
     # ---------------- raycastFirst (secondary) ----------------
     w1 = W.c1_raycast()
     g1 = w1.geometry
     idx1 = g1.index.copy()
     tree1 = M.build(g1.positions, idx1, strategy=M.SAH, target_leaf_size=10)
+    t_up = time.perf_counter()
     mesh1 = ctx.upload(tree1, g1.positions, idx1)
+    ctx.synchronize()
+    upload1_ms = (time.perf_counter() - t_up) * 1e3
     nr = w1.origins.shape[0]
     d_o = torch.from_numpy(w1.origins).to(dev)
     d_d = torch.from_numpy(w1.directions).to(dev)
@@ -914,13 +995,40 @@ def main():
         tris = float(ref["tris"].mean())
         bq = 32.0 + 32.0 * nodes + (3 * IDX_BYTES + 36) * tris
         achieved = n * bq / (kern_ms / 1000.0) / 1e9
-        traffic = profiled_traffic()
+        traffic = profiled_value("traffic_bytes_per_launch")
+        valu = profiled_value("valu_insts_per_launch")
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": round(traffic) if traffic else None,
                 "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
                 "algorithmic_bytes_per_launch": round(n * bq),
                 "bytes_per_query": round(bq, 1), "nodes_per_query": round(nodes, 2),
-                "tris_per_query": round(tris, 2), "kernel_ms": round(kern_ms, 4)}
+                "tris_per_query": round(tris, 2), "kernel_ms": round(kern_ms, 4),
+                "kernel": "k_closest_point_packet"}
+        if valu:
+            # the measured limiter: VALU issue (HBM traffic is a small fraction of the algorithmic
+            # bytes: node and triangle reads hit in the caches)
+            valu_ms = valu * 4 / (SIMDS * CLOCK_GHZ * 1e9) * 1e3
+            roof["limiter"] = "VALU issue"
+            roof["valu_insts_per_launch"] = round(valu)
+            roof["valu_issue_ms"] = round(valu_ms, 3)
+            roof["valu_issue_frac"] = round(valu_ms / kern_ms, 3)
+        # error of every result against the FP64 oracle on the sample (faces: the parity rule's
+        # rounding ties aside; point and barycoord where the faces agree)
+        gsel = slice(None, None, args.cpu_sample)
+        gface = d_face.cpu().numpy().view(np.uint32)[gsel]
+        gdist = d_dist.cpu().numpy()[gsel]
+        gpt = d_pt.cpu().numpy()[gsel]
+        gbc = d_bc.cpu().numpy()[gsel]
+        eq = gface == ref["face"]
+        tri = idx.reshape(-1, 3).astype(np.int64)[ref["face"][eq].astype(np.int64)]
+        pos64 = g.positions.reshape(-1, 3).astype(np.float64)
+        rbc = barycoords64(ref["point"][eq], pos64[tri[:, 0]], pos64[tri[:, 1]], pos64[tri[:, 2]])
+        errors_c2 = {"queries": int(sample.shape[0]), "face_equal": round(float(eq.mean()), 6),
+                     "max_distance_rel_err": max_rel(gdist, ref["distance"]),
+                     "max_point_rel_err": max_rel(gpt[eq], ref["point"][eq]),
+                     "max_barycoord_abs_err": float(np.max(np.abs(gbc[eq] - rbc))) if eq.any() else 0.0,
+                     "note": "against the FP64 C oracle on every cpu-sample query; point / barycoord where the "
+                             "faces agree (elsewhere the two faces are at the same distance to rounding)"}
         oracle_rate = sample.shape[0] / cpu_s / 1e6
         # the reference side in its own language: the JS restatement in Node, single thread over a
         # strided sample, then a worker pool over every query
@@ -957,6 +1065,12 @@ def main():
         r1 = om1.raycast_first(w1.origins, w1.directions, threads=threads)
         gf = r_face.cpu().numpy().view(np.uint32)
         extra["raycast_face_agreement"] = float((gf == r1["face"]).mean())
+        hit = (gf == r1["face"]) & (r1["face"] != O.NO_HIT)
+        extra["errors_vs_oracle"] = {
+            "rays": int(nr), "hits": int(hit.sum()),
+            "max_distance_rel_err": max_rel(r_dist.cpu().numpy()[hit], r1["distance"][hit]),
+            "max_point_rel_err": max_rel(r_pt.cpu().numpy()[hit], r1["point"][hit]),
+            "max_barycoord_abs_err": float(np.max(np.abs(r_bc.cpu().numpy()[hit] - r1["barycoord"][hit])))}
         extra["raycast_nodes_per_ray"] = round(float(r1["nodes"].mean()), 2)
         extra["raycast_tris_per_ray"] = round(float(r1["tris"].mean()), 2)
         # algorithmic bytes per ray: 56 B of I/O, 32 B per node box, 42 B per triangle (3 x u16 + 36 B)
@@ -992,12 +1106,33 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "closest_point": {"mode": "packets of 64 queries (mbvh_ctx_set_point_lanes 64)",
+                          "kernel_ms": round(kern_ms, 4),
+                          "quad_mode_kernel_ms": round(float(np.mean(kern4)), 4),
+                          "quad_results_identical": quad_same,
+                          "errors_vs_oracle": errors_c2 if not args.no_cpu_baseline else None},
+        "upload": {"closest_mesh_ms": round(upload_ms, 3), "raycast_mesh_ms": round(upload1_ms, 3),
+                   "note": "mbvh_mesh_upload wall time: reference node buffers, index and positions in; the "
+                           "binary-wide and 4-wide layouts derived on the host and copied to HBM"},
         "host_api": host_api,
         "raycast_first": {"workload": "TorusKnot(10,3,400,100) 80k tris SAH leaf10, 65,536 rays from r=40",
                           "mrays_per_s": round(mrays, 2), "mrays_per_s_kernel_only": round(mrays_kernel, 2),
                           "kernel_ms": round(float(np.mean(kern1)), 4), **extra},
     }
     if not args.only_primary:
+        rng = np.random.default_rng(5)
+        lo1, hi1 = g1.bounds()
+        pts1 = (lo1 + (hi1 - lo1) * rng.random((4096, 3))).astype(np.float32)
+        pc = percall_latency(g1.positions, idx1, np.concatenate([w1.origins[:4096], w1.directions[:4096]]), pts1)
+        if pc is not None:
+            js_single = (extra.get("cpu_baseline") or {}).get("single_thread_mrays_per_s")
+            line["per_call"] = {
+                "workload": "TorusKnot(10,3,400,100) 80k tris: one C1 ray per MeshBVH.raycastFirst call, one point "
+                            "in the mesh box per closestPointToPoint call, from Node through the addon",
+                **pc,
+                "reference_js_us_per_ray_single_thread": round(1.0 / js_single, 3) if js_single else None,
+                "note": "each call is a synchronous device round trip; the batch methods amortise it"}
+            log(f"per-call {line['per_call']}")
         line["intersects_box"] = bench_boxes(torch, ctx, stream, mesh, g, idx, tree)
         log(f"boxes {line['intersects_box']}")
         line["closest_point_to_geometry"] = bench_geometry(torch, M, ctx, stream, mesh, g, idx, tree)

@@ -273,3 +273,35 @@ def test_packets_equal_quads_bit_for_bit():
                 assert np.array_equal(a4[key].view(np.uint32) if a4[key].dtype != np.uint32 else a4[key],
                                       a64[key].view(np.uint32) if a64[key].dtype != np.uint32 else a64[key]), (name, k, key)
         print(name, "packets == quads on", [len(x["face"]) for x in out[64]], flush=True)
+
+
+def test_packets_redo_overflowing_lists():
+    """A cone of 16 triangles around one apex, queried from above: every triangle's closest point is
+    the apex, so all 16 tie and overflow a packet's candidate list (8 entries); those queries are
+    finished by the quad kernel and must equal its results and the oracle's face (the reference's
+    first in visit order)."""
+    k = 16
+    ang = np.arange(k) * 2 * np.pi / k
+    ring = np.stack([np.cos(ang), np.sin(ang), np.full(k, -0.5)], axis=1)
+    pos = np.concatenate([np.zeros((1, 3)), ring]).astype(np.float32)
+    tri = np.stack([np.zeros(k, int), 1 + np.arange(k), 1 + (np.arange(k) + 1) % k], axis=1)
+    geo = G.Geometry(pos, tri.reshape(-1).astype(np.uint16))
+    rng = np.random.default_rng(2)
+    above = np.concatenate([rng.uniform(-0.05, 0.05, (5000, 2)), rng.uniform(0.5, 3.0, (5000, 1))], axis=1)
+    pts = np.concatenate([above, rng.uniform(-2, 2, (3000, 3))]).astype(np.float32)
+    g2, idx, tree = build_mesh(geo)
+    om = O.OracleMesh(g2.positions, idx, tree.roots)
+    ref = om.closest_point(pts)
+    out = {}
+    for lanes in (4, 64):
+        c = M.Context(0)
+        c.set_point_lanes(lanes)
+        mesh = c.upload(tree, g2.positions, idx)
+        out[lanes] = mesh.closest_point(pts)
+        mesh.free()
+        c.close()
+    for key in ("face", "distance", "point", "barycoord"):
+        assert np.array_equal(out[4][key], out[64][key]), key
+    apex = np.all(out[64]["point"][:5000] == 0, axis=1)
+    assert apex.mean() > 0.9
+    assert np.array_equal(out[64]["face"][:5000][apex], ref["face"][:5000][apex])

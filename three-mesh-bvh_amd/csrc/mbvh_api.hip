@@ -45,6 +45,7 @@ struct mbvh_ctx {
   DeviceBuffer build_ws;        // device build workspace
   DeviceBuffer geo_ws;          // closestPointToGeometry workspace (per other triangle + answer)
   DeviceBuffer lo_stage[3];     // host-API lo parts of double-precision inputs (origins / points, directions, windows)
+  DeviceBuffer packet[4];       // packet closest point, per chunk: candidate lists, per-query metadata, counters + redo list, pool
   uint64_t hits_total = 0;
   std::mutex lock;              // host entry points on one context are serialised
 };
@@ -554,6 +555,8 @@ extern "C" void mbvh_ctx_destroy(mbvh_ctx* ctx) {
     if (b.ptr) (void)hipFree(b.ptr);
   for (auto& b : ctx->lo_stage)
     if (b.ptr) (void)hipFree(b.ptr);
+  for (auto& b : ctx->packet)
+    if (b.ptr) (void)hipFree(b.ptr);
   if (ctx->grad_ws.ptr) (void)hipFree(ctx->grad_ws.ptr);
   if (ctx->order_ws.ptr) (void)hipFree(ctx->order_ws.ptr);
   if (ctx->build_ws.ptr) (void)hipFree(ctx->build_ws.ptr);
@@ -577,6 +580,15 @@ extern "C" int mbvh_ctx_set_point_lanes(mbvh_ctx* ctx, int32_t lanes) {
   if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_ctx_set_point_lanes: invalid context handle");
   if (lanes != 0 && lanes != 1 && lanes != 4 && lanes != 64)
     return set_error(MBVH_ERR_BAD_ARGS, "mbvh_ctx_set_point_lanes: lanes must be 0 (automatic), 1, 4 or 64");
+  if (lanes == 64) {
+    // the packets' buffers are allocated here, so that queries allocate nothing
+    std::lock_guard<std::mutex> guard(ctx->lock);
+    MBVH_HIP(hipSetDevice(ctx->device));
+    if (int rc = ensure_buffer(ctx->packet[0], kPacketChunk * kPacketCand * sizeof(uint2))) return rc;
+    if (int rc = ensure_buffer(ctx->packet[1], kPacketChunk * sizeof(uint4))) return rc;
+    if (int rc = ensure_buffer(ctx->packet[2], 512 + kPacketChunk * sizeof(uint32_t))) return rc;
+    if (int rc = ensure_buffer(ctx->packet[3], size_t{kPacketPool} * sizeof(uint4))) return rc;
+  }
   ctx->point_lanes = lanes;
   return MBVH_OK;
 }
@@ -1005,7 +1017,8 @@ int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uin
   if (n && (!q || !face)) return set_error(MBVH_ERR_BAD_ARGS, "closest_point: NULL point or face buffer");
   // packets and quads only without a minThreshold (its early exit follows the binary order); packets
   // only where the tree's 4-wide stack fits the packet kernel's
-  const bool packet = !fixed && min_t == 0.0f && ctx->point_lanes == 64 && mesh->max_stack4 <= kPacketStackEntries;
+  const bool packet = !fixed && min_t == 0.0f && ctx->point_lanes == 64 && mesh->max_stack4 <= kPacketStackEntries &&
+                      ctx->packet[3].ptr && mesh->max_stack4 <= kQuadLdsStack + ctx->stack_ovf_entries;
   const bool quads = min_t == 0.0f && !packet && (ctx->point_lanes == 4 || ctx->point_lanes == 0 || ctx->point_lanes == 64);
   uint64_t threads = quads ? launch_threads(4 * n, ctx->max_threads) : launch_threads(n, ctx->max_threads);
   ClosestPointArgs a{};
@@ -1048,9 +1061,29 @@ int closest_point_impl(mbvh_ctx* ctx, const mbvh_mesh* mesh, const float* q, uin
   a.out_point = point;
   a.out_bary = bary;
   if (packet) {
-    a.tile0 = 0;
-    a.tiles = (n + 63) / 64;
-    MBVH_HIP(launch_closest_point_packet(a, threads, s));
+    // per chunk: the packets (lists), the exact resolution, then the quad kernel on the queries whose
+    // lists overflowed (a device-side count: nothing to do when it is 0)
+    a.cand = static_cast<uint2*>(ctx->packet[0].ptr);
+    a.cand_meta = static_cast<uint4*>(ctx->packet[1].ptr);
+    a.redo_count = static_cast<uint32_t*>(ctx->packet[2].ptr);
+    a.pool_count = a.redo_count + 64;  // its own 256-byte line
+    a.redo = a.redo_count + 128;
+    a.pool = static_cast<uint4*>(ctx->packet[3].ptr);
+    a.pool_cap = kPacketPool;
+    const uint64_t tiles_total = (n + 63) / 64, chunk_tiles = kPacketChunk / 64;
+    for (uint64_t t0 = 0; t0 < tiles_total; t0 += chunk_tiles) {
+      a.tile0 = t0;
+      a.tiles = std::min(chunk_tiles, tiles_total - t0);
+      MBVH_HIP(launch_closest_point_packet(a, launch_threads(a.tiles * 64, ctx->max_threads), s));
+      ClosestPointArgs r = a;
+      // rare (only when the candidate pool runs out): a small grid, whose waves leave at once when
+      // the redo count is 0
+      const uint64_t rt = launch_threads(4 * std::min<uint64_t>(a.tiles * 64, 4096), ctx->max_threads);
+      r.stack = stack_of(ctx, rt / 4);
+      r.order = a.redo;
+      r.n_dev = a.redo_count;
+      MBVH_HIP(launch_closest_point4(r, rt, s));
+    }
   } else if (quads)
     MBVH_HIP(launch_closest_point4(a, threads, s));
   else

@@ -88,9 +88,24 @@ struct ClosestPointArgs {
   float* out_dist;
   float* out_point;
   float* out_bary;
-  // packet kernel only: the launch covers packets [tile0, tile0 + tiles) of 64 queries in visit order
+  // packet kernels only: the launch covers packets [tile0, tile0 + tiles) of 64 queries in visit
+  // order. cand: kPacketCand candidates (slot, FP32 lower bound bits) per query of the launch;
+  // cand_meta: per query (listed count or a mark, FP32 upper bound bits, head of its further
+  // candidates in pool or kPoolEnd, 0); pool: further candidates (slot, lower bound bits, next, 0),
+  // pool_count of pool_cap used; redo / redo_count: queries left when the pool ran out, finished by
+  // the quad kernel
   uint64_t tile0;
   uint64_t tiles;
+  uint2* cand;
+  uint4* cand_meta;
+  uint4* pool;
+  uint32_t* pool_count;
+  uint32_t pool_cap;
+  uint32_t* redo;
+  uint32_t* redo_count;
+  // quad kernel: when set, the number of queries to visit (device-side, e.g. a redo list's length);
+  // n stays the number of points (the bound of the indices in order)
+  const uint32_t* n_dev;
 };
 
 // ---- refit ------------------------------------------------------------------------------------
@@ -444,8 +459,15 @@ hipError_t launch_closest_point(const ClosestPointArgs& a, uint64_t threads, hip
 // four lanes per query on the 4-wide tree (minThreshold must be 0); `threads` lanes resident
 hipError_t launch_closest_point4(const ClosestPointArgs& a, uint64_t threads, hipStream_t s);
 // One wave of 64 queries per packet walking the 4-wide tree together (coherent queries); the tree's
-// 4-wide stack must fit kPacketStackEntries.
+// 4-wide stack must fit kPacketStackEntries. The traversal lists each query's candidates, a second
+// launch resolves them exactly; queries whose candidates overflow the list go to a redo list that
+// the caller finishes with launch_closest_point4 (n_dev = redo_count, order = redo).
 constexpr uint32_t kPacketStackEntries = 64;
+constexpr uint32_t kPacketCand = 4;  // the LDS list per query (kCand)
+// queries per packet launch: the buffers (48 bytes per query, plus the pool) are sized for it. Each
+// launch ends with its slowest packets, so fewer, larger launches finish sooner.
+constexpr uint64_t kPacketChunk = uint64_t{1} << 24;
+constexpr uint32_t kPacketPool = 1u << 23;  // further candidates per launch
 hipError_t launch_closest_point_packet(const ClosestPointArgs& a, uint64_t threads, hipStream_t s);
 // fill == false: count hits per ray into a.counts; fill == true: write them at a.offsets.
 hipError_t launch_raycast_all(const RaycastAllArgs& a, bool fill, uint64_t threads, hipStream_t s);

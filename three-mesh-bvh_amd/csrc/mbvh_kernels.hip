@@ -1585,14 +1585,15 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
   const ff max_t2 = max_finite ? thr_sq(A.max_threshold, A.max_lo) : ff{kInf, 0.0f};
   constexpr float kScoreSlack = 1.0f - 4.0f * kU;
   const float qnan = __int_as_float(0x7fc00000);
+  const uint64_t nv = A.n_dev ? static_cast<uint64_t>(*A.n_dev) : A.n;  // queries to visit
 
   while (true) {
     uint32_t tile = 0;
     if ((lane & 63u) == 0) tile = atomicAdd(A.queue, 1u);
     tile = __shfl(tile, 0, 64);
-    if (static_cast<uint64_t>(tile) * 16 >= A.n) break;  // every wave reaches this exit
+    if (static_cast<uint64_t>(tile) * 16 >= nv) break;  // every wave reaches this exit
     const uint64_t visit = static_cast<uint64_t>(tile) * 16 + wq;
-    if (visit >= A.n) continue;
+    if (visit >= nv) continue;
     const uint64_t qi = A.order ? A.order[visit] : visit;
     if (qi >= A.n) continue;
     const f3 q = mk(A.points[3 * qi + 0], A.points[3 * qi + 1], A.points[3 * qi + 2]);
@@ -1731,6 +1732,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_closest_point4(ClosestPointArgs A
 // kernel only for trees whose 4-wide stack fits it.
 // ------------------------------------------------------------------------------------------
 constexpr int kPacketStack = static_cast<int>(kPacketStackEntries);
+constexpr uint32_t kPacketRedoMark = 0xFFFFFFFFu;   // cand_meta: the query went to the redo list
+constexpr uint32_t kPacketStackMark = 0xFFFFFFFEu;  // cand_meta: the packet's stack overflowed
+constexpr uint32_t kPoolEnd = 0xFFFFFFFFu;          // end of a query's chain in the pool
 
 __device__ __forceinline__ float4 pick4(uint32_t k, float4 a, float4 b, float4 c, float4 d) {
   return k == 0 ? a : (k == 1 ? b : (k == 2 ? c : d));
@@ -1745,7 +1749,6 @@ __global__ __launch_bounds__(kBlock, 8) void k_closest_point_packet(ClosestPoint
   const uint32_t lane = threadIdx.x, wl = lane & 63u;
   float4* const stk = pstack + (lane >> 6) * kPacketStack * 2;
   const MeshView M = A.mesh;
-  const TieTree T = tie_tree(M);
   ConstF* const nodes4 = (ConstF*)(M.nodes4);
   ConstF* const tris = (ConstF*)(M.tris);
   const bool max_finite = A.max_threshold < kInf;
@@ -1773,10 +1776,13 @@ __global__ __launch_bounds__(kBlock, 8) void k_closest_point_packet(ClosestPoint
     if (act == 0) continue;
     // the lane whose scores order the children: lane 32 if its query is live, else the first live one
     const uint32_t leader = (act >> 32) & 1u ? 32u : static_cast<uint32_t>(__builtin_ctzll(act));
+    // this query's candidates: a kCand-entry list in LDS, compacted against the bound when full; a
+    // candidate that finds it full of live ones goes onto the query's chain in the global pool
     Cands cl;
     cl.lane = lane;
     cl.n = 0;
-    bool overflow = false;
+    uint32_t head = kPoolEnd;
+    bool redo = false, overflow = false;
     for (uint32_t r = 0; r < M.n_roots && !overflow; r++) {
       uint32_t ref = __builtin_amdgcn_readfirstlane(M.roots4[r]);
       int sp = 0;
@@ -1872,17 +1878,18 @@ __global__ __launch_bounds__(kBlock, 8) void k_closest_point_packet(ClosestPoint
                 if (add) U = fminf(U, d2 + e2);
               }
               if (add) {
-                if (cl.n == kCand && !cl.compact(U)) {
-                  const Winner w = resolve_point<kLo, true>(cl, U, q, ql, M.tris, T);
-                  cl.n = 0;
-                  if (w.w != kNoHit) {
-                    const float dw = ff_to_float(w.v);
-                    U = fminf(U, dw * (1.0f + 4.0f * kU));
-                    cl.put(0, w.w, dw * (1.0f - 4.0f * kU));
-                    cl.n = 1;
+                if (cl.n < kCand || cl.compact(U)) {
+                  cl.put(cl.n++, s, lb);
+                } else {
+                  const uint32_t at = atomicAdd(A.pool_count, 1u);
+                  if (at < A.pool_cap) {
+                    A.pool[at] = make_uint4(s, __float_as_uint(lb), head, 0u);
+                    head = at;
+                  } else {
+                    redo = true;  // the pool ran out: the quad kernel finishes this query
+                    U = -1.0f;
                   }
                 }
-                cl.put(cl.n++, s, lb);
               }
             }
           }
@@ -1904,28 +1911,88 @@ __global__ __launch_bounds__(kBlock, 8) void k_closest_point_packet(ClosestPoint
         }
       }
     }
-    Winner w = {kNoHit, {kInf, 0.0f}, 0.0f};
-    if (active && !overflow && cl.n > 0) w = resolve_point<kLo, true>(cl, U, q, ql, M.tris, T);
     if (active) {
-      uint32_t out_face = overflow ? kOverflow : kNoHit;
-      float out_dist = kInf;
-      float px = qnan, py = qnan, pz = qnan, bw = qnan, bu = qnan, bv = qnan;
-      if (!overflow && w.w != kNoHit) finish_point<kLo>(M.tris, w.w, q, ql, out_face, out_dist, px, py, pz, bw, bu, bv);
-      A.out_face[qi] = out_face;
-      if (A.out_dist) A.out_dist[qi] = out_dist;
-      if (A.out_point) {
-        A.out_point[3 * qi + 0] = px;
-        A.out_point[3 * qi + 1] = py;
-        A.out_point[3 * qi + 2] = pz;
-      }
-      if (A.out_bary) {
-        A.out_bary[3 * qi + 0] = bw;
-        A.out_bary[3 * qi + 1] = bu;
-        A.out_bary[3 * qi + 2] = bv;
-      }
+      const uint64_t v = visit - A.tile0 * 64;
+      for (int c = 0; c < cl.n; c++) A.cand[v * kPacketCand + c] = cl.get(c);
+      A.cand_meta[v] = make_uint4(overflow ? kPacketStackMark : (redo ? kPacketRedoMark : static_cast<uint32_t>(cl.n)),
+                                  __float_as_uint(U), head, 0u);
+      if (redo) A.redo[atomicAdd(A.redo_count, 1u)] = static_cast<uint32_t>(qi);
     }
   }
 }
+
+// The exact resolution of the packets' candidate lists: one lane per query, the smallest exact
+// squared distance from q + ql, ties (point_tie) to the triangle the reference meets first; the
+// reported distance, point and barycoord in double-float.
+template <bool kLo>
+__global__ __launch_bounds__(kBlock) void k_closest_point_resolve(ClosestPointArgs A) {
+  const uint64_t v = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (v >= A.tiles * 64) return;
+  const uint64_t visit = A.tile0 * 64 + v;
+  if (visit >= A.n) return;
+  const uint64_t qi = A.order ? A.order[visit] : visit;
+  if (qi >= A.n) return;
+  const uint4 meta = A.cand_meta[v];
+  if (meta.x == kPacketRedoMark) return;  // finished by the quad kernel
+  const MeshView M = A.mesh;
+  const TieTree T = tie_tree(M);
+  const f3 q = mk(A.points[3 * qi + 0], A.points[3 * qi + 1], A.points[3 * qi + 2]);
+  const f3 ql = kLo ? load3(A.points_lo, qi) : mk(0.0f, 0.0f, 0.0f);
+  const float U = __uint_as_float(meta.y);
+  const float qnan = __int_as_float(0x7fc00000);
+  uint32_t out_face = meta.x == kPacketStackMark ? kOverflow : kNoHit;
+  float out_dist = kInf;
+  float px = qnan, py = qnan, pz = qnan, bw = qnan, bu = qnan, bv = qnan;
+  if (meta.x != kPacketStackMark) {
+    const uint2* const row = A.cand + v * kPacketCand;
+    uint32_t win = kNoHit;
+    ff best_d2 = {kInf, 0.0f};
+    float best_err = 0.0f;
+    uint32_t link = meta.z;
+    // the listed entries, then the pool chain
+    for (uint32_t c = 0; c < meta.x || link != kPoolEnd; c++) {
+      uint2 e;
+      if (c < meta.x) {
+        e = row[c];
+      } else {
+        const uint4 pe = A.pool[link];
+        e = make_uint2(pe.x, pe.y);
+        link = pe.z;
+      }
+      if (__uint_as_float(e.y) > U) continue;
+      const Tri tr = load_tri(M.tris, e.x);
+      const ClosestFF cf = closest_on_triangle_ff<kLo>(q, ql, tr.a(), tr.b(), tr.c());
+      if (!(cf.dist2.hi == cf.dist2.hi)) continue;  // degenerate: never wins
+      bool take = win == kNoHit;
+      if (!take) {
+        if (point_tie(cf.dist2, cf.err, best_d2, best_err))
+          take = e.x != win && point_first_of<kLo>(T, q, ql, e.x, win);
+        else
+          take = ff_sign(ff_sub(cf.dist2, best_d2)) < 0;
+      }
+      if (take) {
+        win = e.x;
+        best_d2 = cf.dist2;
+        best_err = cf.err;
+      }
+    }
+    if (win != kNoHit) finish_point<kLo>(M.tris, win, q, ql, out_face, out_dist, px, py, pz, bw, bu, bv);
+  }
+  A.out_face[qi] = out_face;
+  if (A.out_dist) A.out_dist[qi] = out_dist;
+  if (A.out_point) {
+    A.out_point[3 * qi + 0] = px;
+    A.out_point[3 * qi + 1] = py;
+    A.out_point[3 * qi + 2] = pz;
+  }
+  if (A.out_bary) {
+    A.out_bary[3 * qi + 0] = bw;
+    A.out_bary[3 * qi + 1] = bu;
+    A.out_bary[3 * qi + 2] = bv;
+  }
+}
+
+
 
 // One lane per query on the binary tree in the reference's own visit order (minThreshold's early
 // exit returns the first triangle under it in that order): children ordered by their squared box
@@ -3223,13 +3290,19 @@ hipError_t launch_sphere_decide(const SphereArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_closest_point_packet(const ClosestPointArgs& a, uint64_t threads, hipStream_t s) {
-  if (a.n == 0) return hipSuccess;
+  if (a.n == 0 || a.tiles == 0) return hipSuccess;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.queue);
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.redo_count);
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.pool_count);
   const dim3 grid(static_cast<uint32_t>(threads / kBlock));
-  if (a.points_lo)
+  const dim3 rgrid(static_cast<uint32_t>((a.tiles * 64 + kBlock - 1) / kBlock));
+  if (a.points_lo) {
     hipLaunchKernelGGL(k_closest_point_packet<true>, grid, dim3(kBlock), 0, s, a);
-  else
+    hipLaunchKernelGGL(k_closest_point_resolve<true>, rgrid, dim3(kBlock), 0, s, a);
+  } else {
     hipLaunchKernelGGL(k_closest_point_packet<false>, grid, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_closest_point_resolve<false>, rgrid, dim3(kBlock), 0, s, a);
+  }
   return hipGetLastError();
 }
 
