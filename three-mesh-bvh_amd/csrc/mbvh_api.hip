@@ -17,6 +17,10 @@ using namespace mbvh;
 namespace {
 
 constexpr uint64_t kMaxThreadsPerCU = 2048;  // 32 waves of 64 lanes
+// host calls of at most this many queries use the context's pinned block: 12 input and 8 output
+// words per query (origins, directions and their lo parts; face, distance, point, barycoord)
+constexpr uint64_t kSmallQueries = 256;
+constexpr size_t kSmallBytes = kSmallQueries * 20 * sizeof(float);
 
 struct DeviceBuffer {
   void* ptr = nullptr;
@@ -45,6 +49,10 @@ struct mbvh_ctx {
   DeviceBuffer build_ws;        // device build workspace
   DeviceBuffer geo_ws;          // closestPointToGeometry workspace (per other triangle + answer)
   DeviceBuffer lo_stage[3];     // host-API lo parts of double-precision inputs (origins / points, directions, windows)
+  // small host calls (one ray or point from a per-call API): inputs and outputs in one pinned host
+  // block the kernels read and write in place (no staging copies); host and device addresses
+  void* small_host = nullptr;
+  void* small_dev = nullptr;
   DeviceBuffer packet[4];       // packet closest point, per chunk: candidate lists, per-query metadata, counters + redo list, pool
   uint64_t hits_total = 0;
   std::mutex lock;              // host entry points on one context are serialised
@@ -536,6 +544,15 @@ extern "C" int mbvh_ctx_create(int32_t device, mbvh_ctx** out_ctx) {
     delete c;
     return MBVH_ERR_OOM;
   }
+  // the pinned block of small calls (without it, small calls take the staging path)
+  if (hipHostMalloc(&c->small_host, kSmallBytes, hipHostMallocMapped) == hipSuccess) {
+    if (hipHostGetDevicePointer(&c->small_dev, c->small_host, 0) != hipSuccess) {
+      (void)hipHostFree(c->small_host);
+      c->small_host = c->small_dev = nullptr;
+    }
+  } else {
+    c->small_host = nullptr;
+  }
   *out_ctx = c;
   return MBVH_OK;
 }
@@ -557,6 +574,7 @@ extern "C" void mbvh_ctx_destroy(mbvh_ctx* ctx) {
     if (b.ptr) (void)hipFree(b.ptr);
   for (auto& b : ctx->packet)
     if (b.ptr) (void)hipFree(b.ptr);
+  if (ctx->small_host) (void)hipHostFree(ctx->small_host);
   if (ctx->grad_ws.ptr) (void)hipFree(ctx->grad_ws.ptr);
   if (ctx->order_ws.ptr) (void)hipFree(ctx->order_ws.ptr);
   if (ctx->build_ws.ptr) (void)hipFree(ctx->build_ws.ptr);
@@ -1144,6 +1162,34 @@ int host_raycast_first(const char* fn, mbvh_ctx* ctx, const mbvh_mesh* mesh, con
   int rc = set_device(ctx);
   if (rc) return rc;
   const size_t v3 = n * 3 * sizeof(float);
+  if (n > 0 && n <= kSmallQueries && ctx->small_host) {
+    // one pinned block: the kernels read the rays and write the results there, no copies
+    float* h = static_cast<float*>(ctx->small_host);
+    float* d = static_cast<float*>(ctx->small_dev);
+    const uint64_t k = 3 * n;
+    std::memcpy(h, origins, v3);
+    std::memcpy(h + k, directions, v3);
+    LoInputs lo;
+    lo.near_lo = near_lo;
+    lo.far_lo = far_lo;
+    if (origins_lo) {
+      std::memcpy(h + 2 * k, origins_lo, v3);
+      std::memcpy(h + 3 * k, directions_lo, v3);
+      lo.a = d + 2 * k;
+      lo.b = d + 3 * k;
+    }
+    float* out = d + 4 * k;  // face n, distance n, point 3n, barycoord 3n
+    rc = raycast_first_impl(ctx, mesh, d, d + k, n, nullptr, side, near_dist, far_dist,
+                            reinterpret_cast<uint32_t*>(out), out + n, out + 2 * n, out + 5 * n, ctx->stream, lo);
+    if (rc) return rc;
+    MBVH_HIP(hipStreamSynchronize(ctx->stream));
+    const float* ho = h + 4 * k;
+    std::memcpy(face_index, ho, n * sizeof(uint32_t));
+    if (distance) std::memcpy(distance, ho + n, n * sizeof(float));
+    if (point) std::memcpy(point, ho + 2 * n, v3);
+    if (barycoord) std::memcpy(barycoord, ho + 5 * n, v3);
+    return check_no_overflow(face_index, n, fn);
+  }
   if ((rc = ensure_buffer(ctx->staging[0], v3)) || (rc = ensure_buffer(ctx->staging[1], v3)) ||
       (rc = ensure_buffer(ctx->staging[2], n * sizeof(uint32_t))) ||
       (rc = ensure_buffer(ctx->staging[3], n * sizeof(float))) || (rc = ensure_buffer(ctx->staging[4], v3)) ||
@@ -1319,6 +1365,31 @@ int host_closest_point(const char* fn, mbvh_ctx* ctx, const mbvh_mesh* mesh, con
   int rc = set_device(ctx);
   if (rc) return rc;
   const size_t v3 = n * 3 * sizeof(float);
+  if (n > 0 && n <= kSmallQueries && ctx->small_host) {
+    // one pinned block: the kernels read the points and write the results there, no copies
+    float* h = static_cast<float*>(ctx->small_host);
+    float* d = static_cast<float*>(ctx->small_dev);
+    const uint64_t k = 3 * n;
+    std::memcpy(h, points, v3);
+    LoInputs lo;
+    lo.near_lo = min_lo;
+    lo.far_lo = max_lo;
+    if (points_lo) {
+      std::memcpy(h + k, points_lo, v3);
+      lo.a = d + k;
+    }
+    float* out = d + 4 * k;  // face n, distance n, point 3n, barycoord 3n
+    rc = closest_point_impl(ctx, mesh, d, n, nullptr, min_threshold, max_threshold, reinterpret_cast<uint32_t*>(out),
+                            out + n, out + 2 * n, out + 5 * n, ctx->stream, nullptr, lo);
+    if (rc) return rc;
+    MBVH_HIP(hipStreamSynchronize(ctx->stream));
+    const float* ho = h + 4 * k;
+    std::memcpy(face_index, ho, n * sizeof(uint32_t));
+    if (distance) std::memcpy(distance, ho + n, n * sizeof(float));
+    if (closest) std::memcpy(closest, ho + 2 * n, v3);
+    if (barycoord) std::memcpy(barycoord, ho + 5 * n, v3);
+    return check_no_overflow(face_index, n, fn);
+  }
   if ((rc = ensure_buffer(ctx->staging[0], v3)) || (rc = ensure_buffer(ctx->staging[2], n * sizeof(uint32_t))) ||
       (rc = ensure_buffer(ctx->staging[3], n * sizeof(float))) || (rc = ensure_buffer(ctx->staging[4], v3)) ||
       (rc = ensure_buffer(ctx->staging[5], v3)))
