@@ -493,6 +493,29 @@ int mbvh_mesh_export_textures(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* posit
 int mbvh_mesh_export_nodes_d(mbvh_ctx* ctx, mbvh_mesh* mesh, uint32_t* nodes, uint64_t capacity_bytes, void* stream);
 int mbvh_mesh_export_nodes(mbvh_ctx* ctx, mbvh_mesh* mesh, uint32_t* nodes, uint64_t capacity_bytes);
 
+/*
+ * Imports into a device mesh (no counterpart in the reference, which only packs; the inverses of the
+ * two packings above). mbvh_mesh_import_nodes reads a WebGPU BVHNode buffer of n_nodes 8-word
+ * records holding a mesh's root subtrees back to back (appendBVHSubtree,
+ * src/webgpu/utils/packBVHBufferUtils.js:216-280): each root's extent is its subtree's span, leaf
+ * triangle offsets lose `geometry_offset`, and a root axis written [1, -1] for empty bounds becomes
+ * [+inf, -inf] again. mbvh_mesh_import_textures reads MeshBVHUniformStruct's bvhBounds (RGBA32F,
+ * node i's min at texel 2i, max at 2i + 1) and bvhContents (RG32UI: a leaf (0xFFFF0000 | count,
+ * offset), an internal node (split axis, right child offset)) of one root
+ * (src/webgl/MeshBVHUniformStruct.js:129-210). The index / indirect / position arguments are those
+ * of mbvh_mesh_upload. Host arrays; the conversion runs on the host, then the mesh is uploaded.
+ */
+int mbvh_mesh_import_nodes(mbvh_ctx* ctx, const uint32_t* nodes, uint64_t n_nodes, uint32_t geometry_offset,
+                           const void* index, int32_t index_elt_bytes, uint64_t index_count, const void* indirect,
+                           int32_t indirect_elt_bytes, uint64_t indirect_count, const float* positions,
+                           uint64_t vertex_count, uint32_t position_stride, uint32_t position_offset,
+                           mbvh_mesh** out_mesh);
+int mbvh_mesh_import_textures(mbvh_ctx* ctx, const float* bvh_bounds, const uint32_t* bvh_contents, uint64_t n_nodes,
+                              const void* index, int32_t index_elt_bytes, uint64_t index_count, const void* indirect,
+                              int32_t indirect_elt_bytes, uint64_t indirect_count, const float* positions,
+                              uint64_t vertex_count, uint32_t position_stride, uint32_t position_offset,
+                              mbvh_mesh** out_mesh);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Scenes: closest hit over all instances in one launch                                      */
 /* ------------------------------------------------------------------------------------------ */

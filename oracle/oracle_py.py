@@ -355,6 +355,31 @@ def webgpu_nodes(roots: Sequence[np.ndarray], geometry_offset: int = 0) -> np.nd
     return np.concatenate(parts) if parts else np.zeros(0, np.uint8)
 
 
+def webgpu_nodes_to_roots(words: np.ndarray, geometry_offset: int = 0):
+    """The inverse of webgpu_nodes: the reference root buffers packed back to back in a BVHNode buffer
+    (each root's extent its subtree's span: the right spine's relative offsets plus one), leaf offsets
+    less geometry_offset, an empty root's [1, -1] axes back to [inf, -inf]."""
+    w = np.ascontiguousarray(words)
+    w = (w.view(np.uint32) if w.dtype == np.uint8 else w.astype(np.uint32)).reshape(-1, 8)
+    roots, start = [], 0
+    while start < w.shape[0]:
+        j, size = start, 1
+        while (int(w[j, 7]) >> 16) != 0xFFFF:
+            r = int(w[j, 6])
+            j += r
+            size += r
+        u = w[start:start + size].copy()
+        f = u.view(np.float32)
+        for c in range(3):
+            if f[0, c] > f[0, c + 3]:
+                f[0, c], f[0, c + 3] = np.inf, -np.inf
+        leaf = (u[:, 7] >> 16) == 0xFFFF
+        u[leaf, 6] -= np.uint32(geometry_offset)
+        roots.append(u.view(np.uint8).reshape(-1))
+        start += size
+    return roots
+
+
 def box_triangle(box_min, box_max, box_to_mesh, a, b, c, clip=False) -> int:
     """One box against one triangle: OrientedBox's separating-axis test (clip=False) or the independent
     clipping formulation (clip=True; -1 for a singular matrix)."""

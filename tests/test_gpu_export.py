@@ -2,7 +2,9 @@
 _nodes) through the C ABI: MeshBVHUniformStruct's bvhBounds / bvhContents / index / position
 textures and the WebGPU BVHNode buffer equal the oracle's restatement of the reference's packing of
 the same tree and geometry (uint16 and uint32 index, indirect, non-indexed, groups, after a refit);
-the device form equals the host form."""
+the device form equals the host form. Imports (mbvh_mesh_import_nodes / _textures): the packed
+buffers back to device meshes whose node buffers equal the original roots (and the oracle's inverse
+restatement) and whose queries equal the original mesh's bit for bit."""
 import numpy as np
 import pytest
 
@@ -98,3 +100,60 @@ def test_device_form(ctx):
     assert np.array_equal(position.cpu().numpy().view(np.uint32), host["position"].view(np.uint32))
     assert np.array_equal(nodes.cpu().numpy(), tree.roots[0])
     mesh.free()
+
+
+def _same_queries(a, b, g):
+    lo, hi = g.bounds()
+    rng = np.random.default_rng(8)
+    o = (lo + (hi - lo) * rng.random((2000, 3)) * 3 - (hi - lo)).astype(np.float32)
+    t = (lo + (hi - lo) * rng.random((2000, 3))).astype(np.float32)
+    d = (t - o) / np.linalg.norm(t - o, axis=1, keepdims=True)
+    ra, rb, ca, cb = a.raycast_first(o, d), b.raycast_first(o, d), a.closest_point(t), b.closest_point(t)
+    for k in ("face", "distance", "point", "barycoord"):  # bit for bit (misses: NaN points)
+        assert np.array_equal(np.asarray(ra[k]).view(np.uint32), np.asarray(rb[k]).view(np.uint32)), k
+        assert np.array_equal(np.asarray(ca[k]).view(np.uint32), np.asarray(cb[k]).view(np.uint32)), k
+
+
+@pytest.mark.parametrize("name,make,indirect,ranges", [
+    ("torusknot_u16", lambda: G.torus_knot(10, 3, 200, 40), False, None),
+    ("sphere_u32_groups_empty", lambda: G.sphere(1, 300, 150), False, "empty"),
+    ("indirect", lambda: G.torus(1, 0.4, 30, 90), True, None),
+])
+def test_import_nodes_and_textures(ctx, name, make, indirect, ranges):
+    print(f"case {name}", flush=True)
+    g0 = make()
+    n = g0.tri_count
+    rr = [(0, n // 3), (n // 3, 0), (n // 3, n - n // 3)] if ranges == "empty" else None
+    g, idx, tree = build_mesh(g0, indirect=indirect, ranges=rr)
+    mesh = ctx.upload(tree, g.positions, idx)
+    for off in (0, 12345):
+        words = O.webgpu_nodes(tree.roots, geometry_offset=off)
+        back = O.webgpu_nodes_to_roots(words, off)
+        assert len(back) == len(tree.roots)
+        for a, b in zip(back, tree.roots):
+            assert np.array_equal(a, np.asarray(b).view(np.uint8))
+        imp = ctx.import_nodes(words, g.positions, idx, geometry_offset=off, indirect=tree.indirect)
+        for a, b in zip(imp.copy_roots(), tree.roots):
+            assert np.array_equal(a, np.asarray(b).view(np.uint8))
+        _same_queries(mesh, imp, g)
+        imp.free()
+    if len(tree.roots) == 1:
+        tex = mesh.export_textures(g.positions)
+        nn = tree.roots[0].nbytes // 32
+        assert np.array_equal(O.textures_to_root(tex["bounds"], tex["contents"], nn), np.asarray(tree.roots[0]).view(np.uint8))
+        imp = ctx.import_textures(tex["bounds"], tex["contents"], nn, g.positions, idx, indirect=tree.indirect)
+        assert np.array_equal(imp.copy_roots()[0], np.asarray(tree.roots[0]).view(np.uint8))
+        _same_queries(mesh, imp, g)
+        imp.free()
+    mesh.free()
+
+
+def test_import_rejects_bad_buffers(ctx):
+    g, idx, tree = build_mesh(G.torus_knot(1, 0.3, 40, 8))
+    words = O.webgpu_nodes(tree.roots).view(np.uint32).reshape(-1, 8).copy()
+    bad = words.copy()
+    bad[0, 6] = bad.shape[0] + 5  # the root's right child beyond the buffer
+    with pytest.raises(M.MbvhError):
+        ctx.import_nodes(bad, g.positions, idx)
+    with pytest.raises(M.MbvhError):  # leaf offsets below the geometry offset
+        ctx.import_nodes(words, g.positions, idx, geometry_offset=7)

@@ -70,3 +70,20 @@ def test_round_trip_and_webgpu_nodes():
     fixed = O.webgpu_nodes([empty], geometry_offset=7)
     assert fixed.view(np.float32)[:6].tolist() == [1, 1, 1, -1, -1, -1]
     assert fixed.view(np.uint32)[6] == 7
+
+
+def test_webgpu_nodes_round_trip():
+    """webgpu_nodes_to_roots inverts webgpu_nodes: several roots (one empty, whose [inf, -inf] bounds
+    the packing writes as [1, -1]) and a nonzero geometry offset."""
+    from mbvh_amd import geometry as G
+    from helpers import build_mesh
+    g0 = G.sphere(1, 40, 20)
+    n = g0.tri_count
+    g, idx, tree = build_mesh(g0, ranges=[(0, n // 3), (n // 3, 0), (n // 3, n - n // 3)])
+    assert len(tree.roots) == 3
+    w = O.webgpu_nodes(tree.roots, geometry_offset=999)
+    assert w.view(np.float32)[8 * (tree.roots[0].nbytes // 32):][:6].tolist() == [1, 1, 1, -1, -1, -1]
+    back = O.webgpu_nodes_to_roots(w, 999)
+    assert len(back) == 3
+    for a, b in zip(back, tree.roots):
+        assert np.array_equal(a, np.asarray(b).view(np.uint8))

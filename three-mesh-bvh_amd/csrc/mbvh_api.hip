@@ -936,6 +936,100 @@ extern "C" int mbvh_mesh_refit_d(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* po
                        stream ? static_cast<hipStream_t>(stream) : ctx->stream);
 }
 
+// ---- imports: the node words of the WebGPU / WebGL packings back to reference node buffers ----
+
+namespace {
+constexpr uint32_t kLeafWord = 0xFFFF0000u;  // IS_LEAF: the upper half-word of word 7 set
+
+int import_roots(const char* fn, std::vector<std::vector<uint32_t>>& roots, mbvh_ctx* ctx, const void* index,
+                 int32_t index_elt_bytes, uint64_t index_count, const void* indirect, int32_t indirect_elt_bytes,
+                 uint64_t indirect_count, const float* positions, uint64_t vertex_count, uint32_t position_stride,
+                 uint32_t position_offset, mbvh_mesh** out_mesh) {
+  if (roots.empty()) return set_error(MBVH_ERR_BAD_ARGS, "%s: no nodes", fn);
+  std::vector<const void*> ptrs;
+  std::vector<uint64_t> bytes;
+  for (const auto& r : roots) {
+    ptrs.push_back(r.data());
+    bytes.push_back(r.size() * sizeof(uint32_t));
+  }
+  return mbvh_mesh_upload(ctx, ptrs.data(), bytes.data(), static_cast<uint32_t>(roots.size()), index, index_elt_bytes,
+                          index_count, indirect, indirect_elt_bytes, indirect_count, positions, vertex_count,
+                          position_stride, position_offset, out_mesh);
+}
+}  // namespace
+
+extern "C" int mbvh_mesh_import_nodes(mbvh_ctx* ctx, const uint32_t* nodes, uint64_t n_nodes, uint32_t geometry_offset,
+                                      const void* index, int32_t index_elt_bytes, uint64_t index_count,
+                                      const void* indirect, int32_t indirect_elt_bytes, uint64_t indirect_count,
+                                      const float* positions, uint64_t vertex_count, uint32_t position_stride,
+                                      uint32_t position_offset, mbvh_mesh** out_mesh) {
+  clear_error();
+  const char* fn = "mbvh_mesh_import_nodes";
+  if (!out_mesh) return set_error(MBVH_ERR_BAD_ARGS, "%s: out_mesh is NULL", fn);
+  *out_mesh = nullptr;
+  if (!nodes || n_nodes == 0) return set_error(MBVH_ERR_BAD_ARGS, "%s: no nodes", fn);
+  std::vector<std::vector<uint32_t>> roots;
+  uint64_t start = 0;
+  while (start < n_nodes) {
+    // a subtree's extent: the right spine's offsets plus its last leaf
+    uint64_t j = start, size = 1;
+    while ((nodes[8 * j + 7] & kLeafWord) != kLeafWord) {
+      const uint64_t r = nodes[8 * j + 6];
+      if (r == 0 || j + r >= n_nodes)
+        return set_error(MBVH_ERR_BAD_ARGS, "%s: node %llu's right child %llu is outside the %llu nodes", fn,
+                         static_cast<unsigned long long>(j), static_cast<unsigned long long>(j + r),
+                         static_cast<unsigned long long>(n_nodes));
+      j += r;
+      size += r;
+    }
+    std::vector<uint32_t> root(nodes + 8 * start, nodes + 8 * (start + size));
+    float b[6];
+    std::memcpy(b, root.data(), sizeof(b));
+    for (int c = 0; c < 3; c++)
+      if (b[c] > b[c + 3]) {  // the packing's [1, -1] for an empty root
+        b[c] = std::numeric_limits<float>::infinity();
+        b[c + 3] = -std::numeric_limits<float>::infinity();
+      }
+    std::memcpy(root.data(), b, sizeof(b));
+    for (uint64_t k = 0; k < size; k++) {
+      uint32_t* w = root.data() + 8 * k;
+      if ((w[7] & kLeafWord) == kLeafWord) {
+        if (w[6] < geometry_offset)
+          return set_error(MBVH_ERR_BAD_ARGS, "%s: node %llu's triangle offset %u is below the geometry offset %u", fn,
+                           static_cast<unsigned long long>(start + k), w[6], geometry_offset);
+        w[6] -= geometry_offset;
+      }
+    }
+    roots.push_back(std::move(root));
+    start += size;
+  }
+  return import_roots(fn, roots, ctx, index, index_elt_bytes, index_count, indirect, indirect_elt_bytes, indirect_count,
+                      positions, vertex_count, position_stride, position_offset, out_mesh);
+}
+
+extern "C" int mbvh_mesh_import_textures(mbvh_ctx* ctx, const float* bvh_bounds, const uint32_t* bvh_contents,
+                                         uint64_t n_nodes, const void* index, int32_t index_elt_bytes,
+                                         uint64_t index_count, const void* indirect, int32_t indirect_elt_bytes,
+                                         uint64_t indirect_count, const float* positions, uint64_t vertex_count,
+                                         uint32_t position_stride, uint32_t position_offset, mbvh_mesh** out_mesh) {
+  clear_error();
+  const char* fn = "mbvh_mesh_import_textures";
+  if (!out_mesh) return set_error(MBVH_ERR_BAD_ARGS, "%s: out_mesh is NULL", fn);
+  *out_mesh = nullptr;
+  if (!bvh_bounds || !bvh_contents || n_nodes == 0) return set_error(MBVH_ERR_BAD_ARGS, "%s: no nodes", fn);
+  std::vector<std::vector<uint32_t>> roots(1, std::vector<uint32_t>(8 * n_nodes));
+  uint32_t* w = roots[0].data();
+  for (uint64_t i = 0; i < n_nodes; i++, w += 8) {
+    std::memcpy(w, bvh_bounds + 8 * i, 3 * sizeof(float));         // texel 2i: min.xyz
+    std::memcpy(w + 3, bvh_bounds + 8 * i + 4, 3 * sizeof(float));  // texel 2i + 1: max.xyz
+    const uint32_t c0 = bvh_contents[2 * i], c1 = bvh_contents[2 * i + 1];
+    w[6] = c1;
+    w[7] = (c0 & kLeafWord) == kLeafWord ? (kLeafWord | (c0 & 0xFFFFu)) : c0;
+  }
+  return import_roots(fn, roots, ctx, index, index_elt_bytes, index_count, indirect, indirect_elt_bytes, indirect_count,
+                      positions, vertex_count, position_stride, position_offset, out_mesh);
+}
+
 extern "C" int mbvh_mesh_copy_roots(mbvh_ctx* ctx, const mbvh_mesh* mesh, void* const* roots_out,
                                     const uint64_t* root_bytes, uint32_t n_roots) {
   clear_error();

@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "mbvh_closest_point_to_geometry", "mbvh_closest_point_to_geometry_d",
     "mbvh_scene_closest_point", "mbvh_scene_closest_point_d", "mbvh_mesh_texture_sizes", "mbvh_mesh_export_textures",
     "mbvh_mesh_export_textures_d", "mbvh_mesh_export_nodes", "mbvh_mesh_export_nodes_d",
+    "mbvh_mesh_import_nodes", "mbvh_mesh_import_textures",
     "mbvh_batch_autotune", "mbvh_batch_graph_create", "mbvh_batch_graph_launch", "mbvh_batch_graph_free",
     "mbvh_instance_rays_d", "mbvh_instance_hits_d",
     "mbvh_grad_workspace_bytes", "mbvh_raycast_first_bwd", "mbvh_raycast_first_bwd_d",
@@ -150,6 +151,10 @@ def library() -> C.CDLL:
         "mbvh_mesh_export_textures": (C.c_int, [vp, vp, vp, u32, u32, vp, vp, vp, vp]),
         "mbvh_mesh_export_textures_d": (C.c_int, [vp, vp, vp, u32, u32, vp, vp, vp, vp, vp]),
         "mbvh_mesh_export_nodes": (C.c_int, [vp, vp, vp, u64]),
+        "mbvh_mesh_import_nodes": (C.c_int, [vp, vp, u64, u32, vp, i32, u64, vp, i32, u64, vp, u64, u32, u32,
+                                             C.POINTER(vp)]),
+        "mbvh_mesh_import_textures": (C.c_int, [vp, vp, vp, u64, vp, i32, u64, vp, i32, u64, vp, u64, u32, u32,
+                                                C.POINTER(vp)]),
         "mbvh_mesh_export_nodes_d": (C.c_int, [vp, vp, vp, u64, vp]),
         "mbvh_batch_graph_launch": (C.c_int, [vp, vp]),
         "mbvh_batch_graph_free": (None, [vp]),
@@ -300,6 +305,45 @@ class Context:
     def batch(self, meshes: Sequence["Mesh"]) -> "Batch":
         return Batch(self, meshes)
 
+    def import_nodes(self, nodes: np.ndarray, positions: np.ndarray, index: Optional[np.ndarray],
+                     geometry_offset: int = 0, indirect: Optional[np.ndarray] = None) -> "Mesh":
+        """A device mesh from a WebGPU BVHNode buffer (uint32 words, 8 per node, the mesh's roots back
+        to back; leaf offsets carrying `geometry_offset`), as mbvh_mesh_import_nodes."""
+        words = np.ascontiguousarray(nodes)
+        words = (words.view(np.uint32) if words.dtype == np.uint8 else words.astype(np.uint32)).reshape(-1)
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+        h = C.c_void_p()
+        _check(self._lib.mbvh_mesh_import_nodes(
+            self.handle, _ptr(words), words.size // 8, geometry_offset,
+            _ptr(index), 0 if index is None else index.itemsize, 0 if index is None else index.size,
+            _ptr(indirect), 0 if indirect is None else indirect.itemsize, 0 if indirect is None else indirect.size,
+            _ptr(pos), pos.size // 3, 3, 0, C.byref(h)))
+        # the roots' extents, as the import found them (each subtree's span)
+        w = words.reshape(-1, 8)
+        sizes, start = [], 0
+        while start < w.shape[0]:
+            size = _subtree_span(w, start)
+            sizes.append(size * 32)
+            start += size
+        return Mesh._adopt(self, h, sizes)
+
+    def import_textures(self, bvh_bounds: np.ndarray, bvh_contents: np.ndarray, n_nodes: int, positions: np.ndarray,
+                        index: Optional[np.ndarray], indirect: Optional[np.ndarray] = None) -> "Mesh":
+        """A device mesh from MeshBVHUniformStruct's bvhBounds (float32, 4 per texel) and bvhContents
+        (uint32, 2 per texel) of one root of n_nodes nodes, as mbvh_mesh_import_textures."""
+        b = np.ascontiguousarray(bvh_bounds, np.float32).reshape(-1)
+        c = np.ascontiguousarray(bvh_contents, np.uint32).reshape(-1)
+        if b.size < 8 * n_nodes or c.size < 2 * n_nodes:
+            raise ValueError("textures smaller than n_nodes nodes")
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+        h = C.c_void_p()
+        _check(self._lib.mbvh_mesh_import_textures(
+            self.handle, _ptr(b), _ptr(c), n_nodes,
+            _ptr(index), 0 if index is None else index.itemsize, 0 if index is None else index.size,
+            _ptr(indirect), 0 if indirect is None else indirect.itemsize, 0 if indirect is None else indirect.size,
+            _ptr(pos), pos.size // 3, 3, 0, C.byref(h)))
+        return Mesh._adopt(self, h, [n_nodes * 32])
+
     def instance_rays_d(self, inverse_ptr: int, n_instances: int, origins_ptr: int, dirs_ptr: int, ray_inst_ptr: int,
                         n: int, out_origins_ptr: int, out_dirs_ptr: int, stream: int = 0) -> None:
         v = lambda x: C.c_void_p(x or None)
@@ -338,8 +382,27 @@ def _geometry_hit(r):
             "other_face_index": int(r["other_face_index"])}
 
 
+def _subtree_span(words: np.ndarray, start: int) -> int:
+    """Nodes in the depth-first subtree at `start` of (n, 8) node words: the right spine's offsets
+    plus its last leaf."""
+    j, size = start, 1
+    while (int(words[j, 7]) & 0xFFFF0000) != 0xFFFF0000:
+        r = int(words[j, 6])
+        j += r
+        size += r
+    return size
+
+
 class Mesh:
     """Device-resident BVH (mbvh_mesh)."""
+
+    @classmethod
+    def _adopt(cls, ctx: "Context", handle, root_bytes) -> "Mesh":
+        m = cls.__new__(cls)
+        m.ctx = ctx
+        m.handle = handle
+        m.root_bytes = list(root_bytes)
+        return m
 
     def __init__(self, ctx: Context, tree: Tree, positions: np.ndarray, index: Optional[np.ndarray],
                  stride: int = 3, offset: int = 0):
