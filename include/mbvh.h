@@ -207,6 +207,23 @@ int mbvh_mesh_refit(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint
  * Queries launched after it on the same stream see the refit geometry. */
 int mbvh_mesh_refit_d(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint64_t vertex_count,
                       uint32_t position_stride, uint32_t position_offset, void* stream);
+/*
+ * Partial refit (MeshBVH.refit(nodeIndices), src/core/MeshBVH.js:504-508 over
+ * src/core/cast/refit.template.js:4-196): node_ids holds n_ids node indices counted over all roots
+ * (root r's first node is the node count of the roots before it; the numbers shapecast reports,
+ * src/core/BVH.js:394-409). The refit visits the nodes the reference visits for that set: every
+ * root; below a visited node, the children in the set, or both (and then their whole subtrees) when
+ * neither is. Nodes not visited keep their boxes and count as unchanged. Ids that name no node
+ * match nothing; repeated ids are harmless. node_ids NULL is a full refit (nodeIndices = null).
+ * The triangles' device copies take the new positions in any case. Otherwise as mbvh_mesh_refit;
+ * the _d form takes device positions and device ids, as mbvh_mesh_refit_d.
+ */
+int mbvh_mesh_refit_nodes(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint64_t vertex_count,
+                          uint32_t position_stride, uint32_t position_offset, const uint32_t* node_ids, uint64_t n_ids,
+                          void* const* roots_out, const uint64_t* root_bytes, uint32_t n_roots);
+int mbvh_mesh_refit_nodes_d(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint64_t vertex_count,
+                            uint32_t position_stride, uint32_t position_offset, const uint32_t* node_ids, uint64_t n_ids,
+                            void* stream);
 /* Copies the mesh's current reference-format node buffers (as refit) into roots_out[r]
  * (root_bytes[r] bytes each, n_roots entries; NULL entries skipped). Synchronous. */
 int mbvh_mesh_copy_roots(mbvh_ctx* ctx, const mbvh_mesh* mesh, void* const* roots_out, const uint64_t* root_bytes,

@@ -810,13 +810,30 @@ int orc_validate_root(const orc_mesh* m, uint32_t root, orc_tree_stats* st) {
 }
 
 /* ------------------------------------------------------------------------------------------ */
-/* refit (src/core/cast/refit.template.js:4-196 with nodeIndices = null)                       */
+/* refit (src/core/cast/refit.template.js:4-196)                                               */
 /* ------------------------------------------------------------------------------------------ */
+
+/* The node set of a partial refit: member[id] != 0 for the ids in nodeIndices, an id being the
+ * node's index over all roots (this root's first node is `base`); NULL for nodeIndices = null. */
+typedef struct {
+  const uint8_t* member;
+  uint64_t n_member;
+  uint64_t base;
+} refit_set;
+
+static int refit_has(const refit_set* set, uint64_t node) {
+  const uint64_t id = set->base + node;
+  return id < set->n_member && set->member[id] != 0;
+}
 
 /* One node of _traverse: a leaf's box is the min / max over its triangles' vertices in slot order
  * (strict < and >), written only if a component differs (!==); an internal node is the union of its
- * children, recomputed only when a child changed. Returns whether the node changed. */
-static int refit_node(const orc_mesh* m, uint8_t* nodes, uint64_t n_nodes, uint64_t node, int depth) {
+ * children, recomputed only when a child changed. With a node set (refit.template.js:136-172), an
+ * internal node not forced visits the children in the set, or, when neither is, both, forced (and a
+ * forced node forces its children); a child not visited counts as unchanged. Returns whether the
+ * node changed. */
+static int refit_node(const orc_mesh* m, uint8_t* nodes, uint64_t n_nodes, uint64_t node, int depth,
+                      const refit_set* set, int force) {
   if (node >= n_nodes || depth > 4096) return 0;
   float* box = (float*)(nodes + 32 * node);
   uint16_t flag, count;
@@ -849,8 +866,15 @@ static int refit_node(const orc_mesh* m, uint8_t* nodes, uint64_t n_nodes, uint6
     return changed;
   }
   const uint64_t left = node + 1, right = node + word6;
-  const int lc = refit_node(m, nodes, n_nodes, left, depth + 1);
-  const int rc = refit_node(m, nodes, n_nodes, right, depth + 1);
+  int visit_left = 1, visit_right = 1, force_children = force;
+  if (set && !force) {
+    const int has_left = refit_has(set, left), has_right = refit_has(set, right);
+    force_children = !has_left && !has_right;
+    visit_left = force_children || has_left;
+    visit_right = force_children || has_right;
+  }
+  const int lc = visit_left ? refit_node(m, nodes, n_nodes, left, depth + 1, set, force_children) : 0;
+  const int rc = visit_right ? refit_node(m, nodes, n_nodes, right, depth + 1, set, force_children) : 0;
   if (lc || rc) {
     const float* lb = (const float*)(nodes + 32 * left);
     const float* rb = (const float*)(nodes + 32 * right);
@@ -863,7 +887,13 @@ static int refit_node(const orc_mesh* m, uint8_t* nodes, uint64_t n_nodes, uint6
 }
 
 void orc_refit_root(const orc_mesh* m, uint8_t* nodes, uint64_t n_bytes) {
-  if (n_bytes >= 32) refit_node(m, nodes, n_bytes / 32, 0, 0);
+  if (n_bytes >= 32) refit_node(m, nodes, n_bytes / 32, 0, 0, NULL, 0);
+}
+
+void orc_refit_root_nodes(const orc_mesh* m, uint8_t* nodes, uint64_t n_bytes, uint64_t base, const uint8_t* member,
+                          uint64_t n_member) {
+  const refit_set set = {member, n_member, base};
+  if (n_bytes >= 32) refit_node(m, nodes, n_bytes / 32, 0, 0, member ? &set : NULL, 0);
 }
 
 /* ------------------------------------------------------------------------------------------ */

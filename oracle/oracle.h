@@ -110,6 +110,10 @@ int orc_validate_root(const orc_mesh* m, uint32_t root, orc_tree_stats* stats);
  * of one root's node buffer `nodes` (n_bytes, modified in place) from the mesh's positions, index
  * and indirect buffer. */
 void orc_refit_root(const orc_mesh* m, uint8_t* nodes, uint64_t n_bytes);
+/* refit(nodeIndices) for one root whose first node has index `base` over all roots: member[id] != 0
+ * (id < n_member) for the ids in the set; member NULL is nodeIndices = null. */
+void orc_refit_root_nodes(const orc_mesh* m, uint8_t* nodes, uint64_t n_bytes, uint64_t base, const uint8_t* member,
+                          uint64_t n_member);
 
 /* MeshBVH.intersectsSphere by brute force over the geometry's triangles with the reference's
  * sphereIntersectTriangle (src/math/MathUtilities.js:160-204): segment closest points, then the

@@ -93,6 +93,8 @@ def library():
         lib.orc_closest_to_geometry.restype = u32
         lib.orc_refit_root.argtypes = [vp, vp, C.c_uint64]
         lib.orc_refit_root.restype = None
+        lib.orc_refit_root_nodes.argtypes = [vp, vp, C.c_uint64, C.c_uint64, vp, C.c_uint64]
+        lib.orc_refit_root_nodes.restype = None
         lib.orc_validate_root.argtypes = [vp, u32, C.POINTER(TreeStats)]
         lib.orc_validate_root.restype = C.c_int
         _lib = lib
@@ -258,13 +260,27 @@ class OracleMesh:
         return np.array([bool(library().orc_intersects_box(self.ref, _p(mn[i]), _p(mx[i]), _p(m[i])))
                          for i in range(mn.shape[0])])
 
-    def refit(self):
-        """The roots after MeshBVH.refit over this mesh's positions (new arrays; self.roots unchanged)."""
+    def refit(self, node_ids=None):
+        """The roots after MeshBVH.refit(nodeIndices) over this mesh's positions (new arrays; self.roots
+        unchanged). node_ids: None (a full refit) or the node indices over all roots (refit.template.js
+        numbers root r's nodes from the node count of the roots before it)."""
+        total = sum(r.nbytes // 32 for r in self.roots)
+        member = None
+        if node_ids is not None:
+            ids = np.asarray(list(node_ids), dtype=np.int64).reshape(-1)
+            member = np.zeros(max(total, 1), np.uint8)
+            member[ids[(ids >= 0) & (ids < total)]] = 1
         out = []
+        base = 0
         for r in self.roots:
             nodes = r.copy()
-            library().orc_refit_root(self.ref, nodes.ctypes.data, nodes.nbytes)
+            if member is None:
+                library().orc_refit_root(self.ref, nodes.ctypes.data, nodes.nbytes)
+            else:
+                library().orc_refit_root_nodes(self.ref, nodes.ctypes.data, nodes.nbytes, base, member.ctypes.data,
+                                               member.size)
             out.append(nodes)
+            base += r.nbytes // 32
         return out
 
     def validate_root(self, root: int = 0):

@@ -405,7 +405,6 @@ for (let v = 0; v < positions.length / 3; v++) {
   positions[3 * v + 2] = z + 0.25 * Math.sin(0.3 * x);
 }
 bvh.refit();
-assert.throws(() => bvh.refit([0, 1]), /nodeIndices/);
 writeArr('refit_positions.bin', positions);
 writeArr('refit_root0.bin', new Uint8Array(bvh._roots[0]));
 const rr = bvh.raycastFirstBatch(origins, dirs);
@@ -413,5 +412,14 @@ writeArr('refit_rf_face.bin', rr.faceIndex);
 writeArr('refit_rf_dist.bin', rr.distance);
 writeArr('refit_rf_point.bin', rr.point);
 writeArr('refit_rf_bary.bin', rr.barycoord);
+
+// --- partial refit: a Set of node ids (node 1, every 7th node, values that name no node) -----
+for (let v = 0; v < positions.length / 3; v += 11) positions[3 * v + 1] += 0.5;
+const nNodes = bvh._roots[0].byteLength / 32;
+const partialIds = new Set([-1, 2.5, 'x', nNodes + 3, 1]);  // node 1: the root's left child
+for (let i = 0; i < nNodes; i += 7) partialIds.add(i);
+bvh.refit(partialIds);
+writeArr('partial_positions.bin', positions);
+writeArr('partial_root0.bin', new Uint8Array(bvh._roots[0]));
 
 console.log(JSON.stringify({ ok: true, rays: nRays, points: nPts, worldChecked, batchedChecked, presenceDiffers }));

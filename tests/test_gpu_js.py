@@ -189,3 +189,10 @@ def test_js_api_matches_oracle(tmp_path):
             "barycoord": np.fromfile(d / "refit_rf_bary.bin", np.float32).reshape(-1, 3)}
     print("js refit raycastFirst", check_raycast_parity(om2, w.origins, w.directions, got2,
                                                         om2.raycast_first(w.origins, w.directions), "js_refit"))
+    # MeshBVH.refit(Set of node ids) after a second edit: the oracle's partial refit of the same set
+    # (the values that name no node are dropped on both sides)
+    pos3 = np.fromfile(d / "partial_positions.bin", np.float32)
+    root3 = np.fromfile(d / "partial_root0.bin", np.uint8)
+    ids = {1} | set(range(0, root2.size // 32, 7))
+    assert np.array_equal(root3, O.OracleMesh(pos3, idx, [root2]).refit(ids)[0])
+    assert not np.array_equal(root3, O.OracleMesh(pos3, idx, [root2]).refit()[0])  # it was partial

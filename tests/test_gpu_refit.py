@@ -153,3 +153,126 @@ def test_refit_device_form_in_an_optimisation_step(ctx):
     _same(got, host.raycast_first(o2, d2))
     host.free()
     mesh.free()
+
+
+# ---- partial refit (refit(nodeIndices), refit.template.js:136-172) ----------------------------
+
+def _partial_cases(total, rng):
+    return {
+        "sparse": rng.choice(total, max(1, total // 20), replace=False),
+        "dense": rng.choice(total, total // 2, replace=False),
+        "empty": np.zeros(0, np.int64),
+        "all": np.arange(total),
+        "foreign": np.array([total, total + 7, 2**32 - 1, 3]),  # ids past the tree name nothing
+        "repeated": np.array([1, 1, 2, 2, 2]),
+    }
+
+
+@pytest.mark.parametrize("kind", ["plain", "indirect", "two_roots"])
+def test_partial_refit_matches_oracle(ctx, kind):
+    """Device partial refits of random node sets equal the oracle's byte for byte."""
+    geo = G.torus_knot(10, 3, 200, 24)
+    ranges = None
+    if kind == "two_roots":
+        geo.ensure_index()
+        ntri = geo.index.size // 3
+        ranges = [(0, ntri // 3), (ntri // 3, ntri - ntri // 3)]
+    g, idx, tree = build_mesh(geo, ranges=ranges, indirect=kind == "indirect")
+    total = sum(r.nbytes // 32 for r in tree.roots)
+    rng = np.random.default_rng(21)
+    for name, ids in _partial_cases(total, rng).items():
+        mesh = ctx.upload(tree, g.positions, idx)
+        pos = _deform(g.positions, 22)
+        got = mesh.refit(pos, node_ids=ids)
+        ref = O.OracleMesh(pos, idx, tree.roots, indirect=tree.indirect).refit(ids)
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), name
+        # the query layouts carry the same boxes as a fresh upload of those nodes
+        fresh = ctx.upload(M.Tree(ref, tree.indirect), pos, idx)
+        o, d = _rays(pos.min(axis=0), pos.max(axis=0), 3000, 23)
+        _same(mesh.raycast_first(o, d), fresh.raycast_first(o, d))
+        pts = W.grid_points(pos.min(axis=0), pos.max(axis=0), 12)
+        _same(mesh.closest_point(pts), fresh.closest_point(pts))
+        fresh.free()
+        mesh.free()
+
+
+def _leaves_and_parents(roots):
+    """Per root: (base id, parent of each node (-1 at the root), leaf (node, offset, count) list)."""
+    out, base = [], 0
+    for r in roots:
+        f = r.view(np.float32).reshape(-1, 8)
+        u = r.view(np.uint32).reshape(-1, 8)
+        h = r.view(np.uint16).reshape(-1, 16)
+        n = f.shape[0]
+        parent = np.full(n, -1, np.int64)
+        leaves = []
+        for i in range(n):
+            if h[i, 15] == 0xFFFF:
+                leaves.append((i, int(u[i, 6]), int(h[i, 14])))
+            else:
+                parent[i + 1] = i
+                parent[i + int(u[i, 6])] = i
+        out.append((base, parent, leaves))
+        base += n
+    return out
+
+
+def test_partial_refit_of_changed_paths_is_a_full_refit(ctx):
+    """Moving a few vertices and listing the leaves that use them with their ancestors (what a
+    shapecast over the edited region reports) gives the full refit exactly; queries then match the
+    FP64 oracle on the edited geometry."""
+    geo = G.torus_knot(10, 3, 300, 40)
+    geo.ensure_index()
+    ntri = geo.index.size // 3
+    g, idx, built = build_mesh(geo, ranges=[(0, ntri // 2), (ntri // 2, ntri - ntri // 2)])
+    # tight boxes first (the build's are rounded outwards, so a full refit would rewrite every leaf)
+    tree = M.Tree(O.OracleMesh(g.positions, idx, built.roots).refit(), None)
+    mesh = ctx.upload(tree, g.positions, idx)
+    rng = np.random.default_rng(31)
+    pos = g.positions.reshape(-1, 3).copy()
+    moved = rng.choice(len(pos), 6, replace=False)
+    pos[moved] += rng.normal(0, 0.4, (len(moved), 3)).astype(np.float32)
+    tri = idx.reshape(-1, 3)
+    touched = np.isin(tri, moved).any(axis=1)
+    ids = set()
+    for base, parent, leaves in _leaves_and_parents(tree.roots):
+        for node, off, cnt in leaves:
+            if touched[off:off + cnt].any():
+                j = node
+                while j >= 0:
+                    ids.add(base + j)
+                    j = parent[j]
+    full = O.OracleMesh(pos, idx, tree.roots).refit()
+    part = O.OracleMesh(pos, idx, tree.roots).refit(ids)
+    assert all(np.array_equal(a, b) for a, b in zip(part, full))
+    got = mesh.refit(pos, node_ids=sorted(ids))
+    assert all(np.array_equal(a, b) for a, b in zip(got, full))
+    om = O.OracleMesh(pos, idx, full)
+    o, d = _rays(pos.min(axis=0), pos.max(axis=0), 10000, 32)
+    check_raycast_parity(om, o, d, mesh.raycast_first(o, d), om.raycast_first(o, d), "partial refit rays")
+    pts = W.grid_points(pos.min(axis=0), pos.max(axis=0), 20)
+    check_closest_parity(om, pts, mesh.closest_point(pts), om.closest_point(pts), "partial refit points")
+    mesh.free()
+
+
+def test_partial_refit_device_form(ctx):
+    """Device positions and device node ids (mbvh_mesh_refit_nodes_d) == the host form."""
+    torch = pytest.importorskip("torch")
+    dev = torch.device("cuda:0")
+    g, idx, tree = build_mesh(G.torus_knot(10, 3, 200, 24))
+    total = tree.roots[0].nbytes // 32
+    ids = np.random.default_rng(41).choice(total, total // 10, replace=False).astype(np.uint32)
+    pos = _deform(g.positions, 42)
+    a = ctx.upload(tree, g.positions, idx)
+    b = ctx.upload(tree, g.positions, idx)
+    pos_t = torch.from_numpy(pos).to(dev)
+    ids_t = torch.from_numpy(ids.astype(np.int32)).to(dev)
+    torch.cuda.synchronize()
+    a.refit_nodes_d(pos_t.data_ptr(), pos_t.shape[0], ids_t.data_ptr(), ids.size)
+    ctx.synchronize()
+    want = b.refit(pos, node_ids=ids)
+    assert all(np.array_equal(x, y) for x, y in zip(a.copy_roots(), want))
+    assert all(np.array_equal(x, y) for x, y in zip(want, O.OracleMesh(pos, idx, tree.roots).refit(ids)))
+    a.free()
+    b.free()

@@ -3,8 +3,10 @@ refit is tight and idempotent; after a deformation every leaf box is exactly the
 its triangles' vertex coordinates and every internal box the union of its children (checked with
 numpy, independently of the oracle), and the tree validates; only changed paths are rewritten (an
 inconsistent internal box above unchanged leaves is kept, as the reference keeps it); -0 and +0 are
-equal (a stored -0 survives)."""
+equal (a stored -0 survives). Partial refits (nodeIndices) against boxes worked out by hand on a
+five-node tree and a two-root tree, and the reference's own case (every node listed = a full refit)."""
 import numpy as np
+import pytest
 
 import mbvh_amd as M
 import oracle_py as O
@@ -108,3 +110,86 @@ def test_refit_signed_zero_is_unchanged():
     f[0, 2] = np.float32(-0.0)  # stored -0 for min.z; the vertices give +0
     out = O.OracleMesh(pos, idx, [buf]).refit()[0]
     assert np.signbit(_nodes(out)[0][0, 2])  # -0 == +0: the leaf is unchanged, -0 stays
+
+
+# ---- refit(nodeIndices): refit.template.js:136-172 -------------------------------------------
+
+def _node_bytes(specs):
+    """Reference node buffer from (box6, kind, a, b): kind 'leaf' (a = offset, b = count) or 'inner'
+    (a = right child offset, b = split axis)."""
+    buf = np.zeros(32 * len(specs), np.uint8)
+    f, u, h = _nodes(buf)
+    for i, (box, kind, a, b) in enumerate(specs):
+        f[i, :6] = box
+        u[i, 6] = a
+        if kind == "leaf":
+            h[i, 14], h[i, 15] = b, 0xFFFF
+        else:
+            u[i, 7] = b
+    return buf
+
+
+# three triangles: one in z = 0 near x = 0, one in z = 0 near x = 2, one in z = 5
+_TRIS = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0],
+                  [2, 0, 0], [3, 0, 0], [2, 1, 0],
+                  [0, 0, 5], [1, 0, 5], [0, 1, 5]], np.float32)
+_B0, _B1, _B2 = [0, 0, 0, 1, 1, 0], [2, 0, 0, 3, 1, 0], [0, 0, 5, 1, 1, 5]
+# node 0 = (node 1 = (leaf 2: tri 0, leaf 3: tri 1), leaf 4: tri 2)
+_FIVE = [([0, 0, 0, 3, 1, 5], "inner", 4, 2), ([0, 0, 0, 3, 1, 0], "inner", 2, 0),
+         (_B0, "leaf", 0, 1), (_B1, "leaf", 1, 1), (_B2, "leaf", 2, 1)]
+
+
+def _boxes(buf):
+    return [list(map(float, r)) for r in _nodes(buf)[0][:, :6]]
+
+
+@pytest.mark.parametrize("ids,expect", [
+    # {1}: the root visits node 1 only; node 1 has neither child listed, so both are refit (forced);
+    # leaf 4 keeps its old box and the root is the union of new node 1 and old leaf 4
+    ({1}, [[0, 0, 0, 6, 2, 5], [0, 0, 0, 6, 2, 0], [0, 0, 0, 2, 2, 0], [4, 0, 0, 6, 2, 0], _B2]),
+    # {1, 3}: node 1 visits leaf 3 only; leaf 2 keeps its old box
+    ({1, 3}, [[0, 0, 0, 6, 2, 5], [0, 0, 0, 6, 2, 0], _B0, [4, 0, 0, 6, 2, 0], _B2]),
+    # {4}: the root visits leaf 4 only; node 1's subtree keeps its boxes
+    ({4}, [[0, 0, 0, 3, 2, 10], [0, 0, 0, 3, 1, 0], _B0, _B1, [0, 0, 10, 2, 2, 10]]),
+    # {2}, {0}, {}: neither of the root's children is listed, so everything is refit (forced)
+    ({2}, [[0, 0, 0, 6, 2, 10], [0, 0, 0, 6, 2, 0], [0, 0, 0, 2, 2, 0], [4, 0, 0, 6, 2, 0], [0, 0, 10, 2, 2, 10]]),
+    ({0}, [[0, 0, 0, 6, 2, 10], [0, 0, 0, 6, 2, 0], [0, 0, 0, 2, 2, 0], [4, 0, 0, 6, 2, 0], [0, 0, 10, 2, 2, 10]]),
+    (set(), [[0, 0, 0, 6, 2, 10], [0, 0, 0, 6, 2, 0], [0, 0, 0, 2, 2, 0], [4, 0, 0, 6, 2, 0], [0, 0, 10, 2, 2, 10]]),
+    # ids that name no node never match
+    ({-1, 99, 1}, [[0, 0, 0, 6, 2, 5], [0, 0, 0, 6, 2, 0], [0, 0, 0, 2, 2, 0], [4, 0, 0, 6, 2, 0], _B2]),
+])
+def test_partial_refit_by_hand(ids, expect):
+    idx = np.arange(9, dtype=np.uint16)
+    out = O.OracleMesh(_TRIS * np.float32(2), idx, [_node_bytes(_FIVE)]).refit(ids)[0]
+    assert _boxes(out) == [list(map(float, e)) for e in expect]
+    full = O.OracleMesh(_TRIS * np.float32(2), idx, [_node_bytes(_FIVE)]).refit()[0]
+    if not ids & {1, 3, 4}:
+        assert np.array_equal(out, full)  # a set without the root's children is a full refit
+
+
+def test_partial_refit_numbers_nodes_over_all_roots():
+    """Root r's node j is id (nodes of the roots before r) + j, as shapecast reports them."""
+    ra = _node_bytes([([0, 0, 0, 3, 1, 0], "inner", 2, 0), (_B0, "leaf", 0, 1), (_B1, "leaf", 1, 1)])
+    rb = _node_bytes([(_B2, "leaf", 2, 1)])
+    idx = np.arange(9, dtype=np.uint16)
+    pos = _TRIS * np.float32(2)
+    # {4}: no node of root A (ids 0-2) has a listed child, so root A is refit whole; root B (id 3) is a
+    # leaf root, refit whatever the set holds
+    a, b = O.OracleMesh(pos, idx, [ra, rb]).refit({4})
+    assert _boxes(a) == [[0, 0, 0, 6, 2, 0], [0, 0, 0, 2, 2, 0], [4, 0, 0, 6, 2, 0]]
+    assert _boxes(b) == [[0, 0, 10, 2, 2, 10]]
+    # {2}: root A visits its right leaf only
+    a, _ = O.OracleMesh(pos, idx, [ra, rb]).refit({2})
+    assert _boxes(a) == [[0, 0, 0, 6, 2, 0], _B0, [4, 0, 0, 6, 2, 0]]
+
+
+def test_partial_refit_with_every_node_is_a_full_refit():
+    """The reference's own case (test/MeshBVH.test.js:340-367): every node id from a shapecast, then a
+    scale by 2; the result is the full refit, tight everywhere."""
+    g = G.torus_knot(10, 3, 120, 16)
+    idx, tree = _tree(g)
+    pos = (g.positions.reshape(-1, 3) * np.float32(2)).astype(np.float32)
+    n = tree.roots[0].nbytes // 32
+    part = O.OracleMesh(pos, idx, tree.roots).refit(range(n))[0]
+    assert np.array_equal(part, O.OracleMesh(pos, idx, tree.roots).refit()[0])
+    _check_tight(part, pos, idx)

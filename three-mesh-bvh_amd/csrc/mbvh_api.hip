@@ -865,7 +865,7 @@ int check_roots_out(const char* fn, const mbvh_mesh* mesh, void* const* roots_ou
 }
 
 int enqueue_refit(mbvh_mesh* mesh, const float* d_positions, uint64_t vertex_count, uint32_t stride, uint32_t offset,
-                  hipStream_t s) {
+                  hipStream_t s, const uint32_t* d_node_ids = nullptr, uint64_t n_ids = 0) {
   RefitArgs a{};
   a.ref_nodes = mesh->d_ref_nodes;
   a.wide_off = mesh->d_ref_wide;
@@ -888,6 +888,10 @@ int enqueue_refit(mbvh_mesh* mesh, const float* d_positions, uint64_t vertex_cou
   a.stride = stride;
   a.offset = offset;
   a.coord_bound = mesh->d_coord_max;
+  a.node_ids = d_node_ids;
+  a.n_ids = n_ids;
+  a.root_first = mesh->d_root_first;
+  a.n_roots = mesh->n_roots;
   MBVH_HIP(launch_refit(a, s));
   return MBVH_OK;
 }
@@ -934,6 +938,52 @@ extern "C" int mbvh_mesh_refit_d(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* po
   if ((rc = set_device(ctx))) return rc;
   return enqueue_refit(mesh, positions, vertex_count, position_stride, position_offset,
                        stream ? static_cast<hipStream_t>(stream) : ctx->stream);
+}
+
+// Partial refit: node ids as shapecast reports them (a node's index over all roots). A NULL id list
+// is a full refit (nodeIndices = null); an empty one visits what the reference visits for an empty
+// set (which is everything, as neither of a root's children is listed).
+extern "C" int mbvh_mesh_refit_nodes(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint64_t vertex_count,
+                                     uint32_t position_stride, uint32_t position_offset, const uint32_t* node_ids,
+                                     uint64_t n_ids, void* const* roots_out, const uint64_t* root_bytes,
+                                     uint32_t n_roots) {
+  clear_error();
+  const char* fn = "mbvh_mesh_refit_nodes";
+  int rc = check_refit(fn, ctx, mesh, positions, vertex_count, position_stride, position_offset);
+  if (!rc) rc = check_roots_out(fn, mesh, roots_out, root_bytes, n_roots);
+  if (rc) return rc;
+  if (!node_ids && n_ids) return set_error(MBVH_ERR_BAD_ARGS, "%s: node_ids is NULL", fn);
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t s = ctx->stream;
+  const size_t bytes = vertex_count ? ((vertex_count - 1) * position_stride + position_offset + 3) * sizeof(float) : 0;
+  if ((rc = ensure_buffer(ctx->staging[0], bytes))) return rc;
+  if (bytes) MBVH_HIP(hipMemcpyAsync(ctx->staging[0].ptr, positions, bytes, hipMemcpyHostToDevice, s));
+  const uint32_t* d_ids = nullptr;
+  if (node_ids) {
+    if ((rc = ensure_buffer(ctx->staging[1], (n_ids ? n_ids : 1) * sizeof(uint32_t)))) return rc;
+    if (n_ids) MBVH_HIP(hipMemcpyAsync(ctx->staging[1].ptr, node_ids, n_ids * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    d_ids = static_cast<const uint32_t*>(ctx->staging[1].ptr);
+  }
+  if ((rc = enqueue_refit(mesh, static_cast<const float*>(ctx->staging[0].ptr), vertex_count, position_stride,
+                          position_offset, s, d_ids, n_ids)) ||
+      (rc = enqueue_copy_roots(mesh, roots_out, root_bytes, s)))
+    return rc;
+  MBVH_HIP(hipStreamSynchronize(s));
+  return MBVH_OK;
+}
+
+extern "C" int mbvh_mesh_refit_nodes_d(mbvh_ctx* ctx, mbvh_mesh* mesh, const float* positions, uint64_t vertex_count,
+                                       uint32_t position_stride, uint32_t position_offset, const uint32_t* node_ids,
+                                       uint64_t n_ids, void* stream) {
+  clear_error();
+  const char* fn = "mbvh_mesh_refit_nodes_d";
+  int rc = check_refit(fn, ctx, mesh, positions, vertex_count, position_stride, position_offset);
+  if (rc) return rc;
+  if (!node_ids && n_ids) return set_error(MBVH_ERR_BAD_ARGS, "%s: node_ids is NULL", fn);
+  if ((rc = set_device(ctx))) return rc;
+  return enqueue_refit(mesh, positions, vertex_count, position_stride, position_offset,
+                       stream ? static_cast<hipStream_t>(stream) : ctx->stream, node_ids, n_ids);
 }
 
 // ---- imports: the node words of the WebGPU / WebGL packings back to reference node buffers ----

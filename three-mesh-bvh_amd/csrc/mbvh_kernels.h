@@ -131,6 +131,12 @@ struct RefitArgs {
   uint32_t stride;
   uint32_t offset;
   float* coord_bound;         // the mesh's coordinate bound, recomputed from the positions
+  // partial refit (nodeIndices): node_ids != nullptr restricts the refit to the nodes the reference's
+  // traversal visits for that set; ids count nodes over all roots, ids >= n_nodes match nothing
+  const uint32_t* node_ids;
+  uint64_t n_ids;
+  const uint64_t* root_first;  // first node of each root
+  uint32_t n_roots;
 };
 hipError_t launch_refit(const RefitArgs& a, hipStream_t s);
 

@@ -16,12 +16,22 @@
 // node; k_refit_four then copies the wide boxes into the 4-wide tree, and k_refit_tris re-gathers
 // the triangle corners (their .w words - face id, facing mask, vertex id - are kept). The
 // coordinate bound the query kernels scale their error bounds by is recomputed from the positions.
+//
+// A partial refit (nodeIndices, refit.template.js:136-172) first marks the nodes the reference's
+// traversal visits, top down, one launch per depth: every root is visited; a visited internal node
+// that is not forced visits the children in the set, or, when neither is, both, forced; a forced
+// node forces its children. The per-node word holds the marks until the bottom-up launches replace
+// them by the changed flag; a node not visited keeps its box and counts as unchanged. The triangle
+// corners and the coordinate bound are refreshed everywhere, as the queries read the new positions.
 #include "mbvh_kernels.h"
 
 namespace mbvh {
 namespace {
 
 constexpr int kRefitBlock = 256;
+
+// Per-node marks of a partial refit (the changed flag, written bottom up, is bit 0).
+constexpr uint32_t kRefitChanged = 1u, kRefitVisit = 2u, kRefitMember = 4u, kRefitForce = 8u;
 
 __global__ __launch_bounds__(64) void k_refit_reset_bound(float* coord_bound) {
   if (threadIdx.x == 0) *coord_bound = 0.0f;
@@ -85,6 +95,10 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_leaves(RefitArgs a) {
   const float inf = __builtin_huge_valf();
   for (uint64_t li = static_cast<uint64_t>(blockIdx.x) * kRefitBlock + threadIdx.x; li < a.n_leaves; li += stride) {
     const uint64_t node = a.leaves[li];
+    if (a.node_ids && !(a.changed[node] & kRefitVisit)) {
+      a.changed[node] = 0u;
+      continue;
+    }
     const uint32_t* w = a.ref_nodes + 8 * node;
     const uint32_t off = w[6];
     const uint32_t cnt = w[7] & 0xFFFFu;  // u16[14]: count (u16[15] is the leaf flag)
@@ -100,7 +114,7 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_leaves(RefitArgs a) {
     bool changed = false;
     for (int k = 0; k < 6; k++) changed = changed || !(old[k] == b[k]);
     if (changed) put_box(a, node, b);
-    a.changed[node] = changed ? 1u : 0u;
+    a.changed[node] = changed ? kRefitChanged : 0u;
   }
 }
 
@@ -109,9 +123,13 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_level(RefitArgs a, const 
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kRefitBlock;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kRefitBlock + threadIdx.x; i < count; i += stride) {
     const uint64_t node = nodes[i];
+    if (a.node_ids && !(a.changed[node] & kRefitVisit)) {
+      a.changed[node] = 0u;
+      continue;
+    }
     const uint64_t left = node + 1;
     const uint64_t right = node + a.ref_nodes[8 * node + 6];
-    const bool any = a.changed[left] != 0u || a.changed[right] != 0u;
+    const bool any = ((a.changed[left] | a.changed[right]) & kRefitChanged) != 0u;
     if (any) {
       const float* lb = reinterpret_cast<const float*>(a.ref_nodes + 8 * left);
       const float* rb = reinterpret_cast<const float*>(a.ref_nodes + 8 * right);
@@ -122,7 +140,7 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_level(RefitArgs a, const 
       }
       put_box(a, node, u);
     }
-    a.changed[node] = any ? 1u : 0u;
+    a.changed[node] = any ? kRefitChanged : 0u;
   }
 }
 
@@ -132,6 +150,46 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_four(RefitArgs a) {
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kRefitBlock + threadIdx.x; i < a.n_four; i += stride) {
     const uint2 m = a.four_map[i];  // (float offset in the 4-wide tree, float offset in the wide nodes)
     for (int k = 0; k < 6; k++) a.four[m.x + k] = a.wide[m.y + k];
+  }
+}
+
+__global__ __launch_bounds__(kRefitBlock) void k_refit_mark_clear(uint32_t* marks, uint64_t n) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kRefitBlock;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kRefitBlock + threadIdx.x; i < n; i += stride) marks[i] = 0u;
+}
+
+// Members of the set (plain stores of one value, so repeated ids are harmless).
+__global__ __launch_bounds__(kRefitBlock) void k_refit_mark_ids(RefitArgs a) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kRefitBlock;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kRefitBlock + threadIdx.x; i < a.n_ids; i += stride) {
+    const uint32_t id = a.node_ids[i];
+    if (id < a.n_nodes) a.changed[id] = kRefitMember;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_refit_mark_roots(RefitArgs a) {
+  for (uint32_t r = threadIdx.x; r < a.n_roots; r += 64) a.changed[a.root_first[r]] |= kRefitVisit;
+}
+
+// The children of the visited internal nodes of one depth (each node writes only its own two
+// children, whose member bits it reads first).
+__global__ __launch_bounds__(kRefitBlock) void k_refit_mark_level(RefitArgs a, const uint32_t* nodes, uint64_t count) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kRefitBlock;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kRefitBlock + threadIdx.x; i < count; i += stride) {
+    const uint64_t node = nodes[i];
+    const uint32_t st = a.changed[node];
+    if (!(st & kRefitVisit)) continue;
+    const uint64_t left = node + 1;
+    const uint64_t right = node + a.ref_nodes[8 * node + 6];
+    const uint32_t ml = a.changed[left], mr = a.changed[right];
+    const bool force = (st & kRefitForce) || !((ml | mr) & kRefitMember);
+    if (force) {
+      a.changed[left] = ml | kRefitVisit | kRefitForce;
+      a.changed[right] = mr | kRefitVisit | kRefitForce;
+    } else {
+      if (ml & kRefitMember) a.changed[left] = ml | kRefitVisit;
+      if (mr & kRefitMember) a.changed[right] = mr | kRefitVisit;
+    }
   }
 }
 
@@ -227,6 +285,17 @@ hipError_t launch_refit(const RefitArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_refit_bound, dim3(blocks_for(a.n_vertices) < kBoundBlocks ? blocks_for(a.n_vertices) : kBoundBlocks),
                      dim3(kRefitBlock), 0, s, a);
   if (a.n_slots) hipLaunchKernelGGL(k_refit_tris, dim3(blocks_for(3 * a.n_slots)), dim3(kRefitBlock), 0, s, a);
+  if (a.node_ids) {
+    hipLaunchKernelGGL(k_refit_mark_clear, dim3(blocks_for(a.n_nodes)), dim3(kRefitBlock), 0, s, a.changed, a.n_nodes);
+    if (a.n_ids) hipLaunchKernelGGL(k_refit_mark_ids, dim3(blocks_for(a.n_ids)), dim3(kRefitBlock), 0, s, a);
+    hipLaunchKernelGGL(k_refit_mark_roots, dim3(1), dim3(64), 0, s, a);
+    for (uint32_t d = 0; d < a.n_levels; d++) {
+      const uint64_t count = a.level_off[d + 1] - a.level_off[d];
+      if (count)
+        hipLaunchKernelGGL(k_refit_mark_level, dim3(blocks_for(count)), dim3(kRefitBlock), 0, s, a,
+                           a.levels + a.level_off[d], count);
+    }
+  }
   if (a.n_leaves) hipLaunchKernelGGL(k_refit_leaves, dim3(blocks_for(a.n_leaves)), dim3(kRefitBlock), 0, s, a);
   for (uint32_t d = a.n_levels; d-- > 0;) {
     const uint64_t count = a.level_off[d + 1] - a.level_off[d];

@@ -1046,11 +1046,17 @@ static napi_value js_intersects_sphere(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
-/* refitMesh(mesh, positions, stride, offset, roots[]): device refit from new positions; the refit
- * reference-format nodes are written into the roots' ArrayBuffers */
+/* refitMesh(mesh, positions, stride, offset, roots[], nodeIds = null): device refit from new positions;
+ * the refit reference-format nodes are written into the roots' ArrayBuffers. nodeIds (a Uint32Array
+ * of node indices over all roots) makes it a partial refit. */
 static napi_value js_refit(napi_env env, napi_callback_info info) {
-  napi_value argv[5];
-  if (!get_args(env, info, 5, argv)) return throw_msg(env, "refitMesh: bad arguments");
+  napi_value argv[6];
+  if (!get_args(env, info, 6, argv)) return throw_msg(env, "refitMesh: bad arguments");
+  View ids;
+  memset(&ids, 0, sizeof(ids));
+  const int partial = !is_nullish(env, argv[5]);
+  if (partial && (!get_view(env, argv[5], &ids) || ids.type != napi_uint32_array))
+    return throw_msg(env, "refitMesh: nodeIds must be a Uint32Array or null");
   MeshHandle* h = get_mesh(env, argv[0]);
   if (!h) return throw_msg(env, "refitMesh: invalid mesh");
   View pos;
@@ -1087,12 +1093,17 @@ static napi_value js_refit(napi_env env, napi_callback_info info) {
   }
   mbvh_ctx* ctx = NULL;
   int rc = context_for(env, h->device, &ctx);
-  if (!rc)
+  static const uint32_t no_ids = 0;
+  if (!rc && !partial)
     rc = mbvh_mesh_refit(ctx, h->mesh, (const float*)pos.data, pos.length / (size_t)stride, (uint32_t)stride,
                          (uint32_t)offset, root_ptrs, root_bytes, n_roots);
+  else if (!rc)
+    rc = mbvh_mesh_refit_nodes(ctx, h->mesh, (const float*)pos.data, pos.length / (size_t)stride, (uint32_t)stride,
+                               (uint32_t)offset, ids.length ? (const uint32_t*)ids.data : &no_ids, ids.length,
+                               root_ptrs, root_bytes, n_roots);
   free(root_ptrs);
   free(root_bytes);
-  if (rc) return throw_status(env, rc, "mbvh_mesh_refit");
+  if (rc) return throw_status(env, rc, partial ? "mbvh_mesh_refit_nodes" : "mbvh_mesh_refit");
   return NULL;
 }
 

@@ -32,7 +32,7 @@ EXPORTED_SYMBOLS = (
     "mbvh_ctx_create", "mbvh_ctx_destroy", "mbvh_ctx_synchronize", "mbvh_ctx_set_ray_lanes",
     "mbvh_ctx_set_point_lanes",
     "mbvh_mesh_upload", "mbvh_mesh_free", "mbvh_mesh_info", "mbvh_mesh_set_side_mask", "mbvh_mesh_refit",
-    "mbvh_mesh_refit_d", "mbvh_mesh_copy_roots", "mbvh_intersects_sphere",
+    "mbvh_mesh_refit_d", "mbvh_mesh_refit_nodes", "mbvh_mesh_refit_nodes_d", "mbvh_mesh_copy_roots", "mbvh_intersects_sphere",
     "mbvh_raycast_first", "mbvh_raycast_first_d",
     "mbvh_closest_point", "mbvh_closest_point_d", "mbvh_closest_point_noalloc_d",
     "mbvh_raycast_first_f64", "mbvh_raycast_first_ff_d", "mbvh_closest_point_f64", "mbvh_closest_point_ff_d",
@@ -111,6 +111,8 @@ def library() -> C.CDLL:
         "mbvh_mesh_set_side_mask": (C.c_int, [vp, vp, vp, u64]),
         "mbvh_mesh_refit": (C.c_int, [vp, vp, vp, u64, u32, u32, vp, vp, u32]),
         "mbvh_mesh_refit_d": (C.c_int, [vp, vp, vp, u64, u32, u32, vp]),
+        "mbvh_mesh_refit_nodes": (C.c_int, [vp, vp, vp, u64, u32, u32, vp, u64, vp, vp, u32]),
+        "mbvh_mesh_refit_nodes_d": (C.c_int, [vp, vp, vp, u64, u32, u32, vp, u64, vp]),
         "mbvh_intersects_sphere": (C.c_int, [vp, vp, vp, vp, u64, vp]),
         "mbvh_mesh_copy_roots": (C.c_int, [vp, vp, vp, vp, u32]),
         "mbvh_raycast_first": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp]),
@@ -422,16 +424,24 @@ class Mesh:
         self.handle = h
         self.root_bytes = [r.nbytes for r in tree.roots]
 
-    def refit(self, positions: np.ndarray, stride: int = 3, offset: int = 0, roots: bool = True):
-        """MeshBVH.refit on the device from new positions of the same vertices; returns the refit
-        reference-format root buffers (uint8 arrays) when `roots` is true."""
+    def refit(self, positions: np.ndarray, stride: int = 3, offset: int = 0, roots: bool = True, node_ids=None):
+        """MeshBVH.refit(nodeIndices) on the device from new positions of the same vertices; returns the
+        refit reference-format root buffers (uint8 arrays) when `roots` is true. node_ids: None (a full
+        refit) or node indices counted over all roots (ids outside [0, 2^32) name no node)."""
         pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
         n = len(self.root_bytes)
         outs = [np.empty(b, np.uint8) for b in self.root_bytes] if roots else None
         ptrs = (C.c_void_p * n)(*[o.ctypes.data for o in outs]) if roots else None
         sizes = (C.c_uint64 * n)(*self.root_bytes)
-        _check(self.ctx._lib.mbvh_mesh_refit(self.ctx.handle, self.handle, _ptr(pos), pos.size // stride, stride,
-                                             offset, ptrs, sizes, n))
+        if node_ids is None:
+            _check(self.ctx._lib.mbvh_mesh_refit(self.ctx.handle, self.handle, _ptr(pos), pos.size // stride, stride,
+                                                 offset, ptrs, sizes, n))
+        else:
+            ids = np.asarray(list(node_ids), dtype=np.int64).reshape(-1)
+            ids = np.ascontiguousarray(ids[(ids >= 0) & (ids < 2**32)].astype(np.uint32))
+            keep = np.zeros(1, np.uint32) if ids.size == 0 else ids  # a valid pointer for an empty set
+            _check(self.ctx._lib.mbvh_mesh_refit_nodes(self.ctx.handle, self.handle, _ptr(pos), pos.size // stride,
+                                                       stride, offset, _ptr(keep), ids.size, ptrs, sizes, n))
         return outs
 
     def refit_d(self, positions_ptr: int, vertex_count: int, stride: int = 3, offset: int = 0,
@@ -439,6 +449,13 @@ class Mesh:
         """Refit from device positions, enqueued on `stream` (no synchronisation)."""
         _check(self.ctx._lib.mbvh_mesh_refit_d(self.ctx.handle, self.handle, C.c_void_p(positions_ptr), vertex_count,
                                                stride, offset, C.c_void_p(stream or None)))
+
+    def refit_nodes_d(self, positions_ptr: int, vertex_count: int, node_ids_ptr: int, n_ids: int, stride: int = 3,
+                      offset: int = 0, stream: int = 0) -> None:
+        """Partial refit from device positions and device node ids (uint32), enqueued on `stream`."""
+        _check(self.ctx._lib.mbvh_mesh_refit_nodes_d(self.ctx.handle, self.handle, C.c_void_p(positions_ptr),
+                                                     vertex_count, stride, offset, C.c_void_p(node_ids_ptr), n_ids,
+                                                     C.c_void_p(stream or None)))
 
     def intersects_box(self, box_min: np.ndarray, box_max: np.ndarray, box_to_mesh: np.ndarray) -> np.ndarray:
         """Per box: whether any triangle intersects it (min / max in the box frame, 4x4 column-major

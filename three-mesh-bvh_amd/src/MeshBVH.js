@@ -204,13 +204,19 @@ class MeshBVH {
   // is the min / max of its triangles' vertices and is rewritten only if it differs, an internal
   // node is the union of its children, recomputed only if a child changed). The refit nodes are
   // written back into this._roots, so serialize() and later uploads see them, and the device copy
-  // is updated in place. Partial refits (nodeIndices from shapecast) are not supported.
+  // is updated in place. nodeIndices (a Set or an Array of node indices over all roots, as shapecast
+  // reports them) makes it a partial refit (refit.template.js:136-172): below each root only the
+  // listed children are visited, or both children, whole, when neither is listed; values that are
+  // not node indices match nothing, as in the reference's Set.has.
   refit(nodeIndices = null) {
-    if (nodeIndices !== null && nodeIndices !== undefined) {
-      throw new Error('MeshBVH.refit: partial refit (nodeIndices) is not supported; call refit()');
+    let ids = null;
+    if (nodeIndices) {
+      const list = [];
+      for (const v of nodeIndices) if (Number.isInteger(v) && v >= 0 && v < 4294967296) list.push(v);
+      ids = Uint32Array.from(list);
     }
     const pos = GU.positionLayout(this.geometry);
-    native.load().refitMesh(this._mesh(), pos.array, pos.stride, pos.offset, this._roots);
+    native.load().refitMesh(this._mesh(), pos.array, pos.stride, pos.offset, this._roots, ids);
   }
 
   // Releases the device copy now (otherwise released when the BVH is garbage collected).
