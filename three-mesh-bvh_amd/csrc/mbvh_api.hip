@@ -3199,6 +3199,15 @@ int enqueue_scene_points(mbvh_ctx* ctx, const mbvh_scene* sc, const float* point
   a.out_dist = dist;
   a.out_point = point;
   a.out_bary = bary;
+  // spatial order: lanes of a wave then share instances and tree paths
+  if (n >= kOrderMin && n < (uint64_t{1} << 32)) {
+    int rc;
+    if ((rc = ensure_buffer(ctx->order, n * sizeof(uint32_t))) ||
+        (rc = ensure_buffer(ctx->order_ws, order_workspace_bytes(n))))
+      return rc;
+    MBVH_HIP(launch_order_points(points, n, static_cast<uint32_t*>(ctx->order.ptr), ctx->order_ws.ptr, s));
+    a.order = static_cast<const uint32_t*>(ctx->order.ptr);
+  }
   const uint64_t threads = launch_threads(n, ctx->max_threads);
   a.stack = stack_of(ctx, threads);
   MBVH_HIP(launch_scene_closest_point(a, threads, s));

@@ -287,6 +287,7 @@ struct SceneArgs {
   float* out_point;                // world point
   float* out_bary;
   StackScratch stack;
+  const uint32_t* order;           // closest point: visit order of the points (NULL: 0..n-1)
 };
 hipError_t launch_scene_raycast_first(const SceneArgs& a, uint64_t threads, hipStream_t s);
 // Closest point over the instances: `origins` holds the world points (directions, side, near / far unused).

@@ -3071,7 +3071,8 @@ __global__ __launch_bounds__(kBlock) void k_scene_closest_point(SceneArgs A) {
   st.lane = tid;
   st.capacity = 8 + static_cast<int>(A.stack.ovf_entries);
   const float qnan = __int_as_float(0x7fc00000);
-  for (uint64_t i = gtid; i < A.n; i += static_cast<uint64_t>(gridDim.x) * kBlock) {
+  for (uint64_t k = gtid; k < A.n; k += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    const uint64_t i = A.order ? A.order[k] : k;  // neighbouring lanes take neighbouring points
     const float px = A.origins[3 * i], py = A.origins[3 * i + 1], pz = A.origins[3 * i + 2];
     const D3 p = d3(px, py, pz);
     ScenePointBest b;
