@@ -314,3 +314,33 @@ def test_scene_closest_point_sheared_and_anisotropic(ctx, meshes):
         assert np.linalg.norm(cp - pts[i]) <= ref["distance"][i] * (1 + 1e-6) + 1e-9
     assert np.isin(got["instance"], np.arange(4)).sum() > 20  # the edge-case instances do win points
     sc.free()
+
+
+@pytest.mark.parametrize("kind", ["rigid", "non_uniform", "sheared"])
+def test_scene_closest_point_lanes_agree(ctx, meshes, kind):
+    """One lane and four lanes per point (mbvh_ctx_set_point_lanes 1 / 4) give the same faces,
+    instances, distances, points and barycoords bit for bit (6000 points: the spatial order runs)."""
+    built, ms = meshes
+    geos = [b[0] for b in built]
+    w = W.scene_rays(geos, 20, 8, seed=51, non_uniform=0.5 if kind == "non_uniform" else 0.0)
+    if kind == "sheared":
+        for k in range(0, 20, 3):
+            m = w.matrix_world[k].astype(np.float64).reshape(4, 4).T
+            m[:3, 2] += 0.6 * m[:3, 0]
+            m[:3, 1] *= 7.0
+            w.matrix_world[k] = m.T.reshape(-1).astype(np.float32)
+            w.matrix_world_inverse[k] = np.linalg.inv(m).T.reshape(-1).astype(np.float32)
+    pts = _scene_points(w, 6000, 13)
+    print(f"case scene closest point lanes {kind}", flush=True)
+    sc = ctx.scene(ms, w.matrix_world, w.matrix_world_inverse, w.instance_geometry)
+    try:
+        ctx.set_point_lanes(1)
+        one = sc.closest_point(pts)
+        ctx.set_point_lanes(4)
+        four = sc.closest_point(pts)
+    finally:
+        ctx.set_point_lanes(0)
+        sc.free()
+    assert np.all(one["face"] != M.NO_HIT)
+    for k in ("face", "instance", "distance", "point", "barycoord"):
+        assert np.array_equal(np.asarray(one[k]).view(np.uint32), np.asarray(four[k]).view(np.uint32)), k

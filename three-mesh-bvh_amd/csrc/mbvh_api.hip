@@ -2852,6 +2852,9 @@ extern "C" int mbvh_build_gpu_d(mbvh_ctx* ctx, const float* positions, uint64_t 
 
 // ---- scenes ------------------------------------------------------------------------------------------
 
+// Depth bound of a scene's top-level tree (the scene raycast's per-lane stack holds that many levels).
+constexpr uint32_t kTlasMaxDepth = 24;
+
 struct mbvh_scene {
   uint32_t magic = 0x4E454353u;  // "SCEN"
   mbvh_ctx* ctx = nullptr;
@@ -2865,7 +2868,9 @@ struct mbvh_scene {
   uint32_t* d_geom = nullptr;
   uint32_t* d_tlas = nullptr;
   uint32_t* d_tlas_inst = nullptr;
-  uint32_t max_stack = 0;
+  uint32_t max_stack = 0;         // stack entries of the deepest instance walk (raycast)
+  uint32_t max_stack_points = 0;  // closest point: the top level's far children below them
+  uint32_t max_stack_points4 = 0;  // the same for the four-lane kernel (4-wide instance trees)
 };
 
 namespace {
@@ -2952,12 +2957,13 @@ extern "C" int mbvh_scene_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, 
     return set_error(MBVH_ERR_BAD_ARGS, "%s: NULL matrices or instance geometry", fn);
   if (n_instances >= (1u << 24)) return set_error(MBVH_ERR_LIMIT, "%s: at most 2^24 - 1 instances", fn);
   std::vector<MeshView> views(n_meshes);
-  uint32_t max_stack = 0;
+  uint32_t max_stack = 0, max_stack4 = 0;
   for (uint32_t i = 0; i < n_meshes; i++) {
     if (!valid_mesh(meshes[i]) || meshes[i]->ctx != ctx)
       return set_error(MBVH_ERR_BAD_HANDLE, "%s: mesh %u is not a mesh of this context", fn, i);
     views[i] = view_of(meshes[i]);
     max_stack = std::max(max_stack, meshes[i]->max_stack);
+    max_stack4 = std::max(max_stack4, meshes[i]->max_stack4);
   }
   for (uint32_t i = 0; i < n_instances; i++)
     if (instance_geometry[i] >= n_meshes)
@@ -3027,7 +3033,7 @@ extern "C" int mbvh_scene_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, 
     std::memcpy(nodes.data() + 28, &leaf_word, 4);
   } else {
     const uint32_t range[2] = {0u, n_listed};
-    const mbvh_build_opts opts{MBVH_SAH, 24, 2, 0};  // depth bound = the kernel's TLAS stack
+    const mbvh_build_opts opts{MBVH_SAH, kTlasMaxDepth, 2, 0};  // depth bound = the kernels' TLAS stacks
     mbvh_tree* tlas = nullptr;
     rc = mbvh_build(prim_pos.data(), 3ull * n_listed, 3, 0, index.data(), 4, index.size(), range, 1, &opts, &tlas);
     if (rc) return rc;
@@ -3037,13 +3043,18 @@ extern "C" int mbvh_scene_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, 
   std::vector<uint32_t> slot_inst(std::max<uint32_t>(n_listed, 1), 0u);
   for (uint32_t s = 0; s < n_listed; s++) slot_inst[s] = listed[index[3ull * s] / 3];
 
-  if ((rc = reserve_stack(ctx, max_stack))) return rc;
+  // closest point keeps the top level's far children (at most one per level) below the instance walks
+  const uint32_t max_stack_points = max_stack + kTlasMaxDepth;
+  const uint32_t max_stack_points4 = max_stack4 + kTlasMaxDepth;
+  if ((rc = reserve_stack(ctx, std::max(max_stack_points, max_stack_points4)))) return rc;
   mbvh_scene* sc = new (std::nothrow) mbvh_scene();
   if (!sc) return set_error(MBVH_ERR_OOM, "%s: out of host memory", fn);
   sc->ctx = ctx;
   sc->n_meshes = n_meshes;
   sc->n_instances = n_instances;
   sc->max_stack = max_stack;
+  sc->max_stack_points = max_stack_points;
+  sc->max_stack_points4 = max_stack_points4;
   const size_t mat_bytes = std::max<size_t>(16ull * n_instances * sizeof(float), 64);
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&sc->d_meshes), views.size() * sizeof(MeshView));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_world), mat_bytes);
@@ -3180,7 +3191,9 @@ namespace {
 
 int enqueue_scene_points(mbvh_ctx* ctx, const mbvh_scene* sc, const float* points, uint64_t n, uint32_t* face,
                          uint32_t* inst, float* dist, float* point, float* bary, hipStream_t s) {
-  if (sc->max_stack > 8 + ctx->stack_ovf_entries)
+  const bool quads = ctx->point_lanes != 1;  // four lanes per point unless one was asked for
+  if (quads ? sc->max_stack_points4 > static_cast<uint32_t>(kQuadLdsStack) + ctx->stack_ovf_entries
+            : sc->max_stack_points > 8 + ctx->stack_ovf_entries)
     return set_error(MBVH_ERR_LIMIT, "scene: stack scratch smaller than the tree depth");
   SceneArgs a{};
   a.meshes = sc->d_meshes;
@@ -3208,9 +3221,16 @@ int enqueue_scene_points(mbvh_ctx* ctx, const mbvh_scene* sc, const float* point
     MBVH_HIP(launch_order_points(points, n, static_cast<uint32_t*>(ctx->order.ptr), ctx->order_ws.ptr, s));
     a.order = static_cast<const uint32_t*>(ctx->order.ptr);
   }
-  const uint64_t threads = launch_threads(n, ctx->max_threads);
-  a.stack = stack_of(ctx, threads);
-  MBVH_HIP(launch_scene_closest_point(a, threads, s));
+  if (quads) {
+    const uint64_t threads = launch_threads(4 * n, ctx->max_threads);
+    a.stack = stack_of(ctx, threads / 4);
+    a.queue = static_cast<uint32_t*>(ctx->queue.ptr);
+    MBVH_HIP(launch_scene_closest_point4(a, threads, s));
+  } else {
+    const uint64_t threads = launch_threads(n, ctx->max_threads);
+    a.stack = stack_of(ctx, threads);
+    MBVH_HIP(launch_scene_closest_point(a, threads, s));
+  }
   return MBVH_OK;
 }
 

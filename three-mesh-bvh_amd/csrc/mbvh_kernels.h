@@ -288,10 +288,13 @@ struct SceneArgs {
   float* out_bary;
   StackScratch stack;
   const uint32_t* order;           // closest point: visit order of the points (NULL: 0..n-1)
+  uint32_t* queue;                 // closest point, four lanes: work-queue counter
 };
 hipError_t launch_scene_raycast_first(const SceneArgs& a, uint64_t threads, hipStream_t s);
 // Closest point over the instances: `origins` holds the world points (directions, side, near / far unused).
 hipError_t launch_scene_closest_point(const SceneArgs& a, uint64_t threads, hipStream_t s);
+// The same with four lanes per point (threads = 4 per point, a multiple of the block; a work queue).
+hipError_t launch_scene_closest_point4(const SceneArgs& a, uint64_t threads, hipStream_t s);
 
 // ---- intersectsBox ----------------------------------------------------------------------------------
 struct BoxArgs {

@@ -2995,8 +2995,13 @@ __device__ bool scene_point_instance(const MeshView& M, const float* W, const fl
   auto box_lb = [&](float a, float bb, float c, float d, float e, float g) {
     return local ? local_box_d2_lb(lf, a, bb, c, d, e, g) : world_box_d2_lb(W, px, py, pz, a, bb, c, d, e, g);
   };
+  // the FP32 local point and the absolute error of an FP32 point-to-triangle distance from it: the
+  // distance's roundings (8u of the magnitudes) and the local coordinates' own error bounds
+  const f3 lq = mk(lf.l[0], lf.l[1], lf.l[2]);
+  const float tri_err = 8.0f * kU * (maxabs(lq) + *M.coord_max) + (lf.d[0] + lf.d[1] + lf.d[2]);
+  const int base = st.sp;  // the caller's entries (the top level's) stay below
   for (uint32_t r = 0; r < M.n_roots; r++) {
-    st.sp = 0;
+    st.sp = base;
     uint32_t ref = M.roots[r];
     while (true) {
       if (ref & kRefLeaf) {
@@ -3010,6 +3015,15 @@ __device__ bool scene_point_instance(const MeshView& M, const float* W, const fl
                      fminf(fminf(t.p0.z, t.p1.z), t.p2.z), fmaxf(fmaxf(t.p0.x, t.p1.x), t.p2.x),
                      fmaxf(fmaxf(t.p0.y, t.p1.y), t.p2.y), fmaxf(fmaxf(t.p0.z, t.p1.z), t.p2.z)) > b.d2)
             continue;
+          if (local) {
+            // then the FP32 distance in the instance frame: sigma * (local distance less its error)
+            // bounds the world distance from below (a NaN bound never skips)
+            const f3 c = closest_on_triangle(lq, t.a(), t.b(), t.c());
+            const float ex = lq.x - c.x, ey = lq.y - c.y, ez = lq.z - c.z;
+            const float dl = __builtin_amdgcn_sqrtf(ex * ex + ey * ey + ez * ez);
+            const float lb = dl - (tri_err + dl * (4.0f * kU));
+            if (lb > 0.0f && static_cast<double>(lb * lb * lf.s2) * (1.0 - 1.0 / 16384.0) > b.d2) continue;
+          }
           const D3 q = closest_on_tri_d(p, world_vertex(Wd, t.p0), world_vertex(Wd, t.p1), world_vertex(Wd, t.p2));
           const double d2 = ddist2(p, q);
           const uint32_t face = t.face();
@@ -3041,7 +3055,7 @@ __device__ bool scene_point_instance(const MeshView& M, const float* W, const fl
         }
       }
       ref = kDone;
-      while (st.sp > 0) {
+      while (st.sp > base) {
         const uint2 e = st.pop();
         if (__longlong_as_double(static_cast<long long>(static_cast<uint64_t>(e.y) << 32)) <= b.d2) {
           ref = e.x;
@@ -3056,11 +3070,12 @@ __device__ bool scene_point_instance(const MeshView& M, const float* W, const fl
 
 // One lane per world point: the top level near child first (pruned by the best squared distance,
 // equal bounds still visited so ties resolve the same way every time), each instance leaf walks
-// that instance's tree. Outputs: face (in the instance's geometry), instance, world distance, world
-// point and the barycoords of the point in the world triangle (those of the local triangle too).
+// that instance's tree. The top level's far children and the instance walks' share one stack (its
+// LDS part small, so more waves fit), the instance walks above. Outputs: face (in the instance's
+// geometry), instance, world distance, world point and the barycoords of the point in the world
+// triangle (those of the local triangle too).
 __global__ __launch_bounds__(kBlock) void k_scene_closest_point(SceneArgs A) {
   __shared__ uint2 lds_stack[8 * kBlock];
-  __shared__ uint32_t tstack[kSceneStack * kBlock];
   const uint32_t tid = threadIdx.x;
   const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kBlock + tid;
   StackL<8> st;
@@ -3080,13 +3095,10 @@ __global__ __launch_bounds__(kBlock) void k_scene_closest_point(SceneArgs A) {
     b.inst = b.face = b.slot = kNoHit;
     b.p = d3(0, 0, 0);
     bool overflow = false;
-    int sp = 1;
-    tstack[tid] = 0u;
-    while (sp > 0 && !overflow) {
-      sp--;
-      const uint32_t node = tstack[sp * kBlock + tid];
+    st.sp = 0;
+    uint32_t node = 0;  // the root is always entered (an empty scene's is an empty leaf)
+    while (!overflow) {
       const uint32_t* nw = A.tlas + 8ull * node;
-      if (tlas_box_d2_lb(reinterpret_cast<const float*>(nw), px, py, pz) > b.d2) continue;
       const uint32_t w7 = nw[7];
       if ((w7 >> 16) == 0xFFFFu) {
         const uint32_t off = nw[6], cnt = w7 & 0xFFFFu;
@@ -3095,19 +3107,37 @@ __global__ __launch_bounds__(kBlock) void k_scene_closest_point(SceneArgs A) {
           overflow = !scene_point_instance(A.meshes[A.instance_geometry[inst]], A.world + 16ull * inst,
                                            A.frame + 16ull * inst, A.frame_ok[inst] != 0, inst, p, px, py, pz, st, b);
         }
+        if (overflow) break;
       } else {
-        if (sp + 2 > kSceneStack) {
-          overflow = true;
-          break;
-        }
         const uint32_t left = node + 1, right = node + nw[6];
         const double dl = tlas_box_d2_lb(reinterpret_cast<const float*>(A.tlas + 8ull * left), px, py, pz);
         const double dr = tlas_box_d2_lb(reinterpret_cast<const float*>(A.tlas + 8ull * right), px, py, pz);
-        const bool left_first = dl <= dr;
-        tstack[sp * kBlock + tid] = left_first ? right : left;  // far child below the near one
-        tstack[(sp + 1) * kBlock + tid] = left_first ? left : right;
-        sp += 2;
+        const bool hl = dl <= b.d2, hr = dr <= b.d2;
+        if (hl && hr) {
+          const bool left_first = dl <= dr;
+          // the upper word of a non-negative double rounds it down: still a lower bound when popped
+          if (!st.push(left_first ? right : left,
+                       static_cast<uint32_t>(static_cast<uint64_t>(__double_as_longlong(left_first ? dr : dl)) >> 32))) {
+            overflow = true;
+            break;
+          }
+          node = left_first ? left : right;
+          continue;
+        }
+        if (hl || hr) {
+          node = hl ? left : right;
+          continue;
+        }
       }
+      node = kDone;
+      while (st.sp > 0) {
+        const uint2 e = st.pop();
+        if (__longlong_as_double(static_cast<long long>(static_cast<uint64_t>(e.y) << 32)) <= b.d2) {
+          node = e.x;
+          break;
+        }
+      }
+      if (node == kDone) break;
     }
     uint32_t face = overflow ? kOverflow : b.face;
     float dist = kInf, wx = qnan, wy = qnan, wz = qnan, bw = qnan, bu = qnan, bv = qnan;
@@ -3131,6 +3161,235 @@ __global__ __launch_bounds__(kBlock) void k_scene_closest_point(SceneArgs A) {
       wx = static_cast<float>(b.p.x);
       wy = static_cast<float>(b.p.y);
       wz = static_cast<float>(b.p.z);
+    }
+    A.out_face[i] = face;
+    if (A.out_instance) A.out_instance[i] = overflow ? kNoHit : b.inst;
+    if (A.out_dist) A.out_dist[i] = dist;
+    if (A.out_point) {
+      A.out_point[3 * i] = wx;
+      A.out_point[3 * i + 1] = wy;
+      A.out_point[3 * i + 2] = wz;
+    }
+    if (A.out_bary) {
+      A.out_bary[3 * i] = bw;
+      A.out_bary[3 * i + 1] = bu;
+      A.out_bary[3 * i + 2] = bv;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Scene closest point with four lanes per point (the mesh kernel's quad scheme): the quad walks the
+// top level together (one far child pushed per level), and each instance's 4-wide tree with one
+// child per lane, ranked and pushed by quad_children; a leaf's triangles are split over the lanes.
+// The bounds and the decisions are those of the one-lane kernel (local-frame FP32 lower bounds,
+// FP64 world distances, least (squared distance, instance, face) wins), so the results are the same
+// bit for bit: a pruned child or triangle has a certified lower bound above a distance some
+// triangle reaches. The best is reduced over the quad after every leaf, so the node steps see one
+// bound.
+// ------------------------------------------------------------------------------------------
+struct ScenePick {
+  double d2;
+  uint32_t inst, face, slot;
+};
+
+__device__ __forceinline__ bool scene_pick_less(double d2, uint32_t inst, uint32_t face, const ScenePick& b) {
+  return d2 < b.d2 || (d2 == b.d2 && (inst < b.inst || (inst == b.inst && face < b.face)));
+}
+
+template <int kCtrl>
+__device__ __forceinline__ double quad_dpp_d(double v) {
+  const uint64_t u = static_cast<uint64_t>(__double_as_longlong(v));
+  const uint32_t lo = quad_dpp<kCtrl>(static_cast<uint32_t>(u)), hi = quad_dpp<kCtrl>(static_cast<uint32_t>(u >> 32));
+  return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+
+// The least pick of the quad, in every lane (the order is total, so the lanes agree).
+__device__ __forceinline__ void quad_min_pick(ScenePick& b) {
+  ScenePick o;
+  o.d2 = quad_dpp_d<0xB1>(b.d2);
+  o.inst = quad_xor1(b.inst);
+  o.face = quad_xor1(b.face);
+  o.slot = quad_xor1(b.slot);
+  if (scene_pick_less(o.d2, o.inst, o.face, b)) b = o;
+  o.d2 = quad_dpp_d<0x4E>(b.d2);
+  o.inst = quad_xor2(b.inst);
+  o.face = quad_xor2(b.face);
+  o.slot = quad_xor2(b.slot);
+  if (scene_pick_less(o.d2, o.inst, o.face, b)) b = o;
+}
+
+// Pops the nearest stacked entry whose lower bound (a float rounded down) is within the best.
+__device__ __forceinline__ uint32_t quad_pop_within(QuadStack& st, int base, double best) {
+  while (st.sp > base) {
+    const uint2 e = st.pop();
+    if (static_cast<double>(__uint_as_float(e.y)) <= best) return e.x;
+  }
+  return kDone;
+}
+
+// One instance's 4-wide tree walked by a quad in world space; updates `b` (quad-uniform on return).
+// Returns false on a stack overflow.
+__device__ bool scene_point_instance4(const MeshView& M, const float* W, const float* I, bool frame_ok, uint32_t inst,
+                                      D3 p, float px, float py, float pz, QuadStack& st, uint32_t lane, ScenePick& b) {
+  const uint32_t q4 = lane & 3u;
+  double Wd[16];
+  for (int k = 0; k < 16; k++) Wd[k] = W[k];
+  const LocalFrame lf = local_frame(W, I, px, py, pz);
+  const bool local = frame_ok && lf.s2 > 0.0f;
+  auto box_lb = [&](float a, float bb, float c, float d, float e, float g) {
+    return local ? local_box_d2_lb(lf, a, bb, c, d, e, g) : world_box_d2_lb(W, px, py, pz, a, bb, c, d, e, g);
+  };
+  const f3 lq = mk(lf.l[0], lf.l[1], lf.l[2]);
+  const float tri_err = 8.0f * kU * (maxabs(lq) + *M.coord_max) + (lf.d[0] + lf.d[1] + lf.d[2]);
+  const int base = st.sp;  // the top level's entries stay below
+  bool overflow = false;
+  for (uint32_t r = 0; r < M.n_roots; r++) {
+    st.sp = base;
+    uint32_t ref = M.roots4[r];
+    while (true) {
+      while (!(ref & kRefLeaf) && ref != kDone) {
+        const float4 c0 = M.nodes4[8ull * ref + 2 * q4];
+        const float4 c1 = M.nodes4[8ull * ref + 2 * q4 + 1];
+        const uint32_t cref = __float_as_uint(c1.z);
+        bool in = cref != kDone;
+        float key = kInf;
+        if (in) {
+          const double lb = box_lb(c0.x, c0.y, c0.z, c0.w, c1.x, c1.y);
+          in = lb <= b.d2;
+          key = __double2float_rd(lb);  // a lower bound still, for the test when popped
+        }
+        ref = quad_children(st, lane, in, key, cref, overflow);
+        if (overflow) return false;
+        if (ref == kDone) ref = quad_pop_within(st, base, b.d2);
+      }
+      while (ref & kRefLeaf) {
+        uint32_t off, cnt;
+        leaf_range(ref, M.bigleaf, off, cnt);
+        for (uint32_t s = off + q4; s < off + cnt; s += 4) {
+          const Tri t = load_tri(M.tris, s);
+          if (box_lb(fminf(fminf(t.p0.x, t.p1.x), t.p2.x), fminf(fminf(t.p0.y, t.p1.y), t.p2.y),
+                     fminf(fminf(t.p0.z, t.p1.z), t.p2.z), fmaxf(fmaxf(t.p0.x, t.p1.x), t.p2.x),
+                     fmaxf(fmaxf(t.p0.y, t.p1.y), t.p2.y), fmaxf(fmaxf(t.p0.z, t.p1.z), t.p2.z)) > b.d2)
+            continue;
+          if (local) {
+            const f3 c = closest_on_triangle(lq, t.a(), t.b(), t.c());
+            const float ex = lq.x - c.x, ey = lq.y - c.y, ez = lq.z - c.z;
+            const float dl = __builtin_amdgcn_sqrtf(ex * ex + ey * ey + ez * ez);
+            const float lb = dl - (tri_err + dl * (4.0f * kU));
+            if (lb > 0.0f && static_cast<double>(lb * lb * lf.s2) * (1.0 - 1.0 / 16384.0) > b.d2) continue;
+          }
+          const D3 qd = closest_on_tri_d(p, world_vertex(Wd, t.p0), world_vertex(Wd, t.p1), world_vertex(Wd, t.p2));
+          const double d2 = ddist2(p, qd);
+          const uint32_t face = t.face();
+          if (scene_pick_less(d2, inst, face, b)) {
+            b.d2 = d2;
+            b.inst = inst;
+            b.face = face;
+            b.slot = s;
+          }
+        }
+        quad_min_pick(b);
+        ref = quad_pop_within(st, base, b.d2);
+      }
+      if (ref == kDone) break;
+    }
+  }
+  return !overflow;
+}
+
+__device__ __noinline__ void scene_point_out(const SceneArgs& A, uint64_t i, bool overflow, ScenePick b, D3 p);
+
+__global__ __launch_bounds__(kBlock) void k_scene_closest_point4(SceneArgs A) {
+  __shared__ uint2 qstack[kQuadLds * (kBlock / 4)];
+  const uint32_t lane = threadIdx.x, q4 = lane & 3u, wq = (lane & 63u) >> 2, qid = lane >> 2;
+  QuadStack st;
+  st.lds = qstack;
+  st.ovf = A.stack.ovf;
+  st.ovf_stride = static_cast<uint64_t>(gridDim.x) * (kBlock / 4);
+  st.gquad = static_cast<uint64_t>(blockIdx.x) * (kBlock / 4) + qid;
+  st.qid = qid;
+  st.capacity = kQuadLds + static_cast<int>(A.stack.ovf_entries);
+  while (true) {
+    uint32_t tile = 0;
+    if ((lane & 63u) == 0) tile = atomicAdd(A.queue, 1u);
+    tile = __shfl(tile, 0, 64);
+    if (static_cast<uint64_t>(tile) * 16 >= A.n) break;  // every wave reaches this exit
+    const uint64_t visit = static_cast<uint64_t>(tile) * 16 + wq;
+    if (visit >= A.n) continue;
+    const uint64_t i = A.order ? A.order[visit] : visit;
+    const float px = A.origins[3 * i], py = A.origins[3 * i + 1], pz = A.origins[3 * i + 2];
+    const D3 p = d3(px, py, pz);
+    ScenePick b = {__builtin_huge_val(), kNoHit, kNoHit, kNoHit};
+    bool overflow = false;
+    st.sp = 0;
+    uint32_t node = 0;  // the root is always entered (an empty scene's is an empty leaf)
+    while (true) {
+      const uint32_t* nw = A.tlas + 8ull * node;
+      const uint32_t w7 = nw[7];
+      if ((w7 >> 16) == 0xFFFFu) {
+        const uint32_t off = nw[6], cnt = w7 & 0xFFFFu;
+        for (uint32_t s = off; s < off + cnt && !overflow; s++) {
+          const uint32_t inst = A.tlas_inst[s];
+          overflow = !scene_point_instance4(A.meshes[A.instance_geometry[inst]], A.world + 16ull * inst,
+                                            A.frame + 16ull * inst, A.frame_ok[inst] != 0, inst, p, px, py, pz, st,
+                                            lane, b);
+        }
+        if (overflow) break;
+      } else {
+        const uint32_t left = node + 1, right = node + nw[6];
+        const double dl = tlas_box_d2_lb(reinterpret_cast<const float*>(A.tlas + 8ull * left), px, py, pz);
+        const double dr = tlas_box_d2_lb(reinterpret_cast<const float*>(A.tlas + 8ull * right), px, py, pz);
+        const bool hl = dl <= b.d2, hr = dr <= b.d2;
+        if (hl && hr) {
+          const bool left_first = dl <= dr;
+          if (!st.push(left_first ? right : left, __float_as_uint(__double2float_rd(left_first ? dr : dl)))) {
+            overflow = true;
+            break;
+          }
+          node = left_first ? left : right;
+          continue;
+        }
+        if (hl || hr) {
+          node = hl ? left : right;
+          continue;
+        }
+      }
+      node = quad_pop_within(st, 0, b.d2);
+      if (node == kDone) break;
+    }
+    if (q4 == 0) scene_point_out(A, i, overflow, b, p);
+  }
+}
+
+// Outputs of the four-lane kernel for point i (one lane of the quad).
+__device__ __noinline__ void scene_point_out(const SceneArgs& A, uint64_t i, bool overflow, ScenePick b, D3 p) {
+  const float qnan = __int_as_float(0x7fc00000);
+  {
+    uint32_t face = overflow ? kOverflow : b.face;
+    float dist = kInf, wx = qnan, wy = qnan, wz = qnan, bw = qnan, bu = qnan, bv = qnan;
+    if (!overflow && b.inst != kNoHit) {
+      // the winner's world point again (the same double-precision steps), then three's
+      // Triangle.getBarycoord of the point in the world triangle
+      const MeshView& M = A.meshes[A.instance_geometry[b.inst]];
+      double Wd[16];
+      for (int k = 0; k < 16; k++) Wd[k] = A.world[16ull * b.inst + k];
+      const Tri t = load_tri(M.tris, b.slot);
+      const D3 a = world_vertex(Wd, t.p0), bb = world_vertex(Wd, t.p1), c = world_vertex(Wd, t.p2);
+      const D3 wp = closest_on_tri_d(p, a, bb, c);
+      const D3 v0 = dsub(c, a), v1 = dsub(bb, a), v2 = dsub(wp, a);
+      const double d00 = ddot(v0, v0), d01 = ddot(v0, v1), d02 = ddot(v0, v2), d11 = ddot(v1, v1), d12 = ddot(v1, v2);
+      const double den = d00 * d11 - d01 * d01;
+      if (den != 0) {
+        const double inv = 1 / den, u = (d11 * d02 - d01 * d12) * inv, v = (d00 * d12 - d01 * d02) * inv;
+        bw = static_cast<float>(1 - u - v);
+        bu = static_cast<float>(v);
+        bv = static_cast<float>(u);
+      }
+      dist = static_cast<float>(sqrt(b.d2));
+      wx = static_cast<float>(wp.x);
+      wy = static_cast<float>(wp.y);
+      wz = static_cast<float>(wp.z);
     }
     A.out_face[i] = face;
     if (A.out_instance) A.out_instance[i] = overflow ? kNoHit : b.inst;
@@ -3231,6 +3490,13 @@ hipError_t launch_intersects_box(const BoxArgs& a, uint64_t threads, hipStream_t
 hipError_t launch_scene_closest_point(const SceneArgs& a, uint64_t threads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_scene_closest_point, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_scene_closest_point4(const SceneArgs& a, uint64_t threads, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.queue);
+  hipLaunchKernelGGL(k_scene_closest_point4, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
