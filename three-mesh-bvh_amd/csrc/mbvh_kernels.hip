@@ -2935,6 +2935,19 @@ __device__ __forceinline__ double tlas_box_d2_lb(const float* b, float px, float
 }
 
 __device__ __forceinline__ D3 world_vertex(const double* Wd, float4 v) { return apply_matrix4_d(Wd, v.x, v.y, v.z); }
+// The same from the float matrix, each entry widened to double where it is used (same values).
+__device__ __forceinline__ D3 world_vertex_f(const float* W, float4 v) {
+  MBVH_NO_CONTRACT
+  const double x = v.x, y = v.y, z = v.z;
+  const double w = 1 / (static_cast<double>(W[3]) * x + static_cast<double>(W[7]) * y + static_cast<double>(W[11]) * z +
+                        static_cast<double>(W[15]));
+  return d3((static_cast<double>(W[0]) * x + static_cast<double>(W[4]) * y + static_cast<double>(W[8]) * z +
+             static_cast<double>(W[12])) * w,
+            (static_cast<double>(W[1]) * x + static_cast<double>(W[5]) * y + static_cast<double>(W[9]) * z +
+             static_cast<double>(W[13])) * w,
+            (static_cast<double>(W[2]) * x + static_cast<double>(W[6]) * y + static_cast<double>(W[10]) * z +
+             static_cast<double>(W[14])) * w);
+}
 
 // The point in an instance's frame, for box bounds measured there: for an affine W, the world
 // distance to anything W puts in a box is at least sigma * (the local distance to the box), sigma the
@@ -2945,6 +2958,7 @@ __device__ __forceinline__ D3 world_vertex(const double* Wd, float4 v) { return 
 // caller-supplied inverse is never used for bounds).
 struct LocalFrame {
   float l[3], d[3], s2;
+  float s2u;  // an upper bound of the largest singular value squared (Gershgorin, grown for the roundings)
 };
 __device__ __forceinline__ LocalFrame local_frame(const float* W, const float* I, float px, float py, float pz) {
   LocalFrame f;
@@ -2958,14 +2972,16 @@ __device__ __forceinline__ LocalFrame local_frame(const float* W, const float* I
     for (int j = 0; j < 3; j++)
       g[i][j] = W[4 * i] * W[4 * j] + W[4 * i + 1] * W[4 * j + 1] + W[4 * i + 2] * W[4 * j + 2];
   // each entry's FP32 error is a few ulps of |col i| |col j|: 2^-20 of that per off-diagonal entry
-  float s2 = kInf;
+  float s2 = kInf, s2u = 0.0f;
   for (int i = 0; i < 3; i++) {
     const int j = (i + 1) % 3, k = (i + 2) % 3;
     const float off = fabsf(g[i][j]) + fabsf(g[i][k]) +
                       (sqrtf(g[i][i] * g[j][j]) + sqrtf(g[i][i] * g[k][k])) * (1.0f / 1048576.0f);
     s2 = fminf(s2, g[i][i] * (1.0f - 1.0f / 65536.0f) - off * (1.0f + 1.0f / 65536.0f));
+    s2u = fmaxf(s2u, (g[i][i] + off) * (1.0f + 1.0f / 65536.0f));
   }
   f.s2 = s2 > 0.0f ? s2 : 0.0f;
+  f.s2u = s2u;
   return f;
 }
 
@@ -3230,11 +3246,10 @@ __device__ __forceinline__ uint32_t quad_pop_within(QuadStack& st, int base, dou
 
 // One instance's 4-wide tree walked by a quad in world space; updates `b` (quad-uniform on return).
 // Returns false on a stack overflow.
-__device__ bool scene_point_instance4(const MeshView& M, const float* W, const float* I, bool frame_ok, uint32_t inst,
-                                      D3 p, float px, float py, float pz, QuadStack& st, uint32_t lane, ScenePick& b) {
+__device__ __forceinline__ bool scene_point_instance4(const MeshView& M, const float* W, const float* I, bool frame_ok,
+                                                      uint32_t inst, D3 p, float px, float py, float pz, QuadStack& st,
+                                                      uint32_t lane, ScenePick& b, float& ubw) {
   const uint32_t q4 = lane & 3u;
-  double Wd[16];
-  for (int k = 0; k < 16; k++) Wd[k] = W[k];
   const LocalFrame lf = local_frame(W, I, px, py, pz);
   const bool local = frame_ok && lf.s2 > 0.0f;
   auto box_lb = [&](float a, float bb, float c, float d, float e, float g) {
@@ -3256,12 +3271,12 @@ __device__ bool scene_point_instance4(const MeshView& M, const float* W, const f
         float key = kInf;
         if (in) {
           const double lb = box_lb(c0.x, c0.y, c0.z, c0.w, c1.x, c1.y);
-          in = lb <= b.d2;
+          in = lb <= fmin(b.d2, static_cast<double>(ubw));
           key = __double2float_rd(lb);  // a lower bound still, for the test when popped
         }
         ref = quad_children(st, lane, in, key, cref, overflow);
         if (overflow) return false;
-        if (ref == kDone) ref = quad_pop_within(st, base, b.d2);
+        if (ref == kDone) ref = quad_pop_within(st, base, fmin(b.d2, static_cast<double>(ubw)));
       }
       while (ref & kRefLeaf) {
         uint32_t off, cnt;
@@ -3270,16 +3285,26 @@ __device__ bool scene_point_instance4(const MeshView& M, const float* W, const f
           const Tri t = load_tri(M.tris, s);
           if (box_lb(fminf(fminf(t.p0.x, t.p1.x), t.p2.x), fminf(fminf(t.p0.y, t.p1.y), t.p2.y),
                      fminf(fminf(t.p0.z, t.p1.z), t.p2.z), fmaxf(fmaxf(t.p0.x, t.p1.x), t.p2.x),
-                     fmaxf(fmaxf(t.p0.y, t.p1.y), t.p2.y), fmaxf(fmaxf(t.p0.z, t.p1.z), t.p2.z)) > b.d2)
+                     fmaxf(fmaxf(t.p0.y, t.p1.y), t.p2.y), fmaxf(fmaxf(t.p0.z, t.p1.z), t.p2.z)) >
+              fmin(b.d2, static_cast<double>(ubw)))
             continue;
           if (local) {
+            // sigma_min * (local distance less its error) <= world distance <= sigma_max * (local
+            // distance plus its error): the upper bound joins ubw (a NaN leaves it), and a triangle
+            // whose lower bound is above min(best, ubw) can neither win nor tie
             const f3 c = closest_on_triangle(lq, t.a(), t.b(), t.c());
             const float ex = lq.x - c.x, ey = lq.y - c.y, ez = lq.z - c.z;
             const float dl = __builtin_amdgcn_sqrtf(ex * ex + ey * ey + ez * ez);
-            const float lb = dl - (tri_err + dl * (4.0f * kU));
-            if (lb > 0.0f && static_cast<double>(lb * lb * lf.s2) * (1.0 - 1.0 / 16384.0) > b.d2) continue;
+            const float e = tri_err + dl * (4.0f * kU);
+            const float hi = dl + e;
+            ubw = fminf(ubw, hi * hi * lf.s2u * (1.0f + 1.0f / 16384.0f));
+            const float lb = dl - e;
+            if (lb > 0.0f && static_cast<double>(lb * lb * lf.s2) * (1.0 - 1.0 / 16384.0) >
+                                 fmin(b.d2, static_cast<double>(ubw)))
+              continue;
           }
-          const D3 qd = closest_on_tri_d(p, world_vertex(Wd, t.p0), world_vertex(Wd, t.p1), world_vertex(Wd, t.p2));
+          // the matrix read again here (cached) rather than held in registers through the walk
+          const D3 qd = closest_on_tri_d(p, world_vertex_f(W, t.p0), world_vertex_f(W, t.p1), world_vertex_f(W, t.p2));
           const double d2 = ddist2(p, qd);
           const uint32_t face = t.face();
           if (scene_pick_less(d2, inst, face, b)) {
@@ -3290,7 +3315,9 @@ __device__ bool scene_point_instance4(const MeshView& M, const float* W, const f
           }
         }
         quad_min_pick(b);
-        ref = quad_pop_within(st, base, b.d2);
+        ubw = fminf(ubw, quad_xor1(ubw));
+        ubw = fminf(ubw, quad_xor2(ubw));
+        ref = quad_pop_within(st, base, fmin(b.d2, static_cast<double>(ubw)));
       }
       if (ref == kDone) break;
     }
@@ -3298,9 +3325,9 @@ __device__ bool scene_point_instance4(const MeshView& M, const float* W, const f
   return !overflow;
 }
 
-__device__ __noinline__ void scene_point_out(const SceneArgs& A, uint64_t i, bool overflow, ScenePick b, D3 p);
+__device__ __forceinline__ void scene_point_out(const SceneArgs& A, uint64_t i, bool overflow, ScenePick b, D3 p);
 
-__global__ __launch_bounds__(kBlock) void k_scene_closest_point4(SceneArgs A) {
+__global__ __launch_bounds__(kBlock, 4) void k_scene_closest_point4(SceneArgs A) {
   __shared__ uint2 qstack[kQuadLds * (kBlock / 4)];
   const uint32_t lane = threadIdx.x, q4 = lane & 3u, wq = (lane & 63u) >> 2, qid = lane >> 2;
   QuadStack st;
@@ -3321,6 +3348,7 @@ __global__ __launch_bounds__(kBlock) void k_scene_closest_point4(SceneArgs A) {
     const float px = A.origins[3 * i], py = A.origins[3 * i + 1], pz = A.origins[3 * i + 2];
     const D3 p = d3(px, py, pz);
     ScenePick b = {__builtin_huge_val(), kNoHit, kNoHit, kNoHit};
+    float ubw = kInf;  // an upper bound of the least world squared distance, from FP32 distances
     bool overflow = false;
     st.sp = 0;
     uint32_t node = 0;  // the root is always entered (an empty scene's is an empty leaf)
@@ -3333,14 +3361,15 @@ __global__ __launch_bounds__(kBlock) void k_scene_closest_point4(SceneArgs A) {
           const uint32_t inst = A.tlas_inst[s];
           overflow = !scene_point_instance4(A.meshes[A.instance_geometry[inst]], A.world + 16ull * inst,
                                             A.frame + 16ull * inst, A.frame_ok[inst] != 0, inst, p, px, py, pz, st,
-                                            lane, b);
+                                            lane, b, ubw);
         }
         if (overflow) break;
       } else {
         const uint32_t left = node + 1, right = node + nw[6];
         const double dl = tlas_box_d2_lb(reinterpret_cast<const float*>(A.tlas + 8ull * left), px, py, pz);
         const double dr = tlas_box_d2_lb(reinterpret_cast<const float*>(A.tlas + 8ull * right), px, py, pz);
-        const bool hl = dl <= b.d2, hr = dr <= b.d2;
+        const double bnd = fmin(b.d2, static_cast<double>(ubw));
+        const bool hl = dl <= bnd, hr = dr <= bnd;
         if (hl && hr) {
           const bool left_first = dl <= dr;
           if (!st.push(left_first ? right : left, __float_as_uint(__double2float_rd(left_first ? dr : dl)))) {
@@ -3355,7 +3384,7 @@ __global__ __launch_bounds__(kBlock) void k_scene_closest_point4(SceneArgs A) {
           continue;
         }
       }
-      node = quad_pop_within(st, 0, b.d2);
+      node = quad_pop_within(st, 0, fmin(b.d2, static_cast<double>(ubw)));
       if (node == kDone) break;
     }
     if (q4 == 0) scene_point_out(A, i, overflow, b, p);
@@ -3363,7 +3392,7 @@ __global__ __launch_bounds__(kBlock) void k_scene_closest_point4(SceneArgs A) {
 }
 
 // Outputs of the four-lane kernel for point i (one lane of the quad).
-__device__ __noinline__ void scene_point_out(const SceneArgs& A, uint64_t i, bool overflow, ScenePick b, D3 p) {
+__device__ __forceinline__ void scene_point_out(const SceneArgs& A, uint64_t i, bool overflow, ScenePick b, D3 p) {
   const float qnan = __int_as_float(0x7fc00000);
   {
     uint32_t face = overflow ? kOverflow : b.face;
