@@ -344,3 +344,45 @@ def test_scene_closest_point_lanes_agree(ctx, meshes, kind):
     assert np.all(one["face"] != M.NO_HIT)
     for k in ("face", "instance", "distance", "point", "barycoord"):
         assert np.array_equal(np.asarray(one[k]).view(np.uint32), np.asarray(four[k]).view(np.uint32)), k
+
+
+def test_scene_closest_point_long_walks(ctx):
+    """Points near the centre of a big closed mesh are nearly equidistant from all of its triangles,
+    so their walks run past the four-lane kernel's step budget and finish in the wave-per-point pass:
+    the results equal the one-lane kernel's bit for bit and the oracle's brute force on a sample."""
+    built = [build_mesh(G.sphere(1.5, 400, 200)), build_mesh(G.torus_knot(1.0, 0.3, 120, 16))]
+    ms = [ctx.upload(t, g.positions, idx) for g, idx, t in built]
+    rng = np.random.default_rng(61)
+    mats = []
+    for k, (scale, geo) in enumerate([((1.0, 1.0, 1.0), 0), ((1.3, 0.8, 1.1), 0), ((1.0, 1.0, 1.0), 1)]):
+        m = np.eye(4)
+        m[:3, :3] = np.diag(scale)
+        m[:3, 3] = [6.0 * k, 0.5 * k, -1.0 * k]
+        mats.append(m)
+    world = np.stack([m.T.reshape(-1) for m in mats]).astype(np.float32)
+    inverse = np.stack([np.linalg.inv(m).T.reshape(-1) for m in mats]).astype(np.float32)
+    geometry = np.array([0, 0, 1], np.uint32)
+    d = rng.standard_normal((600, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rho = np.concatenate([rng.uniform(0.0, 0.15, 300) * 1.5, rng.uniform(0.5, 3.0, 300)])
+    centre = world[rng.integers(0, 2, 600), 12:15].astype(np.float64)
+    pts = (centre + d * rho[:, None]).astype(np.float32)
+    print("case scene closest point long walks", flush=True)
+    sc = ctx.scene(ms, world, inverse, geometry)
+    try:
+        ctx.set_point_lanes(1)
+        one = sc.closest_point(pts)
+        ctx.set_point_lanes(0)
+        got = sc.closest_point(pts)
+    finally:
+        ctx.set_point_lanes(0)
+        sc.free()
+    for k in ("face", "instance", "distance", "point", "barycoord"):
+        assert np.array_equal(np.asarray(one[k]).view(np.uint32), np.asarray(got[k]).view(np.uint32)), k
+    oms = [O.OracleMesh(g.positions, idx, t.roots) for g, idx, t in built]
+    sample = np.r_[0:20, 300:320]
+    ref = O.scene_closest_point(oms, world, geometry, None, pts[sample])
+    np.testing.assert_allclose(got["distance"][sample], ref["distance"], rtol=1e-6, atol=1e-7)
+    assert (got["instance"][sample] == ref["instance"]).mean() >= 0.95
+    for m in ms:
+        m.free()

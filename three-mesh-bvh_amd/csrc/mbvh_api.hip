@@ -43,6 +43,7 @@ struct mbvh_ctx {
   DeviceBuffer staging[8];      // host-API input / output staging
   DeviceBuffer order;           // host-API query order
   DeviceBuffer order_ws;        // host-API ordering workspace
+  DeviceBuffer heavy;           // scene closest point: points left to the wave-per-point pass
   DeviceBuffer hits[5];         // host-API all-hits results: offsets, face, dist, point, bary
   DeviceBuffer grad[11];        // host-API backward staging
   DeviceBuffer grad_ws;         // host-API backward workspace
@@ -577,6 +578,7 @@ extern "C" void mbvh_ctx_destroy(mbvh_ctx* ctx) {
   if (ctx->small_host) (void)hipHostFree(ctx->small_host);
   if (ctx->grad_ws.ptr) (void)hipFree(ctx->grad_ws.ptr);
   if (ctx->order_ws.ptr) (void)hipFree(ctx->order_ws.ptr);
+  if (ctx->heavy.ptr) (void)hipFree(ctx->heavy.ptr);
   if (ctx->build_ws.ptr) (void)hipFree(ctx->build_ws.ptr);
   if (ctx->geo_ws.ptr) (void)hipFree(ctx->geo_ws.ptr);
   (void)hipStreamDestroy(ctx->stream);
@@ -2854,6 +2856,10 @@ extern "C" int mbvh_build_gpu_d(mbvh_ctx* ctx, const float* positions, uint64_t 
 
 // Depth bound of a scene's top-level tree (the scene raycast's per-lane stack holds that many levels).
 constexpr uint32_t kTlasMaxDepth = 24;
+// Scene closest point: leaf steps a quad takes on one point before leaving it to a wave of its own,
+// and the persistent waves of that pass.
+constexpr uint32_t kSceneHeavyBudget = 512;
+constexpr uint32_t kSceneWideWaves = 2048;
 
 struct mbvh_scene {
   uint32_t magic = 0x4E454353u;  // "SCEN"
@@ -2871,6 +2877,7 @@ struct mbvh_scene {
   uint32_t max_stack = 0;         // stack entries of the deepest instance walk (raycast)
   uint32_t max_stack_points = 0;  // closest point: the top level's far children below them
   uint32_t max_stack_points4 = 0;  // the same for the four-lane kernel (4-wide instance trees)
+  bool wide_ok = false;            // every instance tree's 4-wide walk fits the wave-per-point frontier
 };
 
 namespace {
@@ -3055,6 +3062,7 @@ extern "C" int mbvh_scene_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, 
   sc->max_stack = max_stack;
   sc->max_stack_points = max_stack_points;
   sc->max_stack_points4 = max_stack_points4;
+  sc->wide_ok = 64ull * (max_stack4 > 0 ? max_stack4 - 1 : 0) + 256 <= kWideFrontier;
   const size_t mat_bytes = std::max<size_t>(16ull * n_instances * sizeof(float), 64);
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&sc->d_meshes), views.size() * sizeof(MeshView));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_world), mat_bytes);
@@ -3225,7 +3233,17 @@ int enqueue_scene_points(mbvh_ctx* ctx, const mbvh_scene* sc, const float* point
     const uint64_t threads = launch_threads(4 * n, ctx->max_threads);
     a.stack = stack_of(ctx, threads / 4);
     a.queue = static_cast<uint32_t*>(ctx->queue.ptr);
+    // points whose walks run past kSceneHeavyBudget leaf steps go to one wave each (their walks can
+    // be thousands of steps, which one quad would take alone while the launch waits for it)
+    if (sc->wide_ok && n < (uint64_t{1} << 32)) {
+      int rc;
+      if ((rc = ensure_buffer(ctx->heavy, n * sizeof(uint32_t)))) return rc;
+      a.heavy = static_cast<uint32_t*>(ctx->heavy.ptr);
+      a.heavy_count = static_cast<uint32_t*>(ctx->queue.ptr) + 32;  // its own line of the counter block
+      a.heavy_budget = kSceneHeavyBudget;
+    }
     MBVH_HIP(launch_scene_closest_point4(a, threads, s));
+    if (a.heavy) MBVH_HIP(launch_scene_closest_point_wide(a, kSceneWideWaves, s));
   } else {
     const uint64_t threads = launch_threads(n, ctx->max_threads);
     a.stack = stack_of(ctx, threads);

@@ -289,12 +289,22 @@ struct SceneArgs {
   StackScratch stack;
   const uint32_t* order;           // closest point: visit order of the points (NULL: 0..n-1)
   uint32_t* queue;                 // closest point, four lanes: work-queue counter
+  // closest point, four lanes: a point still unfinished after `heavy_budget` leaf steps is left to
+  // the wave-per-point pass, its index appended to `heavy` (count in heavy_count); heavy NULL: none
+  uint32_t* heavy;
+  uint32_t* heavy_count;
+  uint32_t heavy_budget;
 };
 hipError_t launch_scene_raycast_first(const SceneArgs& a, uint64_t threads, hipStream_t s);
 // Closest point over the instances: `origins` holds the world points (directions, side, near / far unused).
 hipError_t launch_scene_closest_point(const SceneArgs& a, uint64_t threads, hipStream_t s);
 // The same with four lanes per point (threads = 4 per point, a multiple of the block; a work queue).
 hipError_t launch_scene_closest_point4(const SceneArgs& a, uint64_t threads, hipStream_t s);
+// The points left by the four-lane kernel (a.heavy, *a.heavy_count of them), one wave each over an
+// LDS frontier of kWideFrontier entries; `waves` persistent waves. The instance trees' 4-wide depth
+// must fit that frontier: 64 * (max_stack4 - 1) + 256 <= kWideFrontier.
+constexpr uint32_t kWideFrontier = 4096;
+hipError_t launch_scene_closest_point_wide(const SceneArgs& a, uint32_t waves, hipStream_t s);
 
 // ---- intersectsBox ----------------------------------------------------------------------------------
 struct BoxArgs {

@@ -2184,6 +2184,7 @@ __global__ __launch_bounds__(kBlock) void k_closest_point(ClosestPointArgs A) {
 // ---- scenes: closest first hit over instances (TLAS over instance world boxes) ------------------
 
 constexpr int kSceneStack = 25;  // TLAS stack entries per lane: the TLAS is at most 24 levels deep
+constexpr uint32_t kTlasStack = 24;  // the wave-per-point kernel's top-level stack (one far child per level)
 
 // Entry distance of a world ray into a box of the TLAS (near plane respected).
 __device__ __forceinline__ bool box_entry(f3 o, f3 inv, const float* b, float near_dist, float& t_enter) {
@@ -3247,8 +3248,9 @@ __device__ __forceinline__ uint32_t quad_pop_within(QuadStack& st, int base, dou
 // One instance's 4-wide tree walked by a quad in world space; updates `b` (quad-uniform on return).
 // Returns false on a stack overflow.
 __device__ __forceinline__ bool scene_point_instance4(const MeshView& M, const float* W, const float* I, bool frame_ok,
-                                                      uint32_t inst, D3 p, float px, float py, float pz, QuadStack& st,
-                                                      uint32_t lane, ScenePick& b, float& ubw) {
+                                                      uint32_t inst, float px, float py, float pz, QuadStack& st,
+                                                      uint32_t lane, ScenePick& b, float& ubw, uint32_t& steps,
+                                                      uint32_t budget) {
   const uint32_t q4 = lane & 3u;
   const LocalFrame lf = local_frame(W, I, px, py, pz);
   const bool local = frame_ok && lf.s2 > 0.0f;
@@ -3303,7 +3305,8 @@ __device__ __forceinline__ bool scene_point_instance4(const MeshView& M, const f
                                  fmin(b.d2, static_cast<double>(ubw)))
               continue;
           }
-          // the matrix read again here (cached) rather than held in registers through the walk
+          // the matrix and the point widened again here rather than held in registers through the walk
+          const D3 p = d3(px, py, pz);
           const D3 qd = closest_on_tri_d(p, world_vertex_f(W, t.p0), world_vertex_f(W, t.p1), world_vertex_f(W, t.p2));
           const double d2 = ddist2(p, qd);
           const uint32_t face = t.face();
@@ -3317,10 +3320,12 @@ __device__ __forceinline__ bool scene_point_instance4(const MeshView& M, const f
         quad_min_pick(b);
         ubw = fminf(ubw, quad_xor1(ubw));
         ubw = fminf(ubw, quad_xor2(ubw));
-        ref = quad_pop_within(st, base, fmin(b.d2, static_cast<double>(ubw)));
+        // past the step budget the walk ends here (quad-uniform) and the point goes to a wave of its own
+        ref = ++steps > budget ? kDone : quad_pop_within(st, base, fmin(b.d2, static_cast<double>(ubw)));
       }
       if (ref == kDone) break;
     }
+    if (steps > budget) break;
   }
   return !overflow;
 }
@@ -3346,10 +3351,11 @@ __global__ __launch_bounds__(kBlock, 4) void k_scene_closest_point4(SceneArgs A)
     if (visit >= A.n) continue;
     const uint64_t i = A.order ? A.order[visit] : visit;
     const float px = A.origins[3 * i], py = A.origins[3 * i + 1], pz = A.origins[3 * i + 2];
-    const D3 p = d3(px, py, pz);
     ScenePick b = {__builtin_huge_val(), kNoHit, kNoHit, kNoHit};
     float ubw = kInf;  // an upper bound of the least world squared distance, from FP32 distances
     bool overflow = false;
+    uint32_t steps = 0;
+    const uint32_t budget = A.heavy ? A.heavy_budget : 0xFFFFFFFFu;
     st.sp = 0;
     uint32_t node = 0;  // the root is always entered (an empty scene's is an empty leaf)
     while (true) {
@@ -3357,13 +3363,13 @@ __global__ __launch_bounds__(kBlock, 4) void k_scene_closest_point4(SceneArgs A)
       const uint32_t w7 = nw[7];
       if ((w7 >> 16) == 0xFFFFu) {
         const uint32_t off = nw[6], cnt = w7 & 0xFFFFu;
-        for (uint32_t s = off; s < off + cnt && !overflow; s++) {
+        for (uint32_t s = off; s < off + cnt && !overflow && steps <= budget; s++) {
           const uint32_t inst = A.tlas_inst[s];
           overflow = !scene_point_instance4(A.meshes[A.instance_geometry[inst]], A.world + 16ull * inst,
-                                            A.frame + 16ull * inst, A.frame_ok[inst] != 0, inst, p, px, py, pz, st,
-                                            lane, b, ubw);
+                                            A.frame + 16ull * inst, A.frame_ok[inst] != 0, inst, px, py, pz, st,
+                                            lane, b, ubw, steps, budget);
         }
-        if (overflow) break;
+        if (overflow || steps > budget) break;
       } else {
         const uint32_t left = node + 1, right = node + nw[6];
         const double dl = tlas_box_d2_lb(reinterpret_cast<const float*>(A.tlas + 8ull * left), px, py, pz);
@@ -3387,7 +3393,194 @@ __global__ __launch_bounds__(kBlock, 4) void k_scene_closest_point4(SceneArgs A)
       node = quad_pop_within(st, 0, fmin(b.d2, static_cast<double>(ubw)));
       if (node == kDone) break;
     }
-    if (q4 == 0) scene_point_out(A, i, overflow, b, p);
+    if (q4 == 0) {
+      if (!overflow && steps > budget)
+        A.heavy[atomicAdd(A.heavy_count, 1u)] = static_cast<uint32_t>(i);
+      else
+        scene_point_out(A, i, overflow, b, d3(px, py, pz));
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Scene closest point, one wave per point, for the points the four-lane kernel left (those whose
+// walks are long: a point near the centre of a closed mesh is close to every triangle). The wave
+// walks the top level together; in each instance it keeps an LDS frontier of 4-wide nodes: every
+// round the lanes take the top 64 entries (those whose bound is still within the best), expand
+// nodes (children within the best pushed at offsets from a prefix sum over the wave) or test a
+// leaf's triangles, and the best is reduced over the wave. The frontier, taken 64 at a time from
+// the top, holds at most 192 entries per level of the tree plus 256. Same bounds, same decisions
+// and the same order of picks as the one-lane kernel: the same results bit for bit.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void wave_min_pick(ScenePick& b) {
+  for (int m = 1; m < 64; m <<= 1) {
+    ScenePick o;
+    o.d2 = __shfl_xor(b.d2, m, 64);
+    o.inst = __shfl_xor(b.inst, m, 64);
+    o.face = __shfl_xor(b.face, m, 64);
+    o.slot = __shfl_xor(b.slot, m, 64);
+    if (scene_pick_less(o.d2, o.inst, o.face, b)) b = o;
+  }
+}
+
+__device__ __forceinline__ float wave_min_f(float v) {
+  for (int m = 1; m < 64; m <<= 1) v = fminf(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+// Exclusive prefix sum of c over the wave; *total receives the sum.
+__device__ __forceinline__ uint32_t wave_prefix(uint32_t c, uint32_t lane, uint32_t* total) {
+  uint32_t x = c;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= static_cast<uint32_t>(d)) x += y;
+  }
+  *total = __shfl(x, 63, 64);
+  return x - c;
+}
+
+__global__ __launch_bounds__(64) void k_scene_closest_point_wide(SceneArgs A) {
+  __shared__ uint2 front[kWideFrontier];
+  __shared__ uint2 top[kTlasStack];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t nh = *A.heavy_count;
+  for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+    const uint64_t i = A.heavy[h];
+    const float px = A.origins[3 * i], py = A.origins[3 * i + 1], pz = A.origins[3 * i + 2];
+    const D3 p = d3(px, py, pz);
+    ScenePick b = {__builtin_huge_val(), kNoHit, kNoHit, kNoHit};
+    float ubw = kInf;
+    bool overflow = false;
+    int tsp = 0;
+    uint32_t node = 0;
+    while (true) {  // the top level, the same in every lane
+      const uint32_t* nw = A.tlas + 8ull * node;
+      const uint32_t w7 = nw[7];
+      if ((w7 >> 16) == 0xFFFFu) {
+        const uint32_t off = nw[6], cnt = w7 & 0xFFFFu;
+        for (uint32_t ts = off; ts < off + cnt && !overflow; ts++) {
+          const uint32_t inst = A.tlas_inst[ts];
+          const MeshView& M = A.meshes[A.instance_geometry[inst]];
+          const float* W = A.world + 16ull * inst;
+          const LocalFrame lf = local_frame(W, A.frame + 16ull * inst, px, py, pz);
+          const bool local = A.frame_ok[inst] != 0 && lf.s2 > 0.0f;
+          const f3 lq = mk(lf.l[0], lf.l[1], lf.l[2]);
+          const float tri_err = 8.0f * kU * (maxabs(lq) + *M.coord_max) + (lf.d[0] + lf.d[1] + lf.d[2]);
+          for (uint32_t r = 0; r < M.n_roots && !overflow; r++) {
+            if (lane == 0) front[0] = make_uint2(M.roots4[r], 0u);
+            __syncthreads();
+            uint32_t fc = 1;
+            while (fc > 0) {
+              const uint32_t take = fc < 64u ? fc : 64u;
+              const uint32_t fb = fc - take;
+              const uint2 e = lane < take ? front[fb + lane] : make_uint2(kDone, 0u);
+              __syncthreads();  // every lane has read its entry before the pushes overwrite them
+              const double bnd = fmin(b.d2, static_cast<double>(ubw));
+              uint32_t kids[4];
+              float keys[4];
+              uint32_t nk = 0;
+              if (e.x != kDone && static_cast<double>(__uint_as_float(e.y)) <= bnd) {
+                if (!(e.x & kRefLeaf)) {
+                  for (uint32_t k = 0; k < 4; k++) {
+                    const float4 c0 = M.nodes4[8ull * e.x + 2 * k];
+                    const float4 c1 = M.nodes4[8ull * e.x + 2 * k + 1];
+                    const uint32_t cref = __float_as_uint(c1.z);
+                    if (cref == kDone) continue;
+                    const double lb = local ? local_box_d2_lb(lf, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y)
+                                            : world_box_d2_lb(W, px, py, pz, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y);
+                    if (lb <= bnd) {
+                      kids[nk] = cref;
+                      keys[nk] = __double2float_rd(lb);
+                      nk++;
+                    }
+                  }
+                } else {
+                  uint32_t off2, cnt2;
+                  leaf_range(e.x, M.bigleaf, off2, cnt2);
+                  for (uint32_t s = off2; s < off2 + cnt2; s++) {
+                    const Tri t = load_tri(M.tris, s);
+                    const float mnx = fminf(fminf(t.p0.x, t.p1.x), t.p2.x), mny = fminf(fminf(t.p0.y, t.p1.y), t.p2.y);
+                    const float mnz = fminf(fminf(t.p0.z, t.p1.z), t.p2.z), mxx = fmaxf(fmaxf(t.p0.x, t.p1.x), t.p2.x);
+                    const float mxy = fmaxf(fmaxf(t.p0.y, t.p1.y), t.p2.y), mxz = fmaxf(fmaxf(t.p0.z, t.p1.z), t.p2.z);
+                    const double tb = local ? local_box_d2_lb(lf, mnx, mny, mnz, mxx, mxy, mxz)
+                                            : world_box_d2_lb(W, px, py, pz, mnx, mny, mnz, mxx, mxy, mxz);
+                    if (tb > fmin(b.d2, static_cast<double>(ubw))) continue;
+                    if (local) {
+                      const f3 c = closest_on_triangle(lq, t.a(), t.b(), t.c());
+                      const float ex = lq.x - c.x, ey = lq.y - c.y, ez = lq.z - c.z;
+                      const float dl = __builtin_amdgcn_sqrtf(ex * ex + ey * ey + ez * ez);
+                      const float er = tri_err + dl * (4.0f * kU);
+                      const float hi = dl + er;
+                      ubw = fminf(ubw, hi * hi * lf.s2u * (1.0f + 1.0f / 16384.0f));
+                      const float lb = dl - er;
+                      if (lb > 0.0f && static_cast<double>(lb * lb * lf.s2) * (1.0 - 1.0 / 16384.0) >
+                                           fmin(b.d2, static_cast<double>(ubw)))
+                        continue;
+                    }
+                    const D3 qd = closest_on_tri_d(p, world_vertex_f(W, t.p0), world_vertex_f(W, t.p1),
+                                                   world_vertex_f(W, t.p2));
+                    const double d2 = ddist2(p, qd);
+                    const uint32_t face = t.face();
+                    if (scene_pick_less(d2, inst, face, b)) {
+                      b.d2 = d2;
+                      b.inst = inst;
+                      b.face = face;
+                      b.slot = s;
+                    }
+                  }
+                }
+              }
+              uint32_t total;
+              const uint32_t at = fb + wave_prefix(nk, lane, &total);
+              if (fb + total > kWideFrontier) {
+                overflow = true;  // wave-uniform (total and fb are)
+                break;
+              }
+              for (uint32_t k = 0; k < nk; k++) front[at + k] = make_uint2(kids[k], __float_as_uint(keys[k]));
+              __syncthreads();
+              fc = fb + total;
+              wave_min_pick(b);
+              ubw = wave_min_f(ubw);
+            }
+          }
+        }
+        if (overflow) break;
+      } else {
+        const uint32_t left = node + 1, right = node + nw[6];
+        const double dl = tlas_box_d2_lb(reinterpret_cast<const float*>(A.tlas + 8ull * left), px, py, pz);
+        const double dr = tlas_box_d2_lb(reinterpret_cast<const float*>(A.tlas + 8ull * right), px, py, pz);
+        const double bnd = fmin(b.d2, static_cast<double>(ubw));
+        const bool hl = dl <= bnd, hr = dr <= bnd;
+        if (hl && hr) {
+          const bool left_first = dl <= dr;
+          if (tsp >= static_cast<int>(kTlasStack)) {
+            overflow = true;
+            break;
+          }
+          if (lane == 0) top[tsp] = make_uint2(left_first ? right : left, __float_as_uint(__double2float_rd(left_first ? dr : dl)));
+          tsp++;
+          node = left_first ? left : right;
+          __syncthreads();
+          continue;
+        }
+        if (hl || hr) {
+          node = hl ? left : right;
+          continue;
+        }
+      }
+      node = kDone;
+      const double bnd = fmin(b.d2, static_cast<double>(ubw));
+      while (tsp > 0) {
+        const uint2 e = top[--tsp];
+        if (static_cast<double>(__uint_as_float(e.y)) <= bnd) {
+          node = e.x;
+          break;
+        }
+      }
+      if (node == kDone) break;
+    }
+    if (lane == 0) scene_point_out(A, i, overflow, b, p);
+    __syncthreads();  // the next point's entries overwrite this one's
   }
 }
 
@@ -3525,7 +3718,14 @@ hipError_t launch_scene_closest_point(const SceneArgs& a, uint64_t threads, hipS
 hipError_t launch_scene_closest_point4(const SceneArgs& a, uint64_t threads, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.queue);
+  if (a.heavy) hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.heavy_count);
   hipLaunchKernelGGL(k_scene_closest_point4, dim3(static_cast<uint32_t>(threads / kBlock)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_scene_closest_point_wide(const SceneArgs& a, uint32_t waves, hipStream_t s) {
+  if (a.n == 0 || !a.heavy) return hipSuccess;
+  hipLaunchKernelGGL(k_scene_closest_point_wide, dim3(waves), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
