@@ -51,6 +51,9 @@ def parse():
                     help="time the device BVH build of the 1M-triangle icosphere alone (for profiling) and exit")
     ap.add_argument("--only-geometry", action="store_true",
                     help="time closestPointToGeometry on the closest-point mesh alone (for profiling) and exit")
+    ap.add_argument("--only-scene", action="store_true",
+                    help="time the scene queries (raycast and closest point over the C5 instances) alone (for "
+                         "profiling) and exit")
     ap.add_argument("--c3-rays", type=int, default=1 << 20)
     ap.add_argument("--c4-grid", type=int, default=256)
     ap.add_argument("--c5-rays", type=int, default=1 << 22)
@@ -885,6 +888,23 @@ def main():
     if args.only_differentiable:
         ctx = M.Context(0)
         print(json.dumps(bench_differentiable(torch, M, W, ctx, stream, args)), flush=True)
+        return
+
+    if args.only_scene:
+        from concurrent.futures import ThreadPoolExecutor
+        ctx = M.Context(0)
+        w = W.c5_batched(args.c5_geometries, 1 << 10)
+        geos = [g.copy() for g in w.geometries]
+        for g in geos:
+            g.ensure_index()
+        idxs = [g.index.copy() for g in geos]
+        with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+            trees = list(ex.map(lambda k: M.build(geos[k].positions, idxs[k], strategy=M.SAH, target_leaf_size=10),
+                                range(len(geos))))
+        meshes = [ctx.upload(trees[k], geos[k].positions, idxs[k]) for k in range(len(geos))]
+        batch = ctx.batch(meshes)
+        print(json.dumps({"scene": bench_scene(torch, M, W, ctx, stream, args, meshes, batch, geos, idxs, trees, w)}),
+              flush=True)
         return
 
     if args.only_build:
