@@ -34,7 +34,8 @@ check('exports the query API', () => {
     assert.ok(api[k] !== undefined, k);
   }
   for (const m of ['raycast', 'raycastFirst', 'closestPointToPoint', 'raycastObject3D', 'raycastFirstBatch', 'refit', 'intersectsSphere', 'intersectsSphereBatch',
-    'raycastBatch', 'closestPointToPointBatch', 'getBoundingBox']) {
+    'raycastBatch', 'closestPointToPointBatch', 'getBoundingBox', 'raycastFirstBatchAsync',
+    'closestPointToPointBatchAsync']) {
     assert.strictEqual(typeof api.MeshBVH.prototype[m], 'function', m);
   }
   assert.strictEqual(typeof api.MeshBVH.serialize, 'function');

@@ -422,4 +422,25 @@ bvh.refit(partialIds);
 writeArr('partial_positions.bin', positions);
 writeArr('partial_root0.bin', new Uint8Array(bvh._roots[0]));
 
-console.log(JSON.stringify({ ok: true, rays: nRays, points: nPts, worldChecked, batchedChecked, presenceDiffers }));
+// --- asynchronous batch queries: several in flight, the same bytes as the synchronous calls -------
+(async () => {
+  const sameBytes = (a, b, what) => assert.ok(Buffer.from(a.buffer, a.byteOffset, a.byteLength)
+    .equals(Buffer.from(b.buffer, b.byteOffset, b.byteLength)), what);
+  const pending = [bvh.raycastFirstBatchAsync(origins, dirs), bvh.closestPointToPointBatchAsync(points),
+    bvh.raycastFirstBatchAsync(Float64Array.from(origins), Float64Array.from(dirs))];
+  assert.throws(() => bvh.dispose(), /in flight/);  // the mesh stays alive until they settle
+  const [ra1, ca1, ra2] = await Promise.all(pending);
+  const rs = bvh.raycastFirstBatch(origins, dirs);
+  const cs = bvh.closestPointToPointBatch(points);
+  const rd = bvh.raycastFirstBatch(Float64Array.from(origins), Float64Array.from(dirs));
+  for (const k of ['faceIndex', 'distance', 'point', 'barycoord']) {
+    sameBytes(ra1[k], rs[k], `async raycastFirst ${k}`);
+    sameBytes(ca1[k], cs[k], `async closestPoint ${k}`);
+    sameBytes(ra2[k], rd[k], `async double raycastFirst ${k}`);
+  }
+  console.log(JSON.stringify({ ok: true, rays: nRays, points: nPts, worldChecked, batchedChecked, presenceDiffers,
+    asyncChecked: true }));
+})().catch((e) => {
+  console.error(e);
+  process.exit(1);
+});

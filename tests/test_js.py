@@ -26,7 +26,7 @@ def test_addon_exports():
     names = set(r.stdout.strip().split(","))
     assert {"build", "uploadMesh", "freeMesh", "raycastFirst", "raycast", "closestPoint", "setSideMask",
             "deviceCount", "raycastFirstBackward", "closestPointBackward", "batchCreate", "freeBatch",
-            "batchRaycastFirst", "refitMesh", "intersectsSphere"} <= names
+            "batchRaycastFirst", "refitMesh", "intersectsSphere", "raycastFirstAsync", "closestPointAsync"} <= names
 
 
 @pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")

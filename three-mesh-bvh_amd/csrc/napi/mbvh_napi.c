@@ -25,6 +25,7 @@ typedef struct MeshHandle {
   mbvh_mesh* mesh;
   int device;
   int batch_refs;  /* batches using this mesh: it cannot be freed explicitly meanwhile */
+  int async_refs;  /* asynchronous queries in flight on this mesh: likewise */
 } MeshHandle;
 
 typedef struct BatchHandle {
@@ -347,6 +348,7 @@ static napi_value js_free_mesh(napi_env env, napi_callback_info info) {
   if (napi_get_value_external(env, argv[0], &p) == napi_ok && p) {
     MeshHandle* h = (MeshHandle*)p;
     if (h->batch_refs > 0) return throw_msg(env, "freeMesh: the mesh is used by a batch; free the batch first");
+    if (h->async_refs > 0) return throw_msg(env, "freeMesh: an asynchronous query on the mesh is in flight; await it first");
     if (h->mesh) mbvh_mesh_free(h->mesh);
     h->mesh = NULL;
   }
@@ -797,37 +799,193 @@ static int get_rays(napi_env env, napi_value vo, napi_value vd, View* o, View* d
   return o->type == napi_float32_array || *f64;
 }
 
-/* raycastFirst(mesh, origins, dirs, side, near, far, outFace, outDist, outPoint, outBary); Float64Array
- * rays are traced as the double rays (mbvh_raycast_first_f64). */
-static napi_value js_raycast_first(napi_env env, napi_callback_info info) {
-  napi_value argv[10];
-  get_args(env, info, 10, argv);
-  MeshHandle* h = get_mesh(env, argv[0]);
-  if (!h) return throw_msg(env, "raycastFirst: invalid mesh");
-  View o, d;
-  int f64 = 0;
-  if (!get_rays(env, argv[1], argv[2], &o, &d, &f64))
-    return throw_msg(env, "raycastFirst: origins / directions must be Float32Arrays or Float64Arrays of equal length 3n");
-  const size_t n = o.length / 3;
+/* One first-hit or closest-point query, parsed from its arguments; run on the calling thread
+ * (raycastFirst, closestPoint) or on a libuv worker thread (raycastFirstAsync, closestPointAsync). */
+typedef struct Query {
+  int kind;  /* 0: first hit, 1: closest point */
+  MeshHandle* h;
+  mbvh_ctx* ctx;
+  int f64;
+  const void* a;  /* origins, or points */
+  const void* b;  /* directions */
+  size_t n;
   int32_t side;
-  double nr, fr;
-  if (!get_i32(env, argv[3], &side) || !get_f64(env, argv[4], &nr) || !get_f64(env, argv[5], &fr))
-    return throw_msg(env, "raycastFirst: side / near / far must be numbers");
+  double p0, p1;  /* near / far, or min / max threshold */
   void *face, *dist, *pt, *bc;
-  if (!out_view(env, argv[6], napi_uint32_array, n, &face) || !face ||
-      !out_view(env, argv[7], napi_float32_array, n, &dist) || !out_view(env, argv[8], napi_float32_array, 3 * n, &pt) ||
-      !out_view(env, argv[9], napi_float32_array, 3 * n, &bc))
-    return throw_msg(env, "raycastFirst: output arrays have the wrong type or length");
-  mbvh_ctx* ctx = NULL;
-  int rc = context_for(env, h->device, &ctx);
-  if (!rc && f64)
-    rc = mbvh_raycast_first_f64(ctx, h->mesh, (const double*)o.data, (const double*)d.data, n, side, nr, fr,
-                                (uint32_t*)face, (float*)dist, (float*)pt, (float*)bc);
-  else if (!rc)
-    rc = mbvh_raycast_first(ctx, h->mesh, (const float*)o.data, (const float*)d.data, n, side, to_float(nr),
-                            to_float(fr), (uint32_t*)face, (float*)dist, (float*)pt, (float*)bc);
-  if (rc) return throw_status(env, rc, f64 ? "mbvh_raycast_first_f64" : "mbvh_raycast_first");
+  int rc;
+  char msg[512];  /* mbvh_last_error() of the thread that ran it */
+  napi_deferred deferred;
+  napi_async_work work;
+  napi_ref refs[8];  /* the mesh and the typed arrays, alive until an asynchronous query settles */
+  int n_refs;
+} Query;
+
+/* raycastFirst(mesh, origins, dirs, side, near, far, face, dist, point, bary): origins / dirs either
+ * Float32Arrays (FP32 rays) or Float64Arrays (double rays, traced as the reference traces them) */
+static const char* parse_raycast_first(napi_env env, napi_value* argv, Query* q) {
+  memset(q, 0, sizeof(*q));
+  q->kind = 0;
+  q->h = get_mesh(env, argv[0]);
+  if (!q->h) return "raycastFirst: invalid mesh";
+  View o, d;
+  if (!get_rays(env, argv[1], argv[2], &o, &d, &q->f64))
+    return "raycastFirst: origins / directions must be Float32Arrays or Float64Arrays of equal length 3n";
+  q->a = o.data;
+  q->b = d.data;
+  q->n = o.length / 3;
+  if (!get_i32(env, argv[3], &q->side) || !get_f64(env, argv[4], &q->p0) || !get_f64(env, argv[5], &q->p1))
+    return "raycastFirst: side / near / far must be numbers";
+  if (!out_view(env, argv[6], napi_uint32_array, q->n, &q->face) || !q->face ||
+      !out_view(env, argv[7], napi_float32_array, q->n, &q->dist) ||
+      !out_view(env, argv[8], napi_float32_array, 3 * q->n, &q->pt) ||
+      !out_view(env, argv[9], napi_float32_array, 3 * q->n, &q->bc))
+    return "raycastFirst: output arrays have the wrong type or length";
   return NULL;
+}
+
+/* closestPoint(mesh, points, minThreshold, maxThreshold, face, dist, point, bary): points a
+ * Float32Array or a Float64Array (double points) */
+static const char* parse_closest_point(napi_env env, napi_value* argv, Query* q) {
+  memset(q, 0, sizeof(*q));
+  q->kind = 1;
+  q->h = get_mesh(env, argv[0]);
+  if (!q->h) return "closestPoint: invalid mesh";
+  View p;
+  if (!get_view(env, argv[1], &p) || (p.type != napi_float32_array && p.type != napi_float64_array) || p.length % 3)
+    return "closestPoint: points must be a Float32Array or Float64Array of length 3n";
+  q->f64 = p.type == napi_float64_array;
+  q->a = p.data;
+  q->n = p.length / 3;
+  if (!get_f64(env, argv[2], &q->p0) || !get_f64(env, argv[3], &q->p1)) return "closestPoint: thresholds must be numbers";
+  if (!out_view(env, argv[4], napi_uint32_array, q->n, &q->face) || !q->face ||
+      !out_view(env, argv[5], napi_float32_array, q->n, &q->dist) ||
+      !out_view(env, argv[6], napi_float32_array, 3 * q->n, &q->pt) ||
+      !out_view(env, argv[7], napi_float32_array, 3 * q->n, &q->bc))
+    return "closestPoint: output arrays have the wrong type or length";
+  return NULL;
+}
+
+static const char* query_where(const Query* q) {
+  if (q->kind == 0) return q->f64 ? "mbvh_raycast_first_f64" : "mbvh_raycast_first";
+  return q->f64 ? "mbvh_closest_point_f64" : "mbvh_closest_point";
+}
+
+/* Runs the query through the C ABI (no JavaScript values touched: safe on a worker thread). */
+static void run_query(Query* q) {
+  if (q->kind == 0 && q->f64)
+    q->rc = mbvh_raycast_first_f64(q->ctx, q->h->mesh, (const double*)q->a, (const double*)q->b, q->n, q->side, q->p0,
+                                   q->p1, (uint32_t*)q->face, (float*)q->dist, (float*)q->pt, (float*)q->bc);
+  else if (q->kind == 0)
+    q->rc = mbvh_raycast_first(q->ctx, q->h->mesh, (const float*)q->a, (const float*)q->b, q->n, q->side,
+                               to_float(q->p0), to_float(q->p1), (uint32_t*)q->face, (float*)q->dist, (float*)q->pt,
+                               (float*)q->bc);
+  else if (q->f64)
+    q->rc = mbvh_closest_point_f64(q->ctx, q->h->mesh, (const double*)q->a, q->n, q->p0, q->p1, (uint32_t*)q->face,
+                                   (float*)q->dist, (float*)q->pt, (float*)q->bc);
+  else
+    q->rc = mbvh_closest_point(q->ctx, q->h->mesh, (const float*)q->a, q->n, to_float(q->p0), to_float(q->p1),
+                               (uint32_t*)q->face, (float*)q->dist, (float*)q->pt, (float*)q->bc);
+  if (q->rc) snprintf(q->msg, sizeof(q->msg), "%s", mbvh_last_error());
+}
+
+static napi_value run_sync(napi_env env, napi_callback_info info, size_t argc,
+                           const char* (*parse)(napi_env, napi_value*, Query*)) {
+  napi_value argv[10];
+  get_args(env, info, argc, argv);
+  Query q;
+  const char* bad = parse(env, argv, &q);
+  if (bad) return throw_msg(env, bad);
+  int rc = context_for(env, q.h->device, &q.ctx);
+  if (rc) return throw_status(env, rc, query_where(&q));
+  run_query(&q);
+  if (q.rc) return throw_status(env, q.rc, query_where(&q));
+  return NULL;
+}
+
+static napi_value js_raycast_first(napi_env env, napi_callback_info info) {
+  return run_sync(env, info, 10, parse_raycast_first);
+}
+
+static napi_value js_closest_point(napi_env env, napi_callback_info info) {
+  return run_sync(env, info, 8, parse_closest_point);
+}
+
+/* ---- asynchronous queries: napi_async_work runs the C call on a libuv worker thread (calls on one
+ * device context are serialised by the library), and a Promise settles on the main thread ---- */
+static void async_execute(napi_env env, void* data) {
+  (void)env;
+  run_query((Query*)data);
+}
+
+static void async_complete(napi_env env, napi_status status, void* data) {
+  Query* q = (Query*)data;
+  for (int i = 0; i < q->n_refs; i++) napi_delete_reference(env, q->refs[i]);
+  q->h->async_refs--;
+  if (status != napi_ok && !q->rc) {
+    q->rc = MBVH_ERR_BAD_ARGS;
+    snprintf(q->msg, sizeof(q->msg), "the query was cancelled before it ran");
+  }
+  if (q->rc) {
+    char msg[768], code[32];
+    snprintf(msg, sizeof(msg), "%s failed (mbvh status %d): %s", query_where(q), q->rc, q->msg);
+    snprintf(code, sizeof(code), "MBVH_%d", -q->rc);
+    napi_value m, c, err;
+    napi_create_string_utf8(env, msg, NAPI_AUTO_LENGTH, &m);
+    napi_create_string_utf8(env, code, NAPI_AUTO_LENGTH, &c);
+    napi_create_error(env, c, m, &err);
+    napi_reject_deferred(env, q->deferred, err);
+  } else {
+    napi_value undef;
+    napi_get_undefined(env, &undef);
+    napi_resolve_deferred(env, q->deferred, undef);
+  }
+  napi_delete_async_work(env, q->work);
+  free(q);
+}
+
+static napi_value run_async(napi_env env, napi_callback_info info, size_t argc,
+                            const char* (*parse)(napi_env, napi_value*, Query*), const char* name) {
+  napi_value argv[10];
+  get_args(env, info, argc, argv);
+  Query* q = (Query*)malloc(sizeof(Query));
+  if (!q) return throw_msg(env, "out of memory");
+  const char* bad = parse(env, argv, q);
+  if (bad) {
+    free(q);
+    return throw_msg(env, bad);
+  }
+  int rc = context_for(env, q->h->device, &q->ctx);  /* created here, on the main thread */
+  if (rc) {
+    free(q);
+    return throw_status(env, rc, query_where(q));
+  }
+  /* the mesh external and every typed-array argument stay referenced until the promise settles */
+  for (size_t i = 0; i < argc; i++) {
+    napi_valuetype t;
+    napi_typeof(env, argv[i], &t);
+    if ((t == napi_object || t == napi_external) && q->n_refs < 8) napi_create_reference(env, argv[i], 1, &q->refs[q->n_refs++]);
+  }
+  napi_value promise, resource;
+  napi_create_promise(env, &q->deferred, &promise);
+  napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &resource);
+  if (napi_create_async_work(env, NULL, resource, async_execute, async_complete, q, &q->work) != napi_ok ||
+      napi_queue_async_work(env, q->work) != napi_ok) {
+    for (int i = 0; i < q->n_refs; i++) napi_delete_reference(env, q->refs[i]);
+    free(q);
+    return throw_msg(env, "could not queue the asynchronous query");
+  }
+  q->h->async_refs++;
+  return promise;
+}
+
+/* raycastFirstAsync(same arguments as raycastFirst) -> Promise (resolved once the outputs are written) */
+static napi_value js_raycast_first_async(napi_env env, napi_callback_info info) {
+  return run_async(env, info, 10, parse_raycast_first, "mbvh.raycastFirstAsync");
+}
+
+/* closestPointAsync(same arguments as closestPoint) -> Promise */
+static napi_value js_closest_point_async(napi_env env, napi_callback_info info) {
+  return run_async(env, info, 8, parse_closest_point, "mbvh.closestPointAsync");
 }
 
 /* raycast(mesh, origins, dirs, side, near, far) -> { offsets, faceIndex, distance, point, barycoord } */
@@ -881,38 +1039,6 @@ static napi_value js_raycast(napi_env env, napi_callback_info info) {
   napi_set_named_property(env, result, "point", pt);
   napi_set_named_property(env, result, "barycoord", bc);
   return result;
-}
-
-/* closestPoint(mesh, points, minT, maxT, outFace, outDist, outPoint, outBary); Float64Array points are
- * decided for the double points (mbvh_closest_point_f64). */
-static napi_value js_closest_point(napi_env env, napi_callback_info info) {
-  napi_value argv[8];
-  get_args(env, info, 8, argv);
-  MeshHandle* h = get_mesh(env, argv[0]);
-  if (!h) return throw_msg(env, "closestPoint: invalid mesh");
-  View q;
-  if (!get_view(env, argv[1], &q) || (q.type != napi_float32_array && q.type != napi_float64_array) || q.length % 3)
-    return throw_msg(env, "closestPoint: points must be a Float32Array or Float64Array of length 3n");
-  const int f64 = q.type == napi_float64_array;
-  const size_t n = q.length / 3;
-  double mn, mx;
-  if (!get_f64(env, argv[2], &mn) || !get_f64(env, argv[3], &mx))
-    return throw_msg(env, "closestPoint: thresholds must be numbers");
-  void *face, *dist, *pt, *bc;
-  if (!out_view(env, argv[4], napi_uint32_array, n, &face) || !face ||
-      !out_view(env, argv[5], napi_float32_array, n, &dist) || !out_view(env, argv[6], napi_float32_array, 3 * n, &pt) ||
-      !out_view(env, argv[7], napi_float32_array, 3 * n, &bc))
-    return throw_msg(env, "closestPoint: output arrays have the wrong type or length");
-  mbvh_ctx* ctx = NULL;
-  int rc = context_for(env, h->device, &ctx);
-  if (!rc && f64)
-    rc = mbvh_closest_point_f64(ctx, h->mesh, (const double*)q.data, n, mn, mx, (uint32_t*)face, (float*)dist,
-                                (float*)pt, (float*)bc);
-  else if (!rc)
-    rc = mbvh_closest_point(ctx, h->mesh, (const float*)q.data, n, to_float(mn), to_float(mx), (uint32_t*)face,
-                            (float*)dist, (float*)pt, (float*)bc);
-  if (rc) return throw_status(env, rc, f64 ? "mbvh_closest_point_f64" : "mbvh_closest_point");
-  return NULL;
 }
 
 /* meshInfo(mesh) -> { triCount, vertexCount, maxDepth } */
@@ -1117,6 +1243,8 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("setSideMask", js_set_side_mask);
   EXPORT("raycastFirst", js_raycast_first);
   EXPORT("closestPoint", js_closest_point);
+  EXPORT("raycastFirstAsync", js_raycast_first_async);
+  EXPORT("closestPointAsync", js_closest_point_async);
   EXPORT("raycast", js_raycast);
   EXPORT("raycastFirstBackward", js_raycast_first_bwd);
   EXPORT("batchCreate", js_batch_create);

@@ -295,6 +295,34 @@ class MeshBVH {
     return out;
   }
 
+  // The same two batch queries without blocking the event loop: the device call runs on a libuv
+  // worker thread and the Promise resolves to the same packed results. The input arrays must not
+  // change until it settles; the BVH cannot be disposed meanwhile (dispose() throws).
+  raycastFirstBatchAsync(origins, directions, options = {}) {
+    const [o, d] = queryArrays(origins, directions);
+    const n = o.length / FLOATS_PER_VEC;
+    const side = this._sideFor(options.side !== undefined ? options.side : (options.material !== undefined ? options.material : C.FrontSide));
+    const out = {
+      faceIndex: new Uint32Array(n), distance: new Float32Array(n),
+      point: new Float32Array(3 * n), barycoord: new Float32Array(3 * n),
+    };
+    return native.load().raycastFirstAsync(this._mesh(), o, d, side, options.near !== undefined ? options.near : 0,
+      options.far !== undefined ? options.far : Infinity, out.faceIndex, out.distance, out.point, out.barycoord)
+      .then(() => out);
+  }
+
+  closestPointToPointBatchAsync(points, options = {}) {
+    const q = queryArray(points);
+    const n = q.length / FLOATS_PER_VEC;
+    const out = {
+      faceIndex: new Uint32Array(n), distance: new Float32Array(n),
+      point: new Float32Array(3 * n), barycoord: new Float32Array(3 * n),
+    };
+    return native.load().closestPointAsync(this._mesh(), q, options.minThreshold || 0,
+      options.maxThreshold !== undefined ? options.maxThreshold : Infinity, out.faceIndex, out.distance, out.point,
+      out.barycoord).then(() => out);
+  }
+
   // intersectsSphere for n spheres: centers a Float32Array of length 3n, radii one of length n.
   // Returns a Uint8Array (1: some triangle within the radius, inclusive).
   // intersectsBox (src/core/MeshBVH.js:780-792): box = { min, max } in its own frame, boxToMesh its
