@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define MBVH_ABI_VERSION 2u
+#define MBVH_ABI_VERSION 3u
 #define MBVH_NO_HIT 0xFFFFFFFFu
 
 typedef enum mbvh_status {
@@ -574,15 +574,19 @@ int mbvh_scene_raycast_first_d(mbvh_ctx* ctx, const mbvh_scene* scene, const flo
  * by each instance's matrix, the least world distance wins — for n world points (3n floats). Triangles
  * and distances in double; equal distances go to the lowest instance, then the lowest face. Outputs
  * per point: face_index (in the instance's geometry), instance, world distance, world point and
- * barycoord (NULL outputs other than face_index are skipped). Host-pointer form (synchronous) and
- * device-pointer form (enqueues on `stream`; face 0xFFFFFFFE marks a traversal stack overflow).
+ * barycoord (NULL outputs other than face_index are skipped). Host-pointer form (synchronous; it
+ * orders the points spatially itself) and device-pointer form (enqueues on `stream`; `order`: NULL or
+ * a visit order of the n points such as mbvh_order_points_d writes, which lets the lanes of a wave
+ * share instances and tree paths; face 0xFFFFFFFE marks a traversal stack overflow). Points whose
+ * walks are long finish in a pass of one wave each, through a list the context allocates once: in
+ * calls of up to 2^20 points.
  */
 int mbvh_scene_closest_point(mbvh_ctx* ctx, const mbvh_scene* scene, const float* points, uint64_t n,
                              uint32_t* face_index, uint32_t* instance, float* distance, float* point,
                              float* barycoord);
 int mbvh_scene_closest_point_d(mbvh_ctx* ctx, const mbvh_scene* scene, const float* points, uint64_t n,
-                               uint32_t* face_index, uint32_t* instance, float* distance, float* point,
-                               float* barycoord, void* stream);
+                               const uint32_t* order, uint32_t* face_index, uint32_t* instance, float* distance,
+                               float* point, float* barycoord, void* stream);
 /* Host-pointer form: stages the inputs, launches, copies the results back and synchronises. */
 int mbvh_scene_raycast_first(mbvh_ctx* ctx, const mbvh_scene* scene, const float* origins, const float* directions,
                              uint64_t n, int32_t side, float near_dist, float far_dist, uint32_t* face_index,

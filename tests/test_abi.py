@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_version_and_device_count():
     lib = M.library()
-    assert lib.mbvh_abi_version() == 2
+    assert lib.mbvh_abi_version() == 3
     n = M.device_count()
     assert n >= 0
 

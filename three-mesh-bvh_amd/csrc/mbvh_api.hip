@@ -29,6 +29,10 @@ struct DeviceBuffer {
 
 }  // namespace
 
+// Capacity of the context's list of queries left to a wave each: a launch of more queries runs
+// without that pass.
+constexpr uint64_t kHeavyCap = uint64_t{1} << 20;
+
 struct mbvh_ctx {
   uint32_t magic = kCtxMagic;
   int device = 0;
@@ -43,7 +47,7 @@ struct mbvh_ctx {
   DeviceBuffer staging[8];      // host-API input / output staging
   DeviceBuffer order;           // host-API query order
   DeviceBuffer order_ws;        // host-API ordering workspace
-  DeviceBuffer heavy;           // scene closest point: points left to the wave-per-point pass
+  DeviceBuffer heavy;           // queries left to a wave each (kHeavyCap entries, allocated with the context)
   DeviceBuffer hits[5];         // host-API all-hits results: offsets, face, dist, point, bary
   DeviceBuffer grad[11];        // host-API backward staging
   DeviceBuffer grad_ws;         // host-API backward workspace
@@ -545,6 +549,9 @@ extern "C" int mbvh_ctx_create(int32_t device, mbvh_ctx** out_ctx) {
     delete c;
     return MBVH_ERR_OOM;
   }
+  // the list of queries left to a wave each (long walks), allocated once so that the device-pointer
+  // entry points allocate nothing; without it those passes are off
+  if (ensure_buffer(c->heavy, kHeavyCap * sizeof(uint32_t))) c->heavy = DeviceBuffer{};
   // the pinned block of small calls (without it, small calls take the staging path)
   if (hipHostMalloc(&c->small_host, kSmallBytes, hipHostMallocMapped) == hipSuccess) {
     if (hipHostGetDevicePointer(&c->small_dev, c->small_host, 0) != hipSuccess) {
@@ -3197,8 +3204,8 @@ extern "C" int mbvh_scene_raycast_first(mbvh_ctx* ctx, const mbvh_scene* scene, 
 // ---- scenes: closest point over instances -------------------------------------------------------------
 namespace {
 
-int enqueue_scene_points(mbvh_ctx* ctx, const mbvh_scene* sc, const float* points, uint64_t n, uint32_t* face,
-                         uint32_t* inst, float* dist, float* point, float* bary, hipStream_t s) {
+int enqueue_scene_points(mbvh_ctx* ctx, const mbvh_scene* sc, const float* points, uint64_t n, const uint32_t* order,
+                         uint32_t* face, uint32_t* inst, float* dist, float* point, float* bary, hipStream_t s) {
   const bool quads = ctx->point_lanes != 1;  // four lanes per point unless one was asked for
   if (quads ? sc->max_stack_points4 > static_cast<uint32_t>(kQuadLdsStack) + ctx->stack_ovf_entries
             : sc->max_stack_points > 8 + ctx->stack_ovf_entries)
@@ -3220,24 +3227,14 @@ int enqueue_scene_points(mbvh_ctx* ctx, const mbvh_scene* sc, const float* point
   a.out_dist = dist;
   a.out_point = point;
   a.out_bary = bary;
-  // spatial order: lanes of a wave then share instances and tree paths
-  if (n >= kOrderMin && n < (uint64_t{1} << 32)) {
-    int rc;
-    if ((rc = ensure_buffer(ctx->order, n * sizeof(uint32_t))) ||
-        (rc = ensure_buffer(ctx->order_ws, order_workspace_bytes(n))))
-      return rc;
-    MBVH_HIP(launch_order_points(points, n, static_cast<uint32_t*>(ctx->order.ptr), ctx->order_ws.ptr, s));
-    a.order = static_cast<const uint32_t*>(ctx->order.ptr);
-  }
+  a.order = order;  // a spatial order lets the lanes of a wave share instances and tree paths
   if (quads) {
     const uint64_t threads = launch_threads(4 * n, ctx->max_threads);
     a.stack = stack_of(ctx, threads / 4);
     a.queue = static_cast<uint32_t*>(ctx->queue.ptr);
     // points whose walks run past kSceneHeavyBudget leaf steps go to one wave each (their walks can
     // be thousands of steps, which one quad would take alone while the launch waits for it)
-    if (sc->wide_ok && n < (uint64_t{1} << 32)) {
-      int rc;
-      if ((rc = ensure_buffer(ctx->heavy, n * sizeof(uint32_t)))) return rc;
+    if (sc->wide_ok && ctx->heavy.ptr && n <= kHeavyCap) {
       a.heavy = static_cast<uint32_t*>(ctx->heavy.ptr);
       a.heavy_count = static_cast<uint32_t*>(ctx->queue.ptr) + 32;  // its own line of the counter block
       a.heavy_budget = kSceneHeavyBudget;
@@ -3255,14 +3252,14 @@ int enqueue_scene_points(mbvh_ctx* ctx, const mbvh_scene* sc, const float* point
 }  // namespace
 
 extern "C" int mbvh_scene_closest_point_d(mbvh_ctx* ctx, const mbvh_scene* scene, const float* points, uint64_t n,
-                                          uint32_t* face_index, uint32_t* instance, float* distance, float* point,
-                                          float* barycoord, void* stream) {
+                                          const uint32_t* order, uint32_t* face_index, uint32_t* instance,
+                                          float* distance, float* point, float* barycoord, void* stream) {
   clear_error();
   const char* fn = "mbvh_scene_closest_point_d";
   int rc = check_scene_call(fn, ctx, scene, MBVH_FRONT_SIDE, n, points, points, face_index);
   if (rc) return rc;
   if ((rc = set_device(ctx))) return rc;
-  return enqueue_scene_points(ctx, scene, points, n, face_index, instance, distance, point, barycoord,
+  return enqueue_scene_points(ctx, scene, points, n, order, face_index, instance, distance, point, barycoord,
                               stream ? static_cast<hipStream_t>(stream) : ctx->stream);
 }
 
@@ -3283,7 +3280,9 @@ extern "C" int mbvh_scene_closest_point(mbvh_ctx* ctx, const mbvh_scene* scene, 
       (rc = stage_out(ctx->staging[3], instance, s1, &dI)) || (rc = stage_out(ctx->staging[4], distance, s1, &dT)) ||
       (rc = stage_out(ctx->staging[5], point, v3, &dP)) || (rc = stage_out(ctx->staging[6], barycoord, v3, &dB)))
     return rc;
-  if ((rc = enqueue_scene_points(ctx, scene, static_cast<const float*>(dQ), n, static_cast<uint32_t*>(dF),
+  const uint32_t* order = nullptr;
+  if ((rc = host_order(ctx, static_cast<const float*>(dQ), n, &order))) return rc;
+  if ((rc = enqueue_scene_points(ctx, scene, static_cast<const float*>(dQ), n, order, static_cast<uint32_t*>(dF),
                                  static_cast<uint32_t*>(dI), static_cast<float*>(dT), static_cast<float*>(dP),
                                  static_cast<float*>(dB), s)))
     return rc;

@@ -144,7 +144,7 @@ def library() -> C.CDLL:
         "mbvh_scene_raycast_first_d": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp, vp]),
         "mbvh_scene_raycast_first": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp]),
         "mbvh_scene_closest_point": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp]),
-        "mbvh_scene_closest_point_d": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp, vp]),
+        "mbvh_scene_closest_point_d": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp]),
         "mbvh_intersects_box": (C.c_int, [vp, vp, vp, vp, vp, u64, vp]),
         "mbvh_intersects_box_d": (C.c_int, [vp, vp, vp, vp, vp, u64, vp, vp]),
         "mbvh_closest_point_to_geometry": (C.c_int, [vp, vp, vp, u64, u32, u32, vp, u32, u64, vp, f64, f64, vp]),
@@ -891,11 +891,12 @@ class Scene:
         return out
 
     def closest_point_d(self, points_ptr: int, n: int, face_ptr: int, instance_ptr: int = 0, dist_ptr: int = 0,
-                        point_ptr: int = 0, bary_ptr: int = 0, stream: int = 0) -> None:
+                        point_ptr: int = 0, bary_ptr: int = 0, stream: int = 0, order_ptr: int = 0) -> None:
+        """Device form; order_ptr: an optional visit order (n uint32, e.g. Context.order_points_d)."""
         v = lambda x: C.c_void_p(x or None)
         _check(self.ctx._lib.mbvh_scene_closest_point_d(
-            self.ctx.handle, self.handle, v(points_ptr), n, v(face_ptr), v(instance_ptr), v(dist_ptr), v(point_ptr),
-            v(bary_ptr), v(stream)))
+            self.ctx.handle, self.handle, v(points_ptr), n, v(order_ptr), v(face_ptr), v(instance_ptr), v(dist_ptr),
+            v(point_ptr), v(bary_ptr), v(stream)))
 
     def raycast_first_d(self, origins_ptr: int, dirs_ptr: int, n: int, face_ptr: int, instance_ptr: int = 0,
                         dist_ptr: int = 0, point_ptr: int = 0, bary_ptr: int = 0, side: int = FRONT_SIDE,
