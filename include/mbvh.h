@@ -81,6 +81,11 @@ typedef struct mbvh_build_opts {
   int32_t max_depth;        /* default 40 (Constants.js:76) */
   int32_t target_leaf_size; /* default 10 (Constants.js:77); `maxLeafTris` alias maps here */
   int32_t indirect;         /* 0: permute `index` in place; 1: build an indirect buffer */
+  int32_t strict_leaf_size; /* 0: none; else no leaf holds more primitives (unless max_depth stops it):
+                               an over-full node is split even against the heuristic, at the median
+                               of its order on its box's longest axis when the heuristic finds no
+                               split (the _strictLeafSize of src/core/build/buildTree.js:59-94, which
+                               ClusteredBVH sets to 1); host build only */
 } mbvh_build_opts;
 
 /*
@@ -587,6 +592,28 @@ int mbvh_scene_closest_point(mbvh_ctx* ctx, const mbvh_scene* scene, const float
 int mbvh_scene_closest_point_d(mbvh_ctx* ctx, const mbvh_scene* scene, const float* points, uint64_t n,
                                const uint32_t* order, uint32_t* face_index, uint32_t* instance, float* distance,
                                float* point, float* barycoord, void* stream);
+/*
+ * The scene in the reference's WebGPU packing (BVHComputeData.update, src/webgpu/BVHComputeData.js:217-447,
+ * with every instance referenced whole as ClusteredBVH references instanced objects,
+ * src/webgpu/ClusteredBVH.js:219-229): `nodes` (8 words per BVHNode, src/webgpu/tsl/structs.js:9-14): a
+ * top level with one (instance, root) primitive per leaf, each leaf holding its subtree's absolute
+ * node offset and 0xFF000000 | its transform slot (appendBVHData,
+ * src/webgpu/utils/packBVHBufferUtils.js:44-131), then each referenced (mesh, root) subtree once
+ * (appendBVHSubtree, :216-280, leaf offsets rebased by the mesh's triangle base); `transforms` (36 words
+ * per slot: matrixWorld, its inverse, visible, 3 padding words; writeTransformData,
+ * BVHComputeData.js:491-526); `index` (3 words per triangle: each mesh's triangles in slot order,
+ * rebased by the mesh's vertex base, meshes in the order the top level first names them; an
+ * attribute buffer packed in that mesh order, each mesh's vertices whole, matches it). Instances whose
+ * roots hold no triangles are left out; invisible instances stay, with visible 0. The top level is
+ * this library's SAH build over the primitives' world boxes. Fails with MBVH_ERR_LIMIT when two
+ * primitives with identical boxes would share a leaf. Host arrays; sizes in words from
+ * mbvh_scene_webgpu_sizes; NULL outputs are skipped.
+ */
+int mbvh_scene_webgpu_sizes(mbvh_ctx* ctx, const mbvh_scene* scene, uint64_t* n_nodes, uint64_t* n_transforms,
+                            uint64_t* n_index);
+int mbvh_scene_export_webgpu(mbvh_ctx* ctx, const mbvh_scene* scene, uint32_t* nodes, uint64_t nodes_capacity,
+                             uint32_t* transforms, uint64_t transforms_capacity, uint32_t* index,
+                             uint64_t index_capacity);
 /* Host-pointer form: stages the inputs, launches, copies the results back and synchronises. */
 int mbvh_scene_raycast_first(mbvh_ctx* ctx, const mbvh_scene* scene, const float* origins, const float* directions,
                              uint64_t n, int32_t side, float near_dist, float far_dist, uint32_t* face_index,

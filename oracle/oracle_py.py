@@ -447,3 +447,117 @@ def scene_closest_point(meshes, world, geometry, visible, points):
         dist[i] = np.sqrt(d2.value)
         pts[i] = p
     return {"instance": inst, "face": face, "distance": dist, "point": pts}
+
+
+def _world_box(e, lb):
+    """A local box through an affine column-major matrix: the 8 corners in double, rounded outward
+    to float32 and widened by 2^-20 of their magnitude (the scene's top-level boxes)."""
+    e = np.asarray(e, np.float64)
+    corners = np.array([[lb[3 if c & 1 else 0], lb[4 if c & 2 else 1], lb[5 if c & 4 else 2]] for c in range(8)])
+    out = []
+    for c in corners:
+        w = e[3] * c[0] + e[7] * c[1] + e[11] * c[2] + e[15]
+        out.append([(e[k] * c[0] + e[4 + k] * c[1] + e[8 + k] * c[2] + e[12 + k]) / w for k in range(3)])
+    out = np.array(out)
+    mn, mx = out.min(axis=0), out.max(axis=0)
+    pad = np.maximum(np.maximum(np.abs(mn), np.abs(mx)) * 9.5367431640625e-07, 1e-30)
+    lo = np.nextafter((mn - pad).astype(np.float32), np.float32(-np.inf))
+    hi = np.nextafter((mx + pad).astype(np.float32), np.float32(np.inf))
+    return np.concatenate([lo, hi]).astype(np.float32)
+
+
+def webgpu_scene(roots, slot_vertices, vertex_counts, world, geometry, visible, build):
+    """BVHComputeData.update's packing (src/webgpu/BVHComputeData.js:217-447) of instances referenced
+    whole: one (instance, root) primitive per top-level leaf, the top level built by `build` (positions,
+    index -> root buffer, permuted index) over the primitives' world boxes; appendBVHData
+    (src/webgpu/utils/packBVHBufferUtils.js:44-131) for the top level, appendBVHSubtree (:216-280) for
+    each (mesh, root) the leaves name (first-named first), transforms per slot (matrixWorld, its inverse,
+    visible, padding; BVHComputeData.js:491-526) and the index per mesh rebased by its vertex base
+    (appendIndexData, :293-322). roots[g]: the mesh's root buffers (uint8); slot_vertices[g]: its
+    triangles' vertex ids in slot order (3 per slot). Returns nodes (N, 8) uint32, transforms (S, 36)
+    uint32 with the inverse as float32 from numpy's double inverse, index uint32."""
+    prims, pos = [], []
+    for i, g in enumerate(geometry):
+        for r, rb in enumerate(roots[g]):
+            b = rb.view(np.float32)[:6].astype(np.float64)
+            if not (b[0] <= b[3] and b[1] <= b[4] and b[2] <= b[5]):
+                continue
+            wb = _world_box(world[i], b)
+            prims.append((i, r))
+            pos.append(np.concatenate([wb, wb[:3]]))
+    np_ = len(prims)
+    if np_ == 0:
+        tl = np.zeros(8, np.uint32)
+        tl.view(np.float32)[:3] = np.inf
+        tl.view(np.float32)[3:6] = -np.inf
+        tl[7] = 0xFFFF0000
+        tlas, order = tl.reshape(1, 8), []
+    else:
+        pos = np.array(pos, np.float32).reshape(-1, 3)
+        tb, idx = build(pos, np.arange(3 * np_, dtype=np.uint32))
+        tlas = tb.view(np.uint32).reshape(-1, 8)
+        order = [int(idx[3 * k]) // 3 for k in range(np_)]
+    n_tlas = tlas.shape[0]
+    sub, tri_base, vert_base, mesh_order, sub_order = {}, {}, {}, [], []
+    nxt, ntri, nvert = n_tlas, 0, 0
+    leaf_base = []
+    for k in range(np_):
+        i, r = prims[order[k]]
+        g = int(geometry[i])
+        if g not in tri_base:
+            tri_base[g], vert_base[g] = ntri, nvert
+            ntri += len(slot_vertices[g]) // 3
+            nvert += int(vertex_counts[g])
+            mesh_order.append(g)
+        if (g, r) not in sub:
+            sub[(g, r)] = nxt
+            nxt += roots[g][r].size // 32
+            sub_order.append((g, r))
+        leaf_base.append(sub[(g, r)])
+    nodes = np.zeros((nxt, 8), np.uint32)
+
+    def fix_root(dst, src):
+        f = src.view(np.float32)
+        df = dst.view(np.float32)
+        for c in range(3):
+            if f[c] > f[c + 3]:
+                df[c], df[c + 3] = 1.0, -1.0
+
+    for k in range(n_tlas):
+        src = tlas[k]
+        nodes[k, :6] = src[:6]
+        if k == 0:
+            fix_root(nodes[k], src)
+        if (int(src[7]) >> 16) == 0xFFFF:
+            cnt = int(src[7]) & 0xFFFF
+            slot = 0 if cnt == 0 else int(src[6])
+            nodes[k, 6] = 0 if cnt == 0 else leaf_base[slot]
+            nodes[k, 7] = 0xFF000000 | (slot & 0x00FFFFFF)
+        else:
+            nodes[k, 6:8] = src[6:8]
+    for g, r in sub_order:
+        src = roots[g][r].view(np.uint32).reshape(-1, 8)
+        base = sub[(g, r)]
+        for k in range(src.shape[0]):
+            nodes[base + k, :6] = src[k, :6]
+            if k == 0:
+                fix_root(nodes[base + k], src[k])
+            if (int(src[k, 7]) >> 16) == 0xFFFF:
+                nodes[base + k, 6] = int(src[k, 6]) + tri_base[g]
+                nodes[base + k, 7] = 0xFFFF0000 | (int(src[k, 7]) & 0xFFFF)
+            else:
+                nodes[base + k, 6:8] = src[k, 6:8]
+    tr = np.zeros((max(np_, 1), 36), np.uint32)
+    for k in range(np_):
+        i = prims[order[k]][0]
+        w = np.asarray(world[i], np.float32)
+        tr[k, :16] = w.view(np.uint32)
+        inv = np.linalg.inv(w.astype(np.float64).reshape(4, 4).T).T.reshape(-1).astype(np.float32)
+        tr[k, 16:32] = inv.view(np.uint32)
+        tr[k, 32] = 1 if (visible is None or visible[i]) else 0
+    index = np.zeros(3 * ntri, np.uint32)
+    for g in mesh_order:
+        v = np.asarray(slot_vertices[g], np.uint32)
+        index[3 * tri_base[g]:3 * tri_base[g] + v.size] = v + vert_base[g]
+    return {"nodes": nodes, "transforms": tr, "index": index, "order": [prims[o] for o in order],
+            "mesh_order": mesh_order, "vert_base": vert_base}

@@ -248,6 +248,21 @@ const batchedChecked = bface.filter(f => f !== 0xFFFFFFFF).length, presenceDiffe
 // face and distance up to that rounding (a ray grazing an edge may land on the other side of it).
 {
   const scene = api.SceneBVH.fromBatchedMesh(batched);
+  {  // the scene in the reference's WebGPU packing: tagged top-level leaves inside the buffer
+    const pk = scene.exportWebGPU();
+    const nNodes = pk.nodes.length / 8, nSlots = pk.transforms.length / 36;
+    let topLeaves = 0;
+    for (let k = 0; k < nNodes; k++) {
+      const w7 = pk.nodes[8 * k + 7];
+      if ((w7 >>> 24) === 0xFF && (w7 >>> 16) !== 0xFFFF) {
+        topLeaves++;
+        assert.ok((w7 & 0xFFFFFF) < nSlots && pk.nodes[8 * k + 6] < nNodes, `top-level leaf ${k}`);
+      }
+    }
+    assert.ok(topLeaves > 0 && topLeaves === nSlots, 'one slot per top-level leaf');
+    for (let t = 0; t < nSlots; t++) assert.ok(pk.transforms[36 * t + 32] <= 1, 'visible flag');
+    assert.ok(pk.index.length > 0 && pk.index.length % 3 === 0, 'packed index');
+  }
   const nS = Math.min(K, 128);
   const many = scene.raycastFirstBatch(origins.subarray(0, 3 * nS), dirs.subarray(0, 3 * nS));
   let sceneHits = 0, sceneSame = 0;

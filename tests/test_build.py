@@ -164,3 +164,34 @@ def test_validate_bounds_catches_planted_violations():
     h[leaf, 14] -= 1  # the leaf's last slot is no longer covered
     rc, st = O.OracleMesh(g.positions, idx, [buf]).validate_root(0)
     assert rc != 0 and st["structure_failures"] > 0
+
+
+def _leaves(buf):
+    u = buf.view(np.uint32).reshape(-1, 8)
+    h = buf.view(np.uint16).reshape(-1, 16)
+    return [(i, int(u[i, 6]), int(h[i, 14])) for i in range(u.shape[0]) if h[i, 15] == 0xFFFF]
+
+
+def test_strict_leaf_size_splits_against_the_heuristic():
+    """_strictLeafSize (src/core/build/buildTree.js:59-94): three identical triangles (no usable SAH
+    split, and under the target leaf size of 10) are split anyway, at the median of their order
+    (offset + max(1, floor(count / 2))) on the box's longest axis: root -> (1, 2), then (1, 1)."""
+    tri = np.array([[0, 0, 0], [4, 0, 0], [0, 1, 0]], np.float32)  # longest edge: x
+    pos = np.concatenate([tri, tri, tri]).astype(np.float32)
+    idx = np.arange(9, dtype=np.uint32)
+    plain = M.build(pos, idx.copy(), strategy=M.SAH, target_leaf_size=10)
+    assert _leaves(plain.roots[0]) == [(0, 0, 3)]  # one leaf without the strict size
+    strict = M.build(pos, idx.copy(), strategy=M.SAH, target_leaf_size=10, strict_leaf_size=1)
+    u = strict.roots[0].view(np.uint32).reshape(-1, 8)
+    # root (axis 0): left leaf = slot 0; right = node 2 (axis 0) with leaves slots 1 and 2
+    assert u.shape[0] == 5
+    assert int(u[0, 7]) == 0 and int(u[0, 6]) == 2 and int(u[2, 7]) == 0
+    assert _leaves(strict.roots[0]) == [(1, 0, 1), (3, 1, 1), (4, 2, 1)]
+    # overlapping boxes with a usable-but-costly split and many primitives: every leaf holds one
+    rng = np.random.default_rng(3)
+    c = rng.normal(0, 0.05, (40, 3))
+    boxes = np.concatenate([c - 1, c + 1, c - 1], axis=1).reshape(-1, 3).astype(np.float32)
+    t = M.build(boxes, np.arange(120, dtype=np.uint32), strategy=M.SAH, target_leaf_size=1, strict_leaf_size=1)
+    assert all(cnt == 1 for _, _, cnt in _leaves(t.roots[0]))
+    loose = M.build(boxes, np.arange(120, dtype=np.uint32), strategy=M.SAH, target_leaf_size=1)
+    assert any(cnt > 1 for _, _, cnt in _leaves(loose.roots[0]))  # the heuristic alone keeps some together

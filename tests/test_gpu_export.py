@@ -157,3 +157,140 @@ def test_import_rejects_bad_buffers(ctx):
         ctx.import_nodes(bad, g.positions, idx)
     with pytest.raises(M.MbvhError):  # leaf offsets below the geometry offset
         ctx.import_nodes(words, g.positions, idx, geometry_offset=7)
+
+
+# ---- a scene in the WebGPU packing (BVHComputeData.update with instances referenced whole) ----------
+
+def _packed_first_hits(pk, positions, o, d):
+    """The merged top-level / instance loop of the reference's WebGPU shapecast
+    (src/webgpu/shapecastFns/getShapecastFn.js:97-217) with the first-hit tests, in float64 over the
+    packed buffers: a top-level leaf (0xFF top byte) names its transform slot and subtree; the ray goes
+    into the instance frame by the slot's inverse and the subtree's triangles come from the packed
+    index. Returns the least world distance and the slot that reached it (double-sided)."""
+    nodes, tr, index = pk["nodes"], pk["transforms"], pk["index"]
+    nf = nodes.view(np.float32).astype(np.float64)
+
+    def enters(n, oo, dd):
+        with np.errstate(divide="ignore", invalid="ignore"):
+            inv = 1.0 / dd
+            t0, t1 = (nf[n, 0:3] - oo) * inv, (nf[n, 3:6] - oo) * inv
+        tmin, tmax = np.nanmax(np.minimum(t0, t1)), np.nanmin(np.maximum(t0, t1))
+        return tmin <= tmax and tmax >= 0
+
+    def tri_t(a, b, c, oo, dd):
+        e1, e2 = b - a, c - a
+        p = np.cross(dd, e2)
+        det = e1 @ p
+        if abs(det) < 1e-300:
+            return np.inf
+        tv = oo - a
+        u = (tv @ p) / det
+        q = np.cross(tv, e1)
+        v = (dd @ q) / det
+        t = (e2 @ q) / det
+        return t if (u >= 0 and v >= 0 and u + v <= 1 and t >= 0) else np.inf
+
+    best, best_slot = np.inf, -1
+    stack = [0]
+    while stack:
+        n = stack.pop()
+        if not enters(n, o, d):
+            continue
+        w7 = int(nodes[n, 7])
+        if (w7 >> 24) == 0xFF:  # top-level leaf
+            slot = w7 & 0xFFFFFF
+            W = tr[slot, :16].view(np.float32).astype(np.float64).reshape(4, 4).T
+            I = tr[slot, 16:32].view(np.float32).astype(np.float64).reshape(4, 4).T
+            lo = I[:3, :3] @ o + I[:3, 3]
+            ld = I[:3, :3] @ d
+            sub = [int(nodes[n, 6])]
+            while sub:
+                m = sub.pop()
+                if not enters(m, lo, ld):
+                    continue
+                x7 = int(nodes[m, 7])
+                if (x7 >> 16) == 0xFFFF:
+                    off, cnt = int(nodes[m, 6]), x7 & 0xFFFF
+                    for t_ in range(off, off + cnt):
+                        a, b, c = (positions[index[3 * t_ + j]].astype(np.float64) for j in range(3))
+                        t = tri_t(a, b, c, lo, ld)
+                        if np.isfinite(t):
+                            wp = W[:3, :3] @ (lo + t * ld) + W[:3, 3]
+                            dist = np.linalg.norm(wp - o)
+                            if dist < best:
+                                best, best_slot = dist, slot
+                else:
+                    sub += [m + int(nodes[m, 6]), m + 1]
+        else:
+            stack += [n + int(nodes[n, 6]), n + 1]
+    return best, best_slot
+
+
+def test_scene_webgpu_packing(ctx):
+    """mbvh_scene_export_webgpu == the oracle's restatement of the packing (top level with one
+    (instance, root) per leaf, tagged leaves, each (mesh, root) subtree once with rebased leaves,
+    transforms, rebased index) for plain, two-root and indirect meshes shared by several instances, one
+    of them invisible; and the merged loop over the packed buffers finds the scene raycast's hits."""
+    geo = G.torus_knot(1.0, 0.3, 60, 10)
+    geo.ensure_index()
+    ntri = geo.index.size // 3
+    built = [build_mesh(G.sphere(1.0, 24, 12)), build_mesh(geo, ranges=[(0, ntri // 2), (ntri // 2, ntri - ntri // 2)]),
+             build_mesh(G.torus(1.0, 0.3, 12, 30), indirect=True)]
+    ms = [ctx.upload(t, g.positions, idx) for g, idx, t in built]
+    rng = np.random.default_rng(71)
+    geometry = np.array([0, 1, 2, 1, 0, 2, 1], np.uint32)
+    mats = []
+    for k in range(len(geometry)):
+        m = np.eye(4)
+        a = rng.normal(size=3)
+        c, s_ = np.cos(0.7 * k), np.sin(0.7 * k)
+        m[:3, :3] = np.array([[c, -s_, 0], [s_, c, 0], [0, 0, 1]]) * rng.uniform(0.6, 1.5)
+        m[:3, 3] = a * 4
+        mats.append(m)
+    world = np.stack([m.T.reshape(-1) for m in mats]).astype(np.float32)
+    inverse = np.stack([np.linalg.inv(m).T.reshape(-1) for m in mats]).astype(np.float32)
+    visible = np.ones(len(geometry), np.uint8)
+    visible[4] = 0
+    sc = ctx.scene(ms, world, inverse, geometry, visible)
+    got = sc.export_webgpu()
+    slot_vertices = [_index_triplets(g, idx, t).reshape(-1) for g, idx, t in built]
+
+    def build(pos, idx):
+        tree = M.build(pos, idx, strategy=M.SAH, target_leaf_size=1, max_depth=40, strict_leaf_size=1)
+        return tree.roots[0], idx
+
+    ref = O.webgpu_scene([t.roots for _, _, t in built], slot_vertices, [g.positions.size // 3 for g, _, _ in built],
+                         world, geometry, visible, build)
+    assert np.array_equal(got["nodes"], ref["nodes"])
+    assert np.array_equal(got["index"], ref["index"])
+    assert np.array_equal(got["transforms"][:, :16], ref["transforms"][:, :16])
+    assert np.array_equal(got["transforms"][:, 32:], ref["transforms"][:, 32:])  # visible flag, padding
+    gi = got["transforms"][:, 16:32].view(np.float32)
+    ri = ref["transforms"][:, 16:32].view(np.float32)
+    assert np.all(np.abs(gi - ri) <= 2 * np.spacing(np.maximum(np.abs(gi), np.abs(ri))))  # inverses to an ulp
+    assert sum(int(r.size // 32) for _, _, t in built for r in t.roots) < got["nodes"].shape[0]  # top level first
+    # the packed buffers are usable: the merged loop finds the scene raycast's hits (visible instances)
+    positions = np.concatenate([built[g][0].positions.reshape(-1, 3) for g in ref["mesh_order"]])
+    sc_vis = ctx.scene(ms, world, inverse, geometry)  # the loop below ignores the visible flag
+    n = 200
+    o = rng.normal(0, 12, (n, 3)).astype(np.float32)
+    tgt = world[rng.integers(0, len(geometry), n), 12:15] + rng.normal(0, 0.8, (n, 3)).astype(np.float32)
+    d = (tgt - o) / np.linalg.norm(tgt - o, axis=1, keepdims=True)
+    d = d.astype(np.float32)
+    res = sc_vis.raycast_first(o, d, side=M.DOUBLE_SIDE)
+    order = ref["order"]
+    hits = 0
+    for i in range(n):
+        dist, slot = _packed_first_hits(ref, positions, o[i].astype(np.float64), d[i].astype(np.float64))
+        if res["face"][i] == M.NO_HIT:
+            assert not np.isfinite(dist), i
+            continue
+        hits += 1
+        assert abs(dist - res["distance"][i]) <= 1e-5 * max(dist, 1.0), (i, dist, res["distance"][i])
+        inst = order[slot][0]
+        assert inst == res["instance"][i] or abs(dist - res["distance"][i]) <= 1e-6 * max(dist, 1.0), i
+    assert hits > n // 4
+    sc_vis.free()
+    sc.free()
+    for m in ms:
+        m.free()

@@ -2758,6 +2758,8 @@ extern "C" int mbvh_build_gpu(mbvh_ctx* ctx, const float* positions, uint64_t ve
   *out_tree = nullptr;
   int rc = check_build_args(fn, positions, vertex_count, position_stride, position_offset, index, index_elt_bytes,
                             index_count, root_ranges, n_roots, opts, false);
+  if (!rc && opts->strict_leaf_size != 0)
+    rc = set_error(MBVH_ERR_BAD_ARGS, "%s: strict_leaf_size is a host build option", fn);
   BuildPlan plan;
   if (!rc) rc = plan_build(fn, root_ranges, n_roots, opts, index_elt_bytes, &plan);
   if (rc) return rc;
@@ -2825,6 +2827,8 @@ extern "C" int mbvh_build_gpu_d(mbvh_ctx* ctx, const float* positions, uint64_t 
   if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid context handle", fn);
   int rc = check_build_args(fn, positions, vertex_count, position_stride, position_offset, index, index_elt_bytes,
                             index_count, root_ranges, n_roots, opts, false);
+  if (!rc && opts->strict_leaf_size != 0)
+    rc = set_error(MBVH_ERR_BAD_ARGS, "%s: strict_leaf_size is a host build option", fn);
   BuildPlan plan;
   if (!rc) rc = plan_build(fn, root_ranges, n_roots, opts, index_elt_bytes, &plan);
   if (rc) return rc;
@@ -2885,6 +2889,11 @@ struct mbvh_scene {
   uint32_t max_stack_points = 0;  // closest point: the top level's far children below them
   uint32_t max_stack_points4 = 0;  // the same for the four-lane kernel (4-wide instance trees)
   bool wide_ok = false;            // every instance tree's 4-wide walk fits the wave-per-point frontier
+  // host copies for the WebGPU packing
+  std::vector<const mbvh_mesh*> meshes;
+  std::vector<float> world_h, frame_h;  // 16 per instance: matrixWorld, the library's own inverse
+  std::vector<uint32_t> geom_h;
+  std::vector<uint8_t> visible_h;
 };
 
 namespace {
@@ -2937,6 +2946,32 @@ void free_scene_buffers(mbvh_scene* sc) {
 }
 
 // Union of a mesh's root boxes (the device copy of its reference nodes).
+// World box of a local box (min xyz, max xyz in double) under an affine matrixWorld: the 8 corners
+// through the matrix in double, rounded outward to float and widened by 2^-20 of their magnitude, so
+// that no hit rounds outside it. False if it is not finite.
+bool world_box_of(const float* e, const double* lb, float* fb) {
+  double mn[3] = {std::numeric_limits<double>::infinity(), std::numeric_limits<double>::infinity(),
+                  std::numeric_limits<double>::infinity()};
+  double mx[3] = {-mn[0], -mn[1], -mn[2]};
+  for (int c = 0; c < 8; c++) {
+    const double x = lb[(c & 1) ? 3 : 0], y = lb[(c & 2) ? 4 : 1], z = lb[(c & 4) ? 5 : 2];
+    const double w = e[3] * x + e[7] * y + e[11] * z + e[15];
+    for (int k = 0; k < 3; k++) {
+      const double v = (e[k] * x + e[4 + k] * y + e[8 + k] * z + e[12 + k]) / w;
+      mn[k] = std::min(mn[k], v);
+      mx[k] = std::max(mx[k], v);
+    }
+  }
+  bool finite = true;
+  for (int k = 0; k < 3; k++) {
+    const double pad = std::max(std::max(std::fabs(mn[k]), std::fabs(mx[k])) * 9.5367431640625e-07, 1e-30);
+    fb[k] = std::nextafter(static_cast<float>(mn[k] - pad), -std::numeric_limits<float>::infinity());
+    fb[k + 3] = std::nextafter(static_cast<float>(mx[k] + pad), std::numeric_limits<float>::infinity());
+    finite = finite && std::isfinite(mn[k]) && std::isfinite(mx[k]);
+  }
+  return finite;
+}
+
 int mesh_local_box(const mbvh_mesh* m, double* box) {
   for (int k = 0; k < 3; k++) {
     box[k] = std::numeric_limits<double>::infinity();
@@ -3008,28 +3043,9 @@ extern "C" int mbvh_scene_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, 
     if (visible && !visible[i]) continue;
     const double* lb = &local[6ull * instance_geometry[i]];
     if (!(lb[0] <= lb[3] && lb[1] <= lb[4] && lb[2] <= lb[5])) continue;  // no triangles
-    const float* e = matrix_world + 16ull * i;
-    double mn[3] = {std::numeric_limits<double>::infinity(), std::numeric_limits<double>::infinity(),
-                    std::numeric_limits<double>::infinity()};
-    double mx[3] = {-mn[0], -mn[1], -mn[2]};
-    for (int c = 0; c < 8; c++) {
-      const double x = lb[(c & 1) ? 3 : 0], y = lb[(c & 2) ? 4 : 1], z = lb[(c & 4) ? 5 : 2];
-      const double w = e[3] * x + e[7] * y + e[11] * z + e[15];
-      for (int k = 0; k < 3; k++) {
-        const double v = (e[k] * x + e[4 + k] * y + e[8 + k] * z + e[12 + k]) / w;
-        mn[k] = std::min(mn[k], v);
-        mx[k] = std::max(mx[k], v);
-      }
-    }
-    bool finite = true;
     float fb[6];
-    for (int k = 0; k < 3; k++) {
-      const double pad = std::max(std::max(std::fabs(mn[k]), std::fabs(mx[k])) * 9.5367431640625e-07, 1e-30);
-      fb[k] = std::nextafter(static_cast<float>(mn[k] - pad), -std::numeric_limits<float>::infinity());
-      fb[k + 3] = std::nextafter(static_cast<float>(mx[k] + pad), std::numeric_limits<float>::infinity());
-      finite = finite && std::isfinite(mn[k]) && std::isfinite(mx[k]);
-    }
-    if (!finite) return set_error(MBVH_ERR_BAD_ARGS, "%s: instance %u has a non-finite world box", fn, i);
+    if (!world_box_of(matrix_world + 16ull * i, lb, fb))
+      return set_error(MBVH_ERR_BAD_ARGS, "%s: instance %u has a non-finite world box", fn, i);
     listed.push_back(i);
     const float v[9] = {fb[0], fb[1], fb[2], fb[3], fb[4], fb[5], fb[0], fb[1], fb[2]};
     prim_pos.insert(prim_pos.end(), v, v + 9);
@@ -3070,6 +3086,13 @@ extern "C" int mbvh_scene_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, 
   sc->max_stack_points = max_stack_points;
   sc->max_stack_points4 = max_stack_points4;
   sc->wide_ok = 64ull * (max_stack4 > 0 ? max_stack4 - 1 : 0) + 256 <= kWideFrontier;
+  sc->meshes.assign(meshes, meshes + n_meshes);
+  sc->world_h.assign(matrix_world, matrix_world + 16ull * n_instances);
+  sc->frame_h.assign(frame.begin(), frame.begin() + 16ull * n_instances);
+  sc->geom_h.assign(instance_geometry, instance_geometry + n_instances);
+  sc->visible_h.assign(n_instances, 1);
+  if (visible)
+    for (uint32_t i = 0; i < n_instances; i++) sc->visible_h[i] = visible[i] ? 1 : 0;
   const size_t mat_bytes = std::max<size_t>(16ull * n_instances * sizeof(float), 64);
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&sc->d_meshes), views.size() * sizeof(MeshView));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc->d_world), mat_bytes);
@@ -3096,6 +3119,227 @@ extern "C" int mbvh_scene_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, 
     return set_error(MBVH_ERR_OOM, "%s: device allocation or copy failed (%s)", fn, hipGetErrorString(e));
   }
   *out_scene = sc;
+  return MBVH_OK;
+}
+
+// ---- a scene in the reference's WebGPU packing -------------------------------------------------
+// BVHComputeData.update (src/webgpu/BVHComputeData.js:217-447) with every instance referenced whole,
+// as ClusteredBVH does for instanced and batched objects (src/webgpu/ClusteredBVH.js:219-229): one
+// top-level primitive per (instance, root), one primitive per top-level leaf (_strictLeafSize 1); the
+// top level first (appendBVHData, src/webgpu/utils/packBVHBufferUtils.js:44-131: a leaf holds the
+// absolute node offset of its subtree and 0xFF000000 | transform slot), then each (mesh, root)
+// subtree once, in the order the leaves first name it (appendBVHSubtree, :216-280: leaf offsets
+// rebased by the mesh's triangle base); transforms per slot as writeTransformData
+// (BVHComputeData.js:491-526: matrixWorld, its inverse, visible, three padding words); the index
+// buffer per mesh in the same order, rebased by the mesh's vertex base (appendIndexData,
+// packBVHBufferUtils.js:293-322, with the mesh's triangles in slot order). The top level is this
+// library's SAH build over the primitives' world boxes, so its shape is this build's, not ClusteredBVH's.
+namespace {
+
+struct WebgpuPacking {
+  std::vector<uint32_t> nodes;       // 8 words per node
+  std::vector<uint32_t> transforms;  // 36 words per slot
+  std::vector<uint32_t> index;
+};
+
+int pack_scene_webgpu(const mbvh_scene* sc, WebgpuPacking& out, const char* fn) {
+  struct Host {
+    bool ready = false;
+    std::vector<uint32_t> nodes, vidx;
+  };
+  std::vector<Host> hm(sc->meshes.size());
+  auto fetch = [&](uint32_t g) -> int {
+    if (hm[g].ready) return MBVH_OK;
+    const mbvh_mesh* m = sc->meshes[g];
+    hm[g].nodes.resize(8 * m->n_ref_nodes);
+    hm[g].vidx.resize(3 * m->n_slots);
+    MBVH_HIP(hipMemcpy(hm[g].nodes.data(), m->d_ref_nodes, hm[g].nodes.size() * 4, hipMemcpyDeviceToHost));
+    if (m->n_slots) MBVH_HIP(hipMemcpy(hm[g].vidx.data(), m->d_vidx, hm[g].vidx.size() * 4, hipMemcpyDeviceToHost));
+    hm[g].ready = true;
+    return MBVH_OK;
+  };
+  // primitives: (instance, root) of every instance with a non-empty root
+  std::vector<std::pair<uint32_t, uint32_t>> prims;
+  std::vector<float> prim_pos;
+  int rc;
+  for (uint32_t i = 0; i < sc->n_instances; i++) {
+    const uint32_t g = sc->geom_h[i];
+    if ((rc = fetch(g))) return rc;
+    const mbvh_mesh* m = sc->meshes[g];
+    for (uint32_t r = 0; r < m->root_base.size(); r++) {
+      const float* b = reinterpret_cast<const float*>(hm[g].nodes.data() + 8 * m->root_base[r]);
+      if (!(b[0] <= b[3] && b[1] <= b[4] && b[2] <= b[5])) continue;  // no triangles
+      const double lb[6] = {b[0], b[1], b[2], b[3], b[4], b[5]};
+      float fb[6];
+      if (!world_box_of(sc->world_h.data() + 16ull * i, lb, fb))
+        return set_error(MBVH_ERR_BAD_ARGS, "%s: instance %u has a non-finite world box", fn, i);
+      prims.push_back({i, r});
+      const float v[9] = {fb[0], fb[1], fb[2], fb[3], fb[4], fb[5], fb[0], fb[1], fb[2]};
+      prim_pos.insert(prim_pos.end(), v, v + 9);
+    }
+  }
+  const uint64_t np = prims.size();
+  if (np >= (uint64_t{1} << 24)) return set_error(MBVH_ERR_LIMIT, "%s: at most 2^24 - 1 primitives", fn);
+  std::vector<uint8_t> tlas(kBytesPerNode, 0);
+  std::vector<uint32_t> order;  // primitive of each top-level slot
+  if (np == 0) {  // one empty leaf
+    const float empty[6] = {std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity(),
+                            std::numeric_limits<float>::infinity(), -std::numeric_limits<float>::infinity(),
+                            -std::numeric_limits<float>::infinity(), -std::numeric_limits<float>::infinity()};
+    const uint32_t leaf_word = 0xFFFF0000u;
+    std::memcpy(tlas.data(), empty, sizeof(empty));
+    std::memcpy(tlas.data() + 28, &leaf_word, 4);
+  } else {
+    std::vector<uint32_t> idx(3 * np);
+    for (uint64_t k = 0; k < 3 * np; k++) idx[k] = static_cast<uint32_t>(k);
+    const uint32_t range[2] = {0u, static_cast<uint32_t>(np)};
+    const mbvh_build_opts opts{MBVH_SAH, 40, 1, 0, 1};  // one primitive per leaf (_strictLeafSize 1)
+    mbvh_tree* t = nullptr;
+    if ((rc = mbvh_build(prim_pos.data(), 3 * np, 3, 0, idx.data(), 4, idx.size(), range, 1, &opts, &t))) return rc;
+    tlas = t->roots[0];
+    mbvh_tree_free(t);
+    order.resize(np);
+    for (uint64_t k = 0; k < np; k++) order[k] = idx[3 * k] / 3;
+  }
+  const uint64_t n_tlas = tlas.size() / kBytesPerNode;
+  const uint32_t* tw = reinterpret_cast<const uint32_t*>(tlas.data());
+  for (uint64_t k = 0; k < n_tlas && np; k++)
+    if ((tw[8 * k + 7] >> 16) == 0xFFFFu && (tw[8 * k + 7] & 0xFFFFu) != 1u)
+      return set_error(MBVH_ERR_LIMIT, "%s: instances with identical world boxes share a top-level leaf", fn);
+  // subtrees and geometry in the order the top-level slots first name them
+  std::vector<std::vector<int64_t>> subtree(sc->meshes.size());  // per mesh, per root: node base or -1
+  std::vector<int64_t> tri_base(sc->meshes.size(), -1), vert_base(sc->meshes.size(), -1);
+  std::vector<uint32_t> mesh_order;
+  std::vector<std::pair<uint32_t, uint32_t>> sub_order;  // (mesh, root)
+  uint64_t next_node = n_tlas, next_tri = 0, next_vert = 0;
+  std::vector<uint64_t> leaf_base(np);
+  for (uint64_t k = 0; k < np; k++) {
+    const uint32_t inst = prims[order[k]].first, r = prims[order[k]].second, g = sc->geom_h[inst];
+    const mbvh_mesh* m = sc->meshes[g];
+    if (tri_base[g] < 0) {
+      tri_base[g] = static_cast<int64_t>(next_tri);
+      vert_base[g] = static_cast<int64_t>(next_vert);
+      next_tri += m->n_slots;
+      next_vert += m->vertex_count;
+      mesh_order.push_back(g);
+      subtree[g].assign(m->root_base.size(), -1);
+    }
+    if (subtree[g][r] < 0) {
+      subtree[g][r] = static_cast<int64_t>(next_node);
+      next_node += m->root_nodes[r];
+      sub_order.push_back({g, r});
+    }
+    leaf_base[k] = static_cast<uint64_t>(subtree[g][r]);
+  }
+  if (next_node >= (uint64_t{1} << 32)) return set_error(MBVH_ERR_LIMIT, "%s: more than 2^32 nodes", fn);
+  out.nodes.assign(8 * next_node, 0u);
+  // the top level (appendBVHData)
+  for (uint64_t k = 0; k < n_tlas; k++) {
+    uint32_t* dst = out.nodes.data() + 8 * k;
+    const uint32_t* src = tw + 8 * k;
+    for (int c = 0; c < 6; c++) dst[c] = src[c];
+    if (k == 0) {
+      const float* bf = reinterpret_cast<const float*>(src);
+      for (int c = 0; c < 3; c++)
+        if (bf[c] > bf[c + 3]) {
+          const float one = 1.0f, mone = -1.0f;
+          std::memcpy(&dst[c], &one, 4);
+          std::memcpy(&dst[c + 3], &mone, 4);
+        }
+    }
+    if ((src[7] >> 16) == 0xFFFFu) {
+      const uint32_t cnt = src[7] & 0xFFFFu;
+      const uint32_t slot = cnt == 0 ? 0u : src[6];  // the slot is the leaf's primitive offset
+      dst[6] = cnt == 0 ? 0u : static_cast<uint32_t>(leaf_base[slot]);
+      dst[7] = 0xFF000000u | (slot & 0x00FFFFFFu);
+    } else {
+      dst[6] = src[6];
+      dst[7] = src[7];
+    }
+  }
+  // the subtrees (appendBVHSubtree)
+  for (const auto& gr : sub_order) {
+    const uint32_t g = gr.first, r = gr.second;
+    const mbvh_mesh* m = sc->meshes[g];
+    const uint32_t* src0 = hm[g].nodes.data() + 8 * m->root_base[r];
+    uint32_t* dst0 = out.nodes.data() + 8 * static_cast<uint64_t>(subtree[g][r]);
+    for (uint64_t k = 0; k < m->root_nodes[r]; k++) {
+      const uint32_t* src = src0 + 8 * k;
+      uint32_t* dst = dst0 + 8 * k;
+      for (int c = 0; c < 6; c++) dst[c] = src[c];
+      if (k == 0) {
+        const float* bf = reinterpret_cast<const float*>(src);
+        for (int c = 0; c < 3; c++)
+          if (bf[c] > bf[c + 3]) {
+            const float one = 1.0f, mone = -1.0f;
+            std::memcpy(&dst[c], &one, 4);
+            std::memcpy(&dst[c + 3], &mone, 4);
+          }
+      }
+      if ((src[7] >> 16) == 0xFFFFu) {
+        dst[6] = src[6] + static_cast<uint32_t>(tri_base[g]);
+        dst[7] = 0xFFFF0000u | (src[7] & 0xFFFFu);
+      } else {
+        dst[6] = src[6];
+        dst[7] = src[7];
+      }
+    }
+  }
+  // transforms, one per slot
+  out.transforms.assign(36 * std::max<uint64_t>(np, 1), 0u);
+  for (uint64_t k = 0; k < np; k++) {
+    const uint32_t inst = prims[order[k]].first;
+    std::memcpy(out.transforms.data() + 36 * k, sc->world_h.data() + 16ull * inst, 64);
+    std::memcpy(out.transforms.data() + 36 * k + 16, sc->frame_h.data() + 16ull * inst, 64);
+    out.transforms[36 * k + 32] = sc->visible_h[inst];
+  }
+  // the index, per mesh
+  out.index.assign(3 * next_tri, 0u);
+  for (uint32_t g : mesh_order) {
+    const uint64_t base = 3 * static_cast<uint64_t>(tri_base[g]);
+    for (uint64_t k = 0; k < hm[g].vidx.size(); k++)
+      out.index[base + k] = hm[g].vidx[k] + static_cast<uint32_t>(vert_base[g]);
+  }
+  return MBVH_OK;
+}
+
+}  // namespace
+
+extern "C" int mbvh_scene_webgpu_sizes(mbvh_ctx* ctx, const mbvh_scene* scene, uint64_t* n_nodes,
+                                       uint64_t* n_transforms, uint64_t* n_index) {
+  clear_error();
+  const char* fn = "mbvh_scene_webgpu_sizes";
+  if (!valid_ctx(ctx) || !valid_scene(scene) || scene->ctx != ctx)
+    return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid context or scene handle", fn);
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  WebgpuPacking pk;
+  if ((rc = pack_scene_webgpu(scene, pk, fn))) return rc;
+  if (n_nodes) *n_nodes = pk.nodes.size() / 8;
+  if (n_transforms) *n_transforms = pk.transforms.size() / 36;
+  if (n_index) *n_index = pk.index.size();
+  return MBVH_OK;
+}
+
+extern "C" int mbvh_scene_export_webgpu(mbvh_ctx* ctx, const mbvh_scene* scene, uint32_t* nodes, uint64_t nodes_capacity,
+                                        uint32_t* transforms, uint64_t transforms_capacity, uint32_t* index,
+                                        uint64_t index_capacity) {
+  clear_error();
+  const char* fn = "mbvh_scene_export_webgpu";
+  if (!valid_ctx(ctx) || !valid_scene(scene) || scene->ctx != ctx)
+    return set_error(MBVH_ERR_BAD_HANDLE, "%s: invalid context or scene handle", fn);
+  std::lock_guard<std::mutex> guard(ctx->lock);
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  WebgpuPacking pk;
+  if ((rc = pack_scene_webgpu(scene, pk, fn))) return rc;
+  if ((nodes && nodes_capacity < pk.nodes.size()) || (transforms && transforms_capacity < pk.transforms.size()) ||
+      (index && index_capacity < pk.index.size()))
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: an output holds fewer words than mbvh_scene_webgpu_sizes gives", fn);
+  if (nodes) std::memcpy(nodes, pk.nodes.data(), pk.nodes.size() * 4);
+  if (transforms) std::memcpy(transforms, pk.transforms.data(), pk.transforms.size() * 4);
+  if (index && !pk.index.empty()) std::memcpy(index, pk.index.data(), pk.index.size() * 4);
   return MBVH_OK;
 }
 

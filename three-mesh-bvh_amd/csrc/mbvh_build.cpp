@@ -349,14 +349,19 @@ class TreeBuilder {
 
   // buildTree.js:44-119.
   int split(int32_t node, uint64_t offset, uint64_t count, const float* centroid, int depth) {
-    if (count <= static_cast<uint64_t>(opts_.target_leaf_size) || depth >= opts_.max_depth) {
+    // _strictLeafSize (buildTree.js:59-94): an over-full node is split whatever the heuristic says
+    const bool must_split = opts_.strict_leaf_size > 0 && count > static_cast<uint64_t>(opts_.strict_leaf_size);
+    if ((count <= static_cast<uint64_t>(opts_.target_leaf_size) && !must_split) || depth >= opts_.max_depth) {
       return make_leaf(node, offset, count);
     }
-    const Split s = choose_split(nodes_[node].box, centroid, offset, count);
+    Split s = choose_split(nodes_[node].box, centroid, offset, count);
     uint64_t split_at = 0;
     if (s.axis != -1) split_at = partition(offset, count, s);
     if (s.axis == -1 || split_at == offset || split_at == offset + count) {
-      return make_leaf(node, offset, count);
+      if (!must_split) return make_leaf(node, offset, count);
+      // no usable split: the median of the current order, on the node box's longest axis
+      s.axis = std::max(0, longest_edge(nodes_[node].box));
+      split_at = offset + std::max<uint64_t>(1, count / 2);
     }
     nodes_[node].axis = s.axis;
 
@@ -467,8 +472,8 @@ int check_build_args(const char* fn, const float* positions, uint64_t vertex_cou
                      position_stride, position_offset);
   if (opts->strategy < MBVH_CENTER || opts->strategy > MBVH_SAH)
     return set_error(MBVH_ERR_BAD_ARGS, "%s: invalid build strategy value %d", fn, opts->strategy);
-  if (opts->max_depth < 0 || opts->target_leaf_size < 0)
-    return set_error(MBVH_ERR_BAD_ARGS, "%s: negative max_depth / target_leaf_size", fn);
+  if (opts->max_depth < 0 || opts->target_leaf_size < 0 || opts->strict_leaf_size < 0)
+    return set_error(MBVH_ERR_BAD_ARGS, "%s: negative max_depth / target_leaf_size / strict_leaf_size", fn);
   if (index) {
     if (index_elt_bytes != 2 && index_elt_bytes != 4)
       return set_error(MBVH_ERR_BAD_ARGS, "%s: index element size must be 2 or 4", fn);

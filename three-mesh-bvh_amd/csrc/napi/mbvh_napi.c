@@ -189,7 +189,8 @@ static napi_value js_build(napi_env env, napi_callback_info info) {
   if (!get_named_i32(env, argv[5], "strategy", MBVH_CENTER, &opts.strategy) ||
       !get_named_i32(env, argv[5], "maxDepth", 40, &opts.max_depth) ||
       !get_named_i32(env, argv[5], "targetLeafSize", 10, &opts.target_leaf_size) ||
-      !get_named_i32(env, argv[5], "indirect", 0, &opts.indirect))
+      !get_named_i32(env, argv[5], "indirect", 0, &opts.indirect) ||
+      !get_named_i32(env, argv[5], "strictLeafSize", 0, &opts.strict_leaf_size))
     return throw_msg(env, "build: bad options");
   int32_t device = -1; /* >= 0: build on that device (mbvh_build_gpu), same result */
   if (!get_named_i32(env, argv[5], "device", -1, &device)) return throw_msg(env, "build: bad options");
@@ -1172,6 +1173,42 @@ static napi_value js_intersects_sphere(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* sceneExportWebGPU(scene) -> { nodes, transforms, index } (Uint32Arrays): the scene in the reference's
+ * WebGPU packing (mbvh_scene_export_webgpu) */
+static napi_value new_u32(napi_env env, uint64_t n, uint32_t** data) {
+  napi_value ab, out;
+  void* p = NULL;
+  if (napi_create_arraybuffer(env, (size_t)n * 4, &p, &ab) != napi_ok) return NULL;
+  if (napi_create_typedarray(env, napi_uint32_array, (size_t)n, ab, 0, &out) != napi_ok) return NULL;
+  *data = (uint32_t*)p;
+  return out;
+}
+
+static napi_value js_scene_export_webgpu(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  void* p = NULL;
+  if (napi_get_value_external(env, argv[0], &p) != napi_ok || !p || !((BatchHandle*)p)->scene)
+    return throw_msg(env, "sceneExportWebGPU: invalid scene");
+  BatchHandle* b = (BatchHandle*)p;
+  mbvh_ctx* ctx = NULL;
+  uint64_t nn = 0, nt = 0, ni = 0;
+  int rc = context_for(env, b->device, &ctx);
+  if (!rc) rc = mbvh_scene_webgpu_sizes(ctx, b->scene, &nn, &nt, &ni);
+  if (rc) return throw_status(env, rc, "mbvh_scene_webgpu_sizes");
+  uint32_t *pn = NULL, *pt = NULL, *pi = NULL;
+  napi_value vn = new_u32(env, 8 * nn, &pn), vt = new_u32(env, 36 * nt, &pt), vi = new_u32(env, ni, &pi);
+  if (!vn || !vt || !vi) return throw_msg(env, "sceneExportWebGPU: out of memory");
+  rc = mbvh_scene_export_webgpu(ctx, b->scene, pn, 8 * nn, pt, 36 * nt, pi, ni);
+  if (rc) return throw_status(env, rc, "mbvh_scene_export_webgpu");
+  napi_value out;
+  napi_create_object(env, &out);
+  napi_set_named_property(env, out, "nodes", vn);
+  napi_set_named_property(env, out, "transforms", vt);
+  napi_set_named_property(env, out, "index", vi);
+  return out;
+}
+
 /* refitMesh(mesh, positions, stride, offset, roots[], nodeIds = null): device refit from new positions;
  * the refit reference-format nodes are written into the roots' ArrayBuffers. nodeIds (a Uint32Array
  * of node indices over all roots) makes it a partial refit. */
@@ -1260,6 +1297,7 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("exportNodes", js_export_nodes);
   EXPORT("freeScene", js_free_batch);
   EXPORT("sceneRaycastFirst", js_scene_raycast_first);
+  EXPORT("sceneExportWebGPU", js_scene_export_webgpu);
   EXPORT("closestPointBackward", js_closest_point_bwd);
   EXPORT("refitMesh", js_refit);
   EXPORT("intersectsSphere", js_intersects_sphere);

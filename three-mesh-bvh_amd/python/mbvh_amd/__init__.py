@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = (
     "mbvh_batch_raycast_first", "mbvh_scene_create", "mbvh_scene_free", "mbvh_scene_raycast_first",
     "mbvh_scene_raycast_first_d", "mbvh_intersects_box", "mbvh_intersects_box_d",
     "mbvh_closest_point_to_geometry", "mbvh_closest_point_to_geometry_d",
-    "mbvh_scene_closest_point", "mbvh_scene_closest_point_d", "mbvh_mesh_texture_sizes", "mbvh_mesh_export_textures",
+    "mbvh_scene_closest_point", "mbvh_scene_closest_point_d", "mbvh_scene_webgpu_sizes", "mbvh_scene_export_webgpu", "mbvh_mesh_texture_sizes", "mbvh_mesh_export_textures",
     "mbvh_mesh_export_textures_d", "mbvh_mesh_export_nodes", "mbvh_mesh_export_nodes_d",
     "mbvh_mesh_import_nodes", "mbvh_mesh_import_textures",
     "mbvh_batch_autotune", "mbvh_batch_graph_create", "mbvh_batch_graph_launch", "mbvh_batch_graph_free",
@@ -60,8 +60,8 @@ class MbvhError(RuntimeError):
 
 
 class BuildOpts(C.Structure):
-    _fields_ = [("strategy", C.c_int32), ("max_depth", C.c_int32),
-                ("target_leaf_size", C.c_int32), ("indirect", C.c_int32)]
+    _fields_ = [("strategy", C.c_int32), ("max_depth", C.c_int32), ("target_leaf_size", C.c_int32),
+                ("indirect", C.c_int32), ("strict_leaf_size", C.c_int32)]
 
 
 _lib = None
@@ -145,6 +145,8 @@ def library() -> C.CDLL:
         "mbvh_scene_raycast_first": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp]),
         "mbvh_scene_closest_point": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp]),
         "mbvh_scene_closest_point_d": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp]),
+        "mbvh_scene_webgpu_sizes": (C.c_int, [vp, vp, vp, vp, vp]),
+        "mbvh_scene_export_webgpu": (C.c_int, [vp, vp, vp, u64, vp, u64, vp, u64]),
         "mbvh_intersects_box": (C.c_int, [vp, vp, vp, vp, vp, u64, vp]),
         "mbvh_intersects_box_d": (C.c_int, [vp, vp, vp, vp, vp, u64, vp, vp]),
         "mbvh_closest_point_to_geometry": (C.c_int, [vp, vp, vp, u64, u32, u32, vp, u32, u64, vp, f64, f64, vp]),
@@ -203,7 +205,8 @@ class Tree:
 
 def build(positions: np.ndarray, index: Optional[np.ndarray], ranges: Optional[Sequence] = None,
           strategy: int = SAH, max_depth: int = 40, target_leaf_size: int = 10,
-          indirect: bool = False, stride: int = 3, offset: int = 0, ctx: Optional["Context"] = None) -> Tree:
+          indirect: bool = False, stride: int = 3, offset: int = 0, ctx: Optional["Context"] = None,
+          strict_leaf_size: int = 0) -> Tree:
     """Builds the BVH. A non-indirect build permutes `index` IN PLACE (like the reference).
 
     positions: float32 array with `stride` floats per vertex. ranges: (offset, count) triangle
@@ -222,7 +225,7 @@ def build(positions: np.ndarray, index: Optional[np.ndarray], ranges: Optional[S
     if ranges is None:
         ranges = [(0, ntris)]
     rr = np.asarray(ranges, dtype=np.uint32).reshape(-1)
-    opts = BuildOpts(strategy, max_depth, target_leaf_size, 1 if indirect else 0)
+    opts = BuildOpts(strategy, max_depth, target_leaf_size, 1 if indirect else 0, strict_leaf_size)
     tree = C.c_void_p()
     args = (_ptr(pos), nverts, stride, offset, _ptr(index), 0 if index is None else index.itemsize,
             0 if index is None else index.size, _ptr(rr), len(rr) // 2, C.byref(opts), C.byref(tree))
@@ -889,6 +892,19 @@ class Scene:
             self.ctx.handle, self.handle, _ptr(q), n, _ptr(out["face"]), _ptr(out["instance"]),
             _ptr(out["distance"]), _ptr(out["point"]), _ptr(out["barycoord"])))
         return out
+
+    def export_webgpu(self):
+        """The scene in the reference's WebGPU packing: {'nodes': uint32 (N, 8), 'transforms': uint32 (S, 36),
+        'index': uint32 (3T,)} as mbvh_scene_export_webgpu writes them."""
+        nn, nt, ni = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(self.ctx._lib.mbvh_scene_webgpu_sizes(self.ctx.handle, self.handle, C.byref(nn), C.byref(nt),
+                                                     C.byref(ni)))
+        nodes = np.empty((nn.value, 8), np.uint32)
+        tr = np.empty((nt.value, 36), np.uint32)
+        idx = np.empty(max(ni.value, 1), np.uint32)
+        _check(self.ctx._lib.mbvh_scene_export_webgpu(self.ctx.handle, self.handle, _ptr(nodes), nodes.size, _ptr(tr),
+                                                      tr.size, _ptr(idx), ni.value))
+        return {"nodes": nodes, "transforms": tr, "index": idx[:ni.value]}
 
     def closest_point_d(self, points_ptr: int, n: int, face_ptr: int, instance_ptr: int = 0, dist_ptr: int = 0,
                         point_ptr: int = 0, bary_ptr: int = 0, stream: int = 0, order_ptr: int = 0) -> None:

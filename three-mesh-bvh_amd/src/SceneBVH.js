@@ -136,6 +136,16 @@ class SceneBVH {
     return target;
   }
 
+  // The scene in the reference's WebGPU packing (BVHComputeData.update with every instance referenced
+  // whole, src/webgpu/BVHComputeData.js:217-447): { nodes, transforms, index } Uint32Arrays — BVHNode
+  // words (top level first: a leaf holds its subtree's node offset and 0xFF000000 | transform slot;
+  // then each mesh root once, leaves rebased into the index), 36 words per transform slot (matrixWorld,
+  // its inverse, visible, padding) and the packed index (meshes in the order the top level first
+  // names them, each rebased by the vertex count of the meshes before it).
+  exportWebGPU() {
+    return native.load().sceneExportWebGPU(this._scene);
+  }
+
   dispose() {
     if (this._scene) native.load().freeScene(this._scene);
     this._scene = null;
