@@ -594,26 +594,30 @@ int mbvh_scene_closest_point_d(mbvh_ctx* ctx, const mbvh_scene* scene, const flo
                                float* point, float* barycoord, void* stream);
 /*
  * The scene in the reference's WebGPU packing (BVHComputeData.update, src/webgpu/BVHComputeData.js:217-447,
- * with every instance referenced whole as ClusteredBVH references instanced objects,
- * src/webgpu/ClusteredBVH.js:219-229): `nodes` (8 words per BVHNode, src/webgpu/tsl/structs.js:9-14): a
- * top level with one (instance, root) primitive per leaf, each leaf holding its subtree's absolute
- * node offset and 0xFF000000 | its transform slot (appendBVHData,
- * src/webgpu/utils/packBVHBufferUtils.js:44-131), then each referenced (mesh, root) subtree once
- * (appendBVHSubtree, :216-280, leaf offsets rebased by the mesh's triangle base); `transforms` (36 words
- * per slot: matrixWorld, its inverse, visible, 3 padding words; writeTransformData,
+ * over ClusteredBVH's primitives, src/webgpu/ClusteredBVH.js:207-331). `primitive_limit` 0 references
+ * every instance's roots whole (what ClusteredBVH does for instanced and batched objects and for
+ * scenes of fewer than three meshes); otherwise each root is cut into clusters as _traverseClusters
+ * does (depth first, left first, the first node on each path that is a leaf or spans fewer than
+ * `primitive_limit` triangles; the reference uses 64). `nodes` (8 words per BVHNode,
+ * src/webgpu/tsl/structs.js:9-14): a top level with one (instance, root, cluster) primitive per leaf,
+ * each leaf holding its subtree's absolute node offset and 0xFF000000 | its transform slot
+ * (appendBVHData, src/webgpu/utils/packBVHBufferUtils.js:44-131), then each referenced
+ * (mesh, root, cluster) subtree once, its contiguous node span (getSubtreeNodeCount, :142-161) with
+ * leaf offsets rebased by the mesh's triangle base (appendBVHSubtree, :216-280); `transforms` (36
+ * words per slot: matrixWorld, its inverse, visible, 3 padding words; writeTransformData,
  * BVHComputeData.js:491-526); `index` (3 words per triangle: each mesh's triangles in slot order,
  * rebased by the mesh's vertex base, meshes in the order the top level first names them; an
  * attribute buffer packed in that mesh order, each mesh's vertices whole, matches it). Instances whose
  * roots hold no triangles are left out; invisible instances stay, with visible 0. The top level is
  * this library's SAH build over the primitives' world boxes. Fails with MBVH_ERR_LIMIT when two
  * primitives with identical boxes would share a leaf. Host arrays; sizes in words from
- * mbvh_scene_webgpu_sizes; NULL outputs are skipped.
+ * mbvh_scene_webgpu_sizes (same `primitive_limit`); NULL outputs are skipped.
  */
-int mbvh_scene_webgpu_sizes(mbvh_ctx* ctx, const mbvh_scene* scene, uint64_t* n_nodes, uint64_t* n_transforms,
-                            uint64_t* n_index);
-int mbvh_scene_export_webgpu(mbvh_ctx* ctx, const mbvh_scene* scene, uint32_t* nodes, uint64_t nodes_capacity,
-                             uint32_t* transforms, uint64_t transforms_capacity, uint32_t* index,
-                             uint64_t index_capacity);
+int mbvh_scene_webgpu_sizes(mbvh_ctx* ctx, const mbvh_scene* scene, uint32_t primitive_limit, uint64_t* n_nodes,
+                            uint64_t* n_transforms, uint64_t* n_index);
+int mbvh_scene_export_webgpu(mbvh_ctx* ctx, const mbvh_scene* scene, uint32_t primitive_limit, uint32_t* nodes,
+                             uint64_t nodes_capacity, uint32_t* transforms, uint64_t transforms_capacity,
+                             uint32_t* index, uint64_t index_capacity);
 /* Host-pointer form: stages the inputs, launches, copies the results back and synchronises. */
 int mbvh_scene_raycast_first(mbvh_ctx* ctx, const mbvh_scene* scene, const float* origins, const float* directions,
                              uint64_t n, int32_t side, float near_dist, float far_dist, uint32_t* face_index,

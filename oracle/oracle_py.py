@@ -466,25 +466,69 @@ def _world_box(e, lb):
     return np.concatenate([lo, hi]).astype(np.float32)
 
 
-def webgpu_scene(roots, slot_vertices, vertex_counts, world, geometry, visible, build):
-    """BVHComputeData.update's packing (src/webgpu/BVHComputeData.js:217-447) of instances referenced
-    whole: one (instance, root) primitive per top-level leaf, the top level built by `build` (positions,
-    index -> root buffer, permuted index) over the primitives' world boxes; appendBVHData
-    (src/webgpu/utils/packBVHBufferUtils.js:44-131) for the top level, appendBVHSubtree (:216-280) for
-    each (mesh, root) the leaves name (first-named first), transforms per slot (matrixWorld, its inverse,
-    visible, padding; BVHComputeData.js:491-526) and the index per mesh rebased by its vertex base
+def cluster_cut(nodes, primitive_limit):
+    """ClusteredBVH's _traverseClusters (src/webgpu/ClusteredBVH.js:302-331) over one root's nodes
+    ((N, 8) uint32, relative right child): depth first, left child first, a node is a cluster when it
+    is a leaf or spans fewer than `primitive_limit` triangles (BVHTraversalHelper.getRangeStart /
+    getRangeEnd, src/core/BVHTraversalHelper.js:41-67: the leftmost leaf's offset to the rightmost
+    leaf's end); 0 means no limit (the root whole)."""
+    def is_leaf(n):
+        return (int(nodes[n, 7]) >> 16) == 0xFFFF
+
+    def span(n):
+        lo = hi = n
+        while not is_leaf(lo):
+            lo += 1
+        while not is_leaf(hi):
+            hi += int(nodes[hi, 6])
+        return int(nodes[hi, 6]) + (int(nodes[hi, 7]) & 0xFFFF) - int(nodes[lo, 6])
+
+    out = []
+
+    def walk(n):
+        if is_leaf(n) or primitive_limit == 0 or span(n) < primitive_limit:
+            out.append(n)
+            return
+        walk(n + 1)
+        walk(n + int(nodes[n, 6]))
+
+    walk(0)
+    return out
+
+
+def subtree_node_count(nodes, n):
+    """getSubtreeNodeCount (src/webgpu/utils/packBVHBufferUtils.js:142-161): the contiguous node span
+    of the subtree at n."""
+    if (int(nodes[n, 7]) >> 16) == 0xFFFF:
+        return 1
+    r = int(nodes[n, 6])
+    return r + subtree_node_count(nodes, n + r)
+
+
+def webgpu_scene(roots, slot_vertices, vertex_counts, world, geometry, visible, build, primitive_limit=0):
+    """BVHComputeData.update's packing (src/webgpu/BVHComputeData.js:217-447) of ClusteredBVH's
+    primitives (src/webgpu/ClusteredBVH.js:207-297): per instance and non-empty root, the root whole
+    (primitive_limit 0) or its cluster cut (cluster_cut); one (instance, root, node) primitive per
+    top-level leaf, the top level built by `build` (positions, index -> root buffer, permuted index)
+    over the primitives' world boxes; appendBVHData (src/webgpu/utils/packBVHBufferUtils.js:44-131) for
+    the top level, appendBVHSubtree (:216-280) for each (mesh, root, node) subtree the leaves name
+    (first-named first; its contiguous span), transforms per slot (matrixWorld, its inverse, visible,
+    padding; BVHComputeData.js:491-526) and the index per mesh rebased by its vertex base
     (appendIndexData, :293-322). roots[g]: the mesh's root buffers (uint8); slot_vertices[g]: its
     triangles' vertex ids in slot order (3 per slot). Returns nodes (N, 8) uint32, transforms (S, 36)
-    uint32 with the inverse as float32 from numpy's double inverse, index uint32."""
+    uint32 with the inverse as float32 from numpy's double inverse, index uint32, and the primitive
+    (instance, root, node) of each top-level slot."""
     prims, pos = [], []
     for i, g in enumerate(geometry):
         for r, rb in enumerate(roots[g]):
             b = rb.view(np.float32)[:6].astype(np.float64)
             if not (b[0] <= b[3] and b[1] <= b[4] and b[2] <= b[5]):
                 continue
-            wb = _world_box(world[i], b)
-            prims.append((i, r))
-            pos.append(np.concatenate([wb, wb[:3]]))
+            rn = rb.view(np.uint32).reshape(-1, 8)
+            for n in cluster_cut(rn, primitive_limit):
+                wb = _world_box(world[i], rb.view(np.float32).reshape(-1, 8)[n, :6].astype(np.float64))
+                prims.append((i, r, n))
+                pos.append(np.concatenate([wb, wb[:3]]))
     np_ = len(prims)
     if np_ == 0:
         tl = np.zeros(8, np.uint32)
@@ -502,18 +546,19 @@ def webgpu_scene(roots, slot_vertices, vertex_counts, world, geometry, visible, 
     nxt, ntri, nvert = n_tlas, 0, 0
     leaf_base = []
     for k in range(np_):
-        i, r = prims[order[k]]
+        i, r, n = prims[order[k]]
         g = int(geometry[i])
         if g not in tri_base:
             tri_base[g], vert_base[g] = ntri, nvert
             ntri += len(slot_vertices[g]) // 3
             nvert += int(vertex_counts[g])
             mesh_order.append(g)
-        if (g, r) not in sub:
-            sub[(g, r)] = nxt
-            nxt += roots[g][r].size // 32
-            sub_order.append((g, r))
-        leaf_base.append(sub[(g, r)])
+        if (g, r, n) not in sub:
+            size = subtree_node_count(roots[g][r].view(np.uint32).reshape(-1, 8), n)
+            sub[(g, r, n)] = (nxt, size)
+            nxt += size
+            sub_order.append((g, r, n))
+        leaf_base.append(sub[(g, r, n)][0])
     nodes = np.zeros((nxt, 8), np.uint32)
 
     def fix_root(dst, src):
@@ -535,10 +580,10 @@ def webgpu_scene(roots, slot_vertices, vertex_counts, world, geometry, visible, 
             nodes[k, 7] = 0xFF000000 | (slot & 0x00FFFFFF)
         else:
             nodes[k, 6:8] = src[6:8]
-    for g, r in sub_order:
-        src = roots[g][r].view(np.uint32).reshape(-1, 8)
-        base = sub[(g, r)]
-        for k in range(src.shape[0]):
+    for g, r, n in sub_order:
+        base, size = sub[(g, r, n)]
+        src = roots[g][r].view(np.uint32).reshape(-1, 8)[n:n + size]
+        for k in range(size):
             nodes[base + k, :6] = src[k, :6]
             if k == 0:
                 fix_root(nodes[base + k], src[k])

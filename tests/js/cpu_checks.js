@@ -35,7 +35,7 @@ check('exports the query API', () => {
   }
   for (const m of ['raycast', 'raycastFirst', 'closestPointToPoint', 'raycastObject3D', 'raycastFirstBatch', 'refit', 'intersectsSphere', 'intersectsSphereBatch',
     'raycastBatch', 'closestPointToPointBatch', 'getBoundingBox', 'raycastFirstBatchAsync',
-    'closestPointToPointBatchAsync']) {
+    'closestPointToPointBatchAsync', 'raycastFirstAsync', 'closestPointToPointAsync']) {
     assert.strictEqual(typeof api.MeshBVH.prototype[m], 'function', m);
   }
   assert.strictEqual(typeof api.MeshBVH.serialize, 'function');
@@ -151,15 +151,55 @@ check('getTriangleHitPointInfo: face, material group, normal, barycoord, uv (clo
   assert.ok(near(r1.face.normal.z, -1) && near(r1.uv.x, 0) && near(r1.uv.y, 1));
 });
 
-let failed = 0;
-for (const [name, fn] of checks) {
-  try {
-    fn();
-    console.log(`ok   ${name}`);
-  } catch (e) {
-    failed++;
-    console.log(`FAIL ${name}: ${e && e.stack ? e.stack : e}`);
+// The micro-batch queue over stand-in batch methods (the device ones need a GPU): the calls of one
+// turn become one batch per (side, near, far) or threshold group, answers go back in call order,
+// a failed launch rejects every call of its group, and the next turn starts a new batch.
+check('micro-batched per-call queries group the calls of one turn', async () => {
+  const bvh = new api.MeshBVH(torus(1, 0.4, 12, 48));
+  const launches = [];
+  const result = (n, miss) => ({
+    faceIndex: Uint32Array.from({ length: n }, (_, k) => (miss(k) ? 0xFFFFFFFF : 7)),
+    distance: Float32Array.from({ length: n }, (_, k) => k + 0.5),
+    point: new Float32Array(3 * n).fill(0.25), barycoord: new Float32Array(3 * n).fill(1 / 3),
+  });
+  bvh.raycastFirstBatchAsync = (o, d, opts) => {
+    launches.push({ kind: 'ray', n: o.length / 3, o: Array.from(o), opts });
+    if (opts.far === 13) return Promise.reject(new Error('launch failed'));
+    return Promise.resolve(result(o.length / 3, k => o[3 * k] < 0));
+  };
+  bvh.closestPointToPointBatchAsync = (q, opts) => {
+    launches.push({ kind: 'point', n: q.length / 3, opts });
+    return Promise.resolve(result(q.length / 3, () => false));
+  };
+  const ray = x => ({ origin: { x, y: 0, z: 0 }, direction: { x: 0, y: 0, z: 1 } });
+  const calls = [bvh.raycastFirstAsync(ray(1)), bvh.raycastFirstAsync(ray(-1), api.DoubleSide),
+    bvh.raycastFirstAsync(ray(2)), bvh.raycastFirstAsync(ray(-3)), bvh.raycastFirstAsync(ray(4), api.FrontSide, 0, 13),
+    bvh.closestPointToPointAsync({ x: 1, y: 2, z: 3 }), bvh.closestPointToPointAsync({ x: 1, y: 2, z: 3 }, {}, 0, 2)];
+  assert.strictEqual(launches.length, 0);  // nothing launches before the turn ends
+  const settled = await Promise.allSettled(calls);
+  assert.deepStrictEqual(launches.map(l => [l.kind, l.n]), [['ray', 3], ['ray', 1], ['ray', 1], ['point', 1], ['point', 1]]);
+  assert.deepStrictEqual(launches[0].o, [1, 0, 0, 2, 0, 0, -3, 0, 0]);  // call order within the group
+  assert.strictEqual(launches[1].opts.side, api.DoubleSide);
+  assert.strictEqual(launches[4].opts.maxThreshold, 2);
+  const v = settled.map(x => x.value);
+  assert.ok(v[0].faceIndex === 7 && v[0].distance === 0.5 && v[2].distance === 1.5 && v[3] === null && v[1] === null);
+  assert.ok(settled[4].status === 'rejected' && /launch failed/.test(settled[4].reason.message));
+  assert.ok(v[5].faceIndex === 7 && v[5].point.x === 0.25 && v[6].distance === 0.5);
+  await bvh.raycastFirstAsync(ray(1));
+  assert.strictEqual(launches.length, 6);  // a later turn: a batch of its own
+});
+
+(async () => {
+  let failed = 0;
+  for (const [name, fn] of checks) {
+    try {
+      await fn();
+      console.log(`ok   ${name}`);
+    } catch (e) {
+      failed++;
+      console.log(`FAIL ${name}: ${e && e.stack ? e.stack : e}`);
+    }
   }
-}
-console.log(`${checks.length - failed}/${checks.length} passed`);
-process.exit(failed ? 1 : 0);
+  console.log(`${checks.length - failed}/${checks.length} passed`);
+  process.exit(failed ? 1 : 0);
+})();

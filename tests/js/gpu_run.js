@@ -262,6 +262,10 @@ const batchedChecked = bface.filter(f => f !== 0xFFFFFFFF).length, presenceDiffe
     assert.ok(topLeaves > 0 && topLeaves === nSlots, 'one slot per top-level leaf');
     for (let t = 0; t < nSlots; t++) assert.ok(pk.transforms[36 * t + 32] <= 1, 'visible flag');
     assert.ok(pk.index.length > 0 && pk.index.length % 3 === 0, 'packed index');
+    // cut into clusters of fewer than 64 triangles: more top-level slots, the same geometry
+    const pc = scene.exportWebGPU({ primitiveLimit: 64 });
+    assert.ok(pc.transforms.length / 36 > nSlots, 'clusters add top-level slots');
+    assert.strictEqual(pc.index.length, pk.index.length, 'clusters keep the packed index size');
   }
   const nS = Math.min(K, 128);
   const many = scene.raycastFirstBatch(origins.subarray(0, 3 * nS), dirs.subarray(0, 3 * nS));
@@ -453,8 +457,49 @@ writeArr('partial_root0.bin', new Uint8Array(bvh._roots[0]));
     sameBytes(ca1[k], cs[k], `async closestPoint ${k}`);
     sameBytes(ra2[k], rd[k], `async double raycastFirst ${k}`);
   }
+  // micro-batched per-call queries: the calls of one turn share one launch per (side, near, far) or
+  // threshold group and answer as the synchronous per-call methods do; time per call beside theirs
+  const nCall = Math.min(nRays, 4096);
+  const rayAt = i => ({ origin: { x: origins[3 * i], y: origins[3 * i + 1], z: origins[3 * i + 2] },
+    direction: { x: dirs[3 * i], y: dirs[3 * i + 1], z: dirs[3 * i + 2] } });
+  const sideAt = i => (i % 3 === 2 ? api.DoubleSide : api.FrontSide);
+  const ptAt = i => ({ x: points[3 * (i % nPts)], y: points[3 * (i % nPts) + 1], z: points[3 * (i % nPts) + 2] });
+  const maxAt = i => (i % 4 === 3 ? 0.05 : Infinity);
+  const usSince = t0 => Number(process.hrtime.bigint() - t0) / 1e3 / nCall;
+  let t0 = process.hrtime.bigint();
+  const raySync = Array.from({ length: nCall }, (_, i) => bvh.raycastFirst(rayAt(i), sideAt(i)));
+  const raySyncUs = usSince(t0);
+  t0 = process.hrtime.bigint();
+  const rayQueued = await Promise.all(Array.from({ length: nCall }, (_, i) => bvh.raycastFirstAsync(rayAt(i), sideAt(i))));
+  const rayQueuedUs = usSince(t0);
+  t0 = process.hrtime.bigint();
+  const ptSync = Array.from({ length: nCall }, (_, i) => bvh.closestPointToPoint(ptAt(i), {}, 0, maxAt(i)));
+  const ptSyncUs = usSince(t0);
+  t0 = process.hrtime.bigint();
+  const ptQueued = await Promise.all(Array.from({ length: nCall }, (_, i) => bvh.closestPointToPointAsync(ptAt(i), {}, 0, maxAt(i))));
+  const ptQueuedUs = usSince(t0);
+  let rayHits = 0, ptMisses = 0;
+  for (let i = 0; i < nCall; i++) {
+    const a = raySync[i], b = rayQueued[i];
+    assert.strictEqual(a === null, b === null, `queued ray ${i}: hit or miss`);
+    if (a) {
+      rayHits++;
+      assert.deepStrictEqual([b.faceIndex, b.distance, b.point.x, b.point.y, b.point.z],
+        [a.faceIndex, a.distance, a.point.x, a.point.y, a.point.z], `queued ray ${i}`);
+    }
+    const p = ptSync[i], q = ptQueued[i];
+    assert.strictEqual(p === null, q === null, `queued point ${i}: found or not`);
+    if (p) {
+      assert.deepStrictEqual([q.faceIndex, q.distance, q.point.x, q.point.y, q.point.z],
+        [p.faceIndex, p.distance, p.point.x, p.point.y, p.point.z], `queued point ${i}`);
+    } else {
+      ptMisses++;
+    }
+  }
+  assert.ok(rayHits > 0 && ptMisses > 0 && ptMisses < nCall, 'both outcomes among the queued calls');
   console.log(JSON.stringify({ ok: true, rays: nRays, points: nPts, worldChecked, batchedChecked, presenceDiffers,
-    asyncChecked: true }));
+    asyncChecked: true, microBatchChecked: nCall, perCallUs: { raySync: raySyncUs, rayQueued: rayQueuedUs,
+      pointSync: ptSyncUs, pointQueued: ptQueuedUs } }));
 })().catch((e) => {
   console.error(e);
   process.exit(1);

@@ -226,11 +226,43 @@ def _packed_first_hits(pk, positions, o, d):
     return best, best_slot
 
 
-def test_scene_webgpu_packing(ctx):
+def _check_cluster_cut(nodes, cut, limit):
+    """A cluster cut of one root's nodes covers every triangle once, each cluster is a leaf or spans
+    fewer than `limit` triangles, and its parent spans at least `limit` (the first such node on its path)."""
+    parent, stack = {0: None}, [0]
+    while stack:
+        n = stack.pop()
+        if (int(nodes[n, 7]) >> 16) != 0xFFFF:
+            for c in (n + 1, n + int(nodes[n, 6])):
+                parent[c] = n
+                stack.append(c)
+
+    def tris(n):
+        if (int(nodes[n, 7]) >> 16) == 0xFFFF:
+            return list(range(int(nodes[n, 6]), int(nodes[n, 6]) + (int(nodes[n, 7]) & 0xFFFF)))
+        return tris(n + 1) + tris(n + int(nodes[n, 6]))
+
+    covered = []
+    for n in cut:
+        t = tris(n)
+        covered += t
+        leaf = (int(nodes[n, 7]) >> 16) == 0xFFFF
+        assert leaf or limit == 0 or len(t) < limit, (n, len(t))
+        if limit and parent[n] is not None:
+            assert len(tris(parent[n])) >= limit, n
+        if limit == 0:
+            assert n == 0
+    assert sorted(covered) == tris(0)
+
+
+@pytest.mark.parametrize("limit", [0, 64, 1])
+def test_scene_webgpu_packing(ctx, limit):
     """mbvh_scene_export_webgpu == the oracle's restatement of the packing (top level with one
-    (instance, root) per leaf, tagged leaves, each (mesh, root) subtree once with rebased leaves,
+    (instance, root, cluster) per leaf, tagged leaves, each referenced subtree once with rebased leaves,
     transforms, rebased index) for plain, two-root and indirect meshes shared by several instances, one
-    of them invisible; and the merged loop over the packed buffers finds the scene raycast's hits."""
+    of them invisible, with roots whole (limit 0), cut into clusters of fewer than 64 triangles (the
+    reference's limit) and cut at every leaf (limit 1); the cut is checked against its definition; and
+    the merged loop over the packed buffers finds the scene raycast's hits."""
     geo = G.torus_knot(1.0, 0.3, 60, 10)
     geo.ensure_index()
     ntri = geo.index.size // 3
@@ -252,23 +284,42 @@ def test_scene_webgpu_packing(ctx):
     visible = np.ones(len(geometry), np.uint8)
     visible[4] = 0
     sc = ctx.scene(ms, world, inverse, geometry, visible)
-    got = sc.export_webgpu()
+    got = sc.export_webgpu(limit)
     slot_vertices = [_index_triplets(g, idx, t).reshape(-1) for g, idx, t in built]
+    for _, _, t in built:
+        for rb in t.roots:
+            _check_cluster_cut(rb.view(np.uint32).reshape(-1, 8), O.cluster_cut(rb.view(np.uint32).reshape(-1, 8), limit),
+                               limit)
 
     def build(pos, idx):
         tree = M.build(pos, idx, strategy=M.SAH, target_leaf_size=1, max_depth=40, strict_leaf_size=1)
         return tree.roots[0], idx
 
     ref = O.webgpu_scene([t.roots for _, _, t in built], slot_vertices, [g.positions.size // 3 for g, _, _ in built],
-                         world, geometry, visible, build)
+                         world, geometry, visible, build, primitive_limit=limit)
     assert np.array_equal(got["nodes"], ref["nodes"])
+    n_roots = sum(len(built[g][2].roots) for g in geometry)
+    assert (got["transforms"].shape[0] == n_roots) == (limit == 0)
     assert np.array_equal(got["index"], ref["index"])
     assert np.array_equal(got["transforms"][:, :16], ref["transforms"][:, :16])
     assert np.array_equal(got["transforms"][:, 32:], ref["transforms"][:, 32:])  # visible flag, padding
     gi = got["transforms"][:, 16:32].view(np.float32)
     ri = ref["transforms"][:, 16:32].view(np.float32)
     assert np.all(np.abs(gi - ri) <= 2 * np.spacing(np.maximum(np.abs(gi), np.abs(ri))))  # inverses to an ulp
-    assert sum(int(r.size // 32) for _, _, t in built for r in t.roots) < got["nodes"].shape[0]  # top level first
+    # a top level of 2S - 1 nodes (one primitive per leaf) first, then each named subtree's span once,
+    # and every top-level leaf enters a copy of its (mesh, root, node) subtree's boxes
+    n_slots = got["transforms"].shape[0]
+    spans = {(int(geometry[i]), r, n): O.subtree_node_count(built[int(geometry[i])][2].roots[r].view(np.uint32).reshape(-1, 8), n)
+             for i, r, n in ref["order"]}
+    assert got["nodes"].shape[0] == 2 * n_slots - 1 + sum(spans.values())
+    leaf_of = {int(w) & 0xFFFFFF: j for j, w in enumerate(got["nodes"][:2 * n_slots - 1, 7]) if (int(w) >> 24) == 0xFF}
+    assert sorted(leaf_of) == list(range(n_slots))
+    for k, (i, r, n) in enumerate(ref["order"]):
+        g = int(geometry[i])
+        rn = built[g][2].roots[r].view(np.uint32).reshape(-1, 8)
+        base, size = int(got["nodes"][leaf_of[k], 6]), spans[(g, r, n)]
+        assert base >= 2 * n_slots - 1
+        assert np.array_equal(got["nodes"][base + 1:base + size, :6], rn[n + 1:n + size, :6])
     # the packed buffers are usable: the merged loop finds the scene raycast's hits (visible instances)
     positions = np.concatenate([built[g][0].positions.reshape(-1, 3) for g in ref["mesh_order"]])
     sc_vis = ctx.scene(ms, world, inverse, geometry)  # the loop below ignores the visible flag

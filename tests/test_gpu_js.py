@@ -41,6 +41,7 @@ def test_js_api_matches_oracle(tmp_path):
     print(r.stdout[-2000:], r.stderr[-4000:])
     assert r.returncode == 0
     assert '"asyncChecked":true' in r.stdout  # the asynchronous queries ran and matched
+    assert '"microBatchChecked":' in r.stdout  # the micro-batched per-call queries ran and matched
 
     idx = np.fromfile(d / "bvh_index.bin", dtype=g.index.dtype)
     root = np.fromfile(d / "bvh_root0.bin", dtype=np.uint8)

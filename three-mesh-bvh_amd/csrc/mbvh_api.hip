@@ -3,10 +3,12 @@
 
 #include <algorithm>
 #include <limits>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <tuple>
 #include <vector>
 
 #include "mbvh_common.h"
@@ -2260,7 +2262,7 @@ extern "C" int mbvh_batch_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, 
 
 extern "C" void mbvh_batch_free(mbvh_batch* batch) {
   if (!valid_batch(batch)) return;
-  (void)hipSetDevice(batch->ctx->device);
+  if (valid_ctx(batch->ctx)) (void)hipSetDevice(batch->ctx->device);
   (void)hipDeviceSynchronize();
   (void)hipFree(batch->d_meshes);
   (void)hipFree(batch->d_counter);
@@ -3123,13 +3125,16 @@ extern "C" int mbvh_scene_create(mbvh_ctx* ctx, const mbvh_mesh* const* meshes, 
 }
 
 // ---- a scene in the reference's WebGPU packing -------------------------------------------------
-// BVHComputeData.update (src/webgpu/BVHComputeData.js:217-447) with every instance referenced whole,
-// as ClusteredBVH does for instanced and batched objects (src/webgpu/ClusteredBVH.js:219-229): one
-// top-level primitive per (instance, root), one primitive per top-level leaf (_strictLeafSize 1); the
-// top level first (appendBVHData, src/webgpu/utils/packBVHBufferUtils.js:44-131: a leaf holds the
-// absolute node offset of its subtree and 0xFF000000 | transform slot), then each (mesh, root)
-// subtree once, in the order the leaves first name it (appendBVHSubtree, :216-280: leaf offsets
-// rebased by the mesh's triangle base); transforms per slot as writeTransformData
+// BVHComputeData.update (src/webgpu/BVHComputeData.js:217-447) over ClusteredBVH's primitives
+// (src/webgpu/ClusteredBVH.js:207-297): per instance and root, either the whole root (primitive
+// limit 0, what ClusteredBVH does for instanced and batched objects and for scenes of fewer than three
+// meshes) or the cluster cut of _traverseClusters (:302-331: depth first, left child first, the first
+// node on each path that is a leaf or spans fewer triangles than the limit); one primitive per
+// top-level leaf (_strictLeafSize 1); the top level first (appendBVHData,
+// src/webgpu/utils/packBVHBufferUtils.js:44-131: a leaf holds the absolute node offset of its
+// subtree and 0xFF000000 | transform slot), then each (mesh, root, node) subtree once, in the order
+// the leaves first name it, its contiguous node span copied (getSubtreeNodeCount, :142-161;
+// appendBVHSubtree, :216-280: leaf offsets rebased by the mesh's triangle base); transforms per slot as writeTransformData
 // (BVHComputeData.js:491-526: matrixWorld, its inverse, visible, three padding words); the index
 // buffer per mesh in the same order, rebased by the mesh's vertex base (appendIndexData,
 // packBVHBufferUtils.js:293-322, with the mesh's triangles in slot order). The top level is this
@@ -3142,7 +3147,28 @@ struct WebgpuPacking {
   std::vector<uint32_t> index;
 };
 
-int pack_scene_webgpu(const mbvh_scene* sc, WebgpuPacking& out, const char* fn) {
+// Node span of the subtree at `n` (relative right child in word 6) and the triangles it covers.
+uint64_t subtree_span(const uint32_t* nodes, uint64_t n) {
+  uint64_t span = 1;
+  while ((nodes[8 * n + 7] >> 16) != 0xFFFFu) {
+    span += nodes[8 * n + 6];
+    n += nodes[8 * n + 6];
+  }
+  return span;
+}
+
+uint32_t subtree_triangles(const uint32_t* nodes, uint64_t n) {
+  uint64_t lo = n, hi = n;
+  while ((nodes[8 * lo + 7] >> 16) != 0xFFFFu) lo += 1;
+  while ((nodes[8 * hi + 7] >> 16) != 0xFFFFu) hi += nodes[8 * hi + 6];
+  return nodes[8 * hi + 6] + (nodes[8 * hi + 7] & 0xFFFFu) - nodes[8 * lo + 6];
+}
+
+struct WebgpuPrim {
+  uint32_t instance, root, node;  // node: index within the root's buffer
+};
+
+int pack_scene_webgpu(const mbvh_scene* sc, uint32_t primitive_limit, WebgpuPacking& out, const char* fn) {
   struct Host {
     bool ready = false;
     std::vector<uint32_t> nodes, vidx;
@@ -3158,24 +3184,38 @@ int pack_scene_webgpu(const mbvh_scene* sc, WebgpuPacking& out, const char* fn) 
     hm[g].ready = true;
     return MBVH_OK;
   };
-  // primitives: (instance, root) of every instance with a non-empty root
-  std::vector<std::pair<uint32_t, uint32_t>> prims;
+  // primitives: (instance, root, cluster node) of every instance's non-empty roots
+  std::vector<WebgpuPrim> prims;
   std::vector<float> prim_pos;
+  std::vector<uint64_t> walk;
   int rc;
   for (uint32_t i = 0; i < sc->n_instances; i++) {
     const uint32_t g = sc->geom_h[i];
     if ((rc = fetch(g))) return rc;
     const mbvh_mesh* m = sc->meshes[g];
     for (uint32_t r = 0; r < m->root_base.size(); r++) {
-      const float* b = reinterpret_cast<const float*>(hm[g].nodes.data() + 8 * m->root_base[r]);
-      if (!(b[0] <= b[3] && b[1] <= b[4] && b[2] <= b[5])) continue;  // no triangles
-      const double lb[6] = {b[0], b[1], b[2], b[3], b[4], b[5]};
-      float fb[6];
-      if (!world_box_of(sc->world_h.data() + 16ull * i, lb, fb))
-        return set_error(MBVH_ERR_BAD_ARGS, "%s: instance %u has a non-finite world box", fn, i);
-      prims.push_back({i, r});
-      const float v[9] = {fb[0], fb[1], fb[2], fb[3], fb[4], fb[5], fb[0], fb[1], fb[2]};
-      prim_pos.insert(prim_pos.end(), v, v + 9);
+      const uint32_t* rn = hm[g].nodes.data() + 8 * m->root_base[r];
+      const float* rb = reinterpret_cast<const float*>(rn);
+      if (!(rb[0] <= rb[3] && rb[1] <= rb[4] && rb[2] <= rb[5])) continue;  // no triangles
+      walk.assign(1, 0);
+      while (!walk.empty()) {
+        const uint64_t n = walk.back();
+        walk.pop_back();
+        const bool leaf = (rn[8 * n + 7] >> 16) == 0xFFFFu;
+        if (!leaf && primitive_limit != 0 && subtree_triangles(rn, n) >= primitive_limit) {
+          walk.push_back(n + rn[8 * n + 6]);  // right below left: the left subtree comes first
+          walk.push_back(n + 1);
+          continue;
+        }
+        const float* b = rb + 8 * n;
+        const double lb[6] = {b[0], b[1], b[2], b[3], b[4], b[5]};
+        float fb[6];
+        if (!world_box_of(sc->world_h.data() + 16ull * i, lb, fb))
+          return set_error(MBVH_ERR_BAD_ARGS, "%s: instance %u has a non-finite world box", fn, i);
+        prims.push_back({i, r, static_cast<uint32_t>(n)});
+        const float v[9] = {fb[0], fb[1], fb[2], fb[3], fb[4], fb[5], fb[0], fb[1], fb[2]};
+        prim_pos.insert(prim_pos.end(), v, v + 9);
+      }
     }
   }
   const uint64_t np = prims.size();
@@ -3205,16 +3245,21 @@ int pack_scene_webgpu(const mbvh_scene* sc, WebgpuPacking& out, const char* fn) 
   const uint32_t* tw = reinterpret_cast<const uint32_t*>(tlas.data());
   for (uint64_t k = 0; k < n_tlas && np; k++)
     if ((tw[8 * k + 7] >> 16) == 0xFFFFu && (tw[8 * k + 7] & 0xFFFFu) != 1u)
-      return set_error(MBVH_ERR_LIMIT, "%s: instances with identical world boxes share a top-level leaf", fn);
+      return set_error(MBVH_ERR_LIMIT, "%s: primitives with identical world boxes share a top-level leaf", fn);
   // subtrees and geometry in the order the top-level slots first name them
-  std::vector<std::vector<int64_t>> subtree(sc->meshes.size());  // per mesh, per root: node base or -1
+  struct Subtree {
+    uint32_t mesh, root, node;
+    uint64_t base, span;
+  };
+  std::map<std::tuple<uint32_t, uint32_t, uint32_t>, uint64_t> sub_of;  // (mesh, root, node) -> entry
+  std::vector<Subtree> subs;
   std::vector<int64_t> tri_base(sc->meshes.size(), -1), vert_base(sc->meshes.size(), -1);
   std::vector<uint32_t> mesh_order;
-  std::vector<std::pair<uint32_t, uint32_t>> sub_order;  // (mesh, root)
   uint64_t next_node = n_tlas, next_tri = 0, next_vert = 0;
   std::vector<uint64_t> leaf_base(np);
   for (uint64_t k = 0; k < np; k++) {
-    const uint32_t inst = prims[order[k]].first, r = prims[order[k]].second, g = sc->geom_h[inst];
+    const WebgpuPrim& p = prims[order[k]];
+    const uint32_t g = sc->geom_h[p.instance];
     const mbvh_mesh* m = sc->meshes[g];
     if (tri_base[g] < 0) {
       tri_base[g] = static_cast<int64_t>(next_tri);
@@ -3222,14 +3267,16 @@ int pack_scene_webgpu(const mbvh_scene* sc, WebgpuPacking& out, const char* fn) 
       next_tri += m->n_slots;
       next_vert += m->vertex_count;
       mesh_order.push_back(g);
-      subtree[g].assign(m->root_base.size(), -1);
     }
-    if (subtree[g][r] < 0) {
-      subtree[g][r] = static_cast<int64_t>(next_node);
-      next_node += m->root_nodes[r];
-      sub_order.push_back({g, r});
+    const auto key = std::make_tuple(g, p.root, p.node);
+    auto it = sub_of.find(key);
+    if (it == sub_of.end()) {
+      const uint64_t span = subtree_span(hm[g].nodes.data() + 8 * m->root_base[p.root], p.node);
+      it = sub_of.emplace(key, subs.size()).first;
+      subs.push_back({g, p.root, p.node, next_node, span});
+      next_node += span;
     }
-    leaf_base[k] = static_cast<uint64_t>(subtree[g][r]);
+    leaf_base[k] = subs[it->second].base;
   }
   if (next_node >= (uint64_t{1} << 32)) return set_error(MBVH_ERR_LIMIT, "%s: more than 2^32 nodes", fn);
   out.nodes.assign(8 * next_node, 0u);
@@ -3258,12 +3305,12 @@ int pack_scene_webgpu(const mbvh_scene* sc, WebgpuPacking& out, const char* fn) 
     }
   }
   // the subtrees (appendBVHSubtree)
-  for (const auto& gr : sub_order) {
-    const uint32_t g = gr.first, r = gr.second;
+  for (const Subtree& sb : subs) {
+    const uint32_t g = sb.mesh;
     const mbvh_mesh* m = sc->meshes[g];
-    const uint32_t* src0 = hm[g].nodes.data() + 8 * m->root_base[r];
-    uint32_t* dst0 = out.nodes.data() + 8 * static_cast<uint64_t>(subtree[g][r]);
-    for (uint64_t k = 0; k < m->root_nodes[r]; k++) {
+    const uint32_t* src0 = hm[g].nodes.data() + 8 * (m->root_base[sb.root] + sb.node);
+    uint32_t* dst0 = out.nodes.data() + 8 * sb.base;
+    for (uint64_t k = 0; k < sb.span; k++) {
       const uint32_t* src = src0 + 8 * k;
       uint32_t* dst = dst0 + 8 * k;
       for (int c = 0; c < 6; c++) dst[c] = src[c];
@@ -3288,7 +3335,7 @@ int pack_scene_webgpu(const mbvh_scene* sc, WebgpuPacking& out, const char* fn) 
   // transforms, one per slot
   out.transforms.assign(36 * std::max<uint64_t>(np, 1), 0u);
   for (uint64_t k = 0; k < np; k++) {
-    const uint32_t inst = prims[order[k]].first;
+    const uint32_t inst = prims[order[k]].instance;
     std::memcpy(out.transforms.data() + 36 * k, sc->world_h.data() + 16ull * inst, 64);
     std::memcpy(out.transforms.data() + 36 * k + 16, sc->frame_h.data() + 16ull * inst, 64);
     out.transforms[36 * k + 32] = sc->visible_h[inst];
@@ -3305,8 +3352,8 @@ int pack_scene_webgpu(const mbvh_scene* sc, WebgpuPacking& out, const char* fn) 
 
 }  // namespace
 
-extern "C" int mbvh_scene_webgpu_sizes(mbvh_ctx* ctx, const mbvh_scene* scene, uint64_t* n_nodes,
-                                       uint64_t* n_transforms, uint64_t* n_index) {
+extern "C" int mbvh_scene_webgpu_sizes(mbvh_ctx* ctx, const mbvh_scene* scene, uint32_t primitive_limit,
+                                       uint64_t* n_nodes, uint64_t* n_transforms, uint64_t* n_index) {
   clear_error();
   const char* fn = "mbvh_scene_webgpu_sizes";
   if (!valid_ctx(ctx) || !valid_scene(scene) || scene->ctx != ctx)
@@ -3315,14 +3362,15 @@ extern "C" int mbvh_scene_webgpu_sizes(mbvh_ctx* ctx, const mbvh_scene* scene, u
   int rc = set_device(ctx);
   if (rc) return rc;
   WebgpuPacking pk;
-  if ((rc = pack_scene_webgpu(scene, pk, fn))) return rc;
+  if ((rc = pack_scene_webgpu(scene, primitive_limit, pk, fn))) return rc;
   if (n_nodes) *n_nodes = pk.nodes.size() / 8;
   if (n_transforms) *n_transforms = pk.transforms.size() / 36;
   if (n_index) *n_index = pk.index.size();
   return MBVH_OK;
 }
 
-extern "C" int mbvh_scene_export_webgpu(mbvh_ctx* ctx, const mbvh_scene* scene, uint32_t* nodes, uint64_t nodes_capacity,
+extern "C" int mbvh_scene_export_webgpu(mbvh_ctx* ctx, const mbvh_scene* scene, uint32_t primitive_limit,
+                                        uint32_t* nodes, uint64_t nodes_capacity,
                                         uint32_t* transforms, uint64_t transforms_capacity, uint32_t* index,
                                         uint64_t index_capacity) {
   clear_error();
@@ -3333,7 +3381,7 @@ extern "C" int mbvh_scene_export_webgpu(mbvh_ctx* ctx, const mbvh_scene* scene, 
   int rc = set_device(ctx);
   if (rc) return rc;
   WebgpuPacking pk;
-  if ((rc = pack_scene_webgpu(scene, pk, fn))) return rc;
+  if ((rc = pack_scene_webgpu(scene, primitive_limit, pk, fn))) return rc;
   if ((nodes && nodes_capacity < pk.nodes.size()) || (transforms && transforms_capacity < pk.transforms.size()) ||
       (index && index_capacity < pk.index.size()))
     return set_error(MBVH_ERR_BAD_ARGS, "%s: an output holds fewer words than mbvh_scene_webgpu_sizes gives", fn);

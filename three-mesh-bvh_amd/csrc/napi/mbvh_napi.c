@@ -4,10 +4,14 @@
  * in the library (kernels, upload) and in the JavaScript package (the MeshBVH API).
  *
  * Device contexts are process-wide, one per device, created on first use and destroyed by an
- * environment cleanup hook, so mesh finalizers never outlive their context.
+ * environment cleanup hook. Node runs that hook before the finalizers of externals, so the hook first
+ * releases every scene, batch and mesh still alive (the handles are kept in lists) and the
+ * finalizers that run after it find nothing left to free on a destroyed context.
  */
-#define NAPI_VERSION 3
+#define NAPI_VERSION 4
+#include <math.h>
 #include <node_api.h>
+#include <pthread.h>
 
 #include <stdint.h>
 #include <stdio.h>
@@ -26,6 +30,7 @@ typedef struct MeshHandle {
   int device;
   int batch_refs;  /* batches using this mesh: it cannot be freed explicitly meanwhile */
   int async_refs;  /* asynchronous queries in flight on this mesh: likewise */
+  struct MeshHandle *prev, *next;  /* the live mesh handles */
 } MeshHandle;
 
 typedef struct BatchHandle {
@@ -35,7 +40,40 @@ typedef struct BatchHandle {
   uint32_t n_meshes;
   MeshHandle** meshes;  /* kept alive (and unfreeable) while the batch exists */
   napi_ref* refs;
+  struct BatchHandle *prev, *next;  /* the live batch / scene handles */
 } BatchHandle;
+
+/* Handles whose externals exist, for the environment cleanup hook (main thread only). */
+static MeshHandle* g_meshes;
+static BatchHandle* g_batches;
+
+static void link_mesh(MeshHandle* h) {
+  h->prev = NULL;
+  h->next = g_meshes;
+  if (g_meshes) g_meshes->prev = h;
+  g_meshes = h;
+}
+
+static void unlink_mesh(MeshHandle* h) {
+  if (h->prev) h->prev->next = h->next;
+  else if (g_meshes == h) g_meshes = h->next;
+  if (h->next) h->next->prev = h->prev;
+  h->prev = h->next = NULL;
+}
+
+static void link_batch(BatchHandle* b) {
+  b->prev = NULL;
+  b->next = g_batches;
+  if (g_batches) g_batches->prev = b;
+  g_batches = b;
+}
+
+static void unlink_batch(BatchHandle* b) {
+  if (b->prev) b->prev->next = b->next;
+  else if (g_batches == b) g_batches = b->next;
+  if (b->next) b->next->prev = b->prev;
+  b->prev = b->next = NULL;
+}
 
 /* ------------------------------------------------------------------------------------------ */
 
@@ -143,8 +181,22 @@ static int context_for(napi_env env, int32_t device, mbvh_ctx** out) {
   return MBVH_OK;
 }
 
+static void stop_query_thread(void);
+
 static void cleanup_contexts(void* arg) {
   (void)arg;
+  stop_query_thread();
+  /* scenes and batches before the meshes they hold, every library object before its context */
+  for (BatchHandle* b = g_batches; b; b = b->next) {
+    if (b->batch) mbvh_batch_free(b->batch);
+    if (b->scene) mbvh_scene_free(b->scene);
+    b->batch = NULL;
+    b->scene = NULL;
+  }
+  for (MeshHandle* h = g_meshes; h; h = h->next) {
+    if (h->mesh) mbvh_mesh_free(h->mesh);
+    h->mesh = NULL;
+  }
   for (int i = 0; i < MAX_DEVICES; i++) {
     if (g_ctx[i]) mbvh_ctx_destroy(g_ctx[i]);
     g_ctx[i] = NULL;
@@ -156,6 +208,7 @@ static void finalize_mesh(napi_env env, void* data, void* hint) {
   (void)hint;
   MeshHandle* h = (MeshHandle*)data;
   if (h) {
+    unlink_mesh(h);
     if (h->mesh) mbvh_mesh_free(h->mesh);
     free(h);
   }
@@ -338,6 +391,7 @@ static napi_value js_upload(napi_env env, napi_callback_info info) {
     free(h);
     return throw_msg(env, "uploadMesh: cannot wrap the mesh");
   }
+  link_mesh(h);
   return ext;
 }
 
@@ -376,6 +430,7 @@ static void release_batch(napi_env env, BatchHandle* b) {
 static void finalize_batch(napi_env env, void* data, void* hint) {
   (void)hint;
   BatchHandle* b = (BatchHandle*)data;
+  unlink_batch(b);
   release_batch(env, b);
   free(b);
 }
@@ -447,7 +502,12 @@ static napi_value js_batch_create(napi_env env, napi_callback_info info) {
     return throw_status(env, rc, "mbvh_batch_create");
   }
   napi_value out;
-  napi_create_external(env, b, finalize_batch, NULL, &out);
+  if (napi_create_external(env, b, finalize_batch, NULL, &out) != napi_ok) {
+    release_batch(env, b);
+    free(b);
+    return throw_msg(env, "batchCreate: cannot wrap the handle");
+  }
+  link_batch(b);
   return out;
 }
 
@@ -574,7 +634,12 @@ static napi_value js_scene_create(napi_env env, napi_callback_info info) {
     return throw_status(env, rc, "mbvh_scene_create");
   }
   napi_value out;
-  napi_create_external(env, b, finalize_batch, NULL, &out);
+  if (napi_create_external(env, b, finalize_batch, NULL, &out) != napi_ok) {
+    release_batch(env, b);
+    free(b);
+    return throw_msg(env, "sceneCreate: cannot wrap the handle");
+  }
+  link_batch(b);
   return out;
 }
 
@@ -801,7 +866,7 @@ static int get_rays(napi_env env, napi_value vo, napi_value vd, View* o, View* d
 }
 
 /* One first-hit or closest-point query, parsed from its arguments; run on the calling thread
- * (raycastFirst, closestPoint) or on a libuv worker thread (raycastFirstAsync, closestPointAsync). */
+ * (raycastFirst, closestPoint) or on the addon's query thread (raycastFirstAsync, closestPointAsync). */
 typedef struct Query {
   int kind;  /* 0: first hit, 1: closest point */
   MeshHandle* h;
@@ -816,9 +881,9 @@ typedef struct Query {
   int rc;
   char msg[512];  /* mbvh_last_error() of the thread that ran it */
   napi_deferred deferred;
-  napi_async_work work;
   napi_ref refs[8];  /* the mesh and the typed arrays, alive until an asynchronous query settles */
   int n_refs;
+  struct Query* next;  /* the query thread's FIFO */
 } Query;
 
 /* raycastFirst(mesh, origins, dirs, side, near, far, face, dist, point, bary): origins / dirs either
@@ -911,21 +976,54 @@ static napi_value js_closest_point(napi_env env, napi_callback_info info) {
   return run_sync(env, info, 8, parse_closest_point);
 }
 
-/* ---- asynchronous queries: napi_async_work runs the C call on a libuv worker thread (calls on one
- * device context are serialised by the library), and a Promise settles on the main thread ---- */
-static void async_execute(napi_env env, void* data) {
-  (void)env;
-  run_query((Query*)data);
+/* ---- asynchronous queries: ONE query thread owned by the addon runs the C calls in submission
+ * order (calls on one device context are serialised by the library anyway), and a thread-safe
+ * function settles each Promise on the main thread. Not libuv's worker pool: libuv joins its pool
+ * threads at process exit after the shared libraries' destructors have run, so the per-thread state
+ * the HIP runtime keeps for a thread that called it would be torn down after the runtime itself. The
+ * environment cleanup hook stops and joins the query thread while the runtime is still alive. ---- */
+typedef struct {
+  pthread_mutex_t mu;
+  pthread_cond_t cv;
+  Query *head, *tail;  /* submitted, not yet run */
+  int stop;
+  int started;
+  pthread_t thread;
+  napi_threadsafe_function done;  /* settles a finished query's Promise */
+  uint32_t in_flight;             /* main thread: submitted and not yet settled (keeps the loop alive) */
+} QueryThread;
+
+static QueryThread g_qt = {PTHREAD_MUTEX_INITIALIZER, PTHREAD_COND_INITIALIZER, NULL, NULL, 0, 0, 0, NULL, 0};
+
+static void* query_main(void* arg) {
+  (void)arg;
+  for (;;) {
+    pthread_mutex_lock(&g_qt.mu);
+    while (!g_qt.head && !g_qt.stop) pthread_cond_wait(&g_qt.cv, &g_qt.mu);
+    Query* q = NULL;
+    if (!g_qt.stop) {
+      q = g_qt.head;
+      g_qt.head = q->next;
+      if (!g_qt.head) g_qt.tail = NULL;
+    }
+    pthread_mutex_unlock(&g_qt.mu);
+    if (!q) return NULL;  /* stopping: queries not yet run are dropped with the environment */
+    run_query(q);
+    napi_call_threadsafe_function(g_qt.done, q, napi_tsfn_blocking);
+  }
 }
 
-static void async_complete(napi_env env, napi_status status, void* data) {
+/* Main thread: a finished query's Promise (env NULL: the environment is going away). */
+static void settle_query(napi_env env, napi_value js_cb, void* context, void* data) {
+  (void)js_cb;
+  (void)context;
   Query* q = (Query*)data;
+  if (!env) {
+    free(q);
+    return;
+  }
   for (int i = 0; i < q->n_refs; i++) napi_delete_reference(env, q->refs[i]);
   q->h->async_refs--;
-  if (status != napi_ok && !q->rc) {
-    q->rc = MBVH_ERR_BAD_ARGS;
-    snprintf(q->msg, sizeof(q->msg), "the query was cancelled before it ran");
-  }
   if (q->rc) {
     char msg[768], code[32];
     snprintf(msg, sizeof(msg), "%s failed (mbvh status %d): %s", query_where(q), q->rc, q->msg);
@@ -940,12 +1038,49 @@ static void async_complete(napi_env env, napi_status status, void* data) {
     napi_get_undefined(env, &undef);
     napi_resolve_deferred(env, q->deferred, undef);
   }
-  napi_delete_async_work(env, q->work);
   free(q);
+  if (g_qt.in_flight > 0 && --g_qt.in_flight == 0) napi_unref_threadsafe_function(env, g_qt.done);
+}
+
+static napi_value js_noop(napi_env env, napi_callback_info info) {
+  (void)env;
+  (void)info;
+  return NULL;
+}
+
+/* Main thread: starts the query thread on first use. */
+static int start_query_thread(napi_env env) {
+  if (g_qt.started) return 1;
+  napi_value name, fn;
+  if (napi_create_string_utf8(env, "mbvh.query", NAPI_AUTO_LENGTH, &name) != napi_ok ||
+      napi_create_function(env, "mbvhQueryDone", NAPI_AUTO_LENGTH, js_noop, NULL, &fn) != napi_ok ||
+      napi_create_threadsafe_function(env, fn, NULL, name, 0, 1, NULL, NULL, NULL, settle_query, &g_qt.done) != napi_ok)
+    return 0;
+  napi_unref_threadsafe_function(env, g_qt.done);  /* referenced only while queries are in flight */
+  g_qt.stop = 0;
+  if (pthread_create(&g_qt.thread, NULL, query_main, NULL) != 0) {
+    napi_release_threadsafe_function(g_qt.done, napi_tsfn_abort);
+    g_qt.done = NULL;
+    return 0;
+  }
+  g_qt.started = 1;
+  return 1;
+}
+
+/* Environment cleanup: stop and join the query thread (before any library object is freed). */
+static void stop_query_thread(void) {
+  if (!g_qt.started) return;
+  pthread_mutex_lock(&g_qt.mu);
+  g_qt.stop = 1;
+  pthread_cond_signal(&g_qt.cv);
+  pthread_mutex_unlock(&g_qt.mu);
+  pthread_join(g_qt.thread, NULL);
+  g_qt.started = 0;
 }
 
 static napi_value run_async(napi_env env, napi_callback_info info, size_t argc,
                             const char* (*parse)(napi_env, napi_value*, Query*), const char* name) {
+  (void)name;
   napi_value argv[10];
   get_args(env, info, argc, argv);
   Query* q = (Query*)malloc(sizeof(Query));
@@ -957,8 +1092,13 @@ static napi_value run_async(napi_env env, napi_callback_info info, size_t argc,
   }
   int rc = context_for(env, q->h->device, &q->ctx);  /* created here, on the main thread */
   if (rc) {
+    const char* where = query_where(q);
     free(q);
-    return throw_status(env, rc, query_where(q));
+    return throw_status(env, rc, where);
+  }
+  if (!start_query_thread(env)) {
+    free(q);
+    return throw_msg(env, "could not start the query thread");
   }
   /* the mesh external and every typed-array argument stay referenced until the promise settles */
   for (size_t i = 0; i < argc; i++) {
@@ -966,16 +1106,19 @@ static napi_value run_async(napi_env env, napi_callback_info info, size_t argc,
     napi_typeof(env, argv[i], &t);
     if ((t == napi_object || t == napi_external) && q->n_refs < 8) napi_create_reference(env, argv[i], 1, &q->refs[q->n_refs++]);
   }
-  napi_value promise, resource;
+  napi_value promise;
   napi_create_promise(env, &q->deferred, &promise);
-  napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &resource);
-  if (napi_create_async_work(env, NULL, resource, async_execute, async_complete, q, &q->work) != napi_ok ||
-      napi_queue_async_work(env, q->work) != napi_ok) {
-    for (int i = 0; i < q->n_refs; i++) napi_delete_reference(env, q->refs[i]);
-    free(q);
-    return throw_msg(env, "could not queue the asynchronous query");
-  }
   q->h->async_refs++;
+  if (g_qt.in_flight++ == 0) napi_ref_threadsafe_function(env, g_qt.done);
+  q->next = NULL;
+  pthread_mutex_lock(&g_qt.mu);
+  if (g_qt.tail)
+    g_qt.tail->next = q;
+  else
+    g_qt.head = q;
+  g_qt.tail = q;
+  pthread_cond_signal(&g_qt.cv);
+  pthread_mutex_unlock(&g_qt.mu);
   return promise;
 }
 
@@ -1173,8 +1316,9 @@ static napi_value js_intersects_sphere(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
-/* sceneExportWebGPU(scene) -> { nodes, transforms, index } (Uint32Arrays): the scene in the reference's
- * WebGPU packing (mbvh_scene_export_webgpu) */
+/* sceneExportWebGPU(scene, primitiveLimit = Infinity) -> { nodes, transforms, index } (Uint32Arrays): the
+ * scene in the reference's WebGPU packing (mbvh_scene_export_webgpu); roots are cut into clusters of
+ * fewer than primitiveLimit triangles, Infinity keeps them whole */
 static napi_value new_u32(napi_env env, uint64_t n, uint32_t** data) {
   napi_value ab, out;
   void* p = NULL;
@@ -1185,8 +1329,15 @@ static napi_value new_u32(napi_env env, uint64_t n, uint32_t** data) {
 }
 
 static napi_value js_scene_export_webgpu(napi_env env, napi_callback_info info) {
-  napi_value argv[1];
-  get_args(env, info, 1, argv);
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  uint32_t limit = 0; /* 0: whole roots */
+  if (!is_nullish(env, argv[1])) {
+    double v;
+    if (!get_f64(env, argv[1], &v)) return throw_msg(env, "sceneExportWebGPU: primitiveLimit must be a number");
+    /* a node becomes a cluster when it spans fewer than v triangles: count < v <=> count < ceil(v) */
+    if (v < 4294967296.0) limit = v < 1.0 ? 1u : (uint32_t)ceil(v);
+  }
   void* p = NULL;
   if (napi_get_value_external(env, argv[0], &p) != napi_ok || !p || !((BatchHandle*)p)->scene)
     return throw_msg(env, "sceneExportWebGPU: invalid scene");
@@ -1194,12 +1345,12 @@ static napi_value js_scene_export_webgpu(napi_env env, napi_callback_info info) 
   mbvh_ctx* ctx = NULL;
   uint64_t nn = 0, nt = 0, ni = 0;
   int rc = context_for(env, b->device, &ctx);
-  if (!rc) rc = mbvh_scene_webgpu_sizes(ctx, b->scene, &nn, &nt, &ni);
+  if (!rc) rc = mbvh_scene_webgpu_sizes(ctx, b->scene, limit, &nn, &nt, &ni);
   if (rc) return throw_status(env, rc, "mbvh_scene_webgpu_sizes");
   uint32_t *pn = NULL, *pt = NULL, *pi = NULL;
   napi_value vn = new_u32(env, 8 * nn, &pn), vt = new_u32(env, 36 * nt, &pt), vi = new_u32(env, ni, &pi);
   if (!vn || !vt || !vi) return throw_msg(env, "sceneExportWebGPU: out of memory");
-  rc = mbvh_scene_export_webgpu(ctx, b->scene, pn, 8 * nn, pt, 36 * nt, pi, ni);
+  rc = mbvh_scene_export_webgpu(ctx, b->scene, limit, pn, 8 * nn, pt, 36 * nt, pi, ni);
   if (rc) return throw_status(env, rc, "mbvh_scene_export_webgpu");
   napi_value out;
   napi_create_object(env, &out);

@@ -145,8 +145,8 @@ def library() -> C.CDLL:
         "mbvh_scene_raycast_first": (C.c_int, [vp, vp, vp, vp, u64, i32, f32, f32, vp, vp, vp, vp, vp]),
         "mbvh_scene_closest_point": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp]),
         "mbvh_scene_closest_point_d": (C.c_int, [vp, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp]),
-        "mbvh_scene_webgpu_sizes": (C.c_int, [vp, vp, vp, vp, vp]),
-        "mbvh_scene_export_webgpu": (C.c_int, [vp, vp, vp, u64, vp, u64, vp, u64]),
+        "mbvh_scene_webgpu_sizes": (C.c_int, [vp, vp, u32, vp, vp, vp]),
+        "mbvh_scene_export_webgpu": (C.c_int, [vp, vp, u32, vp, u64, vp, u64, vp, u64]),
         "mbvh_intersects_box": (C.c_int, [vp, vp, vp, vp, vp, u64, vp]),
         "mbvh_intersects_box_d": (C.c_int, [vp, vp, vp, vp, vp, u64, vp, vp]),
         "mbvh_closest_point_to_geometry": (C.c_int, [vp, vp, vp, u64, u32, u32, vp, u32, u64, vp, f64, f64, vp]),
@@ -893,17 +893,18 @@ class Scene:
             _ptr(out["distance"]), _ptr(out["point"]), _ptr(out["barycoord"])))
         return out
 
-    def export_webgpu(self):
+    def export_webgpu(self, primitive_limit: int = 0):
         """The scene in the reference's WebGPU packing: {'nodes': uint32 (N, 8), 'transforms': uint32 (S, 36),
-        'index': uint32 (3T,)} as mbvh_scene_export_webgpu writes them."""
+        'index': uint32 (3T,)} as mbvh_scene_export_webgpu writes them; primitive_limit > 0 cuts the roots
+        into clusters of fewer triangles (0: whole roots)."""
         nn, nt, ni = C.c_uint64(), C.c_uint64(), C.c_uint64()
-        _check(self.ctx._lib.mbvh_scene_webgpu_sizes(self.ctx.handle, self.handle, C.byref(nn), C.byref(nt),
-                                                     C.byref(ni)))
+        _check(self.ctx._lib.mbvh_scene_webgpu_sizes(self.ctx.handle, self.handle, primitive_limit, C.byref(nn),
+                                                     C.byref(nt), C.byref(ni)))
         nodes = np.empty((nn.value, 8), np.uint32)
         tr = np.empty((nt.value, 36), np.uint32)
         idx = np.empty(max(ni.value, 1), np.uint32)
-        _check(self.ctx._lib.mbvh_scene_export_webgpu(self.ctx.handle, self.handle, _ptr(nodes), nodes.size, _ptr(tr),
-                                                      tr.size, _ptr(idx), ni.value))
+        _check(self.ctx._lib.mbvh_scene_export_webgpu(self.ctx.handle, self.handle, primitive_limit, _ptr(nodes),
+                                                      nodes.size, _ptr(tr), tr.size, _ptr(idx), ni.value))
         return {"nodes": nodes, "transforms": tr, "index": idx[:ni.value]}
 
     def closest_point_d(self, points_ptr: int, n: int, face_ptr: int, instance_ptr: int = 0, dist_ptr: int = 0,

@@ -295,8 +295,8 @@ class MeshBVH {
     return out;
   }
 
-  // The same two batch queries without blocking the event loop: the device call runs on a libuv
-  // worker thread and the Promise resolves to the same packed results. The input arrays must not
+  // The same two batch queries without blocking the event loop: the device call runs on the
+  // addon's query thread and the Promise resolves to the same packed results. The input arrays must not
   // change until it settles; the BVH cannot be disposed meanwhile (dispose() throws).
   raycastFirstBatchAsync(origins, directions, options = {}) {
     const [o, d] = queryArrays(origins, directions);
@@ -500,6 +500,78 @@ class MeshBVH {
     target.distance = r.distance[0];
     target.faceIndex = r.faceIndex[0];
     return target;
+  }
+
+  // ---- micro-batched per-call queries -----------------------------------------------------------
+  // raycastFirstAsync / closestPointToPointAsync take what raycastFirst / closestPointToPoint take
+  // and return Promises of what those return. Calls made in one turn of the event loop with the same
+  // material (or side) and near / far, or the same thresholds, are answered by ONE asynchronous batch
+  // launch when the turn ends (setImmediate), so a loop of per-call queries costs one device round
+  // trip instead of one each. The tree is read when the batch runs: a refit later in the same turn
+  // applies to the calls queued before it.
+  raycastFirstAsync(ray, materialOrSide = C.FrontSide, near = 0, far = Infinity) {
+    const { origin: o, direction: d } = ray;
+    return this._enqueue('ray', materialOrSide, near, far, [o.x, o.y, o.z], [d.x, d.y, d.z], ray);
+  }
+
+  closestPointToPointAsync(point, target = {}, minThreshold = 0, maxThreshold = Infinity) {
+    return this._enqueue('point', null, minThreshold, maxThreshold, [point.x, point.y, point.z], null,
+      { point, target });
+  }
+
+  _enqueue(kind, materialOrSide, a, b, first, second, item) {
+    return new Promise((resolve, reject) => {
+      if (!this._queued) this._queued = [];
+      let g = this._queued.find(x => x.kind === kind && x.materialOrSide === materialOrSide &&
+        Object.is(x.a, a) && Object.is(x.b, b));
+      if (!g) {
+        g = { kind, materialOrSide, a, b, first: [], second: [], items: [] };
+        this._queued.push(g);
+      }
+      g.first.push(first[0], first[1], first[2]);
+      if (second) g.second.push(second[0], second[1], second[2]);
+      g.items.push({ item, resolve, reject });
+      if (this._queued.length === 1 && g.items.length === 1) {
+        const later = typeof setImmediate === 'function' ? setImmediate : f => setTimeout(f, 0);
+        later(() => this._flushQueued());
+      }
+    });
+  }
+
+  _flushQueued() {
+    const groups = this._queued || [];
+    this._queued = null;
+    for (const g of groups) {
+      let p;
+      try {
+        p = g.kind === 'ray'
+          ? this.raycastFirstBatchAsync(Float64Array.from(g.first), Float64Array.from(g.second),
+            { side: g.materialOrSide, near: g.a, far: g.b })
+          : this.closestPointToPointBatchAsync(Float64Array.from(g.first), { minThreshold: g.a, maxThreshold: g.b });
+      } catch (e) {
+        p = Promise.reject(e);
+      }
+      p.then((r) => {
+        g.items.forEach(({ item, resolve }, k) => {
+          if (r.faceIndex[k] === C.NO_HIT) {
+            resolve(null);
+            return;
+          }
+          const pt = [r.point[3 * k], r.point[3 * k + 1], r.point[3 * k + 2]];
+          if (g.kind === 'ray') {
+            resolve(this._makeHit(item, g.materialOrSide, r.faceIndex[k], r.distance[k], pt,
+              [r.barycoord[3 * k], r.barycoord[3 * k + 1], r.barycoord[3 * k + 2]]));
+            return;
+          }
+          const { point, target } = item;
+          if (!target.point) target.point = M.makeVec3(point, pt[0], pt[1], pt[2]);
+          else target.point.set(pt[0], pt[1], pt[2]);
+          target.distance = r.distance[k];
+          target.faceIndex = r.faceIndex[k];
+          resolve(target);
+        });
+      }, (e) => g.items.forEach(({ reject }) => reject(e)));
+    }
   }
 
   // raycastObject3D (src/core/MeshBVH.js:447-494): world ray to local, query, hits back to world.

@@ -136,14 +136,18 @@ class SceneBVH {
     return target;
   }
 
-  // The scene in the reference's WebGPU packing (BVHComputeData.update with every instance referenced
-  // whole, src/webgpu/BVHComputeData.js:217-447): { nodes, transforms, index } Uint32Arrays — BVHNode
-  // words (top level first: a leaf holds its subtree's node offset and 0xFF000000 | transform slot;
-  // then each mesh root once, leaves rebased into the index), 36 words per transform slot (matrixWorld,
-  // its inverse, visible, padding) and the packed index (meshes in the order the top level first
-  // names them, each rebased by the vertex count of the meshes before it).
-  exportWebGPU() {
-    return native.load().sceneExportWebGPU(this._scene);
+  // The scene in the reference's WebGPU packing (BVHComputeData.update over ClusteredBVH's primitives,
+  // src/webgpu/BVHComputeData.js:217-447, src/webgpu/ClusteredBVH.js:207-331): { nodes, transforms,
+  // index } Uint32Arrays — BVHNode words (top level first: a leaf holds its subtree's node offset and
+  // 0xFF000000 | transform slot; then each referenced subtree once, leaves rebased into the index), 36
+  // words per transform slot (matrixWorld, its inverse, visible, padding) and the packed index (meshes
+  // in the order the top level first names them, each rebased by the vertex count of the meshes before
+  // it). primitiveLimit as BVHComputeData chooses it: a BatchedMesh's instances, and scenes of fewer
+  // than three meshes, are referenced whole (Infinity); other scenes cut every root into clusters of
+  // fewer than 64 triangles.
+  exportWebGPU({ primitiveLimit } = {}) {
+    if (primitiveLimit === undefined) primitiveLimit = this.object || this.instances.length < 3 ? Infinity : 64;
+    return native.load().sceneExportWebGPU(this._scene, primitiveLimit);
   }
 
   dispose() {
