@@ -1854,12 +1854,23 @@ __global__ __launch_bounds__(kBlock, 8) void k_closest_point_packet(ClosestPoint
             const float raw = box_score(q, cb0.x, cb0.y, cb0.z, cb0.w, cb1.x, cb1.y);
             reach = below_max<kLo>(true, raw, max_t2, q, ql, cb0.x, cb0.y, cb0.z, cb0.w, cb1.x, cb1.y);
           }
+          // the next triangle's scalar loads are issued before this one is tested, so their latency
+          // overlaps the test instead of stalling the wave at the top of every iteration
+          Tri nx;
+          if (cnt) {
+            ConstF* const tp = tris + 12ull * off;
+            nx.p0 = cload4(tp);
+            nx.p1 = cload4(tp + 4);
+            nx.p2 = cload4(tp + 8);
+          }
           for (uint32_t s = off; s < off + cnt; s++) {
-            ConstF* const tp = tris + 12ull * s;
-            Tri tr;
-            tr.p0 = cload4(tp);
-            tr.p1 = cload4(tp + 4);
-            tr.p2 = cload4(tp + 8);
+            const Tri tr = nx;
+            if (s + 1 < off + cnt) {
+              ConstF* const tp = tris + 12ull * (s + 1);
+              nx.p0 = cload4(tp);
+              nx.p1 = cload4(tp + 4);
+              nx.p2 = cload4(tp + 8);
+            }
             const float sb = box_score(q, fminf(fminf(tr.p0.x, tr.p1.x), tr.p2.x), fminf(fminf(tr.p0.y, tr.p1.y), tr.p2.y),
                                        fminf(fminf(tr.p0.z, tr.p1.z), tr.p2.z), fmaxf(fmaxf(tr.p0.x, tr.p1.x), tr.p2.x),
                                        fmaxf(fmaxf(tr.p0.y, tr.p1.y), tr.p2.y), fmaxf(fmaxf(tr.p0.z, tr.p1.z), tr.p2.z));
