@@ -1854,19 +1854,23 @@ __global__ __launch_bounds__(kBlock, 8) void k_closest_point_packet(ClosestPoint
             const float raw = box_score(q, cb0.x, cb0.y, cb0.z, cb0.w, cb1.x, cb1.y);
             reach = below_max<kLo>(true, raw, max_t2, q, ql, cb0.x, cb0.y, cb0.z, cb0.w, cb1.x, cb1.y);
           }
-          // the next triangle's scalar loads are issued before this one is tested, so their latency
+          // the leaf (and so its range) is the packet's: the walk stays in scalar registers, and the
+          // next triangle's scalar loads are issued before this one is tested so that their latency
           // overlaps the test instead of stalling the wave at the top of every iteration
+          off = __builtin_amdgcn_readfirstlane(off);
+          cnt = __builtin_amdgcn_readfirstlane(cnt);
+          ConstF* tp = tris + 12ull * off;
           Tri nx;
           if (cnt) {
-            ConstF* const tp = tris + 12ull * off;
             nx.p0 = cload4(tp);
             nx.p1 = cload4(tp + 4);
             nx.p2 = cload4(tp + 8);
           }
-          for (uint32_t s = off; s < off + cnt; s++) {
+          for (uint32_t k = 0; k < cnt; k++) {
+            const uint32_t s = off + k;
             const Tri tr = nx;
-            if (s + 1 < off + cnt) {
-              ConstF* const tp = tris + 12ull * (s + 1);
+            tp += 12;
+            if (k + 1 < cnt) {
               nx.p0 = cload4(tp);
               nx.p1 = cload4(tp + 4);
               nx.p2 = cload4(tp + 8);
