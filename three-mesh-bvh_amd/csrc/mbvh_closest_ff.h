@@ -31,7 +31,7 @@ __device__ __forceinline__ bool ff_ge0(ff x) { return ff_sign(x) >= 0; }
 // of the products and sums: each component within 2^-45 (|e1| + |e2|) + 2^-46 (|base| + |qp|),
 // which bounds the squared distance's error by 2^-43 (d R + d^2), R = |base|_1 + |e1|_1 + |e2|_1.
 template <bool kLo>
-__device__ __noinline__ ClosestFF closest_on_triangle_ff(f3 p, f3 pl, f3 a, f3 b, f3 c) {
+__device__ __forceinline__ ClosestFF closest_on_triangle_ff_inline(f3 p, f3 pl, f3 a, f3 b, f3 c) {
   auto diff = [&](f3 v) {  // p + pl - v
     ff3 r = ff3_diff(p, v);
     if (kLo) r = {ff_add(r.x, ff_from(pl.x)), ff_add(r.y, ff_from(pl.y)), ff_add(r.z, ff_from(pl.z))};
@@ -131,6 +131,13 @@ __device__ __noinline__ ClosestFF closest_on_triangle_ff(f3 p, f3 pl, f3 a, f3 b
   r.bu = bu;
   r.bv = bv;
   return r;
+}
+
+// The same, as one out-of-line function: callers that reach it from deep inside a traversal keep
+// their register budget; a kernel whose loop is mostly this call takes the inline form.
+template <bool kLo>
+__device__ __noinline__ ClosestFF closest_on_triangle_ff(f3 p, f3 pl, f3 a, f3 b, f3 c) {
+  return closest_on_triangle_ff_inline<kLo>(p, pl, a, b, c);
 }
 
 }  // namespace

@@ -1976,7 +1976,7 @@ __global__ __launch_bounds__(kBlock) void k_closest_point_resolve(ClosestPointAr
       }
       if (__uint_as_float(e.y) > U) continue;
       const Tri tr = load_tri(M.tris, e.x);
-      const ClosestFF cf = closest_on_triangle_ff<kLo>(q, ql, tr.a(), tr.b(), tr.c());
+      const ClosestFF cf = closest_on_triangle_ff_inline<kLo>(q, ql, tr.a(), tr.b(), tr.c());
       if (!(cf.dist2.hi == cf.dist2.hi)) continue;  // degenerate: never wins
       bool take = win == kNoHit;
       if (!take) {
