@@ -1792,6 +1792,9 @@ __global__ __launch_bounds__(kBlock, 8) void k_closest_point_packet(ClosestPoint
       while (ref != kDone) {
         if (!(ref & kRefLeaf)) {
           // ---- node step: the four children, scored by every lane ----
+          // all four records are loaded before any is scored (one scalar round trip per node, not
+          // one per child), and the maxThreshold decisions, which may call out of line, come after
+          ref = __builtin_amdgcn_readfirstlane(ref);
           ConstF* const nd = nodes4 + 32ull * ref;
           uint32_t keys[4];
           uint64_t need_any = 0;
@@ -1800,16 +1803,28 @@ __global__ __launch_bounds__(kBlock, 8) void k_closest_point_packet(ClosestPoint
           for (int k = 0; k < 4; k++) {
             rec0[k] = cload4(nd + 8 * k);
             rec1[k] = cload4(nd + 8 * k + 4);
+          }
+          float raw[4];
+          bool in[4];
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
             const uint32_t cref = __float_as_uint(rec1[k].z);
-            const float raw = box_score(q, rec0[k].x, rec0[k].y, rec0[k].z, rec0[k].w, rec1[k].x, rec1[k].y);
-            bool in = cref != kDone && raw * kScoreSlack <= U;
-            if (in && max_finite)
-              in = below_max<kLo>(true, raw, max_t2, q, ql, rec0[k].x, rec0[k].y, rec0[k].z, rec0[k].w, rec1[k].x,
-                                  rec1[k].y);
-            const uint64_t m = __ballot(in);
+            raw[k] = box_score(q, rec0[k].x, rec0[k].y, rec0[k].z, rec0[k].w, rec1[k].x, rec1[k].y);
+            in[k] = cref != kDone && raw[k] * kScoreSlack <= U;
+          }
+          if (max_finite) {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+              if (in[k])
+                in[k] = below_max<kLo>(true, raw[k], max_t2, q, ql, rec0[k].x, rec0[k].y, rec0[k].z, rec0[k].w,
+                                       rec1[k].x, rec1[k].y);
+          }
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const uint64_t m = __ballot(in[k]);
             need_any |= m ? (1ull << k) : 0ull;
             // the leader's score (bits of a non-negative float order as unsigned) with the child's index
-            const uint32_t kb = __builtin_amdgcn_readlane(__float_as_uint(raw), leader);
+            const uint32_t kb = __builtin_amdgcn_readlane(__float_as_uint(raw[k]), leader);
             keys[k] = m ? ((kb & ~3u) | static_cast<uint32_t>(k)) : 0xFFFFFFFFu;
           }
           // sort the (at most four) entered children nearest first
