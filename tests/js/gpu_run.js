@@ -308,9 +308,9 @@ const batchedChecked = bface.filter(f => f !== 0xFFFFFFFF).length, presenceDiffe
       best = Math.min(best, Math.hypot(wp[0] - px, wp[1] - py, wp[2] - pz));
     });
     const d = cpb.distance[i];
-    assert.ok(d <= best * (1 + 1e-4) + 1e-5, `scene closest point ${i}: ${d} vs ${best}`);
+    assert.ok(d <= best * (1 + 1e-5) + 1e-6, `scene closest point ${i}: ${d} vs ${best}`);
     const q = [cpb.point[3 * i], cpb.point[3 * i + 1], cpb.point[3 * i + 2]];
-    assert.ok(Math.abs(Math.hypot(q[0] - px, q[1] - py, q[2] - pz) - d) <= 1e-4 * Math.max(d, 1));
+    assert.ok(Math.abs(Math.hypot(q[0] - px, q[1] - py, q[2] - pz) - d) <= 1e-5 * Math.max(d, 1));
     const one = scene.closestPointToPoint({ x: px, y: py, z: pz });
     assert.ok(one && one.distance === d && one.batchId === cpb.instance[i] && one.faceIndex === cpb.faceIndex[i]);
   }
