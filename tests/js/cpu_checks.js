@@ -30,7 +30,8 @@ function check(name, fn) { checks.push([name, fn]); }
 
 check('exports the query API', () => {
   for (const k of ['MeshBVH', 'acceleratedRaycast', 'computeBoundsTree', 'disposeBoundsTree',
-    'computeBatchedBoundsTree', 'disposeBatchedBoundsTree', 'getTriangleHitPointInfo', 'SAH', 'CENTER', 'AVERAGE']) {
+    'computeBatchedBoundsTree', 'disposeBatchedBoundsTree', 'getTriangleHitPointInfo', 'SAH', 'CENTER', 'AVERAGE',
+    'setQueryLanes']) {
     assert.ok(api[k] !== undefined, k);
   }
   for (const m of ['raycast', 'raycastFirst', 'closestPointToPoint', 'raycastObject3D', 'raycastFirstBatch', 'refit', 'intersectsSphere', 'intersectsSphereBatch',

@@ -58,6 +58,15 @@ writeArr('rf_dist.bin', rf.distance);
 writeArr('rf_point.bin', rf.point);
 writeArr('rf_bary.bin', rf.barycoord);
 const cp = bvh.closestPointToPointBatch(points);
+{  // packets of 64 queries (setQueryLanes): the same answers, bit for bit
+  api.setQueryLanes({ closestPoint: 64 });
+  const pk = bvh.closestPointToPointBatch(points);
+  api.setQueryLanes({ closestPoint: 0 });
+  for (const k of ['faceIndex', 'distance', 'point', 'barycoord']) {
+    assert.ok(Buffer.from(pk[k].buffer).equals(Buffer.from(cp[k].buffer)), `packets == quads: ${k}`);
+  }
+  assert.throws(() => api.setQueryLanes({ closestPoint: 3 }), /lanes must be/);
+}
 writeArr('cp_face.bin', cp.faceIndex);
 writeArr('cp_dist.bin', cp.distance);
 writeArr('cp_point.bin', cp.point);

@@ -26,7 +26,8 @@ def test_addon_exports():
     names = set(r.stdout.strip().split(","))
     assert {"build", "uploadMesh", "freeMesh", "raycastFirst", "raycast", "closestPoint", "setSideMask",
             "deviceCount", "raycastFirstBackward", "closestPointBackward", "batchCreate", "freeBatch",
-            "batchRaycastFirst", "refitMesh", "intersectsSphere", "raycastFirstAsync", "closestPointAsync"} <= names
+            "batchRaycastFirst", "refitMesh", "intersectsSphere", "raycastFirstAsync", "closestPointAsync",
+            "sceneExportWebGPU", "setQueryLanes"} <= names
 
 
 @pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")

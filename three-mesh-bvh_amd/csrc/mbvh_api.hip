@@ -600,6 +600,7 @@ extern "C" int mbvh_ctx_set_ray_lanes(mbvh_ctx* ctx, int32_t lanes) {
   if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_ctx_set_ray_lanes: invalid context handle");
   if (lanes != 0 && lanes != 1 && lanes != 4)
     return set_error(MBVH_ERR_BAD_ARGS, "mbvh_ctx_set_ray_lanes: lanes must be 0 (automatic), 1 or 4");
+  std::lock_guard<std::mutex> guard(ctx->lock);  // host calls in flight on other threads read it
   ctx->ray_lanes = lanes;
   return MBVH_OK;
 }
@@ -609,9 +610,9 @@ extern "C" int mbvh_ctx_set_point_lanes(mbvh_ctx* ctx, int32_t lanes) {
   if (!valid_ctx(ctx)) return set_error(MBVH_ERR_BAD_HANDLE, "mbvh_ctx_set_point_lanes: invalid context handle");
   if (lanes != 0 && lanes != 1 && lanes != 4 && lanes != 64)
     return set_error(MBVH_ERR_BAD_ARGS, "mbvh_ctx_set_point_lanes: lanes must be 0 (automatic), 1, 4 or 64");
+  std::lock_guard<std::mutex> guard(ctx->lock);  // host calls in flight on other threads read it
   if (lanes == 64) {
     // the packets' buffers are allocated here, so that queries allocate nothing
-    std::lock_guard<std::mutex> guard(ctx->lock);
     MBVH_HIP(hipSetDevice(ctx->device));
     if (int rc = ensure_buffer(ctx->packet[0], kPacketChunk * kPacketCand * sizeof(uint2))) return rc;
     if (int rc = ensure_buffer(ctx->packet[1], kPacketChunk * sizeof(uint4))) return rc;

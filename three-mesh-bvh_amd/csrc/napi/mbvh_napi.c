@@ -307,6 +307,27 @@ static napi_value js_device_count(napi_env env, napi_callback_info info) {
   return out;
 }
 
+/* setQueryLanes(device, pointLanes, rayLanes): lanes per query of the device's context
+ * (mbvh_ctx_set_point_lanes / mbvh_ctx_set_ray_lanes); a null or undefined count is left as it is */
+static napi_value js_set_query_lanes(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  int32_t device, lanes;
+  if (!get_i32(env, argv[0], &device)) return throw_msg(env, "setQueryLanes: device must be an integer");
+  mbvh_ctx* ctx = NULL;
+  int rc = context_for(env, device, &ctx);
+  if (rc) return throw_status(env, rc, "mbvh_ctx_create");
+  if (!is_nullish(env, argv[1])) {
+    if (!get_i32(env, argv[1], &lanes)) return throw_msg(env, "setQueryLanes: pointLanes must be an integer");
+    if ((rc = mbvh_ctx_set_point_lanes(ctx, lanes))) return throw_status(env, rc, "mbvh_ctx_set_point_lanes");
+  }
+  if (!is_nullish(env, argv[2])) {
+    if (!get_i32(env, argv[2], &lanes)) return throw_msg(env, "setQueryLanes: rayLanes must be an integer");
+    if ((rc = mbvh_ctx_set_ray_lanes(ctx, lanes))) return throw_status(env, rc, "mbvh_ctx_set_ray_lanes");
+  }
+  return NULL;
+}
+
 /* uploadMesh(device, roots[], index|null, indirect|null, positions, stride, offset) -> external */
 static napi_value js_upload(napi_env env, napi_callback_info info) {
   napi_value argv[7];
@@ -1424,6 +1445,7 @@ static napi_value js_refit(napi_env env, napi_callback_info info) {
 static napi_value init(napi_env env, napi_value exports) {
   EXPORT("abiVersion", js_abi_version);
   EXPORT("deviceCount", js_device_count);
+  EXPORT("setQueryLanes", js_set_query_lanes);
   EXPORT("build", js_build);
   EXPORT("uploadMesh", js_upload);
   EXPORT("freeMesh", js_free_mesh);

@@ -34,5 +34,10 @@ module.exports = {
   setDevice: native.setDevice,
   getDevice: native.getDevice,
   deviceCount: () => native.load().deviceCount(),
+  // Lanes per query on the current device (process-wide; no counterpart in the reference):
+  // closest point 0 (automatic: four), 1, 4, or 64 (packets of 64 neighbouring queries sharing one
+  // traversal: grids and SDF sampling, the same answers bit for bit); raycastFirst 0, 1 or 4.
+  setQueryLanes: ({ closestPoint, raycastFirst } = {}) =>
+    native.load().setQueryLanes(native.getDevice(), closestPoint, raycastFirst),
   Vec3,
 };
