@@ -183,8 +183,11 @@ static int context_for(napi_env env, int32_t device, mbvh_ctx** out) {
 
 static void stop_query_thread(void);
 
+static napi_env g_env;
+
 static void cleanup_contexts(void* arg) {
   (void)arg;
+  g_env = NULL;
   stop_query_thread();
   /* scenes and batches before the meshes they hold, every library object before its context */
   for (BatchHandle* b = g_batches; b; b = b->next) {
@@ -1442,7 +1445,17 @@ static napi_value js_refit(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* The environment the process-wide state (contexts, handle lists, query thread) belongs to. */
+static napi_env g_env = NULL;
+
 static napi_value init(napi_env env, napi_value exports) {
+  if (g_env && g_env != env) {
+    napi_throw_error(env, NULL,
+                     "three-mesh-bvh-amd: the addon serves one Node environment per process (its device contexts, "
+                     "handles and query thread are process-wide); load it from one thread only");
+    return NULL;
+  }
+  g_env = env;
   EXPORT("abiVersion", js_abi_version);
   EXPORT("deviceCount", js_device_count);
   EXPORT("setQueryLanes", js_set_query_lanes);
