@@ -132,3 +132,15 @@ def test_js_box_restatement_matches_c_oracle(tmp_path):
     js = np.fromfile(tmp_path / "box_hit.u8", np.uint8).astype(bool)
     ref = O.OracleMesh(g.positions, idx, tree.roots).intersects_box(mn, mx, mats)
     assert np.array_equal(js, ref) and 0.05 < ref.mean() < 0.95
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_addon_refuses_a_second_environment():
+    """The addon's device contexts, handle lists and query thread are process-wide: a worker_threads
+    environment that loads it after the main thread gets an error instead of shared state."""
+    script = (f"const a=require({ADDON!r}); const {{ Worker }}=require('worker_threads');"
+              f"const w=new Worker(\"try {{ require({ADDON!r}); console.log('loaded'); }} catch (e) {{ console.log('refused: ' + e.message); }}\", {{ eval: true }});"
+              "w.on('exit', c => process.exit(c));")
+    r = subprocess.run(["node", "-e", script], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert "refused: three-mesh-bvh-amd: the addon serves one Node environment per process" in r.stdout
