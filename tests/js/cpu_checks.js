@@ -190,6 +190,48 @@ check('micro-batched per-call queries group the calls of one turn', async () => 
   assert.strictEqual(launches.length, 6);  // a later turn: a batch of its own
 });
 
+// Per-group side masks live in the mesh's triangle rows: an asynchronous raycast with another
+// material array waits for the ones in flight, and a synchronous one that would rewrite the mask
+// under them throws (stand-in addon functions; the device ones need a GPU).
+check('material-array masks wait for the asynchronous raycasts reading them', async () => {
+  const g = torus(1, 0.4, 12, 48);
+  const n = g.index.count;
+  g.addGroup(0, n / 2, 0);
+  g.addGroup(n / 2, n / 2, 1);
+  const bvh = new api.MeshBVH(g);
+  bvh._mesh = () => ({});
+  const addon = require('../../three-mesh-bvh_amd/src/native.js').load();
+  const saved = { launch: addon.raycastFirstAsync, mask: addon.setSideMask };
+  const events = [];
+  let pending = [];
+  addon.setSideMask = (mesh, mask) => events.push(`mask ${mask[0]}`);
+  addon.raycastFirstAsync = (mesh, o, d, side) => {
+    events.push(`launch ${side}`);
+    return new Promise(resolve => pending.push(resolve));
+  };
+  try {
+    const mat = side => ({ isMaterial: true, side });
+    const front = mat(api.FrontSide), back = mat(api.BackSide), both = mat(api.DoubleSide);
+    const o = new Float32Array(3), d = new Float32Array([0, 0, 1]);
+    const a = bvh.raycastFirstBatchAsync(o, d, { material: [front, back] });
+    const b = bvh.raycastFirstBatchAsync(o, d, { material: [back, front] });  // another mask: waits
+    const c = bvh.raycastFirstBatchAsync(o, d, { material: [front, back] });  // the same mask: runs now
+    bvh.raycastFirstBatchAsync(o, d, { side: api.DoubleSide }).catch(() => {});  // no mask: runs now
+    assert.throws(() => bvh.raycastFirstBatch(o, d, { material: [both, both] }), /in flight/);
+    assert.deepStrictEqual(events, ['mask 1', 'launch 3', 'launch 3', `launch ${api.DoubleSide}`]);
+    pending.forEach(r => r());
+    pending = [];
+    await Promise.all([a, c]);
+    for (let i = 0; i < 4 && events.length < 6; i++) await new Promise(r => setImmediate(r));
+    assert.deepStrictEqual(events.slice(4), ['mask 2', 'launch 3']);
+    pending.forEach(r => r());
+    await b;
+  } finally {
+    addon.raycastFirstAsync = saved.launch;
+    addon.setSideMask = saved.mask;
+  }
+});
+
 (async () => {
   let failed = 0;
   for (const [name, fn] of checks) {

@@ -232,15 +232,24 @@ class MeshBVH {
 
   // Side argument for the kernels; material arrays become per-face facing masks from the groups
   // (intersectTri's group loop, src/utils/ThreeRayIntersectUtilities.js:171-202).
+  // The mask is written into the mesh's triangle rows on the device, so it cannot change while an
+  // asynchronous query that reads it is in flight (raycastFirstBatchAsync waits for those instead).
+  _maskKeyFor(materialOrSide) {
+    const groups = this.geometry.groups || [];
+    const sides = groups.map(g => sideMask(getSide(materialOrSide[g.materialIndex])));
+    return { groups, sides, key: groups.map((g, i) => `${g.start}:${g.count}:${sides[i]}`).join(',') };
+  }
+
   _sideFor(materialOrSide) {
     if (!Array.isArray(materialOrSide)) {
       const side = getSide(materialOrSide);
       return side === C.BackSide ? C.BackSide : (side === C.DoubleSide ? C.DoubleSide : C.FrontSide);
     }
-    const groups = this.geometry.groups || [];
-    const sides = groups.map(g => sideMask(getSide(materialOrSide[g.materialIndex])));
-    const key = groups.map((g, i) => `${g.start}:${g.count}:${sides[i]}`).join(',');
+    const { groups, sides, key } = this._maskKeyFor(materialOrSide);
     if (key !== this._maskKey) {
+      if (this._maskInFlight > 0) {
+        throw new Error('BVH: an asynchronous raycast with other materials is in flight on this BVH; await it first');
+      }
       const faceCount = (this.geometry.index ? this.geometry.index.count : this.geometry.attributes.position.count) / 3;
       const mask = new Uint8Array(faceCount);
       groups.forEach((g, i) => {
@@ -299,16 +308,27 @@ class MeshBVH {
   // addon's query thread and the Promise resolves to the same packed results. The input arrays must not
   // change until it settles; the BVH cannot be disposed meanwhile (dispose() throws).
   raycastFirstBatchAsync(origins, directions, options = {}) {
+    const mos = options.side !== undefined ? options.side : (options.material !== undefined ? options.material : C.FrontSide);
+    if (Array.isArray(mos) && this._maskInFlight > 0 && this._maskKeyFor(mos).key !== this._maskKey) {
+      // another material array's mask is in use: start once the queries reading it have settled
+      return this._maskSettled.then(() => this.raycastFirstBatchAsync(origins, directions, options));
+    }
     const [o, d] = queryArrays(origins, directions);
     const n = o.length / FLOATS_PER_VEC;
-    const side = this._sideFor(options.side !== undefined ? options.side : (options.material !== undefined ? options.material : C.FrontSide));
+    const side = this._sideFor(mos);
     const out = {
       faceIndex: new Uint32Array(n), distance: new Float32Array(n),
       point: new Float32Array(3 * n), barycoord: new Float32Array(3 * n),
     };
-    return native.load().raycastFirstAsync(this._mesh(), o, d, side, options.near !== undefined ? options.near : 0,
+    const p = native.load().raycastFirstAsync(this._mesh(), o, d, side, options.near !== undefined ? options.near : 0,
       options.far !== undefined ? options.far : Infinity, out.faceIndex, out.distance, out.point, out.barycoord)
       .then(() => out);
+    if (side === 3) {  // reads the per-triangle mask
+      this._maskInFlight = (this._maskInFlight || 0) + 1;
+      const settled = p.then(() => undefined, () => undefined).then(() => { this._maskInFlight--; });
+      this._maskSettled = this._maskSettled ? Promise.all([this._maskSettled, settled]) : settled;
+    }
+    return p;
   }
 
   closestPointToPointBatchAsync(points, options = {}) {
