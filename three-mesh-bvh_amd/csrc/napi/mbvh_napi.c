@@ -960,7 +960,7 @@ static const char* query_where(const Query* q) {
   return q->f64 ? "mbvh_closest_point_f64" : "mbvh_closest_point";
 }
 
-/* Runs the query through the C ABI (no JavaScript values touched: safe on a worker thread). */
+/* Runs the query through the C ABI (no JavaScript values touched: safe on the query thread). */
 static void run_query(Query* q) {
   if (q->kind == 0 && q->f64)
     q->rc = mbvh_raycast_first_f64(q->ctx, q->h->mesh, (const double*)q->a, (const double*)q->b, q->n, q->side, q->p0,
