@@ -3,7 +3,8 @@
 // closestPointToPoint call, as acceleratedRaycast and a three.js app issue them (the JS numbers go
 // through the Node-API addon and the C ABI to the GPU and back, synchronously). Reads a geometry
 // and queries from <dir> (positions.f32, index.u16 or index.u32, rays.f32 = origins then
-// directions, points.f32, meta.json {index_bytes}) and prints one JSON line.
+// directions, points.f32, meta.json {index_bytes}) and prints one JSON line. Also times the same
+// per-call queries through the micro-batched Promise forms (raycastFirstAsync, closestPointToPointAsync).
 const fs = require('fs');
 const path = require('path');
 const api = require(path.join(__dirname, '..', 'src', 'index.js'));
@@ -70,7 +71,36 @@ for (let i = 0; i < calls; i++) {
 }
 const batchRayUs = timeCalls(1, () => bvh.raycastFirstBatch(o, d)) / calls;
 const batchPtUs = timeCalls(1, () => bvh.closestPointToPointBatch(q)) / calls;
-console.log(JSON.stringify({ node: process.version, calls, build_and_upload_ms: buildMs,
-  raycast_first_us_per_call: rayUs, closest_point_us_per_call: ptUs,
-  raycast_first_batch_us_per_query: batchRayUs, closest_point_batch_us_per_query: batchPtUs,
-  hit_fraction: hits / (100 + 5 * calls), found_fraction: found / (100 + 5 * calls) }));
+
+// the same per-call calls through the micro-batched Promise forms: `calls` calls in one turn of the
+// event loop, awaited together (one launch), median of 5 rounds, microseconds per call
+async function timeQueued(make) {
+  await Promise.all(Array.from({ length: 100 }, (_, i) => make(i)));  // warm-up
+  const per = [];
+  for (let rep = 0; rep < 5; rep++) {
+    const a = process.hrtime.bigint();
+    await Promise.all(Array.from({ length: calls }, (_, i) => make(i)));
+    per.push(Number(process.hrtime.bigint() - a) / 1e3 / calls);
+  }
+  per.sort((x, y) => x - y);
+  return per[2];
+}
+(async () => {
+  const queuedRayUs = await timeQueued((i) => {
+    const k = i % nRays;
+    return bvh.raycastFirstAsync({ origin: v(rays[3 * k], rays[3 * k + 1], rays[3 * k + 2]),
+      direction: v(rays[3 * (nRays + k)], rays[3 * (nRays + k) + 1], rays[3 * (nRays + k) + 2]) });
+  });
+  const queuedPtUs = await timeQueued((i) => {
+    const k = i % nPts;
+    return bvh.closestPointToPointAsync(v(points[3 * k], points[3 * k + 1], points[3 * k + 2]));
+  });
+  console.log(JSON.stringify({ node: process.version, calls, build_and_upload_ms: buildMs,
+    raycast_first_us_per_call: rayUs, closest_point_us_per_call: ptUs,
+    raycast_first_batch_us_per_query: batchRayUs, closest_point_batch_us_per_query: batchPtUs,
+    raycast_first_queued_us_per_call: queuedRayUs, closest_point_queued_us_per_call: queuedPtUs,
+    hit_fraction: hits / (100 + 5 * calls), found_fraction: found / (100 + 5 * calls) }));
+})().catch((e) => {
+  console.error(e);
+  process.exit(1);
+});
