@@ -1473,7 +1473,7 @@ __device__ __noinline__ Winner resolve_point(Cands cl, float U, f3 q, f3 ql, con
     const uint2 e = cl.get(i);
     if (__uint_as_float(e.y) > U) continue;
     const Tri tr = load_tri(tris, e.x);
-    const ClosestFF cf = closest_on_triangle_ff<kLo>(q, ql, tr.a(), tr.b(), tr.c());
+    const ClosestFF cf = closest_on_triangle_ff_inline<kLo>(q, ql, tr.a(), tr.b(), tr.c());
     const ff d2 = cf.dist2;
     if (!(d2.hi == d2.hi)) continue;  // degenerate: never wins (reference: `NaN < best` is false)
     bool take = win == kNoHit;
